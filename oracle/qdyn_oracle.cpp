@@ -1,0 +1,850 @@
+// ORACLE — test infrastructure only.  A plain FP64 CPU restatement of serial
+// reference qdyn's MD step for spherical boundaries (the path the GPU engine
+// replaces), order-faithful so that it reproduces reference qdyn on the same
+// inputs.  Only the tests, the smoke check and the benchmark's CPU-baseline
+// leg load this library; the GPU product never links or calls it.
+//
+// Compile with -ffp-contract=off: every expression keeps the Fortran
+// operation order and no multiply-add is fused.
+//
+// Functions and the reference code each one follows (src/md.f90):
+//   md_run step loop            6471-6664
+//   make_pair_lists             5985-6071   cgp_centers 2253-2288
+//   nbpplist_lrf                7998-8188   nbpwlist_lrf 9300-9429
+//   nbqplist                   10215-10322  nbqwlist    10464-10549
+//   nbwwlist_lrf               10897-11021  lrf_update   5896-5981
+//   pot_energy                 16015-16221  pot_energy_nonbonds 16261-16344
+//   nonbond_pp 13069-13155, nonbond_pw 13282-13365, nonbond_qp 13788-13921,
+//   nonbond_ww_spc 15021-15119, nonbond_qw_spc 14411-14570,
+//   nonbond_qq 13491-13636, lrf_taylor 5824-5891
+//   bond 1332-1405, angle 1130-1246, torsion 17907-18052, improper 3130-3272
+//   fix_shell 2816-2862, restrain_solvent 18055-18102, watpol 18372-18541
+//   newvel_ber 6194-6223, shake 17581-17679, temperature 6098-6190
+#include "../qsource_amd/csrc/host/qsys.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+using namespace qsys;
+
+namespace {
+
+const double kPi = 4.0 * std::atan(1.0);
+
+struct NB { int i, j, ljcod; };
+struct NBQ { int iq, j, ljcod; };  // Q-atom number, partner atom (or first atom of water)
+
+struct LRF {
+  double cent[3];
+  double phi0, phi1[3], phi2[9], phi3[27];
+};
+
+struct Energies {
+  // classical
+  double pp_el = 0, pp_vdw = 0, pw_el = 0, pw_vdw = 0, ww_el = 0, ww_vdw = 0;
+  double p_bond = 0, p_angle = 0, p_torsion = 0, p_improper = 0;
+  double w_bond = 0, w_angle = 0, w_torsion = 0, w_improper = 0;
+  double q_bond = 0, q_angle = 0, q_torsion = 0, q_improper = 0;
+  double qx_el = 0, qx_vdw = 0, lrf = 0;
+  double rst_fix = 0, rst_shell = 0, rst_protein = 0, rst_radial = 0, rst_wpol = 0, rst_total = 0;
+  double potential = 0, kinetic = 0;
+};
+struct QEnergies {
+  double qq_el = 0, qq_vdw = 0, qp_el = 0, qp_vdw = 0, qw_el = 0, qw_vdw = 0;
+  double qx_el = 0, qx_vdw = 0, total = 0, restraint = 0;
+  double bond = 0, angle = 0, torsion = 0, improper = 0;
+};
+
+struct Oracle {
+  Setup s;
+  std::vector<double> x, v, xx, d;
+  std::vector<NB> nbpp, nbpw, nbww;
+  std::vector<NBQ> nbqp, nbqw;
+  bool qp_done = false, qw_done = false;
+  std::vector<LRF> lrf;
+  Energies E;
+  std::vector<QEnergies> EQ;
+  double Temp = 0, Tfree = 0, Temp_solute = 0, Tfree_solute = 0, Texcl_solute = 0;
+  double Temp_solvent = 0, Tfree_solvent = 0, Texcl_solvent = 0;
+  double Tscale_solute = 0, Tscale_solvent = 0, Ekinmax = 0;
+  int istep = 0;
+  // watpol work arrays
+  std::vector<double> theta, theta0, tdum;
+  std::vector<std::vector<int>> list_sh, nsort;
+  std::vector<double> cgp_cent_dummy;
+  int last_shake_iter = 0;
+  std::string err;
+
+  const Topology& t() const { return s.top; }
+  int ncgp() const { return (int)s.top.cgp.size(); }
+  int natyps() const { return s.natyps; }
+  int ljcod(int a, int b) const { return s.ljcod[a * s.natyps + b]; }
+  const AtomType& typ(int i) const { return s.top.types[s.iac[i]]; }
+
+  explicit Oracle(const std::string& inp) : s(prepare(inp)) {
+    x = s.x; v = s.v;
+    xx.assign(x.size(), 0.0);
+    d.assign(x.size(), 0.0);
+    lrf.assign(ncgp(), LRF{});
+    EQ.assign(s.nstates, QEnergies{});
+    theta.assign(s.nwat, 0.0); theta0.assign(s.nwat, 0.0); tdum.assign(s.nwat, 0.0);
+    list_sh.assign(s.wshell.size(), {});
+    nsort.assign(s.wshell.size(), {});
+    Ekinmax = 1000.0 * s.ndegf * kBoltz * s.inp.temp0 / 2.0 / (double)s.natom;
+    temperature();
+  }
+
+  // ---------------------------------------------------------------- lists
+  bool excluded_pair(int i, int j) const {
+    if (std::abs(j - i) <= kMaxNbrRange) {
+      if (i < j) return (s.top.listex[i] >> (j - i - 1)) & 1u;
+      return (s.top.listex[j] >> (i - j - 1)) & 1u;
+    }
+    for (auto& p : s.top.listexlong)
+      if ((p.first == i && p.second == j) || (p.first == j && p.second == i)) return true;
+    return false;
+  }
+  bool pair14(int i, int j) const {
+    if (std::abs(j - i) <= kMaxNbrRange) {
+      if (i < j) return (s.top.list14[i] >> (j - i - 1)) & 1u;
+      return (s.top.list14[j] >> (i - j - 1)) & 1u;
+    }
+    bool r = false;
+    for (auto& p : s.top.list14long)
+      if ((p.first == i && p.second == j) || (p.first == j && p.second == i)) r = true;
+    return r;
+  }
+  static bool skip_parity(int ig, int jg) {
+    // 1-based parity rule (src/md.f90:10977-10980) with 0-based arguments
+    int a = ig + 1, b = jg + 1;
+    return (a > b && (a + b) % 2 == 0) || (a < b && (a + b) % 2 == 1);
+  }
+
+  void cgp_centers() {
+    const auto& T = s.top;
+    for (int ig = 0; ig < ncgp(); ++ig) {
+      LRF& L = lrf[ig];
+      L.cent[0] = L.cent[1] = L.cent[2] = 0;
+      L.phi0 = 0;
+      for (double& p : L.phi1) p = 0;
+      for (double& p : L.phi2) p = 0;
+      for (double& p : L.phi3) p = 0;
+      for (int a = T.cgp[ig].first; a <= T.cgp[ig].last; ++a) {
+        int i3 = 3 * T.cgpatom[a];
+        L.cent[0] = L.cent[0] + x[i3];
+        L.cent[1] = L.cent[1] + x[i3 + 1];
+        L.cent[2] = L.cent[2] + x[i3 + 2];
+      }
+      double n = (double)(T.cgp[ig].last - T.cgp[ig].first + 1);
+      for (int k = 0; k < 3; ++k) L.cent[k] = L.cent[k] / n;
+    }
+  }
+
+  void lrf_update(int g1, int g2) {
+    const auto& T = s.top;
+    LRF& L = lrf[g2];
+    for (int ia = T.cgp[g1].first; ia <= T.cgp[g1].last; ++ia) {
+      int i = T.cgpatom[ia];
+      if (s.iqatom[i] != 0) continue;
+      int i3 = 3 * i;
+      double dr[3] = {x[i3] - L.cent[0] - 0.0, x[i3 + 1] - L.cent[1] - 0.0, x[i3 + 2] - L.cent[2] - 0.0};
+      double r2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+      double f0 = s.crg[i] / (r2 * std::sqrt(r2));
+      double f1 = 3.0 * f0 / r2;
+      double f2 = -f1 / r2;
+      L.phi0 = L.phi0 + f0 * r2;
+      for (int k = 0; k < 3; ++k) L.phi1[k] = L.phi1[k] - f0 * dr[k];
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+          double v2 = L.phi2[3 * a + b] + f1 * dr[a] * dr[b];
+          L.phi2[3 * a + b] = (a == b) ? v2 - f0 : v2;
+        }
+      // phi3(9a+3b+c) += f2*(5 dr_a dr_b dr_c - r2*(delta terms))
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+          for (int c = 0; c < 3; ++c) {
+            double t5 = 5.0 * dr[a] * dr[b] * dr[c];
+            double term;
+            if (a == b && b == c) term = t5 - r2 * 3.0 * dr[a];
+            else if (a == b) term = t5 - r2 * dr[c];
+            else if (a == c) term = t5 - r2 * dr[b];
+            else if (b == c) term = t5 - r2 * dr[a];
+            else term = t5;
+            L.phi3[9 * a + 3 * b + c] = L.phi3[9 * a + 3 * b + c] + f2 * term;
+          }
+    }
+  }
+
+  void nbwwlist_lrf() {
+    nbww.clear();
+    const double rcut2 = s.inp.rcww * s.inp.rcww;
+    const double rclrf2 = s.inp.rclrf * s.inp.rclrf;
+    const int ns = s.nat_solute;
+    for (int iw = 0; iw < s.nwat; ++iw) {
+      int is = ns + 3 * iw;
+      if (s.excl[is]) continue;
+      int is3 = 3 * is;
+      int ig = s.iwhich_cgp[is];
+      for (int jw = 0; jw < s.nwat; ++jw) {
+        int ja = ns + 3 * jw;
+        if (s.excl[ja]) continue;
+        int jg = s.iwhich_cgp[ja];
+        int j3 = 3 * ja;
+        if (skip_parity(iw, jw) || iw == jw) continue;
+        double r2 = (x[is3] - x[j3]) * (x[is3] - x[j3]) + (x[is3 + 1] - x[j3 + 1]) * (x[is3 + 1] - x[j3 + 1])
+                  + (x[is3 + 2] - x[j3 + 2]) * (x[is3 + 2] - x[j3 + 2]);
+        if (r2 <= rcut2) {
+          for (int la = 0; la < 3; ++la)
+            for (int ka = 0; ka < 3; ++ka)
+              nbww.push_back({ns + 3 * iw + la, ns + 3 * jw + ka, ljcod(s.iac[ns + la], s.iac[ns + ka])});
+        } else if (r2 <= rclrf2) {
+          lrf_update(ig, jg);
+          lrf_update(jg, ig);
+        }
+      }
+    }
+  }
+
+  void make_pair_lists() {
+    if (s.inp.use_lrf) {
+      cgp_centers();
+      nbpplist_lrf();
+      nbpwlist_lrf();
+      nbqplist();
+      nbwwlist_lrf();
+    } else {
+      throw std::runtime_error("runs without LRF are not supported");
+    }
+    nbqwlist();
+  }
+
+  // solute lists: filled in below
+  void nbpplist_lrf();
+  void nbpwlist_lrf();
+  void nbqplist();
+  void nbqwlist();
+
+  // ---------------------------------------------------------------- forces
+  void nonbond_ww_spc() {
+    int ichg = 1;
+    const size_t n = nbww.size();
+    for (size_t iw = 0; iw < n; iw += 3) {
+      if (ichg > 3) ichg = 1;
+      int i = nbww[iw].i, j = nbww[iw].j;
+      int i3 = 3 * i;
+      double c1 = s.chg_solv[ichg - 1];
+      for (int ip = 0; ip < 3; ++ip) {
+        int ja = j + ip, j3 = 3 * ja;
+        double c2 = s.chg_solv[ip];
+        double dx1 = x[j3] - x[i3], dx2 = x[j3 + 1] - x[i3 + 1], dx3 = x[j3 + 2] - x[i3 + 2];
+        double r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
+        double r6 = r2 * r2 * r2;
+        r2 = 1.0 / r2;
+        double r = std::sqrt(r2);
+        double Vel = c1 * c2 * r;
+        E.ww_el = E.ww_el + Vel;
+        double dv = r2 * (-Vel);
+        if (ichg == 1 && ip == 0) {
+          int iLJ = ljcod(s.iac[i], s.iac[j]);
+          double a1 = s.alj_solv[0][iLJ - 1], a2 = s.alj_solv[0][iLJ - 1];
+          double b1 = s.blj_solv[0][iLJ - 1], b2 = s.blj_solv[0][iLJ - 1];
+          r6 = 1.0 / r6;
+          double Va = a1 * a2 * r6 * r6;
+          double Vb = b1 * b2 * r6;
+          E.ww_vdw = E.ww_vdw + Va - Vb;
+          dv = dv + r2 * (-12.0 * Va + 6.0 * Vb);
+        }
+        d[i3] = d[i3] - dv * dx1; d[i3 + 1] = d[i3 + 1] - dv * dx2; d[i3 + 2] = d[i3 + 2] - dv * dx3;
+        d[j3] = d[j3] + dv * dx1; d[j3 + 1] = d[j3 + 1] + dv * dx2; d[j3 + 2] = d[j3 + 2] + dv * dx3;
+      }
+      ++ichg;
+    }
+  }
+
+  void lrf_taylor() {
+    for (int i = 0; i < s.natom; ++i) {
+      if (s.excl[i] || s.iqatom[i] != 0) continue;
+      int i3 = 3 * i;
+      const LRF& L = lrf[s.iwhich_cgp[i]];
+      double dr[3] = {L.cent[0] - x[i3], L.cent[1] - x[i3 + 1], L.cent[2] - x[i3 + 2]};
+      const double* p2 = L.phi2;
+      const double* p3 = L.phi3;
+      double Vij = L.phi0 + L.phi1[0] * dr[0] + L.phi1[1] * dr[1] + L.phi1[2] * dr[2]
+                 + 0.5 * (p2[0] * dr[0] + p2[1] * dr[1] + p2[2] * dr[2]) * dr[0]
+                 + 0.5 * (p2[3] * dr[0] + p2[4] * dr[1] + p2[5] * dr[2]) * dr[1]
+                 + 0.5 * (p2[6] * dr[0] + p2[7] * dr[1] + p2[8] * dr[2]) * dr[2];
+      E.lrf = E.lrf + 0.5 * s.crg[i] * Vij;
+      double df[3];
+      for (int a = 0; a < 3; ++a) {
+        const double* q = p3 + 9 * a;
+        df[a] = L.phi1[a] + p2[3 * a] * dr[0] + p2[3 * a + 1] * dr[1] + p2[3 * a + 2] * dr[2]
+              + 0.5 * (q[0] * dr[0] + q[1] * dr[1] + q[2] * dr[2]) * dr[0]
+              + 0.5 * (q[3] * dr[0] + q[4] * dr[1] + q[5] * dr[2]) * dr[1]
+              + 0.5 * (q[6] * dr[0] + q[7] * dr[1] + q[8] * dr[2]) * dr[2];
+      }
+      d[i3] = d[i3] - s.crg[i] * df[0];
+      d[i3 + 1] = d[i3 + 1] - s.crg[i] * df[1];
+      d[i3 + 2] = d[i3 + 2] - s.crg[i] * df[2];
+    }
+  }
+
+  void nonbond_pp();
+  void nonbond_pw();
+  void nonbond_qp();
+  void nonbond_qw_spc();
+  void nonbond_qq();
+  double bond(int a, int b);
+  double angle(int a, int b);
+  double torsion(int a, int b);
+  double improper(int a, int b);
+
+  void fix_shell() {
+    for (int i = 0; i < s.natom; ++i) {
+      if (!(s.excl[i] || s.shell[i])) continue;
+      int i3 = 3 * i;
+      double fk;
+      if (s.excl[i]) {
+        fk = s.inp.fk_fix;
+        if (s.inp.freeze) {
+          v[i3] = v[i3 + 1] = v[i3 + 2] = 0;
+          x[i3] = s.xtop[i3]; x[i3 + 1] = s.xtop[i3 + 1]; x[i3 + 2] = s.xtop[i3 + 2];
+          fk = 0;
+        }
+      } else {
+        fk = s.inp.fk_pshell;
+      }
+      double dr[3] = {x[i3] - s.xtop[i3], x[i3 + 1] - s.xtop[i3 + 1], x[i3 + 2] - s.xtop[i3 + 2]};
+      double r2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+      double erst = 0.5 * fk * r2;
+      if (s.excl[i]) E.rst_fix = E.rst_fix + erst;
+      if (s.shell[i]) E.rst_shell = E.rst_shell + erst;
+      d[i3] = d[i3] + fk * dr[0];
+      d[i3 + 1] = d[i3 + 1] + fk * dr[1];
+      d[i3 + 2] = d[i3 + 2] + fk * dr[2];
+    }
+  }
+
+  void restrain_solvent() {
+    double shift = s.fk_wsphere != 0 ? std::sqrt(kBoltz * Tfree / s.fk_wsphere) : 0.0;
+    const auto& T = s.top;
+    for (int iw = T.ncgp_solute; iw < ncgp(); ++iw) {
+      int i = T.cgp[iw].iswitch;
+      if (s.excl[i]) continue;
+      int i3 = 3 * i;
+      double dr[3] = {x[i3] - T.xwcent[0], x[i3 + 1] - T.xwcent[1], x[i3 + 2] - T.xwcent[2]};
+      double b = std::sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+      double db = b - (s.rwat - shift);
+      double erst, dv;
+      if (db > 0) {
+        erst = 0.5 * s.fk_wsphere * db * db - s.dwmz;
+        dv = s.fk_wsphere * db / b;
+      } else if (b > 0) {
+        double fexp = std::exp(s.awmz * db);
+        erst = s.dwmz * (fexp * fexp - 2.0 * fexp);
+        dv = -2.0 * s.dwmz * s.awmz * (fexp - fexp * fexp) / b;
+      } else {
+        dv = 0; erst = 0;
+      }
+      E.rst_radial = E.rst_radial + erst;
+      d[i3] = d[i3] + dv * dr[0];
+      d[i3 + 1] = d[i3 + 1] + dv * dr[1];
+      d[i3 + 2] = d[i3 + 2] + dv * dr[2];
+    }
+  }
+
+  void watpol() {
+    const int nsh = (int)s.wshell.size();
+    const auto& xw = s.top.xwcent;
+    for (auto& w : s.wshell) w.n_insh = 0;
+    for (auto& l : list_sh) l.clear();
+    for (int iw = 0; iw < s.nwat; ++iw) {
+      theta[iw] = 0; theta0[iw] = 0;
+      int i = s.nat_solute + 3 * iw;
+      if (s.excl[i]) continue;
+      int i3 = 3 * i;
+      double rmu[3], rcu[3];
+      for (int k = 0; k < 3; ++k) rmu[k] = x[i3 + 3 + k] + x[i3 + 6 + k] - 2.0 * x[i3 + k];
+      double rm = std::sqrt(rmu[0] * rmu[0] + rmu[1] * rmu[1] + rmu[2] * rmu[2]);
+      for (int k = 0; k < 3; ++k) rmu[k] = rmu[k] / rm;
+      for (int k = 0; k < 3; ++k) rcu[k] = x[i3 + k] - xw[k];
+      double rc = std::sqrt(rcu[0] * rcu[0] + rcu[1] * rcu[1] + rcu[2] * rcu[2]);
+      for (int k = 0; k < 3; ++k) rcu[k] = rcu[k] / rc;
+      double scp = rmu[0] * rcu[0] + rmu[1] * rcu[1] + rmu[2] * rcu[2];
+      if (scp > 1.0) scp = 1.0;
+      if (scp < -1.0) scp = -1.0;
+      theta[iw] = std::acos(scp);
+      tdum[iw] = theta[iw];
+      if (rc > s.wshell[nsh - 1].rout - s.wshell[nsh - 1].dr) {
+        int is;
+        for (is = nsh - 1; is >= 1; --is)
+          if (rc <= s.wshell[is].rout) break;
+        if (is < 1) is = 0;
+        s.wshell[is].n_insh += 1;
+        list_sh[is].push_back(iw);
+      }
+    }
+    for (int is = 0; is < nsh; ++is) {
+      int n = s.wshell[is].n_insh;
+      nsort[is].assign(n, 0);
+      int jmin = 0;
+      for (int il = 0; il < n; ++il) {
+        double tmin = 2.0 * kPi;
+        for (int jl = 0; jl < n; ++jl) {
+          int jw = list_sh[is][jl];
+          if (tdum[jw] < tmin) { jmin = jw; tmin = theta[jw]; }
+        }
+        nsort[is][il] = jmin;
+        tdum[jmin] = 99999.0;
+      }
+    }
+    if (istep != 0 && istep % kItdisUpdate == 0) {
+      for (auto& w : s.wshell) {
+        w.avtheta = w.avtheta / (double)kItdisUpdate;
+        w.avn_insh = w.avn_insh / (double)kItdisUpdate;
+        w.theta_corr = w.theta_corr + w.avtheta - std::acos(w.cstb);
+        w.avtheta = 0; w.avn_insh = 0;
+      }
+    }
+    for (int is = 0; is < nsh; ++is) {
+      WaterShell& w = s.wshell[is];
+      if (w.n_insh == 0) continue;
+      double avtdum = 0;
+      for (int il = 0; il < w.n_insh; ++il) {
+        int iw = nsort[is][il];
+        double arg = 1.0 + (1.0 - 2.0 * (double)(il + 1)) / (double)w.n_insh;
+        double th0 = std::acos(arg);
+        th0 = th0 - 3.0 * std::sin(th0) * w.cstb / 2.0;
+        if (th0 < 0) th0 = 0;
+        if (th0 > kPi) th0 = kPi;
+        theta0[il] = th0;
+        avtdum = avtdum + theta[iw];
+        double dth = theta[iw] - th0 + w.theta_corr;
+        E.rst_wpol = E.rst_wpol + 0.5 * s.fkwpol * dth * dth;
+        double dv = s.fkwpol * (theta[iw] - th0 + w.theta_corr);
+        int i = s.nat_solute + 3 * iw;
+        int i3 = 3 * i;
+        double rmu[3], rcu[3];
+        for (int k = 0; k < 3; ++k) rmu[k] = x[i3 + 3 + k] + x[i3 + 6 + k] - 2.0 * x[i3 + k];
+        double rm = std::sqrt(rmu[0] * rmu[0] + rmu[1] * rmu[1] + rmu[2] * rmu[2]);
+        for (int k = 0; k < 3; ++k) rmu[k] = rmu[k] / rm;
+        for (int k = 0; k < 3; ++k) rcu[k] = x[i3 + k] - xw[k];
+        double rc = std::sqrt(rcu[0] * rcu[0] + rcu[1] * rcu[1] + rcu[2] * rcu[2]);
+        for (int k = 0; k < 3; ++k) rcu[k] = rcu[k] / rc;
+        double scp = rmu[0] * rcu[0] + rmu[1] * rcu[1] + rmu[2] * rcu[2];
+        if (scp > 1.0) scp = 1.0;
+        if (scp < -1.0) scp = -1.0;
+        double f0 = std::sin(std::acos(scp));
+        if (std::fabs(f0) < kQrealEps) f0 = kQrealEps;
+        f0 = -1.0 / f0;
+        f0 = dv * f0;
+        double f1[9], f2[3];
+        for (int k = 0; k < 3; ++k) {
+          f1[k] = -2.0 * (rcu[k] - rmu[k] * scp) / rm;
+          f1[3 + k] = (rcu[k] - rmu[k] * scp) / rm;
+          f1[6 + k] = (rcu[k] - rmu[k] * scp) / rm;
+          f2[k] = (rmu[k] - rcu[k] * scp) / rc;
+        }
+        for (int k = 0; k < 3; ++k) d[i3 + k] = d[i3 + k] + f0 * (f1[k] + f2[k]);
+        for (int k = 3; k < 9; ++k) d[i3 + k] = d[i3 + k] + f0 * (f1[k]);
+      }
+      w.avtheta = w.avtheta + avtdum / (double)w.n_insh;
+      w.avn_insh = w.avn_insh + w.n_insh;
+    }
+  }
+
+  void pot_energy() {
+    E = Energies{};
+    for (auto& q : EQ) q = QEnergies{};
+    std::fill(d.begin(), d.end(), 0.0);
+    // pot_energy_nonbonds (sphere, geometric rule)
+    nonbond_pp();
+    nonbond_pw();
+    nonbond_qp();
+    if (s.natom > s.nat_solute) {
+      nonbond_ww_spc();
+      nonbond_qw_spc();
+    }
+    if (s.inp.use_lrf) lrf_taylor();
+    // bonded (FF_GROMOS/default impropers)
+    E.p_bond = bond(0, s.nbonds_solute);
+    E.w_bond = bond(s.nbonds_solute, (int)s.bnd.size());
+    E.p_angle = angle(0, s.nangles_solute);
+    E.w_angle = angle(s.nangles_solute, (int)s.ang.size());
+    E.p_torsion = torsion(0, s.ntors_solute);
+    E.w_torsion = torsion(s.ntors_solute, (int)s.tor.size());
+    E.p_improper = improper(0, s.nimps_solute);
+    E.w_improper = improper(s.nimps_solute, (int)s.imp.size());
+    fix_shell();
+    if (s.nwat > 0) {
+      restrain_solvent();
+      if (s.wpol_restr) watpol();
+    }
+    nonbond_qq();
+    for (int st = 0; st < s.nstates; ++st) {
+      QEnergies& q = EQ[st];
+      q.qx_el = q.qq_el + q.qp_el + q.qw_el;
+      q.qx_vdw = q.qq_vdw + q.qp_vdw + q.qw_vdw;
+      q.total = q.bond + q.angle + q.torsion + q.improper + q.qx_el + q.qx_vdw + q.restraint;
+      double lam = s.lambda[st];
+      E.q_bond = E.q_bond + q.bond * lam;
+      E.q_angle = E.q_angle + q.angle * lam;
+      E.q_torsion = E.q_torsion + q.torsion * lam;
+      E.q_improper = E.q_improper + q.improper * lam;
+      E.qx_el = E.qx_el + q.qx_el * lam;
+      E.qx_vdw = E.qx_vdw + q.qx_vdw * lam;
+      E.rst_protein = E.rst_protein + q.restraint * lam;
+    }
+    E.rst_total = E.rst_fix + E.rst_shell + E.rst_protein + E.rst_radial + E.rst_wpol;
+    E.potential = E.p_bond + E.w_bond + E.p_angle + E.w_angle + E.p_torsion + E.p_improper + E.pp_el
+                + E.pp_vdw + E.pw_el + E.pw_vdw + E.ww_el + E.ww_vdw + E.q_bond + E.q_angle
+                + E.q_torsion + E.q_improper + E.qx_el + E.qx_vdw + E.rst_total + E.lrf;
+  }
+
+  // ---------------------------------------------------------------- update
+  void newvel_ber(double step, double tscale, int a0, int a1) {
+    for (int i = a0; i < a1; ++i) {
+      int i3 = 3 * i;
+      for (int k = 0; k < 3; ++k) {
+        v[i3 + k] = (v[i3 + k] - d[i3 + k] * s.winv[i] * step) * tscale;
+        xx[i3 + k] = x[i3 + k];
+        x[i3 + k] = x[i3 + k] + v[i3 + k] * s.dt;
+      }
+    }
+  }
+
+  int shake() {
+    int total = 0;
+    for (const ShakeMol& m : s.shake_mol) {
+      int nc = (int)m.bond.size();
+      std::vector<char> ready(nc, 0);
+      int nits = 0;
+      for (;;) {
+        for (int ic = 0; ic < nc; ++ic) {
+          if (ready[ic]) continue;
+          int i = m.bond[ic].i, j = m.bond[ic].j, i3 = 3 * i, j3 = 3 * j;
+          double a0 = x[i3] - x[j3], a1 = x[i3 + 1] - x[j3 + 1], a2 = x[i3 + 2] - x[j3 + 2];
+          double xij2 = a0 * a0 + a1 * a1 + a2 * a2;
+          double diff = m.bond[ic].dist2 - xij2;
+          if (std::fabs(diff) < kShakeTol * m.bond[ic].dist2) ready[ic] = 1;
+          double b0 = xx[i3] - xx[j3], b1 = xx[i3 + 1] - xx[j3 + 1], b2 = xx[i3 + 2] - xx[j3 + 2];
+          double scp = a0 * b0 + a1 * b1 + a2 * b2;
+          double corr = diff / (2.0 * scp * (s.winv[i] + s.winv[j]));
+          x[i3] = x[i3] + b0 * corr * s.winv[i];
+          x[i3 + 1] = x[i3 + 1] + b1 * corr * s.winv[i];
+          x[i3 + 2] = x[i3 + 2] + b2 * corr * s.winv[i];
+          x[j3] = x[j3] - b0 * corr * s.winv[j];
+          x[j3 + 1] = x[j3 + 1] - b1 * corr * s.winv[j];
+          x[j3 + 2] = x[j3 + 2] - b2 * corr * s.winv[j];
+        }
+        ++nits;
+        bool all = true;
+        for (int ic = 0; ic < nc; ++ic) all = all && ready[ic];
+        if (all) break;
+        if (nits >= kShakeMaxIter) return -1;
+      }
+      total += nits;
+    }
+    return total;
+  }
+
+  void temperature() {
+    Temp = 0; Temp_solute = 0; Tfree_solute = 0; Texcl_solute = 0;
+    for (int i = 0; i < s.nat_solute; ++i) {
+      int i3 = 3 * i;
+      double ek = 0.5 * typ(i).mass * (v[i3] * v[i3] + v[i3 + 1] * v[i3 + 1] + v[i3 + 2] * v[i3 + 2]);
+      Temp_solute = Temp_solute + ek;
+      if (!s.excl[i]) Tfree_solute = Tfree_solute + ek; else Texcl_solute = Texcl_solute + ek;
+    }
+    Tfree_solvent = 0; Temp_solvent = 0; Texcl_solvent = 0;
+    for (int i = s.nat_solute; i < s.natom; ++i) {
+      int i3 = 3 * i;
+      double ek = 0.5 * typ(i).mass * (v[i3] * v[i3] + v[i3 + 1] * v[i3 + 1] + v[i3 + 2] * v[i3 + 2]);
+      Temp_solvent = Temp_solvent + ek;
+      if (!s.excl[i]) Tfree_solvent = Tfree_solvent + ek; else Texcl_solvent = Texcl_solvent + ek;
+    }
+    Tfree = Tfree_solvent + Tfree_solute;
+    Temp = Temp_solute + Temp_solvent;
+    E.kinetic = Temp;
+    Temp = 2.0 * Temp / kBoltz / (double)s.ndegf;
+    Tfree = 2.0 * Tfree / kBoltz / (double)s.ndegfree;
+    if (s.detail_temps) {
+      Temp_solute = 2.0 * Temp_solute / kBoltz / (double)s.ndegf_solute;
+      Tfree_solute = 2.0 * Tfree_solute / kBoltz / (double)s.ndegfree_solute;
+      Temp_solvent = 2.0 * Temp_solvent / kBoltz / (double)s.ndegf_solvent;
+      Tfree_solvent = 2.0 * Tfree_solvent / kBoltz / (double)s.ndegfree_solvent;
+    }
+    const double T0 = s.inp.temp0, dt = s.dt, tau = s.tau_t;
+    if (s.separate_scaling) {
+      if (Tfree_solvent != 0) Tscale_solvent = T0 / Tfree_solvent - 1.0;
+      Tscale_solvent = std::sqrt(1 + dt / tau * Tscale_solvent);
+      if (Tfree_solute != 0) Tscale_solute = T0 / Tfree_solute - 1.0;
+      Tscale_solute = std::sqrt(1 + dt / tau * Tscale_solute);
+    } else {
+      if (Tfree != 0) Tscale_solvent = T0 / Tfree - 1.0;
+      Tscale_solvent = std::sqrt(1 + dt / tau * Tscale_solvent);
+      Tscale_solute = Tscale_solvent;
+    }
+  }
+
+  // one md_run iteration (src/md.f90:6471-6612) without output
+  bool step() {
+    if (istep % s.inp.nbcycle == 0) make_pair_lists();
+    pot_energy();
+    newvel_ber(s.dt, Tscale_solute, 0, s.nat_solute);
+    newvel_ber(s.dt, Tscale_solvent, s.nat_solute, s.natom);
+    if (s.shake_constraints > 0) {
+      int n = shake();
+      if (n < 0) { err = "shake failure"; return false; }
+      last_shake_iter = n / s.nmol;
+      for (size_t k = 0; k < x.size(); ++k) v[k] = (x[k] - xx[k]) / s.dt;
+    }
+    temperature();
+    ++istep;
+    return true;
+  }
+};
+
+// ------------------------------------------------------------------ solute
+void Oracle::nbpplist_lrf() {
+  nbpp.clear();
+  const auto& T = s.top;
+  const double rcut2 = s.inp.rcpp * s.inp.rcpp, rclrf2 = s.inp.rclrf * s.inp.rclrf;
+  for (int ig = 0; ig < T.ncgp_solute; ++ig) {
+    int ia = T.cgp[ig].iswitch;
+    if (s.excl[ia]) continue;
+    int i3 = 3 * ia;
+    for (int jg = 0; jg < T.ncgp_solute; ++jg) {
+      if (skip_parity(ig, jg)) continue;
+      int ja = T.cgp[jg].iswitch;
+      if (s.excl[ja]) continue;
+      int j3 = 3 * ja;
+      double r2 = (x[i3] - x[j3]) * (x[i3] - x[j3]) + (x[i3 + 1] - x[j3 + 1]) * (x[i3 + 1] - x[j3 + 1])
+                + (x[i3 + 2] - x[j3 + 2]) * (x[i3 + 2] - x[j3 + 2]);
+      if (r2 <= rcut2) {
+        for (int a = T.cgp[ig].first; a <= T.cgp[ig].last; ++a) {
+          int i = T.cgpatom[a];
+          if (s.iqatom[i] != 0) continue;
+          for (int b = T.cgp[jg].first; b <= T.cgp[jg].last; ++b) {
+            int j = T.cgpatom[b];
+            if (ig == jg && i >= j) continue;
+            if (s.iqatom[j] != 0) continue;
+            int lj = ljcod(s.iac[i], s.iac[j]);
+            if (s.crg[i] * s.crg[j] == 0.0 && typ(i).avdw[lj - 1] * typ(j).avdw[lj - 1] == 0.0 &&
+                typ(i).bvdw[lj - 1] * typ(j).bvdw[lj - 1] == 0.0)
+              continue;
+            if (excluded_pair(i, j)) continue;
+            nbpp.push_back({i, j, pair14(i, j) ? 3 : lj});
+          }
+        }
+      } else if (r2 <= rclrf2) {
+        lrf_update(ig, jg);
+        lrf_update(jg, ig);
+      }
+    }
+  }
+}
+
+void Oracle::nbpwlist_lrf() {
+  nbpw.clear();
+  const auto& T = s.top;
+  const double rcut2 = s.inp.rcpw * s.inp.rcpw, rclrf2 = s.inp.rclrf * s.inp.rclrf;
+  for (int ig = 0; ig < T.ncgp_solute; ++ig) {
+    int ia = T.cgp[ig].iswitch;
+    if (s.excl[ia]) continue;
+    int i3 = 3 * ia;
+    for (int jw = 0; jw < s.nwat; ++jw) {
+      int ja = s.nat_solute + 3 * jw;
+      if (s.excl[ja]) continue;
+      int j3 = 3 * ja;
+      int jg = s.iwhich_cgp[ja];
+      double r2 = (x[i3] - x[j3]) * (x[i3] - x[j3]) + (x[i3 + 1] - x[j3 + 1]) * (x[i3 + 1] - x[j3 + 1])
+                + (x[i3 + 2] - x[j3 + 2]) * (x[i3 + 2] - x[j3 + 2]);
+      if (r2 <= rcut2) {
+        for (int a = T.cgp[ig].first; a <= T.cgp[ig].last; ++a) {
+          int i = T.cgpatom[a];
+          if (s.iqatom[i] != 0) continue;
+          for (int k = 0; k < 3; ++k) {
+            int j = ja + k;
+            nbpw.push_back({i, j, ljcod(s.iac[i], s.iac[j])});
+          }
+        }
+      } else if (r2 <= rclrf2) {
+        lrf_update(ig, jg);
+        lrf_update(jg, ig);
+      }
+    }
+  }
+}
+
+void Oracle::nbqplist() {
+  if (s.nqat == 0) return;
+  throw std::runtime_error("Q-atom lists not yet restated");
+}
+void Oracle::nbqwlist() {
+  if (s.nqat == 0) return;
+  throw std::runtime_error("Q-atom lists not yet restated");
+}
+void Oracle::nonbond_pp() {
+  for (const NB& p : nbpp) {
+    int i = p.i, j = p.j, i3 = 3 * i, j3 = 3 * j;
+    double dx1 = x[j3] - x[i3], dx2 = x[j3 + 1] - x[i3 + 1], dx3 = x[j3 + 2] - x[i3 + 2];
+    double r2 = 1.0 / (dx1 * dx1 + dx2 * dx2 + dx3 * dx3);
+    double r = std::sqrt(r2);
+    double r6 = r2 * r2 * r2;
+    double Vel = s.crg[i] * s.crg[j] * r;
+    if (p.ljcod == 3) Vel = Vel * s.top.el14_scale;
+    double Va = typ(i).avdw[p.ljcod - 1] * typ(j).avdw[p.ljcod - 1] * r6 * r6;
+    double Vb = typ(i).bvdw[p.ljcod - 1] * typ(j).bvdw[p.ljcod - 1] * r6;
+    double dv = r2 * (-Vel - 12.0 * Va + 6.0 * Vb);
+    d[i3] = d[i3] - dv * dx1; d[i3 + 1] = d[i3 + 1] - dv * dx2; d[i3 + 2] = d[i3 + 2] - dv * dx3;
+    d[j3] = d[j3] + dv * dx1; d[j3 + 1] = d[j3 + 1] + dv * dx2; d[j3 + 2] = d[j3 + 2] + dv * dx3;
+    E.pp_el = E.pp_el + Vel;
+    E.pp_vdw = E.pp_vdw + Va - Vb;
+  }
+}
+void Oracle::nonbond_pw() {
+  for (const NB& p : nbpw) {
+    int i = p.i, j = p.j, i3 = 3 * i, j3 = 3 * j;
+    double dx1 = x[j3] - x[i3], dx2 = x[j3 + 1] - x[i3 + 1], dx3 = x[j3 + 2] - x[i3 + 2];
+    double r2 = 1.0 / (dx1 * dx1 + dx2 * dx2 + dx3 * dx3);
+    double r = std::sqrt(r2);
+    double r6 = r2 * r2 * r2;
+    double Vel = s.crg[i] * s.crg[j] * r;
+    double Va = typ(i).avdw[p.ljcod - 1] * typ(j).avdw[p.ljcod - 1] * r6 * r6;
+    double Vb = typ(i).bvdw[p.ljcod - 1] * typ(j).bvdw[p.ljcod - 1] * r6;
+    double dv = r2 * (-Vel - 12.0 * Va + 6.0 * Vb);
+    d[i3] = d[i3] - dv * dx1; d[i3 + 1] = d[i3 + 1] - dv * dx2; d[i3 + 2] = d[i3 + 2] - dv * dx3;
+    d[j3] = d[j3] + dv * dx1; d[j3 + 1] = d[j3 + 1] + dv * dx2; d[j3 + 2] = d[j3 + 2] + dv * dx3;
+    E.pw_el = E.pw_el + Vel;
+    E.pw_vdw = E.pw_vdw + Va - Vb;
+  }
+}
+void Oracle::nonbond_qp() {}
+void Oracle::nonbond_qw_spc() {}
+void Oracle::nonbond_qq() {}
+double Oracle::bond(int a, int b) {
+  if (a < b) throw std::runtime_error("solute bonds not yet restated");
+  return 0.0;
+}
+double Oracle::angle(int a, int b) {
+  if (a < b) throw std::runtime_error("solute angles not yet restated");
+  return 0.0;
+}
+double Oracle::torsion(int a, int b) {
+  if (a < b) throw std::runtime_error("solute torsions not yet restated");
+  return 0.0;
+}
+double Oracle::improper(int a, int b) {
+  if (a < b) throw std::runtime_error("solute impropers not yet restated");
+  return 0.0;
+}
+
+thread_local std::string g_err;
+
+}  // namespace
+
+// ------------------------------------------------------------------ C API
+extern "C" {
+
+void* oracle_create(const char* inp) {
+  try {
+    return new Oracle(inp);
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+const char* oracle_last_error() { return g_err.c_str(); }
+void oracle_destroy(void* h) { delete static_cast<Oracle*>(h); }
+int oracle_natom(void* h) { return static_cast<Oracle*>(h)->s.natom; }
+int oracle_nstates(void* h) { return static_cast<Oracle*>(h)->s.nstates; }
+
+// run n steps; returns 0 on success
+int oracle_run(void* h, int n) {
+  auto* o = static_cast<Oracle*>(h);
+  try {
+    for (int k = 0; k < n; ++k)
+      if (!o->step()) { g_err = o->err; return 1; }
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 2;
+  }
+  return 0;
+}
+// the final list build + energy of md_run (src/md.f90:6690-6700)
+int oracle_final_energy(void* h) {
+  auto* o = static_cast<Oracle*>(h);
+  try {
+    o->make_pair_lists();
+    o->pot_energy();
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 2;
+  }
+  return 0;
+}
+void oracle_get_xv(void* h, double* x, double* v) {
+  auto* o = static_cast<Oracle*>(h);
+  std::memcpy(x, o->x.data(), 8 * o->x.size());
+  std::memcpy(v, o->v.data(), 8 * o->v.size());
+}
+void oracle_get_d(void* h, double* d) {
+  auto* o = static_cast<Oracle*>(h);
+  std::memcpy(d, o->d.data(), 8 * o->d.size());
+}
+// energies in write_out order; returns the count written (<= cap)
+int oracle_get_energies(void* h, double* e, int cap) {
+  auto* o = static_cast<Oracle*>(h);
+  const Energies& E = o->E;
+  double v[] = {E.pp_el, E.pp_vdw, E.p_bond, E.p_angle, E.p_torsion, E.p_improper,
+                E.ww_el, E.ww_vdw, E.w_bond, E.w_angle, E.w_torsion, E.w_improper,
+                E.pw_el, E.pw_vdw, E.lrf,
+                E.qx_el, E.qx_vdw, E.q_bond, E.q_angle, E.q_torsion, E.q_improper,
+                E.rst_total, E.rst_fix, E.rst_radial, E.rst_wpol, E.rst_shell, E.rst_protein,
+                E.potential + E.kinetic, E.potential, E.kinetic};
+  int n = (int)(sizeof(v) / sizeof(v[0]));
+  if (n > cap) n = cap;
+  std::memcpy(e, v, 8 * n);
+  return n;
+}
+// per-state Q energies: qq_el qq_vdw qp_el qp_vdw qw_el qw_vdw total (7 per state)
+int oracle_get_qenergies(void* h, double* e, int cap) {
+  auto* o = static_cast<Oracle*>(h);
+  int k = 0;
+  for (auto& q : o->EQ) {
+    double v[] = {q.qq_el, q.qq_vdw, q.qp_el, q.qp_vdw, q.qw_el, q.qw_vdw, q.total};
+    for (double z : v) if (k < cap) e[k++] = z;
+  }
+  return k;
+}
+// list sizes: nbpp nbpw nbww nbqp nbqw
+void oracle_list_sizes(void* h, long* n) {
+  auto* o = static_cast<Oracle*>(h);
+  n[0] = (long)o->nbpp.size(); n[1] = (long)o->nbpw.size(); n[2] = (long)o->nbww.size();
+  n[3] = (long)o->nbqp.size(); n[4] = (long)o->nbqw.size();
+}
+// copy list `which` (0 pp, 1 pw, 2 ww) as (i, j, ljcod) int triples
+void oracle_get_list(void* h, int which, int* out) {
+  auto* o = static_cast<Oracle*>(h);
+  const std::vector<NB>& L = which == 0 ? o->nbpp : (which == 1 ? o->nbpw : o->nbww);
+  for (size_t k = 0; k < L.size(); ++k) {
+    out[3 * k] = L[k].i; out[3 * k + 1] = L[k].j; out[3 * k + 2] = L[k].ljcod;
+  }
+}
+// LRF multipoles: 43 doubles per charge group (centre, phi0, phi1, phi2, phi3)
+void oracle_get_lrf(void* h, double* out) {
+  auto* o = static_cast<Oracle*>(h);
+  for (size_t g = 0; g < o->lrf.size(); ++g) std::memcpy(out + 43 * g, &o->lrf[g], 43 * 8);
+}
+double oracle_temperature(void* h) { return static_cast<Oracle*>(h)->Temp; }
+void oracle_theta_corr(void* h, double* out) {
+  auto* o = static_cast<Oracle*>(h);
+  for (size_t k = 0; k < o->s.wshell.size(); ++k) out[k] = o->s.wshell[k].theta_corr;
+}
+int oracle_nshells(void* h) { return (int)static_cast<Oracle*>(h)->s.wshell.size(); }
+
+}  // extern "C"

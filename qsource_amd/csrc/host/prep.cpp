@@ -1,0 +1,445 @@
+// qdyn's initialisation sequence for a spherical-boundary run
+// (src/qdyn.f90:108-132): topology conversions, restrained shell, restart or
+// topology coordinates, FEP charges, solvent parameters and water shells,
+// inverse masses, charge scaling, SHAKE molecules and degrees of freedom,
+// Q-atom connectivity and Q-Q lists, topology shrinking, and Maxwell
+// velocities with initial SHAKE.  The result is a Setup that both the CPU
+// oracle and the GPU engine start from.
+#include "qsys.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+
+namespace qsys {
+
+static const double kPi = 4.0 * std::atan(1.0);        // src/md.f90:527
+static const double kDeg2Rad = kPi / 180.0;             // src/md.f90:528
+
+static std::string dir_of(const std::string& p) {
+  size_t k = p.find_last_of('/');
+  return k == std::string::npos ? std::string(".") : p.substr(0, k);
+}
+static std::string join(const std::string& d, const std::string& f) {
+  if (f.empty() || f[0] == '/') return f;
+  return d + "/" + f;
+}
+
+// randm (src/md.f90:17543-17577)
+double Randm::next(int& ig) {
+  const int m = 100000000, m1 = 10000, mult = 31415821;
+  if (!started) { started = true; irand = std::abs(ig) % m; }
+  int irandh = irand / m1, irandl = irand % m1;
+  int multh = mult / m1, multl = mult % m1;
+  irand = ((irandh * multl + irandl * multh) % m1) * m1 + irandl * multl;
+  irand = (irand + 1) % m;
+  double r = (double)(irand / 10) * 10 / (double)m;
+  if (r <= 0.0 || r > 1.0) r = 0.0;
+  ig = irand;
+  return r;
+}
+
+// shake (src/md.f90:17581-17679), used here for initial_shaking only
+int shake_cpu(const Setup& s, const std::vector<double>& xx, std::vector<double>& x) {
+  int total = 0;
+  for (const ShakeMol& m : s.shake_mol) {
+    int nc = (int)m.bond.size();
+    std::vector<char> ready(nc, 0);
+    int nits = 0;
+    for (;;) {
+      for (int ic = 0; ic < nc; ++ic) {
+        if (ready[ic]) continue;
+        int i = m.bond[ic].i, j = m.bond[ic].j;
+        int i3 = 3 * i, j3 = 3 * j;
+        double xij0 = x[i3] - x[j3], xij1 = x[i3+1] - x[j3+1], xij2_ = x[i3+2] - x[j3+2];
+        double xij2 = xij0 * xij0 + xij1 * xij1 + xij2_ * xij2_;
+        double diff = m.bond[ic].dist2 - xij2;
+        if (std::fabs(diff) < kShakeTol * m.bond[ic].dist2) ready[ic] = 1;
+        double xx0 = xx[i3] - xx[j3], xx1 = xx[i3+1] - xx[j3+1], xx2 = xx[i3+2] - xx[j3+2];
+        double scp = xij0 * xx0 + xij1 * xx1 + xij2_ * xx2;
+        double corr = diff / (2.0 * scp * (s.winv[i] + s.winv[j]));
+        x[i3]   = x[i3]   + xx0 * corr * s.winv[i];
+        x[i3+1] = x[i3+1] + xx1 * corr * s.winv[i];
+        x[i3+2] = x[i3+2] + xx2 * corr * s.winv[i];
+        x[j3]   = x[j3]   - xx0 * corr * s.winv[j];
+        x[j3+1] = x[j3+1] - xx1 * corr * s.winv[j];
+        x[j3+2] = x[j3+2] - xx2 * corr * s.winv[j];
+      }
+      ++nits;
+      bool all = true;
+      for (int ic = 0; ic < nc; ++ic) all = all && ready[ic];
+      if (all) break;
+      if (nits >= kShakeMaxIter) return -1;
+    }
+    total += nits;
+  }
+  return total;
+}
+
+Setup prepare(const std::string& input_path) {
+  Setup s;
+  const std::string dir = dir_of(input_path);
+  s.inp = read_input(input_path);
+  MDInput& in = s.inp;
+  s.top = read_topology(join(dir, in.top_file));
+  Topology& t = s.top;
+  if (t.solvent_type != 0) throw std::runtime_error("only SPC-type solvent (solvent_type 0) is supported");
+  if (t.ivdw_rule != 1) throw std::runtime_error("only the geometric vdW rule is supported");
+  if (t.iuse_switch_atom != 1) throw std::runtime_error("only switching-atom charge-group cut-offs are supported");
+  s.natom = t.nat_pro;
+  s.nat_solute = t.nat_solute;
+  s.solv_atom = t.solv_atom;
+  if (s.natom > s.nat_solute && s.solv_atom != 3) throw std::runtime_error("solvent must be 3-atom water");
+  s.nwat = (s.natom - s.nat_solute) / s.solv_atom;
+  s.nmol = (int)t.istart_mol.size();
+  t.istart_mol.push_back(s.natom);
+
+  // topology(): unit conversions and LJ code table (src/md.f90:17819-17905)
+  for (auto& a : t.anglib) a.ang0 = kDeg2Rad * a.ang0;
+  for (auto& q : t.torlib) { q.paths = 1.0 / q.paths; q.deltor = kDeg2Rad * q.deltor; }
+  for (auto& q : t.implib) q.imp0 = kDeg2Rad * q.imp0;
+  s.natyps = (int)t.types.size();
+  s.ljcod.assign((size_t)s.natyps * s.natyps, 1);
+  for (auto& p : t.lj2) {
+    s.ljcod[p.first * s.natyps + p.second] = 2;
+    s.ljcod[p.second * s.natyps + p.first] = 2;
+  }
+  s.iac = t.iac;
+  s.heavy.resize(s.natom);
+  for (int i = 0; i < s.natom; ++i) s.heavy[i] = !(t.types[s.iac[i]].mass < 4.0);
+  s.excl = t.excl;
+  s.dt = 0.020462 * in.stepsize;
+  s.tau_t = 0.020462 * in.tau_t;
+  if (in.temp0 <= 0) throw std::runtime_error("no dynamics at zero temperature");
+  if (s.tau_t < s.dt) throw std::runtime_error("bath coupling time shorter than the step");
+
+  // prep_coord: restrained shell on topology coordinates (src/md.f90:16595-16622)
+  s.xtop = t.xtop;
+  s.rexcl_i = in.have_shell_radius ? in.shell_radius : 0.85;
+  if (s.rexcl_i < 0) throw std::runtime_error("inner radius of restrained shell must be >= 0");
+  s.shell.assign(s.natom, 0);
+  if (s.rexcl_i > t.rexcl_o) throw std::runtime_error("inner radius of restrained shell must be < exclusion radius");
+  if (s.rexcl_i <= 1.0) s.rexcl_i = s.rexcl_i * t.rexcl_o;
+  {
+    double rin2 = s.rexcl_i * s.rexcl_i;
+    for (int ig = 0; ig < t.ncgp_solute; ++ig) {
+      int sw = t.cgp[ig].iswitch;
+      if (!s.excl[sw] && s.heavy[sw]) {
+        int i3 = 3 * sw;
+        double r2 = (s.xtop[i3] - t.xpcent[0]) * (s.xtop[i3] - t.xpcent[0])
+                  + (s.xtop[i3+1] - t.xpcent[1]) * (s.xtop[i3+1] - t.xpcent[1])
+                  + (s.xtop[i3+2] - t.xpcent[2]) * (s.xtop[i3+2] - t.xpcent[2]);
+        if (r2 > rin2)
+          for (int a = t.cgp[ig].first; a <= t.cgp[ig].last; ++a) s.shell[t.cgpatom[a]] = 1;
+      }
+    }
+  }
+  std::vector<double> theta_corr_rst;
+  bool restart = !in.restart_file.empty();
+  if (restart) {
+    read_restart(join(dir, in.restart_file), s.natom, s.x, s.v, &theta_corr_rst);
+  } else {
+    s.x = s.xtop;
+    s.v.assign(3 * s.natom, 0.0);
+  }
+
+  // get_fep: Q atoms and their per-state charges (src/md.f90:2884-2990)
+  s.iqatom.assign(s.natom, 0);
+  s.crg = t.crg;
+  if (!in.fep_file.empty()) {
+    FepData f = read_fep(join(dir, in.fep_file), s.natom);
+    s.nstates = f.nstates;
+    if (!in.lambdas.empty()) {
+      if ((int)in.lambdas.size() != s.nstates)
+        throw std::runtime_error("number of lambdas differs from FEP states");
+      s.lambda = in.lambdas;
+    } else {
+      if (s.nstates != 1) throw std::runtime_error("[lambdas] required for more than one state");
+      s.lambda = {1.0};
+    }
+    s.nqat = (int)f.iqseq.size();
+    s.iqseq = f.iqseq;
+    for (int iq = 0; iq < s.nqat; ++iq) {
+      if (s.iqseq[iq] >= s.nat_solute) throw std::runtime_error("Q atoms must be solute atoms");
+      s.iqatom[s.iqseq[iq]] = iq + 1;
+    }
+    s.qcrg.assign((size_t)s.nqat * s.nstates, 0.0);
+    for (int iq = 0; iq < s.nqat; ++iq)
+      for (int st = 0; st < s.nstates; ++st)
+        s.qcrg[iq * s.nstates + st] = f.have_charges ? f.qcrg[iq][st] : t.crg[s.iqseq[iq]];
+  } else if (!in.lambdas.empty()) {
+    throw std::runtime_error("[lambdas] given without an FEP file");
+  }
+
+  // prep_sim: solvent parameters from the first water (src/md.f90:16704-16713)
+  if (s.nwat > 0) {
+    for (int iw = 0; iw < 3; ++iw) {
+      int a = s.nat_solute + iw;
+      s.chg_solv[iw] = s.crg[a];
+      for (int k = 0; k < 2; ++k) {
+        s.alj_solv[iw][k] = t.types[s.iac[a]].avdw[k];
+        s.blj_solv[iw][k] = t.types[s.iac[a]].bvdw[k];
+      }
+    }
+  }
+  // wat_sphere (src/md.f90:18106-18185)
+  s.rwat = t.rwat;
+  if (in.rwat_in > 0) s.rwat = in.rwat_in;
+  {
+    double crgtot = 0, crgexcl = 0;
+    for (int i = 0; i < s.nat_solute; ++i) {
+      if (!s.excl[i]) { if (s.iqatom[i] == 0) crgtot = crgtot + s.crg[i]; }
+      else crgexcl = crgexcl + s.crg[i];
+    }
+    double crgqtot = 0;
+    for (int iq = 0; iq < s.nqat; ++iq)
+      for (int st = 0; st < s.nstates; ++st) {
+        crgtot = crgtot + s.qcrg[iq * s.nstates + st] * s.lambda[st];
+        crgqtot = crgqtot + s.qcrg[iq * s.nstates + st] * s.lambda[st];
+      }
+    if (!in.wpol_born) crgqtot = 0;
+    s.crgtot = crgtot;
+    s.crgqtot = crgqtot;
+  }
+  s.fk_wsphere = in.fk_wsphere == -1 ? 60.0 : in.fk_wsphere;
+  s.fkwpol = in.fkwpol == -1 ? 20.0 : in.fkwpol;
+  s.dwmz = in.dwmz == -1 ? 0.26 * std::exp(-0.19 * (s.rwat - 15.0)) + 0.74 : in.dwmz;
+  s.awmz = in.awmz == -1 ? 0.2 / (1.0 + std::exp(0.4 * (s.rwat - 25.0))) + 0.3 : in.awmz;
+  s.wpol_restr = in.wpol_restr;
+  // wat_shells (src/md.f90:18189-18368)
+  if (s.nwat > 0 && s.wpol_restr) {
+    int bndcodw = t.bnd.back().cod, angcodw = t.ang.back().cod;
+    s.mu_w = -s.chg_solv[0] * t.bondlib[bndcodw - 1].b0 * std::cos(t.anglib[angcodw - 1].ang0 / 2);
+    double drs = kWpolrLayer / kDrout;
+    int nsh = (int)(-0.5 + std::sqrt(2 * drs + 0.25));
+    s.wshell.resize(nsh);
+    bool tc_ok = restart && (int)theta_corr_rst.size() == nsh;
+    double rout = s.rwat;
+    int nmax = 0;
+    for (int is = 0; is < nsh; ++is) {
+      WaterShell& w = s.wshell[is];
+      w.avtheta = 0; w.avn_insh = 0; w.n_insh = 0;
+      w.theta_corr = tc_ok ? theta_corr_rst[is] : 0.0;
+      w.rout = rout;
+      double dr = kDrout * (is + 1);
+      double ri = rout - dr;
+      w.dr = dr;
+      double vshell = rout * rout * rout - ri * ri * ri;
+      int n_insh = (int)(4 * kPi / 3 * vshell * kRhoWat);
+      if (n_insh > nmax) nmax = n_insh;
+      double rshell = std::pow(0.5 * (rout * rout * rout + ri * ri * ri), 1.0 / 3.0);
+      w.cstb = s.crgqtot * 0.98750 / (kRhoWat * s.mu_w * 4.0 * kPi * rshell * rshell);
+      rout = rout - dr;
+    }
+    s.n_max_insh = (int)(nmax * 1.5);
+  }
+  s.iwhich_cgp.assign(s.natom, 0);
+  for (int ig = 0; ig < (int)t.cgp.size(); ++ig)
+    for (int a = t.cgp[ig].first; a <= t.cgp[ig].last; ++a) s.iwhich_cgp[t.cgpatom[a]] = ig;
+  s.winv.resize(s.natom);
+  for (int i = 0; i < s.natom; ++i) s.winv[i] = 1.0 / t.types[s.iac[i]].mass;
+  const double sq = std::sqrt(t.coulomb_constant);
+  for (auto& c : s.crg) c = c * sq;
+  if (s.nwat > 0) for (auto& c : s.chg_solv) c = c * sq;
+  for (auto& c : s.qcrg) c = c * sq;
+
+  // init_shake (src/md.f90:4242-4426)
+  std::vector<Bond> bnd = t.bnd;
+  std::vector<Angle> ang = t.ang;
+  {
+    const auto& ist = t.istart_mol;
+    std::vector<ShakeMol> mols(s.nmol);
+    double exclshk = 0;
+    int mol = -1;
+    for (auto& b : bnd) {
+      int ia = b.i, ja = b.j;
+      while (ia >= ist[mol + 1]) ++mol;
+      if (b.cod == 0) continue;
+      bool hyd = in.shake_hydrogens && (!s.heavy[ia] || !s.heavy[ja]);
+      if (hyd || (in.shake_solute && ia < s.nat_solute) || (in.shake_solvent && ia >= s.nat_solute)) {
+        if (s.excl[ia]) exclshk += 0.5;
+        if (s.excl[ja]) exclshk += 0.5;
+        mols[mol].bond.push_back({ia, ja, t.bondlib[b.cod - 1].b0 * t.bondlib[b.cod - 1].b0});
+        b.cod = -1;
+      }
+    }
+    int solute_shake = 0;
+    for (int m = 0; m < s.nmol - s.nwat; ++m) solute_shake += (int)mols[m].bond.size();
+    // stable compaction of molecules with constraints (src/md.f90:4363-4378)
+    int trg = 0, src = 1;
+    while (src < s.nmol) {
+      if (mols[trg].bond.empty()) {
+        mols[trg] = mols[src];
+        mols[src].bond.clear();
+        ++src;
+      } else {
+        ++trg;
+        if (trg == src) ++src;
+      }
+    }
+    int nmols = s.nmol > 0 ? trg + 1 : 0;
+    mols.resize(nmols);
+    s.shake_mol = mols;
+    int nc = 0;
+    for (auto& m : s.shake_mol) nc += (int)m.bond.size();
+    s.shake_constraints = nc;
+    s.exclshk = exclshk;
+    int nexats = 0;
+    for (int i = 0; i < s.natom; ++i) nexats += s.excl[i] ? 1 : 0;
+    s.ndegf = 3 * s.natom - nc;
+    s.ndegfree = (int)(s.ndegf - 3 * t.nexats + exclshk);
+    s.ndegf_solvent = s.ndegf - 3 * s.nat_solute + solute_shake;
+    s.ndegf_solute = s.ndegf - s.ndegf_solvent;
+    s.ndegfree_solvent = 3 * (s.natom - s.nat_solute) - (nc - solute_shake);
+    s.ndegfree_solute = s.ndegfree - s.ndegfree_solvent;
+    s.separate_scaling = in.separate_scaling;
+    if (s.ndegfree_solvent * s.ndegfree_solute == 0) {
+      s.detail_temps = false;
+      s.separate_scaling = false;
+    } else {
+      s.detail_temps = true;
+    }
+    // clear angles whose i-k pair is constrained
+    for (auto& m : s.shake_mol)
+      for (auto& c : m.bond)
+        for (auto& a : ang)
+          if ((a.i == c.i && a.k == c.j) || (a.i == c.j && a.k == c.i)) { a.cod = 0; break; }
+  }
+
+  // make_nbqqlist (src/md.f90:9582-9804): Q connectivity over solute bonds
+  if (s.nqat > 0) {
+    const int ns = s.nat_solute, nq = s.nqat, nst = s.nstates;
+    s.qconn.assign((size_t)nst * nq * ns, 9);
+    std::vector<std::vector<int>> nbr(ns);
+    for (int b = 0; b < t.nbonds_solute; ++b) {
+      if (bnd[b].cod == 0) continue;
+      nbr[bnd[b].i].push_back(bnd[b].j);
+      nbr[bnd[b].j].push_back(bnd[b].i);
+    }
+    auto qc = [&](int st, int iq, int a) -> int& { return s.qconn[((size_t)st * nq + iq) * ns + a]; };
+    for (int iq = 0; iq < nq; ++iq)
+      for (int st = 0; st < nst; ++st) {
+        int o = s.iqseq[iq];
+        qc(st, iq, o) = 1;
+        std::vector<int> frontier{o};
+        for (int lvl = 2; lvl <= 4; ++lvl) {
+          std::vector<int> nxt;
+          for (int u : frontier)
+            for (int w : nbr[u])
+              if (qc(st, iq, w) > lvl) { qc(st, iq, w) = lvl; nxt.push_back(w); }
+          frontier = nxt;
+        }
+      }
+    s.nbqq.assign(nst, {});
+    for (int iq = 0; iq < nq - 1; ++iq)
+      for (int jq = iq + 1; jq < nq; ++jq) {
+        int j = s.iqseq[jq];
+        for (int st = 0; st < nst; ++st) {
+          int c = qc(st, iq, j);
+          if (c > 3) {
+            NbqPair p{iq, j, jq, 0, 1.0};
+            p.ljcod = (c == 4) ? 3 : s.ljcod[s.iac[s.iqseq[iq]] * s.natyps + s.iac[j]];
+            s.nbqq[st].push_back(p);
+          }
+        }
+      }
+    for (int j = 0; j < ns; ++j) {
+      if (s.iqatom[j] > 0) continue;
+      bool any = false;
+      for (int st = 0; st < nst && !any; ++st)
+        for (int iq = 0; iq < nq && !any; ++iq) any = qc(st, iq, j) <= 3;
+      if (!any) continue;
+      for (int iq = 0; iq < nq; ++iq)
+        for (int st = 0; st < nst; ++st) {
+          int c = qc(st, iq, j);
+          if (c >= 4) {
+            NbqPair p{iq, j, -1, 0, 1.0};
+            p.ljcod = (c == 4) ? 3 : s.ljcod[s.iac[s.iqseq[iq]] * s.natyps + s.iac[j]];
+            s.nbqq[st].push_back(p);
+          }
+        }
+    }
+  }
+
+  // shrink_topology (src/md.f90:17683-17783): swap-with-last removal
+  {
+    int nb = (int)bnd.size(), nbs = t.nbonds_solute;
+    int i = 0;
+    while (i < nb) {
+      if (bnd[i].cod <= 0) {
+        if (i < nbs) { bnd[i] = bnd[nbs - 1]; bnd[nbs - 1] = bnd[nb - 1]; --nbs; }
+        else bnd[i] = bnd[nb - 1];
+        --nb;
+        continue;
+      }
+      ++i;
+    }
+    bnd.resize(nb);
+    s.bnd = bnd; s.nbonds_solute = nbs;
+    int na = (int)ang.size(), nas = t.nangles_solute;
+    i = 0;
+    while (i < na) {
+      if (ang[i].cod == 0) {
+        if (i < nas) { ang[i] = ang[nas - 1]; ang[nas - 1] = ang[na - 1]; --nas; }
+        else ang[i] = ang[na - 1];
+        --na;
+        continue;
+      }
+      ++i;
+    }
+    ang.resize(na);
+    s.ang = ang; s.nangles_solute = nas;
+    auto shrink4 = [&](std::vector<Torsion> v, int nsol, std::vector<Torsion>& out, int& nsol_out) {
+      int n = (int)v.size();
+      int k = 0;
+      while (k < n) {
+        bool all_ex = in.exclude_bonded && s.excl[v[k].i] && s.excl[v[k].j] && s.excl[v[k].k] && s.excl[v[k].l];
+        if (all_ex || v[k].cod == 0) {
+          if (k < nsol) { v[k] = v[nsol - 1]; v[nsol - 1] = v[n - 1]; --nsol; }
+          else v[k] = v[n - 1];
+          --n;
+          continue;
+        }
+        ++k;
+      }
+      v.resize(n);
+      out = v; nsol_out = nsol;
+    };
+    shrink4(t.tor, t.ntors_solute, s.tor, s.ntors_solute);
+    shrink4(t.imp, t.nimps_solute, s.imp, s.nimps_solute);
+  }
+
+  // maxwell, initial_shaking, stop_cm_translation (src/qdyn.f90:128-132)
+  if (in.iseed > 0) {
+    Randm rnd;
+    int ig = in.iseed;
+    double kT = kBoltz * in.tmaxw;
+    for (int i = 0; i < s.natom; ++i) {
+      double sd = std::sqrt(kT / t.types[s.iac[i]].mass);
+      for (int j = 0; j < 3; ++j) {
+        double a = 0.0;
+        for (int k = 0; k < 12; ++k) a = a + rnd.next(ig);
+        s.v[3 * i + j] = (a - 6.0) * sd + 0.0;
+      }
+    }
+    {
+      std::vector<double> xx = s.x;
+      if (shake_cpu(s, xx, s.x) < 0) throw std::runtime_error("initial shake failure");
+      for (int k = 0; k < 3 * s.natom; ++k) xx[k] = s.x[k] - s.dt * s.v[k];
+      if (shake_cpu(s, s.x, xx) < 0) throw std::runtime_error("initial shake failure");
+      for (int k = 0; k < 3 * s.natom; ++k) s.v[k] = (s.x[k] - xx[k]) / s.dt;
+    }
+    double totmass = 0, vcm[3] = {0, 0, 0};
+    for (int i = 0; i < s.natom; ++i) {
+      double rm = t.types[s.iac[i]].mass;
+      totmass = totmass + rm;
+      for (int j = 0; j < 3; ++j) vcm[j] = vcm[j] + rm * s.v[3 * i + j];
+    }
+    for (int j = 0; j < 3; ++j) vcm[j] = vcm[j] / totmass;
+    for (int i = 0; i < s.natom; ++i)
+      for (int j = 0; j < 3; ++j) s.v[3 * i + j] = s.v[3 * i + j] - vcm[j];
+  }
+  return s;
+}
+
+}  // namespace qsys

@@ -1,0 +1,188 @@
+// Host-side description of one qdyn simulation: the topology, MD input and
+// FEP data as read from qdyn's own files, and the prepared per-run state
+// (scaled charges, SHAKE molecules, degrees of freedom, water shells, Q-atom
+// lists) that qdyn's initialisation builds before md_run.
+//
+// Layout conventions: atom, charge-group and Q-atom indices are 0-based;
+// coordinates are AoS FP64 [x0,y0,z0,x1,...] of length 3N exactly as qdyn's
+// x(:) (src/md.f90:309-313); the gradient d = dV/dx (force = -d).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace qsys {
+
+constexpr int kMaxNbrRange = 25;   // src/topo.f90:36
+constexpr int kMaxStates = 10;     // src/qatom.f90:39-41
+constexpr double kBoltz = 0.001986;        // src/md.f90:54
+constexpr double kRhoWat = 0.0335;         // src/md.f90:53
+constexpr double kShakeTol = 0.0001;       // src/md.f90:460
+constexpr int kShakeMaxIter = 1000;        // src/md.f90:461
+constexpr double kWpolrLayer = 3.0001;     // src/md.f90:190
+constexpr double kDrout = 0.5;             // src/md.f90:191
+constexpr int kItdisUpdate = 100;          // src/md.f90:189
+constexpr double kQrealEps = 1.0e-14;      // src/sizes.f90:48
+
+struct AtomType {
+  double mass;
+  double avdw[3];
+  double bvdw[3];
+};
+
+struct Bond { int i, j, cod; };
+struct Angle { int i, j, k, cod; };
+struct Torsion { int i, j, k, l, cod; };
+struct BondLib { double fk, b0; };
+struct AngLib { double fk, ang0, ureyfk, ureyr0; };
+struct TorLib { double fk, rmult, deltor, paths; };
+struct ImpLib { double fk, imp0; };
+struct Cgp { int iswitch, first, last; };  // first..last index cgpatom
+
+// Contents of a qdyn topology file (src/topo.f90:515-1074), 0-based.
+struct Topology {
+  double version = 0;
+  int ff_type = 1;
+  int nat_pro = 0, nat_solute = 0, solv_atom = 3;
+  std::vector<double> xtop;
+  std::vector<int> iac;  // 0-based type index
+  std::vector<Bond> bnd; int nbonds_solute = 0;
+  std::vector<BondLib> bondlib;
+  std::vector<Angle> ang; int nangles_solute = 0;
+  std::vector<AngLib> anglib;
+  std::vector<Torsion> tor; int ntors_solute = 0;
+  std::vector<TorLib> torlib;
+  std::vector<Torsion> imp; int nimps_solute = 0;
+  std::vector<ImpLib> implib; int imp_type = 1;
+  std::vector<double> crg;
+  std::vector<Cgp> cgp; int ncgp_solute = 0; int iuse_switch_atom = 1;
+  std::vector<int> cgpatom;
+  std::vector<AtomType> types;
+  int ivdw_rule = 1;
+  double el14_scale = 0.5, coulomb_constant = 332.0;
+  std::vector<std::pair<int,int>> lj2;
+  // list14/listex: bit k-1 of row i set <=> (i, i+k) listed; rows nat_solute
+  std::vector<uint32_t> list14, listex;
+  std::vector<std::pair<int,int>> list14long, listexlong;
+  std::vector<int> res_start; int nres_solute = 0;
+  std::vector<int> istart_mol;  // 0-based start atom of each molecule
+  int solvent_type = 0;
+  bool use_pbc = false;
+  double rexcl_o = 0, rexcl_i = 0, rwat = 0;
+  double xpcent[3] = {0,0,0}, xwcent[3] = {0,0,0};
+  int nexats = 0, nexwat = 0;
+  std::vector<char> excl;
+};
+
+// The [MD]/[cut-offs]/[sphere]/[solvent]/[intervals]/[files]/[lambdas]
+// settings of a qdyn input file (src/md.f90:4453-5358) that this engine uses.
+struct MDInput {
+  int nsteps = 0;
+  double stepsize = 1.0;       // fs
+  double temp0 = 300.0;
+  double tau_t = 100.0;        // fs, converted by 0.020462 at prep
+  bool separate_scaling = true;
+  int iseed = 1;
+  bool have_tmaxw = false; double tmaxw = 0;
+  bool shake_solvent = true, shake_solute = false, shake_hydrogens = false;
+  bool use_lrf = true;
+  double rcpp = 10, rcww = 10, rcpw = 10, rcq = 99, rclrf = 99;
+  bool have_shell_radius = false; double shell_radius = 0.85;
+  double fk_pshell = 10.0, fk_fix = 200.0;
+  bool freeze = false, exclude_bonded = false;
+  double rwat_in = -1, fk_wsphere = -1, dwmz = -1, awmz = -1, fkwpol = -1;
+  bool wpol_restr = true, wpol_born = true;
+  int nbcycle = 10, iout_cycle = 10, itemp_cycle = 10, iene_cycle = 0, itrj_cycle = 0;
+  std::string top_file, restart_file, final_file, energy_file, fep_file;
+  std::vector<double> lambdas;
+  std::string thermostat = "berendsen", integrator = "leap-frog";
+};
+
+// FEP file content used by this engine: [FEP] states, [atoms] and
+// [change_charges] (src/qatom.f90:301-760).
+struct FepData {
+  int nstates = 0;
+  std::vector<int> iqseq;                 // topology atom (0-based) per Q atom
+  std::vector<std::vector<double>> qcrg;  // [nqat][nstates], unscaled
+  bool have_charges = false;
+};
+
+struct ShakeBond { int i, j; double dist2; };
+struct ShakeMol { std::vector<ShakeBond> bond; };
+
+struct WaterShell {
+  double rout, dr, cstb, avtheta, avn_insh, theta_corr;
+  int n_insh;
+};
+
+// Q-Q / Q-neighbour pair for one state (NBQ_TYPE, src/md.f90:347-352).
+struct NbqPair { int iq, j, jq, ljcod; double el_scale; };
+
+// Everything md_run needs, after initialize/topology/prep_coord/get_fep/
+// prep_sim/init_shake/make_nbqqlist/shrink_topology (src/qdyn.f90:108-132).
+struct Setup {
+  Topology top;
+  MDInput inp;
+  int natom = 0, nat_solute = 0, nwat = 0, solv_atom = 3;
+  std::vector<double> x, v, xtop, winv;
+  std::vector<double> crg;            // scaled by sqrt(coulomb_constant)
+  std::vector<int> iac;
+  std::vector<char> heavy, excl, shell;
+  std::vector<int> iwhich_cgp;
+  std::vector<int> ljcod;             // natyps*natyps, values 1 or 2
+  int natyps = 0;
+  double dt = 0, tau_t = 0;
+  // solvent
+  double chg_solv[3] = {0,0,0}, alj_solv[3][2] = {{0}}, blj_solv[3][2] = {{0}};
+  double rwat = 0, fk_wsphere = 0, dwmz = 0, awmz = 0, fkwpol = 0;
+  bool wpol_restr = true;
+  double crgtot = 0, crgqtot = 0, mu_w = 0;
+  std::vector<WaterShell> wshell; int n_max_insh = 0;
+  double rexcl_i = 0;
+  // bonded terms after shrink_topology
+  std::vector<Bond> bnd; int nbonds_solute = 0;
+  std::vector<Angle> ang; int nangles_solute = 0;
+  std::vector<Torsion> tor; int ntors_solute = 0;
+  std::vector<Torsion> imp; int nimps_solute = 0;
+  // SHAKE
+  std::vector<ShakeMol> shake_mol; int shake_constraints = 0; int nmol = 0;
+  int ndegf = 0, ndegfree = 0, ndegf_solute = 0, ndegf_solvent = 0;
+  int ndegfree_solute = 0, ndegfree_solvent = 0;
+  double exclshk = 0;
+  bool detail_temps = false, separate_scaling = true;
+  // Q atoms
+  int nqat = 0, nstates = 0;
+  std::vector<int> iqseq, iqatom;     // iqatom: Q index+1 per atom (0 = not Q)
+  std::vector<double> qcrg;           // [iq*nstates+s], scaled
+  std::vector<double> lambda;
+  std::vector<int> qconn;             // [s][iq][atom] connectivity (1..9)
+  std::vector<std::vector<NbqPair>> nbqq;  // per state
+};
+
+// Readers.  Each throws std::runtime_error with a message on bad input.
+Topology read_topology(const std::string& path);
+MDInput read_input(const std::string& path);
+FepData read_fep(const std::string& path, int natom);
+bool read_restart(const std::string& path, int natom, std::vector<double>& x,
+                  std::vector<double>& v, std::vector<double>* theta_corr);
+void write_restart(const std::string& path, const std::vector<double>& x,
+                   const std::vector<double>& v, const std::vector<double>* theta_corr);
+
+// qdyn's integer LCG (randm, src/md.f90:17543-17577) with its saved state.
+struct Randm {
+  int irand = 0; bool started = false;
+  double next(int& ig);
+};
+
+// Build the Setup from an input file (paths in it are relative to its
+// directory).  Runs the whole qdyn initialisation sequence, including
+// Maxwell velocities, initial SHAKE and centre-of-mass stop when the input
+// asks for them (those call the engine-independent CPU SHAKE below).
+Setup prepare(const std::string& input_path);
+
+// Plain CPU SHAKE used only at initialisation (initial_shaking,
+// src/md.f90:4430-4450); returns total iterations or -1 on failure.
+int shake_cpu(const Setup& s, const std::vector<double>& xx, std::vector<double>& x);
+
+}  // namespace qsys

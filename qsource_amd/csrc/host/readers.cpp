@@ -1,0 +1,615 @@
+// Readers for qdyn's topology, input, FEP and restart files.
+//
+// The topology reader follows the record structure of topo_read
+// (src/topo.f90:515-1074): every Fortran READ starts at a new record (line)
+// and a list-directed READ of n items continues over as many records as it
+// needs, discarding the rest of the last one.  The input reader follows the
+// sectioned key/value format of prmfile (src/prmfile.f90:834-963) and the
+// keys and defaults of initialize (src/md.f90:4453-5358).
+#include "qsys.hpp"
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <stdexcept>
+
+namespace qsys {
+namespace {
+
+struct Records {
+  std::vector<std::string> lines;
+  size_t pos = 0;
+  explicit Records(const std::string& path) {
+    std::ifstream f(path);
+    if (!f) throw std::runtime_error("cannot open " + path);
+    std::string l;
+    while (std::getline(f, l)) {
+      if (!l.empty() && l.back() == '\r') l.pop_back();
+      lines.push_back(l);
+    }
+  }
+  bool eof() const { return pos >= lines.size(); }
+  const std::string& next() {
+    if (pos >= lines.size()) throw std::runtime_error("unexpected end of topology file");
+    return lines[pos++];
+  }
+  void skip() { next(); }
+  static void tokens(const std::string& l, std::vector<std::string>& out) {
+    size_t i = 0;
+    while (i < l.size()) {
+      while (i < l.size() && (std::isspace((unsigned char)l[i]) || l[i] == ',')) ++i;
+      if (i >= l.size()) break;
+      size_t j = i;
+      while (j < l.size() && !std::isspace((unsigned char)l[j]) && l[j] != ',') ++j;
+      out.push_back(l.substr(i, j - i));
+      i = j;
+    }
+  }
+  // list-directed READ of n values starting at a new record
+  std::vector<std::string> items(size_t n) {
+    std::vector<std::string> out;
+    while (out.size() < n) {
+      std::vector<std::string> t;
+      tokens(next(), t);
+      for (auto& s : t) {
+        if (out.size() < n) out.push_back(s);
+      }
+    }
+    return out;
+  }
+};
+
+double to_d(const std::string& s) {
+  std::string t = s;
+  for (auto& c : t) if (c == 'd' || c == 'D') c = 'e';
+  char* end = nullptr;
+  double v = std::strtod(t.c_str(), &end);
+  if (end == t.c_str()) throw std::runtime_error("bad number '" + s + "'");
+  return v;
+}
+long to_i(const std::string& s) {
+  char* end = nullptr;
+  long v = std::strtol(s.c_str(), &end, 10);
+  if (end == s.c_str() || (*end && *end != '.')) throw std::runtime_error("bad integer '" + s + "'");
+  return v;
+}
+// leading integers of a record, stopping at the first non-integer token
+std::vector<long> lead_ints(const std::string& l) {
+  std::vector<std::string> t;
+  Records::tokens(l, t);
+  std::vector<long> out;
+  for (auto& s : t) {
+    char* end = nullptr;
+    long v = std::strtol(s.c_str(), &end, 10);
+    if (end == s.c_str() || *end) break;
+    out.push_back(v);
+  }
+  return out;
+}
+std::vector<double> lead_reals(const std::string& l) {
+  std::vector<std::string> t;
+  Records::tokens(l, t);
+  std::vector<double> out;
+  for (auto& s : t) {
+    std::string u = s;
+    for (auto& c : u) if (c == 'd' || c == 'D') c = 'e';
+    char* end = nullptr;
+    double v = std::strtod(u.c_str(), &end);
+    if (end == u.c_str() || *end) break;
+    out.push_back(v);
+  }
+  return out;
+}
+
+// (80i1)/(80l1) fixed-format bit tables: nat_solute rows of kMaxNbrRange
+void read_bits(Records& r, int nrows, std::vector<uint32_t>& rows, bool logical) {
+  rows.assign(nrows, 0u);
+  size_t total = (size_t)nrows * kMaxNbrRange, k = 0;
+  while (k < total) {
+    const std::string& l = r.next();
+    for (size_t c = 0; c < 80 && k < total; ++c, ++k) {
+      char ch = c < l.size() ? l[c] : ' ';
+      bool on = logical ? (ch == 'T' || ch == 't') : (ch == '1');
+      if (on) rows[k / kMaxNbrRange] |= 1u << (k % kMaxNbrRange);
+    }
+  }
+}
+
+}  // namespace
+
+Topology read_topology(const std::string& path) {
+  Records r(path);
+  Topology t;
+  std::string title = r.next();
+  if (title.rfind("Q topology file", 0) == 0) {
+    for (;;) {
+      std::string l = r.next();
+      std::vector<std::string> tk;
+      Records::tokens(l, tk);
+      if (tk.empty()) continue;
+      std::string key = tk[0];
+      for (auto& c : key) c = (char)std::toupper((unsigned char)c);
+      if (key == "VERSION" && tk.size() > 1) t.version = to_d(tk[1]);
+      else if (key == "FF_TYPE" && tk.size() > 1) t.ff_type = (int)to_i(tk[1]);
+      else if (key == "END") break;
+    }
+  } else {
+    t.version = 2;
+  }
+  if (t.version < 4.15) throw std::runtime_error("topology version < 4.15");
+  {
+    auto v = lead_ints(r.next());
+    if (v.empty()) throw std::runtime_error("bad atom count record");
+    t.nat_pro = (int)v[0];
+    t.nat_solute = v.size() > 1 ? (int)v[1] : t.nat_pro;
+    t.solv_atom = v.size() > 2 ? (int)v[2] : 3;
+  }
+  int N = t.nat_pro;
+  {
+    auto it = r.items(3 * (size_t)N);
+    t.xtop.resize(3 * N);
+    for (int i = 0; i < 3 * N; ++i) t.xtop[i] = to_d(it[i]);
+  }
+  r.skip();
+  {
+    auto it = r.items(N);
+    t.iac.resize(N);
+    for (int i = 0; i < N; ++i) t.iac[i] = (int)to_i(it[i]) - 1;
+  }
+  // bonds
+  {
+    auto v = lead_ints(r.next());
+    int nb = (int)v.at(0);
+    t.nbonds_solute = v.size() > 1 ? (int)v[1] : nb;
+    auto it = r.items(3 * (size_t)nb);
+    t.bnd.resize(nb);
+    for (int b = 0; b < nb; ++b)
+      t.bnd[b] = {(int)to_i(it[3*b]) - 1, (int)to_i(it[3*b+1]) - 1, (int)to_i(it[3*b+2])};
+    int nc = (int)lead_ints(r.next()).at(0);
+    t.bondlib.resize(nc);
+    for (int c = 0; c < nc; ++c) {
+      auto rv = lead_reals(r.next());
+      t.bondlib[c] = {rv.at(1), rv.at(2)};
+    }
+  }
+  // angles
+  {
+    auto v = lead_ints(r.next());
+    int na = (int)v.at(0);
+    t.nangles_solute = v.size() > 1 ? (int)v[1] : na;
+    auto it = r.items(4 * (size_t)na);
+    t.ang.resize(na);
+    for (int a = 0; a < na; ++a)
+      t.ang[a] = {(int)to_i(it[4*a]) - 1, (int)to_i(it[4*a+1]) - 1, (int)to_i(it[4*a+2]) - 1,
+                  (int)to_i(it[4*a+3])};
+    int nc = (int)lead_ints(r.next()).at(0);
+    t.anglib.resize(nc);
+    for (int c = 0; c < nc; ++c) {
+      auto rv = lead_reals(r.next());
+      AngLib al{rv.at(1), rv.at(2), 0.0, 0.0};
+      if (rv.size() > 3) al.ureyfk = rv[3];
+      if (rv.size() > 4) al.ureyr0 = rv[4];
+      t.anglib[c] = al;
+    }
+  }
+  // torsions
+  {
+    auto v = lead_ints(r.next());
+    int nt = (int)v.at(0);
+    t.ntors_solute = v.size() > 1 ? (int)v[1] : nt;
+    auto it = r.items(5 * (size_t)nt);
+    t.tor.resize(nt);
+    for (int a = 0; a < nt; ++a)
+      t.tor[a] = {(int)to_i(it[5*a]) - 1, (int)to_i(it[5*a+1]) - 1, (int)to_i(it[5*a+2]) - 1,
+                  (int)to_i(it[5*a+3]) - 1, (int)to_i(it[5*a+4])};
+    int nc = (int)to_i(r.items(1)[0]);
+    t.torlib.resize(nc);
+    for (int c = 0; c < nc; ++c) {
+      auto it2 = r.items(5);
+      t.torlib[c] = {to_d(it2[1]), to_d(it2[2]), to_d(it2[3]), to_d(it2[4])};
+    }
+  }
+  // impropers
+  {
+    auto v = lead_ints(r.next());
+    int ni = (int)v.at(0);
+    t.nimps_solute = v.size() > 1 ? (int)v[1] : ni;
+    auto it = r.items(5 * (size_t)ni);
+    t.imp.resize(ni);
+    for (int a = 0; a < ni; ++a)
+      t.imp[a] = {(int)to_i(it[5*a]) - 1, (int)to_i(it[5*a+1]) - 1, (int)to_i(it[5*a+2]) - 1,
+                  (int)to_i(it[5*a+3]) - 1, (int)to_i(it[5*a+4])};
+    auto v2 = lead_ints(r.next());
+    int nc = (int)v2.at(0);
+    t.imp_type = v2.size() > 1 ? (int)v2[1] : 1;
+    t.implib.resize(nc);
+    for (int c = 0; c < nc; ++c) {
+      auto it2 = r.items(3);
+      t.implib[c] = {to_d(it2[1]), to_d(it2[2])};
+    }
+  }
+  // charges
+  {
+    r.skip();
+    auto it = r.items(N);
+    t.crg.resize(N);
+    for (int i = 0; i < N; ++i) t.crg[i] = to_d(it[i]);
+  }
+  // charge groups
+  {
+    auto v = lead_ints(r.next());
+    int ncgp = (int)v.at(0);
+    t.ncgp_solute = v.size() > 1 ? (int)v[1] : ncgp;
+    t.iuse_switch_atom = v.size() > 2 ? (int)v[2] : 1;
+    t.cgp.resize(ncgp);
+    int k = 0;
+    for (int g = 0; g < ncgp; ++g) {
+      auto h = r.items(2);
+      int n = (int)to_i(h[0]);
+      t.cgp[g] = {(int)to_i(h[1]) - 1, k, k + n - 1};
+      auto at = r.items(n);
+      for (int a = 0; a < n; ++a) t.cgpatom.push_back((int)to_i(at[a]) - 1);
+      k += n;
+    }
+  }
+  // atom types
+  {
+    int nt = (int)to_i(r.items(1)[0]);
+    t.types.resize(nt);
+    t.ivdw_rule = (int)to_i(r.items(1)[0]);
+    auto rv = lead_reals(r.next());
+    t.el14_scale = rv.at(0);
+    t.coulomb_constant = (rv.size() > 1 && rv[1] > 0) ? rv[1] : 332.0;
+    r.skip();
+    auto m = r.items(nt);
+    for (int i = 0; i < nt; ++i) t.types[i].mass = to_d(m[i]);
+    for (int j = 0; j < 3; ++j) {
+      r.skip();
+      auto a = r.items(nt);
+      for (int i = 0; i < nt; ++i) t.types[i].avdw[j] = to_d(a[i]);
+      r.skip();
+      auto b = r.items(nt);
+      for (int i = 0; i < nt; ++i) t.types[i].bvdw[j] = to_d(b[i]);
+    }
+    int nlj2 = (int)to_i(r.items(1)[0]);
+    for (int i = 0; i < nlj2; ++i) {
+      auto p = r.items(2);
+      t.lj2.push_back({(int)to_i(p[0]) - 1, (int)to_i(p[1]) - 1});
+    }
+  }
+  // neighbour / exclusion lists
+  {
+    r.items(1);
+    if (t.nat_solute > 0) read_bits(r, t.nat_solute, t.list14, false);
+    else t.list14.clear();
+    int n14long = (int)to_i(r.items(1)[0]);
+    for (int i = 0; i < n14long; ++i) {
+      auto p = r.items(2);
+      t.list14long.push_back({(int)to_i(p[0]) - 1, (int)to_i(p[1]) - 1});
+    }
+    r.items(1);
+    if (t.nat_solute > 0) read_bits(r, t.nat_solute, t.listex, false);
+    int nexlong = (int)to_i(r.items(1)[0]);
+    for (int i = 0; i < nexlong; ++i) {
+      auto p = r.items(2);
+      t.listexlong.push_back({(int)to_i(p[0]) - 1, (int)to_i(p[1]) - 1});
+    }
+  }
+  // residues and molecules
+  {
+    std::string l = r.next();
+    if (l.find_first_not_of(" \t") == std::string::npos) l = r.next();
+    auto v = lead_ints(l);
+    int nres = (int)v.at(0);
+    t.nres_solute = v.size() > 1 ? (int)v[1] : nres;
+    if (nres > 0) {
+      auto it = r.items(nres);
+      for (int i = 0; i < nres; ++i) t.res_start.push_back((int)to_i(it[i]) - 1);
+    }
+    r.skip();
+    for (int i = 0; i < (nres + 15) / 16; ++i) r.skip();
+    int nmol = (int)to_i(r.items(1)[0]);
+    if (nmol > 0) {
+      auto it = r.items(nmol);
+      for (int i = 0; i < nmol; ++i) t.istart_mol.push_back((int)to_i(it[i]) - 1);
+    }
+    int nt = (int)t.types.size();
+    r.skip();
+    for (int i = 0; i < (nt + 7) / 8; ++i) r.skip();
+    r.skip();
+    for (int i = 0; i < (nt + 12) / 13; ++i) r.skip();
+  }
+  if (t.version < 4) throw std::runtime_error("topology version < 4 has no boundary data");
+  t.solvent_type = (int)to_i(r.items(1)[0]);
+  {
+    std::string l = r.next();
+    std::vector<std::string> tk;
+    Records::tokens(l, tk);
+    if (!tk.empty() && tk[0] == "PBC") throw std::runtime_error("periodic boundary topologies are not supported");
+    auto rv = lead_reals(l);
+    if (t.version < 5.01) {
+      t.rexcl_o = rv.at(0); t.rexcl_i = rv.at(1); t.rwat = rv.at(2);
+    } else {
+      t.rexcl_o = rv.at(0); t.rwat = rv.at(1);
+    }
+    auto a = r.items(3);
+    for (int k = 0; k < 3; ++k) t.xpcent[k] = to_d(a[k]);
+    auto b = r.items(3);
+    for (int k = 0; k < 3; ++k) t.xwcent[k] = to_d(b[k]);
+    auto c = r.items(2);
+    t.nexats = (int)to_i(c[0]);
+    t.nexwat = (int)to_i(c[1]);
+    t.excl.assign(N, 0);
+    int k = 0;
+    while (k < N) {
+      const std::string& fl = r.next();
+      for (size_t q = 0; q < 80 && k < N; ++q, ++k) {
+        char ch = q < fl.size() ? fl[q] : 'F';
+        t.excl[k] = (ch == 'T' || ch == 't');
+      }
+    }
+  }
+  return t;
+}
+
+// --------------------------------------------------------------------------
+// input file (prmfile semantics)
+namespace {
+struct Sections {
+  std::map<std::string, std::vector<std::string>> sec;
+  explicit Sections(const std::string& path) {
+    std::ifstream f(path);
+    if (!f) throw std::runtime_error("cannot open " + path);
+    std::string l, cur;
+    bool any = false;
+    while (std::getline(f, l)) {
+      if (!l.empty() && l.back() == '\r') l.pop_back();
+      size_t s = l.find_first_not_of(" \t");
+      if (s == std::string::npos) continue;
+      char c = l[s];
+      if (c == '!' || c == '#' || c == '*') continue;
+      if (c == '[') {
+        size_t e = l.find(']');
+        cur = l.substr(s + 1, e - s - 1);
+        for (auto& ch : cur) ch = (char)std::toupper((unsigned char)ch);
+        sec[cur];
+        any = true;
+        continue;
+      }
+      if (!any) throw std::runtime_error("data outside a section in " + path);
+      // strip a trailing comment: '!' anywhere, '#'/'*' after whitespace
+      size_t cs = std::string::npos;
+      for (size_t k = s + 1; k < l.size(); ++k) {
+        if (l[k] == '!' || ((l[k] == '#' || l[k] == '*') && (l[k-1] == ' ' || l[k-1] == '\t'))) {
+          cs = k; break;
+        }
+      }
+      if (cs != std::string::npos) l = l.substr(0, cs);
+      sec[cur].push_back(l);
+    }
+  }
+  bool has(const std::string& s) const { return sec.count(s) > 0; }
+  bool get(const std::string& s, const std::string& key, std::string& val) const {
+    auto it = sec.find(s);
+    if (it == sec.end()) return false;
+    for (auto& l : it->second) {
+      std::vector<std::string> tk;
+      Records::tokens(l, tk);
+      if (tk.size() >= 2 && tk[0] == key) { val = tk[1]; return true; }
+    }
+    return false;
+  }
+};
+bool parse_onoff(const std::string& v, bool& out) {
+  if (v == "on" || v == "ON" || v == "1") { out = true; return true; }
+  if (v == "off" || v == "OFF" || v == "0") { out = false; return true; }
+  return false;
+}
+}  // namespace
+
+MDInput read_input(const std::string& path) {
+  Sections S(path);
+  MDInput m;
+  std::string v;
+  if (S.has("PBC")) throw std::runtime_error("periodic boundary runs are not supported");
+  if (!S.has("MD")) throw std::runtime_error("input has no [MD] section");
+  if (!S.get("MD", "steps", v)) throw std::runtime_error("[MD] steps missing");
+  m.nsteps = (int)to_i(v);
+  if (!S.get("MD", "stepsize", v)) throw std::runtime_error("[MD] stepsize missing");
+  m.stepsize = to_d(v);
+  if (!S.get("MD", "temperature", v)) throw std::runtime_error("[MD] temperature missing");
+  m.temp0 = to_d(v);
+  if (S.get("MD", "bath_coupling", v)) m.tau_t = to_d(v);
+  if (S.get("MD", "thermostat", v)) m.thermostat = v;
+  if (m.thermostat != "berendsen")
+    throw std::runtime_error("only the berendsen thermostat is supported");
+  if (S.get("MD", "integrator", v)) m.integrator = v;
+  if (m.integrator != "leap-frog") throw std::runtime_error("only the leap-frog integrator is supported");
+  if (S.get("MD", "separate_scaling", v)) parse_onoff(v, m.separate_scaling);
+  m.iseed = 1;
+  if (S.get("MD", "random_seed", v)) m.iseed = (int)to_i(v);
+  if (S.get("MD", "initial_temperature", v)) { m.have_tmaxw = true; m.tmaxw = to_d(v); }
+  else m.iseed = 0;
+  if (S.get("MD", "shake_solvent", v)) parse_onoff(v, m.shake_solvent);
+  if (S.get("MD", "shake_solute", v)) parse_onoff(v, m.shake_solute);
+  if (S.get("MD", "shake_hydrogens", v)) parse_onoff(v, m.shake_hydrogens);
+  if (S.get("MD", "lrf", v)) parse_onoff(v, m.use_lrf);
+  if (S.has("CUT-OFFS")) {
+    if (S.get("CUT-OFFS", "solute_solute", v)) m.rcpp = to_d(v);
+    if (S.get("CUT-OFFS", "solvent_solvent", v)) m.rcww = to_d(v);
+    if (S.get("CUT-OFFS", "solute_solvent", v)) m.rcpw = to_d(v);
+    if (S.get("CUT-OFFS", "q_atom", v)) m.rcq = to_d(v);
+    if (m.use_lrf && S.get("CUT-OFFS", "lrf", v)) m.rclrf = to_d(v);
+  }
+  if (m.use_lrf && (m.rclrf < m.rcpp || m.rclrf < m.rcpw || m.rclrf < m.rcww))
+    throw std::runtime_error("LRF cut-off must not be smaller than solute or solvent cut-offs");
+  if (S.has("SPHERE")) {
+    if (S.get("SPHERE", "shell_radius", v)) { m.have_shell_radius = true; m.shell_radius = to_d(v); }
+    if (S.get("SPHERE", "shell_force", v)) m.fk_pshell = to_d(v);
+    if (S.get("SPHERE", "excluded_force", v)) m.fk_fix = to_d(v);
+    if (m.fk_fix <= 0) m.fk_fix = 200.0;
+    if (S.get("SPHERE", "excluded_freeze", v)) parse_onoff(v, m.freeze);
+    if (S.get("SPHERE", "exclude_bonded", v)) parse_onoff(v, m.exclude_bonded);
+  }
+  std::string ws = S.has("SOLVENT") ? "SOLVENT" : (S.has("WATER") ? "WATER" : "");
+  if (!ws.empty()) {
+    if (S.get(ws, "radius", v)) m.rwat_in = to_d(v);
+    if (S.get(ws, "radial_force", v)) m.fk_wsphere = to_d(v);
+    if (S.get(ws, "polarisation", v)) parse_onoff(v, m.wpol_restr);
+    m.wpol_born = m.wpol_restr;
+    if (S.get(ws, "charge_correction", v)) parse_onoff(v, m.wpol_born);
+    if (S.get(ws, "polarisation_force", v)) m.fkwpol = to_d(v);
+    if (S.get(ws, "morse_depth", v)) m.dwmz = to_d(v);
+    if (S.get(ws, "morse_width", v)) m.awmz = to_d(v);
+  }
+  if (S.has("INTERVALS")) {
+    if (S.get("INTERVALS", "non_bond", v)) m.nbcycle = (int)to_i(v);
+    if (S.get("INTERVALS", "output", v)) m.iout_cycle = (int)to_i(v);
+    m.itemp_cycle = m.iout_cycle;
+    if (S.get("INTERVALS", "temperature", v)) m.itemp_cycle = (int)to_i(v);
+    if (S.get("INTERVALS", "energy", v)) m.iene_cycle = (int)to_i(v);
+    if (S.get("INTERVALS", "trajectory", v)) m.itrj_cycle = (int)to_i(v);
+  }
+  if (!S.has("FILES")) throw std::runtime_error("input has no [files] section");
+  if (!S.get("FILES", "topology", m.top_file)) throw std::runtime_error("[files] topology missing");
+  S.get("FILES", "restart", m.restart_file);
+  S.get("FILES", "final", m.final_file);
+  S.get("FILES", "energy", m.energy_file);
+  S.get("FILES", "fep", m.fep_file);
+  if (m.restart_file.empty() && m.iseed == 0)
+    throw std::runtime_error("need a restart file or random_seed + initial_temperature");
+  if (S.has("LAMBDAS")) {
+    for (auto& l : S.sec.at("LAMBDAS")) {
+      std::vector<std::string> tk;
+      Records::tokens(l, tk);
+      for (auto& s : tk) m.lambdas.push_back(to_d(s));
+    }
+  }
+  return m;
+}
+
+// --------------------------------------------------------------------------
+FepData read_fep(const std::string& path, int natom) {
+  Sections S(path);
+  FepData f;
+  std::string v;
+  f.nstates = 1;
+  if (S.get("FEP", "states", v)) f.nstates = (int)to_i(v);
+  if (f.nstates < 1 || f.nstates > kMaxStates) throw std::runtime_error("bad number of FEP states");
+  if (S.get("FEP", "offset", v) || S.get("FEP", "offset_name", v) || S.get("FEP", "offset_residue", v))
+    throw std::runtime_error("FEP atom offsets are not supported");
+  if (!S.has("ATOMS")) throw std::runtime_error("FEP file has no [atoms] section");
+  for (auto& l : S.sec.at("ATOMS")) {
+    std::vector<std::string> tk;
+    Records::tokens(l, tk);
+    if (tk.size() < 2) continue;
+    int iq = (int)to_i(tk[0]);
+    int at = (int)to_i(tk[1]);
+    if (iq != (int)f.iqseq.size() + 1) throw std::runtime_error("Q atoms must be numbered 1..n in order");
+    if (at < 1 || at > natom) throw std::runtime_error("invalid Q-atom topology number");
+    f.iqseq.push_back(at - 1);
+  }
+  int nq = (int)f.iqseq.size();
+  f.qcrg.assign(nq, std::vector<double>(f.nstates, 0.0));
+  for (const char* unsup : {"CHANGE_BONDS", "CHANGE_ANGLES", "CHANGE_TORSIONS", "CHANGE_IMPROPERS",
+                            "ATOM_TYPES", "CHANGE_ATOMS", "SOFT_PAIRS", "EXCLUDED_PAIRS",
+                            "OFF-DIAGONALS", "SHAKE_CONSTRAINTS", "MONITOR_GROUPS", "SOFTCORE"}) {
+    if (S.has(unsup) && !S.sec.at(unsup).empty())
+      throw std::runtime_error(std::string("FEP section [") + unsup + "] is not supported");
+  }
+  if (S.has("CHANGE_CHARGES")) {
+    f.have_charges = true;
+    for (auto& l : S.sec.at("CHANGE_CHARGES")) {
+      std::vector<std::string> tk;
+      Records::tokens(l, tk);
+      if (tk.empty()) continue;
+      int iq = (int)to_i(tk[0]);
+      if (iq < 1 || iq > nq || (int)tk.size() < 1 + f.nstates) throw std::runtime_error("bad [change_charges] line");
+      for (int s = 0; s < f.nstates; ++s) f.qcrg[iq - 1][s] = to_d(tk[1 + s]);
+    }
+  }
+  return f;
+}
+
+// --------------------------------------------------------------------------
+// restart (Fortran unformatted sequential records, 4-byte markers)
+namespace {
+bool read_rec(FILE* fp, std::vector<char>& buf) {
+  int32_t n;
+  if (std::fread(&n, 4, 1, fp) != 1) return false;
+  buf.resize(n);
+  if (n && std::fread(buf.data(), 1, n, fp) != (size_t)n) return false;
+  int32_t n2;
+  if (std::fread(&n2, 4, 1, fp) != 1 || n2 != n) return false;
+  return true;
+}
+void write_rec(FILE* fp, const void* p, int32_t n) {
+  std::fwrite(&n, 4, 1, fp);
+  if (n) std::fwrite(p, 1, n, fp);
+  std::fwrite(&n, 4, 1, fp);
+}
+}  // namespace
+
+bool read_restart(const std::string& path, int natom, std::vector<double>& x,
+                  std::vector<double>& v, std::vector<double>* theta_corr) {
+  FILE* fp = std::fopen(path.c_str(), "rb");
+  if (!fp) throw std::runtime_error("cannot open restart " + path);
+  std::vector<char> b;
+  if (!read_rec(fp, b) || b.size() != 4) { std::fclose(fp); throw std::runtime_error("bad restart header"); }
+  int32_t canary;
+  std::memcpy(&canary, b.data(), 4);
+  if (canary != -1337) { std::fclose(fp); throw std::runtime_error("restart is not double precision (-1337)"); }
+  auto arr = [&](std::vector<double>& out) {
+    if (!read_rec(fp, b)) throw std::runtime_error("short restart");
+    int32_t nat3;
+    std::memcpy(&nat3, b.data(), 4);
+    if (nat3 != 3 * natom) throw std::runtime_error("wrong number of atoms in restart file");
+    out.resize(nat3);
+    std::memcpy(out.data(), b.data() + 4, 8 * (size_t)nat3);
+  };
+  arr(x);
+  arr(v);
+  if (theta_corr) {
+    theta_corr->clear();
+    if (read_rec(fp, b) && b.size() >= 4) {
+      int32_t n;
+      std::memcpy(&n, b.data(), 4);
+      if (b.size() == 4 + 8 * (size_t)n) {
+        theta_corr->resize(n);
+        std::memcpy(theta_corr->data(), b.data() + 4, 8 * (size_t)n);
+      }
+    }
+  }
+  std::fclose(fp);
+  return true;
+}
+
+void write_restart(const std::string& path, const std::vector<double>& x,
+                   const std::vector<double>& v, const std::vector<double>* theta_corr) {
+  FILE* fp = std::fopen(path.c_str(), "wb");
+  if (!fp) throw std::runtime_error("cannot write " + path);
+  int32_t canary = -1337;
+  write_rec(fp, &canary, 4);
+  for (const auto* a : {&x, &v}) {
+    std::vector<char> b(4 + 8 * a->size());
+    int32_t n = (int32_t)a->size();
+    std::memcpy(b.data(), &n, 4);
+    std::memcpy(b.data() + 4, a->data(), 8 * a->size());
+    write_rec(fp, b.data(), (int32_t)b.size());
+  }
+  if (theta_corr && !theta_corr->empty()) {
+    std::vector<char> b(4 + 8 * theta_corr->size());
+    int32_t n = (int32_t)theta_corr->size();
+    std::memcpy(b.data(), &n, 4);
+    std::memcpy(b.data() + 4, theta_corr->data(), 8 * theta_corr->size());
+    write_rec(fp, b.data(), (int32_t)b.size());
+  }
+  std::fclose(fp);
+}
+
+}  // namespace qsys

@@ -1,0 +1,652 @@
+"""Synthetic qdyn systems: TIP3P droplets with optional branched solutes and
+FEP Q regions, written in qdyn's own file formats.
+
+Every system is generated from a seeded numpy ``default_rng`` and written as
+the four files qdyn reads:
+
+* the topology (``.top``), in the layout read by ``topo_read``
+  (src/topo.f90:515-1074);
+* the MD input (``.inp``), sectioned key/value as read by ``initialize``
+  (src/md.f90:4453-5358);
+* the FEP file (``.fep``): ``[FEP] states``, ``[atoms]`` and
+  ``[change_charges]`` as read by ``qatom_load_atoms``/``qatom_load_fep``
+  (src/qatom.f90:301-760);
+* optionally a restart (``.re``): Fortran unformatted sequential records,
+  canary -1337, ``nat3, x``, ``nat3, v``, ``nwpolr_shell, theta_corr``
+  (src/md.f90:18705-18736).
+
+The same files drive the reference qdyn (CPU) and this engine, so parity is
+checked on identical inputs.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+import os
+import struct
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+# TIP3P geometry and parameters
+R_OH = 0.9572
+THETA_HOH = 104.52
+Q_O, Q_H = -0.834, 0.417
+SIGMA_O, EPS_O = 3.1506, 0.1521
+MASS_O, MASS_H = 15.9994, 1.008
+DENSITY = 0.0334  # molecules / A^3
+MAX_NBR_RANGE = 25  # src/topo.f90:36
+
+
+def lj_sqrt_ab(sigma: float, eps: float):
+    """Geometric-rule topology parameters: sqrt(A)=2 sqrt(eps) sigma^6,
+    sqrt(B)=2 sqrt(eps) sigma^3, so A_ij=avdw_i*avdw_j (src/md.f90:13340)."""
+    return 2.0 * math.sqrt(eps) * sigma ** 6, 2.0 * math.sqrt(eps) * sigma ** 3
+
+
+@dataclasses.dataclass
+class System:
+    """A qdyn topology plus run settings (all atom indices are 1-based, as in
+    the files)."""
+    x: np.ndarray                  # (N,3) coordinates
+    iac: np.ndarray                # (N,) atom type codes, 1-based
+    crg: np.ndarray                # (N,) charges
+    nat_solute: int
+    masses: np.ndarray             # (natyps,)
+    avdw: np.ndarray               # (natyps,3)
+    bvdw: np.ndarray               # (natyps,3)
+    tac: List[str]
+    bonds: np.ndarray              # (nb,3) i j cod
+    nbonds_solute: int
+    bondlib: np.ndarray            # (nbc,2) fk b0
+    angles: np.ndarray             # (na,4)
+    nangles_solute: int
+    anglib: np.ndarray             # (nac,2) fk theta0(deg)
+    tors: np.ndarray               # (nt,5)
+    ntors_solute: int
+    torlib: np.ndarray             # (ntc,4) fk rmult deltor(deg) paths
+    imps: np.ndarray               # (ni,5)
+    nimps_solute: int
+    implib: np.ndarray             # (nic,2) fk imp0(deg)
+    cgps: List[Sequence[int]]      # atoms of each charge group; first = switch
+    ncgp_solute: int
+    list14: np.ndarray             # (nat_solute, 25) bool: list14[i-1, k-1] <=> (i, i+k) 1-4
+    listex: np.ndarray             # (nat_solute, 25) bool
+    list14long: np.ndarray         # (n,2)
+    listexlong: np.ndarray         # (n,2)
+    res_start: np.ndarray
+    res_names: List[str]
+    nres_solute: int
+    mol_start: np.ndarray
+    rexcl_o: float
+    rwat: float
+    xpcent: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(3))
+    xwcent: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(3))
+    el14_scale: float = 0.5
+    coulomb_constant: float = 332.0
+    # Q region (optional)
+    qatoms: Optional[np.ndarray] = None    # (nqat,) topology atom numbers
+    qcrg: Optional[np.ndarray] = None      # (nqat, nstates)
+    # off-diagonal (EVB) couplings, optional: rows (i, j, k, l, A, mu)
+    offdiag: Optional[list] = None
+
+    @property
+    def natom(self) -> int:
+        return int(self.x.shape[0])
+
+    @property
+    def nwat(self) -> int:
+        return (self.natom - self.nat_solute) // 3
+
+
+# ---------------------------------------------------------------------------
+# geometry
+
+
+def _random_rotations(rng, n):
+    """Uniform random rotation matrices (Shoemake quaternions)."""
+    u1, u2, u3 = rng.random(n), rng.random(n), rng.random(n)
+    q = np.stack([np.sqrt(1 - u1) * np.sin(2 * np.pi * u2),
+                  np.sqrt(1 - u1) * np.cos(2 * np.pi * u2),
+                  np.sqrt(u1) * np.sin(2 * np.pi * u3),
+                  np.sqrt(u1) * np.cos(2 * np.pi * u3)], axis=1)
+    a, b, c, d = q[:, 3], q[:, 0], q[:, 1], q[:, 2]
+    R = np.empty((n, 3, 3))
+    R[:, 0, 0] = a * a + b * b - c * c - d * d
+    R[:, 0, 1] = 2 * (b * c - a * d)
+    R[:, 0, 2] = 2 * (b * d + a * c)
+    R[:, 1, 0] = 2 * (b * c + a * d)
+    R[:, 1, 1] = a * a - b * b + c * c - d * d
+    R[:, 1, 2] = 2 * (c * d - a * b)
+    R[:, 2, 0] = 2 * (b * d - a * c)
+    R[:, 2, 1] = 2 * (c * d + a * b)
+    R[:, 2, 2] = a * a - b * b - c * c + d * d
+    return R
+
+
+def water_lattice(radius: float, rng, avoid: Optional[np.ndarray] = None,
+                  avoid_r: float = 2.6):
+    """Waters on a cubic lattice of spacing DENSITY^(-1/3) with O inside the
+    sphere, random orientations; lattice sites within avoid_r of any solute
+    atom are dropped.  Returns (nw,3,3) coordinates O,H1,H2."""
+    a = DENSITY ** (-1.0 / 3.0)
+    n = int(math.ceil(radius / a)) + 1
+    g = np.arange(-n, n + 1) * a
+    O = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    O = O[np.linalg.norm(O, axis=1) <= radius]
+    if avoid is not None and len(avoid):
+        d2 = ((O[:, None, :] - avoid[None, :, :]) ** 2).sum(-1)
+        O = O[d2.min(1) > avoid_r ** 2]
+    nw = O.shape[0]
+    half = math.radians(THETA_HOH) / 2
+    local = np.array([[0, 0, 0],
+                      [R_OH * math.sin(half), 0, R_OH * math.cos(half)],
+                      [-R_OH * math.sin(half), 0, R_OH * math.cos(half)]])
+    rot = _random_rotations(rng, nw)
+    w = O[:, None, :] + np.einsum("nij,kj->nki", rot, local)
+    return w
+
+
+def _branched_solute(n_atoms: int, rmax: float, rng, bond=1.53):
+    """A linear backbone with one-atom branches on every other backbone atom,
+    grown as a self-avoiding walk confined to r < rmax.  Atoms are numbered in
+    growth order (backbone atom, then its branch).  Returns coords (n,3) and
+    the bond list (0-based pairs)."""
+    coords: List[np.ndarray] = []
+    bonds = []
+    backbone: List[int] = []
+
+    def ok(p, skip=()):
+        if np.linalg.norm(p) > rmax:
+            return False
+        for k, c in enumerate(coords):
+            if k in skip:
+                continue
+            if np.sum((c - p) ** 2) < 2.4 ** 2:
+                return False
+        return True
+
+    def place_from(parent, grand, tries=200):
+        for _ in range(tries):
+            if grand is None:
+                d = rng.normal(size=3)
+            else:
+                u = coords[parent] - coords[grand]
+                u /= np.linalg.norm(u)
+                # cone at ~70 deg from the previous bond (109.5 deg angle)
+                t = rng.normal(size=3)
+                t -= t.dot(u) * u
+                t /= np.linalg.norm(t)
+                ang = math.radians(180 - 111.0 + rng.normal(0, 4))
+                d = math.cos(ang) * u + math.sin(ang) * t
+            d /= np.linalg.norm(d)
+            p = coords[parent] + bond * d
+            skip = {parent} if grand is None else {parent, grand}
+            if ok(p, skip):
+                return p
+        return None
+
+    coords.append(np.zeros(3))
+    backbone.append(0)
+    while len(coords) < n_atoms:
+        parent = backbone[-1]
+        grand = backbone[-2] if len(backbone) > 1 else None
+        p = place_from(parent, grand)
+        if p is None:
+            raise RuntimeError("solute growth failed; increase rmax")
+        coords.append(p)
+        i = len(coords) - 1
+        bonds.append((parent, i))
+        backbone.append(i)
+        if len(backbone) % 2 == 0 and len(coords) < n_atoms:
+            q = place_from(i, parent)
+            if q is None:
+                raise RuntimeError("branch growth failed")
+            coords.append(q)
+            bonds.append((i, len(coords) - 1))
+    c = np.array(coords)
+    c -= c.mean(0)
+    return c, bonds
+
+
+def _graph_terms(n, bonds):
+    nbr = [[] for _ in range(n)]
+    for i, j in bonds:
+        nbr[i].append(j)
+        nbr[j].append(i)
+    angles = []
+    for j in range(n):
+        for a in range(len(nbr[j])):
+            for b in range(a + 1, len(nbr[j])):
+                angles.append((nbr[j][a], j, nbr[j][b]))
+    tors = []
+    for j, k in bonds:
+        for i in nbr[j]:
+            if i == k:
+                continue
+            for l in nbr[k]:
+                if l == j or l == i:
+                    continue
+                tors.append((i, j, k, l))
+    # topological distance up to 3
+    dist = {}
+    for s in range(n):
+        frontier = {s}
+        seen = {s: 0}
+        for d in range(1, 4):
+            nxt = set()
+            for u in frontier:
+                for w in nbr[u]:
+                    if w not in seen:
+                        seen[w] = d
+                        nxt.add(w)
+            frontier = nxt
+        for w, d in seen.items():
+            if w > s and d > 0:
+                dist[(s, w)] = d
+    return angles, tors, dist
+
+
+# ---------------------------------------------------------------------------
+# system builders
+
+
+def water_droplet(radius: float = 15.0, seed: int = 1001) -> System:
+    """TIP3P droplet (configuration 1: R=15 A, ~470 waters)."""
+    return solvated_system(radius=radius, n_solute=0, seed=seed)
+
+
+def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
+                    n_states: int = 0, n_types: int = 250,
+                    q_sd: float = 0.3) -> System:
+    """A TIP3P sphere of the given radius around an optional branched solute
+    (charges N(0,q_sd) neutralised per 4-atom charge group, sigma~U[2.5,3.8],
+    eps~U[0.05,0.2] quantised into at most n_types atom types), with an
+    optional Q region of n_q solute atoms and n_states FEP states whose
+    charges are drawn independently and neutralised per 4-atom block."""
+    rng = np.random.default_rng(seed)
+    # --- solute
+    if n_solute > 0:
+        rmax = max(6.0, 0.55 * radius)
+        for _ in range(20):
+            try:
+                sx, sb = _branched_solute(n_solute, rmax, rng)
+                break
+            except RuntimeError:
+                rmax *= 1.1
+        angles0, tors0, dist = _graph_terms(n_solute, sb)
+    else:
+        sx = np.zeros((0, 3))
+        sb, angles0, tors0, dist = [], [], [], {}
+    ns = n_solute
+    wat = water_lattice(radius, rng, avoid=sx if ns else None)
+    nw = wat.shape[0]
+    x = np.concatenate([sx, wat.reshape(-1, 3)], 0)
+    N = ns + 3 * nw
+
+    # --- atom types: water O, water H, then quantised solute types
+    masses = [MASS_O, MASS_H]
+    a_O, b_O = lj_sqrt_ab(SIGMA_O, EPS_O)
+    avdw = [[a_O] * 3, [0.0] * 3]
+    bvdw = [[b_O] * 3, [0.0] * 3]
+    tac = ["OW", "HW"]
+    iac = np.zeros(N, dtype=np.int64)
+    crg = np.zeros(N)
+    if ns:
+        sig = rng.uniform(2.5, 3.8, ns)
+        eps = rng.uniform(0.05, 0.2, ns)
+        ntyp = min(n_types - 2, ns)
+        # quantise (sigma, eps) on a grid: sqrt(ntyp) bins each
+        nb = max(1, int(math.floor(math.sqrt(ntyp))))
+        si = np.minimum((sig - 2.5) / 1.3 * nb, nb - 1).astype(int)
+        ei = np.minimum((eps - 0.05) / 0.15 * nb, nb - 1).astype(int)
+        key = si * nb + ei
+        uniq = sorted(set(key.tolist()))
+        tmap = {k: t for t, k in enumerate(uniq)}
+        for k in uniq:
+            s = 2.5 + (k // nb + 0.5) * 1.3 / nb
+            e = 0.05 + (k % nb + 0.5) * 0.15 / nb
+            a, b = lj_sqrt_ab(s, e)
+            avdw.append([a, a, a / math.sqrt(2.0)])
+            bvdw.append([b, b, b / math.sqrt(2.0)])
+            masses.append(12.011)
+            tac.append("S%d" % len(tac))
+        iac[:ns] = np.array([3 + tmap[k] for k in key.tolist()])
+        q = rng.normal(0.0, q_sd, ns)
+        for g0 in range(0, ns, 4):
+            g1 = min(ns, g0 + 4)
+            q[g0:g1] -= q[g0:g1].mean()
+        crg[:ns] = np.round(q, 4)
+        # keep groups exactly neutral after rounding
+        for g0 in range(0, ns, 4):
+            g1 = min(ns, g0 + 4)
+            crg[g1 - 1] -= round(float(crg[g0:g1].sum()), 4)
+    iac[ns::3] = 1
+    iac[ns + 1::3] = 2
+    iac[ns + 2::3] = 2
+    crg[ns::3] = Q_O
+    crg[ns + 1::3] = Q_H
+    crg[ns + 2::3] = Q_H
+
+    # --- bonded terms: solute harmonic bonds/angles, 3-fold torsions
+    bondlib = [[600.0, 1.53]]
+    bonds = [(i + 1, j + 1, 1) for i, j in sb]
+    anglib = [[80.0, 111.0]]
+    angles = [(i + 1, j + 1, k + 1, 1) for i, j, k in angles0]
+    torlib = [[0.3, 3.0, 0.0, 1.0]]
+    tors = [(i + 1, j + 1, k + 1, l + 1, 1) for i, j, k, l in tors0]
+    nbs, nas, nts = len(bonds), len(angles), len(tors)
+    # water: O-H1, O-H2, H1-H2 (last bond of the topology is H1-H2), one angle
+    hh = 2 * R_OH * math.sin(math.radians(THETA_HOH) / 2)
+    bondlib += [[1106.0, R_OH], [0.0, round(hh, 4)]]
+    anglib += [[200.0, THETA_HOH]]
+    for w in range(nw):
+        o = ns + 3 * w + 1
+        bonds += [(o, o + 1, 2), (o, o + 2, 2), (o + 1, o + 2, 3)]
+        angles += [(o + 1, o, o + 2, 2)]
+
+    # --- charge groups: solute blocks of 4 (first atom switches), waters
+    cgps = [list(range(g + 1, min(ns, g + 4) + 1)) for g in range(0, ns, 4)]
+    ncgp_solute = len(cgps)
+    cgps += [[ns + 3 * w + 1, ns + 3 * w + 2, ns + 3 * w + 3] for w in range(nw)]
+
+    # --- neighbour lists
+    list14 = np.zeros((ns, MAX_NBR_RANGE), dtype=bool)
+    listex = np.zeros((ns, MAX_NBR_RANGE), dtype=bool)
+    l14long, lexlong = [], []
+    for (i, j), d in dist.items():
+        k = j - i
+        if d <= 2:
+            if k <= MAX_NBR_RANGE:
+                listex[i, k - 1] = True
+            else:
+                lexlong.append((i + 1, j + 1))
+        elif d == 3:
+            if k <= MAX_NBR_RANGE:
+                list14[i, k - 1] = True
+            else:
+                l14long.append((i + 1, j + 1))
+
+    # --- residues/molecules: solute is one residue and one molecule
+    res_start = ([1] if ns else []) + [ns + 3 * w + 1 for w in range(nw)]
+    res_names = (["SOL"] if ns else []) + ["HOH"] * nw
+    mol_start = list(res_start)
+
+    sysm = System(
+        x=x, iac=iac, crg=crg, nat_solute=ns,
+        masses=np.array(masses), avdw=np.array(avdw), bvdw=np.array(bvdw), tac=tac,
+        bonds=np.array(bonds, dtype=np.int64).reshape(-1, 3), nbonds_solute=nbs,
+        bondlib=np.array(bondlib),
+        angles=np.array(angles, dtype=np.int64).reshape(-1, 4), nangles_solute=nas,
+        anglib=np.array(anglib),
+        tors=np.array(tors, dtype=np.int64).reshape(-1, 5), ntors_solute=nts,
+        torlib=np.array(torlib),
+        imps=np.zeros((0, 5), dtype=np.int64), nimps_solute=0,
+        implib=np.zeros((0, 2)),
+        cgps=cgps, ncgp_solute=ncgp_solute,
+        list14=list14, listex=listex,
+        list14long=np.array(l14long, dtype=np.int64).reshape(-1, 2),
+        listexlong=np.array(lexlong, dtype=np.int64).reshape(-1, 2),
+        res_start=np.array(res_start), res_names=res_names,
+        nres_solute=1 if ns else 0, mol_start=np.array(mol_start),
+        rexcl_o=radius, rwat=radius)
+
+    if n_q > 0 and n_states > 0:
+        # Q region: the n_q solute atoms nearest the centre, in whole
+        # 4-atom charge groups
+        gc = np.array([sx[np.array(g) - 1].mean(0) for g in cgps[:ncgp_solute]])
+        order = np.argsort(np.linalg.norm(gc, axis=1))
+        qa = []
+        for gi in order:
+            qa += cgps[gi]
+            if len(qa) >= n_q:
+                break
+        qa = np.array(sorted(qa[:n_q]))
+        qc = np.zeros((len(qa), n_states))
+        for s in range(n_states):
+            q = rng.normal(0.0, q_sd, len(qa))
+            for b0 in range(0, len(qa), 4):
+                q[b0:b0 + 4] -= q[b0:b0 + 4].mean()
+            q = np.round(q, 4)
+            for b0 in range(0, len(qa), 4):
+                b1 = min(len(qa), b0 + 4)
+                q[b1 - 1] -= round(float(q[b0:b1].sum()), 4)
+            qc[:, s] = q
+        sysm.qatoms = qa
+        sysm.qcrg = qc
+    return sysm
+
+
+# ---------------------------------------------------------------------------
+# writers
+
+
+def _fmt_list(vals, per_line, fmt):
+    out = []
+    for k in range(0, len(vals), per_line):
+        out.append("".join(fmt % v for v in vals[k:k + per_line]))
+    return out
+
+
+def write_top(s: System, path: str) -> None:
+    """Write a qdyn topology (version 5.06 layout, sphere boundary)."""
+    L = ["Q topology file",
+         "TITLE      synthetic",
+         "DATE       20260101",
+         "VERSION     5.06",
+         "FORCEFIELD synthetic",
+         "FF_TYPE    1",
+         "END        of header"]
+    N = s.natom
+    L.append("%8d%8d%8d = Total no. of atoms, no. of solute atoms, solvent atoms" % (N, s.nat_solute, 3))
+    L += _fmt_list([v for v in s.x.reshape(-1)], 6, "%12.6f")
+    L.append("%8d = No. of integer atom codes. iac's:" % N)
+    L += _fmt_list(list(s.iac), 16, "%5d")
+    L.append("%8d%8d = No. of bonds, no. of solute bonds. i - j - icode: (5 per line)" % (len(s.bonds), s.nbonds_solute))
+    L += _fmt_list([tuple(b) for b in s.bonds], 5, "%6d%6d%4d")
+    L.append("%8d = No. of bond codes. Parameters:" % len(s.bondlib))
+    for k, (fk, b0) in enumerate(s.bondlib):
+        L.append("%5d%10.3f%11.4f" % (k + 1, fk, b0))
+    L.append("%8d%8d = No. of angles, no. of solute angles. i - j - k - icode: (3 per line)" % (len(s.angles), s.nangles_solute))
+    L += _fmt_list([tuple(a) for a in s.angles], 3, "%6d%6d%6d%4d")
+    L.append("%8d = No. of angle codes. Parameters:" % len(s.anglib))
+    for k, (fk, t0) in enumerate(s.anglib):
+        L.append("%5d%10.3f%10.3f" % (k + 1, fk, t0))
+    L.append("%8d%8d = No. of torsions, solute torsions. i - j - k - l - icode: (2 per line)" % (len(s.tors), s.ntors_solute))
+    L += _fmt_list([tuple(t) for t in s.tors], 2, "%6d%6d%6d%6d%4d")
+    L.append("%8d = No. of torsion codes. Parameters:" % len(s.torlib))
+    for k, (fk, m, dt, p) in enumerate(s.torlib):
+        L.append("%5d%10.3f%10.3f%10.3f%10.3f" % (k + 1, fk, m, dt, p))
+    L.append("%8d%8d = No. of impropers, solute impr. i - j - k - l - icode: (2 per line)" % (len(s.imps), s.nimps_solute))
+    L += _fmt_list([tuple(t) for t in s.imps], 2, "%6d%6d%6d%6d%4d")
+    L.append("%8d%8d = No. of improper codes, type (1=harmonic,2=periodic). Parameters:" % (len(s.implib), 1))
+    for k, (fk, i0) in enumerate(s.implib):
+        L.append("%5d%10.3f%10.3f" % (k + 1, fk, i0))
+    L.append("%8d = No. of atomic charges" % N)
+    L += _fmt_list(list(s.crg), 10, "%9.4f")
+    L.append("%8d%8d%8d = No. of charge groups, no. of solute cgps, switch atoms flag." % (len(s.cgps), s.ncgp_solute, 1))
+    for g in s.cgps:
+        L.append("%6d%6d" % (len(g), g[0]))
+        L += _fmt_list(list(g), 16, "%6d")
+    nt = len(s.masses)
+    L.append("%8d = No. of atom types" % nt)
+    L.append("%8d = vdW combination rule (1 = Geom. / 2 = Arit.)" % 1)
+    L.append("%9.5f%10.4f = Electrostatic 1-4 scaling factor and  Coulomb constant" % (s.el14_scale, s.coulomb_constant))
+    L.append("Masses:")
+    L += _fmt_list(list(s.masses), 10, "%10.4f")
+    for j in range(3):
+        L.append("sqrt (Aii) normal:" if j == 0 else ("sqrt (Aii) polar:" if j == 1 else "sqrt (Aii) 1-4:"))
+        L += _fmt_list(list(s.avdw[:, j]), 8, "%13.5f")
+        L.append("sqrt (Bii) normal:" if j == 0 else ("sqrt (Bii) polar:" if j == 1 else "sqrt (Bii) 1-4:"))
+        L += _fmt_list(list(s.bvdw[:, j]), 8, "%13.5f")
+    L.append("%8d = No. of type-2 vdW interactions. pairs:" % 0)
+    ns = s.nat_solute
+
+    def bits(arr):
+        flat = "".join("1" if b else "0" for b in arr.reshape(-1))
+        return [flat[k:k + 80] for k in range(0, len(flat), 80)]
+    L.append("%8d = No. of 1-4 neighbours. nborlist (range=max_nbr_range):" % int(s.list14.sum()))
+    if ns:
+        L += bits(s.list14)
+    L.append("%8d = No. of long 1-4 nbrs (>max_nbr_range). pairlist:" % len(s.list14long))
+    for i, j in s.list14long:
+        L.append("%6d%6d" % (i, j))
+    L.append("%8d = No. of exclusions. exclusion list (range=max_nbr_range):" % int(s.listex.sum()))
+    if ns:
+        L += bits(s.listex)
+    L.append("%8d = No. of long exclusions (>max_nbr_range). pairlist:" % len(s.listexlong))
+    for i, j in s.listexlong:
+        L.append("%6d%6d" % (i, j))
+    L.append("%8d%8d = No. of residues, No of solute residues. start atoms:" % (len(s.res_start), s.nres_solute))
+    L += _fmt_list(list(s.res_start), 13, "%6d")
+    L.append("Sequence:")
+    L += _fmt_list(list(s.res_names), 16, "%-4s ")
+    L.append("%8d = No. of separate molecules. start atoms:" % len(s.mol_start))
+    L += _fmt_list(list(s.mol_start), 13, "%6d")
+    L.append("%8d = No. of atom types:" % nt)
+    L += _fmt_list(list(s.tac), 8, "%-8s ")
+    L.append("%8d = No. of SYBYL atom types:" % nt)
+    L += _fmt_list(["" for _ in range(nt)], 13, "%-5s ")
+    L.append("%8d = solvent type (0=SPC,1=3-atom,2=general)" % 0)
+    L.append("%8.3f%8.3f = Exclusion, solvent radii" % (s.rexcl_o, s.rwat))
+    L.append("%8.3f%8.3f%8.3f = Solute centre" % tuple(s.xpcent))
+    L.append("%8.3f%8.3f%8.3f = Solvent centre" % tuple(s.xwcent))
+    L.append("%8d%8d = No. of excluded atoms (incl. water), no. of excluded waters" % (0, 0))
+    flags = "F" * N
+    L += [flags[k:k + 80] for k in range(0, N, 80)]
+    with open(path, "w") as f:
+        f.write("\n".join(L) + "\n")
+
+
+def write_fep(s: System, path: str) -> None:
+    """FEP file with [FEP] states, [atoms] (Q number, topology atom) and
+    [change_charges] (Q number, charge per state)."""
+    nq, ns = s.qcrg.shape
+    L = ["[FEP]", "states %d" % ns, "", "[atoms]"]
+    for k, a in enumerate(s.qatoms):
+        L.append("%d %d" % (k + 1, a))
+    L += ["", "[change_charges]"]
+    for k in range(nq):
+        L.append("%d " % (k + 1) + " ".join("%.4f" % c for c in s.qcrg[k]))
+    with open(path, "w") as f:
+        f.write("\n".join(L) + "\n")
+
+
+def write_inp(path: str, top: str, final: str, *, steps: int, stepsize: float = 1.0,
+              temperature: float = 300.0, bath_coupling: float = 100.0,
+              random_seed: Optional[int] = 1, initial_temperature: Optional[float] = 300.0,
+              restart: Optional[str] = None, fep: Optional[str] = None,
+              lambdas: Optional[Sequence[float]] = None, energy: Optional[str] = None,
+              energy_interval: int = 0, output: int = 10, non_bond: int = 10,
+              rcpp: float = 10.0, rcww: float = 10.0, rcpw: float = 10.0,
+              rcq: float = 99.0, rclrf: float = 99.0, lrf: bool = True,
+              shake_solvent: bool = True, shake_solute: bool = False,
+              shake_hydrogens: bool = False, separate_scaling: bool = True,
+              radial_force: Optional[float] = 60.0, polarisation: bool = True,
+              polarisation_force: Optional[float] = 20.0,
+              shell_force: Optional[float] = None, shell_radius: Optional[float] = None,
+              temp_interval: Optional[int] = None) -> None:
+    on = lambda b: "on" if b else "off"
+    L = ["[MD]", "steps %d" % steps, "stepsize %.6g" % stepsize,
+         "temperature %.6g" % temperature, "bath_coupling %.6g" % bath_coupling,
+         "separate_scaling %s" % on(separate_scaling)]
+    if initial_temperature is not None:
+        L += ["random_seed %d" % random_seed, "initial_temperature %.6g" % initial_temperature]
+    L += ["shake_solvent %s" % on(shake_solvent), "shake_solute %s" % on(shake_solute),
+          "shake_hydrogens %s" % on(shake_hydrogens), "lrf %s" % on(lrf), "",
+          "[cut-offs]", "solute_solute %.6g" % rcpp, "solvent_solvent %.6g" % rcww,
+          "solute_solvent %.6g" % rcpw, "q_atom %.6g" % rcq]
+    if lrf:
+        L.append("lrf %.6g" % rclrf)
+    L += ["", "[sphere]"]
+    if shell_radius is not None:
+        L.append("shell_radius %.6g" % shell_radius)
+    if shell_force is not None:
+        L.append("shell_force %.6g" % shell_force)
+    L += ["", "[solvent]"]
+    if radial_force is not None:
+        L.append("radial_force %.6g" % radial_force)
+    L.append("polarisation %s" % on(polarisation))
+    if polarisation_force is not None:
+        L.append("polarisation_force %.6g" % polarisation_force)
+    L += ["", "[intervals]", "non_bond %d" % non_bond, "output %d" % output]
+    if temp_interval is not None:
+        L.append("temperature %d" % temp_interval)
+    if energy:
+        L.append("energy %d" % energy_interval)
+    L += ["", "[files]", "topology %s" % top, "final %s" % final]
+    if restart:
+        L.append("restart %s" % restart)
+    if energy:
+        L.append("energy %s" % energy)
+    if fep:
+        L.append("fep %s" % fep)
+    if lambdas is not None:
+        L += ["", "[lambdas]", " ".join("%.6f" % l for l in lambdas)]
+    with open(path, "w") as f:
+        f.write("\n".join(L) + "\n")
+
+
+# ---------------------------------------------------------------------------
+# restart files (Fortran unformatted sequential: 4-byte record markers)
+
+
+def _rec(payload: bytes) -> bytes:
+    n = struct.pack("<i", len(payload))
+    return n + payload + n
+
+
+def write_restart(path: str, x: np.ndarray, v: np.ndarray,
+                  theta_corr: Optional[np.ndarray] = None) -> None:
+    x = np.ascontiguousarray(x, dtype="<f8").reshape(-1)
+    v = np.ascontiguousarray(v, dtype="<f8").reshape(-1)
+    nat3 = x.size
+    out = _rec(struct.pack("<i", -1337))
+    out += _rec(struct.pack("<i", nat3) + x.tobytes())
+    out += _rec(struct.pack("<i", nat3) + v.tobytes())
+    if theta_corr is not None:
+        tc = np.ascontiguousarray(theta_corr, dtype="<f8")
+        out += _rec(struct.pack("<i", tc.size) + tc.tobytes())
+    with open(path, "wb") as f:
+        f.write(out)
+
+
+def read_restart(path: str):
+    """Returns (x (N,3), v (N,3), theta_corr or None)."""
+    with open(path, "rb") as f:
+        data = f.read()
+    recs = []
+    p = 0
+    while p < len(data):
+        (n,) = struct.unpack_from("<i", data, p)
+        recs.append(data[p + 4:p + 4 + n])
+        p += 8 + n
+    if struct.unpack("<i", recs[0])[0] != -1337:
+        raise ValueError("restart file without a double-precision canary")
+    nat3 = struct.unpack_from("<i", recs[1])[0]
+    x = np.frombuffer(recs[1], dtype="<f8", offset=4, count=nat3).copy()
+    v = np.frombuffer(recs[2], dtype="<f8", offset=4, count=nat3).copy()
+    tc = None
+    if len(recs) > 3:
+        nsh = struct.unpack_from("<i", recs[3])[0]
+        tc = np.frombuffer(recs[3], dtype="<f8", offset=4, count=nsh).copy()
+    return x.reshape(-1, 3), v.reshape(-1, 3), tc
+
+
+def write_system(s: System, directory: str, name: str, **inp) -> dict:
+    """Write name.top (+ name.fep) and name.inp into directory; returns paths."""
+    os.makedirs(directory, exist_ok=True)
+    top = os.path.join(directory, name + ".top")
+    write_top(s, top)
+    paths = {"top": top}
+    if s.qatoms is not None:
+        fep = os.path.join(directory, name + ".fep")
+        write_fep(s, fep)
+        paths["fep"] = fep
+        inp.setdefault("fep", name + ".fep")
+    inpp = os.path.join(directory, name + ".inp")
+    inp.setdefault("steps", 100)
+    write_inp(inpp, name + ".top", name + ".re", **inp)
+    paths["inp"] = inpp
+    paths["re"] = os.path.join(directory, name + ".re")
+    return paths
