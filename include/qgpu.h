@@ -1,0 +1,113 @@
+/*
+ * qgpu — C ABI of the MI355X engine for qdyn's nonbonded / SHAKE / step path.
+ *
+ * The reference has no FFI for this path: its step is a set of argument-less
+ * subroutines on module globals in src/md.f90.  Each entry point below
+ * replaces one reference call site (cited per function); a Fortran host binds
+ * them through ISO_C_BINDING (see the interface module in INTEGRATION.md).
+ *
+ * Conventions
+ *  - every function returns 0 on success, nonzero on failure; the message is
+ *    available from qgpu_last_error() (the Fortran side calls die(msg) on a
+ *    nonzero status, as src/md.f90:554-613 does);
+ *  - coordinate, velocity and gradient arrays are FP64, AoS, length 3*natom,
+ *    in qdyn's own order (x(:), v(:), d(:) of src/md.f90:309-313); d is the
+ *    gradient dV/dx (force = -d);
+ *  - one handle holds a batch of instances (one instance = one qdyn input:
+ *    topology + FEP file + lambdas); instance indices are 0-based;
+ *  - the library owns device memory; no torch types cross this boundary.
+ */
+#ifndef QGPU_H
+#define QGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qgpu_handle qgpu_handle;
+
+enum qgpu_precision {
+  QGPU_FP64 = 0,   /* plain: all arithmetic FP64, reproduces the reference */
+  QGPU_MIXED = 1   /* FP32 pair arithmetic, FP64 accumulation/SHAKE/integrator/LRF */
+};
+
+/* Energy terms returned by qgpu_energies, in this order (E%... of
+ * src/nrgy.f90:48-63 as printed by write_out, src/md.f90:18545-18691). */
+enum qgpu_energy_term {
+  QGPU_E_PP_EL, QGPU_E_PP_VDW, QGPU_E_P_BOND, QGPU_E_P_ANGLE, QGPU_E_P_TORSION, QGPU_E_P_IMPROPER,
+  QGPU_E_WW_EL, QGPU_E_WW_VDW, QGPU_E_W_BOND, QGPU_E_W_ANGLE, QGPU_E_W_TORSION, QGPU_E_W_IMPROPER,
+  QGPU_E_PW_EL, QGPU_E_PW_VDW, QGPU_E_LRF,
+  QGPU_E_QX_EL, QGPU_E_QX_VDW, QGPU_E_Q_BOND, QGPU_E_Q_ANGLE, QGPU_E_Q_TORSION, QGPU_E_Q_IMPROPER,
+  QGPU_E_RST_TOTAL, QGPU_E_RST_FIX, QGPU_E_RST_RADIAL, QGPU_E_RST_WPOL, QGPU_E_RST_SHELL,
+  QGPU_E_RST_PROTEIN,
+  QGPU_E_TOTAL, QGPU_E_POTENTIAL, QGPU_E_KINETIC,
+  QGPU_E_COUNT
+};
+
+/* Replaces qdyn's initialisation sequence (src/qdyn.f90:108-141:
+ * initialize, topology, prep_coord, get_fep, prep_sim, init_shake,
+ * make_nbqqlist, shrink_topology, maxwell/initial_shaking/
+ * stop_cm_translation).  Reads each qdyn input file (and the topology, FEP
+ * and restart files it names) and uploads the systems as one batch. */
+int qgpu_create(const char* const* input_files, int n_instances, int precision,
+                qgpu_handle** out);
+
+/* Releases the handle and its device memory. */
+int qgpu_destroy(qgpu_handle* h);
+
+/* Number of atoms of instance k (nat_pro, src/topo.f90:104). */
+int qgpu_natom(qgpu_handle* h, int k, int* natom);
+
+/* Replaces make_pair_lists (src/md.f90:5985-6071, called at :6505): charge-
+ * group centres, solute/solvent pair lists and LRF multipoles, from the
+ * instance coordinates currently on the device. */
+int qgpu_make_pair_lists(qgpu_handle* h);
+
+/* Replaces pot_energy_nonbonds + restraints + nonbond_qq (src/md.f90:
+ * 16096-16140): computes the gradient for every instance on the device.
+ * The gradient is fetched with qgpu_get_gradient. */
+int qgpu_pot_energy(qgpu_handle* h);
+
+/* Replaces md_run's loop body (src/md.f90:6471-6612): nsteps steps of
+ * list update (every NBcycle), forces, leapfrog/Berendsen, SHAKE and
+ * temperature, with x, v and d resident on the device. */
+int qgpu_step(qgpu_handle* h, int nsteps);
+
+/* State transfer of instance k (x, v: 3*natom FP64 each). */
+int qgpu_get_state(qgpu_handle* h, int k, double* x, double* v);
+int qgpu_set_state(qgpu_handle* h, int k, const double* x, const double* v);
+
+/* Gradient d (3*natom) of instance k from the last force evaluation. */
+int qgpu_get_gradient(qgpu_handle* h, int k, double* d);
+
+/* Energy terms of instance k from the last force evaluation, ordered as
+ * enum qgpu_energy_term; writes min(cap, QGPU_E_COUNT) values. */
+int qgpu_energies(qgpu_handle* h, int k, double* e, int cap);
+
+/* Per-state Q-atom energies of instance k (EQ(s), src/qatom.f90:197-200):
+ * 7 values per state: qq_el qq_vdw qp_el qp_vdw qw_el qw_vdw total. */
+int qgpu_qenergies(qgpu_handle* h, int k, double* e, int cap);
+
+/* Exports pair list `which` of instance k in the reference's layout and
+ * order (NB_TYPE i, j, LJcod; 0-based atoms): 0 = nbpp, 1 = nbpw, 2 = nbww
+ * (src/md.f90:329-433).  Call with out == NULL to get the count in *n. */
+int qgpu_get_pairlist(qgpu_handle* h, int k, int which, int* out, long* n);
+
+/* Current temperature (Temp, src/md.f90:6159-6163) of instance k. */
+int qgpu_temperature(qgpu_handle* h, int k, double* temp);
+
+/* Water-polarisation shell corrections theta_corr (restart record,
+ * src/md.f90:18729-18731) of instance k; *n gets the shell count. */
+int qgpu_theta_corr(qgpu_handle* h, int k, double* out, int* n);
+
+/* Average device time per step (ms) of the last qgpu_step call. */
+int qgpu_last_step_ms(qgpu_handle* h, double* ms);
+
+/* Message of the last failure in this thread ("" when none). */
+const char* qgpu_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QGPU_H */

@@ -457,7 +457,9 @@ struct Oracle {
   }
 
   void pot_energy() {
+    const double kinetic = E.kinetic;  // pot_energy leaves E%kinetic alone (src/md.f90:16025)
     E = Energies{};
+    E.kinetic = kinetic;
     for (auto& q : EQ) q = QEnergies{};
     std::fill(d.begin(), d.end(), 0.0);
     // pot_energy_nonbonds (sphere, geometric rule)
@@ -697,8 +699,8 @@ void Oracle::nonbond_pp() {
     double r6 = r2 * r2 * r2;
     double Vel = s.crg[i] * s.crg[j] * r;
     if (p.ljcod == 3) Vel = Vel * s.top.el14_scale;
-    double Va = typ(i).avdw[p.ljcod - 1] * typ(j).avdw[p.ljcod - 1] * r6 * r6;
-    double Vb = typ(i).bvdw[p.ljcod - 1] * typ(j).bvdw[p.ljcod - 1] * r6;
+    double Va = r6 * r6 * typ(i).avdw[p.ljcod - 1] * typ(j).avdw[p.ljcod - 1];
+    double Vb = r6 * typ(i).bvdw[p.ljcod - 1] * typ(j).bvdw[p.ljcod - 1];
     double dv = r2 * (-Vel - 12.0 * Va + 6.0 * Vb);
     d[i3] = d[i3] - dv * dx1; d[i3 + 1] = d[i3 + 1] - dv * dx2; d[i3 + 2] = d[i3 + 2] - dv * dx3;
     d[j3] = d[j3] + dv * dx1; d[j3 + 1] = d[j3 + 1] + dv * dx2; d[j3 + 2] = d[j3 + 2] + dv * dx3;
@@ -710,12 +712,16 @@ void Oracle::nonbond_pw() {
   for (const NB& p : nbpw) {
     int i = p.i, j = p.j, i3 = 3 * i, j3 = 3 * j;
     double dx1 = x[j3] - x[i3], dx2 = x[j3 + 1] - x[i3 + 1], dx3 = x[j3 + 2] - x[i3 + 2];
-    double r2 = 1.0 / (dx1 * dx1 + dx2 * dx2 + dx3 * dx3);
+    double aLJ = typ(i).avdw[p.ljcod - 1] * typ(j).avdw[p.ljcod - 1];
+    double bLJ = typ(i).bvdw[p.ljcod - 1] * typ(j).bvdw[p.ljcod - 1];
+    double r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
+    r2 = 1.0 / r2;
     double r = std::sqrt(r2);
     double r6 = r2 * r2 * r2;
+    double r12 = r6 * r6;
     double Vel = s.crg[i] * s.crg[j] * r;
-    double Va = typ(i).avdw[p.ljcod - 1] * typ(j).avdw[p.ljcod - 1] * r6 * r6;
-    double Vb = typ(i).bvdw[p.ljcod - 1] * typ(j).bvdw[p.ljcod - 1] * r6;
+    double Va = aLJ * r12;
+    double Vb = bLJ * r6;
     double dv = r2 * (-Vel - 12.0 * Va + 6.0 * Vb);
     d[i3] = d[i3] - dv * dx1; d[i3 + 1] = d[i3 + 1] - dv * dx2; d[i3 + 2] = d[i3 + 2] - dv * dx3;
     d[j3] = d[j3] + dv * dx1; d[j3 + 1] = d[j3 + 1] + dv * dx2; d[j3 + 2] = d[j3 + 2] + dv * dx3;

@@ -1,0 +1,158 @@
+// Device-side layout of one simulation instance (one system at one set of
+// lambdas) and the host engine that drives a batch of them on one GPU.
+//
+// HBM layout per instance (all FP64 unless noted):
+//   x, v, xx, d      3N AoS, same element order as qdyn's x(:)/v(:)/d(:)
+//   xtop             3N, topology coordinates (restraint reference)
+//   crg, winv, mass  N (crg already scaled by sqrt(coulomb_constant))
+//   flags            N bytes: bit0 excluded, bit1 restrained shell,
+//                    bit2 Q atom, bit3 heavy
+//   lrf              ncgp x 44: centre(3) phi0 phi1(3) phi2(9) phi3(27) pad
+//   ww_nb            nwat x ww_cap int32 neighbour waters (full, ascending)
+//   wp_nb            nwat x wp_cap int32 solute charge groups within Rcpw
+//   pp_nb            ncgp_solute x pp_cap int32 solute charge groups within Rcpp
+//   pw_nb            ncgp_solute x pw_cap int32 waters within Rcpw
+// Lists are "full" (each owner sees all its partners) so that every atom's
+// gradient is produced by exactly one wavefront without atomics; the
+// reference's half lists (charge-group parity ownership, src/md.f90:
+// 10977-10980) are recovered by the ownership predicate and are what the
+// energies sum over and what the list export returns.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "host/qsys.hpp"
+
+namespace qgpu {
+
+enum : unsigned char { F_EXCL = 1, F_SHELL = 2, F_Q = 4, F_HEAVY = 8 };
+
+constexpr int kLrfStride = 44;
+constexpr int kNumShellMax = 8;
+
+// per-instance energy slots reduced on demand (index into DevSys::eacc)
+enum EnergyTerm {
+  E_PP_EL, E_PP_VDW, E_PW_EL, E_PW_VDW, E_WW_EL, E_WW_VDW, E_LRF,
+  E_P_BOND, E_P_ANGLE, E_P_TORSION, E_P_IMPROPER,
+  E_RST_FIX, E_RST_SHELL, E_RST_RADIAL, E_RST_WPOL,
+  E_KINETIC,
+  E_NTERMS
+};
+// per-state Q terms: qq_el qq_vdw qp_el qp_vdw qw_el qw_vdw
+constexpr int kQTerms = 6;
+
+struct DevShell {
+  double rout, dr, cstb, avtheta, avn_insh, theta_corr;
+  int n_insh;
+  int pad;
+};
+
+struct DevSys {
+  int natom, nat_solute, nwat, ncgp, ncgp_solute;
+  int nqat, nstates;
+  int precision;                 // 0 = FP64, 1 = mixed FP32 pair / FP64 accumulation
+  double *x, *v, *xx, *d, *xtop;
+  const double *crg, *winv, *mass;
+  const unsigned char* flags;
+  const int *iac;                // type index per atom
+  const int *cgp_first, *cgp_last, *cgp_sw, *cgpatom, *iwhich_cgp;
+  double* lrf;
+  // type tables (natyps x 3 each), ljcod natyps x natyps
+  int natyps;
+  const double *avdw, *bvdw;
+  const int* ljcod;
+  // solute exclusion / 1-4 bit rows and long lists
+  const unsigned* listex; const unsigned* list14;
+  const int* exlong; int nexlong;      // pairs (i,j)
+  const int* l14long; int n14long;
+  double el14_scale;
+  // lists
+  int ww_cap, wp_cap, pp_cap, pw_cap;
+  int *ww_cnt, *ww_nb, *wp_cnt, *wp_nb, *pp_cnt, *pp_nb, *pw_cnt, *pw_nb;
+  int* overflow;                 // set to 1 when a list row exceeds its cap
+  // cut-offs squared
+  double rcpp2, rcpw2, rcww2, rclrf2;
+  bool use_lrf;
+  // solvent
+  double chg_solv[3];
+  double aOO, bOO;               // alj_solv(1,iLJ)^2, blj_solv(1,iLJ)^2 for iLJ=ljcod(O,O)
+  double rwat, fk_wsphere, dwmz, awmz, fkwpol;
+  double xwcent[3];
+  int wpol_restr;
+  int nshell;
+  DevShell* shell;
+  double* theta;                 // per water
+  int* shell_of;                 // per water, -1 when outside the shells
+  // restraints
+  double fk_fix, fk_pshell;
+  int freeze;
+  // Q atoms
+  const int* iqseq;
+  const double* qcrg;            // nqat x nstates
+  const double* lambda;
+  // integrator
+  double dt;
+  double temp0, tau_t;
+  int separate_scaling, detail_temps;
+  int ndegf, ndegfree, ndegf_solute, ndegf_solvent, ndegfree_solute, ndegfree_solvent;
+  // SHAKE units: every atom belongs to exactly one unit (a SHAKE molecule or
+  // a single unconstrained atom); unit u spans atoms unit_atoms[unit_first[u]
+  // .. unit_first[u+1]) and constraints sh_*[con_first[u] .. con_first[u+1])
+  int nunits;
+  int has_shake;                 // any SHAKE constraint: v = (x - xx)/dt for every atom
+  const int *unit_first, *unit_atoms, *con_first, *sh_i, *sh_j;
+  const double* sh_d2;
+  int* shake_fail;
+  // energies: eacc[term] partial sums written per item, reduced on demand
+  double* e_water;     // nwat x 4 : ww_el, ww_vdw, lrf(water atoms), radial
+  double* e_solute;    // ncgp_solute x 8 : pp_el pp_vdw pw_el pw_vdw lrf fix shell spare
+  double* e_wpol;      // nshell
+  double* ke_part;     // kinetic partial sums: nblocks x 6
+  int ke_blocks;
+  double* scal;        // [0]Tscale_solute [1]Tscale_solvent [2]Tfree [3]Temp [4]Ekin [5..] per-class temps
+  double* energies;    // E_NTERMS reduced values
+};
+
+// One GPU batch of instances sharing every launch.
+class Batch {
+ public:
+  Batch(const std::vector<std::string>& inputs, int precision, int device);
+  ~Batch();
+  int size() const { return (int)inst_.size(); }
+  void step(int nsteps);
+  void make_lists();
+  void forces();
+  void get_state(int k, double* x, double* v);
+  void set_state(int k, const double* x, const double* v);
+  void get_gradient(int k, double* d);
+  void energies(int k, double* e, int cap);
+  void export_list(int k, int which, std::vector<int>& out);
+  double temperature(int k);
+  void theta_corr(int k, double* out);
+  int istep() const { return istep_; }
+  const qsys::Setup& setup(int k) const { return *setups_[k]; }
+  hipStream_t stream() const { return stream_; }
+  // timing helpers
+  hipEvent_t ev0, ev1;
+
+ private:
+  struct Inst;
+  void upload(int k);
+  void launch_lists();
+  void launch_forces();
+  void launch_integrate();
+  void reduce_energies();
+  std::vector<qsys::Setup*> setups_;
+  std::vector<Inst*> inst_;
+  DevSys* d_sys_ = nullptr;
+  std::vector<DevSys> h_sys_;
+  hipStream_t stream_ = nullptr;
+  int istep_ = 0;
+  int precision_ = 0;
+  int max_nwat_ = 0, max_ncgp_ = 0, max_ncgp_solute_ = 0, max_units_ = 0, max_natom_ = 0;
+};
+
+}  // namespace qgpu

@@ -1,0 +1,148 @@
+// Device helpers shared by the engine's kernels: wavefront/workgroup
+// reductions, the charge-group parity ownership rule, the LRF multipole
+// accumulator and Taylor evaluation, and the water-dipole geometry of the
+// polarisation restraint.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "engine.hpp"
+
+namespace qgpu {
+
+constexpr double kPiD = 3.14159265358979323846;
+
+__device__ __forceinline__ double psqrt(double v) { return sqrt(v); }
+__device__ __forceinline__ float psqrt(float v) { return sqrtf(v); }
+
+// 64-lane butterfly sum; every lane returns the total
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <int NW>
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  v = wave_sum(v);
+  const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  if (ln == 0) red[wv] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < NW; ++w) t += red[w];
+  __syncthreads();
+  return t;  // valid in thread 0
+}
+__device__ __forceinline__ double block_sum_256(double v, double* red) { return block_sum<4>(v, red); }
+__device__ __forceinline__ double block_sum_1024(double v, double* red) { return block_sum<16>(v, red); }
+
+// The reference visits a pair of charge groups / waters (a, b), 1-based,
+// only when (a < b and a+b even) or (a > b and a+b odd)
+// (src/md.f90:7700-7702, 10977-10980).  With 0-based indices the parity of
+// a+b is unchanged.
+__device__ __forceinline__ bool owns_pair(int a, int b) {
+  return (a < b) == (((a + b) & 1) == 0);
+}
+
+// LRF multipole accumulator for one charge-group centre: phi0, phi1[3] and
+// the distinct components of the symmetric phi2 (6) and phi3 (10) tensors,
+// accumulated in FP64 as lrf_update (src/md.f90:5896-5981) forms them.
+struct LrfAcc {
+  double p0, p1[3], p2[6], p3[10];
+  __device__ void zero() {
+    p0 = 0; for (int k = 0; k < 3; ++k) p1[k] = 0;
+    for (int k = 0; k < 6; ++k) p2[k] = 0;
+    for (int k = 0; k < 10; ++k) p3[k] = 0;
+  }
+  __device__ void add(double q, double d0, double d1, double d2) {
+    double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+    double f0 = q / (r2 * sqrt(r2));
+    double f1 = 3.0 * f0 / r2;
+    double f2 = -f1 / r2;
+    p0 = p0 + f0 * r2;
+    p1[0] = p1[0] - f0 * d0; p1[1] = p1[1] - f0 * d1; p1[2] = p1[2] - f0 * d2;
+    p2[0] = p2[0] + f1 * d0 * d0 - f0;
+    p2[1] = p2[1] + f1 * d0 * d1;
+    p2[2] = p2[2] + f1 * d0 * d2;
+    p2[3] = p2[3] + f1 * d1 * d1 - f0;
+    p2[4] = p2[4] + f1 * d1 * d2;
+    p2[5] = p2[5] + f1 * d2 * d2 - f0;
+    p3[0] = p3[0] + f2 * (5.0 * d0 * d0 * d0 - r2 * 3.0 * d0);   // 111
+    p3[1] = p3[1] + f2 * (5.0 * d0 * d0 * d1 - r2 * d1);         // 112
+    p3[2] = p3[2] + f2 * (5.0 * d0 * d0 * d2 - r2 * d2);         // 113
+    p3[3] = p3[3] + f2 * (5.0 * d0 * d1 * d1 - r2 * d0);         // 122
+    p3[4] = p3[4] + f2 * (5.0 * d0 * d1 * d2);                   // 123
+    p3[5] = p3[5] + f2 * (5.0 * d0 * d2 * d2 - r2 * d0);         // 133
+    p3[6] = p3[6] + f2 * (5.0 * d1 * d1 * d1 - r2 * 3.0 * d1);   // 222
+    p3[7] = p3[7] + f2 * (5.0 * d1 * d1 * d2 - r2 * d2);         // 223
+    p3[8] = p3[8] + f2 * (5.0 * d1 * d2 * d2 - r2 * d1);         // 233
+    p3[9] = p3[9] + f2 * (5.0 * d2 * d2 * d2 - r2 * 3.0 * d2);   // 333
+  }
+  __device__ void wave_reduce() {
+    p0 = wave_sum(p0);
+    for (int k = 0; k < 3; ++k) p1[k] = wave_sum(p1[k]);
+    for (int k = 0; k < 6; ++k) p2[k] = wave_sum(p2[k]);
+    for (int k = 0; k < 10; ++k) p3[k] = wave_sum(p3[k]);
+  }
+  // write phi0, phi1, full phi2(9) and phi3(27) after the centre (L[0..2])
+  __device__ void store(double* L) const {
+    L[3] = p0;
+    L[4] = p1[0]; L[5] = p1[1]; L[6] = p1[2];
+    const int i2[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+    for (int k = 0; k < 9; ++k) L[7 + k] = p2[i2[k]];
+    // index of sorted (a<=b<=c) triple in p3
+    const int i3[27] = {0, 1, 2, 1, 3, 4, 2, 4, 5,
+                        1, 3, 4, 3, 6, 7, 4, 7, 8,
+                        2, 4, 5, 4, 7, 8, 5, 8, 9};
+    for (int k = 0; k < 27; ++k) L[16 + k] = p3[i3[k]];
+  }
+};
+
+// Add the non-Q atoms of charge group g (as seen from centre c) to acc.
+__device__ __forceinline__ void lrf_gather_group(const DevSys& S, int g, double c0, double c1,
+                                                 double c2, LrfAcc& acc) {
+  for (int a = S.cgp_first[g]; a <= S.cgp_last[g]; ++a) {
+    const int i = S.cgpatom[a];
+    if (S.flags[i] & F_Q) continue;
+    acc.add(S.crg[i], S.x[3 * i] - c0, S.x[3 * i + 1] - c1, S.x[3 * i + 2] - c2);
+  }
+}
+
+// lrf_taylor for one atom at xa with its group's multipoles L: returns the
+// potential Vij and the field df (src/md.f90:5843-5888).
+__device__ __forceinline__ double lrf_taylor_atom(const double* L, const double* xa, double* df) {
+  const double dr0 = L[0] - xa[0], dr1 = L[1] - xa[1], dr2 = L[2] - xa[2];
+  const double* p1 = L + 4;
+  const double* p2 = L + 7;
+  const double* p3 = L + 16;
+  double V = L[3] + p1[0] * dr0 + p1[1] * dr1 + p1[2] * dr2
+           + 0.5 * (p2[0] * dr0 + p2[1] * dr1 + p2[2] * dr2) * dr0
+           + 0.5 * (p2[3] * dr0 + p2[4] * dr1 + p2[5] * dr2) * dr1
+           + 0.5 * (p2[6] * dr0 + p2[7] * dr1 + p2[8] * dr2) * dr2;
+  for (int a = 0; a < 3; ++a) {
+    const double* q = p3 + 9 * a;
+    df[a] = p1[a] + p2[3 * a] * dr0 + p2[3 * a + 1] * dr1 + p2[3 * a + 2] * dr2
+          + 0.5 * (q[0] * dr0 + q[1] * dr1 + q[2] * dr2) * dr0
+          + 0.5 * (q[3] * dr0 + q[4] * dr1 + q[5] * dr2) * dr1
+          + 0.5 * (q[6] * dr0 + q[7] * dr1 + q[8] * dr2) * dr2;
+  }
+  return V;
+}
+
+// Water dipole unit vector, radial unit vector and their cosine for the
+// water whose O is atom i (src/md.f90:18392-18412).
+__device__ __forceinline__ void watpol_geometry(const DevSys& S, int i, double* rmu, double* rcu,
+                                                double& rm, double& rc, double& scp) {
+  const double* x = S.x + 3 * i;
+  for (int k = 0; k < 3; ++k) rmu[k] = x[3 + k] + x[6 + k] - 2.0 * x[k];
+  rm = sqrt(rmu[0] * rmu[0] + rmu[1] * rmu[1] + rmu[2] * rmu[2]);
+  for (int k = 0; k < 3; ++k) rmu[k] = rmu[k] / rm;
+  for (int k = 0; k < 3; ++k) rcu[k] = x[k] - S.xwcent[k];
+  rc = sqrt(rcu[0] * rcu[0] + rcu[1] * rcu[1] + rcu[2] * rcu[2]);
+  for (int k = 0; k < 3; ++k) rcu[k] = rcu[k] / rc;
+  scp = rmu[0] * rcu[0] + rmu[1] * rcu[1] + rmu[2] * rcu[2];
+  if (scp > 1.0) scp = 1.0;
+  if (scp < -1.0) scp = -1.0;
+}
+
+}  // namespace qgpu

@@ -1,0 +1,165 @@
+"""Python view of the qgpu C ABI (include/qgpu.h) over ctypes.
+
+This is the host-side mirror of qdyn's step interface for tests, the smoke
+check and the benchmark.  It loads the in-tree ``libqgpu.so`` and fails
+loudly when the library is missing; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Sequence
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libqgpu.so")
+
+QGPU_FP64 = 0
+QGPU_MIXED = 1
+
+ENERGY_NAMES = [
+    "pp_el", "pp_vdw", "p_bond", "p_angle", "p_torsion", "p_improper",
+    "ww_el", "ww_vdw", "w_bond", "w_angle", "w_torsion", "w_improper",
+    "pw_el", "pw_vdw", "lrf",
+    "qx_el", "qx_vdw", "q_bond", "q_angle", "q_torsion", "q_improper",
+    "rst_total", "rst_fix", "rst_radial", "rst_wpol", "rst_shell", "rst_protein",
+    "total", "potential", "kinetic",
+]
+
+# exported symbols declared in include/qgpu.h
+SYMBOLS = [
+    "qgpu_create", "qgpu_destroy", "qgpu_natom", "qgpu_make_pair_lists", "qgpu_pot_energy",
+    "qgpu_step", "qgpu_get_state", "qgpu_set_state", "qgpu_get_gradient", "qgpu_energies",
+    "qgpu_qenergies", "qgpu_get_pairlist", "qgpu_temperature", "qgpu_theta_corr",
+    "qgpu_last_step_ms", "qgpu_last_error",
+]
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load libqgpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError("libqgpu.so not built: run __graft_entry__.build() (or qsource_amd/build.py)")
+    lib = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    dp = ctypes.POINTER(ctypes.c_double)
+    lib.qgpu_create.argtypes = [ctypes.POINTER(ctypes.c_char_p), ctypes.c_int, ctypes.c_int,
+                                ctypes.POINTER(P)]
+    lib.qgpu_destroy.argtypes = [P]
+    lib.qgpu_natom.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    lib.qgpu_make_pair_lists.argtypes = [P]
+    lib.qgpu_pot_energy.argtypes = [P]
+    lib.qgpu_step.argtypes = [P, ctypes.c_int]
+    lib.qgpu_get_state.argtypes = [P, ctypes.c_int, dp, dp]
+    lib.qgpu_set_state.argtypes = [P, ctypes.c_int, dp, dp]
+    lib.qgpu_get_gradient.argtypes = [P, ctypes.c_int, dp]
+    lib.qgpu_energies.argtypes = [P, ctypes.c_int, dp, ctypes.c_int]
+    lib.qgpu_qenergies.argtypes = [P, ctypes.c_int, dp, ctypes.c_int]
+    lib.qgpu_get_pairlist.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                      ctypes.POINTER(ctypes.c_long)]
+    lib.qgpu_temperature.argtypes = [P, ctypes.c_int, dp]
+    lib.qgpu_theta_corr.argtypes = [P, ctypes.c_int, dp, ctypes.POINTER(ctypes.c_int)]
+    lib.qgpu_last_step_ms.argtypes = [P, dp]
+    lib.qgpu_last_error.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class QgpuError(RuntimeError):
+    pass
+
+
+class Engine:
+    """A batch of qdyn instances (one per input file) on the GPU."""
+
+    def __init__(self, inputs: Sequence[str], precision: int = QGPU_FP64):
+        self.lib = load_library()
+        arr = (ctypes.c_char_p * len(inputs))(*[os.fsencode(p) for p in inputs])
+        h = ctypes.c_void_p()
+        self._check(self.lib.qgpu_create(arr, len(inputs), precision, ctypes.byref(h)))
+        self.h = h
+        self.n = len(inputs)
+        self.natom = []
+        for k in range(self.n):
+            n = ctypes.c_int()
+            self._check(self.lib.qgpu_natom(self.h, k, ctypes.byref(n)))
+            self.natom.append(n.value)
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise QgpuError(self.lib.qgpu_last_error().decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.qgpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n: int = 1):
+        self._check(self.lib.qgpu_step(self.h, n))
+
+    def make_pair_lists(self):
+        self._check(self.lib.qgpu_make_pair_lists(self.h))
+
+    def pot_energy(self):
+        self._check(self.lib.qgpu_pot_energy(self.h))
+
+    def state(self, k: int = 0):
+        x = np.zeros(3 * self.natom[k])
+        v = np.zeros(3 * self.natom[k])
+        self._check(self.lib.qgpu_get_state(self.h, k, _dp(x), _dp(v)))
+        return x, v
+
+    def set_state(self, k: int, x: np.ndarray, v: np.ndarray):
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        v = np.ascontiguousarray(v, dtype=np.float64).ravel()
+        self._check(self.lib.qgpu_set_state(self.h, k, _dp(x), _dp(v)))
+
+    def gradient(self, k: int = 0):
+        d = np.zeros(3 * self.natom[k])
+        self._check(self.lib.qgpu_get_gradient(self.h, k, _dp(d)))
+        return d
+
+    def energies(self, k: int = 0) -> dict:
+        e = np.zeros(len(ENERGY_NAMES))
+        self._check(self.lib.qgpu_energies(self.h, k, _dp(e), len(e)))
+        return dict(zip(ENERGY_NAMES, e.tolist()))
+
+    def pairlist(self, k: int, which: int) -> np.ndarray:
+        n = ctypes.c_long(0)
+        self._check(self.lib.qgpu_get_pairlist(self.h, k, which, None, ctypes.byref(n)))
+        out = np.zeros(3 * max(n.value, 1), dtype=np.int32)
+        self._check(self.lib.qgpu_get_pairlist(self.h, k, which,
+                                               out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                               ctypes.byref(n)))
+        return out[:3 * n.value].reshape(-1, 3)
+
+    def temperature(self, k: int = 0) -> float:
+        t = ctypes.c_double()
+        self._check(self.lib.qgpu_temperature(self.h, k, ctypes.byref(t)))
+        return t.value
+
+    def theta_corr(self, k: int = 0) -> np.ndarray:
+        n = ctypes.c_int()
+        self._check(self.lib.qgpu_theta_corr(self.h, k, None, ctypes.byref(n)))
+        out = np.zeros(max(n.value, 1))
+        self._check(self.lib.qgpu_theta_corr(self.h, k, _dp(out), ctypes.byref(n)))
+        return out[:n.value]
+
+    def last_step_ms(self) -> float:
+        t = ctypes.c_double()
+        self._check(self.lib.qgpu_last_step_ms(self.h, ctypes.byref(t)))
+        return t.value

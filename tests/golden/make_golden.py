@@ -1,0 +1,102 @@
+"""Writes the golden fixtures of reference qdyn runs used by the CPU tests.
+
+Runs the reference's serial qdyn5 (built from the reference sources by
+oracle/Makefile.ref into oracle/_ref/) on systems written by
+qsource_amd.gensys, and stores, per case:
+
+* ``energies``: the write_out energy summary of every step, as printed by
+  qdyn (F12.2): ww_el ww_vdw lrf rst_total rst_fix rst_radial rst_wpol
+  total potential kinetic, plus pp/pw/Q terms where present;
+* ``x``, ``v``: the final restart coordinates and velocities (FP64);
+* ``theta_corr``: the water-polarisation shell corrections of the restart;
+* ``top_sha``: SHA-256 of the generated topology, so a change of the
+  generator is detected instead of silently comparing different systems.
+
+Usage:  python tests/golden/make_golden.py
+"""
+import hashlib
+import os
+import re
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+from qsource_amd import gensys  # noqa: E402
+
+QDYN = os.path.join(ROOT, "oracle", "_ref", "qdyn5")
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+CASES = {
+    # name: (builder kwargs, input kwargs)
+    "c1_r15_100": (dict(radius=15.0, n_solute=0, seed=1001), dict(steps=100)),
+    "c1_r10_250": (dict(radius=10.0, n_solute=0, seed=1001), dict(steps=250)),
+}
+
+ROW = re.compile(r"^(solute|solvent|solute-solvent|LRF|Q-atom|restraints|SUM)\s+(.*)$")
+
+
+def parse_summaries(log: str):
+    """Energy summaries of every step: dict per step."""
+    out = []
+    for blk in re.split(r"=+\s+Energy summary at step\s+\d+\s+=+|=+\s+FINAL\s+Energy summary\s+=+", log)[1:]:
+        rows = {}
+        for line in blk.splitlines():
+            m = ROW.match(line.strip())
+            if m and m.group(1) not in rows:
+                rows[m.group(1)] = [float(t) for t in m.group(2).split()]
+            if line.startswith("===") and rows:
+                break
+        out.append(rows)
+    return out
+
+
+def flatten(rows):
+    sv = rows.get("solvent", [0] * 6)
+    so = rows.get("solute", [0] * 6)
+    pw = rows.get("solute-solvent", [0, 0])
+    lrf = rows.get("LRF", [0])[0]
+    q = rows.get("Q-atom", [0] * 6)
+    r = rows["restraints"]
+    s = rows["SUM"]
+    return [sv[0], sv[1], lrf, r[0], r[1], r[2], r[3], s[0], s[1], s[2],
+            so[0], so[1], so[2], so[3], so[4], so[5], pw[0], pw[1], q[0], q[1], r[4], r[5]]
+
+
+COLUMNS = ["ww_el", "ww_vdw", "lrf", "rst_total", "rst_fix", "rst_radial", "rst_wpol",
+           "total", "potential", "kinetic", "pp_el", "pp_vdw", "p_bond", "p_angle",
+           "p_torsion", "p_improper", "pw_el", "pw_vdw", "qx_el", "qx_vdw",
+           "rst_shell", "rst_protein"]
+
+
+def build_case(name, sysargs, inpargs, workdir):
+    s = gensys.solvated_system(**sysargs)
+    paths = gensys.write_system(s, workdir, name, output=1, temp_interval=1, **inpargs)
+    return s, paths
+
+
+def main():
+    if not os.path.exists(QDYN):
+        sys.exit("reference qdyn5 not built: make -f oracle/Makefile.ref OUT=oracle/_ref")
+    for name, (sysargs, inpargs) in CASES.items():
+        with tempfile.TemporaryDirectory() as d:
+            s, paths = build_case(name, sysargs, inpargs, d)
+            r = subprocess.run([QDYN, name + ".inp"], cwd=d, stdout=subprocess.PIPE,
+                               stderr=subprocess.STDOUT, text=True)
+            if "terminated normally" not in r.stdout:
+                sys.exit("reference run failed for %s:\n%s" % (name, r.stdout[-3000:]))
+            summ = parse_summaries(r.stdout)
+            en = np.array([flatten(x) for x in summ])
+            x, v, tc = gensys.read_restart(paths["re"])
+            sha = hashlib.sha256(open(paths["top"], "rb").read()).hexdigest()
+            np.savez_compressed(os.path.join(OUT, name + ".npz"), energies=en, columns=np.array(COLUMNS),
+                                x=x, v=v, theta_corr=tc if tc is not None else np.zeros(0),
+                                top_sha=np.array(sha))
+            print(name, en.shape, x.shape, sha[:12])
+
+
+if __name__ == "__main__":
+    main()
