@@ -671,7 +671,11 @@ void Oracle::nbpwlist_lrf() {
           if (s.iqatom[i] != 0) continue;
           for (int k = 0; k < 3; ++k) {
             int j = ja + k;
-            nbpw.push_back({i, j, ljcod(s.iac[i], s.iac[j])});
+            int lj = ljcod(s.iac[i], s.iac[j]);
+            if (s.crg[i] * s.crg[j] == 0.0 && typ(i).avdw[lj - 1] * typ(j).avdw[lj - 1] == 0.0 &&
+                typ(i).bvdw[lj - 1] * typ(j).bvdw[lj - 1] == 0.0)
+              continue;
+            nbpw.push_back({i, j, lj});
           }
         }
       } else if (r2 <= rclrf2) {
@@ -682,13 +686,59 @@ void Oracle::nbpwlist_lrf() {
   }
 }
 
-void Oracle::nbqplist() {
-  if (s.nqat == 0) return;
-  throw std::runtime_error("Q-atom lists not yet restated");
+// connectivity of solute atom a to Q atom iq in state st (make_qconn)
+static inline int qconn_of(const Setup& s, int st, int iq, int a) {
+  return s.qconn[((size_t)st * s.nqat + iq) * s.nat_solute + a];
 }
+
+void Oracle::nbqplist() {
+  // static once Rcq > rexcl_o (src/md.f90:10245)
+  if (qp_done && s.inp.rcq > s.top.rexcl_o) return;
+  nbqp.clear();
+  if (s.nqat == 0) { qp_done = true; return; }
+  const auto& T = s.top;
+  const double rcut2 = s.inp.rcq * s.inp.rcq;
+  for (int ig = 0; ig < T.ncgp_solute; ++ig) {
+    int ia = T.cgp[ig].iswitch;
+    if (s.excl[ia]) continue;
+    int i3 = 3 * ia;
+    double r2 = (x[i3] - T.xpcent[0]) * (x[i3] - T.xpcent[0]) + (x[i3 + 1] - T.xpcent[1]) * (x[i3 + 1] - T.xpcent[1])
+              + (x[i3 + 2] - T.xpcent[2]) * (x[i3 + 2] - T.xpcent[2]);
+    if (r2 > rcut2) continue;
+    for (int a = T.cgp[ig].first; a <= T.cgp[ig].last; ++a) {
+      int i = T.cgpatom[a];
+      bool conn = false;
+      for (int st = 0; st < s.nstates && !conn; ++st)
+        for (int iq = 0; iq < s.nqat && !conn; ++iq) conn = qconn_of(s, st, iq, i) <= 3;
+      if (conn) continue;
+      for (int iq = 0; iq < s.nqat; ++iq) {
+        int lj = ljcod(s.iac[i], s.iac[s.iqseq[iq]]);
+        if (qconn_of(s, 0, iq, i) == 4) lj = 3;
+        nbqp.push_back({iq, i, lj});
+      }
+    }
+  }
+  qp_done = true;
+}
+
 void Oracle::nbqwlist() {
-  if (s.nqat == 0) return;
-  throw std::runtime_error("Q-atom lists not yet restated");
+  if (qw_done && s.inp.rcq > s.top.rexcl_o) return;
+  nbqw.clear();
+  if (s.nqat == 0) { qw_done = true; return; }
+  const auto& T = s.top;
+  const double rcut2 = s.inp.rcq * s.inp.rcq;
+  for (int iw = 0; iw < s.nwat; ++iw) {
+    int ia = s.nat_solute + 3 * iw;
+    if (s.excl[ia]) continue;
+    int i3 = 3 * ia;
+    double r2 = (x[i3] - T.xpcent[0]) * (x[i3] - T.xpcent[0]) + (x[i3 + 1] - T.xpcent[1]) * (x[i3 + 1] - T.xpcent[1])
+              + (x[i3 + 2] - T.xpcent[2]) * (x[i3 + 2] - T.xpcent[2]);
+    if (r2 <= rcut2)
+      for (int iq = 0; iq < s.nqat; ++iq)
+        for (int is = 0; is < 3; ++is)
+          nbqw.push_back({iq, ia + is, ljcod(s.iac[s.nat_solute + is], s.iac[s.iqseq[iq]])});
+  }
+  qw_done = true;
 }
 void Oracle::nonbond_pp() {
   for (const NB& p : nbpp) {
@@ -729,23 +779,228 @@ void Oracle::nonbond_pw() {
     E.pw_vdw = E.pw_vdw + Va - Vb;
   }
 }
-void Oracle::nonbond_qp() {}
-void Oracle::nonbond_qw_spc() {}
-void Oracle::nonbond_qq() {}
+void Oracle::nonbond_qp() {
+  const int ns = s.nstates;
+  for (const NBQ& p : nbqp) {
+    int iq = p.iq, i = s.iqseq[iq], j = p.j, i3 = 3 * i, j3 = 3 * j;
+    int iLJ = p.ljcod;
+    double dx1 = x[j3] - x[i3], dx2 = x[j3 + 1] - x[i3 + 1], dx3 = x[j3 + 2] - x[i3 + 2];
+    double r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
+    r2 = 1.0 / r2;
+    double r6 = r2 * r2 * r2;
+    double r = std::sqrt(r2);
+    for (int st = 0; st < ns; ++st) {
+      double aLJ = typ(i).avdw[iLJ - 1] * typ(j).avdw[iLJ - 1];
+      double bLJ = typ(i).bvdw[iLJ - 1] * typ(j).bvdw[iLJ - 1];
+      double Vel = s.qcrg[iq * ns + st] * s.crg[j] * r;
+      if (iLJ == 3) Vel = Vel * s.top.el14_scale;
+      double Va = aLJ * r6 * r6;
+      double Vb = bLJ * r6;
+      double dv = r2 * (-Vel - 12.0 * Va + 6.0 * Vb) * s.lambda[st];
+      d[i3] = d[i3] - dv * dx1; d[i3 + 1] = d[i3 + 1] - dv * dx2; d[i3 + 2] = d[i3 + 2] - dv * dx3;
+      d[j3] = d[j3] + dv * dx1; d[j3 + 1] = d[j3 + 1] + dv * dx2; d[j3 + 2] = d[j3 + 2] + dv * dx3;
+      EQ[st].qp_el = EQ[st].qp_el + Vel;
+      EQ[st].qp_vdw = EQ[st].qp_vdw + Va - Vb;
+    }
+  }
+}
+
+void Oracle::nonbond_qw_spc() {
+  const int ns = s.nstates;
+  for (size_t iw = 0; iw < nbqw.size(); iw += 3) {
+    int iq = nbqw[iw].iq, i = s.iqseq[iq], i3 = 3 * i;
+    int j = nbqw[iw].j, j3 = 3 * j;
+    double dx1 = x[j3] - x[i3], dx2 = x[j3 + 1] - x[i3 + 1], dx3 = x[j3 + 2] - x[i3 + 2];
+    double r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
+    double r6_hc = r2 * r2 * r2;
+    r2 = 1.0 / r2;
+    double r = std::sqrt(r2);
+    double r6_sc = r6_hc;
+    int iLJ = nbqw[iw].ljcod;
+    double r6 = 1.0 / r6_hc;
+    double aLJ = typ(i).avdw[iLJ - 1];
+    double bLJ = typ(i).bvdw[iLJ - 1];
+    double Va = aLJ * s.alj_solv[0][iLJ - 1] * r6 * r6;
+    double Vb = bLJ * s.blj_solv[0][iLJ - 1] * r6;
+    double dv = 0;
+    for (int st = 0; st < ns; ++st) {
+      double Vel = s.chg_solv[0] * s.qcrg[iq * ns + st] * r;
+      dv = dv + r2 * (-Vel - ((12.0 * Va - 6.0 * Vb) * (r6_hc / r6_sc))) * s.lambda[st];
+      EQ[st].qw_el = EQ[st].qw_el + Vel;
+      EQ[st].qw_vdw = EQ[st].qw_vdw + Va - Vb;
+    }
+    d[i3] = d[i3] - dv * dx1; d[i3 + 1] = d[i3 + 1] - dv * dx2; d[i3 + 2] = d[i3 + 2] - dv * dx3;
+    d[j3] = d[j3] + dv * dx1; d[j3 + 1] = d[j3 + 1] + dv * dx2; d[j3 + 2] = d[j3 + 2] + dv * dx3;
+    for (int ip = 1; ip < 3; ++ip) {
+      int ja = j + ip, ja3 = 3 * ja;
+      double ex1 = x[ja3] - x[i3], ex2 = x[ja3 + 1] - x[i3 + 1], ex3 = x[ja3 + 2] - x[i3 + 2];
+      double q2 = 1.0 / (ex1 * ex1 + ex2 * ex2 + ex3 * ex3);
+      double rr = std::sqrt(q2);
+      double dvh = 0;
+      for (int st = 0; st < ns; ++st) {
+        double Vel = s.chg_solv[ip] * s.qcrg[iq * ns + st] * rr;
+        EQ[st].qw_el = EQ[st].qw_el + Vel;
+        dvh = dvh - q2 * Vel * s.lambda[st];
+      }
+      d[i3] = d[i3] - dvh * ex1; d[i3 + 1] = d[i3 + 1] - dvh * ex2; d[i3 + 2] = d[i3 + 2] - dvh * ex3;
+      d[ja3] = d[ja3] + dvh * ex1; d[ja3 + 1] = d[ja3 + 1] + dvh * ex2; d[ja3 + 2] = d[ja3 + 2] + dvh * ex3;
+    }
+  }
+}
+
+void Oracle::nonbond_qq() {
+  const int ns = s.nstates;
+  for (int st = 0; st < ns; ++st) {
+    for (const NbqPair& p : s.nbqq[st]) {
+      int iq = p.iq, i = s.iqseq[iq], j = p.j, jq = p.jq;
+      int i3 = 3 * i, j3 = 3 * j;
+      int iLJ = p.ljcod;
+      double qi = s.qcrg[iq * ns + st];
+      double el_scale = p.el_scale;
+      double aLJ = typ(i).avdw[iLJ - 1] * typ(j).avdw[iLJ - 1];
+      double bLJ = typ(i).bvdw[iLJ - 1] * typ(j).bvdw[iLJ - 1];
+      double qj = jq >= 0 ? s.qcrg[jq * ns + st] : s.crg[j];
+      double dx1 = x[j3] - x[i3], dx2 = x[j3 + 1] - x[i3 + 1], dx3 = x[j3 + 2] - x[i3 + 2];
+      double r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
+      double r6_hc = r2 * r2 * r2;
+      double r6 = r2 * r2 * r2 + 0.0;
+      r6 = 1.0 / r6;
+      double r12 = r6 * r6;
+      r2 = 1.0 / r2;
+      double r = std::sqrt(r2);
+      double Vel = qi * qj * r * el_scale;
+      if (iLJ == 3) Vel = Vel * s.top.el14_scale;
+      double Va = aLJ * r12;
+      double Vb = bLJ * r6;
+      double dv = r2 * (-Vel - (12.0 * Va - 6.0 * Vb) * r6_hc * r6) * s.lambda[st];
+      d[i3] = d[i3] - dv * dx1; d[i3 + 1] = d[i3 + 1] - dv * dx2; d[i3 + 2] = d[i3 + 2] - dv * dx3;
+      d[j3] = d[j3] + dv * dx1; d[j3 + 1] = d[j3 + 1] + dv * dx2; d[j3 + 2] = d[j3 + 2] + dv * dx3;
+      if (jq >= 0) {
+        EQ[st].qq_el = EQ[st].qq_el + Vel;
+        EQ[st].qq_vdw = EQ[st].qq_vdw + Va - Vb;
+      } else {
+        EQ[st].qp_el = EQ[st].qp_el + Vel;
+        EQ[st].qp_vdw = EQ[st].qp_vdw + Va - Vb;
+      }
+    }
+  }
+}
+
 double Oracle::bond(int a, int b) {
-  if (a < b) throw std::runtime_error("solute bonds not yet restated");
-  return 0.0;
+  double e = 0.0;
+  for (int ib = a; ib < b; ++ib) {
+    const Bond& B = s.bnd[ib];
+    const BondLib& L = s.top.bondlib[B.cod - 1];
+    int i3 = 3 * B.i, j3 = 3 * B.j;
+    double r0 = x[j3] - x[i3], r1 = x[j3 + 1] - x[i3 + 1], r2 = x[j3 + 2] - x[i3 + 2];
+    double bb = std::sqrt(r0 * r0 + r1 * r1 + r2 * r2);
+    double db = bb - L.b0;
+    e = e + 0.5 * L.fk * (db * db);
+    double dv = L.fk * db / bb;
+    d[j3] = d[j3] + r0 * dv; d[j3 + 1] = d[j3 + 1] + r1 * dv; d[j3 + 2] = d[j3 + 2] + r2 * dv;
+    d[i3] = d[i3] - r0 * dv; d[i3 + 1] = d[i3 + 1] - r1 * dv; d[i3 + 2] = d[i3 + 2] - r2 * dv;
+  }
+  return e;
 }
+
 double Oracle::angle(int a, int b) {
-  if (a < b) throw std::runtime_error("solute angles not yet restated");
-  return 0.0;
+  double e = 0.0;
+  for (int ia = a; ia < b; ++ia) {
+    const Angle& A = s.ang[ia];
+    const AngLib& L = s.top.anglib[A.cod - 1];
+    int i3 = 3 * A.i, j3 = 3 * A.j, k3 = 3 * A.k;
+    double rji[3], rjk[3];
+    for (int c = 0; c < 3; ++c) { rji[c] = x[i3 + c] - x[j3 + c]; rjk[c] = x[k3 + c] - x[j3 + c]; }
+    double bji2inv = 1.0 / (rji[0] * rji[0] + rji[1] * rji[1] + rji[2] * rji[2]);
+    double bjk2inv = 1.0 / (rjk[0] * rjk[0] + rjk[1] * rjk[1] + rjk[2] * rjk[2]);
+    double bjiinv = std::sqrt(bji2inv), bjkinv = std::sqrt(bjk2inv);
+    double scp = (rji[0] * rjk[0] + rji[1] * rjk[1] + rji[2] * rjk[2]);
+    scp = scp * bjiinv * bjkinv;
+    if (scp > 1.0) scp = 1.0;
+    else if (scp < -1.0) scp = -1.0;
+    double angv = std::acos(scp);
+    double da = angv - L.ang0;
+    e = e + 0.5 * L.fk * (da * da);
+    double dv = L.fk * da;
+    double f1 = std::sin(angv);
+    if (std::fabs(f1) < kQrealEps) f1 = kQrealEps;
+    f1 = -1.0 / f1;
+    double di[3], dk[3];
+    for (int c = 0; c < 3; ++c) {
+      di[c] = f1 * (rjk[c] * bjiinv * bjkinv - scp * rji[c] * bji2inv);
+      dk[c] = f1 * (rji[c] * bjiinv * bjkinv - scp * rjk[c] * bjk2inv);
+    }
+    for (int c = 0; c < 3; ++c) d[i3 + c] = d[i3 + c] + dv * di[c];
+    for (int c = 0; c < 3; ++c) d[k3 + c] = d[k3 + c] + dv * dk[c];
+    for (int c = 0; c < 3; ++c) d[j3 + c] = d[j3 + c] - dv * (di[c] + dk[c]);
+  }
+  return e;
 }
+
 double Oracle::torsion(int a, int b) {
-  if (a < b) throw std::runtime_error("solute torsions not yet restated");
-  return 0.0;
+  double e = 0.0;
+  for (int ip = b - 1; ip >= a; --ip) {
+    const Torsion& t = s.tor[ip];
+    const TorLib& L = s.top.torlib[t.cod - 1];
+    int i3 = 3 * t.i, j3 = 3 * t.j, k3 = 3 * t.k, l3 = 3 * t.l;
+    double rji[3], rjk[3], rkl[3], rnj[3], rnk[3];
+    for (int c = 0; c < 3; ++c) {
+      rji[c] = x[i3 + c] - x[j3 + c];
+      rjk[c] = x[k3 + c] - x[j3 + c];
+      rkl[c] = x[l3 + c] - x[k3 + c];
+    }
+    rnj[0] = rji[1] * rjk[2] - rji[2] * rjk[1];
+    rnj[1] = rji[2] * rjk[0] - rji[0] * rjk[2];
+    rnj[2] = rji[0] * rjk[1] - rji[1] * rjk[0];
+    rnk[0] = -rjk[1] * rkl[2] + rjk[2] * rkl[1];
+    rnk[1] = -rjk[2] * rkl[0] + rjk[0] * rkl[2];
+    rnk[2] = -rjk[0] * rkl[1] + rjk[1] * rkl[0];
+    double bj2inv = 1.0 / (rnj[0] * rnj[0] + rnj[1] * rnj[1] + rnj[2] * rnj[2]);
+    double bk2inv = 1.0 / (rnk[0] * rnk[0] + rnk[1] * rnk[1] + rnk[2] * rnk[2]);
+    double bjinv = std::sqrt(bj2inv), bkinv = std::sqrt(bk2inv);
+    double scp = (rnj[0] * rnk[0] + rnj[1] * rnk[1] + rnj[2] * rnk[2]) * (bjinv * bkinv);
+    double phi;
+    if (scp > 1.0) { scp = 1.0; phi = std::acos(1.0); }
+    else if (scp < -1.0) { scp = -1.0; phi = std::acos(-1.0); }
+    else phi = std::acos(scp);
+    if (rjk[0] * (rnj[1] * rnk[2] - rnj[2] * rnk[1]) + rjk[1] * (rnj[2] * rnk[0] - rnj[0] * rnk[2]) +
+            rjk[2] * (rnj[0] * rnk[1] - rnj[1] * rnk[0]) < 0)
+      phi = -phi;
+    double arg = L.rmult * phi - L.deltor;
+    e = e + L.fk * (1.0 + std::cos(arg)) * L.paths;
+    double dv = -L.rmult * L.fk * std::sin(arg) * L.paths;
+    double f1 = std::sin(phi);
+    if (std::fabs(f1) < kQrealEps) f1 = kQrealEps;
+    f1 = -1.0 / f1;
+    double di[3], dl[3], rki[3], rlj[3], dp[12];
+    for (int c = 0; c < 3; ++c) {
+      di[c] = f1 * (rnk[c] * (bjinv * bkinv) - scp * rnj[c] * bj2inv);
+      dl[c] = f1 * (rnj[c] * (bjinv * bkinv) - scp * rnk[c] * bk2inv);
+      rki[c] = rji[c] - rjk[c];
+      rlj[c] = -rjk[c] - rkl[c];
+    }
+    dp[0] = rjk[1] * di[2] - rjk[2] * di[1];
+    dp[1] = rjk[2] * di[0] - rjk[0] * di[2];
+    dp[2] = rjk[0] * di[1] - rjk[1] * di[0];
+    dp[3] = rki[1] * di[2] - rki[2] * di[1] + rkl[1] * dl[2] - rkl[2] * dl[1];
+    dp[4] = rki[2] * di[0] - rki[0] * di[2] + rkl[2] * dl[0] - rkl[0] * dl[2];
+    dp[5] = rki[0] * di[1] - rki[1] * di[0] + rkl[0] * dl[1] - rkl[1] * dl[0];
+    dp[6] = rlj[1] * dl[2] - rlj[2] * dl[1] - rji[1] * di[2] + rji[2] * di[1];
+    dp[7] = rlj[2] * dl[0] - rlj[0] * dl[2] - rji[2] * di[0] + rji[0] * di[2];
+    dp[8] = rlj[0] * dl[1] - rlj[1] * dl[0] - rji[0] * di[1] + rji[1] * di[0];
+    dp[9] = rjk[1] * dl[2] - rjk[2] * dl[1];
+    dp[10] = rjk[2] * dl[0] - rjk[0] * dl[2];
+    dp[11] = rjk[0] * dl[1] - rjk[1] * dl[0];
+    for (int c = 0; c < 3; ++c) d[i3 + c] = d[i3 + c] + dv * dp[c];
+    for (int c = 0; c < 3; ++c) d[j3 + c] = d[j3 + c] + dv * dp[3 + c];
+    for (int c = 0; c < 3; ++c) d[k3 + c] = d[k3 + c] + dv * dp[6 + c];
+    for (int c = 0; c < 3; ++c) d[l3 + c] = d[l3 + c] + dv * dp[9 + c];
+  }
+  return e;
 }
+
 double Oracle::improper(int a, int b) {
-  if (a < b) throw std::runtime_error("solute impropers not yet restated");
+  if (a < b) throw std::runtime_error("improper torsions are not restated (none in these systems)");
   return 0.0;
 }
 

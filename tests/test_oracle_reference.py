@@ -20,12 +20,16 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 CASES = {
     "c1_r15_100": (dict(radius=15.0, n_solute=0, seed=1001), dict(steps=100)),
     "c1_r10_250": (dict(radius=10.0, n_solute=0, seed=1001), dict(steps=250)),
+    "c2_r14_q8_30": (dict(radius=14.0, n_solute=40, seed=2001, n_q=8, n_states=2),
+                     dict(steps=30, lambdas=[0.7, 0.3])),
 }
 
 ORACLE_COLS = {"ww_el": "ww_el", "ww_vdw": "ww_vdw", "lrf": "lrf", "rst_total": "rst_total",
                "rst_fix": "rst_fix", "rst_radial": "rst_radial", "rst_wpol": "rst_wpol",
                "total": "total", "potential": "potential", "kinetic": "kinetic",
-               "pp_el": "pp_el", "pp_vdw": "pp_vdw", "pw_el": "pw_el", "pw_vdw": "pw_vdw"}
+               "pp_el": "pp_el", "pp_vdw": "pp_vdw", "pw_el": "pw_el", "pw_vdw": "pw_vdw",
+               "p_bond": "p_bond", "p_angle": "p_angle", "p_torsion": "p_torsion",
+               "qx_el": "qx_el", "qx_vdw": "qx_vdw"}
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
