@@ -16,7 +16,7 @@ PKG = os.path.join(ROOT, "qsource_amd")
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libqgpu.so")
 
-HIP_SOURCES = ["engine.hip", "kernels_water.hip"]
+HIP_SOURCES = ["engine.hip", "kernels_water.hip", "kernels_solute.hip"]
 HOST_SOURCES = ["host/readers.cpp", "host/prep.cpp"]
 
 # -packed-fp32-ops off: hand-off of FP32 pairs to v_pk_* is disabled;

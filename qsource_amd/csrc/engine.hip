@@ -3,6 +3,7 @@
 // exports the C ABI declared in include/qgpu.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -24,6 +25,11 @@ __global__ void k_watpol_force(const DevSys*, int);
 __global__ void k_integrate(const DevSys*);
 __global__ void k_temperature(const DevSys*, int);
 __global__ void k_energy_reduce(const DevSys*);
+__global__ void k_solute_lists(const DevSys*);
+template <typename P> __global__ void k_solute_forces(const DevSys*);
+template <typename P> __global__ void k_q_forces(const DevSys*);
+__global__ void k_qq_forces(const DevSys*);
+__global__ void k_bonded(const DevSys*);
 
 #define HIPCHECK(expr)                                                            \
   do {                                                                            \
@@ -125,9 +131,14 @@ Batch::~Batch() {
 void Batch::upload(int k) {
   const qsys::Setup& s = *setups_[k];
   const qsys::Topology& t = s.top;
-  if (s.nat_solute > 0) throw std::runtime_error("solute atoms are not yet supported by the GPU engine");
-  if (s.nqat > 0) throw std::runtime_error("Q atoms are not yet supported by the GPU engine");
   if (s.inp.freeze) throw std::runtime_error("excluded_freeze is not supported");
+  if (!s.imp.empty()) throw std::runtime_error("improper torsions are not supported by the GPU engine");
+  if (s.nqat > 0 && !(s.inp.rcq > t.rexcl_o))
+    throw std::runtime_error("Q-atom cut-off must exceed the exclusion radius (static Q lists)");
+  if (s.nstates > kMaxQStates) throw std::runtime_error("too many FEP states");
+  for (int g = 0; g < t.ncgp_solute; ++g)
+    if (t.cgp[g].last - t.cgp[g].first + 1 > kMaxGroupAtoms)
+      throw std::runtime_error("solute charge groups of more than 8 atoms are not supported");
   if (!s.inp.use_lrf) throw std::runtime_error("runs without LRF are not supported");
   auto* I = new Inst();
   inst_.push_back(I);
@@ -170,6 +181,14 @@ void Batch::upload(int k) {
   D.bvdw = I->upload(bv);
   D.ljcod = I->upload(s.ljcod);
   D.el14_scale = t.el14_scale;
+  // exclusion and 1-4 tables (solute rows) and long lists
+  D.listex = I->upload(std::vector<unsigned>(t.listex.begin(), t.listex.end()));
+  D.list14 = I->upload(std::vector<unsigned>(t.list14.begin(), t.list14.end()));
+  std::vector<int> exl, l14l;
+  for (auto& p : t.listexlong) { exl.push_back(p.first); exl.push_back(p.second); }
+  for (auto& p : t.list14long) { l14l.push_back(p.first); l14l.push_back(p.second); }
+  D.exlong = I->upload(exl); D.nexlong = (int)t.listexlong.size();
+  D.l14long = I->upload(l14l); D.n14long = (int)t.list14long.size();
   // lists
   D.ww_cap = std::max(1, std::min(s.nwat, 448));
   D.wp_cap = std::max(1, std::min(t.ncgp_solute, 1024));
@@ -207,6 +226,122 @@ void Batch::upload(int k) {
   D.theta = I->alloc<double>(s.nwat);
   D.shell_of = I->alloc<int>(s.nwat);
   D.fk_fix = s.inp.fk_fix; D.fk_pshell = s.inp.fk_pshell; D.freeze = s.inp.freeze;
+  for (int k2 = 0; k2 < 2; ++k2) { D.alj_O[k2] = s.alj_solv[0][k2]; D.blj_O[k2] = s.blj_solv[0][k2]; }
+  // Q atoms: static Q-solute and Q-water sets as nbqplist/nbqwlist build
+  // them at the first list update (src/md.f90:10215-10322, 10464-10549)
+  D.iqseq = I->upload(s.iqseq);
+  D.qcrg = I->upload(s.qcrg);
+  D.lambda = I->upload(s.lambda);
+  {
+    const int nq = s.nqat, nsl = s.nat_solute;
+    auto qconn = [&](int st, int iq, int a) { return s.qconn[((size_t)st * nq + iq) * nsl + a]; };
+    std::vector<signed char> qp_lj((size_t)std::max(1, nsl * nq), 0), qw_lj(std::max(1, nq), 0);
+    std::vector<int> qp_atoms, qw_waters;
+    const double rcq2 = s.inp.rcq * s.inp.rcq;
+    auto dist2c = [&](int a) {
+      double d0 = s.x[3 * a] - t.xpcent[0], d1 = s.x[3 * a + 1] - t.xpcent[1], d2 = s.x[3 * a + 2] - t.xpcent[2];
+      return d0 * d0 + d1 * d1 + d2 * d2;
+    };
+    if (nq > 0) {
+      for (int g = 0; g < t.ncgp_solute; ++g) {
+        int sw = t.cgp[g].iswitch;
+        if (s.excl[sw] || dist2c(sw) > rcq2) continue;
+        for (int a = t.cgp[g].first; a <= t.cgp[g].last; ++a) {
+          int i = t.cgpatom[a];
+          bool conn = false;
+          for (int st = 0; st < s.nstates && !conn; ++st)
+            for (int iq = 0; iq < nq && !conn; ++iq) conn = qconn(st, iq, i) <= 3;
+          if (conn) continue;
+          qp_atoms.push_back(i);
+          flags[i] |= F_QP;
+          for (int iq = 0; iq < nq; ++iq) {
+            int lj = s.ljcod[s.iac[i] * s.natyps + s.iac[s.iqseq[iq]]];
+            if (qconn(0, iq, i) == 4) lj = 3;
+            qp_lj[(size_t)i * nq + iq] = (signed char)lj;
+          }
+        }
+      }
+      std::sort(qp_atoms.begin(), qp_atoms.end());
+      for (int iw = 0; iw < s.nwat; ++iw) {
+        int o = s.nat_solute + 3 * iw;
+        if (s.excl[o] || dist2c(o) > rcq2) continue;
+        qw_waters.push_back(iw);
+        flags[o] |= F_QW;
+      }
+      for (int iq = 0; iq < nq; ++iq)
+        qw_lj[iq] = (signed char)s.ljcod[s.iac[s.nat_solute] * s.natyps + s.iac[s.iqseq[iq]]];
+      HIPCHECK(hipMemcpy(const_cast<unsigned char*>(D.flags), flags.data(), N, hipMemcpyHostToDevice));
+    }
+    D.qp_lj = I->upload(qp_lj);
+    D.qw_lj = I->upload(qw_lj);
+    D.nqp_atoms = (int)qp_atoms.size();
+    D.nqw_waters = (int)qw_waters.size();
+    D.qp_atoms = I->upload(qp_atoms);
+    D.qw_waters = I->upload(qw_waters);
+    D.e_q = I->alloc<double>((size_t)std::max(1, nq * s.nstates) * 4);
+    // Q-Q / Q-neighbour pairs, flattened over states, with atom incidence
+    std::vector<int> qst, qiq, qj, qjq, qlj;
+    std::vector<double> qel;
+    for (int st = 0; st < s.nstates; ++st)
+      for (auto& pr : s.nbqq[st]) {
+        qst.push_back(st); qiq.push_back(pr.iq); qj.push_back(pr.j); qjq.push_back(pr.jq);
+        qlj.push_back(pr.ljcod); qel.push_back(pr.el_scale);
+      }
+    D.nqq = (int)qst.size();
+    D.qq_state = I->upload(qst); D.qq_iq = I->upload(qiq); D.qq_j = I->upload(qj);
+    D.qq_jq = I->upload(qjq); D.qq_lj = I->upload(qlj); D.qq_elscale = I->upload(qel);
+    D.e_qq = I->alloc<double>((size_t)std::max(1, D.nqq) * 2);
+    std::vector<std::vector<int>> inc(N);
+    for (int p = 0; p < D.nqq; ++p) {
+      inc[s.iqseq[qiq[p]]].push_back(p);
+      inc[qj[p]].push_back(p);
+    }
+    std::vector<int> qa, qf, qpairs;
+    for (int i = 0; i < N; ++i)
+      if (!inc[i].empty()) {
+        qa.push_back(i);
+        qf.push_back((int)qpairs.size());
+        qpairs.insert(qpairs.end(), inc[i].begin(), inc[i].end());
+      }
+    qf.push_back((int)qpairs.size());
+    D.nqq_atoms = (int)qa.size();
+    D.qq_atom = I->upload(qa); D.qq_first = I->upload(qf); D.qq_pair = I->upload(qpairs);
+    max_nqat_ = std::max(max_nqat_, nq);
+    max_nqq_atoms_ = std::max(max_nqq_atoms_, D.nqq_atoms);
+  }
+  // bonded terms after shrink_topology, with atom incidence
+  {
+    std::vector<int> bv, av2, tv;
+    std::vector<double> bl, al, tl;
+    for (auto& b : s.bnd) { bv.push_back(b.i); bv.push_back(b.j); bv.push_back(b.cod); }
+    for (auto& a : s.ang) { av2.push_back(a.i); av2.push_back(a.j); av2.push_back(a.k); av2.push_back(a.cod); }
+    for (auto& q : s.tor) { tv.push_back(q.i); tv.push_back(q.j); tv.push_back(q.k); tv.push_back(q.l); tv.push_back(q.cod); }
+    for (auto& b : t.bondlib) { bl.push_back(b.fk); bl.push_back(b.b0); }
+    for (auto& a : t.anglib) { al.push_back(a.fk); al.push_back(a.ang0); }
+    for (auto& q : t.torlib) { tl.push_back(q.fk); tl.push_back(q.rmult); tl.push_back(q.deltor); tl.push_back(q.paths); }
+    D.bnd = I->upload(bv); D.ang = I->upload(av2); D.tor = I->upload(tv);
+    D.bondlib = I->upload(bl); D.anglib = I->upload(al); D.torlib = I->upload(tl);
+    D.nbnd = (int)s.bnd.size(); D.nbnd_solute = s.nbonds_solute;
+    D.nang = (int)s.ang.size(); D.nang_solute = s.nangles_solute;
+    D.ntor = (int)s.tor.size(); D.ntor_solute = s.ntors_solute;
+    std::vector<std::vector<int>> inc(N);
+    for (int n = 0; n < D.nbnd; ++n) for (int a : {s.bnd[n].i, s.bnd[n].j}) inc[a].push_back((0 << 28) | n);
+    for (int n = 0; n < D.nang; ++n) for (int a : {s.ang[n].i, s.ang[n].j, s.ang[n].k}) inc[a].push_back((1 << 28) | n);
+    for (int n = 0; n < D.ntor; ++n)
+      for (int a : {s.tor[n].i, s.tor[n].j, s.tor[n].k, s.tor[n].l}) inc[a].push_back((2 << 28) | n);
+    std::vector<int> ba, bf, bt;
+    for (int i = 0; i < N; ++i)
+      if (!inc[i].empty()) {
+        ba.push_back(i);
+        bf.push_back((int)bt.size());
+        bt.insert(bt.end(), inc[i].begin(), inc[i].end());
+      }
+    bf.push_back((int)bt.size());
+    D.nbonded_atoms = (int)ba.size();
+    D.bd_atom = I->upload(ba); D.bd_first = I->upload(bf); D.bd_term = I->upload(bt);
+    D.e_bonded = I->alloc<double>((size_t)std::max(1, D.nbonded_atoms) * 6);
+    max_nbonded_atoms_ = std::max(max_nbonded_atoms_, D.nbonded_atoms);
+  }
   // integrator
   D.dt = s.dt; D.temp0 = s.inp.temp0; D.tau_t = s.tau_t;
   D.separate_scaling = s.separate_scaling; D.detail_temps = s.detail_temps;
@@ -260,6 +395,7 @@ void Batch::upload(int k) {
   std::vector<double> e0(E_NTERMS, 0.0);
   e0[E_KINETIC] = scal[4];
   D.energies = I->upload(e0);
+  D.qenergies = I->alloc<double>((size_t)std::max(1, s.nstates) * kQTerms);
   I->natom = N; I->nwat = s.nwat;
   // the kinetic partial buffer needs the batch-wide unit count: allocate
   // generously from this instance's own count, re-checked at launch
@@ -277,6 +413,10 @@ void Batch::launch_lists() {
     dim3 g1((max_nwat_ * 64 + 255) / 256, ni);
     hipLaunchKernelGGL(k_water_lists, g1, dim3(256), 0, stream_, d_sys_);
   }
+  if (max_ncgp_solute_ > 0) {
+    dim3 g2((max_ncgp_solute_ * 64 + 255) / 256, ni);
+    hipLaunchKernelGGL(k_solute_lists, g2, dim3(256), 0, stream_, d_sys_);
+  }
 }
 
 void Batch::launch_forces() {
@@ -287,6 +427,26 @@ void Batch::launch_forces() {
       hipLaunchKernelGGL(k_water_forces<double>, g1, dim3(256), 0, stream_, d_sys_);
     else
       hipLaunchKernelGGL(k_water_forces<float>, g1, dim3(256), 0, stream_, d_sys_);
+  }
+  if (max_ncgp_solute_ > 0) {
+    dim3 gs((max_ncgp_solute_ * 64 + 255) / 256, ni);
+    if (precision_ == 0)
+      hipLaunchKernelGGL(k_solute_forces<double>, gs, dim3(256), 0, stream_, d_sys_);
+    else
+      hipLaunchKernelGGL(k_solute_forces<float>, gs, dim3(256), 0, stream_, d_sys_);
+  }
+  if (max_nbonded_atoms_ > 0)
+    hipLaunchKernelGGL(k_bonded, dim3((max_nbonded_atoms_ + 255) / 256, ni), dim3(256), 0, stream_, d_sys_);
+  if (max_nqat_ > 0) {
+    dim3 gq((max_nqat_ * 64 + 255) / 256, ni);
+    if (precision_ == 0)
+      hipLaunchKernelGGL(k_q_forces<double>, gq, dim3(256), 0, stream_, d_sys_);
+    else
+      hipLaunchKernelGGL(k_q_forces<float>, gq, dim3(256), 0, stream_, d_sys_);
+  }
+  if (max_nqq_atoms_ > 0)
+    hipLaunchKernelGGL(k_qq_forces, dim3((max_nqq_atoms_ + 255) / 256, ni), dim3(256), 0, stream_, d_sys_);
+  if (max_nwat_ > 0) {
     dim3 g2((max_nwat_ + 255) / 256, ni);
     hipLaunchKernelGGL(k_watpol_theta, g2, dim3(256), 0, stream_, d_sys_);
     int update = (istep_ != 0 && istep_ % qsys::kItdisUpdate == 0) ? 1 : 0;
@@ -354,9 +514,21 @@ void Batch::energies(int k, double* out, int cap) {
   HIPCHECK(hipStreamSynchronize(stream_));
   double E[E_NTERMS];
   HIPCHECK(hipMemcpy(E, h_sys_[k].energies, sizeof(E), hipMemcpyDeviceToHost));
-  // bonded and Q terms are zero until their kernels exist
-  double p_bond = 0, w_bond = 0, p_angle = 0, w_angle = 0, p_tor = 0, w_tor = 0, p_imp = 0, w_imp = 0;
+  const qsys::Setup& s = *setups_[k];
+  std::vector<double> q((size_t)std::max(1, s.nstates) * kQTerms, 0.0);
+  if (s.nstates > 0)
+    HIPCHECK(hipMemcpy(q.data(), h_sys_[k].qenergies, 8 * q.size(), hipMemcpyDeviceToHost));
+  double p_bond = E[E_P_BOND], w_bond = E[E_W_BOND], p_angle = E[E_P_ANGLE], w_angle = E[E_W_ANGLE];
+  double p_tor = E[E_P_TORSION], w_tor = E[E_W_TORSION], p_imp = 0, w_imp = 0;
+  // lambda-weighted Q sums of pot_energy (src/md.f90:16186-16211); no Q
+  // bonded terms or Q restraints in the supported FEP files
   double q_bond = 0, q_angle = 0, q_tor = 0, q_imp = 0, qx_el = 0, qx_vdw = 0, rst_protein = 0;
+  for (int st = 0; st < s.nstates; ++st) {
+    const double* e = &q[(size_t)st * kQTerms];
+    double el = e[0] + e[2] + e[4], vdw = e[1] + e[3] + e[5];
+    qx_el = qx_el + el * s.lambda[st];
+    qx_vdw = qx_vdw + vdw * s.lambda[st];
+  }
   double rst_total = E[E_RST_FIX] + E[E_RST_SHELL] + rst_protein + E[E_RST_RADIAL] + E[E_RST_WPOL];
   double pot = p_bond + w_bond + p_angle + w_angle + p_tor + p_imp + E[E_PP_EL] + E[E_PP_VDW] +
                E[E_PW_EL] + E[E_PW_VDW] + E[E_WW_EL] + E[E_WW_VDW] + q_bond + q_angle + q_tor + q_imp +
@@ -369,6 +541,27 @@ void Batch::energies(int k, double* out, int cap) {
                             rst_protein, pot + E[E_KINETIC], pot, E[E_KINETIC]};
   int n = std::min(cap, (int)QGPU_E_COUNT);
   std::memcpy(out, v, 8 * n);
+}
+
+// per state: qq_el qq_vdw qp_el qp_vdw qw_el qw_vdw total (EQ(s)%total,
+// src/md.f90:16190-16194) from the last energy reduction
+void Batch::qenergies(int k, double* out, int cap) {
+  const int ni = (int)h_sys_.size();
+  hipLaunchKernelGGL(k_energy_reduce, dim3(1, ni), dim3(256), 0, stream_, d_sys_);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(stream_));
+  const qsys::Setup& s = *setups_[k];
+  std::vector<double> q((size_t)std::max(1, s.nstates) * kQTerms, 0.0);
+  if (s.nstates > 0)
+    HIPCHECK(hipMemcpy(q.data(), h_sys_[k].qenergies, 8 * q.size(), hipMemcpyDeviceToHost));
+  int w = 0;
+  for (int st = 0; st < s.nstates; ++st) {
+    const double* e = &q[(size_t)st * kQTerms];
+    double qx_el = e[0] + e[2] + e[4], qx_vdw = e[1] + e[3] + e[5];
+    double tot = 0.0 + 0.0 + 0.0 + 0.0 + qx_el + qx_vdw + 0.0;
+    double v[7] = {e[0], e[1], e[2], e[3], e[4], e[5], tot};
+    for (int c = 0; c < 7 && w < cap; ++c) out[w++] = v[c];
+  }
 }
 
 void Batch::export_list(int k, int which, std::vector<int>& out) {
@@ -394,7 +587,64 @@ void Batch::export_list(int k, int which, std::vector<int>& out) {
           }
       }
   } else if (which == 0 || which == 1) {
-    // no solute lists yet
+    const qsys::Topology& t = s.top;
+    const int nc = t.ncgp_solute;
+    const int cap = which == 0 ? D.pp_cap : D.pw_cap;
+    std::vector<int> cnt(nc), nb((size_t)nc * cap);
+    if (nc > 0) {
+      HIPCHECK(hipMemcpy(cnt.data(), which == 0 ? D.pp_cnt : D.pw_cnt, 4 * cnt.size(), hipMemcpyDeviceToHost));
+      HIPCHECK(hipMemcpy(nb.data(), which == 0 ? D.pp_nb : D.pw_nb, 4 * nb.size(), hipMemcpyDeviceToHost));
+    }
+    auto lj_of = [&](int i, int j) { return s.ljcod[s.iac[i] * s.natyps + s.iac[j]]; };
+    auto zero = [&](int i, int j, int lj) {
+      const auto& ti = t.types[s.iac[i]];
+      const auto& tj = t.types[s.iac[j]];
+      return s.crg[i] * s.crg[j] == 0.0 && ti.avdw[lj - 1] * tj.avdw[lj - 1] == 0.0 &&
+             ti.bvdw[lj - 1] * tj.bvdw[lj - 1] == 0.0;
+    };
+    auto in_bits = [&](const std::vector<uint32_t>& rows, const std::vector<std::pair<int, int>>& lng,
+                       int i, int j) {
+      if (std::abs(j - i) <= qsys::kMaxNbrRange) {
+        int lo = std::min(i, j), hi = std::max(i, j);
+        return ((rows[lo] >> (hi - lo - 1)) & 1u) != 0;
+      }
+      for (auto& p : lng)
+        if ((p.first == i && p.second == j) || (p.first == j && p.second == i)) return true;
+      return false;
+    };
+    for (int g = 0; g < nc; ++g)
+      for (int k2 = 0; k2 < cnt[g]; ++k2) {
+        int other = nb[(size_t)g * cap + k2];
+        if (which == 0) {
+          bool own = other == g || ((g < other) == (((g + other) & 1) == 0));
+          if (!own) continue;
+          for (int a = t.cgp[g].first; a <= t.cgp[g].last; ++a) {
+            int i = t.cgpatom[a];
+            if (s.iqatom[i]) continue;
+            for (int b = t.cgp[other].first; b <= t.cgp[other].last; ++b) {
+              int j = t.cgpatom[b];
+              if (other == g && i >= j) continue;
+              if (s.iqatom[j]) continue;
+              int lj = lj_of(i, j);
+              if (zero(i, j, lj)) continue;
+              if (in_bits(t.listex, t.listexlong, i, j)) continue;
+              if (in_bits(t.list14, t.list14long, i, j)) lj = 3;
+              out.push_back(i); out.push_back(j); out.push_back(lj);
+            }
+          }
+        } else {
+          int jo = s.nat_solute + 3 * other;
+          for (int a = t.cgp[g].first; a <= t.cgp[g].last; ++a) {
+            int i = t.cgpatom[a];
+            if (s.iqatom[i]) continue;
+            for (int ip = 0; ip < 3; ++ip) {
+              int lj = lj_of(i, jo + ip);
+              if (zero(i, jo + ip, lj)) continue;
+              out.push_back(i); out.push_back(jo + ip); out.push_back(lj);
+            }
+          }
+        }
+      }
   } else {
     throw std::runtime_error("unknown list");
   }
@@ -511,11 +761,7 @@ int qgpu_energies(qgpu_handle* h, int k, double* e, int cap) {
   return guarded([&] { CHECK_K(h, k); h->b->energies(k, e, cap); });
 }
 int qgpu_qenergies(qgpu_handle* h, int k, double* e, int cap) {
-  return guarded([&] {
-    CHECK_K(h, k);
-    (void)e; (void)cap;
-    if (h->b->setup(k).nstates > 0) throw std::runtime_error("Q energies are not yet available");
-  });
+  return guarded([&] { CHECK_K(h, k); h->b->qenergies(k, e, cap); });
 }
 int qgpu_get_pairlist(qgpu_handle* h, int k, int which, int* out, long* n) {
   return guarded([&] {

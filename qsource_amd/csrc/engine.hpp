@@ -28,7 +28,12 @@
 
 namespace qgpu {
 
-enum : unsigned char { F_EXCL = 1, F_SHELL = 2, F_Q = 4, F_HEAVY = 8 };
+// F_QP: solute atom in the static Q-solute list set (nbqplist);
+// F_QW: water (flag on its O) in the static Q-water list set (nbqwlist)
+enum : unsigned char { F_EXCL = 1, F_SHELL = 2, F_Q = 4, F_HEAVY = 8, F_QP = 16, F_QW = 32 };
+
+constexpr int kMaxGroupAtoms = 8;   // atoms per solute charge group handled per wavefront
+constexpr int kMaxQStates = 10;
 
 constexpr int kLrfStride = 44;
 constexpr int kNumShellMax = 8;
@@ -36,7 +41,7 @@ constexpr int kNumShellMax = 8;
 // per-instance energy slots reduced on demand (index into DevSys::eacc)
 enum EnergyTerm {
   E_PP_EL, E_PP_VDW, E_PW_EL, E_PW_VDW, E_WW_EL, E_WW_VDW, E_LRF,
-  E_P_BOND, E_P_ANGLE, E_P_TORSION, E_P_IMPROPER,
+  E_P_BOND, E_P_ANGLE, E_P_TORSION, E_P_IMPROPER, E_W_BOND, E_W_ANGLE, E_W_TORSION,
   E_RST_FIX, E_RST_SHELL, E_RST_RADIAL, E_RST_WPOL,
   E_KINETIC,
   E_NTERMS
@@ -93,6 +98,31 @@ struct DevSys {
   const int* iqseq;
   const double* qcrg;            // nqat x nstates
   const double* lambda;
+  const signed char* qp_lj;      // nat_solute x nqat LJ code of the (Q, solute atom) pair
+  const signed char* qw_lj;      // nqat: LJ code of the (water O, Q atom) pair
+  double alj_O[2], blj_O[2];     // aLJ_solv(1,:), bLJ_solv(1,:) of the water O
+  int nqp_atoms, nqw_waters;
+  const int* qp_atoms;           // solute atoms in the Q-solute list, ascending
+  const int* qw_waters;          // waters in the Q-water list, ascending
+  double* e_q;                   // nqat x nstates x 4: qp_el qp_vdw qw_el qw_vdw
+  // Q-Q and Q-neighbour pairs of all states (nbqq), and per-atom incidence
+  int nqq;                       // total pairs over states
+  const int *qq_state, *qq_iq, *qq_j, *qq_jq, *qq_lj;
+  const double* qq_elscale;
+  double* e_qq;                  // nqq x 2: el, vdw (of the pair)
+  int nqq_atoms;
+  const int *qq_atom, *qq_first, *qq_pair;   // CSR: atom -> its pairs
+  // bonded terms (after shrink_topology) and per-atom incidence
+  const int* bnd;                // nbnd x 3 (i, j, cod)
+  const int* ang;                // nang x 4
+  const int* tor;                // ntor x 5
+  const double* bondlib;         // fk, b0
+  const double* anglib;          // fk, ang0
+  const double* torlib;          // fk, rmult, deltor, paths
+  int nbnd, nbnd_solute, nang, nang_solute, ntor, ntor_solute;
+  int nbonded_atoms;
+  const int *bd_atom, *bd_first, *bd_term;    // CSR: atom -> (type<<28 | term index)
+  double* e_bonded;              // nbonded_atoms x 6: p_bond p_angle p_torsion w_bond w_angle w_torsion
   // integrator
   double dt;
   double temp0, tau_t;
@@ -114,6 +144,7 @@ struct DevSys {
   int ke_blocks;
   double* scal;        // [0]Tscale_solute [1]Tscale_solvent [2]Tfree [3]Temp [4]Ekin [5..] per-class temps
   double* energies;    // E_NTERMS reduced values
+  double* qenergies;   // nstates x kQTerms reduced values
 };
 
 // One GPU batch of instances sharing every launch.
@@ -129,6 +160,7 @@ class Batch {
   void set_state(int k, const double* x, const double* v);
   void get_gradient(int k, double* d);
   void energies(int k, double* e, int cap);
+  void qenergies(int k, double* e, int cap);
   void export_list(int k, int which, std::vector<int>& out);
   double temperature(int k);
   void theta_corr(int k, double* out);
@@ -153,6 +185,7 @@ class Batch {
   int istep_ = 0;
   int precision_ = 0;
   int max_nwat_ = 0, max_ncgp_ = 0, max_ncgp_solute_ = 0, max_units_ = 0, max_natom_ = 0;
+  int max_nqat_ = 0, max_nqq_atoms_ = 0, max_nbonded_atoms_ = 0;
 };
 
 }  // namespace qgpu

@@ -1,0 +1,540 @@
+// CDNA4 kernels for the solute and Q-atom side of qdyn's MD step: solute
+// charge-group lists with the LRF gather, solute nonbonded forces (solute-
+// solute, solute-water, LRF Taylor, shell restraints and the solute side of
+// the Q-solute terms), the Q-atom kernel (Q-solute and Q-water terms seen
+// from the Q atom, per-state energies), the Q-Q / Q-neighbour kernel and the
+// classical bonded kernel.  All are owner-computes: each atom's gradient is
+// written by one lane, so results are independent of scheduling.
+#include "kernels_common.hpp"
+
+namespace qgpu {
+
+__device__ __forceinline__ bool bit_excluded(const DevSys& S, int i, int j) {
+  int lo = i < j ? i : j, hi = i < j ? j : i;
+  if (hi - lo <= qsys::kMaxNbrRange) return (S.listex[lo] >> (hi - lo - 1)) & 1u;
+  for (int n = 0; n < S.nexlong; ++n) {
+    int a = S.exlong[2 * n], b = S.exlong[2 * n + 1];
+    if ((a == i && b == j) || (a == j && b == i)) return true;
+  }
+  return false;
+}
+__device__ __forceinline__ bool bit_14(const DevSys& S, int i, int j) {
+  int lo = i < j ? i : j, hi = i < j ? j : i;
+  if (hi - lo <= qsys::kMaxNbrRange) return (S.list14[lo] >> (hi - lo - 1)) & 1u;
+  bool r = false;
+  for (int n = 0; n < S.n14long; ++n) {
+    int a = S.l14long[2 * n], b = S.l14long[2 * n + 1];
+    if ((a == i && b == j) || (a == j && b == i)) r = true;
+  }
+  return r;
+}
+__device__ __forceinline__ bool zero_interaction(const DevSys& S, int i, int j, int lj) {
+  const int ti = S.iac[i], tj = S.iac[j];
+  return S.crg[i] * S.crg[j] == 0.0 && S.avdw[ti * 3 + lj] * S.avdw[tj * 3 + lj] == 0.0 &&
+         S.bvdw[ti * 3 + lj] * S.bvdw[tj * 3 + lj] == 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Solute lists (nbpplist_lrf src/md.f90:7998-8188 and the solute side of
+// nbpwlist_lrf :9300-9429): one wavefront per solute charge group.
+__global__ __launch_bounds__(256) void k_solute_lists(const DevSys* __restrict__ sys) {
+  const DevSys& S = sys[blockIdx.y];
+  const int lane = threadIdx.x & 63;
+  const int g = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (g >= S.ncgp_solute) return;
+  const int sw = S.cgp_sw[g];
+  if (S.flags[sw] & F_EXCL) {
+    if (lane == 0) { S.pp_cnt[g] = 0; S.pw_cnt[g] = 0; }
+    return;
+  }
+  const double xs0 = S.x[3 * sw], xs1 = S.x[3 * sw + 1], xs2 = S.x[3 * sw + 2];
+  double* Lg = S.lrf + (size_t)g * kLrfStride;
+  const double c0 = Lg[0], c1 = Lg[1], c2 = Lg[2];
+  LrfAcc acc;
+  acc.zero();
+  int n_pp = 0;
+  int* prow = S.pp_nb + (size_t)g * S.pp_cap;
+  for (int base = 0; base < S.ncgp_solute; base += 64) {
+    int jg = base + lane;
+    bool inside = false, far = false;
+    if (jg < S.ncgp_solute) {
+      int js = S.cgp_sw[jg];
+      if (!(S.flags[js] & F_EXCL)) {
+        double a = xs0 - S.x[3 * js], b = xs1 - S.x[3 * js + 1], c = xs2 - S.x[3 * js + 2];
+        double r2 = a * a + b * b + c * c;
+        inside = r2 <= S.rcpp2;
+        far = !inside && r2 <= S.rclrf2;
+      }
+    }
+    unsigned long long m = __ballot(inside);
+    if (inside) {
+      int pos = n_pp + __popcll(m & ((1ull << lane) - 1ull));
+      if (pos < S.pp_cap) prow[pos] = jg;
+    }
+    n_pp += __popcll(m);
+    if (far && S.use_lrf) lrf_gather_group(S, jg, c0, c1, c2, acc);
+  }
+  int n_pw = 0;
+  int* wrow = S.pw_nb + (size_t)g * S.pw_cap;
+  for (int base = 0; base < S.nwat; base += 64) {
+    int jw = base + lane;
+    bool inside = false, far = false;
+    int jo = S.nat_solute + 3 * jw;
+    if (jw < S.nwat && !(S.flags[jo] & F_EXCL)) {
+      double a = xs0 - S.x[3 * jo], b = xs1 - S.x[3 * jo + 1], c = xs2 - S.x[3 * jo + 2];
+      double r2 = a * a + b * b + c * c;
+      inside = r2 <= S.rcpw2;
+      far = !inside && r2 <= S.rclrf2;
+    }
+    unsigned long long m = __ballot(inside);
+    if (inside) {
+      int pos = n_pw + __popcll(m & ((1ull << lane) - 1ull));
+      if (pos < S.pw_cap) wrow[pos] = jw;
+    }
+    n_pw += __popcll(m);
+    if (far && S.use_lrf) lrf_gather_group(S, S.iwhich_cgp[jo], c0, c1, c2, acc);
+  }
+  acc.wave_reduce();
+  if (lane == 0) {
+    if (S.use_lrf) acc.store(Lg);
+    S.pp_cnt[g] = n_pp < S.pp_cap ? n_pp : S.pp_cap;
+    S.pw_cnt[g] = n_pw < S.pw_cap ? n_pw : S.pw_cap;
+    if (n_pp > S.pp_cap || n_pw > S.pw_cap) atomicOr(S.overflow, 1);
+  }
+}
+
+// One solute-solute atom pair as nonbond_pp forms it (src/md.f90:13107-13150)
+// with (ia, ja) in the reference's orientation; returns dv and dx = x_ja - x_ia.
+template <typename P>
+__device__ __forceinline__ P pp_pair(const DevSys& S, int ia, int ja, int lj, P& dx1, P& dx2, P& dx3,
+                                     P& Vel, P& Va, P& Vb) {
+  dx1 = (P)(S.x[3 * ja] - S.x[3 * ia]);
+  dx2 = (P)(S.x[3 * ja + 1] - S.x[3 * ia + 1]);
+  dx3 = (P)(S.x[3 * ja + 2] - S.x[3 * ia + 2]);
+  P r2 = (P)1 / (dx1 * dx1 + dx2 * dx2 + dx3 * dx3);
+  P r = psqrt(r2);
+  P r6 = r2 * r2 * r2;
+  Vel = (P)S.crg[ia] * (P)S.crg[ja] * r;
+  int l = lj - 1;
+  if (lj == 3) Vel = Vel * (P)S.el14_scale;
+  const int ti = S.iac[ia], tj = S.iac[ja];
+  Va = r6 * r6 * (P)S.avdw[ti * 3 + l] * (P)S.avdw[tj * 3 + l];
+  Vb = r6 * (P)S.bvdw[ti * 3 + l] * (P)S.bvdw[tj * 3 + l];
+  return r2 * (-Vel - (P)12 * Va + (P)6 * Vb);
+}
+
+// ---------------------------------------------------------------------------
+// Solute forces: one wavefront per solute charge group (nonbond_pp
+// :13069-13155, nonbond_pw :13282-13365, lrf_taylor, fix_shell :2816-2862,
+// and the solute-atom side of nonbond_qp :13788-13921).
+template <typename P>
+__global__ __launch_bounds__(256) void k_solute_forces(const DevSys* __restrict__ sys) {
+  const DevSys& S = sys[blockIdx.y];
+  const int lane = threadIdx.x & 63;
+  const int g = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (g >= S.ncgp_solute) return;
+  const int f = S.cgp_first[g], na = S.cgp_last[g] - f + 1;
+  double gr[kMaxGroupAtoms][3];
+  for (int a = 0; a < kMaxGroupAtoms; ++a) gr[a][0] = gr[a][1] = gr[a][2] = 0.0;
+  double e_ppel = 0, e_ppvdw = 0, e_pwel = 0, e_pwvdw = 0;
+  // solute-solute
+  const int npp = S.pp_cnt[g];
+  const int* prow = S.pp_nb + (size_t)g * S.pp_cap;
+  for (int k = lane; k < npp; k += 64) {
+    const int jg = prow[k];
+    const bool same = jg == g;
+    const bool own = same || owns_pair(g, jg);
+    for (int a = 0; a < na; ++a) {
+      const int i = S.cgpatom[f + a];
+      if (S.flags[i] & F_Q) continue;
+      for (int b = S.cgp_first[jg]; b <= S.cgp_last[jg]; ++b) {
+        const int j = S.cgpatom[b];
+        if (j == i || (S.flags[j] & F_Q)) continue;
+        int lj = S.ljcod[S.iac[i] * S.natyps + S.iac[j]];
+        if (zero_interaction(S, i, j, lj - 1)) continue;
+        if (bit_excluded(S, i, j)) continue;
+        if (bit_14(S, i, j)) lj = 3;
+        // reference orientation: the owner group's atom first; within one
+        // group the lower atom index first
+        const bool me_first = same ? (i < j) : own;
+        const int ia = me_first ? i : j, ja = me_first ? j : i;
+        P dx1, dx2, dx3, Vel, Va, Vb;
+        P dv = pp_pair<P>(S, ia, ja, lj, dx1, dx2, dx3, Vel, Va, Vb);
+        if (me_first) {
+          gr[a][0] = gr[a][0] - (double)(dv * dx1);
+          gr[a][1] = gr[a][1] - (double)(dv * dx2);
+          gr[a][2] = gr[a][2] - (double)(dv * dx3);
+          e_ppel = e_ppel + (double)Vel;
+          e_ppvdw = e_ppvdw + (double)Va - (double)Vb;
+        } else {
+          gr[a][0] = gr[a][0] + (double)(dv * dx1);
+          gr[a][1] = gr[a][1] + (double)(dv * dx2);
+          gr[a][2] = gr[a][2] + (double)(dv * dx3);
+        }
+      }
+    }
+  }
+  // solute-water (solute atom is i of the reference pair)
+  const int npw = S.pw_cnt[g];
+  const int* wrow = S.pw_nb + (size_t)g * S.pw_cap;
+  for (int k = lane; k < npw; k += 64) {
+    const int jo = S.nat_solute + 3 * wrow[k];
+    for (int a = 0; a < na; ++a) {
+      const int i = S.cgpatom[f + a];
+      if (S.flags[i] & F_Q) continue;
+      const int ti = S.iac[i];
+      for (int ip = 0; ip < 3; ++ip) {
+        const int j = jo + ip;
+        const int tj = S.iac[j];
+        const int lj = S.ljcod[ti * S.natyps + tj] - 1;
+        if (zero_interaction(S, i, j, lj)) continue;
+        const P aLJ = (P)(S.avdw[ti * 3 + lj] * S.avdw[tj * 3 + lj]);
+        const P bLJ = (P)(S.bvdw[ti * 3 + lj] * S.bvdw[tj * 3 + lj]);
+        P dx1 = (P)(S.x[3 * j] - S.x[3 * i]), dx2 = (P)(S.x[3 * j + 1] - S.x[3 * i + 1]),
+          dx3 = (P)(S.x[3 * j + 2] - S.x[3 * i + 2]);
+        P r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
+        r2 = (P)1 / r2;
+        P r = psqrt(r2);
+        P r6 = r2 * r2 * r2;
+        P r12 = r6 * r6;
+        P Vel = (P)S.crg[i] * (P)S.crg[j] * r;
+        P Va = aLJ * r12;
+        P Vb = bLJ * r6;
+        P dv = r2 * (-Vel - (P)12 * Va + (P)6 * Vb);
+        gr[a][0] = gr[a][0] - (double)(dv * dx1);
+        gr[a][1] = gr[a][1] - (double)(dv * dx2);
+        gr[a][2] = gr[a][2] - (double)(dv * dx3);
+        e_pwel = e_pwel + (double)Vel;
+        e_pwvdw = e_pwvdw + (double)Va - (double)Vb;
+      }
+    }
+  }
+  for (int a = 0; a < na; ++a)
+    for (int c = 0; c < 3; ++c) gr[a][c] = wave_sum(gr[a][c]);
+  e_ppel = wave_sum(e_ppel); e_ppvdw = wave_sum(e_ppvdw);
+  e_pwel = wave_sum(e_pwel); e_pwvdw = wave_sum(e_pwvdw);
+  if (lane != 0) return;
+  double e_lrf = 0, e_fix = 0, e_shell = 0;
+  const double* L = S.lrf + (size_t)g * kLrfStride;
+  for (int a = 0; a < na; ++a) {
+    const int i = S.cgpatom[f + a];
+    const unsigned char fl = S.flags[i];
+    const double* xi = S.x + 3 * i;
+    if (S.use_lrf && !(fl & (F_EXCL | F_Q))) {
+      double df[3];
+      double V = lrf_taylor_atom(L, xi, df);
+      e_lrf = e_lrf + 0.5 * S.crg[i] * V;
+      for (int c = 0; c < 3; ++c) gr[a][c] = gr[a][c] - S.crg[i] * df[c];
+    }
+    if (fl & (F_EXCL | F_SHELL)) {
+      const double fk = (fl & F_EXCL) ? S.fk_fix : S.fk_pshell;
+      double dr0 = xi[0] - S.xtop[3 * i], dr1 = xi[1] - S.xtop[3 * i + 1], dr2 = xi[2] - S.xtop[3 * i + 2];
+      double erst = 0.5 * fk * (dr0 * dr0 + dr1 * dr1 + dr2 * dr2);
+      if (fl & F_EXCL) e_fix = e_fix + erst;
+      if (fl & F_SHELL) e_shell = e_shell + erst;
+      gr[a][0] = gr[a][0] + fk * dr0;
+      gr[a][1] = gr[a][1] + fk * dr1;
+      gr[a][2] = gr[a][2] + fk * dr2;
+    }
+    // Q-solute pairs seen from the solute atom (Q atom is i of the pair)
+    if ((fl & F_QP) && !(fl & F_Q)) {
+      const int tj = S.iac[i];
+      for (int iq = 0; iq < S.nqat; ++iq) {
+        const int qi = S.iqseq[iq];
+        const int lj = S.qp_lj[(size_t)i * S.nqat + iq];
+        const int l = lj - 1;
+        const int ti = S.iac[qi];
+        P dx1 = (P)(xi[0] - S.x[3 * qi]), dx2 = (P)(xi[1] - S.x[3 * qi + 1]), dx3 = (P)(xi[2] - S.x[3 * qi + 2]);
+        P r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
+        r2 = (P)1 / r2;
+        P r6 = r2 * r2 * r2;
+        P r = psqrt(r2);
+        const P aLJ = (P)(S.avdw[ti * 3 + l] * S.avdw[tj * 3 + l]);
+        const P bLJ = (P)(S.bvdw[ti * 3 + l] * S.bvdw[tj * 3 + l]);
+        for (int st = 0; st < S.nstates; ++st) {
+          P Vel = (P)S.qcrg[iq * S.nstates + st] * (P)S.crg[i] * r;
+          if (lj == 3) Vel = Vel * (P)S.el14_scale;
+          P Va = aLJ * r6 * r6;
+          P Vb = bLJ * r6;
+          P dv = r2 * (-Vel - (P)12 * Va + (P)6 * Vb) * (P)S.lambda[st];
+          gr[a][0] = gr[a][0] + (double)(dv * dx1);
+          gr[a][1] = gr[a][1] + (double)(dv * dx2);
+          gr[a][2] = gr[a][2] + (double)(dv * dx3);
+        }
+      }
+    }
+    for (int c = 0; c < 3; ++c) S.d[3 * i + c] = gr[a][c];
+  }
+  double* es = S.e_solute + (size_t)g * 8;
+  es[0] = e_ppel; es[1] = e_ppvdw; es[2] = e_pwel; es[3] = e_pwvdw;
+  es[4] = e_lrf; es[5] = e_fix; es[6] = e_shell; es[7] = 0.0;
+}
+template __global__ void k_solute_forces<double>(const DevSys*);
+template __global__ void k_solute_forces<float>(const DevSys*);
+
+// ---------------------------------------------------------------------------
+// Q atoms: one wavefront per Q atom; lanes take the Q-solute partners and
+// the Q-water waters (nonbond_qp, nonbond_qw_spc :14411-14570) and the
+// wavefront reduces the Q atom's gradient and its per-state energies.
+template <typename P>
+__global__ __launch_bounds__(256) void k_q_forces(const DevSys* __restrict__ sys) {
+  const DevSys& S = sys[blockIdx.y];
+  const int lane = threadIdx.x & 63;
+  const int iq = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (iq >= S.nqat) return;
+  const int ns = S.nstates;
+  const int qi = S.iqseq[iq];
+  const double xq0 = S.x[3 * qi], xq1 = S.x[3 * qi + 1], xq2 = S.x[3 * qi + 2];
+  const int ti = S.iac[qi];
+  double g0 = 0, g1 = 0, g2 = 0;
+  double e[kMaxQStates][4];
+  for (int st = 0; st < kMaxQStates; ++st) e[st][0] = e[st][1] = e[st][2] = e[st][3] = 0.0;
+  P qs[kMaxQStates], lam[kMaxQStates];
+  for (int st = 0; st < ns; ++st) { qs[st] = (P)S.qcrg[iq * ns + st]; lam[st] = (P)S.lambda[st]; }
+  for (int k = lane; k < S.nqp_atoms; k += 64) {
+    const int j = S.qp_atoms[k];
+    const int lj = S.qp_lj[(size_t)j * S.nqat + iq];
+    const int l = lj - 1;
+    const int tj = S.iac[j];
+    P dx1 = (P)(S.x[3 * j] - xq0), dx2 = (P)(S.x[3 * j + 1] - xq1), dx3 = (P)(S.x[3 * j + 2] - xq2);
+    P r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
+    r2 = (P)1 / r2;
+    P r6 = r2 * r2 * r2;
+    P r = psqrt(r2);
+    const P aLJ = (P)(S.avdw[ti * 3 + l] * S.avdw[tj * 3 + l]);
+    const P bLJ = (P)(S.bvdw[ti * 3 + l] * S.bvdw[tj * 3 + l]);
+    for (int st = 0; st < ns; ++st) {
+      P Vel = qs[st] * (P)S.crg[j] * r;
+      if (lj == 3) Vel = Vel * (P)S.el14_scale;
+      P Va = aLJ * r6 * r6;
+      P Vb = bLJ * r6;
+      P dv = r2 * (-Vel - (P)12 * Va + (P)6 * Vb) * lam[st];
+      g0 = g0 - (double)(dv * dx1);
+      g1 = g1 - (double)(dv * dx2);
+      g2 = g2 - (double)(dv * dx3);
+      e[st][0] = e[st][0] + (double)Vel;
+      e[st][1] = e[st][1] + (double)Va - (double)Vb;
+    }
+  }
+  const int lo = S.qw_lj[iq];  // LJ code of the (water O, Q atom) pair
+  const P aO = (P)S.avdw[ti * 3 + lo - 1], bO = (P)S.bvdw[ti * 3 + lo - 1];
+  const P aS = (P)S.alj_O[lo - 1], bS = (P)S.blj_O[lo - 1];
+  for (int k = lane; k < S.nqw_waters; k += 64) {
+    const int jo = S.nat_solute + 3 * S.qw_waters[k];
+    P dx1 = (P)(S.x[3 * jo] - xq0), dx2 = (P)(S.x[3 * jo + 1] - xq1), dx3 = (P)(S.x[3 * jo + 2] - xq2);
+    P r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
+    P r6_hc = r2 * r2 * r2;
+    r2 = (P)1 / r2;
+    P r = psqrt(r2);
+    P r6 = (P)1 / r6_hc;
+    P Va = aO * aS * r6 * r6;
+    P Vb = bO * bS * r6;
+    P dv = 0;
+    for (int st = 0; st < ns; ++st) {
+      P Vel = (P)S.chg_solv[0] * qs[st] * r;
+      dv = dv + r2 * (-Vel - ((P)12 * Va - (P)6 * Vb) * (r6_hc / r6_hc)) * lam[st];
+      e[st][2] = e[st][2] + (double)Vel;
+      e[st][3] = e[st][3] + (double)Va - (double)Vb;
+    }
+    g0 = g0 - (double)(dv * dx1);
+    g1 = g1 - (double)(dv * dx2);
+    g2 = g2 - (double)(dv * dx3);
+    for (int ip = 1; ip < 3; ++ip) {
+      const int ja = jo + ip;
+      P ex1 = (P)(S.x[3 * ja] - xq0), ex2 = (P)(S.x[3 * ja + 1] - xq1), ex3 = (P)(S.x[3 * ja + 2] - xq2);
+      P q2 = (P)1 / (ex1 * ex1 + ex2 * ex2 + ex3 * ex3);
+      P rr = psqrt(q2);
+      P dvh = 0;
+      for (int st = 0; st < ns; ++st) {
+        P Vel = (P)S.chg_solv[ip] * qs[st] * rr;
+        e[st][2] = e[st][2] + (double)Vel;
+        dvh = dvh - q2 * Vel * lam[st];
+      }
+      g0 = g0 - (double)(dvh * ex1);
+      g1 = g1 - (double)(dvh * ex2);
+      g2 = g2 - (double)(dvh * ex3);
+    }
+  }
+  g0 = wave_sum(g0); g1 = wave_sum(g1); g2 = wave_sum(g2);
+  for (int st = 0; st < ns; ++st)
+    for (int c = 0; c < 4; ++c) e[st][c] = wave_sum(e[st][c]);
+  if (lane == 0) {
+    S.d[3 * qi] = S.d[3 * qi] + g0;
+    S.d[3 * qi + 1] = S.d[3 * qi + 1] + g1;
+    S.d[3 * qi + 2] = S.d[3 * qi + 2] + g2;
+    for (int st = 0; st < ns; ++st)
+      for (int c = 0; c < 4; ++c) S.e_q[((size_t)iq * ns + st) * 4 + c] = e[st][c];
+  }
+}
+template __global__ void k_q_forces<double>(const DevSys*);
+template __global__ void k_q_forces<float>(const DevSys*);
+
+// ---------------------------------------------------------------------------
+// Q-Q and Q-neighbour pairs (nonbond_qq :13491-13636): one thread per atom
+// that takes part in any pair; each thread evaluates its pairs in the
+// reference orientation (Q atom iq first) and keeps its own gradient; the
+// pair energy is stored by the Q-atom side.  FP64 in both precisions (the
+// list holds at most a few thousand pairs).
+__global__ __launch_bounds__(256) void k_qq_forces(const DevSys* __restrict__ sys) {
+  const DevSys& S = sys[blockIdx.y];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= S.nqq_atoms) return;
+  const int me = S.qq_atom[t];
+  double g0 = 0, g1 = 0, g2 = 0;
+  for (int k = S.qq_first[t]; k < S.qq_first[t + 1]; ++k) {
+    const int p = S.qq_pair[k];
+    const int st = S.qq_state[p], iq = S.qq_iq[p], j = S.qq_j[p], jq = S.qq_jq[p], lj = S.qq_lj[p];
+    const int i = S.iqseq[iq];
+    const int l = lj - 1;
+    const double qi = S.qcrg[iq * S.nstates + st];
+    const double aLJ = S.avdw[S.iac[i] * 3 + l] * S.avdw[S.iac[j] * 3 + l];
+    const double bLJ = S.bvdw[S.iac[i] * 3 + l] * S.bvdw[S.iac[j] * 3 + l];
+    const double qj = jq >= 0 ? S.qcrg[jq * S.nstates + st] : S.crg[j];
+    const double dx1 = S.x[3 * j] - S.x[3 * i], dx2 = S.x[3 * j + 1] - S.x[3 * i + 1],
+                 dx3 = S.x[3 * j + 2] - S.x[3 * i + 2];
+    double r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
+    const double r6_hc = r2 * r2 * r2;
+    double r6 = r2 * r2 * r2 + 0.0;
+    r6 = 1.0 / r6;
+    const double r12 = r6 * r6;
+    r2 = 1.0 / r2;
+    const double r = sqrt(r2);
+    double Vel = qi * qj * r * S.qq_elscale[p];
+    if (lj == 3) Vel = Vel * S.el14_scale;
+    const double Va = aLJ * r12, Vb = bLJ * r6;
+    const double dv = r2 * (-Vel - (12.0 * Va - 6.0 * Vb) * r6_hc * r6) * S.lambda[st];
+    if (me == i) {
+      g0 = g0 - dv * dx1; g1 = g1 - dv * dx2; g2 = g2 - dv * dx3;
+      S.e_qq[2 * p] = Vel;
+      S.e_qq[2 * p + 1] = Va - Vb;
+    } else {
+      g0 = g0 + dv * dx1; g1 = g1 + dv * dx2; g2 = g2 + dv * dx3;
+    }
+  }
+  S.d[3 * me] = S.d[3 * me] + g0;
+  S.d[3 * me + 1] = S.d[3 * me + 1] + g1;
+  S.d[3 * me + 2] = S.d[3 * me + 2] + g2;
+}
+
+// ---------------------------------------------------------------------------
+// Classical bonded terms (bond :1332-1405, angle :1130-1246, torsion
+// :17907-18052): one thread per atom that takes part in a term; every term
+// is evaluated in full by each of its atoms and the atom keeps its own
+// share; the term energy is stored by its first atom.
+__global__ __launch_bounds__(256) void k_bonded(const DevSys* __restrict__ sys) {
+  const DevSys& S = sys[blockIdx.y];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= S.nbonded_atoms) return;
+  const int me = S.bd_atom[t];
+  double g[3] = {0, 0, 0};
+  double e[6] = {0, 0, 0, 0, 0, 0};
+  const double* x = S.x;
+  for (int k = S.bd_first[t]; k < S.bd_first[t + 1]; ++k) {
+    const int code = S.bd_term[k];
+    const int type = code >> 28, n = code & 0x0fffffff;
+    if (type == 0) {
+      const int i = S.bnd[3 * n], j = S.bnd[3 * n + 1], cod = S.bnd[3 * n + 2] - 1;
+      const double fk = S.bondlib[2 * cod], b0 = S.bondlib[2 * cod + 1];
+      double r0 = x[3 * j] - x[3 * i], r1 = x[3 * j + 1] - x[3 * i + 1], r2 = x[3 * j + 2] - x[3 * i + 2];
+      double b = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
+      double db = b - b0;
+      double dv = fk * db / b;
+      if (me == j) { g[0] += r0 * dv; g[1] += r1 * dv; g[2] += r2 * dv; }
+      else { g[0] -= r0 * dv; g[1] -= r1 * dv; g[2] -= r2 * dv; }
+      if (me == i) e[n < S.nbnd_solute ? 0 : 3] += 0.5 * fk * (db * db);
+    } else if (type == 1) {
+      const int i = S.ang[4 * n], j = S.ang[4 * n + 1], kk = S.ang[4 * n + 2], cod = S.ang[4 * n + 3] - 1;
+      const double fk = S.anglib[2 * cod], a0 = S.anglib[2 * cod + 1];
+      double rji[3], rjk[3];
+      for (int c = 0; c < 3; ++c) { rji[c] = x[3 * i + c] - x[3 * j + c]; rjk[c] = x[3 * kk + c] - x[3 * j + c]; }
+      double bji2inv = 1.0 / (rji[0] * rji[0] + rji[1] * rji[1] + rji[2] * rji[2]);
+      double bjk2inv = 1.0 / (rjk[0] * rjk[0] + rjk[1] * rjk[1] + rjk[2] * rjk[2]);
+      double bjiinv = sqrt(bji2inv), bjkinv = sqrt(bjk2inv);
+      double scp = (rji[0] * rjk[0] + rji[1] * rjk[1] + rji[2] * rjk[2]);
+      scp = scp * bjiinv * bjkinv;
+      if (scp > 1.0) scp = 1.0;
+      else if (scp < -1.0) scp = -1.0;
+      double angv = acos(scp);
+      double da = angv - a0;
+      double dv = fk * da;
+      double f1 = sin(angv);
+      if (fabs(f1) < qsys::kQrealEps) f1 = qsys::kQrealEps;
+      f1 = -1.0 / f1;
+      double di[3], dk[3];
+      for (int c = 0; c < 3; ++c) {
+        di[c] = f1 * (rjk[c] * bjiinv * bjkinv - scp * rji[c] * bji2inv);
+        dk[c] = f1 * (rji[c] * bjiinv * bjkinv - scp * rjk[c] * bjk2inv);
+      }
+      if (me == i) for (int c = 0; c < 3; ++c) g[c] += dv * di[c];
+      else if (me == kk) for (int c = 0; c < 3; ++c) g[c] += dv * dk[c];
+      else for (int c = 0; c < 3; ++c) g[c] -= dv * (di[c] + dk[c]);
+      if (me == i) e[n < S.nang_solute ? 1 : 4] += 0.5 * fk * (da * da);
+    } else {
+      const int* T = S.tor + 5 * n;
+      const int i = T[0], j = T[1], kk = T[2], l = T[3], cod = T[4] - 1;
+      const double* L = S.torlib + 4 * cod;
+      double rji[3], rjk[3], rkl[3], rnj[3], rnk[3];
+      for (int c = 0; c < 3; ++c) {
+        rji[c] = x[3 * i + c] - x[3 * j + c];
+        rjk[c] = x[3 * kk + c] - x[3 * j + c];
+        rkl[c] = x[3 * l + c] - x[3 * kk + c];
+      }
+      rnj[0] = rji[1] * rjk[2] - rji[2] * rjk[1];
+      rnj[1] = rji[2] * rjk[0] - rji[0] * rjk[2];
+      rnj[2] = rji[0] * rjk[1] - rji[1] * rjk[0];
+      rnk[0] = -rjk[1] * rkl[2] + rjk[2] * rkl[1];
+      rnk[1] = -rjk[2] * rkl[0] + rjk[0] * rkl[2];
+      rnk[2] = -rjk[0] * rkl[1] + rjk[1] * rkl[0];
+      double bj2inv = 1.0 / (rnj[0] * rnj[0] + rnj[1] * rnj[1] + rnj[2] * rnj[2]);
+      double bk2inv = 1.0 / (rnk[0] * rnk[0] + rnk[1] * rnk[1] + rnk[2] * rnk[2]);
+      double bjinv = sqrt(bj2inv), bkinv = sqrt(bk2inv);
+      double scp = (rnj[0] * rnk[0] + rnj[1] * rnk[1] + rnj[2] * rnk[2]) * (bjinv * bkinv);
+      double phi;
+      if (scp > 1.0) { scp = 1.0; phi = acos(1.0); }
+      else if (scp < -1.0) { scp = -1.0; phi = acos(-1.0); }
+      else phi = acos(scp);
+      if (rjk[0] * (rnj[1] * rnk[2] - rnj[2] * rnk[1]) + rjk[1] * (rnj[2] * rnk[0] - rnj[0] * rnk[2]) +
+              rjk[2] * (rnj[0] * rnk[1] - rnj[1] * rnk[0]) < 0)
+        phi = -phi;
+      double arg = L[1] * phi - L[2];
+      double dv = -L[1] * L[0] * sin(arg) * L[3];
+      double f1 = sin(phi);
+      if (fabs(f1) < qsys::kQrealEps) f1 = qsys::kQrealEps;
+      f1 = -1.0 / f1;
+      double di[3], dl[3], rki[3], rlj[3];
+      for (int c = 0; c < 3; ++c) {
+        di[c] = f1 * (rnk[c] * (bjinv * bkinv) - scp * rnj[c] * bj2inv);
+        dl[c] = f1 * (rnj[c] * (bjinv * bkinv) - scp * rnk[c] * bk2inv);
+        rki[c] = rji[c] - rjk[c];
+        rlj[c] = -rjk[c] - rkl[c];
+      }
+      double dp[3];
+      if (me == i) {
+        dp[0] = rjk[1] * di[2] - rjk[2] * di[1];
+        dp[1] = rjk[2] * di[0] - rjk[0] * di[2];
+        dp[2] = rjk[0] * di[1] - rjk[1] * di[0];
+      } else if (me == j) {
+        dp[0] = rki[1] * di[2] - rki[2] * di[1] + rkl[1] * dl[2] - rkl[2] * dl[1];
+        dp[1] = rki[2] * di[0] - rki[0] * di[2] + rkl[2] * dl[0] - rkl[0] * dl[2];
+        dp[2] = rki[0] * di[1] - rki[1] * di[0] + rkl[0] * dl[1] - rkl[1] * dl[0];
+      } else if (me == kk) {
+        dp[0] = rlj[1] * dl[2] - rlj[2] * dl[1] - rji[1] * di[2] + rji[2] * di[1];
+        dp[1] = rlj[2] * dl[0] - rlj[0] * dl[2] - rji[2] * di[0] + rji[0] * di[2];
+        dp[2] = rlj[0] * dl[1] - rlj[1] * dl[0] - rji[0] * di[1] + rji[1] * di[0];
+      } else {
+        dp[0] = rjk[1] * dl[2] - rjk[2] * dl[1];
+        dp[1] = rjk[2] * dl[0] - rjk[0] * dl[2];
+        dp[2] = rjk[0] * dl[1] - rjk[1] * dl[0];
+      }
+      for (int c = 0; c < 3; ++c) g[c] += dv * dp[c];
+      if (me == i) e[n < S.ntor_solute ? 2 : 5] += L[0] * (1.0 + cos(arg)) * L[3];
+    }
+  }
+  S.d[3 * me] = S.d[3 * me] + g[0];
+  S.d[3 * me + 1] = S.d[3 * me + 1] + g[1];
+  S.d[3 * me + 2] = S.d[3 * me + 2] + g[2];
+  double* eb = S.e_bonded + (size_t)t * 6;
+  for (int c = 0; c < 6; ++c) eb[c] = e[c];
+}
+
+}  // namespace qgpu

@@ -244,6 +244,46 @@ __global__ __launch_bounds__(256) void k_water_forces(const DevSys* __restrict__
       g[1] = g[1] + dv * dr1;
       g[2] = g[2] + dv * dr2;
     }
+    // Q-water pairs seen from the water (nonbond_qw_spc, src/md.f90:14411-
+    // 14570, the Q atom is i of each pair): O with LJ + Coulomb, H Coulomb
+    if (S.flags[io] & F_QW) {
+      const int ns = S.nstates;
+      for (int iq = 0; iq < S.nqat; ++iq) {
+        const int qi = S.iqseq[iq];
+        const double* xq = S.x + 3 * qi;
+        const int lo = S.qw_lj[iq];
+        const int ti = S.iac[qi];
+        P dx1 = (P)(xi[0] - xq[0]), dx2 = (P)(xi[1] - xq[1]), dx3 = (P)(xi[2] - xq[2]);
+        P r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
+        P r6_hc = r2 * r2 * r2;
+        r2 = (P)1 / r2;
+        P r = psqrt(r2);
+        P r6 = (P)1 / r6_hc;
+        P Va = (P)S.avdw[ti * 3 + lo - 1] * (P)S.alj_O[lo - 1] * r6 * r6;
+        P Vb = (P)S.bvdw[ti * 3 + lo - 1] * (P)S.blj_O[lo - 1] * r6;
+        P dv = 0;
+        for (int st = 0; st < ns; ++st) {
+          P Vel = q[0] * (P)S.qcrg[iq * ns + st] * r;
+          dv = dv + r2 * (-Vel - ((P)12 * Va - (P)6 * Vb) * (r6_hc / r6_hc)) * (P)S.lambda[st];
+        }
+        g[0] = g[0] + (double)(dv * dx1);
+        g[1] = g[1] + (double)(dv * dx2);
+        g[2] = g[2] + (double)(dv * dx3);
+        for (int ip = 1; ip < 3; ++ip) {
+          P ex1 = (P)(xi[3 * ip] - xq[0]), ex2 = (P)(xi[3 * ip + 1] - xq[1]), ex3 = (P)(xi[3 * ip + 2] - xq[2]);
+          P q2 = (P)1 / (ex1 * ex1 + ex2 * ex2 + ex3 * ex3);
+          P rr = psqrt(q2);
+          P dvh = 0;
+          for (int st = 0; st < ns; ++st) {
+            P Vel = q[ip] * (P)S.qcrg[iq * ns + st] * rr;
+            dvh = dvh - q2 * Vel * (P)S.lambda[st];
+          }
+          g[3 * ip] = g[3 * ip] + (double)(dvh * ex1);
+          g[3 * ip + 1] = g[3 * ip + 1] + (double)(dvh * ex2);
+          g[3 * ip + 2] = g[3 * ip + 2] + (double)(dvh * ex3);
+        }
+      }
+    }
     for (int k = 0; k < 9; ++k) S.d[3 * io + k] = g[k];
     double* ew = S.e_water + (size_t)iw * 4;
     ew[0] = e_el; ew[1] = e_vdw; ew[2] = e_lrf; ew[3] = e_rad;
@@ -502,9 +542,35 @@ __global__ __launch_bounds__(256) void k_energy_reduce(const DevSys* __restrict_
   double p[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int g = threadIdx.x; g < S.ncgp_solute; g += blockDim.x)
     for (int c = 0; c < 8; ++c) p[c] += S.e_solute[(size_t)g * 8 + c];
+  double b[6] = {0, 0, 0, 0, 0, 0};
+  for (int t = threadIdx.x; t < S.nbonded_atoms; t += blockDim.x)
+    for (int c = 0; c < 6; ++c) b[c] += S.e_bonded[(size_t)t * 6 + c];
   for (int c = 0; c < 4; ++c) w[c] = block_sum_256(w[c], red[c]);
   for (int c = 0; c < 8; ++c) p[c] = block_sum_256(p[c], red[4 + c]);
+  for (int c = 0; c < 6; ++c) b[c] = block_sum_256(b[c], red[c]);
+  // per-state Q energies: qq, qp (Q-neighbour pairs of nonbond_qq and the
+  // Q-solute list), qw
+  for (int st = 0; st < S.nstates; ++st) {
+    double q[6] = {0, 0, 0, 0, 0, 0};
+    for (int iq = threadIdx.x; iq < S.nqat; iq += blockDim.x) {
+      const double* e = S.e_q + ((size_t)iq * S.nstates + st) * 4;
+      q[2] += e[0]; q[3] += e[1]; q[4] += e[2]; q[5] += e[3];
+    }
+    for (int pr = threadIdx.x; pr < S.nqq; pr += blockDim.x) {
+      if (S.qq_state[pr] != st) continue;
+      const int o = S.qq_jq[pr] >= 0 ? 0 : 2;
+      q[o] += S.e_qq[2 * pr];
+      q[o + 1] += S.e_qq[2 * pr + 1];
+    }
+    for (int c = 0; c < 6; ++c) {
+      double v = block_sum_256(q[c], red[c]);
+      if (threadIdx.x == 0) S.qenergies[st * kQTerms + c] = v;
+    }
+  }
   if (threadIdx.x == 0) {
+    S.energies[E_P_BOND] = b[0]; S.energies[E_P_ANGLE] = b[1]; S.energies[E_P_TORSION] = b[2];
+    S.energies[E_W_BOND] = b[3]; S.energies[E_W_ANGLE] = b[4]; S.energies[E_W_TORSION] = b[5];
+    S.energies[E_P_IMPROPER] = 0.0;
     double* E = S.energies;
     E[E_WW_EL] = w[0]; E[E_WW_VDW] = w[1]; E[E_LRF] = w[2] + p[4]; E[E_RST_RADIAL] = w[3];
     E[E_PP_EL] = p[0]; E[E_PP_VDW] = p[1]; E[E_PW_EL] = p[2]; E[E_PW_VDW] = p[3];

@@ -138,6 +138,12 @@ class Engine:
         self._check(self.lib.qgpu_energies(self.h, k, _dp(e), len(e)))
         return dict(zip(ENERGY_NAMES, e.tolist()))
 
+    def qenergies(self, k: int = 0) -> np.ndarray:
+        """Per-state Q energies: rows of qq_el qq_vdw qp_el qp_vdw qw_el qw_vdw total."""
+        e = np.zeros(7 * 16)
+        self._check(self.lib.qgpu_qenergies(self.h, k, _dp(e), len(e)))
+        return e
+
     def pairlist(self, k: int, which: int) -> np.ndarray:
         n = ctypes.c_long(0)
         self._check(self.lib.qgpu_get_pairlist(self.h, k, which, None, ctypes.byref(n)))

@@ -125,7 +125,7 @@ def _random_rotations(rng, n):
 
 
 def water_lattice(radius: float, rng, avoid: Optional[np.ndarray] = None,
-                  avoid_r: float = 2.6):
+                  avoid_r: float = 3.0):
     """Waters on a cubic lattice of spacing DENSITY^(-1/3) with O inside the
     sphere, random orientations; lattice sites within avoid_r of any solute
     atom are dropped.  Returns (nw,3,3) coordinates O,H1,H2."""
@@ -147,33 +147,43 @@ def water_lattice(radius: float, rng, avoid: Optional[np.ndarray] = None,
     return w
 
 
-def _branched_solute(n_atoms: int, rmax: float, rng, bond=1.53):
-    """A linear backbone with one-atom branches on every other backbone atom,
-    grown as a self-avoiding walk confined to r < rmax.  Atoms are numbered in
-    growth order (backbone atom, then its branch).  Returns coords (n,3) and
-    the bond list (0-based pairs)."""
-    coords: List[np.ndarray] = []
+def _branched_solute(n_atoms: int, rmax: float, rng, bond=1.53, min_dist=3.2):
+    """A branched chain grown as a self-avoiding walk confined to r < rmax:
+    the chain grows from its tip, a one-atom branch is added on every other
+    backbone atom, and when the tip is trapped the walk resumes from another
+    backbone atom with a free valence (a branch point).  Atoms closer than
+    three bonds to the new atom may sit closer than min_dist; all others may
+    not.  Atoms are numbered in growth order.  Returns coords (n,3) and the
+    bond list (0-based pairs)."""
+    coords = np.zeros((n_atoms, 3))
+    nbr: List[List[int]] = [[] for _ in range(n_atoms)]
     bonds = []
-    backbone: List[int] = []
+    backbone: List[int] = [0]
+    n = 1
 
-    def ok(p, skip=()):
-        if np.linalg.norm(p) > rmax:
-            return False
-        for k, c in enumerate(coords):
-            if k in skip:
-                continue
-            if np.sum((c - p) ** 2) < 2.4 ** 2:
-                return False
-        return True
+    def near(parent):
+        # atoms within two bonds of the parent: 1-2, 1-3 and 1-4 of the new atom
+        s1 = {parent}
+        s2 = set(nbr[parent])
+        s3 = set(k for a in s2 for k in nbr[a])
+        return s1 | s2 | s3
 
-    def place_from(parent, grand, tries=200):
+    def place_from(parent, tries=60):
+        skip = near(parent)
+        others = np.array([k for k in range(n) if k not in skip], dtype=int)
+        oc = coords[others] if len(others) else np.zeros((0, 3))
+        # 1-3 partners at >= 2.35 A (angle >~ 100 deg), 1-4 partners >= 2.6 A
+        n13 = np.array(nbr[parent], dtype=int)
+        n14 = np.array(sorted(skip - {parent} - set(nbr[parent])), dtype=int)
+        c13 = coords[n13] if len(n13) else np.zeros((0, 3))
+        c14 = coords[n14] if len(n14) else np.zeros((0, 3))
+        prev = nbr[parent][0] if nbr[parent] else None
         for _ in range(tries):
-            if grand is None:
+            if prev is None:
                 d = rng.normal(size=3)
             else:
-                u = coords[parent] - coords[grand]
+                u = coords[parent] - coords[prev]
                 u /= np.linalg.norm(u)
-                # cone at ~70 deg from the previous bond (109.5 deg angle)
                 t = rng.normal(size=3)
                 t -= t.dot(u) * u
                 t /= np.linalg.norm(t)
@@ -181,30 +191,45 @@ def _branched_solute(n_atoms: int, rmax: float, rng, bond=1.53):
                 d = math.cos(ang) * u + math.sin(ang) * t
             d /= np.linalg.norm(d)
             p = coords[parent] + bond * d
-            skip = {parent} if grand is None else {parent, grand}
-            if ok(p, skip):
-                return p
+            if np.linalg.norm(p) > rmax:
+                continue
+            if len(oc) and ((oc - p) ** 2).sum(1).min() < min_dist ** 2:
+                continue
+            if len(c13) and ((c13 - p) ** 2).sum(1).min() < 2.40 ** 2:
+                continue
+            if len(c14) and ((c14 - p) ** 2).sum(1).min() < 2.9 ** 2:
+                continue
+            return p
         return None
 
-    coords.append(np.zeros(3))
-    backbone.append(0)
-    while len(coords) < n_atoms:
-        parent = backbone[-1]
-        grand = backbone[-2] if len(backbone) > 1 else None
-        p = place_from(parent, grand)
+    def add(parent, p):
+        nonlocal n
+        coords[n] = p
+        nbr[parent].append(n)
+        nbr[n].append(parent)
+        bonds.append((parent, n))
+        n += 1
+        return n - 1
+
+    tip = 0
+    stuck = 0
+    while n < n_atoms:
+        p = place_from(tip)
         if p is None:
-            raise RuntimeError("solute growth failed; increase rmax")
-        coords.append(p)
-        i = len(coords) - 1
-        bonds.append((parent, i))
+            stuck += 1
+            if stuck > 5000:
+                raise RuntimeError("solute growth failed; increase rmax")
+            open_atoms = [a for a in backbone if len(nbr[a]) < 3]
+            tip = open_atoms[int(rng.integers(len(open_atoms)))]
+            continue
+        i = add(tip, p)
         backbone.append(i)
-        if len(backbone) % 2 == 0 and len(coords) < n_atoms:
-            q = place_from(i, parent)
-            if q is None:
-                raise RuntimeError("branch growth failed")
-            coords.append(q)
-            bonds.append((i, len(coords) - 1))
-    c = np.array(coords)
+        tip = i
+        if len(backbone) % 2 == 0 and n < n_atoms:
+            q = place_from(i)
+            if q is not None:
+                add(i, q)
+    c = coords.copy()
     c -= c.mean(0)
     return c, bonds
 
@@ -267,13 +292,15 @@ def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
     rng = np.random.default_rng(seed)
     # --- solute
     if n_solute > 0:
-        rmax = max(6.0, 0.55 * radius)
-        for _ in range(20):
+        rmax = max(6.0, radius - 5.0)
+        for attempt in range(20):
             try:
                 sx, sb = _branched_solute(n_solute, rmax, rng)
                 break
             except RuntimeError:
-                rmax *= 1.1
+                if attempt == 19:
+                    raise
+                rmax *= 1.05
         angles0, tors0, dist = _graph_terms(n_solute, sb)
     else:
         sx = np.zeros((0, 3))
@@ -307,8 +334,10 @@ def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
             s = 2.5 + (k // nb + 0.5) * 1.3 / nb
             e = 0.05 + (k % nb + 0.5) * 0.15 / nb
             a, b = lj_sqrt_ab(s, e)
-            avdw.append([a, a, a / math.sqrt(2.0)])
-            bvdw.append([b, b, b / math.sqrt(2.0)])
+            # 1-4 pairs (LJ type 3): sigma x 0.8, eps x 0.5
+            a14, b14 = lj_sqrt_ab(0.8 * s, 0.5 * e)
+            avdw.append([a, a, a14])
+            bvdw.append([b, b, b14])
             masses.append(12.011)
             tac.append("S%d" % len(tac))
         iac[:ns] = np.array([3 + tmap[k] for k in key.tolist()])

@@ -21,7 +21,8 @@ CASES = {
     "c1_r15_100": (dict(radius=15.0, n_solute=0, seed=1001), dict(steps=100)),
     "c1_r10_250": (dict(radius=10.0, n_solute=0, seed=1001), dict(steps=250)),
     "c2_r14_q8_30": (dict(radius=14.0, n_solute=40, seed=2001, n_q=8, n_states=2),
-                     dict(steps=30, lambdas=[0.7, 0.3])),
+                     dict(steps=30, lambdas=[0.7, 0.3], stepsize=0.5,
+                          temperature=50.0, bath_coupling=2.0, initial_temperature=50.0)),
 }
 
 ORACLE_COLS = {"ww_el": "ww_el", "ww_vdw": "ww_vdw", "lrf": "lrf", "rst_total": "rst_total",
