@@ -58,6 +58,10 @@ int qgpu_destroy(qgpu_handle* h);
 /* Number of atoms of instance k (nat_pro, src/topo.f90:104). */
 int qgpu_natom(qgpu_handle* h, int k, int* natom);
 
+/* Number of FEP states of instance k (nstates, src/qatom.f90:301-420;
+ * 0 without an FEP file). */
+int qgpu_nstates(qgpu_handle* h, int k, int* nstates);
+
 /* Replaces make_pair_lists (src/md.f90:5985-6071, called at :6505): charge-
  * group centres, solute/solvent pair lists and LRF multipoles, from the
  * instance coordinates currently on the device. */

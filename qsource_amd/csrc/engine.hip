@@ -721,6 +721,10 @@ int qgpu_natom(qgpu_handle* h, int k, int* natom) {
   return guarded([&] { CHECK_K(h, k); *natom = h->b->setup(k).natom; });
 }
 
+int qgpu_nstates(qgpu_handle* h, int k, int* nstates) {
+  return guarded([&] { CHECK_K(h, k); *nstates = h->b->setup(k).nstates; });
+}
+
 int qgpu_make_pair_lists(qgpu_handle* h) {
   return guarded([&] {
     if (!h) throw std::runtime_error("null handle");

@@ -28,7 +28,7 @@ ENERGY_NAMES = [
 
 # exported symbols declared in include/qgpu.h
 SYMBOLS = [
-    "qgpu_create", "qgpu_destroy", "qgpu_natom", "qgpu_make_pair_lists", "qgpu_pot_energy",
+    "qgpu_create", "qgpu_destroy", "qgpu_natom", "qgpu_nstates", "qgpu_make_pair_lists", "qgpu_pot_energy",
     "qgpu_step", "qgpu_get_state", "qgpu_set_state", "qgpu_get_gradient", "qgpu_energies",
     "qgpu_qenergies", "qgpu_get_pairlist", "qgpu_temperature", "qgpu_theta_corr",
     "qgpu_last_step_ms", "qgpu_last_error",
@@ -51,6 +51,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
                                 ctypes.POINTER(P)]
     lib.qgpu_destroy.argtypes = [P]
     lib.qgpu_natom.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    lib.qgpu_nstates.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     lib.qgpu_make_pair_lists.argtypes = [P]
     lib.qgpu_pot_energy.argtypes = [P]
     lib.qgpu_step.argtypes = [P, ctypes.c_int]
@@ -88,10 +89,13 @@ class Engine:
         self.h = h
         self.n = len(inputs)
         self.natom = []
+        self.nstates = []
         for k in range(self.n):
             n = ctypes.c_int()
             self._check(self.lib.qgpu_natom(self.h, k, ctypes.byref(n)))
             self.natom.append(n.value)
+            self._check(self.lib.qgpu_nstates(self.h, k, ctypes.byref(n)))
+            self.nstates.append(n.value)
 
     def _check(self, rc: int):
         if rc != 0:
@@ -140,9 +144,9 @@ class Engine:
 
     def qenergies(self, k: int = 0) -> np.ndarray:
         """Per-state Q energies: rows of qq_el qq_vdw qp_el qp_vdw qw_el qw_vdw total."""
-        e = np.zeros(7 * 16)
+        e = np.zeros(7 * max(1, self.nstates[k]))
         self._check(self.lib.qgpu_qenergies(self.h, k, _dp(e), len(e)))
-        return e
+        return e[:7 * self.nstates[k]].reshape(-1, 7)
 
     def pairlist(self, k: int, which: int) -> np.ndarray:
         n = ctypes.c_long(0)
