@@ -92,6 +92,19 @@ int qgpu_energies(qgpu_handle* h, int k, double* e, int cap);
  * 7 values per state: qq_el qq_vdw qp_el qp_vdw qw_el qw_vdw total. */
 int qgpu_qenergies(qgpu_handle* h, int k, double* e, int cap);
 
+/* Replaces the energy-file writes of md_run (put_ene every iene_cycle
+ * steps, src/md.f90:6628-6630) for batched runs: every `iene` steps
+ * (istep > 0) each instance's per-state Q energies are recorded on the
+ * device, up to max_frames frames, with no host synchronisation inside
+ * qgpu_step.  iene = 0 switches recording off. */
+int qgpu_set_energy_interval(qgpu_handle* h, int iene, int max_frames);
+
+/* Recorded frames of instance k: nframes x nstates x 14 values in put_ene
+ * order (total, q bond/angle/torsion/improper, qx el/vdw, qq el/vdw,
+ * qp el/vdw, qw el/vdw, restraint; src/nrgy.f90:128-130).  Writes at most
+ * `cap` doubles; *nframes gets the number recorded. */
+int qgpu_energy_series(qgpu_handle* h, int k, double* out, int cap, int* nframes);
+
 /* Exports pair list `which` of instance k in the reference's layout and
  * order (NB_TYPE i, j, LJcod; 0-based atoms): 0 = nbpp, 1 = nbpw, 2 = nbww
  * (src/md.f90:329-433).  Call with out == NULL to get the count in *n. */
@@ -103,6 +116,15 @@ int qgpu_temperature(qgpu_handle* h, int k, double* temp);
 /* Water-polarisation shell corrections theta_corr (restart record,
  * src/md.f90:18729-18731) of instance k; *n gets the shell count. */
 int qgpu_theta_corr(qgpu_handle* h, int k, double* out, int* n);
+
+/* Runs nsteps steps (no graph) with a HIP event pair around every kernel
+ * launch on the engine's stream and writes the average ms per launch of
+ * each kernel, in this order: cgp_centers, water_lists, solute_lists,
+ * water_forces, solute_forces, bonded, q_forces, qq_forces, watpol_theta,
+ * watpol_force, energy_record, integrate, temperature (0 if not launched).
+ * A measurement aid with no counterpart in the reference (its -DPROFILING
+ * build times whole sections, src/md.f90:6370-6381). */
+int qgpu_profile_kernels(qgpu_handle* h, int nsteps, double* ms, int cap);
 
 /* Average device time per step (ms) of the last qgpu_step call. */
 int qgpu_last_step_ms(qgpu_handle* h, double* ms);

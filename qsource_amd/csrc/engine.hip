@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -21,10 +22,10 @@ __global__ void k_cgp_centers(const DevSys*);
 __global__ void k_water_lists(const DevSys*);
 template <typename P> __global__ void k_water_forces(const DevSys*);
 __global__ void k_watpol_theta(const DevSys*);
-__global__ void k_watpol_force(const DevSys*, int);
+__global__ void k_watpol_force(const DevSys*);
 __global__ void k_integrate(const DevSys*);
 __global__ void k_temperature(const DevSys*, int);
-__global__ void k_energy_reduce(const DevSys*);
+__global__ void k_energy_reduce(const DevSys*, int);
 __global__ void k_solute_lists(const DevSys*);
 template <typename P> __global__ void k_solute_forces(const DevSys*);
 template <typename P> __global__ void k_q_forces(const DevSys*);
@@ -111,16 +112,64 @@ Batch::Batch(const std::vector<std::string>& inputs, int precision, int device) 
     max_ncgp_solute_ = std::max(max_ncgp_solute_, s->top.ncgp_solute);
     max_natom_ = std::max(max_natom_, s->natom);
   }
+  nbcycle_ = setups_[0]->inp.nbcycle;
+  for (auto* s : setups_)
+    if (s->inp.nbcycle != nbcycle_ || nbcycle_ < 1)
+      throw std::runtime_error("all instances of a batch need the same non_bond interval");
+  const char* ng = std::getenv("QGPU_NO_GRAPH");
+  use_graph_ = !(ng && ng[0] == '1');
+  HIPCHECK(hipMalloc(&d_step_, sizeof(int)));
+  HIPCHECK(hipMemset(d_step_, 0, sizeof(int)));
   h_sys_.resize(setups_.size());
-  for (size_t k = 0; k < setups_.size(); ++k) upload((int)k);
+  for (size_t k = 0; k < setups_.size(); ++k) {
+    upload((int)k);
+    h_sys_[k].step_ctr = d_step_;
+  }
   HIPCHECK(hipMalloc(&d_sys_, h_sys_.size() * sizeof(DevSys)));
+  sync_sys();
+}
+
+void Batch::sync_sys() {
   HIPCHECK(hipMemcpy(d_sys_, h_sys_.data(), h_sys_.size() * sizeof(DevSys), hipMemcpyHostToDevice));
 }
 
+void Batch::set_energy_interval(int iene, int max_frames) {
+  if (iene < 0 || max_frames < 0) throw std::runtime_error("negative energy interval or frame count");
+  HIPCHECK(hipStreamSynchronize(stream_));
+  for (size_t k = 0; k < h_sys_.size(); ++k) {
+    DevSys& D = h_sys_[k];
+    D.iene = iene;
+    D.eseries_cap = max_frames;
+    D.eseries = inst_[k]->alloc<double>((size_t)std::max(1, max_frames) * std::max(1, D.nstates) * kEneRecord);
+    D.eseries_n = inst_[k]->alloc<int>(1);
+  }
+  iene_ = iene;
+  sync_sys();
+  if (graph_exec_) {
+    HIPCHECK(hipGraphExecDestroy(graph_exec_));
+    graph_exec_ = nullptr;
+  }
+}
+
+int Batch::energy_series(int k, double* out, int cap) {
+  HIPCHECK(hipStreamSynchronize(stream_));
+  const DevSys& D = h_sys_[k];
+  if (!D.eseries_n) return 0;
+  int n = 0;
+  HIPCHECK(hipMemcpy(&n, D.eseries_n, sizeof(int), hipMemcpyDeviceToHost));
+  n = std::min(n, D.eseries_cap);
+  size_t per = (size_t)D.nstates * kEneRecord;
+  size_t want = std::min((size_t)cap / std::max<size_t>(per, 1), (size_t)n);
+  if (out && want) HIPCHECK(hipMemcpy(out, D.eseries, want * per * 8, hipMemcpyDeviceToHost));
+  return n;
+}
+
 Batch::~Batch() {
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
   for (auto* i : inst_) delete i;
   for (auto* s : setups_) delete s;
   if (d_sys_) (void)hipFree(d_sys_);
+  if (d_step_) (void)hipFree(d_step_);
   if (stream_) {
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
@@ -408,14 +457,14 @@ void Batch::upload(int k) {
 void Batch::launch_lists() {
   const int ni = (int)h_sys_.size();
   dim3 g0((max_ncgp_ + 255) / 256, ni);
-  hipLaunchKernelGGL(k_cgp_centers, g0, dim3(256), 0, stream_, d_sys_);
+  { pb(0); hipLaunchKernelGGL(k_cgp_centers, g0, dim3(256), 0, stream_, d_sys_); pe(0); }
   if (max_nwat_ > 0) {
     dim3 g1((max_nwat_ * 64 + 255) / 256, ni);
-    hipLaunchKernelGGL(k_water_lists, g1, dim3(256), 0, stream_, d_sys_);
+    { pb(1); hipLaunchKernelGGL(k_water_lists, g1, dim3(256), 0, stream_, d_sys_); pe(1); }
   }
   if (max_ncgp_solute_ > 0) {
     dim3 g2((max_ncgp_solute_ * 64 + 255) / 256, ni);
-    hipLaunchKernelGGL(k_solute_lists, g2, dim3(256), 0, stream_, d_sys_);
+    { pb(2); hipLaunchKernelGGL(k_solute_lists, g2, dim3(256), 0, stream_, d_sys_); pe(2); }
   }
 }
 
@@ -424,57 +473,117 @@ void Batch::launch_forces() {
   if (max_nwat_ > 0) {
     dim3 g1((max_nwat_ * 64 + 255) / 256, ni);
     if (precision_ == 0)
-      hipLaunchKernelGGL(k_water_forces<double>, g1, dim3(256), 0, stream_, d_sys_);
+      { pb(3); hipLaunchKernelGGL(k_water_forces<double>, g1, dim3(256), 0, stream_, d_sys_); pe(3); }
     else
-      hipLaunchKernelGGL(k_water_forces<float>, g1, dim3(256), 0, stream_, d_sys_);
+      { pb(3); hipLaunchKernelGGL(k_water_forces<float>, g1, dim3(256), 0, stream_, d_sys_); pe(3); }
   }
   if (max_ncgp_solute_ > 0) {
     dim3 gs((max_ncgp_solute_ * 64 + 255) / 256, ni);
     if (precision_ == 0)
-      hipLaunchKernelGGL(k_solute_forces<double>, gs, dim3(256), 0, stream_, d_sys_);
+      { pb(4); hipLaunchKernelGGL(k_solute_forces<double>, gs, dim3(256), 0, stream_, d_sys_); pe(4); }
     else
-      hipLaunchKernelGGL(k_solute_forces<float>, gs, dim3(256), 0, stream_, d_sys_);
+      { pb(4); hipLaunchKernelGGL(k_solute_forces<float>, gs, dim3(256), 0, stream_, d_sys_); pe(4); }
   }
   if (max_nbonded_atoms_ > 0)
-    hipLaunchKernelGGL(k_bonded, dim3((max_nbonded_atoms_ + 255) / 256, ni), dim3(256), 0, stream_, d_sys_);
+    { pb(5); hipLaunchKernelGGL(k_bonded, dim3((max_nbonded_atoms_ + 255) / 256, ni), dim3(256), 0, stream_, d_sys_); pe(5); }
   if (max_nqat_ > 0) {
     dim3 gq((max_nqat_ * 64 + 255) / 256, ni);
     if (precision_ == 0)
-      hipLaunchKernelGGL(k_q_forces<double>, gq, dim3(256), 0, stream_, d_sys_);
+      { pb(6); hipLaunchKernelGGL(k_q_forces<double>, gq, dim3(256), 0, stream_, d_sys_); pe(6); }
     else
-      hipLaunchKernelGGL(k_q_forces<float>, gq, dim3(256), 0, stream_, d_sys_);
+      { pb(6); hipLaunchKernelGGL(k_q_forces<float>, gq, dim3(256), 0, stream_, d_sys_); pe(6); }
   }
   if (max_nqq_atoms_ > 0)
-    hipLaunchKernelGGL(k_qq_forces, dim3((max_nqq_atoms_ + 255) / 256, ni), dim3(256), 0, stream_, d_sys_);
+    { pb(7); hipLaunchKernelGGL(k_qq_forces, dim3((max_nqq_atoms_ + 255) / 256, ni), dim3(256), 0, stream_, d_sys_); pe(7); }
   if (max_nwat_ > 0) {
     dim3 g2((max_nwat_ + 255) / 256, ni);
-    hipLaunchKernelGGL(k_watpol_theta, g2, dim3(256), 0, stream_, d_sys_);
-    int update = (istep_ != 0 && istep_ % qsys::kItdisUpdate == 0) ? 1 : 0;
+    { pb(8); hipLaunchKernelGGL(k_watpol_theta, g2, dim3(256), 0, stream_, d_sys_); pe(8); }
     dim3 g3(kNumShellMax, ni);
-    hipLaunchKernelGGL(k_watpol_force, g3, dim3(1024), 0, stream_, d_sys_, update);
+    { pb(9); hipLaunchKernelGGL(k_watpol_force, g3, dim3(1024), 0, stream_, d_sys_); pe(9); }
   }
 }
 
 void Batch::launch_integrate() {
   const int ni = (int)h_sys_.size();
-  int blocks = (std::max(max_units_, 1) + 255) / 256;
-  for (size_t k = 0; k < h_sys_.size(); ++k)
-    if (blocks > h_sys_[k].ke_blocks) throw std::runtime_error("kinetic partial buffer too small");
-  hipLaunchKernelGGL(k_integrate, dim3(blocks, ni), dim3(256), 0, stream_, d_sys_);
-  hipLaunchKernelGGL(k_temperature, dim3(1, ni), dim3(256), 0, stream_, d_sys_, blocks);
+  const int blocks = (std::max(max_units_, 1) + 255) / 256;
+  // energy-file frame of this step (the kernel checks the step counter)
+  if (iene_ > 0) { pb(10); hipLaunchKernelGGL(k_energy_reduce, dim3(1, ni), dim3(256), 0, stream_, d_sys_, 1); pe(10); }
+  { pb(11); hipLaunchKernelGGL(k_integrate, dim3(blocks, ni), dim3(256), 0, stream_, d_sys_); pe(11); }
+  { pb(12); hipLaunchKernelGGL(k_temperature, dim3(1, ni), dim3(256), 0, stream_, d_sys_, blocks); pe(12); }
 }
 
 void Batch::make_lists() { launch_lists(); }
 void Batch::forces() { launch_forces(); }
 
+void Batch::one_step() {
+  if (istep_ % nbcycle_ == 0) launch_lists();
+  launch_forces();
+  launch_integrate();
+  ++istep_;
+}
+
+// One list-update cycle (NBcycle steps) captured as a hipGraph: the kernels
+// read the step number from the device counter, so the same graph serves
+// every cycle.  Rebuilt when the energy interval changes.
+void Batch::launch_cycle_graph() {
+  if (!graph_exec_) {
+    hipGraph_t g = nullptr;
+    HIPCHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < nbcycle_; ++k) {
+      if (k == 0) launch_lists();
+      launch_forces();
+      launch_integrate();
+    }
+    HIPCHECK(hipStreamEndCapture(stream_, &g));
+    HIPCHECK(hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0));
+    HIPCHECK(hipGraphDestroy(g));
+  }
+  HIPCHECK(hipGraphLaunch(graph_exec_, stream_));
+  istep_ += nbcycle_;
+}
+
+void Batch::profile(int nsteps, double* ms, int cap) {
+  HIPCHECK(hipStreamSynchronize(stream_));
+  for (auto& e : pev_)
+    if (!e) HIPCHECK(hipEventCreate(&e));
+  double tot[kNumKernels] = {0};
+  int cnt[kNumKernels] = {0};
+  prof_ = true;
+  try {
+    for (int n = 0; n < nsteps; ++n) {
+      for (int& u : pused_) u = 0;
+      one_step();
+      HIPCHECK(hipStreamSynchronize(stream_));
+      for (int k = 0; k < kNumKernels; ++k)
+        if (pused_[k]) {
+          float t = 0;
+          HIPCHECK(hipEventElapsedTime(&t, pev_[2 * k], pev_[2 * k + 1]));
+          tot[k] += t;
+          cnt[k] += 1;
+        }
+    }
+  } catch (...) {
+    prof_ = false;
+    throw;
+  }
+  prof_ = false;
+  for (int k = 0; k < kNumKernels && k < cap; ++k) ms[k] = cnt[k] ? tot[k] / cnt[k] : 0.0;
+}
+
 void Batch::step(int nsteps) {
-  const int nb = setups_[0]->inp.nbcycle;
   HIPCHECK(hipEventRecord(ev0, stream_));
-  for (int n = 0; n < nsteps; ++n) {
-    if (istep_ % nb == 0) launch_lists();
-    launch_forces();
-    launch_integrate();
-    ++istep_;
+  int n = 0;
+  while (n < nsteps && (!use_graph_ || istep_ % nbcycle_ != 0 || nsteps - n < nbcycle_)) {
+    one_step();
+    ++n;
+  }
+  while (use_graph_ && nsteps - n >= nbcycle_) {
+    launch_cycle_graph();
+    n += nbcycle_;
+  }
+  while (n < nsteps) {
+    one_step();
+    ++n;
   }
   HIPCHECK(hipEventRecord(ev1, stream_));
   HIPCHECK(hipGetLastError());
@@ -509,7 +618,7 @@ void Batch::get_gradient(int k, double* d) {
 
 void Batch::energies(int k, double* out, int cap) {
   const int ni = (int)h_sys_.size();
-  hipLaunchKernelGGL(k_energy_reduce, dim3(1, ni), dim3(256), 0, stream_, d_sys_);
+  hipLaunchKernelGGL(k_energy_reduce, dim3(1, ni), dim3(256), 0, stream_, d_sys_, 0);
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(stream_));
   double E[E_NTERMS];
@@ -547,7 +656,7 @@ void Batch::energies(int k, double* out, int cap) {
 // src/md.f90:16190-16194) from the last energy reduction
 void Batch::qenergies(int k, double* out, int cap) {
   const int ni = (int)h_sys_.size();
-  hipLaunchKernelGGL(k_energy_reduce, dim3(1, ni), dim3(256), 0, stream_, d_sys_);
+  hipLaunchKernelGGL(k_energy_reduce, dim3(1, ni), dim3(256), 0, stream_, d_sys_, 0);
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(stream_));
   const qsys::Setup& s = *setups_[k];
@@ -766,6 +875,24 @@ int qgpu_energies(qgpu_handle* h, int k, double* e, int cap) {
 }
 int qgpu_qenergies(qgpu_handle* h, int k, double* e, int cap) {
   return guarded([&] { CHECK_K(h, k); h->b->qenergies(k, e, cap); });
+}
+int qgpu_profile_kernels(qgpu_handle* h, int nsteps, double* ms, int cap) {
+  return guarded([&] {
+    if (!h) throw std::runtime_error("null handle");
+    h->b->profile(nsteps, ms, cap);
+  });
+}
+int qgpu_set_energy_interval(qgpu_handle* h, int iene, int max_frames) {
+  return guarded([&] {
+    if (!h) throw std::runtime_error("null handle");
+    h->b->set_energy_interval(iene, max_frames);
+  });
+}
+int qgpu_energy_series(qgpu_handle* h, int k, double* out, int cap, int* nframes) {
+  return guarded([&] {
+    CHECK_K(h, k);
+    *nframes = h->b->energy_series(k, out, cap);
+  });
 }
 int qgpu_get_pairlist(qgpu_handle* h, int k, int which, int* out, long* n) {
   return guarded([&] {

@@ -145,7 +145,19 @@ struct DevSys {
   double* scal;        // [0]Tscale_solute [1]Tscale_solvent [2]Tfree [3]Temp [4]Ekin [5..] per-class temps
   double* energies;    // E_NTERMS reduced values
   double* qenergies;   // nstates x kQTerms reduced values
+  // step counter shared by the batch (incremented once per step on the
+  // device, so a captured multi-step graph needs no host-side step number)
+  int* step_ctr;
+  // energy-file frames recorded on the device every iene steps (istep > 0,
+  // as md_run calls put_ene, src/md.f90:6628-6630): per frame and state the
+  // 14 values of a put_ene record (src/nrgy.f90:128-130)
+  int iene;
+  int eseries_cap;
+  double* eseries;
+  int* eseries_n;
 };
+
+constexpr int kEneRecord = 14;
 
 // One GPU batch of instances sharing every launch.
 class Batch {
@@ -170,8 +182,34 @@ class Batch {
   // timing helpers
   hipEvent_t ev0, ev1;
 
+  // energy-file frames recorded on the device every iene steps
+  void set_energy_interval(int iene, int max_frames);
+  int energy_series(int k, double* out, int cap);
+  void set_graphs(bool on) { use_graph_ = on; }
+  // per-kernel average device time (ms per launch) over nsteps steps run
+  // with an event pair around every launch; order of kKernelNames
+  void profile(int nsteps, double* ms, int cap);
+  static constexpr int kNumKernels = 13;
+
  private:
+  void pb(int id) {
+    if (prof_) hipEventRecord(pev_[2 * id], stream_);
+  }
+  void pe(int id) {
+    if (prof_) { hipEventRecord(pev_[2 * id + 1], stream_); pused_[id] = 1; }
+  }
+  bool prof_ = false;
+  hipEvent_t pev_[2 * 13] = {};
+  int pused_[13] = {};
   struct Inst;
+  void one_step();
+  void launch_cycle_graph();
+  void sync_sys();
+  int nbcycle_ = 10;
+  int iene_ = 0;
+  bool use_graph_ = true;
+  hipGraphExec_t graph_exec_ = nullptr;
+  int* d_step_ = nullptr;
   void upload(int k);
   void launch_lists();
   void launch_forces();

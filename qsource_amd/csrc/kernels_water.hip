@@ -321,10 +321,12 @@ __global__ void k_watpol_theta(const DevSys* __restrict__ sys) {
 // each member's position in qdyn's selection sort (ascending theta, ties to
 // the lower list index) is its rank; the target angle follows from the rank.
 constexpr int kShellCap = 4096;
-__global__ __launch_bounds__(1024) void k_watpol_force(const DevSys* __restrict__ sys, int update) {
+__global__ __launch_bounds__(1024) void k_watpol_force(const DevSys* __restrict__ sys) {
   const DevSys& S = sys[blockIdx.y];
   const int is = blockIdx.x;
   if (!S.wpol_restr || is >= S.nshell) return;
+  const int istep = *S.step_ctr;
+  const bool update = istep != 0 && istep % qsys::kItdisUpdate == 0;
   __shared__ int mem_iw[kShellCap];
   __shared__ double mem_th[kShellCap];
   __shared__ int s_cnt;
@@ -530,12 +532,22 @@ __global__ __launch_bounds__(256) void k_temperature(const DevSys* __restrict__ 
     tsu = tsv;
   }
   S.scal[0] = tsu; S.scal[1] = tsv; S.scal[2] = tfree; S.scal[3] = temp;
+  // end of the step: advance the batch's step counter once
+  if (blockIdx.y == 0) *S.step_ctr = *S.step_ctr + 1;
 }
 
 // Sum the per-item energy partials into the instance's energy terms.
-__global__ __launch_bounds__(256) void k_energy_reduce(const DevSys* __restrict__ sys) {
+// With record != 0 it runs only on energy-file steps (step counter a
+// positive multiple of iene) and also appends the frame's put_ene records.
+__global__ __launch_bounds__(256) void k_energy_reduce(const DevSys* __restrict__ sys, int record) {
   const DevSys& S = sys[blockIdx.y];
   __shared__ double red[12][4];
+  int frame = -1;
+  if (record) {
+    const int st = *S.step_ctr;
+    if (S.iene <= 0 || st <= 0 || st % S.iene != 0) return;
+    frame = st / S.iene - 1;
+  }
   double w[4] = {0, 0, 0, 0};
   for (int iw = threadIdx.x; iw < S.nwat; iw += blockDim.x)
     for (int c = 0; c < 4; ++c) w[c] += S.e_water[(size_t)iw * 4 + c];
@@ -563,10 +575,23 @@ __global__ __launch_bounds__(256) void k_energy_reduce(const DevSys* __restrict_
       q[o + 1] += S.e_qq[2 * pr + 1];
     }
     for (int c = 0; c < 6; ++c) {
-      double v = block_sum_256(q[c], red[c]);
-      if (threadIdx.x == 0) S.qenergies[st * kQTerms + c] = v;
+      q[c] = block_sum_256(q[c], red[c]);
+      if (threadIdx.x == 0) S.qenergies[st * kQTerms + c] = q[c];
+    }
+    if (threadIdx.x == 0 && frame >= 0 && frame < S.eseries_cap) {
+      // put_ene record: total, q(bond angle torsion improper), qx, qq, qp,
+      // qw (el, vdw each), restraint (src/nrgy.f90:128-130; EQ totals as
+      // pot_energy forms them, src/md.f90:16186-16194)
+      double* r = S.eseries + ((size_t)frame * S.nstates + st) * kEneRecord;
+      const double qx_el = q[0] + q[2] + q[4], qx_vdw = q[1] + q[3] + q[5];
+      r[0] = 0.0 + 0.0 + 0.0 + 0.0 + qx_el + qx_vdw + 0.0;
+      r[1] = 0.0; r[2] = 0.0; r[3] = 0.0; r[4] = 0.0;
+      r[5] = qx_el; r[6] = qx_vdw;
+      r[7] = q[0]; r[8] = q[1]; r[9] = q[2]; r[10] = q[3]; r[11] = q[4]; r[12] = q[5];
+      r[13] = 0.0;
     }
   }
+  if (threadIdx.x == 0 && frame >= 0) *S.eseries_n = frame + 1;
   if (threadIdx.x == 0) {
     S.energies[E_P_BOND] = b[0]; S.energies[E_P_ANGLE] = b[1]; S.energies[E_P_TORSION] = b[2];
     S.energies[E_W_BOND] = b[3]; S.energies[E_W_ANGLE] = b[4]; S.energies[E_W_TORSION] = b[5];

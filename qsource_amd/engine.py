@@ -31,8 +31,13 @@ SYMBOLS = [
     "qgpu_create", "qgpu_destroy", "qgpu_natom", "qgpu_nstates", "qgpu_make_pair_lists", "qgpu_pot_energy",
     "qgpu_step", "qgpu_get_state", "qgpu_set_state", "qgpu_get_gradient", "qgpu_energies",
     "qgpu_qenergies", "qgpu_get_pairlist", "qgpu_temperature", "qgpu_theta_corr",
-    "qgpu_last_step_ms", "qgpu_last_error",
+    "qgpu_last_step_ms", "qgpu_last_error", "qgpu_set_energy_interval", "qgpu_energy_series",
+    "qgpu_profile_kernels",
 ]
+
+KERNEL_NAMES = ["cgp_centers", "water_lists", "solute_lists", "water_forces", "solute_forces",
+                "bonded", "q_forces", "qq_forces", "watpol_theta", "watpol_force",
+                "energy_record", "integrate", "temperature"]
 
 _lib = None
 
@@ -65,6 +70,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.qgpu_temperature.argtypes = [P, ctypes.c_int, dp]
     lib.qgpu_theta_corr.argtypes = [P, ctypes.c_int, dp, ctypes.POINTER(ctypes.c_int)]
     lib.qgpu_last_step_ms.argtypes = [P, dp]
+    lib.qgpu_set_energy_interval.argtypes = [P, ctypes.c_int, ctypes.c_int]
+    lib.qgpu_profile_kernels.argtypes = [P, ctypes.c_int, dp, ctypes.c_int]
+    lib.qgpu_energy_series.argtypes = [P, ctypes.c_int, dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     lib.qgpu_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
@@ -168,6 +176,25 @@ class Engine:
         out = np.zeros(max(n.value, 1))
         self._check(self.lib.qgpu_theta_corr(self.h, k, _dp(out), ctypes.byref(n)))
         return out[:n.value]
+
+    def set_energy_interval(self, iene: int, max_frames: int):
+        self._check(self.lib.qgpu_set_energy_interval(self.h, iene, max_frames))
+        self._max_frames = max_frames
+
+    def energy_series(self, k: int = 0) -> np.ndarray:
+        """(nframes, nstates, 14) put_ene records recorded on the device."""
+        ns = max(1, self.nstates[k])
+        cap = max(1, getattr(self, "_max_frames", 0)) * ns * 14
+        out = np.zeros(cap)
+        n = ctypes.c_int()
+        self._check(self.lib.qgpu_energy_series(self.h, k, _dp(out), cap, ctypes.byref(n)))
+        return out[:n.value * ns * 14].reshape(n.value, ns, 14)
+
+    def profile_kernels(self, nsteps: int) -> dict:
+        """Average ms per launch of each kernel over nsteps event-timed steps."""
+        ms = np.zeros(len(KERNEL_NAMES))
+        self._check(self.lib.qgpu_profile_kernels(self.h, nsteps, _dp(ms), len(ms)))
+        return dict(zip(KERNEL_NAMES, ms.tolist()))
 
     def last_step_ms(self) -> float:
         t = ctypes.c_double()

@@ -113,3 +113,27 @@ def test_repeat_run_bitwise_deterministic(c2_small):
     xa, va = a.state(0)
     xb, vb = b.state(0)
     assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+
+
+def test_energy_series_and_graph(c2_small, monkeypatch):
+    _say("case: device energy frames every 10 steps (graph on/off), R=14 solute + Q, FP64")
+    o = Oracle(c2_small["inp"])
+    ref = []
+    for k in range(50):
+        o.run(1)
+        if k > 0 and k % 10 == 0:
+            ref.append(o.qenergies()[:, 6].copy())
+    ref = np.array(ref)
+    runs = {}
+    for graph in ("1", "0"):
+        monkeypatch.setenv("QGPU_NO_GRAPH", "0" if graph == "1" else "1")
+        e = qe.Engine([c2_small["inp"]], qe.QGPU_FP64)
+        e.set_energy_interval(10, 16)
+        e.step(50)
+        fr = e.energy_series(0)
+        assert fr.shape == (4, 2, 14)
+        assert np.allclose(fr[:, :, 0], ref, rtol=1e-9, atol=1e-9), (fr[:, :, 0], ref)
+        runs[graph] = (fr, e.state(0))
+    # the captured multi-step graph and per-step launches are the same kernels
+    assert np.array_equal(runs["1"][0], runs["0"][0])
+    assert np.array_equal(runs["1"][1][0], runs["0"][1][0])
