@@ -96,7 +96,8 @@ int qgpu_qenergies(qgpu_handle* h, int k, double* e, int cap);
  * steps, src/md.f90:6628-6630) for batched runs: every `iene` steps
  * (istep > 0) each instance's per-state Q energies are recorded on the
  * device, up to max_frames frames, with no host synchronisation inside
- * qgpu_step.  iene = 0 switches recording off. */
+ * qgpu_step.  Steps are counted from this call (the reference counts from
+ * the start of the run).  iene = 0 switches recording off. */
 int qgpu_set_energy_interval(qgpu_handle* h, int iene, int max_frames);
 
 /* Recorded frames of instance k: nframes x nstates x 14 values in put_ene
@@ -109,6 +110,11 @@ int qgpu_energy_series(qgpu_handle* h, int k, double* out, int cap, int* nframes
  * order (NB_TYPE i, j, LJcod; 0-based atoms): 0 = nbpp, 1 = nbpw, 2 = nbww
  * (src/md.f90:329-433).  Call with out == NULL to get the count in *n. */
 int qgpu_get_pairlist(qgpu_handle* h, int k, int which, int* out, long* n);
+
+/* Sizes of the pair lists of instance k in the reference's layout (the
+ * counts md_run prints after make_pair_lists, src/md.f90:6508-6512):
+ * counts[0..4] = nbpp, nbpw, nbww, nbqp, and Q-water atom pairs. */
+int qgpu_list_counts(qgpu_handle* h, int k, long* counts);
 
 /* Current temperature (Temp, src/md.f90:6159-6163) of instance k. */
 int qgpu_temperature(qgpu_handle* h, int k, double* temp);

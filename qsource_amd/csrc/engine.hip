@@ -139,6 +139,7 @@ void Batch::set_energy_interval(int iene, int max_frames) {
   for (size_t k = 0; k < h_sys_.size(); ++k) {
     DevSys& D = h_sys_[k];
     D.iene = iene;
+    D.eseries_base = istep_;
     D.eseries_cap = max_frames;
     D.eseries = inst_[k]->alloc<double>((size_t)std::max(1, max_frames) * std::max(1, D.nstates) * kEneRecord);
     D.eseries_n = inst_[k]->alloc<int>(1);
@@ -759,6 +760,29 @@ void Batch::export_list(int k, int which, std::vector<int>& out) {
   }
 }
 
+void Batch::list_counts(int k, long* counts) {
+  std::vector<int> v;
+  export_list(k, 0, v);
+  counts[0] = (long)(v.size() / 3);
+  export_list(k, 1, v);
+  counts[1] = (long)(v.size() / 3);
+  const DevSys& D = h_sys_[k];
+  std::vector<int> cnt(D.nwat), nb((size_t)D.nwat * D.ww_cap);
+  if (D.nwat > 0) {
+    HIPCHECK(hipMemcpy(cnt.data(), D.ww_cnt, 4 * cnt.size(), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(nb.data(), D.ww_nb, 4 * nb.size(), hipMemcpyDeviceToHost));
+  }
+  long ww = 0;
+  for (int iw = 0; iw < D.nwat; ++iw)
+    for (int c = 0; c < cnt[iw]; ++c) {
+      int jw = nb[(size_t)iw * D.ww_cap + c];
+      if ((iw < jw) == (((iw + jw) & 1) == 0)) ++ww;
+    }
+  counts[2] = 9 * ww;
+  counts[3] = (long)D.nqp_atoms * D.nqat;
+  counts[4] = 3L * D.nqw_waters * D.nqat;
+}
+
 double Batch::temperature(int k) {
   HIPCHECK(hipStreamSynchronize(stream_));
   double sc[4];
@@ -892,6 +916,12 @@ int qgpu_energy_series(qgpu_handle* h, int k, double* out, int cap, int* nframes
   return guarded([&] {
     CHECK_K(h, k);
     *nframes = h->b->energy_series(k, out, cap);
+  });
+}
+int qgpu_list_counts(qgpu_handle* h, int k, long* counts) {
+  return guarded([&] {
+    CHECK_K(h, k);
+    h->b->list_counts(k, counts);
   });
 }
 int qgpu_get_pairlist(qgpu_handle* h, int k, int which, int* out, long* n) {

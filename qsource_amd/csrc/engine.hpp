@@ -152,6 +152,7 @@ struct DevSys {
   // as md_run calls put_ene, src/md.f90:6628-6630): per frame and state the
   // 14 values of a put_ene record (src/nrgy.f90:128-130)
   int iene;
+  int eseries_base;    // step counter when recording was switched on
   int eseries_cap;
   double* eseries;
   int* eseries_n;
@@ -174,6 +175,7 @@ class Batch {
   void energies(int k, double* e, int cap);
   void qenergies(int k, double* e, int cap);
   void export_list(int k, int which, std::vector<int>& out);
+  void list_counts(int k, long* counts);
   double temperature(int k);
   void theta_corr(int k, double* out);
   int istep() const { return istep_; }

@@ -544,7 +544,7 @@ __global__ __launch_bounds__(256) void k_energy_reduce(const DevSys* __restrict_
   __shared__ double red[12][4];
   int frame = -1;
   if (record) {
-    const int st = *S.step_ctr;
+    const int st = *S.step_ctr - S.eseries_base;
     if (S.iene <= 0 || st <= 0 || st % S.iene != 0) return;
     frame = st / S.iene - 1;
   }

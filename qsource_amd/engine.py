@@ -32,7 +32,7 @@ SYMBOLS = [
     "qgpu_step", "qgpu_get_state", "qgpu_set_state", "qgpu_get_gradient", "qgpu_energies",
     "qgpu_qenergies", "qgpu_get_pairlist", "qgpu_temperature", "qgpu_theta_corr",
     "qgpu_last_step_ms", "qgpu_last_error", "qgpu_set_energy_interval", "qgpu_energy_series",
-    "qgpu_profile_kernels",
+    "qgpu_profile_kernels", "qgpu_list_counts",
 ]
 
 KERNEL_NAMES = ["cgp_centers", "water_lists", "solute_lists", "water_forces", "solute_forces",
@@ -72,6 +72,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.qgpu_last_step_ms.argtypes = [P, dp]
     lib.qgpu_set_energy_interval.argtypes = [P, ctypes.c_int, ctypes.c_int]
     lib.qgpu_profile_kernels.argtypes = [P, ctypes.c_int, dp, ctypes.c_int]
+    lib.qgpu_list_counts.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_long)]
     lib.qgpu_energy_series.argtypes = [P, ctypes.c_int, dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     lib.qgpu_last_error.restype = ctypes.c_char_p
     _lib = lib
@@ -164,6 +165,12 @@ class Engine:
                                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                                ctypes.byref(n)))
         return out[:3 * n.value].reshape(-1, 3)
+
+    def list_counts(self, k: int = 0) -> dict:
+        """Reference-layout list sizes: nbpp, nbpw, nbww, nbqp, qw atom pairs."""
+        c = (ctypes.c_long * 5)()
+        self._check(self.lib.qgpu_list_counts(self.h, k, c))
+        return dict(zip(["pp", "pw", "ww", "qp", "qw"], list(c)))
 
     def temperature(self, k: int = 0) -> float:
         t = ctypes.c_double()
