@@ -128,6 +128,8 @@ def cpu_baseline(d, steps):
     for line in r.stdout.splitlines():
         if "Total time of main loop" in line:
             loop = float(line.split()[-1].replace("(s)", ""))
+    if not loop:
+        return None
     # ns/day at the production step of 1 fs (time per step is independent of dt)
     return {"value": steps * 1.0 * 1e-6 * 86400.0 / loop, "unit": "ns/day", "cores": 1,
             "kind": "reference",
@@ -198,7 +200,7 @@ def main():
         ep, wallp = run_batch([inputs[groups[1][10]]], qe.QGPU_FP64, max(200, args.steps // 4), 50)
         plain = max(200, args.steps // 4) * 1e-6 * 86400.0 / wallp
         ep.close()
-        cpu = cpu_baseline(d, args.cpu_steps)
+        cpu = cpu_baseline(d, args.cpu_steps) if args.cpu_steps > 0 else None
     cfg["precision"] = "mixed"
     out.update({"value": value, "ms_per_step": wall / args.steps * 1e3, "config": cfg,
                 "roofline": roof, "cpu_baseline": cpu, "kernel_ms": prof,
