@@ -401,6 +401,7 @@ void Batch::upload(int k) {
   // SHAKE units
   std::vector<int> ufirst, uatoms, cfirst, shi, shj;
   std::vector<double> shd;
+  std::vector<signed char> ukind;
   std::vector<char> used(N, 0);
   for (auto& m : s.shake_mol) {
     if (m.bond.size() > 64) throw std::runtime_error("SHAKE molecules with more than 64 constraints are not supported");
@@ -413,6 +414,13 @@ void Batch::upload(int k) {
       shi.push_back(c.i); shj.push_back(c.j); shd.push_back(c.dist2);
     }
     std::sort(atoms.begin(), atoms.end());
+    bool water = atoms.size() == 3 && m.bond.size() == 3 && atoms[1] == atoms[0] + 1 && atoms[2] == atoms[0] + 2;
+    if (water) {
+      const int a = atoms[0];
+      water = m.bond[0].i == a && m.bond[0].j == a + 1 && m.bond[1].i == a && m.bond[1].j == a + 2 &&
+              m.bond[2].i == a + 1 && m.bond[2].j == a + 2;
+    }
+    ukind.push_back(water ? 1 : 0);
     uatoms.insert(uatoms.end(), atoms.begin(), atoms.end());
   }
   for (int i = 0; i < N; ++i)
@@ -420,7 +428,9 @@ void Batch::upload(int k) {
       ufirst.push_back((int)uatoms.size());
       cfirst.push_back((int)shi.size());
       uatoms.push_back(i);
+      ukind.push_back(0);
     }
+  D.unit_kind = I->upload(ukind);
   ufirst.push_back((int)uatoms.size());
   cfirst.push_back((int)shi.size());
   D.nunits = (int)ufirst.size() - 1;

@@ -134,6 +134,9 @@ struct DevSys {
   int nunits;
   int has_shake;                 // any SHAKE constraint: v = (x - xx)/dt for every atom
   const int *unit_first, *unit_atoms, *con_first, *sh_i, *sh_j;
+  // 1 for a water unit (atoms a, a+1, a+2 with constraints (a,a+1),
+  // (a,a+2), (a+1,a+2) in that order): integrated entirely in registers
+  const signed char* unit_kind;
   const double* sh_d2;
   int* shake_fail;
   // energies: eacc[term] partial sums written per item, reduced on demand

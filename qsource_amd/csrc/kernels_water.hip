@@ -115,12 +115,28 @@ __global__ __launch_bounds__(256) void k_water_lists(const DevSys* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// Inverse distance powers of one atom pair from the squared distance r2sq:
+// returns 1/r^2 and sets rinv = 1/r.  FP64 keeps the reference's
+// operations (one/r2, then sqrt, src/md.f90:15074-15080); FP32 uses the
+// hardware reciprocal square root (v_rsq_f32, 1 ulp).
+__device__ __forceinline__ double inv_r2(double r2sq, double& rinv) {
+  double r2 = 1.0 / r2sq;
+  rinv = sqrt(r2);
+  return r2;
+}
+__device__ __forceinline__ float inv_r2(float r2sq, float& rinv) {
+  rinv = __builtin_amdgcn_rsqf(r2sq);
+  return rinv * rinv;
+}
+
 // Water forces (nonbond_ww_spc src/md.f90:15021-15119, water side of
-// nonbond_pw :13282-13365, lrf_taylor :5824-5891 and restrain_solvent
-// :18055-18102).  One wavefront per water; each lane takes one partner water
-// (9 atom pairs, LJ on O-O only) or one solute atom pair set; the owner
-// water's 9 gradient components are shuffle-reduced and written once.
-// P is the pair-arithmetic type: double (plain) or float (mixed).
+// nonbond_pw :13282-13365 and nonbond_qw_spc :14411-14570, lrf_taylor
+// :5824-5891, restrain_solvent :18055-18102).  One wavefront per water:
+// lanes take the partner waters (9 atom pairs each, LJ on O-O only), the
+// listed solute groups, the Q atoms, and the per-atom LRF and restraint
+// terms; the water's 9 gradient components are shuffle-reduced and written
+// once.  P is the pair-arithmetic type: double (plain) or float (mixed, the
+// 9 contributions of one partner are summed in FP32 and accumulated in FP64).
 template <typename P>
 __global__ __launch_bounds__(256) void k_water_forces(const DevSys* __restrict__ sys) {
   const DevSys& S = sys[blockIdx.y];
@@ -130,7 +146,7 @@ __global__ __launch_bounds__(256) void k_water_forces(const DevSys* __restrict__
   const int io = S.nat_solute + 3 * iw;
   double g[9];
   for (int k = 0; k < 9; ++k) g[k] = 0.0;
-  double e_el = 0.0, e_vdw = 0.0;
+  double e_el = 0.0, e_vdw = 0.0, e_lrf = 0.0, e_rad = 0.0;
   double xi[9];
   for (int k = 0; k < 9; ++k) xi[k] = S.x[3 * io + k];
   const P q[3] = {(P)S.chg_solv[0], (P)S.chg_solv[1], (P)S.chg_solv[2]};
@@ -144,33 +160,41 @@ __global__ __launch_bounds__(256) void k_water_forces(const DevSys* __restrict__
     const bool own = owns_pair(iw, jw);
     double xj[9];
     for (int t = 0; t < 9; ++t) xj[t] = S.x[3 * jo + t];
+    P el = 0, vdw = 0;
     for (int la = 0; la < 3; ++la) {
+      P gl0 = 0, gl1 = 0, gl2 = 0;
       for (int ip = 0; ip < 3; ++ip) {
         P dx1 = (P)(xj[3 * ip] - xi[3 * la]);
         P dx2 = (P)(xj[3 * ip + 1] - xi[3 * la + 1]);
         P dx3 = (P)(xj[3 * ip + 2] - xi[3 * la + 2]);
         P r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
         P r6 = r2 * r2 * r2;
-        r2 = (P)1 / r2;
-        P r = psqrt(r2);
+        P r;
+        r2 = inv_r2(r2, r);
         P Vel = q[la] * q[ip] * r;
         P dv = r2 * (-Vel);
         if (la == 0 && ip == 0) {
           r6 = (P)1 / r6;
           P Va = aOO * r6 * r6;
           P Vb = bOO * r6;
-          if (own) e_vdw = e_vdw + (double)Va - (double)Vb;
+          vdw = vdw + Va - Vb;
           dv = dv + r2 * ((P)-12 * Va + (P)6 * Vb);
         }
-        if (own) e_el = e_el + (double)Vel;
-        g[3 * la] = g[3 * la] - (double)(dv * dx1);
-        g[3 * la + 1] = g[3 * la + 1] - (double)(dv * dx2);
-        g[3 * la + 2] = g[3 * la + 2] - (double)(dv * dx3);
+        el = el + Vel;
+        gl0 = gl0 - dv * dx1;
+        gl1 = gl1 - dv * dx2;
+        gl2 = gl2 - dv * dx3;
       }
+      g[3 * la] = g[3 * la] + (double)gl0;
+      g[3 * la + 1] = g[3 * la + 1] + (double)gl1;
+      g[3 * la + 2] = g[3 * la + 2] + (double)gl2;
+    }
+    if (own) {
+      e_el = e_el + (double)el;
+      e_vdw = e_vdw + (double)vdw;
     }
   }
   // water-solute: every non-Q atom of each listed solute group
-  double e_pwel = 0.0, e_pwvdw = 0.0;
   const int nwp = S.wp_cnt[iw];
   const int* wrow = S.wp_nb + (size_t)iw * S.wp_cap;
   for (int k = lane; k < nwp; k += 64) {
@@ -189,8 +213,8 @@ __global__ __launch_bounds__(256) void k_water_forces(const DevSys* __restrict__
         const P aLJ = (P)(S.avdw[ti * 3 + lj] * S.avdw[tj * 3 + lj]);
         const P bLJ = (P)(S.bvdw[ti * 3 + lj] * S.bvdw[tj * 3 + lj]);
         P r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
-        r2 = (P)1 / r2;
-        P r = psqrt(r2);
+        P r;
+        r2 = inv_r2(r2, r);
         P r6 = r2 * r2 * r2;
         P r12 = r6 * r6;
         P Vel = (P)S.crg[i] * (P)S.crg[j] * r;
@@ -203,87 +227,91 @@ __global__ __launch_bounds__(256) void k_water_forces(const DevSys* __restrict__
       }
     }
   }
+  // Q-water pairs seen from the water, one Q atom per lane (the Q atom is i
+  // of each pair): O with LJ + Coulomb, H Coulomb only
+  if (S.flags[io] & F_QW) {
+    const int ns = S.nstates;
+    for (int iq = lane; iq < S.nqat; iq += 64) {
+      const int qi = S.iqseq[iq];
+      const double* xq = S.x + 3 * qi;
+      const int lo = S.qw_lj[iq];
+      const int ti = S.iac[qi];
+      P dx1 = (P)(xi[0] - xq[0]), dx2 = (P)(xi[1] - xq[1]), dx3 = (P)(xi[2] - xq[2]);
+      P r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
+      P r6_hc = r2 * r2 * r2;
+      P r;
+      r2 = inv_r2(r2, r);
+      P r6 = (P)1 / r6_hc;
+      P Va = (P)S.avdw[ti * 3 + lo - 1] * (P)S.alj_O[lo - 1] * r6 * r6;
+      P Vb = (P)S.bvdw[ti * 3 + lo - 1] * (P)S.blj_O[lo - 1] * r6;
+      P dv = 0;
+      for (int st = 0; st < ns; ++st) {
+        P Vel = q[0] * (P)S.qcrg[iq * ns + st] * r;
+        dv = dv + r2 * (-Vel - ((P)12 * Va - (P)6 * Vb) * (r6_hc / r6_hc)) * (P)S.lambda[st];
+      }
+      g[0] = g[0] + (double)(dv * dx1);
+      g[1] = g[1] + (double)(dv * dx2);
+      g[2] = g[2] + (double)(dv * dx3);
+      for (int ip = 1; ip < 3; ++ip) {
+        P ex1 = (P)(xi[3 * ip] - xq[0]), ex2 = (P)(xi[3 * ip + 1] - xq[1]), ex3 = (P)(xi[3 * ip + 2] - xq[2]);
+        P rr;
+        P q2 = inv_r2(ex1 * ex1 + ex2 * ex2 + ex3 * ex3, rr);
+        P dvh = 0;
+        for (int st = 0; st < ns; ++st) {
+          P Vel = q[ip] * (P)S.qcrg[iq * ns + st] * rr;
+          dvh = dvh - q2 * Vel * (P)S.lambda[st];
+        }
+        g[3 * ip] = g[3 * ip] + (double)(dvh * ex1);
+        g[3 * ip + 1] = g[3 * ip + 1] + (double)(dvh * ex2);
+        g[3 * ip + 2] = g[3 * ip + 2] + (double)(dvh * ex3);
+      }
+    }
+  }
+  // LRF Taylor expansion, one water atom per lane 0..2
+  if (S.use_lrf && lane < 3) {
+    const int i = io + lane;
+    if ((S.flags[i] & (F_EXCL | F_Q)) == 0) {
+      const double* L = S.lrf + (size_t)S.iwhich_cgp[io] * kLrfStride;
+      double df[3];
+      const double xa[3] = {xi[3 * lane], xi[3 * lane + 1], xi[3 * lane + 2]};
+      double V = lrf_taylor_atom(L, xa, df);
+      e_lrf = 0.5 * S.crg[i] * V;
+      for (int c = 0; c < 3; ++c) {
+        const double f = S.crg[i] * df[c];
+        if (lane == 0) g[c] = g[c] - f;
+        else if (lane == 1) g[3 + c] = g[3 + c] - f;
+        else g[6 + c] = g[6 + c] - f;
+      }
+    }
+  }
+  // radial solvent restraint on the water O (switching atom), lane 3
+  if (lane == 3 && !(S.flags[io] & F_EXCL)) {
+    double shift = S.fk_wsphere != 0.0 ? sqrt(qsys::kBoltz * S.scal[2] / S.fk_wsphere) : 0.0;
+    double dr0 = xi[0] - S.xwcent[0], dr1 = xi[1] - S.xwcent[1], dr2 = xi[2] - S.xwcent[2];
+    double b = sqrt(dr0 * dr0 + dr1 * dr1 + dr2 * dr2);
+    double db = b - (S.rwat - shift);
+    double erst, dv;
+    if (db > 0) {
+      erst = 0.5 * S.fk_wsphere * db * db - S.dwmz;
+      dv = S.fk_wsphere * db / b;
+    } else if (b > 0.0) {
+      double fexp = exp(S.awmz * db);
+      erst = S.dwmz * (fexp * fexp - 2.0 * fexp);
+      dv = -2.0 * S.dwmz * S.awmz * (fexp - fexp * fexp) / b;
+    } else {
+      erst = 0.0; dv = 0.0;
+    }
+    e_rad = erst;
+    g[0] = g[0] + dv * dr0;
+    g[1] = g[1] + dv * dr1;
+    g[2] = g[2] + dv * dr2;
+  }
   for (int k = 0; k < 9; ++k) g[k] = wave_sum(g[k]);
   e_el = wave_sum(e_el);
   e_vdw = wave_sum(e_vdw);
+  e_lrf = wave_sum(e_lrf);
+  e_rad = wave_sum(e_rad);
   if (lane == 0) {
-    double e_lrf = 0.0, e_rad = 0.0;
-    // LRF Taylor expansion for the three water atoms
-    if (S.use_lrf) {
-      const double* L = S.lrf + (size_t)S.iwhich_cgp[io] * kLrfStride;
-      for (int a = 0; a < 3; ++a) {
-        const int i = io + a;
-        if ((S.flags[i] & (F_EXCL | F_Q)) != 0) continue;
-        double df[3];
-        double V = lrf_taylor_atom(L, xi + 3 * a, df);
-        e_lrf = e_lrf + 0.5 * S.crg[i] * V;
-        g[3 * a] = g[3 * a] - S.crg[i] * df[0];
-        g[3 * a + 1] = g[3 * a + 1] - S.crg[i] * df[1];
-        g[3 * a + 2] = g[3 * a + 2] - S.crg[i] * df[2];
-      }
-    }
-    // radial solvent restraint on the water O (switching atom)
-    if (!(S.flags[io] & F_EXCL)) {
-      double shift = S.fk_wsphere != 0.0 ? sqrt(qsys::kBoltz * S.scal[2] / S.fk_wsphere) : 0.0;
-      double dr0 = xi[0] - S.xwcent[0], dr1 = xi[1] - S.xwcent[1], dr2 = xi[2] - S.xwcent[2];
-      double b = sqrt(dr0 * dr0 + dr1 * dr1 + dr2 * dr2);
-      double db = b - (S.rwat - shift);
-      double erst, dv;
-      if (db > 0) {
-        erst = 0.5 * S.fk_wsphere * db * db - S.dwmz;
-        dv = S.fk_wsphere * db / b;
-      } else if (b > 0.0) {
-        double fexp = exp(S.awmz * db);
-        erst = S.dwmz * (fexp * fexp - 2.0 * fexp);
-        dv = -2.0 * S.dwmz * S.awmz * (fexp - fexp * fexp) / b;
-      } else {
-        erst = 0.0; dv = 0.0;
-      }
-      e_rad = erst;
-      g[0] = g[0] + dv * dr0;
-      g[1] = g[1] + dv * dr1;
-      g[2] = g[2] + dv * dr2;
-    }
-    // Q-water pairs seen from the water (nonbond_qw_spc, src/md.f90:14411-
-    // 14570, the Q atom is i of each pair): O with LJ + Coulomb, H Coulomb
-    if (S.flags[io] & F_QW) {
-      const int ns = S.nstates;
-      for (int iq = 0; iq < S.nqat; ++iq) {
-        const int qi = S.iqseq[iq];
-        const double* xq = S.x + 3 * qi;
-        const int lo = S.qw_lj[iq];
-        const int ti = S.iac[qi];
-        P dx1 = (P)(xi[0] - xq[0]), dx2 = (P)(xi[1] - xq[1]), dx3 = (P)(xi[2] - xq[2]);
-        P r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
-        P r6_hc = r2 * r2 * r2;
-        r2 = (P)1 / r2;
-        P r = psqrt(r2);
-        P r6 = (P)1 / r6_hc;
-        P Va = (P)S.avdw[ti * 3 + lo - 1] * (P)S.alj_O[lo - 1] * r6 * r6;
-        P Vb = (P)S.bvdw[ti * 3 + lo - 1] * (P)S.blj_O[lo - 1] * r6;
-        P dv = 0;
-        for (int st = 0; st < ns; ++st) {
-          P Vel = q[0] * (P)S.qcrg[iq * ns + st] * r;
-          dv = dv + r2 * (-Vel - ((P)12 * Va - (P)6 * Vb) * (r6_hc / r6_hc)) * (P)S.lambda[st];
-        }
-        g[0] = g[0] + (double)(dv * dx1);
-        g[1] = g[1] + (double)(dv * dx2);
-        g[2] = g[2] + (double)(dv * dx3);
-        for (int ip = 1; ip < 3; ++ip) {
-          P ex1 = (P)(xi[3 * ip] - xq[0]), ex2 = (P)(xi[3 * ip + 1] - xq[1]), ex3 = (P)(xi[3 * ip + 2] - xq[2]);
-          P q2 = (P)1 / (ex1 * ex1 + ex2 * ex2 + ex3 * ex3);
-          P rr = psqrt(q2);
-          P dvh = 0;
-          for (int st = 0; st < ns; ++st) {
-            P Vel = q[ip] * (P)S.qcrg[iq * ns + st] * rr;
-            dvh = dvh - q2 * Vel * (P)S.lambda[st];
-          }
-          g[3 * ip] = g[3 * ip] + (double)(dvh * ex1);
-          g[3 * ip + 1] = g[3 * ip + 1] + (double)(dvh * ex2);
-          g[3 * ip + 2] = g[3 * ip + 2] + (double)(dvh * ex3);
-        }
-      }
-    }
     for (int k = 0; k < 9; ++k) S.d[3 * io + k] = g[k];
     double* ew = S.e_water + (size_t)iw * 4;
     ew[0] = e_el; ew[1] = e_vdw; ew[2] = e_lrf; ew[3] = e_rad;
@@ -426,7 +454,62 @@ __global__ __launch_bounds__(256) void k_integrate(const DevSys* __restrict__ sy
   const DevSys& S = sys[blockIdx.y];
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
   double ke[6] = {0, 0, 0, 0, 0, 0};  // solute tot, solute free, solvent tot, solvent free, spare
-  if (u < S.nunits) {
+  if (u < S.nunits && S.unit_kind[u] == 1) {
+    // water: the same arithmetic as the generic path below, with the 9
+    // coordinates, the 9 pre-step coordinates and the velocities in registers
+    const double dt = S.dt;
+    const int a = S.unit_atoms[S.unit_first[u]];
+    const double ts = a < S.nat_solute ? S.scal[0] : S.scal[1];
+    double x[9], xo[9], v[9], w[3];
+    for (int k = 0; k < 3; ++k) w[k] = S.winv[a + k];
+    for (int c = 0; c < 9; ++c) {
+      const int ic = 3 * a + c;
+      double vv = (S.v[ic] - S.d[ic] * w[c / 3] * dt) * ts;
+      v[c] = vv;
+      xo[c] = S.x[ic];
+      x[c] = xo[c] + vv * dt;
+    }
+    const int c0 = S.con_first[u];
+    const double d2[3] = {S.sh_d2[c0], S.sh_d2[c0 + 1], S.sh_d2[c0 + 2]};
+    const int ci[3] = {0, 0, 1}, cj[3] = {1, 2, 2};
+    unsigned ready = 0u;
+    int nits = 0;
+    for (;;) {
+      for (int ic = 0; ic < 3; ++ic) {
+        if (ready & (1u << ic)) continue;
+        const int i = ci[ic], j = cj[ic];
+        double a0 = x[3 * i] - x[3 * j], a1 = x[3 * i + 1] - x[3 * j + 1], a2 = x[3 * i + 2] - x[3 * j + 2];
+        double xij2 = a0 * a0 + a1 * a1 + a2 * a2;
+        double diff = d2[ic] - xij2;
+        if (fabs(diff) < qsys::kShakeTol * d2[ic]) ready |= 1u << ic;
+        double b0 = xo[3 * i] - xo[3 * j], b1 = xo[3 * i + 1] - xo[3 * j + 1], b2 = xo[3 * i + 2] - xo[3 * j + 2];
+        double scp = a0 * b0 + a1 * b1 + a2 * b2;
+        double corr = diff / (2.0 * scp * (w[i] + w[j]));
+        x[3 * i] = x[3 * i] + b0 * corr * w[i];
+        x[3 * i + 1] = x[3 * i + 1] + b1 * corr * w[i];
+        x[3 * i + 2] = x[3 * i + 2] + b2 * corr * w[i];
+        x[3 * j] = x[3 * j] - b0 * corr * w[j];
+        x[3 * j + 1] = x[3 * j + 1] - b1 * corr * w[j];
+        x[3 * j + 2] = x[3 * j + 2] - b2 * corr * w[j];
+      }
+      ++nits;
+      if (ready == 7u) break;
+      if (nits >= qsys::kShakeMaxIter) { atomicOr(S.shake_fail, 1); break; }
+    }
+    for (int c = 0; c < 9; ++c) {
+      const int ic = 3 * a + c;
+      if (S.has_shake) v[c] = (x[c] - xo[c]) / dt;
+      S.x[ic] = x[c];
+      S.v[ic] = v[c];
+    }
+    for (int k = 0; k < 3; ++k) {
+      const int i = a + k;
+      const double ek = 0.5 * S.mass[i] * (v[3 * k] * v[3 * k] + v[3 * k + 1] * v[3 * k + 1] + v[3 * k + 2] * v[3 * k + 2]);
+      const bool ex = (S.flags[i] & F_EXCL) != 0;
+      if (i < S.nat_solute) { ke[0] += ek; if (!ex) ke[1] += ek; }
+      else { ke[2] += ek; if (!ex) ke[3] += ek; }
+    }
+  } else if (u < S.nunits) {
     const double dt = S.dt;
     const double ts_solute = S.scal[0], ts_solvent = S.scal[1];
     const int a0 = S.unit_first[u], a1 = S.unit_first[u + 1];

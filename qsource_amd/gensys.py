@@ -234,6 +234,46 @@ def _branched_solute(n_atoms: int, rmax: float, rng, bond=1.53, min_dist=3.2):
     return c, bonds
 
 
+def _renumber_and_group(coords, bonds, size=4):
+    """Renumbers the solute atoms in depth-first order of the bond tree (so
+    bonded atoms get close indices) and partitions them, in that order, into
+    connected charge groups of at most `size` atoms: a group is closed when
+    it is full or when the next atom is not bonded to any of its members.
+    Compact groups keep every atom near its group centre, which the LRF
+    Taylor expansion about the group centre requires."""
+    n = len(coords)
+    nbr = [[] for _ in range(n)]
+    for i, j in bonds:
+        nbr[i].append(j)
+        nbr[j].append(i)
+    order, seen, stack = [], [False] * n, [0]
+    while stack:
+        a = stack.pop()
+        if seen[a]:
+            continue
+        seen[a] = True
+        order.append(a)
+        for b in sorted(nbr[a], reverse=True):
+            if not seen[b]:
+                stack.append(b)
+    new = {old: k for k, old in enumerate(order)}
+    c2 = coords[order]
+    b2 = sorted((min(new[i], new[j]), max(new[i], new[j])) for i, j in bonds)
+    nb2 = [set() for _ in range(n)]
+    for i, j in b2:
+        nb2[i].add(j)
+        nb2[j].add(i)
+    groups, cur = [], []
+    for a in range(n):
+        if cur and (len(cur) >= size or not any(a in nb2[m] for m in cur)):
+            groups.append(cur)
+            cur = []
+        cur.append(a)
+    if cur:
+        groups.append(cur)
+    return c2, b2, groups
+
+
 def _graph_terms(n, bonds):
     nbr = [[] for _ in range(n)]
     for i, j in bonds:
@@ -301,10 +341,11 @@ def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
                 if attempt == 19:
                     raise
                 rmax *= 1.05
+        sx, sb, groups0 = _renumber_and_group(sx, sb)
         angles0, tors0, dist = _graph_terms(n_solute, sb)
     else:
         sx = np.zeros((0, 3))
-        sb, angles0, tors0, dist = [], [], [], {}
+        sb, angles0, tors0, dist, groups0 = [], [], [], {}, []
     ns = n_solute
     wat = water_lattice(radius, rng, avoid=sx if ns else None)
     nw = wat.shape[0]
@@ -342,14 +383,12 @@ def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
             tac.append("S%d" % len(tac))
         iac[:ns] = np.array([3 + tmap[k] for k in key.tolist()])
         q = rng.normal(0.0, q_sd, ns)
-        for g0 in range(0, ns, 4):
-            g1 = min(ns, g0 + 4)
-            q[g0:g1] -= q[g0:g1].mean()
+        for grp in groups0:
+            q[grp] -= q[grp].mean()
         crg[:ns] = np.round(q, 4)
         # keep groups exactly neutral after rounding
-        for g0 in range(0, ns, 4):
-            g1 = min(ns, g0 + 4)
-            crg[g1 - 1] -= round(float(crg[g0:g1].sum()), 4)
+        for grp in groups0:
+            crg[grp[-1]] -= round(float(crg[grp].sum()), 4)
     iac[ns::3] = 1
     iac[ns + 1::3] = 2
     iac[ns + 2::3] = 2
@@ -374,8 +413,9 @@ def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
         bonds += [(o, o + 1, 2), (o, o + 2, 2), (o + 1, o + 2, 3)]
         angles += [(o + 1, o, o + 2, 2)]
 
-    # --- charge groups: solute blocks of 4 (first atom switches), waters
-    cgps = [list(range(g + 1, min(ns, g + 4) + 1)) for g in range(0, ns, 4)]
+    # --- charge groups: connected solute groups of <= 4 atoms (first atom
+    # switches), waters
+    cgps = [[a + 1 for a in grp] for grp in groups0]
     ncgp_solute = len(cgps)
     cgps += [[ns + 3 * w + 1, ns + 3 * w + 2, ns + 3 * w + 3] for w in range(nw)]
 
