@@ -28,7 +28,8 @@ __global__ void k_temperature(const DevSys*, int);
 __global__ void k_energy_reduce(const DevSys*, int);
 __global__ void k_solute_lists(const DevSys*);
 template <typename P> __global__ void k_solute_forces(const DevSys*);
-template <typename P> __global__ void k_q_forces(const DevSys*);
+template <typename P, int NS> __global__ void k_q_forces(const DevSys*);
+__global__ void k_q_reduce(const DevSys*);
 __global__ void k_qq_forces(const DevSys*);
 __global__ void k_bonded(const DevSys*);
 
@@ -111,6 +112,8 @@ Batch::Batch(const std::vector<std::string>& inputs, int precision, int device) 
     max_ncgp_ = std::max(max_ncgp_, (int)s->top.cgp.size());
     max_ncgp_solute_ = std::max(max_ncgp_solute_, s->top.ncgp_solute);
     max_natom_ = std::max(max_natom_, s->natom);
+    max_nat_solute_ = std::max(max_nat_solute_, s->nat_solute);
+    max_nstates_ = std::max(max_nstates_, s->nstates);
   }
   nbcycle_ = setups_[0]->inp.nbcycle;
   for (auto* s : setups_)
@@ -234,11 +237,29 @@ void Batch::upload(int k) {
   // exclusion and 1-4 tables (solute rows) and long lists
   D.listex = I->upload(std::vector<unsigned>(t.listex.begin(), t.listex.end()));
   D.list14 = I->upload(std::vector<unsigned>(t.list14.begin(), t.list14.end()));
-  std::vector<int> exl, l14l;
-  for (auto& p : t.listexlong) { exl.push_back(p.first); exl.push_back(p.second); }
-  for (auto& p : t.list14long) { l14l.push_back(p.first); l14l.push_back(p.second); }
-  D.exlong = I->upload(exl); D.nexlong = (int)t.listexlong.size();
-  D.l14long = I->upload(l14l); D.n14long = (int)t.list14long.size();
+  {
+    // long exclusion / 1-4 pairs as a CSR over the lower atom
+    std::vector<std::vector<int>> rows(std::max(1, s.nat_solute));
+    auto add = [&](int i, int j, int bit) {
+      int lo = std::min(i, j), hi = std::max(i, j);
+      for (int& c : rows[lo])
+        if ((c >> 2) == hi) { c |= bit; return; }
+      rows[lo].push_back(hi << 2 | bit);
+    };
+    for (auto& p : t.listexlong) add(p.first, p.second, 1);
+    for (auto& p : t.list14long) add(p.first, p.second, 2);
+    std::vector<int> first(1, 0), code;
+    for (auto& r : rows) {
+      code.insert(code.end(), r.begin(), r.end());
+      first.push_back((int)code.size());
+    }
+    D.exl_first = I->upload(first);
+    D.exl_code = I->upload(code);
+    std::vector<int> slot(std::max(1, s.nat_solute), 0);
+    for (int g = 0; g < t.ncgp_solute; ++g)
+      for (int a = t.cgp[g].first; a <= t.cgp[g].last; ++a) slot[t.cgpatom[a]] = a - t.cgp[g].first;
+    D.cgp_slot = I->upload(slot);
+  }
   // lists
   D.ww_cap = std::max(1, std::min(s.nwat, 448));
   D.wp_cap = std::max(1, std::min(t.ncgp_solute, 1024));
@@ -250,6 +271,7 @@ void Batch::upload(int k) {
   D.wp_nb = I->alloc<int>((size_t)s.nwat * D.wp_cap);
   D.pp_cnt = I->alloc<int>(t.ncgp_solute);
   D.pp_nb = I->alloc<int>((size_t)t.ncgp_solute * D.pp_cap);
+  D.pp_mask = I->alloc<ulonglong2>((size_t)t.ncgp_solute * D.pp_cap);
   D.pw_cnt = I->alloc<int>(t.ncgp_solute);
   D.pw_nb = I->alloc<int>((size_t)t.ncgp_solute * D.pw_cap);
   D.overflow = I->alloc<int>(1);
@@ -329,6 +351,7 @@ void Batch::upload(int k) {
     D.qp_atoms = I->upload(qp_atoms);
     D.qw_waters = I->upload(qw_waters);
     D.e_q = I->alloc<double>((size_t)std::max(1, nq * s.nstates) * 4);
+    D.q_part = I->alloc<double>((size_t)std::max(1, nq) * kQChunks * kQPartStride);
     // Q-Q / Q-neighbour pairs, flattened over states, with atom incidence
     std::vector<int> qst, qiq, qj, qjq, qlj;
     std::vector<double> qel;
@@ -445,7 +468,7 @@ void Batch::upload(int k) {
   D.shake_fail = I->alloc<int>(1);
   // energies and reductions
   D.e_water = I->alloc<double>((size_t)std::max(1, s.nwat) * 4);
-  D.e_solute = I->alloc<double>((size_t)std::max(1, t.ncgp_solute) * 8);
+  D.e_solute = I->alloc<double>((size_t)std::max(1, s.nat_solute) * 8);
   D.e_wpol = I->alloc<double>(std::max(1, D.nshell));
   I->ke_blocks = 0;
   D.ke_part = nullptr;  // sized after all instances are known (see below)
@@ -489,7 +512,7 @@ void Batch::launch_forces() {
       { pb(3); hipLaunchKernelGGL(k_water_forces<float>, g1, dim3(256), 0, stream_, d_sys_); pe(3); }
   }
   if (max_ncgp_solute_ > 0) {
-    dim3 gs((max_ncgp_solute_ * 64 + 255) / 256, ni);
+    dim3 gs((max_nat_solute_ * 64 + 255) / 256, ni);
     if (precision_ == 0)
       { pb(4); hipLaunchKernelGGL(k_solute_forces<double>, gs, dim3(256), 0, stream_, d_sys_); pe(4); }
     else
@@ -498,11 +521,19 @@ void Batch::launch_forces() {
   if (max_nbonded_atoms_ > 0)
     { pb(5); hipLaunchKernelGGL(k_bonded, dim3((max_nbonded_atoms_ + 255) / 256, ni), dim3(256), 0, stream_, d_sys_); pe(5); }
   if (max_nqat_ > 0) {
-    dim3 gq((max_nqat_ * 64 + 255) / 256, ni);
-    if (precision_ == 0)
-      { pb(6); hipLaunchKernelGGL(k_q_forces<double>, gq, dim3(256), 0, stream_, d_sys_); pe(6); }
-    else
-      { pb(6); hipLaunchKernelGGL(k_q_forces<float>, gq, dim3(256), 0, stream_, d_sys_); pe(6); }
+    dim3 gq((max_nqat_ * kQChunks * 64 + 255) / 256, ni);
+    pb(6);
+    if (precision_ == 0) {
+      if (max_nstates_ <= 2) hipLaunchKernelGGL((k_q_forces<double, 2>), gq, dim3(256), 0, stream_, d_sys_);
+      else if (max_nstates_ <= 4) hipLaunchKernelGGL((k_q_forces<double, 4>), gq, dim3(256), 0, stream_, d_sys_);
+      else hipLaunchKernelGGL((k_q_forces<double, kMaxQStates>), gq, dim3(256), 0, stream_, d_sys_);
+    } else {
+      if (max_nstates_ <= 2) hipLaunchKernelGGL((k_q_forces<float, 2>), gq, dim3(256), 0, stream_, d_sys_);
+      else if (max_nstates_ <= 4) hipLaunchKernelGGL((k_q_forces<float, 4>), gq, dim3(256), 0, stream_, d_sys_);
+      else hipLaunchKernelGGL((k_q_forces<float, kMaxQStates>), gq, dim3(256), 0, stream_, d_sys_);
+    }
+    hipLaunchKernelGGL(k_q_reduce, dim3((max_nqat_ + 63) / 64, ni), dim3(64), 0, stream_, d_sys_);
+    pe(6);
   }
   if (max_nqq_atoms_ > 0)
     { pb(7); hipLaunchKernelGGL(k_qq_forces, dim3((max_nqq_atoms_ + 255) / 256, ni), dim3(256), 0, stream_, d_sys_); pe(7); }

@@ -34,6 +34,10 @@ enum : unsigned char { F_EXCL = 1, F_SHELL = 2, F_Q = 4, F_HEAVY = 8, F_QP = 16,
 
 constexpr int kMaxGroupAtoms = 8;   // atoms per solute charge group handled per wavefront
 constexpr int kMaxQStates = 10;
+// wavefronts per Q atom in the Q-atom kernel, and the per-wavefront partial
+// record: gradient (3) and per-state qp_el qp_vdw qw_el qw_vdw
+constexpr int kQChunks = 8;
+constexpr int kQPartStride = 3 + 4 * kMaxQStates;
 
 constexpr int kLrfStride = 44;
 constexpr int kNumShellMax = 8;
@@ -69,14 +73,20 @@ struct DevSys {
   int natyps;
   const double *avdw, *bvdw;
   const int* ljcod;
-  // solute exclusion / 1-4 bit rows and long lists
+  // solute exclusion / 1-4 bit rows (pairs at most kMaxNbrRange apart) and,
+  // for pairs further apart, a per-atom CSR over the lower atom of each pair:
+  // exl_code[exl_first[lo] .. exl_first[lo+1]) = hi << 2 | 1 (excluded) | 2 (1-4)
   const unsigned* listex; const unsigned* list14;
-  const int* exlong; int nexlong;      // pairs (i,j)
-  const int* l14long; int n14long;
+  const int *exl_first, *exl_code;
   double el14_scale;
+  const int* cgp_slot;           // solute atom -> its position within its charge group
   // lists
   int ww_cap, wp_cap, pp_cap, pw_cap;
   int *ww_cnt, *ww_nb, *wp_cnt, *wp_nb, *pp_cnt, *pp_nb, *pw_cnt, *pw_nb;
+  // per pp entry (group g, neighbour group jg): bit a*8+b of .x set when the
+  // atom pair (slot a of g, slot b of jg) is not evaluated (same atom, Q atom,
+  // zero interaction or excluded), of .y when it is a 1-4 pair
+  ulonglong2* pp_mask;
   int* overflow;                 // set to 1 when a list row exceeds its cap
   // cut-offs squared
   double rcpp2, rcpw2, rcww2, rclrf2;
@@ -105,6 +115,7 @@ struct DevSys {
   const int* qp_atoms;           // solute atoms in the Q-solute list, ascending
   const int* qw_waters;          // waters in the Q-water list, ascending
   double* e_q;                   // nqat x nstates x 4: qp_el qp_vdw qw_el qw_vdw
+  double* q_part;                // nqat x kQChunks x kQPartStride partial sums
   // Q-Q and Q-neighbour pairs of all states (nbqq), and per-atom incidence
   int nqq;                       // total pairs over states
   const int *qq_state, *qq_iq, *qq_j, *qq_jq, *qq_lj;
@@ -141,7 +152,7 @@ struct DevSys {
   int* shake_fail;
   // energies: eacc[term] partial sums written per item, reduced on demand
   double* e_water;     // nwat x 4 : ww_el, ww_vdw, lrf(water atoms), radial
-  double* e_solute;    // ncgp_solute x 8 : pp_el pp_vdw pw_el pw_vdw lrf fix shell spare
+  double* e_solute;    // nat_solute x 8 : pp_el pp_vdw pw_el pw_vdw lrf fix shell spare
   double* e_wpol;      // nshell
   double* ke_part;     // kinetic partial sums: nblocks x 6
   int ke_blocks;
@@ -228,6 +239,7 @@ class Batch {
   int istep_ = 0;
   int precision_ = 0;
   int max_nwat_ = 0, max_ncgp_ = 0, max_ncgp_solute_ = 0, max_units_ = 0, max_natom_ = 0;
+  int max_nat_solute_ = 0, max_nstates_ = 0;
   int max_nqat_ = 0, max_nqq_atoms_ = 0, max_nbonded_atoms_ = 0;
 };
 

@@ -15,6 +15,20 @@ constexpr double kPiD = 3.14159265358979323846;
 __device__ __forceinline__ double psqrt(double v) { return sqrt(v); }
 __device__ __forceinline__ float psqrt(float v) { return sqrtf(v); }
 
+// Inverse distance powers of one atom pair from the squared distance r2sq:
+// returns 1/r^2 and sets rinv = 1/r.  FP64 keeps the reference's
+// operations (one/r2, then sqrt, src/md.f90:15074-15080); FP32 uses the
+// hardware reciprocal square root (v_rsq_f32, 1 ulp).
+__device__ __forceinline__ double inv_r2(double r2sq, double& rinv) {
+  double r2 = 1.0 / r2sq;
+  rinv = sqrt(r2);
+  return r2;
+}
+__device__ __forceinline__ float inv_r2(float r2sq, float& rinv) {
+  rinv = __builtin_amdgcn_rsqf(r2sq);
+  return rinv * rinv;
+}
+
 // 64-lane butterfly sum; every lane returns the total
 __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

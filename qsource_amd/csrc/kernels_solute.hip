@@ -9,29 +9,48 @@
 
 namespace qgpu {
 
-__device__ __forceinline__ bool bit_excluded(const DevSys& S, int i, int j) {
-  int lo = i < j ? i : j, hi = i < j ? j : i;
-  if (hi - lo <= qsys::kMaxNbrRange) return (S.listex[lo] >> (hi - lo - 1)) & 1u;
-  for (int n = 0; n < S.nexlong; ++n) {
-    int a = S.exlong[2 * n], b = S.exlong[2 * n + 1];
-    if ((a == i && b == j) || (a == j && b == i)) return true;
+// Exclusion (bit 0) and 1-4 (bit 1) status of a solute atom pair: the bit
+// rows for pairs at most kMaxNbrRange apart, else the lower atom's long-pair
+// row (listex/list14 and listexlong/list14long, src/md.f90:7745-7760).
+__device__ __forceinline__ int pair_status(const DevSys& S, int i, int j) {
+  const int lo = i < j ? i : j, hi = i < j ? j : i;
+  if (hi - lo <= qsys::kMaxNbrRange)
+    return (int)((S.listex[lo] >> (hi - lo - 1)) & 1u) | (int)(((S.list14[lo] >> (hi - lo - 1)) & 1u) << 1);
+  for (int n = S.exl_first[lo]; n < S.exl_first[lo + 1]; ++n) {
+    const int c = S.exl_code[n];
+    if ((c >> 2) == hi) return c & 3;
   }
-  return false;
-}
-__device__ __forceinline__ bool bit_14(const DevSys& S, int i, int j) {
-  int lo = i < j ? i : j, hi = i < j ? j : i;
-  if (hi - lo <= qsys::kMaxNbrRange) return (S.list14[lo] >> (hi - lo - 1)) & 1u;
-  bool r = false;
-  for (int n = 0; n < S.n14long; ++n) {
-    int a = S.l14long[2 * n], b = S.l14long[2 * n + 1];
-    if ((a == i && b == j) || (a == j && b == i)) r = true;
-  }
-  return r;
+  return 0;
 }
 __device__ __forceinline__ bool zero_interaction(const DevSys& S, int i, int j, int lj) {
   const int ti = S.iac[i], tj = S.iac[j];
   return S.crg[i] * S.crg[j] == 0.0 && S.avdw[ti * 3 + lj] * S.avdw[tj * 3 + lj] == 0.0 &&
          S.bvdw[ti * 3 + lj] * S.bvdw[tj * 3 + lj] == 0.0;
+}
+
+// Skip and 1-4 masks of the atom pairs of solute groups (g, jg), with the
+// skip rules of nbpplist (src/md.f90:8080-8120): the same atom, a Q atom on
+// either side, a zero interaction, an excluded pair.
+__device__ ulonglong2 pair_masks(const DevSys& S, int g, int jg) {
+  unsigned long long skip = 0, m14 = 0;
+  const int fi = S.cgp_first[g], ni = S.cgp_last[g] - fi + 1;
+  const int fj = S.cgp_first[jg], nj = S.cgp_last[jg] - fj + 1;
+  for (int a = 0; a < ni; ++a) {
+    const int i = S.cgpatom[fi + a];
+    const bool qi = (S.flags[i] & F_Q) != 0;
+    for (int b = 0; b < nj; ++b) {
+      const int j = S.cgpatom[fj + b];
+      const unsigned long long bit = 1ull << (a * kMaxGroupAtoms + b);
+      if (qi || j == i || (S.flags[j] & F_Q)) { skip |= bit; continue; }
+      const int lj = S.ljcod[S.iac[i] * S.natyps + S.iac[j]];
+      if (zero_interaction(S, i, j, lj - 1)) { skip |= bit; continue; }
+      const int st = pair_status(S, i, j);
+      if (st & 1) { skip |= bit; continue; }
+      if (st & 2) m14 |= bit;
+    }
+  }
+  for (int a = ni; a < kMaxGroupAtoms; ++a) skip |= 0xffull << (a * kMaxGroupAtoms);
+  return make_ulonglong2(skip, m14);
 }
 
 // ---------------------------------------------------------------------------
@@ -69,7 +88,10 @@ __global__ __launch_bounds__(256) void k_solute_lists(const DevSys* __restrict__
     unsigned long long m = __ballot(inside);
     if (inside) {
       int pos = n_pp + __popcll(m & ((1ull << lane) - 1ull));
-      if (pos < S.pp_cap) prow[pos] = jg;
+      if (pos < S.pp_cap) {
+        prow[pos] = jg;
+        S.pp_mask[(size_t)g * S.pp_cap + pos] = pair_masks(S, g, jg);
+      }
     }
     n_pp += __popcll(m);
     if (far && S.use_lrf) lrf_gather_group(S, jg, c0, c1, c2, acc);
@@ -124,65 +146,79 @@ __device__ __forceinline__ P pp_pair(const DevSys& S, int ia, int ja, int lj, P&
 }
 
 // ---------------------------------------------------------------------------
-// Solute forces: one wavefront per solute charge group (nonbond_pp
-// :13069-13155, nonbond_pw :13282-13365, lrf_taylor, fix_shell :2816-2862,
-// and the solute-atom side of nonbond_qp :13788-13921).
+// Solute forces: one wavefront per solute atom (nonbond_pp :13069-13155,
+// nonbond_pw :13282-13365, lrf_taylor, fix_shell :2816-2862, and the
+// solute-atom side of nonbond_qp :13788-13921).  Lanes take the neighbour
+// groups of the atom's charge group (pair status from the list-time masks),
+// the listed waters, and the Q atoms; lane 1 adds the LRF Taylor term and
+// lane 2 the shell/fix restraint; the atom's gradient is shuffle-reduced and
+// written once.  A pair's energy is counted by the atom that is i of the
+// reference pair: the owner group's atom, or the lower index in one group.
 template <typename P>
 __global__ __launch_bounds__(256) void k_solute_forces(const DevSys* __restrict__ sys) {
   const DevSys& S = sys[blockIdx.y];
   const int lane = threadIdx.x & 63;
-  const int g = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  if (g >= S.ncgp_solute) return;
-  const int f = S.cgp_first[g], na = S.cgp_last[g] - f + 1;
-  double gr[kMaxGroupAtoms][3];
-  for (int a = 0; a < kMaxGroupAtoms; ++a) gr[a][0] = gr[a][1] = gr[a][2] = 0.0;
-  double e_ppel = 0, e_ppvdw = 0, e_pwel = 0, e_pwvdw = 0;
-  // solute-solute
-  const int npp = S.pp_cnt[g];
-  const int* prow = S.pp_nb + (size_t)g * S.pp_cap;
-  for (int k = lane; k < npp; k += 64) {
-    const int jg = prow[k];
-    const bool same = jg == g;
-    const bool own = same || owns_pair(g, jg);
-    for (int a = 0; a < na; ++a) {
-      const int i = S.cgpatom[f + a];
-      if (S.flags[i] & F_Q) continue;
-      for (int b = S.cgp_first[jg]; b <= S.cgp_last[jg]; ++b) {
-        const int j = S.cgpatom[b];
-        if (j == i || (S.flags[j] & F_Q)) continue;
-        int lj = S.ljcod[S.iac[i] * S.natyps + S.iac[j]];
-        if (zero_interaction(S, i, j, lj - 1)) continue;
-        if (bit_excluded(S, i, j)) continue;
-        if (bit_14(S, i, j)) lj = 3;
-        // reference orientation: the owner group's atom first; within one
-        // group the lower atom index first
+  const int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (i >= S.nat_solute) return;
+  const int g = S.iwhich_cgp[i];
+  const int a = S.cgp_slot[i];
+  const unsigned char fl = S.flags[i];
+  const double xi0 = S.x[3 * i], xi1 = S.x[3 * i + 1], xi2 = S.x[3 * i + 2];
+  const int ti = S.iac[i];
+  const P qi = (P)S.crg[i];
+  double g0 = 0, g1 = 0, g2 = 0;
+  double e_ppel = 0, e_ppvdw = 0, e_pwel = 0, e_pwvdw = 0, e_lrf = 0, e_fix = 0, e_shell = 0;
+  if (!(fl & F_Q)) {
+    // solute-solute
+    const int npp = S.pp_cnt[g];
+    const int* prow = S.pp_nb + (size_t)g * S.pp_cap;
+    const ulonglong2* mrow = S.pp_mask + (size_t)g * S.pp_cap;
+    for (int k = lane; k < npp; k += 64) {
+      const int jg = prow[k];
+      const ulonglong2 mk = mrow[k];
+      unsigned todo = (unsigned)(~mk.x >> (a * kMaxGroupAtoms)) & 0xffu;
+      const unsigned is14 = (unsigned)(mk.y >> (a * kMaxGroupAtoms)) & 0xffu;
+      const bool same = jg == g;
+      const bool own = same || owns_pair(g, jg);
+      const int fj = S.cgp_first[jg];
+      while (todo) {
+        const int b = __builtin_ctz(todo);
+        todo &= todo - 1;
+        const int j = S.cgpatom[fj + b];
+        const int tj = S.iac[j];
+        const int lj = ((is14 >> b) & 1) ? 3 : S.ljcod[ti * S.natyps + tj];
+        const int l = lj - 1;
         const bool me_first = same ? (i < j) : own;
-        const int ia = me_first ? i : j, ja = me_first ? j : i;
-        P dx1, dx2, dx3, Vel, Va, Vb;
-        P dv = pp_pair<P>(S, ia, ja, lj, dx1, dx2, dx3, Vel, Va, Vb);
+        // dx = x_j - x_i; the reference pair (ia, ja) has dx of the same sign
+        // when i is first, and the opposite sign otherwise
+        P dx1 = (P)(S.x[3 * j] - xi0), dx2 = (P)(S.x[3 * j + 1] - xi1), dx3 = (P)(S.x[3 * j + 2] - xi2);
+        if (!me_first) { dx1 = -dx1; dx2 = -dx2; dx3 = -dx3; }
+        P r;
+        P r2 = inv_r2(dx1 * dx1 + dx2 * dx2 + dx3 * dx3, r);
+        P r6 = r2 * r2 * r2;
+        P Vel = qi * (P)S.crg[j] * r;
+        if (lj == 3) Vel = Vel * (P)S.el14_scale;
+        const P Va = r6 * r6 * (P)S.avdw[ti * 3 + l] * (P)S.avdw[tj * 3 + l];
+        const P Vb = r6 * (P)S.bvdw[ti * 3 + l] * (P)S.bvdw[tj * 3 + l];
+        const P dv = r2 * (-Vel - (P)12 * Va + (P)6 * Vb);
         if (me_first) {
-          gr[a][0] = gr[a][0] - (double)(dv * dx1);
-          gr[a][1] = gr[a][1] - (double)(dv * dx2);
-          gr[a][2] = gr[a][2] - (double)(dv * dx3);
+          g0 = g0 - (double)(dv * dx1);
+          g1 = g1 - (double)(dv * dx2);
+          g2 = g2 - (double)(dv * dx3);
           e_ppel = e_ppel + (double)Vel;
           e_ppvdw = e_ppvdw + (double)Va - (double)Vb;
         } else {
-          gr[a][0] = gr[a][0] + (double)(dv * dx1);
-          gr[a][1] = gr[a][1] + (double)(dv * dx2);
-          gr[a][2] = gr[a][2] + (double)(dv * dx3);
+          g0 = g0 + (double)(dv * dx1);
+          g1 = g1 + (double)(dv * dx2);
+          g2 = g2 + (double)(dv * dx3);
         }
       }
     }
-  }
-  // solute-water (solute atom is i of the reference pair)
-  const int npw = S.pw_cnt[g];
-  const int* wrow = S.pw_nb + (size_t)g * S.pw_cap;
-  for (int k = lane; k < npw; k += 64) {
-    const int jo = S.nat_solute + 3 * wrow[k];
-    for (int a = 0; a < na; ++a) {
-      const int i = S.cgpatom[f + a];
-      if (S.flags[i] & F_Q) continue;
-      const int ti = S.iac[i];
+    // solute-water (the solute atom is i of the reference pair)
+    const int npw = S.pw_cnt[g];
+    const int* wrow = S.pw_nb + (size_t)g * S.pw_cap;
+    for (int k = lane; k < npw; k += 64) {
+      const int jo = S.nat_solute + 3 * wrow[k];
       for (int ip = 0; ip < 3; ++ip) {
         const int j = jo + ip;
         const int tj = S.iac[j];
@@ -190,121 +226,129 @@ __global__ __launch_bounds__(256) void k_solute_forces(const DevSys* __restrict_
         if (zero_interaction(S, i, j, lj)) continue;
         const P aLJ = (P)(S.avdw[ti * 3 + lj] * S.avdw[tj * 3 + lj]);
         const P bLJ = (P)(S.bvdw[ti * 3 + lj] * S.bvdw[tj * 3 + lj]);
-        P dx1 = (P)(S.x[3 * j] - S.x[3 * i]), dx2 = (P)(S.x[3 * j + 1] - S.x[3 * i + 1]),
-          dx3 = (P)(S.x[3 * j + 2] - S.x[3 * i + 2]);
-        P r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
-        r2 = (P)1 / r2;
-        P r = psqrt(r2);
+        P dx1 = (P)(S.x[3 * j] - xi0), dx2 = (P)(S.x[3 * j + 1] - xi1), dx3 = (P)(S.x[3 * j + 2] - xi2);
+        P r;
+        P r2 = inv_r2(dx1 * dx1 + dx2 * dx2 + dx3 * dx3, r);
         P r6 = r2 * r2 * r2;
         P r12 = r6 * r6;
-        P Vel = (P)S.crg[i] * (P)S.crg[j] * r;
+        P Vel = qi * (P)S.crg[j] * r;
         P Va = aLJ * r12;
         P Vb = bLJ * r6;
         P dv = r2 * (-Vel - (P)12 * Va + (P)6 * Vb);
-        gr[a][0] = gr[a][0] - (double)(dv * dx1);
-        gr[a][1] = gr[a][1] - (double)(dv * dx2);
-        gr[a][2] = gr[a][2] - (double)(dv * dx3);
+        g0 = g0 - (double)(dv * dx1);
+        g1 = g1 - (double)(dv * dx2);
+        g2 = g2 - (double)(dv * dx3);
         e_pwel = e_pwel + (double)Vel;
         e_pwvdw = e_pwvdw + (double)Va - (double)Vb;
       }
     }
-  }
-  for (int a = 0; a < na; ++a)
-    for (int c = 0; c < 3; ++c) gr[a][c] = wave_sum(gr[a][c]);
-  e_ppel = wave_sum(e_ppel); e_ppvdw = wave_sum(e_ppvdw);
-  e_pwel = wave_sum(e_pwel); e_pwvdw = wave_sum(e_pwvdw);
-  if (lane != 0) return;
-  double e_lrf = 0, e_fix = 0, e_shell = 0;
-  const double* L = S.lrf + (size_t)g * kLrfStride;
-  for (int a = 0; a < na; ++a) {
-    const int i = S.cgpatom[f + a];
-    const unsigned char fl = S.flags[i];
-    const double* xi = S.x + 3 * i;
-    if (S.use_lrf && !(fl & (F_EXCL | F_Q))) {
-      double df[3];
-      double V = lrf_taylor_atom(L, xi, df);
-      e_lrf = e_lrf + 0.5 * S.crg[i] * V;
-      for (int c = 0; c < 3; ++c) gr[a][c] = gr[a][c] - S.crg[i] * df[c];
-    }
-    if (fl & (F_EXCL | F_SHELL)) {
-      const double fk = (fl & F_EXCL) ? S.fk_fix : S.fk_pshell;
-      double dr0 = xi[0] - S.xtop[3 * i], dr1 = xi[1] - S.xtop[3 * i + 1], dr2 = xi[2] - S.xtop[3 * i + 2];
-      double erst = 0.5 * fk * (dr0 * dr0 + dr1 * dr1 + dr2 * dr2);
-      if (fl & F_EXCL) e_fix = e_fix + erst;
-      if (fl & F_SHELL) e_shell = e_shell + erst;
-      gr[a][0] = gr[a][0] + fk * dr0;
-      gr[a][1] = gr[a][1] + fk * dr1;
-      gr[a][2] = gr[a][2] + fk * dr2;
-    }
-    // Q-solute pairs seen from the solute atom (Q atom is i of the pair)
-    if ((fl & F_QP) && !(fl & F_Q)) {
-      const int tj = S.iac[i];
-      for (int iq = 0; iq < S.nqat; ++iq) {
-        const int qi = S.iqseq[iq];
+    // Q-solute pairs seen from the solute atom (the Q atom is i of the pair)
+    if (fl & F_QP) {
+      for (int iq = lane; iq < S.nqat; iq += 64) {
+        const int qa = S.iqseq[iq];
         const int lj = S.qp_lj[(size_t)i * S.nqat + iq];
         const int l = lj - 1;
-        const int ti = S.iac[qi];
-        P dx1 = (P)(xi[0] - S.x[3 * qi]), dx2 = (P)(xi[1] - S.x[3 * qi + 1]), dx3 = (P)(xi[2] - S.x[3 * qi + 2]);
-        P r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
-        r2 = (P)1 / r2;
+        const int tq = S.iac[qa];
+        P dx1 = (P)(xi0 - S.x[3 * qa]), dx2 = (P)(xi1 - S.x[3 * qa + 1]), dx3 = (P)(xi2 - S.x[3 * qa + 2]);
+        P r;
+        P r2 = inv_r2(dx1 * dx1 + dx2 * dx2 + dx3 * dx3, r);
         P r6 = r2 * r2 * r2;
-        P r = psqrt(r2);
-        const P aLJ = (P)(S.avdw[ti * 3 + l] * S.avdw[tj * 3 + l]);
-        const P bLJ = (P)(S.bvdw[ti * 3 + l] * S.bvdw[tj * 3 + l]);
+        const P aLJ = (P)(S.avdw[tq * 3 + l] * S.avdw[ti * 3 + l]);
+        const P bLJ = (P)(S.bvdw[tq * 3 + l] * S.bvdw[ti * 3 + l]);
         for (int st = 0; st < S.nstates; ++st) {
-          P Vel = (P)S.qcrg[iq * S.nstates + st] * (P)S.crg[i] * r;
+          P Vel = (P)S.qcrg[iq * S.nstates + st] * qi * r;
           if (lj == 3) Vel = Vel * (P)S.el14_scale;
           P Va = aLJ * r6 * r6;
           P Vb = bLJ * r6;
           P dv = r2 * (-Vel - (P)12 * Va + (P)6 * Vb) * (P)S.lambda[st];
-          gr[a][0] = gr[a][0] + (double)(dv * dx1);
-          gr[a][1] = gr[a][1] + (double)(dv * dx2);
-          gr[a][2] = gr[a][2] + (double)(dv * dx3);
+          g0 = g0 + (double)(dv * dx1);
+          g1 = g1 + (double)(dv * dx2);
+          g2 = g2 + (double)(dv * dx3);
         }
       }
     }
-    for (int c = 0; c < 3; ++c) S.d[3 * i + c] = gr[a][c];
+    // LRF Taylor expansion about the group centre
+    if (lane == 1 && S.use_lrf && !(fl & F_EXCL)) {
+      const double* L = S.lrf + (size_t)g * kLrfStride;
+      const double xa[3] = {xi0, xi1, xi2};
+      double df[3];
+      const double V = lrf_taylor_atom(L, xa, df);
+      e_lrf = 0.5 * S.crg[i] * V;
+      g0 = g0 - S.crg[i] * df[0];
+      g1 = g1 - S.crg[i] * df[1];
+      g2 = g2 - S.crg[i] * df[2];
+    }
   }
-  double* es = S.e_solute + (size_t)g * 8;
-  es[0] = e_ppel; es[1] = e_ppvdw; es[2] = e_pwel; es[3] = e_pwvdw;
-  es[4] = e_lrf; es[5] = e_fix; es[6] = e_shell; es[7] = 0.0;
+  // harmonic restraint of excluded and shell atoms to the topology position
+  if (lane == 2 && (fl & (F_EXCL | F_SHELL))) {
+    const double fk = (fl & F_EXCL) ? S.fk_fix : S.fk_pshell;
+    const double dr0 = xi0 - S.xtop[3 * i], dr1 = xi1 - S.xtop[3 * i + 1], dr2 = xi2 - S.xtop[3 * i + 2];
+    const double erst = 0.5 * fk * (dr0 * dr0 + dr1 * dr1 + dr2 * dr2);
+    if (fl & F_EXCL) e_fix = erst;
+    if (fl & F_SHELL) e_shell = erst;
+    g0 = g0 + fk * dr0;
+    g1 = g1 + fk * dr1;
+    g2 = g2 + fk * dr2;
+  }
+  g0 = wave_sum(g0); g1 = wave_sum(g1); g2 = wave_sum(g2);
+  e_ppel = wave_sum(e_ppel); e_ppvdw = wave_sum(e_ppvdw);
+  e_pwel = wave_sum(e_pwel); e_pwvdw = wave_sum(e_pwvdw);
+  e_lrf = wave_sum(e_lrf); e_fix = wave_sum(e_fix); e_shell = wave_sum(e_shell);
+  if (lane == 0) {
+    S.d[3 * i] = g0; S.d[3 * i + 1] = g1; S.d[3 * i + 2] = g2;
+    double* es = S.e_solute + (size_t)i * 8;
+    es[0] = e_ppel; es[1] = e_ppvdw; es[2] = e_pwel; es[3] = e_pwvdw;
+    es[4] = e_lrf; es[5] = e_fix; es[6] = e_shell; es[7] = 0.0;
+  }
 }
 template __global__ void k_solute_forces<double>(const DevSys*);
 template __global__ void k_solute_forces<float>(const DevSys*);
 
 // ---------------------------------------------------------------------------
-// Q atoms: one wavefront per Q atom; lanes take the Q-solute partners and
-// the Q-water waters (nonbond_qp, nonbond_qw_spc :14411-14570) and the
-// wavefront reduces the Q atom's gradient and its per-state energies.
-template <typename P>
+// Q atoms (nonbond_qp :13788-13921, nonbond_qw_spc :14411-14570 seen from
+// the Q atom): kQChunks wavefronts per Q atom, each taking an interleaved
+// share of the Q-solute partners and the Q-water waters (the Q-water list
+// spans the whole sphere); each wavefront writes its gradient and per-state
+// energies to q_part, and k_q_reduce sums the chunks in a fixed order.
+// NS bounds the number of FEP states at compile time so that the per-state
+// accumulators stay in registers.
+template <typename P, int NS>
 __global__ __launch_bounds__(256) void k_q_forces(const DevSys* __restrict__ sys) {
   const DevSys& S = sys[blockIdx.y];
   const int lane = threadIdx.x & 63;
-  const int iq = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int wv = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int iq = wv / kQChunks, chunk = wv % kQChunks;
   if (iq >= S.nqat) return;
   const int ns = S.nstates;
   const int qi = S.iqseq[iq];
   const double xq0 = S.x[3 * qi], xq1 = S.x[3 * qi + 1], xq2 = S.x[3 * qi + 2];
   const int ti = S.iac[qi];
   double g0 = 0, g1 = 0, g2 = 0;
-  double e[kMaxQStates][4];
-  for (int st = 0; st < kMaxQStates; ++st) e[st][0] = e[st][1] = e[st][2] = e[st][3] = 0.0;
-  P qs[kMaxQStates], lam[kMaxQStates];
-  for (int st = 0; st < ns; ++st) { qs[st] = (P)S.qcrg[iq * ns + st]; lam[st] = (P)S.lambda[st]; }
-  for (int k = lane; k < S.nqp_atoms; k += 64) {
+  double e[NS][4];
+  P qs[NS], lam[NS];
+#pragma unroll
+  for (int st = 0; st < NS; ++st) {
+    e[st][0] = e[st][1] = e[st][2] = e[st][3] = 0.0;
+    qs[st] = st < ns ? (P)S.qcrg[iq * ns + st] : (P)0;
+    lam[st] = st < ns ? (P)S.lambda[st] : (P)0;
+  }
+  const int stride = 64 * kQChunks;
+  for (int k = chunk * 64 + lane; k < S.nqp_atoms; k += stride) {
     const int j = S.qp_atoms[k];
     const int lj = S.qp_lj[(size_t)j * S.nqat + iq];
     const int l = lj - 1;
     const int tj = S.iac[j];
     P dx1 = (P)(S.x[3 * j] - xq0), dx2 = (P)(S.x[3 * j + 1] - xq1), dx3 = (P)(S.x[3 * j + 2] - xq2);
-    P r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
-    r2 = (P)1 / r2;
+    P r;
+    P r2 = inv_r2(dx1 * dx1 + dx2 * dx2 + dx3 * dx3, r);
     P r6 = r2 * r2 * r2;
-    P r = psqrt(r2);
     const P aLJ = (P)(S.avdw[ti * 3 + l] * S.avdw[tj * 3 + l]);
     const P bLJ = (P)(S.bvdw[ti * 3 + l] * S.bvdw[tj * 3 + l]);
-    for (int st = 0; st < ns; ++st) {
-      P Vel = qs[st] * (P)S.crg[j] * r;
+    const P cj = (P)S.crg[j];
+#pragma unroll
+    for (int st = 0; st < NS; ++st) {
+      if (st >= ns) continue;
+      P Vel = qs[st] * cj * r;
       if (lj == 3) Vel = Vel * (P)S.el14_scale;
       P Va = aLJ * r6 * r6;
       P Vb = bLJ * r6;
@@ -319,19 +363,22 @@ __global__ __launch_bounds__(256) void k_q_forces(const DevSys* __restrict__ sys
   const int lo = S.qw_lj[iq];  // LJ code of the (water O, Q atom) pair
   const P aO = (P)S.avdw[ti * 3 + lo - 1], bO = (P)S.bvdw[ti * 3 + lo - 1];
   const P aS = (P)S.alj_O[lo - 1], bS = (P)S.blj_O[lo - 1];
-  for (int k = lane; k < S.nqw_waters; k += 64) {
+  const P cO = (P)S.chg_solv[0];
+  for (int k = chunk * 64 + lane; k < S.nqw_waters; k += stride) {
     const int jo = S.nat_solute + 3 * S.qw_waters[k];
     P dx1 = (P)(S.x[3 * jo] - xq0), dx2 = (P)(S.x[3 * jo + 1] - xq1), dx3 = (P)(S.x[3 * jo + 2] - xq2);
     P r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
     P r6_hc = r2 * r2 * r2;
-    r2 = (P)1 / r2;
-    P r = psqrt(r2);
+    P r;
+    r2 = inv_r2(r2, r);
     P r6 = (P)1 / r6_hc;
     P Va = aO * aS * r6 * r6;
     P Vb = bO * bS * r6;
     P dv = 0;
-    for (int st = 0; st < ns; ++st) {
-      P Vel = (P)S.chg_solv[0] * qs[st] * r;
+#pragma unroll
+    for (int st = 0; st < NS; ++st) {
+      if (st >= ns) continue;
+      P Vel = cO * qs[st] * r;
       dv = dv + r2 * (-Vel - ((P)12 * Va - (P)6 * Vb) * (r6_hc / r6_hc)) * lam[st];
       e[st][2] = e[st][2] + (double)Vel;
       e[st][3] = e[st][3] + (double)Va - (double)Vb;
@@ -342,11 +389,14 @@ __global__ __launch_bounds__(256) void k_q_forces(const DevSys* __restrict__ sys
     for (int ip = 1; ip < 3; ++ip) {
       const int ja = jo + ip;
       P ex1 = (P)(S.x[3 * ja] - xq0), ex2 = (P)(S.x[3 * ja + 1] - xq1), ex3 = (P)(S.x[3 * ja + 2] - xq2);
-      P q2 = (P)1 / (ex1 * ex1 + ex2 * ex2 + ex3 * ex3);
-      P rr = psqrt(q2);
+      P rr;
+      P q2 = inv_r2(ex1 * ex1 + ex2 * ex2 + ex3 * ex3, rr);
+      const P ch = (P)S.chg_solv[ip];
       P dvh = 0;
-      for (int st = 0; st < ns; ++st) {
-        P Vel = (P)S.chg_solv[ip] * qs[st] * rr;
+#pragma unroll
+      for (int st = 0; st < NS; ++st) {
+        if (st >= ns) continue;
+        P Vel = ch * qs[st] * rr;
         e[st][2] = e[st][2] + (double)Vel;
         dvh = dvh - q2 * Vel * lam[st];
       }
@@ -356,18 +406,40 @@ __global__ __launch_bounds__(256) void k_q_forces(const DevSys* __restrict__ sys
     }
   }
   g0 = wave_sum(g0); g1 = wave_sum(g1); g2 = wave_sum(g2);
-  for (int st = 0; st < ns; ++st)
-    for (int c = 0; c < 4; ++c) e[st][c] = wave_sum(e[st][c]);
-  if (lane == 0) {
-    S.d[3 * qi] = S.d[3 * qi] + g0;
-    S.d[3 * qi + 1] = S.d[3 * qi + 1] + g1;
-    S.d[3 * qi + 2] = S.d[3 * qi + 2] + g2;
-    for (int st = 0; st < ns; ++st)
-      for (int c = 0; c < 4; ++c) S.e_q[((size_t)iq * ns + st) * 4 + c] = e[st][c];
+  double* out = S.q_part + (size_t)wv * kQPartStride;
+  if (lane == 0) { out[0] = g0; out[1] = g1; out[2] = g2; }
+#pragma unroll
+  for (int st = 0; st < NS; ++st) {
+    if (st >= ns) continue;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const double v = wave_sum(e[st][c]);
+      if (lane == 0) out[3 + st * 4 + c] = v;
+    }
   }
 }
-template __global__ void k_q_forces<double>(const DevSys*);
-template __global__ void k_q_forces<float>(const DevSys*);
+template __global__ void k_q_forces<double, 2>(const DevSys*);
+template __global__ void k_q_forces<float, 2>(const DevSys*);
+template __global__ void k_q_forces<double, 4>(const DevSys*);
+template __global__ void k_q_forces<float, 4>(const DevSys*);
+template __global__ void k_q_forces<double, kMaxQStates>(const DevSys*);
+template __global__ void k_q_forces<float, kMaxQStates>(const DevSys*);
+
+// Sum of the Q-atom chunks: gradient added to d, per-state energies stored.
+__global__ __launch_bounds__(64) void k_q_reduce(const DevSys* __restrict__ sys) {
+  const DevSys& S = sys[blockIdx.y];
+  const int iq = blockIdx.x * blockDim.x + threadIdx.x;
+  if (iq >= S.nqat) return;
+  const int ns = S.nstates;
+  const int n = 3 + 4 * ns;
+  const int qi = S.iqseq[iq];
+  for (int c = 0; c < n; ++c) {
+    double v = 0.0;
+    for (int k = 0; k < kQChunks; ++k) v = v + S.q_part[(size_t)(iq * kQChunks + k) * kQPartStride + c];
+    if (c < 3) S.d[3 * qi + c] = S.d[3 * qi + c] + v;
+    else S.e_q[(size_t)iq * ns * 4 + (c - 3)] = v;
+  }
+}
 
 // ---------------------------------------------------------------------------
 // Q-Q and Q-neighbour pairs (nonbond_qq :13491-13636): one thread per atom

@@ -115,20 +115,6 @@ __global__ __launch_bounds__(256) void k_water_lists(const DevSys* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// Inverse distance powers of one atom pair from the squared distance r2sq:
-// returns 1/r^2 and sets rinv = 1/r.  FP64 keeps the reference's
-// operations (one/r2, then sqrt, src/md.f90:15074-15080); FP32 uses the
-// hardware reciprocal square root (v_rsq_f32, 1 ulp).
-__device__ __forceinline__ double inv_r2(double r2sq, double& rinv) {
-  double r2 = 1.0 / r2sq;
-  rinv = sqrt(r2);
-  return r2;
-}
-__device__ __forceinline__ float inv_r2(float r2sq, float& rinv) {
-  rinv = __builtin_amdgcn_rsqf(r2sq);
-  return rinv * rinv;
-}
-
 // Water forces (nonbond_ww_spc src/md.f90:15021-15119, water side of
 // nonbond_pw :13282-13365 and nonbond_qw_spc :14411-14570, lrf_taylor
 // :5824-5891, restrain_solvent :18055-18102).  One wavefront per water:
@@ -635,8 +621,8 @@ __global__ __launch_bounds__(256) void k_energy_reduce(const DevSys* __restrict_
   for (int iw = threadIdx.x; iw < S.nwat; iw += blockDim.x)
     for (int c = 0; c < 4; ++c) w[c] += S.e_water[(size_t)iw * 4 + c];
   double p[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int g = threadIdx.x; g < S.ncgp_solute; g += blockDim.x)
-    for (int c = 0; c < 8; ++c) p[c] += S.e_solute[(size_t)g * 8 + c];
+  for (int i = threadIdx.x; i < S.nat_solute; i += blockDim.x)
+    for (int c = 0; c < 8; ++c) p[c] += S.e_solute[(size_t)i * 8 + c];
   double b[6] = {0, 0, 0, 0, 0, 0};
   for (int t = threadIdx.x; t < S.nbonded_atoms; t += blockDim.x)
     for (int c = 0; c < 6; ++c) b[c] += S.e_bonded[(size_t)t * 6 + c];
