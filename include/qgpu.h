@@ -108,7 +108,9 @@ int qgpu_energy_series(qgpu_handle* h, int k, double* out, int cap, int* nframes
 
 /* Exports pair list `which` of instance k in the reference's layout and
  * order (NB_TYPE i, j, LJcod; 0-based atoms): 0 = nbpp, 1 = nbpw, 2 = nbww
- * (src/md.f90:329-433).  Call with out == NULL to get the count in *n. */
+ * (src/md.f90:329-433); 3 = nbpp as the force kernel evaluates it (charge-
+ * group list plus the per-entry pair masks formed on the device), which
+ * equals list 0.  Call with out == NULL to get the count in *n. */
 int qgpu_get_pairlist(qgpu_handle* h, int k, int which, int* out, long* n);
 
 /* Sizes of the pair lists of instance k in the reference's layout (the

@@ -737,6 +737,32 @@ void Batch::export_list(int k, int which, std::vector<int>& out) {
             out.push_back(s.ljcod[s.iac[ns + la] * s.natyps + s.iac[ns + ka]]);
           }
       }
+  } else if (which == 3) {
+    // nbpp as the force kernel sees it: group list plus the device pair masks
+    const qsys::Topology& t = s.top;
+    const int nc = t.ncgp_solute;
+    std::vector<int> cnt(nc), nb((size_t)nc * D.pp_cap);
+    std::vector<ulonglong2> mk((size_t)nc * D.pp_cap);
+    if (nc > 0) {
+      HIPCHECK(hipMemcpy(cnt.data(), D.pp_cnt, 4 * cnt.size(), hipMemcpyDeviceToHost));
+      HIPCHECK(hipMemcpy(nb.data(), D.pp_nb, 4 * nb.size(), hipMemcpyDeviceToHost));
+      HIPCHECK(hipMemcpy(mk.data(), D.pp_mask, sizeof(ulonglong2) * mk.size(), hipMemcpyDeviceToHost));
+    }
+    for (int g = 0; g < nc; ++g)
+      for (int k2 = 0; k2 < cnt[g]; ++k2) {
+        const int other = nb[(size_t)g * D.pp_cap + k2];
+        const ulonglong2 m = mk[(size_t)g * D.pp_cap + k2];
+        if (!(other == g || ((g < other) == (((g + other) & 1) == 0)))) continue;
+        for (int a = 0; a < kMaxGroupAtoms; ++a)
+          for (int b = 0; b < kMaxGroupAtoms; ++b) {
+            const int bit = a * kMaxGroupAtoms + b;
+            if ((m.x >> bit) & 1ull) continue;
+            const int i = t.cgpatom[t.cgp[g].first + a], j = t.cgpatom[t.cgp[other].first + b];
+            if (other == g && i >= j) continue;
+            out.push_back(i); out.push_back(j);
+            out.push_back(((m.y >> bit) & 1ull) ? 3 : s.ljcod[s.iac[i] * s.natyps + s.iac[j]]);
+          }
+      }
   } else if (which == 0 || which == 1) {
     const qsys::Topology& t = s.top;
     const int nc = t.ncgp_solute;

@@ -35,9 +35,12 @@ __device__ ulonglong2 pair_masks(const DevSys& S, int g, int jg) {
   unsigned long long skip = 0, m14 = 0;
   const int fi = S.cgp_first[g], ni = S.cgp_last[g] - fi + 1;
   const int fj = S.cgp_first[jg], nj = S.cgp_last[jg] - fj + 1;
+  // slots past the end of either group are never evaluated
+  const unsigned long long absent_cols = 0xffull & ~((1ull << nj) - 1ull);
   for (int a = 0; a < ni; ++a) {
     const int i = S.cgpatom[fi + a];
     const bool qi = (S.flags[i] & F_Q) != 0;
+    skip |= absent_cols << (a * kMaxGroupAtoms);
     for (int b = 0; b < nj; ++b) {
       const int j = S.cgpatom[fj + b];
       const unsigned long long bit = 1ull << (a * kMaxGroupAtoms + b);

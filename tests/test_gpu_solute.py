@@ -60,6 +60,19 @@ def test_solute_pairlists_bitexact(c2_small):
         got = e.pairlist(0, which)
         assert ref.shape[0] > 0
         assert np.array_equal(got, ref), which
+    # the pair status the force kernel uses (device masks) gives the same nbpp
+    assert np.array_equal(e.pairlist(0, 3), o.pairlist(0))
+
+
+def test_solute_pair_masks_long_exclusions(c2):
+    # 300-atom branched solute: many excluded and 1-4 pairs lie more than
+    # 25 atoms apart and go through the long-pair lookup
+    _say("case: nbpp from device pair masks, C2 R=25 + 300-atom solute")
+    o, e = _step0(c2, qe.QGPU_FP64)
+    ref = o.pairlist(0)
+    assert np.array_equal(e.pairlist(0, 0), ref)
+    assert np.array_equal(e.pairlist(0, 3), ref)
+    assert ((np.abs(ref[:, 0] - ref[:, 1]) > 25) & (ref[:, 2] == 3)).sum() > 0
 
 
 def test_step0_fp64(c2_small):
