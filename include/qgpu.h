@@ -4,7 +4,7 @@
  * The reference has no FFI for this path: its step is a set of argument-less
  * subroutines on module globals in src/md.f90.  Each entry point below
  * replaces one reference call site (cited per function); a Fortran host binds
- * them through ISO_C_BINDING (see the interface module in INTEGRATION.md).
+ * them through ISO_C_BINDING (interface module qsource_amd/fortran/qgpu.f90).
  *
  * Conventions
  *  - every function returns 0 on success, nonzero on failure; the message is

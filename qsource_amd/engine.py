@@ -48,7 +48,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     if _lib is not None:
         return _lib
     if not os.path.exists(path):
-        raise RuntimeError("libqgpu.so not built: run __graft_entry__.build() (or qsource_amd/build.py)")
+        raise RuntimeError("libqgpu.so not built: run python -m qsource_amd.build")
     lib = ctypes.CDLL(path)
     P = ctypes.c_void_p
     dp = ctypes.POINTER(ctypes.c_double)
