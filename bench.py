@@ -13,8 +13,12 @@ the device every 10 steps.
 
 value = instances x steps x dt_fs x 1e-6 x 86400 / wall seconds of the
 timed region (mixed precision: FP32 pair arithmetic, FP64 accumulation).
-The same windows run in FP64 give the reference dG per system (Zwanzig and
-BAR over the recorded frames); |ddG| = |dG_mixed - dG_fp64| is reported.
+Accuracy gate: every recorded frame of the mixed run is also evaluated with
+FP64 pair arithmetic on the same coordinates; |ddG| = |dG(mixed dU) -
+dG(FP64 dU)| over identical frames (Zwanzig and BAR) must stay below 0.05
+kcal/mol.  The same windows run as an FP64 batch give independent
+trajectories; their dG difference is reported with its bootstrap standard
+error (it is dominated by sampling noise at these run lengths).
 The plain FP64 engine on one window is timed as the unbatched baseline, and
 the reference's serial qdyn5 on one window of the R=22 system (host cores,
 1 thread) as the CPU baseline.
@@ -84,18 +88,66 @@ def run_batch(inputs, precision, steps, warmup, iene=10):
     return e, wall
 
 
-def dg_per_system(e, groups, lam, temperature=300.0):
+def series(e, groups, fp64=False):
+    """Per system, per window: (nframes, nstates) EQ totals of the frames."""
+    return [[e.energy_series(k, fp64=fp64)[:, :, 0] for k in g] for g in groups]
+
+
+def dg_of(ser, lam, temperature=300.0):
+    """(Zwanzig, BAR) dG per system."""
     from qsource_amd import fep
     out = []
-    for g in groups:
-        v = [e.energy_series(k)[:, :, 0] for k in g]
-        z = fep.zwanzig(lam, v, temperature)
+    for v in ser:
+        z = fep.zwanzig(lam, v, temperature)["total"]
         try:
             b = fep.bar(lam, v, temperature)["total"]
         except ValueError:
             b = float("nan")
-        out.append((z["total"], b))
+        out.append((z, b))
     return out
+
+
+def zwanzig_stderr(ser, lam, temperature=300.0, block=10, nboot=200, seed=7):
+    """Standard error of the Zwanzig dG per system: bootstrap over blocks of
+    `block` consecutive frames within every window (frames are correlated)."""
+    from qsource_amd import fep
+    rng = np.random.default_rng(seed)
+    out = []
+    for v in ser:
+        vals = []
+        for _ in range(nboot):
+            w = []
+            for e in v:
+                nb = max(1, len(e) // block)
+                pick = rng.integers(0, nb, nb)
+                w.append(np.concatenate([e[b * block:(b + 1) * block] for b in pick]))
+            vals.append(fep.zwanzig(lam, w, temperature)["total"])
+        out.append(float(np.std(vals)))
+    return out
+
+
+def force_error(inputs, states):
+    """Mixed vs FP64 gradient and energies on identical coordinates (one
+    window per system): relative RMS force error and per-term energy
+    differences."""
+    from qsource_amd import engine as qe
+    res = []
+    engines = [qe.Engine(inputs, p) for p in (qe.QGPU_FP64, qe.QGPU_MIXED)]
+    for e in engines:
+        for k, (x, v) in enumerate(states):
+            e.set_state(k, x, v)
+        e.make_pair_lists()
+        e.pot_energy()
+    for k in range(len(inputs)):
+        d64, d32 = engines[0].gradient(k), engines[1].gradient(k)
+        e64, e32 = engines[0].energies(k), engines[1].energies(k)
+        rms = float(np.sqrt(np.mean((d32 - d64) ** 2)) / np.sqrt(np.mean(d64 ** 2)))
+        terms = {t: float(e32[t] - e64[t]) for t in ("ww_el", "ww_vdw", "pw_el", "pw_vdw", "pp_el",
+                                                      "pp_vdw", "qx_el", "qx_vdw", "lrf", "potential")}
+        res.append({"force_rms_rel": rms, "energy_diff": terms})
+    for e in engines:
+        e.close()
+    return res
 
 
 def water_force_flops(e, k):
@@ -166,7 +218,17 @@ def main():
         em, wall = run_batch(inputs, qe.QGPU_MIXED, args.steps, args.warmup)
         value = n * args.steps * cfg["stepsize_fs"] * 1e-6 * 86400.0 / wall
         log("mixed: %.3f ms/step, %.1f ns/day aggregate" % (wall / args.steps * 1e3, value))
-        dg_m = dg_per_system(em, groups, lam)
+        ser_m = series(em, groups)
+        ser_m64 = series(em, groups, fp64=True)
+        dg_m = dg_of(ser_m, lam)
+        dg_m64 = dg_of(ser_m64, lam)
+        ddg_frames = [abs(a[0] - b[0]) for a, b in zip(dg_m, dg_m64)]
+        ddg_frames_bar = [abs(a[1] - b[1]) for a, b in zip(dg_m, dg_m64)]
+        du_err = max(float(np.abs((a @ np.array([-1.0, 1.0])) - (b @ np.array([-1.0, 1.0]))).max())
+                     for sm, sf in zip(ser_m, ser_m64) for a, b in zip(sm, sf))
+        log("ddG on identical frames (mixed vs FP64 dU): %s" % ddg_frames)
+        mid = [g[len(g) // 2] for g in groups]
+        states = [em.state(k) for k in mid]
         prof = em.profile_kernels(args.profile_steps)
         dom = max((k for k in prof if k not in ("cgp_centers", "water_lists", "solute_lists")),
                   key=lambda k: prof[k])
@@ -182,20 +244,28 @@ def main():
                 "traffic": None, "flop_per_launch": flops, "water_pairs_per_launch": wpairs,
                 "kernel_ms": prof["water_forces"], "dominant_kernel": dom}
         em.close()
-        fp64 = {}
+        ferr = force_error([inputs[k] for k in mid], states)
+        log("force RMS error (mixed vs FP64, same coordinates): %s" % [f["force_rms_rel"] for f in ferr])
+        acc = {"dG_mixed_zwanzig": [a[0] for a in dg_m], "dG_mixed_bar": [a[1] for a in dg_m],
+               "dG_fp64_same_frames_zwanzig": [b[0] for b in dg_m64],
+               "dG_fp64_same_frames_bar": [b[1] for b in dg_m64],
+               "abs_ddG_same_frames_zwanzig": ddg_frames, "abs_ddG_same_frames_bar": ddg_frames_bar,
+               "max_abs_dU_error_kcal": du_err,
+               "gate_ddG_lt_0.05": bool(max(ddg_frames + ddg_frames_bar) < 0.05),
+               "force_error_one_window_per_system": ferr}
         if not args.skip_fp64:
-            log("fp64 batch")
+            log("fp64 batch (independent trajectories)")
             ef, wall64 = run_batch(inputs, qe.QGPU_FP64, args.steps, args.warmup)
-            dg_f = dg_per_system(ef, groups, lam)
+            ser_f = series(ef, groups)
             ef.close()
-            ddg = [abs(a[0] - b[0]) for a, b in zip(dg_m, dg_f)]
-            ddg_bar = [abs(a[1] - b[1]) for a, b in zip(dg_m, dg_f)]
-            fp64 = {"fp64_batched_ns_per_day": n * args.steps * 1e-6 * 86400.0 / wall64,
-                    "dG_mixed_zwanzig": [a[0] for a in dg_m], "dG_fp64_zwanzig": [b[0] for b in dg_f],
-                    "dG_mixed_bar": [a[1] for a in dg_m], "dG_fp64_bar": [b[1] for b in dg_f],
-                    "abs_ddG_zwanzig": ddg, "abs_ddG_bar": ddg_bar,
-                    "gate_ddG_lt_0.05": bool(max(ddg) < 0.05)}
-            log("ddG (Zwanzig) per system: %s" % ddg)
+            dg_f = dg_of(ser_f, lam)
+            se = [float(np.hypot(a, b)) for a, b in zip(zwanzig_stderr(ser_m, lam), zwanzig_stderr(ser_f, lam))]
+            acc.update({"fp64_batched_ns_per_day": n * args.steps * 1e-6 * 86400.0 / wall64,
+                        "dG_fp64_traj_zwanzig": [b[0] for b in dg_f], "dG_fp64_traj_bar": [b[1] for b in dg_f],
+                        "abs_ddG_traj_zwanzig": [abs(a[0] - b[0]) for a, b in zip(dg_m, dg_f)],
+                        "stderr_ddG_traj_zwanzig": se})
+            log("ddG of independent trajectories (Zwanzig) per system: %s +- %s" %
+                (acc["abs_ddG_traj_zwanzig"], se))
         # plain FP64 engine, one window at a time
         ep, wallp = run_batch([inputs[groups[1][10]]], qe.QGPU_FP64, max(200, args.steps // 4), 50)
         plain = max(200, args.steps // 4) * 1e-6 * 86400.0 / wallp
@@ -206,7 +276,7 @@ def main():
                 "roofline": roof, "cpu_baseline": cpu, "kernel_ms": prof,
                 "plain_fp64_single_window_ns_per_day": plain,
                 "speedup_vs_plain_fp64": value / plain})
-    out.update(fp64)
+    out["accuracy"] = acc
     print(json.dumps(out))
 
 

@@ -106,6 +106,13 @@ int qgpu_set_energy_interval(qgpu_handle* h, int iene, int max_frames);
  * `cap` doubles; *nframes gets the number recorded. */
 int qgpu_energy_series(qgpu_handle* h, int k, double* out, int cap, int* nframes);
 
+/* Mixed-precision batches only: the same frames as qgpu_energy_series, with
+ * the per-state Q energies re-evaluated on the frame's coordinates with FP64
+ * pair arithmetic (the plain engine's arithmetic).  Gives the FP64 dU of
+ * exactly the configurations the mixed run sampled, for the precision error
+ * of dU and dG.  No counterpart in the reference (it has one precision). */
+int qgpu_energy_series_fp64(qgpu_handle* h, int k, double* out, int cap, int* nframes);
+
 /* Exports pair list `which` of instance k in the reference's layout and
  * order (NB_TYPE i, j, LJcod; 0-based atoms): 0 = nbpp, 1 = nbpw, 2 = nbww
  * (src/md.f90:329-433); 3 = nbpp as the force kernel evaluates it (charge-

@@ -19,6 +19,7 @@ namespace qgpu {
 
 // kernels (kernels_water.hip, kernels_solute.hip)
 __global__ void k_cgp_centers(const DevSys*);
+__global__ void k_x_to_f32(const DevSys*);
 __global__ void k_water_lists(const DevSys*);
 template <typename P> __global__ void k_water_forces(const DevSys*);
 __global__ void k_watpol_theta(const DevSys*);
@@ -28,8 +29,8 @@ __global__ void k_temperature(const DevSys*, int);
 __global__ void k_energy_reduce(const DevSys*, int);
 __global__ void k_solute_lists(const DevSys*);
 template <typename P> __global__ void k_solute_forces(const DevSys*);
-template <typename P, int NS> __global__ void k_q_forces(const DevSys*);
-__global__ void k_q_reduce(const DevSys*);
+template <typename P, int NS> __global__ void k_q_forces(const DevSys*, int);
+__global__ void k_q_reduce(const DevSys*, int);
 __global__ void k_qq_forces(const DevSys*);
 __global__ void k_bonded(const DevSys*);
 
@@ -130,10 +131,18 @@ Batch::Batch(const std::vector<std::string>& inputs, int precision, int device) 
   }
   HIPCHECK(hipMalloc(&d_sys_, h_sys_.size() * sizeof(DevSys)));
   sync_sys();
+  refresh_xf();
 }
 
 void Batch::sync_sys() {
   HIPCHECK(hipMemcpy(d_sys_, h_sys_.data(), h_sys_.size() * sizeof(DevSys), hipMemcpyHostToDevice));
+}
+
+void Batch::refresh_xf() {
+  hipLaunchKernelGGL(k_x_to_f32, dim3((3 * max_natom_ + 255) / 256, (int)h_sys_.size()), dim3(256), 0,
+                     stream_, d_sys_);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(stream_));
 }
 
 void Batch::set_energy_interval(int iene, int max_frames) {
@@ -145,6 +154,9 @@ void Batch::set_energy_interval(int iene, int max_frames) {
     D.eseries_base = istep_;
     D.eseries_cap = max_frames;
     D.eseries = inst_[k]->alloc<double>((size_t)std::max(1, max_frames) * std::max(1, D.nstates) * kEneRecord);
+    D.eseries64 = precision_ == 1 && D.nqat > 0
+                      ? inst_[k]->alloc<double>((size_t)std::max(1, max_frames) * std::max(1, D.nstates) * kEneRecord)
+                      : nullptr;
     D.eseries_n = inst_[k]->alloc<int>(1);
   }
   iene_ = iene;
@@ -155,16 +167,18 @@ void Batch::set_energy_interval(int iene, int max_frames) {
   }
 }
 
-int Batch::energy_series(int k, double* out, int cap) {
+int Batch::energy_series(int k, double* out, int cap, bool fp64) {
   HIPCHECK(hipStreamSynchronize(stream_));
   const DevSys& D = h_sys_[k];
   if (!D.eseries_n) return 0;
+  if (fp64 && !D.eseries64)
+    throw std::runtime_error("FP64 re-evaluated frames exist only for mixed-precision batches with Q atoms");
   int n = 0;
   HIPCHECK(hipMemcpy(&n, D.eseries_n, sizeof(int), hipMemcpyDeviceToHost));
   n = std::min(n, D.eseries_cap);
   size_t per = (size_t)D.nstates * kEneRecord;
   size_t want = std::min((size_t)cap / std::max<size_t>(per, 1), (size_t)n);
-  if (out && want) HIPCHECK(hipMemcpy(out, D.eseries, want * per * 8, hipMemcpyDeviceToHost));
+  if (out && want) HIPCHECK(hipMemcpy(out, fp64 ? D.eseries64 : D.eseries, want * per * 8, hipMemcpyDeviceToHost));
   return n;
 }
 
@@ -203,6 +217,7 @@ void Batch::upload(int k) {
   D.precision = precision_;
   const int N = s.natom;
   D.x = I->upload(s.x);
+  D.xf = I->alloc<float>(3 * N);
   D.v = I->upload(s.v);
   D.xx = I->alloc<double>(3 * N);
   D.d = I->alloc<double>(3 * N);
@@ -352,6 +367,10 @@ void Batch::upload(int k) {
     D.qw_waters = I->upload(qw_waters);
     D.e_q = I->alloc<double>((size_t)std::max(1, nq * s.nstates) * 4);
     D.q_part = I->alloc<double>((size_t)std::max(1, nq) * kQChunks * kQPartStride);
+    if (precision_ == 1) {
+      D.q_part64 = I->alloc<double>((size_t)std::max(1, nq) * kQChunks * kQPartStride);
+      D.e_q64 = I->alloc<double>((size_t)std::max(1, nq * s.nstates) * 4);
+    }
     // Q-Q / Q-neighbour pairs, flattened over states, with atom incidence
     std::vector<int> qst, qiq, qj, qjq, qlj;
     std::vector<double> qel;
@@ -493,7 +512,7 @@ void Batch::launch_lists() {
   dim3 g0((max_ncgp_ + 255) / 256, ni);
   { pb(0); hipLaunchKernelGGL(k_cgp_centers, g0, dim3(256), 0, stream_, d_sys_); pe(0); }
   if (max_nwat_ > 0) {
-    dim3 g1((max_nwat_ * 64 + 255) / 256, ni);
+    dim3 g1((max_nwat_ * 8 + 255) / 256, ni);
     { pb(1); hipLaunchKernelGGL(k_water_lists, g1, dim3(256), 0, stream_, d_sys_); pe(1); }
   }
   if (max_ncgp_solute_ > 0) {
@@ -505,7 +524,7 @@ void Batch::launch_lists() {
 void Batch::launch_forces() {
   const int ni = (int)h_sys_.size();
   if (max_nwat_ > 0) {
-    dim3 g1((max_nwat_ * 64 + 255) / 256, ni);
+    dim3 g1((max_nwat_ * 8 + 255) / 256, ni);
     if (precision_ == 0)
       { pb(3); hipLaunchKernelGGL(k_water_forces<double>, g1, dim3(256), 0, stream_, d_sys_); pe(3); }
     else
@@ -522,18 +541,29 @@ void Batch::launch_forces() {
     { pb(5); hipLaunchKernelGGL(k_bonded, dim3((max_nbonded_atoms_ + 255) / 256, ni), dim3(256), 0, stream_, d_sys_); pe(5); }
   if (max_nqat_ > 0) {
     dim3 gq((max_nqat_ * kQChunks * 64 + 255) / 256, ni);
+    auto q_fp64 = [&](int shadow) {
+      if (max_nstates_ <= 2) hipLaunchKernelGGL((k_q_forces<double, 2>), gq, dim3(256), 0, stream_, d_sys_, shadow);
+      else if (max_nstates_ <= 4) hipLaunchKernelGGL((k_q_forces<double, 4>), gq, dim3(256), 0, stream_, d_sys_, shadow);
+      else hipLaunchKernelGGL((k_q_forces<double, kMaxQStates>), gq, dim3(256), 0, stream_, d_sys_, shadow);
+    };
+    const dim3 gr((max_nqat_ + 63) / 64, ni);
     pb(6);
     if (precision_ == 0) {
-      if (max_nstates_ <= 2) hipLaunchKernelGGL((k_q_forces<double, 2>), gq, dim3(256), 0, stream_, d_sys_);
-      else if (max_nstates_ <= 4) hipLaunchKernelGGL((k_q_forces<double, 4>), gq, dim3(256), 0, stream_, d_sys_);
-      else hipLaunchKernelGGL((k_q_forces<double, kMaxQStates>), gq, dim3(256), 0, stream_, d_sys_);
+      q_fp64(0);
     } else {
-      if (max_nstates_ <= 2) hipLaunchKernelGGL((k_q_forces<float, 2>), gq, dim3(256), 0, stream_, d_sys_);
-      else if (max_nstates_ <= 4) hipLaunchKernelGGL((k_q_forces<float, 4>), gq, dim3(256), 0, stream_, d_sys_);
-      else hipLaunchKernelGGL((k_q_forces<float, kMaxQStates>), gq, dim3(256), 0, stream_, d_sys_);
+      if (max_nstates_ <= 2) hipLaunchKernelGGL((k_q_forces<float, 2>), gq, dim3(256), 0, stream_, d_sys_, 0);
+      else if (max_nstates_ <= 4) hipLaunchKernelGGL((k_q_forces<float, 4>), gq, dim3(256), 0, stream_, d_sys_, 0);
+      else hipLaunchKernelGGL((k_q_forces<float, kMaxQStates>), gq, dim3(256), 0, stream_, d_sys_, 0);
     }
-    hipLaunchKernelGGL(k_q_reduce, dim3((max_nqat_ + 63) / 64, ni), dim3(64), 0, stream_, d_sys_);
+    hipLaunchKernelGGL(k_q_reduce, gr, dim3(64), 0, stream_, d_sys_, 0);
     pe(6);
+    // mixed precision with energy frames: the Q energies of each recorded
+    // frame are also evaluated with FP64 pair arithmetic (the kernels return
+    // at once on other steps)
+    if (precision_ == 1 && iene_ > 0) {
+      q_fp64(1);
+      hipLaunchKernelGGL(k_q_reduce, gr, dim3(64), 0, stream_, d_sys_, 1);
+    }
   }
   if (max_nqq_atoms_ > 0)
     { pb(7); hipLaunchKernelGGL(k_qq_forces, dim3((max_nqq_atoms_ + 255) / 256, ni), dim3(256), 0, stream_, d_sys_); pe(7); }
@@ -651,6 +681,7 @@ void Batch::set_state(int k, const double* x, const double* v) {
   size_t n = 3 * (size_t)h_sys_[k].natom;
   if (x) HIPCHECK(hipMemcpy(h_sys_[k].x, x, n * 8, hipMemcpyHostToDevice));
   if (v) HIPCHECK(hipMemcpy(h_sys_[k].v, v, n * 8, hipMemcpyHostToDevice));
+  if (x) refresh_xf();
 }
 
 void Batch::get_gradient(int k, double* d) {
@@ -983,6 +1014,12 @@ int qgpu_energy_series(qgpu_handle* h, int k, double* out, int cap, int* nframes
   return guarded([&] {
     CHECK_K(h, k);
     *nframes = h->b->energy_series(k, out, cap);
+  });
+}
+int qgpu_energy_series_fp64(qgpu_handle* h, int k, double* out, int cap, int* nframes) {
+  return guarded([&] {
+    CHECK_K(h, k);
+    *nframes = h->b->energy_series(k, out, cap, true);
   });
 }
 int qgpu_list_counts(qgpu_handle* h, int k, long* counts) {

@@ -64,6 +64,7 @@ struct DevSys {
   int nqat, nstates;
   int precision;                 // 0 = FP64, 1 = mixed FP32 pair / FP64 accumulation
   double *x, *v, *xx, *d, *xtop;
+  float* xf;                     // FP32 copy of x for the mixed-precision pair kernels
   const double *crg, *winv, *mass;
   const unsigned char* flags;
   const int *iac;                // type index per atom
@@ -116,6 +117,7 @@ struct DevSys {
   const int* qw_waters;          // waters in the Q-water list, ascending
   double* e_q;                   // nqat x nstates x 4: qp_el qp_vdw qw_el qw_vdw
   double* q_part;                // nqat x kQChunks x kQPartStride partial sums
+  double *q_part64, *e_q64;      // mixed precision: FP64 re-evaluation on energy-file steps
   // Q-Q and Q-neighbour pairs of all states (nbqq), and per-atom incidence
   int nqq;                       // total pairs over states
   const int *qq_state, *qq_iq, *qq_j, *qq_jq, *qq_lj;
@@ -169,6 +171,7 @@ struct DevSys {
   int eseries_base;    // step counter when recording was switched on
   int eseries_cap;
   double* eseries;
+  double* eseries64;   // mixed precision: the same frames with FP64 Q pair arithmetic
   int* eseries_n;
 };
 
@@ -200,7 +203,7 @@ class Batch {
 
   // energy-file frames recorded on the device every iene steps
   void set_energy_interval(int iene, int max_frames);
-  int energy_series(int k, double* out, int cap);
+  int energy_series(int k, double* out, int cap, bool fp64 = false);
   void set_graphs(bool on) { use_graph_ = on; }
   // per-kernel average device time (ms per launch) over nsteps steps run
   // with an event pair around every launch; order of kKernelNames
@@ -221,6 +224,7 @@ class Batch {
   void one_step();
   void launch_cycle_graph();
   void sync_sys();
+  void refresh_xf();
   int nbcycle_ = 10;
   int iene_ = 0;
   bool use_graph_ = true;

@@ -29,6 +29,41 @@ __device__ __forceinline__ float inv_r2(float r2sq, float& rinv) {
   return rinv * rinv;
 }
 
+// Coordinates read by the pair arithmetic: FP64 x, or its FP32 copy
+template <typename P>
+__device__ __forceinline__ const P* pair_x(const DevSys& S);
+template <>
+__device__ __forceinline__ const double* pair_x<double>(const DevSys& S) { return S.x; }
+template <>
+__device__ __forceinline__ const float* pair_x<float>(const DevSys& S) { return S.xf; }
+
+// c - a*b and the squared norm: FP64 as separate operations in the
+// reference's order (d1*d1 + d2*d2 + d3*d3), FP32 with fused multiply-add
+__device__ __forceinline__ double pmsub(double a, double b, double c) { return c - a * b; }
+__device__ __forceinline__ float pmsub(float a, float b, float c) { return __builtin_fmaf(-a, b, c); }
+__device__ __forceinline__ double pnorm2(double a, double b, double c) { return a * a + b * b + c * c; }
+__device__ __forceinline__ float pnorm2(float a, float b, float c) {
+  return __builtin_fmaf(c, c, __builtin_fmaf(b, b, a * a));
+}
+
+// Sum over an aligned group of 8 lanes; every lane of the group returns the
+// same total
+__device__ __forceinline__ double seg8_sum(double v) {
+  v += __shfl_xor(v, 1, 64);
+  v += __shfl_xor(v, 2, 64);
+  v += __shfl_xor(v, 4, 64);
+  return v;
+}
+
+// True on a step whose energy-file frame is recorded (step counter a positive
+// multiple of iene since recording started) when the batch also keeps the
+// FP64 re-evaluation of its Q energies.
+__device__ __forceinline__ bool shadow_record_step(const DevSys& S) {
+  if (!S.eseries64 || S.iene <= 0) return false;
+  const int st = *S.step_ctr - S.eseries_base;
+  return st > 0 && st % S.iene == 0;
+}
+
 // 64-lane butterfly sum; every lane returns the total
 __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -69,11 +104,13 @@ struct LrfAcc {
     for (int k = 0; k < 10; ++k) p3[k] = 0;
   }
   __device__ void add(double q, double d0, double d1, double d2) {
-    double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-    double f0 = q / (r2 * sqrt(r2));
-    double f1 = 3.0 * f0 / r2;
-    double f2 = -f1 / r2;
-    p0 = p0 + f0 * r2;
+    const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+    const double ri2 = 1.0 / r2;
+    const double ri = sqrt(ri2);
+    const double f0 = q * ri * ri2;      // q / r^3
+    const double f1 = 3.0 * f0 * ri2;    // 3 q / r^5
+    const double f2 = -f1 * ri2;         // -3 q / r^7
+    p0 = p0 + q * ri;
     p1[0] = p1[0] - f0 * d0; p1[1] = p1[1] - f0 * d1; p1[2] = p1[2] - f0 * d2;
     p2[0] = p2[0] + f1 * d0 * d0 - f0;
     p2[1] = p2[1] + f1 * d0 * d1;
@@ -91,6 +128,13 @@ struct LrfAcc {
     p3[7] = p3[7] + f2 * (5.0 * d1 * d1 * d2 - r2 * d2);         // 223
     p3[8] = p3[8] + f2 * (5.0 * d1 * d2 * d2 - r2 * d1);         // 233
     p3[9] = p3[9] + f2 * (5.0 * d2 * d2 * d2 - r2 * 3.0 * d2);   // 333
+  }
+  // sum over an aligned group of 8 lanes
+  __device__ void seg8_reduce() {
+    p0 = seg8_sum(p0);
+    for (int k = 0; k < 3; ++k) p1[k] = seg8_sum(p1[k]);
+    for (int k = 0; k < 6; ++k) p2[k] = seg8_sum(p2[k]);
+    for (int k = 0; k < 10; ++k) p3[k] = seg8_sum(p3[k]);
   }
   __device__ void wave_reduce() {
     p0 = wave_sum(p0);

@@ -128,26 +128,6 @@ __global__ __launch_bounds__(256) void k_solute_lists(const DevSys* __restrict__
   }
 }
 
-// One solute-solute atom pair as nonbond_pp forms it (src/md.f90:13107-13150)
-// with (ia, ja) in the reference's orientation; returns dv and dx = x_ja - x_ia.
-template <typename P>
-__device__ __forceinline__ P pp_pair(const DevSys& S, int ia, int ja, int lj, P& dx1, P& dx2, P& dx3,
-                                     P& Vel, P& Va, P& Vb) {
-  dx1 = (P)(S.x[3 * ja] - S.x[3 * ia]);
-  dx2 = (P)(S.x[3 * ja + 1] - S.x[3 * ia + 1]);
-  dx3 = (P)(S.x[3 * ja + 2] - S.x[3 * ia + 2]);
-  P r2 = (P)1 / (dx1 * dx1 + dx2 * dx2 + dx3 * dx3);
-  P r = psqrt(r2);
-  P r6 = r2 * r2 * r2;
-  Vel = (P)S.crg[ia] * (P)S.crg[ja] * r;
-  int l = lj - 1;
-  if (lj == 3) Vel = Vel * (P)S.el14_scale;
-  const int ti = S.iac[ia], tj = S.iac[ja];
-  Va = r6 * r6 * (P)S.avdw[ti * 3 + l] * (P)S.avdw[tj * 3 + l];
-  Vb = r6 * (P)S.bvdw[ti * 3 + l] * (P)S.bvdw[tj * 3 + l];
-  return r2 * (-Vel - (P)12 * Va + (P)6 * Vb);
-}
-
 // ---------------------------------------------------------------------------
 // Solute forces: one wavefront per solute atom (nonbond_pp :13069-13155,
 // nonbond_pw :13282-13365, lrf_taylor, fix_shell :2816-2862, and the
@@ -316,8 +296,9 @@ template __global__ void k_solute_forces<float>(const DevSys*);
 // NS bounds the number of FEP states at compile time so that the per-state
 // accumulators stay in registers.
 template <typename P, int NS>
-__global__ __launch_bounds__(256) void k_q_forces(const DevSys* __restrict__ sys) {
+__global__ __launch_bounds__(256) void k_q_forces(const DevSys* __restrict__ sys, int shadow) {
   const DevSys& S = sys[blockIdx.y];
+  if (shadow && !shadow_record_step(S)) return;
   const int lane = threadIdx.x & 63;
   const int wv = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int iq = wv / kQChunks, chunk = wv % kQChunks;
@@ -409,7 +390,7 @@ __global__ __launch_bounds__(256) void k_q_forces(const DevSys* __restrict__ sys
     }
   }
   g0 = wave_sum(g0); g1 = wave_sum(g1); g2 = wave_sum(g2);
-  double* out = S.q_part + (size_t)wv * kQPartStride;
+  double* out = (shadow ? S.q_part64 : S.q_part) + (size_t)wv * kQPartStride;
   if (lane == 0) { out[0] = g0; out[1] = g1; out[2] = g2; }
 #pragma unroll
   for (int st = 0; st < NS; ++st) {
@@ -421,26 +402,31 @@ __global__ __launch_bounds__(256) void k_q_forces(const DevSys* __restrict__ sys
     }
   }
 }
-template __global__ void k_q_forces<double, 2>(const DevSys*);
-template __global__ void k_q_forces<float, 2>(const DevSys*);
-template __global__ void k_q_forces<double, 4>(const DevSys*);
-template __global__ void k_q_forces<float, 4>(const DevSys*);
-template __global__ void k_q_forces<double, kMaxQStates>(const DevSys*);
-template __global__ void k_q_forces<float, kMaxQStates>(const DevSys*);
+template __global__ void k_q_forces<double, 2>(const DevSys*, int);
+template __global__ void k_q_forces<float, 2>(const DevSys*, int);
+template __global__ void k_q_forces<double, 4>(const DevSys*, int);
+template __global__ void k_q_forces<float, 4>(const DevSys*, int);
+template __global__ void k_q_forces<double, kMaxQStates>(const DevSys*, int);
+template __global__ void k_q_forces<float, kMaxQStates>(const DevSys*, int);
 
 // Sum of the Q-atom chunks: gradient added to d, per-state energies stored.
-__global__ __launch_bounds__(64) void k_q_reduce(const DevSys* __restrict__ sys) {
+// With shadow != 0 (FP64 re-evaluation of a mixed-precision batch on an
+// energy-file step) only the energies are stored, into e_q64.
+__global__ __launch_bounds__(64) void k_q_reduce(const DevSys* __restrict__ sys, int shadow) {
   const DevSys& S = sys[blockIdx.y];
+  if (shadow && !shadow_record_step(S)) return;
   const int iq = blockIdx.x * blockDim.x + threadIdx.x;
   if (iq >= S.nqat) return;
   const int ns = S.nstates;
   const int n = 3 + 4 * ns;
   const int qi = S.iqseq[iq];
-  for (int c = 0; c < n; ++c) {
+  const double* part = shadow ? S.q_part64 : S.q_part;
+  double* eq = shadow ? S.e_q64 : S.e_q;
+  for (int c = shadow ? 3 : 0; c < n; ++c) {
     double v = 0.0;
-    for (int k = 0; k < kQChunks; ++k) v = v + S.q_part[(size_t)(iq * kQChunks + k) * kQPartStride + c];
+    for (int k = 0; k < kQChunks; ++k) v = v + part[(size_t)(iq * kQChunks + k) * kQPartStride + c];
     if (c < 3) S.d[3 * qi + c] = S.d[3 * qi + c] + v;
-    else S.e_q[(size_t)iq * ns * 4 + (c - 3)] = v;
+    else eq[(size_t)iq * ns * 4 + (c - 3)] = v;
   }
 }
 

@@ -32,219 +32,240 @@ __global__ void k_cgp_centers(const DevSys* __restrict__ sys) {
   for (int k = 3; k < kLrfStride; ++k) L[k] = 0.0;
 }
 
+// FP32 copy of the coordinates read by the mixed-precision pair kernels
+// (refreshed by k_integrate every step, and by this kernel after host writes).
+__global__ void k_x_to_f32(const DevSys* __restrict__ sys) {
+  const DevSys& S = sys[blockIdx.y];
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < 3 * S.natom) S.xf[c] = (float)S.x[c];
+}
+
 // ---------------------------------------------------------------------------
 // Water lists (nbwwlist_lrf src/md.f90:10897-11021 and the water side of
-// nbpwlist_lrf :9300-9429).  One wavefront per water: lanes scan the other
-// waters and the solute charge groups; partners inside the cut-off are
-// appended in ascending order by a ballot compaction, partners between the
-// cut-off and RcLRF contribute their non-Q atoms' multipoles to this water's
-// charge-group centre (gathered in registers, shuffle-reduced once).
+// nbpwlist_lrf :9300-9429).  Eight waters per wavefront, eight lanes per
+// water: each iteration tests the wavefront's 8 waters against 8 partner
+// waters (or solute charge groups), so every partner's coordinates are read
+// once for 8 waters.  Partners inside the cut-off are appended to the water's
+// row in ascending order (ballot over the water's 8 lanes); partners between
+// the cut-off and RcLRF add their non-Q atoms' multipoles about this water's
+// charge-group centre (20 FP64 accumulators per lane, summed over the 8
+// lanes at the end).
 __global__ __launch_bounds__(256) void k_water_lists(const DevSys* __restrict__ sys) {
   const DevSys& S = sys[blockIdx.y];
-  const int lane = threadIdx.x & 63;
-  const int iw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  if (iw >= S.nwat) return;
-  const int io = S.nat_solute + 3 * iw;
-  if (S.flags[io] & F_EXCL) {
-    if (lane == 0) { S.ww_cnt[iw] = 0; S.wp_cnt[iw] = 0; }
-    return;
-  }
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if ((t >> 6) * 8 >= S.nwat) return;   // whole wavefronts past the last water
+  const int lane = t & 63, sub = t & 7, seg = lane & ~7;
+  const int iw = t >> 3;
+  const int io = S.nat_solute + 3 * (iw < S.nwat ? iw : 0);
+  const bool active = iw < S.nwat && !(S.flags[io] & F_EXCL);
   const double xi0 = S.x[3 * io], xi1 = S.x[3 * io + 1], xi2 = S.x[3 * io + 2];
   const int ig = S.iwhich_cgp[io];
   double* Lg = S.lrf + (size_t)ig * kLrfStride;
   const double c0 = Lg[0], c1 = Lg[1], c2 = Lg[2];
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const unsigned long long mine = 0xffull << seg;
   LrfAcc acc;
   acc.zero();
   // waters
   int n_in = 0;
-  int* row = S.ww_nb + (size_t)iw * S.ww_cap;
-  for (int base = 0; base < S.nwat; base += 64) {
-    int jw = base + lane;
+  int* row = S.ww_nb + (size_t)(iw < S.nwat ? iw : 0) * S.ww_cap;
+  for (int base = 0; base < S.nwat; base += 8) {
+    const int jw = base + sub;
     bool inside = false, far = false;
-    if (jw < S.nwat && jw != iw) {
-      int jo = S.nat_solute + 3 * jw;
+    if (active && jw < S.nwat && jw != iw) {
+      const int jo = S.nat_solute + 3 * jw;
       if (!(S.flags[jo] & F_EXCL)) {
-        double a = xi0 - S.x[3 * jo], b = xi1 - S.x[3 * jo + 1], c = xi2 - S.x[3 * jo + 2];
-        double r2 = a * a + b * b + c * c;
+        const double a = xi0 - S.x[3 * jo], b = xi1 - S.x[3 * jo + 1], c = xi2 - S.x[3 * jo + 2];
+        const double r2 = a * a + b * b + c * c;
         inside = r2 <= S.rcww2;
         far = !inside && r2 <= S.rclrf2;
       }
     }
-    unsigned long long m = __ballot(inside);
+    const unsigned long long m = __ballot(inside) & mine;
     if (inside) {
-      int pos = n_in + __popcll(m & ((1ull << lane) - 1ull));
+      const int pos = n_in + __popcll(m & below);
       if (pos < S.ww_cap) row[pos] = jw;
     }
     n_in += __popcll(m);
-    if (far && S.use_lrf) {
-      int jg = S.iwhich_cgp[S.nat_solute + 3 * jw];
-      lrf_gather_group(S, jg, c0, c1, c2, acc);
-    }
+    if (far && S.use_lrf) lrf_gather_group(S, S.iwhich_cgp[S.nat_solute + 3 * jw], c0, c1, c2, acc);
   }
   // solute charge groups (water side of the solute-water list)
   int n_wp = 0;
-  int* wrow = S.wp_nb + (size_t)iw * S.wp_cap;
-  for (int base = 0; base < S.ncgp_solute; base += 64) {
-    int g = base + lane;
+  int* wrow = S.wp_nb + (size_t)(iw < S.nwat ? iw : 0) * S.wp_cap;
+  for (int base = 0; base < S.ncgp_solute; base += 8) {
+    const int g = base + sub;
     bool inside = false, far = false;
-    if (g < S.ncgp_solute) {
-      int sw = S.cgp_sw[g];
+    if (active && g < S.ncgp_solute) {
+      const int sw = S.cgp_sw[g];
       if (!(S.flags[sw] & F_EXCL)) {
         // r2 as the reference forms it: solute switch atom minus water O
-        double a = S.x[3 * sw] - xi0, b = S.x[3 * sw + 1] - xi1, c = S.x[3 * sw + 2] - xi2;
-        double r2 = a * a + b * b + c * c;
+        const double a = S.x[3 * sw] - xi0, b = S.x[3 * sw + 1] - xi1, c = S.x[3 * sw + 2] - xi2;
+        const double r2 = a * a + b * b + c * c;
         inside = r2 <= S.rcpw2;
         far = !inside && r2 <= S.rclrf2;
       }
     }
-    unsigned long long m = __ballot(inside);
+    const unsigned long long m = __ballot(inside) & mine;
     if (inside) {
-      int pos = n_wp + __popcll(m & ((1ull << lane) - 1ull));
+      const int pos = n_wp + __popcll(m & below);
       if (pos < S.wp_cap) wrow[pos] = g;
     }
     n_wp += __popcll(m);
     if (far && S.use_lrf) lrf_gather_group(S, g, c0, c1, c2, acc);
   }
-  acc.wave_reduce();
-  if (lane == 0) {
-    if (S.use_lrf) acc.store(Lg);
-    S.ww_cnt[iw] = n_in < S.ww_cap ? n_in : S.ww_cap;
-    S.wp_cnt[iw] = n_wp < S.wp_cap ? n_wp : S.wp_cap;
-    if (n_in > S.ww_cap || n_wp > S.wp_cap) atomicOr(S.overflow, 1);
+  acc.seg8_reduce();
+  if (iw < S.nwat && sub == 0) {
+    if (!active) {
+      S.ww_cnt[iw] = 0;
+      S.wp_cnt[iw] = 0;
+    } else {
+      if (S.use_lrf) acc.store(Lg);
+      S.ww_cnt[iw] = n_in < S.ww_cap ? n_in : S.ww_cap;
+      S.wp_cnt[iw] = n_wp < S.wp_cap ? n_wp : S.wp_cap;
+      if (n_in > S.ww_cap || n_wp > S.wp_cap) atomicOr(S.overflow, 1);
+    }
   }
 }
 
 // ---------------------------------------------------------------------------
 // Water forces (nonbond_ww_spc src/md.f90:15021-15119, water side of
 // nonbond_pw :13282-13365 and nonbond_qw_spc :14411-14570, lrf_taylor
-// :5824-5891, restrain_solvent :18055-18102).  One wavefront per water:
-// lanes take the partner waters (9 atom pairs each, LJ on O-O only), the
-// listed solute groups, the Q atoms, and the per-atom LRF and restraint
-// terms; the water's 9 gradient components are shuffle-reduced and written
-// once.  P is the pair-arithmetic type: double (plain) or float (mixed, the
-// 9 contributions of one partner are summed in FP32 and accumulated in FP64).
+// :5824-5891, restrain_solvent :18055-18102).  Eight waters per wavefront,
+// eight lanes per water: the lanes of a water take its partner waters (9 atom
+// pairs each, LJ on O-O only), its listed solute groups and the Q atoms in
+// turn, lanes 0-2 add the LRF Taylor term of atoms O, H1, H2 and lane 3 the
+// radial restraint; the water's 9 gradient components are summed over its 8
+// lanes and written once.  P is the pair-arithmetic type: double (plain,
+// reference operation order inside each pair) or float (mixed: FP32
+// coordinates and FMA pair arithmetic, each partner's contribution
+// accumulated in FP64).
 template <typename P>
 __global__ __launch_bounds__(256) void k_water_forces(const DevSys* __restrict__ sys) {
   const DevSys& S = sys[blockIdx.y];
-  const int lane = threadIdx.x & 63;
-  const int iw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  if (iw >= S.nwat) return;
-  const int io = S.nat_solute + 3 * iw;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if ((t >> 6) * 8 >= S.nwat) return;   // whole wavefronts past the last water
+  const int sub = t & 7;
+  const int iw = t >> 3;
+  const bool active = iw < S.nwat;
+  const int io = S.nat_solute + 3 * (active ? iw : 0);
+  const P* X = pair_x<P>(S);
   double g[9];
   for (int k = 0; k < 9; ++k) g[k] = 0.0;
   double e_el = 0.0, e_vdw = 0.0, e_lrf = 0.0, e_rad = 0.0;
-  double xi[9];
-  for (int k = 0; k < 9; ++k) xi[k] = S.x[3 * io + k];
-  const P q[3] = {(P)S.chg_solv[0], (P)S.chg_solv[1], (P)S.chg_solv[2]};
+  P xi[9];
+  for (int k = 0; k < 9; ++k) xi[k] = X[3 * io + k];
+  P qq[9];
+  for (int la = 0; la < 3; ++la)
+    for (int ip = 0; ip < 3; ++ip) qq[3 * la + ip] = (P)(S.chg_solv[la] * S.chg_solv[ip]);
   const P aOO = (P)S.aOO, bOO = (P)S.bOO;
   // water-water
-  const int nn = S.ww_cnt[iw];
+  const int nn = active ? S.ww_cnt[iw] : 0;
   const int* row = S.ww_nb + (size_t)iw * S.ww_cap;
-  for (int k = lane; k < nn; k += 64) {
+  for (int k = sub; k < nn; k += 8) {
     const int jw = row[k];
     const int jo = S.nat_solute + 3 * jw;
-    const bool own = owns_pair(iw, jw);
-    double xj[9];
-    for (int t = 0; t < 9; ++t) xj[t] = S.x[3 * jo + t];
+    P xj[9];
+    for (int u = 0; u < 9; ++u) xj[u] = X[3 * jo + u];
     P el = 0, vdw = 0;
     for (int la = 0; la < 3; ++la) {
       P gl0 = 0, gl1 = 0, gl2 = 0;
       for (int ip = 0; ip < 3; ++ip) {
-        P dx1 = (P)(xj[3 * ip] - xi[3 * la]);
-        P dx2 = (P)(xj[3 * ip + 1] - xi[3 * la + 1]);
-        P dx3 = (P)(xj[3 * ip + 2] - xi[3 * la + 2]);
-        P r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
-        P r6 = r2 * r2 * r2;
+        const P dx1 = xj[3 * ip] - xi[3 * la];
+        const P dx2 = xj[3 * ip + 1] - xi[3 * la + 1];
+        const P dx3 = xj[3 * ip + 2] - xi[3 * la + 2];
+        const P r2sq = pnorm2(dx1, dx2, dx3);
         P r;
-        r2 = inv_r2(r2, r);
-        P Vel = q[la] * q[ip] * r;
+        const P r2 = inv_r2(r2sq, r);
+        const P Vel = qq[3 * la + ip] * r;
         P dv = r2 * (-Vel);
         if (la == 0 && ip == 0) {
-          r6 = (P)1 / r6;
-          P Va = aOO * r6 * r6;
-          P Vb = bOO * r6;
+          P r6;
+          if constexpr (sizeof(P) == 8) r6 = (P)1 / (r2sq * r2sq * r2sq);
+          else r6 = r2 * r2 * r2;
+          const P Va = aOO * r6 * r6;
+          const P Vb = bOO * r6;
           vdw = vdw + Va - Vb;
           dv = dv + r2 * ((P)-12 * Va + (P)6 * Vb);
         }
         el = el + Vel;
-        gl0 = gl0 - dv * dx1;
-        gl1 = gl1 - dv * dx2;
-        gl2 = gl2 - dv * dx3;
+        gl0 = pmsub(dv, dx1, gl0);
+        gl1 = pmsub(dv, dx2, gl1);
+        gl2 = pmsub(dv, dx3, gl2);
       }
       g[3 * la] = g[3 * la] + (double)gl0;
       g[3 * la + 1] = g[3 * la + 1] + (double)gl1;
       g[3 * la + 2] = g[3 * la + 2] + (double)gl2;
     }
-    if (own) {
+    if (owns_pair(iw, jw)) {
       e_el = e_el + (double)el;
       e_vdw = e_vdw + (double)vdw;
     }
   }
-  // water-solute: every non-Q atom of each listed solute group
-  const int nwp = S.wp_cnt[iw];
+  // water-solute: every non-Q atom of each listed solute group (the solute
+  // atom is i of the reference pair)
+  const int nwp = active ? S.wp_cnt[iw] : 0;
   const int* wrow = S.wp_nb + (size_t)iw * S.wp_cap;
-  for (int k = lane; k < nwp; k += 64) {
+  for (int k = sub; k < nwp; k += 8) {
     const int gs = wrow[k];
     for (int a = S.cgp_first[gs]; a <= S.cgp_last[gs]; ++a) {
       const int i = S.cgpatom[a];
       if (S.flags[i] & F_Q) continue;
-      const double xs0 = S.x[3 * i], xs1 = S.x[3 * i + 1], xs2 = S.x[3 * i + 2];
+      const P xs0 = X[3 * i], xs1 = X[3 * i + 1], xs2 = X[3 * i + 2];
       const int ti = S.iac[i];
+      const P qs = (P)S.crg[i];
       for (int ip = 0; ip < 3; ++ip) {
         const int j = io + ip;
         const int tj = S.iac[j];
         const int lj = S.ljcod[ti * S.natyps + tj] - 1;
-        // reference orientation: i = solute atom, j = water atom
-        P dx1 = (P)(xi[3 * ip] - xs0), dx2 = (P)(xi[3 * ip + 1] - xs1), dx3 = (P)(xi[3 * ip + 2] - xs2);
+        const P dx1 = xi[3 * ip] - xs0, dx2 = xi[3 * ip + 1] - xs1, dx3 = xi[3 * ip + 2] - xs2;
         const P aLJ = (P)(S.avdw[ti * 3 + lj] * S.avdw[tj * 3 + lj]);
         const P bLJ = (P)(S.bvdw[ti * 3 + lj] * S.bvdw[tj * 3 + lj]);
-        P r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
         P r;
-        r2 = inv_r2(r2, r);
-        P r6 = r2 * r2 * r2;
-        P r12 = r6 * r6;
-        P Vel = (P)S.crg[i] * (P)S.crg[j] * r;
-        P Va = aLJ * r12;
-        P Vb = bLJ * r6;
-        P dv = r2 * (-Vel - (P)12 * Va + (P)6 * Vb);
+        const P r2 = inv_r2(pnorm2(dx1, dx2, dx3), r);
+        const P r6 = r2 * r2 * r2;
+        const P r12 = r6 * r6;
+        const P Vel = qs * (P)S.crg[j] * r;
+        const P Va = aLJ * r12;
+        const P Vb = bLJ * r6;
+        const P dv = r2 * (-Vel - (P)12 * Va + (P)6 * Vb);
         g[3 * ip] = g[3 * ip] + (double)(dv * dx1);
         g[3 * ip + 1] = g[3 * ip + 1] + (double)(dv * dx2);
         g[3 * ip + 2] = g[3 * ip + 2] + (double)(dv * dx3);
       }
     }
   }
-  // Q-water pairs seen from the water, one Q atom per lane (the Q atom is i
-  // of each pair): O with LJ + Coulomb, H Coulomb only
-  if (S.flags[io] & F_QW) {
+  // Q-water pairs seen from the water, Q atoms taken in turn by the lanes
+  // (the Q atom is i of each pair): O with LJ + Coulomb, H Coulomb only
+  if (active && (S.flags[io] & F_QW)) {
     const int ns = S.nstates;
-    for (int iq = lane; iq < S.nqat; iq += 64) {
+    for (int iq = sub; iq < S.nqat; iq += 8) {
       const int qi = S.iqseq[iq];
-      const double* xq = S.x + 3 * qi;
+      const P xq0 = X[3 * qi], xq1 = X[3 * qi + 1], xq2 = X[3 * qi + 2];
       const int lo = S.qw_lj[iq];
       const int ti = S.iac[qi];
-      P dx1 = (P)(xi[0] - xq[0]), dx2 = (P)(xi[1] - xq[1]), dx3 = (P)(xi[2] - xq[2]);
-      P r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
-      P r6_hc = r2 * r2 * r2;
+      const P dx1 = xi[0] - xq0, dx2 = xi[1] - xq1, dx3 = xi[2] - xq2;
+      const P r2sq = pnorm2(dx1, dx2, dx3);
+      const P r6_hc = r2sq * r2sq * r2sq;
       P r;
-      r2 = inv_r2(r2, r);
-      P r6 = (P)1 / r6_hc;
-      P Va = (P)S.avdw[ti * 3 + lo - 1] * (P)S.alj_O[lo - 1] * r6 * r6;
-      P Vb = (P)S.bvdw[ti * 3 + lo - 1] * (P)S.blj_O[lo - 1] * r6;
+      const P r2 = inv_r2(r2sq, r);
+      const P r6 = (P)1 / r6_hc;
+      const P Va = (P)S.avdw[ti * 3 + lo - 1] * (P)S.alj_O[lo - 1] * r6 * r6;
+      const P Vb = (P)S.bvdw[ti * 3 + lo - 1] * (P)S.blj_O[lo - 1] * r6;
       P dv = 0;
       for (int st = 0; st < ns; ++st) {
-        P Vel = q[0] * (P)S.qcrg[iq * ns + st] * r;
+        const P Vel = (P)S.chg_solv[0] * (P)S.qcrg[iq * ns + st] * r;
         dv = dv + r2 * (-Vel - ((P)12 * Va - (P)6 * Vb) * (r6_hc / r6_hc)) * (P)S.lambda[st];
       }
       g[0] = g[0] + (double)(dv * dx1);
       g[1] = g[1] + (double)(dv * dx2);
       g[2] = g[2] + (double)(dv * dx3);
       for (int ip = 1; ip < 3; ++ip) {
-        P ex1 = (P)(xi[3 * ip] - xq[0]), ex2 = (P)(xi[3 * ip + 1] - xq[1]), ex3 = (P)(xi[3 * ip + 2] - xq[2]);
+        const P ex1 = xi[3 * ip] - xq0, ex2 = xi[3 * ip + 1] - xq1, ex3 = xi[3 * ip + 2] - xq2;
         P rr;
-        P q2 = inv_r2(ex1 * ex1 + ex2 * ex2 + ex3 * ex3, rr);
+        const P q2 = inv_r2(pnorm2(ex1, ex2, ex3), rr);
         P dvh = 0;
         for (int st = 0; st < ns; ++st) {
-          P Vel = q[ip] * (P)S.qcrg[iq * ns + st] * rr;
+          const P Vel = (P)S.chg_solv[ip] * (P)S.qcrg[iq * ns + st] * rr;
           dvh = dvh - q2 * Vel * (P)S.lambda[st];
         }
         g[3 * ip] = g[3 * ip] + (double)(dvh * ex1);
@@ -253,35 +274,36 @@ __global__ __launch_bounds__(256) void k_water_forces(const DevSys* __restrict__
       }
     }
   }
-  // LRF Taylor expansion, one water atom per lane 0..2
-  if (S.use_lrf && lane < 3) {
-    const int i = io + lane;
+  // LRF Taylor expansion (FP64), atom `sub` of the water on lanes 0..2
+  if (active && S.use_lrf && sub < 3) {
+    const int i = io + sub;
     if ((S.flags[i] & (F_EXCL | F_Q)) == 0) {
       const double* L = S.lrf + (size_t)S.iwhich_cgp[io] * kLrfStride;
+      const double xa[3] = {S.x[3 * i], S.x[3 * i + 1], S.x[3 * i + 2]};
       double df[3];
-      const double xa[3] = {xi[3 * lane], xi[3 * lane + 1], xi[3 * lane + 2]};
-      double V = lrf_taylor_atom(L, xa, df);
+      const double V = lrf_taylor_atom(L, xa, df);
       e_lrf = 0.5 * S.crg[i] * V;
       for (int c = 0; c < 3; ++c) {
         const double f = S.crg[i] * df[c];
-        if (lane == 0) g[c] = g[c] - f;
-        else if (lane == 1) g[3 + c] = g[3 + c] - f;
+        if (sub == 0) g[c] = g[c] - f;
+        else if (sub == 1) g[3 + c] = g[3 + c] - f;
         else g[6 + c] = g[6 + c] - f;
       }
     }
   }
-  // radial solvent restraint on the water O (switching atom), lane 3
-  if (lane == 3 && !(S.flags[io] & F_EXCL)) {
-    double shift = S.fk_wsphere != 0.0 ? sqrt(qsys::kBoltz * S.scal[2] / S.fk_wsphere) : 0.0;
-    double dr0 = xi[0] - S.xwcent[0], dr1 = xi[1] - S.xwcent[1], dr2 = xi[2] - S.xwcent[2];
-    double b = sqrt(dr0 * dr0 + dr1 * dr1 + dr2 * dr2);
-    double db = b - (S.rwat - shift);
+  // radial solvent restraint on the water O (switching atom), lane 3 (FP64)
+  if (active && sub == 3 && !(S.flags[io] & F_EXCL)) {
+    const double shift = S.fk_wsphere != 0.0 ? sqrt(qsys::kBoltz * S.scal[2] / S.fk_wsphere) : 0.0;
+    const double dr0 = S.x[3 * io] - S.xwcent[0], dr1 = S.x[3 * io + 1] - S.xwcent[1],
+                 dr2 = S.x[3 * io + 2] - S.xwcent[2];
+    const double b = sqrt(dr0 * dr0 + dr1 * dr1 + dr2 * dr2);
+    const double db = b - (S.rwat - shift);
     double erst, dv;
     if (db > 0) {
       erst = 0.5 * S.fk_wsphere * db * db - S.dwmz;
       dv = S.fk_wsphere * db / b;
     } else if (b > 0.0) {
-      double fexp = exp(S.awmz * db);
+      const double fexp = exp(S.awmz * db);
       erst = S.dwmz * (fexp * fexp - 2.0 * fexp);
       dv = -2.0 * S.dwmz * S.awmz * (fexp - fexp * fexp) / b;
     } else {
@@ -292,12 +314,12 @@ __global__ __launch_bounds__(256) void k_water_forces(const DevSys* __restrict__
     g[1] = g[1] + dv * dr1;
     g[2] = g[2] + dv * dr2;
   }
-  for (int k = 0; k < 9; ++k) g[k] = wave_sum(g[k]);
-  e_el = wave_sum(e_el);
-  e_vdw = wave_sum(e_vdw);
-  e_lrf = wave_sum(e_lrf);
-  e_rad = wave_sum(e_rad);
-  if (lane == 0) {
+  for (int k = 0; k < 9; ++k) g[k] = seg8_sum(g[k]);
+  e_el = seg8_sum(e_el);
+  e_vdw = seg8_sum(e_vdw);
+  e_lrf = seg8_sum(e_lrf);
+  e_rad = seg8_sum(e_rad);
+  if (active && sub == 0) {
     for (int k = 0; k < 9; ++k) S.d[3 * io + k] = g[k];
     double* ew = S.e_water + (size_t)iw * 4;
     ew[0] = e_el; ew[1] = e_vdw; ew[2] = e_lrf; ew[3] = e_rad;
@@ -486,6 +508,7 @@ __global__ __launch_bounds__(256) void k_integrate(const DevSys* __restrict__ sy
       const int ic = 3 * a + c;
       if (S.has_shake) v[c] = (x[c] - xo[c]) / dt;
       S.x[ic] = x[c];
+      S.xf[ic] = (float)x[c];
       S.v[ic] = v[c];
     }
     for (int k = 0; k < 3; ++k) {
@@ -554,6 +577,7 @@ __global__ __launch_bounds__(256) void k_integrate(const DevSys* __restrict__ sy
         const int ic = 3 * i + c;
         if (S.has_shake) S.v[ic] = (S.x[ic] - S.xx[ic]) / dt;
         vv[c] = S.v[ic];
+        S.xf[ic] = (float)S.x[ic];
       }
       const double ek = 0.5 * S.mass[i] * (vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2]);
       const bool ex = (S.flags[i] & F_EXCL) != 0;
@@ -630,11 +654,15 @@ __global__ __launch_bounds__(256) void k_energy_reduce(const DevSys* __restrict_
   for (int c = 0; c < 8; ++c) p[c] = block_sum_256(p[c], red[4 + c]);
   for (int c = 0; c < 6; ++c) b[c] = block_sum_256(b[c], red[c]);
   // per-state Q energies: qq, qp (Q-neighbour pairs of nonbond_qq and the
-  // Q-solute list), qw
+  // Q-solute list), qw; pass 1 forms the FP64 re-evaluated frame of a
+  // mixed-precision batch from e_q64 (Q-Q pairs are FP64 in both)
+  for (int pass = 0; pass < 2; ++pass)
   for (int st = 0; st < S.nstates; ++st) {
+    if (pass == 1 && !(frame >= 0 && S.eseries64)) break;
+    const double* eq = pass == 0 ? S.e_q : S.e_q64;
     double q[6] = {0, 0, 0, 0, 0, 0};
     for (int iq = threadIdx.x; iq < S.nqat; iq += blockDim.x) {
-      const double* e = S.e_q + ((size_t)iq * S.nstates + st) * 4;
+      const double* e = eq + ((size_t)iq * S.nstates + st) * 4;
       q[2] += e[0]; q[3] += e[1]; q[4] += e[2]; q[5] += e[3];
     }
     for (int pr = threadIdx.x; pr < S.nqq; pr += blockDim.x) {
@@ -645,13 +673,13 @@ __global__ __launch_bounds__(256) void k_energy_reduce(const DevSys* __restrict_
     }
     for (int c = 0; c < 6; ++c) {
       q[c] = block_sum_256(q[c], red[c]);
-      if (threadIdx.x == 0) S.qenergies[st * kQTerms + c] = q[c];
+      if (threadIdx.x == 0 && pass == 0) S.qenergies[st * kQTerms + c] = q[c];
     }
     if (threadIdx.x == 0 && frame >= 0 && frame < S.eseries_cap) {
       // put_ene record: total, q(bond angle torsion improper), qx, qq, qp,
       // qw (el, vdw each), restraint (src/nrgy.f90:128-130; EQ totals as
       // pot_energy forms them, src/md.f90:16186-16194)
-      double* r = S.eseries + ((size_t)frame * S.nstates + st) * kEneRecord;
+      double* r = (pass == 0 ? S.eseries : S.eseries64) + ((size_t)frame * S.nstates + st) * kEneRecord;
       const double qx_el = q[0] + q[2] + q[4], qx_vdw = q[1] + q[3] + q[5];
       r[0] = 0.0 + 0.0 + 0.0 + 0.0 + qx_el + qx_vdw + 0.0;
       r[1] = 0.0; r[2] = 0.0; r[3] = 0.0; r[4] = 0.0;

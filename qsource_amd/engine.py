@@ -32,6 +32,7 @@ SYMBOLS = [
     "qgpu_step", "qgpu_get_state", "qgpu_set_state", "qgpu_get_gradient", "qgpu_energies",
     "qgpu_qenergies", "qgpu_get_pairlist", "qgpu_temperature", "qgpu_theta_corr",
     "qgpu_last_step_ms", "qgpu_last_error", "qgpu_set_energy_interval", "qgpu_energy_series",
+    "qgpu_energy_series_fp64",
     "qgpu_profile_kernels", "qgpu_list_counts",
 ]
 
@@ -74,6 +75,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.qgpu_profile_kernels.argtypes = [P, ctypes.c_int, dp, ctypes.c_int]
     lib.qgpu_list_counts.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_long)]
     lib.qgpu_energy_series.argtypes = [P, ctypes.c_int, dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    lib.qgpu_energy_series_fp64.argtypes = [P, ctypes.c_int, dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     lib.qgpu_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
@@ -188,13 +190,16 @@ class Engine:
         self._check(self.lib.qgpu_set_energy_interval(self.h, iene, max_frames))
         self._max_frames = max_frames
 
-    def energy_series(self, k: int = 0) -> np.ndarray:
-        """(nframes, nstates, 14) put_ene records recorded on the device."""
+    def energy_series(self, k: int = 0, fp64: bool = False) -> np.ndarray:
+        """(nframes, nstates, 14) put_ene records recorded on the device.
+        With fp64=True (mixed-precision batches): the same frames with the Q
+        energies re-evaluated with FP64 pair arithmetic."""
         ns = max(1, self.nstates[k])
         cap = max(1, getattr(self, "_max_frames", 0)) * ns * 14
         out = np.zeros(cap)
         n = ctypes.c_int()
-        self._check(self.lib.qgpu_energy_series(self.h, k, _dp(out), cap, ctypes.byref(n)))
+        fn = self.lib.qgpu_energy_series_fp64 if fp64 else self.lib.qgpu_energy_series
+        self._check(fn(self.h, k, _dp(out), cap, ctypes.byref(n)))
         return out[:n.value * ns * 14].reshape(n.value, ns, 14)
 
     def profile_kernels(self, nsteps: int) -> dict:

@@ -150,3 +150,30 @@ def test_energy_series_and_graph(c2_small, monkeypatch):
     # the captured multi-step graph and per-step launches are the same kernels
     assert np.array_equal(runs["1"][0], runs["0"][0])
     assert np.array_equal(runs["1"][1][0], runs["0"][1][0])
+
+
+def test_mixed_frames_fp64_reevaluation(c2_small):
+    """A mixed-precision run records each energy frame twice: with its own
+    FP32 pair arithmetic and re-evaluated in FP64 on the same coordinates.
+    The FP64 frame equals the plain engine's Q energies at those coordinates;
+    the mixed frame differs from it only at FP32 round-off."""
+    _say("case: mixed frames + FP64 re-evaluation, R=14 + 40-atom solute + 8 Q")
+    em = qe.Engine([c2_small["inp"]], qe.QGPU_MIXED)
+    em.set_energy_interval(5, 4)
+    em.step(5)
+    x, v = em.state(0)
+    em.step(1)                       # frame 0 is recorded on these coordinates
+    f32 = em.energy_series(0)
+    f64 = em.energy_series(0, fp64=True)
+    assert f32.shape == f64.shape and f32.shape[0] == 1
+    ep = qe.Engine([c2_small["inp"]], qe.QGPU_FP64)
+    ep.set_state(0, x, v)
+    ep.make_pair_lists()
+    ep.pot_energy()
+    q = np.array(ep.qenergies(0)).reshape(-1, 7)
+    # put_ene columns 7..12 are qq el/vdw, qp el/vdw, qw el/vdw
+    assert np.allclose(f64[0, :, 7:13], q[:, :6], rtol=1e-12, atol=1e-10), (f64[0, :, 7:13], q[:, :6])
+    d = np.abs(f32[0, :, 7:13] - f64[0, :, 7:13])
+    assert d.max() <= 1e-4 * max(1.0, np.abs(f64[0, :, 7:13]).max()), d
+    em.close()
+    ep.close()
