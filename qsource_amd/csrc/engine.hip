@@ -20,14 +20,14 @@ namespace qgpu {
 // kernels (kernels_water.hip, kernels_solute.hip)
 __global__ void k_cgp_centers(const DevSys*);
 __global__ void k_x_to_f32(const DevSys*);
-__global__ void k_water_lists(const DevSys*);
+template <typename P> __global__ void k_water_lists(const DevSys*);
 template <typename P> __global__ void k_water_forces(const DevSys*);
 __global__ void k_watpol_theta(const DevSys*);
 __global__ void k_watpol_force(const DevSys*);
 __global__ void k_integrate(const DevSys*);
 __global__ void k_temperature(const DevSys*, int);
 __global__ void k_energy_reduce(const DevSys*, int);
-__global__ void k_solute_lists(const DevSys*);
+template <typename P> __global__ void k_solute_lists(const DevSys*);
 template <typename P> __global__ void k_solute_forces(const DevSys*);
 template <typename P, int NS> __global__ void k_q_forces(const DevSys*, int);
 __global__ void k_q_reduce(const DevSys*, int);
@@ -513,11 +513,17 @@ void Batch::launch_lists() {
   { pb(0); hipLaunchKernelGGL(k_cgp_centers, g0, dim3(256), 0, stream_, d_sys_); pe(0); }
   if (max_nwat_ > 0) {
     dim3 g1((max_nwat_ * 8 + 255) / 256, ni);
-    { pb(1); hipLaunchKernelGGL(k_water_lists, g1, dim3(256), 0, stream_, d_sys_); pe(1); }
+    pb(1);
+    if (precision_ == 0) hipLaunchKernelGGL(k_water_lists<double>, g1, dim3(256), 0, stream_, d_sys_);
+    else hipLaunchKernelGGL(k_water_lists<float>, g1, dim3(256), 0, stream_, d_sys_);
+    pe(1);
   }
   if (max_ncgp_solute_ > 0) {
     dim3 g2((max_ncgp_solute_ * 64 + 255) / 256, ni);
-    { pb(2); hipLaunchKernelGGL(k_solute_lists, g2, dim3(256), 0, stream_, d_sys_); pe(2); }
+    pb(2);
+    if (precision_ == 0) hipLaunchKernelGGL(k_solute_lists<double>, g2, dim3(256), 0, stream_, d_sys_);
+    else hipLaunchKernelGGL(k_solute_lists<float>, g2, dim3(256), 0, stream_, d_sys_);
+    pe(2);
   }
 }
 

@@ -96,39 +96,61 @@ __device__ __forceinline__ bool owns_pair(int a, int b) {
 // LRF multipole accumulator for one charge-group centre: phi0, phi1[3] and
 // the distinct components of the symmetric phi2 (6) and phi3 (10) tensors,
 // accumulated in FP64 as lrf_update (src/md.f90:5896-5981) forms them.
-struct LrfAcc {
-  double p0, p1[3], p2[6], p3[10];
+// Unique multipole components of charges about a centre (lrf_update,
+// src/md.f90:5927-5977): phi0 = sum q/r, phi1 = -sum q d/r^3,
+// phi2_ab = sum (3 q d_a d_b/r^5 - q delta_ab/r^3),
+// phi3_abc = sum -3 q (5 d_a d_b d_c - r^2 (delta_ab d_c + delta_ac d_b +
+// delta_bc d_a))/r^7, with d = atom - centre; p2 holds (00 01 02 11 12 22),
+// p3 holds (000 001 002 011 012 022 111 112 122 222).
+template <typename T>
+struct LrfTerms {
+  T p0, p1[3], p2[6], p3[10];
   __device__ void zero() {
     p0 = 0; for (int k = 0; k < 3; ++k) p1[k] = 0;
     for (int k = 0; k < 6; ++k) p2[k] = 0;
     for (int k = 0; k < 10; ++k) p3[k] = 0;
   }
-  __device__ void add(double q, double d0, double d1, double d2) {
-    const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-    const double ri2 = 1.0 / r2;
-    const double ri = sqrt(ri2);
-    const double f0 = q * ri * ri2;      // q / r^3
-    const double f1 = 3.0 * f0 * ri2;    // 3 q / r^5
-    const double f2 = -f1 * ri2;         // -3 q / r^7
+  __device__ void add(T q, T d0, T d1, T d2) {
+#pragma clang fp contract(fast)
+    const T r2 = d0 * d0 + d1 * d1 + d2 * d2;
+    T ri;
+    const T ri2 = inv_r2(r2, ri);
+    const T f0 = q * ri * ri2;          // q / r^3
+    const T f1 = (T)3 * f0 * ri2;       // 3 q / r^5
+    const T t = (T)-5 * f1 * ri2;       // 5 * (-3 q / r^7); -3q/r^7 * r^2 = -f1
     p0 = p0 + q * ri;
     p1[0] = p1[0] - f0 * d0; p1[1] = p1[1] - f0 * d1; p1[2] = p1[2] - f0 * d2;
-    p2[0] = p2[0] + f1 * d0 * d0 - f0;
-    p2[1] = p2[1] + f1 * d0 * d1;
-    p2[2] = p2[2] + f1 * d0 * d2;
-    p2[3] = p2[3] + f1 * d1 * d1 - f0;
-    p2[4] = p2[4] + f1 * d1 * d2;
-    p2[5] = p2[5] + f1 * d2 * d2 - f0;
-    p3[0] = p3[0] + f2 * (5.0 * d0 * d0 * d0 - r2 * 3.0 * d0);   // 111
-    p3[1] = p3[1] + f2 * (5.0 * d0 * d0 * d1 - r2 * d1);         // 112
-    p3[2] = p3[2] + f2 * (5.0 * d0 * d0 * d2 - r2 * d2);         // 113
-    p3[3] = p3[3] + f2 * (5.0 * d0 * d1 * d1 - r2 * d0);         // 122
-    p3[4] = p3[4] + f2 * (5.0 * d0 * d1 * d2);                   // 123
-    p3[5] = p3[5] + f2 * (5.0 * d0 * d2 * d2 - r2 * d0);         // 133
-    p3[6] = p3[6] + f2 * (5.0 * d1 * d1 * d1 - r2 * 3.0 * d1);   // 222
-    p3[7] = p3[7] + f2 * (5.0 * d1 * d1 * d2 - r2 * d2);         // 223
-    p3[8] = p3[8] + f2 * (5.0 * d1 * d2 * d2 - r2 * d1);         // 233
-    p3[9] = p3[9] + f2 * (5.0 * d2 * d2 * d2 - r2 * 3.0 * d2);   // 333
+    const T d00 = d0 * d0, d01 = d0 * d1, d02 = d0 * d2, d11 = d1 * d1, d12 = d1 * d2, d22 = d2 * d2;
+    p2[0] = p2[0] + (f1 * d00 - f0);
+    p2[1] = p2[1] + f1 * d01;
+    p2[2] = p2[2] + f1 * d02;
+    p2[3] = p2[3] + (f1 * d11 - f0);
+    p2[4] = p2[4] + f1 * d12;
+    p2[5] = p2[5] + (f1 * d22 - f0);
+    const T t00 = t * d00, t01 = t * d01, t11 = t * d11, t22 = t * d22;
+    p3[0] = p3[0] + (t00 * d0 + (T)3 * f1 * d0);   // 000
+    p3[1] = p3[1] + (t00 * d1 + f1 * d1);          // 001
+    p3[2] = p3[2] + (t00 * d2 + f1 * d2);          // 002
+    p3[3] = p3[3] + (t11 * d0 + f1 * d0);          // 011
+    p3[4] = p3[4] + t01 * d2;                      // 012
+    p3[5] = p3[5] + (t22 * d0 + f1 * d0);          // 022
+    p3[6] = p3[6] + (t11 * d1 + (T)3 * f1 * d1);   // 111
+    p3[7] = p3[7] + (t11 * d2 + f1 * d2);          // 112
+    p3[8] = p3[8] + (t22 * d1 + f1 * d1);          // 122
+    p3[9] = p3[9] + (t22 * d2 + (T)3 * f1 * d2);   // 222
   }
+};
+
+// FP64 accumulator of the 20 unique components
+struct LrfAcc : LrfTerms<double> {
+  // add a partner group's components computed in FP32
+  __device__ void add(const LrfTerms<float>& g) {
+    p0 = p0 + (double)g.p0;
+    for (int k = 0; k < 3; ++k) p1[k] = p1[k] + (double)g.p1[k];
+    for (int k = 0; k < 6; ++k) p2[k] = p2[k] + (double)g.p2[k];
+    for (int k = 0; k < 10; ++k) p3[k] = p3[k] + (double)g.p3[k];
+  }
+  using LrfTerms<double>::add;
   // sum over an aligned group of 8 lanes
   __device__ void seg8_reduce() {
     p0 = seg8_sum(p0);
@@ -157,12 +179,26 @@ struct LrfAcc {
 };
 
 // Add the non-Q atoms of charge group g (as seen from centre c) to acc.
+template <typename P>
 __device__ __forceinline__ void lrf_gather_group(const DevSys& S, int g, double c0, double c1,
                                                  double c2, LrfAcc& acc) {
-  for (int a = S.cgp_first[g]; a <= S.cgp_last[g]; ++a) {
-    const int i = S.cgpatom[a];
-    if (S.flags[i] & F_Q) continue;
-    acc.add(S.crg[i], S.x[3 * i] - c0, S.x[3 * i + 1] - c1, S.x[3 * i + 2] - c2);
+  if constexpr (sizeof(P) == 8) {
+    for (int a = S.cgp_first[g]; a <= S.cgp_last[g]; ++a) {
+      const int i = S.cgpatom[a];
+      if (S.flags[i] & F_Q) continue;
+      acc.add(S.crg[i], S.x[3 * i] - c0, S.x[3 * i + 1] - c1, S.x[3 * i + 2] - c2);
+    }
+  } else {
+    // mixed precision: the group's terms in FP32, accumulated in FP64
+    LrfTerms<float> part;
+    part.zero();
+    for (int a = S.cgp_first[g]; a <= S.cgp_last[g]; ++a) {
+      const int i = S.cgpatom[a];
+      if (S.flags[i] & F_Q) continue;
+      part.add((float)S.crg[i], (float)(S.x[3 * i] - c0), (float)(S.x[3 * i + 1] - c1),
+               (float)(S.x[3 * i + 2] - c2));
+    }
+    acc.add(part);
   }
 }
 

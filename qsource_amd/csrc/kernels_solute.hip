@@ -58,7 +58,9 @@ __device__ ulonglong2 pair_masks(const DevSys& S, int g, int jg) {
 
 // ---------------------------------------------------------------------------
 // Solute lists (nbpplist_lrf src/md.f90:7998-8188 and the solute side of
-// nbpwlist_lrf :9300-9429): one wavefront per solute charge group.
+// nbpwlist_lrf :9300-9429): one wavefront per solute charge group; P as in
+// k_water_lists (precision of the LRF multipole terms).
+template <typename P>
 __global__ __launch_bounds__(256) void k_solute_lists(const DevSys* __restrict__ sys) {
   const DevSys& S = sys[blockIdx.y];
   const int lane = threadIdx.x & 63;
@@ -97,7 +99,7 @@ __global__ __launch_bounds__(256) void k_solute_lists(const DevSys* __restrict__
       }
     }
     n_pp += __popcll(m);
-    if (far && S.use_lrf) lrf_gather_group(S, jg, c0, c1, c2, acc);
+    if (far && S.use_lrf) lrf_gather_group<P>(S, jg, c0, c1, c2, acc);
   }
   int n_pw = 0;
   int* wrow = S.pw_nb + (size_t)g * S.pw_cap;
@@ -117,7 +119,7 @@ __global__ __launch_bounds__(256) void k_solute_lists(const DevSys* __restrict__
       if (pos < S.pw_cap) wrow[pos] = jw;
     }
     n_pw += __popcll(m);
-    if (far && S.use_lrf) lrf_gather_group(S, S.iwhich_cgp[jo], c0, c1, c2, acc);
+    if (far && S.use_lrf) lrf_gather_group<P>(S, S.iwhich_cgp[jo], c0, c1, c2, acc);
   }
   acc.wave_reduce();
   if (lane == 0) {
@@ -127,6 +129,8 @@ __global__ __launch_bounds__(256) void k_solute_lists(const DevSys* __restrict__
     if (n_pp > S.pp_cap || n_pw > S.pw_cap) atomicOr(S.overflow, 1);
   }
 }
+template __global__ void k_solute_lists<double>(const DevSys*);
+template __global__ void k_solute_lists<float>(const DevSys*);
 
 // ---------------------------------------------------------------------------
 // Solute forces: one wavefront per solute atom (nonbond_pp :13069-13155,

@@ -49,7 +49,10 @@ __global__ void k_x_to_f32(const DevSys* __restrict__ sys) {
 // row in ascending order (ballot over the water's 8 lanes); partners between
 // the cut-off and RcLRF add their non-Q atoms' multipoles about this water's
 // charge-group centre (20 FP64 accumulators per lane, summed over the 8
-// lanes at the end).
+// lanes at the end).  P = float evaluates each partner group's multipole
+// terms in FP32 before adding them to the FP64 sums (mixed precision); list
+// membership is decided in FP64 in both.
+template <typename P>
 __global__ __launch_bounds__(256) void k_water_lists(const DevSys* __restrict__ sys) {
   const DevSys& S = sys[blockIdx.y];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -87,7 +90,7 @@ __global__ __launch_bounds__(256) void k_water_lists(const DevSys* __restrict__ 
       if (pos < S.ww_cap) row[pos] = jw;
     }
     n_in += __popcll(m);
-    if (far && S.use_lrf) lrf_gather_group(S, S.iwhich_cgp[S.nat_solute + 3 * jw], c0, c1, c2, acc);
+    if (far && S.use_lrf) lrf_gather_group<P>(S, S.iwhich_cgp[S.nat_solute + 3 * jw], c0, c1, c2, acc);
   }
   // solute charge groups (water side of the solute-water list)
   int n_wp = 0;
@@ -111,7 +114,7 @@ __global__ __launch_bounds__(256) void k_water_lists(const DevSys* __restrict__ 
       if (pos < S.wp_cap) wrow[pos] = g;
     }
     n_wp += __popcll(m);
-    if (far && S.use_lrf) lrf_gather_group(S, g, c0, c1, c2, acc);
+    if (far && S.use_lrf) lrf_gather_group<P>(S, g, c0, c1, c2, acc);
   }
   acc.seg8_reduce();
   if (iw < S.nwat && sub == 0) {
@@ -126,6 +129,8 @@ __global__ __launch_bounds__(256) void k_water_lists(const DevSys* __restrict__ 
     }
   }
 }
+template __global__ void k_water_lists<double>(const DevSys*);
+template __global__ void k_water_lists<float>(const DevSys*);
 
 // ---------------------------------------------------------------------------
 // Water forces (nonbond_ww_spc src/md.f90:15021-15119, water side of
@@ -154,9 +159,13 @@ __global__ __launch_bounds__(256) void k_water_forces(const DevSys* __restrict__
   double e_el = 0.0, e_vdw = 0.0, e_lrf = 0.0, e_rad = 0.0;
   P xi[9];
   for (int k = 0; k < 9; ++k) xi[k] = X[3 * io + k];
+  // charge products formed in P from the P charges: with q_O = -2 q_H exact
+  // in either precision, all nine products share one rounding, so the large
+  // O-O / O-H / H-H sums still cancel as they do in FP64
+  const P qs3[3] = {(P)S.chg_solv[0], (P)S.chg_solv[1], (P)S.chg_solv[2]};
   P qq[9];
   for (int la = 0; la < 3; ++la)
-    for (int ip = 0; ip < 3; ++ip) qq[3 * la + ip] = (P)(S.chg_solv[la] * S.chg_solv[ip]);
+    for (int ip = 0; ip < 3; ++ip) qq[3 * la + ip] = qs3[la] * qs3[ip];
   const P aOO = (P)S.aOO, bOO = (P)S.bOO;
   // water-water
   const int nn = active ? S.ww_cnt[iw] : 0;

@@ -93,6 +93,13 @@ def test_mixed_precision_c1(c1):
     rms = np.sqrt(np.mean(do ** 2))
     err = np.sqrt(np.mean((dg - do) ** 2))
     assert err <= 1e-5 * rms, (err, rms)
+    # FP32 pair terms and FP32 LRF multipole terms, FP64 sums: per-term
+    # energies close to the FP64 oracle (the ww sums cancel ~1e6 kcal/mol of
+    # per-type terms down to ~50, so they get an absolute bound)
+    eo, eg = o.energies(), e.energies(0)
+    print({t: eg[t] - eo[t] for t in ("ww_el", "ww_vdw", "lrf", "rst_radial")})
+    for t, tol in (("ww_el", 2e-3), ("ww_vdw", 2e-3), ("lrf", 1e-5), ("rst_radial", 1e-6)):
+        assert abs(eg[t] - eo[t]) <= tol * max(1.0, abs(eo[t])), (t, eg[t], eo[t])
     e.step(100)
     o.run(100)
     pg, po = e.energies(0)["potential"], o.energies()["potential"]
