@@ -302,6 +302,18 @@ void Batch::upload(int k) {
     int lj = s.ljcod[s.iac[o] * s.natyps + s.iac[o]];
     D.aOO = s.alj_solv[0][lj - 1] * s.alj_solv[0][lj - 1];
     D.bOO = s.blj_solv[0][lj - 1] * s.blj_solv[0][lj - 1];
+    // standard waters: one charge group of (O, H1, H2) in this order with the
+    // solvent charges and no Q atom; the LRF gather then reads them directly
+    bool std_w = true;
+    for (int w = 0; w < s.nwat && std_w; ++w) {
+      const int a = s.nat_solute + 3 * w;
+      const auto& g = t.cgp[s.iwhich_cgp[a]];
+      std_w = g.last - g.first == 2;
+      for (int k = 0; k < 3 && std_w; ++k)
+        std_w = t.cgpatom[g.first + k] == a + k && s.iwhich_cgp[a + k] == s.iwhich_cgp[a] &&
+                !s.iqatom[a + k] && s.crg[a + k] == s.chg_solv[k];
+    }
+    D.std_water = std_w ? 1 : 0;
   }
   D.rwat = s.rwat; D.fk_wsphere = s.fk_wsphere; D.dwmz = s.dwmz; D.awmz = s.awmz; D.fkwpol = s.fkwpol;
   for (int c = 0; c < 3; ++c) D.xwcent[c] = t.xwcent[c];
@@ -537,7 +549,7 @@ void Batch::launch_forces() {
       { pb(3); hipLaunchKernelGGL(k_water_forces<float>, g1, dim3(256), 0, stream_, d_sys_); pe(3); }
   }
   if (max_ncgp_solute_ > 0) {
-    dim3 gs((max_nat_solute_ * 64 + 255) / 256, ni);
+    dim3 gs((max_nat_solute_ * 8 + 255) / 256, ni);
     if (precision_ == 0)
       { pb(4); hipLaunchKernelGGL(k_solute_forces<double>, gs, dim3(256), 0, stream_, d_sys_); pe(4); }
     else

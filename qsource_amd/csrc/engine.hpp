@@ -94,6 +94,7 @@ struct DevSys {
   bool use_lrf;
   // solvent
   double chg_solv[3];
+  int std_water;                 // every water is one charge group (O, H1, H2) with chg_solv, no Q atom
   double aOO, bOO;               // alj_solv(1,iLJ)^2, blj_solv(1,iLJ)^2 for iLJ=ljcod(O,O)
   double rwat, fk_wsphere, dwmz, awmz, fkwpol;
   double xwcent[3];

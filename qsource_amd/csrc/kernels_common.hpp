@@ -93,9 +93,6 @@ __device__ __forceinline__ bool owns_pair(int a, int b) {
   return (a < b) == (((a + b) & 1) == 0);
 }
 
-// LRF multipole accumulator for one charge-group centre: phi0, phi1[3] and
-// the distinct components of the symmetric phi2 (6) and phi3 (10) tensors,
-// accumulated in FP64 as lrf_update (src/md.f90:5896-5981) forms them.
 // Unique multipole components of charges about a centre (lrf_update,
 // src/md.f90:5927-5977): phi0 = sum q/r, phi1 = -sum q d/r^3,
 // phi2_ab = sum (3 q d_a d_b/r^5 - q delta_ab/r^3),
@@ -198,6 +195,26 @@ __device__ __forceinline__ void lrf_gather_group(const DevSys& S, int g, double 
       part.add((float)S.crg[i], (float)(S.x[3 * i] - c0), (float)(S.x[3 * i + 1] - c1),
                (float)(S.x[3 * i + 2] - c2));
     }
+    acc.add(part);
+  }
+}
+
+// lrf_gather_group for a standard water (its charge group is its own three
+// atoms jo, jo+1, jo+2 with the solvent charges, none of them a Q atom):
+// the same terms in the same order, without the group/atom indirection.
+template <typename P>
+__device__ __forceinline__ void lrf_gather_water(const DevSys& S, int jo, double c0, double c1,
+                                                 double c2, LrfAcc& acc) {
+  const double* xj = S.x + 3 * jo;
+  if constexpr (sizeof(P) == 8) {
+    for (int k = 0; k < 3; ++k)
+      acc.add(S.chg_solv[k], xj[3 * k] - c0, xj[3 * k + 1] - c1, xj[3 * k + 2] - c2);
+  } else {
+    LrfTerms<float> part;
+    part.zero();
+    for (int k = 0; k < 3; ++k)
+      part.add((float)S.chg_solv[k], (float)(xj[3 * k] - c0), (float)(xj[3 * k + 1] - c1),
+               (float)(xj[3 * k + 2] - c2));
     acc.add(part);
   }
 }

@@ -119,7 +119,10 @@ __global__ __launch_bounds__(256) void k_solute_lists(const DevSys* __restrict__
       if (pos < S.pw_cap) wrow[pos] = jw;
     }
     n_pw += __popcll(m);
-    if (far && S.use_lrf) lrf_gather_group<P>(S, S.iwhich_cgp[jo], c0, c1, c2, acc);
+    if (far && S.use_lrf) {
+      if (S.std_water) lrf_gather_water<P>(S, jo, c0, c1, c2, acc);
+      else lrf_gather_group<P>(S, S.iwhich_cgp[jo], c0, c1, c2, acc);
+    }
   }
   acc.wave_reduce();
   if (lane == 0) {
@@ -133,34 +136,39 @@ template __global__ void k_solute_lists<double>(const DevSys*);
 template __global__ void k_solute_lists<float>(const DevSys*);
 
 // ---------------------------------------------------------------------------
-// Solute forces: one wavefront per solute atom (nonbond_pp :13069-13155,
-// nonbond_pw :13282-13365, lrf_taylor, fix_shell :2816-2862, and the
-// solute-atom side of nonbond_qp :13788-13921).  Lanes take the neighbour
-// groups of the atom's charge group (pair status from the list-time masks),
-// the listed waters, and the Q atoms; lane 1 adds the LRF Taylor term and
-// lane 2 the shell/fix restraint; the atom's gradient is shuffle-reduced and
-// written once.  A pair's energy is counted by the atom that is i of the
-// reference pair: the owner group's atom, or the lower index in one group.
+// Solute forces (nonbond_pp :13069-13155, nonbond_pw :13282-13365,
+// lrf_taylor, fix_shell :2816-2862, and the solute-atom side of nonbond_qp
+// :13788-13921).  Eight solute atoms per wavefront, eight lanes per atom: the
+// lanes of an atom take the neighbour groups of its charge group (pair status
+// from the list-time masks), the listed waters and the Q atoms in turn; lane
+// 1 adds the LRF Taylor term and lane 2 the shell/fix restraint; the atom's
+// gradient is summed over its 8 lanes and written once.  A pair's energy is
+// counted by the atom that is i of the reference pair: the owner group's
+// atom, or the lower index within one group.
 template <typename P>
 __global__ __launch_bounds__(256) void k_solute_forces(const DevSys* __restrict__ sys) {
   const DevSys& S = sys[blockIdx.y];
-  const int lane = threadIdx.x & 63;
-  const int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  if (i >= S.nat_solute) return;
-  const int g = S.iwhich_cgp[i];
-  const int a = S.cgp_slot[i];
-  const unsigned char fl = S.flags[i];
-  const double xi0 = S.x[3 * i], xi1 = S.x[3 * i + 1], xi2 = S.x[3 * i + 2];
-  const int ti = S.iac[i];
-  const P qi = (P)S.crg[i];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if ((t >> 6) * 8 >= S.nat_solute) return;   // whole wavefronts past the last atom
+  const int sub = t & 7;
+  const int i = t >> 3;
+  const bool active = i < S.nat_solute;
+  const int ia = active ? i : 0;
+  const int g = S.iwhich_cgp[ia];
+  const int a = S.cgp_slot[ia];
+  const unsigned char fl = S.flags[ia];
+  const P* X = pair_x<P>(S);
+  const P xi0 = X[3 * ia], xi1 = X[3 * ia + 1], xi2 = X[3 * ia + 2];
+  const int ti = S.iac[ia];
+  const P qi = (P)S.crg[ia];
   double g0 = 0, g1 = 0, g2 = 0;
   double e_ppel = 0, e_ppvdw = 0, e_pwel = 0, e_pwvdw = 0, e_lrf = 0, e_fix = 0, e_shell = 0;
-  if (!(fl & F_Q)) {
+  if (active && !(fl & F_Q)) {
     // solute-solute
     const int npp = S.pp_cnt[g];
     const int* prow = S.pp_nb + (size_t)g * S.pp_cap;
     const ulonglong2* mrow = S.pp_mask + (size_t)g * S.pp_cap;
-    for (int k = lane; k < npp; k += 64) {
+    for (int k = sub; k < npp; k += 8) {
       const int jg = prow[k];
       const ulonglong2 mk = mrow[k];
       unsigned todo = (unsigned)(~mk.x >> (a * kMaxGroupAtoms)) & 0xffu;
@@ -178,11 +186,11 @@ __global__ __launch_bounds__(256) void k_solute_forces(const DevSys* __restrict_
         const bool me_first = same ? (i < j) : own;
         // dx = x_j - x_i; the reference pair (ia, ja) has dx of the same sign
         // when i is first, and the opposite sign otherwise
-        P dx1 = (P)(S.x[3 * j] - xi0), dx2 = (P)(S.x[3 * j + 1] - xi1), dx3 = (P)(S.x[3 * j + 2] - xi2);
+        P dx1 = X[3 * j] - xi0, dx2 = X[3 * j + 1] - xi1, dx3 = X[3 * j + 2] - xi2;
         if (!me_first) { dx1 = -dx1; dx2 = -dx2; dx3 = -dx3; }
         P r;
-        P r2 = inv_r2(dx1 * dx1 + dx2 * dx2 + dx3 * dx3, r);
-        P r6 = r2 * r2 * r2;
+        const P r2 = inv_r2(pnorm2(dx1, dx2, dx3), r);
+        const P r6 = r2 * r2 * r2;
         P Vel = qi * (P)S.crg[j] * r;
         if (lj == 3) Vel = Vel * (P)S.el14_scale;
         const P Va = r6 * r6 * (P)S.avdw[ti * 3 + l] * (P)S.avdw[tj * 3 + l];
@@ -204,24 +212,24 @@ __global__ __launch_bounds__(256) void k_solute_forces(const DevSys* __restrict_
     // solute-water (the solute atom is i of the reference pair)
     const int npw = S.pw_cnt[g];
     const int* wrow = S.pw_nb + (size_t)g * S.pw_cap;
-    for (int k = lane; k < npw; k += 64) {
+    for (int k = sub; k < npw; k += 8) {
       const int jo = S.nat_solute + 3 * wrow[k];
       for (int ip = 0; ip < 3; ++ip) {
         const int j = jo + ip;
         const int tj = S.iac[j];
         const int lj = S.ljcod[ti * S.natyps + tj] - 1;
-        if (zero_interaction(S, i, j, lj)) continue;
+        if (zero_interaction(S, ia, j, lj)) continue;
         const P aLJ = (P)(S.avdw[ti * 3 + lj] * S.avdw[tj * 3 + lj]);
         const P bLJ = (P)(S.bvdw[ti * 3 + lj] * S.bvdw[tj * 3 + lj]);
-        P dx1 = (P)(S.x[3 * j] - xi0), dx2 = (P)(S.x[3 * j + 1] - xi1), dx3 = (P)(S.x[3 * j + 2] - xi2);
+        const P dx1 = X[3 * j] - xi0, dx2 = X[3 * j + 1] - xi1, dx3 = X[3 * j + 2] - xi2;
         P r;
-        P r2 = inv_r2(dx1 * dx1 + dx2 * dx2 + dx3 * dx3, r);
-        P r6 = r2 * r2 * r2;
-        P r12 = r6 * r6;
-        P Vel = qi * (P)S.crg[j] * r;
-        P Va = aLJ * r12;
-        P Vb = bLJ * r6;
-        P dv = r2 * (-Vel - (P)12 * Va + (P)6 * Vb);
+        const P r2 = inv_r2(pnorm2(dx1, dx2, dx3), r);
+        const P r6 = r2 * r2 * r2;
+        const P r12 = r6 * r6;
+        const P Vel = qi * (P)S.crg[j] * r;
+        const P Va = aLJ * r12;
+        const P Vb = bLJ * r6;
+        const P dv = r2 * (-Vel - (P)12 * Va + (P)6 * Vb);
         g0 = g0 - (double)(dv * dx1);
         g1 = g1 - (double)(dv * dx2);
         g2 = g2 - (double)(dv * dx3);
@@ -231,33 +239,33 @@ __global__ __launch_bounds__(256) void k_solute_forces(const DevSys* __restrict_
     }
     // Q-solute pairs seen from the solute atom (the Q atom is i of the pair)
     if (fl & F_QP) {
-      for (int iq = lane; iq < S.nqat; iq += 64) {
+      for (int iq = sub; iq < S.nqat; iq += 8) {
         const int qa = S.iqseq[iq];
         const int lj = S.qp_lj[(size_t)i * S.nqat + iq];
         const int l = lj - 1;
         const int tq = S.iac[qa];
-        P dx1 = (P)(xi0 - S.x[3 * qa]), dx2 = (P)(xi1 - S.x[3 * qa + 1]), dx3 = (P)(xi2 - S.x[3 * qa + 2]);
+        const P dx1 = xi0 - X[3 * qa], dx2 = xi1 - X[3 * qa + 1], dx3 = xi2 - X[3 * qa + 2];
         P r;
-        P r2 = inv_r2(dx1 * dx1 + dx2 * dx2 + dx3 * dx3, r);
-        P r6 = r2 * r2 * r2;
+        const P r2 = inv_r2(pnorm2(dx1, dx2, dx3), r);
+        const P r6 = r2 * r2 * r2;
         const P aLJ = (P)(S.avdw[tq * 3 + l] * S.avdw[ti * 3 + l]);
         const P bLJ = (P)(S.bvdw[tq * 3 + l] * S.bvdw[ti * 3 + l]);
         for (int st = 0; st < S.nstates; ++st) {
           P Vel = (P)S.qcrg[iq * S.nstates + st] * qi * r;
           if (lj == 3) Vel = Vel * (P)S.el14_scale;
-          P Va = aLJ * r6 * r6;
-          P Vb = bLJ * r6;
-          P dv = r2 * (-Vel - (P)12 * Va + (P)6 * Vb) * (P)S.lambda[st];
+          const P Va = aLJ * r6 * r6;
+          const P Vb = bLJ * r6;
+          const P dv = r2 * (-Vel - (P)12 * Va + (P)6 * Vb) * (P)S.lambda[st];
           g0 = g0 + (double)(dv * dx1);
           g1 = g1 + (double)(dv * dx2);
           g2 = g2 + (double)(dv * dx3);
         }
       }
     }
-    // LRF Taylor expansion about the group centre
-    if (lane == 1 && S.use_lrf && !(fl & F_EXCL)) {
+    // LRF Taylor expansion about the group centre (FP64)
+    if (sub == 1 && S.use_lrf && !(fl & F_EXCL)) {
       const double* L = S.lrf + (size_t)g * kLrfStride;
-      const double xa[3] = {xi0, xi1, xi2};
+      const double xa[3] = {S.x[3 * i], S.x[3 * i + 1], S.x[3 * i + 2]};
       double df[3];
       const double V = lrf_taylor_atom(L, xa, df);
       e_lrf = 0.5 * S.crg[i] * V;
@@ -267,9 +275,10 @@ __global__ __launch_bounds__(256) void k_solute_forces(const DevSys* __restrict_
     }
   }
   // harmonic restraint of excluded and shell atoms to the topology position
-  if (lane == 2 && (fl & (F_EXCL | F_SHELL))) {
+  if (active && sub == 2 && (fl & (F_EXCL | F_SHELL))) {
     const double fk = (fl & F_EXCL) ? S.fk_fix : S.fk_pshell;
-    const double dr0 = xi0 - S.xtop[3 * i], dr1 = xi1 - S.xtop[3 * i + 1], dr2 = xi2 - S.xtop[3 * i + 2];
+    const double dr0 = S.x[3 * i] - S.xtop[3 * i], dr1 = S.x[3 * i + 1] - S.xtop[3 * i + 1],
+                 dr2 = S.x[3 * i + 2] - S.xtop[3 * i + 2];
     const double erst = 0.5 * fk * (dr0 * dr0 + dr1 * dr1 + dr2 * dr2);
     if (fl & F_EXCL) e_fix = erst;
     if (fl & F_SHELL) e_shell = erst;
@@ -277,11 +286,11 @@ __global__ __launch_bounds__(256) void k_solute_forces(const DevSys* __restrict_
     g1 = g1 + fk * dr1;
     g2 = g2 + fk * dr2;
   }
-  g0 = wave_sum(g0); g1 = wave_sum(g1); g2 = wave_sum(g2);
-  e_ppel = wave_sum(e_ppel); e_ppvdw = wave_sum(e_ppvdw);
-  e_pwel = wave_sum(e_pwel); e_pwvdw = wave_sum(e_pwvdw);
-  e_lrf = wave_sum(e_lrf); e_fix = wave_sum(e_fix); e_shell = wave_sum(e_shell);
-  if (lane == 0) {
+  g0 = seg8_sum(g0); g1 = seg8_sum(g1); g2 = seg8_sum(g2);
+  e_ppel = seg8_sum(e_ppel); e_ppvdw = seg8_sum(e_ppvdw);
+  e_pwel = seg8_sum(e_pwel); e_pwvdw = seg8_sum(e_pwvdw);
+  e_lrf = seg8_sum(e_lrf); e_fix = seg8_sum(e_fix); e_shell = seg8_sum(e_shell);
+  if (active && sub == 0) {
     S.d[3 * i] = g0; S.d[3 * i + 1] = g1; S.d[3 * i + 2] = g2;
     double* es = S.e_solute + (size_t)i * 8;
     es[0] = e_ppel; es[1] = e_ppvdw; es[2] = e_pwel; es[3] = e_pwvdw;

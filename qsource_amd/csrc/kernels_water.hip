@@ -90,7 +90,11 @@ __global__ __launch_bounds__(256) void k_water_lists(const DevSys* __restrict__ 
       if (pos < S.ww_cap) row[pos] = jw;
     }
     n_in += __popcll(m);
-    if (far && S.use_lrf) lrf_gather_group<P>(S, S.iwhich_cgp[S.nat_solute + 3 * jw], c0, c1, c2, acc);
+    if (far && S.use_lrf) {
+      const int jo = S.nat_solute + 3 * jw;
+      if (S.std_water) lrf_gather_water<P>(S, jo, c0, c1, c2, acc);
+      else lrf_gather_group<P>(S, S.iwhich_cgp[jo], c0, c1, c2, acc);
+    }
   }
   // solute charge groups (water side of the solute-water list)
   int n_wp = 0;
