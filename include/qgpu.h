@@ -135,7 +135,8 @@ int qgpu_theta_corr(qgpu_handle* h, int k, double* out, int* n);
 /* Runs nsteps steps (no graph) with a HIP event pair around every kernel
  * launch on the engine's stream and writes the average ms per launch of
  * each kernel, in this order: cgp_centers, water_lists, solute_lists,
- * water_forces, solute_forces, bonded, q_forces, qq_forces, watpol_theta,
+ * water_forces, solute_forces, terms (bonded terms and Q-Q pairs), q_forces,
+ * term_gather, watpol_theta,
  * watpol_force, energy_record, integrate, temperature (0 if not launched),
  * less the measured time of an empty event pair on the same stream.
  * A measurement aid with no counterpart in the reference (its -DPROFILING

@@ -31,8 +31,8 @@ template <typename P> __global__ void k_solute_lists(const DevSys*);
 template <typename P> __global__ void k_solute_forces(const DevSys*);
 template <typename P, int NS> __global__ void k_q_forces(const DevSys*, int);
 __global__ void k_q_reduce(const DevSys*, int);
-__global__ void k_qq_forces(const DevSys*);
-__global__ void k_bonded(const DevSys*);
+__global__ void k_terms(const DevSys*);
+__global__ void k_term_gather(const DevSys*);
 
 #define HIPCHECK(expr)                                                            \
   do {                                                                            \
@@ -331,6 +331,7 @@ void Batch::upload(int k) {
   D.iqseq = I->upload(s.iqseq);
   D.qcrg = I->upload(s.qcrg);
   D.lambda = I->upload(s.lambda);
+  std::vector<int> qq_atom_i, qq_atom_j;   // atoms of each Q pair, for the term slots
   {
     const int nq = s.nqat, nsl = s.nat_solute;
     auto qconn = [&](int st, int iq, int a) { return s.qconn[((size_t)st * nq + iq) * nsl + a]; };
@@ -395,23 +396,8 @@ void Batch::upload(int k) {
     D.qq_state = I->upload(qst); D.qq_iq = I->upload(qiq); D.qq_j = I->upload(qj);
     D.qq_jq = I->upload(qjq); D.qq_lj = I->upload(qlj); D.qq_elscale = I->upload(qel);
     D.e_qq = I->alloc<double>((size_t)std::max(1, D.nqq) * 2);
-    std::vector<std::vector<int>> inc(N);
-    for (int p = 0; p < D.nqq; ++p) {
-      inc[s.iqseq[qiq[p]]].push_back(p);
-      inc[qj[p]].push_back(p);
-    }
-    std::vector<int> qa, qf, qpairs;
-    for (int i = 0; i < N; ++i)
-      if (!inc[i].empty()) {
-        qa.push_back(i);
-        qf.push_back((int)qpairs.size());
-        qpairs.insert(qpairs.end(), inc[i].begin(), inc[i].end());
-      }
-    qf.push_back((int)qpairs.size());
-    D.nqq_atoms = (int)qa.size();
-    D.qq_atom = I->upload(qa); D.qq_first = I->upload(qf); D.qq_pair = I->upload(qpairs);
+    for (int p = 0; p < D.nqq; ++p) { qq_atom_i.push_back(s.iqseq[qiq[p]]); qq_atom_j.push_back(qj[p]); }
     max_nqat_ = std::max(max_nqat_, nq);
-    max_nqq_atoms_ = std::max(max_nqq_atoms_, D.nqq_atoms);
   }
   // bonded terms after shrink_topology, with atom incidence
   {
@@ -428,23 +414,30 @@ void Batch::upload(int k) {
     D.nbnd = (int)s.bnd.size(); D.nbnd_solute = s.nbonds_solute;
     D.nang = (int)s.ang.size(); D.nang_solute = s.nangles_solute;
     D.ntor = (int)s.tor.size(); D.ntor_solute = s.ntors_solute;
+    // term slots (bonds, angles, torsions, then the Q pairs) and the CSR of
+    // every atom's slots in slot order
+    D.e_bterm = I->alloc<double>((size_t)std::max(1, D.nbnd + D.nang + D.ntor));
     std::vector<std::vector<int>> inc(N);
-    for (int n = 0; n < D.nbnd; ++n) for (int a : {s.bnd[n].i, s.bnd[n].j}) inc[a].push_back((0 << 28) | n);
-    for (int n = 0; n < D.nang; ++n) for (int a : {s.ang[n].i, s.ang[n].j, s.ang[n].k}) inc[a].push_back((1 << 28) | n);
+    int slot = 0;
+    for (int n = 0; n < D.nbnd; ++n) for (int a : {s.bnd[n].i, s.bnd[n].j}) inc[a].push_back(slot++);
+    for (int n = 0; n < D.nang; ++n) for (int a : {s.ang[n].i, s.ang[n].j, s.ang[n].k}) inc[a].push_back(slot++);
     for (int n = 0; n < D.ntor; ++n)
-      for (int a : {s.tor[n].i, s.tor[n].j, s.tor[n].k, s.tor[n].l}) inc[a].push_back((2 << 28) | n);
-    std::vector<int> ba, bf, bt;
+      for (int a : {s.tor[n].i, s.tor[n].j, s.tor[n].k, s.tor[n].l}) inc[a].push_back(slot++);
+    for (int p = 0; p < D.nqq; ++p)
+      for (int a : {qq_atom_i[p], qq_atom_j[p]}) inc[a].push_back(slot++);
+    std::vector<int> sa, sf, sl;
     for (int i = 0; i < N; ++i)
       if (!inc[i].empty()) {
-        ba.push_back(i);
-        bf.push_back((int)bt.size());
-        bt.insert(bt.end(), inc[i].begin(), inc[i].end());
+        sa.push_back(i);
+        sf.push_back((int)sl.size());
+        sl.insert(sl.end(), inc[i].begin(), inc[i].end());
       }
-    bf.push_back((int)bt.size());
-    D.nbonded_atoms = (int)ba.size();
-    D.bd_atom = I->upload(ba); D.bd_first = I->upload(bf); D.bd_term = I->upload(bt);
-    D.e_bonded = I->alloc<double>((size_t)std::max(1, D.nbonded_atoms) * 6);
-    max_nbonded_atoms_ = std::max(max_nbonded_atoms_, D.nbonded_atoms);
+    sf.push_back((int)sl.size());
+    D.slot_g = I->alloc<double>((size_t)std::max(1, slot) * 3);
+    D.nslot_atoms = (int)sa.size();
+    D.slot_atom = I->upload(sa); D.slot_first = I->upload(sf); D.slot_list = I->upload(sl);
+    max_terms_ = std::max(max_terms_, D.nbnd + D.nang + D.ntor + D.nqq);
+    max_slot_atoms_ = std::max(max_slot_atoms_, D.nslot_atoms);
   }
   // integrator
   D.dt = s.dt; D.temp0 = s.inp.temp0; D.tau_t = s.tau_t;
@@ -555,8 +548,8 @@ void Batch::launch_forces() {
     else
       { pb(4); hipLaunchKernelGGL(k_solute_forces<float>, gs, dim3(256), 0, stream_, d_sys_); pe(4); }
   }
-  if (max_nbonded_atoms_ > 0)
-    { pb(5); hipLaunchKernelGGL(k_bonded, dim3((max_nbonded_atoms_ + 255) / 256, ni), dim3(256), 0, stream_, d_sys_); pe(5); }
+  if (max_terms_ > 0)
+    { pb(5); hipLaunchKernelGGL(k_terms, dim3((max_terms_ + 255) / 256, ni), dim3(256), 0, stream_, d_sys_); pe(5); }
   if (max_nqat_ > 0) {
     dim3 gq((max_nqat_ * kQChunks * 64 + 255) / 256, ni);
     auto q_fp64 = [&](int shadow) {
@@ -583,8 +576,8 @@ void Batch::launch_forces() {
       hipLaunchKernelGGL(k_q_reduce, gr, dim3(64), 0, stream_, d_sys_, 1);
     }
   }
-  if (max_nqq_atoms_ > 0)
-    { pb(7); hipLaunchKernelGGL(k_qq_forces, dim3((max_nqq_atoms_ + 255) / 256, ni), dim3(256), 0, stream_, d_sys_); pe(7); }
+  if (max_slot_atoms_ > 0)
+    { pb(7); hipLaunchKernelGGL(k_term_gather, dim3((max_slot_atoms_ + 255) / 256, ni), dim3(256), 0, stream_, d_sys_); pe(7); }
   if (max_nwat_ > 0) {
     dim3 g2((max_nwat_ + 255) / 256, ni);
     { pb(8); hipLaunchKernelGGL(k_watpol_theta, g2, dim3(256), 0, stream_, d_sys_); pe(8); }

@@ -124,8 +124,6 @@ struct DevSys {
   const int *qq_state, *qq_iq, *qq_j, *qq_jq, *qq_lj;
   const double* qq_elscale;
   double* e_qq;                  // nqq x 2: el, vdw (of the pair)
-  int nqq_atoms;
-  const int *qq_atom, *qq_first, *qq_pair;   // CSR: atom -> its pairs
   // bonded terms (after shrink_topology) and per-atom incidence
   const int* bnd;                // nbnd x 3 (i, j, cod)
   const int* ang;                // nang x 4
@@ -134,9 +132,13 @@ struct DevSys {
   const double* anglib;          // fk, ang0
   const double* torlib;          // fk, rmult, deltor, paths
   int nbnd, nbnd_solute, nang, nang_solute, ntor, ntor_solute;
-  int nbonded_atoms;
-  const int *bd_atom, *bd_first, *bd_term;    // CSR: atom -> (type<<28 | term index)
-  double* e_bonded;              // nbonded_atoms x 6: p_bond p_angle p_torsion w_bond w_angle w_torsion
+  double* e_bterm;               // energy per bonded term (bonds, angles, torsions)
+  // gradient slots of the term kernel: 3 FP64 per (term, atom of the term),
+  // bonds 2, angles 3, torsions 4, Q pairs 2 slots, consecutive per term;
+  // CSR atom -> its slots, summed in this order
+  double* slot_g;
+  int nslot_atoms;
+  const int *slot_atom, *slot_first, *slot_list;
   // integrator
   double dt;
   double temp0, tau_t;
@@ -245,7 +247,7 @@ class Batch {
   int precision_ = 0;
   int max_nwat_ = 0, max_ncgp_ = 0, max_ncgp_solute_ = 0, max_units_ = 0, max_natom_ = 0;
   int max_nat_solute_ = 0, max_nstates_ = 0;
-  int max_nqat_ = 0, max_nqq_atoms_ = 0, max_nbonded_atoms_ = 0;
+  int max_nqat_ = 0, max_terms_ = 0, max_slot_atoms_ = 0;
 };
 
 }  // namespace qgpu

@@ -2,9 +2,10 @@
 // charge-group lists with the LRF gather, solute nonbonded forces (solute-
 // solute, solute-water, LRF Taylor, shell restraints and the solute side of
 // the Q-solute terms), the Q-atom kernel (Q-solute and Q-water terms seen
-// from the Q atom, per-state energies), the Q-Q / Q-neighbour kernel and the
-// classical bonded kernel.  All are owner-computes: each atom's gradient is
-// written by one lane, so results are independent of scheduling.
+// from the Q atom, per-state energies), and the term kernel for bonded terms
+// and Q-Q / Q-neighbour pairs with its fixed-order gather.  Every atom's
+// gradient is written by one lane or summed in a fixed order, so results
+// are independent of scheduling.
 #include "kernels_common.hpp"
 
 namespace qgpu {
@@ -444,19 +445,122 @@ __global__ __launch_bounds__(64) void k_q_reduce(const DevSys* __restrict__ sys,
 }
 
 // ---------------------------------------------------------------------------
-// Q-Q and Q-neighbour pairs (nonbond_qq :13491-13636): one thread per atom
-// that takes part in any pair; each thread evaluates its pairs in the
-// reference orientation (Q atom iq first) and keeps its own gradient; the
-// pair energy is stored by the Q-atom side.  FP64 in both precisions (the
-// list holds at most a few thousand pairs).
-__global__ __launch_bounds__(256) void k_qq_forces(const DevSys* __restrict__ sys) {
+// Bonded terms (bond :1332-1405, angle :1130-1246, torsion :17907-18052) and
+// Q-Q / Q-neighbour pairs (nonbond_qq :13491-13636), one thread per term:
+// the term's energy is stored per term and the gradient contribution of
+// each of its atoms into that atom's slot (slots of term n are consecutive:
+// bonds i j, angles i j k, torsions i j k l, Q pairs i j); k_term_gather
+// then adds every atom's slots to d in a fixed order.  FP64 in both
+// precisions.
+__device__ __forceinline__ void put_slot(const DevSys& S, int slot, double g0, double g1, double g2) {
+  double* o = S.slot_g + 3 * (size_t)slot;
+  o[0] = g0; o[1] = g1; o[2] = g2;
+}
+
+__global__ __launch_bounds__(256) void k_terms(const DevSys* __restrict__ sys) {
   const DevSys& S = sys[blockIdx.y];
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= S.nqq_atoms) return;
-  const int me = S.qq_atom[t];
-  double g0 = 0, g1 = 0, g2 = 0;
-  for (int k = S.qq_first[t]; k < S.qq_first[t + 1]; ++k) {
-    const int p = S.qq_pair[k];
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const double* x = S.x;
+  if (n < S.nbnd) {
+    const int i = S.bnd[3 * n], j = S.bnd[3 * n + 1], cod = S.bnd[3 * n + 2] - 1;
+    const double fk = S.bondlib[2 * cod], b0 = S.bondlib[2 * cod + 1];
+    const double r0 = x[3 * j] - x[3 * i], r1 = x[3 * j + 1] - x[3 * i + 1], r2 = x[3 * j + 2] - x[3 * i + 2];
+    const double b = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
+    const double db = b - b0;
+    const double dv = fk * db / b;
+    put_slot(S, 2 * n, -r0 * dv, -r1 * dv, -r2 * dv);
+    put_slot(S, 2 * n + 1, r0 * dv, r1 * dv, r2 * dv);
+    S.e_bterm[n] = 0.5 * fk * (db * db);
+    return;
+  }
+  n -= S.nbnd;
+  if (n < S.nang) {
+    const int i = S.ang[4 * n], j = S.ang[4 * n + 1], kk = S.ang[4 * n + 2], cod = S.ang[4 * n + 3] - 1;
+    const double fk = S.anglib[2 * cod], a0 = S.anglib[2 * cod + 1];
+    double rji[3], rjk[3];
+    for (int c = 0; c < 3; ++c) { rji[c] = x[3 * i + c] - x[3 * j + c]; rjk[c] = x[3 * kk + c] - x[3 * j + c]; }
+    const double bji2inv = 1.0 / (rji[0] * rji[0] + rji[1] * rji[1] + rji[2] * rji[2]);
+    const double bjk2inv = 1.0 / (rjk[0] * rjk[0] + rjk[1] * rjk[1] + rjk[2] * rjk[2]);
+    const double bjiinv = sqrt(bji2inv), bjkinv = sqrt(bjk2inv);
+    double scp = (rji[0] * rjk[0] + rji[1] * rjk[1] + rji[2] * rjk[2]);
+    scp = scp * bjiinv * bjkinv;
+    if (scp > 1.0) scp = 1.0;
+    else if (scp < -1.0) scp = -1.0;
+    const double angv = acos(scp);
+    const double da = angv - a0;
+    const double dv = fk * da;
+    double f1 = sin(angv);
+    if (fabs(f1) < qsys::kQrealEps) f1 = qsys::kQrealEps;
+    f1 = -1.0 / f1;
+    double di[3], dk[3];
+    for (int c = 0; c < 3; ++c) {
+      di[c] = f1 * (rjk[c] * bjiinv * bjkinv - scp * rji[c] * bji2inv);
+      dk[c] = f1 * (rji[c] * bjiinv * bjkinv - scp * rjk[c] * bjk2inv);
+    }
+    const int s0 = 2 * S.nbnd + 3 * n;
+    put_slot(S, s0, dv * di[0], dv * di[1], dv * di[2]);
+    put_slot(S, s0 + 1, -dv * (di[0] + dk[0]), -dv * (di[1] + dk[1]), -dv * (di[2] + dk[2]));
+    put_slot(S, s0 + 2, dv * dk[0], dv * dk[1], dv * dk[2]);
+    S.e_bterm[S.nbnd + n] = 0.5 * fk * (da * da);
+    return;
+  }
+  n -= S.nang;
+  if (n < S.ntor) {
+    const int* T = S.tor + 5 * n;
+    const int i = T[0], j = T[1], kk = T[2], l = T[3], cod = T[4] - 1;
+    const double* L = S.torlib + 4 * cod;
+    double rji[3], rjk[3], rkl[3], rnj[3], rnk[3];
+    for (int c = 0; c < 3; ++c) {
+      rji[c] = x[3 * i + c] - x[3 * j + c];
+      rjk[c] = x[3 * kk + c] - x[3 * j + c];
+      rkl[c] = x[3 * l + c] - x[3 * kk + c];
+    }
+    rnj[0] = rji[1] * rjk[2] - rji[2] * rjk[1];
+    rnj[1] = rji[2] * rjk[0] - rji[0] * rjk[2];
+    rnj[2] = rji[0] * rjk[1] - rji[1] * rjk[0];
+    rnk[0] = -rjk[1] * rkl[2] + rjk[2] * rkl[1];
+    rnk[1] = -rjk[2] * rkl[0] + rjk[0] * rkl[2];
+    rnk[2] = -rjk[0] * rkl[1] + rjk[1] * rkl[0];
+    const double bj2inv = 1.0 / (rnj[0] * rnj[0] + rnj[1] * rnj[1] + rnj[2] * rnj[2]);
+    const double bk2inv = 1.0 / (rnk[0] * rnk[0] + rnk[1] * rnk[1] + rnk[2] * rnk[2]);
+    const double bjinv = sqrt(bj2inv), bkinv = sqrt(bk2inv);
+    double scp = (rnj[0] * rnk[0] + rnj[1] * rnk[1] + rnj[2] * rnk[2]) * (bjinv * bkinv);
+    double phi;
+    if (scp > 1.0) { scp = 1.0; phi = acos(1.0); }
+    else if (scp < -1.0) { scp = -1.0; phi = acos(-1.0); }
+    else phi = acos(scp);
+    if (rjk[0] * (rnj[1] * rnk[2] - rnj[2] * rnk[1]) + rjk[1] * (rnj[2] * rnk[0] - rnj[0] * rnk[2]) +
+            rjk[2] * (rnj[0] * rnk[1] - rnj[1] * rnk[0]) < 0)
+      phi = -phi;
+    const double arg = L[1] * phi - L[2];
+    const double dv = -L[1] * L[0] * sin(arg) * L[3];
+    double f1 = sin(phi);
+    if (fabs(f1) < qsys::kQrealEps) f1 = qsys::kQrealEps;
+    f1 = -1.0 / f1;
+    double di[3], dl[3], rki[3], rlj[3];
+    for (int c = 0; c < 3; ++c) {
+      di[c] = f1 * (rnk[c] * (bjinv * bkinv) - scp * rnj[c] * bj2inv);
+      dl[c] = f1 * (rnj[c] * (bjinv * bkinv) - scp * rnk[c] * bk2inv);
+      rki[c] = rji[c] - rjk[c];
+      rlj[c] = -rjk[c] - rkl[c];
+    }
+    const int s0 = 2 * S.nbnd + 3 * S.nang + 4 * n;
+    put_slot(S, s0, dv * (rjk[1] * di[2] - rjk[2] * di[1]), dv * (rjk[2] * di[0] - rjk[0] * di[2]),
+             dv * (rjk[0] * di[1] - rjk[1] * di[0]));
+    put_slot(S, s0 + 1, dv * (rki[1] * di[2] - rki[2] * di[1] + rkl[1] * dl[2] - rkl[2] * dl[1]),
+             dv * (rki[2] * di[0] - rki[0] * di[2] + rkl[2] * dl[0] - rkl[0] * dl[2]),
+             dv * (rki[0] * di[1] - rki[1] * di[0] + rkl[0] * dl[1] - rkl[1] * dl[0]));
+    put_slot(S, s0 + 2, dv * (rlj[1] * dl[2] - rlj[2] * dl[1] - rji[1] * di[2] + rji[2] * di[1]),
+             dv * (rlj[2] * dl[0] - rlj[0] * dl[2] - rji[2] * di[0] + rji[0] * di[2]),
+             dv * (rlj[0] * dl[1] - rlj[1] * dl[0] - rji[0] * di[1] + rji[1] * di[0]));
+    put_slot(S, s0 + 3, dv * (rjk[1] * dl[2] - rjk[2] * dl[1]), dv * (rjk[2] * dl[0] - rjk[0] * dl[2]),
+             dv * (rjk[0] * dl[1] - rjk[1] * dl[0]));
+    S.e_bterm[S.nbnd + S.nang + n] = L[0] * (1.0 + cos(arg)) * L[3];
+    return;
+  }
+  n -= S.ntor;
+  if (n < S.nqq) {
+    const int p = n;
     const int st = S.qq_state[p], iq = S.qq_iq[p], j = S.qq_j[p], jq = S.qq_jq[p], lj = S.qq_lj[p];
     const int i = S.iqseq[iq];
     const int l = lj - 1;
@@ -464,8 +568,7 @@ __global__ __launch_bounds__(256) void k_qq_forces(const DevSys* __restrict__ sy
     const double aLJ = S.avdw[S.iac[i] * 3 + l] * S.avdw[S.iac[j] * 3 + l];
     const double bLJ = S.bvdw[S.iac[i] * 3 + l] * S.bvdw[S.iac[j] * 3 + l];
     const double qj = jq >= 0 ? S.qcrg[jq * S.nstates + st] : S.crg[j];
-    const double dx1 = S.x[3 * j] - S.x[3 * i], dx2 = S.x[3 * j + 1] - S.x[3 * i + 1],
-                 dx3 = S.x[3 * j + 2] - S.x[3 * i + 2];
+    const double dx1 = x[3 * j] - x[3 * i], dx2 = x[3 * j + 1] - x[3 * i + 1], dx3 = x[3 * j + 2] - x[3 * i + 2];
     double r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
     const double r6_hc = r2 * r2 * r2;
     double r6 = r2 * r2 * r2 + 0.0;
@@ -477,138 +580,28 @@ __global__ __launch_bounds__(256) void k_qq_forces(const DevSys* __restrict__ sy
     if (lj == 3) Vel = Vel * S.el14_scale;
     const double Va = aLJ * r12, Vb = bLJ * r6;
     const double dv = r2 * (-Vel - (12.0 * Va - 6.0 * Vb) * r6_hc * r6) * S.lambda[st];
-    if (me == i) {
-      g0 = g0 - dv * dx1; g1 = g1 - dv * dx2; g2 = g2 - dv * dx3;
-      S.e_qq[2 * p] = Vel;
-      S.e_qq[2 * p + 1] = Va - Vb;
-    } else {
-      g0 = g0 + dv * dx1; g1 = g1 + dv * dx2; g2 = g2 + dv * dx3;
-    }
+    const int s0 = 2 * S.nbnd + 3 * S.nang + 4 * S.ntor + 2 * p;
+    put_slot(S, s0, -dv * dx1, -dv * dx2, -dv * dx3);
+    put_slot(S, s0 + 1, dv * dx1, dv * dx2, dv * dx3);
+    S.e_qq[2 * p] = Vel;
+    S.e_qq[2 * p + 1] = Va - Vb;
+  }
+}
+
+// Adds each atom's term slots to its gradient, in slot order.
+__global__ __launch_bounds__(256) void k_term_gather(const DevSys* __restrict__ sys) {
+  const DevSys& S = sys[blockIdx.y];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= S.nslot_atoms) return;
+  const int me = S.slot_atom[t];
+  double g0 = 0, g1 = 0, g2 = 0;
+  for (int k = S.slot_first[t]; k < S.slot_first[t + 1]; ++k) {
+    const double* o = S.slot_g + 3 * (size_t)S.slot_list[k];
+    g0 = g0 + o[0]; g1 = g1 + o[1]; g2 = g2 + o[2];
   }
   S.d[3 * me] = S.d[3 * me] + g0;
   S.d[3 * me + 1] = S.d[3 * me + 1] + g1;
   S.d[3 * me + 2] = S.d[3 * me + 2] + g2;
-}
-
-// ---------------------------------------------------------------------------
-// Classical bonded terms (bond :1332-1405, angle :1130-1246, torsion
-// :17907-18052): one thread per atom that takes part in a term; every term
-// is evaluated in full by each of its atoms and the atom keeps its own
-// share; the term energy is stored by its first atom.
-__global__ __launch_bounds__(256) void k_bonded(const DevSys* __restrict__ sys) {
-  const DevSys& S = sys[blockIdx.y];
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= S.nbonded_atoms) return;
-  const int me = S.bd_atom[t];
-  double g[3] = {0, 0, 0};
-  double e[6] = {0, 0, 0, 0, 0, 0};
-  const double* x = S.x;
-  for (int k = S.bd_first[t]; k < S.bd_first[t + 1]; ++k) {
-    const int code = S.bd_term[k];
-    const int type = code >> 28, n = code & 0x0fffffff;
-    if (type == 0) {
-      const int i = S.bnd[3 * n], j = S.bnd[3 * n + 1], cod = S.bnd[3 * n + 2] - 1;
-      const double fk = S.bondlib[2 * cod], b0 = S.bondlib[2 * cod + 1];
-      double r0 = x[3 * j] - x[3 * i], r1 = x[3 * j + 1] - x[3 * i + 1], r2 = x[3 * j + 2] - x[3 * i + 2];
-      double b = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
-      double db = b - b0;
-      double dv = fk * db / b;
-      if (me == j) { g[0] += r0 * dv; g[1] += r1 * dv; g[2] += r2 * dv; }
-      else { g[0] -= r0 * dv; g[1] -= r1 * dv; g[2] -= r2 * dv; }
-      if (me == i) e[n < S.nbnd_solute ? 0 : 3] += 0.5 * fk * (db * db);
-    } else if (type == 1) {
-      const int i = S.ang[4 * n], j = S.ang[4 * n + 1], kk = S.ang[4 * n + 2], cod = S.ang[4 * n + 3] - 1;
-      const double fk = S.anglib[2 * cod], a0 = S.anglib[2 * cod + 1];
-      double rji[3], rjk[3];
-      for (int c = 0; c < 3; ++c) { rji[c] = x[3 * i + c] - x[3 * j + c]; rjk[c] = x[3 * kk + c] - x[3 * j + c]; }
-      double bji2inv = 1.0 / (rji[0] * rji[0] + rji[1] * rji[1] + rji[2] * rji[2]);
-      double bjk2inv = 1.0 / (rjk[0] * rjk[0] + rjk[1] * rjk[1] + rjk[2] * rjk[2]);
-      double bjiinv = sqrt(bji2inv), bjkinv = sqrt(bjk2inv);
-      double scp = (rji[0] * rjk[0] + rji[1] * rjk[1] + rji[2] * rjk[2]);
-      scp = scp * bjiinv * bjkinv;
-      if (scp > 1.0) scp = 1.0;
-      else if (scp < -1.0) scp = -1.0;
-      double angv = acos(scp);
-      double da = angv - a0;
-      double dv = fk * da;
-      double f1 = sin(angv);
-      if (fabs(f1) < qsys::kQrealEps) f1 = qsys::kQrealEps;
-      f1 = -1.0 / f1;
-      double di[3], dk[3];
-      for (int c = 0; c < 3; ++c) {
-        di[c] = f1 * (rjk[c] * bjiinv * bjkinv - scp * rji[c] * bji2inv);
-        dk[c] = f1 * (rji[c] * bjiinv * bjkinv - scp * rjk[c] * bjk2inv);
-      }
-      if (me == i) for (int c = 0; c < 3; ++c) g[c] += dv * di[c];
-      else if (me == kk) for (int c = 0; c < 3; ++c) g[c] += dv * dk[c];
-      else for (int c = 0; c < 3; ++c) g[c] -= dv * (di[c] + dk[c]);
-      if (me == i) e[n < S.nang_solute ? 1 : 4] += 0.5 * fk * (da * da);
-    } else {
-      const int* T = S.tor + 5 * n;
-      const int i = T[0], j = T[1], kk = T[2], l = T[3], cod = T[4] - 1;
-      const double* L = S.torlib + 4 * cod;
-      double rji[3], rjk[3], rkl[3], rnj[3], rnk[3];
-      for (int c = 0; c < 3; ++c) {
-        rji[c] = x[3 * i + c] - x[3 * j + c];
-        rjk[c] = x[3 * kk + c] - x[3 * j + c];
-        rkl[c] = x[3 * l + c] - x[3 * kk + c];
-      }
-      rnj[0] = rji[1] * rjk[2] - rji[2] * rjk[1];
-      rnj[1] = rji[2] * rjk[0] - rji[0] * rjk[2];
-      rnj[2] = rji[0] * rjk[1] - rji[1] * rjk[0];
-      rnk[0] = -rjk[1] * rkl[2] + rjk[2] * rkl[1];
-      rnk[1] = -rjk[2] * rkl[0] + rjk[0] * rkl[2];
-      rnk[2] = -rjk[0] * rkl[1] + rjk[1] * rkl[0];
-      double bj2inv = 1.0 / (rnj[0] * rnj[0] + rnj[1] * rnj[1] + rnj[2] * rnj[2]);
-      double bk2inv = 1.0 / (rnk[0] * rnk[0] + rnk[1] * rnk[1] + rnk[2] * rnk[2]);
-      double bjinv = sqrt(bj2inv), bkinv = sqrt(bk2inv);
-      double scp = (rnj[0] * rnk[0] + rnj[1] * rnk[1] + rnj[2] * rnk[2]) * (bjinv * bkinv);
-      double phi;
-      if (scp > 1.0) { scp = 1.0; phi = acos(1.0); }
-      else if (scp < -1.0) { scp = -1.0; phi = acos(-1.0); }
-      else phi = acos(scp);
-      if (rjk[0] * (rnj[1] * rnk[2] - rnj[2] * rnk[1]) + rjk[1] * (rnj[2] * rnk[0] - rnj[0] * rnk[2]) +
-              rjk[2] * (rnj[0] * rnk[1] - rnj[1] * rnk[0]) < 0)
-        phi = -phi;
-      double arg = L[1] * phi - L[2];
-      double dv = -L[1] * L[0] * sin(arg) * L[3];
-      double f1 = sin(phi);
-      if (fabs(f1) < qsys::kQrealEps) f1 = qsys::kQrealEps;
-      f1 = -1.0 / f1;
-      double di[3], dl[3], rki[3], rlj[3];
-      for (int c = 0; c < 3; ++c) {
-        di[c] = f1 * (rnk[c] * (bjinv * bkinv) - scp * rnj[c] * bj2inv);
-        dl[c] = f1 * (rnj[c] * (bjinv * bkinv) - scp * rnk[c] * bk2inv);
-        rki[c] = rji[c] - rjk[c];
-        rlj[c] = -rjk[c] - rkl[c];
-      }
-      double dp[3];
-      if (me == i) {
-        dp[0] = rjk[1] * di[2] - rjk[2] * di[1];
-        dp[1] = rjk[2] * di[0] - rjk[0] * di[2];
-        dp[2] = rjk[0] * di[1] - rjk[1] * di[0];
-      } else if (me == j) {
-        dp[0] = rki[1] * di[2] - rki[2] * di[1] + rkl[1] * dl[2] - rkl[2] * dl[1];
-        dp[1] = rki[2] * di[0] - rki[0] * di[2] + rkl[2] * dl[0] - rkl[0] * dl[2];
-        dp[2] = rki[0] * di[1] - rki[1] * di[0] + rkl[0] * dl[1] - rkl[1] * dl[0];
-      } else if (me == kk) {
-        dp[0] = rlj[1] * dl[2] - rlj[2] * dl[1] - rji[1] * di[2] + rji[2] * di[1];
-        dp[1] = rlj[2] * dl[0] - rlj[0] * dl[2] - rji[2] * di[0] + rji[0] * di[2];
-        dp[2] = rlj[0] * dl[1] - rlj[1] * dl[0] - rji[0] * di[1] + rji[1] * di[0];
-      } else {
-        dp[0] = rjk[1] * dl[2] - rjk[2] * dl[1];
-        dp[1] = rjk[2] * dl[0] - rjk[0] * dl[2];
-        dp[2] = rjk[0] * dl[1] - rjk[1] * dl[0];
-      }
-      for (int c = 0; c < 3; ++c) g[c] += dv * dp[c];
-      if (me == i) e[n < S.ntor_solute ? 2 : 5] += L[0] * (1.0 + cos(arg)) * L[3];
-    }
-  }
-  S.d[3 * me] = S.d[3 * me] + g[0];
-  S.d[3 * me + 1] = S.d[3 * me + 1] + g[1];
-  S.d[3 * me + 2] = S.d[3 * me + 2] + g[2];
-  double* eb = S.e_bonded + (size_t)t * 6;
-  for (int c = 0; c < 6; ++c) eb[c] = e[c];
 }
 
 }  // namespace qgpu

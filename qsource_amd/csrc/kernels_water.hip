@@ -660,9 +660,12 @@ __global__ __launch_bounds__(256) void k_energy_reduce(const DevSys* __restrict_
   double p[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int i = threadIdx.x; i < S.nat_solute; i += blockDim.x)
     for (int c = 0; c < 8; ++c) p[c] += S.e_solute[(size_t)i * 8 + c];
+  // bonded terms: p_bond p_angle p_torsion w_bond w_angle w_torsion
   double b[6] = {0, 0, 0, 0, 0, 0};
-  for (int t = threadIdx.x; t < S.nbonded_atoms; t += blockDim.x)
-    for (int c = 0; c < 6; ++c) b[c] += S.e_bonded[(size_t)t * 6 + c];
+  for (int n = threadIdx.x; n < S.nbnd; n += blockDim.x) b[n < S.nbnd_solute ? 0 : 3] += S.e_bterm[n];
+  for (int n = threadIdx.x; n < S.nang; n += blockDim.x) b[n < S.nang_solute ? 1 : 4] += S.e_bterm[S.nbnd + n];
+  for (int n = threadIdx.x; n < S.ntor; n += blockDim.x)
+    b[n < S.ntor_solute ? 2 : 5] += S.e_bterm[S.nbnd + S.nang + n];
   for (int c = 0; c < 4; ++c) w[c] = block_sum_256(w[c], red[c]);
   for (int c = 0; c < 8; ++c) p[c] = block_sum_256(p[c], red[4 + c]);
   for (int c = 0; c < 6; ++c) b[c] = block_sum_256(b[c], red[c]);

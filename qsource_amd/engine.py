@@ -37,7 +37,7 @@ SYMBOLS = [
 ]
 
 KERNEL_NAMES = ["cgp_centers", "water_lists", "solute_lists", "water_forces", "solute_forces",
-                "bonded", "q_forces", "qq_forces", "watpol_theta", "watpol_force",
+                "terms", "q_forces", "term_gather", "watpol_theta", "watpol_force",
                 "energy_record", "integrate", "temperature"]
 
 _lib = None
