@@ -313,7 +313,29 @@ void Batch::upload(int k) {
         std_w = t.cgpatom[g.first + k] == a + k && s.iwhich_cgp[a + k] == s.iwhich_cgp[a] &&
                 !s.iqatom[a + k] && s.crg[a + k] == s.chg_solv[k];
     }
+    for (int w = 0; w < s.nwat && std_w; ++w)
+      for (int k = 0; k < 3 && std_w; ++k) std_w = s.iac[s.nat_solute + 3 * w + k] == s.iac[s.nat_solute + k];
     D.std_water = std_w ? 1 : 0;
+    if (std_w && s.nat_solute > 0) {
+      std::vector<double> wd((size_t)s.nat_solute * 6);
+      std::vector<float> wf((size_t)s.nat_solute * 6);
+      std::vector<unsigned char> wz(s.nat_solute, 0);
+      for (int i = 0; i < s.nat_solute; ++i) {
+        const int ti = s.iac[i];
+        for (int k = 0; k < 3; ++k) {
+          const int j = s.nat_solute + k, tj = s.iac[j];
+          const int lj = s.ljcod[ti * s.natyps + tj] - 1;
+          const double a = t.types[ti].avdw[lj] * t.types[tj].avdw[lj];
+          const double b = t.types[ti].bvdw[lj] * t.types[tj].bvdw[lj];
+          wd[6 * i + k] = a; wd[6 * i + 3 + k] = b;
+          wf[6 * i + k] = (float)a; wf[6 * i + 3 + k] = (float)b;
+          if (s.crg[i] * s.crg[j] == 0.0 && a == 0.0 && b == 0.0) wz[i] |= (unsigned char)(1u << k);
+        }
+      }
+      D.swp_d = I->upload(wd);
+      D.swp_f = I->upload(wf);
+      D.swp_zero = I->upload(wz);
+    }
   }
   D.rwat = s.rwat; D.fk_wsphere = s.fk_wsphere; D.dwmz = s.dwmz; D.awmz = s.awmz; D.fkwpol = s.fkwpol;
   for (int c = 0; c < 3; ++c) D.xwcent[c] = t.xwcent[c];

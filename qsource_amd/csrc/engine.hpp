@@ -95,6 +95,12 @@ struct DevSys {
   // solvent
   double chg_solv[3];
   int std_water;                 // every water is one charge group (O, H1, H2) with chg_solv, no Q atom
+  // with standard waters: per solute atom i the LJ products with water atoms
+  // O, H1, H2 (aLJ[3] then bLJ[3]) in FP64 and FP32, and a 3-bit mask of
+  // the water atoms it has a zero interaction with (nbpwlist's skip rule)
+  const double* swp_d;
+  const float* swp_f;
+  const unsigned char* swp_zero;
   double aOO, bOO;               // alj_solv(1,iLJ)^2, blj_solv(1,iLJ)^2 for iLJ=ljcod(O,O)
   double rwat, fk_wsphere, dwmz, awmz, fkwpol;
   double xwcent[3];

@@ -37,6 +37,14 @@ __device__ __forceinline__ const double* pair_x<double>(const DevSys& S) { retur
 template <>
 __device__ __forceinline__ const float* pair_x<float>(const DevSys& S) { return S.xf; }
 
+// Per-solute-atom LJ products with the water atoms, in the pair precision
+template <typename P>
+__device__ __forceinline__ const P* solute_water_lj(const DevSys& S);
+template <>
+__device__ __forceinline__ const double* solute_water_lj<double>(const DevSys& S) { return S.swp_d; }
+template <>
+__device__ __forceinline__ const float* solute_water_lj<float>(const DevSys& S) { return S.swp_f; }
+
 // c - a*b and the squared norm: FP64 as separate operations in the
 // reference's order (d1*d1 + d2*d2 + d3*d3), FP32 with fused multiply-add
 __device__ __forceinline__ double pmsub(double a, double b, double c) { return c - a * b; }

@@ -146,8 +146,9 @@ template __global__ void k_solute_lists<float>(const DevSys*);
 // gradient is summed over its 8 lanes and written once.  A pair's energy is
 // counted by the atom that is i of the reference pair: the owner group's
 // atom, or the lower index within one group.
+// FP32 pair arithmetic fits 6 wavefronts per SIMD without spilling
 template <typename P>
-__global__ __launch_bounds__(256) void k_solute_forces(const DevSys* __restrict__ sys) {
+__global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(const DevSys* __restrict__ sys) {
   const DevSys& S = sys[blockIdx.y];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if ((t >> 6) * 8 >= S.nat_solute) return;   // whole wavefronts past the last atom
@@ -213,15 +214,26 @@ __global__ __launch_bounds__(256) void k_solute_forces(const DevSys* __restrict_
     // solute-water (the solute atom is i of the reference pair)
     const int npw = S.pw_cnt[g];
     const int* wrow = S.pw_nb + (size_t)g * S.pw_cap;
+    // standard waters: LJ products and zero-interaction mask of this atom
+    // with O, H1, H2 from the per-atom table
+    const P* wlj = S.std_water ? solute_water_lj<P>(S) + 6 * (size_t)i : nullptr;
+    const unsigned wzero = S.std_water ? S.swp_zero[i] : 0u;
     for (int k = sub; k < npw; k += 8) {
       const int jo = S.nat_solute + 3 * wrow[k];
       for (int ip = 0; ip < 3; ++ip) {
         const int j = jo + ip;
-        const int tj = S.iac[j];
-        const int lj = S.ljcod[ti * S.natyps + tj] - 1;
-        if (zero_interaction(S, ia, j, lj)) continue;
-        const P aLJ = (P)(S.avdw[ti * 3 + lj] * S.avdw[tj * 3 + lj]);
-        const P bLJ = (P)(S.bvdw[ti * 3 + lj] * S.bvdw[tj * 3 + lj]);
+        P aLJ, bLJ;
+        if (wlj) {
+          if ((wzero >> ip) & 1u) continue;
+          aLJ = wlj[ip];
+          bLJ = wlj[3 + ip];
+        } else {
+          const int tj = S.iac[j];
+          const int lj = S.ljcod[ti * S.natyps + tj] - 1;
+          if (zero_interaction(S, ia, j, lj)) continue;
+          aLJ = (P)(S.avdw[ti * 3 + lj] * S.avdw[tj * 3 + lj]);
+          bLJ = (P)(S.bvdw[ti * 3 + lj] * S.bvdw[tj * 3 + lj]);
+        }
         const P dx1 = X[3 * j] - xi0, dx2 = X[3 * j + 1] - xi1, dx3 = X[3 * j + 2] - xi2;
         P r;
         const P r2 = inv_r2(pnorm2(dx1, dx2, dx3), r);

@@ -148,8 +148,9 @@ template __global__ void k_water_lists<float>(const DevSys*);
 // reference operation order inside each pair) or float (mixed: FP32
 // coordinates and FMA pair arithmetic, each partner's contribution
 // accumulated in FP64).
+// FP32 pair arithmetic fits 6 wavefronts per SIMD without spilling
 template <typename P>
-__global__ __launch_bounds__(256) void k_water_forces(const DevSys* __restrict__ sys) {
+__global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_water_forces(const DevSys* __restrict__ sys) {
   const DevSys& S = sys[blockIdx.y];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if ((t >> 6) * 8 >= S.nwat) return;   // whole wavefronts past the last water
@@ -226,13 +227,20 @@ __global__ __launch_bounds__(256) void k_water_forces(const DevSys* __restrict__
       const P xs0 = X[3 * i], xs1 = X[3 * i + 1], xs2 = X[3 * i + 2];
       const int ti = S.iac[i];
       const P qs = (P)S.crg[i];
+      const P* wlj = S.std_water ? solute_water_lj<P>(S) + 6 * (size_t)i : nullptr;
       for (int ip = 0; ip < 3; ++ip) {
         const int j = io + ip;
-        const int tj = S.iac[j];
-        const int lj = S.ljcod[ti * S.natyps + tj] - 1;
         const P dx1 = xi[3 * ip] - xs0, dx2 = xi[3 * ip + 1] - xs1, dx3 = xi[3 * ip + 2] - xs2;
-        const P aLJ = (P)(S.avdw[ti * 3 + lj] * S.avdw[tj * 3 + lj]);
-        const P bLJ = (P)(S.bvdw[ti * 3 + lj] * S.bvdw[tj * 3 + lj]);
+        P aLJ, bLJ;
+        if (wlj) {
+          aLJ = wlj[ip];
+          bLJ = wlj[3 + ip];
+        } else {
+          const int tj = S.iac[j];
+          const int lj = S.ljcod[ti * S.natyps + tj] - 1;
+          aLJ = (P)(S.avdw[ti * 3 + lj] * S.avdw[tj * 3 + lj]);
+          bLJ = (P)(S.bvdw[ti * 3 + lj] * S.bvdw[tj * 3 + lj]);
+        }
         P r;
         const P r2 = inv_r2(pnorm2(dx1, dx2, dx3), r);
         const P r6 = r2 * r2 * r2;
