@@ -455,9 +455,13 @@ void Batch::upload(int k) {
         sl.insert(sl.end(), inc[i].begin(), inc[i].end());
       }
     sf.push_back((int)sl.size());
+    // each term slot is stored at its position in the atom's CSR run, so the
+    // gather reads every atom's contributions contiguously
+    std::vector<int> pos(std::max(1, slot), 0);
+    for (size_t k = 0; k < sl.size(); ++k) pos[sl[k]] = (int)k;
     D.slot_g = I->alloc<double>((size_t)std::max(1, slot) * 3);
     D.nslot_atoms = (int)sa.size();
-    D.slot_atom = I->upload(sa); D.slot_first = I->upload(sf); D.slot_list = I->upload(sl);
+    D.slot_atom = I->upload(sa); D.slot_first = I->upload(sf); D.slot_pos = I->upload(pos);
     max_terms_ = std::max(max_terms_, D.nbnd + D.nang + D.ntor + D.nqq);
     max_slot_atoms_ = std::max(max_slot_atoms_, D.nslot_atoms);
   }

@@ -140,11 +140,12 @@ struct DevSys {
   int nbnd, nbnd_solute, nang, nang_solute, ntor, ntor_solute;
   double* e_bterm;               // energy per bonded term (bonds, angles, torsions)
   // gradient slots of the term kernel: 3 FP64 per (term, atom of the term),
-  // bonds 2, angles 3, torsions 4, Q pairs 2 slots, consecutive per term;
-  // CSR atom -> its slots, summed in this order
+  // numbered bonds 2, angles 3, torsions 4, Q pairs 2 per term; slot n is
+  // stored at slot_pos[n], so that atom slot_atom[t]'s contributions occupy
+  // slot_g[slot_first[t] .. slot_first[t+1]) in slot order
   double* slot_g;
   int nslot_atoms;
-  const int *slot_atom, *slot_first, *slot_list;
+  const int *slot_atom, *slot_first, *slot_pos;
   // integrator
   double dt;
   double temp0, tau_t;

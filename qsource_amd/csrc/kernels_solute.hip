@@ -465,7 +465,7 @@ __global__ __launch_bounds__(64) void k_q_reduce(const DevSys* __restrict__ sys,
 // then adds every atom's slots to d in a fixed order.  FP64 in both
 // precisions.
 __device__ __forceinline__ void put_slot(const DevSys& S, int slot, double g0, double g1, double g2) {
-  double* o = S.slot_g + 3 * (size_t)slot;
+  double* o = S.slot_g + 3 * (size_t)S.slot_pos[slot];
   o[0] = g0; o[1] = g1; o[2] = g2;
 }
 
@@ -607,9 +607,10 @@ __global__ __launch_bounds__(256) void k_term_gather(const DevSys* __restrict__ 
   if (t >= S.nslot_atoms) return;
   const int me = S.slot_atom[t];
   double g0 = 0, g1 = 0, g2 = 0;
-  for (int k = S.slot_first[t]; k < S.slot_first[t + 1]; ++k) {
-    const double* o = S.slot_g + 3 * (size_t)S.slot_list[k];
-    g0 = g0 + o[0]; g1 = g1 + o[1]; g2 = g2 + o[2];
+  const double* o = S.slot_g + 3 * (size_t)S.slot_first[t];
+  const int n = S.slot_first[t + 1] - S.slot_first[t];
+  for (int k = 0; k < n; ++k) {
+    g0 = g0 + o[3 * k]; g1 = g1 + o[3 * k + 1]; g2 = g2 + o[3 * k + 2];
   }
   S.d[3 * me] = S.d[3 * me] + g0;
   S.d[3 * me + 1] = S.d[3 * me + 1] + g1;
