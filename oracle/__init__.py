@@ -40,7 +40,7 @@ def _load():
         for name in ["oracle_destroy", "oracle_natom", "oracle_nstates", "oracle_run",
                      "oracle_final_energy", "oracle_get_xv", "oracle_get_d", "oracle_get_energies",
                      "oracle_get_qenergies", "oracle_list_sizes", "oracle_get_list", "oracle_get_lrf",
-                     "oracle_temperature", "oracle_theta_corr", "oracle_nshells"]:
+                     "oracle_temperature", "oracle_theta_corr", "oracle_nshells", "oracle_offdiag"]:
             getattr(lib, name).argtypes = None
         lib.oracle_temperature.restype = ctypes.c_double
         _lib = lib
@@ -121,3 +121,11 @@ class Oracle:
         out = np.zeros(max(n, 1))
         self.lib.oracle_theta_corr(self.h, _dp(out))
         return out[:n]
+
+    def offdiag(self) -> np.ndarray:
+        """(noffd, 2): H_ij and r_kl of each [off_diagonals] line at the
+        current coordinates."""
+        h = np.zeros(16)
+        r = np.zeros(16)
+        n = self.lib.oracle_offdiag(self.h, _dp(h), _dp(r), ctypes.c_int(16))
+        return np.stack([h[:n], r[:n]], axis=1)

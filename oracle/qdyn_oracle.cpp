@@ -84,6 +84,10 @@ struct Oracle {
   int natyps() const { return s.natyps; }
   int ljcod(int a, int b) const { return s.ljcod[a * s.natyps + b]; }
   const AtomType& typ(int i) const { return s.top.types[s.iac[i]]; }
+  // sc_lookup(iq, column, state), 0-based column (topology types, then Q atoms)
+  double sc(int iq, int col, int st) const {
+    return s.sc_lookup[((size_t)iq * (s.natyps + s.nqat) + col) * s.nstates + st];
+  }
 
   explicit Oracle(const std::string& inp) : s(prepare(inp)) {
     x = s.x; v = s.v;
@@ -301,6 +305,12 @@ struct Oracle {
   double angle(int a, int b);
   double torsion(int a, int b);
   double improper(int a, int b);
+  double improper2(int a, int b);
+  void nonbond_qp_qvdw();
+  void offdiag();
+  // EVB off-diagonal elements of the last offdiag() (per [off_diagonals]
+  // line: Hij and r_kl)
+  std::vector<double> offd_hij, offd_rkl;
 
   void fix_shell() {
     for (int i = 0; i < s.natom; ++i) {
@@ -465,7 +475,8 @@ struct Oracle {
     // pot_energy_nonbonds (sphere, geometric rule)
     nonbond_pp();
     nonbond_pw();
-    nonbond_qp();
+    if (s.qvdw_flag) nonbond_qp_qvdw();
+    else nonbond_qp();
     if (s.natom > s.nat_solute) {
       nonbond_ww_spc();
       nonbond_qw_spc();
@@ -478,8 +489,15 @@ struct Oracle {
     E.w_angle = angle(s.nangles_solute, (int)s.ang.size());
     E.p_torsion = torsion(0, s.ntors_solute);
     E.w_torsion = torsion(s.ntors_solute, (int)s.tor.size());
-    E.p_improper = improper(0, s.nimps_solute);
-    E.w_improper = improper(s.nimps_solute, (int)s.imp.size());
+    // pot_energy_bonds (src/md.f90:16224-16256): harmonic impropers for
+    // GROMOS, periodic for AMBER
+    if (s.ff_type == 2) {
+      E.p_improper = improper2(0, s.nimps_solute);
+      E.w_improper = improper2(s.nimps_solute, (int)s.imp.size());
+    } else {
+      E.p_improper = improper(0, s.nimps_solute);
+      E.w_improper = improper(s.nimps_solute, (int)s.imp.size());
+    }
     fix_shell();
     if (s.nwat > 0) {
       restrain_solvent();
@@ -817,13 +835,26 @@ void Oracle::nonbond_qw_spc() {
     double r = std::sqrt(r2);
     double r6_sc = r6_hc;
     int iLJ = nbqw[iw].ljcod;
-    double r6 = 1.0 / r6_hc;
-    double aLJ = typ(i).avdw[iLJ - 1];
-    double bLJ = typ(i).bvdw[iLJ - 1];
-    double Va = aLJ * s.alj_solv[0][iLJ - 1] * r6 * r6;
-    double Vb = bLJ * s.blj_solv[0][iLJ - 1] * r6;
+    double Va = 0, Vb = 0;
+    if (!s.qvdw_flag) {
+      double r6 = 1.0 / r6_hc;
+      double aLJ = typ(i).avdw[iLJ - 1];
+      double bLJ = typ(i).bvdw[iLJ - 1];
+      Va = aLJ * s.alj_solv[0][iLJ - 1] * r6 * r6;
+      Vb = bLJ * s.blj_solv[0][iLJ - 1] * r6;
+    }
     double dv = 0;
     for (int st = 0; st < ns; ++st) {
+      if (s.qvdw_flag) {
+        // Q-atom type of this state, LJ code 1, soft-core column iLJ
+        // (src/md.f90:14497-14505)
+        const int qt = s.qiac[iq * ns + st];
+        double aLJ = s.qavdw[3 * qt];
+        double bLJ = s.qbvdw[3 * qt];
+        r6_sc = r6_hc + sc(iq, iLJ - 1, st);
+        Va = aLJ * s.alj_solv[0][iLJ - 1] / (r6_sc * r6_sc);
+        Vb = bLJ * s.blj_solv[0][iLJ - 1] / (r6_sc);
+      }
       double Vel = s.chg_solv[0] * s.qcrg[iq * ns + st] * r;
       dv = dv + r2 * (-Vel - ((12.0 * Va - 6.0 * Vb) * (r6_hc / r6_sc))) * s.lambda[st];
       EQ[st].qw_el = EQ[st].qw_el + Vel;
@@ -857,22 +888,53 @@ void Oracle::nonbond_qq() {
       int iLJ = p.ljcod;
       double qi = s.qcrg[iq * ns + st];
       double el_scale = p.el_scale;
-      double aLJ = typ(i).avdw[iLJ - 1] * typ(j).avdw[iLJ - 1];
-      double bLJ = typ(i).bvdw[iLJ - 1] * typ(j).bvdw[iLJ - 1];
-      double qj = jq >= 0 ? s.qcrg[jq * ns + st] : s.crg[j];
+      double aLJ, bLJ, qj;
+      if (!s.qvdw_flag) {
+        aLJ = typ(i).avdw[iLJ - 1] * typ(j).avdw[iLJ - 1];
+        bLJ = typ(i).bvdw[iLJ - 1] * typ(j).bvdw[iLJ - 1];
+        qj = jq >= 0 ? s.qcrg[jq * ns + st] : s.crg[j];
+      } else {
+        // Q-atom types (src/md.f90:13531-13550)
+        const int qti = s.qiac[iq * ns + st];
+        aLJ = s.qavdw[3 * qti + iLJ - 1];
+        bLJ = s.qbvdw[3 * qti + iLJ - 1];
+        if (jq >= 0) {
+          const int qtj = s.qiac[jq * ns + st];
+          aLJ = aLJ * s.qavdw[3 * qtj + iLJ - 1];
+          bLJ = bLJ * s.qbvdw[3 * qtj + iLJ - 1];
+          qj = s.qcrg[jq * ns + st];
+        } else {
+          if (iLJ == 2) {
+            aLJ = s.qavdw[3 * qti];
+            bLJ = s.qbvdw[3 * qti];
+          }
+          aLJ = aLJ * typ(j).avdw[iLJ - 1];
+          bLJ = bLJ * typ(j).bvdw[iLJ - 1];
+          qj = s.crg[j];
+        }
+      }
       double dx1 = x[j3] - x[i3], dx2 = x[j3 + 1] - x[i3 + 1], dx3 = x[j3 + 2] - x[i3 + 2];
       double r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
       double r6_hc = r2 * r2 * r2;
-      double r6 = r2 * r2 * r2 + 0.0;
+      // soft-core column natyps + jq (1-based jq, 0 for a non-Q atom: the
+      // reference then reads the column of the last topology type)
+      double r6 = r2 * r2 * r2 + sc(iq, s.natyps + jq, st);
       r6 = 1.0 / r6;
       double r12 = r6 * r6;
       r2 = 1.0 / r2;
       double r = std::sqrt(r2);
       double Vel = qi * qj * r * el_scale;
       if (iLJ == 3) Vel = Vel * s.top.el14_scale;
-      double Va = aLJ * r12;
-      double Vb = bLJ * r6;
-      double dv = r2 * (-Vel - (12.0 * Va - 6.0 * Vb) * r6_hc * r6) * s.lambda[st];
+      double Va, Vb, dv;
+      if (s.qvdw_flag && jq >= 0 && iLJ == 2) {
+        Va = aLJ * std::exp(-bLJ / r);
+        Vb = 0.0;
+        dv = r2 * (-Vel - bLJ * Va / r) * s.lambda[st];
+      } else {
+        Va = aLJ * r12;
+        Vb = bLJ * r6;
+        dv = r2 * (-Vel - (12.0 * Va - 6.0 * Vb) * r6_hc * r6) * s.lambda[st];
+      }
       d[i3] = d[i3] - dv * dx1; d[i3 + 1] = d[i3 + 1] - dv * dx2; d[i3 + 2] = d[i3 + 2] - dv * dx3;
       d[j3] = d[j3] + dv * dx1; d[j3 + 1] = d[j3 + 1] + dv * dx2; d[j3 + 2] = d[j3 + 2] + dv * dx3;
       if (jq >= 0) {
@@ -999,9 +1061,139 @@ double Oracle::torsion(int a, int b) {
   return e;
 }
 
+// nonbond_qp_qvdw (src/md.f90:14097-14230): Q-solute pairs with the Q atom's
+// type of each state and soft-core r^6 + alpha.
+void Oracle::nonbond_qp_qvdw() {
+  const int ns = s.nstates;
+  for (const NBQ& p : nbqp) {
+    int iq = p.iq, i = s.iqseq[iq], j = p.j, i3 = 3 * i, j3 = 3 * j;
+    int iacj = s.iac[j];
+    int iLJ = p.ljcod;
+    int qLJ = iLJ == 2 ? 1 : iLJ;
+    double dx1 = x[j3] - x[i3], dx2 = x[j3 + 1] - x[i3 + 1], dx3 = x[j3 + 2] - x[i3 + 2];
+    double r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
+    double r6 = r2 * r2 * r2;
+    r2 = 1.0 / r2;
+    double r = std::sqrt(r2);
+    for (int st = 0; st < ns; ++st) {
+      const int iaci = s.qiac[iq * ns + st];
+      double aLJ = s.qavdw[3 * iaci + qLJ - 1] * typ(j).avdw[iLJ - 1];
+      double bLJ = s.qbvdw[3 * iaci + qLJ - 1] * typ(j).bvdw[iLJ - 1];
+      double Vel = s.qcrg[iq * ns + st] * s.crg[j] * r;
+      if (iLJ == 3) Vel = Vel * s.top.el14_scale;
+      double r6_sc = r6 + sc(iq, iacj, st);
+      double Va = aLJ / (r6_sc * r6_sc);
+      double Vb = bLJ / (r6_sc);
+      double dv = r2 * (-Vel - (12.0 * Va - 6.0 * Vb) * (r6 / r6_sc)) * s.lambda[st];
+      d[i3] = d[i3] - dv * dx1; d[i3 + 1] = d[i3 + 1] - dv * dx2; d[i3 + 2] = d[i3 + 2] - dv * dx3;
+      d[j3] = d[j3] + dv * dx1; d[j3 + 1] = d[j3 + 1] + dv * dx2; d[j3 + 2] = d[j3 + 2] + dv * dx3;
+      EQ[st].qp_el = EQ[st].qp_el + Vel;
+      EQ[st].qp_vdw = EQ[st].qp_vdw + Va - Vb;
+    }
+  }
+}
+
+// Dihedral geometry shared by the improper forms (as in torsion): phi and
+// the 12 derivative components dp of phi scaled by -1/sin(phi).
+static double dihedral(const std::vector<double>& x, int i, int j, int k, int l, double dp[12]) {
+  const int i3 = 3 * i, j3 = 3 * j, k3 = 3 * k, l3 = 3 * l;
+  double rji[3], rjk[3], rkl[3], rnj[3], rnk[3];
+  for (int c = 0; c < 3; ++c) {
+    rji[c] = x[i3 + c] - x[j3 + c];
+    rjk[c] = x[k3 + c] - x[j3 + c];
+    rkl[c] = x[l3 + c] - x[k3 + c];
+  }
+  rnj[0] = rji[1] * rjk[2] - rji[2] * rjk[1];
+  rnj[1] = rji[2] * rjk[0] - rji[0] * rjk[2];
+  rnj[2] = rji[0] * rjk[1] - rji[1] * rjk[0];
+  rnk[0] = -rjk[1] * rkl[2] + rjk[2] * rkl[1];
+  rnk[1] = -rjk[2] * rkl[0] + rjk[0] * rkl[2];
+  rnk[2] = -rjk[0] * rkl[1] + rjk[1] * rkl[0];
+  double bj2inv = 1.0 / (rnj[0] * rnj[0] + rnj[1] * rnj[1] + rnj[2] * rnj[2]);
+  double bk2inv = 1.0 / (rnk[0] * rnk[0] + rnk[1] * rnk[1] + rnk[2] * rnk[2]);
+  double bjinv = std::sqrt(bj2inv), bkinv = std::sqrt(bk2inv);
+  double scp = (rnj[0] * rnk[0] + rnj[1] * rnk[1] + rnj[2] * rnk[2]) * (bjinv * bkinv);
+  if (scp > 1.0) scp = 1.0;
+  else if (scp < -1.0) scp = -1.0;
+  double phi = std::acos(scp);
+  if (rjk[0] * (rnj[1] * rnk[2] - rnj[2] * rnk[1]) + rjk[1] * (rnj[2] * rnk[0] - rnj[0] * rnk[2]) +
+          rjk[2] * (rnj[0] * rnk[1] - rnj[1] * rnk[0]) < 0)
+    phi = -phi;
+  double f1 = std::sin(phi);
+  if (std::fabs(f1) < kQrealEps) f1 = kQrealEps;
+  f1 = -1.0 / f1;
+  double di[3], dl[3], rki[3], rlj[3];
+  for (int c = 0; c < 3; ++c) {
+    di[c] = f1 * (rnk[c] * bjinv * bkinv - scp * rnj[c] * bj2inv);
+    dl[c] = f1 * (rnj[c] * bjinv * bkinv - scp * rnk[c] * bk2inv);
+    rki[c] = rji[c] - rjk[c];
+    rlj[c] = -rjk[c] - rkl[c];
+  }
+  dp[0] = rjk[1] * di[2] - rjk[2] * di[1];
+  dp[1] = rjk[2] * di[0] - rjk[0] * di[2];
+  dp[2] = rjk[0] * di[1] - rjk[1] * di[0];
+  dp[3] = rki[1] * di[2] - rki[2] * di[1] + rkl[1] * dl[2] - rkl[2] * dl[1];
+  dp[4] = rki[2] * di[0] - rki[0] * di[2] + rkl[2] * dl[0] - rkl[0] * dl[2];
+  dp[5] = rki[0] * di[1] - rki[1] * di[0] + rkl[0] * dl[1] - rkl[1] * dl[0];
+  dp[6] = rlj[1] * dl[2] - rlj[2] * dl[1] - rji[1] * di[2] + rji[2] * di[1];
+  dp[7] = rlj[2] * dl[0] - rlj[0] * dl[2] - rji[2] * di[0] + rji[0] * di[2];
+  dp[8] = rlj[0] * dl[1] - rlj[1] * dl[0] - rji[0] * di[1] + rji[1] * di[0];
+  dp[9] = rjk[1] * dl[2] - rjk[2] * dl[1];
+  dp[10] = rjk[2] * dl[0] - rjk[0] * dl[2];
+  dp[11] = rjk[0] * dl[1] - rjk[1] * dl[0];
+  return phi;
+}
+
+// improper (src/md.f90:3130-3272): harmonic in the wrapped angle
+// phi - imp0, terms visited from the last to the first.
 double Oracle::improper(int a, int b) {
-  if (a < b) throw std::runtime_error("improper torsions are not restated (none in these systems)");
-  return 0.0;
+  double e = 0.0;
+  for (int ip = b - 1; ip >= a; --ip) {
+    const Torsion& t = s.imp[ip];
+    const ImpLib& L = s.top.implib[t.cod - 1];
+    double dp[12];
+    double phi = dihedral(x, t.i, t.j, t.k, t.l, dp);
+    double arg = phi - L.imp0;
+    arg = arg - 2.0 * kPi * std::round(arg / (2.0 * kPi));
+    double dv = L.fk * arg;
+    e = e + 0.5 * dv * arg;
+    const int at[4] = {t.i, t.j, t.k, t.l};
+    for (int m = 0; m < 4; ++m)
+      for (int c = 0; c < 3; ++c) d[3 * at[m] + c] = d[3 * at[m] + c] + dv * dp[3 * m + c];
+  }
+  return e;
+}
+
+// improper2 (src/md.f90:3276-3416): periodic fk (1 + cos(2 phi - imp0)).
+double Oracle::improper2(int a, int b) {
+  double e = 0.0;
+  for (int ip = b - 1; ip >= a; --ip) {
+    const Torsion& t = s.imp[ip];
+    const ImpLib& L = s.top.implib[t.cod - 1];
+    double dp[12];
+    double phi = dihedral(x, t.i, t.j, t.k, t.l, dp);
+    double arg = 2 * phi - L.imp0;
+    e = e + L.fk * (1 + std::cos(arg));
+    double dv = -2 * L.fk * std::sin(arg);
+    const int at[4] = {t.i, t.j, t.k, t.l};
+    for (int m = 0; m < 4; ++m)
+      for (int c = 0; c < 3; ++c) d[3 * at[m] + c] = d[3 * at[m] + c] + dv * dp[3 * m + c];
+  }
+  return e;
+}
+
+// offdiag (src/md.f90:15444-15470): H_ij = A exp(-mu r_kl) per function.
+void Oracle::offdiag() {
+  offd_hij.assign(s.offd.size(), 0.0);
+  offd_rkl.assign(s.offd.size(), 0.0);
+  for (size_t io = 0; io < s.offd.size(); ++io) {
+    const OffDiag& o = s.offd[io];
+    int k3 = 3 * s.iqseq[o.k], l3 = 3 * s.iqseq[o.l];
+    double r = std::sqrt((x[l3] - x[k3]) * (x[l3] - x[k3]) + (x[l3 + 1] - x[k3 + 1]) * (x[l3 + 1] - x[k3 + 1]) +
+                         (x[l3 + 2] - x[k3 + 2]) * (x[l3 + 2] - x[k3 + 2]));
+    offd_hij[io] = o.A * std::exp(-o.mu * r);
+    offd_rkl[io] = r;
+  }
 }
 
 thread_local std::string g_err;
@@ -1102,6 +1294,16 @@ void oracle_get_lrf(void* h, double* out) {
   for (size_t g = 0; g < o->lrf.size(); ++g) std::memcpy(out + 43 * g, &o->lrf[g], 43 * 8);
 }
 double oracle_temperature(void* h) { return static_cast<Oracle*>(h)->Temp; }
+// EVB off-diagonal elements at the current coordinates: Hij and r_kl per
+// [off_diagonals] line; returns the count.
+int oracle_offdiag(void* h, double* hij, double* rkl, int cap) {
+  Oracle* o = static_cast<Oracle*>(h);
+  o->offdiag();
+  int n = (int)o->offd_hij.size();
+  for (int k = 0; k < n && k < cap; ++k) { hij[k] = o->offd_hij[k]; rkl[k] = o->offd_rkl[k]; }
+  return n;
+}
+
 void oracle_theta_corr(void* h, double* out) {
   auto* o = static_cast<Oracle*>(h);
   for (size_t k = 0; k < o->s.wshell.size(); ++k) out[k] = o->s.wshell[k].theta_corr;

@@ -143,6 +143,8 @@ Setup prepare(const std::string& input_path) {
     s.v.assign(3 * s.natom, 0.0);
   }
 
+  s.ff_type = t.ff_type;
+  if (s.ff_type == 3) throw std::runtime_error("CHARMM force fields (Urey-Bradley terms) are not supported");
   // get_fep: Q atoms and their per-state charges (src/md.f90:2884-2990)
   s.iqatom.assign(s.natom, 0);
   s.crg = t.crg;
@@ -167,6 +169,54 @@ Setup prepare(const std::string& input_path) {
     for (int iq = 0; iq < s.nqat; ++iq)
       for (int st = 0; st < s.nstates; ++st)
         s.qcrg[iq * s.nstates + st] = f.have_charges ? f.qcrg[iq][st] : t.crg[s.iqseq[iq]];
+    // Q-atom types (qvdw_flag) and the soft-core table
+    s.qvdw_flag = f.qvdw;
+    s.offd = f.offd;
+    if (f.qvdw) {
+      if (t.ivdw_rule != 1) throw std::runtime_error("Q-atom types need the geometric combination rule");
+      s.qiac.assign((size_t)s.nqat * s.nstates, 0);
+      for (int iq = 0; iq < s.nqat; ++iq)
+        for (int st = 0; st < s.nstates; ++st) s.qiac[iq * s.nstates + st] = f.qiac[iq][st];
+      for (size_t k = 0; k < f.qtac.size(); ++k)
+        for (int c = 0; c < 3; ++c) { s.qavdw.push_back(f.qavdw[k][c]); s.qbvdw.push_back(f.qbvdw[k][c]); }
+    }
+    const int nt = (int)t.types.size(), nq = s.nqat, nst = s.nstates, ncol = nt + nq;
+    s.sc_lookup.assign((size_t)nq * ncol * nst, 0.0);
+    if (f.softcore) {
+      // src/qatom.f90:1898-1970: explicit alphas, or alphas from the
+      // desired maximum vdW potential (geometric rule)
+      auto sc = [&](int i, int col, int st) -> double& { return s.sc_lookup[((size_t)i * ncol + col) * nst + st]; };
+      auto qa = [&](int i, int st) { return f.qavdw[f.qiac[i][st]][0]; };
+      auto qb = [&](int i, int st) { return f.qbvdw[f.qiac[i][st]][0]; };
+      for (int i = 0; i < nq; ++i)
+        for (int st = 0; st < nst; ++st) {
+          const double am = f.alpha_max[i][st];
+          for (int j = 0; j < nt; ++j) {
+            if (f.softcore_max_potential) {
+              if (am != 0) {
+                const double aq = qa(i, st), bq = qb(i, st), aj = t.types[j].avdw[0], bj = t.types[j].bvdw[0];
+                sc(i, j, st) = (-bq * bj + std::sqrt(bq * bq * bj * bj + 4 * am * aq * aj)) / (2 * am);
+              }
+            } else {
+              sc(i, j, st) = am;
+            }
+          }
+          for (int j = 0; j < nq; ++j) {
+            const double aj_max = f.alpha_max[j][st];
+            if (am > 1e-6 || aj_max > 1e-6) {
+              double at;
+              if (f.softcore_max_potential) {
+                at = std::min(am, aj_max);
+                if (am == 0 || aj_max == 0) at = std::max(am, aj_max);
+                const double aq = qa(i, st), bq = qb(i, st), aj = qa(j, st), bj = qb(j, st);
+                sc(i, nt + j, st) = (-bq * bj + std::sqrt(bq * bq * bj * bj + 4 * at * aq * aj)) / (2 * at);
+              } else {
+                sc(i, nt + j, st) = std::max(am, aj_max);
+              }
+            }
+          }
+        }
+    }
   } else if (!in.lambdas.empty()) {
     throw std::runtime_error("[lambdas] given without an FEP file");
   }
@@ -338,7 +388,9 @@ Setup prepare(const std::string& input_path) {
           int c = qc(st, iq, j);
           if (c > 3) {
             NbqPair p{iq, j, jq, 0, 1.0};
-            p.ljcod = (c == 4) ? 3 : s.ljcod[s.iac[s.iqseq[iq]] * s.natyps + s.iac[j]];
+            // LJ code: 1-4 pairs 3; with Q-atom types 1 (2 only for soft
+            // pairs, not read here); else the topology code
+            p.ljcod = (c == 4) ? 3 : s.qvdw_flag ? 1 : s.ljcod[s.iac[s.iqseq[iq]] * s.natyps + s.iac[j]];
             s.nbqq[st].push_back(p);
           }
         }
@@ -354,7 +406,7 @@ Setup prepare(const std::string& input_path) {
           int c = qc(st, iq, j);
           if (c >= 4) {
             NbqPair p{iq, j, -1, 0, 1.0};
-            p.ljcod = (c == 4) ? 3 : s.ljcod[s.iac[s.iqseq[iq]] * s.natyps + s.iac[j]];
+            p.ljcod = (c == 4) ? 3 : s.qvdw_flag ? 1 : s.ljcod[s.iac[s.iqseq[iq]] * s.natyps + s.iac[j]];
             s.nbqq[st].push_back(p);
           }
         }

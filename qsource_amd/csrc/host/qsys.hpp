@@ -10,6 +10,7 @@
 
 #include <cstdint>
 #include <string>
+#include <array>
 #include <vector>
 
 namespace qsys {
@@ -99,13 +100,28 @@ struct MDInput {
   std::string thermostat = "berendsen", integrator = "leap-frog";
 };
 
-// FEP file content used by this engine: [FEP] states, [atoms] and
-// [change_charges] (src/qatom.f90:301-760).
+// EVB off-diagonal function H_ij = A exp(-mu r_kl) (src/qatom.f90:1809-1850);
+// states and Q atoms 0-based.
+struct OffDiag { int i, j, k, l; double A, mu; };
+
+// FEP file content used by this engine: [FEP] states and
+// softcore_use_max_potential, [atoms], [change_charges], [atom_types],
+// [change_atoms], [softcore] and [off_diagonals] (src/qatom.f90:301-1975).
 struct FepData {
   int nstates = 0;
   std::vector<int> iqseq;                 // topology atom (0-based) per Q atom
   std::vector<std::vector<double>> qcrg;  // [nqat][nstates], unscaled
   bool have_charges = false;
+  // Q-atom LJ types: qavdw/qbvdw [nqlib][3] (LJ codes 1..3), qmass; with
+  // [change_atoms] every Q atom has a type per state (qvdw_flag)
+  std::vector<std::string> qtac;
+  std::vector<std::array<double, 3>> qavdw, qbvdw;
+  std::vector<double> qmass;
+  bool qvdw = false;
+  std::vector<std::vector<int>> qiac;     // [nqat][nstates], 0-based rows of qtac
+  bool softcore = false, softcore_max_potential = false;
+  std::vector<std::vector<double>> alpha_max;  // [nqat][nstates]
+  std::vector<OffDiag> offd;
 };
 
 struct ShakeBond { int i, j; double dist2; };
@@ -158,6 +174,16 @@ struct Setup {
   std::vector<double> lambda;
   std::vector<int> qconn;             // [s][iq][atom] connectivity (1..9)
   std::vector<std::vector<NbqPair>> nbqq;  // per state
+  // Q-atom types (qvdw_flag): qiac [iq*nstates+s] (0-based), qavdw/qbvdw
+  // [type*3 + LJ code-1]; soft-core table sc_lookup(iq, column, s) with
+  // columns 0..natyps-1 topology types and natyps+jq Q atoms, stored
+  // [(iq*(natyps+nqat) + column)*nstates + s] (src/qatom.f90:1852-1975)
+  bool qvdw_flag = false;
+  std::vector<int> qiac;
+  std::vector<double> qavdw, qbvdw;
+  std::vector<double> sc_lookup;
+  std::vector<OffDiag> offd;
+  int ff_type = 1;
 };
 
 // Readers.  Each throws std::runtime_error with a message on bad input.

@@ -517,8 +517,10 @@ FepData read_fep(const std::string& path, int natom) {
   int nq = (int)f.iqseq.size();
   f.qcrg.assign(nq, std::vector<double>(f.nstates, 0.0));
   for (const char* unsup : {"CHANGE_BONDS", "CHANGE_ANGLES", "CHANGE_TORSIONS", "CHANGE_IMPROPERS",
-                            "ATOM_TYPES", "CHANGE_ATOMS", "SOFT_PAIRS", "EXCLUDED_PAIRS",
-                            "OFF-DIAGONALS", "SHAKE_CONSTRAINTS", "MONITOR_GROUPS", "SOFTCORE"}) {
+                            "BOND_TYPES", "ANGLE_TYPES", "TORSION_TYPES", "IMPROPER_TYPES",
+                            "ANGLE_COUPLINGS", "TORSION_COUPLINGS", "IMPROPER_COUPLINGS",
+                            "SOFT_PAIRS", "EXCLUDED_PAIRS", "EL_SCALE", "SHAKE_CONSTRAINTS",
+                            "MONITOR_GROUPS", "MONITOR_GROUP_PAIRS"}) {
     if (S.has(unsup) && !S.sec.at(unsup).empty())
       throw std::runtime_error(std::string("FEP section [") + unsup + "] is not supported");
   }
@@ -532,6 +534,75 @@ FepData read_fep(const std::string& path, int natom) {
       if (iq < 1 || iq > nq || (int)tk.size() < 1 + f.nstates) throw std::runtime_error("bad [change_charges] line");
       for (int s = 0; s < f.nstates; ++s) f.qcrg[iq - 1][s] = to_d(tk[1 + s]);
     }
+  }
+  // Q-atom type library: name, (A, B) for LJ codes 1..3, mass
+  if (S.has("ATOM_TYPES")) {
+    for (auto& l : S.sec.at("ATOM_TYPES")) {
+      std::vector<std::string> tk;
+      Records::tokens(l, tk);
+      if (tk.empty()) continue;
+      if (tk.size() < 8) throw std::runtime_error("bad [atom_types] line in " + path);
+      f.qtac.push_back(tk[0]);
+      f.qavdw.push_back({to_d(tk[1]), to_d(tk[3]), to_d(tk[5])});
+      f.qbvdw.push_back({to_d(tk[2]), to_d(tk[4]), to_d(tk[6])});
+      f.qmass.push_back(to_d(tk[7]));
+    }
+  }
+  // per-state Q-atom types: a line for every Q atom (qvdw_flag)
+  if (S.has("CHANGE_ATOMS") && !S.sec.at("CHANGE_ATOMS").empty()) {
+    if (f.qtac.empty())
+      throw std::runtime_error("[change_atoms] needs an [atom_types] library (topology types are not supported)");
+    f.qvdw = true;
+    f.qiac.assign(nq, std::vector<int>(f.nstates, -1));
+    for (auto& l : S.sec.at("CHANGE_ATOMS")) {
+      std::vector<std::string> tk;
+      Records::tokens(l, tk);
+      if (tk.empty()) continue;
+      int iq = (int)to_i(tk[0]);
+      if (iq < 1 || iq > nq || (int)tk.size() < 1 + f.nstates) throw std::runtime_error("bad [change_atoms] line");
+      for (int s = 0; s < f.nstates; ++s) {
+        auto it = std::find(f.qtac.begin(), f.qtac.end(), tk[1 + s]);
+        if (it == f.qtac.end()) throw std::runtime_error("Q-atom type " + tk[1 + s] + " has not been defined");
+        f.qiac[iq - 1][s] = (int)(it - f.qtac.begin());
+      }
+    }
+    for (auto& r : f.qiac)
+      for (int t : r)
+        if (t < 0) throw std::runtime_error("Atom types of Q-atoms must be given for every Q-atom");
+  }
+  // soft-core alphas for every Q atom and state
+  if (S.get("FEP", "softcore_use_max_potential", v)) parse_onoff(v, f.softcore_max_potential);
+  if (S.has("SOFTCORE") && !S.sec.at("SOFTCORE").empty()) {
+    if (!f.qvdw) throw std::runtime_error("Q-atom types must be redefined in [change_atoms] for [softcore]");
+    f.softcore = true;
+    f.alpha_max.assign(nq, std::vector<double>(f.nstates, 0.0));
+    int n = 0;
+    for (auto& l : S.sec.at("SOFTCORE")) {
+      std::vector<std::string> tk;
+      Records::tokens(l, tk);
+      if (tk.empty()) continue;
+      int iq = (int)to_i(tk[0]);
+      if (iq < 1 || iq > nq || (int)tk.size() < 1 + f.nstates) throw std::runtime_error("bad [softcore] line");
+      for (int s = 0; s < f.nstates; ++s) f.alpha_max[iq - 1][s] = to_d(tk[1 + s]);
+      ++n;
+    }
+    if (n != nq) throw std::runtime_error("Alpha must be given for every Q-atom");
+  }
+  // EVB off-diagonal functions
+  if (S.has("OFF_DIAGONALS")) {
+    for (auto& l : S.sec.at("OFF_DIAGONALS")) {
+      std::vector<std::string> tk;
+      Records::tokens(l, tk);
+      if (tk.empty()) continue;
+      if (tk.size() < 6) throw std::runtime_error("bad [off_diagonals] line");
+      OffDiag o{(int)to_i(tk[0]) - 1, (int)to_i(tk[1]) - 1, (int)to_i(tk[2]) - 1, (int)to_i(tk[3]) - 1,
+                to_d(tk[4]), to_d(tk[5])};
+      if (o.i < 0 || o.i >= f.nstates || o.j < 0 || o.j >= f.nstates || o.i == o.j)
+        throw std::runtime_error("Invalid combination of states in [off_diagonals]");
+      if (o.k < 0 || o.k >= nq || o.l < 0 || o.l >= nq) throw std::runtime_error("Invalid q-atom number in [off_diagonals]");
+      f.offd.push_back(o);
+    }
+    if ((int)f.offd.size() > kMaxStates) throw std::runtime_error("Too many off-diagonal functions");
   }
   return f;
 }

@@ -13,7 +13,7 @@
 from __future__ import annotations
 
 import struct
-from typing import List, Sequence, Tuple
+from typing import Optional, List, Sequence, Tuple
 
 import numpy as np
 
@@ -26,10 +26,12 @@ def _rec(payload: bytes) -> bytes:
 
 
 def write_energy_file(path: str, lambdas: Sequence[float], frames: np.ndarray,
-                      version: str = "qgpu") -> None:
+                      version: str = "qgpu", offd: Optional[np.ndarray] = None) -> None:
     """frames: (nframes, nstates, 14) rows of
     total q(bond angle torsion improper) qx(el vdw) qq(el vdw) qp(el vdw)
-    qw(el vdw) restraint, for one energy array (no group contributions)."""
+    qw(el vdw) restraint, for one energy array (no group contributions);
+    offd: optional (nframes, noffd, 4) off-diagonal entries (state i, state
+    j, Hij, rkl), written as OFFDIAG_SAVE records (src/nrgy.f90:73-78)."""
     nstates = len(lambdas)
     # canary 1337, arrays=1, totresid=1, types(1)=NOGC(=3), numres(1)=1,
     # resid(1)=-1, gcnum(1)=-1, version char(80)
@@ -40,9 +42,44 @@ def write_energy_file(path: str, lambdas: Sequence[float], frames: np.ndarray,
             row = f[s]
             out.append(_rec(struct.pack("<i", s + 1) + struct.pack("<d", lambdas[s]) +
                             np.asarray(row, dtype="<f8").tobytes()))
-        out.append(_rec(b""))  # no off-diagonal elements
+        if offd is None or len(offd) == 0:
+            out.append(_rec(b""))  # no off-diagonal elements
+        else:
+            k = len(out) // (nstates + 1)
+            out.append(_rec(b"".join(struct.pack("<iidd", int(o[0]), int(o[1]), float(o[2]), float(o[3]))
+                                     for o in offd[k])))
     with open(path, "wb") as fh:
         fh.write(b"".join(out))
+
+
+def read_energy_frames(path: str, nstates: int):
+    """All frames of a qdyn energy file (put_ene, src/nrgy.f90:113-142):
+    returns (lambdas (nstates,), rows (nframes, nstates, 14), offd (nframes,
+    noffd, 4) with state i, state j, Hij, rkl)."""
+    data = open(path, "rb").read()
+    recs: List[bytes] = []
+    p = 0
+    while p < len(data):
+        (n,) = struct.unpack_from("<i", data, p)
+        recs.append(data[p + 4:p + 4 + n])
+        p += 8 + n
+    body = recs[1:]
+    per = nstates + 1
+    nfr = len(body) // per
+    lam = np.zeros(nstates)
+    rows = np.zeros((nfr, nstates, 14))
+    noffd = len(body[nstates]) // 24 if nfr else 0
+    offd = np.zeros((nfr, noffd, 4))
+    for k in range(nfr):
+        for s in range(nstates):
+            r = body[k * per + s]
+            lam[s] = struct.unpack_from("<d", r, 4)[0]
+            rows[k, s] = np.frombuffer(r, dtype="<f8", count=14, offset=12)
+        ro = body[k * per + nstates]
+        for m in range(noffd):
+            i, j, h, rkl = struct.unpack_from("<iidd", ro, 24 * m)
+            offd[k, m] = (i, j, h, rkl)
+    return lam, rows, offd
 
 
 def read_energy_file(path: str, nstates: int) -> Tuple[np.ndarray, np.ndarray]:

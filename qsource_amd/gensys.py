@@ -87,6 +87,14 @@ class System:
     # Q region (optional)
     qatoms: Optional[np.ndarray] = None    # (nqat,) topology atom numbers
     qcrg: Optional[np.ndarray] = None      # (nqat, nstates)
+    # optional per-state Q-atom types ([atom_types] + [change_atoms]):
+    # qtypes rows (name, A1, B1, A2, B2, A3, B3, mass), qtype_of (nqat, nstates)
+    # 0-based rows of qtypes
+    qtypes: Optional[list] = None
+    qtype_of: Optional[np.ndarray] = None
+    softcore: Optional[np.ndarray] = None  # (nqat, nstates) alphas ([softcore])
+    offdiag: Optional[list] = None         # (state_i, state_j, q_k, q_l, A, mu), 1-based
+    ff_type: int = 1                       # 1 GROMOS (harmonic impropers), 2 AMBER (periodic)
     # off-diagonal (EVB) couplings, optional: rows (i, j, k, l, A, mu)
     offdiag: Optional[list] = None
 
@@ -323,12 +331,22 @@ def water_droplet(radius: float = 15.0, seed: int = 1001) -> System:
 
 def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
                     n_states: int = 0, n_types: int = 250,
-                    q_sd: float = 0.3) -> System:
+                    q_sd: float = 0.3, q_types: bool = False,
+                    softcore: Optional[float] = None, n_offdiag: int = 0,
+                    impropers: int = 0) -> System:
     """A TIP3P sphere of the given radius around an optional branched solute
     (charges N(0,q_sd) neutralised per 4-atom charge group, sigma~U[2.5,3.8],
     eps~U[0.05,0.2] quantised into at most n_types atom types), with an
     optional Q region of n_q solute atoms and n_states FEP states whose
-    charges are drawn independently and neutralised per 4-atom block."""
+    charges are drawn independently and neutralised per 4-atom block.
+
+    q_types: every Q atom also gets an LJ type per state (sigma, eps drawn as
+    for the solute), which switches qdyn to Q-atom types (qvdw);
+    softcore: with q_types, soft-core alphas drawn from U[0, softcore] per Q
+    atom and state; n_offdiag: EVB couplings A*exp(-mu r) with
+    A~U[10,60], mu~U[0,0.5] between random state pairs and Q-atom pairs;
+    impropers: 1 adds harmonic (GROMOS) and 2 periodic (AMBER) impropers on
+    the solute's branch points."""
     rng = np.random.default_rng(seed)
     # --- solute
     if n_solute > 0:
@@ -483,6 +501,38 @@ def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
             qc[:, s] = q
         sysm.qatoms = qa
         sysm.qcrg = qc
+        if q_types:
+            rows, tof = [], np.zeros((len(qa), n_states), dtype=np.int64)
+            for k, a in enumerate(qa):
+                mass = float(sysm.masses[sysm.iac[a - 1] - 1])
+                for st in range(n_states):
+                    sig, eps = rng.uniform(2.5, 3.8), rng.uniform(0.05, 0.2)
+                    a1, b1 = lj_sqrt_ab(sig, eps)
+                    a3, b3 = lj_sqrt_ab(0.8 * sig, 0.5 * eps)
+                    tof[k, st] = len(rows)
+                    rows.append(("Q%d_%d" % (k + 1, st + 1), a1, b1, a1, b1, a3, b3, mass))
+            sysm.qtypes, sysm.qtype_of = rows, tof
+            if softcore:
+                sysm.softcore = np.round(rng.uniform(0.0, softcore, (len(qa), n_states)), 2)
+        if n_offdiag > 0 and n_states > 1:
+            off = []
+            for _ in range(n_offdiag):
+                si, sj = rng.choice(n_states, 2, replace=False)
+                qk, ql = rng.choice(len(qa), 2, replace=False)
+                off.append((int(si) + 1, int(sj) + 1, int(qk) + 1, int(ql) + 1,
+                            round(float(rng.uniform(10, 60)), 2), round(float(rng.uniform(0, 0.5)), 3)))
+            sysm.offdiag = off
+    if impropers and n_solute > 0:
+        # one improper per branch point j with neighbours (i, k, l): i-j-k-l
+        nbr = [[] for _ in range(n_solute)]
+        for i, j in sb:
+            nbr[i].append(j)
+            nbr[j].append(i)
+        imps = [(nb[0] + 1, j + 1, nb[1] + 1, nb[2] + 1, 1) for j, nb in enumerate(nbr) if len(nb) >= 3]
+        sysm.imps = np.array(imps, dtype=np.int64).reshape(-1, 5)
+        sysm.nimps_solute = len(imps)
+        sysm.ff_type = 1 if impropers == 1 else 2
+        sysm.implib = np.array([[3.0, 0.0]]) if impropers == 1 else np.array([[1.5, 180.0]])
     return sysm
 
 
@@ -504,7 +554,7 @@ def write_top(s: System, path: str) -> None:
          "DATE       20260101",
          "VERSION     5.06",
          "FORCEFIELD synthetic",
-         "FF_TYPE    1",
+         "FF_TYPE    %d" % s.ff_type,
          "END        of header"]
     N = s.natom
     L.append("%8d%8d%8d = Total no. of atoms, no. of solute atoms, solvent atoms" % (N, s.nat_solute, 3))
@@ -528,7 +578,7 @@ def write_top(s: System, path: str) -> None:
         L.append("%5d%10.3f%10.3f%10.3f%10.3f" % (k + 1, fk, m, dt, p))
     L.append("%8d%8d = No. of impropers, solute impr. i - j - k - l - icode: (2 per line)" % (len(s.imps), s.nimps_solute))
     L += _fmt_list([tuple(t) for t in s.imps], 2, "%6d%6d%6d%6d%4d")
-    L.append("%8d%8d = No. of improper codes, type (1=harmonic,2=periodic). Parameters:" % (len(s.implib), 1))
+    L.append("%8d%8d = No. of improper codes, type (1=harmonic,2=periodic). Parameters:" % (len(s.implib), s.ff_type))
     for k, (fk, i0) in enumerate(s.implib):
         L.append("%5d%10.3f%10.3f" % (k + 1, fk, i0))
     L.append("%8d = No. of atomic charges" % N)
@@ -588,15 +638,36 @@ def write_top(s: System, path: str) -> None:
 
 
 def write_fep(s: System, path: str) -> None:
-    """FEP file with [FEP] states, [atoms] (Q number, topology atom) and
-    [change_charges] (Q number, charge per state)."""
+    """FEP file with [FEP] states, [atoms] (Q number, topology atom),
+    [change_charges] (Q number, charge per state) and, when present,
+    [atom_types] / [change_atoms] (Q-atom LJ types per state), [softcore]
+    (alpha per state) and [off_diagonals] (states i j, Q atoms k l, A, mu),
+    in the layout qatom_load_fep reads (src/qatom.f90:700-1975)."""
     nq, ns = s.qcrg.shape
-    L = ["[FEP]", "states %d" % ns, "", "[atoms]"]
+    L = ["[FEP]", "states %d" % ns]
+    if s.softcore is not None:
+        L.append("softcore_use_max_potential off")
+    L += ["", "[atoms]"]
     for k, a in enumerate(s.qatoms):
         L.append("%d %d" % (k + 1, a))
     L += ["", "[change_charges]"]
     for k in range(nq):
         L.append("%d " % (k + 1) + " ".join("%.4f" % c for c in s.qcrg[k]))
+    if s.qtypes is not None:
+        L += ["", "[atom_types]"]
+        for name, a1, b1, a2, b2, a3, b3, m in s.qtypes:
+            L.append("%-8s %12.4f %10.4f %12.4f %10.4f %12.4f %10.4f %8.3f" % (name, a1, b1, a2, b2, a3, b3, m))
+        L += ["", "[change_atoms]"]
+        for k in range(nq):
+            L.append("%d " % (k + 1) + " ".join(s.qtypes[t][0] for t in s.qtype_of[k]))
+    if s.softcore is not None:
+        L += ["", "[softcore]"]
+        for k in range(nq):
+            L.append("%d " % (k + 1) + " ".join("%.2f" % a for a in s.softcore[k]))
+    if s.offdiag:
+        L += ["", "[off_diagonals]"]
+        for si, sj, qk, ql, A, mu in s.offdiag:
+            L.append("%d %d %d %d %.2f %.3f" % (si, sj, qk, ql, A, mu))
     with open(path, "w") as f:
         f.write("\n".join(L) + "\n")
 

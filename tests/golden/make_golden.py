@@ -10,7 +10,10 @@ qsource_amd.gensys, and stores, per case:
 * ``x``, ``v``: the final restart coordinates and velocities (FP64);
 * ``theta_corr``: the water-polarisation shell corrections of the restart;
 * ``top_sha``: SHA-256 of the generated topology, so a change of the
-  generator is detected instead of silently comparing different systems.
+  generator is detected instead of silently comparing different systems;
+* ``eq_frames``, ``offd_frames`` (cases that write an energy file): the
+  reference's energy-file frames (per-state Q energies in FP64 and the
+  off-diagonal elements) of every recorded step.
 
 Usage:  python tests/golden/make_golden.py
 """
@@ -25,7 +28,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
-from qsource_amd import gensys  # noqa: E402
+from qsource_amd import fep, gensys  # noqa: E402
 
 QDYN = os.path.join(ROOT, "oracle", "_ref", "qdyn5")
 OUT = os.path.dirname(os.path.abspath(__file__))
@@ -37,6 +40,19 @@ CASES = {
     "c2_r14_q8_30": (dict(radius=14.0, n_solute=40, seed=2001, n_q=8, n_states=2),
                      dict(steps=30, lambdas=[0.7, 0.3], stepsize=0.5,
                           temperature=50.0, bath_coupling=2.0, initial_temperature=50.0)),
+    # EVB-style Q region: 4 states with per-state Q-atom types (qvdw),
+    # soft-core alphas and off-diagonal couplings; harmonic (GROMOS) or
+    # periodic (AMBER) impropers; energy file written every step
+    "c5_r12_q8s4_gromos_30": (dict(radius=12.0, n_solute=40, seed=2101, n_q=8, n_states=4, q_types=True,
+                                   softcore=20.0, n_offdiag=3, impropers=1),
+                              dict(steps=30, lambdas=[0.4, 0.3, 0.2, 0.1], stepsize=0.5, temperature=50.0,
+                                   bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
+                                   energy_interval=1)),
+    "c5_r12_q8s4_amber_30": (dict(radius=12.0, n_solute=40, seed=2102, n_q=8, n_states=4, q_types=True,
+                                  softcore=20.0, n_offdiag=3, impropers=2),
+                             dict(steps=30, lambdas=[0.25, 0.25, 0.25, 0.25], stepsize=0.5, temperature=50.0,
+                                  bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
+                                  energy_interval=1)),
 }
 
 ROW = re.compile(r"^(solute|solvent|solute-solvent|LRF|Q-atom|restraints|SUM)\s+(.*)$")
@@ -95,9 +111,16 @@ def main():
             en = np.array([flatten(x) for x in summ])
             x, v, tc = gensys.read_restart(paths["re"])
             sha = hashlib.sha256(open(paths["top"], "rb").read()).hexdigest()
+            extra = {}
+            if inpargs.get("energy"):
+                # the reference's own energy file: per-state Q energies (FP64)
+                # and off-diagonal elements of every recorded step
+                _, rows, offd = fep.read_energy_frames(os.path.join(d, inpargs["energy"]),
+                                                       len(inpargs["lambdas"]))
+                extra = dict(eq_frames=rows, offd_frames=offd)
             np.savez_compressed(os.path.join(OUT, name + ".npz"), energies=en, columns=np.array(COLUMNS),
                                 x=x, v=v, theta_corr=tc if tc is not None else np.zeros(0),
-                                top_sha=np.array(sha))
+                                top_sha=np.array(sha), **extra)
             print(name, en.shape, x.shape, sha[:12])
 
 

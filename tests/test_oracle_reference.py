@@ -23,6 +23,16 @@ CASES = {
     "c2_r14_q8_30": (dict(radius=14.0, n_solute=40, seed=2001, n_q=8, n_states=2),
                      dict(steps=30, lambdas=[0.7, 0.3], stepsize=0.5,
                           temperature=50.0, bath_coupling=2.0, initial_temperature=50.0)),
+    "c5_r12_q8s4_gromos_30": (dict(radius=12.0, n_solute=40, seed=2101, n_q=8, n_states=4, q_types=True,
+                                   softcore=20.0, n_offdiag=3, impropers=1),
+                              dict(steps=30, lambdas=[0.4, 0.3, 0.2, 0.1], stepsize=0.5, temperature=50.0,
+                                   bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
+                                   energy_interval=1)),
+    "c5_r12_q8s4_amber_30": (dict(radius=12.0, n_solute=40, seed=2102, n_q=8, n_states=4, q_types=True,
+                                  softcore=20.0, n_offdiag=3, impropers=2),
+                             dict(steps=30, lambdas=[0.25, 0.25, 0.25, 0.25], stepsize=0.5, temperature=50.0,
+                                  bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
+                                  energy_interval=1)),
 }
 
 ORACLE_COLS = {"ww_el": "ww_el", "ww_vdw": "ww_vdw", "lrf": "lrf", "rst_total": "rst_total",
@@ -30,7 +40,7 @@ ORACLE_COLS = {"ww_el": "ww_el", "ww_vdw": "ww_vdw", "lrf": "lrf", "rst_total": 
                "total": "total", "potential": "potential", "kinetic": "kinetic",
                "pp_el": "pp_el", "pp_vdw": "pp_vdw", "pw_el": "pw_el", "pw_vdw": "pw_vdw",
                "p_bond": "p_bond", "p_angle": "p_angle", "p_torsion": "p_torsion",
-               "qx_el": "qx_el", "qx_vdw": "qx_vdw"}
+               "p_improper": "p_improper", "qx_el": "qx_el", "qx_vdw": "qx_vdw"}
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
@@ -58,4 +68,31 @@ def test_oracle_matches_reference_qdyn(name, tmp_path):
     x, v = o.state()
     assert np.array_equal(x, g["x"].ravel()), "final coordinates differ from reference qdyn"
     assert np.array_equal(v, g["v"].ravel()), "final velocities differ from reference qdyn"
+
+
+ENERGY_FILE_CASES = sorted(k for k, (_, inp) in CASES.items() if inp.get("energy"))
+
+
+@pytest.mark.parametrize("name", ENERGY_FILE_CASES)
+def test_oracle_energy_file_frames(name, tmp_path):
+    """Per-state Q energies and EVB off-diagonal elements of every recorded
+    step, against the reference's own energy file (FP64 values): frame k is
+    written at step k (k >= 1) from the coordinates that step starts with."""
+    g = np.load(os.path.join(GOLD, name + ".npz"), allow_pickle=False)
+    sysargs, inpargs = CASES[name]
+    s = gensys.solvated_system(**sysargs)
+    paths = gensys.write_system(s, str(tmp_path), name, output=1, temp_interval=1, **inpargs)
+    rows, offd = g["eq_frames"], g["offd_frames"]
+    assert rows.shape[0] == inpargs["steps"] - 1 and offd.shape[1] == sysargs["n_offdiag"]
+    o = Oracle(paths["inp"])
+    o.run(1)
+    for k in range(1, inpargs["steps"]):
+        od = o.offdiag()
+        assert np.allclose(od[:, 0], offd[k - 1, :, 2], rtol=1e-13, atol=0), (k, od, offd[k - 1])
+        assert np.allclose(od[:, 1], offd[k - 1, :, 3], rtol=1e-13, atol=0)
+        o.run(1)
+        q = o.qenergies()           # (nstates, 7): qq el/vdw, qp el/vdw, qw el/vdw, total
+        ref = rows[k - 1]           # put_ene columns 7..12: qq, qp, qw (el, vdw); 0: total
+        assert np.allclose(q[:, :6], ref[:, 7:13], rtol=1e-12, atol=1e-10), (k, q[:, :6], ref[:, 7:13])
+        assert np.allclose(q[:, 6], ref[:, 0], rtol=1e-12, atol=1e-10)
     assert np.array_equal(o.theta_corr(), g["theta_corr"])
