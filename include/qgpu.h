@@ -113,6 +113,16 @@ int qgpu_energy_series(qgpu_handle* h, int k, double* out, int cap, int* nframes
  * of dU and dG.  No counterpart in the reference (it has one precision). */
 int qgpu_energy_series_fp64(qgpu_handle* h, int k, double* out, int cap, int* nframes);
 
+/* Number of EVB off-diagonal functions ([off_diagonals] lines of the FEP
+ * file, src/qatom.f90:1809-1850) of instance k. */
+int qgpu_noffdiag(qgpu_handle* h, int k, int* n);
+
+/* Off-diagonal elements of the recorded energy frames of instance k:
+ * nframes x noffd x (Hij, r_kl), computed on the frame's coordinates as
+ * offdiag does (src/md.f90:15444-15470) and written after each frame's state
+ * records by put_ene (src/nrgy.f90:136-138).  Writes at most `cap` doubles. */
+int qgpu_offdiag_series(qgpu_handle* h, int k, double* out, int cap, int* nframes);
+
 /* Exports pair list `which` of instance k in the reference's layout and
  * order (NB_TYPE i, j, LJcod; 0-based atoms): 0 = nbpp, 1 = nbpw, 2 = nbww
  * (src/md.f90:329-433); 3 = nbpp as the force kernel evaluates it (charge-

@@ -157,6 +157,7 @@ void Batch::set_energy_interval(int iene, int max_frames) {
     D.eseries64 = precision_ == 1 && D.nqat > 0
                       ? inst_[k]->alloc<double>((size_t)std::max(1, max_frames) * std::max(1, D.nstates) * kEneRecord)
                       : nullptr;
+    D.eseries_offd = D.noffd > 0 ? inst_[k]->alloc<double>((size_t)std::max(1, max_frames) * D.noffd * 2) : nullptr;
     D.eseries_n = inst_[k]->alloc<int>(1);
   }
   iene_ = iene;
@@ -182,6 +183,19 @@ int Batch::energy_series(int k, double* out, int cap, bool fp64) {
   return n;
 }
 
+int Batch::offdiag_series(int k, double* out, int cap) {
+  HIPCHECK(hipStreamSynchronize(stream_));
+  const DevSys& D = h_sys_[k];
+  if (!D.eseries_n) return 0;
+  int n = 0;
+  HIPCHECK(hipMemcpy(&n, D.eseries_n, sizeof(int), hipMemcpyDeviceToHost));
+  n = std::min(n, D.eseries_cap);
+  const size_t per = (size_t)D.noffd * 2;
+  const size_t want = per ? std::min((size_t)cap / per, (size_t)n) : 0;
+  if (out && want) HIPCHECK(hipMemcpy(out, D.eseries_offd, want * per * 8, hipMemcpyDeviceToHost));
+  return n;
+}
+
 Batch::~Batch() {
   if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
   for (auto* i : inst_) delete i;
@@ -199,7 +213,6 @@ void Batch::upload(int k) {
   const qsys::Setup& s = *setups_[k];
   const qsys::Topology& t = s.top;
   if (s.inp.freeze) throw std::runtime_error("excluded_freeze is not supported");
-  if (!s.imp.empty()) throw std::runtime_error("improper torsions are not supported by the GPU engine");
   if (s.nqat > 0 && !(s.inp.rcq > t.rexcl_o))
     throw std::runtime_error("Q-atom cut-off must exceed the exclusion radius (static Q lists)");
   if (s.nstates > kMaxQStates) throw std::runtime_error("too many FEP states");
@@ -353,6 +366,23 @@ void Batch::upload(int k) {
   D.iqseq = I->upload(s.iqseq);
   D.qcrg = I->upload(s.qcrg);
   D.lambda = I->upload(s.lambda);
+  D.qvdw = s.qvdw_flag ? 1 : 0;
+  D.qiac = I->upload(s.qiac);
+  D.qavdw = I->upload(s.qavdw);
+  D.qbvdw = I->upload(s.qbvdw);
+  D.sc_ncol = s.natyps + s.nqat;
+  D.sc_lookup = I->upload(s.sc_lookup);
+  {
+    std::vector<int> kl;
+    std::vector<double> amu;
+    for (const auto& o : s.offd) {
+      kl.push_back(s.iqseq[o.k]); kl.push_back(s.iqseq[o.l]);
+      amu.push_back(o.A); amu.push_back(o.mu);
+    }
+    D.noffd = (int)s.offd.size();
+    D.offd_kl = I->upload(kl);
+    D.offd_amu = I->upload(amu);
+  }
   std::vector<int> qq_atom_i, qq_atom_j;   // atoms of each Q pair, for the term slots
   {
     const int nq = s.nqat, nsl = s.nat_solute;
@@ -431,6 +461,13 @@ void Batch::upload(int k) {
     for (auto& b : t.bondlib) { bl.push_back(b.fk); bl.push_back(b.b0); }
     for (auto& a : t.anglib) { al.push_back(a.fk); al.push_back(a.ang0); }
     for (auto& q : t.torlib) { tl.push_back(q.fk); tl.push_back(q.rmult); tl.push_back(q.deltor); tl.push_back(q.paths); }
+    std::vector<int> iv;
+    std::vector<double> il;
+    for (auto& q : s.imp) { iv.push_back(q.i); iv.push_back(q.j); iv.push_back(q.k); iv.push_back(q.l); iv.push_back(q.cod); }
+    for (auto& q : t.implib) { il.push_back(q.fk); il.push_back(q.imp0); }
+    D.imp = I->upload(iv); D.implib = I->upload(il);
+    D.nimp = (int)s.imp.size(); D.nimp_solute = s.nimps_solute;
+    D.ff_type = s.ff_type;
     D.bnd = I->upload(bv); D.ang = I->upload(av2); D.tor = I->upload(tv);
     D.bondlib = I->upload(bl); D.anglib = I->upload(al); D.torlib = I->upload(tl);
     D.nbnd = (int)s.bnd.size(); D.nbnd_solute = s.nbonds_solute;
@@ -438,13 +475,15 @@ void Batch::upload(int k) {
     D.ntor = (int)s.tor.size(); D.ntor_solute = s.ntors_solute;
     // term slots (bonds, angles, torsions, then the Q pairs) and the CSR of
     // every atom's slots in slot order
-    D.e_bterm = I->alloc<double>((size_t)std::max(1, D.nbnd + D.nang + D.ntor));
+    D.e_bterm = I->alloc<double>((size_t)std::max(1, D.nbnd + D.nang + D.ntor + D.nimp));
     std::vector<std::vector<int>> inc(N);
     int slot = 0;
     for (int n = 0; n < D.nbnd; ++n) for (int a : {s.bnd[n].i, s.bnd[n].j}) inc[a].push_back(slot++);
     for (int n = 0; n < D.nang; ++n) for (int a : {s.ang[n].i, s.ang[n].j, s.ang[n].k}) inc[a].push_back(slot++);
     for (int n = 0; n < D.ntor; ++n)
       for (int a : {s.tor[n].i, s.tor[n].j, s.tor[n].k, s.tor[n].l}) inc[a].push_back(slot++);
+    for (int n = 0; n < D.nimp; ++n)
+      for (int a : {s.imp[n].i, s.imp[n].j, s.imp[n].k, s.imp[n].l}) inc[a].push_back(slot++);
     for (int p = 0; p < D.nqq; ++p)
       for (int a : {qq_atom_i[p], qq_atom_j[p]}) inc[a].push_back(slot++);
     std::vector<int> sa, sf, sl;
@@ -462,7 +501,7 @@ void Batch::upload(int k) {
     D.slot_g = I->alloc<double>((size_t)std::max(1, slot) * 3);
     D.nslot_atoms = (int)sa.size();
     D.slot_atom = I->upload(sa); D.slot_first = I->upload(sf); D.slot_pos = I->upload(pos);
-    max_terms_ = std::max(max_terms_, D.nbnd + D.nang + D.ntor + D.nqq);
+    max_terms_ = std::max(max_terms_, D.nbnd + D.nang + D.ntor + D.nimp + D.nqq);
     max_slot_atoms_ = std::max(max_slot_atoms_, D.nslot_atoms);
   }
   // integrator
@@ -750,7 +789,7 @@ void Batch::energies(int k, double* out, int cap) {
   if (s.nstates > 0)
     HIPCHECK(hipMemcpy(q.data(), h_sys_[k].qenergies, 8 * q.size(), hipMemcpyDeviceToHost));
   double p_bond = E[E_P_BOND], w_bond = E[E_W_BOND], p_angle = E[E_P_ANGLE], w_angle = E[E_W_ANGLE];
-  double p_tor = E[E_P_TORSION], w_tor = E[E_W_TORSION], p_imp = 0, w_imp = 0;
+  double p_tor = E[E_P_TORSION], w_tor = E[E_W_TORSION], p_imp = E[E_P_IMPROPER], w_imp = E[E_W_IMPROPER];
   // lambda-weighted Q sums of pot_energy (src/md.f90:16186-16211); no Q
   // bonded terms or Q restraints in the supported FEP files
   double q_bond = 0, q_angle = 0, q_tor = 0, q_imp = 0, qx_el = 0, qx_vdw = 0, rst_protein = 0;
@@ -1069,6 +1108,15 @@ int qgpu_energy_series_fp64(qgpu_handle* h, int k, double* out, int cap, int* nf
   return guarded([&] {
     CHECK_K(h, k);
     *nframes = h->b->energy_series(k, out, cap, true);
+  });
+}
+int qgpu_noffdiag(qgpu_handle* h, int k, int* n) {
+  return guarded([&] { CHECK_K(h, k); *n = h->b->noffdiag(k); });
+}
+int qgpu_offdiag_series(qgpu_handle* h, int k, double* out, int cap, int* nframes) {
+  return guarded([&] {
+    CHECK_K(h, k);
+    *nframes = h->b->offdiag_series(k, out, cap);
   });
 }
 int qgpu_list_counts(qgpu_handle* h, int k, long* counts) {

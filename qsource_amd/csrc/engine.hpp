@@ -45,7 +45,7 @@ constexpr int kNumShellMax = 8;
 // per-instance energy slots reduced on demand (index into DevSys::eacc)
 enum EnergyTerm {
   E_PP_EL, E_PP_VDW, E_PW_EL, E_PW_VDW, E_WW_EL, E_WW_VDW, E_LRF,
-  E_P_BOND, E_P_ANGLE, E_P_TORSION, E_P_IMPROPER, E_W_BOND, E_W_ANGLE, E_W_TORSION,
+  E_P_BOND, E_P_ANGLE, E_P_TORSION, E_P_IMPROPER, E_W_BOND, E_W_ANGLE, E_W_TORSION, E_W_IMPROPER,
   E_RST_FIX, E_RST_SHELL, E_RST_RADIAL, E_RST_WPOL,
   E_KINETIC,
   E_NTERMS
@@ -122,6 +122,14 @@ struct DevSys {
   int nqp_atoms, nqw_waters;
   const int* qp_atoms;           // solute atoms in the Q-solute list, ascending
   const int* qw_waters;          // waters in the Q-water list, ascending
+  // Q-atom LJ types (qvdw_flag): qiac [iq*nstates+s]; qavdw/qbvdw [type*3 +
+  // LJ code-1]; soft-core sc_lookup [(iq*sc_ncol + column)*nstates + s] with
+  // columns natyps topology types then nqat Q atoms (all 0 without softcore)
+  int qvdw;
+  const int* qiac;
+  const double *qavdw, *qbvdw;
+  const double* sc_lookup;
+  int sc_ncol;
   double* e_q;                   // nqat x nstates x 4: qp_el qp_vdw qw_el qw_vdw
   double* q_part;                // nqat x kQChunks x kQPartStride partial sums
   double *q_part64, *e_q64;      // mixed precision: FP64 re-evaluation on energy-file steps
@@ -137,10 +145,13 @@ struct DevSys {
   const double* bondlib;         // fk, b0
   const double* anglib;          // fk, ang0
   const double* torlib;          // fk, rmult, deltor, paths
-  int nbnd, nbnd_solute, nang, nang_solute, ntor, ntor_solute;
-  double* e_bterm;               // energy per bonded term (bonds, angles, torsions)
+  const int* imp;                // nimp x 5 (i, j, k, l, cod)
+  const double* implib;          // fk, imp0 (radians)
+  int ff_type;                   // 2: periodic impropers (AMBER), else harmonic
+  int nbnd, nbnd_solute, nang, nang_solute, ntor, ntor_solute, nimp, nimp_solute;
+  double* e_bterm;               // energy per bonded term (bonds, angles, torsions, impropers)
   // gradient slots of the term kernel: 3 FP64 per (term, atom of the term),
-  // numbered bonds 2, angles 3, torsions 4, Q pairs 2 per term; slot n is
+  // numbered bonds 2, angles 3, torsions 4, impropers 4, Q pairs 2 per term; slot n is
   // stored at slot_pos[n], so that atom slot_atom[t]'s contributions occupy
   // slot_g[slot_first[t] .. slot_first[t+1]) in slot order
   double* slot_g;
@@ -182,6 +193,12 @@ struct DevSys {
   int eseries_cap;
   double* eseries;
   double* eseries64;   // mixed precision: the same frames with FP64 Q pair arithmetic
+  // EVB off-diagonal functions (Q atoms k, l; A, mu) and, per recorded
+  // frame, Hij and r_kl of each (offdiag, src/md.f90:15444-15470)
+  int noffd;
+  const int* offd_kl;
+  const double* offd_amu;
+  double* eseries_offd;
   int* eseries_n;
 };
 
@@ -214,6 +231,8 @@ class Batch {
   // energy-file frames recorded on the device every iene steps
   void set_energy_interval(int iene, int max_frames);
   int energy_series(int k, double* out, int cap, bool fp64 = false);
+  int offdiag_series(int k, double* out, int cap);
+  int noffdiag(int k) const { return h_sys_[k].noffd; }
   void set_graphs(bool on) { use_graph_ = on; }
   // per-kernel average device time (ms per launch) over nsteps steps run
   // with an event pair around every launch; order of kKernelNames

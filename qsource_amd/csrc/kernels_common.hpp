@@ -72,6 +72,44 @@ __device__ __forceinline__ bool shadow_record_step(const DevSys& S) {
   return st > 0 && st % S.iene == 0;
 }
 
+// Soft-core alpha sc_lookup(iq, column, state); column 0-based over the
+// topology types, then natyps + the Q atom.
+__device__ __forceinline__ double sc_alpha(const DevSys& S, int iq, int col, int st) {
+  return S.sc_lookup[((size_t)iq * S.sc_ncol + col) * S.nstates + st];
+}
+
+// One state of a Q-solute pair with Q-atom types (nonbond_qp_qvdw,
+// src/md.f90:14150-14172): Q atom iq, partner of type tj and charge cj, LJ
+// code lj; r2i = 1/r^2, ri = 1/r, r6h = r^6.  Returns dv (times lambda).
+template <typename P>
+__device__ __forceinline__ P qp_qvdw_state(const DevSys& S, int iq, int st, int tj, int lj, P cj, P r2i,
+                                           P ri, P r6h, P& Vel, P& Va, P& Vb) {
+  const int qt = S.qiac[iq * S.nstates + st];
+  const int qlj = lj == 2 ? 1 : lj;
+  const P aLJ = (P)(S.qavdw[3 * qt + qlj - 1] * S.avdw[tj * 3 + lj - 1]);
+  const P bLJ = (P)(S.qbvdw[3 * qt + qlj - 1] * S.bvdw[tj * 3 + lj - 1]);
+  Vel = (P)S.qcrg[iq * S.nstates + st] * cj * ri;
+  if (lj == 3) Vel = Vel * (P)S.el14_scale;
+  const P r6s = r6h + (P)sc_alpha(S, iq, tj, st);
+  Va = aLJ / (r6s * r6s);
+  Vb = bLJ / r6s;
+  return r2i * (-Vel - ((P)12 * Va - (P)6 * Vb) * (r6h / r6s)) * (P)S.lambda[st];
+}
+
+// One state of the Q-atom / water-O pair with Q-atom types (nonbond_qw_spc,
+// src/md.f90:14495-14512): LJ code 1 of the Q type with the water O's
+// aLJ_solv(1, lo), soft-core column lo.  Returns dv (times lambda).
+template <typename P>
+__device__ __forceinline__ P qw_qvdw_state(const DevSys& S, int iq, int st, int lo, P r2i, P ri, P r6h,
+                                           P& Vel, P& Va, P& Vb) {
+  const int qt = S.qiac[iq * S.nstates + st];
+  const P r6s = r6h + (P)sc_alpha(S, iq, lo - 1, st);
+  Va = (P)S.qavdw[3 * qt] * (P)S.alj_O[lo - 1] / (r6s * r6s);
+  Vb = (P)S.qbvdw[3 * qt] * (P)S.blj_O[lo - 1] / r6s;
+  Vel = (P)S.chg_solv[0] * (P)S.qcrg[iq * S.nstates + st] * ri;
+  return r2i * (-Vel - ((P)12 * Va - (P)6 * Vb) * (r6h / r6s)) * (P)S.lambda[st];
+}
+
 // 64-lane butterfly sum; every lane returns the total
 __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

@@ -258,11 +258,23 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(c
         const int l = lj - 1;
         const int tq = S.iac[qa];
         const P dx1 = xi0 - X[3 * qa], dx2 = xi1 - X[3 * qa + 1], dx3 = xi2 - X[3 * qa + 2];
+        const P r2sq = pnorm2(dx1, dx2, dx3);
         P r;
-        const P r2 = inv_r2(pnorm2(dx1, dx2, dx3), r);
+        const P r2 = inv_r2(r2sq, r);
         const P r6 = r2 * r2 * r2;
         const P aLJ = (P)(S.avdw[tq * 3 + l] * S.avdw[ti * 3 + l]);
         const P bLJ = (P)(S.bvdw[tq * 3 + l] * S.bvdw[ti * 3 + l]);
+        if (S.qvdw) {
+          const P r6h = r2sq * r2sq * r2sq;
+          for (int st = 0; st < S.nstates; ++st) {
+            P Vel, Va, Vb;
+            const P dv = qp_qvdw_state<P>(S, iq, st, ti, lj, qi, r2, r, r6h, Vel, Va, Vb);
+            g0 = g0 + (double)(dv * dx1);
+            g1 = g1 + (double)(dv * dx2);
+            g2 = g2 + (double)(dv * dx3);
+          }
+          continue;
+        }
         for (int st = 0; st < S.nstates; ++st) {
           P Vel = (P)S.qcrg[iq * S.nstates + st] * qi * r;
           if (lj == 3) Vel = Vel * (P)S.el14_scale;
@@ -349,12 +361,28 @@ __global__ __launch_bounds__(256) void k_q_forces(const DevSys* __restrict__ sys
     const int l = lj - 1;
     const int tj = S.iac[j];
     P dx1 = (P)(S.x[3 * j] - xq0), dx2 = (P)(S.x[3 * j + 1] - xq1), dx3 = (P)(S.x[3 * j + 2] - xq2);
+    const P r2sq = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
     P r;
-    P r2 = inv_r2(dx1 * dx1 + dx2 * dx2 + dx3 * dx3, r);
+    P r2 = inv_r2(r2sq, r);
     P r6 = r2 * r2 * r2;
     const P aLJ = (P)(S.avdw[ti * 3 + l] * S.avdw[tj * 3 + l]);
     const P bLJ = (P)(S.bvdw[ti * 3 + l] * S.bvdw[tj * 3 + l]);
     const P cj = (P)S.crg[j];
+    if (S.qvdw) {
+      const P r6h = r2sq * r2sq * r2sq;
+#pragma unroll
+      for (int st = 0; st < NS; ++st) {
+        if (st >= ns) continue;
+        P Vel, Va, Vb;
+        const P dv = qp_qvdw_state<P>(S, iq, st, tj, lj, cj, r2, r, r6h, Vel, Va, Vb);
+        g0 = g0 - (double)(dv * dx1);
+        g1 = g1 - (double)(dv * dx2);
+        g2 = g2 - (double)(dv * dx3);
+        e[st][0] = e[st][0] + (double)Vel;
+        e[st][1] = e[st][1] + (double)Va - (double)Vb;
+      }
+      continue;
+    }
 #pragma unroll
     for (int st = 0; st < NS; ++st) {
       if (st >= ns) continue;
@@ -385,13 +413,24 @@ __global__ __launch_bounds__(256) void k_q_forces(const DevSys* __restrict__ sys
     P Va = aO * aS * r6 * r6;
     P Vb = bO * bS * r6;
     P dv = 0;
+    if (S.qvdw) {
 #pragma unroll
-    for (int st = 0; st < NS; ++st) {
-      if (st >= ns) continue;
-      P Vel = cO * qs[st] * r;
-      dv = dv + r2 * (-Vel - ((P)12 * Va - (P)6 * Vb) * (r6_hc / r6_hc)) * lam[st];
-      e[st][2] = e[st][2] + (double)Vel;
-      e[st][3] = e[st][3] + (double)Va - (double)Vb;
+      for (int st = 0; st < NS; ++st) {
+        if (st >= ns) continue;
+        P Vel;
+        dv = dv + qw_qvdw_state<P>(S, iq, st, lo, r2, r, r6_hc, Vel, Va, Vb);
+        e[st][2] = e[st][2] + (double)Vel;
+        e[st][3] = e[st][3] + (double)Va - (double)Vb;
+      }
+    } else {
+#pragma unroll
+      for (int st = 0; st < NS; ++st) {
+        if (st >= ns) continue;
+        P Vel = cO * qs[st] * r;
+        dv = dv + r2 * (-Vel - ((P)12 * Va - (P)6 * Vb) * (r6_hc / r6_hc)) * lam[st];
+        e[st][2] = e[st][2] + (double)Vel;
+        e[st][3] = e[st][3] + (double)Va - (double)Vb;
+      }
     }
     g0 = g0 - (double)(dv * dx1);
     g1 = g1 - (double)(dv * dx2);
@@ -457,16 +496,67 @@ __global__ __launch_bounds__(64) void k_q_reduce(const DevSys* __restrict__ sys,
 }
 
 // ---------------------------------------------------------------------------
-// Bonded terms (bond :1332-1405, angle :1130-1246, torsion :17907-18052) and
-// Q-Q / Q-neighbour pairs (nonbond_qq :13491-13636), one thread per term:
-// the term's energy is stored per term and the gradient contribution of
-// each of its atoms into that atom's slot (slots of term n are consecutive:
-// bonds i j, angles i j k, torsions i j k l, Q pairs i j); k_term_gather
+// Bonded terms (bond :1332-1405, angle :1130-1246, torsion :17907-18052,
+// improper :3130-3416) and Q-Q / Q-neighbour pairs (nonbond_qq
+// :13491-13636), one thread per term: the term's energy is stored per term
+// and the gradient contribution of each of its atoms into that atom's slot
+// (slots of term n are consecutive: bonds i j, angles i j k, torsions and
+// impropers i j k l, Q pairs i j); k_term_gather
 // then adds every atom's slots to d in a fixed order.  FP64 in both
 // precisions.
 __device__ __forceinline__ void put_slot(const DevSys& S, int slot, double g0, double g1, double g2) {
   double* o = S.slot_g + 3 * (size_t)S.slot_pos[slot];
   o[0] = g0; o[1] = g1; o[2] = g2;
+}
+
+// Dihedral angle i-j-k-l and the 12 components of its derivative times
+// -1/sin(phi), as torsion/improper form them (src/md.f90:3160-3244).
+__device__ double dihedral_dp(const double* x, int i, int j, int k, int l, double dp[12]) {
+  double rji[3], rjk[3], rkl[3], rnj[3], rnk[3];
+  for (int c = 0; c < 3; ++c) {
+    rji[c] = x[3 * i + c] - x[3 * j + c];
+    rjk[c] = x[3 * k + c] - x[3 * j + c];
+    rkl[c] = x[3 * l + c] - x[3 * k + c];
+  }
+  rnj[0] = rji[1] * rjk[2] - rji[2] * rjk[1];
+  rnj[1] = rji[2] * rjk[0] - rji[0] * rjk[2];
+  rnj[2] = rji[0] * rjk[1] - rji[1] * rjk[0];
+  rnk[0] = -rjk[1] * rkl[2] + rjk[2] * rkl[1];
+  rnk[1] = -rjk[2] * rkl[0] + rjk[0] * rkl[2];
+  rnk[2] = -rjk[0] * rkl[1] + rjk[1] * rkl[0];
+  const double bj2inv = 1.0 / (rnj[0] * rnj[0] + rnj[1] * rnj[1] + rnj[2] * rnj[2]);
+  const double bk2inv = 1.0 / (rnk[0] * rnk[0] + rnk[1] * rnk[1] + rnk[2] * rnk[2]);
+  const double bjinv = sqrt(bj2inv), bkinv = sqrt(bk2inv);
+  double scp = (rnj[0] * rnk[0] + rnj[1] * rnk[1] + rnj[2] * rnk[2]) * (bjinv * bkinv);
+  if (scp > 1.0) scp = 1.0;
+  else if (scp < -1.0) scp = -1.0;
+  double phi = acos(scp);
+  if (rjk[0] * (rnj[1] * rnk[2] - rnj[2] * rnk[1]) + rjk[1] * (rnj[2] * rnk[0] - rnj[0] * rnk[2]) +
+          rjk[2] * (rnj[0] * rnk[1] - rnj[1] * rnk[0]) < 0)
+    phi = -phi;
+  double f1 = sin(phi);
+  if (fabs(f1) < qsys::kQrealEps) f1 = qsys::kQrealEps;
+  f1 = -1.0 / f1;
+  double di[3], dl[3], rki[3], rlj[3];
+  for (int c = 0; c < 3; ++c) {
+    di[c] = f1 * (rnk[c] * bjinv * bkinv - scp * rnj[c] * bj2inv);
+    dl[c] = f1 * (rnj[c] * bjinv * bkinv - scp * rnk[c] * bk2inv);
+    rki[c] = rji[c] - rjk[c];
+    rlj[c] = -rjk[c] - rkl[c];
+  }
+  dp[0] = rjk[1] * di[2] - rjk[2] * di[1];
+  dp[1] = rjk[2] * di[0] - rjk[0] * di[2];
+  dp[2] = rjk[0] * di[1] - rjk[1] * di[0];
+  dp[3] = rki[1] * di[2] - rki[2] * di[1] + rkl[1] * dl[2] - rkl[2] * dl[1];
+  dp[4] = rki[2] * di[0] - rki[0] * di[2] + rkl[2] * dl[0] - rkl[0] * dl[2];
+  dp[5] = rki[0] * di[1] - rki[1] * di[0] + rkl[0] * dl[1] - rkl[1] * dl[0];
+  dp[6] = rlj[1] * dl[2] - rlj[2] * dl[1] - rji[1] * di[2] + rji[2] * di[1];
+  dp[7] = rlj[2] * dl[0] - rlj[0] * dl[2] - rji[2] * di[0] + rji[0] * di[2];
+  dp[8] = rlj[0] * dl[1] - rlj[1] * dl[0] - rji[0] * di[1] + rji[1] * di[0];
+  dp[9] = rjk[1] * dl[2] - rjk[2] * dl[1];
+  dp[10] = rjk[2] * dl[0] - rjk[0] * dl[2];
+  dp[11] = rjk[0] * dl[1] - rjk[1] * dl[0];
+  return phi;
 }
 
 __global__ __launch_bounds__(256) void k_terms(const DevSys* __restrict__ sys) {
@@ -571,28 +661,84 @@ __global__ __launch_bounds__(256) void k_terms(const DevSys* __restrict__ sys) {
     return;
   }
   n -= S.ntor;
+  if (n < S.nimp) {
+    // improper (src/md.f90:3130-3272, harmonic in the wrapped angle) or
+    // improper2 (:3276-3416, periodic fk (1 + cos(2 phi - imp0)))
+    const int* T = S.imp + 5 * n;
+    const double* L = S.implib + 2 * (T[4] - 1);
+    double dp[12];
+    const double phi = dihedral_dp(x, T[0], T[1], T[2], T[3], dp);
+    double dv, e;
+    if (S.ff_type == 2) {
+      const double arg = 2 * phi - L[1];
+      e = L[0] * (1 + cos(arg));
+      dv = -2 * L[0] * sin(arg);
+    } else {
+      double arg = phi - L[1];
+      arg = arg - 2.0 * kPiD * round(arg / (2.0 * kPiD));
+      dv = L[0] * arg;
+      e = 0.5 * dv * arg;
+    }
+    const int s0 = 2 * S.nbnd + 3 * S.nang + 4 * S.ntor + 4 * n;
+    for (int m = 0; m < 4; ++m) put_slot(S, s0 + m, dv * dp[3 * m], dv * dp[3 * m + 1], dv * dp[3 * m + 2]);
+    S.e_bterm[S.nbnd + S.nang + S.ntor + n] = e;
+    return;
+  }
+  n -= S.nimp;
   if (n < S.nqq) {
     const int p = n;
     const int st = S.qq_state[p], iq = S.qq_iq[p], j = S.qq_j[p], jq = S.qq_jq[p], lj = S.qq_lj[p];
     const int i = S.iqseq[iq];
     const int l = lj - 1;
     const double qi = S.qcrg[iq * S.nstates + st];
-    const double aLJ = S.avdw[S.iac[i] * 3 + l] * S.avdw[S.iac[j] * 3 + l];
-    const double bLJ = S.bvdw[S.iac[i] * 3 + l] * S.bvdw[S.iac[j] * 3 + l];
-    const double qj = jq >= 0 ? S.qcrg[jq * S.nstates + st] : S.crg[j];
+    double aLJ, bLJ, qj;
+    if (!S.qvdw) {
+      aLJ = S.avdw[S.iac[i] * 3 + l] * S.avdw[S.iac[j] * 3 + l];
+      bLJ = S.bvdw[S.iac[i] * 3 + l] * S.bvdw[S.iac[j] * 3 + l];
+      qj = jq >= 0 ? S.qcrg[jq * S.nstates + st] : S.crg[j];
+    } else {
+      // Q-atom types (src/md.f90:13531-13550)
+      const int qti = S.qiac[iq * S.nstates + st];
+      aLJ = S.qavdw[3 * qti + l];
+      bLJ = S.qbvdw[3 * qti + l];
+      if (jq >= 0) {
+        const int qtj = S.qiac[jq * S.nstates + st];
+        aLJ = aLJ * S.qavdw[3 * qtj + l];
+        bLJ = bLJ * S.qbvdw[3 * qtj + l];
+        qj = S.qcrg[jq * S.nstates + st];
+      } else {
+        if (lj == 2) {
+          aLJ = S.qavdw[3 * qti];
+          bLJ = S.qbvdw[3 * qti];
+        }
+        aLJ = aLJ * S.avdw[S.iac[j] * 3 + l];
+        bLJ = bLJ * S.bvdw[S.iac[j] * 3 + l];
+        qj = S.crg[j];
+      }
+    }
     const double dx1 = x[3 * j] - x[3 * i], dx2 = x[3 * j + 1] - x[3 * i + 1], dx3 = x[3 * j + 2] - x[3 * i + 2];
     double r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
     const double r6_hc = r2 * r2 * r2;
-    double r6 = r2 * r2 * r2 + 0.0;
+    // soft-core column natyps + jq (jq = -1 for a non-Q partner reads the
+    // last topology type's column, as the reference's natyps + 0)
+    double r6 = r2 * r2 * r2 + sc_alpha(S, iq, S.natyps + jq, st);
     r6 = 1.0 / r6;
     const double r12 = r6 * r6;
     r2 = 1.0 / r2;
     const double r = sqrt(r2);
     double Vel = qi * qj * r * S.qq_elscale[p];
     if (lj == 3) Vel = Vel * S.el14_scale;
-    const double Va = aLJ * r12, Vb = bLJ * r6;
-    const double dv = r2 * (-Vel - (12.0 * Va - 6.0 * Vb) * r6_hc * r6) * S.lambda[st];
-    const int s0 = 2 * S.nbnd + 3 * S.nang + 4 * S.ntor + 2 * p;
+    double Va, Vb, dv;
+    if (S.qvdw && jq >= 0 && lj == 2) {
+      Va = aLJ * exp(-bLJ / r);
+      Vb = 0.0;
+      dv = r2 * (-Vel - bLJ * Va / r) * S.lambda[st];
+    } else {
+      Va = aLJ * r12;
+      Vb = bLJ * r6;
+      dv = r2 * (-Vel - (12.0 * Va - 6.0 * Vb) * r6_hc * r6) * S.lambda[st];
+    }
+    const int s0 = 2 * S.nbnd + 3 * S.nang + 4 * S.ntor + 4 * S.nimp + 2 * p;
     put_slot(S, s0, -dv * dx1, -dv * dx2, -dv * dx3);
     put_slot(S, s0 + 1, dv * dx1, dv * dx2, dv * dx3);
     S.e_qq[2 * p] = Vel;

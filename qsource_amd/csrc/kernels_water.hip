@@ -270,12 +270,19 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_water_forces(co
       P r;
       const P r2 = inv_r2(r2sq, r);
       const P r6 = (P)1 / r6_hc;
-      const P Va = (P)S.avdw[ti * 3 + lo - 1] * (P)S.alj_O[lo - 1] * r6 * r6;
-      const P Vb = (P)S.bvdw[ti * 3 + lo - 1] * (P)S.blj_O[lo - 1] * r6;
       P dv = 0;
-      for (int st = 0; st < ns; ++st) {
-        const P Vel = (P)S.chg_solv[0] * (P)S.qcrg[iq * ns + st] * r;
-        dv = dv + r2 * (-Vel - ((P)12 * Va - (P)6 * Vb) * (r6_hc / r6_hc)) * (P)S.lambda[st];
+      if (S.qvdw) {
+        for (int st = 0; st < ns; ++st) {
+          P Vel, Va, Vb;
+          dv = dv + qw_qvdw_state<P>(S, iq, st, lo, r2, r, r6_hc, Vel, Va, Vb);
+        }
+      } else {
+        const P Va = (P)S.avdw[ti * 3 + lo - 1] * (P)S.alj_O[lo - 1] * r6 * r6;
+        const P Vb = (P)S.bvdw[ti * 3 + lo - 1] * (P)S.blj_O[lo - 1] * r6;
+        for (int st = 0; st < ns; ++st) {
+          const P Vel = (P)S.chg_solv[0] * (P)S.qcrg[iq * ns + st] * r;
+          dv = dv + r2 * (-Vel - ((P)12 * Va - (P)6 * Vb) * (r6_hc / r6_hc)) * (P)S.lambda[st];
+        }
       }
       g[0] = g[0] + (double)(dv * dx1);
       g[1] = g[1] + (double)(dv * dx2);
@@ -674,9 +681,13 @@ __global__ __launch_bounds__(256) void k_energy_reduce(const DevSys* __restrict_
   for (int n = threadIdx.x; n < S.nang; n += blockDim.x) b[n < S.nang_solute ? 1 : 4] += S.e_bterm[S.nbnd + n];
   for (int n = threadIdx.x; n < S.ntor; n += blockDim.x)
     b[n < S.ntor_solute ? 2 : 5] += S.e_bterm[S.nbnd + S.nang + n];
+  double bi[2] = {0, 0};   // impropers: solute, solvent
+  for (int n = threadIdx.x; n < S.nimp; n += blockDim.x)
+    bi[n < S.nimp_solute ? 0 : 1] += S.e_bterm[S.nbnd + S.nang + S.ntor + n];
   for (int c = 0; c < 4; ++c) w[c] = block_sum_256(w[c], red[c]);
   for (int c = 0; c < 8; ++c) p[c] = block_sum_256(p[c], red[4 + c]);
   for (int c = 0; c < 6; ++c) b[c] = block_sum_256(b[c], red[c]);
+  for (int c = 0; c < 2; ++c) bi[c] = block_sum_256(bi[c], red[6 + c]);
   // per-state Q energies: qq, qp (Q-neighbour pairs of nonbond_qq and the
   // Q-solute list), qw; pass 1 forms the FP64 re-evaluated frame of a
   // mixed-precision batch from e_q64 (Q-Q pairs are FP64 in both)
@@ -712,11 +723,22 @@ __global__ __launch_bounds__(256) void k_energy_reduce(const DevSys* __restrict_
       r[13] = 0.0;
     }
   }
+  // EVB off-diagonal elements of the frame (offdiag, src/md.f90:15444-15470)
+  if (threadIdx.x == 0 && frame >= 0 && frame < S.eseries_cap && S.eseries_offd) {
+    for (int io = 0; io < S.noffd; ++io) {
+      const int k3 = 3 * S.offd_kl[2 * io], l3 = 3 * S.offd_kl[2 * io + 1];
+      const double a = S.x[l3] - S.x[k3], b2 = S.x[l3 + 1] - S.x[k3 + 1], c = S.x[l3 + 2] - S.x[k3 + 2];
+      const double r = sqrt(a * a + b2 * b2 + c * c);
+      S.eseries_offd[((size_t)frame * S.noffd + io) * 2] = S.offd_amu[2 * io] * exp(-S.offd_amu[2 * io + 1] * r);
+      S.eseries_offd[((size_t)frame * S.noffd + io) * 2 + 1] = r;
+    }
+  }
   if (threadIdx.x == 0 && frame >= 0) *S.eseries_n = frame + 1;
   if (threadIdx.x == 0) {
     S.energies[E_P_BOND] = b[0]; S.energies[E_P_ANGLE] = b[1]; S.energies[E_P_TORSION] = b[2];
     S.energies[E_W_BOND] = b[3]; S.energies[E_W_ANGLE] = b[4]; S.energies[E_W_TORSION] = b[5];
-    S.energies[E_P_IMPROPER] = 0.0;
+    S.energies[E_P_IMPROPER] = bi[0];
+    S.energies[E_W_IMPROPER] = bi[1];
     double* E = S.energies;
     E[E_WW_EL] = w[0]; E[E_WW_VDW] = w[1]; E[E_LRF] = w[2] + p[4]; E[E_RST_RADIAL] = w[3];
     E[E_PP_EL] = p[0]; E[E_PP_VDW] = p[1]; E[E_PW_EL] = p[2]; E[E_PW_VDW] = p[3];

@@ -32,7 +32,7 @@ SYMBOLS = [
     "qgpu_step", "qgpu_get_state", "qgpu_set_state", "qgpu_get_gradient", "qgpu_energies",
     "qgpu_qenergies", "qgpu_get_pairlist", "qgpu_temperature", "qgpu_theta_corr",
     "qgpu_last_step_ms", "qgpu_last_error", "qgpu_set_energy_interval", "qgpu_energy_series",
-    "qgpu_energy_series_fp64",
+    "qgpu_energy_series_fp64", "qgpu_noffdiag", "qgpu_offdiag_series",
     "qgpu_profile_kernels", "qgpu_list_counts",
 ]
 
@@ -76,6 +76,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.qgpu_list_counts.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_long)]
     lib.qgpu_energy_series.argtypes = [P, ctypes.c_int, dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     lib.qgpu_energy_series_fp64.argtypes = [P, ctypes.c_int, dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    lib.qgpu_noffdiag.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    lib.qgpu_offdiag_series.argtypes = [P, ctypes.c_int, dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     lib.qgpu_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
@@ -189,6 +191,19 @@ class Engine:
     def set_energy_interval(self, iene: int, max_frames: int):
         self._check(self.lib.qgpu_set_energy_interval(self.h, iene, max_frames))
         self._max_frames = max_frames
+
+    def offdiag_series(self, k: int = 0) -> np.ndarray:
+        """(nframes, noffd, 2): EVB off-diagonal Hij and r_kl of each recorded
+        frame."""
+        m = ctypes.c_int()
+        self._check(self.lib.qgpu_noffdiag(self.h, k, ctypes.byref(m)))
+        cap = max(1, getattr(self, "_max_frames", 0)) * max(1, m.value) * 2
+        out = np.zeros(cap)
+        n = ctypes.c_int()
+        self._check(self.lib.qgpu_offdiag_series(self.h, k, _dp(out), cap, ctypes.byref(n)))
+        if m.value == 0:
+            return np.zeros((n.value, 0, 2))
+        return out[:n.value * m.value * 2].reshape(n.value, m.value, 2)
 
     def energy_series(self, k: int = 0, fp64: bool = False) -> np.ndarray:
         """(nframes, nstates, 14) put_ene records recorded on the device.
