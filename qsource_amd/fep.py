@@ -37,7 +37,7 @@ def write_energy_file(path: str, lambdas: Sequence[float], frames: np.ndarray,
     # resid(1)=-1, gcnum(1)=-1, version char(80)
     head = struct.pack("<iiiiiii", 1337, 1, 1, 3, 1, -1, -1) + version.encode()[:80].ljust(80)
     out = [_rec(head)]
-    for f in frames:
+    for k, f in enumerate(frames):
         for s in range(nstates):
             row = f[s]
             out.append(_rec(struct.pack("<i", s + 1) + struct.pack("<d", lambdas[s]) +
@@ -45,7 +45,6 @@ def write_energy_file(path: str, lambdas: Sequence[float], frames: np.ndarray,
         if offd is None or len(offd) == 0:
             out.append(_rec(b""))  # no off-diagonal elements
         else:
-            k = len(out) // (nstates + 1)
             out.append(_rec(b"".join(struct.pack("<iidd", int(o[0]), int(o[1]), float(o[2]), float(o[3]))
                                      for o in offd[k])))
     with open(path, "wb") as fh:
