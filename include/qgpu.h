@@ -156,6 +156,48 @@ int qgpu_profile_kernels(qgpu_handle* h, int nsteps, double* ms, int cap);
 /* Average device time per step (ms) of the last qgpu_step call. */
 int qgpu_last_step_ms(qgpu_handle* h, double* ms);
 
+/* ---- Learned far-field surrogate ------------------------------------------
+ * No counterpart in the reference: an alternative to the LRF Taylor far field
+ * (lrf_update/lrf_taylor, src/md.f90:5824-5981).  A 4-128-128-3 SiLU MLP maps
+ * per-charge-group features (net charge, dipole along and across the radial
+ * direction, radial position / sphere radius) to the far-field potential and
+ * the field along and across the radial direction at the group centre.  The
+ * 128x128 layer runs on BF16 matrix cores with FP32 accumulation; training
+ * (Adam, mean squared error of standardised outputs) runs on the GPU. */
+typedef struct qgpu_mlp qgpu_mlp;
+
+int qgpu_mlp_create(int seed, qgpu_mlp** out);
+int qgpu_mlp_destroy(qgpu_mlp* m);
+/* Number of parameters (W1[128][4] b1 W2[128][128] b2 W3[3][128] b3). */
+int qgpu_mlp_nparams(int* n);
+/* FP32 parameters in that order; norm (14 floats): input means and standard
+ * deviations (4 + 4), output means and standard deviations (3 + 3); may be
+ * NULL. */
+int qgpu_mlp_get_params(qgpu_mlp* m, float* params, float* norm);
+int qgpu_mlp_set_params(qgpu_mlp* m, const float* params, const float* norm);
+/* Training samples (X: n x 4 features, Y: n x 3 targets); sets the
+ * normalisation from their means and standard deviations. */
+int qgpu_mlp_set_data(qgpu_mlp* m, const float* X, const float* Y, int n);
+/* Adam epochs over shuffled mini-batches; *loss = last epoch's mean squared
+ * error of the standardised outputs. */
+int qgpu_mlp_train(qgpu_mlp* m, int epochs, int batch, float lr, int seed, double* loss);
+/* Gradient of the mean squared error over (X, Y) w.r.t. all parameters. */
+int qgpu_mlp_gradient(qgpu_mlp* m, const float* X, const float* Y, int n, float* grad, double* loss);
+/* Predictions (n x 3, de-standardised) for features X (n x 4). */
+int qgpu_mlp_predict(qgpu_mlp* m, const float* X, int n, float* Y);
+const char* qgpu_mlp_last_error(void);
+
+/* Far-field samples of instance k at its current coordinates (pair lists and
+ * LRF multipoles rebuilt, so each target is the exact potential and field of
+ * all far charges at the group centre): X (ncgp x 4), Y (ncgp x 3); *n =
+ * ncgp.  qgpu_far_lrf_estimate gives, for the same groups, the estimate of
+ * the LRF multipoles of the last list update at the current centres. */
+int qgpu_far_samples(qgpu_handle* h, int k, float* X, float* Y, int cap, int* n);
+int qgpu_far_lrf_estimate(qgpu_handle* h, int k, float* X, float* Y, int cap, int* n);
+/* Far field of every instance from model m (kept alive by the caller)
+ * instead of the LRF multipoles; m = NULL restores LRF. */
+int qgpu_set_far_model(qgpu_handle* h, qgpu_mlp* m);
+
 /* Message of the last failure in this thread ("" when none). */
 const char* qgpu_last_error(void);
 

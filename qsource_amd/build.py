@@ -16,7 +16,8 @@ PKG = os.path.join(ROOT, "qsource_amd")
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libqgpu.so")
 
-HIP_SOURCES = ["engine.hip", "kernels_water.hip", "kernels_solute.hip"]
+HIP_SOURCES = ["engine.hip", "kernels_water.hip", "kernels_solute.hip", "kernels_far.hip",
+               "kernels_mlp.hip", "mlp_host.hip"]
 HOST_SOURCES = ["host/readers.cpp", "host/prep.cpp"]
 
 # -packed-fp32-ops off: hand-off of FP32 pairs to v_pk_* is disabled;

@@ -14,6 +14,7 @@
 
 #include "../../include/qgpu.h"
 #include "engine.hpp"
+#include "mlp_host.hpp"
 
 namespace qgpu {
 
@@ -33,6 +34,10 @@ template <typename P, int NS> __global__ void k_q_forces(const DevSys*, int);
 __global__ void k_q_reduce(const DevSys*, int);
 __global__ void k_terms(const DevSys*);
 __global__ void k_term_gather(const DevSys*);
+__global__ void k_far_features(const DevSys*);
+__global__ void k_far_targets(const DevSys*);
+__global__ void k_far_lrf_eval(const DevSys*);
+__global__ void k_far_apply(const DevSys*, const float*, const float*);
 
 #define HIPCHECK(expr)                                                            \
   do {                                                                            \
@@ -198,6 +203,9 @@ int Batch::offdiag_series(int k, double* out, int cap) {
 
 Batch::~Batch() {
   if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  if (far_x_) (void)hipFree(far_x_);
+  if (far_y_) (void)hipFree(far_y_);
+  if (far_frame_) (void)hipFree(far_frame_);
   for (auto* i : inst_) delete i;
   for (auto* s : setups_) delete s;
   if (d_sys_) (void)hipFree(d_sys_);
@@ -559,6 +567,7 @@ void Batch::upload(int k) {
   D.e_water = I->alloc<double>((size_t)std::max(1, s.nwat) * 4);
   D.e_solute = I->alloc<double>((size_t)std::max(1, s.nat_solute) * 8);
   D.e_wpol = I->alloc<double>(std::max(1, D.nshell));
+  D.e_far = I->alloc<double>(std::max(1, D.ncgp));
   I->ke_blocks = 0;
   D.ke_part = nullptr;  // sized after all instances are known (see below)
   std::vector<double> scal(8, 0.0);
@@ -648,6 +657,67 @@ void Batch::launch_forces() {
     { pb(8); hipLaunchKernelGGL(k_watpol_theta, g2, dim3(256), 0, stream_, d_sys_); pe(8); }
     dim3 g3(kNumShellMax, ni);
     { pb(9); hipLaunchKernelGGL(k_watpol_force, g3, dim3(1024), 0, stream_, d_sys_); pe(9); }
+  }
+  if (far_model_) launch_far_model();
+}
+
+// Batch-wide far-field arrays (one row per charge group of every instance).
+void Batch::ensure_far() {
+  if (far_x_) return;
+  int off = 0;
+  for (auto& D : h_sys_) { D.far_off = off; off += D.ncgp; }
+  far_total_ = off;
+  HIPCHECK(hipMalloc(&far_x_, (size_t)std::max(1, off) * 4 * sizeof(float)));
+  HIPCHECK(hipMalloc(&far_y_, (size_t)std::max(1, off) * 3 * sizeof(float)));
+  HIPCHECK(hipMalloc(&far_frame_, (size_t)std::max(1, off) * 9 * sizeof(double)));
+  for (auto& D : h_sys_) { D.far_x = far_x_; D.far_y = far_y_; D.far_frame = far_frame_; }
+  sync_sys();
+}
+
+// Learned far field of every instance: group features, one batched MLP
+// forward pass over all groups, per-atom energy and gradient.
+void Batch::launch_far_model() {
+  const int ni = (int)h_sys_.size();
+  const dim3 g((max_ncgp_ + 255) / 256, ni);
+  hipLaunchKernelGGL(k_far_features, g, dim3(256), 0, stream_, d_sys_);
+  far_model_->predict_dev(far_x_, far_total_, far_y_, stream_);
+  hipLaunchKernelGGL(k_far_apply, g, dim3(256), 0, stream_, d_sys_, far_model_->norm + 8, far_model_->ystd_d);
+}
+
+int Batch::far_samples(int k, float* X, float* Y, int cap, bool lrf_estimate) {
+  ensure_far();
+  const int ni = (int)h_sys_.size();
+  const dim3 g((max_ncgp_ + 255) / 256, ni);
+  if (!lrf_estimate) {
+    if (far_model_) throw std::runtime_error("exact far-field samples need the LRF multipoles (no model attached)");
+    launch_lists();
+    hipLaunchKernelGGL(k_far_features, g, dim3(256), 0, stream_, d_sys_);
+    hipLaunchKernelGGL(k_far_targets, g, dim3(256), 0, stream_, d_sys_);
+  } else {
+    hipLaunchKernelGGL(k_far_features, g, dim3(256), 0, stream_, d_sys_);
+    hipLaunchKernelGGL(k_far_lrf_eval, g, dim3(256), 0, stream_, d_sys_);
+  }
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(stream_));
+  const DevSys& D = h_sys_[k];
+  const int n = std::min(cap, D.ncgp);
+  if (X) HIPCHECK(hipMemcpy(X, far_x_ + (size_t)D.far_off * 4, (size_t)n * 4 * sizeof(float), hipMemcpyDeviceToHost));
+  if (Y) HIPCHECK(hipMemcpy(Y, far_y_ + (size_t)D.far_off * 3, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToHost));
+  return D.ncgp;
+}
+
+void Batch::set_far_model(Mlp* m) {
+  HIPCHECK(hipStreamSynchronize(stream_));
+  ensure_far();
+  far_model_ = m;
+  for (size_t k = 0; k < h_sys_.size(); ++k) {
+    h_sys_[k].far_mode = m ? 1 : 0;
+    h_sys_[k].use_lrf = m ? false : setups_[k]->inp.use_lrf;
+  }
+  sync_sys();
+  if (graph_exec_) {
+    HIPCHECK(hipGraphExecDestroy(graph_exec_));
+    graph_exec_ = nullptr;
   }
 }
 
@@ -1118,6 +1188,15 @@ int qgpu_offdiag_series(qgpu_handle* h, int k, double* out, int cap, int* nframe
     CHECK_K(h, k);
     *nframes = h->b->offdiag_series(k, out, cap);
   });
+}
+int qgpu_far_samples(qgpu_handle* h, int k, float* X, float* Y, int cap, int* n) {
+  return guarded([&] { CHECK_K(h, k); *n = h->b->far_samples(k, X, Y, cap, false); });
+}
+int qgpu_far_lrf_estimate(qgpu_handle* h, int k, float* X, float* Y, int cap, int* n) {
+  return guarded([&] { CHECK_K(h, k); *n = h->b->far_samples(k, X, Y, cap, true); });
+}
+int qgpu_set_far_model(qgpu_handle* h, qgpu_mlp* m) {
+  return guarded([&] { h->b->set_far_model(m ? m->m : nullptr); });
 }
 int qgpu_list_counts(qgpu_handle* h, int k, long* counts) {
   return guarded([&] {

@@ -199,6 +199,15 @@ struct DevSys {
   const int* offd_kl;
   const double* offd_amu;
   double* eseries_offd;
+  // far field: far_mode 1 replaces the LRF multipoles and Taylor terms by the
+  // learned surrogate; per group features far_x (4 floats), outputs far_y (3
+  // floats), frame far_frame (e_r, t, centre: 9 doubles) at offset far_off
+  // of the batch-wide arrays; e_far per group energy
+  int far_mode;
+  int far_off;
+  float *far_x, *far_y;
+  double* far_frame;
+  double* e_far;
   int* eseries_n;
 };
 
@@ -232,6 +241,11 @@ class Batch {
   void set_energy_interval(int iene, int max_frames);
   int energy_series(int k, double* out, int cap, bool fp64 = false);
   int offdiag_series(int k, double* out, int cap);
+  // far-field samples of instance k at its current coordinates: features and
+  // the exact far-field targets (lists rebuilt), or the estimate of the
+  // current LRF multipoles; returns the number of charge groups
+  int far_samples(int k, float* X, float* Y, int cap, bool lrf_estimate);
+  void set_far_model(class Mlp* m);
   int noffdiag(int k) const { return h_sys_[k].noffd; }
   void set_graphs(bool on) { use_graph_ = on; }
   // per-kernel average device time (ms per launch) over nsteps steps run
@@ -263,6 +277,12 @@ class Batch {
   void launch_lists();
   void launch_forces();
   void launch_integrate();
+  void ensure_far();
+  void launch_far_model();
+  class Mlp* far_model_ = nullptr;
+  float *far_x_ = nullptr, *far_y_ = nullptr;
+  double* far_frame_ = nullptr;
+  int far_total_ = 0;
   void reduce_energies();
   std::vector<qsys::Setup*> setups_;
   std::vector<Inst*> inst_;

@@ -676,6 +676,9 @@ __global__ __launch_bounds__(256) void k_energy_reduce(const DevSys* __restrict_
   for (int i = threadIdx.x; i < S.nat_solute; i += blockDim.x)
     for (int c = 0; c < 8; ++c) p[c] += S.e_solute[(size_t)i * 8 + c];
   // bonded terms: p_bond p_angle p_torsion w_bond w_angle w_torsion
+  double ef = 0.0;   // learned far field (far_mode)
+  if (S.far_mode)
+    for (int g = threadIdx.x; g < S.ncgp; g += blockDim.x) ef += S.e_far[g];
   double b[6] = {0, 0, 0, 0, 0, 0};
   for (int n = threadIdx.x; n < S.nbnd; n += blockDim.x) b[n < S.nbnd_solute ? 0 : 3] += S.e_bterm[n];
   for (int n = threadIdx.x; n < S.nang; n += blockDim.x) b[n < S.nang_solute ? 1 : 4] += S.e_bterm[S.nbnd + n];
@@ -688,6 +691,7 @@ __global__ __launch_bounds__(256) void k_energy_reduce(const DevSys* __restrict_
   for (int c = 0; c < 8; ++c) p[c] = block_sum_256(p[c], red[4 + c]);
   for (int c = 0; c < 6; ++c) b[c] = block_sum_256(b[c], red[c]);
   for (int c = 0; c < 2; ++c) bi[c] = block_sum_256(bi[c], red[6 + c]);
+  ef = block_sum_256(ef, red[8]);
   // per-state Q energies: qq, qp (Q-neighbour pairs of nonbond_qq and the
   // Q-solute list), qw; pass 1 forms the FP64 re-evaluated frame of a
   // mixed-precision batch from e_q64 (Q-Q pairs are FP64 in both)
@@ -740,7 +744,7 @@ __global__ __launch_bounds__(256) void k_energy_reduce(const DevSys* __restrict_
     S.energies[E_P_IMPROPER] = bi[0];
     S.energies[E_W_IMPROPER] = bi[1];
     double* E = S.energies;
-    E[E_WW_EL] = w[0]; E[E_WW_VDW] = w[1]; E[E_LRF] = w[2] + p[4]; E[E_RST_RADIAL] = w[3];
+    E[E_WW_EL] = w[0]; E[E_WW_VDW] = w[1]; E[E_LRF] = w[2] + p[4] + ef; E[E_RST_RADIAL] = w[3];
     E[E_PP_EL] = p[0]; E[E_PP_VDW] = p[1]; E[E_PW_EL] = p[2]; E[E_PW_VDW] = p[3];
     E[E_RST_FIX] = p[5]; E[E_RST_SHELL] = p[6];
     double ew = 0.0;

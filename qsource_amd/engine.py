@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from typing import Sequence
+from typing import Optional, Sequence
 
 import numpy as np
 
@@ -33,6 +33,9 @@ SYMBOLS = [
     "qgpu_qenergies", "qgpu_get_pairlist", "qgpu_temperature", "qgpu_theta_corr",
     "qgpu_last_step_ms", "qgpu_last_error", "qgpu_set_energy_interval", "qgpu_energy_series",
     "qgpu_energy_series_fp64", "qgpu_noffdiag", "qgpu_offdiag_series",
+    "qgpu_mlp_create", "qgpu_mlp_destroy", "qgpu_mlp_nparams", "qgpu_mlp_get_params", "qgpu_mlp_set_params",
+    "qgpu_mlp_set_data", "qgpu_mlp_train", "qgpu_mlp_gradient", "qgpu_mlp_predict", "qgpu_mlp_last_error",
+    "qgpu_far_samples", "qgpu_far_lrf_estimate", "qgpu_set_far_model",
     "qgpu_profile_kernels", "qgpu_list_counts",
 ]
 
@@ -78,6 +81,21 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.qgpu_energy_series_fp64.argtypes = [P, ctypes.c_int, dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     lib.qgpu_noffdiag.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     lib.qgpu_offdiag_series.argtypes = [P, ctypes.c_int, dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    fp = ctypes.POINTER(ctypes.c_float)
+    ip = ctypes.POINTER(ctypes.c_int)
+    lib.qgpu_mlp_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+    lib.qgpu_mlp_destroy.argtypes = [P]
+    lib.qgpu_mlp_nparams.argtypes = [ip]
+    lib.qgpu_mlp_get_params.argtypes = [P, fp, fp]
+    lib.qgpu_mlp_set_params.argtypes = [P, fp, fp]
+    lib.qgpu_mlp_set_data.argtypes = [P, fp, fp, ctypes.c_int]
+    lib.qgpu_mlp_train.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_int, dp]
+    lib.qgpu_mlp_gradient.argtypes = [P, fp, fp, ctypes.c_int, fp, dp]
+    lib.qgpu_mlp_predict.argtypes = [P, fp, ctypes.c_int, fp]
+    lib.qgpu_mlp_last_error.restype = ctypes.c_char_p
+    lib.qgpu_far_samples.argtypes = [P, ctypes.c_int, fp, fp, ctypes.c_int, ip]
+    lib.qgpu_far_lrf_estimate.argtypes = [P, ctypes.c_int, fp, fp, ctypes.c_int, ip]
+    lib.qgpu_set_far_model.argtypes = [P, P]
     lib.qgpu_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
@@ -85,6 +103,10 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
 
 def _dp(a: np.ndarray):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 
 
 class QgpuError(RuntimeError):
@@ -227,3 +249,84 @@ class Engine:
         t = ctypes.c_double()
         self._check(self.lib.qgpu_last_step_ms(self.h, ctypes.byref(t)))
         return t.value
+
+
+class FarFieldMLP:
+    """The learned far-field surrogate (4-128-128-3 SiLU MLP, BF16 matrix-core
+    kernels) trained and evaluated on the GPU."""
+
+    def __init__(self, seed: int = 1):
+        self.lib = load_library()
+        h = ctypes.c_void_p()
+        self._check(self.lib.qgpu_mlp_create(seed, ctypes.byref(h)))
+        self.h = h
+        n = ctypes.c_int()
+        self.lib.qgpu_mlp_nparams(ctypes.byref(n))
+        self.nparams = n.value
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise QgpuError(self.lib.qgpu_mlp_last_error().decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.qgpu_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        """(FP32 parameters, normalisation: xmean[4] xstd[4] ymean[3] ystd[3])."""
+        p = np.zeros(self.nparams, dtype=np.float32)
+        nm = np.zeros(14, dtype=np.float32)
+        self._check(self.lib.qgpu_mlp_get_params(self.h, _fp(p), _fp(nm)))
+        return p, nm
+
+    def set_params(self, p: np.ndarray, norm: Optional[np.ndarray] = None):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        nm = None if norm is None else np.ascontiguousarray(norm, dtype=np.float32)
+        self._check(self.lib.qgpu_mlp_set_params(self.h, _fp(p), _fp(nm) if nm is not None else None))
+
+    def set_data(self, X: np.ndarray, Y: np.ndarray):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.ascontiguousarray(Y, dtype=np.float32)
+        self._check(self.lib.qgpu_mlp_set_data(self.h, _fp(X), _fp(Y), len(X)))
+
+    def train(self, epochs: int, batch: int = 1024, lr: float = 2e-3, seed: int = 1) -> float:
+        loss = ctypes.c_double()
+        self._check(self.lib.qgpu_mlp_train(self.h, epochs, batch, ctypes.c_float(lr), seed, ctypes.byref(loss)))
+        return loss.value
+
+    def gradient(self, X: np.ndarray, Y: np.ndarray):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.ascontiguousarray(Y, dtype=np.float32)
+        g = np.zeros(self.nparams, dtype=np.float32)
+        loss = ctypes.c_double()
+        self._check(self.lib.qgpu_mlp_gradient(self.h, _fp(X), _fp(Y), len(X), _fp(g), ctypes.byref(loss)))
+        return g, loss.value
+
+    def predict(self, X: np.ndarray) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((len(X), 3), dtype=np.float32)
+        self._check(self.lib.qgpu_mlp_predict(self.h, _fp(X), len(X), _fp(Y)))
+        return Y
+
+
+def far_samples(engine: "Engine", k: int = 0, lrf_estimate: bool = False):
+    """Features (ncgp, 4) and far-field (ncgp, 3) of instance k: exact at the
+    current coordinates, or the current LRF multipoles' estimate."""
+    cap = 1 << 20
+    X = np.zeros((cap, 4), dtype=np.float32)
+    Y = np.zeros((cap, 3), dtype=np.float32)
+    n = ctypes.c_int()
+    fn = engine.lib.qgpu_far_lrf_estimate if lrf_estimate else engine.lib.qgpu_far_samples
+    engine._check(fn(engine.h, k, _fp(X), _fp(Y), cap, ctypes.byref(n)))
+    return X[:n.value].copy(), Y[:n.value].copy()
+
+
+def set_far_model(engine: "Engine", model: Optional[FarFieldMLP]):
+    engine._check(engine.lib.qgpu_set_far_model(engine.h, model.h if model is not None else None))
