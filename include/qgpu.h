@@ -89,7 +89,10 @@ int qgpu_get_gradient(qgpu_handle* h, int k, double* d);
 int qgpu_energies(qgpu_handle* h, int k, double* e, int cap);
 
 /* Per-state Q-atom energies of instance k (EQ(s), src/qatom.f90:197-200):
- * 7 values per state: qq_el qq_vdw qp_el qp_vdw qw_el qw_vdw total. */
+ * 12 values per state: qq_el qq_vdw qp_el qp_vdw qw_el qw_vdw total, then
+ * the Q-bonded bond angle torsion improper (qbond/qangle/qtorsion/qimproper,
+ * src/md.f90:17022-17540) and the state's restraint energy (p_restrain,
+ * src/md.f90:15473-15802). */
 int qgpu_qenergies(qgpu_handle* h, int k, double* e, int cap);
 
 /* Replaces the energy-file writes of md_run (put_ene every iene_cycle

@@ -98,9 +98,11 @@ class Oracle:
         return dict(zip(ENERGY_NAMES, e.tolist()))
 
     def qenergies(self) -> np.ndarray:
-        e = np.zeros(7 * max(self.nstates, 1))
+        """Per state: qq_el qq_vdw qp_el qp_vdw qw_el qw_vdw total bond angle
+        torsion improper restraint."""
+        e = np.zeros(12 * max(self.nstates, 1))
         n = self.lib.oracle_get_qenergies(self.h, _dp(e), ctypes.c_int(len(e)))
-        return e[:n].reshape(-1, 7)
+        return e[:n].reshape(-1, 12)
 
     def list_sizes(self):
         n = (ctypes.c_long * 5)()

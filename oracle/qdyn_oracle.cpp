@@ -19,6 +19,9 @@
 //   nonbond_qq 13491-13636, lrf_taylor 5824-5891
 //   bond 1332-1405, angle 1130-1246, torsion 17907-18052, improper 3130-3272
 //   fix_shell 2816-2862, restrain_solvent 18055-18102, watpol 18372-18541
+//   p_restrain 15473-15802
+//   qbond 17192-17245, qangle 17022-17114, qtorsion 17395-17540,
+//   qimproper 17249-17392
 //   newvel_ber 6194-6223, shake 17581-17679, temperature 6098-6190
 #include "../qsource_amd/csrc/host/qsys.hpp"
 
@@ -312,6 +315,15 @@ struct Oracle {
   // line: Hij and r_kl)
   std::vector<double> offd_hij, offd_rkl;
 
+  // Morse factors of the Q bonds, kept between calls as the reference's
+  // module arrays EMorseD and dMorse_i/j (src/qatom.f90:201-203), zero at start
+  std::vector<double> emorse, dmorse_i, dmorse_j;
+  void qbond(int st);
+  void qangle(int st);
+  void qtorsion(int st);
+  void qimproper(int st);
+  void p_restrain();
+
   void fix_shell() {
     for (int i = 0; i < s.natom; ++i) {
       if (!(s.excl[i] || s.shell[i])) continue;
@@ -499,11 +511,18 @@ struct Oracle {
       E.w_improper = improper(s.nimps_solute, (int)s.imp.size());
     }
     fix_shell();
+    p_restrain();
     if (s.nwat > 0) {
       restrain_solvent();
       if (s.wpol_restr) watpol();
     }
     nonbond_qq();
+    for (int st = 0; st < s.nstates; ++st) {
+      qbond(st);
+      qangle(st);
+      qtorsion(st);
+      qimproper(st);
+    }
     for (int st = 0; st < s.nstates; ++st) {
       QEnergies& q = EQ[st];
       q.qx_el = q.qq_el + q.qp_el + q.qw_el;
@@ -1182,6 +1201,337 @@ double Oracle::improper2(int a, int b) {
   return e;
 }
 
+// qbond (src/md.f90:17192-17245): Morse + harmonic Q bonds of one state;
+// sets the bond's Morse factor and its gradient for coupled terms.
+void Oracle::qbond(int st) {
+  const size_t nb = s.qbnd.size();
+  if (emorse.size() != nb) { emorse.assign(nb, 0.0); dmorse_i.assign(3 * nb, 0.0); dmorse_j.assign(3 * nb, 0.0); }
+  const double lam = s.lambda[st];
+  for (size_t ib = 0; ib < nb; ++ib) {
+    const QTerm& b = s.qbnd[ib];
+    const int ic = b.cod[st];
+    if (ic <= 0) continue;
+    const QBondLib& L = s.qbondlib[ic - 1];
+    const int i3 = 3 * b.i, j3 = 3 * b.j;
+    double rij[3] = {x[j3] - x[i3], x[j3 + 1] - x[i3 + 1], x[j3 + 2] - x[i3 + 2]};
+    double bl = std::sqrt(rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2]);
+    double db = bl - L.r0;
+    double fexp = std::exp(-L.amz * db);
+    double be = L.dmz * (fexp * fexp - 2.0 * fexp) + 0.5 * L.fk * (db * db);
+    emorse[ib] = -(fexp * fexp - 2.0 * fexp);
+    EQ[st].bond = EQ[st].bond + be;
+    double dv = (2.0 * L.dmz * L.amz * (fexp - fexp * fexp) + L.fk * db) * lam / bl;
+    for (int c = 0; c < 3; ++c) {
+      d[i3 + c] = d[i3 + c] - dv * rij[c];
+      d[j3 + c] = d[j3 + c] + dv * rij[c];
+    }
+    if (db > 0) {
+      emorse[ib] = -(fexp * fexp - 2.0 * fexp);
+      for (int c = 0; c < 3; ++c) {
+        dmorse_i[3 * ib + c] = +2.0 * L.amz * (fexp - fexp * fexp) * lam / bl * rij[c];
+        dmorse_j[3 * ib + c] = -2.0 * L.amz * (fexp - fexp * fexp) * lam / bl * rij[c];
+      }
+    } else {
+      emorse[ib] = 1.0;
+      for (int c = 0; c < 3; ++c) { dmorse_i[3 * ib + c] = 0.0; dmorse_j[3 * ib + c] = 0.0; }
+    }
+  }
+}
+
+// qangle (src/md.f90:17022-17114)
+void Oracle::qangle(int st) {
+  const double lam = s.lambda[st];
+  for (size_t ia = 0; ia < s.qang.size(); ++ia) {
+    const QTerm& a = s.qang[ia];
+    const int ic = a.cod[st];
+    if (ic <= 0) continue;
+    const QAngLib& L = s.qanglib[ic - 1];
+    const int ib = s.qang_cb[ia];
+    double gamma = 1.0;
+    if (ib >= 0) {
+      gamma = emorse[ib];
+      if (s.qang_cf[ia] == 1) gamma = 1.0 - gamma;
+    }
+    const int i3 = 3 * a.i, j3 = 3 * a.j, k3 = 3 * a.k;
+    double rji[3], rjk[3];
+    for (int c = 0; c < 3; ++c) { rji[c] = x[i3 + c] - x[j3 + c]; rjk[c] = x[k3 + c] - x[j3 + c]; }
+    double bji = std::sqrt(rji[0] * rji[0] + rji[1] * rji[1] + rji[2] * rji[2]);
+    double bjk = std::sqrt(rjk[0] * rjk[0] + rjk[1] * rjk[1] + rjk[2] * rjk[2]);
+    double scp = (rji[0] * rjk[0] + rji[1] * rjk[1] + rji[2] * rjk[2]);
+    scp = scp / (bji * bjk);
+    if (scp > 1.0) scp = 1.0;
+    if (scp < -1.0) scp = -1.0;
+    double ang = std::acos(scp);
+    double da = ang - L.ang0;
+    double ae = 0.5 * L.fk * (da * da);
+    EQ[st].angle = EQ[st].angle + ae * gamma;
+    double dv = gamma * L.fk * da * lam;
+    double f1 = std::sin(ang);
+    if (std::fabs(f1) < kQrealEps) f1 = kQrealEps;
+    f1 = -1.0 / f1;
+    double di[3], dk[3];
+    for (int c = 0; c < 3; ++c) {
+      di[c] = f1 * (rjk[c] / (bji * bjk) - scp * rji[c] / (bji * bji));
+      dk[c] = f1 * (rji[c] / (bji * bjk) - scp * rjk[c] / (bjk * bjk));
+    }
+    for (int c = 0; c < 3; ++c) {
+      d[i3 + c] = d[i3 + c] + dv * di[c];
+      d[k3 + c] = d[k3 + c] + dv * dk[c];
+      d[j3 + c] = d[j3 + c] - dv * (di[c] + dk[c]);
+    }
+    if (ib >= 0) {
+      const int bi3 = 3 * s.qbnd[ib].i, bj3 = 3 * s.qbnd[ib].j;
+      for (int c = 0; c < 3; ++c) d[bi3 + c] = d[bi3 + c] + dmorse_i[3 * ib + c] * ae;
+      for (int c = 0; c < 3; ++c) d[bj3 + c] = d[bj3 + c] + dmorse_j[3 * ib + c] * ae;
+    }
+  }
+}
+
+// Dihedral geometry as qtorsion/qimproper form it: divisions by bj*bk and
+// bj**2 (src/md.f90:17295-17338).
+static double qdihedral(const std::vector<double>& x, int i, int j, int k, int l, double dp[12]) {
+  const int i3 = 3 * i, j3 = 3 * j, k3 = 3 * k, l3 = 3 * l;
+  double rji[3], rjk[3], rkl[3], rnj[3], rnk[3];
+  for (int c = 0; c < 3; ++c) {
+    rji[c] = x[i3 + c] - x[j3 + c];
+    rjk[c] = x[k3 + c] - x[j3 + c];
+    rkl[c] = x[l3 + c] - x[k3 + c];
+  }
+  rnj[0] = rji[1] * rjk[2] - rji[2] * rjk[1];
+  rnj[1] = rji[2] * rjk[0] - rji[0] * rjk[2];
+  rnj[2] = rji[0] * rjk[1] - rji[1] * rjk[0];
+  rnk[0] = -rjk[1] * rkl[2] + rjk[2] * rkl[1];
+  rnk[1] = -rjk[2] * rkl[0] + rjk[0] * rkl[2];
+  rnk[2] = -rjk[0] * rkl[1] + rjk[1] * rkl[0];
+  double bj = std::sqrt(rnj[0] * rnj[0] + rnj[1] * rnj[1] + rnj[2] * rnj[2]);
+  double bk = std::sqrt(rnk[0] * rnk[0] + rnk[1] * rnk[1] + rnk[2] * rnk[2]);
+  double scp = (rnj[0] * rnk[0] + rnj[1] * rnk[1] + rnj[2] * rnk[2]) / (bj * bk);
+  if (scp > 1.0) scp = 1.0;
+  if (scp < -1.0) scp = -1.0;
+  double phi = std::acos(scp);
+  double sgn = rjk[0] * (rnj[1] * rnk[2] - rnj[2] * rnk[1]) + rjk[1] * (rnj[2] * rnk[0] - rnj[0] * rnk[2]) +
+               rjk[2] * (rnj[0] * rnk[1] - rnj[1] * rnk[0]);
+  if (sgn < 0) phi = -phi;
+  double f1 = std::sin(phi);
+  if (std::fabs(f1) < kQrealEps) f1 = kQrealEps;
+  f1 = -1.0 / f1;
+  double di[3], dl[3], rki[3], rlj[3];
+  for (int c = 0; c < 3; ++c) {
+    di[c] = f1 * (rnk[c] / (bj * bk) - scp * rnj[c] / (bj * bj));
+    dl[c] = f1 * (rnj[c] / (bj * bk) - scp * rnk[c] / (bk * bk));
+    rki[c] = rji[c] - rjk[c];
+    rlj[c] = -rjk[c] - rkl[c];
+  }
+  dp[0] = rjk[1] * di[2] - rjk[2] * di[1];
+  dp[1] = rjk[2] * di[0] - rjk[0] * di[2];
+  dp[2] = rjk[0] * di[1] - rjk[1] * di[0];
+  dp[3] = rki[1] * di[2] - rki[2] * di[1] + rkl[1] * dl[2] - rkl[2] * dl[1];
+  dp[4] = rki[2] * di[0] - rki[0] * di[2] + rkl[2] * dl[0] - rkl[0] * dl[2];
+  dp[5] = rki[0] * di[1] - rki[1] * di[0] + rkl[0] * dl[1] - rkl[1] * dl[0];
+  dp[6] = rlj[1] * dl[2] - rlj[2] * dl[1] - rji[1] * di[2] + rji[2] * di[1];
+  dp[7] = rlj[2] * dl[0] - rlj[0] * dl[2] - rji[2] * di[0] + rji[0] * di[2];
+  dp[8] = rlj[0] * dl[1] - rlj[1] * dl[0] - rji[0] * di[1] + rji[1] * di[0];
+  dp[9] = rjk[1] * dl[2] - rjk[2] * dl[1];
+  dp[10] = rjk[2] * dl[0] - rjk[0] * dl[2];
+  dp[11] = rjk[0] * dl[1] - rjk[1] * dl[0];
+  return phi;
+}
+
+// qtorsion (src/md.f90:17395-17540)
+void Oracle::qtorsion(int st) {
+  const double lam = s.lambda[st];
+  for (size_t ip = 0; ip < s.qtor.size(); ++ip) {
+    const QTerm& t = s.qtor[ip];
+    const int ic = t.cod[st];
+    if (ic <= 0) continue;
+    const QTorLib& L = s.qtorlib[ic - 1];
+    const int ib = s.qtor_cb[ip];
+    double gamma = 1.0;
+    if (ib >= 0) {
+      gamma = emorse[ib];
+      if (s.qtor_cf[ip] == 1) gamma = 1 - gamma;
+    }
+    double dp[12];
+    double phi = qdihedral(x, t.i, t.j, t.k, t.l, dp);
+    double arg = L.rmult * phi - L.deltor;
+    double pe = L.fk * (1.0 + std::cos(arg));
+    EQ[st].torsion = EQ[st].torsion + pe * gamma;
+    double dv = -L.rmult * L.fk * std::sin(arg) * gamma * lam;
+    const int at[4] = {t.i, t.j, t.k, t.l};
+    for (int m = 0; m < 4; ++m)
+      for (int c = 0; c < 3; ++c) d[3 * at[m] + c] = d[3 * at[m] + c] + dv * dp[3 * m + c];
+    if (ib >= 0) {
+      const int bi3 = 3 * s.qbnd[ib].i, bj3 = 3 * s.qbnd[ib].j;
+      for (int c = 0; c < 3; ++c) d[bi3 + c] = d[bi3 + c] + dmorse_i[3 * ib + c] * pe;
+      for (int c = 0; c < 3; ++c) d[bj3 + c] = d[bj3 + c] + dmorse_j[3 * ib + c] * pe;
+    }
+  }
+}
+
+// qimproper (src/md.f90:17249-17392): harmonic in the wrapped angle
+void Oracle::qimproper(int st) {
+  const double lam = s.lambda[st];
+  for (size_t ip = 0; ip < s.qimp.size(); ++ip) {
+    const QTerm& t = s.qimp[ip];
+    const int ic = t.cod[st];
+    if (ic <= 0) continue;
+    const QImpLib& L = s.qimplib[ic - 1];
+    const int ib = s.qimp_cb[ip];
+    double gamma = 1.0;
+    if (ib >= 0) {
+      gamma = emorse[ib];
+      if (s.qimp_cf[ip] == 1) gamma = 1 - gamma;
+    }
+    double dp[12];
+    double phi = qdihedral(x, t.i, t.j, t.k, t.l, dp);
+    double arg = phi - L.imp0;
+    arg = arg - 2.0 * kPi * std::round(arg / (2.0 * kPi));
+    double dv = L.fk * arg;
+    double pe = 0.5 * dv * arg;
+    EQ[st].improper = EQ[st].improper + pe * gamma;
+    dv = dv * gamma * lam;
+    const int at[4] = {t.i, t.j, t.k, t.l};
+    for (int m = 0; m < 4; ++m)
+      for (int c = 0; c < 3; ++c) d[3 * at[m] + c] = d[3 * at[m] + c] + dv * dp[3 * m + c];
+    if (ib >= 0) {
+      const int bi3 = 3 * s.qbnd[ib].i, bj3 = 3 * s.qbnd[ib].j;
+      for (int c = 0; c < 3; ++c) d[bi3 + c] = d[bi3 + c] + dmorse_i[3 * ib + c] * pe;
+      for (int c = 0; c < 3; ++c) d[bj3 + c] = d[bj3 + c] + dmorse_j[3 * ib + c] * pe;
+    }
+  }
+}
+
+// p_restrain (src/md.f90:15473-15802), spherical boundary
+void Oracle::p_restrain() {
+  const double deg2rad = kPi / 180.0;
+  auto add_state = [&](int istate, double edum) {
+    if (istate == 0) {
+      for (int k = 0; k < s.nstates; ++k) EQ[k].restraint = EQ[k].restraint + edum;
+      if (s.nstates == 0) E.rst_protein = E.rst_protein + edum;
+    } else {
+      EQ[istate - 1].restraint = EQ[istate - 1].restraint + edum;
+    }
+  };
+  auto wgt_of = [&](int istate) { return istate != 0 ? s.lambda[istate - 1] : 1.0; };
+  for (const RstSeq& r : s.rstseq) {
+    const double fk = r.fk;
+    auto use = [&](int i) { return s.heavy[i] || (r.ih == 1 && !(s.excl[i] && s.freeze)); };
+    if (r.to_centre == 1 || r.to_centre == 2) {
+      double dr[3] = {0, 0, 0};
+      double norm = 0;
+      int n = 0;
+      for (int i = r.i; i <= r.j; ++i) {
+        if (!use(i)) continue;
+        const double w = r.to_centre == 1 ? 1.0 : s.mass[i];
+        if (r.to_centre == 1) {
+          ++n;
+          for (int c = 0; c < 3; ++c) dr[c] = dr[c] + x[3 * i + c] - s.xtop[3 * i + c];
+        } else {
+          norm = norm + w;
+          for (int c = 0; c < 3; ++c) dr[c] = dr[c] + (x[3 * i + c] - s.xtop[3 * i + c]) * w;
+        }
+      }
+      if (r.to_centre == 1 ? n > 0 : norm > 0) {
+        for (int c = 0; c < 3; ++c) dr[c] = r.to_centre == 1 ? dr[c] / n : dr[c] / norm;
+        double r2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+        E.rst_protein = E.rst_protein + 0.5 * fk * r2;
+        for (int i = r.i; i <= r.j; ++i) {
+          if (!use(i)) continue;
+          for (int c = 0; c < 3; ++c)
+            d[3 * i + c] = r.to_centre == 1 ? d[3 * i + c] + fk * dr[c] * s.mass[i] / 12.010
+                                            : d[3 * i + c] + fk * dr[c];
+        }
+      }
+    } else {
+      for (int i = r.i; i <= r.j; ++i) {
+        if (!use(i)) continue;
+        const int i3 = 3 * i;
+        double dr[3] = {x[i3] - s.xtop[i3], x[i3 + 1] - s.xtop[i3 + 1], x[i3 + 2] - s.xtop[i3 + 2]};
+        double r2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+        E.rst_protein = E.rst_protein + 0.5 * fk * r2;
+        for (int c = 0; c < 3; ++c) d[i3 + c] = d[i3 + c] + fk * dr[c];
+      }
+    }
+  }
+  for (const RstPos& r : s.rstpos) {
+    const int i3 = 3 * r.i;
+    double dr[3] = {x[i3] - r.x[0], x[i3 + 1] - r.x[1], x[i3 + 2] - r.x[2]};
+    const double wgt = wgt_of(r.ipsi);
+    double edum = 0.5 * r.fk[0] * (dr[0] * dr[0]) + 0.5 * r.fk[1] * (dr[1] * dr[1]) + 0.5 * r.fk[2] * (dr[2] * dr[2]);
+    for (int c = 0; c < 3; ++c) d[i3 + c] = d[i3 + c] + r.fk[c] * dr[c] * wgt;
+    add_state(r.ipsi, edum);
+  }
+  for (const RstDis& r : s.rstdis) {
+    const int i3 = 3 * r.i, j3 = 3 * r.j;
+    double dr[3] = {x[j3] - x[i3], x[j3 + 1] - x[i3 + 1], x[j3 + 2] - x[i3 + 2]};
+    const double wgt = wgt_of(r.ipsi);
+    double b = std::sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+    double db;
+    if (b < r.d1) db = b - r.d1;
+    else if (b > r.d2) db = b - r.d2;
+    else continue;
+    double edum = 0.5 * r.fk * (db * db);
+    double dv = wgt * r.fk * db / b;
+    for (int c = 0; c < 3; ++c) {
+      d[j3 + c] = d[j3 + c] + dr[c] * dv;
+      d[i3 + c] = d[i3 + c] - dr[c] * dv;
+    }
+    add_state(r.ipsi, edum);
+  }
+  for (const RstAng& r : s.rstang) {
+    const int i3 = 3 * r.i, j3 = 3 * r.j, k3 = 3 * r.k;
+    double dr[3], dr2[3];
+    for (int c = 0; c < 3; ++c) { dr[c] = x[i3 + c] - x[j3 + c]; dr2[c] = x[k3 + c] - x[j3 + c]; }
+    const double wgt = wgt_of(r.ipsi);
+    double r2ij = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+    double r2jk = dr2[0] * dr2[0] + dr2[1] * dr2[1] + dr2[2] * dr2[2];
+    double rij = std::sqrt(r2ij), rjk = std::sqrt(r2jk);
+    double scp = (dr[0] * dr2[0] + dr[1] * dr2[1] + dr[2] * dr2[2]);
+    scp = scp / (rij * rjk);
+    if (scp > 1.0) scp = 1.0;
+    if (scp < -1.0) scp = -1.0;
+    double theta = std::acos(scp);
+    double db = theta - (r.ang) * deg2rad;
+    double edum = 0.5 * r.fk * (db * db);
+    double dv = wgt * r.fk * db;
+    double f1 = std::sin(theta);
+    if (std::fabs(f1) < kQrealEps) f1 = kQrealEps;
+    f1 = -1.0 / f1;
+    double di[3], dk[3];
+    for (int c = 0; c < 3; ++c) {
+      di[c] = f1 * (dr2[c] / (rij * rjk) - scp * dr[c] / r2ij);
+      dk[c] = f1 * (dr[c] / (rij * rjk) - scp * dr2[c] / r2jk);
+    }
+    for (int c = 0; c < 3; ++c) {
+      d[i3 + c] = d[i3 + c] + dv * di[c];
+      d[k3 + c] = d[k3 + c] + dv * dk[c];
+      d[j3 + c] = d[j3 + c] - dv * (di[c] + dk[c]);
+    }
+    add_state(r.ipsi, edum);
+  }
+  for (const RstWal& r : s.rstwal) {
+    const double fk = r.fk;
+    for (int i = r.i; i <= r.j; ++i) {
+      if (!(s.heavy[i] || r.ih == 1)) continue;
+      const int i3 = 3 * i;
+      double dr[3] = {x[i3] - s.top.xwcent[0], x[i3 + 1] - s.top.xwcent[1], x[i3 + 2] - s.top.xwcent[2]};
+      double b = std::sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+      double db = b - r.d;
+      double erst, dv;
+      if (db > 0) {
+        erst = 0.5 * fk * (db * db) - r.dMorse;
+        dv = fk * db / b;
+      } else {
+        double fexp = std::exp(r.aMorse * db);
+        erst = r.dMorse * (fexp * fexp - 2.0 * fexp);
+        dv = -2.0 * r.dMorse * r.aMorse * (fexp - fexp * fexp) / b;
+      }
+      E.rst_protein = E.rst_protein + erst;
+      for (int c = 0; c < 3; ++c) d[i3 + c] = d[i3 + c] + dv * dr[c];
+    }
+  }
+}
+
 // offdiag (src/md.f90:15444-15470): H_ij = A exp(-mu r_kl) per function.
 void Oracle::offdiag() {
   offd_hij.assign(s.offd.size(), 0.0);
@@ -1264,12 +1614,14 @@ int oracle_get_energies(void* h, double* e, int cap) {
   std::memcpy(e, v, 8 * n);
   return n;
 }
-// per-state Q energies: qq_el qq_vdw qp_el qp_vdw qw_el qw_vdw total (7 per state)
+// per-state Q energies: qq_el qq_vdw qp_el qp_vdw qw_el qw_vdw total bond
+// angle torsion improper restraint (12 per state)
 int oracle_get_qenergies(void* h, double* e, int cap) {
   auto* o = static_cast<Oracle*>(h);
   int k = 0;
   for (auto& q : o->EQ) {
-    double v[] = {q.qq_el, q.qq_vdw, q.qp_el, q.qp_vdw, q.qw_el, q.qw_vdw, q.total};
+    double v[] = {q.qq_el, q.qq_vdw, q.qp_el, q.qp_vdw, q.qw_el, q.qw_vdw, q.total,
+                  q.bond, q.angle, q.torsion, q.improper, q.restraint};
     for (double z : v) if (k < cap) e[k++] = z;
   }
   return k;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -481,8 +482,123 @@ void Batch::upload(int k) {
     D.nbnd = (int)s.bnd.size(); D.nbnd_solute = s.nbonds_solute;
     D.nang = (int)s.ang.size(); D.nang_solute = s.nangles_solute;
     D.ntor = (int)s.tor.size(); D.ntor_solute = s.ntors_solute;
-    // term slots (bonds, angles, torsions, then the Q pairs) and the CSR of
-    // every atom's slots in slot order
+    // extra terms: Q bonds, angles, torsions, impropers of every state where
+    // present, then the restraints of p_restrain; each with the atoms of
+    // its gradient slots
+    std::vector<int> xt;
+    std::vector<double> xpar;
+    std::vector<std::vector<int>> xatoms;
+    std::vector<int> rst_atoms;
+    auto add_xt = [&](std::array<int, kXtInts> r, std::array<double, kXtPars> pr, std::vector<int> atoms) {
+      xt.insert(xt.end(), r.begin(), r.end());
+      xpar.insert(xpar.end(), pr.begin(), pr.end());
+      xatoms.push_back(std::move(atoms));
+    };
+    const int nst = s.nstates;
+    std::vector<int> qb_atoms;
+    for (size_t ib = 0; ib < s.qbnd.size(); ++ib) {
+      const qsys::QTerm& b = s.qbnd[ib];
+      qb_atoms.push_back(b.i); qb_atoms.push_back(b.j);
+      int last = -1;
+      for (int st = 0; st < nst; ++st) if (b.cod[st] > 0) last = st;
+      for (int st = 0; st < nst; ++st) {
+        if (b.cod[st] <= 0) continue;
+        const qsys::QBondLib& L = s.qbondlib[b.cod[st] - 1];
+        add_xt({XT_QBOND, b.i, b.j, (int)ib, st == last ? 1 : 0, st, 0, 0, 0, 0},
+               {L.dmz, L.amz, L.r0, L.fk, 0, 0, 0, 0}, {b.i, b.j});
+      }
+    }
+    // the state whose Morse factor a coupled term of state st sees
+    auto coupled = [&](int cb, int st, std::array<double, kXtPars>& pr) {
+      if (cb < 0) return -1;
+      for (int sp = st; sp >= 0; --sp)
+        if (s.qbnd[cb].cod[sp] > 0) {
+          const qsys::QBondLib& L = s.qbondlib[s.qbnd[cb].cod[sp] - 1];
+          pr[4] = L.dmz; pr[5] = L.amz; pr[6] = L.r0; pr[7] = L.fk;
+          return sp;
+        }
+      return -1;
+    };
+    auto with_bond = [&](std::vector<int> a, int cb) {
+      if (cb >= 0) { a.push_back(s.qbnd[cb].i); a.push_back(s.qbnd[cb].j); }
+      return a;
+    };
+    for (size_t ia = 0; ia < s.qang.size(); ++ia) {
+      const qsys::QTerm& q = s.qang[ia];
+      const int cb = s.qang_cb[ia];
+      for (int st = 0; st < nst; ++st) {
+        if (q.cod[st] <= 0) continue;
+        const qsys::QAngLib& L = s.qanglib[q.cod[st] - 1];
+        std::array<double, kXtPars> pr{L.fk, L.ang0, 0, 0, 0, 0, 0, 0};
+        const int seff = coupled(cb, st, pr);
+        add_xt({XT_QANGLE, q.i, q.j, q.k, 0, st, cb, s.qang_cf[ia], seff, 0}, pr, with_bond({q.i, q.j, q.k}, cb));
+      }
+    }
+    for (size_t it = 0; it < s.qtor.size(); ++it) {
+      const qsys::QTerm& q = s.qtor[it];
+      const int cb = s.qtor_cb[it];
+      for (int st = 0; st < nst; ++st) {
+        if (q.cod[st] <= 0) continue;
+        const qsys::QTorLib& L = s.qtorlib[q.cod[st] - 1];
+        std::array<double, kXtPars> pr{L.fk, L.rmult, L.deltor, 0, 0, 0, 0, 0};
+        const int seff = coupled(cb, st, pr);
+        add_xt({XT_QTORSION, q.i, q.j, q.k, q.l, st, cb, s.qtor_cf[it], seff, 0}, pr,
+               with_bond({q.i, q.j, q.k, q.l}, cb));
+      }
+    }
+    for (size_t it = 0; it < s.qimp.size(); ++it) {
+      const qsys::QTerm& q = s.qimp[it];
+      const int cb = s.qimp_cb[it];
+      for (int st = 0; st < nst; ++st) {
+        if (q.cod[st] <= 0) continue;
+        const qsys::QImpLib& L = s.qimplib[q.cod[st] - 1];
+        std::array<double, kXtPars> pr{L.fk, L.imp0, 0, 0, 0, 0, 0, 0};
+        const int seff = coupled(cb, st, pr);
+        add_xt({XT_QIMPROPER, q.i, q.j, q.k, q.l, st, cb, s.qimp_cf[it], seff, 0}, pr,
+               with_bond({q.i, q.j, q.k, q.l}, cb));
+      }
+    }
+    const double deg2rad = 4.0 * std::atan(1.0) / 180.0;
+    for (const qsys::RstSeq& r : s.rstseq) {
+      std::vector<int> use;
+      for (int i = r.i; i <= r.j; ++i)
+        if (s.heavy[i] || (r.ih == 1 && !(s.excl[i] && s.freeze))) use.push_back(i);
+      if (r.to_centre == 1 || r.to_centre == 2) {
+        if (use.empty()) continue;
+        if (r.to_centre == 2) {
+          double tm = 0;
+          for (int i : use) tm = tm + t.types[s.iac[i]].mass;
+          if (!(tm > 0)) continue;
+        }
+        add_xt({XT_RST_SEQC, (int)rst_atoms.size(), (int)use.size(), r.to_centre, 0, -1, 0, 0, 0, 0},
+               {r.fk, 0, 0, 0, 0, 0, 0, 0}, use);
+        rst_atoms.insert(rst_atoms.end(), use.begin(), use.end());
+      } else {
+        for (int i : use) add_xt({XT_RST_SEQ, i, 0, 0, 0, -1, 0, 0, 0, 0}, {r.fk, 0, 0, 0, 0, 0, 0, 0}, {i});
+      }
+    }
+    for (const qsys::RstPos& r : s.rstpos)
+      add_xt({XT_RST_POS, r.i, 0, 0, 0, r.ipsi, 0, 0, 0, 0}, {r.x[0], r.x[1], r.x[2], r.fk[0], r.fk[1], r.fk[2], 0, 0},
+             {r.i});
+    for (const qsys::RstDis& r : s.rstdis)
+      add_xt({XT_RST_DIST, r.i, r.j, 0, 0, r.ipsi, 0, 0, 0, 0}, {r.d1, r.d2, r.fk, 0, 0, 0, 0, 0}, {r.i, r.j});
+    for (const qsys::RstAng& r : s.rstang)
+      add_xt({XT_RST_ANG, r.i, r.j, r.k, 0, r.ipsi, 0, 0, 0, 0}, {r.ang * deg2rad, r.fk, 0, 0, 0, 0, 0, 0},
+             {r.i, r.j, r.k});
+    for (const qsys::RstWal& r : s.rstwal)
+      for (int i = r.i; i <= r.j; ++i)
+        if (s.heavy[i] || r.ih == 1)
+          add_xt({XT_RST_WALL, i, 0, 0, 0, -1, 0, 0, 0, 0}, {r.d, r.fk, r.aMorse, r.dMorse, 0, 0, 0, 0}, {i});
+    D.nxt = (int)xatoms.size();
+    D.nqbond = (int)s.qbnd.size();
+    D.qb_atoms = I->upload(qb_atoms);
+    D.qb_cur = I->alloc<double>((size_t)std::max(1, D.nqbond) * kMorseRec);
+    D.qb_prev = I->alloc<double>((size_t)std::max(1, D.nqbond) * kMorseRec);
+    D.rst_atoms = I->upload(rst_atoms);
+    D.e_xt = I->alloc<double>((size_t)std::max(1, D.nxt));
+    D.freeze = s.freeze;
+    // term slots (bonds, angles, torsions, impropers, the Q pairs, then the
+    // extra terms) and the CSR of every atom's slots in slot order
     D.e_bterm = I->alloc<double>((size_t)std::max(1, D.nbnd + D.nang + D.ntor + D.nimp));
     std::vector<std::vector<int>> inc(N);
     int slot = 0;
@@ -494,6 +610,12 @@ void Batch::upload(int k) {
       for (int a : {s.imp[n].i, s.imp[n].j, s.imp[n].k, s.imp[n].l}) inc[a].push_back(slot++);
     for (int p = 0; p < D.nqq; ++p)
       for (int a : {qq_atom_i[p], qq_atom_j[p]}) inc[a].push_back(slot++);
+    for (int n = 0; n < D.nxt; ++n) {
+      xt[(size_t)kXtInts * n + 9] = slot;
+      for (int a : xatoms[n]) inc[a].push_back(slot++);
+    }
+    D.xt = I->upload(xt);
+    D.xp = I->upload(xpar);
     std::vector<int> sa, sf, sl;
     for (int i = 0; i < N; ++i)
       if (!inc[i].empty()) {
@@ -509,8 +631,8 @@ void Batch::upload(int k) {
     D.slot_g = I->alloc<double>((size_t)std::max(1, slot) * 3);
     D.nslot_atoms = (int)sa.size();
     D.slot_atom = I->upload(sa); D.slot_first = I->upload(sf); D.slot_pos = I->upload(pos);
-    max_terms_ = std::max(max_terms_, D.nbnd + D.nang + D.ntor + D.nimp + D.nqq);
-    max_slot_atoms_ = std::max(max_slot_atoms_, D.nslot_atoms);
+    max_terms_ = std::max(max_terms_, D.nbnd + D.nang + D.ntor + D.nimp + D.nqq + D.nxt);
+    max_slot_atoms_ = std::max(max_slot_atoms_, std::max(D.nslot_atoms, D.nqbond));
   }
   // integrator
   D.dt = s.dt; D.temp0 = s.inp.temp0; D.tau_t = s.tau_t;
@@ -860,14 +982,19 @@ void Batch::energies(int k, double* out, int cap) {
     HIPCHECK(hipMemcpy(q.data(), h_sys_[k].qenergies, 8 * q.size(), hipMemcpyDeviceToHost));
   double p_bond = E[E_P_BOND], w_bond = E[E_W_BOND], p_angle = E[E_P_ANGLE], w_angle = E[E_W_ANGLE];
   double p_tor = E[E_P_TORSION], w_tor = E[E_W_TORSION], p_imp = E[E_P_IMPROPER], w_imp = E[E_W_IMPROPER];
-  // lambda-weighted Q sums of pot_energy (src/md.f90:16186-16211); no Q
-  // bonded terms or Q restraints in the supported FEP files
-  double q_bond = 0, q_angle = 0, q_tor = 0, q_imp = 0, qx_el = 0, qx_vdw = 0, rst_protein = 0;
+  // lambda-weighted Q sums of pot_energy (src/md.f90:16186-16211)
+  double q_bond = 0, q_angle = 0, q_tor = 0, q_imp = 0, qx_el = 0, qx_vdw = 0, rst_protein = E[E_RST_PROTEIN];
   for (int st = 0; st < s.nstates; ++st) {
     const double* e = &q[(size_t)st * kQTerms];
+    const double lam = s.lambda[st];
     double el = e[0] + e[2] + e[4], vdw = e[1] + e[3] + e[5];
-    qx_el = qx_el + el * s.lambda[st];
-    qx_vdw = qx_vdw + vdw * s.lambda[st];
+    q_bond = q_bond + e[6] * lam;
+    q_angle = q_angle + e[7] * lam;
+    q_tor = q_tor + e[8] * lam;
+    q_imp = q_imp + e[9] * lam;
+    qx_el = qx_el + el * lam;
+    qx_vdw = qx_vdw + vdw * lam;
+    rst_protein = rst_protein + e[10] * lam;
   }
   double rst_total = E[E_RST_FIX] + E[E_RST_SHELL] + rst_protein + E[E_RST_RADIAL] + E[E_RST_WPOL];
   double pot = p_bond + w_bond + p_angle + w_angle + p_tor + p_imp + E[E_PP_EL] + E[E_PP_VDW] +
@@ -884,7 +1011,8 @@ void Batch::energies(int k, double* out, int cap) {
 }
 
 // per state: qq_el qq_vdw qp_el qp_vdw qw_el qw_vdw total (EQ(s)%total,
-// src/md.f90:16190-16194) from the last energy reduction
+// src/md.f90:16190-16194), Q bond, angle, torsion, improper, restraint;
+// from the last energy reduction
 void Batch::qenergies(int k, double* out, int cap) {
   const int ni = (int)h_sys_.size();
   hipLaunchKernelGGL(k_energy_reduce, dim3(1, ni), dim3(256), 0, stream_, d_sys_, 0);
@@ -898,9 +1026,9 @@ void Batch::qenergies(int k, double* out, int cap) {
   for (int st = 0; st < s.nstates; ++st) {
     const double* e = &q[(size_t)st * kQTerms];
     double qx_el = e[0] + e[2] + e[4], qx_vdw = e[1] + e[3] + e[5];
-    double tot = 0.0 + 0.0 + 0.0 + 0.0 + qx_el + qx_vdw + 0.0;
-    double v[7] = {e[0], e[1], e[2], e[3], e[4], e[5], tot};
-    for (int c = 0; c < 7 && w < cap; ++c) out[w++] = v[c];
+    double tot = e[6] + e[7] + e[8] + e[9] + qx_el + qx_vdw + e[10];
+    double v[12] = {e[0], e[1], e[2], e[3], e[4], e[5], tot, e[6], e[7], e[8], e[9], e[10]};
+    for (int c = 0; c < 12 && w < cap; ++c) out[w++] = v[c];
   }
 }
 

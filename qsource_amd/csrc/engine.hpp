@@ -46,12 +46,25 @@ constexpr int kNumShellMax = 8;
 enum EnergyTerm {
   E_PP_EL, E_PP_VDW, E_PW_EL, E_PW_VDW, E_WW_EL, E_WW_VDW, E_LRF,
   E_P_BOND, E_P_ANGLE, E_P_TORSION, E_P_IMPROPER, E_W_BOND, E_W_ANGLE, E_W_TORSION, E_W_IMPROPER,
-  E_RST_FIX, E_RST_SHELL, E_RST_RADIAL, E_RST_WPOL,
+  E_RST_FIX, E_RST_SHELL, E_RST_RADIAL, E_RST_WPOL, E_RST_PROTEIN,
   E_KINETIC,
   E_NTERMS
 };
-// per-state Q terms: qq_el qq_vdw qp_el qp_vdw qw_el qw_vdw
-constexpr int kQTerms = 6;
+// per-state Q terms: qq_el qq_vdw qp_el qp_vdw qw_el qw_vdw, Q-bonded
+// bond angle torsion improper, restraint
+constexpr int kQTerms = 11;
+
+// extra FP64 terms of the term kernel: Q-bonded terms per state
+// (src/md.f90:17022-17540) and the restraints of p_restrain (:15473-15802).
+// Record of kXtInts ints: kind, atoms a0..a3, state, aux0, aux1, aux2,
+// first slot; kXtPars doubles of parameters.
+enum XtKind {
+  XT_QBOND = 1, XT_QANGLE, XT_QTORSION, XT_QIMPROPER,
+  XT_RST_SEQ, XT_RST_SEQC, XT_RST_POS, XT_RST_DIST, XT_RST_ANG, XT_RST_WALL
+};
+constexpr int kXtInts = 10;
+constexpr int kXtPars = 8;
+constexpr int kMorseRec = 7;   // Morse factor and its gradient on the bond's two atoms
 
 struct DevShell {
   double rout, dr, cstb, avtheta, avn_insh, theta_corr;
@@ -111,7 +124,6 @@ struct DevSys {
   int* shell_of;                 // per water, -1 when outside the shells
   // restraints
   double fk_fix, fk_pshell;
-  int freeze;
   // Q atoms
   const int* iqseq;
   const double* qcrg;            // nqat x nstates
@@ -157,6 +169,21 @@ struct DevSys {
   double* slot_g;
   int nslot_atoms;
   const int *slot_atom, *slot_first, *slot_pos;
+  // extra terms (XtKind records) with per-item energies.  Q bonds carry
+  // their Morse factor (EMorseD, dMorse_i/j, src/qatom.f90:201-203) to the
+  // coupled Q angles, torsions and impropers: a coupled term recomputes the
+  // factor at the latest state <= its own where the bond is present; with
+  // none it takes the value the bond's last present state left in the
+  // previous evaluation (qb_prev, kept by the reference's module arrays)
+  int nxt;
+  const int* xt;
+  const double* xp;
+  double* e_xt;
+  int nqbond;
+  const int* qb_atoms;           // nqbond x 2
+  double *qb_cur, *qb_prev;      // nqbond x kMorseRec
+  const int* rst_atoms;          // atoms of the centre restraints
+  int freeze;
   // integrator
   double dt;
   double temp0, tau_t;

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <stdexcept>
 
 namespace qsys {
@@ -76,6 +77,176 @@ int shake_cpu(const Setup& s, const std::vector<double>& xx, std::vector<double>
   return total;
 }
 
+// qatom_load_fep's completion of Q terms around breaking bonds
+// (src/qatom.f90:957-1046, 1180-1265, 1323-1467, 1512-1676) and get_fep's
+// removal of the redefined topology terms (src/md.f90:2937-3030).
+static void q_bonded_setup(Setup& s, const FepData& f) {
+  Topology& t = s.top;
+  const int nst = s.nstates;
+  s.qbondlib = f.qbondlib; s.qanglib = f.qanglib; s.qtorlib = f.qtorlib; s.qimplib = f.qimplib;
+  s.qbnd = f.qbnd; s.qang = f.qang; s.qtor = f.qtor; s.qimp = f.qimp;
+  for (const QTerm& b : s.qbnd)
+    if (b.i >= s.nat_solute || b.j >= s.nat_solute) throw std::runtime_error("Q bonds must join solute atoms");
+  // breaking bonds: the first adjacent pair of states where the bond is
+  // present in one and absent in the other (the reference's second test,
+  // (a .eqv. b .eqv. .false.), equals a .neqv. b)
+  struct Break { int i, j, brk, set; };
+  std::vector<Break> brk;
+  for (const QTerm& b : s.qbnd)
+    for (int st = 0; st + 1 < nst; ++st) {
+      bool a0 = b.cod[st] > 0, a1 = b.cod[st + 1] > 0;
+      if (a0 != a1) { brk.push_back({b.i, b.j, a0 ? st + 1 : st, a0 ? st : st + 1}); break; }
+    }
+  auto new_term = [&](int i, int j, int k, int l, int set, int code) {
+    QTerm q{i, j, k, l, std::vector<int>(nst, 0)};
+    q.cod[set] = code;
+    return q;
+  };
+  // angles of the topology at a breaking bond that the FEP file does not redefine
+  {
+    std::vector<int> hits;
+    std::vector<const Break*> hb;
+    for (const Break& b : brk)
+      for (size_t a = 0; a < t.ang.size(); ++a) {
+        const Angle& g = t.ang[a];
+        if (((g.i == b.i && g.j == b.j) || (g.j == b.i && g.k == b.j) || (g.k == b.i && g.j == b.j) ||
+             (g.j == b.i && g.i == b.j)) && g.cod != 0) { hits.push_back((int)a); hb.push_back(&b); }
+      }
+    for (size_t h = 0; h < hits.size(); ++h) {
+      const Angle& g = t.ang[hits[h]];
+      bool set = false;
+      for (const QTerm& q : s.qang)
+        if ((g.i == q.i && g.j == q.j && g.k == q.k) || (g.i == q.k && g.j == q.j && g.k == q.i)) set = true;
+      if (set) continue;
+      const AngLib& L = t.anglib[g.cod - 1];
+      if (L.ureyfk != 0.0) throw std::runtime_error("Urey-Bradley Q-angle terms (CHARMM) are not supported");
+      s.qanglib.push_back({L.fk, L.ang0, L.ureyfk, L.ureyr0});
+      s.qang.push_back(new_term(g.i, g.j, g.k, 0, hb[h]->set, (int)s.qanglib.size()));
+    }
+  }
+  // torsions
+  {
+    std::vector<int> hits;
+    std::vector<const Break*> hb;
+    for (const Break& b : brk)
+      for (size_t a = 0; a < t.tor.size(); ++a) {
+        const Torsion& g = t.tor[a];
+        if (((g.i == b.i && g.j == b.j) || (g.j == b.i && g.k == b.j) || (g.k == b.i && g.l == b.j) ||
+             (g.l == b.i && g.k == b.j) || (g.k == b.i && g.j == b.j) || (g.j == b.i && g.i == b.j)) &&
+            g.cod != 0) { hits.push_back((int)a); hb.push_back(&b); }
+      }
+    for (size_t h = 0; h < hits.size(); ++h) {
+      const Torsion& g = t.tor[hits[h]];
+      bool set = false;
+      for (const QTerm& q : s.qtor)
+        if ((g.i == q.i && g.j == q.j && g.k == q.k && g.l == q.l) ||
+            (g.i == q.l && g.j == q.k && g.k == q.j && g.l == q.i)) set = true;
+      if (set) continue;
+      const TorLib& L = t.torlib[g.cod - 1];
+      s.qtorlib.push_back({L.fk, L.rmult, L.deltor});
+      s.qtor.push_back(new_term(g.i, g.j, g.k, g.l, hb[h]->set, (int)s.qtorlib.size()));
+    }
+  }
+  // impropers; the reference compares against the first nqtor Q impropers
+  // (src/qatom.f90:1591), which this set-up repeats where it stays in range
+  {
+    std::vector<int> hits;
+    std::vector<const Break*> hb;
+    for (const Break& b : brk)
+      for (size_t a = 0; a < t.imp.size(); ++a) {
+        const Torsion& g = t.imp[a];
+        if (((g.i == b.i && g.j == b.j) || (g.i == b.i && g.j == b.i) || (g.j == b.i && g.k == b.j) ||
+             (g.k == b.i && g.j == b.j) || (g.j == b.i && g.l == b.j) || (g.l == b.i && g.j == b.j)) &&
+            g.cod != 0) { hits.push_back((int)a); hb.push_back(&b); }
+      }
+    const int nqtor = (int)s.qtor.size();
+    for (size_t h = 0; h < hits.size(); ++h) {
+      const Torsion& g = t.imp[hits[h]];
+      if (nqtor > (int)s.qimp.size())
+        throw std::runtime_error("impropers at a breaking bond with fewer Q impropers than Q torsions are not supported");
+      bool set = false;
+      for (int j = 0; j < nqtor; ++j) {
+        const QTerm& q = s.qimp[j];
+        if ((g.i == q.i && g.j == q.j && g.k == q.k && g.l == q.l) ||
+            (g.i == q.k && g.j == q.j && g.k == q.i && g.l == q.l) ||
+            (g.i == q.k && g.j == q.j && g.k == q.l && g.l == q.i) ||
+            (g.i == q.l && g.j == q.j && g.k == q.i && g.l == q.k) ||
+            (g.i == q.l && g.j == q.j && g.k == q.k && g.l == q.i)) set = true;
+      }
+      if (set) continue;
+      const ImpLib& L = t.implib[g.cod - 1];
+      s.qimplib.push_back({L.fk, L.imp0});
+      s.qimp.push_back(new_term(g.i, g.j, g.k, g.l, hb[h]->set, (int)s.qimplib.size()));
+    }
+  }
+  // couplings: an angle takes its first listed coupling, torsions and
+  // impropers their last (src/md.f90:17042-17052, 17264-17273, 17420-17429)
+  auto coupl = [&](const std::vector<QCoupling>& cl, int nterm, bool first, std::vector<int>& cb, std::vector<int>& cf) {
+    cb.assign(nterm, -1); cf.assign(nterm, 0);
+    for (const QCoupling& c : cl) {
+      if (c.term >= nterm) throw std::runtime_error("Q-term coupling refers to a missing term");
+      if (first && cb[c.term] >= 0) continue;
+      cb[c.term] = c.bond; cf[c.term] = c.flag;
+    }
+  };
+  coupl(f.ang_coupl, (int)s.qang.size(), true, s.qang_cb, s.qang_cf);
+  coupl(f.tor_coupl, (int)s.qtor.size(), false, s.qtor_cb, s.qtor_cf);
+  coupl(f.imp_coupl, (int)s.qimp.size(), false, s.qimp_cb, s.qimp_cf);
+  // remove the redefined interactions from the topology
+  for (Bond& b : t.bnd)
+    for (const QTerm& q : s.qbnd)
+      if ((b.i == q.i && b.j == q.j) || (b.i == q.j && b.j == q.i)) b.cod = 0;
+  for (Angle& a : t.ang)
+    for (const QTerm& q : s.qang)
+      if ((a.i == q.i && a.j == q.j && a.k == q.k) || (a.i == q.k && a.j == q.j && a.k == q.i)) a.cod = 0;
+  for (Torsion& a : t.tor)
+    for (const QTerm& q : s.qtor)
+      if (((a.i == q.i && a.j == q.j && a.k == q.k && a.l == q.l) ||
+           (a.i == q.l && a.j == q.k && a.k == q.j && a.l == q.i)) && a.cod != 0) a.cod = 0;
+  for (Torsion& a : t.imp)
+    for (const QTerm& q : s.qimp)
+      if ((a.i == q.i && a.j == q.j && a.k == q.k && a.l == q.l) ||
+          (a.i == q.k && a.j == q.j && a.k == q.i && a.l == q.l) ||
+          (a.i == q.k && a.j == q.j && a.k == q.l && a.l == q.i) ||
+          (a.i == q.l && a.j == q.j && a.k == q.i && a.l == q.k) ||
+          (a.i == q.l && a.j == q.j && a.k == q.k && a.l == q.i)) a.cod = 0;
+}
+
+// Extra restraints: atom numbers resolved and checked (src/md.f90:16431-16446).
+static void restraints_setup(Setup& s) {
+  const MDInput& in = s.inp;
+  const Topology& t = s.top;
+  s.rstseq = in.rstseq; s.rstpos = in.rstpos; s.rstdis = in.rstdis; s.rstang = in.rstang; s.rstwal = in.rstwal;
+  auto resolve = [&](const std::string& txt) {
+    size_t c = txt.find(':');
+    if (c == std::string::npos) return (int)std::strtol(txt.c_str(), nullptr, 10) - 1;
+    int res = (int)std::strtol(txt.substr(0, c).c_str(), nullptr, 10);
+    int at = (int)std::strtol(txt.substr(c + 1).c_str(), nullptr, 10);
+    const int nres = (int)t.res_start.size();
+    if (res < 1 || res > nres) return -1;
+    int first = t.res_start[res - 1], next = res < nres ? t.res_start[res] : s.natom;
+    if (at < 1 || at > next - first) return -1;
+    return first + at - 1;
+  };
+  for (size_t k = 0; k < s.rstdis.size(); ++k)
+    if (!in.rstdis_text[k].first.empty()) {
+      s.rstdis[k].i = resolve(in.rstdis_text[k].first);
+      s.rstdis[k].j = resolve(in.rstdis_text[k].second);
+    }
+  auto ok = [&](int a) { return a >= 0 && a < s.natom; };
+  auto okst = [&](int p) { return p >= 0 && p <= s.nstates; };
+  for (auto& r : s.rstseq)
+    if (!ok(r.i) || !ok(r.j)) throw std::runtime_error("invalid atom in [sequence_restraints]");
+  for (auto& r : s.rstpos)
+    if (!ok(r.i) || !okst(r.ipsi)) throw std::runtime_error("invalid atom or state in [atom_restraints]");
+  for (auto& r : s.rstdis)
+    if (!ok(r.i) || !ok(r.j) || !okst(r.ipsi)) throw std::runtime_error("invalid atom or state in [distance_restraints]");
+  for (auto& r : s.rstang)
+    if (!ok(r.i) || !ok(r.j) || !ok(r.k) || !okst(r.ipsi)) throw std::runtime_error("invalid atom or state in [angle_restraints]");
+  for (auto& r : s.rstwal)
+    if (!ok(r.i) || !ok(r.j)) throw std::runtime_error("invalid atom in [wall_restraints]");
+}
+
 Setup prepare(const std::string& input_path) {
   Setup s;
   const std::string dir = dir_of(input_path);
@@ -113,8 +284,18 @@ Setup prepare(const std::string& input_path) {
   if (in.temp0 <= 0) throw std::runtime_error("no dynamics at zero temperature");
   if (s.tau_t < s.dt) throw std::runtime_error("bath coupling time shorter than the step");
 
-  // prep_coord: restrained shell on topology coordinates (src/md.f90:16595-16622)
+  // prep_coord: topology coordinates for restraining, optionally refreshed
+  // from an external restart-format file (src/md.f90:16380-16425), then the
+  // restrained shell on them (src/md.f90:16595-16622)
   s.xtop = t.xtop;
+  if (!in.restraint_file.empty()) {
+    std::vector<double> xr, vr;
+    read_restart(join(dir, in.restraint_file), s.natom, xr, vr, nullptr);
+    s.xtop = xr;
+  }
+  s.mass.resize(s.natom);
+  for (int i = 0; i < s.natom; ++i) s.mass[i] = t.types[s.iac[i]].mass;
+  s.freeze = in.freeze;
   s.rexcl_i = in.have_shell_radius ? in.shell_radius : 0.85;
   if (s.rexcl_i < 0) throw std::runtime_error("inner radius of restrained shell must be >= 0");
   s.shell.assign(s.natom, 0);
@@ -217,9 +398,11 @@ Setup prepare(const std::string& input_path) {
           }
         }
     }
+    q_bonded_setup(s, f);
   } else if (!in.lambdas.empty()) {
     throw std::runtime_error("[lambdas] given without an FEP file");
   }
+  restraints_setup(s);
 
   // prep_sim: solvent parameters from the first water (src/md.f90:16704-16713)
   if (s.nwat > 0) {
@@ -360,15 +543,20 @@ Setup prepare(const std::string& input_path) {
   if (s.nqat > 0) {
     const int ns = s.nat_solute, nq = s.nqat, nst = s.nstates;
     s.qconn.assign((size_t)nst * nq * ns, 9);
-    std::vector<std::vector<int>> nbr(ns);
+    // find_bonded (src/md.f90:9626-9681): solute bonds that were not
+    // redefined (shaken ones included) plus the Q bonds present in the state
+    std::vector<std::vector<int>> nbr0(ns);
     for (int b = 0; b < t.nbonds_solute; ++b) {
       if (bnd[b].cod == 0) continue;
-      nbr[bnd[b].i].push_back(bnd[b].j);
-      nbr[bnd[b].j].push_back(bnd[b].i);
+      nbr0[bnd[b].i].push_back(bnd[b].j);
+      nbr0[bnd[b].j].push_back(bnd[b].i);
     }
     auto qc = [&](int st, int iq, int a) -> int& { return s.qconn[((size_t)st * nq + iq) * ns + a]; };
-    for (int iq = 0; iq < nq; ++iq)
-      for (int st = 0; st < nst; ++st) {
+    for (int st = 0; st < nst; ++st) {
+      std::vector<std::vector<int>> nbr = nbr0;
+      for (const QTerm& b : s.qbnd)
+        if (b.cod[st] > 0) { nbr[b.i].push_back(b.j); nbr[b.j].push_back(b.i); }
+      for (int iq = 0; iq < nq; ++iq) {
         int o = s.iqseq[iq];
         qc(st, iq, o) = 1;
         std::vector<int> frontier{o};
@@ -380,6 +568,7 @@ Setup prepare(const std::string& input_path) {
           frontier = nxt;
         }
       }
+    }
     s.nbqq.assign(nst, {});
     for (int iq = 0; iq < nq - 1; ++iq)
       for (int jq = iq + 1; jq < nq; ++jq) {

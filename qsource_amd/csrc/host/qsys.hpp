@@ -78,6 +78,14 @@ struct Topology {
 
 // The [MD]/[cut-offs]/[sphere]/[solvent]/[intervals]/[files]/[lambdas]
 // settings of a qdyn input file (src/md.f90:4453-5358) that this engine uses.
+// Extra restraints of the input file (src/md.f90:259-296), 0-based atoms;
+// ipsi is the FEP state (1-based, 0 = every state) as in the file.
+struct RstSeq { int i, j; double fk; int ih, to_centre; };
+struct RstPos { int i; double x[3], fk[3]; int ipsi; };
+struct RstDis { int i, j; double d1, d2, fk; int ipsi; };
+struct RstAng { int i, j, k; double ang, fk; int ipsi; };   // ang in degrees
+struct RstWal { int i, j; double d, fk, aMorse, dMorse; int ih; };
+
 struct MDInput {
   int nsteps = 0;
   double stepsize = 1.0;       // fs
@@ -98,7 +106,26 @@ struct MDInput {
   std::string top_file, restart_file, final_file, energy_file, fep_file;
   std::vector<double> lambdas;
   std::string thermostat = "berendsen", integrator = "leap-frog";
+  std::string restraint_file;            // [files] restraint: xtop from a restart file
+  std::vector<RstSeq> rstseq;
+  std::vector<RstPos> rstpos;
+  std::vector<RstDis> rstdis;
+  std::vector<RstAng> rstang;
+  std::vector<RstWal> rstwal;
+  // residue:atom texts of distance restraints, resolved against the topology
+  std::vector<std::pair<std::string, std::string>> rstdis_text;
 };
+
+// Q-bonded terms of the FEP file (src/qatom.f90:920-1790): atoms 0-based,
+// per-state type codes 1-based (0 = term absent in that state).
+struct QBondLib { double dmz, amz, r0, fk; };
+struct QAngLib { double fk, ang0, ureyfk, ureyr0; };   // ang0 in radians
+struct QTorLib { double fk, rmult, deltor; };          // deltor in radians
+struct QImpLib { double fk, imp0; };                   // imp0 in radians
+struct QTerm { int i, j, k, l; std::vector<int> cod; };
+// coupling of a Q angle/torsion/improper to a Morse Q bond: term, bond
+// (0-based), flag 1 = scale by 1 - the bond's Morse factor
+struct QCoupling { int term, bond, flag; };
 
 // EVB off-diagonal function H_ij = A exp(-mu r_kl) (src/qatom.f90:1809-1850);
 // states and Q atoms 0-based.
@@ -122,6 +149,12 @@ struct FepData {
   bool softcore = false, softcore_max_potential = false;
   std::vector<std::vector<double>> alpha_max;  // [nqat][nstates]
   std::vector<OffDiag> offd;
+  std::vector<QBondLib> qbondlib;
+  std::vector<QAngLib> qanglib;
+  std::vector<QTorLib> qtorlib;
+  std::vector<QImpLib> qimplib;
+  std::vector<QTerm> qbnd, qang, qtor, qimp;
+  std::vector<QCoupling> ang_coupl, tor_coupl, imp_coupl;
 };
 
 struct ShakeBond { int i, j; double dist2; };
@@ -184,6 +217,23 @@ struct Setup {
   std::vector<double> sc_lookup;
   std::vector<OffDiag> offd;
   int ff_type = 1;
+  // Q-bonded terms after the break-bond completion of qatom_load_fep, with
+  // their libraries; coupled bond per angle/torsion/improper (-1 = none)
+  // and its flag
+  std::vector<QBondLib> qbondlib;
+  std::vector<QAngLib> qanglib;
+  std::vector<QTorLib> qtorlib;
+  std::vector<QImpLib> qimplib;
+  std::vector<QTerm> qbnd, qang, qtor, qimp;
+  std::vector<int> qang_cb, qang_cf, qtor_cb, qtor_cf, qimp_cb, qimp_cf;
+  // extra restraints (p_restrain), atoms resolved
+  std::vector<RstSeq> rstseq;
+  std::vector<RstPos> rstpos;
+  std::vector<RstDis> rstdis;
+  std::vector<RstAng> rstang;
+  std::vector<RstWal> rstwal;
+  std::vector<double> mass;           // topology type mass per atom (iaclib(iac)%mass)
+  bool freeze = false;
 };
 
 // Readers.  Each throws std::runtime_error with a message on bad input.

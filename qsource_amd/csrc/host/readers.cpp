@@ -412,6 +412,50 @@ bool parse_onoff(const std::string& v, bool& out) {
 }
 }  // namespace
 
+// Extra restraints (src/md.f90:5211-5354); atoms 0-based.
+static void read_restraints(const Sections& S, MDInput& m) {
+  auto lines = [&](const char* name) {
+    std::vector<std::vector<std::string>> out;
+    if (!S.has(name)) return out;
+    for (auto& l : S.sec.at(name)) {
+      std::vector<std::string> tk;
+      Records::tokens(l, tk);
+      if (!tk.empty()) out.push_back(tk);
+    }
+    return out;
+  };
+  auto atom = [](const std::string& t) { return (int)to_i(t) - 1; };
+  // "i j fk ih [to_centre]" (to_centre 0 unless given)
+  for (auto& tk : lines("SEQUENCE_RESTRAINTS")) {
+    if (tk.size() < 4) throw std::runtime_error("bad [sequence_restraints] line");
+    m.rstseq.push_back({atom(tk[0]), atom(tk[1]), to_d(tk[2]), (int)to_i(tk[3]),
+                        tk.size() >= 5 ? (int)to_i(tk[4]) : 0});
+  }
+  for (auto& tk : lines("ATOM_RESTRAINTS")) {
+    if (tk.size() < 8) throw std::runtime_error("Invalid atom restraint data.");
+    RstPos r{atom(tk[0]), {to_d(tk[1]), to_d(tk[2]), to_d(tk[3])}, {to_d(tk[4]), to_d(tk[5]), to_d(tk[6])},
+             (int)to_i(tk[7])};
+    m.rstpos.push_back(r);
+  }
+  // "i j d1 d2 fk ipsi"; i and j may be residue:atom (resolved at set-up)
+  for (auto& tk : lines("DISTANCE_RESTRAINTS")) {
+    if (tk.size() < 6) throw std::runtime_error("Invalid distance restraint data.");
+    bool text = tk[0].find(':') != std::string::npos || tk[1].find(':') != std::string::npos;
+    m.rstdis.push_back({text ? -1 : atom(tk[0]), text ? -1 : atom(tk[1]), to_d(tk[2]), to_d(tk[3]),
+                        to_d(tk[4]), (int)to_i(tk[5])});
+    m.rstdis_text.push_back(text ? std::make_pair(tk[0], tk[1]) : std::make_pair(std::string(), std::string()));
+  }
+  for (auto& tk : lines("ANGLE_RESTRAINTS")) {
+    if (tk.size() < 6) throw std::runtime_error("Invalid angle restraint data.");
+    m.rstang.push_back({atom(tk[0]), atom(tk[1]), atom(tk[2]), to_d(tk[3]), to_d(tk[4]), (int)to_i(tk[5])});
+  }
+  for (auto& tk : lines("WALL_RESTRAINTS")) {
+    if (tk.size() < 7) throw std::runtime_error("Invalid wall restraint data.");
+    m.rstwal.push_back({atom(tk[0]), atom(tk[1]), to_d(tk[2]), to_d(tk[3]), to_d(tk[4]), to_d(tk[5]),
+                        (int)to_i(tk[6])});
+  }
+}
+
 MDInput read_input(const std::string& path) {
   Sections S(path);
   MDInput m;
@@ -490,10 +534,114 @@ MDInput read_input(const std::string& path) {
       for (auto& s : tk) m.lambdas.push_back(to_d(s));
     }
   }
+  S.get("FILES", "restraint", m.restraint_file);
+  read_restraints(S, m);
   return m;
 }
 
 // --------------------------------------------------------------------------
+namespace {
+const double kPiR = 4.0 * std::atan(1.0);
+const double kDeg2RadR = kPiR / 180.0;     // src/qatom.f90:218-219
+
+// Lines of a type section "[xxx_types]": type number (1-based) then values;
+// types may come in any order (prm_max_enum, src/qatom.f90:923).
+template <class Lib, class Fill>
+void read_types(const Sections& S, const char* name, std::vector<Lib>& lib, std::vector<char>& have, Fill fill) {
+  if (!S.has(name)) return;
+  for (auto& l : S.sec.at(name)) {
+    std::vector<std::string> tk;
+    Records::tokens(l, tk);
+    if (tk.empty()) continue;
+    int j = (int)to_i(tk[0]);
+    if (j < 1) throw std::runtime_error(std::string("bad type number in [") + name + "]");
+    if ((int)lib.size() < j) { lib.resize(j); have.resize(j, 0); }
+    fill(tk, lib[j - 1]);
+    have[j - 1] = 1;
+  }
+}
+
+// "[change_xxx]" lines: natoms topology atoms then one type code per state.
+void read_changes(const Sections& S, const char* name, int natoms, int nstates, int natom,
+                  const std::vector<char>& have, std::vector<QTerm>& out) {
+  if (!S.has(name)) return;
+  for (auto& l : S.sec.at(name)) {
+    std::vector<std::string> tk;
+    Records::tokens(l, tk);
+    if (tk.empty()) continue;
+    if ((int)tk.size() < natoms + nstates) throw std::runtime_error(std::string("bad [") + name + "] line");
+    int a[4] = {0, 0, 0, 0};
+    for (int m = 0; m < natoms; ++m) {
+      a[m] = (int)to_i(tk[m]) - 1;
+      if (a[m] < 0 || a[m] >= natom) throw std::runtime_error(std::string("invalid atom in [") + name + "]");
+    }
+    QTerm t{a[0], a[1], a[2], a[3], std::vector<int>(nstates, 0)};
+    for (int st = 0; st < nstates; ++st) {
+      int c = (int)to_i(tk[natoms + st]);
+      if (c < 0 || (c > 0 && (c > (int)have.size() || !have[c - 1])))
+        throw std::runtime_error(std::string("undefined type code in [") + name + "]");
+      t.cod[st] = c;
+    }
+    out.push_back(t);
+  }
+}
+
+void read_couplings(const Sections& S, const char* name, int nterms, const FepData& f,
+                    std::vector<QCoupling>& out) {
+  if (!S.has(name)) return;
+  for (auto& l : S.sec.at(name)) {
+    std::vector<std::string> tk;
+    Records::tokens(l, tk);
+    if (tk.empty()) continue;
+    if (tk.size() < 2) throw std::runtime_error(std::string("bad [") + name + "] line");
+    QCoupling c{(int)to_i(tk[0]) - 1, (int)to_i(tk[1]) - 1, tk.size() >= 3 ? (int)to_i(tk[2]) : 0};
+    if (c.term < 0 || c.term >= nterms || c.bond < 0 || c.bond >= (int)f.qbnd.size())
+      throw std::runtime_error(std::string("invalid term or bond in [") + name + "]");
+    // coupled bonds must be Morse in every state where present
+    // (bond_harmonic_in_any_state, src/qatom.f90:2039-2054)
+    for (int st = 0; st < f.nstates; ++st) {
+      int cod = f.qbnd[c.bond].cod[st];
+      if (cod != 0 && f.qbondlib[cod - 1].fk != 0.0)
+        throw std::runtime_error(std::string("coupled Q bond is harmonic in [") + name + "]");
+    }
+    out.push_back(c);
+  }
+}
+}  // namespace
+
+// Q-bonded sections of the FEP file (src/qatom.f90:920-1790).
+static void read_fep_bonded(const Sections& S, FepData& f, int natom) {
+  std::vector<char> hb, ha, ht, hi;
+  // [bond_types]: Morse "j Dmz amz r0", else harmonic "j fk r0"
+  read_types(S, "BOND_TYPES", f.qbondlib, hb, [](const std::vector<std::string>& tk, QBondLib& b) {
+    if (tk.size() >= 4) b = {to_d(tk[1]), to_d(tk[2]), to_d(tk[3]), 0.0};
+    else if (tk.size() == 3) b = {0.0, 0.0, to_d(tk[2]), to_d(tk[1])};
+    else throw std::runtime_error("bad [bond_types] line");
+  });
+  // [angle_types]: "j fk theta0 [ureyfk urey_r0]"
+  read_types(S, "ANGLE_TYPES", f.qanglib, ha, [](const std::vector<std::string>& tk, QAngLib& a) {
+    if (tk.size() < 3) throw std::runtime_error("bad [angle_types] line");
+    a = {to_d(tk[1]), kDeg2RadR * to_d(tk[2]), tk.size() >= 4 ? to_d(tk[3]) : 0.0, tk.size() >= 5 ? to_d(tk[4]) : 0.0};
+  });
+  read_types(S, "TORSION_TYPES", f.qtorlib, ht, [](const std::vector<std::string>& tk, QTorLib& t) {
+    if (tk.size() < 4) throw std::runtime_error("bad [torsion_types] line");
+    t = {to_d(tk[1]), to_d(tk[2]), kDeg2RadR * to_d(tk[3])};
+  });
+  read_types(S, "IMPROPER_TYPES", f.qimplib, hi, [](const std::vector<std::string>& tk, QImpLib& t) {
+    if (tk.size() < 3) throw std::runtime_error("bad [improper_types] line");
+    t = {to_d(tk[1]), kDeg2RadR * to_d(tk[2])};
+  });
+  for (auto& a : f.qanglib)
+    if (a.ureyfk != 0.0) throw std::runtime_error("Urey-Bradley Q-angle terms (CHARMM) are not supported");
+  read_changes(S, "CHANGE_BONDS", 2, f.nstates, natom, hb, f.qbnd);
+  read_changes(S, "CHANGE_ANGLES", 3, f.nstates, natom, ha, f.qang);
+  read_changes(S, "CHANGE_TORSIONS", 4, f.nstates, natom, ht, f.qtor);
+  read_changes(S, "CHANGE_IMPROPERS", 4, f.nstates, natom, hi, f.qimp);
+  read_couplings(S, "ANGLE_COUPLINGS", (int)f.qang.size(), f, f.ang_coupl);
+  read_couplings(S, "TORSION_COUPLINGS", (int)f.qtor.size(), f, f.tor_coupl);
+  read_couplings(S, "IMPROPER_COUPLINGS", (int)f.qimp.size(), f, f.imp_coupl);
+}
+
 FepData read_fep(const std::string& path, int natom) {
   Sections S(path);
   FepData f;
@@ -516,10 +664,7 @@ FepData read_fep(const std::string& path, int natom) {
   }
   int nq = (int)f.iqseq.size();
   f.qcrg.assign(nq, std::vector<double>(f.nstates, 0.0));
-  for (const char* unsup : {"CHANGE_BONDS", "CHANGE_ANGLES", "CHANGE_TORSIONS", "CHANGE_IMPROPERS",
-                            "BOND_TYPES", "ANGLE_TYPES", "TORSION_TYPES", "IMPROPER_TYPES",
-                            "ANGLE_COUPLINGS", "TORSION_COUPLINGS", "IMPROPER_COUPLINGS",
-                            "SOFT_PAIRS", "EXCLUDED_PAIRS", "EL_SCALE", "SHAKE_CONSTRAINTS",
+  for (const char* unsup : {"SOFT_PAIRS", "EXCLUDED_PAIRS", "EL_SCALE", "SHAKE_CONSTRAINTS",
                             "MONITOR_GROUPS", "MONITOR_GROUP_PAIRS"}) {
     if (S.has(unsup) && !S.sec.at(unsup).empty())
       throw std::runtime_error(std::string("FEP section [") + unsup + "] is not supported");
@@ -588,6 +733,7 @@ FepData read_fep(const std::string& path, int natom) {
     }
     if (n != nq) throw std::runtime_error("Alpha must be given for every Q-atom");
   }
+  read_fep_bonded(S, f, natom);
   // EVB off-diagonal functions
   if (S.has("OFF_DIAGONALS")) {
     for (auto& l : S.sec.at("OFF_DIAGONALS")) {

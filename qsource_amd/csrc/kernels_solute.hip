@@ -559,6 +559,289 @@ __device__ double dihedral_dp(const double* x, int i, int j, int k, int l, doubl
   return phi;
 }
 
+// Dihedral as qtorsion/qimproper form it (divisions by bj*bk and bj**2,
+// src/md.f90:17295-17338): phi and its derivative components times -1/sin.
+__device__ double qdihedral_dp(const double* x, int i, int j, int k, int l, double dp[12]) {
+  double rji[3], rjk[3], rkl[3], rnj[3], rnk[3];
+  for (int c = 0; c < 3; ++c) {
+    rji[c] = x[3 * i + c] - x[3 * j + c];
+    rjk[c] = x[3 * k + c] - x[3 * j + c];
+    rkl[c] = x[3 * l + c] - x[3 * k + c];
+  }
+  rnj[0] = rji[1] * rjk[2] - rji[2] * rjk[1];
+  rnj[1] = rji[2] * rjk[0] - rji[0] * rjk[2];
+  rnj[2] = rji[0] * rjk[1] - rji[1] * rjk[0];
+  rnk[0] = -rjk[1] * rkl[2] + rjk[2] * rkl[1];
+  rnk[1] = -rjk[2] * rkl[0] + rjk[0] * rkl[2];
+  rnk[2] = -rjk[0] * rkl[1] + rjk[1] * rkl[0];
+  const double bj = sqrt(rnj[0] * rnj[0] + rnj[1] * rnj[1] + rnj[2] * rnj[2]);
+  const double bk = sqrt(rnk[0] * rnk[0] + rnk[1] * rnk[1] + rnk[2] * rnk[2]);
+  double scp = (rnj[0] * rnk[0] + rnj[1] * rnk[1] + rnj[2] * rnk[2]) / (bj * bk);
+  if (scp > 1.0) scp = 1.0;
+  if (scp < -1.0) scp = -1.0;
+  double phi = acos(scp);
+  if (rjk[0] * (rnj[1] * rnk[2] - rnj[2] * rnk[1]) + rjk[1] * (rnj[2] * rnk[0] - rnj[0] * rnk[2]) +
+          rjk[2] * (rnj[0] * rnk[1] - rnj[1] * rnk[0]) < 0)
+    phi = -phi;
+  double f1 = sin(phi);
+  if (fabs(f1) < qsys::kQrealEps) f1 = qsys::kQrealEps;
+  f1 = -1.0 / f1;
+  double di[3], dl[3], rki[3], rlj[3];
+  for (int c = 0; c < 3; ++c) {
+    di[c] = f1 * (rnk[c] / (bj * bk) - scp * rnj[c] / (bj * bj));
+    dl[c] = f1 * (rnj[c] / (bj * bk) - scp * rnk[c] / (bk * bk));
+    rki[c] = rji[c] - rjk[c];
+    rlj[c] = -rjk[c] - rkl[c];
+  }
+  dp[0] = rjk[1] * di[2] - rjk[2] * di[1];
+  dp[1] = rjk[2] * di[0] - rjk[0] * di[2];
+  dp[2] = rjk[0] * di[1] - rjk[1] * di[0];
+  dp[3] = rki[1] * di[2] - rki[2] * di[1] + rkl[1] * dl[2] - rkl[2] * dl[1];
+  dp[4] = rki[2] * di[0] - rki[0] * di[2] + rkl[2] * dl[0] - rkl[0] * dl[2];
+  dp[5] = rki[0] * di[1] - rki[1] * di[0] + rkl[0] * dl[1] - rkl[1] * dl[0];
+  dp[6] = rlj[1] * dl[2] - rlj[2] * dl[1] - rji[1] * di[2] + rji[2] * di[1];
+  dp[7] = rlj[2] * dl[0] - rlj[0] * dl[2] - rji[2] * di[0] + rji[0] * di[2];
+  dp[8] = rlj[0] * dl[1] - rlj[1] * dl[0] - rji[0] * di[1] + rji[1] * di[0];
+  dp[9] = rjk[1] * dl[2] - rjk[2] * dl[1];
+  dp[10] = rjk[2] * dl[0] - rjk[0] * dl[2];
+  dp[11] = rjk[0] * dl[1] - rjk[1] * dl[0];
+  return phi;
+}
+
+// Morse Q bond ib of parameters P = (Dmz, amz, r0, fk) at state weight lam
+// (qbond, src/md.f90:17213-17242): energy, gradient factor dv (gradient on
+// j = dv * rij), and the Morse factor record (EMorseD, dMorse_i, dMorse_j).
+__device__ double qbond_eval(const DevSys& S, int ib, const double* P, double lam, double rij[3], double& dv,
+                             double rec[kMorseRec]) {
+  const double* x = S.x;
+  const int i3 = 3 * S.qb_atoms[2 * ib], j3 = 3 * S.qb_atoms[2 * ib + 1];
+  for (int c = 0; c < 3; ++c) rij[c] = x[j3 + c] - x[i3 + c];
+  const double b = sqrt(rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2]);
+  const double db = b - P[2];
+  const double fexp = exp(-P[1] * db);
+  const double be = P[0] * (fexp * fexp - 2.0 * fexp) + 0.5 * P[3] * (db * db);
+  dv = (2.0 * P[0] * P[1] * (fexp - fexp * fexp) + P[3] * db) * lam / b;
+  if (db > 0) {
+    rec[0] = -(fexp * fexp - 2.0 * fexp);
+    for (int c = 0; c < 3; ++c) {
+      rec[1 + c] = +2.0 * P[1] * (fexp - fexp * fexp) * lam / b * rij[c];
+      rec[4 + c] = -2.0 * P[1] * (fexp - fexp * fexp) * lam / b * rij[c];
+    }
+  } else {
+    rec[0] = 1.0;
+    for (int c = 0; c < 6; ++c) rec[1 + c] = 0.0;
+  }
+  return be;
+}
+
+// Morse factor record of the bond a coupled term sees (see DevSys::xt).
+__device__ void coupled_morse(const DevSys& S, const int* R, const double* P, double rec[kMorseRec]) {
+  const int ib = R[6], seff = R[8];
+  if (seff >= 0) {
+    double rij[3], dv;
+    qbond_eval(S, ib, P + 4, S.lambda[seff], rij, dv, rec);
+  } else {
+    for (int c = 0; c < kMorseRec; ++c) rec[c] = S.qb_prev[kMorseRec * ib + c];
+  }
+}
+
+// One extra term (XtKind record n): gradient into its slots, energy e_xt[n].
+__device__ void xterm(const DevSys& S, int n) {
+  const int* R = S.xt + (size_t)kXtInts * n;
+  const double* P = S.xp + (size_t)kXtPars * n;
+  const double* x = S.x;
+  const int kind = R[0], s0 = R[9];
+  double e = 0.0;
+  switch (kind) {
+    case XT_QBOND: {
+      // R: kind i j ib last-present-state state
+      double rij[3], dv, rec[kMorseRec];
+      e = qbond_eval(S, R[3], P, S.lambda[R[5]], rij, dv, rec);
+      put_slot(S, s0, -dv * rij[0], -dv * rij[1], -dv * rij[2]);
+      put_slot(S, s0 + 1, dv * rij[0], dv * rij[1], dv * rij[2]);
+      if (R[4]) for (int c = 0; c < kMorseRec; ++c) S.qb_cur[kMorseRec * R[3] + c] = rec[c];
+      break;
+    }
+    case XT_QANGLE: {
+      // qangle (src/md.f90:17022-17114); R[6] coupled bond, R[7] flag, R[8] its state
+      double gamma = 1.0, rec[kMorseRec];
+      if (R[6] >= 0) {
+        coupled_morse(S, R, P, rec);
+        gamma = rec[0];
+        if (R[7] == 1) gamma = 1.0 - gamma;
+      }
+      const int i = R[1], j = R[2], k = R[3];
+      double rji[3], rjk[3];
+      for (int c = 0; c < 3; ++c) { rji[c] = x[3 * i + c] - x[3 * j + c]; rjk[c] = x[3 * k + c] - x[3 * j + c]; }
+      const double bji = sqrt(rji[0] * rji[0] + rji[1] * rji[1] + rji[2] * rji[2]);
+      const double bjk = sqrt(rjk[0] * rjk[0] + rjk[1] * rjk[1] + rjk[2] * rjk[2]);
+      double scp = (rji[0] * rjk[0] + rji[1] * rjk[1] + rji[2] * rjk[2]);
+      scp = scp / (bji * bjk);
+      if (scp > 1.0) scp = 1.0;
+      if (scp < -1.0) scp = -1.0;
+      const double ang = acos(scp);
+      const double da = ang - P[1];
+      const double ae = 0.5 * P[0] * (da * da);
+      e = ae * gamma;
+      const double dv = gamma * P[0] * da * S.lambda[R[5]];
+      double f1 = sin(ang);
+      if (fabs(f1) < qsys::kQrealEps) f1 = qsys::kQrealEps;
+      f1 = -1.0 / f1;
+      double di[3], dk[3];
+      for (int c = 0; c < 3; ++c) {
+        di[c] = f1 * (rjk[c] / (bji * bjk) - scp * rji[c] / (bji * bji));
+        dk[c] = f1 * (rji[c] / (bji * bjk) - scp * rjk[c] / (bjk * bjk));
+      }
+      put_slot(S, s0, dv * di[0], dv * di[1], dv * di[2]);
+      put_slot(S, s0 + 1, -dv * (di[0] + dk[0]), -dv * (di[1] + dk[1]), -dv * (di[2] + dk[2]));
+      put_slot(S, s0 + 2, dv * dk[0], dv * dk[1], dv * dk[2]);
+      if (R[6] >= 0) {
+        put_slot(S, s0 + 3, rec[1] * ae, rec[2] * ae, rec[3] * ae);
+        put_slot(S, s0 + 4, rec[4] * ae, rec[5] * ae, rec[6] * ae);
+      }
+      break;
+    }
+    case XT_QTORSION:
+    case XT_QIMPROPER: {
+      // qtorsion (src/md.f90:17395-17540), qimproper (:17249-17392)
+      double gamma = 1.0, rec[kMorseRec];
+      if (R[6] >= 0) {
+        coupled_morse(S, R, P, rec);
+        gamma = rec[0];
+        if (R[7] == 1) gamma = 1 - gamma;
+      }
+      double dp[12];
+      const double phi = qdihedral_dp(x, R[1], R[2], R[3], R[4], dp);
+      double pe, dv;
+      if (kind == XT_QTORSION) {
+        const double arg = P[1] * phi - P[2];
+        pe = P[0] * (1.0 + cos(arg));
+        dv = -P[1] * P[0] * sin(arg) * gamma * S.lambda[R[5]];
+      } else {
+        double arg = phi - P[1];
+        arg = arg - 2.0 * kPiD * round(arg / (2.0 * kPiD));
+        dv = P[0] * arg;
+        pe = 0.5 * dv * arg;
+        dv = dv * gamma * S.lambda[R[5]];
+      }
+      e = pe * gamma;
+      for (int m = 0; m < 4; ++m) put_slot(S, s0 + m, dv * dp[3 * m], dv * dp[3 * m + 1], dv * dp[3 * m + 2]);
+      if (R[6] >= 0) {
+        put_slot(S, s0 + 4, rec[1] * pe, rec[2] * pe, rec[3] * pe);
+        put_slot(S, s0 + 5, rec[4] * pe, rec[5] * pe, rec[6] * pe);
+      }
+      break;
+    }
+    case XT_RST_SEQ: {
+      // sequence restraint of one atom to its topology position (:15549-15575)
+      const int i3 = 3 * R[1];
+      const double d0 = x[i3] - S.xtop[i3], d1 = x[i3 + 1] - S.xtop[i3 + 1], d2 = x[i3 + 2] - S.xtop[i3 + 2];
+      e = 0.5 * P[0] * (d0 * d0 + d1 * d1 + d2 * d2);
+      put_slot(S, s0, P[0] * d0, P[0] * d1, P[0] * d2);
+      break;
+    }
+    case XT_RST_SEQC: {
+      // sequence restraint of a geometric (R[3] = 1) or mass (2) centre
+      // (:15486-15547): R[1] first entry in rst_atoms, R[2] count
+      const int* A = S.rst_atoms + R[1];
+      const int na = R[2];
+      double dr[3] = {0, 0, 0}, norm = 0;
+      for (int m = 0; m < na; ++m) {
+        const int i3 = 3 * A[m];
+        const double w = R[3] == 1 ? 1.0 : S.mass[A[m]];
+        for (int c = 0; c < 3; ++c)
+          dr[c] = R[3] == 1 ? dr[c] + x[i3 + c] - S.xtop[i3 + c] : dr[c] + (x[i3 + c] - S.xtop[i3 + c]) * w;
+        norm = norm + w;
+      }
+      for (int c = 0; c < 3; ++c) dr[c] = dr[c] / norm;
+      e = 0.5 * P[0] * (dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+      for (int m = 0; m < na; ++m) {
+        const double ms = S.mass[A[m]];
+        if (R[3] == 1) put_slot(S, s0 + m, P[0] * dr[0] * ms / 12.010, P[0] * dr[1] * ms / 12.010, P[0] * dr[2] * ms / 12.010);
+        else put_slot(S, s0 + m, P[0] * dr[0], P[0] * dr[1], P[0] * dr[2]);
+      }
+      break;
+    }
+    case XT_RST_POS: {
+      // position restraint (:15580-15619), P: x0[3] fk[3]
+      const int i3 = 3 * R[1];
+      const double wgt = R[5] != 0 ? S.lambda[R[5] - 1] : 1.0;
+      double dr[3];
+      for (int c = 0; c < 3; ++c) dr[c] = x[i3 + c] - P[c];
+      e = 0.5 * P[3] * (dr[0] * dr[0]) + 0.5 * P[4] * (dr[1] * dr[1]) + 0.5 * P[5] * (dr[2] * dr[2]);
+      put_slot(S, s0, P[3] * dr[0] * wgt, P[4] * dr[1] * wgt, P[5] * dr[2] * wgt);
+      break;
+    }
+    case XT_RST_DIST: {
+      // flat-bottom distance restraint (:15622-15670), P: d1 d2 fk
+      const int i3 = 3 * R[1], j3 = 3 * R[2];
+      const double wgt = R[5] != 0 ? S.lambda[R[5] - 1] : 1.0;
+      double dr[3];
+      for (int c = 0; c < 3; ++c) dr[c] = x[j3 + c] - x[i3 + c];
+      const double b = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+      double db = 0.0;
+      if (b < P[0]) db = b - P[0];
+      else if (b > P[1]) db = b - P[1];
+      const double dv = db != 0.0 ? wgt * P[2] * db / b : 0.0;
+      e = db != 0.0 ? 0.5 * P[2] * (db * db) : 0.0;
+      put_slot(S, s0, -dr[0] * dv, -dr[1] * dv, -dr[2] * dv);
+      put_slot(S, s0 + 1, dr[0] * dv, dr[1] * dv, dr[2] * dv);
+      break;
+    }
+    case XT_RST_ANG: {
+      // angle restraint (:15673-15762), P: theta0 (radians) fk
+      const int i3 = 3 * R[1], j3 = 3 * R[2], k3 = 3 * R[3];
+      const double wgt = R[5] != 0 ? S.lambda[R[5] - 1] : 1.0;
+      double dr[3], dr2[3];
+      for (int c = 0; c < 3; ++c) { dr[c] = x[i3 + c] - x[j3 + c]; dr2[c] = x[k3 + c] - x[j3 + c]; }
+      const double r2ij = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+      const double r2jk = dr2[0] * dr2[0] + dr2[1] * dr2[1] + dr2[2] * dr2[2];
+      const double rij = sqrt(r2ij), rjk = sqrt(r2jk);
+      double scp = (dr[0] * dr2[0] + dr[1] * dr2[1] + dr[2] * dr2[2]);
+      scp = scp / (rij * rjk);
+      if (scp > 1.0) scp = 1.0;
+      if (scp < -1.0) scp = -1.0;
+      const double theta = acos(scp);
+      const double db = theta - P[0];
+      e = 0.5 * P[1] * (db * db);
+      const double dv = wgt * P[1] * db;
+      double f1 = sin(theta);
+      if (fabs(f1) < qsys::kQrealEps) f1 = qsys::kQrealEps;
+      f1 = -1.0 / f1;
+      double di[3], dk[3];
+      for (int c = 0; c < 3; ++c) {
+        di[c] = f1 * (dr2[c] / (rij * rjk) - scp * dr[c] / r2ij);
+        dk[c] = f1 * (dr[c] / (rij * rjk) - scp * dr2[c] / r2jk);
+      }
+      put_slot(S, s0, dv * di[0], dv * di[1], dv * di[2]);
+      put_slot(S, s0 + 1, -dv * (di[0] + dk[0]), -dv * (di[1] + dk[1]), -dv * (di[2] + dk[2]));
+      put_slot(S, s0 + 2, dv * dk[0], dv * dk[1], dv * dk[2]);
+      break;
+    }
+    case XT_RST_WALL: {
+      // half-harmonic wall with Morse attraction (:15765-15797), P: d fk aMorse dMorse
+      const int i3 = 3 * R[1];
+      double dr[3];
+      for (int c = 0; c < 3; ++c) dr[c] = x[i3 + c] - S.xwcent[c];
+      const double b = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+      const double db = b - P[0];
+      double dv;
+      if (db > 0) {
+        e = 0.5 * P[1] * (db * db) - P[3];
+        dv = P[1] * db / b;
+      } else {
+        const double fexp = exp(P[2] * db);
+        e = P[3] * (fexp * fexp - 2.0 * fexp);
+        dv = -2.0 * P[3] * P[2] * (fexp - fexp * fexp) / b;
+      }
+      put_slot(S, s0, dv * dr[0], dv * dr[1], dv * dr[2]);
+      break;
+    }
+    default: break;
+  }
+  S.e_xt[n] = e;
+}
+
 __global__ __launch_bounds__(256) void k_terms(const DevSys* __restrict__ sys) {
   const DevSys& S = sys[blockIdx.y];
   int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -743,13 +1026,19 @@ __global__ __launch_bounds__(256) void k_terms(const DevSys* __restrict__ sys) {
     put_slot(S, s0 + 1, dv * dx1, dv * dx2, dv * dx3);
     S.e_qq[2 * p] = Vel;
     S.e_qq[2 * p + 1] = Va - Vb;
+    return;
   }
+  n -= S.nqq;
+  if (n < S.nxt) xterm(S, n);
 }
 
-// Adds each atom's term slots to its gradient, in slot order.
+// Adds each atom's term slots to its gradient, in slot order, and keeps the
+// Q bonds' Morse factors of this evaluation for the next one.
 __global__ __launch_bounds__(256) void k_term_gather(const DevSys* __restrict__ sys) {
   const DevSys& S = sys[blockIdx.y];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < S.nqbond)
+    for (int c = 0; c < kMorseRec; ++c) S.qb_prev[kMorseRec * t + c] = S.qb_cur[kMorseRec * t + c];
   if (t >= S.nslot_atoms) return;
   const int me = S.slot_atom[t];
   double g0 = 0, g1 = 0, g2 = 0;

@@ -692,6 +692,16 @@ __global__ __launch_bounds__(256) void k_energy_reduce(const DevSys* __restrict_
   for (int c = 0; c < 6; ++c) b[c] = block_sum_256(b[c], red[c]);
   for (int c = 0; c < 2; ++c) bi[c] = block_sum_256(bi[c], red[6 + c]);
   ef = block_sum_256(ef, red[8]);
+  // extra terms: restraints outside the FEP states (sequence, wall, and the
+  // state-independent ones of a run without states, src/md.f90:15612-15618)
+  double rp = 0.0;
+  for (int n = threadIdx.x; n < S.nxt; n += blockDim.x) {
+    const int kind = S.xt[kXtInts * n], tag = S.xt[kXtInts * n + 5];
+    if (kind == XT_RST_SEQ || kind == XT_RST_SEQC || kind == XT_RST_WALL ||
+        (kind >= XT_RST_POS && kind <= XT_RST_ANG && tag == 0 && S.nstates == 0))
+      rp += S.e_xt[n];
+  }
+  rp = block_sum_256(rp, red[9]);
   // per-state Q energies: qq, qp (Q-neighbour pairs of nonbond_qq and the
   // Q-solute list), qw; pass 1 forms the FP64 re-evaluated frame of a
   // mixed-precision batch from e_q64 (Q-Q pairs are FP64 in both)
@@ -699,7 +709,16 @@ __global__ __launch_bounds__(256) void k_energy_reduce(const DevSys* __restrict_
   for (int st = 0; st < S.nstates; ++st) {
     if (pass == 1 && !(frame >= 0 && S.eseries64)) break;
     const double* eq = pass == 0 ? S.e_q : S.e_q64;
-    double q[6] = {0, 0, 0, 0, 0, 0};
+    // qq qp qw (el, vdw), Q bond angle torsion improper, restraint
+    double q[kQTerms] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int n = threadIdx.x; n < S.nxt; n += blockDim.x) {
+      const int kind = S.xt[kXtInts * n], tag = S.xt[kXtInts * n + 5];
+      if (kind <= XT_QIMPROPER) {
+        if (tag == st) q[5 + kind] += S.e_xt[n];
+      } else if (kind >= XT_RST_POS && kind <= XT_RST_ANG && (tag == 0 || tag == st + 1)) {
+        q[10] += S.e_xt[n];
+      }
+    }
     for (int iq = threadIdx.x; iq < S.nqat; iq += blockDim.x) {
       const double* e = eq + ((size_t)iq * S.nstates + st) * 4;
       q[2] += e[0]; q[3] += e[1]; q[4] += e[2]; q[5] += e[3];
@@ -710,7 +729,7 @@ __global__ __launch_bounds__(256) void k_energy_reduce(const DevSys* __restrict_
       q[o] += S.e_qq[2 * pr];
       q[o + 1] += S.e_qq[2 * pr + 1];
     }
-    for (int c = 0; c < 6; ++c) {
+    for (int c = 0; c < kQTerms; ++c) {
       q[c] = block_sum_256(q[c], red[c]);
       if (threadIdx.x == 0 && pass == 0) S.qenergies[st * kQTerms + c] = q[c];
     }
@@ -720,11 +739,11 @@ __global__ __launch_bounds__(256) void k_energy_reduce(const DevSys* __restrict_
       // pot_energy forms them, src/md.f90:16186-16194)
       double* r = (pass == 0 ? S.eseries : S.eseries64) + ((size_t)frame * S.nstates + st) * kEneRecord;
       const double qx_el = q[0] + q[2] + q[4], qx_vdw = q[1] + q[3] + q[5];
-      r[0] = 0.0 + 0.0 + 0.0 + 0.0 + qx_el + qx_vdw + 0.0;
-      r[1] = 0.0; r[2] = 0.0; r[3] = 0.0; r[4] = 0.0;
+      r[0] = q[6] + q[7] + q[8] + q[9] + qx_el + qx_vdw + q[10];
+      r[1] = q[6]; r[2] = q[7]; r[3] = q[8]; r[4] = q[9];
       r[5] = qx_el; r[6] = qx_vdw;
       r[7] = q[0]; r[8] = q[1]; r[9] = q[2]; r[10] = q[3]; r[11] = q[4]; r[12] = q[5];
-      r[13] = 0.0;
+      r[13] = q[10];
     }
   }
   // EVB off-diagonal elements of the frame (offdiag, src/md.f90:15444-15470)
@@ -746,7 +765,7 @@ __global__ __launch_bounds__(256) void k_energy_reduce(const DevSys* __restrict_
     double* E = S.energies;
     E[E_WW_EL] = w[0]; E[E_WW_VDW] = w[1]; E[E_LRF] = w[2] + p[4] + ef; E[E_RST_RADIAL] = w[3];
     E[E_PP_EL] = p[0]; E[E_PP_VDW] = p[1]; E[E_PW_EL] = p[2]; E[E_PW_VDW] = p[3];
-    E[E_RST_FIX] = p[5]; E[E_RST_SHELL] = p[6];
+    E[E_RST_FIX] = p[5]; E[E_RST_SHELL] = p[6]; E[E_RST_PROTEIN] = rp;
     double ew = 0.0;
     for (int s = 0; s < S.nshell; ++s) ew = ew + S.e_wpol[s];
     E[E_RST_WPOL] = S.wpol_restr ? ew : 0.0;

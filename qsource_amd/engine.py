@@ -178,10 +178,11 @@ class Engine:
         return dict(zip(ENERGY_NAMES, e.tolist()))
 
     def qenergies(self, k: int = 0) -> np.ndarray:
-        """Per-state Q energies: rows of qq_el qq_vdw qp_el qp_vdw qw_el qw_vdw total."""
-        e = np.zeros(7 * max(1, self.nstates[k]))
+        """Per-state Q energies: rows of qq_el qq_vdw qp_el qp_vdw qw_el qw_vdw
+        total bond angle torsion improper restraint."""
+        e = np.zeros(12 * max(1, self.nstates[k]))
         self._check(self.lib.qgpu_qenergies(self.h, k, _dp(e), len(e)))
-        return e[:7 * self.nstates[k]].reshape(-1, 7)
+        return e[:12 * self.nstates[k]].reshape(-1, 12)
 
     def pairlist(self, k: int, which: int) -> np.ndarray:
         n = ctypes.c_long(0)

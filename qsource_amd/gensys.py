@@ -95,8 +95,23 @@ class System:
     softcore: Optional[np.ndarray] = None  # (nqat, nstates) alphas ([softcore])
     offdiag: Optional[list] = None         # (state_i, state_j, q_k, q_l, A, mu), 1-based
     ff_type: int = 1                       # 1 GROMOS (harmonic impropers), 2 AMBER (periodic)
-    # off-diagonal (EVB) couplings, optional: rows (i, j, k, l, A, mu)
-    offdiag: Optional[list] = None
+    # Q-bonded FEP sections (optional): type rows and terms with one type
+    # code per state (0 = absent): qbond_types (Dmz, alpha, r0) with qbonds
+    # (i, j, codes); qangle_types (fk, theta0) with qangles (i, j, k, codes);
+    # qtorsion_types (fk, mult, delta) with qtorsions (i, j, k, l, codes);
+    # qimproper_types (fk, imp0) with qimpropers (i, j, k, l, codes);
+    # qcouplings {"angle"|"torsion"|"improper": [(term, bond, flag)]}, 1-based
+    qbond_types: Optional[list] = None
+    qbonds: Optional[list] = None
+    qangle_types: Optional[list] = None
+    qangles: Optional[list] = None
+    qtorsion_types: Optional[list] = None
+    qtorsions: Optional[list] = None
+    qimproper_types: Optional[list] = None
+    qimpropers: Optional[list] = None
+    qcouplings: Optional[dict] = None
+    # input-file restraints ([sequence|atom|distance|angle|wall]_restraints rows)
+    restraints: Optional[dict] = None
 
     @property
     def natom(self) -> int:
@@ -333,7 +348,7 @@ def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
                     n_states: int = 0, n_types: int = 250,
                     q_sd: float = 0.3, q_types: bool = False,
                     softcore: Optional[float] = None, n_offdiag: int = 0,
-                    impropers: int = 0) -> System:
+                    impropers: int = 0, q_bonded: bool = False, restraints: bool = False) -> System:
     """A TIP3P sphere of the given radius around an optional branched solute
     (charges N(0,q_sd) neutralised per 4-atom charge group, sigma~U[2.5,3.8],
     eps~U[0.05,0.2] quantised into at most n_types atom types), with an
@@ -533,7 +548,167 @@ def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
         sysm.nimps_solute = len(imps)
         sysm.ff_type = 1 if impropers == 1 else 2
         sysm.implib = np.array([[3.0, 0.0]]) if impropers == 1 else np.array([[1.5, 180.0]])
+    if q_bonded and sysm.qatoms is not None and n_states >= 2:
+        _q_bonded_terms(sysm, rng, n_states)
+    if restraints and n_solute >= 40:
+        sysm.restraints = _example_restraints(sysm, max(n_states, 0))
     return sysm
+
+
+def _example_restraints(sysm: System, n_states: int) -> dict:
+    """One of every p_restrain kind on the solute, around its current
+    geometry: sequence restraints to topology positions (heavy atoms), to the
+    geometric and to the mass centre; position restraints; flat-bottom
+    distance restraints active below and above their range and one in range
+    (residue:atom notation); angle restraints; a wall with Morse attraction.
+    FEP states are used where the system has them."""
+    x = sysm.x
+    st = lambda k: k if n_states >= k else 0
+    dist = lambda i, j: float(np.linalg.norm(x[i - 1] - x[j - 1]))
+
+    def ang(i, j, k):
+        u, w = x[i - 1] - x[j - 1], x[k - 1] - x[j - 1]
+        return float(np.degrees(np.arccos(np.dot(u, w) / np.linalg.norm(u) / np.linalg.norm(w))))
+
+    f = lambda v: "%.4f" % v
+    return {
+        "sequence": [(1, 10, 2.0, 0), (11, 20, 1.5, 1, 1), (21, 30, 1.0, 0, 2)],
+        "atom": [(5, f(x[4, 0] + 0.3), f(x[4, 1] - 0.2), f(x[4, 2] + 0.1), 1.0, 2.0, 3.0, 0),
+                 (12, f(x[11, 0] - 0.2), f(x[11, 1] + 0.1), f(x[11, 2]), 2.0, 2.0, 2.0, st(2))],
+        "distance": [(3, 15, f(dist(3, 15) - 0.5), f(dist(3, 15) - 0.2), 5.0, 0),
+                     (7, 33, f(dist(7, 33) + 0.3), f(dist(7, 33) + 1.0), 4.0, st(1)),
+                     ("1:12", "1:25", 0.0, f(dist(12, 25) + 0.5), 2.0, st(2))],
+        "angle": [(2, 9, 4, f(ang(2, 9, 4) + 5.0), 10.0, st(1)), (6, 17, 8, f(ang(6, 17, 8) - 3.0), 20.0, 0)],
+        "wall": [(1, 40, 9.0, 5.0, 1.5, 0.3, 1)],
+    }
+
+
+def _q_bonded_terms(sysm: System, rng, n_states: int) -> None:
+    """Q-bonded FEP sections over the Q region (EVB-style): every solute
+    bond, angle, torsion and improper whose atoms are all Q atoms gets a type
+    per state; one topology bond breaks (Morse, present in the first state
+    only) and one bond forms between the closest Q atoms not bonded in the
+    topology (present in the last state only, r0 at their distance, with a
+    new angle through it); the terms through the breaking bond are absent
+    where it is, and angles/torsions/impropers are coupled to both bonds,
+    including terms present in every state.  A bond that a topology improper
+    touches is not chosen to break."""
+    qs = set(int(a) for a in sysm.qatoms)
+    sb = [(int(i), int(j)) for i, j, _ in sysm.bonds[:sysm.nbonds_solute]]
+    qb = [b for b in sb if b[0] in qs and b[1] in qs]
+    imp_atoms = [set(int(a) for a in r[:4]) for r in sysm.imps]
+    free = [b for b in qb if not any(b[0] in ia and b[1] in ia for ia in imp_atoms)]
+    nbr = {a: set() for a in range(1, sysm.nat_solute + 1)}
+    for i, j in sb:
+        nbr[i].add(j)
+        nbr[j].add(i)
+    # prefer a breaking bond with non-Q neighbours: the topology terms
+    # through it that the FEP file does not redefine become Q terms
+    outer = [b for b in free if (nbr[b[0]] | nbr[b[1]]) - qs]
+    brk = outer[0] if outer else (free[0] if free else None)
+    # forming bond: closest Q pair at topological distance >= 3
+    x = sysm.x
+    frm, frm_r = None, 1e9
+    ql = sorted(qs)
+    for m, i in enumerate(ql):
+        for j in ql[m + 1:]:
+            if j in nbr[i] or nbr[i] & nbr[j]:
+                continue
+            r = float(np.linalg.norm(x[i - 1] - x[j - 1]))
+            if r < frm_r:
+                frm, frm_r = (i, j), r
+    bonds = list(qb) + ([frm] if frm else [])
+    present = {b: [True] * n_states for b in bonds}
+    if brk:
+        present[brk] = [st == 0 for st in range(n_states)]
+    if frm:
+        present[frm] = [st == n_states - 1 for st in range(n_states)]
+
+    def has(atoms, b):
+        return any({atoms[m], atoms[m + 1]} == set(b) for m in range(len(atoms) - 1))
+
+    def term_present(atoms):
+        return [all(present[b][st] for b in bonds if has(atoms, b)) for st in range(n_states)]
+
+    btypes, qbonds = [], []
+    for b in bonds:
+        codes = []
+        for st in range(n_states):
+            if present[b][st]:
+                r0 = round(frm_r, 3) if b == frm else round(1.53 * float(rng.uniform(0.97, 1.03)), 3)
+                btypes.append((round(float(rng.uniform(60, 110)), 2), round(float(rng.uniform(1.6, 2.4)), 3), r0))
+                codes.append(len(btypes))
+            else:
+                codes.append(0)
+        qbonds.append((b[0], b[1], codes))
+    atypes, qangles = [], []
+    for i, j, k, _ in sysm.angles[:sysm.nangles_solute]:
+        a = (int(i), int(j), int(k))
+        if not all(v in qs for v in a):
+            continue
+        pres = term_present(a)
+        codes = []
+        for st in range(n_states):
+            if pres[st]:
+                atypes.append((round(float(rng.uniform(50, 90)), 2), round(111.0 + float(rng.uniform(-4, 4)), 2)))
+                codes.append(len(atypes))
+            else:
+                codes.append(0)
+        qangles.append(a + (codes,))
+    if frm:
+        # a new angle n - i - j through the forming bond, at its current value
+        i, j = frm
+        n = min(nbr[i])
+        u, w = x[n - 1] - x[i - 1], x[j - 1] - x[i - 1]
+        th = float(np.degrees(np.arccos(np.dot(u, w) / np.linalg.norm(u) / np.linalg.norm(w))))
+        atypes.append((40.0, round(th, 2)))
+        qangles.append((n, i, j, [0] * (n_states - 1) + [len(atypes)]))
+    ttypes, qtors = [], []
+    for i, j, k, l, _ in sysm.tors[:sysm.ntors_solute]:
+        t = (int(i), int(j), int(k), int(l))
+        if not all(v in qs for v in t):
+            continue
+        pres = term_present(t)
+        codes = []
+        for st in range(n_states):
+            if pres[st]:
+                ttypes.append((round(float(rng.uniform(0.2, 1.5)), 3), float(rng.integers(1, 4)),
+                               float(rng.choice([0.0, 180.0]))))
+                codes.append(len(ttypes))
+            else:
+                codes.append(0)
+        qtors.append(t + (codes,))
+    itypes, qimps = [], []
+    for r in sysm.imps[:sysm.nimps_solute]:
+        t = tuple(int(v) for v in r[:4])
+        if not all(v in qs for v in t):
+            continue
+        codes = []
+        for st in range(n_states):
+            itypes.append((round(float(rng.uniform(2, 6)), 3), round(float(rng.uniform(-10, 10)), 2)))
+            codes.append(len(itypes))
+        qimps.append(t + (codes,))
+    coupl = {"angle": [], "torsion": [], "improper": []}
+    ib = {b: n + 1 for n, b in enumerate(bonds)}
+    for name, terms in (("angle", qangles), ("torsion", qtors), ("improper", qimps)):
+        always = [n for n, t in enumerate(terms) if all(c > 0 for c in t[-1])]
+        for n, t in enumerate(terms):
+            atoms = t[:-1]
+            if brk and has(atoms, brk):
+                coupl[name].append((n + 1, ib[brk], 0))
+            elif frm and has(atoms, frm):
+                coupl[name].append((n + 1, ib[frm], 1))
+        # terms present in every state coupled to the forming bond (the first
+        # state sees the factor the bond left in the previous evaluation)
+        # and to the breaking bond (later states see the first state's)
+        for m, bnd, flag in zip(always[:2], (frm, brk), (0, 1)):
+            if bnd and not any(c[0] == m + 1 for c in coupl[name]):
+                coupl[name].append((m + 1, ib[bnd], flag))
+    sysm.qbond_types, sysm.qbonds = btypes, qbonds
+    sysm.qangle_types, sysm.qangles = atypes, qangles
+    sysm.qtorsion_types, sysm.qtorsions = ttypes, qtors
+    sysm.qimproper_types, sysm.qimpropers = itypes, qimps
+    sysm.qcouplings = coupl
 
 
 # ---------------------------------------------------------------------------
@@ -668,6 +843,26 @@ def write_fep(s: System, path: str) -> None:
         L += ["", "[off_diagonals]"]
         for si, sj, qk, ql, A, mu in s.offdiag:
             L.append("%d %d %d %d %.2f %.3f" % (si, sj, qk, ql, A, mu))
+    if s.qbonds is not None:
+        sec = [("bond_types", s.qbond_types, "change_bonds", s.qbonds),
+               ("angle_types", s.qangle_types, "change_angles", s.qangles),
+               ("torsion_types", s.qtorsion_types, "change_torsions", s.qtorsions),
+               ("improper_types", s.qimproper_types, "change_impropers", s.qimpropers)]
+        for tname, types, cname, terms in sec:
+            if not terms:
+                continue
+            L += ["", "[%s]" % tname]
+            for k, row in enumerate(types):
+                L.append("%d " % (k + 1) + " ".join("%.6g" % v for v in row))
+            L += ["", "[%s]" % cname]
+            for t in terms:
+                L.append(" ".join("%d" % a for a in t[:-1]) + "  " + " ".join("%d" % c for c in t[-1]))
+        for name in ("angle", "torsion", "improper"):
+            rows = (s.qcouplings or {}).get(name) or []
+            if rows:
+                L += ["", "[%s_couplings]" % name]
+                for term, bond, flag in rows:
+                    L.append("%d %d %d" % (term, bond, flag))
     with open(path, "w") as f:
         f.write("\n".join(L) + "\n")
 
@@ -685,7 +880,9 @@ def write_inp(path: str, top: str, final: str, *, steps: int, stepsize: float = 
               radial_force: Optional[float] = 60.0, polarisation: bool = True,
               polarisation_force: Optional[float] = 20.0,
               shell_force: Optional[float] = None, shell_radius: Optional[float] = None,
-              temp_interval: Optional[int] = None) -> None:
+              temp_interval: Optional[int] = None, restraints: Optional[dict] = None) -> None:
+    """qdyn input file; `restraints` maps sequence/atom/distance/angle/wall
+    to lists of rows written into the [<name>_restraints] sections."""
     on = lambda b: "on" if b else "off"
     L = ["[MD]", "steps %d" % steps, "stepsize %.6g" % stepsize,
          "temperature %.6g" % temperature, "bath_coupling %.6g" % bath_coupling,
@@ -723,6 +920,11 @@ def write_inp(path: str, top: str, final: str, *, steps: int, stepsize: float = 
         L.append("fep %s" % fep)
     if lambdas is not None:
         L += ["", "[lambdas]", " ".join("%.6f" % l for l in lambdas)]
+    for name in ("sequence", "atom", "distance", "angle", "wall"):
+        rows = (restraints or {}).get(name)
+        if rows:
+            L += ["", "[%s_restraints]" % name]
+            L += [" ".join(str(v) for v in row) for row in rows]
     with open(path, "w") as f:
         f.write("\n".join(L) + "\n")
 
@@ -786,6 +988,8 @@ def write_system(s: System, directory: str, name: str, **inp) -> dict:
         inp.setdefault("fep", name + ".fep")
     inpp = os.path.join(directory, name + ".inp")
     inp.setdefault("steps", 100)
+    if s.restraints:
+        inp.setdefault("restraints", s.restraints)
     write_inp(inpp, name + ".top", name + ".re", **inp)
     paths["inp"] = inpp
     paths["re"] = os.path.join(directory, name + ".re")

@@ -53,6 +53,20 @@ CASES = {
                              dict(steps=30, lambdas=[0.25, 0.25, 0.25, 0.25], stepsize=0.5, temperature=50.0,
                                   bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
                                   energy_interval=1)),
+    # Q-bonded terms (Morse bonds, one breaking and one forming, Q angles,
+    # torsions, impropers, couplings) and every kind of input restraint
+    "c6_r12_q8s2_qbonded_30": (dict(radius=12.0, n_solute=40, seed=2201, n_q=8, n_states=2, q_types=True,
+                                    softcore=60.0, impropers=1, q_bonded=True, restraints=True),
+                               dict(steps=30, lambdas=[0.6, 0.4], stepsize=0.5, temperature=50.0,
+                                    bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
+                                    energy_interval=1)),
+    # a breaking bond with non-Q neighbours: topology torsions through it
+    # become Q terms (qatom_load_fep's completion); periodic impropers
+    "c6_r12_q6s2_qbreak_30": (dict(radius=12.0, n_solute=40, seed=2202, n_q=6, n_states=2, q_types=True,
+                                   softcore=60.0, impropers=2, q_bonded=True, restraints=True),
+                              dict(steps=30, lambdas=[0.5, 0.5], stepsize=0.5, temperature=50.0,
+                                   bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
+                                   energy_interval=1)),
 }
 
 ROW = re.compile(r"^(solute|solvent|solute-solvent|LRF|Q-atom|restraints|SUM)\s+(.*)$")
@@ -82,13 +96,14 @@ def flatten(rows):
     r = rows["restraints"]
     s = rows["SUM"]
     return [sv[0], sv[1], lrf, r[0], r[1], r[2], r[3], s[0], s[1], s[2],
-            so[0], so[1], so[2], so[3], so[4], so[5], pw[0], pw[1], q[0], q[1], r[4], r[5]]
+            so[0], so[1], so[2], so[3], so[4], so[5], pw[0], pw[1], q[0], q[1], r[4], r[5],
+            q[2], q[3], q[4], q[5]]
 
 
 COLUMNS = ["ww_el", "ww_vdw", "lrf", "rst_total", "rst_fix", "rst_radial", "rst_wpol",
            "total", "potential", "kinetic", "pp_el", "pp_vdw", "p_bond", "p_angle",
            "p_torsion", "p_improper", "pw_el", "pw_vdw", "qx_el", "qx_vdw",
-           "rst_shell", "rst_protein"]
+           "rst_shell", "rst_protein", "q_bond", "q_angle", "q_torsion", "q_improper"]
 
 
 def build_case(name, sysargs, inpargs, workdir):

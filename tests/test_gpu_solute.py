@@ -170,7 +170,7 @@ def test_mixed_frames_fp64_reevaluation(c2_small):
     ep.set_state(0, x, v)
     ep.make_pair_lists()
     ep.pot_energy()
-    q = np.array(ep.qenergies(0)).reshape(-1, 7)
+    q = np.array(ep.qenergies(0)).reshape(-1, 12)
     # put_ene columns 7..12 are qq el/vdw, qp el/vdw, qw el/vdw
     assert np.allclose(f64[0, :, 7:13], q[:, :6], rtol=1e-12, atol=1e-10), (f64[0, :, 7:13], q[:, :6])
     d = np.abs(f32[0, :, 7:13] - f64[0, :, 7:13])

@@ -2,7 +2,8 @@
 tests/golden/make_golden.py from the reference's own serial qdyn5).
 
 Pins: every step's printed energy summary (F12.2) and the final restart
-(x, v, theta_corr) bit-for-bit.  A restart that is bitwise identical after
+(x, v, theta_corr) bit-for-bit; for the energy-file cases every recorded
+frame's per-state Q energies (nonbonded, Q-bonded, restraint) in FP64.  A restart that is bitwise identical after
 100-250 steps of chaotic MD leaves no room for a dropped term, a wrong sign
 or index, or a changed operation order anywhere on the path.
 """
@@ -33,6 +34,20 @@ CASES = {
                              dict(steps=30, lambdas=[0.25, 0.25, 0.25, 0.25], stepsize=0.5, temperature=50.0,
                                   bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
                                   energy_interval=1)),
+    # Q-bonded terms (Morse bonds, one breaking and one forming, Q angles,
+    # torsions, impropers, couplings) and every kind of input restraint
+    "c6_r12_q8s2_qbonded_30": (dict(radius=12.0, n_solute=40, seed=2201, n_q=8, n_states=2, q_types=True,
+                                    softcore=60.0, impropers=1, q_bonded=True, restraints=True),
+                               dict(steps=30, lambdas=[0.6, 0.4], stepsize=0.5, temperature=50.0,
+                                    bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
+                                    energy_interval=1)),
+    # a breaking bond with non-Q neighbours: topology torsions through it
+    # become Q terms (qatom_load_fep's completion); periodic impropers
+    "c6_r12_q6s2_qbreak_30": (dict(radius=12.0, n_solute=40, seed=2202, n_q=6, n_states=2, q_types=True,
+                                   softcore=60.0, impropers=2, q_bonded=True, restraints=True),
+                              dict(steps=30, lambdas=[0.5, 0.5], stepsize=0.5, temperature=50.0,
+                                   bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
+                                   energy_interval=1)),
 }
 
 ORACLE_COLS = {"ww_el": "ww_el", "ww_vdw": "ww_vdw", "lrf": "lrf", "rst_total": "rst_total",
@@ -40,7 +55,9 @@ ORACLE_COLS = {"ww_el": "ww_el", "ww_vdw": "ww_vdw", "lrf": "lrf", "rst_total": 
                "total": "total", "potential": "potential", "kinetic": "kinetic",
                "pp_el": "pp_el", "pp_vdw": "pp_vdw", "pw_el": "pw_el", "pw_vdw": "pw_vdw",
                "p_bond": "p_bond", "p_angle": "p_angle", "p_torsion": "p_torsion",
-               "p_improper": "p_improper", "qx_el": "qx_el", "qx_vdw": "qx_vdw"}
+               "p_improper": "p_improper", "qx_el": "qx_el", "qx_vdw": "qx_vdw",
+               "rst_shell": "rst_shell", "rst_protein": "rst_protein", "q_bond": "q_bond",
+               "q_angle": "q_angle", "q_torsion": "q_torsion", "q_improper": "q_improper"}
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
@@ -83,7 +100,7 @@ def test_oracle_energy_file_frames(name, tmp_path):
     s = gensys.solvated_system(**sysargs)
     paths = gensys.write_system(s, str(tmp_path), name, output=1, temp_interval=1, **inpargs)
     rows, offd = g["eq_frames"], g["offd_frames"]
-    assert rows.shape[0] == inpargs["steps"] - 1 and offd.shape[1] == sysargs["n_offdiag"]
+    assert rows.shape[0] == inpargs["steps"] - 1 and offd.shape[1] == sysargs.get("n_offdiag", 0)
     o = Oracle(paths["inp"])
     o.run(1)
     for k in range(1, inpargs["steps"]):
@@ -91,8 +108,13 @@ def test_oracle_energy_file_frames(name, tmp_path):
         assert np.allclose(od[:, 0], offd[k - 1, :, 2], rtol=1e-13, atol=0), (k, od, offd[k - 1])
         assert np.allclose(od[:, 1], offd[k - 1, :, 3], rtol=1e-13, atol=0)
         o.run(1)
-        q = o.qenergies()           # (nstates, 7): qq el/vdw, qp el/vdw, qw el/vdw, total
-        ref = rows[k - 1]           # put_ene columns 7..12: qq, qp, qw (el, vdw); 0: total
+        # (nstates, 12): qq el/vdw, qp el/vdw, qw el/vdw, total, Q bond angle
+        # torsion improper, restraint; put_ene columns: 0 total, 1-4 Q
+        # bonded, 7-12 qq qp qw (el, vdw), 13 restraint
+        q = o.qenergies()
+        ref = rows[k - 1]
         assert np.allclose(q[:, :6], ref[:, 7:13], rtol=1e-12, atol=1e-10), (k, q[:, :6], ref[:, 7:13])
         assert np.allclose(q[:, 6], ref[:, 0], rtol=1e-12, atol=1e-10)
+        assert np.allclose(q[:, 7:11], ref[:, 1:5], rtol=1e-12, atol=1e-10), (k, q[:, 7:11], ref[:, 1:5])
+        assert np.allclose(q[:, 11], ref[:, 13], rtol=1e-12, atol=1e-10), (k, q[:, 11], ref[:, 13])
     assert np.array_equal(o.theta_corr(), g["theta_corr"])
