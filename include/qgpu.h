@@ -201,6 +201,30 @@ int qgpu_far_lrf_estimate(qgpu_handle* h, int k, float* X, float* Y, int cap, in
  * instead of the LRF multipoles; m = NULL restores LRF. */
 int qgpu_set_far_model(qgpu_handle* h, qgpu_mlp* m);
 
+/* State indices (i, j, 1-based) of every [off_diagonals] function of
+ * instance k, as the energy file's off-diagonal record holds them
+ * (OFFDIAG_SAVE, src/nrgy.f90:73-78); *n = number of functions. */
+int qgpu_offdiag_states(qgpu_handle* h, int k, int* ij, int cap, int* n);
+
+/* ---- qdyn's output files (host only; no device needed) ------------------
+ * Trajectory: replaces trj_initialize + trj_commit_mask + trj_create of
+ * init_trj (src/md.f90:16668-16684, src/trj.f90:72-204) for the run of
+ * `input_path`: the [trajectory_atoms] mask (all atoms when absent) and the
+ * DCD header; trj_path NULL takes [files] trajectory of the input.
+ * qgpu_trj_write replaces write_trj/trj_write (src/md.f90:18695-18701,
+ * src/trj.f90:211-233): one frame of x (3*natom FP64) as REAL(4) records. */
+typedef struct qgpu_trj qgpu_trj;
+int qgpu_trj_create(const char* input_path, const char* trj_path, qgpu_trj** out);
+int qgpu_trj_natoms(qgpu_trj* t);
+int qgpu_trj_write(qgpu_trj* t, const double* x);
+int qgpu_trj_close(qgpu_trj* t);
+
+/* Restart file as write_xfin writes it (src/md.f90:18705-18736): canary
+ * -1337, x, v (3*natom FP64 each) and, when nshell > 0, the water
+ * polarisation shells' theta_corr. */
+int qgpu_write_restart(const char* path, int natom, const double* x, const double* v, int nshell,
+                       const double* theta_corr);
+
 /* Message of the last failure in this thread ("" when none). */
 const char* qgpu_last_error(void);
 

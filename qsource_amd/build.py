@@ -18,7 +18,7 @@ LIB = os.path.join(PKG, "libqgpu.so")
 
 HIP_SOURCES = ["engine.hip", "kernels_water.hip", "kernels_solute.hip", "kernels_far.hip",
                "kernels_mlp.hip", "mlp_host.hip"]
-HOST_SOURCES = ["host/readers.cpp", "host/prep.cpp"]
+HOST_SOURCES = ["host/readers.cpp", "host/prep.cpp", "host/mdfiles.cpp"]
 
 # -packed-fp32-ops off: hand-off of FP32 pairs to v_pk_* is disabled;
 # -ffp-contract=off: FP64 expressions keep the reference's rounding steps.

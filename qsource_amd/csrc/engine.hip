@@ -1361,6 +1361,67 @@ int qgpu_last_step_ms(qgpu_handle* h, double* ms) {
     *ms = h->last_ms;
   });
 }
+int qgpu_offdiag_states(qgpu_handle* h, int k, int* ij, int cap, int* n) {
+  return guarded([&] {
+    CHECK_K(h, k);
+    const auto& od = h->b->setup(k).offd;
+    *n = (int)od.size();
+    for (int m = 0; m < (int)od.size() && 2 * m + 1 < cap; ++m) { ij[2 * m] = od[m].i + 1; ij[2 * m + 1] = od[m].j + 1; }
+  });
+}
+
+}  // extern "C"
+
+struct qgpu_trj {
+  qsys::Setup setup;
+  std::unique_ptr<qsys::DcdWriter> w;
+};
+
+extern "C" {
+
+int qgpu_trj_create(const char* input_path, const char* trj_path, qgpu_trj** out) {
+  return guarded([&] {
+    if (!input_path || !out) throw std::runtime_error("null argument");
+    auto t = std::make_unique<qgpu_trj>();
+    t->setup = qsys::prepare(input_path);
+    std::string path;
+    if (trj_path) {
+      path = trj_path;
+    } else {
+      const std::string ip(input_path), f = t->setup.inp.trj_file;
+      if (f.empty()) throw std::runtime_error("the input names no trajectory file");
+      const size_t sl = ip.find_last_of('/');
+      path = (f[0] == '/' || sl == std::string::npos) ? f : ip.substr(0, sl) + "/" + f;
+    }
+    t->w = std::make_unique<qsys::DcdWriter>(t->setup, path);
+    *out = t.release();
+  });
+}
+
+int qgpu_trj_natoms(qgpu_trj* t) { return t && t->w ? t->w->natoms() : 0; }
+
+int qgpu_trj_write(qgpu_trj* t, const double* x) {
+  return guarded([&] {
+    if (!t || !t->w || !x) throw std::runtime_error("invalid trajectory or null coordinates");
+    t->w->write(x);
+  });
+}
+
+int qgpu_trj_close(qgpu_trj* t) {
+  return guarded([&] { delete t; });
+}
+
+int qgpu_write_restart(const char* path, int natom, const double* x, const double* v, int nshell,
+                       const double* theta_corr) {
+  return guarded([&] {
+    if (!path || !x || !v || natom <= 0 || nshell < 0) throw std::runtime_error("invalid restart arguments");
+    std::vector<double> xv(x, x + 3 * (size_t)natom), vv(v, v + 3 * (size_t)natom);
+    std::vector<double> tc;
+    if (nshell > 0) tc.assign(theta_corr, theta_corr + nshell);
+    qsys::write_restart(path, xv, vv, nshell > 0 ? &tc : nullptr);
+  });
+}
+
 const char* qgpu_last_error(void) { return g_err.c_str(); }
 
 }  // extern "C"

@@ -9,6 +9,7 @@
 #pragma once
 
 #include <cstdint>
+#include <cstdio>
 #include <string>
 #include <array>
 #include <vector>
@@ -107,6 +108,8 @@ struct MDInput {
   std::vector<double> lambdas;
   std::string thermostat = "berendsen", integrator = "leap-frog";
   std::string restraint_file;            // [files] restraint: xtop from a restart file
+  std::string trj_file;                  // [files] trajectory
+  std::vector<std::string> trajectory_atoms;  // [trajectory_atoms] mask rows
   std::vector<RstSeq> rstseq;
   std::vector<RstPos> rstpos;
   std::vector<RstDis> rstdis;
@@ -244,6 +247,25 @@ bool read_restart(const std::string& path, int natom, std::vector<double>& x,
                   std::vector<double>& v, std::vector<double>* theta_corr);
 void write_restart(const std::string& path, const std::vector<double>& x,
                    const std::vector<double>& v, const std::vector<double>* theta_corr);
+
+// Trajectory atom mask of the [trajectory_atoms] rows (mask_add/update,
+// src/mask.f90:129-361): per atom 1 if written; "all" when rows is empty.
+std::vector<char> trajectory_mask(const Setup& s, const std::vector<std::string>& rows);
+
+// DCD trajectory of qdyn's trj module (trj_initialize/trj_create/trj_write,
+// src/trj.f90:72-233): header records, then per frame the masked x, y and z
+// as REAL(4) records.
+class DcdWriter {
+ public:
+  DcdWriter(const Setup& s, const std::string& path);
+  ~DcdWriter();
+  void write(const double* x);
+  int natoms() const { return (int)atoms_.size(); }
+
+ private:
+  FILE* fp_ = nullptr;
+  std::vector<int> atoms_;
+};
 
 // qdyn's integer LCG (randm, src/md.f90:17543-17577) with its saved state.
 struct Randm {

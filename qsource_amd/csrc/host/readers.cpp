@@ -535,6 +535,14 @@ MDInput read_input(const std::string& path) {
     }
   }
   S.get("FILES", "restraint", m.restraint_file);
+  S.get("FILES", "trajectory", m.trj_file);
+  if (m.itrj_cycle > 0 && m.trj_file.empty())
+    throw std::runtime_error("Trajectory file name required to write trajectory!");
+  if (S.has("TRAJECTORY_ATOMS"))
+    for (auto l : S.sec.at("TRAJECTORY_ATOMS")) {
+      size_t a = l.find_first_not_of(" \t"), b = l.find_last_not_of(" \t");
+      if (a != std::string::npos) m.trajectory_atoms.push_back(l.substr(a, b - a + 1));
+    }
   read_restraints(S, m);
   return m;
 }

@@ -36,7 +36,8 @@ SYMBOLS = [
     "qgpu_mlp_create", "qgpu_mlp_destroy", "qgpu_mlp_nparams", "qgpu_mlp_get_params", "qgpu_mlp_set_params",
     "qgpu_mlp_set_data", "qgpu_mlp_train", "qgpu_mlp_gradient", "qgpu_mlp_predict", "qgpu_mlp_last_error",
     "qgpu_far_samples", "qgpu_far_lrf_estimate", "qgpu_set_far_model",
-    "qgpu_profile_kernels", "qgpu_list_counts",
+    "qgpu_profile_kernels", "qgpu_list_counts", "qgpu_offdiag_states",
+    "qgpu_trj_create", "qgpu_trj_natoms", "qgpu_trj_write", "qgpu_trj_close", "qgpu_write_restart",
 ]
 
 KERNEL_NAMES = ["cgp_centers", "water_lists", "solute_lists", "water_forces", "solute_forces",
@@ -96,6 +97,12 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.qgpu_far_samples.argtypes = [P, ctypes.c_int, fp, fp, ctypes.c_int, ip]
     lib.qgpu_far_lrf_estimate.argtypes = [P, ctypes.c_int, fp, fp, ctypes.c_int, ip]
     lib.qgpu_set_far_model.argtypes = [P, P]
+    lib.qgpu_offdiag_states.argtypes = [P, ctypes.c_int, ip, ctypes.c_int, ip]
+    lib.qgpu_trj_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(P)]
+    lib.qgpu_trj_natoms.argtypes = [P]
+    lib.qgpu_trj_write.argtypes = [P, dp]
+    lib.qgpu_trj_close.argtypes = [P]
+    lib.qgpu_write_restart.argtypes = [ctypes.c_char_p, ctypes.c_int, dp, dp, ctypes.c_int, dp]
     lib.qgpu_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
@@ -228,6 +235,14 @@ class Engine:
             return np.zeros((n.value, 0, 2))
         return out[:n.value * m.value * 2].reshape(n.value, m.value, 2)
 
+    def offdiag_states(self, k: int = 0) -> np.ndarray:
+        """(noffd, 2) 1-based state pairs (i, j) of the off-diagonal functions."""
+        ij = np.zeros(2 * 16, dtype=np.int32)
+        n = ctypes.c_int()
+        self._check(self.lib.qgpu_offdiag_states(self.h, k, ij.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                                 len(ij), ctypes.byref(n)))
+        return ij[:2 * n.value].reshape(-1, 2)
+
     def energy_series(self, k: int = 0, fp64: bool = False) -> np.ndarray:
         """(nframes, nstates, 14) put_ene records recorded on the device.
         With fp64=True (mixed-precision batches): the same frames with the Q
@@ -331,3 +346,46 @@ def far_samples(engine: "Engine", k: int = 0, lrf_estimate: bool = False):
 
 def set_far_model(engine: "Engine", model: Optional[FarFieldMLP]):
     engine._check(engine.lib.qgpu_set_far_model(engine.h, model.h if model is not None else None))
+
+
+class Trajectory:
+    """qdyn DCD trajectory of the run of `input_path` (mask of its
+    [trajectory_atoms] section; written to `path`, or to the input's
+    [files] trajectory)."""
+
+    def __init__(self, input_path: str, path: Optional[str] = None):
+        self.lib = load_library()
+        h = ctypes.c_void_p()
+        rc = self.lib.qgpu_trj_create(os.fsencode(input_path), os.fsencode(path) if path else None,
+                                      ctypes.byref(h))
+        if rc != 0:
+            raise QgpuError(self.lib.qgpu_last_error().decode())
+        self.h = h
+        self.natoms = self.lib.qgpu_trj_natoms(self.h)
+
+    def write(self, x: np.ndarray):
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        if self.lib.qgpu_trj_write(self.h, _dp(x)) != 0:
+            raise QgpuError(self.lib.qgpu_last_error().decode())
+
+    def close(self):
+        if self.h:
+            self.lib.qgpu_trj_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def write_restart(path: str, x: np.ndarray, v: np.ndarray, theta_corr: Optional[np.ndarray] = None):
+    """Restart file as qdyn's write_xfin writes it."""
+    lib = load_library()
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    v = np.ascontiguousarray(v, dtype=np.float64).ravel()
+    tc = np.ascontiguousarray(theta_corr if theta_corr is not None else np.zeros(0), dtype=np.float64)
+    if lib.qgpu_write_restart(os.fsencode(path), x.size // 3, _dp(x), _dp(v), tc.size,
+                              _dp(tc) if tc.size else None) != 0:
+        raise QgpuError(lib.qgpu_last_error().decode())

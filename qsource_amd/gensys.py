@@ -880,9 +880,12 @@ def write_inp(path: str, top: str, final: str, *, steps: int, stepsize: float = 
               radial_force: Optional[float] = 60.0, polarisation: bool = True,
               polarisation_force: Optional[float] = 20.0,
               shell_force: Optional[float] = None, shell_radius: Optional[float] = None,
-              temp_interval: Optional[int] = None, restraints: Optional[dict] = None) -> None:
+              temp_interval: Optional[int] = None, restraints: Optional[dict] = None,
+              trajectory: Optional[str] = None, trajectory_interval: int = 0,
+              trajectory_atoms: Optional[Sequence[str]] = None) -> None:
     """qdyn input file; `restraints` maps sequence/atom/distance/angle/wall
-    to lists of rows written into the [<name>_restraints] sections."""
+    to lists of rows written into the [<name>_restraints] sections;
+    `trajectory_atoms` rows form the [trajectory_atoms] mask."""
     on = lambda b: "on" if b else "off"
     L = ["[MD]", "steps %d" % steps, "stepsize %.6g" % stepsize,
          "temperature %.6g" % temperature, "bath_coupling %.6g" % bath_coupling,
@@ -911,7 +914,11 @@ def write_inp(path: str, top: str, final: str, *, steps: int, stepsize: float = 
         L.append("temperature %d" % temp_interval)
     if energy:
         L.append("energy %d" % energy_interval)
+    if trajectory:
+        L.append("trajectory %d" % trajectory_interval)
     L += ["", "[files]", "topology %s" % top, "final %s" % final]
+    if trajectory:
+        L.append("trajectory %s" % trajectory)
     if restart:
         L.append("restart %s" % restart)
     if energy:
@@ -920,6 +927,8 @@ def write_inp(path: str, top: str, final: str, *, steps: int, stepsize: float = 
         L.append("fep %s" % fep)
     if lambdas is not None:
         L += ["", "[lambdas]", " ".join("%.6f" % l for l in lambdas)]
+    if trajectory_atoms:
+        L += ["", "[trajectory_atoms]"] + list(trajectory_atoms)
     for name in ("sequence", "atom", "distance", "angle", "wall"):
         rows = (restraints or {}).get(name)
         if rows:

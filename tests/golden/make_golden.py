@@ -13,7 +13,10 @@ qsource_amd.gensys, and stores, per case:
   generator is detected instead of silently comparing different systems;
 * ``eq_frames``, ``offd_frames`` (cases that write an energy file): the
   reference's energy-file frames (per-state Q energies in FP64 and the
-  off-diagonal elements) of every recorded step.
+  off-diagonal elements) of every recorded step;
+* ``re_bytes``, ``en_bytes``, ``dcd_bytes`` (cases that write a
+  trajectory): the reference's restart, energy and trajectory files as
+  written, for byte-exact checks of this package's writers.
 
 Usage:  python tests/golden/make_golden.py
 """
@@ -59,7 +62,8 @@ CASES = {
                                     softcore=60.0, impropers=1, q_bonded=True, restraints=True),
                                dict(steps=30, lambdas=[0.6, 0.4], stepsize=0.5, temperature=50.0,
                                     bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
-                                    energy_interval=1)),
+                                    energy_interval=1, trajectory="case.dcd", trajectory_interval=10,
+                                    trajectory_atoms=["1 20", "not heavy", "residue 2 5"])),
     # a breaking bond with non-Q neighbours: topology torsions through it
     # become Q terms (qatom_load_fep's completion); periodic impropers
     "c6_r12_q6s2_qbreak_30": (dict(radius=12.0, n_solute=40, seed=2202, n_q=6, n_states=2, q_types=True,
@@ -133,6 +137,10 @@ def main():
                 _, rows, offd = fep.read_energy_frames(os.path.join(d, inpargs["energy"]),
                                                        len(inpargs["lambdas"]))
                 extra = dict(eq_frames=rows, offd_frames=offd)
+            if inpargs.get("trajectory"):
+                raw = lambda f: np.frombuffer(open(os.path.join(d, f), "rb").read(), dtype=np.uint8)
+                extra.update(re_bytes=raw(name + ".re"), en_bytes=raw(inpargs["energy"]),
+                             dcd_bytes=raw(inpargs["trajectory"]))
             np.savez_compressed(os.path.join(OUT, name + ".npz"), energies=en, columns=np.array(COLUMNS),
                                 x=x, v=v, theta_corr=tc if tc is not None else np.zeros(0),
                                 top_sha=np.array(sha), **extra)

@@ -40,7 +40,8 @@ CASES = {
                                     softcore=60.0, impropers=1, q_bonded=True, restraints=True),
                                dict(steps=30, lambdas=[0.6, 0.4], stepsize=0.5, temperature=50.0,
                                     bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
-                                    energy_interval=1)),
+                                    energy_interval=1, trajectory="case.dcd", trajectory_interval=10,
+                                    trajectory_atoms=["1 20", "not heavy", "residue 2 5"])),
     # a breaking bond with non-Q neighbours: topology torsions through it
     # become Q terms (qatom_load_fep's completion); periodic impropers
     "c6_r12_q6s2_qbreak_30": (dict(radius=12.0, n_solute=40, seed=2202, n_q=6, n_states=2, q_types=True,
