@@ -208,7 +208,7 @@ struct Oracle {
           for (int la = 0; la < 3; ++la)
             for (int ka = 0; ka < 3; ++ka)
               nbww.push_back({ns + 3 * iw + la, ns + 3 * jw + ka, ljcod(s.iac[ns + la], s.iac[ns + ka])});
-        } else if (r2 <= rclrf2) {
+        } else if (s.inp.use_lrf && r2 <= rclrf2) {
           lrf_update(ig, jg);
           lrf_update(jg, ig);
         }
@@ -216,16 +216,15 @@ struct Oracle {
     }
   }
 
+  // make_pair_lists (src/md.f90:6030-6058): with LRF off, nbpplist,
+  // nbpwlist and nbwwlist (:7631-7806, :9047-9164, :10698-10791) build the
+  // same lists as their _lrf forms and collect no multipoles
   void make_pair_lists() {
-    if (s.inp.use_lrf) {
-      cgp_centers();
-      nbpplist_lrf();
-      nbpwlist_lrf();
-      nbqplist();
-      nbwwlist_lrf();
-    } else {
-      throw std::runtime_error("runs without LRF are not supported");
-    }
+    if (s.inp.use_lrf) cgp_centers();
+    nbpplist_lrf();
+    nbpwlist_lrf();
+    nbqplist();
+    nbwwlist_lrf();
     nbqwlist();
   }
 
@@ -679,7 +678,7 @@ void Oracle::nbpplist_lrf() {
             nbpp.push_back({i, j, pair14(i, j) ? 3 : lj});
           }
         }
-      } else if (r2 <= rclrf2) {
+      } else if (s.inp.use_lrf && r2 <= rclrf2) {
         lrf_update(ig, jg);
         lrf_update(jg, ig);
       }
@@ -715,7 +714,7 @@ void Oracle::nbpwlist_lrf() {
             nbpw.push_back({i, j, lj});
           }
         }
-      } else if (r2 <= rclrf2) {
+      } else if (s.inp.use_lrf && r2 <= rclrf2) {
         lrf_update(ig, jg);
         lrf_update(jg, ig);
       }
@@ -931,6 +930,12 @@ void Oracle::nonbond_qq() {
           bLJ = bLJ * typ(j).bvdw[iLJ - 1];
           qj = s.crg[j];
         }
+      }
+      if (s.qq_lib_charges) {
+        // nonbond_qq_lib_charges (src/md.f90:13640-13700): library charges
+        // for Q-Q pairs, the state's charge of i against non-Q atoms
+        qi = jq >= 0 ? s.crg[i] : s.qcrg[iq * ns + st];
+        qj = s.crg[j];
       }
       double dx1 = x[j3] - x[i3], dx2 = x[j3 + 1] - x[i3 + 1], dx3 = x[j3 + 2] - x[i3 + 2];
       double r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;

@@ -228,7 +228,6 @@ void Batch::upload(int k) {
   for (int g = 0; g < t.ncgp_solute; ++g)
     if (t.cgp[g].last - t.cgp[g].first + 1 > kMaxGroupAtoms)
       throw std::runtime_error("solute charge groups of more than 8 atoms are not supported");
-  if (!s.inp.use_lrf) throw std::runtime_error("runs without LRF are not supported");
   auto* I = new Inst();
   inst_.push_back(I);
   DevSys& D = h_sys_[k];
@@ -380,6 +379,7 @@ void Batch::upload(int k) {
   D.qavdw = I->upload(s.qavdw);
   D.qbvdw = I->upload(s.qbvdw);
   D.sc_ncol = s.natyps + s.nqat;
+  D.qq_lib = s.qq_lib_charges ? 1 : 0;
   D.sc_lookup = I->upload(s.sc_lookup);
   {
     std::vector<int> kl;

@@ -138,6 +138,7 @@ struct DevSys {
   // LJ code-1]; soft-core sc_lookup [(iq*sc_ncol + column)*nstates + s] with
   // columns natyps topology types then nqat Q atoms (all 0 without softcore)
   int qvdw;
+  int qq_lib;                    // nonbond_qq_lib_charges: library charges for Q-Q pairs
   const int* qiac;
   const double *qavdw, *qbvdw;
   const double* sc_lookup;

@@ -329,6 +329,9 @@ Setup prepare(const std::string& input_path) {
   // get_fep: Q atoms and their per-state charges (src/md.f90:2884-2990)
   s.iqatom.assign(s.natom, 0);
   s.crg = t.crg;
+  std::vector<std::pair<int, int>> s_soft_pairs;
+  std::vector<FepData::ElScale> s_el_scale;
+  std::vector<FepData::ExSpec> s_exspec;
   if (!in.fep_file.empty()) {
     FepData f = read_fep(join(dir, in.fep_file), s.natom);
     s.nstates = f.nstates;
@@ -399,6 +402,23 @@ Setup prepare(const std::string& input_path) {
         }
     }
     q_bonded_setup(s, f);
+    s.qq_lib_charges = f.qq_lib_charges;
+    // special exclusions involving non-Q atoms: all states or none, then
+    // into the topology's exclusion lists (src/md.f90:3039-3066)
+    for (const auto& e : f.exspec) {
+      if (e.i < 0 || e.i >= s.natom || e.j < 0 || e.j >= s.natom) throw std::runtime_error("invalid special exclusion data");
+      if (s.iqatom[e.i] == 0 || s.iqatom[e.j] == 0) {
+        bool any = false, all = true;
+        for (char fl : e.flag) { any = any || fl; all = all && fl; }
+        if (!any) continue;
+        if (!all) throw std::runtime_error("Non-Q-atom special exclusion pairs must be on in all or no states");
+        if (e.i >= s.nat_solute || e.j >= s.nat_solute) throw std::runtime_error("special exclusions must join solute atoms");
+        const int lo = std::min(e.i, e.j), hi = std::max(e.i, e.j);
+        if (hi - lo <= kMaxNbrRange) t.listex[lo] |= 1u << (hi - lo - 1);
+        else t.listexlong.push_back({e.i, e.j});
+      }
+    }
+    s_soft_pairs = f.soft_pairs; s_el_scale = f.el_scale; s_exspec = f.exspec;
   } else if (!in.lambdas.empty()) {
     throw std::runtime_error("[lambdas] given without an FEP file");
   }
@@ -569,6 +589,18 @@ Setup prepare(const std::string& input_path) {
         }
       }
     }
+    // special exclusions of Q atoms: connectivity 0 in the flagged states
+    // (make_qconn, src/md.f90:9598-9616)
+    for (const auto& e : s_exspec) {
+      for (int side = 0; side < 2; ++side) {
+        const int a = side ? e.j : e.i, b = side ? e.i : e.j;
+        const int iq = s.iqatom[a];
+        if (iq == 0) continue;
+        if (b >= ns) throw std::runtime_error("special exclusions of Q atoms must be with solute atoms");
+        for (int st = 0; st < nst; ++st)
+          if (e.flag[st]) qc(st, iq - 1, b) = 0;
+      }
+    }
     s.nbqq.assign(nst, {});
     for (int iq = 0; iq < nq - 1; ++iq)
       for (int jq = iq + 1; jq < nq; ++jq) {
@@ -577,9 +609,17 @@ Setup prepare(const std::string& input_path) {
           int c = qc(st, iq, j);
           if (c > 3) {
             NbqPair p{iq, j, jq, 0, 1.0};
-            // LJ code: 1-4 pairs 3; with Q-atom types 1 (2 only for soft
-            // pairs, not read here); else the topology code
-            p.ljcod = (c == 4) ? 3 : s.qvdw_flag ? 1 : s.ljcod[s.iac[s.iqseq[iq]] * s.natyps + s.iac[j]];
+            // LJ code: 1-4 pairs 3; with Q-atom types 1, or 2 for a soft
+            // pair; else the topology code (nbqqlist, src/md.f90:9734-9778)
+            if (c == 4) p.ljcod = 3;
+            else if (!s.qvdw_flag) p.ljcod = s.ljcod[s.iac[s.iqseq[iq]] * s.natyps + s.iac[j]];
+            else {
+              p.ljcod = 1;
+              for (const auto& sp : s_soft_pairs)
+                if ((iq == sp.first && jq == sp.second) || (jq == sp.first && iq == sp.second)) { p.ljcod = 2; break; }
+            }
+            for (const auto& es : s_el_scale)
+              if ((iq == es.iq && jq == es.jq) || (iq == es.jq && jq == es.iq)) { p.el_scale = es.scale[st]; break; }
             s.nbqq[st].push_back(p);
           }
         }

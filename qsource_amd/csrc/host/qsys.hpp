@@ -158,6 +158,15 @@ struct FepData {
   std::vector<QImpLib> qimplib;
   std::vector<QTerm> qbnd, qang, qtor, qimp;
   std::vector<QCoupling> ang_coupl, tor_coupl, imp_coupl;
+  // [soft_pairs] Q-Q pairs with exponential repulsion (0-based Q numbers);
+  // [el_scale] Q-Q electrostatic scale per state; [excluded_pairs] atom
+  // pairs (0-based) excluded in the states flagged; qq_use_library_charges
+  std::vector<std::pair<int, int>> soft_pairs;
+  struct ElScale { int iq, jq; std::vector<double> scale; };
+  std::vector<ElScale> el_scale;
+  struct ExSpec { int i, j; std::vector<char> flag; };
+  std::vector<ExSpec> exspec;
+  bool qq_lib_charges = false;
 };
 
 struct ShakeBond { int i, j; double dist2; };
@@ -219,6 +228,7 @@ struct Setup {
   std::vector<double> qavdw, qbvdw;
   std::vector<double> sc_lookup;
   std::vector<OffDiag> offd;
+  bool qq_lib_charges = false;         // nonbond_qq_lib_charges
   int ff_type = 1;
   // Q-bonded terms after the break-bond completion of qatom_load_fep, with
   // their libraries; coupled bond per angle/torsion/improper (-1 = none)

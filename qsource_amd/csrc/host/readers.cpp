@@ -672,8 +672,8 @@ FepData read_fep(const std::string& path, int natom) {
   }
   int nq = (int)f.iqseq.size();
   f.qcrg.assign(nq, std::vector<double>(f.nstates, 0.0));
-  for (const char* unsup : {"SOFT_PAIRS", "EXCLUDED_PAIRS", "EL_SCALE", "SHAKE_CONSTRAINTS",
-                            "MONITOR_GROUPS", "MONITOR_GROUP_PAIRS"}) {
+  if (S.get("FEP", "qq_use_library_charges", v)) parse_onoff(v, f.qq_lib_charges);
+  for (const char* unsup : {"SHAKE_CONSTRAINTS", "MONITOR_GROUPS", "MONITOR_GROUP_PAIRS"}) {
     if (S.has(unsup) && !S.sec.at(unsup).empty())
       throw std::runtime_error(std::string("FEP section [") + unsup + "] is not supported");
   }
@@ -742,6 +742,42 @@ FepData read_fep(const std::string& path, int natom) {
     if (n != nq) throw std::runtime_error("Alpha must be given for every Q-atom");
   }
   read_fep_bonded(S, f, natom);
+  // [soft_pairs] "iq jq" (src/qatom.f90:832-858)
+  auto rows = [&](const char* name) {
+    std::vector<std::vector<std::string>> out;
+    if (!S.has(name)) return out;
+    for (auto& l : S.sec.at(name)) {
+      std::vector<std::string> tk;
+      Records::tokens(l, tk);
+      if (!tk.empty()) out.push_back(tk);
+    }
+    return out;
+  };
+  for (auto& tk : rows("SOFT_PAIRS")) {
+    int a = (int)to_i(tk.at(0)), b = tk.size() > 1 ? (int)to_i(tk[1]) : 0;
+    if (a < 1 || a > nq || b < 1 || b > nq) throw std::runtime_error("Invalid q-atom number in [soft_pairs]");
+    f.soft_pairs.push_back({a - 1, b - 1});
+  }
+  // [el_scale] "iq jq scale(1..nstates)" (src/qatom.f90:862-888)
+  for (auto& tk : rows("EL_SCALE")) {
+    if ((int)tk.size() < 2 + f.nstates) throw std::runtime_error("bad [el_scale] line");
+    int a = (int)to_i(tk[0]), b = (int)to_i(tk[1]);
+    if (a < 1 || a > nq || b < 1 || b > nq) throw std::runtime_error("Invalid q-atom number in [el_scale]");
+    FepData::ElScale e{a - 1, b - 1, {}};
+    for (int st = 0; st < f.nstates; ++st) e.scale.push_back(to_d(tk[2 + st]));
+    f.el_scale.push_back(e);
+  }
+  // [excluded_pairs] "i j flag(1..nstates)", flags 0/1 (src/qatom.f90:891-918)
+  for (auto& tk : rows("EXCLUDED_PAIRS")) {
+    if ((int)tk.size() < 2 + f.nstates) throw std::runtime_error("bad [excluded_pairs] line");
+    FepData::ExSpec e{(int)to_i(tk[0]) - 1, (int)to_i(tk[1]) - 1, {}};
+    for (int st = 0; st < f.nstates; ++st) {
+      long fl = to_i(tk[2 + st]);
+      if (fl != 0 && fl != 1) throw std::runtime_error("Special exclusion state flags are invalid.");
+      e.flag.push_back((char)fl);
+    }
+    f.exspec.push_back(e);
+  }
   // EVB off-diagonal functions
   if (S.has("OFF_DIAGONALS")) {
     for (auto& l : S.sec.at("OFF_DIAGONALS")) {

@@ -974,6 +974,7 @@ __global__ __launch_bounds__(256) void k_terms(const DevSys* __restrict__ sys) {
     const int i = S.iqseq[iq];
     const int l = lj - 1;
     const double qi = S.qcrg[iq * S.nstates + st];
+    double qi_used = qi;
     double aLJ, bLJ, qj;
     if (!S.qvdw) {
       aLJ = S.avdw[S.iac[i] * 3 + l] * S.avdw[S.iac[j] * 3 + l];
@@ -999,6 +1000,11 @@ __global__ __launch_bounds__(256) void k_terms(const DevSys* __restrict__ sys) {
         qj = S.crg[j];
       }
     }
+    if (S.qq_lib) {
+      // nonbond_qq_lib_charges (src/md.f90:13640-13700)
+      qi_used = jq >= 0 ? S.crg[i] : qi;
+      qj = S.crg[j];
+    }
     const double dx1 = x[3 * j] - x[3 * i], dx2 = x[3 * j + 1] - x[3 * i + 1], dx3 = x[3 * j + 2] - x[3 * i + 2];
     double r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
     const double r6_hc = r2 * r2 * r2;
@@ -1009,7 +1015,7 @@ __global__ __launch_bounds__(256) void k_terms(const DevSys* __restrict__ sys) {
     const double r12 = r6 * r6;
     r2 = 1.0 / r2;
     const double r = sqrt(r2);
-    double Vel = qi * qj * r * S.qq_elscale[p];
+    double Vel = qi_used * qj * r * S.qq_elscale[p];
     if (lj == 3) Vel = Vel * S.el14_scale;
     double Va, Vb, dv;
     if (S.qvdw && jq >= 0 && lj == 2) {

@@ -112,6 +112,12 @@ class System:
     qcouplings: Optional[dict] = None
     # input-file restraints ([sequence|atom|distance|angle|wall]_restraints rows)
     restraints: Optional[dict] = None
+    # FEP Q-Q options: [soft_pairs] (iq, jq), [el_scale] (iq, jq, scales),
+    # [excluded_pairs] (i, j, flags) rows, 1-based; qq_use_library_charges
+    soft_pairs: Optional[list] = None
+    el_scale: Optional[list] = None
+    excluded_pairs: Optional[list] = None
+    qq_lib_charges: bool = False
 
     @property
     def natom(self) -> int:
@@ -348,7 +354,8 @@ def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
                     n_states: int = 0, n_types: int = 250,
                     q_sd: float = 0.3, q_types: bool = False,
                     softcore: Optional[float] = None, n_offdiag: int = 0,
-                    impropers: int = 0, q_bonded: bool = False, restraints: bool = False) -> System:
+                    impropers: int = 0, q_bonded: bool = False, restraints: bool = False,
+                    fep_pairs: bool = False, qq_lib_charges: bool = False) -> System:
     """A TIP3P sphere of the given radius around an optional branched solute
     (charges N(0,q_sd) neutralised per 4-atom charge group, sigma~U[2.5,3.8],
     eps~U[0.05,0.2] quantised into at most n_types atom types), with an
@@ -552,7 +559,74 @@ def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
         _q_bonded_terms(sysm, rng, n_states)
     if restraints and n_solute >= 40:
         sysm.restraints = _example_restraints(sysm, max(n_states, 0))
+    if fep_pairs and sysm.qatoms is not None and n_states >= 2:
+        _fep_pairs(sysm, rng, n_states)
+    sysm.qq_lib_charges = bool(qq_lib_charges and sysm.qatoms is not None)
     return sysm
+
+
+def _fep_pairs(sysm: System, rng, n_states: int) -> None:
+    """[soft_pairs], [el_scale] and [excluded_pairs] over the Q region: the
+    two Q-Q pairs furthest apart in the bond graph repel softly (with Q-atom
+    types, whose LJ-code-2 parameters become C, alpha of C_i C_j
+    exp(-alpha_i alpha_j r)); two pairs get per-state electrostatic scales;
+    one Q-Q pair is excluded in the last state only, and a Q/non-Q and a
+    non-Q/non-Q solute pair (the latter more than 25 atoms apart) in every
+    state."""
+    ns = sysm.nat_solute
+    nbr = [set() for _ in range(ns + 1)]
+    for i, j, _ in sysm.bonds[:sysm.nbonds_solute]:
+        nbr[int(i)].add(int(j))
+        nbr[int(j)].add(int(i))
+
+    def dist(a):
+        d = {a: 0}
+        front = [a]
+        while front:
+            nxt = []
+            for u in front:
+                for w in nbr[u]:
+                    if w not in d:
+                        d[w] = d[u] + 1
+                        nxt.append(w)
+            front = nxt
+        return d
+
+    qa = [int(a) for a in sysm.qatoms]
+    pairs = []
+    for m, a in enumerate(qa):
+        da = dist(a)
+        for n in range(m + 1, len(qa)):
+            pairs.append((da.get(qa[n], 99), m + 1, n + 1))
+    pairs.sort(key=lambda t: (-t[0], t[1], t[2]))
+    far = [(m, n) for d, m, n in pairs if d >= 4]
+    if sysm.qtypes is not None:
+        rows = []
+        for name, a1, b1, a2, b2, a3, b3, mass in sysm.qtypes:
+            rows.append((name, a1, b1, round(float(np.sqrt(rng.uniform(500, 2000))), 4),
+                         round(float(np.sqrt(rng.uniform(2.0, 3.0))), 4), a3, b3, mass))
+        sysm.qtypes = rows
+        sysm.soft_pairs = far[:2]
+    sysm.el_scale = [(m, n, [round(float(rng.uniform(0.5, 1.0)), 2) for _ in range(n_states)])
+                     for m, n in (far[2:4] if len(far) >= 4 else far[:2])]
+    ex = []
+    if len(far) > 4:
+        m, n = far[4]
+        ex.append((qa[m - 1], qa[n - 1], [0] * (n_states - 1) + [1]))
+    qs = set(qa)
+    d0 = dist(qa[0])
+    nq = [a for a in range(1, ns + 1) if a not in qs and d0.get(a, 99) >= 4 and abs(a - qa[0]) <= 25]
+    if nq:
+        ex.append((qa[0], nq[0], [1] * n_states))
+    for a in range(1, ns + 1):
+        if a in qs:
+            continue
+        da = dist(a)
+        b = [c for c in range(a + 26, ns + 1) if c not in qs and da.get(c, 99) >= 4]
+        if b:
+            ex.append((a, b[0], [1] * n_states))
+            break
+    sysm.excluded_pairs = ex
 
 
 def _example_restraints(sysm: System, n_states: int) -> dict:
@@ -822,6 +896,8 @@ def write_fep(s: System, path: str) -> None:
     L = ["[FEP]", "states %d" % ns]
     if s.softcore is not None:
         L.append("softcore_use_max_potential off")
+    if s.qq_lib_charges:
+        L.append("qq_use_library_charges on")
     L += ["", "[atoms]"]
     for k, a in enumerate(s.qatoms):
         L.append("%d %d" % (k + 1, a))
@@ -843,6 +919,13 @@ def write_fep(s: System, path: str) -> None:
         L += ["", "[off_diagonals]"]
         for si, sj, qk, ql, A, mu in s.offdiag:
             L.append("%d %d %d %d %.2f %.3f" % (si, sj, qk, ql, A, mu))
+    if s.soft_pairs:
+        L += ["", "[soft_pairs]"] + ["%d %d" % p for p in s.soft_pairs]
+    if s.el_scale:
+        L += ["", "[el_scale]"] + ["%d %d " % (m, n) + " ".join("%.2f" % v for v in sc) for m, n, sc in s.el_scale]
+    if s.excluded_pairs:
+        L += ["", "[excluded_pairs]"] + ["%d %d " % (i, j) + " ".join("%d" % f for f in fl)
+                                         for i, j, fl in s.excluded_pairs]
     if s.qbonds is not None:
         sec = [("bond_types", s.qbond_types, "change_bonds", s.qbonds),
                ("angle_types", s.qangle_types, "change_angles", s.qangles),

@@ -1,0 +1,105 @@
+"""GPU parity on input variants of the step against the FP64 oracle (pinned
+bitwise to reference qdyn on the same systems by test_oracle_reference.py):
+
+* LRF off (plain cut-off lists, no multipoles or Taylor term,
+  src/md.f90:6030-6058): step 0 energies and gradient within 1e-12, 30 FP64
+  steps within 1e-9; mixed precision gradient RMS error below 1e-5 of the
+  gradient RMS;
+* the FEP file's Q-Q pair options: [soft_pairs] (exponential repulsion with
+  Q-atom types), [el_scale], [excluded_pairs] (Q-Q in one state, Q/non-Q
+  and non-Q/non-Q in all) and qq_use_library_charges
+  (nonbond_qq_lib_charges, src/md.f90:13640-13786): step 0 within 1e-12,
+  30 FP64 steps within 1e-9.
+"""
+import numpy as np
+import pytest
+
+from oracle import Oracle
+from qsource_amd import engine as qe
+from qsource_amd import gensys
+
+pytestmark = pytest.mark.gpu
+
+COOL = dict(stepsize=0.5, temperature=50.0, bath_coupling=2.0, initial_temperature=50.0)
+TERMS = ["pp_el", "pp_vdw", "pw_el", "pw_vdw", "ww_el", "ww_vdw", "lrf", "qx_el", "qx_vdw",
+         "rst_radial", "rst_wpol", "potential"]
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _compare(paths, precision=qe.QGPU_FP64):
+    o = Oracle(paths["inp"])
+    o.final_energy()
+    e = qe.Engine([paths["inp"]], precision)
+    e.make_pair_lists()
+    e.pot_energy()
+    eo, eg = o.energies(), e.energies(0)
+    do, dg = o.gradient(), e.gradient(0)
+    e.close()
+    return eo, eg, do, dg
+
+
+def test_no_lrf(tmp_path):
+    _say("case: LRF off, R=14 + 40-atom solute + 8 Q, FP64 step 0, 30 steps, mixed step 0")
+    s = gensys.solvated_system(radius=14.0, n_solute=40, seed=2003, n_q=8, n_states=2)
+    paths = gensys.write_system(s, str(tmp_path), "nolrf", steps=30, lambdas=[0.5, 0.5], lrf=False, **COOL)
+    eo, eg, do, dg = _compare(paths)
+    assert eo["lrf"] == 0.0 and eg["lrf"] == 0.0
+    for t in TERMS:
+        assert abs(eg[t] - eo[t]) <= 1e-12 * max(1.0, abs(eo[t])), (t, eg[t], eo[t])
+    assert np.abs(dg - do).max() <= 1e-12 * np.abs(do).max()
+    o = Oracle(paths["inp"])
+    o.run(30)
+    e = qe.Engine([paths["inp"]], qe.QGPU_FP64)
+    e.step(30)
+    xo, vo = o.state()
+    xg, vg = e.state(0)
+    assert np.abs(xg - xo).max() <= 1e-9 * np.abs(xo).max()
+    assert np.abs(vg - vo).max() <= 1e-9 * np.abs(vo).max()
+    e.close()
+    eo, eg, do, dg = _compare(paths, qe.QGPU_MIXED)
+    rms = np.sqrt(np.mean(do ** 2))
+    assert np.sqrt(np.mean((dg - do) ** 2)) <= 1e-5 * rms
+
+
+FEP_PAIRS = {
+    "fepx": dict(radius=12.0, n_solute=40, seed=2301, n_q=8, n_states=2, impropers=1, fep_pairs=True,
+                 q_types=True, softcore=40.0),
+    "qqlib": dict(radius=12.0, n_solute=40, seed=2301, n_q=8, n_states=2, impropers=1, fep_pairs=True,
+                  qq_lib_charges=True),
+}
+
+
+@pytest.mark.parametrize("name", sorted(FEP_PAIRS))
+def test_fep_pair_options(tmp_path, name):
+    """[soft_pairs], [el_scale], [excluded_pairs] and qq_use_library_charges:
+    step 0 energies, per-state Q energies and gradient within 1e-12; 30 FP64
+    steps within 1e-9."""
+    _say("case: FEP pair options (%s), R=12 + 40-atom solute + 8 Q, FP64" % name)
+    s = gensys.solvated_system(**FEP_PAIRS[name])
+    paths = gensys.write_system(s, str(tmp_path), name, steps=30, lambdas=[0.5, 0.5], **COOL)
+    o = Oracle(paths["inp"])
+    o.final_energy()
+    e = qe.Engine([paths["inp"]], qe.QGPU_FP64)
+    e.make_pair_lists()
+    e.pot_energy()
+    eo, eg = o.energies(), e.energies(0)
+    for t in TERMS:
+        assert abs(eg[t] - eo[t]) <= 1e-12 * max(1.0, abs(eo[t])), (t, eg[t], eo[t])
+    qo = o.qenergies()
+    qg = np.array(e.qenergies(0)).reshape(qo.shape)
+    assert np.allclose(qg, qo, rtol=1e-12, atol=1e-11), (qg, qo)
+    do, dg = o.gradient(), e.gradient(0)
+    assert np.abs(dg - do).max() <= 1e-12 * np.abs(do).max()
+    e.close()
+    o = Oracle(paths["inp"])
+    o.run(30)
+    e = qe.Engine([paths["inp"]], qe.QGPU_FP64)
+    e.step(30)
+    xo, vo = o.state()
+    xg, vg = e.state(0)
+    assert np.abs(xg - xo).max() <= 1e-9 * np.abs(xo).max()
+    assert np.abs(vg - vo).max() <= 1e-9 * np.abs(vo).max()
+    e.close()

@@ -24,6 +24,10 @@ CASES = {
     "c2_r14_q8_30": (dict(radius=14.0, n_solute=40, seed=2001, n_q=8, n_states=2),
                      dict(steps=30, lambdas=[0.7, 0.3], stepsize=0.5,
                           temperature=50.0, bath_coupling=2.0, initial_temperature=50.0)),
+    # plain cut-off: pair lists without LRF multipoles, no Taylor term
+    "c2_r14_q8_nolrf_30": (dict(radius=14.0, n_solute=40, seed=2003, n_q=8, n_states=2),
+                           dict(steps=30, lambdas=[0.5, 0.5], stepsize=0.5, lrf=False,
+                                temperature=50.0, bath_coupling=2.0, initial_temperature=50.0)),
     "c5_r12_q8s4_gromos_30": (dict(radius=12.0, n_solute=40, seed=2101, n_q=8, n_states=4, q_types=True,
                                    softcore=20.0, n_offdiag=3, impropers=1),
                               dict(steps=30, lambdas=[0.4, 0.3, 0.2, 0.1], stepsize=0.5, temperature=50.0,
@@ -42,6 +46,19 @@ CASES = {
                                     bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
                                     energy_interval=1, trajectory="case.dcd", trajectory_interval=10,
                                     trajectory_atoms=["1 20", "not heavy", "residue 2 5"])),
+    # Q-Q pair options: soft repulsion pairs (Q-atom types), per-state
+    # electrostatic scales, special exclusions (Q-Q in one state; Q/non-Q and
+    # non-Q/non-Q in all); and the same with library charges for Q-Q pairs
+    "c7_r12_q8s2_fepx_30": (dict(radius=12.0, n_solute=40, seed=2301, n_q=8, n_states=2, impropers=1,
+                                 fep_pairs=True, q_types=True, softcore=40.0),
+                            dict(steps=30, lambdas=[0.5, 0.5], stepsize=0.5, temperature=50.0,
+                                 bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
+                                 energy_interval=1)),
+    "c7_r12_q8s2_qqlib_30": (dict(radius=12.0, n_solute=40, seed=2301, n_q=8, n_states=2, impropers=1,
+                                  fep_pairs=True, qq_lib_charges=True),
+                             dict(steps=30, lambdas=[0.5, 0.5], stepsize=0.5, temperature=50.0,
+                                  bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
+                                  energy_interval=1)),
     # a breaking bond with non-Q neighbours: topology torsions through it
     # become Q terms (qatom_load_fep's completion); periodic impropers
     "c6_r12_q6s2_qbreak_30": (dict(radius=12.0, n_solute=40, seed=2202, n_q=6, n_states=2, q_types=True,
