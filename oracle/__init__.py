@@ -115,6 +115,13 @@ class Oracle:
         self.lib.oracle_get_list(self.h, ctypes.c_int(which), out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
         return out[:3 * n].reshape(-1, 3)
 
+    def lrf(self, ncgp: int) -> np.ndarray:
+        """(ncgp, 43) LRF records of the last list build: centre, phi0,
+        phi1(3), phi2(9), phi3(27)."""
+        out = np.zeros(43 * ncgp)
+        self.lib.oracle_get_lrf(self.h, _dp(out))
+        return out.reshape(ncgp, 43)
+
     def temperature(self) -> float:
         return self.lib.oracle_temperature(self.h)
 

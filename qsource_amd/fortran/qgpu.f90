@@ -72,6 +72,72 @@ module qgpu
       real(c_double), intent(out) :: e(*)
     end function
 
+    integer(c_int) function qgpu_natom(handle, k, natom) bind(C, name='qgpu_natom')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle
+      integer(c_int), value :: k
+      integer(c_int), intent(out) :: natom
+    end function
+
+    integer(c_int) function qgpu_temperature(handle, k, temp) bind(C, name='qgpu_temperature')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: handle
+      integer(c_int), value :: k
+      real(c_double), intent(out) :: temp
+    end function
+
+    integer(c_int) function qgpu_theta_corr(handle, k, tc, n) bind(C, name='qgpu_theta_corr')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: handle
+      integer(c_int), value :: k
+      real(c_double), intent(out) :: tc(*)
+      integer(c_int), intent(out) :: n
+    end function
+
+    ! energy file frames kept on the device (put_ene every iene steps)
+    integer(c_int) function qgpu_set_energy_interval(handle, iene, max_frames) &
+        bind(C, name='qgpu_set_energy_interval')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle
+      integer(c_int), value :: iene, max_frames
+    end function
+
+    integer(c_int) function qgpu_energy_series(handle, k, out, cap, nframes) &
+        bind(C, name='qgpu_energy_series')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: handle
+      integer(c_int), value :: k, cap
+      real(c_double), intent(out) :: out(*)
+      integer(c_int), intent(out) :: nframes
+    end function
+
+    ! restart (write_xfin) and DCD trajectory (trj_create/trj_write) writers
+    integer(c_int) function qgpu_write_restart(path, natom, x, v, nshell, theta_corr) &
+        bind(C, name='qgpu_write_restart')
+      import :: c_int, c_char, c_double
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: natom, nshell
+      real(c_double), intent(in) :: x(*), v(*), theta_corr(*)
+    end function
+
+    integer(c_int) function qgpu_trj_create(input_path, trj_path, trj) bind(C, name='qgpu_trj_create')
+      import :: c_int, c_char, c_ptr
+      character(kind=c_char), intent(in) :: input_path(*)
+      type(c_ptr), value :: trj_path
+      type(c_ptr), intent(out) :: trj
+    end function
+
+    integer(c_int) function qgpu_trj_write(trj, x) bind(C, name='qgpu_trj_write')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: trj
+      real(c_double), intent(in) :: x(*)
+    end function
+
+    integer(c_int) function qgpu_trj_close(trj) bind(C, name='qgpu_trj_close')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: trj
+    end function
+
     type(c_ptr) function qgpu_last_error() bind(C, name='qgpu_last_error')
       import :: c_ptr
     end function

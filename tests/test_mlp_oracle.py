@@ -1,10 +1,14 @@
 """The CPU restatement of the far-field surrogate network (oracle/mlp.py) is
 the checker of the GPU training kernels; its analytic gradient is pinned
-here by central finite differences in FP64, and its BF16 rounding by exact
-BF16 values."""
+here by central finite differences in FP64, its BF16 rounding by exact
+BF16 values, and its exact far-field targets (far_field_exact) by the
+oracle's LRF multipoles phi0/phi1 of a fresh list build (which
+test_oracle_reference.py pins bitwise to reference qdyn through the
+trajectories they produce)."""
 import numpy as np
 
-from oracle import mlp
+from oracle import Oracle, mlp
+from qsource_amd import gensys
 
 
 def _setup(seed=5, n=37):
@@ -40,3 +44,26 @@ def test_bf16_rounding_is_exact_on_representable_values_and_ties_to_even():
     assert mlp.bf16(np.float32(1.0 + 2.0 ** -8)) == 1.0
     # 1 + 3 * 2^-8 lies halfway between 1 + 2^-7 and 1 + 2^-6: ties to even (1 + 2^-6)
     assert mlp.bf16(np.float32(1.0 + 3 * 2.0 ** -8)) == 1.0 + 2.0 ** -6
+
+
+def test_far_field_exact_equals_lrf_multipoles(tmp_path):
+    """phi0 and phi1 of lrf_update summed over every far group are the
+    potential and field at the group centre: far_field_exact must give the
+    same numbers from its own brute-force sum."""
+    s = gensys.solvated_system(radius=12.0, n_solute=40, seed=2001, n_q=8, n_states=2)
+    paths = gensys.write_system(s, str(tmp_path), "ff", steps=10, lambdas=[0.5, 0.5])
+    o = Oracle(paths["inp"])
+    o.final_energy()                      # list build at the starting coordinates
+    x, _ = o.state()
+    groups = [[a - 1 for a in g] for g in s.cgps]
+    switch = [g[0] for g in groups]
+    qat = np.zeros(s.natom, dtype=bool)
+    qat[np.asarray(s.qatoms) - 1] = True
+    crg = np.round(s.crg, 4) * np.sqrt(332.0)
+    cent, phi, E = mlp.far_field_exact(x, crg, groups, switch, np.zeros(s.natom, bool), qat,
+                                       10.0, 10.0, 10.0, s.nat_solute)
+    L = o.lrf(len(groups))
+    assert np.allclose(cent, L[:, 0:3], rtol=0, atol=1e-12)
+    assert np.abs(phi).max() > 0.1
+    assert np.allclose(phi, L[:, 3], rtol=1e-10, atol=1e-10 * np.abs(phi).max())
+    assert np.allclose(E, L[:, 4:7], rtol=1e-10, atol=1e-10 * np.abs(E).max())
