@@ -201,6 +201,23 @@ int qgpu_far_lrf_estimate(qgpu_handle* h, int k, float* X, float* Y, int cap, in
  * instead of the LRF multipoles; m = NULL restores LRF. */
 int qgpu_set_far_model(qgpu_handle* h, qgpu_mlp* m);
 
+/* Launch-shape tuner (no counterpart in the reference).  The four large
+ * kernels (water lists, solute lists, water forces, solute forces) have
+ * launch shapes: workgroup mapping (0 2-D grid, 1 instance-major work map,
+ * 2 instances kept on one XCD each) x workgroup size (256, 128); for the
+ * two force kernels partner prefetch (0 off, 1 on); for the FP32 water
+ * forces the register budget (0: 6 wavefronts per SIMD, 1: 4).  Every shape
+ * gives bitwise the same results.  Each shape is timed over `reps` launches
+ * on this batch and the fastest is kept; qgpu_step does this by itself at
+ * its first call unless QGPU_AUTOTUNE=0.  Only at a pair-list step (the
+ * tuning launches rebuild the lists).  ms (8 values): per kernel the chosen
+ * and the default shape's ms per launch; shapes (16): per kernel mapping,
+ * workgroup size, prefetch and register budget. */
+int qgpu_autotune(qgpu_handle* h, int reps, double* ms, int* shapes);
+/* Fixes the launch shape of tuned kernel `kernel` (0-3 as above) and
+ * switches the tuning at the first step off. */
+int qgpu_set_launch_shape(qgpu_handle* h, int kernel, int map, int block, int prefetch, int occupancy);
+
 /* State indices (i, j, 1-based) of every [off_diagonals] function of
  * instance k, as the energy file's off-diagonal record holds them
  * (OFFDIAG_SAVE, src/nrgy.f90:73-78); *n = number of functions. */

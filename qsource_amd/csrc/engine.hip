@@ -22,15 +22,15 @@ namespace qgpu {
 // kernels (kernels_water.hip, kernels_solute.hip)
 __global__ void k_cgp_centers(const DevSys*);
 __global__ void k_x_to_f32(const DevSys*);
-template <typename P> __global__ void k_water_lists(const DevSys*);
-template <typename P> __global__ void k_water_forces(const DevSys*);
+template <typename P, int MAPPED> __global__ void k_water_lists(const DevSys*, const int2*);
+template <typename P, int MAPPED, int PF, int W> __global__ void k_water_forces(const DevSys*, const int2*);
 __global__ void k_watpol_theta(const DevSys*);
 __global__ void k_watpol_force(const DevSys*);
 __global__ void k_integrate(const DevSys*);
 __global__ void k_temperature(const DevSys*, int);
 __global__ void k_energy_reduce(const DevSys*, int);
-template <typename P> __global__ void k_solute_lists(const DevSys*);
-template <typename P> __global__ void k_solute_forces(const DevSys*);
+template <typename P, int MAPPED> __global__ void k_solute_lists(const DevSys*, const int2*);
+template <typename P, int MAPPED, int PF> __global__ void k_solute_forces(const DevSys*, const int2*);
 template <typename P, int NS> __global__ void k_q_forces(const DevSys*, int);
 __global__ void k_q_reduce(const DevSys*, int);
 __global__ void k_terms(const DevSys*);
@@ -128,6 +128,8 @@ Batch::Batch(const std::vector<std::string>& inputs, int precision, int device) 
       throw std::runtime_error("all instances of a batch need the same non_bond interval");
   const char* ng = std::getenv("QGPU_NO_GRAPH");
   use_graph_ = !(ng && ng[0] == '1');
+  const char* at = std::getenv("QGPU_AUTOTUNE");
+  autotune_on_ = !(at && at[0] == '0');
   HIPCHECK(hipMalloc(&d_step_, sizeof(int)));
   HIPCHECK(hipMemset(d_step_, 0, sizeof(int)));
   h_sys_.resize(setups_.size());
@@ -204,6 +206,8 @@ int Batch::offdiag_series(int k, double* out, int cap) {
 
 Batch::~Batch() {
   if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  for (auto& m : maps_)
+    if (m.second.first) (void)hipFree(m.second.first);
   if (far_x_) (void)hipFree(far_x_);
   if (far_y_) (void)hipFree(far_y_);
   if (far_frame_) (void)hipFree(far_frame_);
@@ -708,42 +712,207 @@ void Batch::upload(int k) {
   D.ke_blocks = blocks + 4096;
 }
 
+// Items and lanes per item of the tuned kernels, per instance, and the
+// relative work of an instance (for balancing instances over the XCDs).
+static void tuned_items(int kernel, const DevSys& D, int& items, int& lanes, double& work) {
+  switch (kernel) {
+    case Batch::TK_WATER_LISTS:
+      items = D.nwat; lanes = 8; work = (double)D.nwat * (D.nwat + D.ncgp_solute); break;
+    case Batch::TK_SOLUTE_LISTS:
+      items = D.ncgp_solute; lanes = 64; work = (double)D.ncgp_solute * (D.ncgp_solute + D.nwat); break;
+    case Batch::TK_WATER_FORCES:
+      items = D.nwat; lanes = 8; work = (double)D.nwat; break;
+    default:
+      items = D.nat_solute; lanes = 8; work = (double)D.nat_solute; break;
+  }
+  if (items == 0 || (kernel == Batch::TK_SOLUTE_LISTS && D.ncgp_solute == 0)) work = 0;
+}
+
+// Work map of a tuned kernel: one (instance, first item) entry per
+// workgroup.  Map 1 lists the workgroups instance by instance.  Map 2
+// assigns whole instances to the 8 XCDs (largest work first, each to the
+// least loaded XCD) and interleaves the XCD queues, since workgroup b of a
+// 1-D dispatch runs on XCD b mod 8; short queues are padded with empty
+// entries (instance -1).
+const std::pair<int2*, int>& Batch::work_map(int kernel, int map, int block) {
+  const std::array<int, 3> key{kernel, map, block};
+  for (auto& m : maps_)
+    if (m.first == key) return m.second;
+  const int ni = (int)h_sys_.size();
+  std::vector<std::vector<int2>> per(ni);
+  std::vector<double> work(ni);
+  for (int k = 0; k < ni; ++k) {
+    int items, lanes;
+    tuned_items(kernel, h_sys_[k], items, lanes, work[k]);
+    const int ipb = block / lanes;
+    for (int b = 0; b * ipb < items; ++b) per[k].push_back(make_int2(k, b * ipb));
+  }
+  std::vector<int2> table;
+  if (map == 1) {
+    for (auto& v : per) table.insert(table.end(), v.begin(), v.end());
+  } else {
+    constexpr int kXcd = 8;
+    std::vector<int> order(ni);
+    for (int k = 0; k < ni; ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
+    std::vector<double> load(kXcd, 0.0);
+    std::vector<std::vector<int2>> q(kXcd);
+    for (int k : order) {
+      if (per[k].empty()) continue;
+      const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      load[x] += work[k];
+      q[x].insert(q[x].end(), per[k].begin(), per[k].end());
+    }
+    size_t n = 0;
+    for (auto& v : q) n = std::max(n, v.size());
+    table.assign(n * kXcd, make_int2(-1, 0));
+    for (int x = 0; x < kXcd; ++x)
+      for (size_t sI = 0; sI < q[x].size(); ++sI) table[sI * kXcd + x] = q[x][sI];
+  }
+  int2* d = nullptr;
+  if (!table.empty()) {
+    HIPCHECK(hipMalloc(&d, table.size() * sizeof(int2)));
+    HIPCHECK(hipMemcpy(d, table.data(), table.size() * sizeof(int2), hipMemcpyHostToDevice));
+  }
+  maps_.push_back({key, {d, (int)table.size()}});
+  return maps_.back().second;
+}
+
+void Batch::launch_tuned(int kernel, const LaunchShape& sh) {
+  const int ni = (int)h_sys_.size();
+  const int blk = sh.block;
+  int max_items = 0, lanes = 8;
+  for (int k = 0; k < ni; ++k) {
+    int items, l;
+    double w;
+    tuned_items(kernel, h_sys_[k], items, l, w);
+    if (w > 0) max_items = std::max(max_items, items);
+    lanes = l;
+  }
+  if (max_items == 0) return;
+  const int2* map = nullptr;
+  dim3 grid((max_items * lanes + blk - 1) / blk, ni);
+  if (sh.map != 0) {
+    const auto& m = work_map(kernel, sh.map, blk);
+    if (m.second == 0) return;
+    map = m.first;
+    grid = dim3(m.second, 1);
+  }
+  const bool f64 = precision_ == 0;
+  const bool mp = sh.map != 0;
+  const bool pf = sh.pf != 0;
+#define QGPU_LAUNCH(K)                                                                         \
+  do {                                                                                         \
+    if (f64 && mp) hipLaunchKernelGGL((K<double, 1>), grid, dim3(blk), 0, stream_, d_sys_, map); \
+    else if (f64) hipLaunchKernelGGL((K<double, 0>), grid, dim3(blk), 0, stream_, d_sys_, map);  \
+    else if (mp) hipLaunchKernelGGL((K<float, 1>), grid, dim3(blk), 0, stream_, d_sys_, map);    \
+    else hipLaunchKernelGGL((K<float, 0>), grid, dim3(blk), 0, stream_, d_sys_, map);            \
+  } while (0)
+#define QGPU_LAUNCH_PF(K)                                                                              \
+  do {                                                                                                 \
+    if (f64 && mp && pf) hipLaunchKernelGGL((K<double, 1, 1>), grid, dim3(blk), 0, stream_, d_sys_, map); \
+    else if (f64 && mp) hipLaunchKernelGGL((K<double, 1, 0>), grid, dim3(blk), 0, stream_, d_sys_, map);  \
+    else if (f64 && pf) hipLaunchKernelGGL((K<double, 0, 1>), grid, dim3(blk), 0, stream_, d_sys_, map);  \
+    else if (f64) hipLaunchKernelGGL((K<double, 0, 0>), grid, dim3(blk), 0, stream_, d_sys_, map);        \
+    else if (mp && pf) hipLaunchKernelGGL((K<float, 1, 1>), grid, dim3(blk), 0, stream_, d_sys_, map);    \
+    else if (mp) hipLaunchKernelGGL((K<float, 1, 0>), grid, dim3(blk), 0, stream_, d_sys_, map);          \
+    else if (pf) hipLaunchKernelGGL((K<float, 0, 1>), grid, dim3(blk), 0, stream_, d_sys_, map);          \
+    else hipLaunchKernelGGL((K<float, 0, 0>), grid, dim3(blk), 0, stream_, d_sys_, map);                  \
+  } while (0)
+  // water forces: the FP32 build fitted to 6 or 4 wavefronts per SIMD
+#define QGPU_WF(PP, M, F)                                                                                 \
+  do {                                                                                                    \
+    if (sh.occ && sizeof(PP) == 4)                                                                        \
+      hipLaunchKernelGGL((k_water_forces<PP, M, F, 4>), grid, dim3(blk), 0, stream_, d_sys_, map);        \
+    else                                                                                                  \
+      hipLaunchKernelGGL((k_water_forces<PP, M, F, 6>), grid, dim3(blk), 0, stream_, d_sys_, map);        \
+  } while (0)
+  switch (kernel) {
+    case TK_WATER_LISTS: QGPU_LAUNCH(k_water_lists); break;
+    case TK_SOLUTE_LISTS: QGPU_LAUNCH(k_solute_lists); break;
+    case TK_WATER_FORCES:
+      if (f64 && mp && pf) QGPU_WF(double, 1, 1);
+      else if (f64 && mp) QGPU_WF(double, 1, 0);
+      else if (f64 && pf) QGPU_WF(double, 0, 1);
+      else if (f64) QGPU_WF(double, 0, 0);
+      else if (mp && pf) QGPU_WF(float, 1, 1);
+      else if (mp) QGPU_WF(float, 1, 0);
+      else if (pf) QGPU_WF(float, 0, 1);
+      else QGPU_WF(float, 0, 0);
+      break;
+    default: QGPU_LAUNCH_PF(k_solute_forces); break;
+  }
+#undef QGPU_WF
+#undef QGPU_LAUNCH
+#undef QGPU_LAUNCH_PF
+}
+
+void Batch::autotune(int reps, double* ms) {
+  static const int kMaps[3] = {0, 1, 2};
+  static const int kBlocks[2] = {256, 128};
+  // the tuning launches rebuild the pair lists from the current coordinates,
+  // which the next step does too only at a list step
+  if (istep_ % nbcycle_ != 0) throw std::runtime_error("launch shapes can be tuned only at a pair-list step");
+  HIPCHECK(hipStreamSynchronize(stream_));
+  hipEvent_t a, b;
+  HIPCHECK(hipEventCreate(&a));
+  HIPCHECK(hipEventCreate(&b));
+  // lists need the group centres, forces need lists
+  launch_lists();
+  for (int kernel = 0; kernel < TK_COUNT; ++kernel) {
+    double best = 1e30, dflt = 0.0;
+    LaunchShape pick = shape_[kernel];
+    const int npf = (kernel == TK_WATER_FORCES || kernel == TK_SOLUTE_FORCES) ? 2 : 1;
+    const int nocc = (kernel == TK_WATER_FORCES && precision_ == 1) ? 2 : 1;
+    for (int m : kMaps)
+      for (int blk : kBlocks)
+        for (int pf = 0; pf < npf; ++pf)
+        for (int oc = 0; oc < nocc; ++oc) {
+        const LaunchShape sh{m, blk, pf, oc};
+        launch_tuned(kernel, sh);   // warm-up (and map upload)
+        HIPCHECK(hipEventRecord(a, stream_));
+        for (int r = 0; r < reps; ++r) launch_tuned(kernel, sh);
+        HIPCHECK(hipEventRecord(b, stream_));
+        HIPCHECK(hipEventSynchronize(b));
+        float t = 0.f;
+        HIPCHECK(hipEventElapsedTime(&t, a, b));
+        const double per = t / reps;
+        if (m == 0 && blk == 256 && pf == 0 && oc == 0) dflt = per;
+        if (per < best) { best = per; pick = sh; }
+        }
+    shape_[kernel] = pick;
+    if (ms) { ms[2 * kernel] = best; ms[2 * kernel + 1] = dflt; }
+  }
+  HIPCHECK(hipGetLastError());
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  tuned_ = true;
+  // captured graphs hold the old launch shapes
+  if (graph_exec_) { (void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
+}
+
+void Batch::set_shape(int k, int map, int block, int pf, int occ) {
+  if (k < 0 || k >= TK_COUNT || map < 0 || map > 2 || (block != 128 && block != 256) || pf < 0 || pf > 1 ||
+      occ < 0 || occ > 1)
+    throw std::runtime_error("invalid launch shape");
+  shape_[k] = LaunchShape{map, block, (k == TK_WATER_FORCES || k == TK_SOLUTE_FORCES) ? pf : 0,
+                          k == TK_WATER_FORCES ? occ : 0};
+  tuned_ = true;
+  if (graph_exec_) { (void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
+}
+
 void Batch::launch_lists() {
   const int ni = (int)h_sys_.size();
   dim3 g0((max_ncgp_ + 255) / 256, ni);
   { pb(0); hipLaunchKernelGGL(k_cgp_centers, g0, dim3(256), 0, stream_, d_sys_); pe(0); }
-  if (max_nwat_ > 0) {
-    dim3 g1((max_nwat_ * 8 + 255) / 256, ni);
-    pb(1);
-    if (precision_ == 0) hipLaunchKernelGGL(k_water_lists<double>, g1, dim3(256), 0, stream_, d_sys_);
-    else hipLaunchKernelGGL(k_water_lists<float>, g1, dim3(256), 0, stream_, d_sys_);
-    pe(1);
-  }
-  if (max_ncgp_solute_ > 0) {
-    dim3 g2((max_ncgp_solute_ * 64 + 255) / 256, ni);
-    pb(2);
-    if (precision_ == 0) hipLaunchKernelGGL(k_solute_lists<double>, g2, dim3(256), 0, stream_, d_sys_);
-    else hipLaunchKernelGGL(k_solute_lists<float>, g2, dim3(256), 0, stream_, d_sys_);
-    pe(2);
-  }
+  if (max_nwat_ > 0) { pb(1); launch_tuned(TK_WATER_LISTS, shape_[TK_WATER_LISTS]); pe(1); }
+  if (max_ncgp_solute_ > 0) { pb(2); launch_tuned(TK_SOLUTE_LISTS, shape_[TK_SOLUTE_LISTS]); pe(2); }
 }
 
 void Batch::launch_forces() {
   const int ni = (int)h_sys_.size();
-  if (max_nwat_ > 0) {
-    dim3 g1((max_nwat_ * 8 + 255) / 256, ni);
-    if (precision_ == 0)
-      { pb(3); hipLaunchKernelGGL(k_water_forces<double>, g1, dim3(256), 0, stream_, d_sys_); pe(3); }
-    else
-      { pb(3); hipLaunchKernelGGL(k_water_forces<float>, g1, dim3(256), 0, stream_, d_sys_); pe(3); }
-  }
-  if (max_ncgp_solute_ > 0) {
-    dim3 gs((max_nat_solute_ * 8 + 255) / 256, ni);
-    if (precision_ == 0)
-      { pb(4); hipLaunchKernelGGL(k_solute_forces<double>, gs, dim3(256), 0, stream_, d_sys_); pe(4); }
-    else
-      { pb(4); hipLaunchKernelGGL(k_solute_forces<float>, gs, dim3(256), 0, stream_, d_sys_); pe(4); }
-  }
+  if (max_nwat_ > 0) { pb(3); launch_tuned(TK_WATER_FORCES, shape_[TK_WATER_FORCES]); pe(3); }
+  if (max_ncgp_solute_ > 0) { pb(4); launch_tuned(TK_SOLUTE_FORCES, shape_[TK_SOLUTE_FORCES]); pe(4); }
   if (max_terms_ > 0)
     { pb(5); hipLaunchKernelGGL(k_terms, dim3((max_terms_ + 255) / 256, ni), dim3(256), 0, stream_, d_sys_); pe(5); }
   if (max_nqat_ > 0) {
@@ -923,6 +1092,10 @@ void Batch::profile(int nsteps, double* ms, int cap) {
 }
 
 void Batch::step(int nsteps) {
+  if (!tuned_ && autotune_on_ && istep_ % nbcycle_ == 0 && nsteps > 0) autotune(3);
+  // work maps are uploaded here, outside any graph capture
+  for (int k = 0; k < TK_COUNT; ++k)
+    if (shape_[k].map != 0) work_map(k, shape_[k].map, shape_[k].block);
   HIPCHECK(hipEventRecord(ev0, stream_));
   int n = 0;
   while (n < nsteps && (!use_graph_ || istep_ % nbcycle_ != 0 || nsteps - n < nbcycle_)) {
@@ -1361,6 +1534,27 @@ int qgpu_last_step_ms(qgpu_handle* h, double* ms) {
     *ms = h->last_ms;
   });
 }
+int qgpu_autotune(qgpu_handle* h, int reps, double* ms, int* shapes) {
+  return guarded([&] {
+    if (!h) throw std::runtime_error("invalid handle");
+    h->b->autotune(reps > 0 ? reps : 3, ms);
+    if (shapes)
+      for (int k = 0; k < qgpu::Batch::TK_COUNT; ++k) {
+        shapes[4 * k] = h->b->shape(k).map;
+        shapes[4 * k + 1] = h->b->shape(k).block;
+        shapes[4 * k + 2] = h->b->shape(k).pf;
+        shapes[4 * k + 3] = h->b->shape(k).occ;
+      }
+  });
+}
+
+int qgpu_set_launch_shape(qgpu_handle* h, int kernel, int map, int block, int prefetch, int occupancy) {
+  return guarded([&] {
+    if (!h) throw std::runtime_error("invalid handle");
+    h->b->set_shape(kernel, map, block, prefetch, occupancy);
+  });
+}
+
 int qgpu_offdiag_states(qgpu_handle* h, int k, int* ij, int cap, int* n) {
   return guarded([&] {
     CHECK_K(h, k);

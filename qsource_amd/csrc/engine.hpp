@@ -21,7 +21,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "host/qsys.hpp"
@@ -281,6 +283,26 @@ class Batch {
   void profile(int nsteps, double* ms, int cap);
   static constexpr int kNumKernels = 13;
 
+  // Launch shapes of the four large kernels (water and solute lists and
+  // forces): workgroup mapping (0: 2-D grid with instance = blockIdx.y;
+  // 1: instance-major work map without empty workgroups; 2: work map that
+  // keeps each instance's workgroups on one XCD, instances balanced over the
+  // XCDs), workgroup size and, for the force kernels, whether each lane
+  // loads its next partner during the current one.  Every shape computes each item with the same
+  // arithmetic in the same order, so results do not depend on the choice.
+  enum TunedKernel { TK_WATER_LISTS, TK_SOLUTE_LISTS, TK_WATER_FORCES, TK_SOLUTE_FORCES, TK_COUNT };
+  // pf: partner prefetch (force kernels); occ: 1 = water forces fitted to 4
+  // instead of 6 wavefronts per SIMD (FP32)
+  struct LaunchShape { int map = 0; int block = 256; int pf = 0; int occ = 0; };
+  // times every shape of every tuned kernel on this batch (a few launches
+  // each; the kernels' outputs do not change) and keeps the fastest; ms
+  // receives, per kernel, the best and the default shape's time
+  void autotune(int reps, double* ms = nullptr);
+  const LaunchShape& shape(int k) const { return shape_[k]; }
+  // fixes one kernel's launch shape (and switches the tuning at the first step off)
+  void set_shape(int k, int map, int block, int pf, int occ);
+  bool tuned() const { return tuned_; }
+
  private:
   void pb(int id) {
     if (prof_) hipEventRecord(pev_[2 * id], stream_);
@@ -304,6 +326,12 @@ class Batch {
   void upload(int k);
   void launch_lists();
   void launch_forces();
+  void launch_tuned(int kernel, const LaunchShape& sh);
+  const std::pair<int2*, int>& work_map(int kernel, int map, int block);
+  LaunchShape shape_[TK_COUNT];
+  bool tuned_ = false;
+  bool autotune_on_ = true;       // tune at the first step (QGPU_AUTOTUNE=0 switches it off)
+  std::vector<std::pair<std::array<int, 3>, std::pair<int2*, int>>> maps_;
   void launch_integrate();
   void ensure_far();
   void launch_far_model();

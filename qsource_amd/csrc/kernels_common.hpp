@@ -30,6 +30,40 @@ __device__ __forceinline__ float inv_r2(float r2sq, float& rinv) {
 }
 
 // Coordinates read by the pair arithmetic: FP64 x, or its FP32 copy
+// A device array read through the global address space: pointers held in
+// DevSys are generic to the compiler, which then emits flat loads that wait
+// on both the LDS and the vector-memory counters; global loads do not.
+template <typename T>
+using gconst = const __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ gconst<T> gptr(const T* p) {
+  return (gconst<T>)p;
+}
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gptr_w(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+
+// Instance and first item of this workgroup.  MAPPED = 0: the 2-D grid
+// (blockIdx.y = instance, blockIdx.x * items_per_block = first item).
+// MAPPED = 1: one work-map entry per workgroup of a 1-D grid (instance,
+// first item; instance < 0 for an empty slot), which lets the host place
+// each instance's workgroups on the XCDs it chooses and skip empty ones.
+template <int MAPPED>
+__device__ __forceinline__ bool block_work(const DevSys* sys, const int2* map, int items_per_block,
+                                           const DevSys*& S, int& base) {
+  if constexpr (MAPPED != 0) {
+    const int2 m = map[blockIdx.x];
+    if (m.x < 0) return false;
+    S = sys + m.x;
+    base = m.y;
+  } else {
+    S = sys + blockIdx.y;
+    base = (int)blockIdx.x * items_per_block;
+  }
+  return true;
+}
+
 template <typename P>
 __device__ __forceinline__ const P* pair_x(const DevSys& S);
 template <>
@@ -248,19 +282,21 @@ __device__ __forceinline__ void lrf_gather_group(const DevSys& S, int g, double 
 // lrf_gather_group for a standard water (its charge group is its own three
 // atoms jo, jo+1, jo+2 with the solvent charges, none of them a Q atom):
 // the same terms in the same order, without the group/atom indirection.
+// In mixed precision the partner's offsets from the centre come from the
+// FP32 coordinate copy and the FP32 centre (cf).
 template <typename P>
 __device__ __forceinline__ void lrf_gather_water(const DevSys& S, int jo, double c0, double c1,
-                                                 double c2, LrfAcc& acc) {
-  const double* xj = S.x + 3 * jo;
+                                                 double c2, const float* cf, LrfAcc& acc) {
   if constexpr (sizeof(P) == 8) {
+    const gconst<double> xj = gptr(S.x) + 3 * jo;
     for (int k = 0; k < 3; ++k)
       acc.add(S.chg_solv[k], xj[3 * k] - c0, xj[3 * k + 1] - c1, xj[3 * k + 2] - c2);
   } else {
+    const gconst<float> xj = gptr(S.xf) + 3 * jo;
     LrfTerms<float> part;
     part.zero();
     for (int k = 0; k < 3; ++k)
-      part.add((float)S.chg_solv[k], (float)(xj[3 * k] - c0), (float)(xj[3 * k + 1] - c1),
-               (float)(xj[3 * k + 2] - c2));
+      part.add((float)S.chg_solv[k], xj[3 * k] - cf[0], xj[3 * k + 1] - cf[1], xj[3 * k + 2] - cf[2]);
     acc.add(part);
   }
 }

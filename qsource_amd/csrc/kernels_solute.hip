@@ -61,11 +61,14 @@ __device__ ulonglong2 pair_masks(const DevSys& S, int g, int jg) {
 // Solute lists (nbpplist_lrf src/md.f90:7998-8188 and the solute side of
 // nbpwlist_lrf :9300-9429): one wavefront per solute charge group; P as in
 // k_water_lists (precision of the LRF multipole terms).
-template <typename P>
-__global__ __launch_bounds__(256) void k_solute_lists(const DevSys* __restrict__ sys) {
-  const DevSys& S = sys[blockIdx.y];
+template <typename P, int MAPPED>
+__global__ __launch_bounds__(256) void k_solute_lists(const DevSys* __restrict__ sys, const int2* __restrict__ map) {
+  const DevSys* Sp;
+  int base;
+  if (!block_work<MAPPED>(sys, map, blockDim.x >> 6, Sp, base)) return;
+  const DevSys& S = *Sp;
   const int lane = threadIdx.x & 63;
-  const int g = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int g = base + (threadIdx.x >> 6);
   if (g >= S.ncgp_solute) return;
   const int sw = S.cgp_sw[g];
   if (S.flags[sw] & F_EXCL) {
@@ -75,6 +78,7 @@ __global__ __launch_bounds__(256) void k_solute_lists(const DevSys* __restrict__
   const double xs0 = S.x[3 * sw], xs1 = S.x[3 * sw + 1], xs2 = S.x[3 * sw + 2];
   double* Lg = S.lrf + (size_t)g * kLrfStride;
   const double c0 = Lg[0], c1 = Lg[1], c2 = Lg[2];
+  const float cf[3] = {(float)c0, (float)c1, (float)c2};
   LrfAcc acc;
   acc.zero();
   int n_pp = 0;
@@ -121,7 +125,7 @@ __global__ __launch_bounds__(256) void k_solute_lists(const DevSys* __restrict__
     }
     n_pw += __popcll(m);
     if (far && S.use_lrf) {
-      if (S.std_water) lrf_gather_water<P>(S, jo, c0, c1, c2, acc);
+      if (S.std_water) lrf_gather_water<P>(S, jo, c0, c1, c2, cf, acc);
       else lrf_gather_group<P>(S, S.iwhich_cgp[jo], c0, c1, c2, acc);
     }
   }
@@ -133,8 +137,10 @@ __global__ __launch_bounds__(256) void k_solute_lists(const DevSys* __restrict__
     if (n_pp > S.pp_cap || n_pw > S.pw_cap) atomicOr(S.overflow, 1);
   }
 }
-template __global__ void k_solute_lists<double>(const DevSys*);
-template __global__ void k_solute_lists<float>(const DevSys*);
+template __global__ void k_solute_lists<double, 0>(const DevSys*, const int2*);
+template __global__ void k_solute_lists<float, 0>(const DevSys*, const int2*);
+template __global__ void k_solute_lists<double, 1>(const DevSys*, const int2*);
+template __global__ void k_solute_lists<float, 1>(const DevSys*, const int2*);
 
 // ---------------------------------------------------------------------------
 // Solute forces (nonbond_pp :13069-13155, nonbond_pw :13282-13365,
@@ -147,19 +153,23 @@ template __global__ void k_solute_lists<float>(const DevSys*);
 // counted by the atom that is i of the reference pair: the owner group's
 // atom, or the lower index within one group.
 // FP32 pair arithmetic fits 6 wavefronts per SIMD without spilling
-template <typename P>
-__global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(const DevSys* __restrict__ sys) {
-  const DevSys& S = sys[blockIdx.y];
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if ((t >> 6) * 8 >= S.nat_solute) return;   // whole wavefronts past the last atom
+template <typename P, int MAPPED, int PF>
+__global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(const DevSys* __restrict__ sys,
+                                                                               const int2* __restrict__ map) {
+  const DevSys* Sp;
+  int base;
+  if (!block_work<MAPPED>(sys, map, blockDim.x >> 3, Sp, base)) return;
+  const DevSys& S = *Sp;
+  const int t = threadIdx.x;
+  if (base + ((t >> 6) << 3) >= S.nat_solute) return;   // whole wavefronts past the last atom
   const int sub = t & 7;
-  const int i = t >> 3;
+  const int i = base + (t >> 3);
   const bool active = i < S.nat_solute;
   const int ia = active ? i : 0;
   const int g = S.iwhich_cgp[ia];
   const int a = S.cgp_slot[ia];
   const unsigned char fl = S.flags[ia];
-  const P* X = pair_x<P>(S);
+  const gconst<P> X = gptr(pair_x<P>(S));
   const P xi0 = X[3 * ia], xi1 = X[3 * ia + 1], xi2 = X[3 * ia + 2];
   const int ti = S.iac[ia];
   const P qi = (P)S.crg[ia];
@@ -168,21 +178,26 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(c
   if (active && !(fl & F_Q)) {
     // solute-solute
     const int npp = S.pp_cnt[g];
-    const int* prow = S.pp_nb + (size_t)g * S.pp_cap;
-    const ulonglong2* mrow = S.pp_mask + (size_t)g * S.pp_cap;
+    const gconst<int> prow = gptr(S.pp_nb) + (size_t)g * S.pp_cap;
+    // (skip, 1-4) mask pairs of the entries, read as two 64-bit words
+    const gconst<unsigned long long> mrow =
+        gptr(reinterpret_cast<const unsigned long long*>(S.pp_mask + (size_t)g * S.pp_cap));
+    const gconst<int> gatom = gptr(S.cgpatom);
+    const gconst<int> giac = gptr(S.iac);
+    const gconst<double> gcrg = gptr(S.crg);
     for (int k = sub; k < npp; k += 8) {
       const int jg = prow[k];
-      const ulonglong2 mk = mrow[k];
-      unsigned todo = (unsigned)(~mk.x >> (a * kMaxGroupAtoms)) & 0xffu;
-      const unsigned is14 = (unsigned)(mk.y >> (a * kMaxGroupAtoms)) & 0xffu;
+      const unsigned long long mk_x = mrow[2 * k], mk_y = mrow[2 * k + 1];
+      unsigned todo = (unsigned)(~mk_x >> (a * kMaxGroupAtoms)) & 0xffu;
+      const unsigned is14 = (unsigned)(mk_y >> (a * kMaxGroupAtoms)) & 0xffu;
       const bool same = jg == g;
       const bool own = same || owns_pair(g, jg);
       const int fj = S.cgp_first[jg];
       while (todo) {
         const int b = __builtin_ctz(todo);
         todo &= todo - 1;
-        const int j = S.cgpatom[fj + b];
-        const int tj = S.iac[j];
+        const int j = gatom[fj + b];
+        const int tj = giac[j];
         const int lj = ((is14 >> b) & 1) ? 3 : S.ljcod[ti * S.natyps + tj];
         const int l = lj - 1;
         const bool me_first = same ? (i < j) : own;
@@ -193,7 +208,7 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(c
         P r;
         const P r2 = inv_r2(pnorm2(dx1, dx2, dx3), r);
         const P r6 = r2 * r2 * r2;
-        P Vel = qi * (P)S.crg[j] * r;
+        P Vel = qi * (P)gcrg[j] * r;
         if (lj == 3) Vel = Vel * (P)S.el14_scale;
         const P Va = r6 * r6 * (P)S.avdw[ti * 3 + l] * (P)S.avdw[tj * 3 + l];
         const P Vb = r6 * (P)S.bvdw[ti * 3 + l] * (P)S.bvdw[tj * 3 + l];
@@ -213,13 +228,12 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(c
     }
     // solute-water (the solute atom is i of the reference pair)
     const int npw = S.pw_cnt[g];
-    const int* wrow = S.pw_nb + (size_t)g * S.pw_cap;
+    const gconst<int> wrow = gptr(S.pw_nb) + (size_t)g * S.pw_cap;
     // standard waters: LJ products and zero-interaction mask of this atom
     // with O, H1, H2 from the per-atom table
     const P* wlj = S.std_water ? solute_water_lj<P>(S) + 6 * (size_t)i : nullptr;
     const unsigned wzero = S.std_water ? S.swp_zero[i] : 0u;
-    for (int k = sub; k < npw; k += 8) {
-      const int jo = S.nat_solute + 3 * wrow[k];
+    auto pw_water = [&](const int jo, const P* xw) {
       for (int ip = 0; ip < 3; ++ip) {
         const int j = jo + ip;
         P aLJ, bLJ;
@@ -234,7 +248,7 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(c
           aLJ = (P)(S.avdw[ti * 3 + lj] * S.avdw[tj * 3 + lj]);
           bLJ = (P)(S.bvdw[ti * 3 + lj] * S.bvdw[tj * 3 + lj]);
         }
-        const P dx1 = X[3 * j] - xi0, dx2 = X[3 * j + 1] - xi1, dx3 = X[3 * j + 2] - xi2;
+        const P dx1 = xw[3 * ip] - xi0, dx2 = xw[3 * ip + 1] - xi1, dx3 = xw[3 * ip + 2] - xi2;
         P r;
         const P r2 = inv_r2(pnorm2(dx1, dx2, dx3), r);
         const P r6 = r2 * r2 * r2;
@@ -248,6 +262,34 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(c
         g2 = g2 - (double)(dv * dx3);
         e_pwel = e_pwel + (double)Vel;
         e_pwvdw = e_pwvdw + (double)Va - (double)Vb;
+      }
+    };
+    if constexpr (PF != 0) {
+      // next water's index and coordinates loaded during this one's pairs
+      int k = sub;
+      int jon = 0;
+      P xn[9];
+      if (k < npw) {
+        jon = S.nat_solute + 3 * wrow[k];
+        for (int u = 0; u < 9; ++u) xn[u] = X[3 * jon + u];
+      }
+      while (k < npw) {
+        P xw[9];
+        for (int u = 0; u < 9; ++u) xw[u] = xn[u];
+        const int jo = jon;
+        k += 8;
+        if (k < npw) {
+          jon = S.nat_solute + 3 * wrow[k];
+          for (int u = 0; u < 9; ++u) xn[u] = X[3 * jon + u];
+        }
+        pw_water(jo, xw);
+      }
+    } else {
+      for (int k = sub; k < npw; k += 8) {
+        const int jo = S.nat_solute + 3 * wrow[k];
+        P xw[9];
+        for (int u = 0; u < 9; ++u) xw[u] = X[3 * jo + u];
+        pw_water(jo, xw);
       }
     }
     // Q-solute pairs seen from the solute atom (the Q atom is i of the pair)
@@ -322,8 +364,14 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(c
     es[4] = e_lrf; es[5] = e_fix; es[6] = e_shell; es[7] = 0.0;
   }
 }
-template __global__ void k_solute_forces<double>(const DevSys*);
-template __global__ void k_solute_forces<float>(const DevSys*);
+template __global__ void k_solute_forces<double, 0, 0>(const DevSys*, const int2*);
+template __global__ void k_solute_forces<float, 0, 0>(const DevSys*, const int2*);
+template __global__ void k_solute_forces<double, 1, 0>(const DevSys*, const int2*);
+template __global__ void k_solute_forces<float, 1, 0>(const DevSys*, const int2*);
+template __global__ void k_solute_forces<double, 0, 1>(const DevSys*, const int2*);
+template __global__ void k_solute_forces<float, 0, 1>(const DevSys*, const int2*);
+template __global__ void k_solute_forces<double, 1, 1>(const DevSys*, const int2*);
+template __global__ void k_solute_forces<float, 1, 1>(const DevSys*, const int2*);
 
 // ---------------------------------------------------------------------------
 // Q atoms (nonbond_qp :13788-13921, nonbond_qw_spc :14411-14570 seen from

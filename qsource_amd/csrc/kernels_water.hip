@@ -52,33 +52,39 @@ __global__ void k_x_to_f32(const DevSys* __restrict__ sys) {
 // lanes at the end).  P = float evaluates each partner group's multipole
 // terms in FP32 before adding them to the FP64 sums (mixed precision); list
 // membership is decided in FP64 in both.
-template <typename P>
-__global__ __launch_bounds__(256) void k_water_lists(const DevSys* __restrict__ sys) {
-  const DevSys& S = sys[blockIdx.y];
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if ((t >> 6) * 8 >= S.nwat) return;   // whole wavefronts past the last water
+template <typename P, int MAPPED>
+__global__ __launch_bounds__(256) void k_water_lists(const DevSys* __restrict__ sys, const int2* __restrict__ map) {
+  const DevSys* Sp;
+  int base;
+  if (!block_work<MAPPED>(sys, map, blockDim.x >> 3, Sp, base)) return;
+  const DevSys& S = *Sp;
+  const int t = threadIdx.x;
+  if (base + ((t >> 6) << 3) >= S.nwat) return;   // whole wavefronts past the last water
   const int lane = t & 63, sub = t & 7, seg = lane & ~7;
-  const int iw = t >> 3;
+  const int iw = base + (t >> 3);
+  const gconst<double> gx = gptr(S.x);
+  const gconst<unsigned char> gfl = gptr(S.flags);
   const int io = S.nat_solute + 3 * (iw < S.nwat ? iw : 0);
-  const bool active = iw < S.nwat && !(S.flags[io] & F_EXCL);
-  const double xi0 = S.x[3 * io], xi1 = S.x[3 * io + 1], xi2 = S.x[3 * io + 2];
+  const bool active = iw < S.nwat && !(gfl[io] & F_EXCL);
+  const double xi0 = gx[3 * io], xi1 = gx[3 * io + 1], xi2 = gx[3 * io + 2];
   const int ig = S.iwhich_cgp[io];
   double* Lg = S.lrf + (size_t)ig * kLrfStride;
   const double c0 = Lg[0], c1 = Lg[1], c2 = Lg[2];
+  const float cf[3] = {(float)c0, (float)c1, (float)c2};
   const unsigned long long below = (1ull << lane) - 1ull;
   const unsigned long long mine = 0xffull << seg;
   LrfAcc acc;
   acc.zero();
   // waters
   int n_in = 0;
-  int* row = S.ww_nb + (size_t)(iw < S.nwat ? iw : 0) * S.ww_cap;
+  auto row = gptr_w(S.ww_nb + (size_t)(iw < S.nwat ? iw : 0) * S.ww_cap);
   for (int base = 0; base < S.nwat; base += 8) {
     const int jw = base + sub;
     bool inside = false, far = false;
     if (active && jw < S.nwat && jw != iw) {
       const int jo = S.nat_solute + 3 * jw;
-      if (!(S.flags[jo] & F_EXCL)) {
-        const double a = xi0 - S.x[3 * jo], b = xi1 - S.x[3 * jo + 1], c = xi2 - S.x[3 * jo + 2];
+      if (!(gfl[jo] & F_EXCL)) {
+        const double a = xi0 - gx[3 * jo], b = xi1 - gx[3 * jo + 1], c = xi2 - gx[3 * jo + 2];
         const double r2 = a * a + b * b + c * c;
         inside = r2 <= S.rcww2;
         far = !inside && r2 <= S.rclrf2;
@@ -92,21 +98,21 @@ __global__ __launch_bounds__(256) void k_water_lists(const DevSys* __restrict__ 
     n_in += __popcll(m);
     if (far && S.use_lrf) {
       const int jo = S.nat_solute + 3 * jw;
-      if (S.std_water) lrf_gather_water<P>(S, jo, c0, c1, c2, acc);
+      if (S.std_water) lrf_gather_water<P>(S, jo, c0, c1, c2, cf, acc);
       else lrf_gather_group<P>(S, S.iwhich_cgp[jo], c0, c1, c2, acc);
     }
   }
   // solute charge groups (water side of the solute-water list)
   int n_wp = 0;
-  int* wrow = S.wp_nb + (size_t)(iw < S.nwat ? iw : 0) * S.wp_cap;
+  auto wrow = gptr_w(S.wp_nb + (size_t)(iw < S.nwat ? iw : 0) * S.wp_cap);
   for (int base = 0; base < S.ncgp_solute; base += 8) {
     const int g = base + sub;
     bool inside = false, far = false;
     if (active && g < S.ncgp_solute) {
       const int sw = S.cgp_sw[g];
-      if (!(S.flags[sw] & F_EXCL)) {
+      if (!(gfl[sw] & F_EXCL)) {
         // r2 as the reference forms it: solute switch atom minus water O
-        const double a = S.x[3 * sw] - xi0, b = S.x[3 * sw + 1] - xi1, c = S.x[3 * sw + 2] - xi2;
+        const double a = gx[3 * sw] - xi0, b = gx[3 * sw + 1] - xi1, c = gx[3 * sw + 2] - xi2;
         const double r2 = a * a + b * b + c * c;
         inside = r2 <= S.rcpw2;
         far = !inside && r2 <= S.rclrf2;
@@ -133,8 +139,10 @@ __global__ __launch_bounds__(256) void k_water_lists(const DevSys* __restrict__ 
     }
   }
 }
-template __global__ void k_water_lists<double>(const DevSys*);
-template __global__ void k_water_lists<float>(const DevSys*);
+template __global__ void k_water_lists<double, 0>(const DevSys*, const int2*);
+template __global__ void k_water_lists<float, 0>(const DevSys*, const int2*);
+template __global__ void k_water_lists<double, 1>(const DevSys*, const int2*);
+template __global__ void k_water_lists<float, 1>(const DevSys*, const int2*);
 
 // ---------------------------------------------------------------------------
 // Water forces (nonbond_ww_spc src/md.f90:15021-15119, water side of
@@ -148,17 +156,22 @@ template __global__ void k_water_lists<float>(const DevSys*);
 // reference operation order inside each pair) or float (mixed: FP32
 // coordinates and FMA pair arithmetic, each partner's contribution
 // accumulated in FP64).
-// FP32 pair arithmetic fits 6 wavefronts per SIMD without spilling
-template <typename P>
-__global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_water_forces(const DevSys* __restrict__ sys) {
-  const DevSys& S = sys[blockIdx.y];
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if ((t >> 6) * 8 >= S.nwat) return;   // whole wavefronts past the last water
+// W: wavefronts per SIMD the FP32 build is fitted to (6: registers spill to
+// scratch; 4: no spills); which is faster is measured by the launch tuner
+template <typename P, int MAPPED, int PF, int W>
+__global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(const DevSys* __restrict__ sys,
+                                                                              const int2* __restrict__ map) {
+  const DevSys* Sp;
+  int base;
+  if (!block_work<MAPPED>(sys, map, blockDim.x >> 3, Sp, base)) return;
+  const DevSys& S = *Sp;
+  const int t = threadIdx.x;
+  if (base + ((t >> 6) << 3) >= S.nwat) return;   // whole wavefronts past the last water
   const int sub = t & 7;
-  const int iw = t >> 3;
+  const int iw = base + (t >> 3);
   const bool active = iw < S.nwat;
   const int io = S.nat_solute + 3 * (active ? iw : 0);
-  const P* X = pair_x<P>(S);
+  const gconst<P> X = gptr(pair_x<P>(S));
   double g[9];
   for (int k = 0; k < 9; ++k) g[k] = 0.0;
   double e_el = 0.0, e_vdw = 0.0, e_lrf = 0.0, e_rad = 0.0;
@@ -174,12 +187,8 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_water_forces(co
   const P aOO = (P)S.aOO, bOO = (P)S.bOO;
   // water-water
   const int nn = active ? S.ww_cnt[iw] : 0;
-  const int* row = S.ww_nb + (size_t)iw * S.ww_cap;
-  for (int k = sub; k < nn; k += 8) {
-    const int jw = row[k];
-    const int jo = S.nat_solute + 3 * jw;
-    P xj[9];
-    for (int u = 0; u < 9; ++u) xj[u] = X[3 * jo + u];
+  const gconst<int> row = gptr(S.ww_nb) + (size_t)iw * S.ww_cap;
+  auto ww_pair = [&](const int jw, const P* xj) {
     P el = 0, vdw = 0;
     for (int la = 0; la < 3; ++la) {
       P gl0 = 0, gl1 = 0, gl2 = 0;
@@ -214,11 +223,41 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_water_forces(co
       e_el = e_el + (double)el;
       e_vdw = e_vdw + (double)vdw;
     }
+  };
+  if constexpr (PF != 0) {
+    // the next partner's index and coordinates are loaded while this
+    // partner's pairs are computed (same partners, same order)
+    int k = sub;
+    int jwn = 0;
+    P xn[9];
+    if (k < nn) {
+      jwn = row[k];
+      for (int u = 0; u < 9; ++u) xn[u] = X[3 * (S.nat_solute + 3 * jwn) + u];
+    }
+    while (k < nn) {
+      P xj[9];
+      for (int u = 0; u < 9; ++u) xj[u] = xn[u];
+      const int jw = jwn;
+      k += 8;
+      if (k < nn) {
+        jwn = row[k];
+        for (int u = 0; u < 9; ++u) xn[u] = X[3 * (S.nat_solute + 3 * jwn) + u];
+      }
+      ww_pair(jw, xj);
+    }
+  } else {
+    for (int k = sub; k < nn; k += 8) {
+      const int jw = row[k];
+      const int jo = S.nat_solute + 3 * jw;
+      P xj[9];
+      for (int u = 0; u < 9; ++u) xj[u] = X[3 * jo + u];
+      ww_pair(jw, xj);
+    }
   }
   // water-solute: every non-Q atom of each listed solute group (the solute
   // atom is i of the reference pair)
   const int nwp = active ? S.wp_cnt[iw] : 0;
-  const int* wrow = S.wp_nb + (size_t)iw * S.wp_cap;
+  const gconst<int> wrow = gptr(S.wp_nb) + (size_t)iw * S.wp_cap;
   for (int k = sub; k < nwp; k += 8) {
     const int gs = wrow[k];
     for (int a = S.cgp_first[gs]; a <= S.cgp_last[gs]; ++a) {
@@ -353,8 +392,18 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_water_forces(co
     ew[0] = e_el; ew[1] = e_vdw; ew[2] = e_lrf; ew[3] = e_rad;
   }
 }
-template __global__ void k_water_forces<double>(const DevSys*);
-template __global__ void k_water_forces<float>(const DevSys*);
+template __global__ void k_water_forces<double, 0, 0, 6>(const DevSys*, const int2*);
+template __global__ void k_water_forces<double, 0, 1, 6>(const DevSys*, const int2*);
+template __global__ void k_water_forces<double, 1, 0, 6>(const DevSys*, const int2*);
+template __global__ void k_water_forces<double, 1, 1, 6>(const DevSys*, const int2*);
+template __global__ void k_water_forces<float, 0, 0, 6>(const DevSys*, const int2*);
+template __global__ void k_water_forces<float, 0, 0, 4>(const DevSys*, const int2*);
+template __global__ void k_water_forces<float, 0, 1, 6>(const DevSys*, const int2*);
+template __global__ void k_water_forces<float, 0, 1, 4>(const DevSys*, const int2*);
+template __global__ void k_water_forces<float, 1, 0, 6>(const DevSys*, const int2*);
+template __global__ void k_water_forces<float, 1, 0, 4>(const DevSys*, const int2*);
+template __global__ void k_water_forces<float, 1, 1, 6>(const DevSys*, const int2*);
+template __global__ void k_water_forces<float, 1, 1, 4>(const DevSys*, const int2*);
 
 // ---------------------------------------------------------------------------
 // Water polarisation restraint (watpol, src/md.f90:18372-18541), part 1:

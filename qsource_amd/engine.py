@@ -38,7 +38,9 @@ SYMBOLS = [
     "qgpu_far_samples", "qgpu_far_lrf_estimate", "qgpu_set_far_model",
     "qgpu_profile_kernels", "qgpu_list_counts", "qgpu_offdiag_states",
     "qgpu_trj_create", "qgpu_trj_natoms", "qgpu_trj_write", "qgpu_trj_close", "qgpu_write_restart",
+    "qgpu_autotune", "qgpu_set_launch_shape",
 ]
+TUNED_KERNELS = ["water_lists", "solute_lists", "water_forces", "solute_forces"]
 
 KERNEL_NAMES = ["cgp_centers", "water_lists", "solute_lists", "water_forces", "solute_forces",
                 "terms", "q_forces", "term_gather", "watpol_theta", "watpol_force",
@@ -98,6 +100,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.qgpu_far_lrf_estimate.argtypes = [P, ctypes.c_int, fp, fp, ctypes.c_int, ip]
     lib.qgpu_set_far_model.argtypes = [P, P]
     lib.qgpu_offdiag_states.argtypes = [P, ctypes.c_int, ip, ctypes.c_int, ip]
+    lib.qgpu_autotune.argtypes = [P, ctypes.c_int, dp, ip]
+    lib.qgpu_set_launch_shape.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     lib.qgpu_trj_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(P)]
     lib.qgpu_trj_natoms.argtypes = [P]
     lib.qgpu_trj_write.argtypes = [P, dp]
@@ -234,6 +238,21 @@ class Engine:
         if m.value == 0:
             return np.zeros((n.value, 0, 2))
         return out[:n.value * m.value * 2].reshape(n.value, m.value, 2)
+
+    def autotune(self, reps: int = 3) -> dict:
+        """Times every launch shape of the four large kernels on this batch
+        and keeps the fastest; returns per kernel (mapping, workgroup size,
+        prefetch, register budget, chosen ms, default ms)."""
+        ms = np.zeros(8)
+        sh = np.zeros(16, dtype=np.int32)
+        self._check(self.lib.qgpu_autotune(self.h, reps, _dp(ms), sh.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return {name: (int(sh[4 * k]), int(sh[4 * k + 1]), int(sh[4 * k + 2]), int(sh[4 * k + 3]),
+                       float(ms[2 * k]), float(ms[2 * k + 1]))
+                for k, name in enumerate(TUNED_KERNELS)}
+
+    def set_launch_shape(self, kernel: str, mapping: int, block: int, prefetch: int = 0, occupancy: int = 0):
+        self._check(self.lib.qgpu_set_launch_shape(self.h, TUNED_KERNELS.index(kernel), mapping, block, prefetch,
+                                                   occupancy))
 
     def offdiag_states(self, k: int = 0) -> np.ndarray:
         """(noffd, 2) 1-based state pairs (i, j) of the off-diagonal functions."""
