@@ -44,6 +44,19 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def measured_traffic(name="k_water_forces_c3_63win_mixed_pmc_traffic.md"):
+    """HBM bytes per launch of the FP32 water-force kernel, from the
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (two counter runs,
+    calibrated on a plain copy); None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def build_c3(d, seed_base=3000, equil=True):
     """Writes the 63 window inputs; returns (inputs, window lambdas per
     system, cfg)."""
@@ -249,7 +262,8 @@ def main():
         ach = flops / (prof["water_forces"] * 1e-3) / 1e12 if prof["water_forces"] > 0 else 0.0
         roof = {"bound": "valu", "kernel": "water_forces", "achieved": ach,
                 "peak": PEAK_TFLOPS["mixed"], "unit": "TFLOP/s", "frac": ach / PEAK_TFLOPS["mixed"],
-                "traffic": None, "flop_per_launch": flops, "water_pairs_per_launch": wpairs,
+                "traffic": measured_traffic(), "traffic_source": "profiles/k_water_forces_c3_63win_mixed_pmc_traffic.md",
+                "flop_per_launch": flops, "water_pairs_per_launch": wpairs,
                 "kernel_ms": prof["water_forces"], "dominant_kernel": dom}
         em.close()
         ferr = force_error([inputs[k] for k in mid], states)

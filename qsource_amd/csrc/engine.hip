@@ -870,13 +870,16 @@ void Batch::autotune(int reps, double* ms) {
         for (int oc = 0; oc < nocc; ++oc) {
         const LaunchShape sh{m, blk, pf, oc};
         launch_tuned(kernel, sh);   // warm-up (and map upload)
-        HIPCHECK(hipEventRecord(a, stream_));
-        for (int r = 0; r < reps; ++r) launch_tuned(kernel, sh);
-        HIPCHECK(hipEventRecord(b, stream_));
-        HIPCHECK(hipEventSynchronize(b));
-        float t = 0.f;
-        HIPCHECK(hipEventElapsedTime(&t, a, b));
-        const double per = t / reps;
+        double per = 1e30;
+        for (int pass = 0; pass < 2; ++pass) {   // the faster of two timed passes
+          HIPCHECK(hipEventRecord(a, stream_));
+          for (int r = 0; r < reps; ++r) launch_tuned(kernel, sh);
+          HIPCHECK(hipEventRecord(b, stream_));
+          HIPCHECK(hipEventSynchronize(b));
+          float t = 0.f;
+          HIPCHECK(hipEventElapsedTime(&t, a, b));
+          per = std::min(per, (double)t / reps);
+        }
         if (m == 0 && blk == 256 && pf == 0 && oc == 0) dflt = per;
         if (per < best) { best = per; pick = sh; }
         }
