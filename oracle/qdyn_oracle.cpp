@@ -16,6 +16,7 @@
 //   pot_energy                 16015-16221  pot_energy_nonbonds 16261-16344
 //   nonbond_pp 13069-13155, nonbond_pw 13282-13365, nonbond_qp 13788-13921,
 //   nonbond_ww_spc 15021-15119, nonbond_qw_spc 14411-14570,
+//   nonbond_3atomsolvent 15238-15333, nonbond_qw_3atom 14749-14874,
 //   nonbond_qq 13491-13636, lrf_taylor 5824-5891
 //   bond 1332-1405, angle 1130-1246, torsion 17907-18052, improper 3130-3272
 //   fix_shell 2816-2862, restrain_solvent 18055-18102, watpol 18372-18541
@@ -302,6 +303,44 @@ struct Oracle {
   void nonbond_pw();
   void nonbond_qp();
   void nonbond_qw_spc();
+  void nonbond_qw_3atom();
+
+  // nonbond_3atomsolvent (src/md.f90:15238-15333): LJ between every atom
+  // pair of two waters, the LJ code from atom i and the other water's
+  // first atom, parameters of the atoms' positions in the water
+  void nonbond_3atomsolvent() {
+    int ichg = 1;
+    const size_t n = nbww.size();
+    for (size_t iw = 0; iw < n; iw += 3) {
+      if (ichg > 3) ichg = 1;
+      int i = nbww[iw].i, j = nbww[iw].j;
+      int i3 = 3 * i;
+      double c1 = s.chg_solv[ichg - 1];
+      for (int ip = 0; ip < 3; ++ip) {
+        int ja = j + ip, j3 = 3 * ja;
+        double c2 = s.chg_solv[ip];
+        double dx1 = x[j3] - x[i3], dx2 = x[j3 + 1] - x[i3 + 1], dx3 = x[j3 + 2] - x[i3 + 2];
+        double r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
+        double r6 = r2 * r2 * r2;
+        r2 = 1.0 / r2;
+        double r = std::sqrt(r2);
+        double Vel = c1 * c2 * r;
+        double dv = r2 * (-Vel);
+        int iLJ = ljcod(s.iac[i], s.iac[j]);
+        double a1 = s.alj_solv[ichg - 1][iLJ - 1], a2 = s.alj_solv[ip][iLJ - 1];
+        double b1 = s.blj_solv[ichg - 1][iLJ - 1], b2 = s.blj_solv[ip][iLJ - 1];
+        r6 = 1.0 / r6;
+        double Va = a1 * a2 * r6 * r6;
+        double Vb = b1 * b2 * r6;
+        E.ww_el = E.ww_el + Vel;
+        E.ww_vdw = E.ww_vdw + Va - Vb;
+        dv = dv + r2 * (-12.0 * Va + 6.0 * Vb);
+        d[i3] = d[i3] - dv * dx1; d[i3 + 1] = d[i3 + 1] - dv * dx2; d[i3 + 2] = d[i3 + 2] - dv * dx3;
+        d[j3] = d[j3] + dv * dx1; d[j3 + 1] = d[j3 + 1] + dv * dx2; d[j3 + 2] = d[j3 + 2] + dv * dx3;
+      }
+      ++ichg;
+    }
+  }
   void nonbond_qq();
   double bond(int a, int b);
   double angle(int a, int b);
@@ -489,8 +528,13 @@ struct Oracle {
     if (s.qvdw_flag) nonbond_qp_qvdw();
     else nonbond_qp();
     if (s.natom > s.nat_solute) {
-      nonbond_ww_spc();
-      nonbond_qw_spc();
+      if (s.top.solvent_type == 1) {
+        nonbond_3atomsolvent();
+        nonbond_qw_3atom();
+      } else {
+        nonbond_ww_spc();
+        nonbond_qw_spc();
+      }
     }
     if (s.inp.use_lrf) lrf_taylor();
     // bonded (FF_GROMOS/default impropers)
@@ -893,6 +937,51 @@ void Oracle::nonbond_qw_spc() {
       }
       d[i3] = d[i3] - dvh * ex1; d[i3 + 1] = d[i3 + 1] - dvh * ex2; d[i3 + 2] = d[i3 + 2] - dvh * ex3;
       d[ja3] = d[ja3] + dvh * ex1; d[ja3 + 1] = d[ja3 + 1] + dvh * ex2; d[ja3 + 2] = d[ja3 + 2] + dvh * ex3;
+    }
+  }
+}
+
+// nonbond_qw_3atom (src/md.f90:14749-14874): LJ and Coulomb on every water
+// atom, the LJ code of each (water atom, Q atom) pair from the list
+void Oracle::nonbond_qw_3atom() {
+  const int ns = s.nstates;
+  for (size_t iw = 0; iw < nbqw.size(); iw += 3) {
+    int iq = nbqw[iw].iq, i = s.iqseq[iq], i3 = 3 * i;
+    int j = nbqw[iw].j;
+    for (int ip = 0; ip < 3; ++ip) {
+      int ja = j + ip, j3 = 3 * ja;
+      double dx1 = x[j3] - x[i3], dx2 = x[j3 + 1] - x[i3 + 1], dx3 = x[j3 + 2] - x[i3 + 2];
+      double r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
+      double r6_hc = r2 * r2 * r2;
+      r2 = 1.0 / r2;
+      double r = std::sqrt(r2);
+      double r6_sc = r6_hc;
+      int iLJ = nbqw[iw + ip].ljcod;
+      double Va = 0, Vb = 0;
+      if (!s.qvdw_flag) {
+        double r6 = 1.0 / r6_hc;
+        double aLJ = typ(i).avdw[iLJ - 1];
+        double bLJ = typ(i).bvdw[iLJ - 1];
+        Va = aLJ * s.alj_solv[ip][iLJ - 1] * r6 * r6;
+        Vb = bLJ * s.blj_solv[ip][iLJ - 1] * r6;
+      }
+      double dv = 0;
+      for (int st = 0; st < ns; ++st) {
+        if (s.qvdw_flag) {
+          const int qt = s.qiac[iq * ns + st];
+          double aLJ = s.qavdw[3 * qt];
+          double bLJ = s.qbvdw[3 * qt];
+          r6_sc = r6_hc + sc(iq, iLJ - 1, st);
+          Va = aLJ * s.alj_solv[ip][iLJ - 1] / (r6_sc * r6_sc);
+          Vb = bLJ * s.blj_solv[ip][iLJ - 1] / (r6_sc);
+        }
+        double Vel = s.chg_solv[ip] * s.qcrg[iq * ns + st] * r;
+        EQ[st].qw_el = EQ[st].qw_el + Vel;
+        EQ[st].qw_vdw = EQ[st].qw_vdw + Va - Vb;
+        dv = dv + r2 * (-Vel - ((12.0 * Va - 6.0 * Vb) * (r6_hc / r6_sc))) * s.lambda[st];
+      }
+      d[i3] = d[i3] - dv * dx1; d[i3 + 1] = d[i3 + 1] - dv * dx2; d[i3 + 2] = d[i3 + 2] - dv * dx3;
+      d[j3] = d[j3] + dv * dx1; d[j3 + 1] = d[j3 + 1] + dv * dx2; d[j3 + 2] = d[j3 + 2] + dv * dx3;
     }
   }
 }

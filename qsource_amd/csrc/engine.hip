@@ -23,7 +23,7 @@ namespace qgpu {
 __global__ void k_cgp_centers(const DevSys*);
 __global__ void k_x_to_f32(const DevSys*);
 template <typename P, int MAPPED> __global__ void k_water_lists(const DevSys*, const int2*);
-template <typename P, int MAPPED, int PF, int W> __global__ void k_water_forces(const DevSys*, const int2*);
+template <typename P, int MAPPED, int PF, int W, int AA> __global__ void k_water_forces(const DevSys*, const int2*);
 __global__ void k_watpol_theta(const DevSys*);
 __global__ void k_watpol_force(const DevSys*);
 __global__ void k_integrate(const DevSys*);
@@ -109,6 +109,16 @@ Batch::Batch(const std::vector<std::string>& inputs, int precision, int device) 
   if (inputs.empty()) throw std::runtime_error("no instances");
   if (precision != 0 && precision != 1) throw std::runtime_error("precision must be QGPU_FP64 or QGPU_MIXED");
   precision_ = precision;
+  try {
+    init(inputs, device);
+  } catch (...) {
+    // a constructor that throws runs no destructor: free what was made
+    release();
+    throw;
+  }
+}
+
+void Batch::init(const std::vector<std::string>& inputs, int device) {
   HIPCHECK(hipSetDevice(device));
   HIPCHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   HIPCHECK(hipEventCreate(&ev0));
@@ -122,6 +132,16 @@ Batch::Batch(const std::vector<std::string>& inputs, int precision, int device) 
     max_nat_solute_ = std::max(max_nat_solute_, s->nat_solute);
     max_nstates_ = std::max(max_nstates_, s->nstates);
   }
+  // one water-force kernel serves the whole batch: SPC and all-atom solvents
+  // cannot share a batch
+  int stype = -1;
+  for (auto* s : setups_) {
+    if (s->nwat == 0) continue;
+    if (stype >= 0 && s->top.solvent_type != stype)
+      throw std::runtime_error("instances of a batch must share the solvent type (SPC or all-atom)");
+    stype = s->top.solvent_type;
+  }
+  allatom_ = stype == 1;
   nbcycle_ = setups_[0]->inp.nbcycle;
   for (auto* s : setups_)
     if (s->inp.nbcycle != nbcycle_ || nbcycle_ < 1)
@@ -204,8 +224,11 @@ int Batch::offdiag_series(int k, double* out, int cap) {
   return n;
 }
 
-Batch::~Batch() {
+Batch::~Batch() { release(); }
+
+void Batch::release() {
   if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  graph_exec_ = nullptr;
   for (auto& m : maps_)
     if (m.second.first) (void)hipFree(m.second.first);
   if (far_x_) (void)hipFree(far_x_);
@@ -216,10 +239,13 @@ Batch::~Batch() {
   if (d_sys_) (void)hipFree(d_sys_);
   if (d_step_) (void)hipFree(d_step_);
   if (stream_) {
-    (void)hipEventDestroy(ev0);
-    (void)hipEventDestroy(ev1);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
     (void)hipStreamDestroy(stream_);
   }
+  maps_.clear(); inst_.clear(); setups_.clear();
+  far_x_ = far_y_ = nullptr; far_frame_ = nullptr;
+  d_sys_ = nullptr; d_step_ = nullptr; stream_ = nullptr; ev0 = ev1 = nullptr;
 }
 
 void Batch::upload(int k) {
@@ -327,6 +353,22 @@ void Batch::upload(int k) {
     int lj = s.ljcod[s.iac[o] * s.natyps + s.iac[o]];
     D.aOO = s.alj_solv[0][lj - 1] * s.alj_solv[0][lj - 1];
     D.bOO = s.blj_solv[0][lj - 1] * s.blj_solv[0][lj - 1];
+    D.solv_allatom = t.solvent_type == 1 ? 1 : 0;
+    if (D.solv_allatom) {
+      // the kernels take one LJ code per position in the water: every water
+      // must carry the first water's atom types
+      for (int w = 1; w < s.nwat; ++w)
+        for (int k = 0; k < 3; ++k)
+          if (s.iac[o + 3 * w + k] != s.iac[o + k])
+            throw std::runtime_error("all-atom solvent: waters with differing atom types are not supported");
+      for (int la = 0; la < 3; ++la) {
+        const int c = s.ljcod[s.iac[o + la] * s.natyps + s.iac[o]] - 1;
+        for (int ka = 0; ka < 3; ++ka) {
+          D.aww[3 * la + ka] = s.alj_solv[la][c] * s.alj_solv[ka][c];
+          D.bww[3 * la + ka] = s.blj_solv[la][c] * s.blj_solv[ka][c];
+        }
+      }
+    }
     // standard waters: one charge group of (O, H1, H2) in this order with the
     // solvent charges and no Q atom; the LRF gather then reads them directly
     bool std_w = true;
@@ -372,7 +414,8 @@ void Batch::upload(int k) {
   D.theta = I->alloc<double>(s.nwat);
   D.shell_of = I->alloc<int>(s.nwat);
   D.fk_fix = s.inp.fk_fix; D.fk_pshell = s.inp.fk_pshell; D.freeze = s.inp.freeze;
-  for (int k2 = 0; k2 < 2; ++k2) { D.alj_O[k2] = s.alj_solv[0][k2]; D.blj_O[k2] = s.blj_solv[0][k2]; }
+  for (int a = 0; a < 3; ++a)
+    for (int k2 = 0; k2 < 2; ++k2) { D.alj_w[a][k2] = s.alj_solv[a][k2]; D.blj_w[a][k2] = s.blj_solv[a][k2]; }
   // Q atoms: static Q-solute and Q-water sets as nbqplist/nbqwlist build
   // them at the first list update (src/md.f90:10215-10322, 10464-10549)
   D.iqseq = I->upload(s.iqseq);
@@ -400,7 +443,7 @@ void Batch::upload(int k) {
   {
     const int nq = s.nqat, nsl = s.nat_solute;
     auto qconn = [&](int st, int iq, int a) { return s.qconn[((size_t)st * nq + iq) * nsl + a]; };
-    std::vector<signed char> qp_lj((size_t)std::max(1, nsl * nq), 0), qw_lj(std::max(1, nq), 0);
+    std::vector<signed char> qp_lj((size_t)std::max(1, nsl * nq), 0), qw_lj(std::max(1, 3 * nq), 0);
     std::vector<int> qp_atoms, qw_waters;
     const double rcq2 = s.inp.rcq * s.inp.rcq;
     auto dist2c = [&](int a) {
@@ -433,8 +476,10 @@ void Batch::upload(int k) {
         qw_waters.push_back(iw);
         flags[o] |= F_QW;
       }
-      for (int iq = 0; iq < nq; ++iq)
-        qw_lj[iq] = (signed char)s.ljcod[s.iac[s.nat_solute] * s.natyps + s.iac[s.iqseq[iq]]];
+      // nbqwlist's code per water atom: ljcod(iac(water atom ip), iac(Q atom))
+      for (int iq = 0; iq < nq && s.nwat > 0; ++iq)
+        for (int ip = 0; ip < 3; ++ip)
+          qw_lj[3 * iq + ip] = (signed char)s.ljcod[s.iac[s.nat_solute + ip] * s.natyps + s.iac[s.iqseq[iq]]];
       HIPCHECK(hipMemcpy(const_cast<unsigned char*>(D.flags), flags.data(), N, hipMemcpyHostToDevice));
     }
     D.qp_lj = I->upload(qp_lj);
@@ -822,10 +867,14 @@ void Batch::launch_tuned(int kernel, const LaunchShape& sh) {
   // water forces: the FP32 build fitted to 6 or 4 wavefronts per SIMD
 #define QGPU_WF(PP, M, F)                                                                                 \
   do {                                                                                                    \
-    if (sh.occ && sizeof(PP) == 4)                                                                        \
-      hipLaunchKernelGGL((k_water_forces<PP, M, F, 4>), grid, dim3(blk), 0, stream_, d_sys_, map);        \
+    if (sh.occ && sizeof(PP) == 4 && allatom_)                                                            \
+      hipLaunchKernelGGL((k_water_forces<PP, M, F, 4, 1>), grid, dim3(blk), 0, stream_, d_sys_, map);     \
+    else if (sh.occ && sizeof(PP) == 4)                                                                   \
+      hipLaunchKernelGGL((k_water_forces<PP, M, F, 4, 0>), grid, dim3(blk), 0, stream_, d_sys_, map);     \
+    else if (allatom_)                                                                                    \
+      hipLaunchKernelGGL((k_water_forces<PP, M, F, 6, 1>), grid, dim3(blk), 0, stream_, d_sys_, map);     \
     else                                                                                                  \
-      hipLaunchKernelGGL((k_water_forces<PP, M, F, 6>), grid, dim3(blk), 0, stream_, d_sys_, map);        \
+      hipLaunchKernelGGL((k_water_forces<PP, M, F, 6, 0>), grid, dim3(blk), 0, stream_, d_sys_, map);     \
   } while (0)
   switch (kernel) {
     case TK_WATER_LISTS: QGPU_LAUNCH(k_water_lists); break;

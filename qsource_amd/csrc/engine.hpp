@@ -117,6 +117,11 @@ struct DevSys {
   const float* swp_f;
   const unsigned char* swp_zero;
   double aOO, bOO;               // alj_solv(1,iLJ)^2, blj_solv(1,iLJ)^2 for iLJ=ljcod(O,O)
+  // all-atom solvent LJ (topology solvent type 1): LJ between every atom
+  // pair of two waters, A of pair (la, ka) = aLJ_solv(la,iLJ) aLJ_solv(ka,iLJ)
+  // with iLJ = ljcod(atom la, water O) (src/md.f90:15291-15300)
+  int solv_allatom;
+  double aww[9], bww[9];
   double rwat, fk_wsphere, dwmz, awmz, fkwpol;
   double xwcent[3];
   int wpol_restr;
@@ -131,8 +136,8 @@ struct DevSys {
   const double* qcrg;            // nqat x nstates
   const double* lambda;
   const signed char* qp_lj;      // nat_solute x nqat LJ code of the (Q, solute atom) pair
-  const signed char* qw_lj;      // nqat: LJ code of the (water O, Q atom) pair
-  double alj_O[2], blj_O[2];     // aLJ_solv(1,:), bLJ_solv(1,:) of the water O
+  const signed char* qw_lj;      // nqat x 3: LJ code of the (water atom ip, Q atom) pair
+  double alj_w[3][2], blj_w[3][2]; // aLJ_solv(ip,:), bLJ_solv(ip,:) of the three water atoms
   int nqp_atoms, nqw_waters;
   const int* qp_atoms;           // solute atoms in the Q-solute list, ascending
   const int* qw_waters;          // waters in the Q-water list, ascending
@@ -248,6 +253,8 @@ class Batch {
  public:
   Batch(const std::vector<std::string>& inputs, int precision, int device);
   ~Batch();
+  Batch(const Batch&) = delete;
+  Batch& operator=(const Batch&) = delete;
   int size() const { return (int)inst_.size(); }
   void step(int nsteps);
   void make_lists();
@@ -265,7 +272,7 @@ class Batch {
   const qsys::Setup& setup(int k) const { return *setups_[k]; }
   hipStream_t stream() const { return stream_; }
   // timing helpers
-  hipEvent_t ev0, ev1;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
   // energy-file frames recorded on the device every iene steps
   void set_energy_interval(int iene, int max_frames);
@@ -314,11 +321,14 @@ class Batch {
   hipEvent_t pev_[2 * 13] = {};
   int pused_[13] = {};
   struct Inst;
+  void init(const std::vector<std::string>& inputs, int device);
+  void release();
   void one_step();
   void launch_cycle_graph();
   void sync_sys();
   void refresh_xf();
   int nbcycle_ = 10;
+  bool allatom_ = false;   // all-atom solvent LJ (topology solvent type 1)
   int iene_ = 0;
   bool use_graph_ = true;
   hipGraphExec_t graph_exec_ = nullptr;

@@ -254,7 +254,8 @@ Setup prepare(const std::string& input_path) {
   MDInput& in = s.inp;
   s.top = read_topology(join(dir, in.top_file));
   Topology& t = s.top;
-  if (t.solvent_type != 0) throw std::runtime_error("only SPC-type solvent (solvent_type 0) is supported");
+  if (t.solvent_type != 0 && t.solvent_type != 1)
+    throw std::runtime_error("only SPC (solvent_type 0) and 3-atom all-LJ (1) solvents are supported");
   if (t.ivdw_rule != 1) throw std::runtime_error("only the geometric vdW rule is supported");
   if (t.iuse_switch_atom != 1) throw std::runtime_error("only switching-atom charge-group cut-offs are supported");
   s.natom = t.nat_pro;
@@ -332,6 +333,7 @@ Setup prepare(const std::string& input_path) {
   std::vector<std::pair<int, int>> s_soft_pairs;
   std::vector<FepData::ElScale> s_el_scale;
   std::vector<FepData::ExSpec> s_exspec;
+  std::vector<FepData::QShake> s_qshake;
   if (!in.fep_file.empty()) {
     FepData f = read_fep(join(dir, in.fep_file), s.natom);
     s.nstates = f.nstates;
@@ -418,7 +420,7 @@ Setup prepare(const std::string& input_path) {
         else t.listexlong.push_back({e.i, e.j});
       }
     }
-    s_soft_pairs = f.soft_pairs; s_el_scale = f.el_scale; s_exspec = f.exspec;
+    s_soft_pairs = f.soft_pairs; s_el_scale = f.el_scale; s_exspec = f.exspec; s_qshake = f.qshake;
   } else if (!in.lambdas.empty()) {
     throw std::runtime_error("[lambdas] given without an FEP file");
   }
@@ -515,6 +517,20 @@ Setup prepare(const std::string& input_path) {
         mols[mol].bond.push_back({ia, ja, t.bondlib[b.cod - 1].b0 * t.bondlib[b.cod - 1].b0});
         b.cod = -1;
       }
+    }
+    // extra constraints of the FEP file, at the lambda-weighted distance:
+    // a pair already constrained is overwritten in place, otherwise appended
+    // to the molecule of its first atom (src/md.f90:4328-4354)
+    for (const auto& q : s_qshake) {
+      int m = 0;
+      while (q.i >= ist[m + 1]) ++m;
+      double d = 0.0;
+      for (int st = 0; st < s.nstates; ++st) d = d + s.lambda[st] * q.dist[st];
+      ShakeBond nb{q.i, q.j, d * d};
+      bool found = false;
+      for (auto& c : mols[m].bond)
+        if ((c.i == q.i && c.j == q.j) || (c.i == q.j && c.j == q.i)) { c = nb; found = true; break; }
+      if (!found) mols[m].bond.push_back(nb);
     }
     int solute_shake = 0;
     for (int m = 0; m < s.nmol - s.nwat; ++m) solute_shake += (int)mols[m].bond.size();

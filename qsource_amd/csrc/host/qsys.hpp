@@ -167,6 +167,9 @@ struct FepData {
   struct ExSpec { int i, j; std::vector<char> flag; };
   std::vector<ExSpec> exspec;
   bool qq_lib_charges = false;
+  // [shake_constraints]: atom pair (0-based) and distance per state
+  struct QShake { int i, j; std::vector<double> dist; };
+  std::vector<QShake> qshake;
 };
 
 struct ShakeBond { int i, j; double dist2; };

@@ -673,7 +673,7 @@ FepData read_fep(const std::string& path, int natom) {
   int nq = (int)f.iqseq.size();
   f.qcrg.assign(nq, std::vector<double>(f.nstates, 0.0));
   if (S.get("FEP", "qq_use_library_charges", v)) parse_onoff(v, f.qq_lib_charges);
-  for (const char* unsup : {"SHAKE_CONSTRAINTS", "MONITOR_GROUPS", "MONITOR_GROUP_PAIRS"}) {
+  for (const char* unsup : {"MONITOR_GROUPS", "MONITOR_GROUP_PAIRS"}) {
     if (S.has(unsup) && !S.sec.at(unsup).empty())
       throw std::runtime_error(std::string("FEP section [") + unsup + "] is not supported");
   }
@@ -766,6 +766,14 @@ FepData read_fep(const std::string& path, int natom) {
     FepData::ElScale e{a - 1, b - 1, {}};
     for (int st = 0; st < f.nstates; ++st) e.scale.push_back(to_d(tk[2 + st]));
     f.el_scale.push_back(e);
+  }
+  // [shake_constraints] "i j distance(1..nstates)" (src/qatom.f90:1792-1806)
+  for (auto& tk : rows("SHAKE_CONSTRAINTS")) {
+    if ((int)tk.size() < 2 + f.nstates) throw std::runtime_error("bad [shake_constraints] line");
+    FepData::QShake q{(int)to_i(tk[0]) - 1, (int)to_i(tk[1]) - 1, {}};
+    if (q.i < 0 || q.i >= natom || q.j < 0 || q.j >= natom) throw std::runtime_error("invalid atom in [shake_constraints]");
+    for (int st = 0; st < f.nstates; ++st) q.dist.push_back(to_d(tk[2 + st]));
+    f.qshake.push_back(q);
   }
   // [excluded_pairs] "i j flag(1..nstates)", flags 0/1 (src/qatom.f90:891-918)
   for (auto& tk : rows("EXCLUDED_PAIRS")) {

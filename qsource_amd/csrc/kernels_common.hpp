@@ -130,17 +130,19 @@ __device__ __forceinline__ P qp_qvdw_state(const DevSys& S, int iq, int st, int 
   return r2i * (-Vel - ((P)12 * Va - (P)6 * Vb) * (r6h / r6s)) * (P)S.lambda[st];
 }
 
-// One state of the Q-atom / water-O pair with Q-atom types (nonbond_qw_spc,
-// src/md.f90:14495-14512): LJ code 1 of the Q type with the water O's
-// aLJ_solv(1, lo), soft-core column lo.  Returns dv (times lambda).
+// One state of the Q-atom / water-atom pair with Q-atom types
+// (nonbond_qw_spc src/md.f90:14495-14512 for the O, nonbond_qw_3atom
+// :14831-14838 for any atom ip of an all-atom solvent): LJ code 1 of the Q
+// type with the water atom's aLJ_solv(ip, lo), soft-core column lo.
+// Returns dv (times lambda).
 template <typename P>
-__device__ __forceinline__ P qw_qvdw_state(const DevSys& S, int iq, int st, int lo, P r2i, P ri, P r6h,
+__device__ __forceinline__ P qw_qvdw_state(const DevSys& S, int iq, int st, int ip, int lo, P r2i, P ri, P r6h,
                                            P& Vel, P& Va, P& Vb) {
   const int qt = S.qiac[iq * S.nstates + st];
   const P r6s = r6h + (P)sc_alpha(S, iq, lo - 1, st);
-  Va = (P)S.qavdw[3 * qt] * (P)S.alj_O[lo - 1] / (r6s * r6s);
-  Vb = (P)S.qbvdw[3 * qt] * (P)S.blj_O[lo - 1] / r6s;
-  Vel = (P)S.chg_solv[0] * (P)S.qcrg[iq * S.nstates + st] * ri;
+  Va = (P)S.qavdw[3 * qt] * (P)S.alj_w[ip][lo - 1] / (r6s * r6s);
+  Vb = (P)S.qbvdw[3 * qt] * (P)S.blj_w[ip][lo - 1] / r6s;
+  Vel = (P)S.chg_solv[ip] * (P)S.qcrg[iq * S.nstates + st] * ri;
   return r2i * (-Vel - ((P)12 * Va - (P)6 * Vb) * (r6h / r6s)) * (P)S.lambda[st];
 }
 

@@ -446,9 +446,10 @@ __global__ __launch_bounds__(256) void k_q_forces(const DevSys* __restrict__ sys
       e[st][1] = e[st][1] + (double)Va - (double)Vb;
     }
   }
-  const int lo = S.qw_lj[iq];  // LJ code of the (water O, Q atom) pair
+  const int lo = S.qw_lj[3 * iq];  // LJ code of the (water O, Q atom) pair
   const P aO = (P)S.avdw[ti * 3 + lo - 1], bO = (P)S.bvdw[ti * 3 + lo - 1];
-  const P aS = (P)S.alj_O[lo - 1], bS = (P)S.blj_O[lo - 1];
+  const P aS = (P)S.alj_w[0][lo - 1], bS = (P)S.blj_w[0][lo - 1];
+  const bool allatom = S.solv_allatom != 0;
   const P cO = (P)S.chg_solv[0];
   for (int k = chunk * 64 + lane; k < S.nqw_waters; k += stride) {
     const int jo = S.nat_solute + 3 * S.qw_waters[k];
@@ -466,7 +467,7 @@ __global__ __launch_bounds__(256) void k_q_forces(const DevSys* __restrict__ sys
       for (int st = 0; st < NS; ++st) {
         if (st >= ns) continue;
         P Vel;
-        dv = dv + qw_qvdw_state<P>(S, iq, st, lo, r2, r, r6_hc, Vel, Va, Vb);
+        dv = dv + qw_qvdw_state<P>(S, iq, st, 0, lo, r2, r, r6_hc, Vel, Va, Vb);
         e[st][2] = e[st][2] + (double)Vel;
         e[st][3] = e[st][3] + (double)Va - (double)Vb;
       }
@@ -487,15 +488,45 @@ __global__ __launch_bounds__(256) void k_q_forces(const DevSys* __restrict__ sys
       const int ja = jo + ip;
       P ex1 = (P)(S.x[3 * ja] - xq0), ex2 = (P)(S.x[3 * ja + 1] - xq1), ex3 = (P)(S.x[3 * ja + 2] - xq2);
       P rr;
-      P q2 = inv_r2(ex1 * ex1 + ex2 * ex2 + ex3 * ex3, rr);
+      const P h2 = ex1 * ex1 + ex2 * ex2 + ex3 * ex3;
+      P q2 = inv_r2(h2, rr);
       const P ch = (P)S.chg_solv[ip];
       P dvh = 0;
+      if (allatom) {
+        // all-atom solvent: the H atoms carry LJ too, with their own LJ code
+        // (nonbond_qw_3atom, src/md.f90:14801-14858)
+        const int lh = S.qw_lj[3 * iq + ip];
+        const P h6_hc = h2 * h2 * h2;
+        if (S.qvdw) {
 #pragma unroll
-      for (int st = 0; st < NS; ++st) {
-        if (st >= ns) continue;
-        P Vel = ch * qs[st] * rr;
-        e[st][2] = e[st][2] + (double)Vel;
-        dvh = dvh - q2 * Vel * lam[st];
+          for (int st = 0; st < NS; ++st) {
+            if (st >= ns) continue;
+            P Vel;
+            dvh = dvh + qw_qvdw_state<P>(S, iq, st, ip, lh, q2, rr, h6_hc, Vel, Va, Vb);
+            e[st][2] = e[st][2] + (double)Vel;
+            e[st][3] = e[st][3] + (double)Va - (double)Vb;
+          }
+        } else {
+          const P h6 = (P)1 / h6_hc;
+          const P Vah = (P)S.avdw[ti * 3 + lh - 1] * (P)S.alj_w[ip][lh - 1] * h6 * h6;
+          const P Vbh = (P)S.bvdw[ti * 3 + lh - 1] * (P)S.blj_w[ip][lh - 1] * h6;
+#pragma unroll
+          for (int st = 0; st < NS; ++st) {
+            if (st >= ns) continue;
+            P Vel = ch * qs[st] * rr;
+            dvh = dvh + q2 * (-Vel - ((P)12 * Vah - (P)6 * Vbh) * (h6_hc / h6_hc)) * lam[st];
+            e[st][2] = e[st][2] + (double)Vel;
+            e[st][3] = e[st][3] + (double)Vah - (double)Vbh;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int st = 0; st < NS; ++st) {
+          if (st >= ns) continue;
+          P Vel = ch * qs[st] * rr;
+          e[st][2] = e[st][2] + (double)Vel;
+          dvh = dvh - q2 * Vel * lam[st];
+        }
       }
       g0 = g0 - (double)(dvh * ex1);
       g1 = g1 - (double)(dvh * ex2);

@@ -157,8 +157,11 @@ template __global__ void k_water_lists<float, 1>(const DevSys*, const int2*);
 // coordinates and FMA pair arithmetic, each partner's contribution
 // accumulated in FP64).
 // W: wavefronts per SIMD the FP32 build is fitted to (6: registers spill to
-// scratch; 4: no spills); which is faster is measured by the launch tuner
-template <typename P, int MAPPED, int PF, int W>
+// scratch; 4: no spills); which is faster is measured by the launch tuner.
+// AA: all-atom solvent LJ (nonbond_3atomsolvent :15238-15333 and
+// nonbond_qw_3atom :14749-14874): LJ on all nine atom pairs of two waters
+// and between each Q atom and every water atom.
+template <typename P, int MAPPED, int PF, int W, int AA>
 __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(const DevSys* __restrict__ sys,
                                                                               const int2* __restrict__ map) {
   const DevSys* Sp;
@@ -185,6 +188,9 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(co
   for (int la = 0; la < 3; ++la)
     for (int ip = 0; ip < 3; ++ip) qq[3 * la + ip] = qs3[la] * qs3[ip];
   const P aOO = (P)S.aOO, bOO = (P)S.bOO;
+  P aww[9], bww[9];
+  if constexpr (AA != 0)
+    for (int k = 0; k < 9; ++k) { aww[k] = (P)S.aww[k]; bww[k] = (P)S.bww[k]; }
   // water-water
   const int nn = active ? S.ww_cnt[iw] : 0;
   const gconst<int> row = gptr(S.ww_nb) + (size_t)iw * S.ww_cap;
@@ -201,12 +207,12 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(co
         const P r2 = inv_r2(r2sq, r);
         const P Vel = qq[3 * la + ip] * r;
         P dv = r2 * (-Vel);
-        if (la == 0 && ip == 0) {
+        if (AA != 0 || (la == 0 && ip == 0)) {
           P r6;
           if constexpr (sizeof(P) == 8) r6 = (P)1 / (r2sq * r2sq * r2sq);
           else r6 = r2 * r2 * r2;
-          const P Va = aOO * r6 * r6;
-          const P Vb = bOO * r6;
+          const P Va = (AA != 0 ? aww[3 * la + ip] : aOO) * r6 * r6;
+          const P Vb = (AA != 0 ? bww[3 * la + ip] : bOO) * r6;
           vdw = vdw + Va - Vb;
           dv = dv + r2 * ((P)-12 * Va + (P)6 * Vb);
         }
@@ -301,7 +307,7 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(co
     for (int iq = sub; iq < S.nqat; iq += 8) {
       const int qi = S.iqseq[iq];
       const P xq0 = X[3 * qi], xq1 = X[3 * qi + 1], xq2 = X[3 * qi + 2];
-      const int lo = S.qw_lj[iq];
+      const int lo = S.qw_lj[3 * iq];
       const int ti = S.iac[qi];
       const P dx1 = xi[0] - xq0, dx2 = xi[1] - xq1, dx3 = xi[2] - xq2;
       const P r2sq = pnorm2(dx1, dx2, dx3);
@@ -313,11 +319,11 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(co
       if (S.qvdw) {
         for (int st = 0; st < ns; ++st) {
           P Vel, Va, Vb;
-          dv = dv + qw_qvdw_state<P>(S, iq, st, lo, r2, r, r6_hc, Vel, Va, Vb);
+          dv = dv + qw_qvdw_state<P>(S, iq, st, 0, lo, r2, r, r6_hc, Vel, Va, Vb);
         }
       } else {
-        const P Va = (P)S.avdw[ti * 3 + lo - 1] * (P)S.alj_O[lo - 1] * r6 * r6;
-        const P Vb = (P)S.bvdw[ti * 3 + lo - 1] * (P)S.blj_O[lo - 1] * r6;
+        const P Va = (P)S.avdw[ti * 3 + lo - 1] * (P)S.alj_w[0][lo - 1] * r6 * r6;
+        const P Vb = (P)S.bvdw[ti * 3 + lo - 1] * (P)S.blj_w[0][lo - 1] * r6;
         for (int st = 0; st < ns; ++st) {
           const P Vel = (P)S.chg_solv[0] * (P)S.qcrg[iq * ns + st] * r;
           dv = dv + r2 * (-Vel - ((P)12 * Va - (P)6 * Vb) * (r6_hc / r6_hc)) * (P)S.lambda[st];
@@ -329,11 +335,32 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(co
       for (int ip = 1; ip < 3; ++ip) {
         const P ex1 = xi[3 * ip] - xq0, ex2 = xi[3 * ip + 1] - xq1, ex3 = xi[3 * ip + 2] - xq2;
         P rr;
-        const P q2 = inv_r2(pnorm2(ex1, ex2, ex3), rr);
+        const P h2 = pnorm2(ex1, ex2, ex3);
+        const P q2 = inv_r2(h2, rr);
         P dvh = 0;
-        for (int st = 0; st < ns; ++st) {
-          const P Vel = (P)S.chg_solv[ip] * (P)S.qcrg[iq * ns + st] * rr;
-          dvh = dvh - q2 * Vel * (P)S.lambda[st];
+        if constexpr (AA != 0) {
+          // the H carries LJ too, with its own LJ code
+          const int lh = S.qw_lj[3 * iq + ip];
+          const P h6_hc = h2 * h2 * h2;
+          if (S.qvdw) {
+            for (int st = 0; st < ns; ++st) {
+              P Vel, Va, Vb;
+              dvh = dvh + qw_qvdw_state<P>(S, iq, st, ip, lh, q2, rr, h6_hc, Vel, Va, Vb);
+            }
+          } else {
+            const P h6 = (P)1 / h6_hc;
+            const P Va = (P)S.avdw[ti * 3 + lh - 1] * (P)S.alj_w[ip][lh - 1] * h6 * h6;
+            const P Vb = (P)S.bvdw[ti * 3 + lh - 1] * (P)S.blj_w[ip][lh - 1] * h6;
+            for (int st = 0; st < ns; ++st) {
+              const P Vel = (P)S.chg_solv[ip] * (P)S.qcrg[iq * ns + st] * rr;
+              dvh = dvh + q2 * (-Vel - ((P)12 * Va - (P)6 * Vb) * (h6_hc / h6_hc)) * (P)S.lambda[st];
+            }
+          }
+        } else {
+          for (int st = 0; st < ns; ++st) {
+            const P Vel = (P)S.chg_solv[ip] * (P)S.qcrg[iq * ns + st] * rr;
+            dvh = dvh - q2 * Vel * (P)S.lambda[st];
+          }
         }
         g[3 * ip] = g[3 * ip] + (double)(dvh * ex1);
         g[3 * ip + 1] = g[3 * ip + 1] + (double)(dvh * ex2);
@@ -392,18 +419,30 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(co
     ew[0] = e_el; ew[1] = e_vdw; ew[2] = e_lrf; ew[3] = e_rad;
   }
 }
-template __global__ void k_water_forces<double, 0, 0, 6>(const DevSys*, const int2*);
-template __global__ void k_water_forces<double, 0, 1, 6>(const DevSys*, const int2*);
-template __global__ void k_water_forces<double, 1, 0, 6>(const DevSys*, const int2*);
-template __global__ void k_water_forces<double, 1, 1, 6>(const DevSys*, const int2*);
-template __global__ void k_water_forces<float, 0, 0, 6>(const DevSys*, const int2*);
-template __global__ void k_water_forces<float, 0, 0, 4>(const DevSys*, const int2*);
-template __global__ void k_water_forces<float, 0, 1, 6>(const DevSys*, const int2*);
-template __global__ void k_water_forces<float, 0, 1, 4>(const DevSys*, const int2*);
-template __global__ void k_water_forces<float, 1, 0, 6>(const DevSys*, const int2*);
-template __global__ void k_water_forces<float, 1, 0, 4>(const DevSys*, const int2*);
-template __global__ void k_water_forces<float, 1, 1, 6>(const DevSys*, const int2*);
-template __global__ void k_water_forces<float, 1, 1, 4>(const DevSys*, const int2*);
+template __global__ void k_water_forces<double, 0, 0, 6, 0>(const DevSys*, const int2*);
+template __global__ void k_water_forces<double, 0, 0, 6, 1>(const DevSys*, const int2*);
+template __global__ void k_water_forces<double, 0, 1, 6, 0>(const DevSys*, const int2*);
+template __global__ void k_water_forces<double, 0, 1, 6, 1>(const DevSys*, const int2*);
+template __global__ void k_water_forces<double, 1, 0, 6, 0>(const DevSys*, const int2*);
+template __global__ void k_water_forces<double, 1, 0, 6, 1>(const DevSys*, const int2*);
+template __global__ void k_water_forces<double, 1, 1, 6, 0>(const DevSys*, const int2*);
+template __global__ void k_water_forces<double, 1, 1, 6, 1>(const DevSys*, const int2*);
+template __global__ void k_water_forces<float, 0, 0, 6, 0>(const DevSys*, const int2*);
+template __global__ void k_water_forces<float, 0, 0, 6, 1>(const DevSys*, const int2*);
+template __global__ void k_water_forces<float, 0, 0, 4, 0>(const DevSys*, const int2*);
+template __global__ void k_water_forces<float, 0, 0, 4, 1>(const DevSys*, const int2*);
+template __global__ void k_water_forces<float, 0, 1, 6, 0>(const DevSys*, const int2*);
+template __global__ void k_water_forces<float, 0, 1, 6, 1>(const DevSys*, const int2*);
+template __global__ void k_water_forces<float, 0, 1, 4, 0>(const DevSys*, const int2*);
+template __global__ void k_water_forces<float, 0, 1, 4, 1>(const DevSys*, const int2*);
+template __global__ void k_water_forces<float, 1, 0, 6, 0>(const DevSys*, const int2*);
+template __global__ void k_water_forces<float, 1, 0, 6, 1>(const DevSys*, const int2*);
+template __global__ void k_water_forces<float, 1, 0, 4, 0>(const DevSys*, const int2*);
+template __global__ void k_water_forces<float, 1, 0, 4, 1>(const DevSys*, const int2*);
+template __global__ void k_water_forces<float, 1, 1, 6, 0>(const DevSys*, const int2*);
+template __global__ void k_water_forces<float, 1, 1, 6, 1>(const DevSys*, const int2*);
+template __global__ void k_water_forces<float, 1, 1, 4, 0>(const DevSys*, const int2*);
+template __global__ void k_water_forces<float, 1, 1, 4, 1>(const DevSys*, const int2*);
 
 // ---------------------------------------------------------------------------
 // Water polarisation restraint (watpol, src/md.f90:18372-18541), part 1:

@@ -34,6 +34,7 @@ THETA_HOH = 104.52
 Q_O, Q_H = -0.834, 0.417
 SIGMA_O, EPS_O = 3.1506, 0.1521
 MASS_O, MASS_H = 15.9994, 1.008
+SIGMA_H, EPS_H = 0.4, 0.046
 DENSITY = 0.0334  # molecules / A^3
 MAX_NBR_RANGE = 25  # src/topo.f90:36
 
@@ -118,6 +119,11 @@ class System:
     el_scale: Optional[list] = None
     excluded_pairs: Optional[list] = None
     qq_lib_charges: bool = False
+    # [shake_constraints] rows (i, j, distance per state), 1-based
+    shake_constraints: Optional[list] = None
+    # topology solvent type: 0 SPC-like (LJ on the water O only), 1 all-atom
+    # LJ on every solvent atom (TIP3P-like H types)
+    solvent_type: int = 0
 
     @property
     def natom(self) -> int:
@@ -355,7 +361,8 @@ def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
                     q_sd: float = 0.3, q_types: bool = False,
                     softcore: Optional[float] = None, n_offdiag: int = 0,
                     impropers: int = 0, q_bonded: bool = False, restraints: bool = False,
-                    fep_pairs: bool = False, qq_lib_charges: bool = False) -> System:
+                    fep_pairs: bool = False, qq_lib_charges: bool = False,
+                    q_shake: bool = False, solvent_allatom: bool = False) -> System:
     """A TIP3P sphere of the given radius around an optional branched solute
     (charges N(0,q_sd) neutralised per 4-atom charge group, sigma~U[2.5,3.8],
     eps~U[0.05,0.2] quantised into at most n_types atom types), with an
@@ -368,7 +375,9 @@ def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
     atom and state; n_offdiag: EVB couplings A*exp(-mu r) with
     A~U[10,60], mu~U[0,0.5] between random state pairs and Q-atom pairs;
     impropers: 1 adds harmonic (GROMOS) and 2 periodic (AMBER) impropers on
-    the solute's branch points."""
+    the solute's branch points; solvent_allatom: the water H gets the LJ
+    type of CHARMM's TIP3P H (sigma 0.4 A, eps 0.046 kcal/mol) and the
+    topology declares solvent type 1 (all-atom solvent LJ)."""
     rng = np.random.default_rng(seed)
     # --- solute
     if n_solute > 0:
@@ -395,8 +404,9 @@ def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
     # --- atom types: water O, water H, then quantised solute types
     masses = [MASS_O, MASS_H]
     a_O, b_O = lj_sqrt_ab(SIGMA_O, EPS_O)
-    avdw = [[a_O] * 3, [0.0] * 3]
-    bvdw = [[b_O] * 3, [0.0] * 3]
+    a_H, b_H = lj_sqrt_ab(SIGMA_H, EPS_H) if solvent_allatom else (0.0, 0.0)
+    avdw = [[a_O] * 3, [a_H] * 3]
+    bvdw = [[b_O] * 3, [b_H] * 3]
     tac = ["OW", "HW"]
     iac = np.zeros(N, dtype=np.int64)
     crg = np.zeros(N)
@@ -562,6 +572,22 @@ def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
     if fep_pairs and sysm.qatoms is not None and n_states >= 2:
         _fep_pairs(sysm, rng, n_states)
     sysm.qq_lib_charges = bool(qq_lib_charges and sysm.qatoms is not None)
+    if q_shake and sysm.qatoms is not None and n_states >= 2:
+        # a bonded Q pair (1.53 A in the first state, 2 % longer in the
+        # others) and a Q 1-3 pair at its current distance (+0.02 A later)
+        qs = set(int(a) for a in sysm.qatoms)
+        rows = []
+        for i, j, _ in sysm.bonds[:sysm.nbonds_solute]:
+            if int(i) in qs and int(j) in qs:
+                rows.append((int(i), int(j), [1.53] + [round(1.53 * 1.02, 3)] * (n_states - 1)))
+                break
+        for i, j, k, _ in sysm.angles[:sysm.nangles_solute]:
+            if int(i) in qs and int(k) in qs:
+                d = float(np.linalg.norm(sysm.x[int(i) - 1] - sysm.x[int(k) - 1]))
+                rows.append((int(i), int(k), [round(d, 3)] + [round(d + 0.02, 3)] * (n_states - 1)))
+                break
+        sysm.shake_constraints = rows
+    sysm.solvent_type = 1 if solvent_allatom else 0
     return sysm
 
 
@@ -875,7 +901,7 @@ def write_top(s: System, path: str) -> None:
     L += _fmt_list(list(s.tac), 8, "%-8s ")
     L.append("%8d = No. of SYBYL atom types:" % nt)
     L += _fmt_list(["" for _ in range(nt)], 13, "%-5s ")
-    L.append("%8d = solvent type (0=SPC,1=3-atom,2=general)" % 0)
+    L.append("%8d = solvent type (0=SPC,1=3-atom,2=general)" % s.solvent_type)
     L.append("%8.3f%8.3f = Exclusion, solvent radii" % (s.rexcl_o, s.rwat))
     L.append("%8.3f%8.3f%8.3f = Solute centre" % tuple(s.xpcent))
     L.append("%8.3f%8.3f%8.3f = Solvent centre" % tuple(s.xwcent))
@@ -926,6 +952,9 @@ def write_fep(s: System, path: str) -> None:
     if s.excluded_pairs:
         L += ["", "[excluded_pairs]"] + ["%d %d " % (i, j) + " ".join("%d" % f for f in fl)
                                          for i, j, fl in s.excluded_pairs]
+    if s.shake_constraints:
+        L += ["", "[shake_constraints]"] + ["%d %d " % (i, j) + " ".join("%.3f" % v for v in d)
+                                            for i, j, d in s.shake_constraints]
     if s.qbonds is not None:
         sec = [("bond_types", s.qbond_types, "change_bonds", s.qbonds),
                ("angle_types", s.qangle_types, "change_angles", s.qangles),

@@ -81,6 +81,25 @@ CASES = {
                              dict(steps=30, lambdas=[0.5, 0.5], stepsize=0.5, temperature=50.0,
                                   bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
                                   energy_interval=1)),
+    # FEP SHAKE constraints (lambda-weighted distances) with solute SHAKE on:
+    # one overwrites a shaken bond, one adds a 1-3 constraint
+    "c7_r12_q8s2_qshake_30": (dict(radius=12.0, n_solute=40, seed=2401, n_q=8, n_states=2, q_shake=True),
+                              dict(steps=30, lambdas=[0.6, 0.4], stepsize=0.5, temperature=50.0,
+                                   bath_coupling=2.0, initial_temperature=50.0, shake_solute=True,
+                                   energy="case.en", energy_interval=1)),
+    # all-atom solvent LJ (topology solvent type 1, TIP3P-like H types):
+    # LJ between every atom pair of two waters and between the Q atoms and
+    # every water atom, with fixed Q-atom LJ parameters or Q-atom types
+    "c8_r12_q8s2_allatom_30": (dict(radius=12.0, n_solute=40, seed=2501, n_q=8, n_states=2,
+                                    solvent_allatom=True),
+                               dict(steps=30, lambdas=[0.6, 0.4], stepsize=0.5, temperature=50.0,
+                                    bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
+                                    energy_interval=1)),
+    "c8_r12_q8s2_allatom_qvdw_30": (dict(radius=12.0, n_solute=40, seed=2502, n_q=8, n_states=2,
+                                         q_types=True, softcore=30.0, solvent_allatom=True),
+                                    dict(steps=30, lambdas=[0.6, 0.4], stepsize=0.5, temperature=50.0,
+                                         bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
+                                         energy_interval=1)),
     # a breaking bond with non-Q neighbours: topology torsions through it
     # become Q terms (qatom_load_fep's completion); periodic impropers
     "c6_r12_q6s2_qbreak_30": (dict(radius=12.0, n_solute=40, seed=2202, n_q=6, n_states=2, q_types=True,
@@ -136,7 +155,10 @@ def build_case(name, sysargs, inpargs, workdir):
 def main():
     if not os.path.exists(QDYN):
         sys.exit("reference qdyn5 not built: make -f oracle/Makefile.ref OUT=oracle/_ref")
+    only = set(sys.argv[1:])   # case names to (re)make; all when none given
     for name, (sysargs, inpargs) in CASES.items():
+        if only and name not in only:
+            continue
         with tempfile.TemporaryDirectory() as d:
             s, paths = build_case(name, sysargs, inpargs, d)
             r = subprocess.run([QDYN, name + ".inp"], cwd=d, stdout=subprocess.PIPE,

@@ -8,8 +8,15 @@ bitwise to reference qdyn on the same systems by test_oracle_reference.py):
 * the FEP file's Q-Q pair options: [soft_pairs] (exponential repulsion with
   Q-atom types), [el_scale], [excluded_pairs] (Q-Q in one state, Q/non-Q
   and non-Q/non-Q in all) and qq_use_library_charges
-  (nonbond_qq_lib_charges, src/md.f90:13640-13786): step 0 within 1e-12,
-  30 FP64 steps within 1e-9.
+  (nonbond_qq_lib_charges, src/md.f90:13640-13786), and the FEP file's
+  SHAKE constraints with solute SHAKE on (src/md.f90:4328-4354): step 0
+  within 1e-12, 30 FP64 steps within 1e-9;
+* an all-atom solvent (topology solvent type 1, TIP3P-like H with LJ:
+  nonbond_3atomsolvent and nonbond_qw_3atom, src/md.f90:15238-15333,
+  14749-14874) with fixed Q-atom LJ parameters and with Q-atom types: step 0
+  within 1e-12, 30 FP64 steps within 1e-9, mixed precision gradient RMS
+  error below 1e-5 of the gradient RMS; an SPC and an all-atom system in one
+  batch are refused.
 """
 import numpy as np
 import pytest
@@ -69,7 +76,9 @@ FEP_PAIRS = {
                  q_types=True, softcore=40.0),
     "qqlib": dict(radius=12.0, n_solute=40, seed=2301, n_q=8, n_states=2, impropers=1, fep_pairs=True,
                   qq_lib_charges=True),
+    "qshake": dict(radius=12.0, n_solute=40, seed=2401, n_q=8, n_states=2, q_shake=True),
 }
+EXTRA_INP = {"qshake": dict(shake_solute=True)}
 
 
 @pytest.mark.parametrize("name", sorted(FEP_PAIRS))
@@ -79,7 +88,8 @@ def test_fep_pair_options(tmp_path, name):
     steps within 1e-9."""
     _say("case: FEP pair options (%s), R=12 + 40-atom solute + 8 Q, FP64" % name)
     s = gensys.solvated_system(**FEP_PAIRS[name])
-    paths = gensys.write_system(s, str(tmp_path), name, steps=30, lambdas=[0.5, 0.5], **COOL)
+    paths = gensys.write_system(s, str(tmp_path), name, steps=30, lambdas=[0.5, 0.5], **COOL,
+                                **EXTRA_INP.get(name, {}))
     o = Oracle(paths["inp"])
     o.final_energy()
     e = qe.Engine([paths["inp"]], qe.QGPU_FP64)
@@ -103,3 +113,61 @@ def test_fep_pair_options(tmp_path, name):
     assert np.abs(xg - xo).max() <= 1e-9 * np.abs(xo).max()
     assert np.abs(vg - vo).max() <= 1e-9 * np.abs(vo).max()
     e.close()
+
+
+ALLATOM = {
+    "fixed": dict(radius=12.0, n_solute=40, seed=2501, n_q=8, n_states=2, solvent_allatom=True),
+    "qvdw": dict(radius=12.0, n_solute=40, seed=2502, n_q=8, n_states=2, q_types=True, softcore=30.0,
+                 solvent_allatom=True),
+}
+
+
+@pytest.mark.parametrize("name", sorted(ALLATOM))
+def test_allatom_solvent(tmp_path, name):
+    """LJ on every water atom: step 0 energies, per-state Q energies and
+    gradient within 1e-12 of the oracle, 30 FP64 steps within 1e-9, mixed
+    precision gradient within 1e-5 (RMS)."""
+    _say("case: all-atom solvent (%s), R=12 + 40-atom solute + 8 Q, FP64 and mixed" % name)
+    s = gensys.solvated_system(**ALLATOM[name])
+    assert s.solvent_type == 1
+    paths = gensys.write_system(s, str(tmp_path), name, steps=30, lambdas=[0.6, 0.4], **COOL)
+    o = Oracle(paths["inp"])
+    o.final_energy()
+    e = qe.Engine([paths["inp"]], qe.QGPU_FP64)
+    e.make_pair_lists()
+    e.pot_energy()
+    eo, eg = o.energies(), e.energies(0)
+    # the H LJ is present: the SPC formula would miss a visible part of ww_vdw
+    assert eo["ww_vdw"] != 0.0
+    for t in TERMS:
+        assert abs(eg[t] - eo[t]) <= 1e-12 * max(1.0, abs(eo[t])), (t, eg[t], eo[t])
+    qo = o.qenergies()
+    qg = np.array(e.qenergies(0)).reshape(qo.shape)
+    assert np.allclose(qg, qo, rtol=1e-12, atol=1e-11), (qg, qo)
+    do, dg = o.gradient(), e.gradient(0)
+    assert np.abs(dg - do).max() <= 1e-12 * np.abs(do).max()
+    e.close()
+    o = Oracle(paths["inp"])
+    o.run(30)
+    e = qe.Engine([paths["inp"]], qe.QGPU_FP64)
+    e.step(30)
+    xo, vo = o.state()
+    xg, vg = e.state(0)
+    assert np.abs(xg - xo).max() <= 1e-9 * np.abs(xo).max()
+    assert np.abs(vg - vo).max() <= 1e-9 * np.abs(vo).max()
+    e.close()
+    eo, eg, do, dg = _compare(paths, qe.QGPU_MIXED)
+    rms = np.sqrt(np.mean(do ** 2))
+    assert np.sqrt(np.mean((dg - do) ** 2)) <= 1e-5 * rms
+    for t in ("ww_vdw", "qx_vdw"):
+        assert abs(eg[t] - eo[t]) <= 1e-5 * max(1.0, abs(eo[t])), (t, eg[t], eo[t])
+
+
+def test_mixed_solvent_batch_refused(tmp_path):
+    _say("case: SPC and all-atom solvent in one batch (refused at set-up)")
+    a = gensys.solvated_system(radius=10.0, n_solute=0, seed=11)
+    b = gensys.solvated_system(radius=10.0, n_solute=0, seed=12, solvent_allatom=True)
+    pa = gensys.write_system(a, str(tmp_path), "spc", steps=1)
+    pb = gensys.write_system(b, str(tmp_path), "aa", steps=1)
+    with pytest.raises(RuntimeError, match="solvent type"):
+        qe.Engine([pa["inp"], pb["inp"]], qe.QGPU_FP64)
