@@ -123,6 +123,12 @@ void Batch::init(const std::vector<std::string>& inputs, int device) {
   HIPCHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   HIPCHECK(hipEventCreate(&ev0));
   HIPCHECK(hipEventCreate(&ev1));
+  HIPCHECK(hipStreamCreateWithFlags(&stream2_, hipStreamNonBlocking));
+  HIPCHECK(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
+  HIPCHECK(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
+  ls_ = stream_;
+  const char* br = std::getenv("QGPU_BRANCHES");
+  branches_ = !(br && br[0] == '0');
   for (const auto& f : inputs) setups_.push_back(new qsys::Setup(qsys::prepare(f)));
   for (auto* s : setups_) {
     max_nwat_ = std::max(max_nwat_, s->nwat);
@@ -238,6 +244,10 @@ void Batch::release() {
   for (auto* s : setups_) delete s;
   if (d_sys_) (void)hipFree(d_sys_);
   if (d_step_) (void)hipFree(d_step_);
+  if (stream2_) (void)hipStreamDestroy(stream2_);
+  if (ev_fork_) (void)hipEventDestroy(ev_fork_);
+  if (ev_join_) (void)hipEventDestroy(ev_join_);
+  stream2_ = nullptr; ev_fork_ = ev_join_ = nullptr;
   if (stream_) {
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -848,33 +858,33 @@ void Batch::launch_tuned(int kernel, const LaunchShape& sh) {
   const bool pf = sh.pf != 0;
 #define QGPU_LAUNCH(K)                                                                         \
   do {                                                                                         \
-    if (f64 && mp) hipLaunchKernelGGL((K<double, 1>), grid, dim3(blk), 0, stream_, d_sys_, map); \
-    else if (f64) hipLaunchKernelGGL((K<double, 0>), grid, dim3(blk), 0, stream_, d_sys_, map);  \
-    else if (mp) hipLaunchKernelGGL((K<float, 1>), grid, dim3(blk), 0, stream_, d_sys_, map);    \
-    else hipLaunchKernelGGL((K<float, 0>), grid, dim3(blk), 0, stream_, d_sys_, map);            \
+    if (f64 && mp) hipLaunchKernelGGL((K<double, 1>), grid, dim3(blk), 0, ls_, d_sys_, map); \
+    else if (f64) hipLaunchKernelGGL((K<double, 0>), grid, dim3(blk), 0, ls_, d_sys_, map);  \
+    else if (mp) hipLaunchKernelGGL((K<float, 1>), grid, dim3(blk), 0, ls_, d_sys_, map);    \
+    else hipLaunchKernelGGL((K<float, 0>), grid, dim3(blk), 0, ls_, d_sys_, map);            \
   } while (0)
 #define QGPU_LAUNCH_PF(K)                                                                              \
   do {                                                                                                 \
-    if (f64 && mp && pf) hipLaunchKernelGGL((K<double, 1, 1>), grid, dim3(blk), 0, stream_, d_sys_, map); \
-    else if (f64 && mp) hipLaunchKernelGGL((K<double, 1, 0>), grid, dim3(blk), 0, stream_, d_sys_, map);  \
-    else if (f64 && pf) hipLaunchKernelGGL((K<double, 0, 1>), grid, dim3(blk), 0, stream_, d_sys_, map);  \
-    else if (f64) hipLaunchKernelGGL((K<double, 0, 0>), grid, dim3(blk), 0, stream_, d_sys_, map);        \
-    else if (mp && pf) hipLaunchKernelGGL((K<float, 1, 1>), grid, dim3(blk), 0, stream_, d_sys_, map);    \
-    else if (mp) hipLaunchKernelGGL((K<float, 1, 0>), grid, dim3(blk), 0, stream_, d_sys_, map);          \
-    else if (pf) hipLaunchKernelGGL((K<float, 0, 1>), grid, dim3(blk), 0, stream_, d_sys_, map);          \
-    else hipLaunchKernelGGL((K<float, 0, 0>), grid, dim3(blk), 0, stream_, d_sys_, map);                  \
+    if (f64 && mp && pf) hipLaunchKernelGGL((K<double, 1, 1>), grid, dim3(blk), 0, ls_, d_sys_, map); \
+    else if (f64 && mp) hipLaunchKernelGGL((K<double, 1, 0>), grid, dim3(blk), 0, ls_, d_sys_, map);  \
+    else if (f64 && pf) hipLaunchKernelGGL((K<double, 0, 1>), grid, dim3(blk), 0, ls_, d_sys_, map);  \
+    else if (f64) hipLaunchKernelGGL((K<double, 0, 0>), grid, dim3(blk), 0, ls_, d_sys_, map);        \
+    else if (mp && pf) hipLaunchKernelGGL((K<float, 1, 1>), grid, dim3(blk), 0, ls_, d_sys_, map);    \
+    else if (mp) hipLaunchKernelGGL((K<float, 1, 0>), grid, dim3(blk), 0, ls_, d_sys_, map);          \
+    else if (pf) hipLaunchKernelGGL((K<float, 0, 1>), grid, dim3(blk), 0, ls_, d_sys_, map);          \
+    else hipLaunchKernelGGL((K<float, 0, 0>), grid, dim3(blk), 0, ls_, d_sys_, map);                  \
   } while (0)
   // water forces: the FP32 build fitted to 6 or 4 wavefronts per SIMD
 #define QGPU_WF(PP, M, F)                                                                                 \
   do {                                                                                                    \
     if (sh.occ && sizeof(PP) == 4 && allatom_)                                                            \
-      hipLaunchKernelGGL((k_water_forces<PP, M, F, 4, 1>), grid, dim3(blk), 0, stream_, d_sys_, map);     \
+      hipLaunchKernelGGL((k_water_forces<PP, M, F, 4, 1>), grid, dim3(blk), 0, ls_, d_sys_, map);     \
     else if (sh.occ && sizeof(PP) == 4)                                                                   \
-      hipLaunchKernelGGL((k_water_forces<PP, M, F, 4, 0>), grid, dim3(blk), 0, stream_, d_sys_, map);     \
+      hipLaunchKernelGGL((k_water_forces<PP, M, F, 4, 0>), grid, dim3(blk), 0, ls_, d_sys_, map);     \
     else if (allatom_)                                                                                    \
-      hipLaunchKernelGGL((k_water_forces<PP, M, F, 6, 1>), grid, dim3(blk), 0, stream_, d_sys_, map);     \
+      hipLaunchKernelGGL((k_water_forces<PP, M, F, 6, 1>), grid, dim3(blk), 0, ls_, d_sys_, map);     \
     else                                                                                                  \
-      hipLaunchKernelGGL((k_water_forces<PP, M, F, 6, 0>), grid, dim3(blk), 0, stream_, d_sys_, map);     \
+      hipLaunchKernelGGL((k_water_forces<PP, M, F, 6, 0>), grid, dim3(blk), 0, ls_, d_sys_, map);     \
   } while (0)
   switch (kernel) {
     case TK_WATER_LISTS: QGPU_LAUNCH(k_water_lists); break;
@@ -953,54 +963,91 @@ void Batch::set_shape(int k, int map, int block, int pf, int occ) {
   if (graph_exec_) { (void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
 }
 
+// Two branches of independent kernels: the water side stays on the engine's
+// stream, the solute side runs on a second stream forked from it and joined
+// back (in a captured graph: parallel graph branches).  The branches write
+// disjoint buffers.  Profiling runs every kernel on one stream.
+void Batch::fork() {
+  if (prof_ || !branches_) return;
+  HIPCHECK(hipEventRecord(ev_fork_, stream_));
+  HIPCHECK(hipStreamWaitEvent(stream2_, ev_fork_, 0));
+  ls_ = stream2_;
+}
+
+void Batch::join() {
+  if (ls_ == stream_) return;
+  HIPCHECK(hipEventRecord(ev_join_, stream2_));
+  HIPCHECK(hipStreamWaitEvent(stream_, ev_join_, 0));
+  ls_ = stream_;
+}
+
 void Batch::launch_lists() {
   const int ni = (int)h_sys_.size();
   dim3 g0((max_ncgp_ + 255) / 256, ni);
   { pb(0); hipLaunchKernelGGL(k_cgp_centers, g0, dim3(256), 0, stream_, d_sys_); pe(0); }
-  if (max_nwat_ > 0) { pb(1); launch_tuned(TK_WATER_LISTS, shape_[TK_WATER_LISTS]); pe(1); }
+  // water lists (ww, water side of pw, water multipoles) on the engine's
+  // stream; solute lists (pp, solute side of pw, solute multipoles) beside
+  fork();
   if (max_ncgp_solute_ > 0) { pb(2); launch_tuned(TK_SOLUTE_LISTS, shape_[TK_SOLUTE_LISTS]); pe(2); }
+  const hipStream_t side = ls_;
+  ls_ = stream_;
+  if (max_nwat_ > 0) { pb(1); launch_tuned(TK_WATER_LISTS, shape_[TK_WATER_LISTS]); pe(1); }
+  ls_ = side;
+  join();
 }
 
 void Batch::launch_forces() {
   const int ni = (int)h_sys_.size();
-  if (max_nwat_ > 0) { pb(3); launch_tuned(TK_WATER_FORCES, shape_[TK_WATER_FORCES]); pe(3); }
+  // side branch: solute forces (assign the solute atoms' gradient), bonded
+  // and Q-Q terms (term slots), Q-atom forces (added to the Q atoms'
+  // gradient after the solute forces)
+  fork();
   if (max_ncgp_solute_ > 0) { pb(4); launch_tuned(TK_SOLUTE_FORCES, shape_[TK_SOLUTE_FORCES]); pe(4); }
   if (max_terms_ > 0)
-    { pb(5); hipLaunchKernelGGL(k_terms, dim3((max_terms_ + 255) / 256, ni), dim3(256), 0, stream_, d_sys_); pe(5); }
+    { pb(5); hipLaunchKernelGGL(k_terms, dim3((max_terms_ + 255) / 256, ni), dim3(256), 0, ls_, d_sys_); pe(5); }
   if (max_nqat_ > 0) {
     dim3 gq((max_nqat_ * kQChunks * 64 + 255) / 256, ni);
     auto q_fp64 = [&](int shadow) {
-      if (max_nstates_ <= 2) hipLaunchKernelGGL((k_q_forces<double, 2>), gq, dim3(256), 0, stream_, d_sys_, shadow);
-      else if (max_nstates_ <= 4) hipLaunchKernelGGL((k_q_forces<double, 4>), gq, dim3(256), 0, stream_, d_sys_, shadow);
-      else hipLaunchKernelGGL((k_q_forces<double, kMaxQStates>), gq, dim3(256), 0, stream_, d_sys_, shadow);
+      if (max_nstates_ <= 2) hipLaunchKernelGGL((k_q_forces<double, 2>), gq, dim3(256), 0, ls_, d_sys_, shadow);
+      else if (max_nstates_ <= 4) hipLaunchKernelGGL((k_q_forces<double, 4>), gq, dim3(256), 0, ls_, d_sys_, shadow);
+      else hipLaunchKernelGGL((k_q_forces<double, kMaxQStates>), gq, dim3(256), 0, ls_, d_sys_, shadow);
     };
     const dim3 gr((max_nqat_ + 63) / 64, ni);
     pb(6);
     if (precision_ == 0) {
       q_fp64(0);
     } else {
-      if (max_nstates_ <= 2) hipLaunchKernelGGL((k_q_forces<float, 2>), gq, dim3(256), 0, stream_, d_sys_, 0);
-      else if (max_nstates_ <= 4) hipLaunchKernelGGL((k_q_forces<float, 4>), gq, dim3(256), 0, stream_, d_sys_, 0);
-      else hipLaunchKernelGGL((k_q_forces<float, kMaxQStates>), gq, dim3(256), 0, stream_, d_sys_, 0);
+      if (max_nstates_ <= 2) hipLaunchKernelGGL((k_q_forces<float, 2>), gq, dim3(256), 0, ls_, d_sys_, 0);
+      else if (max_nstates_ <= 4) hipLaunchKernelGGL((k_q_forces<float, 4>), gq, dim3(256), 0, ls_, d_sys_, 0);
+      else hipLaunchKernelGGL((k_q_forces<float, kMaxQStates>), gq, dim3(256), 0, ls_, d_sys_, 0);
     }
-    hipLaunchKernelGGL(k_q_reduce, gr, dim3(64), 0, stream_, d_sys_, 0);
+    hipLaunchKernelGGL(k_q_reduce, gr, dim3(64), 0, ls_, d_sys_, 0);
     pe(6);
     // mixed precision with energy frames: the Q energies of each recorded
     // frame are also evaluated with FP64 pair arithmetic (the kernels return
     // at once on other steps)
     if (precision_ == 1 && iene_ > 0) {
       q_fp64(1);
-      hipLaunchKernelGGL(k_q_reduce, gr, dim3(64), 0, stream_, d_sys_, 1);
+      hipLaunchKernelGGL(k_q_reduce, gr, dim3(64), 0, ls_, d_sys_, 1);
     }
   }
-  if (max_slot_atoms_ > 0)
-    { pb(7); hipLaunchKernelGGL(k_term_gather, dim3((max_slot_atoms_ + 255) / 256, ni), dim3(256), 0, stream_, d_sys_); pe(7); }
+  // main branch: water forces (assign the waters' gradient), then the
+  // polarisation restraint adds to it
+  const hipStream_t side = ls_;
+  ls_ = stream_;
   if (max_nwat_ > 0) {
+    pb(3); launch_tuned(TK_WATER_FORCES, shape_[TK_WATER_FORCES]); pe(3);
     dim3 g2((max_nwat_ + 255) / 256, ni);
     { pb(8); hipLaunchKernelGGL(k_watpol_theta, g2, dim3(256), 0, stream_, d_sys_); pe(8); }
     dim3 g3(kNumShellMax, ni);
     { pb(9); hipLaunchKernelGGL(k_watpol_force, g3, dim3(1024), 0, stream_, d_sys_); pe(9); }
   }
+  ls_ = side;
+  join();
+  // term slots may hold water atoms (unconstrained water bonds): gathered
+  // after both branches
+  if (max_slot_atoms_ > 0)
+    { pb(7); hipLaunchKernelGGL(k_term_gather, dim3((max_slot_atoms_ + 255) / 256, ni), dim3(256), 0, stream_, d_sys_); pe(7); }
   if (far_model_) launch_far_model();
 }
 

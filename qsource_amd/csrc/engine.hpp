@@ -323,6 +323,8 @@ class Batch {
   struct Inst;
   void init(const std::vector<std::string>& inputs, int device);
   void release();
+  void fork();
+  void join();
   void one_step();
   void launch_cycle_graph();
   void sync_sys();
@@ -355,6 +357,10 @@ class Batch {
   DevSys* d_sys_ = nullptr;
   std::vector<DevSys> h_sys_;
   hipStream_t stream_ = nullptr;
+  hipStream_t stream2_ = nullptr;   // side branch of the step (solute, Q, terms)
+  hipStream_t ls_ = nullptr;        // stream the next launch goes to
+  hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
+  bool branches_ = true;            // QGPU_BRANCHES=0 runs every kernel on one stream
   int istep_ = 0;
   int precision_ = 0;
   int max_nwat_ = 0, max_ncgp_ = 0, max_ncgp_solute_ = 0, max_units_ = 0, max_natom_ = 0;
