@@ -494,6 +494,28 @@ void Batch::upload(int k) {
     }
     D.qp_lj = I->upload(qp_lj);
     D.qw_lj = I->upload(qw_lj);
+    {
+      // the water side needs only the lambda-weighted sum over states of
+      // each Q-water pair's gradient; with fixed Q LJ parameters the states
+      // differ only in the Q charge
+      std::vector<double> rec((size_t)std::max(1, nq) * kQwRec, 0.0);
+      for (int iq = 0; iq < nq && s.nwat > 0; ++iq) {
+        double* r = rec.data() + (size_t)iq * kQwRec;
+        for (int st = 0; st < s.nstates; ++st) {
+          r[0] += s.lambda[st] * s.qcrg[iq * s.nstates + st];
+          r[1] += s.lambda[st];
+        }
+        const int ti = s.iac[s.iqseq[iq]];
+        for (int ip = 0; ip < 3; ++ip) {
+          const int lj = qw_lj[3 * iq + ip] - 1;
+          r[2 + 2 * ip] = t.types[ti].avdw[lj] * s.alj_solv[ip][lj];
+          r[3 + 2 * ip] = t.types[ti].bvdw[lj] * s.blj_solv[ip][lj];
+        }
+      }
+      std::vector<float> recf(rec.begin(), rec.end());
+      D.qw_rec_d = I->upload(rec);
+      D.qw_rec_f = I->upload(recf);
+    }
     D.nqp_atoms = (int)qp_atoms.size();
     D.nqw_waters = (int)qw_waters.size();
     D.qp_atoms = I->upload(qp_atoms);

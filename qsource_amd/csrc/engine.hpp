@@ -67,6 +67,7 @@ enum XtKind {
 constexpr int kXtInts = 10;
 constexpr int kXtPars = 8;
 constexpr int kMorseRec = 7;   // Morse factor and its gradient on the bond's two atoms
+constexpr int kQwRec = 8;      // Q-water record per Q atom (qw_rec_d / qw_rec_f)
 
 struct DevShell {
   double rout, dr, cstb, avtheta, avn_insh, theta_corr;
@@ -137,6 +138,11 @@ struct DevSys {
   const double* lambda;
   const signed char* qp_lj;      // nat_solute x nqat LJ code of the (Q, solute atom) pair
   const signed char* qw_lj;      // nqat x 3: LJ code of the (water atom ip, Q atom) pair
+  // per Q atom, for the water side of the Q-water forces with fixed Q LJ
+  // parameters: sum_s lambda_s qcrg(iq,s), sum_s lambda_s, then A and B of
+  // the Q atom's pair with water atoms O, H1, H2 (kQwRec values)
+  const double* qw_rec_d;
+  const float* qw_rec_f;
   double alj_w[3][2], blj_w[3][2]; // aLJ_solv(ip,:), bLJ_solv(ip,:) of the three water atoms
   int nqp_atoms, nqw_waters;
   const int* qp_atoms;           // solute atoms in the Q-solute list, ascending

@@ -73,6 +73,13 @@ __device__ __forceinline__ const float* pair_x<float>(const DevSys& S) { return 
 
 // Per-solute-atom LJ products with the water atoms, in the pair precision
 template <typename P>
+__device__ __forceinline__ const P* qw_rec(const DevSys& S);
+template <>
+__device__ __forceinline__ const double* qw_rec<double>(const DevSys& S) { return S.qw_rec_d; }
+template <>
+__device__ __forceinline__ const float* qw_rec<float>(const DevSys& S) { return S.qw_rec_f; }
+
+template <typename P>
 __device__ __forceinline__ const P* solute_water_lj(const DevSys& S);
 template <>
 __device__ __forceinline__ const double* solute_water_lj<double>(const DevSys& S) { return S.swp_d; }

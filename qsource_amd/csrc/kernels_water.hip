@@ -301,70 +301,54 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(co
     }
   }
   // Q-water pairs seen from the water, Q atoms taken in turn by the lanes
-  // (the Q atom is i of each pair): O with LJ + Coulomb, H Coulomb only
+  // (the Q atom is i of each pair): O with LJ + Coulomb, H Coulomb only (LJ
+  // too for an all-atom solvent).  Only the gradient is formed here, so with
+  // fixed Q LJ parameters the states fold into one lambda-weighted charge
+  // (sum_s lambda_s qcrg(iq,s)) and one lambda sum for the LJ part; with
+  // Q-atom types (soft-core per state) each state is evaluated.
   if (active && (S.flags[io] & F_QW)) {
     const int ns = S.nstates;
+    const bool qv = S.qvdw != 0;
     for (int iq = sub; iq < S.nqat; iq += 8) {
       const int qi = S.iqseq[iq];
       const P xq0 = X[3 * qi], xq1 = X[3 * qi + 1], xq2 = X[3 * qi + 2];
-      const int lo = S.qw_lj[3 * iq];
-      const int ti = S.iac[qi];
-      const P dx1 = xi[0] - xq0, dx2 = xi[1] - xq1, dx3 = xi[2] - xq2;
-      const P r2sq = pnorm2(dx1, dx2, dx3);
-      const P r6_hc = r2sq * r2sq * r2sq;
-      P r;
-      const P r2 = inv_r2(r2sq, r);
-      const P r6 = (P)1 / r6_hc;
-      P dv = 0;
-      if (S.qvdw) {
-        for (int st = 0; st < ns; ++st) {
-          P Vel, Va, Vb;
-          dv = dv + qw_qvdw_state<P>(S, iq, st, 0, lo, r2, r, r6_hc, Vel, Va, Vb);
-        }
-      } else {
-        const P Va = (P)S.avdw[ti * 3 + lo - 1] * (P)S.alj_w[0][lo - 1] * r6 * r6;
-        const P Vb = (P)S.bvdw[ti * 3 + lo - 1] * (P)S.blj_w[0][lo - 1] * r6;
-        for (int st = 0; st < ns; ++st) {
-          const P Vel = (P)S.chg_solv[0] * (P)S.qcrg[iq * ns + st] * r;
-          dv = dv + r2 * (-Vel - ((P)12 * Va - (P)6 * Vb) * (r6_hc / r6_hc)) * (P)S.lambda[st];
-        }
-      }
-      g[0] = g[0] + (double)(dv * dx1);
-      g[1] = g[1] + (double)(dv * dx2);
-      g[2] = g[2] + (double)(dv * dx3);
-      for (int ip = 1; ip < 3; ++ip) {
+      const gconst<P> R = gptr(qw_rec<P>(S)) + (size_t)iq * kQwRec;
+      const P qe = R[0], lsum = R[1];
+      for (int ip = 0; ip < 3; ++ip) {
         const P ex1 = xi[3 * ip] - xq0, ex2 = xi[3 * ip + 1] - xq1, ex3 = xi[3 * ip + 2] - xq2;
-        P rr;
         const P h2 = pnorm2(ex1, ex2, ex3);
+        P rr;
         const P q2 = inv_r2(h2, rr);
-        P dvh = 0;
-        if constexpr (AA != 0) {
-          // the H carries LJ too, with its own LJ code
+        const bool lj = ip == 0 || AA != 0;
+        P dv = 0;
+        if (qv) {
           const int lh = S.qw_lj[3 * iq + ip];
           const P h6_hc = h2 * h2 * h2;
-          if (S.qvdw) {
-            for (int st = 0; st < ns; ++st) {
-              P Vel, Va, Vb;
-              dvh = dvh + qw_qvdw_state<P>(S, iq, st, ip, lh, q2, rr, h6_hc, Vel, Va, Vb);
-            }
-          } else {
-            const P h6 = (P)1 / h6_hc;
-            const P Va = (P)S.avdw[ti * 3 + lh - 1] * (P)S.alj_w[ip][lh - 1] * h6 * h6;
-            const P Vb = (P)S.bvdw[ti * 3 + lh - 1] * (P)S.blj_w[ip][lh - 1] * h6;
-            for (int st = 0; st < ns; ++st) {
-              const P Vel = (P)S.chg_solv[ip] * (P)S.qcrg[iq * ns + st] * rr;
-              dvh = dvh + q2 * (-Vel - ((P)12 * Va - (P)6 * Vb) * (h6_hc / h6_hc)) * (P)S.lambda[st];
+          for (int st = 0; st < ns; ++st) {
+            P Vel, Va, Vb;
+            if (lj) {
+              dv = dv + qw_qvdw_state<P>(S, iq, st, ip, lh, q2, rr, h6_hc, Vel, Va, Vb);
+            } else {
+              Vel = (P)S.chg_solv[ip] * (P)S.qcrg[iq * ns + st] * rr;
+              dv = dv - q2 * Vel * (P)S.lambda[st];
             }
           }
         } else {
-          for (int st = 0; st < ns; ++st) {
-            const P Vel = (P)S.chg_solv[ip] * (P)S.qcrg[iq * ns + st] * rr;
-            dvh = dvh - q2 * Vel * (P)S.lambda[st];
+          const P Vel = (P)S.chg_solv[ip] * qe * rr;
+          if (lj) {
+            P h6;
+            if constexpr (sizeof(P) == 8) h6 = (P)1 / (h2 * h2 * h2);
+            else h6 = q2 * q2 * q2;
+            const P Va = R[2 + 2 * ip] * h6 * h6;
+            const P Vb = R[3 + 2 * ip] * h6;
+            dv = q2 * (-Vel - ((P)12 * Va - (P)6 * Vb) * lsum);
+          } else {
+            dv = -q2 * Vel;
           }
         }
-        g[3 * ip] = g[3 * ip] + (double)(dvh * ex1);
-        g[3 * ip + 1] = g[3 * ip + 1] + (double)(dvh * ex2);
-        g[3 * ip + 2] = g[3 * ip + 2] + (double)(dvh * ex3);
+        g[3 * ip] = g[3 * ip] + (double)(dv * ex1);
+        g[3 * ip + 1] = g[3 * ip + 1] + (double)(dv * ex2);
+        g[3 * ip + 2] = g[3 * ip + 2] + (double)(dv * ex3);
       }
     }
   }
