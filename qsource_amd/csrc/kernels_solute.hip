@@ -265,22 +265,25 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(c
       }
     };
     if constexpr (PF != 0) {
-      // next water's index and coordinates loaded during this one's pairs
+      // software pipeline: the next water's coordinates and the index of
+      // the one after are in flight during this water's pairs
       int k = sub;
-      int jon = 0;
+      int jon = 0, jwn2 = 0;
       P xn[9];
       if (k < npw) {
         jon = S.nat_solute + 3 * wrow[k];
         for (int u = 0; u < 9; ++u) xn[u] = X[3 * jon + u];
       }
+      if (k + 8 < npw) jwn2 = wrow[k + 8];
       while (k < npw) {
         P xw[9];
         for (int u = 0; u < 9; ++u) xw[u] = xn[u];
         const int jo = jon;
         k += 8;
         if (k < npw) {
-          jon = S.nat_solute + 3 * wrow[k];
+          jon = S.nat_solute + 3 * jwn2;
           for (int u = 0; u < 9; ++u) xn[u] = X[3 * jon + u];
+          if (k + 8 < npw) jwn2 = wrow[k + 8];
         }
         pw_water(jo, xw);
       }

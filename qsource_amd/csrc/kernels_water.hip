@@ -231,23 +231,27 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(co
     }
   };
   if constexpr (PF != 0) {
-    // the next partner's index and coordinates are loaded while this
-    // partner's pairs are computed (same partners, same order)
+    // software pipeline over the lane's partners (same partners, same
+    // order): while a partner's pairs are computed, the next partner's
+    // coordinates and the index of the one after are in flight, so neither
+    // load is waited on before the arithmetic
     int k = sub;
-    int jwn = 0;
+    int jwn = 0, jwn2 = 0;
     P xn[9];
     if (k < nn) {
       jwn = row[k];
       for (int u = 0; u < 9; ++u) xn[u] = X[3 * (S.nat_solute + 3 * jwn) + u];
     }
+    if (k + 8 < nn) jwn2 = row[k + 8];
     while (k < nn) {
       P xj[9];
       for (int u = 0; u < 9; ++u) xj[u] = xn[u];
       const int jw = jwn;
       k += 8;
       if (k < nn) {
-        jwn = row[k];
+        jwn = jwn2;
         for (int u = 0; u < 9; ++u) xn[u] = X[3 * (S.nat_solute + 3 * jwn) + u];
+        if (k + 8 < nn) jwn2 = row[k + 8];
       }
       ww_pair(jw, xj);
     }
