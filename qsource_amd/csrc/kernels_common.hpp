@@ -264,6 +264,64 @@ struct LrfAcc : LrfTerms<double> {
   }
 };
 
+// Raw moments of charges about a centre, the form in which the
+// mixed-precision list kernels gather each partner group in FP32: with a = q/r, f0 = q/r^3, g1 = q/r^5, t = q/r^7,
+// p0 = sum a, p1 = sum f0 d, f0s = sum f0, s1 = sum g1 d, m2 = sum g1 d d
+// (00 01 02 11 12 22), m3 = sum t d d d (000 001 002 011 012 022 111 112
+// 122 222).  They are linear in the charges; the LRF components follow as
+// phi1 = -p1, phi2_ab = 3 m2_ab - delta_ab f0s and
+// phi3_abc = -15 m3_abc + 3 (delta_ab s1_c + delta_ac s1_b + delta_bc s1_a),
+// the same sums as LrfTerms regrouped (fewer operations per atom).
+template <typename T>
+struct LrfRaw {
+  T p0, p1[3], f0s, s1[3], m2[6], m3[10];
+  __device__ void zero() {
+    p0 = 0; f0s = 0;
+    for (int k = 0; k < 3; ++k) { p1[k] = 0; s1[k] = 0; }
+    for (int k = 0; k < 6; ++k) m2[k] = 0;
+    for (int k = 0; k < 10; ++k) m3[k] = 0;
+  }
+  __device__ void add(T q, T d0, T d1, T d2) {
+#pragma clang fp contract(fast)
+    const T r2 = d0 * d0 + d1 * d1 + d2 * d2;
+    T ri;
+    const T ri2 = inv_r2(r2, ri);
+    const T a = q * ri, f0 = a * ri2, g1 = f0 * ri2, t = g1 * ri2;
+    p0 = p0 + a;
+    f0s = f0s + f0;
+    p1[0] = p1[0] + f0 * d0; p1[1] = p1[1] + f0 * d1; p1[2] = p1[2] + f0 * d2;
+    s1[0] = s1[0] + g1 * d0; s1[1] = s1[1] + g1 * d1; s1[2] = s1[2] + g1 * d2;
+    const T d00 = d0 * d0, d01 = d0 * d1, d02 = d0 * d2, d11 = d1 * d1, d12 = d1 * d2, d22 = d2 * d2;
+    m2[0] = m2[0] + g1 * d00; m2[1] = m2[1] + g1 * d01; m2[2] = m2[2] + g1 * d02;
+    m2[3] = m2[3] + g1 * d11; m2[4] = m2[4] + g1 * d12; m2[5] = m2[5] + g1 * d22;
+    const T t00 = t * d00, t01 = t * d01, t02 = t * d02, t11 = t * d11, t12 = t * d12, t22 = t * d22;
+    m3[0] = m3[0] + t00 * d0; m3[1] = m3[1] + t00 * d1; m3[2] = m3[2] + t00 * d2;
+    m3[3] = m3[3] + t11 * d0; m3[4] = m3[4] + t01 * d2; m3[5] = m3[5] + t22 * d0;
+    m3[6] = m3[6] + t11 * d1; m3[7] = m3[7] + t11 * d2; m3[8] = m3[8] + t22 * d1;
+    m3[9] = m3[9] + t22 * d2;
+    (void)t02; (void)t12;
+  }
+  // the unique LRF components of these moments
+  __device__ LrfTerms<T> terms() const {
+    LrfTerms<T> o;
+    o.p0 = p0;
+    for (int k = 0; k < 3; ++k) o.p1[k] = -p1[k];
+    o.p2[0] = (T)3 * m2[0] - f0s; o.p2[1] = (T)3 * m2[1]; o.p2[2] = (T)3 * m2[2];
+    o.p2[3] = (T)3 * m2[3] - f0s; o.p2[4] = (T)3 * m2[4]; o.p2[5] = (T)3 * m2[5] - f0s;
+    o.p3[0] = (T)-15 * m3[0] + (T)9 * s1[0];   // 000
+    o.p3[1] = (T)-15 * m3[1] + (T)3 * s1[1];   // 001
+    o.p3[2] = (T)-15 * m3[2] + (T)3 * s1[2];   // 002
+    o.p3[3] = (T)-15 * m3[3] + (T)3 * s1[0];   // 011
+    o.p3[4] = (T)-15 * m3[4];                  // 012
+    o.p3[5] = (T)-15 * m3[5] + (T)3 * s1[0];   // 022
+    o.p3[6] = (T)-15 * m3[6] + (T)9 * s1[1];   // 111
+    o.p3[7] = (T)-15 * m3[7] + (T)3 * s1[2];   // 112
+    o.p3[8] = (T)-15 * m3[8] + (T)3 * s1[1];   // 122
+    o.p3[9] = (T)-15 * m3[9] + (T)9 * s1[2];   // 222
+    return o;
+  }
+};
+
 // Add the non-Q atoms of charge group g (as seen from centre c) to acc.
 template <typename P>
 __device__ __forceinline__ void lrf_gather_group(const DevSys& S, int g, double c0, double c1,
@@ -275,8 +333,9 @@ __device__ __forceinline__ void lrf_gather_group(const DevSys& S, int g, double 
       acc.add(S.crg[i], S.x[3 * i] - c0, S.x[3 * i + 1] - c1, S.x[3 * i + 2] - c2);
     }
   } else {
-    // mixed precision: the group's terms in FP32, accumulated in FP64
-    LrfTerms<float> part;
+    // mixed precision: the group's moments in FP32 (raw form, then the
+    // components), accumulated in FP64
+    LrfRaw<float> part;
     part.zero();
     for (int a = S.cgp_first[g]; a <= S.cgp_last[g]; ++a) {
       const int i = S.cgpatom[a];
@@ -284,7 +343,7 @@ __device__ __forceinline__ void lrf_gather_group(const DevSys& S, int g, double 
       part.add((float)S.crg[i], (float)(S.x[3 * i] - c0), (float)(S.x[3 * i + 1] - c1),
                (float)(S.x[3 * i + 2] - c2));
     }
-    acc.add(part);
+    acc.add(part.terms());
   }
 }
 
@@ -302,11 +361,11 @@ __device__ __forceinline__ void lrf_gather_water(const DevSys& S, int jo, double
       acc.add(S.chg_solv[k], xj[3 * k] - c0, xj[3 * k + 1] - c1, xj[3 * k + 2] - c2);
   } else {
     const gconst<float> xj = gptr(S.xf) + 3 * jo;
-    LrfTerms<float> part;
+    LrfRaw<float> part;
     part.zero();
     for (int k = 0; k < 3; ++k)
       part.add((float)S.chg_solv[k], xj[3 * k] - cf[0], xj[3 * k + 1] - cf[1], xj[3 * k + 2] - cf[2]);
-    acc.add(part);
+    acc.add(part.terms());
   }
 }
 
