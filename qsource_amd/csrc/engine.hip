@@ -33,6 +33,7 @@ template <typename P, int MAPPED> __global__ void k_solute_lists(const DevSys*, 
 template <typename P, int MAPPED, int PF> __global__ void k_solute_forces(const DevSys*, const int2*);
 template <typename P, int NS> __global__ void k_q_forces(const DevSys*, int);
 __global__ void k_q_reduce(const DevSys*, int);
+__global__ void k_q_lists(const DevSys*);
 __global__ void k_terms(const DevSys*);
 __global__ void k_term_gather(const DevSys*);
 __global__ void k_far_features(const DevSys*);
@@ -148,6 +149,7 @@ void Batch::init(const std::vector<std::string>& inputs, int device) {
     stype = s->top.solvent_type;
   }
   allatom_ = stype == 1;
+  for (auto* s : setups_) q_dynamic_ = q_dynamic_ || (s->nqat > 0 && !(s->inp.rcq > s->top.rexcl_o));
   nbcycle_ = setups_[0]->inp.nbcycle;
   for (auto* s : setups_)
     if (s->inp.nbcycle != nbcycle_ || nbcycle_ < 1)
@@ -262,8 +264,6 @@ void Batch::upload(int k) {
   const qsys::Setup& s = *setups_[k];
   const qsys::Topology& t = s.top;
   if (s.inp.freeze) throw std::runtime_error("excluded_freeze is not supported");
-  if (s.nqat > 0 && !(s.inp.rcq > t.rexcl_o))
-    throw std::runtime_error("Q-atom cut-off must exceed the exclusion radius (static Q lists)");
   if (s.nstates > kMaxQStates) throw std::runtime_error("too many FEP states");
   for (int g = 0; g < t.ncgp_solute; ++g)
     if (t.cgp[g].last - t.cgp[g].first + 1 > kMaxGroupAtoms)
@@ -460,23 +460,31 @@ void Batch::upload(int k) {
       double d0 = s.x[3 * a] - t.xpcent[0], d1 = s.x[3 * a + 1] - t.xpcent[1], d2 = s.x[3 * a + 2] - t.xpcent[2];
       return d0 * d0 + d1 * d1 + d2 * d2;
     };
+    // atoms that can enter the Q-solute list: not bonded or angled to a Q
+    // atom in any state (nbqplist src/md.f90:10295); their LJ codes do not
+    // depend on the coordinates
+    std::vector<unsigned char> qp_cand(std::max(1, nsl), 0);
     if (nq > 0) {
+      for (int i = 0; i < nsl; ++i) {
+        bool conn = false;
+        for (int st = 0; st < s.nstates && !conn; ++st)
+          for (int iq = 0; iq < nq && !conn; ++iq) conn = qconn(st, iq, i) <= 3;
+        if (conn) continue;
+        qp_cand[i] = 1;
+        for (int iq = 0; iq < nq; ++iq) {
+          int lj = s.ljcod[s.iac[i] * s.natyps + s.iac[s.iqseq[iq]]];
+          if (qconn(0, iq, i) == 4) lj = 3;
+          qp_lj[(size_t)i * nq + iq] = (signed char)lj;
+        }
+      }
       for (int g = 0; g < t.ncgp_solute; ++g) {
         int sw = t.cgp[g].iswitch;
         if (s.excl[sw] || dist2c(sw) > rcq2) continue;
         for (int a = t.cgp[g].first; a <= t.cgp[g].last; ++a) {
           int i = t.cgpatom[a];
-          bool conn = false;
-          for (int st = 0; st < s.nstates && !conn; ++st)
-            for (int iq = 0; iq < nq && !conn; ++iq) conn = qconn(st, iq, i) <= 3;
-          if (conn) continue;
+          if (!qp_cand[i]) continue;
           qp_atoms.push_back(i);
           flags[i] |= F_QP;
-          for (int iq = 0; iq < nq; ++iq) {
-            int lj = s.ljcod[s.iac[i] * s.natyps + s.iac[s.iqseq[iq]]];
-            if (qconn(0, iq, i) == 4) lj = 3;
-            qp_lj[(size_t)i * nq + iq] = (signed char)lj;
-          }
         }
       }
       std::sort(qp_atoms.begin(), qp_atoms.end());
@@ -516,8 +524,22 @@ void Batch::upload(int k) {
       D.qw_rec_d = I->upload(rec);
       D.qw_rec_f = I->upload(recf);
     }
-    D.nqp_atoms = (int)qp_atoms.size();
-    D.nqw_waters = (int)qw_waters.size();
+    // Rcq > rexcl_o: the sets above hold for the whole run (nbqplist and
+    // nbqwlist return after their first call); otherwise k_q_lists rebuilds
+    // them at every list update, in capacity-sized arrays
+    D.q_dynamic = (nq > 0 && !(s.inp.rcq > t.rexcl_o)) ? 1 : 0;
+    D.rcq2 = rcq2;
+    {
+      std::vector<int> cnt = {(int)qp_atoms.size(), (int)qw_waters.size()};
+      D.q_counts = I->upload(cnt);
+    }
+    if (D.q_dynamic) {
+      qp_atoms.resize(std::max<size_t>(qp_atoms.size(), (size_t)nsl), 0);
+      qw_waters.resize(std::max<size_t>(qw_waters.size(), (size_t)s.nwat), 0);
+    }
+    D.qp_cand = I->upload(qp_cand);
+    D.qp_gin = I->alloc<unsigned char>(std::max(1, t.ncgp_solute));
+    for (int c = 0; c < 3; ++c) D.xpcent[c] = t.xpcent[c];
     D.qp_atoms = I->upload(qp_atoms);
     D.qw_waters = I->upload(qw_waters);
     D.e_q = I->alloc<double>((size_t)std::max(1, nq * s.nstates) * 4);
@@ -1011,6 +1033,7 @@ void Batch::launch_lists() {
   // stream; solute lists (pp, solute side of pw, solute multipoles) beside
   fork();
   if (max_ncgp_solute_ > 0) { pb(2); launch_tuned(TK_SOLUTE_LISTS, shape_[TK_SOLUTE_LISTS]); pe(2); }
+  if (q_dynamic_) hipLaunchKernelGGL(k_q_lists, dim3(1, ni), dim3(1024), 0, ls_, d_sys_);
   const hipStream_t side = ls_;
   ls_ = stream_;
   if (max_nwat_ > 0) { pb(1); launch_tuned(TK_WATER_LISTS, shape_[TK_WATER_LISTS]); pe(1); }
@@ -1457,8 +1480,10 @@ void Batch::list_counts(int k, long* counts) {
       if ((iw < jw) == (((iw + jw) & 1) == 0)) ++ww;
     }
   counts[2] = 9 * ww;
-  counts[3] = (long)D.nqp_atoms * D.nqat;
-  counts[4] = 3L * D.nqw_waters * D.nqat;
+  int qc[2] = {0, 0};
+  HIPCHECK(hipMemcpy(qc, D.q_counts, sizeof(qc), hipMemcpyDeviceToHost));
+  counts[3] = (long)qc[0] * D.nqat;
+  counts[4] = 3L * qc[1] * D.nqat;
 }
 
 double Batch::temperature(int k) {

@@ -144,9 +144,14 @@ struct DevSys {
   const double* qw_rec_d;
   const float* qw_rec_f;
   double alj_w[3][2], blj_w[3][2]; // aLJ_solv(ip,:), bLJ_solv(ip,:) of the three water atoms
-  int nqp_atoms, nqw_waters;
-  const int* qp_atoms;           // solute atoms in the Q-solute list, ascending
-  const int* qw_waters;          // waters in the Q-water list, ascending
+  int* q_counts;                 // [0] solute atoms in the Q-solute list, [1] waters in the Q-water list
+  int q_dynamic;                 // Rcq <= rexcl_o: both lists rebuilt at every list update (k_q_lists)
+  double rcq2;
+  const unsigned char* qp_cand;  // nat_solute: atom may enter the Q-solute list
+  unsigned char* qp_gin;         // ncgp_solute: group's switch atom within Rcq (k_q_lists)
+  double xpcent[3];              // solute centre (topology)
+  int* qp_atoms;                 // solute atoms in the Q-solute list, ascending
+  int* qw_waters;                // waters in the Q-water list, ascending
   // Q-atom LJ types (qvdw_flag): qiac [iq*nstates+s]; qavdw/qbvdw [type*3 +
   // LJ code-1]; soft-core sc_lookup [(iq*sc_ncol + column)*nstates + s] with
   // columns natyps topology types then nqat Q atoms (all 0 without softcore)
@@ -337,6 +342,7 @@ class Batch {
   void refresh_xf();
   int nbcycle_ = 10;
   bool allatom_ = false;   // all-atom solvent LJ (topology solvent type 1)
+  bool q_dynamic_ = false; // some instance rebuilds its Q lists at list updates
   int iene_ = 0;
   bool use_graph_ = true;
   hipGraphExec_t graph_exec_ = nullptr;

@@ -406,7 +406,8 @@ __global__ __launch_bounds__(256) void k_q_forces(const DevSys* __restrict__ sys
     lam[st] = st < ns ? (P)S.lambda[st] : (P)0;
   }
   const int stride = 64 * kQChunks;
-  for (int k = chunk * 64 + lane; k < S.nqp_atoms; k += stride) {
+  const int nqp = S.q_counts[0], nqw = S.q_counts[1];
+  for (int k = chunk * 64 + lane; k < nqp; k += stride) {
     const int j = S.qp_atoms[k];
     const int lj = S.qp_lj[(size_t)j * S.nqat + iq];
     const int l = lj - 1;
@@ -454,7 +455,7 @@ __global__ __launch_bounds__(256) void k_q_forces(const DevSys* __restrict__ sys
   const P aS = (P)S.alj_w[0][lo - 1], bS = (P)S.blj_w[0][lo - 1];
   const bool allatom = S.solv_allatom != 0;
   const P cO = (P)S.chg_solv[0];
-  for (int k = chunk * 64 + lane; k < S.nqw_waters; k += stride) {
+  for (int k = chunk * 64 + lane; k < nqw; k += stride) {
     const int jo = S.nat_solute + 3 * S.qw_waters[k];
     P dx1 = (P)(S.x[3 * jo] - xq0), dx2 = (P)(S.x[3 * jo + 1] - xq1), dx3 = (P)(S.x[3 * jo + 2] - xq2);
     P r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
@@ -555,6 +556,69 @@ template __global__ void k_q_forces<double, 4>(const DevSys*, int);
 template __global__ void k_q_forces<float, 4>(const DevSys*, int);
 template __global__ void k_q_forces<double, kMaxQStates>(const DevSys*, int);
 template __global__ void k_q_forces<float, kMaxQStates>(const DevSys*, int);
+
+// Q-solute and Q-water lists rebuilt from the current coordinates when
+// Rcq <= rexcl_o (nbqplist src/md.f90:10215-10322, nbqwlist :10464-10549):
+// the non-connected atoms of each solute charge group whose switch atom lies
+// within Rcq of the solute centre, and the waters whose O does, neither
+// excluded.  One workgroup per instance compacts them in ascending order
+// (wavefront ballots, then the waves in order) and sets the F_QP / F_QW
+// flags the force kernels read.
+__device__ __forceinline__ int ordered_compact(bool take, int* wave_cnt, int& total) {
+  const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  const unsigned long long m = __ballot(take);
+  if (ln == 0) wave_cnt[wv] = __popcll(m);
+  __syncthreads();
+  int off = total;
+  for (int w = 0; w < wv; ++w) off += wave_cnt[w];
+  int all = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) all += wave_cnt[w];
+  const int pos = off + __popcll(m & ((1ull << ln) - 1ull));
+  __syncthreads();
+  total += all;
+  return pos;
+}
+
+__global__ __launch_bounds__(1024) void k_q_lists(const DevSys* __restrict__ sys) {
+  const DevSys& S = sys[blockIdx.y];
+  if (!S.q_dynamic) return;
+  __shared__ int wave_cnt[16];
+  unsigned char* fl = const_cast<unsigned char*>(S.flags);
+  const double p0 = S.xpcent[0], p1 = S.xpcent[1], p2 = S.xpcent[2];
+  auto inside = [&](int a) {
+    const double d0 = S.x[3 * a] - p0, d1 = S.x[3 * a + 1] - p1, d2 = S.x[3 * a + 2] - p2;
+    return !(fl[a] & F_EXCL) && d0 * d0 + d1 * d1 + d2 * d2 <= S.rcq2;
+  };
+  // groups first (their switch atoms' flags are read before any flag of
+  // this instance is rewritten), then the atoms
+  for (int g = threadIdx.x; g < S.ncgp_solute; g += blockDim.x) S.qp_gin[g] = inside(S.cgp_sw[g]) ? 1 : 0;
+  __threadfence_block();
+  __syncthreads();
+  int nqp = 0;
+  for (int base = 0; base < S.nat_solute; base += blockDim.x) {
+    const int i = base + threadIdx.x;
+    bool take = false;
+    if (i < S.nat_solute) {
+      take = S.qp_cand[i] && S.qp_gin[S.iwhich_cgp[i]];
+      fl[i] = (unsigned char)(take ? (fl[i] | F_QP) : (fl[i] & ~F_QP));
+    }
+    const int pos = ordered_compact(take, wave_cnt, nqp);
+    if (take) S.qp_atoms[pos] = i;
+  }
+  int nqw = 0;
+  for (int base = 0; base < S.nwat; base += blockDim.x) {
+    const int iw = base + threadIdx.x;
+    bool take = false;
+    if (iw < S.nwat) {
+      const int o = S.nat_solute + 3 * iw;
+      take = inside(o);
+      fl[o] = (unsigned char)(take ? (fl[o] | F_QW) : (fl[o] & ~F_QW));
+    }
+    const int pos = ordered_compact(take, wave_cnt, nqw);
+    if (take) S.qw_waters[pos] = iw;
+  }
+  if (threadIdx.x == 0) { S.q_counts[0] = nqp; S.q_counts[1] = nqw; }
+}
 
 // Sum of the Q-atom chunks: gradient added to d, per-state energies stored.
 // With shadow != 0 (FP64 re-evaluation of a mixed-precision batch on an

@@ -16,7 +16,11 @@ bitwise to reference qdyn on the same systems by test_oracle_reference.py):
   14749-14874) with fixed Q-atom LJ parameters and with Q-atom types: step 0
   within 1e-12, 30 FP64 steps within 1e-9, mixed precision gradient RMS
   error below 1e-5 of the gradient RMS; an SPC and an all-atom system in one
-  batch are refused.
+  batch are refused;
+* Rcq below the exclusion radius (nbqplist / nbqwlist rebuilt at every list
+  update, src/md.f90:10245, 10488): the Q list sizes equal the oracle's at
+  every list update while waters leave the Q sphere, 60 FP64 steps within
+  1e-9, per-state Q energies within 1e-9.
 """
 import numpy as np
 import pytest
@@ -171,3 +175,36 @@ def test_mixed_solvent_batch_refused(tmp_path):
     pb = gensys.write_system(b, str(tmp_path), "aa", steps=1)
     with pytest.raises(RuntimeError, match="solvent type"):
         qe.Engine([pa["inp"], pb["inp"]], qe.QGPU_FP64)
+
+
+def test_dynamic_q_lists(tmp_path):
+    """Q-solute and Q-water lists rebuilt on the device every 5 steps (Rcq 7
+    inside a 14 A sphere at 300 K): list sizes match the oracle's at each
+    update, and the run stays within 1e-9 of the FP64 oracle."""
+    _say("case: dynamic Q lists (Rcq 7 < rexcl_o 14), R=14 + 40-atom solute + 8 Q, FP64 60 steps")
+    s = gensys.solvated_system(radius=14.0, n_solute=40, seed=2601, n_q=8, n_states=2)
+    paths = gensys.write_system(s, str(tmp_path), "dynq", steps=60, lambdas=[0.6, 0.4], stepsize=1.0,
+                                temperature=300.0, bath_coupling=10.0, initial_temperature=300.0,
+                                rcq=7.0, non_bond=5)
+    o = Oracle(paths["inp"])
+    e = qe.Engine([paths["inp"]], qe.QGPU_FP64)
+    sizes_o, sizes_g = [], []
+    for blk in range(12):
+        if blk == 0:
+            o.final_energy()
+            e.make_pair_lists()
+        sizes_o.append(list(o.list_sizes()[3:]))
+        c = e.list_counts(0)
+        sizes_g.append([c["qp"], c["qw"]])
+        o.run(5)
+        e.step(5)
+    assert sizes_g == sizes_o, (sizes_g, sizes_o)
+    assert len(set(tuple(v) for v in sizes_o)) > 1, "the Q lists did not change during the run"
+    xo, vo = o.state()
+    xg, vg = e.state(0)
+    assert np.abs(xg - xo).max() <= 1e-9 * np.abs(xo).max()
+    assert np.abs(vg - vo).max() <= 1e-9 * np.abs(vo).max()
+    qo = o.qenergies()
+    qg = np.array(e.qenergies(0)).reshape(qo.shape)
+    assert np.allclose(qg, qo, rtol=1e-9, atol=1e-8), (qg, qo)
+    e.close()

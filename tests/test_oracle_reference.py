@@ -77,6 +77,12 @@ CASES = {
                                     dict(steps=30, lambdas=[0.6, 0.4], stepsize=0.5, temperature=50.0,
                                          bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
                                          energy_interval=1)),
+    # Rcq below the exclusion radius: the Q-solute and Q-water lists are
+    # rebuilt at every list update (every 5 steps) and change during the run
+    "c9_r14_q8_dynq_60": (dict(radius=14.0, n_solute=40, seed=2601, n_q=8, n_states=2),
+                          dict(steps=60, lambdas=[0.6, 0.4], stepsize=1.0, temperature=300.0,
+                               bath_coupling=10.0, initial_temperature=300.0, energy="case.en",
+                               energy_interval=1, rcq=7.0, non_bond=5)),
     # a breaking bond with non-Q neighbours: topology torsions through it
     # become Q terms (qatom_load_fep's completion); periodic impropers
     "c6_r12_q6s2_qbreak_30": (dict(radius=12.0, n_solute=40, seed=2202, n_q=6, n_states=2, q_types=True,
