@@ -34,6 +34,7 @@ template <typename P, int MAPPED, int PF> __global__ void k_solute_forces(const 
 template <typename P, int NS> __global__ void k_q_forces(const DevSys*, int);
 __global__ void k_q_reduce(const DevSys*, int);
 __global__ void k_q_lists(const DevSys*);
+__global__ void k_freeze_x(const DevSys*);
 __global__ void k_terms(const DevSys*);
 __global__ void k_term_gather(const DevSys*);
 __global__ void k_far_features(const DevSys*);
@@ -149,7 +150,10 @@ void Batch::init(const std::vector<std::string>& inputs, int device) {
     stype = s->top.solvent_type;
   }
   allatom_ = stype == 1;
-  for (auto* s : setups_) q_dynamic_ = q_dynamic_ || (s->nqat > 0 && !(s->inp.rcq > s->top.rexcl_o));
+  for (auto* s : setups_) {
+    q_dynamic_ = q_dynamic_ || (s->nqat > 0 && !(s->inp.rcq > s->top.rexcl_o));
+    freeze_ = freeze_ || s->inp.freeze;
+  }
   nbcycle_ = setups_[0]->inp.nbcycle;
   for (auto* s : setups_)
     if (s->inp.nbcycle != nbcycle_ || nbcycle_ < 1)
@@ -263,7 +267,6 @@ void Batch::release() {
 void Batch::upload(int k) {
   const qsys::Setup& s = *setups_[k];
   const qsys::Topology& t = s.top;
-  if (s.inp.freeze) throw std::runtime_error("excluded_freeze is not supported");
   if (s.nstates > kMaxQStates) throw std::runtime_error("too many FEP states");
   for (int g = 0; g < t.ncgp_solute; ++g)
     if (t.cgp[g].last - t.cgp[g].first + 1 > kMaxGroupAtoms)
@@ -283,6 +286,7 @@ void Batch::upload(int k) {
   D.xx = I->alloc<double>(3 * N);
   D.d = I->alloc<double>(3 * N);
   D.xtop = I->upload(s.xtop);
+  D.xq = s.inp.freeze ? I->alloc<double>(3 * (size_t)N) : D.x;
   D.crg = I->upload(s.crg);
   D.winv = I->upload(s.winv);
   std::vector<double> mass(N);
@@ -789,7 +793,7 @@ void Batch::upload(int k) {
   D.sh_d2 = I->upload(shd);
   D.shake_fail = I->alloc<int>(1);
   // energies and reductions
-  D.e_water = I->alloc<double>((size_t)std::max(1, s.nwat) * 4);
+  D.e_water = I->alloc<double>((size_t)std::max(1, s.nwat) * kEWater);
   D.e_solute = I->alloc<double>((size_t)std::max(1, s.nat_solute) * 8);
   D.e_wpol = I->alloc<double>(std::max(1, D.nshell));
   D.e_far = I->alloc<double>(std::max(1, D.ncgp));
@@ -1043,6 +1047,9 @@ void Batch::launch_lists() {
 
 void Batch::launch_forces() {
   const int ni = (int)h_sys_.size();
+  // excluded_freeze: the coordinates the terms after fix_shell see
+  if (freeze_)
+    hipLaunchKernelGGL(k_freeze_x, dim3((3 * max_natom_ + 255) / 256, ni), dim3(256), 0, stream_, d_sys_);
   // side branch: solute forces (assign the solute atoms' gradient), bonded
   // and Q-Q terms (term slots), Q-atom forces (added to the Q atoms'
   // gradient after the solute forces)

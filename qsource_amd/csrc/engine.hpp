@@ -68,6 +68,7 @@ constexpr int kXtInts = 10;
 constexpr int kXtPars = 8;
 constexpr int kMorseRec = 7;   // Morse factor and its gradient on the bond's two atoms
 constexpr int kQwRec = 8;      // Q-water record per Q atom (qw_rec_d / qw_rec_f)
+constexpr int kEWater = 5;     // energy partials per water
 
 struct DevShell {
   double rout, dr, cstb, avtheta, avn_insh, theta_corr;
@@ -81,6 +82,8 @@ struct DevSys {
   int precision;                 // 0 = FP64, 1 = mixed FP32 pair / FP64 accumulation
   double *x, *v, *xx, *d, *xtop;
   float* xf;                     // FP32 copy of x for the mixed-precision pair kernels
+  double* xq;                    // coordinates of the terms after fix_shell: x, or with
+                                 // excluded_freeze a copy holding the excluded atoms at xtop
   const double *crg, *winv, *mass;
   const unsigned char* flags;
   const int *iac;                // type index per atom
@@ -220,7 +223,7 @@ struct DevSys {
   const double* sh_d2;
   int* shake_fail;
   // energies: eacc[term] partial sums written per item, reduced on demand
-  double* e_water;     // nwat x 4 : ww_el, ww_vdw, lrf(water atoms), radial
+  double* e_water;     // nwat x kEWater : ww_el, ww_vdw, lrf(water atoms), radial, excluded-atom restraint
   double* e_solute;    // nat_solute x 8 : pp_el pp_vdw pw_el pw_vdw lrf fix shell spare
   double* e_wpol;      // nshell
   double* ke_part;     // kinetic partial sums: nblocks x 6
@@ -343,6 +346,7 @@ class Batch {
   int nbcycle_ = 10;
   bool allatom_ = false;   // all-atom solvent LJ (topology solvent type 1)
   bool q_dynamic_ = false; // some instance rebuilds its Q lists at list updates
+  bool freeze_ = false;    // some instance has excluded_freeze on
   int iene_ = 0;
   bool use_graph_ = true;
   hipGraphExec_t graph_exec_ = nullptr;

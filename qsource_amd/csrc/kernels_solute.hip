@@ -345,8 +345,9 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(c
     }
   }
   // harmonic restraint of excluded and shell atoms to the topology position
+  // (excluded_freeze: the atom is put back there instead, no restraint)
   if (active && sub == 2 && (fl & (F_EXCL | F_SHELL))) {
-    const double fk = (fl & F_EXCL) ? S.fk_fix : S.fk_pshell;
+    const double fk = (fl & F_EXCL) ? (S.freeze ? 0.0 : S.fk_fix) : S.fk_pshell;
     const double dr0 = S.x[3 * i] - S.xtop[3 * i], dr1 = S.x[3 * i + 1] - S.xtop[3 * i + 1],
                  dr2 = S.x[3 * i + 2] - S.xtop[3 * i + 2];
     const double erst = 0.5 * fk * (dr0 * dr0 + dr1 * dr1 + dr2 * dr2);
@@ -759,7 +760,7 @@ __device__ double qdihedral_dp(const double* x, int i, int j, int k, int l, doub
 // j = dv * rij), and the Morse factor record (EMorseD, dMorse_i, dMorse_j).
 __device__ double qbond_eval(const DevSys& S, int ib, const double* P, double lam, double rij[3], double& dv,
                              double rec[kMorseRec]) {
-  const double* x = S.x;
+  const double* x = S.xq;
   const int i3 = 3 * S.qb_atoms[2 * ib], j3 = 3 * S.qb_atoms[2 * ib + 1];
   for (int c = 0; c < 3; ++c) rij[c] = x[j3 + c] - x[i3 + c];
   const double b = sqrt(rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2]);
@@ -795,7 +796,7 @@ __device__ void coupled_morse(const DevSys& S, const int* R, const double* P, do
 __device__ void xterm(const DevSys& S, int n) {
   const int* R = S.xt + (size_t)kXtInts * n;
   const double* P = S.xp + (size_t)kXtPars * n;
-  const double* x = S.x;
+  const double* x = S.xq;
   const int kind = R[0], s0 = R[9];
   double e = 0.0;
   switch (kind) {
@@ -1114,6 +1115,9 @@ __global__ __launch_bounds__(256) void k_terms(const DevSys* __restrict__ sys) {
     return;
   }
   n -= S.nimp;
+  // the terms below follow fix_shell in pot_energy: with excluded_freeze
+  // they see the excluded atoms back at their topology positions
+  x = S.xq;
   if (n < S.nqq) {
     const int p = n;
     const int st = S.qq_state[p], iq = S.qq_iq[p], j = S.qq_j[p], jq = S.qq_jq[p], lj = S.qq_lj[p];

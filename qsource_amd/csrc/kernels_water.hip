@@ -40,6 +40,16 @@ __global__ void k_x_to_f32(const DevSys* __restrict__ sys) {
   if (c < 3 * S.natom) S.xf[c] = (float)S.x[c];
 }
 
+// excluded_freeze (fix_shell src/md.f90:2831-2839): the terms evaluated
+// after fix_shell in pot_energy see the excluded atoms at their topology
+// positions; xq holds those coordinates.
+__global__ void k_freeze_x(const DevSys* __restrict__ sys) {
+  const DevSys& S = sys[blockIdx.y];
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (!S.freeze || c >= 3 * S.natom) return;
+  S.xq[c] = (S.flags[c / 3] & F_EXCL) ? S.xtop[c] : S.x[c];
+}
+
 // ---------------------------------------------------------------------------
 // Water lists (nbwwlist_lrf src/md.f90:10897-11021 and the water side of
 // nbpwlist_lrf :9300-9429).  Eight waters per wavefront, eight lanes per
@@ -396,15 +406,30 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(co
     g[1] = g[1] + dv * dr1;
     g[2] = g[2] + dv * dr2;
   }
+  // excluded water atoms: fix_shell's restraint to the topology position
+  // (src/md.f90:2816-2862), atom sub-4 on lanes 4..6 (FP64)
+  double e_fix = 0.0;
+  if (active && sub >= 4 && sub < 7 && (S.flags[io + sub - 4] & F_EXCL)) {
+    const int i = io + sub - 4;
+    const double fk = S.freeze ? 0.0 : S.fk_fix;
+    const double dr0 = S.x[3 * i] - S.xtop[3 * i], dr1 = S.x[3 * i + 1] - S.xtop[3 * i + 1],
+                 dr2 = S.x[3 * i + 2] - S.xtop[3 * i + 2];
+    e_fix = 0.5 * fk * (dr0 * dr0 + dr1 * dr1 + dr2 * dr2);
+    const int a = sub - 4;
+    g[3 * a] = g[3 * a] + fk * dr0;
+    g[3 * a + 1] = g[3 * a + 1] + fk * dr1;
+    g[3 * a + 2] = g[3 * a + 2] + fk * dr2;
+  }
   for (int k = 0; k < 9; ++k) g[k] = seg8_sum(g[k]);
   e_el = seg8_sum(e_el);
   e_vdw = seg8_sum(e_vdw);
   e_lrf = seg8_sum(e_lrf);
   e_rad = seg8_sum(e_rad);
+  e_fix = seg8_sum(e_fix);
   if (active && sub == 0) {
     for (int k = 0; k < 9; ++k) S.d[3 * io + k] = g[k];
-    double* ew = S.e_water + (size_t)iw * 4;
-    ew[0] = e_el; ew[1] = e_vdw; ew[2] = e_lrf; ew[3] = e_rad;
+    double* ew = S.e_water + (size_t)iw * kEWater;
+    ew[0] = e_el; ew[1] = e_vdw; ew[2] = e_lrf; ew[3] = e_rad; ew[4] = e_fix;
   }
 }
 template __global__ void k_water_forces<double, 0, 0, 6, 0>(const DevSys*, const int2*);
@@ -576,9 +601,12 @@ __global__ __launch_bounds__(256) void k_integrate(const DevSys* __restrict__ sy
     for (int k = 0; k < 3; ++k) w[k] = S.winv[a + k];
     for (int c = 0; c < 9; ++c) {
       const int ic = 3 * a + c;
-      double vv = (S.v[ic] - S.d[ic] * w[c / 3] * dt) * ts;
+      // excluded_freeze: an excluded atom restarts from rest at its
+      // topology position (fix_shell, src/md.f90:2831-2839)
+      const bool frz = S.freeze && (S.flags[a + c / 3] & F_EXCL);
+      double vv = ((frz ? 0.0 : S.v[ic]) - S.d[ic] * w[c / 3] * dt) * ts;
       v[c] = vv;
-      xo[c] = S.x[ic];
+      xo[c] = frz ? S.xtop[ic] : S.x[ic];
       x[c] = xo[c] + vv * dt;
     }
     const int c0 = S.con_first[u];
@@ -630,12 +658,14 @@ __global__ __launch_bounds__(256) void k_integrate(const DevSys* __restrict__ sy
       const int i = S.unit_atoms[k];
       const double ts = i < S.nat_solute ? ts_solute : ts_solvent;
       const double w = S.winv[i];
+      const bool frz = S.freeze && (S.flags[i] & F_EXCL);
       for (int c = 0; c < 3; ++c) {
         const int ic = 3 * i + c;
-        double vv = (S.v[ic] - S.d[ic] * w * dt) * ts;
+        double vv = ((frz ? 0.0 : S.v[ic]) - S.d[ic] * w * dt) * ts;
+        const double xo = frz ? S.xtop[ic] : S.x[ic];
         S.v[ic] = vv;
-        S.xx[ic] = S.x[ic];
-        S.x[ic] = S.x[ic] + vv * dt;
+        S.xx[ic] = xo;
+        S.x[ic] = xo + vv * dt;
       }
     }
     const int c0 = S.con_first[u], c1 = S.con_first[u + 1];
@@ -745,9 +775,9 @@ __global__ __launch_bounds__(256) void k_energy_reduce(const DevSys* __restrict_
     if (S.iene <= 0 || st <= 0 || st % S.iene != 0) return;
     frame = st / S.iene - 1;
   }
-  double w[4] = {0, 0, 0, 0};
+  double w[kEWater] = {0, 0, 0, 0, 0};
   for (int iw = threadIdx.x; iw < S.nwat; iw += blockDim.x)
-    for (int c = 0; c < 4; ++c) w[c] += S.e_water[(size_t)iw * 4 + c];
+    for (int c = 0; c < kEWater; ++c) w[c] += S.e_water[(size_t)iw * kEWater + c];
   double p[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int i = threadIdx.x; i < S.nat_solute; i += blockDim.x)
     for (int c = 0; c < 8; ++c) p[c] += S.e_solute[(size_t)i * 8 + c];
@@ -765,6 +795,7 @@ __global__ __launch_bounds__(256) void k_energy_reduce(const DevSys* __restrict_
     bi[n < S.nimp_solute ? 0 : 1] += S.e_bterm[S.nbnd + S.nang + S.ntor + n];
   for (int c = 0; c < 4; ++c) w[c] = block_sum_256(w[c], red[c]);
   for (int c = 0; c < 8; ++c) p[c] = block_sum_256(p[c], red[4 + c]);
+  w[4] = block_sum_256(w[4], red[0]);
   for (int c = 0; c < 6; ++c) b[c] = block_sum_256(b[c], red[c]);
   for (int c = 0; c < 2; ++c) bi[c] = block_sum_256(bi[c], red[6 + c]);
   ef = block_sum_256(ef, red[8]);
@@ -841,7 +872,7 @@ __global__ __launch_bounds__(256) void k_energy_reduce(const DevSys* __restrict_
     double* E = S.energies;
     E[E_WW_EL] = w[0]; E[E_WW_VDW] = w[1]; E[E_LRF] = w[2] + p[4] + ef; E[E_RST_RADIAL] = w[3];
     E[E_PP_EL] = p[0]; E[E_PP_VDW] = p[1]; E[E_PW_EL] = p[2]; E[E_PW_VDW] = p[3];
-    E[E_RST_FIX] = p[5]; E[E_RST_SHELL] = p[6]; E[E_RST_PROTEIN] = rp;
+    E[E_RST_FIX] = p[5] + w[4]; E[E_RST_SHELL] = p[6]; E[E_RST_PROTEIN] = rp;
     double ew = 0.0;
     for (int s = 0; s < S.nshell; ++s) ew = ew + S.e_wpol[s];
     E[E_RST_WPOL] = S.wpol_restr ? ew : 0.0;

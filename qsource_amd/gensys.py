@@ -124,6 +124,8 @@ class System:
     # topology solvent type: 0 SPC-like (LJ on the water O only), 1 all-atom
     # LJ on every solvent atom (TIP3P-like H types)
     solvent_type: int = 0
+    # excluded atoms (topology flags, one per atom), None = none
+    excl: Optional[np.ndarray] = None
 
     @property
     def natom(self) -> int:
@@ -362,7 +364,9 @@ def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
                     softcore: Optional[float] = None, n_offdiag: int = 0,
                     impropers: int = 0, q_bonded: bool = False, restraints: bool = False,
                     fep_pairs: bool = False, qq_lib_charges: bool = False,
-                    q_shake: bool = False, solvent_allatom: bool = False) -> System:
+                    q_shake: bool = False, solvent_allatom: bool = False,
+                    solute_rmax: Optional[float] = None, exclude_outside: bool = False,
+                    exclude_water_radius: Optional[float] = None) -> System:
     """A TIP3P sphere of the given radius around an optional branched solute
     (charges N(0,q_sd) neutralised per 4-atom charge group, sigma~U[2.5,3.8],
     eps~U[0.05,0.2] quantised into at most n_types atom types), with an
@@ -377,11 +381,15 @@ def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
     impropers: 1 adds harmonic (GROMOS) and 2 periodic (AMBER) impropers on
     the solute's branch points; solvent_allatom: the water H gets the LJ
     type of CHARMM's TIP3P H (sigma 0.4 A, eps 0.046 kcal/mol) and the
-    topology declares solvent type 1 (all-atom solvent LJ)."""
+    topology declares solvent type 1 (all-atom solvent LJ); solute_rmax: the
+    solute grows within this radius instead of radius - 5 (so it may reach
+    through the sphere boundary); exclude_outside: every solute charge group
+    whose switch atom lies beyond the exclusion radius is excluded;
+    exclude_water_radius: waters whose O lies beyond it are excluded too."""
     rng = np.random.default_rng(seed)
     # --- solute
     if n_solute > 0:
-        rmax = max(6.0, radius - 5.0)
+        rmax = solute_rmax if solute_rmax is not None else max(6.0, radius - 5.0)
         for attempt in range(20):
             try:
                 sx, sb = _branched_solute(n_solute, rmax, rng)
@@ -588,6 +596,18 @@ def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
                 break
         sysm.shake_constraints = rows
     sysm.solvent_type = 1 if solvent_allatom else 0
+    if (exclude_outside and ns) or exclude_water_radius is not None:
+        excl = np.zeros(N, dtype=bool)
+        if exclude_outside:
+            for g in cgps[:ncgp_solute]:
+                if np.linalg.norm(x[g[0] - 1] - sysm.xpcent) > sysm.rexcl_o:
+                    excl[np.array(g) - 1] = True
+        if exclude_water_radius is not None:
+            for w in range(nw):
+                o = ns + 3 * w
+                if np.linalg.norm(x[o] - sysm.xwcent) > exclude_water_radius:
+                    excl[o:o + 3] = True
+        sysm.excl = excl
     return sysm
 
 
@@ -905,8 +925,10 @@ def write_top(s: System, path: str) -> None:
     L.append("%8.3f%8.3f = Exclusion, solvent radii" % (s.rexcl_o, s.rwat))
     L.append("%8.3f%8.3f%8.3f = Solute centre" % tuple(s.xpcent))
     L.append("%8.3f%8.3f%8.3f = Solvent centre" % tuple(s.xwcent))
-    L.append("%8d%8d = No. of excluded atoms (incl. water), no. of excluded waters" % (0, 0))
-    flags = "F" * N
+    ex = s.excl if s.excl is not None else np.zeros(N, dtype=bool)
+    nexwat = sum(1 for w in range(s.nwat) if ex[s.nat_solute + 3 * w])
+    L.append("%8d%8d = No. of excluded atoms (incl. water), no. of excluded waters" % (int(ex.sum()), nexwat))
+    flags = "".join("T" if e else "F" for e in ex)
     L += [flags[k:k + 80] for k in range(0, N, 80)]
     with open(path, "w") as f:
         f.write("\n".join(L) + "\n")
@@ -992,6 +1014,7 @@ def write_inp(path: str, top: str, final: str, *, steps: int, stepsize: float = 
               radial_force: Optional[float] = 60.0, polarisation: bool = True,
               polarisation_force: Optional[float] = 20.0,
               shell_force: Optional[float] = None, shell_radius: Optional[float] = None,
+              excluded_freeze: bool = False, excluded_force: Optional[float] = None,
               temp_interval: Optional[int] = None, restraints: Optional[dict] = None,
               trajectory: Optional[str] = None, trajectory_interval: int = 0,
               trajectory_atoms: Optional[Sequence[str]] = None) -> None:
@@ -1015,6 +1038,10 @@ def write_inp(path: str, top: str, final: str, *, steps: int, stepsize: float = 
         L.append("shell_radius %.6g" % shell_radius)
     if shell_force is not None:
         L.append("shell_force %.6g" % shell_force)
+    if excluded_force is not None:
+        L.append("excluded_force %.6g" % excluded_force)
+    if excluded_freeze:
+        L.append("excluded_freeze on")
     L += ["", "[solvent]"]
     if radial_force is not None:
         L.append("radial_force %.6g" % radial_force)

@@ -20,7 +20,11 @@ bitwise to reference qdyn on the same systems by test_oracle_reference.py):
 * Rcq below the exclusion radius (nbqplist / nbqwlist rebuilt at every list
   update, src/md.f90:10245, 10488): the Q list sizes equal the oracle's at
   every list update while waters leave the Q sphere, 60 FP64 steps within
-  1e-9, per-state Q energies within 1e-9.
+  1e-9, per-state Q energies within 1e-9;
+* excluded atoms (solute groups beyond the exclusion radius, waters near
+  the boundary) and the restrained shell (fix_shell, src/md.f90:2816-2862),
+  with and without excluded_freeze: step 0 within 1e-12, 30 FP64 steps
+  within 1e-9, mixed precision gradient within 1e-5 (RMS).
 """
 import numpy as np
 import pytest
@@ -208,3 +212,37 @@ def test_dynamic_q_lists(tmp_path):
     qg = np.array(e.qenergies(0)).reshape(qo.shape)
     assert np.allclose(qg, qo, rtol=1e-9, atol=1e-8), (qg, qo)
     e.close()
+
+
+@pytest.mark.parametrize("freeze", [False, True])
+def test_excluded_atoms(tmp_path, freeze):
+    """Excluded solute groups and waters with the excluded-atom restraint (or
+    frozen at their topology positions) and shell atoms with the shell
+    restraint: the GPU step against the oracle."""
+    _say("case: excluded atoms, R=12 + 120-atom solute reaching past the boundary, freeze=%s" % freeze)
+    s = gensys.solvated_system(radius=12.0, n_solute=120, seed=2702, n_q=8, n_states=2, solute_rmax=16.0,
+                               exclude_outside=True, exclude_water_radius=11.3)
+    assert s.excl[:s.nat_solute].any() and s.excl[s.nat_solute:].any()
+    paths = gensys.write_system(s, str(tmp_path), "excl", steps=30, lambdas=[0.6, 0.4], excluded_freeze=freeze,
+                                **COOL)
+    eo, eg, do, dg = _compare(paths)
+    for t in TERMS + ["rst_fix", "rst_shell"]:
+        assert abs(eg[t] - eo[t]) <= 1e-12 * max(1.0, abs(eo[t])), (t, eg[t], eo[t])
+    assert np.abs(dg - do).max() <= 1e-12 * np.abs(do).max()
+    o = Oracle(paths["inp"])
+    o.run(30)
+    e = qe.Engine([paths["inp"]], qe.QGPU_FP64)
+    e.step(30)
+    eo2, eg2 = o.energies(), e.energies(0)
+    xo, vo = o.state()
+    xg, vg = e.state(0)
+    assert np.abs(xg - xo).max() <= 1e-9 * np.abs(xo).max()
+    assert np.abs(vg - vo).max() <= 1e-9 * np.abs(vo).max()
+    for t in ("rst_fix", "rst_shell", "potential"):
+        assert abs(eg2[t] - eo2[t]) <= 1e-9 * max(1.0, abs(eo2[t])), (t, eg2[t], eo2[t])
+    if not freeze:
+        assert eo2["rst_fix"] > 0.0
+    e.close()
+    eo, eg, do, dg = _compare(paths, qe.QGPU_MIXED)
+    rms = np.sqrt(np.mean(do ** 2))
+    assert np.sqrt(np.mean((dg - do) ** 2)) <= 1e-5 * rms

@@ -83,6 +83,20 @@ CASES = {
                           dict(steps=60, lambdas=[0.6, 0.4], stepsize=1.0, temperature=300.0,
                                bath_coupling=10.0, initial_temperature=300.0, energy="case.en",
                                energy_interval=1, rcq=7.0, non_bond=5)),
+    # a solute reaching through the sphere boundary: charge groups beyond
+    # the exclusion radius excluded (fix_shell's excluded-atom restraint),
+    # heavy groups beyond 0.85 rexcl_o in the restrained shell, waters near
+    # the boundary excluded; and the same with excluded_freeze
+    "c10_r12_excl_30": (dict(radius=12.0, n_solute=120, seed=2702, n_q=8, n_states=2, solute_rmax=16.0,
+                             exclude_outside=True, exclude_water_radius=11.3),
+                        dict(steps=30, lambdas=[0.6, 0.4], stepsize=0.5, temperature=50.0,
+                             bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
+                             energy_interval=1)),
+    "c10_r12_excl_freeze_30": (dict(radius=12.0, n_solute=120, seed=2702, n_q=8, n_states=2, solute_rmax=16.0,
+                                    exclude_outside=True, exclude_water_radius=11.3),
+                               dict(steps=30, lambdas=[0.6, 0.4], stepsize=0.5, temperature=50.0,
+                                    bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
+                                    energy_interval=1, excluded_freeze=True)),
     # a breaking bond with non-Q neighbours: topology torsions through it
     # become Q terms (qatom_load_fep's completion); periodic impropers
     "c6_r12_q6s2_qbreak_30": (dict(radius=12.0, n_solute=40, seed=2202, n_q=6, n_states=2, q_types=True,
