@@ -366,7 +366,7 @@ def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
                     fep_pairs: bool = False, qq_lib_charges: bool = False,
                     q_shake: bool = False, solvent_allatom: bool = False,
                     solute_rmax: Optional[float] = None, exclude_outside: bool = False,
-                    exclude_water_radius: Optional[float] = None) -> System:
+                    exclude_water_radius: Optional[float] = None, hydrogens: bool = False) -> System:
     """A TIP3P sphere of the given radius around an optional branched solute
     (charges N(0,q_sd) neutralised per 4-atom charge group, sigma~U[2.5,3.8],
     eps~U[0.05,0.2] quantised into at most n_types atom types), with an
@@ -385,7 +385,9 @@ def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
     solute grows within this radius instead of radius - 5 (so it may reach
     through the sphere boundary); exclude_outside: every solute charge group
     whose switch atom lies beyond the exclusion radius is excluded;
-    exclude_water_radius: waters whose O lies beyond it are excluded too."""
+    exclude_water_radius: waters whose O lies beyond it are excluded too;
+    hydrogens: about half of the solute's terminal atoms become hydrogens
+    (mass 1.008, sigma 2.4 A, eps 0.022, own atom type and bond type)."""
     rng = np.random.default_rng(seed)
     # --- solute
     if n_solute > 0:
@@ -447,6 +449,22 @@ def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
         # keep groups exactly neutral after rounding
         for grp in groups0:
             crg[grp[-1]] -= round(float(crg[grp].sum()), 4)
+    hset = set()
+    if hydrogens and ns:
+        deg = np.zeros(ns, dtype=int)
+        for i, j in sb:
+            deg[i] += 1
+            deg[j] += 1
+        hset = set(a for a in range(1, ns) if deg[a] == 1 and rng.random() < 0.5)
+        if hset:
+            a_h, b_h = lj_sqrt_ab(2.4, 0.022)
+            a14, b14 = lj_sqrt_ab(0.8 * 2.4, 0.5 * 0.022)
+            avdw.append([a_h, a_h, a14])
+            bvdw.append([b_h, b_h, b14])
+            masses.append(1.008)
+            tac.append("HS")
+            for a in hset:
+                iac[a] = len(masses)
     iac[ns::3] = 1
     iac[ns + 1::3] = 2
     iac[ns + 2::3] = 2
@@ -465,6 +483,10 @@ def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
     # water: O-H1, O-H2, H1-H2 (last bond of the topology is H1-H2), one angle
     hh = 2 * R_OH * math.sin(math.radians(THETA_HOH) / 2)
     bondlib += [[1106.0, R_OH], [0.0, round(hh, 4)]]
+    if hset:
+        # solute bonds to a hydrogen: their own type (code 4)
+        bondlib.append([340.0, 1.53])
+        bonds = [(i, j, 4 if (i - 1 in hset or j - 1 in hset) else c) for i, j, c in bonds]
     anglib += [[200.0, THETA_HOH]]
     for w in range(nw):
         o = ns + 3 * w + 1
@@ -1015,6 +1037,7 @@ def write_inp(path: str, top: str, final: str, *, steps: int, stepsize: float = 
               polarisation_force: Optional[float] = 20.0,
               shell_force: Optional[float] = None, shell_radius: Optional[float] = None,
               excluded_freeze: bool = False, excluded_force: Optional[float] = None,
+              exclude_bonded: bool = False,
               temp_interval: Optional[int] = None, restraints: Optional[dict] = None,
               trajectory: Optional[str] = None, trajectory_interval: int = 0,
               trajectory_atoms: Optional[Sequence[str]] = None) -> None:
@@ -1042,6 +1065,8 @@ def write_inp(path: str, top: str, final: str, *, steps: int, stepsize: float = 
         L.append("excluded_force %.6g" % excluded_force)
     if excluded_freeze:
         L.append("excluded_freeze on")
+    if exclude_bonded:
+        L.append("exclude_bonded on")
     L += ["", "[solvent]"]
     if radial_force is not None:
         L.append("radial_force %.6g" % radial_force)

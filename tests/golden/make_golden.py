@@ -120,6 +120,24 @@ CASES = {
                                dict(steps=30, lambdas=[0.6, 0.4], stepsize=0.5, temperature=50.0,
                                     bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
                                     energy_interval=1, excluded_freeze=True)),
+    # solute hydrogens (non-heavy atoms): SHAKE of bonds to hydrogen with one
+    # Berendsen scale for solute and solvent; hydrogens with sequence
+    # restraints, Q-atom types and solute SHAKE; bonded terms of excluded
+    # atoms removed (exclude_bonded)
+    "c11_r12_hyd_shake_30": (dict(radius=12.0, n_solute=60, seed=2801, n_q=8, n_states=2, hydrogens=True),
+                             dict(steps=30, lambdas=[0.6, 0.4], stepsize=0.5, temperature=50.0,
+                                  bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
+                                  energy_interval=1, shake_hydrogens=True, separate_scaling=False)),
+    "c11_r12_hyd_rst_30": (dict(radius=12.0, n_solute=60, seed=2802, n_q=8, n_states=2, hydrogens=True,
+                                restraints=True, q_types=True, softcore=30.0),
+                           dict(steps=30, lambdas=[0.6, 0.4], stepsize=0.5, temperature=50.0,
+                                bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
+                                energy_interval=1, shake_solute=True, shake_hydrogens=True)),
+    "c11_r12_hyd_exclbonded_30": (dict(radius=12.0, n_solute=120, seed=2803, n_q=8, n_states=2, hydrogens=True,
+                                       solute_rmax=16.0, exclude_outside=True),
+                                  dict(steps=30, lambdas=[0.6, 0.4], stepsize=0.5, temperature=50.0,
+                                       bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
+                                       energy_interval=1, exclude_bonded=True)),
     # a breaking bond with non-Q neighbours: topology torsions through it
     # become Q terms (qatom_load_fep's completion); periodic impropers
     "c6_r12_q6s2_qbreak_30": (dict(radius=12.0, n_solute=40, seed=2202, n_q=6, n_states=2, q_types=True,

@@ -24,7 +24,10 @@ bitwise to reference qdyn on the same systems by test_oracle_reference.py):
 * excluded atoms (solute groups beyond the exclusion radius, waters near
   the boundary) and the restrained shell (fix_shell, src/md.f90:2816-2862),
   with and without excluded_freeze: step 0 within 1e-12, 30 FP64 steps
-  within 1e-9, mixed precision gradient within 1e-5 (RMS).
+  within 1e-9, mixed precision gradient within 1e-5 (RMS);
+* solute hydrogens: SHAKE of bonds to hydrogen with one Berendsen scale,
+  sequence restraints with hydrogens and solute SHAKE, exclude_bonded:
+  30 FP64 steps within 1e-9.
 """
 import numpy as np
 import pytest
@@ -246,3 +249,34 @@ def test_excluded_atoms(tmp_path, freeze):
     eo, eg, do, dg = _compare(paths, qe.QGPU_MIXED)
     rms = np.sqrt(np.mean(do ** 2))
     assert np.sqrt(np.mean((dg - do) ** 2)) <= 1e-5 * rms
+
+
+HYDROGENS = {
+    "shake_h": (dict(radius=12.0, n_solute=60, seed=2801, n_q=8, n_states=2, hydrogens=True),
+                dict(shake_hydrogens=True, separate_scaling=False)),
+    "rst_h": (dict(radius=12.0, n_solute=60, seed=2802, n_q=8, n_states=2, hydrogens=True, restraints=True,
+                   q_types=True, softcore=30.0), dict(shake_solute=True, shake_hydrogens=True)),
+    "exclude_bonded": (dict(radius=12.0, n_solute=120, seed=2803, n_q=8, n_states=2, hydrogens=True,
+                            solute_rmax=16.0, exclude_outside=True), dict(exclude_bonded=True)),
+}
+
+
+@pytest.mark.parametrize("name", sorted(HYDROGENS))
+def test_hydrogen_variants(tmp_path, name):
+    _say("case: solute hydrogens (%s), R=12, FP64 step 0 and 30 steps" % name)
+    sysargs, inpargs = HYDROGENS[name]
+    s = gensys.solvated_system(**sysargs)
+    paths = gensys.write_system(s, str(tmp_path), name, steps=30, lambdas=[0.6, 0.4], **COOL, **inpargs)
+    eo, eg, do, dg = _compare(paths)
+    for t in TERMS:
+        assert abs(eg[t] - eo[t]) <= 1e-12 * max(1.0, abs(eo[t])), (t, eg[t], eo[t])
+    assert np.abs(dg - do).max() <= 1e-12 * np.abs(do).max()
+    o = Oracle(paths["inp"])
+    o.run(30)
+    e = qe.Engine([paths["inp"]], qe.QGPU_FP64)
+    e.step(30)
+    xo, vo = o.state()
+    xg, vg = e.state(0)
+    assert np.abs(xg - xo).max() <= 1e-9 * np.abs(xo).max()
+    assert np.abs(vg - vo).max() <= 1e-9 * np.abs(vo).max()
+    e.close()
