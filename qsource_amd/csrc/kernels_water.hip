@@ -415,10 +415,14 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(co
     const double dr0 = S.x[3 * i] - S.xtop[3 * i], dr1 = S.x[3 * i + 1] - S.xtop[3 * i + 1],
                  dr2 = S.x[3 * i + 2] - S.xtop[3 * i + 2];
     e_fix = 0.5 * fk * (dr0 * dr0 + dr1 * dr1 + dr2 * dr2);
-    const int a = sub - 4;
-    g[3 * a] = g[3 * a] + fk * dr0;
-    g[3 * a + 1] = g[3 * a + 1] + fk * dr1;
-    g[3 * a + 2] = g[3 * a + 2] + fk * dr2;
+    // constant indices only: a lane-dependent index would move g to scratch
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      if (sub - 4 == a) {
+        g[3 * a] = g[3 * a] + fk * dr0;
+        g[3 * a + 1] = g[3 * a + 1] + fk * dr1;
+        g[3 * a + 2] = g[3 * a + 2] + fk * dr2;
+      }
   }
   for (int k = 0; k < 9; ++k) g[k] = seg8_sum(g[k]);
   e_el = seg8_sum(e_el);
