@@ -34,6 +34,7 @@ template <typename P, int MAPPED, int PF> __global__ void k_solute_forces(const 
 template <typename P, int NS> __global__ void k_q_forces(const DevSys*, int);
 __global__ void k_q_reduce(const DevSys*, int);
 __global__ void k_q_lists(const DevSys*);
+__global__ void k_term_sum(const DevSys*);
 __global__ void k_freeze_x(const DevSys*);
 __global__ void k_terms(const DevSys*);
 __global__ void k_term_gather(const DevSys*);
@@ -738,6 +739,13 @@ void Batch::upload(int k) {
     D.slot_g = I->alloc<double>((size_t)std::max(1, slot) * 3);
     D.nslot_atoms = (int)sa.size();
     D.slot_atom = I->upload(sa); D.slot_first = I->upload(sf); D.slot_pos = I->upload(pos);
+    {
+      // the integrator gathers the slots of the atoms it moves
+      std::vector<int> of(N, -1);
+      for (size_t k = 0; k < sa.size(); ++k) of[sa[k]] = (int)k;
+      D.slot_of_atom = I->upload(of);
+      D.slot_sum = I->alloc<double>((size_t)std::max<size_t>(1, sa.size()) * 3);
+    }
     max_terms_ = std::max(max_terms_, D.nbnd + D.nang + D.ntor + D.nimp + D.nqq + D.nxt);
     max_slot_atoms_ = std::max(max_slot_atoms_, std::max(D.nslot_atoms, D.nqbond));
   }
@@ -1045,7 +1053,7 @@ void Batch::launch_lists() {
   join();
 }
 
-void Batch::launch_forces() {
+void Batch::launch_forces(bool step) {
   const int ni = (int)h_sys_.size();
   // excluded_freeze: the coordinates the terms after fix_shell see
   if (freeze_)
@@ -1057,6 +1065,8 @@ void Batch::launch_forces() {
   if (max_ncgp_solute_ > 0) { pb(4); launch_tuned(TK_SOLUTE_FORCES, shape_[TK_SOLUTE_FORCES]); pe(4); }
   if (max_terms_ > 0)
     { pb(5); hipLaunchKernelGGL(k_terms, dim3((max_terms_ + 255) / 256, ni), dim3(256), 0, ls_, d_sys_); pe(5); }
+  if (max_slot_atoms_ > 0 && step)
+    { pb(7); hipLaunchKernelGGL(k_term_sum, dim3((max_slot_atoms_ + 255) / 256, ni), dim3(256), 0, ls_, d_sys_); pe(7); }
   if (max_nqat_ > 0) {
     dim3 gq((max_nqat_ * kQChunks * 64 + 255) / 256, ni);
     auto q_fp64 = [&](int shadow) {
@@ -1098,7 +1108,9 @@ void Batch::launch_forces() {
   join();
   // term slots may hold water atoms (unconstrained water bonds): gathered
   // after both branches
-  if (max_slot_atoms_ > 0)
+  // within a step the integrator adds the term slots itself (one launch
+  // less on the step's critical path)
+  if (max_slot_atoms_ > 0 && !step)
     { pb(7); hipLaunchKernelGGL(k_term_gather, dim3((max_slot_atoms_ + 255) / 256, ni), dim3(256), 0, stream_, d_sys_); pe(7); }
   if (far_model_) launch_far_model();
 }
@@ -1173,11 +1185,11 @@ void Batch::launch_integrate() {
 }
 
 void Batch::make_lists() { launch_lists(); }
-void Batch::forces() { launch_forces(); }
+void Batch::forces() { launch_forces(false); }
 
 void Batch::one_step() {
   if (istep_ % nbcycle_ == 0) launch_lists();
-  launch_forces();
+  launch_forces(true);
   launch_integrate();
   ++istep_;
 }
@@ -1191,7 +1203,7 @@ void Batch::launch_cycle_graph() {
     HIPCHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < nbcycle_; ++k) {
       if (k == 0) launch_lists();
-      launch_forces();
+      launch_forces(true);
       launch_integrate();
     }
     HIPCHECK(hipStreamEndCapture(stream_, &g));

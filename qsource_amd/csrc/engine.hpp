@@ -82,6 +82,8 @@ struct DevSys {
   int precision;                 // 0 = FP64, 1 = mixed FP32 pair / FP64 accumulation
   double *x, *v, *xx, *d, *xtop;
   float* xf;                     // FP32 copy of x for the mixed-precision pair kernels
+  const int* slot_of_atom;       // N: the atom's index in slot_atom, or -1
+  double* slot_sum;              // nslot_atoms x 3: each slot atom's summed slots (k_term_sum)
   double* xq;                    // coordinates of the terms after fix_shell: x, or with
                                  // excluded_freeze a copy holding the excluded atoms at xtop
   const double *crg, *winv, *mass;
@@ -353,7 +355,7 @@ class Batch {
   int* d_step_ = nullptr;
   void upload(int k);
   void launch_lists();
-  void launch_forces();
+  void launch_forces(bool step);
   void launch_tuned(int kernel, const LaunchShape& sh);
   const std::pair<int2*, int>& work_map(int kernel, int map, int block);
   LaunchShape shape_[TK_COUNT];

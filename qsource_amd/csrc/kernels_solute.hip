@@ -1188,6 +1188,27 @@ __global__ __launch_bounds__(256) void k_terms(const DevSys* __restrict__ sys) {
   if (n < S.nxt) xterm(S, n);
 }
 
+// Within a step: each slot atom's term slots summed in slot order into
+// slot_sum (the integrator adds it to d), and the Q bonds' Morse factors of
+// this evaluation kept for the next one.  Runs right after k_terms, beside
+// the water forces.
+__global__ __launch_bounds__(256) void k_term_sum(const DevSys* __restrict__ sys) {
+  const DevSys& S = sys[blockIdx.y];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < S.nqbond)
+    for (int c = 0; c < kMorseRec; ++c) S.qb_prev[kMorseRec * t + c] = S.qb_cur[kMorseRec * t + c];
+  if (t >= S.nslot_atoms) return;
+  double g0 = 0, g1 = 0, g2 = 0;
+  const double* o = S.slot_g + 3 * (size_t)S.slot_first[t];
+  const int n = S.slot_first[t + 1] - S.slot_first[t];
+  for (int k = 0; k < n; ++k) {
+    g0 = g0 + o[3 * k]; g1 = g1 + o[3 * k + 1]; g2 = g2 + o[3 * k + 2];
+  }
+  S.slot_sum[3 * t] = g0;
+  S.slot_sum[3 * t + 1] = g1;
+  S.slot_sum[3 * t + 2] = g2;
+}
+
 // Adds each atom's term slots to its gradient, in slot order, and keeps the
 // Q bonds' Morse factors of this evaluation for the next one.
 __global__ __launch_bounds__(256) void k_term_gather(const DevSys* __restrict__ sys) {

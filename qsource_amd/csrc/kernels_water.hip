@@ -591,6 +591,18 @@ __global__ __launch_bounds__(1024) void k_watpol_force(const DevSys* __restrict_
 // (shake :17581-17679), v = (x - xx)/dt, and the kinetic-energy partial
 // sums of temperature (:6098-6190), fused per SHAKE unit.  Units are SHAKE
 // molecules (waters: 3 atoms, 3 constraints) or single atoms.
+// The gradient of atom a as the step uses it: d plus the sum of the atom's
+// bonded-term slots (k_term_sum, formed beside the water forces); the sum
+// is also stored in d.
+__device__ __forceinline__ void gather_slots(const DevSys& S, int a) {
+  const int t = S.slot_of_atom[a];
+  if (t < 0) return;
+  const double* g = S.slot_sum + 3 * (size_t)t;
+  S.d[3 * a] = S.d[3 * a] + g[0];
+  S.d[3 * a + 1] = S.d[3 * a + 1] + g[1];
+  S.d[3 * a + 2] = S.d[3 * a + 2] + g[2];
+}
+
 __global__ __launch_bounds__(256) void k_integrate(const DevSys* __restrict__ sys) {
   const DevSys& S = sys[blockIdx.y];
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
@@ -603,6 +615,7 @@ __global__ __launch_bounds__(256) void k_integrate(const DevSys* __restrict__ sy
     const double ts = a < S.nat_solute ? S.scal[0] : S.scal[1];
     double x[9], xo[9], v[9], w[3];
     for (int k = 0; k < 3; ++k) w[k] = S.winv[a + k];
+    for (int k = 0; k < 3; ++k) gather_slots(S, a + k);
     for (int c = 0; c < 9; ++c) {
       const int ic = 3 * a + c;
       // excluded_freeze: an excluded atom restarts from rest at its
@@ -660,6 +673,7 @@ __global__ __launch_bounds__(256) void k_integrate(const DevSys* __restrict__ sy
     const int a0 = S.unit_first[u], a1 = S.unit_first[u + 1];
     for (int k = a0; k < a1; ++k) {
       const int i = S.unit_atoms[k];
+      gather_slots(S, i);
       const double ts = i < S.nat_solute ? ts_solute : ts_solvent;
       const double w = S.winv[i];
       const bool frz = S.freeze && (S.flags[i] & F_EXCL);
