@@ -170,6 +170,12 @@ void Batch::init(const std::vector<std::string>& inputs, int device) {
     upload((int)k);
     h_sys_[k].step_ctr = d_step_;
   }
+  // one kinetic-energy partial per integrator workgroup of the widest instance
+  const int kblocks = (std::max(max_units_, 1) + kIntegrateBlock - 1) / kIntegrateBlock;
+  for (size_t k = 0; k < h_sys_.size(); ++k) {
+    h_sys_[k].ke_part = inst_[k]->alloc<double>((size_t)kblocks * 6);
+    h_sys_[k].ke_blocks = kblocks;
+  }
   HIPCHECK(hipMalloc(&d_sys_, h_sys_.size() * sizeof(DevSys)));
   sync_sys();
   refresh_xf();
@@ -806,7 +812,6 @@ void Batch::upload(int k) {
   D.e_wpol = I->alloc<double>(std::max(1, D.nshell));
   D.e_far = I->alloc<double>(std::max(1, D.ncgp));
   I->ke_blocks = 0;
-  D.ke_part = nullptr;  // sized after all instances are known (see below)
   std::vector<double> scal(8, 0.0);
   initial_scales(s, scal.data());
   D.scal = I->upload(scal);
@@ -815,12 +820,10 @@ void Batch::upload(int k) {
   D.energies = I->upload(e0);
   D.qenergies = I->alloc<double>((size_t)std::max(1, s.nstates) * kQTerms);
   I->natom = N; I->nwat = s.nwat;
-  // the kinetic partial buffer needs the batch-wide unit count: allocate
-  // generously from this instance's own count, re-checked at launch
-  int blocks = (std::max(D.nunits, 1) + 255) / 256;
-  I->ke_blocks = blocks;
-  D.ke_part = I->alloc<double>((size_t)blocks * 6 + 6 * 4096);
-  D.ke_blocks = blocks + 4096;
+  // the kinetic partial buffer needs the batch-wide unit count: allocated
+  // once every instance is known (Batch::init)
+  D.ke_part = nullptr;
+  D.ke_blocks = 0;
 }
 
 // Items and lanes per item of the tuned kernels, per instance, and the
@@ -1177,10 +1180,10 @@ void Batch::set_far_model(Mlp* m) {
 
 void Batch::launch_integrate() {
   const int ni = (int)h_sys_.size();
-  const int blocks = (std::max(max_units_, 1) + 255) / 256;
+  const int blocks = (std::max(max_units_, 1) + kIntegrateBlock - 1) / kIntegrateBlock;
   // energy-file frame of this step (the kernel checks the step counter)
   if (iene_ > 0) { pb(10); hipLaunchKernelGGL(k_energy_reduce, dim3(1, ni), dim3(256), 0, stream_, d_sys_, 1); pe(10); }
-  { pb(11); hipLaunchKernelGGL(k_integrate, dim3(blocks, ni), dim3(256), 0, stream_, d_sys_); pe(11); }
+  { pb(11); hipLaunchKernelGGL(k_integrate, dim3(blocks, ni), dim3(kIntegrateBlock), 0, stream_, d_sys_); pe(11); }
   { pb(12); hipLaunchKernelGGL(k_temperature, dim3(1, ni), dim3(256), 0, stream_, d_sys_, blocks); pe(12); }
 }
 

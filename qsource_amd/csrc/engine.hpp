@@ -69,6 +69,7 @@ constexpr int kXtPars = 8;
 constexpr int kMorseRec = 7;   // Morse factor and its gradient on the bond's two atoms
 constexpr int kQwRec = 8;      // Q-water record per Q atom (qw_rec_d / qw_rec_f)
 constexpr int kEWater = 5;     // energy partials per water
+constexpr int kIntegrateBlock = 64;  // threads per k_integrate workgroup
 
 struct DevShell {
   double rout, dr, cstb, avtheta, avn_insh, theta_corr;

@@ -603,7 +603,7 @@ __device__ __forceinline__ void gather_slots(const DevSys& S, int a) {
   S.d[3 * a + 2] = S.d[3 * a + 2] + g[2];
 }
 
-__global__ __launch_bounds__(256) void k_integrate(const DevSys* __restrict__ sys) {
+__global__ __launch_bounds__(kIntegrateBlock) void k_integrate(const DevSys* __restrict__ sys) {
   const DevSys& S = sys[blockIdx.y];
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
   double ke[6] = {0, 0, 0, 0, 0, 0};  // solute tot, solute free, solvent tot, solvent free, spare
@@ -737,9 +737,9 @@ __global__ __launch_bounds__(256) void k_integrate(const DevSys* __restrict__ sy
       else { ke[2] += ek; if (!ex) ke[3] += ek; }
     }
   }
-  __shared__ double red[6][4];
+  __shared__ double red[6][kIntegrateBlock / 64];
   for (int c = 0; c < 4; ++c) {
-    double s = block_sum_256(ke[c], red[c]);
+    double s = block_sum<kIntegrateBlock / 64>(ke[c], red[c]);
     if (threadIdx.x == 0) S.ke_part[(size_t)blockIdx.x * 6 + c] = s;
   }
 }
