@@ -208,6 +208,7 @@ void Batch::set_energy_interval(int iene, int max_frames) {
     D.eseries_n = inst_[k]->alloc<int>(1);
   }
   iene_ = iene;
+  ebase_ = istep_;
   sync_sys();
   if (graph_exec_) {
     HIPCHECK(hipGraphExecDestroy(graph_exec_));
@@ -1056,7 +1057,17 @@ void Batch::launch_lists() {
   join();
 }
 
-void Batch::launch_forces(bool step) {
+// Whether step s can be an energy-file step, for leaving the frame kernels
+// out of the launches of other steps (they check the step counter on the
+// device too).  A captured cycle is replayed every NBcycle steps, so its
+// frame positions are fixed only when iene divides NBcycle.
+bool Batch::record_possible(long s) const {
+  if (iene_ <= 0) return false;
+  if (use_graph_ && nbcycle_ % iene_ != 0) return true;
+  return (s - ebase_) % iene_ == 0;
+}
+
+void Batch::launch_forces(bool step, long s) {
   const int ni = (int)h_sys_.size();
   // excluded_freeze: the coordinates the terms after fix_shell see
   if (freeze_)
@@ -1091,7 +1102,7 @@ void Batch::launch_forces(bool step) {
     // mixed precision with energy frames: the Q energies of each recorded
     // frame are also evaluated with FP64 pair arithmetic (the kernels return
     // at once on other steps)
-    if (precision_ == 1 && iene_ > 0) {
+    if (precision_ == 1 && record_possible(s)) {
       q_fp64(1);
       hipLaunchKernelGGL(k_q_reduce, gr, dim3(64), 0, ls_, d_sys_, 1);
     }
@@ -1178,22 +1189,22 @@ void Batch::set_far_model(Mlp* m) {
   }
 }
 
-void Batch::launch_integrate() {
+void Batch::launch_integrate(long s) {
   const int ni = (int)h_sys_.size();
   const int blocks = (std::max(max_units_, 1) + kIntegrateBlock - 1) / kIntegrateBlock;
   // energy-file frame of this step (the kernel checks the step counter)
-  if (iene_ > 0) { pb(10); hipLaunchKernelGGL(k_energy_reduce, dim3(1, ni), dim3(256), 0, stream_, d_sys_, 1); pe(10); }
+  if (record_possible(s)) { pb(10); hipLaunchKernelGGL(k_energy_reduce, dim3(1, ni), dim3(256), 0, stream_, d_sys_, 1); pe(10); }
   { pb(11); hipLaunchKernelGGL(k_integrate, dim3(blocks, ni), dim3(kIntegrateBlock), 0, stream_, d_sys_); pe(11); }
   { pb(12); hipLaunchKernelGGL(k_temperature, dim3(1, ni), dim3(256), 0, stream_, d_sys_, blocks); pe(12); }
 }
 
 void Batch::make_lists() { launch_lists(); }
-void Batch::forces() { launch_forces(false); }
+void Batch::forces() { launch_forces(false, istep_); }
 
 void Batch::one_step() {
   if (istep_ % nbcycle_ == 0) launch_lists();
-  launch_forces(true);
-  launch_integrate();
+  launch_forces(true, istep_);
+  launch_integrate(istep_);
   ++istep_;
 }
 
@@ -1206,8 +1217,8 @@ void Batch::launch_cycle_graph() {
     HIPCHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < nbcycle_; ++k) {
       if (k == 0) launch_lists();
-      launch_forces(true);
-      launch_integrate();
+      launch_forces(true, istep_ + k);
+      launch_integrate(istep_ + k);
     }
     HIPCHECK(hipStreamEndCapture(stream_, &g));
     HIPCHECK(hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0));

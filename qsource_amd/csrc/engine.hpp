@@ -351,19 +351,21 @@ class Batch {
   bool q_dynamic_ = false; // some instance rebuilds its Q lists at list updates
   bool freeze_ = false;    // some instance has excluded_freeze on
   int iene_ = 0;
+  long ebase_ = 0;         // step of the last set_energy_interval
   bool use_graph_ = true;
   hipGraphExec_t graph_exec_ = nullptr;
   int* d_step_ = nullptr;
   void upload(int k);
   void launch_lists();
-  void launch_forces(bool step);
+  void launch_forces(bool step, long s);
+  bool record_possible(long s) const;
   void launch_tuned(int kernel, const LaunchShape& sh);
   const std::pair<int2*, int>& work_map(int kernel, int map, int block);
   LaunchShape shape_[TK_COUNT];
   bool tuned_ = false;
   bool autotune_on_ = true;       // tune at the first step (QGPU_AUTOTUNE=0 switches it off)
   std::vector<std::pair<std::array<int, 3>, std::pair<int2*, int>>> maps_;
-  void launch_integrate();
+  void launch_integrate(long s);
   void ensure_far();
   void launch_far_model();
   class Mlp* far_model_ = nullptr;
