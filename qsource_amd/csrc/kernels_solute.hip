@@ -233,6 +233,8 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(c
     // with O, H1, H2 from the per-atom table
     const P* wlj = S.std_water ? solute_water_lj<P>(S) + 6 * (size_t)i : nullptr;
     const unsigned wzero = S.std_water ? S.swp_zero[i] : 0u;
+    // standard waters carry the solvent charges (checked at upload)
+    const P qw3[3] = {(P)S.chg_solv[0], (P)S.chg_solv[1], (P)S.chg_solv[2]};
     auto pw_water = [&](const int jo, const P* xw) {
       for (int ip = 0; ip < 3; ++ip) {
         const int j = jo + ip;
@@ -253,7 +255,7 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(c
         const P r2 = inv_r2(pnorm2(dx1, dx2, dx3), r);
         const P r6 = r2 * r2 * r2;
         const P r12 = r6 * r6;
-        const P Vel = qi * (P)S.crg[j] * r;
+        const P Vel = qi * (wlj ? qw3[ip] : (P)S.crg[j]) * r;
         const P Va = aLJ * r12;
         const P Vb = bLJ * r6;
         const P dv = r2 * (-Vel - (P)12 * Va + (P)6 * Vb);
