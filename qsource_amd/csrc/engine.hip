@@ -320,6 +320,9 @@ void Batch::upload(int k) {
   for (auto& ty : t.types) for (int j = 0; j < 3; ++j) { av.push_back(ty.avdw[j]); bv.push_back(ty.bvdw[j]); }
   D.avdw = I->upload(av);
   D.bvdw = I->upload(bv);
+  D.avdw_f = I->upload(std::vector<float>(av.begin(), av.end()));
+  D.bvdw_f = I->upload(std::vector<float>(bv.begin(), bv.end()));
+  D.crg_f = I->upload(std::vector<float>(s.crg.begin(), s.crg.end()));
   D.ljcod = I->upload(s.ljcod);
   D.el14_scale = t.el14_scale;
   // exclusion and 1-4 tables (solute rows) and long lists

@@ -95,6 +95,7 @@ struct DevSys {
   // type tables (natyps x 3 each), ljcod natyps x natyps
   int natyps;
   const double *avdw, *bvdw;
+  const float *avdw_f, *bvdw_f, *crg_f;   // FP32 copies for the mixed-precision solute pairs
   const int* ljcod;
   // solute exclusion / 1-4 bit rows (pairs at most kMaxNbrRange apart) and,
   // for pairs further apart, a per-atom CSR over the lower atom of each pair:

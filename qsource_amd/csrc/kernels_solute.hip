@@ -208,10 +208,21 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(c
         P r;
         const P r2 = inv_r2(pnorm2(dx1, dx2, dx3), r);
         const P r6 = r2 * r2 * r2;
-        P Vel = qi * (P)gcrg[j] * r;
+        P cj, ai, aj, bi, bj;
+        if constexpr (sizeof(P) == 4) {
+          // FP32 copies of the same values (the FP64 ones rounded once)
+          cj = S.crg_f[j];
+          ai = S.avdw_f[ti * 3 + l]; aj = S.avdw_f[tj * 3 + l];
+          bi = S.bvdw_f[ti * 3 + l]; bj = S.bvdw_f[tj * 3 + l];
+        } else {
+          cj = gcrg[j];
+          ai = S.avdw[ti * 3 + l]; aj = S.avdw[tj * 3 + l];
+          bi = S.bvdw[ti * 3 + l]; bj = S.bvdw[tj * 3 + l];
+        }
+        P Vel = qi * cj * r;
         if (lj == 3) Vel = Vel * (P)S.el14_scale;
-        const P Va = r6 * r6 * (P)S.avdw[ti * 3 + l] * (P)S.avdw[tj * 3 + l];
-        const P Vb = r6 * (P)S.bvdw[ti * 3 + l] * (P)S.bvdw[tj * 3 + l];
+        const P Va = r6 * r6 * ai * aj;
+        const P Vb = r6 * bi * bj;
         const P dv = r2 * (-Vel - (P)12 * Va + (P)6 * Vb);
         if (me_first) {
           g0 = g0 - (double)(dv * dx1);
