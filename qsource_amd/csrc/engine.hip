@@ -832,10 +832,11 @@ void Batch::upload(int k) {
 
 // Items and lanes per item of the tuned kernels, per instance, and the
 // relative work of an instance (for balancing instances over the XCDs).
-static void tuned_items(int kernel, const DevSys& D, int& items, int& lanes, double& work) {
+static void tuned_items(int kernel, const DevSys& D, bool f64, int& items, int& lanes, double& work) {
   switch (kernel) {
     case Batch::TK_WATER_LISTS:
-      items = D.nwat; lanes = 8; work = (double)D.nwat * (D.nwat + D.ncgp_solute); break;
+      items = D.nwat; lanes = f64 ? kWLLanes64 : kWLLanes32;
+      work = (double)D.nwat * (D.nwat + D.ncgp_solute); break;
     case Batch::TK_SOLUTE_LISTS:
       items = D.ncgp_solute; lanes = 64; work = (double)D.ncgp_solute * (D.ncgp_solute + D.nwat); break;
     case Batch::TK_WATER_FORCES:
@@ -861,7 +862,7 @@ const std::pair<int2*, int>& Batch::work_map(int kernel, int map, int block) {
   std::vector<double> work(ni);
   for (int k = 0; k < ni; ++k) {
     int items, lanes;
-    tuned_items(kernel, h_sys_[k], items, lanes, work[k]);
+    tuned_items(kernel, h_sys_[k], precision_ == 0, items, lanes, work[k]);
     const int ipb = block / lanes;
     for (int b = 0; b * ipb < items; ++b) per[k].push_back(make_int2(k, b * ipb));
   }
@@ -903,7 +904,7 @@ void Batch::launch_tuned(int kernel, const LaunchShape& sh) {
   for (int k = 0; k < ni; ++k) {
     int items, l;
     double w;
-    tuned_items(kernel, h_sys_[k], items, l, w);
+    tuned_items(kernel, h_sys_[k], precision_ == 0, items, l, w);
     if (w > 0) max_items = std::max(max_items, items);
     lanes = l;
   }

@@ -70,6 +70,8 @@ constexpr int kMorseRec = 7;   // Morse factor and its gradient on the bond's tw
 constexpr int kQwRec = 8;      // Q-water record per Q atom (qw_rec_d / qw_rec_f)
 constexpr int kEWater = 5;     // energy partials per water
 constexpr int kIntegrateBlock = 64;  // threads per k_integrate workgroup
+// lanes per water in k_water_lists: FP64 build, FP32 (mixed) build
+constexpr int kWLLanes64 = 8, kWLLanes32 = 16;
 
 struct DevShell {
   double rout, dr, cstb, avtheta, avn_insh, theta_corr;

@@ -103,6 +103,12 @@ __device__ __forceinline__ double seg8_sum(double v) {
   v += __shfl_xor(v, 4, 64);
   return v;
 }
+// the same over an aligned group of N lanes (N a power of two <= 64)
+template <int N>
+__device__ __forceinline__ double segn_sum(double v) {
+  for (int o = 1; o < N; o <<= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
 
 // True on a step whose energy-file frame is recorded (step counter a positive
 // multiple of iene since recording started) when the batch also keeps the
@@ -243,6 +249,13 @@ struct LrfAcc : LrfTerms<double> {
     for (int k = 0; k < 3; ++k) p1[k] = seg8_sum(p1[k]);
     for (int k = 0; k < 6; ++k) p2[k] = seg8_sum(p2[k]);
     for (int k = 0; k < 10; ++k) p3[k] = seg8_sum(p3[k]);
+  }
+  template <int N>
+  __device__ void segn_reduce() {
+    p0 = segn_sum<N>(p0);
+    for (int k = 0; k < 3; ++k) p1[k] = segn_sum<N>(p1[k]);
+    for (int k = 0; k < 6; ++k) p2[k] = segn_sum<N>(p2[k]);
+    for (int k = 0; k < 10; ++k) p3[k] = segn_sum<N>(p3[k]);
   }
   __device__ void wave_reduce() {
     p0 = wave_sum(p0);

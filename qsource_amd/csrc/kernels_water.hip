@@ -52,10 +52,11 @@ __global__ void k_freeze_x(const DevSys* __restrict__ sys) {
 
 // ---------------------------------------------------------------------------
 // Water lists (nbwwlist_lrf src/md.f90:10897-11021 and the water side of
-// nbpwlist_lrf :9300-9429).  Eight waters per wavefront, eight lanes per
-// water: each iteration tests the wavefront's 8 waters against 8 partner
-// waters (or solute charge groups), so every partner's coordinates are read
-// once for 8 waters.  Partners inside the cut-off are appended to the water's
+// nbpwlist_lrf :9300-9429).  kWLLanes lanes per water (8 in the FP64
+// build, 16 in the FP32 one, whose per-lane state is smaller): each
+// iteration tests the wavefront's waters against kWLLanes partner waters
+// (or solute charge groups), so every partner's coordinates are read once
+// for all the wavefront's waters.  Partners inside the cut-off are appended to the water's
 // row in ascending order (ballot over the water's 8 lanes); partners between
 // the cut-off and RcLRF add their non-Q atoms' multipoles about this water's
 // charge-group centre (20 FP64 accumulators per lane, summed over the 8
@@ -64,14 +65,15 @@ __global__ void k_freeze_x(const DevSys* __restrict__ sys) {
 // membership is decided in FP64 in both.
 template <typename P, int MAPPED>
 __global__ __launch_bounds__(256) void k_water_lists(const DevSys* __restrict__ sys, const int2* __restrict__ map) {
+  constexpr int kWLLanes = sizeof(P) == 4 ? kWLLanes32 : kWLLanes64;
   const DevSys* Sp;
   int base;
-  if (!block_work<MAPPED>(sys, map, blockDim.x >> 3, Sp, base)) return;
+  if (!block_work<MAPPED>(sys, map, blockDim.x / kWLLanes, Sp, base)) return;
   const DevSys& S = *Sp;
   const int t = threadIdx.x;
-  if (base + ((t >> 6) << 3) >= S.nwat) return;   // whole wavefronts past the last water
-  const int lane = t & 63, sub = t & 7, seg = lane & ~7;
-  const int iw = base + (t >> 3);
+  if (base + (t >> 6) * (64 / kWLLanes) >= S.nwat) return;   // whole wavefronts past the last water
+  const int lane = t & 63, sub = t & (kWLLanes - 1), seg = lane & ~(kWLLanes - 1);
+  const int iw = base + t / kWLLanes;
   const gconst<double> gx = gptr(S.x);
   const gconst<unsigned char> gfl = gptr(S.flags);
   const int io = S.nat_solute + 3 * (iw < S.nwat ? iw : 0);
@@ -82,13 +84,13 @@ __global__ __launch_bounds__(256) void k_water_lists(const DevSys* __restrict__ 
   const double c0 = Lg[0], c1 = Lg[1], c2 = Lg[2];
   const float cf[3] = {(float)c0, (float)c1, (float)c2};
   const unsigned long long below = (1ull << lane) - 1ull;
-  const unsigned long long mine = 0xffull << seg;
+  const unsigned long long mine = ((1ull << kWLLanes) - 1ull) << seg;
   LrfAcc acc;
   acc.zero();
   // waters
   int n_in = 0;
   auto row = gptr_w(S.ww_nb + (size_t)(iw < S.nwat ? iw : 0) * S.ww_cap);
-  for (int base = 0; base < S.nwat; base += 8) {
+  for (int base = 0; base < S.nwat; base += kWLLanes) {
     const int jw = base + sub;
     bool inside = false, far = false;
     if (active && jw < S.nwat && jw != iw) {
@@ -115,7 +117,7 @@ __global__ __launch_bounds__(256) void k_water_lists(const DevSys* __restrict__ 
   // solute charge groups (water side of the solute-water list)
   int n_wp = 0;
   auto wrow = gptr_w(S.wp_nb + (size_t)(iw < S.nwat ? iw : 0) * S.wp_cap);
-  for (int base = 0; base < S.ncgp_solute; base += 8) {
+  for (int base = 0; base < S.ncgp_solute; base += kWLLanes) {
     const int g = base + sub;
     bool inside = false, far = false;
     if (active && g < S.ncgp_solute) {
@@ -136,7 +138,7 @@ __global__ __launch_bounds__(256) void k_water_lists(const DevSys* __restrict__ 
     n_wp += __popcll(m);
     if (far && S.use_lrf) lrf_gather_group<P>(S, g, c0, c1, c2, acc);
   }
-  acc.seg8_reduce();
+  acc.segn_reduce<kWLLanes>();
   if (iw < S.nwat && sub == 0) {
     if (!active) {
       S.ww_cnt[iw] = 0;
