@@ -223,6 +223,15 @@ int qgpu_set_launch_shape(qgpu_handle* h, int kernel, int map, int block, int pr
  * (OFFDIAG_SAVE, src/nrgy.f90:73-78); *n = number of functions. */
 int qgpu_offdiag_states(qgpu_handle* h, int k, int* ij, int cap, int* n);
 
+/* Energies of the FEP file's monitored group pairs ([monitor_groups],
+ * [monitor_group_pairs]) at the current coordinates of instance k; replaces
+ * md_run's `call nonbond_monitor` at output steps (src/md.f90:6562-6564,
+ * nonbond_monitor :11228-11350; printed by write_out :18653-18663).  Per pair:
+ * Vwsum, Vwel, Vwvdw (lambda-weighted), then Vel and Vvdw of every state
+ * (3 + 2 nstates values).  *n receives the number of values; out is written
+ * when cap >= *n. */
+int qgpu_monitor(qgpu_handle* h, int k, double* out, int cap, int* n);
+
 /* ---- qdyn's output files (host only; no device needed) ------------------
  * Trajectory: replaces trj_initialize + trj_commit_mask + trj_create of
  * init_trj (src/md.f90:16668-16684, src/trj.f90:72-204) for the run of

@@ -104,6 +104,17 @@ class Oracle:
         n = self.lib.oracle_get_qenergies(self.h, _dp(e), ctypes.c_int(len(e)))
         return e[:n].reshape(-1, 12)
 
+    def monitor(self) -> np.ndarray:
+        """Monitored group pairs at the current coordinates: per pair Vwsum,
+        Vwel, Vwlj, then Vel and Vlj of every state."""
+        cap = 4096
+        out = np.zeros(cap)
+        n = self.lib.oracle_monitor(self.h, _dp(out), ctypes.c_int(cap))
+        if n < 0:
+            raise RuntimeError("too many monitored pairs")
+        ns = max(self.nstates, 1)
+        return out[:n].reshape(-1, 3 + 2 * ns)
+
     def list_sizes(self):
         n = (ctypes.c_long * 5)()
         self.lib.oracle_list_sizes(self.h, n)

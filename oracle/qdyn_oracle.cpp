@@ -1645,6 +1645,97 @@ thread_local std::string g_err;
 }  // namespace
 
 // ------------------------------------------------------------------ C API
+// nonbond_monitor (src/md.f90:11228-11350) at the current coordinates: per
+// monitored pair Vwsum, Vwel, Vwlj, then (Vel, Vlj) of every state.  The
+// reference's working variables (charges, type indices, r6) are carried
+// from one state or partner to the next exactly as there.
+static int oracle_monitor_eval(const Setup& s, const double* x, double* out, int cap) {
+  const int ns = std::max(1, s.nstates);
+  const int stride = 3 + 2 * ns;
+  const int np = (int)s.mon.size();
+  if (np * stride > cap) return -np * stride;
+  const auto& T = s.top;
+  for (int par = 0; par < np; ++par) {
+    const auto& m = s.mon[par];
+    const int ni = (int)m.ai.size(), nj = (int)m.aj.size();
+    std::vector<double> Vel(ns, 0.0), Vlj(ns, 0.0);
+    for (int i = 0; i < ni; ++i) {
+      const int atomi = m.ai[i];
+      const int qatomi = s.iqatom[atomi];
+      double qi = s.crg[atomi];
+      int iaci = s.iac[atomi];
+      for (int j = 0; j < nj; ++j) {
+        const int atomj = m.aj[j];
+        const int qatomj = s.iqatom[atomj];
+        int iacj = s.iac[atomj];
+        double qj = s.crg[atomj];
+        const double dx1 = x[3 * atomi] - x[3 * atomj], dx2 = x[3 * atomi + 1] - x[3 * atomj + 1],
+                     dx3 = x[3 * atomi + 2] - x[3 * atomj + 2];
+        const double r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
+        const double r = std::sqrt(1.0 / r2);
+        double r6 = r2 * r2 * r2;
+        const double r6_hc = r6;
+        r6 = 1.0 / r6;
+        int sc_1 = 0, sc_2 = 0;
+        for (int st = 0; st < ns; ++st) {
+          bool do_sc = false;
+          const int lj = m.ljcod[((size_t)st * ni + i) * nj + j];
+          // topology parameters of the (possibly carried-over) type indices;
+          // with Q-atom types a Q atom's are replaced below
+          auto tav = [&](int ty, int c) { return ty < (int)T.types.size() ? T.types[ty].avdw[c] : 0.0; };
+          auto tbv = [&](int ty, int c) { return ty < (int)T.types.size() ? T.types[ty].bvdw[c] : 0.0; };
+          double aLJi = tav(iaci, lj - 1), bLJi = tbv(iaci, lj - 1);
+          double aLJj = tav(iacj, lj - 1), bLJj = tbv(iacj, lj - 1);
+          if (qatomi) qi = s.qcrg[(qatomi - 1) * s.nstates + st];
+          if (qatomj) qj = s.qcrg[(qatomj - 1) * s.nstates + st];
+          if (s.qvdw_flag) {
+            if (qatomi) {
+              iaci = s.qiac[(qatomi - 1) * s.nstates + st];
+              aLJi = s.qavdw[3 * iaci + lj - 1];
+              bLJi = s.qbvdw[3 * iaci + lj - 1];
+              do_sc = true;
+              sc_1 = qatomi - 1;
+            } else {
+              sc_2 = iaci;
+            }
+            if (qatomj) {
+              iacj = s.qiac[(qatomj - 1) * s.nstates + st];
+              aLJj = s.qavdw[3 * iacj + lj - 1];
+              bLJj = s.qbvdw[3 * iacj + lj - 1];
+              if (do_sc) {
+                sc_2 = qatomj - 1 + s.natyps;
+              } else {
+                do_sc = true;
+                sc_1 = qatomj - 1;
+              }
+            } else {
+              sc_2 = iacj;
+            }
+            if (do_sc) {
+              r6 = r6_hc + s.sc_lookup[((size_t)sc_1 * (s.natyps + s.nqat) + sc_2) * s.nstates + st];
+              r6 = 1.0 / r6;
+            }
+          }
+          const double V = qi * qj * r;
+          const double Vv = aLJi * aLJj * r6 * r6 - bLJi * bLJj * r6;
+          Vel[st] = Vel[st] + V;
+          Vlj[st] = Vlj[st] + Vv;
+        }
+      }
+    }
+    double wel = 0.0, wlj = 0.0;
+    for (int st = 0; st < ns; ++st) {
+      const double lam = st < (int)s.lambda.size() ? s.lambda[st] : 1.0;
+      wel = wel + Vel[st] * lam;
+      wlj = wlj + Vlj[st] * lam;
+    }
+    double* o = out + (size_t)par * stride;
+    o[0] = wlj + wel; o[1] = wel; o[2] = wlj;
+    for (int st = 0; st < ns; ++st) { o[3 + 2 * st] = Vel[st]; o[4 + 2 * st] = Vlj[st]; }
+  }
+  return np * stride;
+}
+
 extern "C" {
 
 void* oracle_create(const char* inp) {
@@ -1755,5 +1846,9 @@ void oracle_theta_corr(void* h, double* out) {
   for (size_t k = 0; k < o->s.wshell.size(); ++k) out[k] = o->s.wshell[k].theta_corr;
 }
 int oracle_nshells(void* h) { return (int)static_cast<Oracle*>(h)->s.wshell.size(); }
+int oracle_monitor(void* h, double* out, int cap) {
+  Oracle* o = static_cast<Oracle*>(h);
+  return oracle_monitor_eval(o->s, o->x.data(), out, cap);
+}
 
 }  // extern "C"

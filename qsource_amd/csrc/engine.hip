@@ -34,6 +34,7 @@ template <typename P, int MAPPED, int PF> __global__ void k_solute_forces(const 
 template <typename P, int NS> __global__ void k_q_forces(const DevSys*, int);
 __global__ void k_q_reduce(const DevSys*, int);
 __global__ void k_q_lists(const DevSys*);
+__global__ void k_monitor(const DevSys*);
 __global__ void k_term_sum(const DevSys*);
 __global__ void k_freeze_x(const DevSys*);
 __global__ void k_terms(const DevSys*);
@@ -754,6 +755,25 @@ void Batch::upload(int k) {
       std::vector<int> of(N, -1);
       for (size_t k = 0; k < sa.size(); ++k) of[sa[k]] = (int)k;
       D.slot_of_atom = I->upload(of);
+    }
+    {
+      // monitored group pairs
+      std::vector<int> at, fi, fj, ni, nj, lo;
+      std::vector<signed char> lj;
+      for (const auto& m : s.mon) {
+        fi.push_back((int)at.size()); at.insert(at.end(), m.ai.begin(), m.ai.end());
+        fj.push_back((int)at.size()); at.insert(at.end(), m.aj.begin(), m.aj.end());
+        ni.push_back((int)m.ai.size()); nj.push_back((int)m.aj.size());
+        lo.push_back((int)lj.size());
+        for (int c : m.ljcod) lj.push_back((signed char)c);
+      }
+      D.mon_n = (int)s.mon.size();
+      D.iqatom_d = I->upload(s.iqatom);
+      D.mon_at = I->upload(at); D.mon_fi = I->upload(fi); D.mon_fj = I->upload(fj);
+      D.mon_ni = I->upload(ni); D.mon_nj = I->upload(nj); D.mon_lo = I->upload(lo);
+      D.mon_lj = I->upload(lj);
+      D.mon_out = I->alloc<double>((size_t)std::max(1, D.mon_n) * (kMon + 2 * std::max(1, s.nstates)));
+      max_mon_ = std::max(max_mon_, D.mon_n);
       D.slot_sum = I->alloc<double>((size_t)std::max<size_t>(1, sa.size()) * 3);
     }
     max_terms_ = std::max(max_terms_, D.nbnd + D.nang + D.ntor + D.nimp + D.nqq + D.nxt);
@@ -1200,6 +1220,23 @@ void Batch::launch_integrate(long s) {
   if (record_possible(s)) { pb(10); hipLaunchKernelGGL(k_energy_reduce, dim3(1, ni), dim3(256), 0, stream_, d_sys_, 1); pe(10); }
   { pb(11); hipLaunchKernelGGL(k_integrate, dim3(blocks, ni), dim3(kIntegrateBlock), 0, stream_, d_sys_); pe(11); }
   { pb(12); hipLaunchKernelGGL(k_temperature, dim3(1, ni), dim3(256), 0, stream_, d_sys_, blocks); pe(12); }
+}
+
+// Energies of the monitored group pairs at the current coordinates
+// (nonbond_monitor, called by md_run at every output step after pot_energy,
+// src/md.f90:6562-6564): per pair Vwsum, Vwel, Vwlj, then Vel and Vlj of
+// every state.  Returns the number of values (written only if cap allows).
+int Batch::monitor(int k, double* out, int cap) {
+  const DevSys& D = h_sys_[k];
+  const int stride = kMon + 2 * std::max(1, D.nstates);
+  const int need = D.mon_n * stride;
+  if (need == 0 || need > cap) return need;
+  HIPCHECK(hipStreamSynchronize(stream_));
+  hipLaunchKernelGGL(k_monitor, dim3(max_mon_, (int)h_sys_.size()), dim3(256), 0, stream_, d_sys_);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpyAsync(out, D.mon_out, need * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  HIPCHECK(hipStreamSynchronize(stream_));
+  return need;
 }
 
 void Batch::make_lists() { launch_lists(); }
@@ -1735,6 +1772,13 @@ int qgpu_set_launch_shape(qgpu_handle* h, int kernel, int map, int block, int pr
   return guarded([&] {
     if (!h) throw std::runtime_error("invalid handle");
     h->b->set_shape(kernel, map, block, prefetch, occupancy);
+  });
+}
+
+int qgpu_monitor(qgpu_handle* h, int k, double* out, int cap, int* n) {
+  return guarded([&] {
+    CHECK_K(h, k);
+    *n = h->b->monitor(k, out, cap);
   });
 }
 

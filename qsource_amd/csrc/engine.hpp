@@ -69,6 +69,7 @@ constexpr int kXtPars = 8;
 constexpr int kMorseRec = 7;   // Morse factor and its gradient on the bond's two atoms
 constexpr int kQwRec = 8;      // Q-water record per Q atom (qw_rec_d / qw_rec_f)
 constexpr int kEWater = 5;     // energy partials per water
+constexpr int kMon = 3;        // Vwsum, Vwel, Vwlj before the per-state values
 constexpr int kIntegrateBlock = 64;  // threads per k_integrate workgroup
 // lanes per water in k_water_lists: FP64 build, FP32 (mixed) build
 constexpr int kWLLanes64 = 8, kWLLanes32 = 16;
@@ -85,6 +86,15 @@ struct DevSys {
   int precision;                 // 0 = FP64, 1 = mixed FP32 pair / FP64 accumulation
   double *x, *v, *xx, *d, *xtop;
   float* xf;                     // FP32 copy of x for the mixed-precision pair kernels
+  // monitored group pairs (nonbond_monitor): atoms of group i and j of pair
+  // p at mon_at[mon_fi[p] ..) and [mon_fj[p] ..), counts mon_ni/mon_nj, LJ
+  // codes [(s*ni + i)*nj + j] from mon_lj[mon_lo[p]]; results kMon + 2 nstates
+  // per pair into mon_out
+  const int* iqatom_d;           // N: Q index + 1 of each atom (0 = not a Q atom)
+  int mon_n;
+  const int *mon_at, *mon_fi, *mon_fj, *mon_ni, *mon_nj, *mon_lo;
+  const signed char* mon_lj;
+  double* mon_out;
   const int* slot_of_atom;       // N: the atom's index in slot_atom, or -1
   double* slot_sum;              // nslot_atoms x 3: each slot atom's summed slots (k_term_sum)
   double* xq;                    // coordinates of the terms after fix_shell: x, or with
@@ -361,6 +371,10 @@ class Batch {
   void upload(int k);
   void launch_lists();
   void launch_forces(bool step, long s);
+ public:
+  int monitor(int k, double* out, int cap);   // monitored pair energies of instance k
+  int max_mon_ = 0;
+ private:
   bool record_possible(long s) const;
   void launch_tuned(int kernel, const LaunchShape& sh);
   const std::pair<int2*, int>& work_map(int kernel, int map, int block);

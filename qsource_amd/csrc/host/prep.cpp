@@ -247,6 +247,51 @@ static void restraints_setup(Setup& s) {
     if (!ok(r.i) || !ok(r.j)) throw std::runtime_error("invalid atom in [wall_restraints]");
 }
 
+// LJ codes of the monitored atom pairs per state (nbmonitorlist,
+// src/md.f90:11151-11226): the topology code; code 1 instead of 2 between a
+// Q and a non-Q atom with Q-atom types; code 3 for 1-4 pairs of two non-Q
+// atoms, looked up as the reference does (list14 row of the larger number
+// for a near pair, the long 1-4 list otherwise); for a pair with a Q atom the
+// code of the first matching pair of the state's Q-Q list.
+static void monitor_setup(Setup& s) {
+  const auto& t = s.top;
+  const int ns = std::max(1, s.nstates);
+  auto is14 = [&](int i, int j) {   // 0-based atoms, i != j
+    const int d = i > j ? i - j : j - i;
+    if (d <= kMaxNbrRange) {
+      const int row = i > j ? i : j;   // the reference's list14(|i-j|, max(i, j))
+      if (row >= (int)t.list14.size()) return false;
+      return ((t.list14[row] >> (d - 1)) & 1u) != 0;
+    }
+    for (auto& p : t.list14long)
+      if ((p.first == i && p.second == j) || (p.first == j && p.second == i)) return true;
+    return false;
+  };
+  for (auto& m : s.mon) {
+    const int ni = (int)m.ai.size(), nj = (int)m.aj.size();
+    m.ljcod.assign((size_t)ns * ni * nj, 1);
+    for (int i = 0; i < ni; ++i)
+      for (int j = 0; j < nj; ++j) {
+        const int a = m.ai[i], b = m.aj[j];
+        if (a == b) throw std::runtime_error("two paired monitor atom groups contain the same atom");
+        for (int st = 0; st < ns; ++st) {
+          int lj = s.ljcod[s.iac[a] * s.natyps + s.iac[b]];
+          const bool qa = s.iqatom[a] != 0, qb = s.iqatom[b] != 0;
+          if (s.qvdw_flag && lj == 2 && qa != qb) lj = 1;
+          if (!qa && !qb) {
+            if (is14(a, b)) lj = 3;
+          } else if (st < (int)s.nbqq.size()) {
+            for (auto& p : s.nbqq[st]) {
+              const int qi = s.iqseq[p.iq];
+              if ((qi == a && p.j == b) || (p.j == a && qi == b)) { lj = p.ljcod; break; }
+            }
+          }
+          m.ljcod[((size_t)st * ni + i) * nj + j] = lj;
+        }
+      }
+  }
+}
+
 Setup prepare(const std::string& input_path) {
   Setup s;
   const std::string dir = dir_of(input_path);
@@ -337,6 +382,12 @@ Setup prepare(const std::string& input_path) {
   if (!in.fep_file.empty()) {
     FepData f = read_fep(join(dir, in.fep_file), s.natom);
     s.nstates = f.nstates;
+    for (auto& pr : f.mon_pairs) {
+      Setup::MonPair m;
+      m.ai = f.mon_groups[pr.first];
+      m.aj = f.mon_groups[pr.second];
+      s.mon.push_back(m);
+    }
     if (!in.lambdas.empty()) {
       if ((int)in.lambdas.size() != s.nstates)
         throw std::runtime_error("number of lambdas differs from FEP states");
@@ -736,6 +787,7 @@ Setup prepare(const std::string& input_path) {
     for (int i = 0; i < s.natom; ++i)
       for (int j = 0; j < 3; ++j) s.v[3 * i + j] = s.v[3 * i + j] - vcm[j];
   }
+  monitor_setup(s);
   return s;
 }
 

@@ -170,6 +170,10 @@ struct FepData {
   // [shake_constraints]: atom pair (0-based) and distance per state
   struct QShake { int i, j; std::vector<double> dist; };
   std::vector<QShake> qshake;
+  // [monitor_groups]: atom groups (0-based atoms); [monitor_group_pairs]:
+  // pairs of 0-based group numbers whose nonbonded energies are monitored
+  std::vector<std::vector<int>> mon_groups;
+  std::vector<std::pair<int, int>> mon_pairs;
 };
 
 struct ShakeBond { int i, j; double dist2; };
@@ -222,6 +226,11 @@ struct Setup {
   std::vector<double> lambda;
   std::vector<int> qconn;             // [s][iq][atom] connectivity (1..9)
   std::vector<std::vector<NbqPair>> nbqq;  // per state
+  // monitored group pairs: the atoms of both groups and the LJ code of each
+  // (state, i, j) atom pair, [(s*ni + i)*nj + j] (nbmonitorlist,
+  // src/md.f90:11151-11226)
+  struct MonPair { std::vector<int> ai, aj; std::vector<int> ljcod; };
+  std::vector<MonPair> mon;
   // Q-atom types (qvdw_flag): qiac [iq*nstates+s] (0-based), qavdw/qbvdw
   // [type*3 + LJ code-1]; soft-core table sc_lookup(iq, column, s) with
   // columns 0..natyps-1 topology types and natyps+jq Q atoms, stored

@@ -673,10 +673,31 @@ FepData read_fep(const std::string& path, int natom) {
   int nq = (int)f.iqseq.size();
   f.qcrg.assign(nq, std::vector<double>(f.nstates, 0.0));
   if (S.get("FEP", "qq_use_library_charges", v)) parse_onoff(v, f.qq_lib_charges);
-  for (const char* unsup : {"MONITOR_GROUPS", "MONITOR_GROUP_PAIRS"}) {
-    if (S.has(unsup) && !S.sec.at(unsup).empty())
-      throw std::runtime_error(std::string("FEP section [") + unsup + "] is not supported");
-  }
+  // monitored atom groups and group pairs (qatom_load_fep, src/qatom.f90:1987-2020)
+  if (S.has("MONITOR_GROUPS"))
+    for (auto& l : S.sec.at("MONITOR_GROUPS")) {
+      std::vector<std::string> tk;
+      Records::tokens(l, tk);
+      if (tk.empty()) continue;
+      std::vector<int> g;
+      for (auto& t : tk) {
+        const int a = (int)to_i(t);
+        if (a < 1 || a > natom) throw std::runtime_error("bad atom in [monitor_groups]");
+        g.push_back(a - 1);
+      }
+      f.mon_groups.push_back(g);
+    }
+  if (S.has("MONITOR_GROUP_PAIRS"))
+    for (auto& l : S.sec.at("MONITOR_GROUP_PAIRS")) {
+      std::vector<std::string> tk;
+      Records::tokens(l, tk);
+      if (tk.empty()) continue;
+      if (tk.size() < 2) throw std::runtime_error("bad [monitor_group_pairs] line");
+      const int gi = (int)to_i(tk[0]), gj = (int)to_i(tk[1]);
+      if (gi < 1 || gj < 1 || gi > (int)f.mon_groups.size() || gj > (int)f.mon_groups.size())
+        throw std::runtime_error("[monitor_group_pairs] names an undefined group");
+      f.mon_pairs.push_back({gi - 1, gj - 1});
+    }
   if (S.has("CHANGE_CHARGES")) {
     f.have_charges = true;
     for (auto& l : S.sec.at("CHANGE_CHARGES")) {

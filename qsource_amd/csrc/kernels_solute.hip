@@ -634,6 +634,84 @@ __global__ __launch_bounds__(1024) void k_q_lists(const DevSys* __restrict__ sys
   if (threadIdx.x == 0) { S.q_counts[0] = nqp; S.q_counts[1] = nqw; }
 }
 
+// nonbond_monitor (src/md.f90:11228-11350): Coulomb and LJ energies of every
+// atom pair of a monitored group pair, per state (Q charges and, with Q-atom
+// types, Q LJ parameters and soft-core of the state), FP64; then the
+// lambda-weighted sums.  One workgroup per (pair, instance); excluded atoms
+// frozen by excluded_freeze are at their topology positions, as after
+// fix_shell.
+__global__ __launch_bounds__(256) void k_monitor(const DevSys* __restrict__ sys) {
+  const DevSys& S = sys[blockIdx.y];
+  const int par = blockIdx.x;
+  if (par >= S.mon_n) return;
+  const int ns = S.nstates > 0 ? S.nstates : 1;
+  const int ni = S.mon_ni[par], nj = S.mon_nj[par];
+  const int* ai = S.mon_at + S.mon_fi[par];
+  const int* aj = S.mon_at + S.mon_fj[par];
+  const signed char* ljc = S.mon_lj + S.mon_lo[par];
+  auto pos = [&](int a, int c) {
+    return (S.freeze && (S.flags[a] & F_EXCL)) ? S.xtop[3 * a + c] : S.x[3 * a + c];
+  };
+  double vel[kMaxQStates], vlj[kMaxQStates];
+  for (int st = 0; st < kMaxQStates; ++st) { vel[st] = 0.0; vlj[st] = 0.0; }
+  for (int p = threadIdx.x; p < ni * nj; p += blockDim.x) {
+    const int i = p / nj, j = p % nj;
+    const int a = ai[i], b = aj[j];
+    const int qa = S.iqatom_d[a], qb = S.iqatom_d[b];
+    const double dx1 = pos(a, 0) - pos(b, 0), dx2 = pos(a, 1) - pos(b, 1), dx3 = pos(a, 2) - pos(b, 2);
+    const double r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
+    const double r = sqrt(1.0 / r2);
+    const double r6_hc = r2 * r2 * r2;
+    for (int st = 0; st < ns; ++st) {
+      const int lj = ljc[((size_t)st * ni + i) * nj + j];
+      const int ta = S.iac[a], tb = S.iac[b];
+      double aLJi = S.avdw[ta * 3 + lj - 1], bLJi = S.bvdw[ta * 3 + lj - 1];
+      double aLJj = S.avdw[tb * 3 + lj - 1], bLJj = S.bvdw[tb * 3 + lj - 1];
+      const double qi = qa ? S.qcrg[(qa - 1) * S.nstates + st] : S.crg[a];
+      const double qj = qb ? S.qcrg[(qb - 1) * S.nstates + st] : S.crg[b];
+      double r6 = 1.0 / r6_hc;
+      if (S.qvdw && (qa || qb)) {
+        int sc1, sc2;
+        if (qa) {
+          const int t = S.qiac[(qa - 1) * S.nstates + st];
+          aLJi = S.qavdw[3 * t + lj - 1]; bLJi = S.qbvdw[3 * t + lj - 1];
+          sc1 = qa - 1;
+          if (qb) {
+            const int u = S.qiac[(qb - 1) * S.nstates + st];
+            aLJj = S.qavdw[3 * u + lj - 1]; bLJj = S.qbvdw[3 * u + lj - 1];
+            sc2 = S.natyps + qb - 1;
+          } else {
+            sc2 = tb;
+          }
+        } else {
+          const int u = S.qiac[(qb - 1) * S.nstates + st];
+          aLJj = S.qavdw[3 * u + lj - 1]; bLJj = S.qbvdw[3 * u + lj - 1];
+          sc1 = qb - 1;
+          sc2 = ta;
+        }
+        r6 = 1.0 / (r6_hc + sc_alpha(S, sc1, sc2, st));
+      }
+      vel[st] = vel[st] + qi * qj * r;
+      vlj[st] = vlj[st] + (aLJi * aLJj * r6 * r6 - bLJi * bLJj * r6);
+    }
+  }
+  __shared__ double red[2 * kMaxQStates][4];
+  double* o = S.mon_out + (size_t)par * (kMon + 2 * ns);
+  double wel = 0.0, wlj = 0.0;
+  for (int st = 0; st < ns; ++st) {
+    const double e1 = block_sum_256(vel[st], red[2 * st]);
+    const double e2 = block_sum_256(vlj[st], red[2 * st + 1]);
+    if (threadIdx.x == 0) {
+      const double lam = S.nstates > 0 ? S.lambda[st] : 1.0;
+      o[kMon + 2 * st] = e1;
+      o[kMon + 2 * st + 1] = e2;
+      wel = wel + e1 * lam;
+      wlj = wlj + e2 * lam;
+    }
+  }
+  if (threadIdx.x == 0) { o[0] = wlj + wel; o[1] = wel; o[2] = wlj; }
+}
+
 // Sum of the Q-atom chunks: gradient added to d, per-state energies stored.
 // With shadow != 0 (FP64 re-evaluation of a mixed-precision batch on an
 // energy-file step) only the energies are stored, into e_q64.

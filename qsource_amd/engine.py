@@ -36,7 +36,7 @@ SYMBOLS = [
     "qgpu_mlp_create", "qgpu_mlp_destroy", "qgpu_mlp_nparams", "qgpu_mlp_get_params", "qgpu_mlp_set_params",
     "qgpu_mlp_set_data", "qgpu_mlp_train", "qgpu_mlp_gradient", "qgpu_mlp_predict", "qgpu_mlp_last_error",
     "qgpu_far_samples", "qgpu_far_lrf_estimate", "qgpu_set_far_model",
-    "qgpu_profile_kernels", "qgpu_list_counts", "qgpu_offdiag_states",
+    "qgpu_profile_kernels", "qgpu_list_counts", "qgpu_offdiag_states", "qgpu_monitor",
     "qgpu_trj_create", "qgpu_trj_natoms", "qgpu_trj_write", "qgpu_trj_close", "qgpu_write_restart",
     "qgpu_autotune", "qgpu_set_launch_shape",
 ]
@@ -100,6 +100,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.qgpu_far_lrf_estimate.argtypes = [P, ctypes.c_int, fp, fp, ctypes.c_int, ip]
     lib.qgpu_set_far_model.argtypes = [P, P]
     lib.qgpu_offdiag_states.argtypes = [P, ctypes.c_int, ip, ctypes.c_int, ip]
+    lib.qgpu_monitor.argtypes = [P, ctypes.c_int, dp, ctypes.c_int, ip]
     lib.qgpu_autotune.argtypes = [P, ctypes.c_int, dp, ip]
     lib.qgpu_set_launch_shape.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     lib.qgpu_trj_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(P)]
@@ -253,6 +254,16 @@ class Engine:
     def set_launch_shape(self, kernel: str, mapping: int, block: int, prefetch: int = 0, occupancy: int = 0):
         self._check(self.lib.qgpu_set_launch_shape(self.h, TUNED_KERNELS.index(kernel), mapping, block, prefetch,
                                                    occupancy))
+
+    def monitor(self, k: int = 0) -> np.ndarray:
+        """Monitored group pairs at the current coordinates: per pair Vwsum,
+        Vwel, Vwvdw, then Vel and Vvdw of every state (nonbond_monitor)."""
+        n = ctypes.c_int()
+        self._check(self.lib.qgpu_monitor(self.h, k, None, 0, ctypes.byref(n)))
+        out = np.zeros(max(n.value, 1))
+        self._check(self.lib.qgpu_monitor(self.h, k, _dp(out), len(out), ctypes.byref(n)))
+        ns = max(self.nstates[k], 1)
+        return out[:n.value].reshape(-1, 3 + 2 * ns) if n.value else np.zeros((0, 3 + 2 * ns))
 
     def offdiag_states(self, k: int = 0) -> np.ndarray:
         """(noffd, 2) 1-based state pairs (i, j) of the off-diagonal functions."""

@@ -111,6 +111,15 @@ module qgpu
       integer(c_int), intent(out) :: nframes
     end function
 
+    ! monitored group pairs (nonbond_monitor): 3 + 2 nstates values per pair
+    integer(c_int) function qgpu_monitor(h, k, out, cap, n) bind(C, name='qgpu_monitor')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: k, cap
+      real(c_double), intent(out) :: out(*)
+      integer(c_int), intent(out) :: n
+    end function
+
     ! restart (write_xfin) and DCD trajectory (trj_create/trj_write) writers
     integer(c_int) function qgpu_write_restart(path, natom, x, v, nshell, theta_corr) &
         bind(C, name='qgpu_write_restart')

@@ -121,6 +121,10 @@ class System:
     qq_lib_charges: bool = False
     # [shake_constraints] rows (i, j, distance per state), 1-based
     shake_constraints: Optional[list] = None
+    # [monitor_groups] (lists of atoms) and [monitor_group_pairs] (group
+    # numbers), 1-based
+    monitor_groups: Optional[list] = None
+    monitor_pairs: Optional[list] = None
     # topology solvent type: 0 SPC-like (LJ on the water O only), 1 all-atom
     # LJ on every solvent atom (TIP3P-like H types)
     solvent_type: int = 0
@@ -366,7 +370,8 @@ def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
                     fep_pairs: bool = False, qq_lib_charges: bool = False,
                     q_shake: bool = False, solvent_allatom: bool = False,
                     solute_rmax: Optional[float] = None, exclude_outside: bool = False,
-                    exclude_water_radius: Optional[float] = None, hydrogens: bool = False) -> System:
+                    exclude_water_radius: Optional[float] = None, hydrogens: bool = False,
+                    monitor: bool = False) -> System:
     """A TIP3P sphere of the given radius around an optional branched solute
     (charges N(0,q_sd) neutralised per 4-atom charge group, sigma~U[2.5,3.8],
     eps~U[0.05,0.2] quantised into at most n_types atom types), with an
@@ -387,7 +392,10 @@ def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
     whose switch atom lies beyond the exclusion radius is excluded;
     exclude_water_radius: waters whose O lies beyond it are excluded too;
     hydrogens: about half of the solute's terminal atoms become hydrogens
-    (mass 1.008, sigma 2.4 A, eps 0.022, own atom type and bond type)."""
+    (mass 1.008, sigma 2.4 A, eps 0.022, own atom type and bond type);
+    monitor: three monitored atom groups (the Q atoms, the solute atoms
+    bonded to them, and the first eight solute atoms) and their three
+    pairs."""
     rng = np.random.default_rng(seed)
     # --- solute
     if n_solute > 0:
@@ -618,6 +626,13 @@ def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
                 break
         sysm.shake_constraints = rows
     sysm.solvent_type = 1 if solvent_allatom else 0
+    if monitor and sysm.qatoms is not None and ns >= 16:
+        qs = set(int(a) for a in sysm.qatoms)
+        nbr = sorted(set(int(b) for i, j, _ in sysm.bonds[:sysm.nbonds_solute] for a, b in ((i, j), (j, i))
+                         if int(a) in qs and int(b) not in qs))
+        first = [a for a in range(1, ns + 1) if a not in qs and a not in nbr][:8]
+        sysm.monitor_groups = [sorted(qs), nbr[:8], first]
+        sysm.monitor_pairs = [(1, 2), (1, 3), (2, 3)]
     if (exclude_outside and ns) or exclude_water_radius is not None:
         excl = np.zeros(N, dtype=bool)
         if exclude_outside:
@@ -996,6 +1011,9 @@ def write_fep(s: System, path: str) -> None:
     if s.excluded_pairs:
         L += ["", "[excluded_pairs]"] + ["%d %d " % (i, j) + " ".join("%d" % f for f in fl)
                                          for i, j, fl in s.excluded_pairs]
+    if s.monitor_groups:
+        L += ["", "[monitor_groups]"] + [" ".join("%d" % a for a in g) for g in s.monitor_groups]
+        L += ["", "[monitor_group_pairs]"] + ["%d %d" % p for p in s.monitor_pairs]
     if s.shake_constraints:
         L += ["", "[shake_constraints]"] + ["%d %d " % (i, j) + " ".join("%.3f" % v for v in d)
                                             for i, j, d in s.shake_constraints]

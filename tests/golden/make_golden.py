@@ -138,6 +138,17 @@ CASES = {
                                   dict(steps=30, lambdas=[0.6, 0.4], stepsize=0.5, temperature=50.0,
                                        bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
                                        energy_interval=1, exclude_bonded=True)),
+    # monitored group pairs (FEP [monitor_groups] / [monitor_group_pairs]),
+    # with fixed Q LJ parameters and with Q-atom types and soft-core
+    "c12_r12_q8s2_monitor_30": (dict(radius=12.0, n_solute=40, seed=2901, n_q=8, n_states=2, monitor=True),
+                                dict(steps=30, lambdas=[0.6, 0.4], stepsize=0.5, temperature=50.0,
+                                     bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
+                                     energy_interval=1)),
+    "c12_r12_q8s2_monitor_qvdw_30": (dict(radius=12.0, n_solute=40, seed=2902, n_q=8, n_states=2, monitor=True,
+                                          q_types=True, softcore=30.0),
+                                     dict(steps=30, lambdas=[0.6, 0.4], stepsize=0.5, temperature=50.0,
+                                          bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
+                                          energy_interval=1)),
     # a breaking bond with non-Q neighbours: topology torsions through it
     # become Q terms (qatom_load_fep's completion); periodic impropers
     "c6_r12_q6s2_qbreak_30": (dict(radius=12.0, n_solute=40, seed=2202, n_q=6, n_states=2, q_types=True,
@@ -146,6 +157,24 @@ CASES = {
                                    bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
                                    energy_interval=1)),
 }
+
+def parse_monitor(log: str):
+    """The monitored-pair table of every energy summary (write_out's formats
+    i2, f10.2, f8.2, f9.2, then 2f8.2 per state; overflowed fields are NaN)."""
+    lines = log.splitlines()
+    out = []
+    for k, l in enumerate(lines):
+        if "Monitoring selected groups" not in l:
+            continue
+        rows, j = [], k + 2
+        while not lines[j].startswith("====="):
+            t = lines[j]
+            f = [t[2:12], t[12:20], t[20:29]] + [t[29 + 8 * q:37 + 8 * q] for q in range((len(t) - 29) // 8)]
+            rows.append([float(x) if "*" not in x else float("nan") for x in f])
+            j += 1
+        out.append(rows)
+    return np.array(out)
+
 
 ROW = re.compile(r"^(solute|solvent|solute-solvent|LRF|Q-atom|restraints|SUM)\s+(.*)$")
 
@@ -214,6 +243,8 @@ def main():
                 _, rows, offd = fep.read_energy_frames(os.path.join(d, inpargs["energy"]),
                                                        len(inpargs["lambdas"]))
                 extra = dict(eq_frames=rows, offd_frames=offd)
+            if sysargs.get("monitor"):
+                extra["monitor"] = parse_monitor(r.stdout)
             if inpargs.get("trajectory"):
                 raw = lambda f: np.frombuffer(open(os.path.join(d, f), "rb").read(), dtype=np.uint8)
                 extra.update(re_bytes=raw(name + ".re"), en_bytes=raw(inpargs["energy"]),

@@ -27,7 +27,10 @@ bitwise to reference qdyn on the same systems by test_oracle_reference.py):
   within 1e-9, mixed precision gradient within 1e-5 (RMS);
 * solute hydrogens: SHAKE of bonds to hydrogen with one Berendsen scale,
   sequence restraints with hydrogens and solute SHAKE, exclude_bonded:
-  30 FP64 steps within 1e-9.
+  30 FP64 steps within 1e-9;
+* monitored group pairs (nonbond_monitor, src/md.f90:11228-11350) with
+  fixed Q LJ parameters and with Q-atom types: the engine's values equal the
+  oracle's within 1e-10 relative after 0 and 15 steps.
 """
 import numpy as np
 import pytest
@@ -279,4 +282,22 @@ def test_hydrogen_variants(tmp_path, name):
     xg, vg = e.state(0)
     assert np.abs(xg - xo).max() <= 1e-9 * np.abs(xo).max()
     assert np.abs(vg - vo).max() <= 1e-9 * np.abs(vo).max()
+    e.close()
+
+
+@pytest.mark.parametrize("qvdw", [False, True])
+def test_monitor_groups(tmp_path, qvdw):
+    _say("case: monitored group pairs, R=12 + 40-atom solute + 8 Q, qvdw=%s, FP64" % qvdw)
+    kw = dict(q_types=True, softcore=30.0) if qvdw else {}
+    s = gensys.solvated_system(radius=12.0, n_solute=40, seed=2902 if qvdw else 2901, n_q=8, n_states=2,
+                               monitor=True, **kw)
+    paths = gensys.write_system(s, str(tmp_path), "mon", steps=15, lambdas=[0.6, 0.4], **COOL)
+    o = Oracle(paths["inp"])
+    e = qe.Engine([paths["inp"]], qe.QGPU_FP64)
+    for k in range(2):
+        mo, mg = o.monitor(), e.monitor(0)
+        assert mg.shape == mo.shape == (3, 7)
+        assert np.allclose(mg, mo, rtol=1e-10, atol=1e-9), (k, mg, mo)
+        o.run(15)
+        e.step(15)
     e.close()

@@ -115,6 +115,17 @@ CASES = {
                                   dict(steps=30, lambdas=[0.6, 0.4], stepsize=0.5, temperature=50.0,
                                        bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
                                        energy_interval=1, exclude_bonded=True)),
+    # monitored group pairs (FEP [monitor_groups] / [monitor_group_pairs]),
+    # with fixed Q LJ parameters and with Q-atom types and soft-core
+    "c12_r12_q8s2_monitor_30": (dict(radius=12.0, n_solute=40, seed=2901, n_q=8, n_states=2, monitor=True),
+                                dict(steps=30, lambdas=[0.6, 0.4], stepsize=0.5, temperature=50.0,
+                                     bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
+                                     energy_interval=1)),
+    "c12_r12_q8s2_monitor_qvdw_30": (dict(radius=12.0, n_solute=40, seed=2902, n_q=8, n_states=2, monitor=True,
+                                          q_types=True, softcore=30.0),
+                                     dict(steps=30, lambdas=[0.6, 0.4], stepsize=0.5, temperature=50.0,
+                                          bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
+                                          energy_interval=1)),
     # a breaking bond with non-Q neighbours: topology torsions through it
     # become Q terms (qatom_load_fep's completion); periodic impropers
     "c6_r12_q6s2_qbreak_30": (dict(radius=12.0, n_solute=40, seed=2202, n_q=6, n_states=2, q_types=True,
@@ -192,3 +203,26 @@ def test_oracle_energy_file_frames(name, tmp_path):
         assert np.allclose(q[:, 7:11], ref[:, 1:5], rtol=1e-12, atol=1e-10), (k, q[:, 7:11], ref[:, 1:5])
         assert np.allclose(q[:, 11], ref[:, 13], rtol=1e-12, atol=1e-10), (k, q[:, 11], ref[:, 13])
     assert np.array_equal(o.theta_corr(), g["theta_corr"])
+
+
+MONITOR_CASES = sorted(k for k, (sa, _) in CASES.items() if sa.get("monitor"))
+
+
+@pytest.mark.parametrize("name", MONITOR_CASES)
+def test_oracle_monitor_matches_reference(name, tmp_path):
+    """nonbond_monitor: the oracle's monitored-pair energies at every output
+    step equal the reference's printed table (2 decimals; overflowed fields
+    skipped)."""
+    g = np.load(os.path.join(GOLD, name + ".npz"), allow_pickle=False)
+    ref = g["monitor"]
+    sysargs, inpargs = CASES[name]
+    s = gensys.solvated_system(**sysargs)
+    paths = gensys.write_system(s, str(tmp_path), name, output=1, temp_interval=1, **inpargs)
+    o = Oracle(paths["inp"])
+    steps = inpargs["steps"]
+    for k in range(steps):
+        m = np.round(o.monitor(), 2)
+        ok = np.isnan(ref[k]) | (np.abs(m - ref[k]) <= 0.0100001)
+        assert ok.all(), (k, m, ref[k])
+        o.run(1)
+    o.close()
