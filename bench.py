@@ -211,7 +211,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--config", default="c3")
-    ap.add_argument("--cpu-steps", type=int, default=300)
+    ap.add_argument("--cpu-steps", type=int, default=1000)
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--skip-fp64", action="store_true")
     args = ap.parse_args()
