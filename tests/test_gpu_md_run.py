@@ -1,4 +1,4 @@
-"""The GPU run driver (qsource_amd.md.run) against the files reference qdyn
+"""The GPU MD run loop (qsource_amd.md.run) against the files reference qdyn
 wrote for the same run (tests/golden): the FP64 engine's final restart
 within 1e-9 of the reference's x and v, the same theta_corr record layout;
 the DCD header byte-identical and every frame within REAL(4) rounding of the
