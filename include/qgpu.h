@@ -72,12 +72,57 @@ int qgpu_make_pair_lists(qgpu_handle* h);
  * The gradient is fetched with qgpu_get_gradient. */
 int qgpu_pot_energy(qgpu_handle* h);
 
+/* Term groups of qgpu_pot_energy_terms. */
+enum qgpu_term_group {
+  QGPU_TERMS_WW = 1,     /* water-water (nonbond_ww_spc / nonbond_3atomsolvent) */
+  QGPU_TERMS_PW = 2,     /* solute-water (nonbond_pw) */
+  QGPU_TERMS_PP = 4,     /* solute-solute (nonbond_pp) */
+  QGPU_TERMS_QX = 8,     /* Q-solute and Q-water (nonbond_qp, nonbond_qw_*) */
+  QGPU_TERMS_LRF = 16,   /* LRF Taylor (lrf_taylor) */
+  QGPU_TERMS_OTHER = 32, /* bonded, Q-Q, Q-bonded, restraints, polarisation */
+  QGPU_TERMS_ALL = 63
+};
+
+/* As qgpu_pot_energy, with only the term groups in `mask` in the gradient
+ * (per-term error reports of the mixed-precision path against the plain
+ * one).  Later evaluations include every term again.  No counterpart in
+ * the reference. */
+int qgpu_pot_energy_terms(qgpu_handle* h, int mask);
+
 /* Replaces md_run's loop body (src/md.f90:6471-6612): nsteps steps of
  * list update (every NBcycle), forces, leapfrog/Berendsen, SHAKE and
- * temperature, with x, v and d resident on the device. */
+ * temperature, with x, v and d resident on the device.
+ * Failures stop the batch where die() stops qdyn (src/md.f90:554-613): a
+ * SHAKE failure (shake, :17645-17673), a NaN temperature (tested at every
+ * step; md_run tests it at printed steps, :6654-6656) or a full water
+ * shell leaves every instance at the failing step and returns nonzero,
+ * and so does every later qgpu_step; qgpu_get_state still returns that
+ * state (for the restart die writes).  A pair-list row that outgrows its
+ * capacity is not a failure: the rows are regrown and the list update is
+ * redone, as the reference reallocates its lists (:900-1128). */
 int qgpu_step(qgpu_handle* h, int nsteps);
 
-/* State transfer of instance k (x, v: 3*natom FP64 each). */
+/* Failure record of instance k: *bits = 1 pair-list row over capacity,
+ * 2 water-shell capacity, 4 SHAKE failure, 8 NaN temperature (0 = none);
+ * *at_step = the step it was found at (-1 when none).  No counterpart in the
+ * reference, whose die() ends the program. */
+int qgpu_instance_status(qgpu_handle* h, int k, int* bits, int* at_step);
+
+/* Row capacities of instance k's water-water, water-solute and solute-water
+ * lists (caps[3]) and the number of times this batch regrew list rows.
+ * The library sizes the rows from the starting coordinates; the reference
+ * sizes its lists from counts at distribute_nonbonds (src/md.f90:2696-2714). */
+int qgpu_list_capacity(qgpu_handle* h, int k, int* caps, int* regrowths);
+
+/* Replaces the row capacities of instance k (0 keeps one); the next list
+ * update fills the new rows and regrows any that are too short. */
+int qgpu_set_list_capacity(qgpu_handle* h, int k, int ww, int wp, int pw);
+
+/* State transfer of instance k (x, v: 3*natom FP64 each; either may be
+ * NULL).  Setting v also sets the temperature, Tfree and the Berendsen
+ * scales of the next step from it, as md_run's first temperature() call
+ * does for the starting velocities (src/md.f90:6400-6404); Tfree enters the
+ * radial water restraint (src/md.f90:18055-18102). */
 int qgpu_get_state(qgpu_handle* h, int k, double* x, double* v);
 int qgpu_set_state(qgpu_handle* h, int k, const double* x, const double* v);
 
@@ -137,6 +182,14 @@ int qgpu_get_pairlist(qgpu_handle* h, int k, int which, int* out, long* n);
  * counts md_run prints after make_pair_lists, src/md.f90:6508-6512):
  * counts[0..4] = nbpp, nbpw, nbww, nbqp, and Q-water atom pairs. */
 int qgpu_list_counts(qgpu_handle* h, int k, long* counts);
+
+/* Work of one list update of instance k at its current coordinates, for
+ * roofline accounting (no counterpart in the reference): counts[0..5] =
+ * atom-to-centre multipole evaluations (lrf_update, src/md.f90:5896-5981)
+ * of the water-side and of the solute-side list kernel, water-water list
+ * entries (both directions), water-solute group entries, solute group pairs
+ * within the cut-off (both directions), solute-water entries. */
+int qgpu_work_counts(qgpu_handle* h, int k, double* counts, int cap);
 
 /* Current temperature (Temp, src/md.f90:6159-6163) of instance k. */
 int qgpu_temperature(qgpu_handle* h, int k, double* temp);

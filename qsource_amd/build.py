@@ -38,19 +38,22 @@ def build_engine(verbose: bool = False) -> str:
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     objdir = os.path.join(PKG, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    objs, cmds = [], []
     for src in HIP_SOURCES + HOST_SOURCES:
         obj = os.path.join(objdir, os.path.basename(src).replace(".", "_") + ".o")
         flags = list(HIP_FLAGS)
         if src.endswith(".cpp"):
             flags = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off"]
-            cmd = ["g++"] + flags + ["-c", os.path.join(CSRC, src), "-o", obj]
+            cmds.append(["g++"] + flags + ["-c", os.path.join(CSRC, src), "-o", obj])
         else:
-            cmd = [hipcc] + flags + ["-c", os.path.join(CSRC, src), "-o", obj]
-        out = _run(cmd)
-        if verbose and out.strip():
-            print(out)
+            cmds.append([hipcc] + flags + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    # the translation units compile independently: a few at a time
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=min(len(cmds), max(1, (os.cpu_count() or 2) // 2))) as ex:
+        for out in ex.map(_run, cmds):
+            if verbose and out.strip():
+                print(out)
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs)
     os.replace(tmp, LIB)

@@ -62,6 +62,16 @@ struct Batch::Inst {
     allocs.push_back(p);
     return static_cast<T*>(p);
   }
+  // frees one allocation made by alloc (buffers that are resized)
+  void release(void* p) {
+    if (!p) return;
+    for (size_t k = 0; k < allocs.size(); ++k)
+      if (allocs[k] == p) {
+        (void)hipFree(p);
+        allocs.erase(allocs.begin() + (long)k);
+        return;
+      }
+  }
   template <typename T>
   T* upload(const std::vector<T>& v) {
     T* p = alloc<T>(v.size());
@@ -73,14 +83,15 @@ struct Batch::Inst {
   }
   int ke_blocks = 0;
   int natom = 0, nwat = 0;
+  int eseries_alloc = -1;   // frames the energy-series buffers hold (-1: none yet)
 };
 
 // Initial temperature() of md_run (src/md.f90:6400-6404): Berendsen scale
 // factors for the first step from the starting velocities.
-static void initial_scales(const qsys::Setup& s, double* scal) {
+static void initial_scales(const qsys::Setup& s, const double* vel, double* scal) {
   double temp_solute = 0, tfree_solute = 0, temp_solvent = 0, tfree_solvent = 0;
   for (int i = 0; i < s.natom; ++i) {
-    const double* v = &s.v[3 * i];
+    const double* v = &vel[3 * i];
     double ek = 0.5 * s.top.types[s.iac[i]].mass * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
     if (i < s.nat_solute) { temp_solute += ek; if (!s.excl[i]) tfree_solute += ek; }
     else { temp_solvent += ek; if (!s.excl[i]) tfree_solvent += ek; }
@@ -107,6 +118,74 @@ static void initial_scales(const qsys::Setup& s, double* scal) {
     tsu = tsv;
   }
   scal[0] = tsu; scal[1] = tsv; scal[2] = tfree; scal[3] = temp;
+}
+
+// Longest rows of the water-water, water-solute and solute-water lists at
+// the starting coordinates (the switch-atom / water-O distance tests of the
+// list kernels), counted over a uniform grid of cut-off sized cells.
+struct ListRows { int ww = 0, wp = 0, pw = 0; };
+static ListRows longest_rows(const qsys::Setup& s) {
+  ListRows out;
+  const qsys::Topology& t = s.top;
+  if (s.nwat == 0) return out;
+  const double rww = s.inp.rcww, rpw = s.inp.rcpw;
+  const double cell = std::max(1.0, std::max(rww, rpw));
+  double lo[3] = {1e300, 1e300, 1e300};
+  std::vector<int> ow;   // water O atoms that take part (not excluded)
+  for (int w = 0; w < s.nwat; ++w) {
+    const int o = s.nat_solute + 3 * w;
+    if (s.excl[o]) continue;
+    ow.push_back(w);
+    for (int c = 0; c < 3; ++c) lo[c] = std::min(lo[c], s.x[3 * o + c]);
+  }
+  std::vector<int> sw;   // solute charge groups by switch atom
+  for (int g = 0; g < t.ncgp_solute; ++g) {
+    const int a = t.cgp[g].iswitch;
+    if (s.excl[a]) continue;
+    sw.push_back(g);
+    for (int c = 0; c < 3; ++c) lo[c] = std::min(lo[c], s.x[3 * a + c]);
+  }
+  auto cell_of = [&](const double* x, int* ci) {
+    for (int c = 0; c < 3; ++c) ci[c] = (int)std::floor((x[c] - lo[c]) / cell);
+  };
+  int dim[3] = {1, 1, 1};
+  auto grow_dim = [&](const double* x) {
+    int ci[3];
+    cell_of(x, ci);
+    for (int c = 0; c < 3; ++c) dim[c] = std::max(dim[c], ci[c] + 1);
+  };
+  for (int w : ow) grow_dim(&s.x[3 * (s.nat_solute + 3 * w)]);
+  for (int g : sw) grow_dim(&s.x[3 * t.cgp[g].iswitch]);
+  const size_t ncell = (size_t)dim[0] * dim[1] * dim[2];
+  std::vector<std::vector<int>> wcell(ncell), gcell(ncell);
+  auto index = [&](const int* ci) { return ((size_t)ci[0] * dim[1] + ci[1]) * dim[2] + ci[2]; };
+  for (int w : ow) { int ci[3]; cell_of(&s.x[3 * (s.nat_solute + 3 * w)], ci); wcell[index(ci)].push_back(w); }
+  for (int g : sw) { int ci[3]; cell_of(&s.x[3 * t.cgp[g].iswitch], ci); gcell[index(ci)].push_back(g); }
+  auto count = [&](const double* x, const std::vector<std::vector<int>>& cells, bool water, double rc, int self) {
+    int ci[3];
+    cell_of(x, ci);
+    int n = 0;
+    for (int a = ci[0] - 1; a <= ci[0] + 1; ++a)
+      for (int b = ci[1] - 1; b <= ci[1] + 1; ++b)
+        for (int c = ci[2] - 1; c <= ci[2] + 1; ++c) {
+          if (a < 0 || b < 0 || c < 0 || a >= dim[0] || b >= dim[1] || c >= dim[2]) continue;
+          const int cc[3] = {a, b, c};
+          for (int m : cells[index(cc)]) {
+            if (water && m == self) continue;
+            const int at = water ? s.nat_solute + 3 * m : t.cgp[m].iswitch;
+            const double d0 = s.x[3 * at] - x[0], d1 = s.x[3 * at + 1] - x[1], d2 = s.x[3 * at + 2] - x[2];
+            if (d0 * d0 + d1 * d1 + d2 * d2 <= rc * rc) ++n;
+          }
+        }
+    return n;
+  };
+  for (int w : ow) {
+    const double* xo = &s.x[3 * (s.nat_solute + 3 * w)];
+    out.ww = std::max(out.ww, count(xo, wcell, true, rww, w));
+    out.wp = std::max(out.wp, count(xo, gcell, false, rpw, -1));
+  }
+  for (int g : sw) out.pw = std::max(out.pw, count(&s.x[3 * t.cgp[g].iswitch], wcell, true, rpw, -1));
+  return out;
 }
 
 Batch::Batch(const std::vector<std::string>& inputs, int precision, int device) {
@@ -164,12 +243,19 @@ void Batch::init(const std::vector<std::string>& inputs, int device) {
   use_graph_ = !(ng && ng[0] == '1');
   const char* at = std::getenv("QGPU_AUTOTUNE");
   autotune_on_ = !(at && at[0] == '0');
-  HIPCHECK(hipMalloc(&d_step_, sizeof(int)));
-  HIPCHECK(hipMemset(d_step_, 0, sizeof(int)));
-  h_sys_.resize(setups_.size());
-  for (size_t k = 0; k < setups_.size(); ++k) {
+  const size_t ni = setups_.size();
+  HIPCHECK(hipMalloc(&d_step_, ni * sizeof(int)));
+  HIPCHECK(hipMemset(d_step_, 0, ni * sizeof(int)));
+  HIPCHECK(hipMalloc(&d_status_, ni * kStatusInts * sizeof(int)));
+  HIPCHECK(hipMemset(d_status_, 0, ni * kStatusInts * sizeof(int)));
+  HIPCHECK(hipMalloc(&d_fail_, sizeof(unsigned long long)));
+  HIPCHECK(hipMemcpy(d_fail_, &kNoFailure, sizeof(unsigned long long), hipMemcpyHostToDevice));
+  h_sys_.resize(ni);
+  for (size_t k = 0; k < ni; ++k) {
     upload((int)k);
-    h_sys_[k].step_ctr = d_step_;
+    h_sys_[k].step_ctr = d_step_ + k;
+    h_sys_[k].status = d_status_ + k * kStatusInts;
+    h_sys_[k].fail = d_fail_;
   }
   // one kinetic-energy partial per integrator workgroup of the widest instance
   const int kblocks = (std::max(max_units_, 1) + kIntegrateBlock - 1) / kIntegrateBlock;
@@ -198,15 +284,22 @@ void Batch::set_energy_interval(int iene, int max_frames) {
   HIPCHECK(hipStreamSynchronize(stream_));
   for (size_t k = 0; k < h_sys_.size(); ++k) {
     DevSys& D = h_sys_[k];
+    Inst& I = *inst_[k];
     D.iene = iene;
     D.eseries_base = istep_;
     D.eseries_cap = max_frames;
-    D.eseries = inst_[k]->alloc<double>((size_t)std::max(1, max_frames) * std::max(1, D.nstates) * kEneRecord);
-    D.eseries64 = precision_ == 1 && D.nqat > 0
-                      ? inst_[k]->alloc<double>((size_t)std::max(1, max_frames) * std::max(1, D.nstates) * kEneRecord)
-                      : nullptr;
-    D.eseries_offd = D.noffd > 0 ? inst_[k]->alloc<double>((size_t)std::max(1, max_frames) * D.noffd * 2) : nullptr;
-    D.eseries_n = inst_[k]->alloc<int>(1);
+    // the frame buffers are kept while they hold max_frames, else replaced
+    if (max_frames > I.eseries_alloc) {
+      I.release(D.eseries); I.release(D.eseries64); I.release(D.eseries_offd); I.release(D.eseries_n);
+      const size_t n = (size_t)std::max(1, max_frames) * std::max(1, D.nstates) * kEneRecord;
+      D.eseries = I.alloc<double>(n);
+      D.eseries64 = precision_ == 1 && D.nqat > 0 ? I.alloc<double>(n) : nullptr;
+      D.eseries_offd = D.noffd > 0 ? I.alloc<double>((size_t)std::max(1, max_frames) * D.noffd * 2) : nullptr;
+      D.eseries_n = I.alloc<int>(1);
+      I.eseries_alloc = max_frames;
+    } else {
+      HIPCHECK(hipMemset(D.eseries_n, 0, sizeof(int)));
+    }
   }
   iene_ = iene;
   ebase_ = istep_;
@@ -259,6 +352,9 @@ void Batch::release() {
   for (auto* s : setups_) delete s;
   if (d_sys_) (void)hipFree(d_sys_);
   if (d_step_) (void)hipFree(d_step_);
+  if (d_status_) (void)hipFree(d_status_);
+  if (d_fail_) (void)hipFree(d_fail_);
+  d_status_ = nullptr; d_fail_ = nullptr;
   if (stream2_) (void)hipStreamDestroy(stream2_);
   if (ev_fork_) (void)hipEventDestroy(ev_fork_);
   if (ev_join_) (void)hipEventDestroy(ev_join_);
@@ -288,6 +384,7 @@ void Batch::upload(int k) {
   D.ncgp = (int)t.cgp.size(); D.ncgp_solute = t.ncgp_solute;
   D.nqat = s.nqat; D.nstates = s.nstates;
   D.precision = precision_;
+  D.term_mask = TM_ALL;
   const int N = s.natom;
   D.x = I->upload(s.x);
   D.xf = I->alloc<float>(3 * N);
@@ -352,11 +449,20 @@ void Batch::upload(int k) {
       for (int a = t.cgp[g].first; a <= t.cgp[g].last; ++a) slot[t.cgpatom[a]] = a - t.cgp[g].first;
     D.cgp_slot = I->upload(slot);
   }
-  // lists
-  D.ww_cap = std::max(1, std::min(s.nwat, 448));
-  D.wp_cap = std::max(1, std::min(t.ncgp_solute, 1024));
+  // lists: row capacities from the longest rows of the starting
+  // coordinates (twice that, for density fluctuations); rows that still
+  // outgrow them are regrown at the failing list update (grow_lists)
+  {
+    const ListRows r = longest_rows(s);
+    auto cap = [](int longest, int most) {
+      const int c = (2 * longest + 32 + 15) / 16 * 16;
+      return std::max(1, std::min(most, c));
+    };
+    D.ww_cap = cap(r.ww, s.nwat);
+    D.wp_cap = cap(r.wp, t.ncgp_solute);
+    D.pw_cap = cap(r.pw, s.nwat);
+  }
   D.pp_cap = std::max(1, t.ncgp_solute);
-  D.pw_cap = std::max(1, std::min(s.nwat, 448));
   D.ww_cnt = I->alloc<int>(s.nwat);
   D.ww_nb = I->alloc<int>((size_t)s.nwat * D.ww_cap);
   D.wp_cnt = I->alloc<int>(s.nwat);
@@ -366,7 +472,6 @@ void Batch::upload(int k) {
   D.pp_mask = I->alloc<ulonglong2>((size_t)t.ncgp_solute * D.pp_cap);
   D.pw_cnt = I->alloc<int>(t.ncgp_solute);
   D.pw_nb = I->alloc<int>((size_t)t.ncgp_solute * D.pw_cap);
-  D.overflow = I->alloc<int>(1);
   D.rcpp2 = s.inp.rcpp * s.inp.rcpp;
   D.rcpw2 = s.inp.rcpw * s.inp.rcpw;
   D.rcww2 = s.inp.rcww * s.inp.rcww;
@@ -829,7 +934,8 @@ void Batch::upload(int k) {
   D.sh_i = I->upload(shi);
   D.sh_j = I->upload(shj);
   D.sh_d2 = I->upload(shd);
-  D.shake_fail = I->alloc<int>(1);
+  D.shake_maxiter = qsys::kShakeMaxIter;
+  if (const char* m = std::getenv("QGPU_SHAKE_MAXITER")) D.shake_maxiter = std::max(1, std::atoi(m));
   // energies and reductions
   D.e_water = I->alloc<double>((size_t)std::max(1, s.nwat) * kEWater);
   D.e_solute = I->alloc<double>((size_t)std::max(1, s.nat_solute) * 8);
@@ -837,7 +943,7 @@ void Batch::upload(int k) {
   D.e_far = I->alloc<double>(std::max(1, D.ncgp));
   I->ke_blocks = 0;
   std::vector<double> scal(8, 0.0);
-  initial_scales(s, scal.data());
+  initial_scales(s, s.v.data(), scal.data());
   D.scal = I->upload(scal);
   std::vector<double> e0(E_NTERMS, 0.0);
   e0[E_KINETIC] = scal[4];
@@ -1239,8 +1345,62 @@ int Batch::monitor(int k, double* out, int cap) {
   return need;
 }
 
-void Batch::make_lists() { launch_lists(); }
-void Batch::forces() { launch_forces(false, istep_); }
+void Batch::status(int k, int* bits, int* at_step) {
+  HIPCHECK(hipStreamSynchronize(stream_));
+  int r[kStatusInts];
+  HIPCHECK(hipMemcpy(r, h_sys_[k].status, sizeof(r), hipMemcpyDeviceToHost));
+  if (bits) *bits = r[0];
+  if (at_step) *at_step = r[1] - 1;
+}
+
+// Replaces instance k's list rows by rows of the given capacities (0 keeps
+// one); the next list update fills them.
+void Batch::set_list_caps(int k, int ww, int wp, int pw) {
+  HIPCHECK(hipStreamSynchronize(stream_));
+  DevSys& D = h_sys_[k];
+  Inst& I = *inst_[k];
+  if (ww < 0 || wp < 0 || pw < 0) throw std::runtime_error("negative list capacity");
+  if (ww) { ww = std::min(ww, std::max(1, D.nwat)); I.release(D.ww_nb); D.ww_nb = I.alloc<int>((size_t)D.nwat * ww); D.ww_cap = ww; }
+  if (wp) { wp = std::min(wp, std::max(1, D.ncgp_solute)); I.release(D.wp_nb); D.wp_nb = I.alloc<int>((size_t)D.nwat * wp); D.wp_cap = wp; }
+  if (pw) { pw = std::min(pw, std::max(1, D.nwat)); I.release(D.pw_nb); D.pw_nb = I.alloc<int>((size_t)D.ncgp_solute * pw); D.pw_cap = pw; }
+  sync_sys();
+}
+
+void Batch::make_lists() {
+  if (!dead_.empty()) throw std::runtime_error(dead_);
+  launch_lists();
+  HIPCHECK(hipStreamSynchronize(stream_));
+  // rows regrown: built again from the same coordinates
+  if (check_failures()) {
+    launch_lists();
+    HIPCHECK(hipStreamSynchronize(stream_));
+    check_failures();
+  }
+}
+// A force evaluation that includes only the term groups of `mask` (TM_*),
+// for per-term diagnostics; the batch then evaluates every term again.
+void Batch::forces_terms(int mask) {
+  if (mask <= 0 || mask > TM_ALL) throw std::runtime_error("invalid term mask");
+  HIPCHECK(hipStreamSynchronize(stream_));
+  for (auto& D : h_sys_) D.term_mask = mask;
+  sync_sys();
+  try {
+    forces();
+  } catch (...) {
+    for (auto& D : h_sys_) D.term_mask = TM_ALL;
+    sync_sys();
+    throw;
+  }
+  for (auto& D : h_sys_) D.term_mask = TM_ALL;
+  sync_sys();
+}
+
+void Batch::forces() {
+  if (!dead_.empty()) throw std::runtime_error(dead_);
+  launch_forces(false, istep_);
+  HIPCHECK(hipStreamSynchronize(stream_));
+  check_failures();
+}
 
 void Batch::one_step() {
   if (istep_ % nbcycle_ == 0) launch_lists();
@@ -1294,6 +1454,7 @@ void Batch::profile(int nsteps, double* ms, int cap) {
     throw;
   }
   prof_ = false;
+  check_failures();
   // cost of an event pair with nothing between them on this stream, removed
   // from every kernel's time
   double ovh = 0.0;
@@ -1309,12 +1470,9 @@ void Batch::profile(int nsteps, double* ms, int cap) {
     ms[k] = cnt[k] ? std::max(0.0, tot[k] / cnt[k] - ovh) : 0.0;
 }
 
-void Batch::step(int nsteps) {
-  if (!tuned_ && autotune_on_ && istep_ % nbcycle_ == 0 && nsteps > 0) autotune(3);
-  // work maps are uploaded here, outside any graph capture
-  for (int k = 0; k < TK_COUNT; ++k)
-    if (shape_[k].map != 0) work_map(k, shape_[k].map, shape_[k].block);
-  HIPCHECK(hipEventRecord(ev0, stream_));
+// Launches nsteps steps: single steps up to a list step, whole NBcycles as
+// the captured graph, single steps for the rest.
+void Batch::launch_steps(int nsteps) {
   int n = 0;
   while (n < nsteps && (!use_graph_ || istep_ % nbcycle_ != 0 || nsteps - n < nbcycle_)) {
     one_step();
@@ -1328,16 +1486,87 @@ void Batch::step(int nsteps) {
     one_step();
     ++n;
   }
-  HIPCHECK(hipEventRecord(ev1, stream_));
-  HIPCHECK(hipGetLastError());
-  HIPCHECK(hipStreamSynchronize(stream_));
-  for (size_t k = 0; k < h_sys_.size(); ++k) {
-    int flags[2] = {0, 0};
-    HIPCHECK(hipMemcpy(&flags[0], h_sys_[k].overflow, 4, hipMemcpyDeviceToHost));
-    HIPCHECK(hipMemcpy(&flags[1], h_sys_[k].shake_fail, 4, hipMemcpyDeviceToHost));
-    if (flags[0]) throw std::runtime_error("pair-list capacity exceeded");
-    if (flags[1]) throw std::runtime_error("shake failure");
+}
+
+// After a synchronisation: whether a kernel recorded a failure.  A list row
+// that outgrew its capacity is regrown and the batch resumes at the list
+// update where every instance stopped (returns true: steps to redo).  Any
+// other failure (SHAKE, NaN temperature, shell capacity) stops the batch for
+// good at the failing step, as die() stops qdyn (src/md.f90:554-613), and
+// throws.
+bool Batch::check_failures() {
+  unsigned long long f = kNoFailure;
+  HIPCHECK(hipMemcpy(&f, d_fail_, sizeof(f), hipMemcpyDeviceToHost));
+  if (f == kNoFailure) return false;
+  const int ni = (int)h_sys_.size();
+  std::vector<int> st((size_t)ni * kStatusInts), ctr(ni);
+  HIPCHECK(hipMemcpy(st.data(), d_status_, st.size() * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(ctr.data(), d_step_, ctr.size() * sizeof(int), hipMemcpyDeviceToHost));
+  std::string msg;
+  bool fatal = false;
+  for (int k = 0; k < ni; ++k) {
+    const int* r = &st[(size_t)k * kStatusInts];
+    if (!r[0]) continue;
+    if (r[0] & ~ST_LIST_OVERFLOW) fatal = true;
+    static const char* kWhat[4] = {"pair-list capacity exceeded", "water-shell capacity exceeded",
+                                   "shake failure", "NaN temperature"};
+    for (int b = 0; b < 4; ++b)
+      if (r[0] & (1 << b))
+        msg += "instance " + std::to_string(k) + ": " + kWhat[b] + " at step " + std::to_string(r[1] - 1) + "; ";
   }
+  // every instance stopped at the same step
+  istep_ = ctr[0];
+  if (fatal) {
+    dead_ = "run stopped: " + msg.substr(0, msg.size() - 2);
+    throw std::runtime_error(dead_);
+  }
+  for (int k = 0; k < ni; ++k) {
+    const int* r = &st[(size_t)k * kStatusInts];
+    if (r[0] & ST_LIST_OVERFLOW) grow_lists(k, r[2], r[3], r[4]);
+  }
+  HIPCHECK(hipMemset(d_status_, 0, st.size() * sizeof(int)));
+  HIPCHECK(hipMemcpy(d_fail_, &kNoFailure, sizeof(kNoFailure), hipMemcpyHostToDevice));
+  return true;
+}
+
+// New list rows for instance k that hold rows of the given lengths (with
+// headroom); the next list update fills them.
+void Batch::grow_lists(int k, int need_ww, int need_wp, int need_pw) {
+  DevSys& D = h_sys_[k];
+  Inst& I = *inst_[k];
+  auto bigger = [](int cap, int need, int most) {
+    if (need <= cap) return cap;
+    return std::max(1, std::min(most, (need + need / 4 + 32 + 15) / 16 * 16));
+  };
+  const int ww = bigger(D.ww_cap, need_ww, D.nwat), wp = bigger(D.wp_cap, need_wp, D.ncgp_solute);
+  const int pw = bigger(D.pw_cap, need_pw, D.nwat);
+  if (ww != D.ww_cap) { I.release(D.ww_nb); D.ww_nb = I.alloc<int>((size_t)D.nwat * ww); D.ww_cap = ww; }
+  if (wp != D.wp_cap) { I.release(D.wp_nb); D.wp_nb = I.alloc<int>((size_t)D.nwat * wp); D.wp_cap = wp; }
+  if (pw != D.pw_cap) { I.release(D.pw_nb); D.pw_nb = I.alloc<int>((size_t)D.ncgp_solute * pw); D.pw_cap = pw; }
+  ++list_regrowths_;
+  sync_sys();
+}
+
+void Batch::step(int nsteps) {
+  if (!dead_.empty()) throw std::runtime_error(dead_);
+  if (!tuned_ && autotune_on_ && istep_ % nbcycle_ == 0 && nsteps > 0) autotune(3);
+  // work maps are uploaded here, outside any graph capture
+  for (int k = 0; k < TK_COUNT; ++k)
+    if (shape_[k].map != 0) work_map(k, shape_[k].map, shape_[k].block);
+  HIPCHECK(hipEventRecord(ev0, stream_));
+  // failures are checked every kCheckSteps steps and at the end; after a
+  // failure every kernel that changes the state returns at once, so the
+  // steps launched before the check do nothing
+  constexpr int kCheckSteps = 1000;
+  const int target = istep_ + nsteps;
+  while (istep_ < target) {
+    launch_steps(std::min(target - istep_, kCheckSteps));
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(stream_));
+    check_failures();
+  }
+  HIPCHECK(hipEventRecord(ev1, stream_));
+  HIPCHECK(hipStreamSynchronize(stream_));
 }
 
 void Batch::get_state(int k, double* x, double* v) {
@@ -1351,7 +1580,17 @@ void Batch::set_state(int k, const double* x, const double* v) {
   HIPCHECK(hipStreamSynchronize(stream_));
   size_t n = 3 * (size_t)h_sys_[k].natom;
   if (x) HIPCHECK(hipMemcpy(h_sys_[k].x, x, n * 8, hipMemcpyHostToDevice));
-  if (v) HIPCHECK(hipMemcpy(h_sys_[k].v, v, n * 8, hipMemcpyHostToDevice));
+  if (v) {
+    HIPCHECK(hipMemcpy(h_sys_[k].v, v, n * 8, hipMemcpyHostToDevice));
+    // temperature of the new velocities, as md_run computes it before its
+    // first step (src/md.f90:6400-6404): the Berendsen scales and Tfree
+    // (which the radial water restraint reads) follow the state
+    std::vector<double> scal(8, 0.0);
+    HIPCHECK(hipMemcpy(scal.data(), h_sys_[k].scal, 8 * sizeof(double), hipMemcpyDeviceToHost));
+    initial_scales(*setups_[k], v, scal.data());
+    HIPCHECK(hipMemcpy(h_sys_[k].scal, scal.data(), 8 * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(h_sys_[k].energies + E_KINETIC, &scal[4], sizeof(double), hipMemcpyHostToDevice));
+  }
   if (x) refresh_xf();
 }
 
@@ -1560,6 +1799,66 @@ void Batch::list_counts(int k, long* counts) {
   counts[4] = 3L * qc[1] * D.nqat;
 }
 
+// Work of one list update of instance k at its current coordinates,
+// counted on the host with the list kernels' tests: [0] atom-to-centre
+// multipole evaluations (lrf_update) of the water-side list kernel,
+// [1] of the solute-side one, [2] water-water list entries (both
+// directions), [3] water-solute group entries, [4] solute group pairs
+// within the cut-off (both directions), [5] solute-water entries.
+void Batch::work_counts(int k, double* out) {
+  const DevSys& D = h_sys_[k];
+  const qsys::Setup& s = *setups_[k];
+  const qsys::Topology& t = s.top;
+  std::vector<double> x(3 * (size_t)D.natom);
+  get_state(k, x.data(), nullptr);
+  for (int c = 0; c < 6; ++c) out[c] = 0.0;
+  auto d2 = [&](int a, int b) {
+    const double p = x[3 * a] - x[3 * b], q = x[3 * a + 1] - x[3 * b + 1], r = x[3 * a + 2] - x[3 * b + 2];
+    return p * p + q * q + r * r;
+  };
+  std::vector<int> nonq(t.cgp.size(), 0);
+  for (size_t g = 0; g < t.cgp.size(); ++g)
+    for (int a = t.cgp[g].first; a <= t.cgp[g].last; ++a) nonq[g] += s.iqatom[t.cgpatom[a]] ? 0 : 1;
+  const bool lrf = D.use_lrf;
+  const int ns = s.nat_solute;
+  for (int iw = 0; iw < s.nwat; ++iw) {
+    const int io = ns + 3 * iw;
+    if (s.excl[io]) continue;
+    for (int jw = 0; jw < s.nwat; ++jw) {
+      const int jo = ns + 3 * jw;
+      if (jw == iw || s.excl[jo]) continue;
+      const double r2 = d2(io, jo);
+      if (r2 <= D.rcww2) out[2] += 1;
+      else if (lrf && r2 <= D.rclrf2) out[0] += nonq[s.iwhich_cgp[jo]];
+    }
+    for (int g = 0; g < t.ncgp_solute; ++g) {
+      const int sw = t.cgp[g].iswitch;
+      if (s.excl[sw]) continue;
+      const double r2 = d2(sw, io);
+      if (r2 <= D.rcpw2) out[3] += 1;
+      else if (lrf && r2 <= D.rclrf2) out[0] += nonq[g];
+    }
+  }
+  for (int g = 0; g < t.ncgp_solute; ++g) {
+    const int sw = t.cgp[g].iswitch;
+    if (s.excl[sw]) continue;
+    for (int jg = 0; jg < t.ncgp_solute; ++jg) {
+      const int sj = t.cgp[jg].iswitch;
+      if (jg == g || s.excl[sj]) continue;
+      const double r2 = d2(sw, sj);
+      if (r2 <= D.rcpp2) out[4] += 1;
+      else if (lrf && r2 <= D.rclrf2) out[1] += nonq[jg];
+    }
+    for (int jw = 0; jw < s.nwat; ++jw) {
+      const int jo = ns + 3 * jw;
+      if (s.excl[jo]) continue;
+      const double r2 = d2(sw, jo);
+      if (r2 <= D.rcpw2) out[5] += 1;
+      else if (lrf && r2 <= D.rclrf2) out[1] += nonq[s.iwhich_cgp[jo]];
+    }
+  }
+}
+
 double Batch::temperature(int k) {
   HIPCHECK(hipStreamSynchronize(stream_));
   double sc[4];
@@ -1646,6 +1945,13 @@ int qgpu_pot_energy(qgpu_handle* h) {
   return guarded([&] {
     if (!h) throw std::runtime_error("null handle");
     h->b->forces();
+  });
+}
+
+int qgpu_pot_energy_terms(qgpu_handle* h, int mask) {
+  return guarded([&] {
+    if (!h) throw std::runtime_error("null handle");
+    h->b->forces_terms(mask);
   });
 }
 
@@ -1780,6 +2086,31 @@ int qgpu_monitor(qgpu_handle* h, int k, double* out, int cap, int* n) {
     CHECK_K(h, k);
     *n = h->b->monitor(k, out, cap);
   });
+}
+
+int qgpu_work_counts(qgpu_handle* h, int k, double* counts, int cap) {
+  return guarded([&] {
+    CHECK_K(h, k);
+    double c[6];
+    h->b->work_counts(k, c);
+    for (int i = 0; i < 6 && i < cap; ++i) counts[i] = c[i];
+  });
+}
+
+int qgpu_instance_status(qgpu_handle* h, int k, int* bits, int* at_step) {
+  return guarded([&] { CHECK_K(h, k); h->b->status(k, bits, at_step); });
+}
+
+int qgpu_list_capacity(qgpu_handle* h, int k, int* caps, int* regrowths) {
+  return guarded([&] {
+    CHECK_K(h, k);
+    if (caps) h->b->list_caps(k, caps);
+    if (regrowths) *regrowths = h->b->list_regrowths();
+  });
+}
+
+int qgpu_set_list_capacity(qgpu_handle* h, int k, int ww, int wp, int pw) {
+  return guarded([&] { CHECK_K(h, k); h->b->set_list_caps(k, ww, wp, pw); });
 }
 
 int qgpu_offdiag_states(qgpu_handle* h, int k, int* ij, int cap, int* n) {

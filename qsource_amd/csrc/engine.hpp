@@ -84,6 +84,7 @@ struct DevSys {
   int natom, nat_solute, nwat, ncgp, ncgp_solute;
   int nqat, nstates;
   int precision;                 // 0 = FP64, 1 = mixed FP32 pair / FP64 accumulation
+  int term_mask;                 // TM_* terms a force evaluation includes (all but in term diagnostics)
   double *x, *v, *xx, *d, *xtop;
   float* xf;                     // FP32 copy of x for the mixed-precision pair kernels
   // monitored group pairs (nonbond_monitor): atoms of group i and j of pair
@@ -123,7 +124,15 @@ struct DevSys {
   // atom pair (slot a of g, slot b of jg) is not evaluated (same atom, Q atom,
   // zero interaction or excluded), of .y when it is a 1-4 pair
   ulonglong2* pp_mask;
-  int* overflow;                 // set to 1 when a list row exceeds its cap
+  // failure record of this instance (kStatusInts ints): [0] ST_* bits,
+  // [1] step + 1 of the first failure, [2..4] longest ww, wp and pw row seen
+  // at the failing list build (the capacity the rows need)
+  int* status;
+  // batch-wide failure key (step * 16 + launch phase of the first failure,
+  // ULLONG_MAX while none): a state-changing kernel of phase p at step s
+  // returns at once when *fail < s * 16 + p, so every instance stops at the
+  // same point of the failing step; 0 halts everything
+  unsigned long long* fail;
   // cut-offs squared
   double rcpp2, rcpw2, rcww2, rclrf2;
   bool use_lrf;
@@ -237,7 +246,7 @@ struct DevSys {
   // (a,a+2), (a+1,a+2) in that order): integrated entirely in registers
   const signed char* unit_kind;
   const double* sh_d2;
-  int* shake_fail;
+  int shake_maxiter;             // SHAKE_MAX_ITER (1000); QGPU_SHAKE_MAXITER lowers it to force a failure
   // energies: eacc[term] partial sums written per item, reduced on demand
   double* e_water;     // nwat x kEWater : ww_el, ww_vdw, lrf(water atoms), radial, excluded-atom restraint
   double* e_solute;    // nat_solute x 8 : pp_el pp_vdw pw_el pw_vdw lrf fix shell spare
@@ -278,6 +287,16 @@ struct DevSys {
 
 constexpr int kEneRecord = 14;
 
+// Failure bits of DevSys::status and the launch phases of the failure key.
+enum : int { ST_LIST_OVERFLOW = 1, ST_SHELL_OVERFLOW = 2, ST_SHAKE = 4, ST_NAN = 8 };
+constexpr int kStatusInts = 8;
+// Term groups of a force evaluation (DevSys::term_mask): water-water,
+// solute-water, solute-solute, Q-atom nonbonded (Q-solute and Q-water),
+// LRF Taylor, and the rest (bonded, Q-Q, Q-bonded, restraints, polarisation)
+enum : int { TM_WW = 1, TM_PW = 2, TM_PP = 4, TM_QX = 8, TM_LRF = 16, TM_OTHER = 32, TM_ALL = 63 };
+enum : int { PH_LISTS = 1, PH_TERMS = 2, PH_WATPOL = 3, PH_RECORD = 4, PH_INTEGRATE = 5, PH_TEMPERATURE = 7 };
+constexpr unsigned long long kNoFailure = ~0ull;
+
 // One GPU batch of instances sharing every launch.
 class Batch {
  public:
@@ -289,6 +308,7 @@ class Batch {
   void step(int nsteps);
   void make_lists();
   void forces();
+  void forces_terms(int mask);
   void get_state(int k, double* x, double* v);
   void set_state(int k, const double* x, const double* v);
   void get_gradient(int k, double* d);
@@ -296,9 +316,17 @@ class Batch {
   void qenergies(int k, double* e, int cap);
   void export_list(int k, int which, std::vector<int>& out);
   void list_counts(int k, long* counts);
+  void work_counts(int k, double* out);
   double temperature(int k);
   void theta_corr(int k, double* out);
   int istep() const { return istep_; }
+  // failure record of instance k: ST_* bits and the step they were found at
+  void status(int k, int* bits, int* at_step);
+  int list_regrowths() const { return list_regrowths_; }
+  void set_list_caps(int k, int ww, int wp, int pw);
+  void list_caps(int k, int* caps) const {
+    caps[0] = h_sys_[k].ww_cap; caps[1] = h_sys_[k].wp_cap; caps[2] = h_sys_[k].pw_cap;
+  }
   const qsys::Setup& setup(int k) const { return *setups_[k]; }
   hipStream_t stream() const { return stream_; }
   // timing helpers
@@ -356,6 +384,9 @@ class Batch {
   void fork();
   void join();
   void one_step();
+  void launch_steps(int nsteps);
+  bool check_failures();
+  void grow_lists(int k, int need_ww, int need_wp, int need_pw);
   void launch_cycle_graph();
   void sync_sys();
   void refresh_xf();
@@ -367,7 +398,10 @@ class Batch {
   long ebase_ = 0;         // step of the last set_energy_interval
   bool use_graph_ = true;
   hipGraphExec_t graph_exec_ = nullptr;
-  int* d_step_ = nullptr;
+  int* d_step_ = nullptr;              // per-instance step counters
+  int* d_status_ = nullptr;            // per-instance failure records (kStatusInts each)
+  unsigned long long* d_fail_ = nullptr;   // batch-wide failure key
+  std::string dead_;                   // set when a failure stopped the batch for good
   void upload(int k);
   void launch_lists();
   void launch_forces(bool step, long s);
@@ -400,6 +434,7 @@ class Batch {
   hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
   bool branches_ = true;            // QGPU_BRANCHES=0 runs every kernel on one stream
   int istep_ = 0;
+  int list_regrowths_ = 0;
   int precision_ = 0;
   int max_nwat_ = 0, max_ncgp_ = 0, max_ncgp_solute_ = 0, max_units_ = 0, max_natom_ = 0;
   int max_nat_solute_ = 0, max_nstates_ = 0;

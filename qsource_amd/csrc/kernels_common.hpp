@@ -110,6 +110,22 @@ __device__ __forceinline__ double segn_sum(double v) {
   return v;
 }
 
+// Failure handling.  A failure found by a launch of phase p at step s is
+// recorded as the key s * 16 + p (the smallest key wins).  A kernel that
+// changes the simulation state returns at once when a launch before it
+// recorded a failure, so the state stops where the failing step found it.
+// Within one launch a key is never below that launch's own key, so all of
+// its workgroups take the same decision.
+__device__ __forceinline__ unsigned long long fail_key(const DevSys& S, int phase) {
+  return (unsigned long long)(unsigned)*S.step_ctr * 16ull + (unsigned long long)phase;
+}
+__device__ __forceinline__ bool halted(const DevSys& S, int phase) { return *S.fail < fail_key(S, phase); }
+__device__ __forceinline__ void record_failure(const DevSys& S, int bit, int phase) {
+  atomicOr(S.status, bit);
+  atomicCAS(S.status + 1, 0, *S.step_ctr + 1);
+  atomicMin(S.fail, fail_key(S, phase));
+}
+
 // True on a step whose energy-file frame is recorded (step counter a positive
 // multiple of iene since recording started) when the batch also keeps the
 // FP64 re-evaluation of its Q energies.

@@ -134,7 +134,10 @@ __global__ __launch_bounds__(256) void k_solute_lists(const DevSys* __restrict__
     if (S.use_lrf) acc.store(Lg);
     S.pp_cnt[g] = n_pp < S.pp_cap ? n_pp : S.pp_cap;
     S.pw_cnt[g] = n_pw < S.pw_cap ? n_pw : S.pw_cap;
-    if (n_pp > S.pp_cap || n_pw > S.pw_cap) atomicOr(S.overflow, 1);
+    if (n_pp > S.pp_cap || n_pw > S.pw_cap) {
+      atomicMax(S.status + 4, n_pw);
+      record_failure(S, ST_LIST_OVERFLOW, PH_LISTS);
+    }
   }
 }
 template __global__ void k_solute_lists<double, 0>(const DevSys*, const int2*);
@@ -177,7 +180,7 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(c
   double e_ppel = 0, e_ppvdw = 0, e_pwel = 0, e_pwvdw = 0, e_lrf = 0, e_fix = 0, e_shell = 0;
   if (active && !(fl & F_Q)) {
     // solute-solute
-    const int npp = S.pp_cnt[g];
+    const int npp = (S.term_mask & TM_PP) ? S.pp_cnt[g] : 0;
     const gconst<int> prow = gptr(S.pp_nb) + (size_t)g * S.pp_cap;
     // (skip, 1-4) mask pairs of the entries, read as two 64-bit words
     const gconst<unsigned long long> mrow =
@@ -238,7 +241,7 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(c
       }
     }
     // solute-water (the solute atom is i of the reference pair)
-    const int npw = S.pw_cnt[g];
+    const int npw = (S.term_mask & TM_PW) ? S.pw_cnt[g] : 0;
     const gconst<int> wrow = gptr(S.pw_nb) + (size_t)g * S.pw_cap;
     // standard waters: LJ products and zero-interaction mask of this atom
     // with O, H1, H2 from the per-atom table
@@ -309,7 +312,7 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(c
       }
     }
     // Q-solute pairs seen from the solute atom (the Q atom is i of the pair)
-    if (fl & F_QP) {
+    if ((fl & F_QP) && (S.term_mask & TM_QX)) {
       for (int iq = sub; iq < S.nqat; iq += 8) {
         const int qa = S.iqseq[iq];
         const int lj = S.qp_lj[(size_t)i * S.nqat + iq];
@@ -346,7 +349,7 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(c
       }
     }
     // LRF Taylor expansion about the group centre (FP64)
-    if (sub == 1 && S.use_lrf && !(fl & F_EXCL)) {
+    if (sub == 1 && S.use_lrf && !(fl & F_EXCL) && (S.term_mask & TM_LRF)) {
       const double* L = S.lrf + (size_t)g * kLrfStride;
       const double xa[3] = {S.x[3 * i], S.x[3 * i + 1], S.x[3 * i + 2]};
       double df[3];
@@ -359,7 +362,7 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(c
   }
   // harmonic restraint of excluded and shell atoms to the topology position
   // (excluded_freeze: the atom is put back there instead, no restraint)
-  if (active && sub == 2 && (fl & (F_EXCL | F_SHELL))) {
+  if (active && sub == 2 && (fl & (F_EXCL | F_SHELL)) && (S.term_mask & TM_OTHER)) {
     const double fk = (fl & F_EXCL) ? (S.freeze ? 0.0 : S.fk_fix) : S.fk_pshell;
     const double dr0 = S.x[3 * i] - S.xtop[3 * i], dr1 = S.x[3 * i + 1] - S.xtop[3 * i + 1],
                  dr2 = S.x[3 * i + 2] - S.xtop[3 * i + 2];
@@ -728,6 +731,7 @@ __global__ __launch_bounds__(64) void k_q_reduce(const DevSys* __restrict__ sys,
   for (int c = shadow ? 3 : 0; c < n; ++c) {
     double v = 0.0;
     for (int k = 0; k < kQChunks; ++k) v = v + part[(size_t)(iq * kQChunks + k) * kQPartStride + c];
+    if (!(S.term_mask & TM_QX)) v = 0.0;
     if (c < 3) S.d[3 * qi + c] = S.d[3 * qi + c] + v;
     else eq[(size_t)iq * ns * 4 + (c - 3)] = v;
   }
@@ -1286,7 +1290,7 @@ __global__ __launch_bounds__(256) void k_terms(const DevSys* __restrict__ sys) {
 __global__ __launch_bounds__(256) void k_term_sum(const DevSys* __restrict__ sys) {
   const DevSys& S = sys[blockIdx.y];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < S.nqbond)
+  if (t < S.nqbond && !halted(S, PH_TERMS))
     for (int c = 0; c < kMorseRec; ++c) S.qb_prev[kMorseRec * t + c] = S.qb_cur[kMorseRec * t + c];
   if (t >= S.nslot_atoms) return;
   double g0 = 0, g1 = 0, g2 = 0;
@@ -1295,6 +1299,7 @@ __global__ __launch_bounds__(256) void k_term_sum(const DevSys* __restrict__ sys
   for (int k = 0; k < n; ++k) {
     g0 = g0 + o[3 * k]; g1 = g1 + o[3 * k + 1]; g2 = g2 + o[3 * k + 2];
   }
+  if (!(S.term_mask & TM_OTHER)) g0 = g1 = g2 = 0.0;
   S.slot_sum[3 * t] = g0;
   S.slot_sum[3 * t + 1] = g1;
   S.slot_sum[3 * t + 2] = g2;
@@ -1305,7 +1310,7 @@ __global__ __launch_bounds__(256) void k_term_sum(const DevSys* __restrict__ sys
 __global__ __launch_bounds__(256) void k_term_gather(const DevSys* __restrict__ sys) {
   const DevSys& S = sys[blockIdx.y];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < S.nqbond)
+  if (t < S.nqbond && !halted(S, PH_TERMS))
     for (int c = 0; c < kMorseRec; ++c) S.qb_prev[kMorseRec * t + c] = S.qb_cur[kMorseRec * t + c];
   if (t >= S.nslot_atoms) return;
   const int me = S.slot_atom[t];
@@ -1315,6 +1320,7 @@ __global__ __launch_bounds__(256) void k_term_gather(const DevSys* __restrict__ 
   for (int k = 0; k < n; ++k) {
     g0 = g0 + o[3 * k]; g1 = g1 + o[3 * k + 1]; g2 = g2 + o[3 * k + 2];
   }
+  if (!(S.term_mask & TM_OTHER)) return;
   S.d[3 * me] = S.d[3 * me] + g0;
   S.d[3 * me + 1] = S.d[3 * me + 1] + g1;
   S.d[3 * me + 2] = S.d[3 * me + 2] + g2;

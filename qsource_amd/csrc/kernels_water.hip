@@ -147,7 +147,11 @@ __global__ __launch_bounds__(256) void k_water_lists(const DevSys* __restrict__ 
       if (S.use_lrf) acc.store(Lg);
       S.ww_cnt[iw] = n_in < S.ww_cap ? n_in : S.ww_cap;
       S.wp_cnt[iw] = n_wp < S.wp_cap ? n_wp : S.wp_cap;
-      if (n_in > S.ww_cap || n_wp > S.wp_cap) atomicOr(S.overflow, 1);
+      if (n_in > S.ww_cap || n_wp > S.wp_cap) {
+        atomicMax(S.status + 2, n_in);
+        atomicMax(S.status + 3, n_wp);
+        record_failure(S, ST_LIST_OVERFLOW, PH_LISTS);
+      }
     }
   }
 }
@@ -204,7 +208,7 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(co
   if constexpr (AA != 0)
     for (int k = 0; k < 9; ++k) { aww[k] = (P)S.aww[k]; bww[k] = (P)S.bww[k]; }
   // water-water
-  const int nn = active ? S.ww_cnt[iw] : 0;
+  const int nn = active && (S.term_mask & TM_WW) ? S.ww_cnt[iw] : 0;
   const gconst<int> row = gptr(S.ww_nb) + (size_t)iw * S.ww_cap;
   auto ww_pair = [&](const int jw, const P* xj) {
     P el = 0, vdw = 0;
@@ -278,7 +282,7 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(co
   }
   // water-solute: every non-Q atom of each listed solute group (the solute
   // atom is i of the reference pair)
-  const int nwp = active ? S.wp_cnt[iw] : 0;
+  const int nwp = active && (S.term_mask & TM_PW) ? S.wp_cnt[iw] : 0;
   const gconst<int> wrow = gptr(S.wp_nb) + (size_t)iw * S.wp_cap;
   for (int k = sub; k < nwp; k += 8) {
     const int gs = wrow[k];
@@ -322,7 +326,7 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(co
   // fixed Q LJ parameters the states fold into one lambda-weighted charge
   // (sum_s lambda_s qcrg(iq,s)) and one lambda sum for the LJ part; with
   // Q-atom types (soft-core per state) each state is evaluated.
-  if (active && (S.flags[io] & F_QW)) {
+  if (active && (S.flags[io] & F_QW) && (S.term_mask & TM_QX)) {
     const int ns = S.nstates;
     const bool qv = S.qvdw != 0;
     for (int iq = sub; iq < S.nqat; iq += 8) {
@@ -369,7 +373,7 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(co
     }
   }
   // LRF Taylor expansion (FP64), atom `sub` of the water on lanes 0..2
-  if (active && S.use_lrf && sub < 3) {
+  if (active && S.use_lrf && sub < 3 && (S.term_mask & TM_LRF)) {
     const int i = io + sub;
     if ((S.flags[i] & (F_EXCL | F_Q)) == 0) {
       const double* L = S.lrf + (size_t)S.iwhich_cgp[io] * kLrfStride;
@@ -386,7 +390,7 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(co
     }
   }
   // radial solvent restraint on the water O (switching atom), lane 3 (FP64)
-  if (active && sub == 3 && !(S.flags[io] & F_EXCL)) {
+  if (active && sub == 3 && !(S.flags[io] & F_EXCL) && (S.term_mask & TM_OTHER)) {
     const double shift = S.fk_wsphere != 0.0 ? sqrt(qsys::kBoltz * S.scal[2] / S.fk_wsphere) : 0.0;
     const double dr0 = S.x[3 * io] - S.xwcent[0], dr1 = S.x[3 * io + 1] - S.xwcent[1],
                  dr2 = S.x[3 * io + 2] - S.xwcent[2];
@@ -411,7 +415,7 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(co
   // excluded water atoms: fix_shell's restraint to the topology position
   // (src/md.f90:2816-2862), atom sub-4 on lanes 4..6 (FP64)
   double e_fix = 0.0;
-  if (active && sub >= 4 && sub < 7 && (S.flags[io + sub - 4] & F_EXCL)) {
+  if (active && sub >= 4 && sub < 7 && (S.flags[io + sub - 4] & F_EXCL) && (S.term_mask & TM_OTHER)) {
     const int i = io + sub - 4;
     const double fk = S.freeze ? 0.0 : S.fk_fix;
     const double dr0 = S.x[3 * i] - S.xtop[3 * i], dr1 = S.x[3 * i + 1] - S.xtop[3 * i + 1],
@@ -495,7 +499,7 @@ constexpr int kShellCap = 4096;
 __global__ __launch_bounds__(1024) void k_watpol_force(const DevSys* __restrict__ sys) {
   const DevSys& S = sys[blockIdx.y];
   const int is = blockIdx.x;
-  if (!S.wpol_restr || is >= S.nshell) return;
+  if (!S.wpol_restr || is >= S.nshell || halted(S, PH_WATPOL)) return;
   const int istep = *S.step_ctr;
   const bool update = istep != 0 && istep % qsys::kItdisUpdate == 0;
   __shared__ int mem_iw[kShellCap];
@@ -530,7 +534,7 @@ __global__ __launch_bounds__(1024) void k_watpol_force(const DevSys* __restrict_
     __syncthreads();
   }
   int n = s_cnt;
-  if (n > kShellCap) { if (threadIdx.x == 0) atomicOr(S.overflow, 2); n = kShellCap; }
+  if (n > kShellCap) { if (threadIdx.x == 0) record_failure(S, ST_SHELL_OVERFLOW, PH_WATPOL); n = kShellCap; }
   // theta_corr update every itdis_update steps (before the forces use it)
   if (update && threadIdx.x == 0) {
     W.avtheta = W.avtheta / (double)qsys::kItdisUpdate;
@@ -567,7 +571,7 @@ __global__ __launch_bounds__(1024) void k_watpol_force(const DevSys* __restrict_
     f0 = -1.0 / f0;
     f0 = dv * f0;
     double* d = S.d + 3 * i;
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 3 && (S.term_mask & TM_OTHER); ++k) {
       double f1a = -2.0 * (rcu[k] - rmu[k] * scp) / rm;
       double f1b = (rcu[k] - rmu[k] * scp) / rm;
       double f2 = (rmu[k] - rcu[k] * scp) / rc;
@@ -607,6 +611,7 @@ __device__ __forceinline__ void gather_slots(const DevSys& S, int a) {
 
 __global__ __launch_bounds__(kIntegrateBlock) void k_integrate(const DevSys* __restrict__ sys) {
   const DevSys& S = sys[blockIdx.y];
+  if (halted(S, PH_INTEGRATE)) return;
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
   double ke[6] = {0, 0, 0, 0, 0, 0};  // solute tot, solute free, solvent tot, solvent free, spare
   if (u < S.nunits && S.unit_kind[u] == 1) {
@@ -653,7 +658,7 @@ __global__ __launch_bounds__(kIntegrateBlock) void k_integrate(const DevSys* __r
       }
       ++nits;
       if (ready == 7u) break;
-      if (nits >= qsys::kShakeMaxIter) { atomicOr(S.shake_fail, 1); break; }
+      if (nits >= S.shake_maxiter) { record_failure(S, ST_SHAKE, PH_INTEGRATE); break; }
     }
     for (int c = 0; c < 9; ++c) {
       const int ic = 3 * a + c;
@@ -721,7 +726,7 @@ __global__ __launch_bounds__(kIntegrateBlock) void k_integrate(const DevSys* __r
         }
         ++nits;
         if ((ready & all) == all) break;
-        if (nits >= qsys::kShakeMaxIter) { atomicOr(S.shake_fail, 1); break; }
+        if (nits >= S.shake_maxiter) { record_failure(S, ST_SHAKE, PH_INTEGRATE); break; }
       }
     }
     for (int k = a0; k < a1; ++k) {
@@ -750,6 +755,12 @@ __global__ __launch_bounds__(kIntegrateBlock) void k_integrate(const DevSys* __r
 // the next step (src/md.f90:6098-6190).  One workgroup per instance.
 __global__ __launch_bounds__(256) void k_temperature(const DevSys* __restrict__ sys, int nblocks) {
   const DevSys& S = sys[blockIdx.y];
+  // a failure earlier in this step: the step is not completed, and every
+  // later launch returns at once (key 0)
+  if (halted(S, PH_TEMPERATURE)) {
+    if (threadIdx.x == 0) atomicMin(S.fail, 0ull);
+    return;
+  }
   __shared__ double red[4][4];
   double p[4] = {0, 0, 0, 0};
   for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
@@ -779,8 +790,11 @@ __global__ __launch_bounds__(256) void k_temperature(const DevSys* __restrict__ 
     tsu = tsv;
   }
   S.scal[0] = tsu; S.scal[1] = tsv; S.scal[2] = tfree; S.scal[3] = temp;
-  // end of the step: advance the batch's step counter once
-  if (blockIdx.y == 0) *S.step_ctr = *S.step_ctr + 1;
+  // a NaN temperature stops the run after this step (md_run's test,
+  // src/md.f90:6654-6656, made at every step)
+  if (temp != temp) record_failure(S, ST_NAN, PH_TEMPERATURE);
+  // end of the step: advance this instance's step counter
+  *S.step_ctr = *S.step_ctr + 1;
 }
 
 // Sum the per-item energy partials into the instance's energy terms.
@@ -791,6 +805,7 @@ __global__ __launch_bounds__(256) void k_energy_reduce(const DevSys* __restrict_
   __shared__ double red[12][4];
   int frame = -1;
   if (record) {
+    if (halted(S, PH_RECORD)) return;
     const int st = *S.step_ctr - S.eseries_base;
     if (S.iene <= 0 || st <= 0 || st % S.iene != 0) return;
     frame = st / S.iene - 1;

@@ -39,7 +39,10 @@ SYMBOLS = [
     "qgpu_profile_kernels", "qgpu_list_counts", "qgpu_offdiag_states", "qgpu_monitor",
     "qgpu_trj_create", "qgpu_trj_natoms", "qgpu_trj_write", "qgpu_trj_close", "qgpu_write_restart",
     "qgpu_autotune", "qgpu_set_launch_shape",
+    "qgpu_instance_status", "qgpu_list_capacity", "qgpu_set_list_capacity", "qgpu_pot_energy_terms",
+    "qgpu_work_counts",
 ]
+TERM_GROUPS = {"ww": 1, "pw": 2, "pp": 4, "qx": 8, "lrf": 16, "other": 32}
 TUNED_KERNELS = ["water_lists", "solute_lists", "water_forces", "solute_forces"]
 
 KERNEL_NAMES = ["cgp_centers", "water_lists", "solute_lists", "water_forces", "solute_forces",
@@ -103,6 +106,11 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.qgpu_monitor.argtypes = [P, ctypes.c_int, dp, ctypes.c_int, ip]
     lib.qgpu_autotune.argtypes = [P, ctypes.c_int, dp, ip]
     lib.qgpu_set_launch_shape.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.qgpu_pot_energy_terms.argtypes = [P, ctypes.c_int]
+    lib.qgpu_work_counts.argtypes = [P, ctypes.c_int, dp, ctypes.c_int]
+    lib.qgpu_instance_status.argtypes = [P, ctypes.c_int, ip, ip]
+    lib.qgpu_list_capacity.argtypes = [P, ctypes.c_int, ip, ip]
+    lib.qgpu_set_list_capacity.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     lib.qgpu_trj_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(P)]
     lib.qgpu_trj_natoms.argtypes = [P]
     lib.qgpu_trj_write.argtypes = [P, dp]
@@ -165,8 +173,13 @@ class Engine:
     def make_pair_lists(self):
         self._check(self.lib.qgpu_make_pair_lists(self.h))
 
-    def pot_energy(self):
-        self._check(self.lib.qgpu_pot_energy(self.h))
+    def pot_energy(self, terms: Optional[Sequence[str]] = None):
+        """Gradient and energies at the current coordinates; with `terms`
+        (names of TERM_GROUPS) only those term groups enter the gradient."""
+        if terms is None:
+            self._check(self.lib.qgpu_pot_energy(self.h))
+        else:
+            self._check(self.lib.qgpu_pot_energy_terms(self.h, sum(TERM_GROUPS[t] for t in set(terms))))
 
     def state(self, k: int = 0):
         x = np.zeros(3 * self.natom[k])
@@ -210,6 +223,13 @@ class Engine:
         c = (ctypes.c_long * 5)()
         self._check(self.lib.qgpu_list_counts(self.h, k, c))
         return dict(zip(["pp", "pw", "ww", "qp", "qw"], list(c)))
+
+    def work_counts(self, k: int = 0) -> dict:
+        """Work of one list update of instance k (roofline accounting)."""
+        c = np.zeros(6)
+        self._check(self.lib.qgpu_work_counts(self.h, k, _dp(c), 6))
+        return dict(zip(["lrf_evals_water_side", "lrf_evals_solute_side", "ww_entries", "wp_entries",
+                         "pp_group_pairs", "pw_entries"], c.tolist()))
 
     def temperature(self, k: int = 0) -> float:
         t = ctypes.c_double()
@@ -295,6 +315,23 @@ class Engine:
         t = ctypes.c_double()
         self._check(self.lib.qgpu_last_step_ms(self.h, ctypes.byref(t)))
         return t.value
+
+    def status(self, k: int = 0):
+        """(failure bits, step found at): 1 list row over capacity, 2 water
+        shell capacity, 4 SHAKE failure, 8 NaN temperature; (0, -1) when none."""
+        b, at = ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.qgpu_instance_status(self.h, k, ctypes.byref(b), ctypes.byref(at)))
+        return b.value, at.value
+
+    def list_capacity(self, k: int = 0):
+        """((ww, wp, pw) row capacities of instance k, list regrowths of the batch)."""
+        caps = (ctypes.c_int * 3)()
+        n = ctypes.c_int()
+        self._check(self.lib.qgpu_list_capacity(self.h, k, caps, ctypes.byref(n)))
+        return tuple(caps), n.value
+
+    def set_list_capacity(self, k: int, ww: int = 0, wp: int = 0, pw: int = 0):
+        self._check(self.lib.qgpu_set_list_capacity(self.h, k, ww, wp, pw))
 
 
 class FarFieldMLP:
