@@ -370,10 +370,10 @@ class Batch {
 
  private:
   void pb(int id) {
-    if (prof_) hipEventRecord(pev_[2 * id], stream_);
+    if (prof_) (void)hipEventRecord(pev_[2 * id], stream_);
   }
   void pe(int id) {
-    if (prof_) { hipEventRecord(pev_[2 * id + 1], stream_); pused_[id] = 1; }
+    if (prof_) { (void)hipEventRecord(pev_[2 * id + 1], stream_); pused_[id] = 1; }
   }
   bool prof_ = false;
   hipEvent_t pev_[2 * 13] = {};
