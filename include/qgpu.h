@@ -239,6 +239,15 @@ int qgpu_mlp_set_data(qgpu_mlp* m, const float* X, const float* Y, int n);
 int qgpu_mlp_train(qgpu_mlp* m, int epochs, int batch, float lr, int seed, double* loss);
 /* Gradient of the mean squared error over (X, Y) w.r.t. all parameters. */
 int qgpu_mlp_gradient(qgpu_mlp* m, const float* X, const float* Y, int n, float* grad, double* loss);
+/* As qgpu_mlp_gradient, allocating no device memory.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph
+ * capture, tight serving or training loops).  The training activations and
+ * the staging copies of X and Y live in a static device array of about
+ * 5.1 MiB per GPU (held until the process ends), for at most 2048 samples
+ * per call; larger n is refused.  Calls that overlap on one GPU share that
+ * array and race.  Same kernels and launch shapes as qgpu_mlp_gradient:
+ * the gradient and loss are bitwise the same. */
+int qgpu_mlp_gradient_noalloc(qgpu_mlp* m, const float* X, const float* Y, int n, float* grad, double* loss);
 /* Predictions (n x 3, de-standardised) for features X (n x 4). */
 int qgpu_mlp_predict(qgpu_mlp* m, const float* X, int n, float* Y);
 const char* qgpu_mlp_last_error(void);

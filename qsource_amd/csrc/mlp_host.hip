@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -39,6 +40,7 @@ static T* dalloc(size_t n) {
 }
 
 Mlp::Mlp(int seed) {
+  MCHECK(hipGetDevice(&device));
   MCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   D.P = dalloc<float>(kMlpNParam);
   D.adam_m = dalloc<float>(kMlpNParam);
@@ -212,6 +214,97 @@ void Mlp::predict_dev(const float* X, int n, float* Y, hipStream_t s) {
   MCHECK(hipGetLastError());
 }
 
+// ---------------------------------------------------------------------------
+// Gradient pass without device allocation.  The training activations and the
+// staging copies of X and Y live in `scratch`, a static device array of
+// every GPU the library runs on (held until the process ends), for up to
+// kNoAllocMaxSamples samples.  Host code never uses the array's name as an
+// address: the address of each device's copy comes from hipGetSymbolAddress
+// (scratch_ptr) and is what the kernels receive.  Concurrent calls on one GPU
+// share the array and race; every buffer a pass reads is written by the
+// same pass first (padded samples included), so nothing needs clearing.
+constexpr int kNoAllocMaxSamples = 2048;
+constexpr size_t kAlign = 256;
+constexpr size_t align_up(size_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
+// bytes for npad samples: X, Y staging; xT, z1T, z2T, dyT, dz2T, dz1T, pred,
+// loss_part (FP32); h1T, dz2Tb (BF16)
+constexpr size_t scratch_bytes(size_t npad) {
+  return align_up(npad * kMlpIn * 4) + align_up(npad * kMlpOut * 4) + align_up(npad * kMlpIn * 4) +
+         4 * align_up(npad * kMlpH * 4) + align_up(npad * kMlpOut * 4) + align_up(npad * kMlpOut * 4) +
+         align_up(npad / 16 * 4) + 2 * align_up(npad * kMlpH * 2);
+}
+constexpr size_t kScratchBytes = scratch_bytes(kNoAllocMaxSamples);
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Device address of this GPU's copy of `scratch`, looked up on the first
+// call on each device (with that device current).
+static unsigned char* scratch_ptr(int device) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static unsigned char* ptr[kMaxDevices] = {};
+  if (device < 0 || device >= kMaxDevices) throw std::runtime_error("device index out of range");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw std::runtime_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    ptr[device] = static_cast<unsigned char*>(p);
+  }
+  return ptr[device];
+}
+
+// Same kernels, launch shapes and sample order as Mlp::pass over X, Y
+// (host arrays, n samples); gradient to G (host), returns the loss.
+static double pass_no_alloc(Mlp& m, const float* X, const float* Y, int n, float* G) {
+  if (n <= 0) throw std::runtime_error("no samples");
+  if (n > kNoAllocMaxSamples)
+    throw std::runtime_error("qgpu_mlp_gradient_noalloc: at most " + std::to_string(kNoAllocMaxSamples) +
+                             " samples per call");
+  int dev = -1;
+  MCHECK(hipGetDevice(&dev));
+  if (dev != m.device) throw std::runtime_error("the current device is not the model's device");
+  unsigned char* base = scratch_ptr(dev);
+  const size_t npad = (size_t)(n + 63) / 64 * 64;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    unsigned char* p = base + off;
+    off += align_up(bytes);
+    return p;
+  };
+  float* dx = reinterpret_cast<float*>(take(npad * kMlpIn * 4));
+  float* dy = reinterpret_cast<float*>(take(npad * kMlpOut * 4));
+  MlpDev D = m.D;
+  D.xT = reinterpret_cast<float*>(take(npad * kMlpIn * 4));
+  D.z1T = reinterpret_cast<float*>(take(npad * kMlpH * 4));
+  D.z2T = reinterpret_cast<float*>(take(npad * kMlpH * 4));
+  D.dz2T = reinterpret_cast<float*>(take(npad * kMlpH * 4));
+  D.dz1T = reinterpret_cast<float*>(take(npad * kMlpH * 4));
+  D.dyT = reinterpret_cast<float*>(take(npad * kMlpOut * 4));
+  float* pred = reinterpret_cast<float*>(take(npad * kMlpOut * 4));
+  float* loss_part = reinterpret_cast<float*>(take(npad / 16 * 4));
+  D.h1T = reinterpret_cast<__bf16*>(take(npad * kMlpH * 2));
+  D.dz2Tb = reinterpret_cast<__bf16*>(take(npad * kMlpH * 2));
+  if (off > kScratchBytes) throw std::runtime_error("scratch layout exceeds the array");
+  MCHECK(hipMemcpyAsync(dx, X, (size_t)n * kMlpIn * sizeof(float), hipMemcpyHostToDevice, m.stream));
+  MCHECK(hipMemcpyAsync(dy, Y, (size_t)n * kMlpOut * sizeof(float), hipMemcpyHostToDevice, m.stream));
+  hipLaunchKernelGGL(k_mlp_fwd, dim3(npad / 64), dim3(256), 0, m.stream, D, dx, (const int*)nullptr, n, (int)npad,
+                     pred, 1);
+  hipLaunchKernelGGL(k_mlp_bwd, dim3(npad / 64), dim3(256), 0, m.stream, D, dy, (const int*)nullptr, n, (int)npad,
+                     pred, loss_part);
+  const int nother = kMlpH * kMlpIn + 2 * kMlpH + kMlpOut * kMlpH + kMlpOut;
+  hipLaunchKernelGGL(k_mlp_wgrad, dim3(64 + (nother + 255) / 256), dim3(256), 0, m.stream, D, (int)npad, m.grad);
+  MCHECK(hipGetLastError());
+  float lp[kNoAllocMaxSamples / 16];
+  MCHECK(hipMemcpyAsync(lp, loss_part, npad / 16 * sizeof(float), hipMemcpyDeviceToHost, m.stream));
+  MCHECK(hipMemcpyAsync(G, m.grad, kMlpNParam * sizeof(float), hipMemcpyDeviceToHost, m.stream));
+  MCHECK(hipStreamSynchronize(m.stream));
+  double s = 0;
+  for (size_t k = 0; k < npad / 16; ++k) s += lp[k];
+  return s / (3.0 * n);
+}
+
 }  // namespace qgpu
 
 // ---------------------------------------------------------------------------
@@ -287,6 +380,13 @@ int qgpu_mlp_gradient(qgpu_mlp* h, const float* X, const float* Y, int n, float*
     MCHECK(hipMemcpy(grad, h->m->grad, qgpu::kMlpNParam * sizeof(float), hipMemcpyDeviceToHost));
     (void)hipFree(dx);
     (void)hipFree(dy);
+  });
+}
+
+int qgpu_mlp_gradient_noalloc(qgpu_mlp* h, const float* X, const float* Y, int n, float* grad, double* loss) {
+  return mguard([&] {
+    if (!h || !X || !Y || !grad || !loss) throw std::runtime_error("null argument");
+    *loss = qgpu::pass_no_alloc(*h->m, X, Y, n, grad);
   });
 }
 

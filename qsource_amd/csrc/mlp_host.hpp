@@ -26,6 +26,7 @@ class Mlp {
 
   MlpDev D{};
   hipStream_t stream = nullptr;
+  int device = 0;                   // device the model's buffers and stream live on
   std::vector<float> norm_h;        // xmean[4] xstd[4] ymean[3] ystd[3]
   float* norm = nullptr;            // device: xmean, 1/xstd, ymean, 1/ystd
   float* ystd_d = nullptr;          // device: ystd (for de-normalising)

@@ -40,7 +40,7 @@ SYMBOLS = [
     "qgpu_trj_create", "qgpu_trj_natoms", "qgpu_trj_write", "qgpu_trj_close", "qgpu_write_restart",
     "qgpu_autotune", "qgpu_set_launch_shape",
     "qgpu_instance_status", "qgpu_list_capacity", "qgpu_set_list_capacity", "qgpu_pot_energy_terms",
-    "qgpu_work_counts",
+    "qgpu_work_counts", "qgpu_mlp_gradient_noalloc",
 ]
 TERM_GROUPS = {"ww": 1, "pw": 2, "pp": 4, "qx": 8, "lrf": 16, "other": 32}
 TUNED_KERNELS = ["water_lists", "solute_lists", "water_forces", "solute_forces"]
