@@ -90,7 +90,7 @@ def build_c3(d, seed_base=3000, equil=True):
         t0 = time.time()
         s = gensys.solvated_system(radius=R, n_solute=ns, seed=seed_base + si, n_q=12, n_states=2)
         name = "s%d" % si
-        rst = equilibrate.equilibrate(s, d, name, lambdas=[0.5, 0.5], seed=11 + si) if equil else None
+        rst = equilibrate.equilibrate(s, d, name, lambdas=[0.5, 0.5], seed=11 + si, progress=log) if equil else None
         log("system %d: R=%.0f A, %d solute atoms, %d atoms, equilibrated in %.1f s" %
             (si, R, ns, s.natom, time.time() - t0))
         group = []

@@ -23,19 +23,19 @@ namespace qgpu {
 __global__ void k_cgp_centers(const DevSys*);
 __global__ void k_x_to_f32(const DevSys*);
 template <typename P, int MAPPED> __global__ void k_water_lists(const DevSys*, const int2*);
-template <typename P, int MAPPED, int PF, int W, int AA> __global__ void k_water_forces(const DevSys*, const int2*);
+template <typename P, int MAPPED, int PF, int W, int AA> __global__ void k_water_forces(const DevSys*, const int2*, int);
 __global__ void k_watpol_theta(const DevSys*);
 __global__ void k_watpol_force(const DevSys*);
+__global__ void k_watpol_gather(const DevSys*);
 __global__ void k_integrate(const DevSys*);
 __global__ void k_temperature(const DevSys*, int);
 __global__ void k_energy_reduce(const DevSys*, int);
 template <typename P, int MAPPED> __global__ void k_solute_lists(const DevSys*, const int2*);
-template <typename P, int MAPPED, int PF> __global__ void k_solute_forces(const DevSys*, const int2*);
+template <typename P, int MAPPED, int PF> __global__ void k_solute_forces(const DevSys*, const int2*, int);
 template <typename P, int NS> __global__ void k_q_forces(const DevSys*, int);
 __global__ void k_q_reduce(const DevSys*, int);
 __global__ void k_q_lists(const DevSys*);
 __global__ void k_monitor(const DevSys*);
-__global__ void k_term_sum(const DevSys*);
 __global__ void k_freeze_x(const DevSys*);
 __global__ void k_terms(const DevSys*);
 __global__ void k_term_gather(const DevSys*);
@@ -304,10 +304,7 @@ void Batch::set_energy_interval(int iene, int max_frames) {
   iene_ = iene;
   ebase_ = istep_;
   sync_sys();
-  if (graph_exec_) {
-    HIPCHECK(hipGraphExecDestroy(graph_exec_));
-    graph_exec_ = nullptr;
-  }
+  drop_graphs();
 }
 
 int Batch::energy_series(int k, double* out, int cap, bool fp64) {
@@ -341,8 +338,7 @@ int Batch::offdiag_series(int k, double* out, int cap) {
 Batch::~Batch() { release(); }
 
 void Batch::release() {
-  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
-  graph_exec_ = nullptr;
+  drop_graphs();
   for (auto& m : maps_)
     if (m.second.first) (void)hipFree(m.second.first);
   if (far_x_) (void)hipFree(far_x_);
@@ -359,6 +355,7 @@ void Batch::release() {
   if (ev_fork_) (void)hipEventDestroy(ev_fork_);
   if (ev_join_) (void)hipEventDestroy(ev_join_);
   stream2_ = nullptr; ev_fork_ = ev_join_ = nullptr;
+
   if (stream_) {
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -388,6 +385,7 @@ void Batch::upload(int k) {
   const int N = s.natom;
   D.x = I->upload(s.x);
   D.xf = I->alloc<float>(3 * N);
+  D.xw = (precision_ == 1 && s.nwat > 0) ? I->alloc<float>((size_t)16 * s.nwat) : nullptr;
   D.v = I->upload(s.v);
   D.xx = I->alloc<double>(3 * N);
   D.d = I->alloc<double>(3 * N);
@@ -544,6 +542,7 @@ void Batch::upload(int k) {
   D.shell = I->upload(sh);
   D.theta = I->alloc<double>(s.nwat);
   D.shell_of = I->alloc<int>(s.nwat);
+  D.wpol_g = I->alloc<double>((size_t)9 * std::max(1, s.nwat));
   D.fk_fix = s.inp.fk_fix; D.fk_pshell = s.inp.fk_pshell; D.freeze = s.inp.freeze;
   for (int a = 0; a < 3; ++a)
     for (int k2 = 0; k2 < 2; ++k2) { D.alj_w[a][k2] = s.alj_solv[a][k2]; D.blj_w[a][k2] = s.blj_solv[a][k2]; }
@@ -879,7 +878,6 @@ void Batch::upload(int k) {
       D.mon_lj = I->upload(lj);
       D.mon_out = I->alloc<double>((size_t)std::max(1, D.mon_n) * (kMon + 2 * std::max(1, s.nstates)));
       max_mon_ = std::max(max_mon_, D.mon_n);
-      D.slot_sum = I->alloc<double>((size_t)std::max<size_t>(1, sa.size()) * 3);
     }
     max_terms_ = std::max(max_terms_, D.nbnd + D.nang + D.ntor + D.nimp + D.nqq + D.nxt);
     max_slot_atoms_ = std::max(max_slot_atoms_, std::max(D.nslot_atoms, D.nqbond));
@@ -1046,6 +1044,7 @@ void Batch::launch_tuned(int kernel, const LaunchShape& sh) {
   const bool f64 = precision_ == 0;
   const bool mp = sh.map != 0;
   const bool pf = sh.pf != 0;
+  const int en = launch_energies_ ? 1 : 0;
 #define QGPU_LAUNCH(K)                                                                         \
   do {                                                                                         \
     if (f64 && mp) hipLaunchKernelGGL((K<double, 1>), grid, dim3(blk), 0, ls_, d_sys_, map); \
@@ -1055,26 +1054,26 @@ void Batch::launch_tuned(int kernel, const LaunchShape& sh) {
   } while (0)
 #define QGPU_LAUNCH_PF(K)                                                                              \
   do {                                                                                                 \
-    if (f64 && mp && pf) hipLaunchKernelGGL((K<double, 1, 1>), grid, dim3(blk), 0, ls_, d_sys_, map); \
-    else if (f64 && mp) hipLaunchKernelGGL((K<double, 1, 0>), grid, dim3(blk), 0, ls_, d_sys_, map);  \
-    else if (f64 && pf) hipLaunchKernelGGL((K<double, 0, 1>), grid, dim3(blk), 0, ls_, d_sys_, map);  \
-    else if (f64) hipLaunchKernelGGL((K<double, 0, 0>), grid, dim3(blk), 0, ls_, d_sys_, map);        \
-    else if (mp && pf) hipLaunchKernelGGL((K<float, 1, 1>), grid, dim3(blk), 0, ls_, d_sys_, map);    \
-    else if (mp) hipLaunchKernelGGL((K<float, 1, 0>), grid, dim3(blk), 0, ls_, d_sys_, map);          \
-    else if (pf) hipLaunchKernelGGL((K<float, 0, 1>), grid, dim3(blk), 0, ls_, d_sys_, map);          \
-    else hipLaunchKernelGGL((K<float, 0, 0>), grid, dim3(blk), 0, ls_, d_sys_, map);                  \
+    if (f64 && mp && pf) hipLaunchKernelGGL((K<double, 1, 1>), grid, dim3(blk), 0, ls_, d_sys_, map, en); \
+    else if (f64 && mp) hipLaunchKernelGGL((K<double, 1, 0>), grid, dim3(blk), 0, ls_, d_sys_, map, en);  \
+    else if (f64 && pf) hipLaunchKernelGGL((K<double, 0, 1>), grid, dim3(blk), 0, ls_, d_sys_, map, en);  \
+    else if (f64) hipLaunchKernelGGL((K<double, 0, 0>), grid, dim3(blk), 0, ls_, d_sys_, map, en);        \
+    else if (mp && pf) hipLaunchKernelGGL((K<float, 1, 1>), grid, dim3(blk), 0, ls_, d_sys_, map, en);    \
+    else if (mp) hipLaunchKernelGGL((K<float, 1, 0>), grid, dim3(blk), 0, ls_, d_sys_, map, en);          \
+    else if (pf) hipLaunchKernelGGL((K<float, 0, 1>), grid, dim3(blk), 0, ls_, d_sys_, map, en);          \
+    else hipLaunchKernelGGL((K<float, 0, 0>), grid, dim3(blk), 0, ls_, d_sys_, map, en);                  \
   } while (0)
   // water forces: the FP32 build fitted to 6 or 4 wavefronts per SIMD
 #define QGPU_WF(PP, M, F)                                                                                 \
   do {                                                                                                    \
     if (sh.occ && sizeof(PP) == 4 && allatom_)                                                            \
-      hipLaunchKernelGGL((k_water_forces<PP, M, F, 4, 1>), grid, dim3(blk), 0, ls_, d_sys_, map);     \
+      hipLaunchKernelGGL((k_water_forces<PP, M, F, 4, 1>), grid, dim3(blk), 0, ls_, d_sys_, map, en);     \
     else if (sh.occ && sizeof(PP) == 4)                                                                   \
-      hipLaunchKernelGGL((k_water_forces<PP, M, F, 4, 0>), grid, dim3(blk), 0, ls_, d_sys_, map);     \
+      hipLaunchKernelGGL((k_water_forces<PP, M, F, 4, 0>), grid, dim3(blk), 0, ls_, d_sys_, map, en);     \
     else if (allatom_)                                                                                    \
-      hipLaunchKernelGGL((k_water_forces<PP, M, F, 6, 1>), grid, dim3(blk), 0, ls_, d_sys_, map);     \
+      hipLaunchKernelGGL((k_water_forces<PP, M, F, 6, 1>), grid, dim3(blk), 0, ls_, d_sys_, map, en);     \
     else                                                                                                  \
-      hipLaunchKernelGGL((k_water_forces<PP, M, F, 6, 0>), grid, dim3(blk), 0, ls_, d_sys_, map);     \
+      hipLaunchKernelGGL((k_water_forces<PP, M, F, 6, 0>), grid, dim3(blk), 0, ls_, d_sys_, map, en);     \
   } while (0)
   switch (kernel) {
     case TK_WATER_LISTS: QGPU_LAUNCH(k_water_lists); break;
@@ -1106,8 +1105,10 @@ void Batch::autotune(int reps, double* ms) {
   hipEvent_t a, b;
   HIPCHECK(hipEventCreate(&a));
   HIPCHECK(hipEventCreate(&b));
-  // lists need the group centres, forces need lists
+  // lists need the group centres, forces need lists; the force kernels are
+  // timed as most steps run them (without the energy partials)
   launch_lists();
+  launch_energies_ = false;
   for (int kernel = 0; kernel < TK_COUNT; ++kernel) {
     double best = 1e30, dflt = 0.0;
     LaunchShape pick = shape_[kernel];
@@ -1140,7 +1141,7 @@ void Batch::autotune(int reps, double* ms) {
   (void)hipEventDestroy(b);
   tuned_ = true;
   // captured graphs hold the old launch shapes
-  if (graph_exec_) { (void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
+  drop_graphs();
 }
 
 void Batch::set_shape(int k, int map, int block, int pf, int occ) {
@@ -1150,7 +1151,7 @@ void Batch::set_shape(int k, int map, int block, int pf, int occ) {
   shape_[k] = LaunchShape{map, block, (k == TK_WATER_FORCES || k == TK_SOLUTE_FORCES) ? pf : 0,
                           k == TK_WATER_FORCES ? occ : 0};
   tuned_ = true;
-  if (graph_exec_) { (void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
+  drop_graphs();
 }
 
 // Two branches of independent kernels: the water side stays on the engine's
@@ -1197,28 +1198,36 @@ bool Batch::record_possible(long s) const {
   return (s - ebase_) % iene_ == 0;
 }
 
-void Batch::launch_forces(bool step, long s) {
+void Batch::drop_graphs() {
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  if (graph_exec_final_) (void)hipGraphExecDestroy(graph_exec_final_);
+  graph_exec_ = graph_exec_final_ = nullptr;
+}
+
+void Batch::launch_forces(bool step, long s, bool energies) {
   const int ni = (int)h_sys_.size();
+  launch_energies_ = energies;
   // excluded_freeze: the coordinates the terms after fix_shell see
   if (freeze_)
     hipLaunchKernelGGL(k_freeze_x, dim3((3 * max_natom_ + 255) / 256, ni), dim3(256), 0, stream_, d_sys_);
-  // side branch: solute forces (assign the solute atoms' gradient), bonded
-  // and Q-Q terms (term slots), Q-atom forces (added to the Q atoms'
+  // side branch: the short bonded/Q-Q term and Q-atom kernels first, so
+  // that they overlap the water forces; then the solute forces (assign the
+  // solute atoms' gradient) and the Q-atom sums (added to the Q atoms'
   // gradient after the solute forces)
   fork();
-  if (max_ncgp_solute_ > 0) { pb(4); launch_tuned(TK_SOLUTE_FORCES, shape_[TK_SOLUTE_FORCES]); pe(4); }
   if (max_terms_ > 0)
     { pb(5); hipLaunchKernelGGL(k_terms, dim3((max_terms_ + 255) / 256, ni), dim3(256), 0, ls_, d_sys_); pe(5); }
-  if (max_slot_atoms_ > 0 && step)
-    { pb(7); hipLaunchKernelGGL(k_term_sum, dim3((max_slot_atoms_ + 255) / 256, ni), dim3(256), 0, ls_, d_sys_); pe(7); }
+  const dim3 gq((max_nqat_ * kQChunks * 64 + 255) / 256, ni), gr((max_nqat_ + 63) / 64, ni);
+  auto q_fp64 = [&](int shadow) {
+    if (max_nstates_ <= 2) hipLaunchKernelGGL((k_q_forces<double, 2>), gq, dim3(256), 0, ls_, d_sys_, shadow);
+    else if (max_nstates_ <= 4) hipLaunchKernelGGL((k_q_forces<double, 4>), gq, dim3(256), 0, ls_, d_sys_, shadow);
+    else hipLaunchKernelGGL((k_q_forces<double, kMaxQStates>), gq, dim3(256), 0, ls_, d_sys_, shadow);
+  };
+  // mixed precision with energy frames: the Q energies of each recorded
+  // frame are also evaluated with FP64 pair arithmetic (the kernels return
+  // at once on other steps)
+  const bool shadow = max_nqat_ > 0 && precision_ == 1 && record_possible(s);
   if (max_nqat_ > 0) {
-    dim3 gq((max_nqat_ * kQChunks * 64 + 255) / 256, ni);
-    auto q_fp64 = [&](int shadow) {
-      if (max_nstates_ <= 2) hipLaunchKernelGGL((k_q_forces<double, 2>), gq, dim3(256), 0, ls_, d_sys_, shadow);
-      else if (max_nstates_ <= 4) hipLaunchKernelGGL((k_q_forces<double, 4>), gq, dim3(256), 0, ls_, d_sys_, shadow);
-      else hipLaunchKernelGGL((k_q_forces<double, kMaxQStates>), gq, dim3(256), 0, ls_, d_sys_, shadow);
-    };
-    const dim3 gr((max_nqat_ + 63) / 64, ni);
     pb(6);
     if (precision_ == 0) {
       q_fp64(0);
@@ -1227,18 +1236,17 @@ void Batch::launch_forces(bool step, long s) {
       else if (max_nstates_ <= 4) hipLaunchKernelGGL((k_q_forces<float, 4>), gq, dim3(256), 0, ls_, d_sys_, 0);
       else hipLaunchKernelGGL((k_q_forces<float, kMaxQStates>), gq, dim3(256), 0, ls_, d_sys_, 0);
     }
-    hipLaunchKernelGGL(k_q_reduce, gr, dim3(64), 0, ls_, d_sys_, 0);
     pe(6);
-    // mixed precision with energy frames: the Q energies of each recorded
-    // frame are also evaluated with FP64 pair arithmetic (the kernels return
-    // at once on other steps)
-    if (precision_ == 1 && record_possible(s)) {
-      q_fp64(1);
-      hipLaunchKernelGGL(k_q_reduce, gr, dim3(64), 0, ls_, d_sys_, 1);
-    }
+    if (shadow) q_fp64(1);
+  }
+  if (max_ncgp_solute_ > 0) { pb(4); launch_tuned(TK_SOLUTE_FORCES, shape_[TK_SOLUTE_FORCES]); pe(4); }
+  if (max_nqat_ > 0) {
+    hipLaunchKernelGGL(k_q_reduce, gr, dim3(64), 0, ls_, d_sys_, 0);
+    if (shadow) hipLaunchKernelGGL(k_q_reduce, gr, dim3(64), 0, ls_, d_sys_, 1);
   }
   // main branch: water forces (assign the waters' gradient), then the
-  // polarisation restraint adds to it
+  // polarisation restraint (its own per-water gradient buffer, which the
+  // integrator adds)
   const hipStream_t side = ls_;
   ls_ = stream_;
   if (max_nwat_ > 0) {
@@ -1246,14 +1254,15 @@ void Batch::launch_forces(bool step, long s) {
     dim3 g2((max_nwat_ + 255) / 256, ni);
     { pb(8); hipLaunchKernelGGL(k_watpol_theta, g2, dim3(256), 0, stream_, d_sys_); pe(8); }
     dim3 g3(kNumShellMax, ni);
-    { pb(9); hipLaunchKernelGGL(k_watpol_force, g3, dim3(1024), 0, stream_, d_sys_); pe(9); }
+    { pb(9); hipLaunchKernelGGL(k_watpol_force, g3, dim3(kWpolBlock), 0, stream_, d_sys_); pe(9); }
   }
   ls_ = side;
   join();
-  // term slots may hold water atoms (unconstrained water bonds): gathered
-  // after both branches
-  // within a step the integrator adds the term slots itself (one launch
-  // less on the step's critical path)
+  // outside a step the polarisation gradient and then the term slots (which
+  // may hold water atoms: unconstrained water bonds) are added here, in the
+  // order the integrator adds them within a step
+  if (max_nwat_ > 0 && !step)
+    hipLaunchKernelGGL(k_watpol_gather, dim3((max_nwat_ + 255) / 256, ni), dim3(256), 0, stream_, d_sys_);
   if (max_slot_atoms_ > 0 && !step)
     { pb(7); hipLaunchKernelGGL(k_term_gather, dim3((max_slot_atoms_ + 255) / 256, ni), dim3(256), 0, stream_, d_sys_); pe(7); }
   if (far_model_) launch_far_model();
@@ -1313,10 +1322,7 @@ void Batch::set_far_model(Mlp* m) {
     h_sys_[k].use_lrf = m ? false : setups_[k]->inp.use_lrf;
   }
   sync_sys();
-  if (graph_exec_) {
-    HIPCHECK(hipGraphExecDestroy(graph_exec_));
-    graph_exec_ = nullptr;
-  }
+  drop_graphs();
 }
 
 void Batch::launch_integrate(long s) {
@@ -1402,9 +1408,9 @@ void Batch::forces() {
   check_failures();
 }
 
-void Batch::one_step() {
+void Batch::one_step(bool energies) {
   if (istep_ % nbcycle_ == 0) launch_lists();
-  launch_forces(true, istep_);
+  launch_forces(true, istep_, energies);
   launch_integrate(istep_);
   ++istep_;
 }
@@ -1412,20 +1418,24 @@ void Batch::one_step() {
 // One list-update cycle (NBcycle steps) captured as a hipGraph: the kernels
 // read the step number from the device counter, so the same graph serves
 // every cycle.  Rebuilt when the energy interval changes.
-void Batch::launch_cycle_graph() {
-  if (!graph_exec_) {
+void Batch::launch_cycle_graph(bool final_energies) {
+  // both variants are captured together, so a run never instantiates one
+  // between timed calls
+  for (int fin = 0; fin < 2; ++fin) {
+    hipGraphExec_t& ge = fin ? graph_exec_final_ : graph_exec_;
+    if (ge) continue;
     hipGraph_t g = nullptr;
     HIPCHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < nbcycle_; ++k) {
       if (k == 0) launch_lists();
-      launch_forces(true, istep_ + k);
+      launch_forces(true, istep_ + k, fin && k == nbcycle_ - 1);
       launch_integrate(istep_ + k);
     }
     HIPCHECK(hipStreamEndCapture(stream_, &g));
-    HIPCHECK(hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0));
+    HIPCHECK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
     HIPCHECK(hipGraphDestroy(g));
   }
-  HIPCHECK(hipGraphLaunch(graph_exec_, stream_));
+  HIPCHECK(hipGraphLaunch(final_energies ? graph_exec_final_ : graph_exec_, stream_));
   istep_ += nbcycle_;
 }
 
@@ -1472,18 +1482,20 @@ void Batch::profile(int nsteps, double* ms, int cap) {
 
 // Launches nsteps steps: single steps up to a list step, whole NBcycles as
 // the captured graph, single steps for the rest.
-void Batch::launch_steps(int nsteps) {
+void Batch::launch_steps(int nsteps, bool final_energies) {
+  // energy partials only in the last step, when final_energies
+  auto last = [&](int n) { return final_energies && n == nsteps - 1; };
   int n = 0;
   while (n < nsteps && (!use_graph_ || istep_ % nbcycle_ != 0 || nsteps - n < nbcycle_)) {
-    one_step();
+    one_step(last(n));
     ++n;
   }
   while (use_graph_ && nsteps - n >= nbcycle_) {
-    launch_cycle_graph();
+    launch_cycle_graph(final_energies && nsteps - n == nbcycle_);
     n += nbcycle_;
   }
   while (n < nsteps) {
-    one_step();
+    one_step(last(n));
     ++n;
   }
 }
@@ -1560,7 +1572,8 @@ void Batch::step(int nsteps) {
   constexpr int kCheckSteps = 1000;
   const int target = istep_ + nsteps;
   while (istep_ < target) {
-    launch_steps(std::min(target - istep_, kCheckSteps));
+    const int m = std::min(target - istep_, kCheckSteps);
+    launch_steps(m, istep_ + m == target);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(stream_));
     check_failures();

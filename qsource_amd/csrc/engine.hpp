@@ -71,6 +71,7 @@ constexpr int kQwRec = 8;      // Q-water record per Q atom (qw_rec_d / qw_rec_f
 constexpr int kEWater = 5;     // energy partials per water
 constexpr int kMon = 3;        // Vwsum, Vwel, Vwlj before the per-state values
 constexpr int kIntegrateBlock = 64;  // threads per k_integrate workgroup
+constexpr int kWpolBlock = 256;      // threads per k_watpol_force workgroup (one per shell)
 // lanes per water in k_water_lists: FP64 build, FP32 (mixed) build
 constexpr int kWLLanes64 = 8, kWLLanes32 = 16;
 
@@ -87,6 +88,10 @@ struct DevSys {
   int term_mask;                 // TM_* terms a force evaluation includes (all but in term diagnostics)
   double *x, *v, *xx, *d, *xtop;
   float* xf;                     // FP32 copy of x for the mixed-precision pair kernels
+  // mixed precision: FP32 coordinates of each water (O, H1, H2) as one
+  // 64-byte record of 16 floats (9 used), so that a partner water is read
+  // with three 16-byte loads of one cache line
+  float* xw;
   // monitored group pairs (nonbond_monitor): atoms of group i and j of pair
   // p at mon_at[mon_fi[p] ..) and [mon_fj[p] ..), counts mon_ni/mon_nj, LJ
   // codes [(s*ni + i)*nj + j] from mon_lj[mon_lo[p]]; results kMon + 2 nstates
@@ -97,7 +102,6 @@ struct DevSys {
   const signed char* mon_lj;
   double* mon_out;
   const int* slot_of_atom;       // N: the atom's index in slot_atom, or -1
-  double* slot_sum;              // nslot_atoms x 3: each slot atom's summed slots (k_term_sum)
   double* xq;                    // coordinates of the terms after fix_shell: x, or with
                                  // excluded_freeze a copy holding the excluded atoms at xtop
   const double *crg, *winv, *mass;
@@ -158,6 +162,8 @@ struct DevSys {
   DevShell* shell;
   double* theta;                 // per water
   int* shell_of;                 // per water, -1 when outside the shells
+  double* wpol_g;                // per water 9: this step's polarisation-restraint gradient
+                                 // (valid for shell members, added to d by the integrator)
   // restraints
   double fk_fix, fk_pshell;
   // Q atoms
@@ -383,11 +389,12 @@ class Batch {
   void release();
   void fork();
   void join();
-  void one_step();
-  void launch_steps(int nsteps);
+  void one_step(bool energies = true);
+  void launch_steps(int nsteps, bool final_energies);
+  void drop_graphs();
   bool check_failures();
   void grow_lists(int k, int need_ww, int need_wp, int need_pw);
-  void launch_cycle_graph();
+  void launch_cycle_graph(bool final_energies);
   void sync_sys();
   void refresh_xf();
   int nbcycle_ = 10;
@@ -397,14 +404,19 @@ class Batch {
   int iene_ = 0;
   long ebase_ = 0;         // step of the last set_energy_interval
   bool use_graph_ = true;
-  hipGraphExec_t graph_exec_ = nullptr;
+  // captured list cycle; the _final one forms the energy partials in its
+  // last step (the energies a caller reads after qgpu_step)
+  hipGraphExec_t graph_exec_ = nullptr, graph_exec_final_ = nullptr;
+  // whether the force kernels of the launches being issued form the
+  // nonbonded/LRF/restraint energy partials (dynamics needs only gradients)
+  bool launch_energies_ = true;
   int* d_step_ = nullptr;              // per-instance step counters
   int* d_status_ = nullptr;            // per-instance failure records (kStatusInts each)
   unsigned long long* d_fail_ = nullptr;   // batch-wide failure key
   std::string dead_;                   // set when a failure stopped the batch for good
   void upload(int k);
   void launch_lists();
-  void launch_forces(bool step, long s);
+  void launch_forces(bool step, long s, bool energies = true);
  public:
   int monitor(int k, double* out, int cap);   // monitored pair energies of instance k
   int max_mon_ = 0;

@@ -64,6 +64,23 @@ __device__ __forceinline__ bool block_work(const DevSys* sys, const int2* map, i
   return true;
 }
 
+// FP32 coordinates (O, H1, H2) of water w from its 64-byte record.
+typedef float f32v4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void water_xf(const DevSys& S, int w, float* o) {
+  const gconst<f32v4> r = gptr(reinterpret_cast<const f32v4*>(S.xw)) + 4 * (size_t)w;
+  const f32v4 a = r[0], b = r[1], c = r[2];
+  o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+  o[4] = b.x; o[5] = b.y; o[6] = b.z; o[7] = b.w;
+  o[8] = c.x;
+}
+// The record of water w from its FP64 coordinates x (O, H1, H2).
+__device__ __forceinline__ void water_xf_store(const DevSys& S, int w, const double* x) {
+  f32v4* r = reinterpret_cast<f32v4*>(S.xw) + 4 * (size_t)w;
+  r[0] = f32v4{(float)x[0], (float)x[1], (float)x[2], (float)x[3]};
+  r[1] = f32v4{(float)x[4], (float)x[5], (float)x[6], (float)x[7]};
+  r[2] = f32v4{(float)x[8], 0.0f, 0.0f, 0.0f};
+}
+
 template <typename P>
 __device__ __forceinline__ const P* pair_x(const DevSys& S);
 template <>
@@ -382,14 +399,17 @@ __device__ __forceinline__ void lrf_gather_group(const DevSys& S, int g, double 
 // In mixed precision the partner's offsets from the centre come from the
 // FP32 coordinate copy and the FP32 centre (cf).
 template <typename P>
-__device__ __forceinline__ void lrf_gather_water(const DevSys& S, int jo, double c0, double c1,
+__device__ __forceinline__ void lrf_gather_water(const DevSys& S, int jo, int jw, double c0, double c1,
                                                  double c2, const float* cf, LrfAcc& acc) {
   if constexpr (sizeof(P) == 8) {
+    (void)jw;
     const gconst<double> xj = gptr(S.x) + 3 * jo;
     for (int k = 0; k < 3; ++k)
       acc.add(S.chg_solv[k], xj[3 * k] - c0, xj[3 * k + 1] - c1, xj[3 * k + 2] - c2);
   } else {
-    const gconst<float> xj = gptr(S.xf) + 3 * jo;
+    (void)jo;
+    float xj[9];
+    water_xf(S, jw, xj);
     LrfRaw<float> part;
     part.zero();
     for (int k = 0; k < 3; ++k)

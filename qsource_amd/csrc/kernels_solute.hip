@@ -125,7 +125,7 @@ __global__ __launch_bounds__(256) void k_solute_lists(const DevSys* __restrict__
     }
     n_pw += __popcll(m);
     if (far && S.use_lrf) {
-      if (S.std_water) lrf_gather_water<P>(S, jo, c0, c1, c2, cf, acc);
+      if (S.std_water) lrf_gather_water<P>(S, jo, jw, c0, c1, c2, cf, acc);
       else lrf_gather_group<P>(S, S.iwhich_cgp[jo], c0, c1, c2, acc);
     }
   }
@@ -156,9 +156,11 @@ template __global__ void k_solute_lists<float, 1>(const DevSys*, const int2*);
 // counted by the atom that is i of the reference pair: the owner group's
 // atom, or the lower index within one group.
 // FP32 pair arithmetic fits 6 wavefronts per SIMD without spilling
+// en as in k_water_forces (energy partials e_solute only when set)
 template <typename P, int MAPPED, int PF>
 __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(const DevSys* __restrict__ sys,
-                                                                               const int2* __restrict__ map) {
+                                                                               const int2* __restrict__ map,
+                                                                               int en) {
   const DevSys* Sp;
   int base;
   if (!block_work<MAPPED>(sys, map, blockDim.x >> 3, Sp, base)) return;
@@ -231,8 +233,10 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(c
           g0 = g0 - (double)(dv * dx1);
           g1 = g1 - (double)(dv * dx2);
           g2 = g2 - (double)(dv * dx3);
-          e_ppel = e_ppel + (double)Vel;
-          e_ppvdw = e_ppvdw + (double)Va - (double)Vb;
+          if (en) {
+            e_ppel = e_ppel + (double)Vel;
+            e_ppvdw = e_ppvdw + (double)Va - (double)Vb;
+          }
         } else {
           g0 = g0 + (double)(dv * dx1);
           g1 = g1 + (double)(dv * dx2);
@@ -276,39 +280,50 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(c
         g0 = g0 - (double)(dv * dx1);
         g1 = g1 - (double)(dv * dx2);
         g2 = g2 - (double)(dv * dx3);
-        e_pwel = e_pwel + (double)Vel;
-        e_pwvdw = e_pwvdw + (double)Va - (double)Vb;
+        if (en) {
+          e_pwel = e_pwel + (double)Vel;
+          e_pwvdw = e_pwvdw + (double)Va - (double)Vb;
+        }
+      }
+    };
+    // coordinates of water jw: FP64, or the FP32 water record
+    auto load_water = [&](const int jw, P* o) {
+      if constexpr (sizeof(P) == 4) {
+        water_xf(S, jw, o);
+      } else {
+        const int jo = S.nat_solute + 3 * jw;
+        for (int u = 0; u < 9; ++u) o[u] = X[3 * jo + u];
       }
     };
     if constexpr (PF != 0) {
       // software pipeline: the next water's coordinates and the index of
       // the one after are in flight during this water's pairs
       int k = sub;
-      int jon = 0, jwn2 = 0;
+      int jwn = 0, jwn2 = 0;
       P xn[9];
       if (k < npw) {
-        jon = S.nat_solute + 3 * wrow[k];
-        for (int u = 0; u < 9; ++u) xn[u] = X[3 * jon + u];
+        jwn = wrow[k];
+        load_water(jwn, xn);
       }
       if (k + 8 < npw) jwn2 = wrow[k + 8];
       while (k < npw) {
         P xw[9];
         for (int u = 0; u < 9; ++u) xw[u] = xn[u];
-        const int jo = jon;
+        const int jo = S.nat_solute + 3 * jwn;
         k += 8;
         if (k < npw) {
-          jon = S.nat_solute + 3 * jwn2;
-          for (int u = 0; u < 9; ++u) xn[u] = X[3 * jon + u];
+          jwn = jwn2;
+          load_water(jwn, xn);
           if (k + 8 < npw) jwn2 = wrow[k + 8];
         }
         pw_water(jo, xw);
       }
     } else {
       for (int k = sub; k < npw; k += 8) {
-        const int jo = S.nat_solute + 3 * wrow[k];
+        const int jw = wrow[k];
         P xw[9];
-        for (int u = 0; u < 9; ++u) xw[u] = X[3 * jo + u];
-        pw_water(jo, xw);
+        load_water(jw, xw);
+        pw_water(S.nat_solute + 3 * jw, xw);
       }
     }
     // Q-solute pairs seen from the solute atom (the Q atom is i of the pair)
@@ -374,24 +389,28 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(c
     g2 = g2 + fk * dr2;
   }
   g0 = seg8_sum(g0); g1 = seg8_sum(g1); g2 = seg8_sum(g2);
-  e_ppel = seg8_sum(e_ppel); e_ppvdw = seg8_sum(e_ppvdw);
-  e_pwel = seg8_sum(e_pwel); e_pwvdw = seg8_sum(e_pwvdw);
-  e_lrf = seg8_sum(e_lrf); e_fix = seg8_sum(e_fix); e_shell = seg8_sum(e_shell);
+  if (en) {
+    e_ppel = seg8_sum(e_ppel); e_ppvdw = seg8_sum(e_ppvdw);
+    e_pwel = seg8_sum(e_pwel); e_pwvdw = seg8_sum(e_pwvdw);
+    e_lrf = seg8_sum(e_lrf); e_fix = seg8_sum(e_fix); e_shell = seg8_sum(e_shell);
+  }
   if (active && sub == 0) {
     S.d[3 * i] = g0; S.d[3 * i + 1] = g1; S.d[3 * i + 2] = g2;
+  }
+  if (active && sub == 0 && en) {
     double* es = S.e_solute + (size_t)i * 8;
     es[0] = e_ppel; es[1] = e_ppvdw; es[2] = e_pwel; es[3] = e_pwvdw;
     es[4] = e_lrf; es[5] = e_fix; es[6] = e_shell; es[7] = 0.0;
   }
 }
-template __global__ void k_solute_forces<double, 0, 0>(const DevSys*, const int2*);
-template __global__ void k_solute_forces<float, 0, 0>(const DevSys*, const int2*);
-template __global__ void k_solute_forces<double, 1, 0>(const DevSys*, const int2*);
-template __global__ void k_solute_forces<float, 1, 0>(const DevSys*, const int2*);
-template __global__ void k_solute_forces<double, 0, 1>(const DevSys*, const int2*);
-template __global__ void k_solute_forces<float, 0, 1>(const DevSys*, const int2*);
-template __global__ void k_solute_forces<double, 1, 1>(const DevSys*, const int2*);
-template __global__ void k_solute_forces<float, 1, 1>(const DevSys*, const int2*);
+template __global__ void k_solute_forces<double, 0, 0>(const DevSys*, const int2*, int);
+template __global__ void k_solute_forces<float, 0, 0>(const DevSys*, const int2*, int);
+template __global__ void k_solute_forces<double, 1, 0>(const DevSys*, const int2*, int);
+template __global__ void k_solute_forces<float, 1, 0>(const DevSys*, const int2*, int);
+template __global__ void k_solute_forces<double, 0, 1>(const DevSys*, const int2*, int);
+template __global__ void k_solute_forces<float, 0, 1>(const DevSys*, const int2*, int);
+template __global__ void k_solute_forces<double, 1, 1>(const DevSys*, const int2*, int);
+template __global__ void k_solute_forces<float, 1, 1>(const DevSys*, const int2*, int);
 
 // ---------------------------------------------------------------------------
 // Q atoms (nonbond_qp :13788-13921, nonbond_qw_spc :14411-14570 seen from
@@ -1281,28 +1300,6 @@ __global__ __launch_bounds__(256) void k_terms(const DevSys* __restrict__ sys) {
   }
   n -= S.nqq;
   if (n < S.nxt) xterm(S, n);
-}
-
-// Within a step: each slot atom's term slots summed in slot order into
-// slot_sum (the integrator adds it to d), and the Q bonds' Morse factors of
-// this evaluation kept for the next one.  Runs right after k_terms, beside
-// the water forces.
-__global__ __launch_bounds__(256) void k_term_sum(const DevSys* __restrict__ sys) {
-  const DevSys& S = sys[blockIdx.y];
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < S.nqbond && !halted(S, PH_TERMS))
-    for (int c = 0; c < kMorseRec; ++c) S.qb_prev[kMorseRec * t + c] = S.qb_cur[kMorseRec * t + c];
-  if (t >= S.nslot_atoms) return;
-  double g0 = 0, g1 = 0, g2 = 0;
-  const double* o = S.slot_g + 3 * (size_t)S.slot_first[t];
-  const int n = S.slot_first[t + 1] - S.slot_first[t];
-  for (int k = 0; k < n; ++k) {
-    g0 = g0 + o[3 * k]; g1 = g1 + o[3 * k + 1]; g2 = g2 + o[3 * k + 2];
-  }
-  if (!(S.term_mask & TM_OTHER)) g0 = g1 = g2 = 0.0;
-  S.slot_sum[3 * t] = g0;
-  S.slot_sum[3 * t + 1] = g1;
-  S.slot_sum[3 * t + 2] = g2;
 }
 
 // Adds each atom's term slots to its gradient, in slot order, and keeps the

@@ -38,6 +38,8 @@ __global__ void k_x_to_f32(const DevSys* __restrict__ sys) {
   const DevSys& S = sys[blockIdx.y];
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c < 3 * S.natom) S.xf[c] = (float)S.x[c];
+  // the water records (mixed precision)
+  if (S.xw && c < S.nwat) water_xf_store(S, c, S.x + 3 * (S.nat_solute + 3 * c));
 }
 
 // excluded_freeze (fix_shell src/md.f90:2831-2839): the terms evaluated
@@ -110,7 +112,7 @@ __global__ __launch_bounds__(256) void k_water_lists(const DevSys* __restrict__ 
     n_in += __popcll(m);
     if (far && S.use_lrf) {
       const int jo = S.nat_solute + 3 * jw;
-      if (S.std_water) lrf_gather_water<P>(S, jo, c0, c1, c2, cf, acc);
+      if (S.std_water) lrf_gather_water<P>(S, jo, jw, c0, c1, c2, cf, acc);
       else lrf_gather_group<P>(S, S.iwhich_cgp[jo], c0, c1, c2, acc);
     }
   }
@@ -177,9 +179,12 @@ template __global__ void k_water_lists<float, 1>(const DevSys*, const int2*);
 // AA: all-atom solvent LJ (nonbond_3atomsolvent :15238-15333 and
 // nonbond_qw_3atom :14749-14874): LJ on all nine atom pairs of two waters
 // and between each Q atom and every water atom.
+// en != 0: the launch also forms the energy partials (e_water); steps whose
+// energies are not read launch with en = 0 (the gradient is the same).
 template <typename P, int MAPPED, int PF, int W, int AA>
 __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(const DevSys* __restrict__ sys,
-                                                                              const int2* __restrict__ map) {
+                                                                              const int2* __restrict__ map,
+                                                                              int en) {
   const DevSys* Sp;
   int base;
   if (!block_work<MAPPED>(sys, map, blockDim.x >> 3, Sp, base)) return;
@@ -194,8 +199,17 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(co
   double g[9];
   for (int k = 0; k < 9; ++k) g[k] = 0.0;
   double e_el = 0.0, e_vdw = 0.0, e_lrf = 0.0, e_rad = 0.0;
+  // coordinates of water w: FP64, or the FP32 water record
+  auto load_water = [&](const int w, P* o) {
+    if constexpr (sizeof(P) == 4) {
+      water_xf(S, w, o);
+    } else {
+      const int wo = S.nat_solute + 3 * w;
+      for (int u = 0; u < 9; ++u) o[u] = X[3 * wo + u];
+    }
+  };
   P xi[9];
-  for (int k = 0; k < 9; ++k) xi[k] = X[3 * io + k];
+  load_water(active ? iw : 0, xi);
   // charge products formed in P from the P charges: with q_O = -2 q_H exact
   // in either precision, all nine products share one rounding, so the large
   // O-O / O-H / H-H sums still cancel as they do in FP64
@@ -241,7 +255,7 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(co
       g[3 * la + 1] = g[3 * la + 1] + (double)gl1;
       g[3 * la + 2] = g[3 * la + 2] + (double)gl2;
     }
-    if (owns_pair(iw, jw)) {
+    if (en && owns_pair(iw, jw)) {
       e_el = e_el + (double)el;
       e_vdw = e_vdw + (double)vdw;
     }
@@ -256,7 +270,7 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(co
     P xn[9];
     if (k < nn) {
       jwn = row[k];
-      for (int u = 0; u < 9; ++u) xn[u] = X[3 * (S.nat_solute + 3 * jwn) + u];
+      load_water(jwn, xn);
     }
     if (k + 8 < nn) jwn2 = row[k + 8];
     while (k < nn) {
@@ -266,7 +280,7 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(co
       k += 8;
       if (k < nn) {
         jwn = jwn2;
-        for (int u = 0; u < 9; ++u) xn[u] = X[3 * (S.nat_solute + 3 * jwn) + u];
+        load_water(jwn, xn);
         if (k + 8 < nn) jwn2 = row[k + 8];
       }
       ww_pair(jw, xj);
@@ -274,9 +288,8 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(co
   } else {
     for (int k = sub; k < nn; k += 8) {
       const int jw = row[k];
-      const int jo = S.nat_solute + 3 * jw;
       P xj[9];
-      for (int u = 0; u < 9; ++u) xj[u] = X[3 * jo + u];
+      load_water(jw, xj);
       ww_pair(jw, xj);
     }
   }
@@ -431,41 +444,45 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(co
       }
   }
   for (int k = 0; k < 9; ++k) g[k] = seg8_sum(g[k]);
-  e_el = seg8_sum(e_el);
-  e_vdw = seg8_sum(e_vdw);
-  e_lrf = seg8_sum(e_lrf);
-  e_rad = seg8_sum(e_rad);
-  e_fix = seg8_sum(e_fix);
+  if (en) {
+    e_el = seg8_sum(e_el);
+    e_vdw = seg8_sum(e_vdw);
+    e_lrf = seg8_sum(e_lrf);
+    e_rad = seg8_sum(e_rad);
+    e_fix = seg8_sum(e_fix);
+  }
   if (active && sub == 0) {
     for (int k = 0; k < 9; ++k) S.d[3 * io + k] = g[k];
-    double* ew = S.e_water + (size_t)iw * kEWater;
-    ew[0] = e_el; ew[1] = e_vdw; ew[2] = e_lrf; ew[3] = e_rad; ew[4] = e_fix;
+    if (en) {
+      double* ew = S.e_water + (size_t)iw * kEWater;
+      ew[0] = e_el; ew[1] = e_vdw; ew[2] = e_lrf; ew[3] = e_rad; ew[4] = e_fix;
+    }
   }
 }
-template __global__ void k_water_forces<double, 0, 0, 6, 0>(const DevSys*, const int2*);
-template __global__ void k_water_forces<double, 0, 0, 6, 1>(const DevSys*, const int2*);
-template __global__ void k_water_forces<double, 0, 1, 6, 0>(const DevSys*, const int2*);
-template __global__ void k_water_forces<double, 0, 1, 6, 1>(const DevSys*, const int2*);
-template __global__ void k_water_forces<double, 1, 0, 6, 0>(const DevSys*, const int2*);
-template __global__ void k_water_forces<double, 1, 0, 6, 1>(const DevSys*, const int2*);
-template __global__ void k_water_forces<double, 1, 1, 6, 0>(const DevSys*, const int2*);
-template __global__ void k_water_forces<double, 1, 1, 6, 1>(const DevSys*, const int2*);
-template __global__ void k_water_forces<float, 0, 0, 6, 0>(const DevSys*, const int2*);
-template __global__ void k_water_forces<float, 0, 0, 6, 1>(const DevSys*, const int2*);
-template __global__ void k_water_forces<float, 0, 0, 4, 0>(const DevSys*, const int2*);
-template __global__ void k_water_forces<float, 0, 0, 4, 1>(const DevSys*, const int2*);
-template __global__ void k_water_forces<float, 0, 1, 6, 0>(const DevSys*, const int2*);
-template __global__ void k_water_forces<float, 0, 1, 6, 1>(const DevSys*, const int2*);
-template __global__ void k_water_forces<float, 0, 1, 4, 0>(const DevSys*, const int2*);
-template __global__ void k_water_forces<float, 0, 1, 4, 1>(const DevSys*, const int2*);
-template __global__ void k_water_forces<float, 1, 0, 6, 0>(const DevSys*, const int2*);
-template __global__ void k_water_forces<float, 1, 0, 6, 1>(const DevSys*, const int2*);
-template __global__ void k_water_forces<float, 1, 0, 4, 0>(const DevSys*, const int2*);
-template __global__ void k_water_forces<float, 1, 0, 4, 1>(const DevSys*, const int2*);
-template __global__ void k_water_forces<float, 1, 1, 6, 0>(const DevSys*, const int2*);
-template __global__ void k_water_forces<float, 1, 1, 6, 1>(const DevSys*, const int2*);
-template __global__ void k_water_forces<float, 1, 1, 4, 0>(const DevSys*, const int2*);
-template __global__ void k_water_forces<float, 1, 1, 4, 1>(const DevSys*, const int2*);
+template __global__ void k_water_forces<double, 0, 0, 6, 0>(const DevSys*, const int2*, int);
+template __global__ void k_water_forces<double, 0, 0, 6, 1>(const DevSys*, const int2*, int);
+template __global__ void k_water_forces<double, 0, 1, 6, 0>(const DevSys*, const int2*, int);
+template __global__ void k_water_forces<double, 0, 1, 6, 1>(const DevSys*, const int2*, int);
+template __global__ void k_water_forces<double, 1, 0, 6, 0>(const DevSys*, const int2*, int);
+template __global__ void k_water_forces<double, 1, 0, 6, 1>(const DevSys*, const int2*, int);
+template __global__ void k_water_forces<double, 1, 1, 6, 0>(const DevSys*, const int2*, int);
+template __global__ void k_water_forces<double, 1, 1, 6, 1>(const DevSys*, const int2*, int);
+template __global__ void k_water_forces<float, 0, 0, 6, 0>(const DevSys*, const int2*, int);
+template __global__ void k_water_forces<float, 0, 0, 6, 1>(const DevSys*, const int2*, int);
+template __global__ void k_water_forces<float, 0, 0, 4, 0>(const DevSys*, const int2*, int);
+template __global__ void k_water_forces<float, 0, 0, 4, 1>(const DevSys*, const int2*, int);
+template __global__ void k_water_forces<float, 0, 1, 6, 0>(const DevSys*, const int2*, int);
+template __global__ void k_water_forces<float, 0, 1, 6, 1>(const DevSys*, const int2*, int);
+template __global__ void k_water_forces<float, 0, 1, 4, 0>(const DevSys*, const int2*, int);
+template __global__ void k_water_forces<float, 0, 1, 4, 1>(const DevSys*, const int2*, int);
+template __global__ void k_water_forces<float, 1, 0, 6, 0>(const DevSys*, const int2*, int);
+template __global__ void k_water_forces<float, 1, 0, 6, 1>(const DevSys*, const int2*, int);
+template __global__ void k_water_forces<float, 1, 0, 4, 0>(const DevSys*, const int2*, int);
+template __global__ void k_water_forces<float, 1, 0, 4, 1>(const DevSys*, const int2*, int);
+template __global__ void k_water_forces<float, 1, 1, 6, 0>(const DevSys*, const int2*, int);
+template __global__ void k_water_forces<float, 1, 1, 6, 1>(const DevSys*, const int2*, int);
+template __global__ void k_water_forces<float, 1, 1, 4, 0>(const DevSys*, const int2*, int);
+template __global__ void k_water_forces<float, 1, 1, 4, 1>(const DevSys*, const int2*, int);
 
 // ---------------------------------------------------------------------------
 // Water polarisation restraint (watpol, src/md.f90:18372-18541), part 1:
@@ -489,6 +506,20 @@ __global__ void k_watpol_theta(const DevSys* __restrict__ sys) {
       if (rc <= S.shell[is].rout) break;
     if (is < 1) is = 0;
     S.shell_of[iw] = is;
+    // the parts of the member's gradient that do not depend on its rank,
+    // kept in its gradient record until the shell kernel completes it:
+    // f1a + f2 on O (3), f1b on each H (3), -1 / sin(theta)
+    double* g = S.wpol_g + 9 * (size_t)iw;
+    for (int k = 0; k < 3; ++k) {
+      double f1a = -2.0 * (rcu[k] - rmu[k] * scp) / rm;
+      double f1b = (rcu[k] - rmu[k] * scp) / rm;
+      double f2 = (rmu[k] - rcu[k] * scp) / rc;
+      g[k] = f1a + f2;
+      g[3 + k] = f1b;
+    }
+    double f0 = sin(acos(scp));
+    if (fabs(f0) < qsys::kQrealEps) f0 = qsys::kQrealEps;
+    g[6] = -1.0 / f0;
   }
 }
 
@@ -496,7 +527,7 @@ __global__ void k_watpol_theta(const DevSys* __restrict__ sys) {
 // each member's position in qdyn's selection sort (ascending theta, ties to
 // the lower list index) is its rank; the target angle follows from the rank.
 constexpr int kShellCap = 4096;
-__global__ __launch_bounds__(1024) void k_watpol_force(const DevSys* __restrict__ sys) {
+__global__ __launch_bounds__(kWpolBlock) void k_watpol_force(const DevSys* __restrict__ sys) {
   const DevSys& S = sys[blockIdx.y];
   const int is = blockIdx.x;
   if (!S.wpol_restr || is >= S.nshell || halted(S, PH_WATPOL)) return;
@@ -505,7 +536,7 @@ __global__ __launch_bounds__(1024) void k_watpol_force(const DevSys* __restrict_
   __shared__ int mem_iw[kShellCap];
   __shared__ double mem_th[kShellCap];
   __shared__ int s_cnt;
-  __shared__ double red[2][16];
+  __shared__ double red[2][kWpolBlock / 64];
   DevShell& W = S.shell[is];
   if (threadIdx.x == 0) s_cnt = 0;
   __syncthreads();
@@ -515,7 +546,7 @@ __global__ __launch_bounds__(1024) void k_watpol_force(const DevSys* __restrict_
     bool mine = iw < S.nwat && S.shell_of[iw] == is;
     // block-wide ordered prefix: wave ballots then a serial pass over waves
     unsigned long long m = __ballot(mine);
-    __shared__ int wave_cnt[16];
+    __shared__ int wave_cnt[kWpolBlock / 64];
     int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
     if (ln == 0) wave_cnt[wv] = __popcll(m);
     __syncthreads();
@@ -563,25 +594,22 @@ __global__ __launch_bounds__(1024) void k_watpol_force(const DevSys* __restrict_
     double dth = th - th0 + tcorr;
     e = e + 0.5 * S.fkwpol * dth * dth;
     double dv = S.fkwpol * (th - th0 + tcorr);
-    const int i = S.nat_solute + 3 * iw;
-    double scp, rc, rm, rmu[3], rcu[3];
-    watpol_geometry(S, i, rmu, rcu, rm, rc, scp);
-    double f0 = sin(acos(scp));
-    if (fabs(f0) < qsys::kQrealEps) f0 = qsys::kQrealEps;
-    f0 = -1.0 / f0;
-    f0 = dv * f0;
-    double* d = S.d + 3 * i;
-    for (int k = 0; k < 3 && (S.term_mask & TM_OTHER); ++k) {
-      double f1a = -2.0 * (rcu[k] - rmu[k] * scp) / rm;
-      double f1b = (rcu[k] - rmu[k] * scp) / rm;
-      double f2 = (rmu[k] - rcu[k] * scp) / rc;
-      d[k] = d[k] + f0 * (f1a + f2);
-      d[3 + k] = d[3 + k] + f0 * (f1b);
-      d[6 + k] = d[6 + k] + f0 * (f1b);
+    // the water's gradient contribution (added to d by the integrator, or
+    // by k_watpol_gather outside a step), completed from the rank-free
+    // parts k_watpol_theta left in the record
+    double* g = S.wpol_g + 9 * (size_t)iw;
+    const double f0 = dv * g[6];
+    const bool on = (S.term_mask & TM_OTHER) != 0;
+    double ga[3], gb[3];
+    for (int k = 0; k < 3; ++k) { ga[k] = g[k]; gb[k] = g[3 + k]; }
+    for (int k = 0; k < 3; ++k) {
+      g[k] = on ? f0 * ga[k] : 0.0;
+      g[3 + k] = on ? f0 * gb[k] : 0.0;
+      g[6 + k] = on ? f0 * gb[k] : 0.0;
     }
   }
-  e = block_sum_1024(e, red[0]);
-  tsum = block_sum_1024(tsum, red[1]);
+  e = block_sum<kWpolBlock / 64>(e, red[0]);
+  tsum = block_sum<kWpolBlock / 64>(tsum, red[1]);
   if (threadIdx.x == 0) {
     W.n_insh = n;
     if (n > 0) {
@@ -592,21 +620,49 @@ __global__ __launch_bounds__(1024) void k_watpol_force(const DevSys* __restrict_
   }
 }
 
+// The polarisation gradient of shell members added to d (force evaluations
+// outside a step; a step's integrator adds it itself).
+__global__ void k_watpol_gather(const DevSys* __restrict__ sys) {
+  const DevSys& S = sys[blockIdx.y];
+  const int iw = blockIdx.x * blockDim.x + threadIdx.x;
+  if (iw >= S.nwat || !S.wpol_restr || S.shell_of[iw] < 0) return;
+  double* d = S.d + 3 * (S.nat_solute + 3 * iw);
+  const double* g = S.wpol_g + 9 * (size_t)iw;
+  for (int c = 0; c < 9; ++c) d[c] = d[c] + g[c];
+}
+
 // ---------------------------------------------------------------------------
 // Leapfrog/Berendsen update (newvel_ber src/md.f90:6194-6223), SHAKE
 // (shake :17581-17679), v = (x - xx)/dt, and the kinetic-energy partial
 // sums of temperature (:6098-6190), fused per SHAKE unit.  Units are SHAKE
 // molecules (waters: 3 atoms, 3 constraints) or single atoms.
-// The gradient of atom a as the step uses it: d plus the sum of the atom's
-// bonded-term slots (k_term_sum, formed beside the water forces); the sum
-// is also stored in d.
+// The gradient of atom a as the step uses it: d plus the polarisation
+// restraint and the sum of the atom's bonded-term slots; the result is
+// also stored in d.
 __device__ __forceinline__ void gather_slots(const DevSys& S, int a) {
+  // the polarisation restraint first (it was added right after the water
+  // forces before it ran beside them)
+  if (S.wpol_restr && a >= S.nat_solute) {
+    const int rel = a - S.nat_solute, iw = rel / 3;
+    if (S.shell_of[iw] >= 0) {
+      const double* g = S.wpol_g + 9 * (size_t)iw + 3 * (rel % 3);
+      S.d[3 * a] = S.d[3 * a] + g[0];
+      S.d[3 * a + 1] = S.d[3 * a + 1] + g[1];
+      S.d[3 * a + 2] = S.d[3 * a + 2] + g[2];
+    }
+  }
+  // the atom's bonded/Q-Q/restraint term slots, summed in slot order
   const int t = S.slot_of_atom[a];
-  if (t < 0) return;
-  const double* g = S.slot_sum + 3 * (size_t)t;
-  S.d[3 * a] = S.d[3 * a] + g[0];
-  S.d[3 * a + 1] = S.d[3 * a + 1] + g[1];
-  S.d[3 * a + 2] = S.d[3 * a + 2] + g[2];
+  if (t < 0 || !(S.term_mask & TM_OTHER)) return;
+  double g0 = 0, g1 = 0, g2 = 0;
+  const double* o = S.slot_g + 3 * (size_t)S.slot_first[t];
+  const int n = S.slot_first[t + 1] - S.slot_first[t];
+  for (int k = 0; k < n; ++k) {
+    g0 = g0 + o[3 * k]; g1 = g1 + o[3 * k + 1]; g2 = g2 + o[3 * k + 2];
+  }
+  S.d[3 * a] = S.d[3 * a] + g0;
+  S.d[3 * a + 1] = S.d[3 * a + 1] + g1;
+  S.d[3 * a + 2] = S.d[3 * a + 2] + g2;
 }
 
 __global__ __launch_bounds__(kIntegrateBlock) void k_integrate(const DevSys* __restrict__ sys) {
@@ -667,6 +723,8 @@ __global__ __launch_bounds__(kIntegrateBlock) void k_integrate(const DevSys* __r
       S.xf[ic] = (float)x[c];
       S.v[ic] = v[c];
     }
+    // solvent waters start at nat_solute, three atoms each
+    if (S.xw && a >= S.nat_solute) water_xf_store(S, (a - S.nat_solute) / 3, x);
     for (int k = 0; k < 3; ++k) {
       const int i = a + k;
       const double ek = 0.5 * S.mass[i] * (v[3 * k] * v[3 * k] + v[3 * k + 1] * v[3 * k + 1] + v[3 * k + 2] * v[3 * k + 2]);
@@ -737,6 +795,10 @@ __global__ __launch_bounds__(kIntegrateBlock) void k_integrate(const DevSys* __r
         if (S.has_shake) S.v[ic] = (S.x[ic] - S.xx[ic]) / dt;
         vv[c] = S.v[ic];
         S.xf[ic] = (float)S.x[ic];
+        if (S.xw && i >= S.nat_solute) {
+          const int rel = i - S.nat_solute;
+          S.xw[16 * (rel / 3) + 3 * (rel % 3) + c] = (float)S.x[ic];
+        }
       }
       const double ek = 0.5 * S.mass[i] * (vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2]);
       const bool ex = (S.flags[i] & F_EXCL) != 0;
@@ -761,6 +823,9 @@ __global__ __launch_bounds__(256) void k_temperature(const DevSys* __restrict__ 
     if (threadIdx.x == 0) atomicMin(S.fail, 0ull);
     return;
   }
+  // the Q bonds' Morse factors of this step's evaluation, kept for the next
+  for (int t = threadIdx.x; t < S.nqbond; t += blockDim.x)
+    for (int c = 0; c < kMorseRec; ++c) S.qb_prev[kMorseRec * t + c] = S.qb_cur[kMorseRec * t + c];
   __shared__ double red[4][4];
   double p[4] = {0, 0, 0, 0};
   for (int b = threadIdx.x; b < nblocks; b += blockDim.x)

@@ -10,7 +10,7 @@ the polarisation-shell theta_corr) to the next.
 from __future__ import annotations
 
 import os
-from typing import Optional, Sequence, Tuple
+from typing import Callable, Optional, Sequence, Tuple
 
 from . import gensys
 from .engine import QGPU_FP64, Engine
@@ -27,9 +27,9 @@ DEFAULT_STAGES: Tuple[Tuple[int, float, float, float], ...] = (
 def equilibrate(system: gensys.System, directory: str, name: str,
                 stages: Sequence[Tuple[int, float, float, float]] = DEFAULT_STAGES,
                 lambdas: Optional[Sequence[float]] = None, seed: int = 1,
-                precision: int = QGPU_FP64) -> str:
+                precision: int = QGPU_FP64, progress: Optional[Callable[[str], None]] = None) -> str:
     """Runs the stages and returns the path of the final restart file (in
-    `directory`, named `name`_eq.re)."""
+    `directory`, named `name`_eq.re); `progress` receives a line per stage."""
     rst = None
     for k, (n, dt, temp, tau) in enumerate(stages):
         kw = dict(steps=n, stepsize=dt, temperature=temp, bath_coupling=tau, output=1000000)
@@ -47,6 +47,8 @@ def equilibrate(system: gensys.System, directory: str, name: str,
         rst = os.path.join(directory, "%s_eq%d.re" % (name, k))
         gensys.write_restart(rst, x, v, tc if len(tc) else None)
         e.close()
+        if progress:
+            progress("%s: equilibration stage %d (%d steps of %.1f fs at %.0f K) done" % (name, k, n, dt, temp))
     final = os.path.join(directory, name + "_eq.re")
     os.replace(rst, final)
     return final
