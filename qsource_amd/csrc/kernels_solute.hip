@@ -1312,9 +1312,16 @@ __global__ __launch_bounds__(256) void k_term_gather(const DevSys* __restrict__ 
   if (t >= S.nslot_atoms) return;
   const int me = S.slot_atom[t];
   double g0 = 0, g1 = 0, g2 = 0;
-  const double* o = S.slot_g + 3 * (size_t)S.slot_first[t];
+  const gconst<double> o = gptr(S.slot_g) + 3 * (size_t)S.slot_first[t];
   const int n = S.slot_first[t + 1] - S.slot_first[t];
-  for (int k = 0; k < n; ++k) {
+  // four slots' loads in flight at a time; the sums keep slot order
+  int k = 0;
+  for (; k + 4 <= n; k += 4) {
+    double v[12];
+    for (int u = 0; u < 12; ++u) v[u] = o[3 * k + u];
+    for (int u = 0; u < 4; ++u) { g0 = g0 + v[3 * u]; g1 = g1 + v[3 * u + 1]; g2 = g2 + v[3 * u + 2]; }
+  }
+  for (; k < n; ++k) {
     g0 = g0 + o[3 * k]; g1 = g1 + o[3 * k + 1]; g2 = g2 + o[3 * k + 2];
   }
   if (!(S.term_mask & TM_OTHER)) return;
