@@ -48,7 +48,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_TFLOPS = {"f64": 78.6, "mixed": 157.3}   # MI355X vector FP64 / FP32 (published)
-FLOP_COUL, FLOP_LJ, FLOP_LRF_UPDATE = 24, 34, 60
+FLOP_COUL, FLOP_LJ, FLOP_LRF_UPDATE, FLOP_LRF_ATOM = 24, 34, 60, 90
 IENE = 10
 SE_GATE = 0.02        # kcal/mol: stderr below which the trajectory ddG gate is established
 DDG_GATE = 0.05
@@ -118,10 +118,10 @@ def build_c5(d, equil=True, windows=16):
     from qsource_amd import equilibrate, gensys
     t0 = time.time()
     s = gensys.evb_system(radius=40.0, n_solute=5000, seed=5000, n_q=24, n_states=4)
+    log("c5 system: R=40 A, 5000 solute atoms, %d atoms, %d waters, built in %.1f s" %
+        (s.natom, s.nwat, time.time() - t0))
     lam = gensys.mapping_lambdas(4, windows)
-    rst = equilibrate.equilibrate(s, d, "c5", lambdas=list(lam[0]), seed=21,
-                                  stages=equilibrate.SHORT_STAGES) if equil else None
-    log("c5 system: R=40 A, 5000 solute atoms, %d atoms, equilibrated in %.1f s" % (s.natom, time.time() - t0))
+    rst = equilibrate.equilibrate(s, d, "c5", lambdas=list(lam[0]), seed=21, progress=log) if equil else None
     inputs = []
     for k in range(windows):
         kw = dict(steps=1000000, output=1000000, lambdas=list(lam[k]), initial_temperature=300.0,
@@ -130,9 +130,24 @@ def build_c5(d, equil=True, windows=16):
             kw["restart"] = os.path.basename(rst)
         inputs.append(gensys.write_system(s, d, "c5_w%02d" % k, **kw)["inp"])
     cfg = {"workload": "evb_mapping_windows", "systems": "R40/5000 solute atoms, EVB 4 states",
-           "atoms_per_system": [s.natom], "windows_per_system": windows, "instances": windows,
-           "q_atoms": 24, "states": 4, "stepsize_fs": 1.0, "nbcycle": 10, "energy_interval": IENE}
+           "atoms_per_system": [s.natom], "waters": s.nwat, "windows_per_system": windows, "instances": windows,
+           "q_atoms": 24, "states": 4, "offdiagonals": len(s.offdiag), "stepsize_fs": 1.0, "nbcycle": 10,
+           "energy_interval": IENE, "far_field": "learned surrogate (BF16-MFMA MLP) in place of LRF"}
     return inputs, [list(range(windows))], [lam], cfg
+
+
+def c5_surrogate(inp, nframes=40, stride=25):
+    """The far-field surrogate of the EVB workload: trained on the GPU on
+    exact FP64 far-field targets of `nframes` frames of the first window."""
+    from qsource_amd import farfield
+    t0 = time.time()
+    Xs, Ys, _ = farfield.sample_frames(inp, nframes, stride)
+    X, Y = np.concatenate(Xs), np.concatenate(Ys)
+    model, loss = farfield.train_surrogate(X, Y, epochs=30, batch=1024, lr=2e-3, seed=11)
+    info = {"training_frames": nframes, "frame_stride_steps": stride, "samples": int(len(X)),
+            "train_loss": loss, "seconds": time.time() - t0}
+    log("surrogate: %s" % info)
+    return model, info
 
 
 # ---------------------------------------------------------------------------
@@ -264,29 +279,43 @@ def force_error(inputs, states, terms=("ww", "pw", "pp", "qx", "lrf")):
 # ---------------------------------------------------------------------------
 # roofline accounting
 
-def kernel_rooflines(e, prof, nbcycle=10):
-    """Roofline entries of the two large kernels, with per-step times:
-    water forces (reference ww water pairs x 226 FLOP per launch) and water
-    lists (lrf_update atom-to-centre evaluations x 60 FLOP per launch)."""
-    fw = fl = 0.0
-    wpairs = 0
-    evals = 0.0
+def kernel_rooflines(e, prof, nbcycle=10, lrf=True):
+    """Roofline entries of the four large kernels, from the reference's
+    list sizes (counted by the engine) and the per-unit FLOP counts:
+    water forces: ww water pairs x 226, solute-water atom pairs x 34 (the
+    water side), Q-water atom pairs x 24 and, with LRF, the Taylor term of
+    each water atom (90); solute forces: solute-solute and solute-water atom
+    pairs x 34 each and the Taylor term of each solute atom; water and solute
+    lists: lrf_update atom-to-centre evaluations x 60."""
+    acc = {"water_forces": 0.0, "solute_forces": 0.0, "water_lists": 0.0, "solute_lists": 0.0}
+    units = {"water_pairs": 0, "pw_atom_pairs": 0, "pp_atom_pairs": 0, "qw_atom_pairs": 0,
+             "lrf_evals_water_side": 0.0, "lrf_evals_solute_side": 0.0}
     for k in range(e.n):
-        ww = e.list_counts(k)["ww"] // 9
-        wpairs += ww
-        fw += ww * (8 * FLOP_COUL + FLOP_LJ)
-        wc = e.work_counts(k)
-        evals += wc["lrf_evals_water_side"]
-    fl = evals * FLOP_LRF_UPDATE
+        lc = e.list_counts(k)
+        ww = lc["ww"] // 9
+        nw = (e.natom[k] - e.nat_solute(k)) // 3
+        units["water_pairs"] += ww
+        units["pw_atom_pairs"] += lc["pw"]
+        units["pp_atom_pairs"] += lc["pp"]
+        units["qw_atom_pairs"] += lc["qw"]
+        acc["water_forces"] += ww * (8 * FLOP_COUL + FLOP_LJ) + lc["pw"] * FLOP_LJ + lc["qw"] * FLOP_COUL + \
+            (3 * nw * FLOP_LRF_ATOM if lrf else 0)
+        acc["solute_forces"] += (lc["pp"] + lc["pw"]) * FLOP_LJ + (e.nat_solute(k) * FLOP_LRF_ATOM if lrf else 0)
+        if lrf:
+            wc = e.work_counts(k)
+            units["lrf_evals_water_side"] += wc["lrf_evals_water_side"]
+            units["lrf_evals_solute_side"] += wc["lrf_evals_solute_side"]
+    acc["water_lists"] = units["lrf_evals_water_side"] * FLOP_LRF_UPDATE
+    acc["solute_lists"] = units["lrf_evals_solute_side"] * FLOP_LRF_UPDATE
     out = {}
-    for name, flop, extra in (("water_forces", fw, {"water_pairs_per_launch": wpairs}),
-                              ("water_lists", fl, {"lrf_evals_per_launch": evals})):
+    for name, flop in acc.items():
         ms = prof.get(name, 0.0)
         ach = flop / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
-        per_step = ms / nbcycle if name == "water_lists" else ms
-        out[name] = dict({"bound": "valu", "kernel": name, "achieved": ach, "peak": PEAK_TFLOPS["mixed"],
-                          "unit": "TFLOP/s", "frac": ach / PEAK_TFLOPS["mixed"], "flop_per_launch": flop,
-                          "kernel_ms": ms, "ms_per_step": per_step}, **extra)
+        per_step = ms / nbcycle if name.endswith("lists") else ms
+        out[name] = {"bound": "valu", "kernel": name, "achieved": ach, "peak": PEAK_TFLOPS["mixed"],
+                     "unit": "TFLOP/s", "frac": ach / PEAK_TFLOPS["mixed"], "flop_per_launch": flop,
+                     "kernel_ms": ms, "ms_per_step": per_step}
+    out["units_per_launch"] = units
     tw = profile_value("k_water_forces_c3_63win_mixed_pmc_traffic.md", "traffic_bytes_per_launch")
     tl = profile_value("k_water_lists_c3_63win_mixed_pmc_traffic.md", "traffic_bytes_per_launch")
     out["water_forces"]["traffic"] = int(tw) if tw else None
@@ -391,8 +420,11 @@ def main():
             log("CPU baseline: oracle, %d windows" % args.cpu_procs)
             cpu = cpu_baseline([inputs[k] for k in groups[1]], args.cpu_seconds, args.cpu_procs)
             log("CPU baseline: %s" % cpu)
+        model = None
+        if args.config == "c5":
+            model, cfg["surrogate"] = c5_surrogate(inputs[0])
         log("mixed batch: %d instances" % n)
-        em, wall, warm = run_batch(inputs, qe.QGPU_MIXED, args.steps, args.warmup)
+        em, wall, warm = run_batch(inputs, qe.QGPU_MIXED, args.steps, args.warmup, far_model=model)
         skip = warm // IENE
         tuned = {k: {"mapping": ["grid2d", "instance_map", "xcd_map"][m], "block": b, "prefetch": pf,
                      "waves_per_simd": (4 if oc else 6) if k == "water_forces" else None,
@@ -410,7 +442,7 @@ def main():
         prof = em.profile_kernels(args.profile_steps)
         step_ms = per_step_ms(prof)
         dom = max(step_ms, key=lambda k: step_ms[k])
-        roofs = kernel_rooflines(em, prof)
+        roofs = kernel_rooflines(em, prof, lrf=model is None)
         roof = roofs.get(dom, roofs["water_forces"])
         roof = dict(roof, dominant_kernel=dom)
         em.close()
@@ -425,7 +457,7 @@ def main():
             "force_error_one_window_per_system": ferr}
         if not args.skip_fp64:
             log("fp64 batch (independent trajectories)")
-            ef, wall64, _ = run_batch(inputs, qe.QGPU_FP64, args.steps, args.warmup)
+            ef, wall64, _ = run_batch(inputs, qe.QGPU_FP64, args.steps, args.warmup, far_model=model)
             ser_f = series(ef, groups, skip)
             ef.close()
             tr = trajectory_ddg(ser_m, ser_f, lams)
@@ -450,6 +482,8 @@ def main():
         ep, wallp, _ = run_batch([inputs[pw[len(pw) // 2]]], qe.QGPU_FP64, max(200, args.steps // 4), 50)
         plain = max(200, args.steps // 4) * 1e-6 * 86400.0 / wallp
         ep.close()
+        if model is not None:
+            model.close()
     cfg["precision"] = "mixed"
     ref_cpu = profile_json("cpu_reference_qdyn5_%s.md" % args.config)
     out.update({"value": value, "ms_per_step": wall / args.steps * 1e3, "config": cfg,

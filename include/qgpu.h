@@ -58,6 +58,11 @@ int qgpu_destroy(qgpu_handle* h);
 /* Number of atoms of instance k (nat_pro, src/topo.f90:104). */
 int qgpu_natom(qgpu_handle* h, int k, int* natom);
 
+/* Sizes of instance k (sizes[0..6]): atoms, solute atoms (nat_solute),
+ * waters, charge groups, solute charge groups (ncgp_solute), Q atoms and
+ * FEP states (src/topo.f90:104-130, src/qatom.f90:301-420). */
+int qgpu_system_sizes(qgpu_handle* h, int k, int* sizes, int cap);
+
 /* Number of FEP states of instance k (nstates, src/qatom.f90:301-420;
  * 0 without an FEP file). */
 int qgpu_nstates(qgpu_handle* h, int k, int* nstates);

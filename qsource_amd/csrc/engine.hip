@@ -1943,6 +1943,15 @@ int qgpu_natom(qgpu_handle* h, int k, int* natom) {
   return guarded([&] { CHECK_K(h, k); *natom = h->b->setup(k).natom; });
 }
 
+int qgpu_system_sizes(qgpu_handle* h, int k, int* sizes, int cap) {
+  return guarded([&] {
+    CHECK_K(h, k);
+    const qsys::Setup& s = h->b->setup(k);
+    const int v[7] = {s.natom, s.nat_solute, s.nwat, (int)s.top.cgp.size(), s.top.ncgp_solute, s.nqat, s.nstates};
+    for (int i = 0; i < 7 && i < cap; ++i) sizes[i] = v[i];
+  });
+}
+
 int qgpu_nstates(qgpu_handle* h, int k, int* nstates) {
   return guarded([&] { CHECK_K(h, k); *nstates = h->b->setup(k).nstates; });
 }

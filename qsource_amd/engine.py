@@ -40,7 +40,7 @@ SYMBOLS = [
     "qgpu_trj_create", "qgpu_trj_natoms", "qgpu_trj_write", "qgpu_trj_close", "qgpu_write_restart",
     "qgpu_autotune", "qgpu_set_launch_shape",
     "qgpu_instance_status", "qgpu_list_capacity", "qgpu_set_list_capacity", "qgpu_pot_energy_terms",
-    "qgpu_work_counts", "qgpu_mlp_gradient_noalloc",
+    "qgpu_work_counts", "qgpu_mlp_gradient_noalloc", "qgpu_system_sizes",
 ]
 TERM_GROUPS = {"ww": 1, "pw": 2, "pp": 4, "qx": 8, "lrf": 16, "other": 32}
 TUNED_KERNELS = ["water_lists", "solute_lists", "water_forces", "solute_forces"]
@@ -107,6 +107,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.qgpu_autotune.argtypes = [P, ctypes.c_int, dp, ip]
     lib.qgpu_set_launch_shape.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     lib.qgpu_pot_energy_terms.argtypes = [P, ctypes.c_int]
+    lib.qgpu_system_sizes.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
     lib.qgpu_work_counts.argtypes = [P, ctypes.c_int, dp, ctypes.c_int]
     lib.qgpu_instance_status.argtypes = [P, ctypes.c_int, ip, ip]
     lib.qgpu_list_capacity.argtypes = [P, ctypes.c_int, ip, ip]
@@ -155,6 +156,16 @@ class Engine:
     def _check(self, rc: int):
         if rc != 0:
             raise QgpuError(self.lib.qgpu_last_error().decode())
+
+    def sizes(self, k: int = 0) -> dict:
+        """Atoms, solute atoms, waters, charge groups, solute charge groups,
+        Q atoms and FEP states of instance k."""
+        v = (ctypes.c_int * 7)()
+        self._check(self.lib.qgpu_system_sizes(self.h, k, v, 7))
+        return dict(zip(["natom", "nat_solute", "nwat", "ncgp", "ncgp_solute", "nqat", "nstates"], list(v)))
+
+    def nat_solute(self, k: int = 0) -> int:
+        return self.sizes(k)["nat_solute"]
 
     def close(self):
         if getattr(self, "h", None):

@@ -176,8 +176,13 @@ def water_lattice(radius: float, rng, avoid: Optional[np.ndarray] = None,
     O = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
     O = O[np.linalg.norm(O, axis=1) <= radius]
     if avoid is not None and len(avoid):
-        d2 = ((O[:, None, :] - avoid[None, :, :]) ** 2).sum(-1)
-        O = O[d2.min(1) > avoid_r ** 2]
+        # in blocks of lattice sites (the full site x atom array of a large
+        # solute would not fit in memory); the same distances as one pass
+        keep = np.zeros(len(O), dtype=bool)
+        for b0 in range(0, len(O), 1024):
+            d2 = ((O[b0:b0 + 1024, None, :] - avoid[None, :, :]) ** 2).sum(-1)
+            keep[b0:b0 + 1024] = d2.min(1) > avoid_r ** 2
+        O = O[keep]
     nw = O.shape[0]
     half = math.radians(THETA_HOH) / 2
     local = np.array([[0, 0, 0],
@@ -201,6 +206,22 @@ def _branched_solute(n_atoms: int, rmax: float, rng, bond=1.53, min_dist=3.2):
     bonds = []
     backbone: List[int] = [0]
     n = 1
+    # placed atoms by cell of edge min_dist: an atom closer than min_dist to
+    # a candidate lies in one of the 27 cells around the candidate's
+    cells: dict = {(0, 0, 0): [0]}
+
+    def cell_of(p):
+        return (int(math.floor(p[0] / min_dist)), int(math.floor(p[1] / min_dist)),
+                int(math.floor(p[2] / min_dist)))
+
+    def close_atoms(p, skip):
+        c = cell_of(p)
+        out = []
+        for a in (c[0] - 1, c[0], c[0] + 1):
+            for b in (c[1] - 1, c[1], c[1] + 1):
+                for d in (c[2] - 1, c[2], c[2] + 1):
+                    out.extend(k for k in cells.get((a, b, d), ()) if k not in skip)
+        return out
 
     def near(parent):
         # atoms within two bonds of the parent: 1-2, 1-3 and 1-4 of the new atom
@@ -211,8 +232,6 @@ def _branched_solute(n_atoms: int, rmax: float, rng, bond=1.53, min_dist=3.2):
 
     def place_from(parent, tries=60):
         skip = near(parent)
-        others = np.array([k for k in range(n) if k not in skip], dtype=int)
-        oc = coords[others] if len(others) else np.zeros((0, 3))
         # 1-3 partners at >= 2.35 A (angle >~ 100 deg), 1-4 partners >= 2.6 A
         n13 = np.array(nbr[parent], dtype=int)
         n14 = np.array(sorted(skip - {parent} - set(nbr[parent])), dtype=int)
@@ -234,7 +253,8 @@ def _branched_solute(n_atoms: int, rmax: float, rng, bond=1.53, min_dist=3.2):
             p = coords[parent] + bond * d
             if np.linalg.norm(p) > rmax:
                 continue
-            if len(oc) and ((oc - p) ** 2).sum(1).min() < min_dist ** 2:
+            close = close_atoms(p, skip)
+            if close and ((coords[close] - p) ** 2).sum(1).min() < min_dist ** 2:
                 continue
             if len(c13) and ((c13 - p) ** 2).sum(1).min() < 2.40 ** 2:
                 continue
@@ -246,6 +266,7 @@ def _branched_solute(n_atoms: int, rmax: float, rng, bond=1.53, min_dist=3.2):
     def add(parent, p):
         nonlocal n
         coords[n] = p
+        cells.setdefault(cell_of(p), []).append(n)
         nbr[parent].append(n)
         nbr[n].append(parent)
         bonds.append((parent, n))
@@ -258,7 +279,7 @@ def _branched_solute(n_atoms: int, rmax: float, rng, bond=1.53, min_dist=3.2):
         p = place_from(tip)
         if p is None:
             stuck += 1
-            if stuck > 5000:
+            if stuck > 50000:
                 raise RuntimeError("solute growth failed; increase rmax")
             open_atoms = [a for a in backbone if len(nbr[a]) < 3]
             tip = open_atoms[int(rng.integers(len(open_atoms)))]
@@ -1164,6 +1185,32 @@ def read_restart(path: str):
         nsh = struct.unpack_from("<i", recs[3])[0]
         tc = np.frombuffer(recs[3], dtype="<f8", offset=4, count=nsh).copy()
     return x.reshape(-1, 3), v.reshape(-1, 3), tc
+
+
+def mapping_lambdas(n_states: int, windows: int) -> np.ndarray:
+    """EVB mapping windows that move the system from state 1 through 2, ...
+    to state n_states: window k sits at t = k / (windows - 1) of the path,
+    on the segment between states s and s + 1 with weight u, so its lambda
+    vector is (1 - u) e_s + u e_{s+1}.  Returns (windows, n_states)."""
+    lam = np.zeros((windows, n_states))
+    seg = n_states - 1
+    for k in range(windows):
+        t = k / max(1, windows - 1) * seg
+        sI = min(seg - 1, int(math.floor(t)))
+        u = t - sI
+        lam[k, sI] = 1.0 - u
+        lam[k, sI + 1] = u
+    return np.round(lam, 6)
+
+
+def evb_system(radius: float, n_solute: int, seed: int, n_q: int = 24, n_states: int = 4) -> System:
+    """The EVB-style workload: a solvated branched solute (as
+    solvated_system) with an n_q-atom Q region of n_states states carrying
+    per-state charges and Q-atom LJ types, and n_states (n_states - 1) / 2
+    off-diagonal couplings A exp(-mu r) (A ~ U[10,60] kcal/mol,
+    mu ~ U[0,0.5] 1/A) between random state pairs and Q-atom pairs."""
+    return solvated_system(radius=radius, n_solute=n_solute, seed=seed, n_q=n_q, n_states=n_states,
+                           q_types=True, n_offdiag=n_states * (n_states - 1) // 2)
 
 
 def write_system(s: System, directory: str, name: str, **inp) -> dict:
