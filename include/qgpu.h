@@ -94,6 +94,31 @@ enum qgpu_term_group {
  * the reference. */
 int qgpu_pot_energy_terms(qgpu_handle* h, int mask);
 
+/* Replaces pot_energy's call of pot_energy_nonbonds (src/md.f90:16096,
+ * 16261-16344): the solute-solute, solute-water, Q-solute, water-water and
+ * Q-water pair terms over the current pair lists and the LRF Taylor term,
+ * for instance k at the coordinates x (3*natom FP64; they become the
+ * instance's coordinates, as the reference's x is the module global).
+ * d (3*natom) accumulates the gradient, as the reference adds into the d
+ * that pot_energy zeroed.  E[7] = E%pp%el, E%pp%vdw, E%pw%el, E%pw%vdw,
+ * E%ww%el, E%ww%vdw, E%LRF; EQ[4*nstates] = per state EQ%qp%el, %qp%vdw,
+ * EQ%qw%el, %qw%vdw (eq_cap >= 4*nstates).  nonbond_qq, the bonded terms
+ * and the restraints are not included (qgpu_pot_energy_terms selects any
+ * group). */
+int qgpu_pot_energy_nonbonds(qgpu_handle* h, int k, const double* x, double* d, double* E, double* EQ,
+                             int eq_cap);
+
+/* Replaces the reference's `integer function shake(xx, x)`
+ * (src/md.f90:17581-17679), which md_run calls after the update
+ * (src/md.f90:6596): every SHAKE molecule of instance k is iterated until
+ * each constraint meets |d0^2 - r^2| < 1e-4 d0^2, moving x (3*natom FP64,
+ * in place) along the bonds of the pre-step coordinates xx.  *niter (may be
+ * NULL) = the iterations summed over the molecules, divided by the number
+ * of molecules (integer division, as the function's result).  A molecule
+ * that needs 1000 iterations fails: the call returns nonzero with
+ * qgpu_last_error() = "shake failure" (die, src/md.f90:17673). */
+int qgpu_shake(qgpu_handle* h, int k, const double* xx, double* x, int* niter);
+
 /* Replaces md_run's loop body (src/md.f90:6471-6612): nsteps steps of
  * list update (every NBcycle), forces, leapfrog/Berendsen, SHAKE and
  * temperature, with x, v and d resident on the device.
@@ -284,6 +309,13 @@ int qgpu_autotune(qgpu_handle* h, int reps, double* ms, int* shapes);
 /* Fixes the launch shape of tuned kernel `kernel` (0-3 as above) and
  * switches the tuning at the first step off. */
 int qgpu_set_launch_shape(qgpu_handle* h, int kernel, int map, int block, int prefetch, int occupancy);
+
+/* Replaces offdiag (src/md.f90:15444-15470, called by md_run at :6568):
+ * every [off_diagonals] function of instance k at its current coordinates,
+ * 6 values each: state i, state j, Q atoms k and l (1-based), Hij =
+ * A exp(-mu r_kl) and r_kl.  *n = number of functions; out is written
+ * while cap allows. */
+int qgpu_offdiag(qgpu_handle* h, int k, double* out, int cap, int* n);
 
 /* State indices (i, j, 1-based) of every [off_diagonals] function of
  * instance k, as the energy file's off-diagonal record holds them

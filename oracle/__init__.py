@@ -40,7 +40,8 @@ def _load():
         for name in ["oracle_destroy", "oracle_natom", "oracle_nstates", "oracle_run",
                      "oracle_final_energy", "oracle_get_xv", "oracle_get_d", "oracle_get_energies",
                      "oracle_get_qenergies", "oracle_list_sizes", "oracle_get_list", "oracle_get_lrf",
-                     "oracle_temperature", "oracle_theta_corr", "oracle_nshells", "oracle_offdiag"]:
+                     "oracle_temperature", "oracle_theta_corr", "oracle_nshells", "oracle_offdiag",
+                     "oracle_sizes", "oracle_pot_energy_nonbonds", "oracle_shake", "oracle_pot_energy"]:
             getattr(lib, name).argtypes = None
         lib.oracle_temperature.restype = ctypes.c_double
         _lib = lib
@@ -75,6 +76,11 @@ class Oracle:
     def run(self, n: int):
         rc = self.lib.oracle_run(self.h, ctypes.c_int(n))
         if rc:
+            raise RuntimeError(self.lib.oracle_last_error().decode())
+
+    def pot_energy(self):
+        """pot_energy at the current coordinates over the current lists."""
+        if self.lib.oracle_pot_energy(self.h):
             raise RuntimeError(self.lib.oracle_last_error().decode())
 
     def final_energy(self):
@@ -132,6 +138,28 @@ class Oracle:
         out = np.zeros(43 * ncgp)
         self.lib.oracle_get_lrf(self.h, _dp(out))
         return out.reshape(ncgp, 43)
+
+    def pot_energy_nonbonds(self, x: np.ndarray, d: np.ndarray):
+        """pot_energy_nonbonds at x over the current lists; adds into d and
+        returns (E7, EQ (nstates, 4)) as the engine's entry point does."""
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        e = np.zeros(7)
+        eq = np.zeros(max(4 * self.nstates, 1))
+        if self.lib.oracle_pot_energy_nonbonds(self.h, _dp(x), _dp(d), _dp(e), _dp(eq)):
+            raise RuntimeError(self.lib.oracle_last_error().decode())
+        return e, eq[:4 * self.nstates].reshape(self.nstates, 4)
+
+    def shake(self, xx: np.ndarray, x: np.ndarray) -> int:
+        xx = np.ascontiguousarray(xx, dtype=np.float64).ravel()
+        n = ctypes.c_int(0)
+        if self.lib.oracle_shake(self.h, _dp(xx), _dp(x), ctypes.byref(n)):
+            raise RuntimeError(self.lib.oracle_last_error().decode())
+        return n.value
+
+    def sizes(self) -> dict:
+        n = (ctypes.c_int * 3)()
+        self.lib.oracle_sizes(self.h, n)
+        return dict(nwat=n[0], nqat=n[1], nat_solute=n[2])
 
     def temperature(self) -> float:
         return self.lib.oracle_temperature(self.h)

@@ -516,13 +516,9 @@ struct Oracle {
     }
   }
 
-  void pot_energy() {
-    const double kinetic = E.kinetic;  // pot_energy leaves E%kinetic alone (src/md.f90:16025)
-    E = Energies{};
-    E.kinetic = kinetic;
-    for (auto& q : EQ) q = QEnergies{};
-    std::fill(d.begin(), d.end(), 0.0);
-    // pot_energy_nonbonds (sphere, geometric rule)
+  // pot_energy_nonbonds (src/md.f90:16261-16344; sphere, geometric rule):
+  // adds into d, E and EQ
+  void pot_energy_nonbonds() {
     nonbond_pp();
     nonbond_pw();
     if (s.qvdw_flag) nonbond_qp_qvdw();
@@ -537,6 +533,15 @@ struct Oracle {
       }
     }
     if (s.inp.use_lrf) lrf_taylor();
+  }
+
+  void pot_energy() {
+    const double kinetic = E.kinetic;  // pot_energy leaves E%kinetic alone (src/md.f90:16025)
+    E = Energies{};
+    E.kinetic = kinetic;
+    for (auto& q : EQ) q = QEnergies{};
+    std::fill(d.begin(), d.end(), 0.0);
+    pot_energy_nonbonds();
     // bonded (FF_GROMOS/default impropers)
     E.p_bond = bond(0, s.nbonds_solute);
     E.w_bond = bond(s.nbonds_solute, (int)s.bnd.size());
@@ -1750,6 +1755,55 @@ const char* oracle_last_error() { return g_err.c_str(); }
 void oracle_destroy(void* h) { delete static_cast<Oracle*>(h); }
 int oracle_natom(void* h) { return static_cast<Oracle*>(h)->s.natom; }
 int oracle_nstates(void* h) { return static_cast<Oracle*>(h)->s.nstates; }
+// pot_energy_nonbonds at coordinates x over the current lists: d += the
+// gradient; E7 = pp el/vdW, pw el/vdW, ww el/vdW, LRF; EQ4 per state = qp
+// el/vdW, qw el/vdW.  x becomes the oracle's coordinates.
+int oracle_pot_energy_nonbonds(void* h, const double* x, double* d, double* E7, double* EQ4) {
+  auto* o = static_cast<Oracle*>(h);
+  try {
+    std::copy(x, x + o->x.size(), o->x.begin());
+    const double kinetic = o->E.kinetic;
+    o->E = Energies{};
+    o->E.kinetic = kinetic;
+    for (auto& q : o->EQ) q = QEnergies{};
+    std::vector<double> d0(o->d.size(), 0.0);
+    std::swap(d0, o->d);
+    o->pot_energy_nonbonds();
+    for (size_t k = 0; k < o->d.size(); ++k) d[k] = d[k] + o->d[k];
+    std::swap(d0, o->d);
+    const Energies& E = o->E;
+    const double e7[7] = {E.pp_el, E.pp_vdw, E.pw_el, E.pw_vdw, E.ww_el, E.ww_vdw, E.lrf};
+    std::copy(e7, e7 + 7, E7);
+    for (int st = 0; st < o->s.nstates; ++st) {
+      const QEnergies& q = o->EQ[st];
+      EQ4[4 * st] = q.qp_el; EQ4[4 * st + 1] = q.qp_vdw; EQ4[4 * st + 2] = q.qw_el; EQ4[4 * st + 3] = q.qw_vdw;
+    }
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 2;
+  }
+  return 0;
+}
+// shake(xx, x) on caller arrays (src/md.f90:17581-17679): 0 with *niter =
+// iterations summed over the molecules / number of molecules, 1 on a failure
+int oracle_shake(void* h, const double* xx, double* x, int* niter) {
+  auto* o = static_cast<Oracle*>(h);
+  std::vector<double> sx(xx, xx + o->x.size()), cx(x, x + o->x.size());
+  std::swap(sx, o->xx);
+  std::swap(cx, o->x);
+  const int n = o->shake();
+  std::swap(sx, o->xx);
+  std::swap(cx, o->x);
+  std::copy(cx.begin(), cx.end(), x);
+  if (n < 0) { g_err = "shake failure"; return 1; }
+  *niter = o->s.nmol > 0 ? n / o->s.nmol : 0;
+  return 0;
+}
+// the sizes write_out's layout depends on: waters, Q atoms, solute atoms
+void oracle_sizes(void* h, int* out) {
+  const auto& s = static_cast<Oracle*>(h)->s;
+  out[0] = s.nwat; out[1] = s.nqat; out[2] = s.nat_solute;
+}
 
 // run n steps; returns 0 on success
 int oracle_run(void* h, int n) {
@@ -1757,6 +1811,17 @@ int oracle_run(void* h, int n) {
   try {
     for (int k = 0; k < n; ++k)
       if (!o->step()) { g_err = o->err; return 1; }
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 2;
+  }
+  return 0;
+}
+// pot_energy at the current coordinates over the current lists
+int oracle_pot_energy(void* h) {
+  auto* o = static_cast<Oracle*>(h);
+  try {
+    o->pot_energy();
   } catch (const std::exception& e) {
     g_err = e.what();
     return 2;

@@ -28,6 +28,7 @@ __global__ void k_watpol_theta(const DevSys*);
 __global__ void k_watpol_force(const DevSys*);
 __global__ void k_watpol_gather(const DevSys*);
 __global__ void k_integrate(const DevSys*);
+__global__ void k_shake(const DevSys*, const double*, double*, int*);
 __global__ void k_temperature(const DevSys*, int);
 __global__ void k_energy_reduce(const DevSys*, int);
 template <typename P, int MAPPED> __global__ void k_solute_lists(const DevSys*, const int2*);
@@ -51,8 +52,47 @@ __global__ void k_far_apply(const DevSys*, const float*, const float*);
       throw std::runtime_error(std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// Device charge groups.  The solute kernels hold the atom pairs of two
+// groups as 8 x 8 bit masks, so a solute group of more than kMaxGroupAtoms
+// atoms is split into consecutive slices of at most kMaxGroupAtoms atoms.
+// Every slice keeps its group's switch atom, so that each list decision
+// (cut-off or LRF, nbpplist_lrf src/md.f90:7998-8188) is the group's, and
+// its group's atom range for the centre (cgp_centers, :2253-2288), so that
+// the LRF expansion of every slice is taken about the group's centre; the
+// slice's own atoms are the ones it brings to pairs and to the LRF sums of
+// other groups.  Water groups follow unchanged.
+struct DevGroups {
+  std::vector<int> first, last, sw, cfirst, clast, ref;   // per device group
+  std::vector<int> which;       // atom -> device group
+  std::vector<int> dev_first;   // reference group -> its first device group (+ end)
+  int nsolute = 0;
+};
+static DevGroups device_groups(const qsys::Setup& s) {
+  const qsys::Topology& t = s.top;
+  DevGroups G;
+  G.which.assign(std::max(1, s.natom), -1);
+  for (int g = 0; g < (int)t.cgp.size(); ++g) {
+    const auto& c = t.cgp[g];
+    G.dev_first.push_back((int)G.first.size());
+    const int step = g < t.ncgp_solute ? kMaxGroupAtoms : std::max(1, c.last - c.first + 1);
+    for (int f = c.first; f <= c.last; f += step) {
+      const int l = std::min(c.last, f + step - 1);
+      const int d = (int)G.first.size();
+      G.first.push_back(f); G.last.push_back(l); G.sw.push_back(c.iswitch);
+      G.cfirst.push_back(c.first); G.clast.push_back(c.last); G.ref.push_back(g);
+      for (int a = f; a <= l; ++a) G.which[t.cgpatom[a]] = d;
+    }
+    if (g + 1 == t.ncgp_solute) G.nsolute = (int)G.first.size();
+  }
+  G.dev_first.push_back((int)G.first.size());
+  for (int i = 0; i < s.natom; ++i)
+    if (G.which[i] < 0) G.which[i] = G.dev_first[s.iwhich_cgp[i]];
+  return G;
+}
+
 struct Batch::Inst {
   std::vector<void*> allocs;
+  DevGroups groups;
   template <typename T>
   T* alloc(size_t n) {
     void* p = nullptr;
@@ -124,7 +164,7 @@ static void initial_scales(const qsys::Setup& s, const double* vel, double* scal
 // the starting coordinates (the switch-atom / water-O distance tests of the
 // list kernels), counted over a uniform grid of cut-off sized cells.
 struct ListRows { int ww = 0, wp = 0, pw = 0; };
-static ListRows longest_rows(const qsys::Setup& s) {
+static ListRows longest_rows(const qsys::Setup& s, const DevGroups& G) {
   ListRows out;
   const qsys::Topology& t = s.top;
   if (s.nwat == 0) return out;
@@ -174,7 +214,8 @@ static ListRows longest_rows(const qsys::Setup& s) {
             if (water && m == self) continue;
             const int at = water ? s.nat_solute + 3 * m : t.cgp[m].iswitch;
             const double d0 = s.x[3 * at] - x[0], d1 = s.x[3 * at + 1] - x[1], d2 = s.x[3 * at + 2] - x[2];
-            if (d0 * d0 + d1 * d1 + d2 * d2 <= rc * rc) ++n;
+            // a split solute group stands in a water's row once per slice
+            if (d0 * d0 + d1 * d1 + d2 * d2 <= rc * rc) n += water ? 1 : G.dev_first[m + 1] - G.dev_first[m];
           }
         }
     return n;
@@ -215,8 +256,9 @@ void Batch::init(const std::vector<std::string>& inputs, int device) {
   for (const auto& f : inputs) setups_.push_back(new qsys::Setup(qsys::prepare(f)));
   for (auto* s : setups_) {
     max_nwat_ = std::max(max_nwat_, s->nwat);
-    max_ncgp_ = std::max(max_ncgp_, (int)s->top.cgp.size());
-    max_ncgp_solute_ = std::max(max_ncgp_solute_, s->top.ncgp_solute);
+    const DevGroups G = device_groups(*s);
+    max_ncgp_ = std::max(max_ncgp_, (int)G.first.size());
+    max_ncgp_solute_ = std::max(max_ncgp_solute_, G.nsolute);
     max_natom_ = std::max(max_natom_, s->natom);
     max_nat_solute_ = std::max(max_nat_solute_, s->nat_solute);
     max_nstates_ = std::max(max_nstates_, s->nstates);
@@ -370,15 +412,14 @@ void Batch::upload(int k) {
   const qsys::Setup& s = *setups_[k];
   const qsys::Topology& t = s.top;
   if (s.nstates > kMaxQStates) throw std::runtime_error("too many FEP states");
-  for (int g = 0; g < t.ncgp_solute; ++g)
-    if (t.cgp[g].last - t.cgp[g].first + 1 > kMaxGroupAtoms)
-      throw std::runtime_error("solute charge groups of more than 8 atoms are not supported");
   auto* I = new Inst();
   inst_.push_back(I);
+  I->groups = device_groups(s);
+  const DevGroups& G = I->groups;
   DevSys& D = h_sys_[k];
   std::memset(&D, 0, sizeof(DevSys));
   D.natom = s.natom; D.nat_solute = s.nat_solute; D.nwat = s.nwat;
-  D.ncgp = (int)t.cgp.size(); D.ncgp_solute = t.ncgp_solute;
+  D.ncgp = (int)G.first.size(); D.ncgp_solute = G.nsolute;
   D.nqat = s.nqat; D.nstates = s.nstates;
   D.precision = precision_;
   D.term_mask = TM_ALL;
@@ -403,13 +444,13 @@ void Batch::upload(int k) {
   D.mass = I->upload(mass);
   D.flags = I->upload(flags);
   D.iac = I->upload(s.iac);
-  std::vector<int> cf, cl, cs;
-  for (auto& g : t.cgp) { cf.push_back(g.first); cl.push_back(g.last); cs.push_back(g.iswitch); }
-  D.cgp_first = I->upload(cf);
-  D.cgp_last = I->upload(cl);
-  D.cgp_sw = I->upload(cs);
+  D.cgp_first = I->upload(G.first);
+  D.cgp_last = I->upload(G.last);
+  D.cgp_sw = I->upload(G.sw);
+  D.cgp_cfirst = I->upload(G.cfirst);
+  D.cgp_clast = I->upload(G.clast);
   D.cgpatom = I->upload(t.cgpatom);
-  D.iwhich_cgp = I->upload(s.iwhich_cgp);
+  D.iwhich_cgp = I->upload(G.which);
   D.lrf = I->alloc<double>((size_t)D.ncgp * kLrfStride);
   D.natyps = s.natyps;
   std::vector<double> av, bv;
@@ -443,33 +484,33 @@ void Batch::upload(int k) {
     D.exl_first = I->upload(first);
     D.exl_code = I->upload(code);
     std::vector<int> slot(std::max(1, s.nat_solute), 0);
-    for (int g = 0; g < t.ncgp_solute; ++g)
-      for (int a = t.cgp[g].first; a <= t.cgp[g].last; ++a) slot[t.cgpatom[a]] = a - t.cgp[g].first;
+    for (int g = 0; g < G.nsolute; ++g)
+      for (int a = G.first[g]; a <= G.last[g]; ++a) slot[t.cgpatom[a]] = a - G.first[g];
     D.cgp_slot = I->upload(slot);
   }
   // lists: row capacities from the longest rows of the starting
   // coordinates (twice that, for density fluctuations); rows that still
   // outgrow them are regrown at the failing list update (grow_lists)
   {
-    const ListRows r = longest_rows(s);
+    const ListRows r = longest_rows(s, G);
     auto cap = [](int longest, int most) {
       const int c = (2 * longest + 32 + 15) / 16 * 16;
       return std::max(1, std::min(most, c));
     };
     D.ww_cap = cap(r.ww, s.nwat);
-    D.wp_cap = cap(r.wp, t.ncgp_solute);
+    D.wp_cap = cap(r.wp, G.nsolute);
     D.pw_cap = cap(r.pw, s.nwat);
   }
-  D.pp_cap = std::max(1, t.ncgp_solute);
+  D.pp_cap = std::max(1, G.nsolute);
   D.ww_cnt = I->alloc<int>(s.nwat);
   D.ww_nb = I->alloc<int>((size_t)s.nwat * D.ww_cap);
   D.wp_cnt = I->alloc<int>(s.nwat);
   D.wp_nb = I->alloc<int>((size_t)s.nwat * D.wp_cap);
-  D.pp_cnt = I->alloc<int>(t.ncgp_solute);
-  D.pp_nb = I->alloc<int>((size_t)t.ncgp_solute * D.pp_cap);
-  D.pp_mask = I->alloc<ulonglong2>((size_t)t.ncgp_solute * D.pp_cap);
-  D.pw_cnt = I->alloc<int>(t.ncgp_solute);
-  D.pw_nb = I->alloc<int>((size_t)t.ncgp_solute * D.pw_cap);
+  D.pp_cnt = I->alloc<int>(G.nsolute);
+  D.pp_nb = I->alloc<int>((size_t)G.nsolute * D.pp_cap);
+  D.pp_mask = I->alloc<ulonglong2>((size_t)G.nsolute * D.pp_cap);
+  D.pw_cnt = I->alloc<int>(G.nsolute);
+  D.pw_nb = I->alloc<int>((size_t)G.nsolute * D.pw_cap);
   D.rcpp2 = s.inp.rcpp * s.inp.rcpp;
   D.rcpw2 = s.inp.rcpw * s.inp.rcpw;
   D.rcww2 = s.inp.rcww * s.inp.rcww;
@@ -658,7 +699,7 @@ void Batch::upload(int k) {
       qw_waters.resize(std::max<size_t>(qw_waters.size(), (size_t)s.nwat), 0);
     }
     D.qp_cand = I->upload(qp_cand);
-    D.qp_gin = I->alloc<unsigned char>(std::max(1, t.ncgp_solute));
+    D.qp_gin = I->alloc<unsigned char>(std::max(1, G.nsolute));
     for (int c = 0; c < 3; ++c) D.xpcent[c] = t.xpcent[c];
     D.qp_atoms = I->upload(qp_atoms);
     D.qw_waters = I->upload(qw_waters);
@@ -1607,6 +1648,60 @@ void Batch::set_state(int k, const double* x, const double* v) {
   if (x) refresh_xf();
 }
 
+int Batch::shake(int k, const double* xx, double* x) {
+  if (!dead_.empty()) throw std::runtime_error(dead_);
+  HIPCHECK(hipStreamSynchronize(stream_));
+  const DevSys& D = h_sys_[k];
+  const size_t n = 3 * (size_t)D.natom;
+  double* buf = nullptr;
+  HIPCHECK(hipMalloc(&buf, 2 * n * sizeof(double) + 2 * sizeof(int)));
+  int res[2] = {0, 0};
+  try {
+    int* cnt = reinterpret_cast<int*>(buf + 2 * n);
+    HIPCHECK(hipMemcpy(buf, xx, n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(buf + n, x, n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(cnt, res, sizeof(res), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_shake, dim3((D.nunits + 255) / 256), dim3(256), 0, stream_, d_sys_ + k, buf, buf + n, cnt);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(stream_));
+    HIPCHECK(hipMemcpy(x, buf + n, n * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(res, cnt, sizeof(res), hipMemcpyDeviceToHost));
+  } catch (...) {
+    (void)hipFree(buf);
+    throw;
+  }
+  HIPCHECK(hipFree(buf));
+  return res[1] ? -1 : res[0];
+}
+
+void Batch::pot_energy_nonbonds(int k, const double* x, double* d, double* e7, double* eq4) {
+  set_state(k, x, nullptr);
+  forces_terms(TM_WW | TM_PW | TM_PP | TM_QX | TM_LRF);
+  const size_t n = 3 * (size_t)h_sys_[k].natom;
+  std::vector<double> g(n);
+  get_gradient(k, g.data());
+  for (size_t i = 0; i < n; ++i) d[i] = d[i] + g[i];
+  double E[QGPU_E_COUNT];
+  energies(k, E, QGPU_E_COUNT);
+  const int pick[7] = {QGPU_E_PP_EL, QGPU_E_PP_VDW, QGPU_E_PW_EL, QGPU_E_PW_VDW,
+                       QGPU_E_WW_EL, QGPU_E_WW_VDW, QGPU_E_LRF};
+  for (int c = 0; c < 7; ++c) e7[c] = E[pick[c]];
+  // per-state Q-solute and Q-water sums of this evaluation's per-Q-atom
+  // energies (the Q-neighbour pairs of nonbond_qq, which pot_energy also
+  // adds into EQ%qp, are not part of pot_energy_nonbonds)
+  const int ns = setups_[k]->nstates, nq = h_sys_[k].nqat;
+  if (ns > 0 && eq4) {
+    std::vector<double> q((size_t)std::max(1, nq) * ns * 4, 0.0);
+    if (nq > 0) HIPCHECK(hipMemcpy(q.data(), h_sys_[k].e_q, 8 * q.size(), hipMemcpyDeviceToHost));
+    for (int st = 0; st < ns; ++st)
+      for (int c = 0; c < 4; ++c) {
+        double sum = 0.0;
+        for (int iq = 0; iq < nq; ++iq) sum += q[((size_t)iq * ns + st) * 4 + c];
+        eq4[4 * st + c] = sum;
+      }
+  }
+}
+
 void Batch::get_gradient(int k, double* d) {
   HIPCHECK(hipStreamSynchronize(stream_));
   HIPCHECK(hipMemcpy(d, h_sys_[k].d, 3 * (size_t)h_sys_[k].natom * 8, hipMemcpyDeviceToHost));
@@ -1698,39 +1793,70 @@ void Batch::export_list(int k, int which, std::vector<int>& out) {
           }
       }
   } else if (which == 3) {
-    // nbpp as the force kernel sees it: group list plus the device pair masks
+    // nbpp as the force kernel sees it: the device rows and pair masks, read
+    // back in reference groups (the slices of a split group joined)
     const qsys::Topology& t = s.top;
-    const int nc = t.ncgp_solute;
-    std::vector<int> cnt(nc), nb((size_t)nc * D.pp_cap);
-    std::vector<ulonglong2> mk((size_t)nc * D.pp_cap);
-    if (nc > 0) {
+    const DevGroups& G = inst_[k]->groups;
+    const int nd = G.nsolute;
+    std::vector<int> cnt(nd), nb((size_t)nd * D.pp_cap);
+    std::vector<ulonglong2> mk((size_t)nd * D.pp_cap);
+    if (nd > 0) {
       HIPCHECK(hipMemcpy(cnt.data(), D.pp_cnt, 4 * cnt.size(), hipMemcpyDeviceToHost));
       HIPCHECK(hipMemcpy(nb.data(), D.pp_nb, 4 * nb.size(), hipMemcpyDeviceToHost));
       HIPCHECK(hipMemcpy(mk.data(), D.pp_mask, sizeof(ulonglong2) * mk.size(), hipMemcpyDeviceToHost));
     }
-    for (int g = 0; g < nc; ++g)
-      for (int k2 = 0; k2 < cnt[g]; ++k2) {
-        const int other = nb[(size_t)g * D.pp_cap + k2];
-        const ulonglong2 m = mk[(size_t)g * D.pp_cap + k2];
+    // the mask of device pair (di, dj) from di's row
+    auto mask_of = [&](int di, int dj, ulonglong2& m) {
+      for (int k2 = 0; k2 < cnt[di]; ++k2)
+        if (nb[(size_t)di * D.pp_cap + k2] == dj) { m = mk[(size_t)di * D.pp_cap + k2]; return true; }
+      return false;
+    };
+    for (int g = 0; g < t.ncgp_solute; ++g) {
+      const int d0 = G.dev_first[g];
+      std::vector<int> others;   // reference groups of the first slice's row, in order
+      for (int k2 = 0; k2 < cnt[d0]; ++k2) {
+        const int o = G.ref[nb[(size_t)d0 * D.pp_cap + k2]];
+        if (others.empty() || others.back() != o) others.push_back(o);
+      }
+      const auto& cg = t.cgp[g];
+      for (int other : others) {
         if (!(other == g || ((g < other) == (((g + other) & 1) == 0)))) continue;
-        for (int a = 0; a < kMaxGroupAtoms; ++a)
-          for (int b = 0; b < kMaxGroupAtoms; ++b) {
-            const int bit = a * kMaxGroupAtoms + b;
-            if ((m.x >> bit) & 1ull) continue;
-            const int i = t.cgpatom[t.cgp[g].first + a], j = t.cgpatom[t.cgp[other].first + b];
+        const auto& co = t.cgp[other];
+        for (int a = cg.first; a <= cg.last; ++a)
+          for (int b = co.first; b <= co.last; ++b) {
+            const int i = t.cgpatom[a], j = t.cgpatom[b];
             if (other == g && i >= j) continue;
+            const int ra = a - cg.first, rb = b - co.first;
+            ulonglong2 m;
+            if (!mask_of(d0 + ra / kMaxGroupAtoms, G.dev_first[other] + rb / kMaxGroupAtoms, m)) continue;
+            const int bit = (ra % kMaxGroupAtoms) * kMaxGroupAtoms + rb % kMaxGroupAtoms;
+            if ((m.x >> bit) & 1ull) continue;
             out.push_back(i); out.push_back(j);
             out.push_back(((m.y >> bit) & 1ull) ? 3 : s.ljcod[s.iac[i] * s.natyps + s.iac[j]]);
           }
       }
+    }
   } else if (which == 0 || which == 1) {
     const qsys::Topology& t = s.top;
-    const int nc = t.ncgp_solute;
+    const DevGroups& G = inst_[k]->groups;
+    const int nd = G.nsolute, nc = t.ncgp_solute;
     const int cap = which == 0 ? D.pp_cap : D.pw_cap;
-    std::vector<int> cnt(nc), nb((size_t)nc * cap);
-    if (nc > 0) {
+    std::vector<int> cnt(nd), nb((size_t)nd * cap);
+    if (nd > 0) {
       HIPCHECK(hipMemcpy(cnt.data(), which == 0 ? D.pp_cnt : D.pw_cnt, 4 * cnt.size(), hipMemcpyDeviceToHost));
       HIPCHECK(hipMemcpy(nb.data(), which == 0 ? D.pp_nb : D.pw_nb, 4 * nb.size(), hipMemcpyDeviceToHost));
+    }
+    // the row of each reference group: its first slice's row, with the
+    // slices of split solute groups joined (every slice of a group shares
+    // the group's switch atom, so all of them stand in the same rows)
+    std::vector<std::vector<int>> row(nc);
+    for (int g = 0; g < nc; ++g) {
+      const int d0 = G.dev_first[g];
+      for (int k2 = 0; k2 < cnt[d0]; ++k2) {
+        int o = nb[(size_t)d0 * cap + k2];
+        if (which == 0) o = G.ref[o];
+        if (which == 1 || row[g].empty() || row[g].back() != o) row[g].push_back(o);
+      }
     }
     auto lj_of = [&](int i, int j) { return s.ljcod[s.iac[i] * s.natyps + s.iac[j]]; };
     auto zero = [&](int i, int j, int lj) {
@@ -1750,8 +1876,7 @@ void Batch::export_list(int k, int which, std::vector<int>& out) {
       return false;
     };
     for (int g = 0; g < nc; ++g)
-      for (int k2 = 0; k2 < cnt[g]; ++k2) {
-        int other = nb[(size_t)g * cap + k2];
+      for (int other : row[g]) {
         if (which == 0) {
           bool own = other == g || ((g < other) == (((g + other) & 1) == 0));
           if (!own) continue;
@@ -2076,6 +2201,26 @@ int qgpu_theta_corr(qgpu_handle* h, int k, double* out, int* n) {
     if (out) h->b->theta_corr(k, out);
   });
 }
+int qgpu_shake(qgpu_handle* h, int k, const double* xx, double* x, int* niter) {
+  return guarded([&] {
+    CHECK_K(h, k);
+    if (!xx || !x) throw std::runtime_error("null coordinate array");
+    const int nmol = h->b->setup(k).nmol;
+    const int total = h->b->shake(k, xx, x);
+    if (total < 0) throw std::runtime_error("shake failure");
+    if (niter) *niter = nmol > 0 ? total / nmol : 0;
+  });
+}
+int qgpu_pot_energy_nonbonds(qgpu_handle* h, int k, const double* x, double* d, double* E, double* EQ,
+                             int eq_cap) {
+  return guarded([&] {
+    CHECK_K(h, k);
+    if (!x || !d || !E) throw std::runtime_error("null array");
+    const int ns = h->b->setup(k).nstates;
+    if (ns > 0 && (!EQ || eq_cap < 4 * ns)) throw std::runtime_error("EQ needs 4 values per FEP state");
+    h->b->pot_energy_nonbonds(k, x, d, E, EQ);
+  });
+}
 int qgpu_last_step_ms(qgpu_handle* h, double* ms) {
   return guarded([&] {
     if (!h) throw std::runtime_error("null handle");
@@ -2133,6 +2278,28 @@ int qgpu_list_capacity(qgpu_handle* h, int k, int* caps, int* regrowths) {
 
 int qgpu_set_list_capacity(qgpu_handle* h, int k, int ww, int wp, int pw) {
   return guarded([&] { CHECK_K(h, k); h->b->set_list_caps(k, ww, wp, pw); });
+}
+
+int qgpu_offdiag(qgpu_handle* h, int k, double* out, int cap, int* n) {
+  return guarded([&] {
+    CHECK_K(h, k);
+    const qsys::Setup& s = h->b->setup(k);
+    const auto& od = s.offd;
+    *n = (int)od.size();
+    if (!out || od.empty()) return;
+    std::vector<double> x(3 * (size_t)s.natom);
+    h->b->get_state(k, x.data(), nullptr);
+    for (int m = 0; m < (int)od.size() && 6 * m + 5 < cap; ++m) {
+      // offdiag (src/md.f90:15444-15470) at the current coordinates
+      const int k3 = 3 * s.iqseq[od[m].k], l3 = 3 * s.iqseq[od[m].l];
+      const double r = std::sqrt((x[l3] - x[k3]) * (x[l3] - x[k3]) + (x[l3 + 1] - x[k3 + 1]) * (x[l3 + 1] - x[k3 + 1]) +
+                                 (x[l3 + 2] - x[k3 + 2]) * (x[l3 + 2] - x[k3 + 2]));
+      out[6 * m] = od[m].i + 1; out[6 * m + 1] = od[m].j + 1;
+      out[6 * m + 2] = od[m].k + 1; out[6 * m + 3] = od[m].l + 1;
+      out[6 * m + 4] = od[m].A * std::exp(-od[m].mu * r);
+      out[6 * m + 5] = r;
+    }
+  });
 }
 
 int qgpu_offdiag_states(qgpu_handle* h, int k, int* ij, int cap, int* n) {

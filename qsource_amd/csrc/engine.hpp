@@ -34,7 +34,7 @@ namespace qgpu {
 // F_QW: water (flag on its O) in the static Q-water list set (nbqwlist)
 enum : unsigned char { F_EXCL = 1, F_SHELL = 2, F_Q = 4, F_HEAVY = 8, F_QP = 16, F_QW = 32 };
 
-constexpr int kMaxGroupAtoms = 8;   // atoms per solute charge group handled per wavefront
+constexpr int kMaxGroupAtoms = 8;   // atoms per device solute group (larger groups are split)
 constexpr int kMaxQStates = 10;
 // wavefronts per Q atom in the Q-atom kernel, and the per-wavefront partial
 // record: gradient (3) and per-state qp_el qp_vdw qw_el qw_vdw
@@ -82,7 +82,7 @@ struct DevShell {
 };
 
 struct DevSys {
-  int natom, nat_solute, nwat, ncgp, ncgp_solute;
+  int natom, nat_solute, nwat, ncgp, ncgp_solute;   // ncgp*: device charge groups
   int nqat, nstates;
   int precision;                 // 0 = FP64, 1 = mixed FP32 pair / FP64 accumulation
   int term_mask;                 // TM_* terms a force evaluation includes (all but in term diagnostics)
@@ -107,7 +107,11 @@ struct DevSys {
   const double *crg, *winv, *mass;
   const unsigned char* flags;
   const int *iac;                // type index per atom
+  // device charge groups (solute groups of more than kMaxGroupAtoms atoms
+  // are split into slices): the group's atoms cgpatom[cgp_first..cgp_last],
+  // switch atom, and the range whose mean is the centre (the whole group)
   const int *cgp_first, *cgp_last, *cgp_sw, *cgpatom, *iwhich_cgp;
+  const int *cgp_cfirst, *cgp_clast;
   double* lrf;
   // type tables (natyps x 3 each), ljcod natyps x natyps
   int natyps;
@@ -325,6 +329,12 @@ class Batch {
   void work_counts(int k, double* out);
   double temperature(int k);
   void theta_corr(int k, double* out);
+  // shake(xx, x) of instance k on host arrays; returns the summed
+  // iterations of its SHAKE molecules, or -1 on a failure
+  int shake(int k, const double* xx, double* x);
+  // pot_energy_nonbonds of instance k at host coordinates x: d += gradient,
+  // e7 = pp, pw, ww (el, vdW) and LRF, eq4 = per state qp and qw (el, vdW)
+  void pot_energy_nonbonds(int k, const double* x, double* d, double* e7, double* eq4);
   int istep() const { return istep_; }
   // failure record of instance k: ST_* bits and the step they were found at
   void status(int k, int* bits, int* at_step);

@@ -11,17 +11,19 @@
 
 namespace qgpu {
 
-// Centre (mean of the group's atoms, as cgp_centers), charge and dipole of
-// the non-Q atoms, and the group frame.
+// Centre (mean of the group's atoms, as cgp_centers; the whole group's for
+// the slices of a split group), charge and dipole of the group's (slice's)
+// non-Q atoms, and the group frame.
 __device__ void far_group_frame(const DevSys& S, int g, double c[3], double& q, double er[3], double t[3],
                                 double& pr, double& pt, double& r) {
   const int f = S.cgp_first[g], l = S.cgp_last[g];
+  const int cf = S.cgp_cfirst[g], cl = S.cgp_clast[g];
   c[0] = c[1] = c[2] = 0.0;
-  for (int a = f; a <= l; ++a) {
+  for (int a = cf; a <= cl; ++a) {
     const int i = S.cgpatom[a];
     c[0] = c[0] + S.x[3 * i]; c[1] = c[1] + S.x[3 * i + 1]; c[2] = c[2] + S.x[3 * i + 2];
   }
-  const double nat = (double)(l - f + 1);
+  const double nat = (double)(cl - cf + 1);
   c[0] = c[0] / nat; c[1] = c[1] / nat; c[2] = c[2] / nat;
   double p[3] = {0.0, 0.0, 0.0};
   q = 0.0;

@@ -11,15 +11,16 @@
 namespace qgpu {
 
 // ---------------------------------------------------------------------------
-// cgp_centers (src/md.f90:2253-2288): centre of each charge group; LRF
-// multipoles zeroed (they are overwritten by the list kernels for every
-// group that is not excluded).
+// cgp_centers (src/md.f90:2253-2288): centre of each charge group (the
+// slices of a split group take the whole group's); LRF multipoles zeroed
+// (they are overwritten by the list kernels for every group that is not
+// excluded).
 __global__ void k_cgp_centers(const DevSys* __restrict__ sys) {
   const DevSys& S = sys[blockIdx.y];
   int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= S.ncgp) return;
   double c0 = 0, c1 = 0, c2 = 0;
-  int f = S.cgp_first[g], l = S.cgp_last[g];
+  int f = S.cgp_cfirst[g], l = S.cgp_clast[g];
   for (int a = f; a <= l; ++a) {
     int i3 = 3 * S.cgpatom[a];
     c0 = c0 + S.x[i3];
@@ -672,6 +673,62 @@ __device__ __forceinline__ void gather_slots(const DevSys& S, int a) {
   S.d[3 * a + 2] = S.d[3 * a + 2] + g2;
 }
 
+// shake (src/md.f90:17581-17679) of one molecule, constraints c0..c1 of
+// the instance's SHAKE tables (at most 64), on coordinates x with the
+// pre-step coordinates xx: returns the molecule's iteration count, or -1
+// when it reaches the iteration limit with a constraint still unmet.
+__device__ __forceinline__ int shake_molecule(const DevSys& S, double* __restrict__ x,
+                                              const double* __restrict__ xx, int c0, int c1) {
+  unsigned long long ready = 0ull;
+  const int nc = c1 - c0;
+  const unsigned long long all = nc >= 64 ? ~0ull : ((1ull << nc) - 1ull);
+  int nits = 0;
+  for (;;) {
+    for (int ic = 0; ic < nc; ++ic) {
+      if (ready & (1ull << ic)) continue;
+      const int i = S.sh_i[c0 + ic], j = S.sh_j[c0 + ic];
+      const double d2 = S.sh_d2[c0 + ic];
+      double* xi = x + 3 * i;
+      double* xj = x + 3 * j;
+      const double* xxi = xx + 3 * i;
+      const double* xxj = xx + 3 * j;
+      double a0_ = xi[0] - xj[0], a1_ = xi[1] - xj[1], a2_ = xi[2] - xj[2];
+      double xij2 = a0_ * a0_ + a1_ * a1_ + a2_ * a2_;
+      double diff = d2 - xij2;
+      if (fabs(diff) < qsys::kShakeTol * d2) ready |= 1ull << ic;
+      double b0 = xxi[0] - xxj[0], b1 = xxi[1] - xxj[1], b2 = xxi[2] - xxj[2];
+      double scp = a0_ * b0 + a1_ * b1 + a2_ * b2;
+      double wi = S.winv[i], wj = S.winv[j];
+      double corr = diff / (2.0 * scp * (wi + wj));
+      xi[0] = xi[0] + b0 * corr * wi;
+      xi[1] = xi[1] + b1 * corr * wi;
+      xi[2] = xi[2] + b2 * corr * wi;
+      xj[0] = xj[0] - b0 * corr * wj;
+      xj[1] = xj[1] - b1 * corr * wj;
+      xj[2] = xj[2] - b2 * corr * wj;
+    }
+    ++nits;
+    if ((ready & all) == all) return nits;
+    if (nits >= S.shake_maxiter) return -1;
+  }
+}
+
+// shake(xx, x) on arrays of one instance outside the step (the boundary
+// call that mirrors the reference's function): one thread per SHAKE unit;
+// out[0] accumulates the molecules' iteration counts, out[1] is set where a
+// molecule fails.
+__global__ __launch_bounds__(256) void k_shake(const DevSys* __restrict__ sys, const double* __restrict__ xx,
+                                               double* __restrict__ x, int* __restrict__ out) {
+  const DevSys& S = *sys;
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= S.nunits) return;
+  const int c0 = S.con_first[u], c1 = S.con_first[u + 1];
+  if (c1 <= c0) return;
+  const int n = shake_molecule(S, x, xx, c0, c1);
+  if (n < 0) atomicOr(out + 1, 1);
+  else atomicAdd(out, n);
+}
+
 __global__ __launch_bounds__(kIntegrateBlock) void k_integrate(const DevSys* __restrict__ sys) {
   const DevSys& S = sys[blockIdx.y];
   if (halted(S, PH_INTEGRATE)) return;
@@ -759,41 +816,7 @@ __global__ __launch_bounds__(kIntegrateBlock) void k_integrate(const DevSys* __r
       }
     }
     const int c0 = S.con_first[u], c1 = S.con_first[u + 1];
-    if (c1 > c0) {
-      // constraints of one molecule: at most 64 handled in registers/bits
-      unsigned long long ready = 0ull;
-      const int nc = c1 - c0;
-      const unsigned long long all = nc >= 64 ? ~0ull : ((1ull << nc) - 1ull);
-      int nits = 0;
-      for (;;) {
-        for (int ic = 0; ic < nc; ++ic) {
-          if (ready & (1ull << ic)) continue;
-          const int i = S.sh_i[c0 + ic], j = S.sh_j[c0 + ic];
-          const double d2 = S.sh_d2[c0 + ic];
-          double* xi = S.x + 3 * i;
-          double* xj = S.x + 3 * j;
-          const double* xxi = S.xx + 3 * i;
-          const double* xxj = S.xx + 3 * j;
-          double a0_ = xi[0] - xj[0], a1_ = xi[1] - xj[1], a2_ = xi[2] - xj[2];
-          double xij2 = a0_ * a0_ + a1_ * a1_ + a2_ * a2_;
-          double diff = d2 - xij2;
-          if (fabs(diff) < qsys::kShakeTol * d2) ready |= 1ull << ic;
-          double b0 = xxi[0] - xxj[0], b1 = xxi[1] - xxj[1], b2 = xxi[2] - xxj[2];
-          double scp = a0_ * b0 + a1_ * b1 + a2_ * b2;
-          double wi = S.winv[i], wj = S.winv[j];
-          double corr = diff / (2.0 * scp * (wi + wj));
-          xi[0] = xi[0] + b0 * corr * wi;
-          xi[1] = xi[1] + b1 * corr * wi;
-          xi[2] = xi[2] + b2 * corr * wi;
-          xj[0] = xj[0] - b0 * corr * wj;
-          xj[1] = xj[1] - b1 * corr * wj;
-          xj[2] = xj[2] - b2 * corr * wj;
-        }
-        ++nits;
-        if ((ready & all) == all) break;
-        if (nits >= S.shake_maxiter) { record_failure(S, ST_SHAKE, PH_INTEGRATE); break; }
-      }
-    }
+    if (c1 > c0 && shake_molecule(S, S.x, S.xx, c0, c1) < 0) record_failure(S, ST_SHAKE, PH_INTEGRATE);
     for (int k = a0; k < a1; ++k) {
       const int i = S.unit_atoms[k];
       double vv[3];

@@ -40,7 +40,8 @@ SYMBOLS = [
     "qgpu_trj_create", "qgpu_trj_natoms", "qgpu_trj_write", "qgpu_trj_close", "qgpu_write_restart",
     "qgpu_autotune", "qgpu_set_launch_shape",
     "qgpu_instance_status", "qgpu_list_capacity", "qgpu_set_list_capacity", "qgpu_pot_energy_terms",
-    "qgpu_work_counts", "qgpu_mlp_gradient_noalloc", "qgpu_system_sizes",
+    "qgpu_work_counts", "qgpu_mlp_gradient_noalloc", "qgpu_system_sizes", "qgpu_offdiag",
+    "qgpu_pot_energy_nonbonds", "qgpu_shake",
 ]
 TERM_GROUPS = {"ww": 1, "pw": 2, "pp": 4, "qx": 8, "lrf": 16, "other": 32}
 TUNED_KERNELS = ["water_lists", "solute_lists", "water_forces", "solute_forces"]
@@ -107,7 +108,10 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.qgpu_autotune.argtypes = [P, ctypes.c_int, dp, ip]
     lib.qgpu_set_launch_shape.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     lib.qgpu_pot_energy_terms.argtypes = [P, ctypes.c_int]
+    lib.qgpu_pot_energy_nonbonds.argtypes = [P, ctypes.c_int, dp, dp, dp, dp, ctypes.c_int]
+    lib.qgpu_shake.argtypes = [P, ctypes.c_int, dp, dp, ip]
     lib.qgpu_system_sizes.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+    lib.qgpu_offdiag.argtypes = [P, ctypes.c_int, dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     lib.qgpu_work_counts.argtypes = [P, ctypes.c_int, dp, ctypes.c_int]
     lib.qgpu_instance_status.argtypes = [P, ctypes.c_int, ip, ip]
     lib.qgpu_list_capacity.argtypes = [P, ctypes.c_int, ip, ip]
@@ -191,6 +195,28 @@ class Engine:
             self._check(self.lib.qgpu_pot_energy(self.h))
         else:
             self._check(self.lib.qgpu_pot_energy_terms(self.h, sum(TERM_GROUPS[t] for t in set(terms))))
+
+    def pot_energy_nonbonds(self, k: int, x: np.ndarray, d: np.ndarray):
+        """pot_energy_nonbonds of instance k at coordinates x (which become
+        its coordinates): adds the gradient into d (in place) and returns
+        (E, EQ): E = pp_el pp_vdw pw_el pw_vdw ww_el ww_vdw lrf, EQ
+        (nstates, 4) = qp_el qp_vdw qw_el qw_vdw."""
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        assert d.dtype == np.float64 and d.flags.c_contiguous and d.size == x.size == 3 * self.natom[k]
+        ns = self.nstates[k]
+        e = np.zeros(7)
+        eq = np.zeros(max(4 * ns, 1))
+        self._check(self.lib.qgpu_pot_energy_nonbonds(self.h, k, _dp(x), _dp(d), _dp(e), _dp(eq), 4 * ns))
+        return e, eq[:4 * ns].reshape(ns, 4)
+
+    def shake(self, k: int, xx: np.ndarray, x: np.ndarray) -> int:
+        """shake(xx, x) of instance k: constrains x in place and returns the
+        average iterations per molecule."""
+        xx = np.ascontiguousarray(xx, dtype=np.float64).ravel()
+        assert x.dtype == np.float64 and x.flags.c_contiguous and x.size == xx.size == 3 * self.natom[k]
+        n = ctypes.c_int(0)
+        self._check(self.lib.qgpu_shake(self.h, k, _dp(xx), _dp(x), ctypes.byref(n)))
+        return n.value
 
     def state(self, k: int = 0):
         x = np.zeros(3 * self.natom[k])
@@ -295,6 +321,15 @@ class Engine:
         self._check(self.lib.qgpu_monitor(self.h, k, _dp(out), len(out), ctypes.byref(n)))
         ns = max(self.nstates[k], 1)
         return out[:n.value].reshape(-1, 3 + 2 * ns) if n.value else np.zeros((0, 3 + 2 * ns))
+
+    def offdiag(self, k: int = 0) -> np.ndarray:
+        """(noffd, 6): state i, state j, Q atoms k, l (1-based), Hij and r_kl
+        of every off-diagonal function at the current coordinates."""
+        n = ctypes.c_int()
+        self._check(self.lib.qgpu_offdiag(self.h, k, None, 0, ctypes.byref(n)))
+        out = np.zeros(6 * max(1, n.value))
+        self._check(self.lib.qgpu_offdiag(self.h, k, _dp(out), len(out), ctypes.byref(n)))
+        return out[:6 * n.value].reshape(-1, 6)
 
     def offdiag_states(self, k: int = 0) -> np.ndarray:
         """(noffd, 2) 1-based state pairs (i, j) of the off-diagonal functions."""

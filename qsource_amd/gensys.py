@@ -392,7 +392,7 @@ def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
                     q_shake: bool = False, solvent_allatom: bool = False,
                     solute_rmax: Optional[float] = None, exclude_outside: bool = False,
                     exclude_water_radius: Optional[float] = None, hydrogens: bool = False,
-                    monitor: bool = False) -> System:
+                    monitor: bool = False, group_size: int = 4) -> System:
     """A TIP3P sphere of the given radius around an optional branched solute
     (charges N(0,q_sd) neutralised per 4-atom charge group, sigma~U[2.5,3.8],
     eps~U[0.05,0.2] quantised into at most n_types atom types), with an
@@ -416,7 +416,8 @@ def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
     (mass 1.008, sigma 2.4 A, eps 0.022, own atom type and bond type);
     monitor: three monitored atom groups (the Q atoms, the solute atoms
     bonded to them, and the first eight solute atoms) and their three
-    pairs."""
+    pairs; group_size: the most atoms of a solute charge group (connected
+    atoms in bond-tree order)."""
     rng = np.random.default_rng(seed)
     # --- solute
     if n_solute > 0:
@@ -429,7 +430,7 @@ def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
                 if attempt == 19:
                     raise
                 rmax *= 1.05
-        sx, sb, groups0 = _renumber_and_group(sx, sb)
+        sx, sb, groups0 = _renumber_and_group(sx, sb, size=group_size)
         angles0, tors0, dist = _graph_terms(n_solute, sb)
     else:
         sx = np.zeros((0, 3))

@@ -1,0 +1,73 @@
+"""The reference's basic test system (tests/basic_tests): a 27-atom ligand,
+every atom a Q atom, in a 16 A water sphere (prep_SPH, 3397 atoms), and the
+first equilibration input eq1.inp (100 steps of 0.1 fs from 0.1 K, LRF on,
+a sequence restraint over atoms 1-46).
+
+eq1.inp carries placeholders that the reference's run_test.sh replaces at
+run time (modify_inputs, tests/basic_tests/run_test.sh:176-189);
+`sph_input` makes the same replacements for the spherical boundary.  The
+data files (prep_SPH/lig_w.top, prep_SPH/lig_w.fep, eq1.inp) are kept
+beside this module as copied by make_golden.py; the reference's own tree
+is read in place where it is present.
+"""
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIXTURE = os.path.join(HERE, "prep_SPH")          # eq1.inp, lig_w.top, lig_w.fep
+REFERENCE = "/root/reference/tests/basic_tests"   # eq1.inp; prep_SPH/lig_w.{top,fep}
+FILES = ("eq1.inp", "lig_w.top", "lig_w.fep")
+
+# eval_test.sh:96-101: the Q-surr. row of the step-0 Q-atom energies of
+# eq1 that every SPH run must print
+INIT_QSURR_SPH = "Q-surr. 1 1.0000      3.12    139.43"
+
+
+def reference_present() -> bool:
+    return all(os.path.exists(p) for p in (os.path.join(REFERENCE, "eq1.inp"),
+                                           os.path.join(REFERENCE, "prep_SPH", "lig_w.top"),
+                                           os.path.join(REFERENCE, "prep_SPH", "lig_w.fep")))
+
+
+def sph_input(inp: str, data_dir: str, dest_dir: str, final: str = "eq1.re",
+              integrator: str = "leap-frog", thermostat: str = "berendsen") -> str:
+    """Writes `inp` into dest_dir as run_test.sh prepares it for the sphere:
+    the boundary becomes [sphere] with shell_radius 16 and shell_force 20,
+    the Q-atom and LRF cut-offs 99, and the integrator and thermostat the
+    given ones; the topology and FEP file are taken from data_dir by
+    absolute path and the final restart goes to dest_dir/final.  Returns
+    the path of the written input."""
+    t = open(inp).read()
+    subst = [("invalid-boundary", "[sphere]\nshell_radius\t16\nshell_force\t20"),
+             ("invalid-qatom", "q_atom\t99"), ("invalid-lrf", "lrf\t99"),
+             ("invalid-integrator", integrator), ("invalid-thermostat", thermostat)]
+    for a, b in subst:
+        assert a in t, a
+        t = t.replace(a, b)
+    out = []
+    for line in t.splitlines():
+        w = line.split()
+        if len(w) == 2 and w[0] in ("topology", "fep"):
+            line = "%-24s %s" % (w[0], os.path.join(os.path.abspath(data_dir), w[1]))
+        elif len(w) == 2 and w[0] == "final":
+            line = "%-24s %s" % (w[0], os.path.join(os.path.abspath(dest_dir), final))
+        out.append(line)
+    path = os.path.join(dest_dir, os.path.basename(inp))
+    with open(path, "w") as f:
+        f.write("\n".join(out) + "\n")
+    return path
+
+
+def benchmark_row(path: str, row: int = 1):
+    """Row `row` of a reference benchmark energy file (the lower and upper
+    bound of QEL QVdW EL VdW QBND QANG QTOR QIMP BND ANG TOR IMP TOT KIN
+    POT): returns {column: (lower, upper)}."""
+    names = None
+    for line in open(path):
+        w = line.split()
+        if line.startswith("#") and "QEL" in line:
+            names = w[1:]
+            continue
+        if w and not line.startswith("#") and int(w[0]) == row:
+            v = [float(t) for t in w[1:]]
+            return {n: (v[2 * i], v[2 * i + 1]) for i, n in enumerate(names)}
+    raise KeyError(row)

@@ -18,11 +18,17 @@ qsource_amd.gensys, and stores, per case:
   trajectory): the reference's restart, energy and trajectory files as
   written, for byte-exact checks of this package's writers.
 
+and, for the reference's own basic test system (prep_SPH, basic_tests.py),
+copies of its data files under prep_SPH/ and ``prep_sph_eq1.npz``: the
+summary blocks qdyn5 printed for eq1 (leap-frog, Berendsen, sphere) and
+its final restart.
+
 Usage:  python tests/golden/make_golden.py
 """
 import hashlib
 import os
 import re
+import shutil
 import subprocess
 import sys
 import tempfile
@@ -32,6 +38,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
 from qsource_amd import fep, gensys  # noqa: E402
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+import basic_tests as bt  # noqa: E402
 
 QDYN = os.path.join(ROOT, "oracle", "_ref", "qdyn5")
 OUT = os.path.dirname(os.path.abspath(__file__))
@@ -149,6 +157,12 @@ CASES = {
                                      dict(steps=30, lambdas=[0.6, 0.4], stepsize=0.5, temperature=50.0,
                                           bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
                                           energy_interval=1)),
+    # solute charge groups of up to 12 atoms (the device splits those of
+    # more than 8 into slices), one of them partly in the Q region
+    "c14_r14_q8s2_group12_30": (dict(radius=14.0, n_solute=80, seed=3201, n_q=8, n_states=2, group_size=12),
+                                dict(steps=30, lambdas=[0.6, 0.4], stepsize=0.5, temperature=50.0,
+                                     bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
+                                     energy_interval=1)),
     # a breaking bond with non-Q neighbours: topology torsions through it
     # become Q terms (qatom_load_fep's completion); periodic impropers
     "c6_r12_q6s2_qbreak_30": (dict(radius=12.0, n_solute=40, seed=2202, n_q=6, n_states=2, q_types=True,
@@ -219,10 +233,82 @@ def build_case(name, sysargs, inpargs, workdir):
     return s, paths
 
 
+# Cases whose printed energy summaries (write_out, src/md.f90:18545-18691)
+# are kept as text: every block from its header to its closing line.
+WRITE_OUT_CASES = {
+    "c13_r12_q8s4_writeout_12": (dict(radius=12.0, n_solute=40, seed=3101, n_q=8, n_states=4, q_types=True,
+                                      softcore=20.0, n_offdiag=3, impropers=1, monitor=True),
+                                 dict(steps=12, lambdas=[0.4, 0.3, 0.2, 0.1], stepsize=0.5, temperature=50.0,
+                                      bath_coupling=2.0, initial_temperature=50.0)),
+}
+
+
+def summary_blocks(log: str):
+    """The text of every write_out block: from an 'Energy summary' header
+    line to the 79-character closing line of '='."""
+    lines = log.splitlines()
+    out, cur = [], None
+    for l in lines:
+        if re.match(r"^=+ (FINAL )?\s*Energy summary", l):
+            cur = [l]
+            continue
+        if cur is not None:
+            cur.append(l)
+            if l == "=" * 79:
+                out.append("\n".join(cur))
+                cur = None
+    return out
+
+
+def make_write_out_cases(only):
+    for name, (sysargs, inpargs) in WRITE_OUT_CASES.items():
+        if only and name not in only:
+            continue
+        with tempfile.TemporaryDirectory() as d:
+            s, paths = build_case(name, sysargs, inpargs, d)
+            r = subprocess.run([QDYN, name + ".inp"], cwd=d, stdout=subprocess.PIPE,
+                               stderr=subprocess.STDOUT, text=True)
+            if "terminated normally" not in r.stdout:
+                sys.exit("reference run failed for %s:\n%s" % (name, r.stdout[-3000:]))
+            blocks = summary_blocks(r.stdout)
+            sha = hashlib.sha256(open(paths["top"], "rb").read()).hexdigest()
+            np.savez_compressed(os.path.join(OUT, name + ".npz"), blocks=np.array(blocks), top_sha=np.array(sha))
+            print(name, len(blocks), "blocks", sha[:12])
+
+
+def make_prep_sph(only):
+    """Copies the basic test system's data files beside basic_tests.py and
+    runs reference qdyn5 on eq1 as run_test.sh prepares it for the sphere
+    with leap-frog and Berendsen: keeps every printed summary block and the
+    final restart."""
+    name = "prep_sph_eq1"
+    if only and name not in only:
+        return
+    if not bt.reference_present():
+        sys.exit("reference basic tests not found under " + bt.REFERENCE)
+    os.makedirs(bt.FIXTURE, exist_ok=True)
+    shutil.copyfile(os.path.join(bt.REFERENCE, "eq1.inp"), os.path.join(bt.FIXTURE, "eq1.inp"))
+    for f in ("lig_w.top", "lig_w.fep"):
+        shutil.copyfile(os.path.join(bt.REFERENCE, "prep_SPH", f), os.path.join(bt.FIXTURE, f))
+    with tempfile.TemporaryDirectory() as d:
+        inp = bt.sph_input(os.path.join(bt.FIXTURE, "eq1.inp"), bt.FIXTURE, d)
+        r = subprocess.run([QDYN, inp], cwd=d, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        if "terminated normally" not in r.stdout:
+            sys.exit("reference run failed for %s:\n%s" % (name, r.stdout[-3000:]))
+        blocks = summary_blocks(r.stdout)
+        x, v, tc = gensys.read_restart(os.path.join(d, "eq1.re"))
+        sha = hashlib.sha256(open(os.path.join(bt.FIXTURE, "lig_w.top"), "rb").read()).hexdigest()
+        np.savez_compressed(os.path.join(OUT, name + ".npz"), blocks=np.array(blocks), x=x, v=v,
+                            theta_corr=tc, top_sha=np.array(sha))
+        print(name, len(blocks), "blocks", x.shape, sha[:12])
+
+
 def main():
     if not os.path.exists(QDYN):
         sys.exit("reference qdyn5 not built: make -f oracle/Makefile.ref OUT=oracle/_ref")
     only = set(sys.argv[1:])   # case names to (re)make; all when none given
+    make_write_out_cases(only)
+    make_prep_sph(only)
     for name, (sysargs, inpargs) in CASES.items():
         if only and name not in only:
             continue

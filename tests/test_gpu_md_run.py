@@ -72,3 +72,51 @@ def test_run_writes_qdyn_files(tmp_path):
     for t in ("q_bond", "rst_protein", "rst_radial", "qx_el", "qx_vdw", "lrf", "pp_el", "ww_el", "pw_vdw"):
         assert abs(en[t] - fin[0][t]) <= 1e-9 * max(1.0, abs(fin[0][t])), (t, en[t], fin[0][t])
     e2.close()
+
+
+WO_NAME = "c13_r12_q8s4_writeout_12"
+WO_SYSARGS = dict(radius=12.0, n_solute=40, seed=3101, n_q=8, n_states=4, q_types=True, softcore=20.0,
+                  n_offdiag=3, impropers=1, monitor=True)
+WO_INPARGS = dict(steps=12, lambdas=[0.4, 0.3, 0.2, 0.1], stepsize=0.5, temperature=50.0, bath_coupling=2.0,
+                  initial_temperature=50.0)
+
+
+def _numbers(block):
+    import re
+    return [float(t) for t in re.findall(r"-?\d+\.\d+", block)]
+
+
+def test_run_prints_energy_summaries(tmp_path):
+    """The run loop's printed summaries (FP64 engine) against the blocks
+    reference qdyn printed for the same run: the same lines, and every
+    printed number within 0.01 of the reference's (the engine's energies
+    agree with the reference's to 1e-9, so only a value on a rounding
+    boundary of the last digit can print differently)."""
+    import io
+    print("case: md.run FP64 printed summaries, 4 states + off-diagonals + monitored pairs, 12 steps", flush=True)
+    g = np.load(os.path.join(GOLD, WO_NAME + ".npz"), allow_pickle=False)
+    ref = [str(b) for b in g["blocks"]]
+    s = gensys.solvated_system(**WO_SYSARGS)
+    paths = gensys.write_system(s, str(tmp_path), WO_NAME, output=1, temp_interval=1, **WO_INPARGS)
+    log = io.StringIO()
+    md.run([paths["inp"]], qe.QGPU_FP64, logs=[log])
+    text = log.getvalue()
+    ours = []
+    cur = None
+    for line in text.splitlines():
+        if "Energy summary" in line and line.startswith("="):
+            cur = [line]
+            continue
+        if cur is not None:
+            cur.append(line)
+            if line == "=" * 79:
+                ours.append("\n".join(cur))
+                cur = None
+    assert len(ours) == len(ref) == 13
+    for k, (a, b) in enumerate(zip(ours, ref)):
+        la, lb = a.splitlines(), b.splitlines()
+        assert len(la) == len(lb), k
+        for x, y in zip(la, lb):
+            assert len(x) == len(y) and x.split()[:1] == y.split()[:1], (k, x, y)
+        na, nb = _numbers(a), _numbers(b)
+        assert len(na) == len(nb) and np.allclose(na, nb, rtol=0, atol=0.0100001), (k, a, b)

@@ -126,6 +126,12 @@ CASES = {
                                      dict(steps=30, lambdas=[0.6, 0.4], stepsize=0.5, temperature=50.0,
                                           bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
                                           energy_interval=1)),
+    # solute charge groups of up to 12 atoms (the device splits those of
+    # more than 8 into slices), one of them partly in the Q region
+    "c14_r14_q8s2_group12_30": (dict(radius=14.0, n_solute=80, seed=3201, n_q=8, n_states=2, group_size=12),
+                                dict(steps=30, lambdas=[0.6, 0.4], stepsize=0.5, temperature=50.0,
+                                     bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
+                                     energy_interval=1)),
     # a breaking bond with non-Q neighbours: topology torsions through it
     # become Q terms (qatom_load_fep's completion); periodic impropers
     "c6_r12_q6s2_qbreak_30": (dict(radius=12.0, n_solute=40, seed=2202, n_q=6, n_states=2, q_types=True,

@@ -1,0 +1,112 @@
+"""The oracle on the reference's own basic test system (tests/basic_tests:
+prep_SPH, eq1.inp), prepared as run_test.sh does for the sphere with
+leap-frog and Berendsen (run_test.sh:176-189).
+
+* Read in place from the reference's tree (skipped where it is absent): the
+  step-0 Q-surr. row equals eval_test.sh's known answer
+  (eval_test.sh:96-101), every column that is exact in row 1 of the six
+  SPH_*_benchmark.en files (lower bound = upper bound: QEL QVdW EL VdW QBND
+  QANG QTOR QIMP BND ANG TOR IMP) equals the oracle's step-0 value at the
+  printed precision, and the step-0 potential lies within that row's POT
+  bounds.  Row 1 is the first summary the reference's scripts grep from
+  eq1.log: step 0 (eval_test.sh:189-198).
+* From the copies kept in tests/golden/prep_SPH (same bytes as the
+  reference's files): 100 steps and the closing energy evaluation give the
+  final restart (x, v, water-polarisation theta_corr) bit for bit as the
+  reference's serial qdyn5 wrote it, and md.write_out of the oracle's
+  energies reproduces every summary block qdyn5 printed, byte for byte.
+"""
+import filecmp
+import hashlib
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import Oracle
+from qsource_amd import md
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+sys.path.insert(0, GOLD)
+import basic_tests as bt  # noqa: E402
+
+EXACT = ["QEL", "QVdW", "EL", "VdW", "QBND", "QANG", "QTOR", "QIMP", "BND", "ANG", "TOR", "IMP"]
+SPH_BENCH = ["SPH_%s_%s_benchmark.en" % (i, t) for i in ("leap-frog", "velocity-verlet")
+             for t in ("berendsen", "langevin", "nose-hoover")]
+
+needs_reference = pytest.mark.skipif(not bt.reference_present(), reason="reference basic tests not present")
+
+
+def _step0(inp):
+    """The oracle's step-0 energies as md_run prints them: pot_energy on the
+    start coordinates, kinetic energy after the first update."""
+    o = Oracle(inp)
+    o.run(1)
+    out = o.energies(), o.qenergies(), o.sizes()
+    o.close()
+    return out
+
+
+def _columns(E, EQ):
+    """The benchmark columns, formed as eval_test.sh forms them from the
+    printed rows (eval_test.sh:189-198)."""
+    return dict(QEL=EQ[0, 2] + EQ[0, 4], QVdW=EQ[0, 3] + EQ[0, 5],
+                EL=E["pp_el"] + E["ww_el"] + E["pw_el"], VdW=E["pp_vdw"] + E["ww_vdw"] + E["pw_vdw"],
+                QBND=E["q_bond"], QANG=E["q_angle"], QTOR=E["q_torsion"], QIMP=E["q_improper"],
+                BND=E["p_bond"], ANG=E["p_angle"], TOR=E["p_torsion"], IMP=E["p_improper"],
+                POT=E["potential"])
+
+
+@needs_reference
+def test_sph_initial_energy_in_place(tmp_path):
+    inp = bt.sph_input(os.path.join(bt.REFERENCE, "eq1.inp"), os.path.join(bt.REFERENCE, "prep_SPH"),
+                       str(tmp_path))
+    E, EQ, z = _step0(inp)
+    text = md.write_out(E, EQ, [1.0], step=0, final=False, use_lrf=True, **z)
+    qsurr = [l for l in text.splitlines() if l.startswith("Q-surr.")]
+    assert qsurr == [bt.INIT_QSURR_SPH], qsurr
+    col = _columns(E, EQ)
+    for f in SPH_BENCH:
+        row = bt.benchmark_row(os.path.join(bt.REFERENCE, f), 1)
+        for c in EXACT:
+            lo, hi = row[c]
+            assert lo == hi, (f, c)
+            assert "%.2f" % col[c] == "%.2f" % lo or (lo == 0 and abs(col[c]) < 0.005), (f, c, col[c], lo)
+        lo, hi = row["POT"]
+        assert lo <= col["POT"] <= hi, (f, col["POT"], lo, hi)
+
+
+@needs_reference
+def test_fixture_copies_are_the_reference_files():
+    assert filecmp.cmp(os.path.join(bt.FIXTURE, "eq1.inp"), os.path.join(bt.REFERENCE, "eq1.inp"), shallow=False)
+    for f in ("lig_w.top", "lig_w.fep"):
+        assert filecmp.cmp(os.path.join(bt.FIXTURE, f), os.path.join(bt.REFERENCE, "prep_SPH", f), shallow=False)
+
+
+def test_sph_eq1_run_matches_reference_qdyn(tmp_path):
+    g = np.load(os.path.join(GOLD, "prep_sph_eq1.npz"), allow_pickle=False)
+    top = os.path.join(bt.FIXTURE, "lig_w.top")
+    assert hashlib.sha256(open(top, "rb").read()).hexdigest() == str(g["top_sha"])
+    blocks = [str(b) for b in g["blocks"]]
+    inp = bt.sph_input(os.path.join(bt.FIXTURE, "eq1.inp"), bt.FIXTURE, str(tmp_path))
+    s = md.read_settings(inp)
+    nsteps, iout = s["steps"], s["iout"]
+    assert (nsteps, iout) == (100, 50) and len(blocks) == nsteps // iout + 1
+    o = Oracle(inp)
+    z = o.sizes()
+    assert (o.natom, z["nat_solute"], z["nwat"], o.nstates) == (3397, 46, 1117, 1)
+    printed = []
+    for k in range(nsteps):
+        o.run(1)
+        if k % iout == 0:
+            printed.append(md.write_out(o.energies(), o.qenergies(), [1.0], step=k, final=False, **z))
+    o.final_energy()
+    printed.append(md.write_out(o.energies(), o.qenergies(), [1.0], step=nsteps, final=True, **z))
+    for k, (a, b) in enumerate(zip(printed, blocks)):
+        assert a == b, "block %d:\n%s\n---- reference ----\n%s" % (k, a, b)
+    x, v = o.state()
+    assert np.array_equal(x, g["x"].ravel()) and np.array_equal(v, g["v"].ravel())
+    assert np.array_equal(o.theta_corr(), g["theta_corr"])
+    assert np.abs(g["theta_corr"]).max() > 0   # the istep = 100 update happened
+    o.close()
