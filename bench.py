@@ -395,6 +395,8 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--accuracy-steps", type=int, default=0,
                     help="long accuracy run: mixed and FP64 batches of this many steps (no timing report)")
+    ap.add_argument("--accuracy-control", action="store_true",
+                    help="with --accuracy-steps: also a mixed and an FP64 batch from other initial velocities")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU.  The lambda windows are independent and would be "
@@ -413,7 +415,7 @@ def main():
             inputs, groups, lams, cfg = build_c5(d)
         n = len(inputs)
         if args.accuracy_steps > 0:
-            accuracy_run(inputs, groups, lams, cfg, args.accuracy_steps)
+            accuracy_run(inputs, groups, lams, cfg, args.accuracy_steps, control=args.accuracy_control)
             return
         cpu = None
         if args.cpu_seconds > 0 and args.config == "c3":
@@ -468,15 +470,21 @@ def main():
         long_name = "ddG_%s_long_run.md" % args.config
         long_run = profile_json(long_name)
         this = acc.get("trajectory_this_run")
+        # the gate on identical frames: the error the mixed arithmetic adds
+        acc["gate_ddG_lt_0.05_precision_only"] = bool(
+            max(acc["precision_only"]["abs_ddG_zwanzig"] + acc["precision_only"]["abs_ddG_bar"]) < DDG_GATE)
+        # the gate on independent trajectories, where sampling noise enters
         if this and this["gate_established"]:
             gate, source = this["gate_pass"], "this run"
         elif long_run:
             gate, source = long_run.get("gate_pass"), "profiles/" + long_name
-            acc["trajectory_long_run"] = long_run
+            acc["trajectory_long_run"] = {k: long_run.get(k) for k in (
+                "steps_per_window", "ddG_zwanzig", "stderr_ddG_zwanzig", "rms_ddG_cross_precision",
+                "rms_ddG_same_precision", "gate_established", "gate_note")}
         else:
             gate, source = None, "unestablished (no run long enough)"
-        acc["gate_ddG_lt_0.05"] = gate
-        acc["gate_source"] = source
+        acc["gate_ddG_lt_0.05_trajectories"] = gate
+        acc["gate_trajectories_source"] = source
         # plain FP64 engine, one window at a time
         pw = groups[min(1, len(groups) - 1)]
         ep, wallp, _ = run_batch([inputs[pw[len(pw) // 2]]], qe.QGPU_FP64, max(200, args.steps // 4), 50)
@@ -494,22 +502,58 @@ def main():
     print(json.dumps(out))
 
 
-def accuracy_run(inputs, groups, lams, cfg, steps):
+def reseeded(inputs, offset):
+    """Copies of the window inputs (same restarts) whose initial velocities
+    come from other seeds: random_seed + offset."""
+    out = []
+    for p in inputs:
+        lines = open(p).read().splitlines()
+        for i, line in enumerate(lines):
+            w = line.split()
+            if len(w) == 2 and w[0] == "random_seed":
+                lines[i] = "%-24s %d" % ("random_seed", int(w[1]) + offset)
+        q = p[:-len(".inp")] + "_b.inp"
+        with open(q, "w") as f:
+            f.write("\n".join(lines) + "\n")
+        out.append(q)
+    return out
+
+
+def accuracy_run(inputs, groups, lams, cfg, steps, control=False):
     """Long mixed and FP64 batches of the workload: the trajectory ddG with
     its bootstrap standard error, printed as one JSON line (committed as the
-    long-run summary that the short benchmark cites)."""
+    long-run summary that the short benchmark cites).  With `control`, a
+    second mixed and a second FP64 batch start from other initial
+    velocities (B): the ddG between runs of the same precision (FP64 B - FP64
+    A, mixed B - mixed A) measures the sampling noise that the cross-precision
+    ddG (mixed - FP64, A and B) is to be compared with."""
     from qsource_amd import engine as qe
     res = {"workload": cfg["workload"], "steps_per_window": steps, "energy_interval": IENE}
     sers = {}
-    for name, prec in (("mixed", qe.QGPU_MIXED), ("fp64", qe.QGPU_FP64)):
+    runs = [("mixed", qe.QGPU_MIXED, inputs), ("fp64", qe.QGPU_FP64, inputs)]
+    if control:
+        other = reseeded(inputs, 500)
+        runs += [("mixed_b", qe.QGPU_MIXED, other), ("fp64_b", qe.QGPU_FP64, other)]
+    for name, prec, inp in runs:
         log("accuracy run: %s batch, %d steps" % (name, steps))
-        e, wall, warm = run_batch(inputs, prec, steps, 200, chunk=20000)
-        res["%s_ns_per_day" % name] = len(inputs) * steps * 1e-6 * 86400.0 / wall
+        e, wall, warm = run_batch(inp, prec, steps, 200, chunk=20000)
+        res["%s_ns_per_day" % name] = len(inp) * steps * 1e-6 * 86400.0 / wall
         sers[name] = series(e, groups, warm // IENE)
         e.close()
         log("  %.1f s" % wall)
     tr = trajectory_ddg(sers["mixed"], sers["fp64"], lams)
     res.update(tr)
+    if control:
+        pairs = {"mixed_b-fp64_b": ("mixed_b", "fp64_b"), "fp64_b-fp64": ("fp64_b", "fp64"),
+                 "mixed_b-mixed": ("mixed_b", "mixed")}
+        res["control"] = {}
+        for key, (a, b) in pairs.items():
+            c = trajectory_ddg(sers[a], sers[b], lams)
+            res["control"][key] = {"ddG_zwanzig": c["ddG_zwanzig"], "stderr_ddG_zwanzig": c["stderr_ddG_zwanzig"]}
+        same = [x for key in ("fp64_b-fp64", "mixed_b-mixed") for x in res["control"][key]["ddG_zwanzig"]]
+        cross = tr["ddG_zwanzig"] + res["control"]["mixed_b-fp64_b"]["ddG_zwanzig"]
+        res["rms_ddG_same_precision"] = float(np.sqrt(np.mean(np.square(same))))
+        res["rms_ddG_cross_precision"] = float(np.sqrt(np.mean(np.square(cross))))
     print(json.dumps(res))
 
 

@@ -26,6 +26,7 @@ template <typename P, int MAPPED> __global__ void k_water_lists(const DevSys*, c
 template <typename P, int MAPPED, int PF, int W, int AA> __global__ void k_water_forces(const DevSys*, const int2*, int);
 __global__ void k_watpol_theta(const DevSys*);
 __global__ void k_watpol_force(const DevSys*);
+__global__ void k_watpol_shell(const DevSys*);
 __global__ void k_watpol_gather(const DevSys*);
 __global__ void k_integrate(const DevSys*);
 __global__ void k_shake(const DevSys*, const double*, double*, int*);
@@ -979,6 +980,8 @@ void Batch::upload(int k) {
   D.e_water = I->alloc<double>((size_t)std::max(1, s.nwat) * kEWater);
   D.e_solute = I->alloc<double>((size_t)std::max(1, s.nat_solute) * 8);
   D.e_wpol = I->alloc<double>(std::max(1, D.nshell));
+  D.wpol_part = I->alloc<double>((size_t)std::max(1, D.nshell) * kWpolSplit * 3);
+  max_nshell_ = std::max(max_nshell_, D.nshell);
   D.e_far = I->alloc<double>(std::max(1, D.ncgp));
   I->ke_blocks = 0;
   std::vector<double> scal(8, 0.0);
@@ -1294,8 +1297,13 @@ void Batch::launch_forces(bool step, long s, bool energies) {
     pb(3); launch_tuned(TK_WATER_FORCES, shape_[TK_WATER_FORCES]); pe(3);
     dim3 g2((max_nwat_ + 255) / 256, ni);
     { pb(8); hipLaunchKernelGGL(k_watpol_theta, g2, dim3(256), 0, stream_, d_sys_); pe(8); }
-    dim3 g3(kNumShellMax, ni);
-    { pb(9); hipLaunchKernelGGL(k_watpol_force, g3, dim3(kWpolBlock), 0, stream_, d_sys_); pe(9); }
+    if (max_nshell_ > 0) {
+      dim3 g3(max_nshell_ * kWpolSplit, ni);
+      pb(9);
+      hipLaunchKernelGGL(k_watpol_force, g3, dim3(kWpolBlock), 0, stream_, d_sys_);
+      hipLaunchKernelGGL(k_watpol_shell, dim3(1, ni), dim3(64), 0, stream_, d_sys_);
+      pe(9);
+    }
   }
   ls_ = side;
   join();

@@ -71,7 +71,8 @@ constexpr int kQwRec = 8;      // Q-water record per Q atom (qw_rec_d / qw_rec_f
 constexpr int kEWater = 5;     // energy partials per water
 constexpr int kMon = 3;        // Vwsum, Vwel, Vwlj before the per-state values
 constexpr int kIntegrateBlock = 64;  // threads per k_integrate workgroup
-constexpr int kWpolBlock = 256;      // threads per k_watpol_force workgroup (one per shell)
+constexpr int kWpolBlock = 256;      // threads per k_watpol_force workgroup
+constexpr int kWpolSplit = 8;        // k_watpol_force workgroups per water shell
 // lanes per water in k_water_lists: FP64 build, FP32 (mixed) build
 constexpr int kWLLanes64 = 8, kWLLanes32 = 16;
 
@@ -261,6 +262,7 @@ struct DevSys {
   double* e_water;     // nwat x kEWater : ww_el, ww_vdw, lrf(water atoms), radial, excluded-atom restraint
   double* e_solute;    // nat_solute x 8 : pp_el pp_vdw pw_el pw_vdw lrf fix shell spare
   double* e_wpol;      // nshell
+  double* wpol_part;   // nshell x kWpolSplit x 3: energy, theta sum, member count of each k_watpol_force workgroup
   double* ke_part;     // kinetic partial sums: nblocks x 6
   int ke_blocks;
   double* scal;        // [0]Tscale_solute [1]Tscale_solvent [2]Tfree [3]Temp [4]Ekin [5..] per-class temps
@@ -459,6 +461,7 @@ class Batch {
   int list_regrowths_ = 0;
   int precision_ = 0;
   int max_nwat_ = 0, max_ncgp_ = 0, max_ncgp_solute_ = 0, max_units_ = 0, max_natom_ = 0;
+  int max_nshell_ = 0;
   int max_nat_solute_ = 0, max_nstates_ = 0;
   int max_nqat_ = 0, max_terms_ = 0, max_slot_atoms_ = 0;
 };

@@ -235,6 +235,13 @@ struct LrfTerms {
     for (int k = 0; k < 6; ++k) p2[k] = 0;
     for (int k = 0; k < 10; ++k) p3[k] = 0;
   }
+  // component-wise sum with another record
+  __device__ void add(const LrfTerms& o) {
+    p0 = p0 + o.p0;
+    for (int k = 0; k < 3; ++k) p1[k] = p1[k] + o.p1[k];
+    for (int k = 0; k < 6; ++k) p2[k] = p2[k] + o.p2[k];
+    for (int k = 0; k < 10; ++k) p3[k] = p3[k] + o.p3[k];
+  }
   __device__ void add(T q, T d0, T d1, T d2) {
 #pragma clang fp contract(fast)
     const T r2 = d0 * d0 + d1 * d1 + d2 * d2;
@@ -368,6 +375,18 @@ struct LrfRaw {
   }
 };
 
+// The FP32 raw moments of the non-Q atoms of charge group g (as seen from
+// centre c), added to part.
+__device__ __forceinline__ void lrf_raw_group(const DevSys& S, int g, double c0, double c1, double c2,
+                                              LrfRaw<float>& part) {
+  for (int a = S.cgp_first[g]; a <= S.cgp_last[g]; ++a) {
+    const int i = S.cgpatom[a];
+    if (S.flags[i] & F_Q) continue;
+    part.add((float)S.crg[i], (float)(S.x[3 * i] - c0), (float)(S.x[3 * i + 1] - c1),
+             (float)(S.x[3 * i + 2] - c2));
+  }
+}
+
 // Add the non-Q atoms of charge group g (as seen from centre c) to acc.
 template <typename P>
 __device__ __forceinline__ void lrf_gather_group(const DevSys& S, int g, double c0, double c1,
@@ -383,12 +402,7 @@ __device__ __forceinline__ void lrf_gather_group(const DevSys& S, int g, double 
     // components), accumulated in FP64
     LrfRaw<float> part;
     part.zero();
-    for (int a = S.cgp_first[g]; a <= S.cgp_last[g]; ++a) {
-      const int i = S.cgpatom[a];
-      if (S.flags[i] & F_Q) continue;
-      part.add((float)S.crg[i], (float)(S.x[3 * i] - c0), (float)(S.x[3 * i + 1] - c1),
-               (float)(S.x[3 * i + 2] - c2));
-    }
+    lrf_raw_group(S, g, c0, c1, c2, part);
     acc.add(part.terms());
   }
 }
@@ -416,6 +430,16 @@ __device__ __forceinline__ void lrf_gather_water(const DevSys& S, int jo, int jw
       part.add((float)S.chg_solv[k], xj[3 * k] - cf[0], xj[3 * k + 1] - cf[1], xj[3 * k + 2] - cf[2]);
     acc.add(part.terms());
   }
+}
+
+// The raw moments of standard water jw's three atoms about the FP32 centre
+// cf, added to part (FP32).
+constexpr int kLrfPend = 8;   // partner tiles whose far waters' FP32 raw moments are summed before conversion
+__device__ __forceinline__ void lrf_raw_water(const DevSys& S, int jw, const float* cf, LrfRaw<float>& part) {
+  float xj[9];
+  water_xf(S, jw, xj);
+  for (int k = 0; k < 3; ++k)
+    part.add((float)S.chg_solv[k], xj[3 * k] - cf[0], xj[3 * k + 1] - cf[1], xj[3 * k + 2] - cf[2]);
 }
 
 // lrf_taylor for one atom at xa with its group's multipoles L: returns the

@@ -90,10 +90,19 @@ __global__ __launch_bounds__(256) void k_water_lists(const DevSys* __restrict__ 
   const unsigned long long mine = ((1ull << kWLLanes) - 1ull) << seg;
   LrfAcc acc;
   acc.zero();
+  // FP32 build: the raw moments of the far standard waters of kLrfPend
+  // partner tiles are summed in FP32, and their components (like those of
+  // other far water groups) into FP32 component sums, which join the FP64
+  // sums once after the water partners.  The rounding of these sums stays well below that
+  // of each partner's FP32 terms, whose atoms' contributions cancel.
+  LrfRaw<float> pend;
+  pend.zero();
+  LrfTerms<float> sum32;
+  sum32.zero();
   // waters
   int n_in = 0;
   auto row = gptr_w(S.ww_nb + (size_t)(iw < S.nwat ? iw : 0) * S.ww_cap);
-  for (int base = 0; base < S.nwat; base += kWLLanes) {
+  for (int base = 0, it = 1; base < S.nwat; base += kWLLanes, ++it) {
     const int jw = base + sub;
     bool inside = false, far = false;
     if (active && jw < S.nwat && jw != iw) {
@@ -113,9 +122,31 @@ __global__ __launch_bounds__(256) void k_water_lists(const DevSys* __restrict__ 
     n_in += __popcll(m);
     if (far && S.use_lrf) {
       const int jo = S.nat_solute + 3 * jw;
-      if (S.std_water) lrf_gather_water<P>(S, jo, jw, c0, c1, c2, cf, acc);
-      else lrf_gather_group<P>(S, S.iwhich_cgp[jo], c0, c1, c2, acc);
+      if constexpr (sizeof(P) == 4) {
+        if (S.std_water) {
+          lrf_raw_water(S, jw, cf, pend);
+        } else {
+          LrfRaw<float> part;
+          part.zero();
+          lrf_raw_group(S, S.iwhich_cgp[jo], c0, c1, c2, part);
+          sum32.add(part.terms());
+        }
+      } else {
+        if (S.std_water) lrf_gather_water<P>(S, jo, jw, c0, c1, c2, cf, acc);
+        else lrf_gather_group<P>(S, S.iwhich_cgp[jo], c0, c1, c2, acc);
+      }
     }
+    if constexpr (sizeof(P) == 4) {
+      // every kLrfPend partner tiles (the same iterations for every lane)
+      if (it % kLrfPend == 0) {
+        sum32.add(pend.terms());
+        pend.zero();
+      }
+    }
+  }
+  if constexpr (sizeof(P) == 4) {
+    sum32.add(pend.terms());
+    acc.add(sum32);
   }
   // solute charge groups (water side of the solute-water list)
   int n_wp = 0;
@@ -524,67 +555,94 @@ __global__ void k_watpol_theta(const DevSys* __restrict__ sys) {
   }
 }
 
-// Part 2: one workgroup per shell.  Members are compacted in water order;
-// each member's position in qdyn's selection sort (ascending theta, ties to
-// the lower list index) is its rank; the target angle follows from the rank.
+// Part 2: kWpolSplit workgroups per shell.  Each compacts the shell's
+// members in water order; a member's position in qdyn's selection sort
+// (ascending theta, ties to the lower list index) is its rank, counted by
+// four lanes over interleaved quarters of the members; the target angle
+// follows from the rank.  Workgroup sp takes the sp-th chunk of the
+// members.  The shell record is read only: its sums (energy, theta, member
+// count) go to per-workgroup partials that k_watpol_shell folds into it in
+// a fixed order after this kernel.
 constexpr int kShellCap = 4096;
 __global__ __launch_bounds__(kWpolBlock) void k_watpol_force(const DevSys* __restrict__ sys) {
   const DevSys& S = sys[blockIdx.y];
-  const int is = blockIdx.x;
+  const int is = blockIdx.x / kWpolSplit, sp = blockIdx.x % kWpolSplit;
   if (!S.wpol_restr || is >= S.nshell || halted(S, PH_WATPOL)) return;
   const int istep = *S.step_ctr;
   const bool update = istep != 0 && istep % qsys::kItdisUpdate == 0;
   __shared__ int mem_iw[kShellCap];
   __shared__ double mem_th[kShellCap];
-  __shared__ int s_cnt;
+  __shared__ int wave_tot[kWpolBlock / 64];
   __shared__ double red[2][kWpolBlock / 64];
-  DevShell& W = S.shell[is];
-  if (threadIdx.x == 0) s_cnt = 0;
-  __syncthreads();
-  // ordered compaction of this shell's members
-  for (int base = 0; base < S.nwat; base += blockDim.x) {
-    int iw = base + threadIdx.x;
-    bool mine = iw < S.nwat && S.shell_of[iw] == is;
-    // block-wide ordered prefix: wave ballots then a serial pass over waves
-    unsigned long long m = __ballot(mine);
-    __shared__ int wave_cnt[kWpolBlock / 64];
-    int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
-    if (ln == 0) wave_cnt[wv] = __popcll(m);
-    __syncthreads();
-    int off = s_cnt;
-    for (int w = 0; w < wv; ++w) off += wave_cnt[w];
-    if (mine) {
-      int pos = off + __popcll(m & ((1ull << ln) - 1ull));
-      if (pos < kShellCap) { mem_iw[pos] = iw; mem_th[pos] = S.theta[iw]; }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      int tot = 0;
-      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) tot += wave_cnt[w];
-      s_cnt += tot;
-    }
-    __syncthreads();
+  // ordered compaction of this shell's members: each thread takes a
+  // contiguous run of waters (all its loads issued together), counts its
+  // members, and writes them after the members of the threads before it
+  // (block-wide exclusive scan of the counts)
+  const int per = (S.nwat + kWpolBlock - 1) / kWpolBlock;
+  const int w0 = threadIdx.x * per, w1 = min(S.nwat, w0 + per);
+  unsigned long long own = 0ull;   // membership of the run's first 64 waters
+  int cnt = 0;
+  for (int c0 = w0; c0 < w1; c0 += 8) {
+    int sh[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) sh[u] = c0 + u < w1 ? S.shell_of[c0 + u] : -1;
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (sh[u] == is) {
+        ++cnt;
+        if (c0 + u - w0 < 64) own |= 1ull << (c0 + u - w0);
+      }
   }
-  int n = s_cnt;
-  if (n > kShellCap) { if (threadIdx.x == 0) record_failure(S, ST_SHELL_OVERFLOW, PH_WATPOL); n = kShellCap; }
-  // theta_corr update every itdis_update steps (before the forces use it)
-  if (update && threadIdx.x == 0) {
-    W.avtheta = W.avtheta / (double)qsys::kItdisUpdate;
-    W.avn_insh = W.avn_insh / (double)qsys::kItdisUpdate;
-    W.theta_corr = W.theta_corr + W.avtheta - acos(W.cstb);
-    W.avtheta = 0.0;
-    W.avn_insh = 0.0;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int incl = cnt;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int v = __shfl_up(incl, o);
+    if (lane >= o) incl += v;
+  }
+  if (lane == 63) wave_tot[wv] = incl;
+  __syncthreads();
+  int off = incl - cnt, n = 0;
+  for (int w = 0; w < kWpolBlock / 64; ++w) {
+    if (w < wv) off += wave_tot[w];
+    n += wave_tot[w];
+  }
+  for (int iw = w0, pos = off; iw < w1 && pos < off + cnt; ++iw) {
+    const bool mem = iw - w0 < 64 ? ((own >> (iw - w0)) & 1ull) != 0 : S.shell_of[iw] == is;
+    if (mem) {
+      if (pos < kShellCap) mem_iw[pos] = iw;
+      ++pos;
+    }
   }
   __syncthreads();
-  const double tcorr = W.theta_corr, cstb = W.cstb;
+  if (n > kShellCap) {
+    if (threadIdx.x == 0 && sp == 0) record_failure(S, ST_SHELL_OVERFLOW, PH_WATPOL);
+    n = kShellCap;
+  }
+  for (int m = threadIdx.x; m < n; m += kWpolBlock) mem_th[m] = S.theta[mem_iw[m]];
+  __syncthreads();
+  // theta_corr as this step uses it: updated every itdis_update steps
+  // (k_watpol_shell stores the same update into the record)
+  const DevShell& W = S.shell[is];
+  const double cstb = W.cstb;
+  double tcorr = W.theta_corr;
+  if (update) tcorr = tcorr + W.avtheta / (double)qsys::kItdisUpdate - acos(cstb);
+  const int chunk = (n + kWpolSplit - 1) / kWpolSplit;
+  const int m0 = sp * chunk, m1 = min(n, m0 + chunk);
+  const int q = threadIdx.x & 3;
   double e = 0.0, tsum = 0.0;
-  for (int m = threadIdx.x; m < n; m += blockDim.x) {
-    const double th = mem_th[m];
+  for (int mb = m0; mb < m1; mb += kWpolBlock / 4) {
+    const int m = mb + (int)(threadIdx.x >> 2);
+    const bool have = m < m1;
+    const double th = have ? mem_th[m] : 0.0;
     int rank = 0;
-    for (int k = 0; k < n; ++k) {
-      double tk = mem_th[k];
-      rank += (tk < th || (tk == th && k < m)) ? 1 : 0;
-    }
+    if (have)
+      for (int k = q; k < n; k += 4) {
+        const double tk = mem_th[k];
+        rank += (tk < th || (tk == th && k < m)) ? 1 : 0;
+      }
+    rank += __shfl_xor(rank, 1);
+    rank += __shfl_xor(rank, 2);
+    if (!have || q != 0) continue;
     const int iw = mem_iw[m];
     double arg = 1.0 + (1.0 - 2.0 * (double)(rank + 1)) / (double)n;
     double th0 = acos(arg);
@@ -612,13 +670,44 @@ __global__ __launch_bounds__(kWpolBlock) void k_watpol_force(const DevSys* __res
   e = block_sum<kWpolBlock / 64>(e, red[0]);
   tsum = block_sum<kWpolBlock / 64>(tsum, red[1]);
   if (threadIdx.x == 0) {
-    W.n_insh = n;
-    if (n > 0) {
-      W.avtheta = W.avtheta + tsum / (double)n;
-      W.avn_insh = W.avn_insh + n;
-    }
-    S.e_wpol[is] = e;
+    double* part = S.wpol_part + 3 * ((size_t)is * kWpolSplit + sp);
+    part[0] = e;
+    part[1] = tsum;
+    part[2] = (double)n;
   }
+}
+
+// The shell records after k_watpol_force (one lane per shell): the
+// itdis_update step's theta_corr update, then this step's mean theta and
+// member count, and the restraint energy, from the workgroups' partials
+// summed in workgroup order.
+__global__ void k_watpol_shell(const DevSys* __restrict__ sys) {
+  const DevSys& S = sys[blockIdx.y];
+  const int is = threadIdx.x;
+  if (!S.wpol_restr || is >= S.nshell || halted(S, PH_WATPOL)) return;
+  const int istep = *S.step_ctr;
+  const bool update = istep != 0 && istep % qsys::kItdisUpdate == 0;
+  DevShell& W = S.shell[is];
+  const double* part = S.wpol_part + 3 * (size_t)is * kWpolSplit;
+  double e = 0.0, tsum = 0.0;
+  for (int sp = 0; sp < kWpolSplit; ++sp) {
+    e = e + part[3 * sp];
+    tsum = tsum + part[3 * sp + 1];
+  }
+  const int n = (int)part[2];
+  if (update) {
+    W.avtheta = W.avtheta / (double)qsys::kItdisUpdate;
+    W.avn_insh = W.avn_insh / (double)qsys::kItdisUpdate;
+    W.theta_corr = W.theta_corr + W.avtheta - acos(W.cstb);
+    W.avtheta = 0.0;
+    W.avn_insh = 0.0;
+  }
+  W.n_insh = n;
+  if (n > 0) {
+    W.avtheta = W.avtheta + tsum / (double)n;
+    W.avn_insh = W.avn_insh + n;
+  }
+  S.e_wpol[is] = e;
 }
 
 // The polarisation gradient of shell members added to d (force evaluations
@@ -638,18 +727,20 @@ __global__ void k_watpol_gather(const DevSys* __restrict__ sys) {
 // sums of temperature (:6098-6190), fused per SHAKE unit.  Units are SHAKE
 // molecules (waters: 3 atoms, 3 constraints) or single atoms.
 // The gradient of atom a as the step uses it: d plus the polarisation
-// restraint and the sum of the atom's bonded-term slots; the result is
-// also stored in d.
-__device__ __forceinline__ void gather_slots(const DevSys& S, int a) {
+// restraint and the sum of the atom's bonded-term slots, formed in
+// registers (the integrator stores it back into d after the update, so no
+// load waits behind a store).
+__device__ __forceinline__ void step_gradient(const DevSys& S, int a, double* g) {
+  g[0] = S.d[3 * a]; g[1] = S.d[3 * a + 1]; g[2] = S.d[3 * a + 2];
   // the polarisation restraint first (it was added right after the water
   // forces before it ran beside them)
   if (S.wpol_restr && a >= S.nat_solute) {
     const int rel = a - S.nat_solute, iw = rel / 3;
     if (S.shell_of[iw] >= 0) {
-      const double* g = S.wpol_g + 9 * (size_t)iw + 3 * (rel % 3);
-      S.d[3 * a] = S.d[3 * a] + g[0];
-      S.d[3 * a + 1] = S.d[3 * a + 1] + g[1];
-      S.d[3 * a + 2] = S.d[3 * a + 2] + g[2];
+      const double* w = S.wpol_g + 9 * (size_t)iw + 3 * (rel % 3);
+      g[0] = g[0] + w[0];
+      g[1] = g[1] + w[1];
+      g[2] = g[2] + w[2];
     }
   }
   // the atom's bonded/Q-Q/restraint term slots, summed in slot order
@@ -668,9 +759,9 @@ __device__ __forceinline__ void gather_slots(const DevSys& S, int a) {
   for (; k < n; ++k) {
     g0 = g0 + o[3 * k]; g1 = g1 + o[3 * k + 1]; g2 = g2 + o[3 * k + 2];
   }
-  S.d[3 * a] = S.d[3 * a] + g0;
-  S.d[3 * a + 1] = S.d[3 * a + 1] + g1;
-  S.d[3 * a + 2] = S.d[3 * a + 2] + g2;
+  g[0] = g[0] + g0;
+  g[1] = g[1] + g1;
+  g[2] = g[2] + g2;
 }
 
 // shake (src/md.f90:17581-17679) of one molecule, constraints c0..c1 of
@@ -740,15 +831,15 @@ __global__ __launch_bounds__(kIntegrateBlock) void k_integrate(const DevSys* __r
     const double dt = S.dt;
     const int a = S.unit_atoms[S.unit_first[u]];
     const double ts = a < S.nat_solute ? S.scal[0] : S.scal[1];
-    double x[9], xo[9], v[9], w[3];
+    double x[9], xo[9], v[9], w[3], gd[9];
     for (int k = 0; k < 3; ++k) w[k] = S.winv[a + k];
-    for (int k = 0; k < 3; ++k) gather_slots(S, a + k);
+    for (int k = 0; k < 3; ++k) step_gradient(S, a + k, gd + 3 * k);
     for (int c = 0; c < 9; ++c) {
       const int ic = 3 * a + c;
       // excluded_freeze: an excluded atom restarts from rest at its
       // topology position (fix_shell, src/md.f90:2831-2839)
       const bool frz = S.freeze && (S.flags[a + c / 3] & F_EXCL);
-      double vv = ((frz ? 0.0 : S.v[ic]) - S.d[ic] * w[c / 3] * dt) * ts;
+      double vv = ((frz ? 0.0 : S.v[ic]) - gd[c] * w[c / 3] * dt) * ts;
       v[c] = vv;
       xo[c] = frz ? S.xtop[ic] : S.x[ic];
       x[c] = xo[c] + vv * dt;
@@ -786,6 +877,7 @@ __global__ __launch_bounds__(kIntegrateBlock) void k_integrate(const DevSys* __r
       S.x[ic] = x[c];
       S.xf[ic] = (float)x[c];
       S.v[ic] = v[c];
+      S.d[ic] = gd[c];
     }
     // solvent waters start at nat_solute, three atoms each
     if (S.xw && a >= S.nat_solute) water_xf_store(S, (a - S.nat_solute) / 3, x);
@@ -802,17 +894,19 @@ __global__ __launch_bounds__(kIntegrateBlock) void k_integrate(const DevSys* __r
     const int a0 = S.unit_first[u], a1 = S.unit_first[u + 1];
     for (int k = a0; k < a1; ++k) {
       const int i = S.unit_atoms[k];
-      gather_slots(S, i);
+      double gd[3];
+      step_gradient(S, i, gd);
       const double ts = i < S.nat_solute ? ts_solute : ts_solvent;
       const double w = S.winv[i];
       const bool frz = S.freeze && (S.flags[i] & F_EXCL);
       for (int c = 0; c < 3; ++c) {
         const int ic = 3 * i + c;
-        double vv = ((frz ? 0.0 : S.v[ic]) - S.d[ic] * w * dt) * ts;
+        double vv = ((frz ? 0.0 : S.v[ic]) - gd[c] * w * dt) * ts;
         const double xo = frz ? S.xtop[ic] : S.x[ic];
         S.v[ic] = vv;
         S.xx[ic] = xo;
         S.x[ic] = xo + vv * dt;
+        S.d[ic] = gd[c];
       }
     }
     const int c0 = S.con_first[u], c1 = S.con_first[u + 1];

@@ -31,6 +31,38 @@ module qgpu
       type(c_ptr), value :: handle
     end function
 
+    ! pot_energy_nonbonds (src/md.f90:16261-16344) at x; d accumulates,
+    ! e(7) = pp, pw, ww (el, vdW) and LRF, eq(4, nstates) = qp, qw (el, vdW)
+    integer(c_int) function qgpu_pot_energy_nonbonds(handle, k, x, d, e, eq, eq_cap) &
+        bind(C, name='qgpu_pot_energy_nonbonds')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: handle
+      integer(c_int), value :: k, eq_cap
+      real(c_double), intent(in) :: x(*)
+      real(c_double), intent(inout) :: d(*)
+      real(c_double), intent(out) :: e(*), eq(*)
+    end function
+
+    ! shake(xx, x) (src/md.f90:17581-17679): x constrained in place
+    integer(c_int) function qgpu_shake(handle, k, xx, x, niter) bind(C, name='qgpu_shake')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: handle
+      integer(c_int), value :: k
+      real(c_double), intent(in) :: xx(*)
+      real(c_double), intent(inout) :: x(*)
+      integer(c_int), intent(out) :: niter
+    end function
+
+    ! failure record of instance k: bits (1 list regrowth, 2 shell capacity,
+    ! 4 SHAKE, 8 NaN temperature) and the step it was found at
+    integer(c_int) function qgpu_instance_status(handle, k, bits, at_step) &
+        bind(C, name='qgpu_instance_status')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle
+      integer(c_int), value :: k
+      integer(c_int), intent(out) :: bits, at_step
+    end function
+
     integer(c_int) function qgpu_step(handle, nsteps) bind(C, name='qgpu_step')
       import :: c_int, c_ptr
       type(c_ptr), value :: handle
