@@ -1010,7 +1010,7 @@ static void tuned_items(int kernel, const DevSys& D, bool f64, int& items, int& 
     case Batch::TK_WATER_FORCES:
       items = D.nwat; lanes = 8; work = (double)D.nwat; break;
     default:
-      items = D.nat_solute; lanes = 8; work = (double)D.nat_solute; break;
+      items = D.nat_solute; lanes = kSFLanes; work = (double)D.nat_solute; break;
   }
   if (items == 0 || (kernel == Batch::TK_SOLUTE_LISTS && D.ncgp_solute == 0)) work = 0;
 }

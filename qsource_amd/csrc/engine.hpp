@@ -72,9 +72,11 @@ constexpr int kEWater = 5;     // energy partials per water
 constexpr int kMon = 3;        // Vwsum, Vwel, Vwlj before the per-state values
 constexpr int kIntegrateBlock = 64;  // threads per k_integrate workgroup
 constexpr int kWpolBlock = 256;      // threads per k_watpol_force workgroup
-constexpr int kWpolSplit = 8;        // k_watpol_force workgroups per water shell
+constexpr int kWpolSplit = 4;        // k_watpol_force workgroups per water shell
 // lanes per water in k_water_lists: FP64 build, FP32 (mixed) build
 constexpr int kWLLanes64 = 8, kWLLanes32 = 16;
+// lanes per solute atom in k_solute_forces
+constexpr int kSFLanes = 16;
 
 struct DevShell {
   double rout, dr, cstb, avtheta, avn_insh, theta_corr;

@@ -148,11 +148,12 @@ template __global__ void k_solute_lists<float, 1>(const DevSys*, const int2*);
 // ---------------------------------------------------------------------------
 // Solute forces (nonbond_pp :13069-13155, nonbond_pw :13282-13365,
 // lrf_taylor, fix_shell :2816-2862, and the solute-atom side of nonbond_qp
-// :13788-13921).  Eight solute atoms per wavefront, eight lanes per atom: the
-// lanes of an atom take the neighbour groups of its charge group (pair status
-// from the list-time masks), the listed waters and the Q atoms in turn; lane
-// 1 adds the LRF Taylor term and lane 2 the shell/fix restraint; the atom's
-// gradient is summed over its 8 lanes and written once.  A pair's energy is
+// :13788-13921).  kSFLanes lanes per solute atom (16: the short rows of a
+// solute atom leave latency to hide, so more, shorter lanes): the lanes of
+// an atom take the neighbour groups of its charge group (pair status from
+// the list-time masks), the listed waters and the Q atoms in turn; lane 1
+// adds the LRF Taylor term and lane 2 the shell/fix restraint; the atom's
+// gradient is summed over its lanes and written once.  A pair's energy is
 // counted by the atom that is i of the reference pair: the owner group's
 // atom, or the lower index within one group.
 // FP32 pair arithmetic fits 6 wavefronts per SIMD without spilling
@@ -163,12 +164,12 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(c
                                                                                int en) {
   const DevSys* Sp;
   int base;
-  if (!block_work<MAPPED>(sys, map, blockDim.x >> 3, Sp, base)) return;
+  if (!block_work<MAPPED>(sys, map, blockDim.x / kSFLanes, Sp, base)) return;
   const DevSys& S = *Sp;
   const int t = threadIdx.x;
-  if (base + ((t >> 6) << 3) >= S.nat_solute) return;   // whole wavefronts past the last atom
-  const int sub = t & 7;
-  const int i = base + (t >> 3);
+  if (base + (t >> 6) * (64 / kSFLanes) >= S.nat_solute) return;   // whole wavefronts past the last atom
+  const int sub = t & (kSFLanes - 1);
+  const int i = base + t / kSFLanes;
   const bool active = i < S.nat_solute;
   const int ia = active ? i : 0;
   const int g = S.iwhich_cgp[ia];
@@ -190,7 +191,7 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(c
     const gconst<int> gatom = gptr(S.cgpatom);
     const gconst<int> giac = gptr(S.iac);
     const gconst<double> gcrg = gptr(S.crg);
-    for (int k = sub; k < npp; k += 8) {
+    for (int k = sub; k < npp; k += kSFLanes) {
       const int jg = prow[k];
       const unsigned long long mk_x = mrow[2 * k], mk_y = mrow[2 * k + 1];
       unsigned todo = (unsigned)(~mk_x >> (a * kMaxGroupAtoms)) & 0xffu;
@@ -305,21 +306,21 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(c
         jwn = wrow[k];
         load_water(jwn, xn);
       }
-      if (k + 8 < npw) jwn2 = wrow[k + 8];
+      if (k + kSFLanes < npw) jwn2 = wrow[k + kSFLanes];
       while (k < npw) {
         P xw[9];
         for (int u = 0; u < 9; ++u) xw[u] = xn[u];
         const int jo = S.nat_solute + 3 * jwn;
-        k += 8;
+        k += kSFLanes;
         if (k < npw) {
           jwn = jwn2;
           load_water(jwn, xn);
-          if (k + 8 < npw) jwn2 = wrow[k + 8];
+          if (k + kSFLanes < npw) jwn2 = wrow[k + kSFLanes];
         }
         pw_water(jo, xw);
       }
     } else {
-      for (int k = sub; k < npw; k += 8) {
+      for (int k = sub; k < npw; k += kSFLanes) {
         const int jw = wrow[k];
         P xw[9];
         load_water(jw, xw);
@@ -328,7 +329,7 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(c
     }
     // Q-solute pairs seen from the solute atom (the Q atom is i of the pair)
     if ((fl & F_QP) && (S.term_mask & TM_QX)) {
-      for (int iq = sub; iq < S.nqat; iq += 8) {
+      for (int iq = sub; iq < S.nqat; iq += kSFLanes) {
         const int qa = S.iqseq[iq];
         const int lj = S.qp_lj[(size_t)i * S.nqat + iq];
         const int l = lj - 1;
@@ -388,11 +389,11 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(c
     g1 = g1 + fk * dr1;
     g2 = g2 + fk * dr2;
   }
-  g0 = seg8_sum(g0); g1 = seg8_sum(g1); g2 = seg8_sum(g2);
+  g0 = segn_sum<kSFLanes>(g0); g1 = segn_sum<kSFLanes>(g1); g2 = segn_sum<kSFLanes>(g2);
   if (en) {
-    e_ppel = seg8_sum(e_ppel); e_ppvdw = seg8_sum(e_ppvdw);
-    e_pwel = seg8_sum(e_pwel); e_pwvdw = seg8_sum(e_pwvdw);
-    e_lrf = seg8_sum(e_lrf); e_fix = seg8_sum(e_fix); e_shell = seg8_sum(e_shell);
+    e_ppel = segn_sum<kSFLanes>(e_ppel); e_ppvdw = segn_sum<kSFLanes>(e_ppvdw);
+    e_pwel = segn_sum<kSFLanes>(e_pwel); e_pwvdw = segn_sum<kSFLanes>(e_pwvdw);
+    e_lrf = segn_sum<kSFLanes>(e_lrf); e_fix = segn_sum<kSFLanes>(e_fix); e_shell = segn_sum<kSFLanes>(e_shell);
   }
   if (active && sub == 0) {
     S.d[3 * i] = g0; S.d[3 * i + 1] = g1; S.d[3 * i + 2] = g2;

@@ -635,11 +635,13 @@ __global__ __launch_bounds__(kWpolBlock) void k_watpol_force(const DevSys* __res
     const bool have = m < m1;
     const double th = have ? mem_th[m] : 0.0;
     int rank = 0;
-    if (have)
+    if (have) {
+#pragma unroll 8
       for (int k = q; k < n; k += 4) {
         const double tk = mem_th[k];
         rank += (tk < th || (tk == th && k < m)) ? 1 : 0;
       }
+    }
     rank += __shfl_xor(rank, 1);
     rank += __shfl_xor(rank, 2);
     if (!have || q != 0) continue;
