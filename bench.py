@@ -1,7 +1,7 @@
 """Benchmark: aggregate ns/day of batched FEP lambda windows on one MI355X.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3|c5]
-                [--accuracy-steps N]
+                [--accuracy-steps N [--accuracy-control]]
 
 Default workload (c3): three solvated systems (TIP3P spheres of radius 18,
 22 and 28 A around branched solutes of 150, 300 and 600 atoms, 12-atom Q
@@ -19,20 +19,27 @@ timed region (mixed precision: FP32 pair arithmetic, FP64 accumulation).
 Energy recording, launch tuning and at least one whole captured list cycle
 happen before the timed region, so it replays the captured step graph.
 
-Accuracy: the dG of each system from the mixed batch is compared with the
-dG of the same windows run as an FP64 batch (independent trajectories,
-Zwanzig and BAR), with a block-bootstrap standard error.  The |ddG| < 0.05
-kcal/mol gate is established only where that standard error is below 0.02
-kcal/mol; a short run cannot reach that, and then the gate is taken from the
-committed long-run summary of the same workload (profiles/, written by
---accuracy-steps).  Every recorded frame of the mixed run is also
-re-evaluated with FP64 pair arithmetic on the same coordinates, which gives
-the precision error of dU and dG alone; the per-term force errors (mixed vs
-FP64 on identical coordinates) are reported per system.
-CPU baseline: the FP64 CPU oracle (an order-faithful restatement of serial
-qdyn) on this host's cores, one window per core of the R = 22 A system; the
-reference qdyn5 itself does not travel here, its timing from the build host
-is read from the committed summary.
+Plain FP64 engine path: the same windows run one at a time in FP64; its
+aggregate rate is timed on one window of every system (the simulated time
+over the summed wall time), and speedup_vs_plain_fp64 = value / that rate.
+
+Accuracy: every recorded frame of the mixed run is also re-evaluated with
+FP64 pair arithmetic on the same coordinates, which gives the precision
+error of dU and dG alone (gate_ddG_lt_0.05_precision_only); the per-term
+force errors (mixed vs FP64 on identical coordinates) are reported per
+system.  The dG of each system from the mixed batch is also compared with
+the dG of the same windows run as an FP64 batch (independent trajectories,
+Zwanzig and BAR), with a block-bootstrap standard error.  That gate
+(gate_ddG_lt_0.05_trajectories) is established only where the standard
+error is below 0.02 kcal/mol; a short run cannot reach that, and then it is
+taken from the committed long-run summary of the same workload (profiles/,
+written by --accuracy-steps, with --accuracy-control for the spread between
+runs of one precision).
+CPU baseline: the reference's serial qdyn5 (oracle/_ref, compiled from the
+reference sources) on this host's cores, one window per core of the R = 22 A
+system, from the same inputs and restart as the GPU batch; where it is not
+built, the FP64 CPU oracle (an order-faithful restatement of serial qdyn).
+The same timing from the build host is read from the committed summary.
 """
 import argparse
 import json
@@ -381,6 +388,55 @@ def cpu_baseline(inputs, seconds=15.0, procs=8):
                       "(longest %.1f s)" % (procs, n, max(secs))}
 
 
+REF_QDYN = os.path.join(ROOT, "oracle", "_ref", "qdyn5")
+
+
+def cpu_reference(inputs, seconds=15.0, procs=8):
+    """The reference's serial qdyn5 (oracle/_ref, compiled from the reference
+    sources) on this host: one window per process, `procs` processes at once,
+    each running about `seconds` of steps from the same input and restart the
+    GPU batch starts from; the rate of each comes from the "Total time of
+    main loop" qdyn5 prints.  None where qdyn5 is not built."""
+    if not os.path.exists(REF_QDYN) or not inputs:
+        return None
+    procs = max(1, min(procs, len(inputs), os.cpu_count() or 1, 16))
+
+    def with_steps(inp, n, tag):
+        lines = open(inp).read().splitlines()
+        for i, line in enumerate(lines):
+            w = line.split()
+            if len(w) == 2 and w[0] == "steps":
+                lines[i] = "steps %d" % n
+        q = inp[:-len(".inp")] + "_%s.inp" % tag
+        with open(q, "w") as f:
+            f.write("\n".join(lines) + "\n")
+        return q
+
+    def run(jobs):
+        ps = [subprocess.Popen([REF_QDYN, os.path.basename(q)], cwd=os.path.dirname(q), stdout=subprocess.PIPE,
+                               stderr=subprocess.STDOUT, text=True) for q in jobs]
+        secs = []
+        for p in ps:
+            out = p.communicate()[0]
+            if "terminated normally" not in out or "Total time of main loop" not in out:
+                log("reference qdyn5 run failed:\n" + out[-2000:])
+                return None
+            secs.append(float(out.split("Total time of main loop:")[1].split("(s)")[0]))
+        return secs
+    cal = run([with_steps(inputs[0], 20, "cal")])
+    if cal is None:
+        return None
+    n = max(20, int(round(seconds / max(cal[0], 1e-3) * 20 / 10.0)) * 10)
+    secs = run([with_steps(inputs[k], n, "ref") for k in range(procs)])
+    if secs is None:
+        return None
+    per = [n * 1e-6 * 86400.0 / s for s in secs]
+    return {"value": sum(per), "unit": "ns/day", "cores": procs, "kind": "reference",
+            "sample": "reference serial qdyn5 (oracle/_ref, built from the reference sources), %d windows of the "
+                      "R=22 A / 300-atom system at once, one process per core, %d steps each at 1 fs / 300 K from "
+                      "the equilibrated restart (main loop %.1f - %.1f s)" % (procs, n, min(secs), max(secs))}
+
+
 # ---------------------------------------------------------------------------
 
 def main():
@@ -419,8 +475,11 @@ def main():
             return
         cpu = None
         if args.cpu_seconds > 0 and args.config == "c3":
-            log("CPU baseline: oracle, %d windows" % args.cpu_procs)
-            cpu = cpu_baseline([inputs[k] for k in groups[1]], args.cpu_seconds, args.cpu_procs)
+            log("CPU baseline: reference qdyn5, %d windows" % args.cpu_procs)
+            cpu = cpu_reference([inputs[k] for k in groups[1]], args.cpu_seconds, args.cpu_procs)
+            if cpu is None:
+                log("CPU baseline: oracle, %d windows" % args.cpu_procs)
+                cpu = cpu_baseline([inputs[k] for k in groups[1]], args.cpu_seconds, args.cpu_procs)
             log("CPU baseline: %s" % cpu)
         model = None
         if args.config == "c5":
@@ -485,11 +544,17 @@ def main():
             gate, source = None, "unestablished (no run long enough)"
         acc["gate_ddG_lt_0.05_trajectories"] = gate
         acc["gate_trajectories_source"] = source
-        # plain FP64 engine, one window at a time
-        pw = groups[min(1, len(groups) - 1)]
-        ep, wallp, _ = run_batch([inputs[pw[len(pw) // 2]]], qe.QGPU_FP64, max(200, args.steps // 4), 50)
-        plain = max(200, args.steps // 4) * 1e-6 * 86400.0 / wallp
-        ep.close()
+        # plain FP64 engine path: the windows run one at a time, so the
+        # job's aggregate rate is the simulated time over the summed wall
+        # time of its windows; one window of every system is timed
+        psteps = max(500, args.steps // 2)
+        plain_walls = []
+        for g in groups:
+            ep, wallp, _ = run_batch([inputs[g[len(g) // 2]]], qe.QGPU_FP64, psteps, 50)
+            plain_walls.append(wallp)
+            ep.close()
+        plain = len(groups) * psteps * 1e-6 * 86400.0 / sum(plain_walls)
+        plain_per_system = [psteps * 1e-6 * 86400.0 / w for w in plain_walls]
         if model is not None:
             model.close()
     cfg["precision"] = "mixed"
@@ -497,7 +562,8 @@ def main():
     out.update({"value": value, "ms_per_step": wall / args.steps * 1e3, "config": cfg,
                 "roofline": roof, "roofline_kernels": roofs, "cpu_baseline": cpu,
                 "cpu_reference_build_host": ref_cpu, "kernel_ms": prof, "kernel_ms_per_step": step_ms,
-                "autotune": tuned, "plain_fp64_single_window_ns_per_day": plain,
+                "autotune": tuned, "plain_fp64_one_window_at_a_time_ns_per_day": plain,
+                "plain_fp64_ns_per_day_per_system": plain_per_system,
                 "speedup_vs_plain_fp64": value / plain, "accuracy": acc})
     print(json.dumps(out))
 
