@@ -35,11 +35,12 @@ error is below 0.02 kcal/mol; a short run cannot reach that, and then it is
 taken from the committed long-run summary of the same workload (profiles/,
 written by --accuracy-steps, with --accuracy-control for the spread between
 runs of one precision).
-CPU baseline: the reference's serial qdyn5 (oracle/_ref, compiled from the
-reference sources) on this host's cores, one window per core of the R = 22 A
-system, from the same inputs and restart as the GPU batch; where it is not
-built, the FP64 CPU oracle (an order-faithful restatement of serial qdyn).
-The same timing from the build host is read from the committed summary.
+CPU baseline: the FP64 CPU oracle (an order-faithful restatement of serial
+qdyn) on this host's cores, one window per core of the R = 22 A system, from
+the same inputs and restart as the GPU batch.  The reference qdyn5 itself
+(oracle/_ref, compiled from the reference sources) stays in the build
+container; its timing there, for 1 and 8 windows at once, is read from the
+committed summary (cpu_reference_build_host).
 """
 import argparse
 import json
@@ -388,55 +389,6 @@ def cpu_baseline(inputs, seconds=15.0, procs=8):
                       "(longest %.1f s)" % (procs, n, max(secs))}
 
 
-REF_QDYN = os.path.join(ROOT, "oracle", "_ref", "qdyn5")
-
-
-def cpu_reference(inputs, seconds=15.0, procs=8):
-    """The reference's serial qdyn5 (oracle/_ref, compiled from the reference
-    sources) on this host: one window per process, `procs` processes at once,
-    each running about `seconds` of steps from the same input and restart the
-    GPU batch starts from; the rate of each comes from the "Total time of
-    main loop" qdyn5 prints.  None where qdyn5 is not built."""
-    if not os.path.exists(REF_QDYN) or not inputs:
-        return None
-    procs = max(1, min(procs, len(inputs), os.cpu_count() or 1, 16))
-
-    def with_steps(inp, n, tag):
-        lines = open(inp).read().splitlines()
-        for i, line in enumerate(lines):
-            w = line.split()
-            if len(w) == 2 and w[0] == "steps":
-                lines[i] = "steps %d" % n
-        q = inp[:-len(".inp")] + "_%s.inp" % tag
-        with open(q, "w") as f:
-            f.write("\n".join(lines) + "\n")
-        return q
-
-    def run(jobs):
-        ps = [subprocess.Popen([REF_QDYN, os.path.basename(q)], cwd=os.path.dirname(q), stdout=subprocess.PIPE,
-                               stderr=subprocess.STDOUT, text=True) for q in jobs]
-        secs = []
-        for p in ps:
-            out = p.communicate()[0]
-            if "terminated normally" not in out or "Total time of main loop" not in out:
-                log("reference qdyn5 run failed:\n" + out[-2000:])
-                return None
-            secs.append(float(out.split("Total time of main loop:")[1].split("(s)")[0]))
-        return secs
-    cal = run([with_steps(inputs[0], 20, "cal")])
-    if cal is None:
-        return None
-    n = max(20, int(round(seconds / max(cal[0], 1e-3) * 20 / 10.0)) * 10)
-    secs = run([with_steps(inputs[k], n, "ref") for k in range(procs)])
-    if secs is None:
-        return None
-    per = [n * 1e-6 * 86400.0 / s for s in secs]
-    return {"value": sum(per), "unit": "ns/day", "cores": procs, "kind": "reference",
-            "sample": "reference serial qdyn5 (oracle/_ref, built from the reference sources), %d windows of the "
-                      "R=22 A / 300-atom system at once, one process per core, %d steps each at 1 fs / 300 K from "
-                      "the equilibrated restart (main loop %.1f - %.1f s)" % (procs, n, min(secs), max(secs))}
-
-
 # ---------------------------------------------------------------------------
 
 def main():
@@ -475,11 +427,8 @@ def main():
             return
         cpu = None
         if args.cpu_seconds > 0 and args.config == "c3":
-            log("CPU baseline: reference qdyn5, %d windows" % args.cpu_procs)
-            cpu = cpu_reference([inputs[k] for k in groups[1]], args.cpu_seconds, args.cpu_procs)
-            if cpu is None:
-                log("CPU baseline: oracle, %d windows" % args.cpu_procs)
-                cpu = cpu_baseline([inputs[k] for k in groups[1]], args.cpu_seconds, args.cpu_procs)
+            log("CPU baseline: oracle, %d windows" % args.cpu_procs)
+            cpu = cpu_baseline([inputs[k] for k in groups[1]], args.cpu_seconds, args.cpu_procs)
             log("CPU baseline: %s" % cpu)
         model = None
         if args.config == "c5":

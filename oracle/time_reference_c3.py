@@ -5,10 +5,11 @@ solute, 12 Q atoms, 2 states), on the host's cores.
 
 The system is built exactly as bench.py builds it and equilibrated with the
 same staged protocol (qsource_amd.equilibrate.DEFAULT_STAGES), each stage run
-by qdyn5 itself.  Then `--windows` lambda windows (0.5 +- k * 0.05) run
-`--steps` production steps (1 fs, 300 K, restart from the equilibration) at
-once, one qdyn5 process per window; each window's rate comes from the
-"Total time of main loop" qdyn5 prints.  Prints one `result:` JSON line.
+by qdyn5 itself.  Then one window alone, and `--windows` lambda windows
+(0.5 +- k * 0.05) at once, one qdyn5 process per window, run `--steps`
+production steps (1 fs, 300 K, restart from the equilibration); each
+window's rate comes from the "Total time of main loop" qdyn5 prints.  Prints
+one `result:` JSON line.
 
 Usage:  python oracle/time_reference_c3.py [--steps 1000] [--windows 8]
 """
@@ -58,24 +59,30 @@ def main():
             run_qdyn(p["inp"], d)
             rst = "eq%d.re" % k
             print("equilibration stage %d done" % k, flush=True)
-        procs = []
-        for w in range(args.windows):
-            lam = 0.5 + 0.05 * (w - args.windows // 2)
-            p = gensys.write_system(s, d, "w%d" % w, steps=args.steps, output=1000000,
-                                    lambdas=[1.0 - lam, lam], initial_temperature=300.0,
-                                    random_seed=101 + w, restart=rst)
-            procs.append(subprocess.Popen([QDYN, os.path.basename(p["inp"])], cwd=d,
-                                          stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True))
-        secs = []
-        for pr in procs:
-            out = pr.communicate()[0]
-            m = LOOP.search(out)
-            if "terminated normally" not in out or not m:
-                sys.exit("qdyn5 window failed:\n" + out[-2000:])
-            secs.append(float(m.group(1)))
+        def windows(count):
+            procs = []
+            for w in range(count):
+                lam = 0.5 + 0.05 * (w - count // 2)
+                p = gensys.write_system(s, d, "w%d" % w, steps=args.steps, output=1000000,
+                                        lambdas=[1.0 - lam, lam], initial_temperature=300.0,
+                                        random_seed=101 + w, restart=rst)
+                procs.append(subprocess.Popen([QDYN, os.path.basename(p["inp"])], cwd=d,
+                                              stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True))
+            secs = []
+            for pr in procs:
+                out = pr.communicate()[0]
+                m = LOOP.search(out)
+                if "terminated normally" not in out or not m:
+                    sys.exit("qdyn5 window failed:\n" + out[-2000:])
+                secs.append(float(m.group(1)))
+            return secs
+        one = windows(1)
+        print("one window done", flush=True)
+        secs = windows(args.windows)
         per = [args.steps * 1e-6 * 86400.0 / t for t in secs]
         res = {"value": sum(per), "unit": "ns/day", "cores": args.windows, "kind": "reference",
                "per_window_ns_per_day": per, "main_loop_seconds": secs,
+               "one_window_alone_ns_per_day": args.steps * 1e-6 * 86400.0 / one[0],
                "sample": "reference serial qdyn5, %d windows of the R=22 A / 300-atom system at once, one "
                          "process per core, %d steps each at 1 fs / 300 K from a qdyn5-equilibrated restart"
                          % (args.windows, args.steps),
