@@ -389,6 +389,54 @@ def cpu_baseline(inputs, seconds=15.0, procs=8):
                       "(longest %.1f s)" % (procs, n, max(secs))}
 
 
+def trace_summary(path):
+    """Per-step timeline of a rocprofv3 kernel trace (CSV) of a bench run:
+    the steady-state steps are the run of mixed-precision water-force
+    launches of the chosen variant; prints the wall time per step, the time
+    per step at least one kernel runs, the idle fraction, and each kernel's
+    busy time per step."""
+    import collections
+    import csv
+    ks = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]),
+                 r["Kernel_Name"].split("(")[0].replace("void ", "").replace("qgpu::", ""))
+                for r in csv.DictReader(open(path)))
+    cnt = collections.Counter(k[2] for k in ks if k[2].startswith("k_water_forces<float"))
+    marker = cnt.most_common(1)[0][0]
+    wf = [k for k in ks if k[2] == marker]
+    runs, cur = [], [wf[0]]
+    for a, b in zip(wf, wf[1:]):
+        if b[0] - a[0] < 4000000:
+            cur.append(b)
+        else:
+            runs.append(cur)
+            cur = [b]
+    runs.append(cur)
+    run = max(runs, key=len)
+    t0, t1 = run[1][0], run[-1][0]
+    nsteps = len(run) - 2
+    inwin = [k for k in ks if k[1] > t0 and k[0] < t1]
+    busy, cs, ce = 0, None, None
+    for s, e in sorted((max(s, t0), min(e, t1)) for s, e, _ in inwin):
+        if cs is None:
+            cs, ce = s, e
+        elif s <= ce:
+            ce = max(ce, e)
+        else:
+            busy += ce - cs
+            cs, ce = s, e
+    if cs is not None:
+        busy += ce - cs
+    per = collections.defaultdict(lambda: [0, 0])
+    for s, e, n in inwin:
+        per[n][0] += min(e, t1) - max(s, t0)
+        per[n][1] += 1
+    out = {"steps": nsteps, "wall_us_per_step": (t1 - t0) / nsteps / 1e3,
+           "busy_us_per_step": busy / nsteps / 1e3, "idle_fraction": 1.0 - busy / (t1 - t0),
+           "kernel_us_per_step": {n: round(v[0] / nsteps / 1e3, 2) for n, v in
+                                  sorted(per.items(), key=lambda x: -x[1][0])}}
+    print(json.dumps(out))
+
+
 # ---------------------------------------------------------------------------
 
 def main():
@@ -405,7 +453,12 @@ def main():
                     help="long accuracy run: mixed and FP64 batches of this many steps (no timing report)")
     ap.add_argument("--accuracy-control", action="store_true",
                     help="with --accuracy-steps: also a mixed and an FP64 batch from other initial velocities")
+    ap.add_argument("--trace-summary", default=None,
+                    help="summarise a rocprofv3 kernel-trace CSV of a bench run (no GPU work)")
     args = ap.parse_args()
+    if args.trace_summary:
+        trace_summary(args.trace_summary)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU.  The lambda windows are independent and would be "
               "sharded across GPUs with no exchange; --gpus %d is not run here." % args.gpus,

@@ -162,8 +162,10 @@ __global__ __launch_bounds__(256) void k_mlp_bwd(MlpDev M, const float* __restri
 // ---------------------------------------------------------------------------
 // Weight gradients.  Blocks 0..63: one 16x16 tile of dW2 per block (wave 0),
 // dW2[o][k] = sum_s dz2[s][o] h1[s][k] on matrix cores over 32-sample chunks
-// (A = dz2^T rows, B = h1 via h1^T rows).  Remaining threads: one parameter
-// of W1, b1, b2, W3, b3 each, summed over the samples in order.
+// (A = dz2^T rows, B = h1 via h1^T rows).  Remaining blocks: one parameter
+// of W1, b1, b2, W3, b3 per wavefront, its lanes summing interleaved
+// samples, then a fixed butterfly over the lanes (so the sum order is fixed
+// and training stays bitwise reproducible).
 __global__ __launch_bounds__(256) void k_mlp_wgrad(MlpDev M, int npad, float* __restrict__ G) {
   if (blockIdx.x < 64) {
     if (threadIdx.x >= 64) return;
@@ -178,29 +180,34 @@ __global__ __launch_bounds__(256) void k_mlp_wgrad(MlpDev M, int npad, float* __
       G[kMlpOffW2 + (ot * 16 + 4 * (l / 16) + ii) * kMlpH + kt * 16 + l % 16] = acc[ii];
     return;
   }
-  const int p = (blockIdx.x - 64) * blockDim.x + threadIdx.x;
+  const int l = threadIdx.x & 63;
+  const int p = (blockIdx.x - 64) * (blockDim.x >> 6) + (int)(threadIdx.x >> 6);
+  if (p >= kMlpOther) return;   // the whole wavefront
   float g = 0.0f;
+  int dst;
   if (p < kMlpH * kMlpIn) {                       // W1[o][i]
     const int o = p / kMlpIn, i = p % kMlpIn;
-    for (int s = 0; s < npad; ++s) g += M.dz1T[(size_t)o * npad + s] * M.xT[(size_t)i * npad + s];
-    G[kMlpOffW1 + p] = g;
+    for (int s = l; s < npad; s += 64) g += M.dz1T[(size_t)o * npad + s] * M.xT[(size_t)i * npad + s];
+    dst = kMlpOffW1 + p;
   } else if (p < kMlpH * kMlpIn + kMlpH) {        // b1
     const int o = p - kMlpH * kMlpIn;
-    for (int s = 0; s < npad; ++s) g += M.dz1T[(size_t)o * npad + s];
-    G[kMlpOffB1 + o] = g;
+    for (int s = l; s < npad; s += 64) g += M.dz1T[(size_t)o * npad + s];
+    dst = kMlpOffB1 + o;
   } else if (p < kMlpH * kMlpIn + 2 * kMlpH) {    // b2
     const int o = p - kMlpH * kMlpIn - kMlpH;
-    for (int s = 0; s < npad; ++s) g += M.dz2T[(size_t)o * npad + s];
-    G[kMlpOffB2 + o] = g;
+    for (int s = l; s < npad; s += 64) g += M.dz2T[(size_t)o * npad + s];
+    dst = kMlpOffB2 + o;
   } else if (p < kMlpH * kMlpIn + 2 * kMlpH + kMlpOut * kMlpH) {   // W3[o][k]
     const int q = p - kMlpH * kMlpIn - 2 * kMlpH, o = q / kMlpH, k = q % kMlpH;
-    for (int s = 0; s < npad; ++s) g += M.dyT[(size_t)o * npad + s] * silu(M.z2T[(size_t)k * npad + s]);
-    G[kMlpOffW3 + q] = g;
-  } else if (p < kMlpH * kMlpIn + 2 * kMlpH + kMlpOut * kMlpH + kMlpOut) {   // b3
+    for (int s = l; s < npad; s += 64) g += M.dyT[(size_t)o * npad + s] * silu(M.z2T[(size_t)k * npad + s]);
+    dst = kMlpOffW3 + q;
+  } else {                                        // b3
     const int o = p - kMlpH * kMlpIn - 2 * kMlpH - kMlpOut * kMlpH;
-    for (int s = 0; s < npad; ++s) g += M.dyT[(size_t)o * npad + s];
-    G[kMlpOffB3 + o] = g;
+    for (int s = l; s < npad; s += 64) g += M.dyT[(size_t)o * npad + s];
+    dst = kMlpOffB3 + o;
   }
+  for (int off = 32; off > 0; off >>= 1) g += __shfl_xor(g, off, 64);
+  if (l == 0) G[dst] = g;
 }
 
 // Adam step on the FP32 master parameters; refreshes the BF16 copies.

@@ -18,6 +18,8 @@ constexpr int kMlpOffB2 = kMlpOffW2 + kMlpH * kMlpH;
 constexpr int kMlpOffW3 = kMlpOffB2 + kMlpH;
 constexpr int kMlpOffB3 = kMlpOffW3 + kMlpOut * kMlpH;
 constexpr int kMlpNParam = kMlpOffB3 + kMlpOut;
+// parameters other than W2 (their gradients are per-wavefront sums)
+constexpr int kMlpOther = kMlpH * kMlpIn + 2 * kMlpH + kMlpOut * kMlpH + kMlpOut;
 
 struct MlpDev {
   float* P;                          // FP32 master parameters

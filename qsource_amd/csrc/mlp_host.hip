@@ -168,8 +168,7 @@ double Mlp::pass(const float* X, const float* Y, const int* idx, int n) {
   const int npad = (n + 63) / 64 * 64;
   hipLaunchKernelGGL(k_mlp_fwd, dim3(npad / 64), dim3(256), 0, stream, D, X, idx, n, npad, pred, 1);
   hipLaunchKernelGGL(k_mlp_bwd, dim3(npad / 64), dim3(256), 0, stream, D, Y, idx, n, npad, pred, loss_part);
-  const int nother = kMlpH * kMlpIn + 2 * kMlpH + kMlpOut * kMlpH + kMlpOut;
-  hipLaunchKernelGGL(k_mlp_wgrad, dim3(64 + (nother + 255) / 256), dim3(256), 0, stream, D, npad, grad);
+  hipLaunchKernelGGL(k_mlp_wgrad, dim3(64 + (kMlpOther + 3) / 4), dim3(256), 0, stream, D, npad, grad);
   MCHECK(hipGetLastError());
   std::vector<float> lp(npad / 16);
   MCHECK(hipMemcpyAsync(lp.data(), loss_part, lp.size() * sizeof(float), hipMemcpyDeviceToHost, stream));
@@ -293,8 +292,7 @@ static double pass_no_alloc(Mlp& m, const float* X, const float* Y, int n, float
                      pred, 1);
   hipLaunchKernelGGL(k_mlp_bwd, dim3(npad / 64), dim3(256), 0, m.stream, D, dy, (const int*)nullptr, n, (int)npad,
                      pred, loss_part);
-  const int nother = kMlpH * kMlpIn + 2 * kMlpH + kMlpOut * kMlpH + kMlpOut;
-  hipLaunchKernelGGL(k_mlp_wgrad, dim3(64 + (nother + 255) / 256), dim3(256), 0, m.stream, D, (int)npad, m.grad);
+  hipLaunchKernelGGL(k_mlp_wgrad, dim3(64 + (kMlpOther + 3) / 4), dim3(256), 0, m.stream, D, (int)npad, m.grad);
   MCHECK(hipGetLastError());
   float lp[kNoAllocMaxSamples / 16];
   MCHECK(hipMemcpyAsync(lp, loss_part, npad / 16 * sizeof(float), hipMemcpyDeviceToHost, m.stream));
