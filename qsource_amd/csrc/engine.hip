@@ -573,6 +573,31 @@ void Batch::upload(int k) {
       D.swp_d = I->upload(wd);
       D.swp_f = I->upload(wf);
       D.swp_zero = I->upload(wz);
+      if (precision_ == 1) {
+        // solute slot records for the water side of the solute-water pairs:
+        // the constant fields here, x y z written with xf
+        int nslot = 0;
+        for (int g = 0; g < G.nsolute; ++g) nslot = std::max(nslot, G.last[g] + 1);
+        std::vector<float> rec((size_t)kSrec * std::max(1, nslot), 0.0f);
+        std::vector<int> of(s.nat_solute, -1);
+        for (int g = 0; g < G.nsolute; ++g)
+          for (int a = G.first[g]; a <= G.last[g]; ++a) {
+            const int i = t.cgpatom[a];
+            if (i < 0 || i >= s.nat_solute) throw std::runtime_error("solute charge group holds a solvent atom");
+            of[i] = a;
+            float* r = rec.data() + (size_t)kSrec * a;
+            for (int c = 0; c < 3; ++c) r[c] = (float)s.x[3 * i + c];
+            r[3] = (float)s.crg[i];
+            for (int c = 0; c < 6; ++c) r[4 + c] = wf[6 * i + c];
+            r[10] = s.iqatom[i] ? 1.0f : 0.0f;
+          }
+        // every solute atom must have its slot (each is in one charge group);
+        // otherwise the pairs take the per-atom path
+        if (std::find(of.begin(), of.end(), -1) == of.end()) {
+          D.srec = I->upload(rec);
+          D.srec_of_atom = I->upload(of);
+        }
+      }
     }
   }
   D.rwat = s.rwat; D.fk_wsphere = s.fk_wsphere; D.dwmz = s.dwmz; D.awmz = s.awmz; D.fkwpol = s.fkwpol;

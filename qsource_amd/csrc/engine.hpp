@@ -68,6 +68,7 @@ constexpr int kXtInts = 10;
 constexpr int kXtPars = 8;
 constexpr int kMorseRec = 7;   // Morse factor and its gradient on the bond's two atoms
 constexpr int kQwRec = 8;      // Q-water record per Q atom (qw_rec_d / qw_rec_f)
+constexpr int kSrec = 12;   // floats per solute slot record (x y z q, a0 a1 a2 b0, b1 b2 skip pad)
 constexpr int kEWater = 5;     // energy partials per water
 constexpr int kMon = 3;        // Vwsum, Vwel, Vwlj before the per-state values
 constexpr int kIntegrateBlock = 64;  // threads per k_integrate workgroup
@@ -156,6 +157,13 @@ struct DevSys {
   const double* swp_d;
   const float* swp_f;
   const unsigned char* swp_zero;
+  // mixed precision with standard waters: one kSrec-float record per solute
+  // charge-group slot a (the atom cgpatom[a]), read by the water side of the
+  // solute-water pairs in slot order: FP32 x, y, z (refreshed with xf), the
+  // charge, the LJ products with water atoms O, H1, H2 (a then b) and 1.0 in
+  // the skip field for a Q atom; srec_of_atom maps a solute atom to its slot
+  float* srec;
+  const int* srec_of_atom;
   double aOO, bOO;               // alj_solv(1,iLJ)^2, blj_solv(1,iLJ)^2 for iLJ=ljcod(O,O)
   // all-atom solvent LJ (topology solvent type 1): LJ between every atom
   // pair of two waters, A of pair (la, ka) = aLJ_solv(la,iLJ) aLJ_solv(ka,iLJ)

@@ -39,6 +39,11 @@ __global__ void k_x_to_f32(const DevSys* __restrict__ sys) {
   const DevSys& S = sys[blockIdx.y];
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c < 3 * S.natom) S.xf[c] = (float)S.x[c];
+  // the solute slot records (mixed precision, standard waters)
+  if (S.srec && c < S.nat_solute) {
+    float* r = S.srec + (size_t)kSrec * S.srec_of_atom[c];
+    for (int k = 0; k < 3; ++k) r[k] = (float)S.x[3 * c + k];
+  }
   // the water records (mixed precision)
   if (S.xw && c < S.nwat) water_xf_store(S, c, S.x + 3 * (S.nat_solute + 3 * c));
 }
@@ -203,26 +208,25 @@ template __global__ void k_water_lists<float, 1>(const DevSys*, const int2*);
 // turn, lanes 0-2 add the LRF Taylor term of atoms O, H1, H2 and lane 3 the
 // radial restraint; the water's 9 gradient components are summed over its 8
 // lanes and written once.  P is the pair-arithmetic type: double (plain,
-// reference operation order inside each pair) or float (mixed: FP32
-// coordinates and FMA pair arithmetic, each partner's contribution
-// accumulated in FP64).
+// reference operation order inside each pair, every partner's contribution
+// added to the FP64 gradient) or float (mixed: FP32 coordinates and FMA pair
+// arithmetic; a lane sums its contributions in FP32 over a tile: all its
+// partner waters, one solute group, or its Q atoms, and adds each tile's sum
+// to the FP64 gradient, so the FP64 gradient is not held in registers while
+// the partner waters are visited).
 // W: wavefronts per SIMD the FP32 build is fitted to (6: registers spill to
 // scratch; 4: no spills); which is faster is measured by the launch tuner.
 // AA: all-atom solvent LJ (nonbond_3atomsolvent :15238-15333 and
 // nonbond_qw_3atom :14749-14874): LJ on all nine atom pairs of two waters
 // and between each Q atom and every water atom.
 // en != 0: the launch also forms the energy partials (e_water); steps whose
-// energies are not read launch with en = 0 (the gradient is the same).
-template <typename P, int MAPPED, int PF, int W, int AA>
-__global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(const DevSys* __restrict__ sys,
-                                                                              const int2* __restrict__ map,
-                                                                              int en) {
-  const DevSys* Sp;
-  int base;
-  if (!block_work<MAPPED>(sys, map, blockDim.x >> 3, Sp, base)) return;
-  const DevSys& S = *Sp;
+// energies are not read launch with en = 0 (the gradient is the same).  The
+// two cases are separate instantiations of the body (EN), so a force-only
+// step carries no energy arithmetic.
+template <typename P, int PF, int AA, bool EN>
+__device__ __forceinline__ void water_forces_body(const DevSys& S, const int base) {
+  constexpr bool F32 = sizeof(P) == 4;
   const int t = threadIdx.x;
-  if (base + ((t >> 6) << 3) >= S.nwat) return;   // whole wavefronts past the last water
   const int sub = t & 7;
   const int iw = base + (t >> 3);
   const bool active = iw < S.nwat;
@@ -230,10 +234,26 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(co
   const gconst<P> X = gptr(pair_x<P>(S));
   double g[9];
   for (int k = 0; k < 9; ++k) g[k] = 0.0;
+  // FP32 build: the current tile's gradient, joined to g by flush()
+  float gf[9];
+  for (int k = 0; k < 9; ++k) gf[k] = 0.0f;
+  auto flush = [&]() {
+    if constexpr (F32) {
+      for (int k = 0; k < 9; ++k) {
+        g[k] = g[k] + (double)gf[k];
+        gf[k] = 0.0f;
+      }
+    }
+  };
+  // add c to gradient component k (a constant index after unrolling)
+  auto gadd = [&](const int k, const P c) {
+    if constexpr (F32) gf[k] = gf[k] + c;
+    else g[k] = g[k] + c;
+  };
   double e_el = 0.0, e_vdw = 0.0, e_lrf = 0.0, e_rad = 0.0;
   // coordinates of water w: FP64, or the FP32 water record
   auto load_water = [&](const int w, P* o) {
-    if constexpr (sizeof(P) == 4) {
+    if constexpr (F32) {
       water_xf(S, w, o);
     } else {
       const int wo = S.nat_solute + 3 * w;
@@ -275,47 +295,55 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(co
           else r6 = r2 * r2 * r2;
           const P Va = (AA != 0 ? aww[3 * la + ip] : aOO) * r6 * r6;
           const P Vb = (AA != 0 ? bww[3 * la + ip] : bOO) * r6;
-          vdw = vdw + Va - Vb;
+          if constexpr (EN) vdw = vdw + Va - Vb;
           dv = dv + r2 * ((P)-12 * Va + (P)6 * Vb);
         }
-        el = el + Vel;
+        if constexpr (EN) el = el + Vel;
         gl0 = pmsub(dv, dx1, gl0);
         gl1 = pmsub(dv, dx2, gl1);
         gl2 = pmsub(dv, dx3, gl2);
       }
-      g[3 * la] = g[3 * la] + (double)gl0;
-      g[3 * la + 1] = g[3 * la + 1] + (double)gl1;
-      g[3 * la + 2] = g[3 * la + 2] + (double)gl2;
+      gadd(3 * la, gl0);
+      gadd(3 * la + 1, gl1);
+      gadd(3 * la + 2, gl2);
     }
-    if (en && owns_pair(iw, jw)) {
+    if (EN && owns_pair(iw, jw)) {
       e_el = e_el + (double)el;
       e_vdw = e_vdw + (double)vdw;
     }
   };
   if constexpr (PF != 0) {
-    // software pipeline over the lane's partners (same partners, same
-    // order): while a partner's pairs are computed, the next partner's
-    // coordinates and the index of the one after are in flight, so neither
-    // load is waited on before the arithmetic
+    // software pipeline over the lane's partners k, k+8, k+16, ... (the
+    // same partners in the same order as without it): two coordinate
+    // buffers alternate, so while one partner's pairs are computed the next
+    // partner's coordinates and the index of the one after are in flight
     int k = sub;
-    int jwn = 0, jwn2 = 0;
-    P xn[9];
+    int j0 = 0, j1 = 0, jn = 0;
+    P x0[9], x1[9];
     if (k < nn) {
-      jwn = row[k];
-      load_water(jwn, xn);
+      j0 = row[k];
+      load_water(j0, x0);
     }
-    if (k + 8 < nn) jwn2 = row[k + 8];
+    if (k + 8 < nn) {
+      j1 = row[k + 8];
+      load_water(j1, x1);
+    }
+    if (k + 16 < nn) jn = row[k + 16];
     while (k < nn) {
-      P xj[9];
-      for (int u = 0; u < 9; ++u) xj[u] = xn[u];
-      const int jw = jwn;
-      k += 8;
-      if (k < nn) {
-        jwn = jwn2;
-        load_water(jwn, xn);
-        if (k + 8 < nn) jwn2 = row[k + 8];
+      ww_pair(j0, x0);
+      if (k + 16 < nn) {
+        j0 = jn;
+        load_water(j0, x0);
       }
-      ww_pair(jw, xj);
+      if (k + 24 < nn) jn = row[k + 24];
+      if (k + 8 >= nn) break;
+      ww_pair(j1, x1);
+      if (k + 24 < nn) {
+        j1 = jn;
+        load_water(j1, x1);
+      }
+      if (k + 32 < nn) jn = row[k + 32];
+      k += 16;
     }
   } else {
     for (int k = sub; k < nn; k += 8) {
@@ -325,10 +353,40 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(co
       ww_pair(jw, xj);
     }
   }
+  flush();
   // water-solute: every non-Q atom of each listed solute group (the solute
   // atom is i of the reference pair)
   const int nwp = active && (S.term_mask & TM_PW) ? S.wp_cnt[iw] : 0;
   const gconst<int> wrow = gptr(S.wp_nb) + (size_t)iw * S.wp_cap;
+  if (F32 && S.srec) {
+    // FP32 records in slot order: the group's atoms are consecutive records,
+    // so no load waits on another (the same pairs and arithmetic as below)
+    for (int k = sub; k < nwp; k += 8) {
+      const int gs = wrow[k];
+      const int a0 = S.cgp_first[gs], a1 = S.cgp_last[gs];
+      const gconst<f32v4> rec = gptr(reinterpret_cast<const f32v4*>(S.srec));
+      for (int a = a0; a <= a1; ++a) {
+        const f32v4 r0 = rec[3 * (size_t)a], r1 = rec[3 * (size_t)a + 1], r2v = rec[3 * (size_t)a + 2];
+        if (r2v.z != 0.0f) continue;   // a Q atom
+        const P ra[3] = {(P)r1.x, (P)r1.y, (P)r1.z}, rb[3] = {(P)r1.w, (P)r2v.x, (P)r2v.y};
+        for (int ip = 0; ip < 3; ++ip) {
+          const P dx1 = xi[3 * ip] - (P)r0.x, dx2 = xi[3 * ip + 1] - (P)r0.y, dx3 = xi[3 * ip + 2] - (P)r0.z;
+          P r;
+          const P r2 = inv_r2(pnorm2(dx1, dx2, dx3), r);
+          const P r6 = r2 * r2 * r2;
+          const P r12 = r6 * r6;
+          const P Vel = (P)r0.w * (P)S.chg_solv[ip] * r;
+          const P Va = ra[ip] * r12;
+          const P Vb = rb[ip] * r6;
+          const P dv = r2 * (-Vel - (P)12 * Va + (P)6 * Vb);
+          gadd(3 * ip, dv * dx1);
+          gadd(3 * ip + 1, dv * dx2);
+          gadd(3 * ip + 2, dv * dx3);
+        }
+      }
+      flush();
+    }
+  } else {
   for (int k = sub; k < nwp; k += 8) {
     const int gs = wrow[k];
     for (int a = S.cgp_first[gs]; a <= S.cgp_last[gs]; ++a) {
@@ -359,11 +417,13 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(co
         const P Va = aLJ * r12;
         const P Vb = bLJ * r6;
         const P dv = r2 * (-Vel - (P)12 * Va + (P)6 * Vb);
-        g[3 * ip] = g[3 * ip] + (double)(dv * dx1);
-        g[3 * ip + 1] = g[3 * ip + 1] + (double)(dv * dx2);
-        g[3 * ip + 2] = g[3 * ip + 2] + (double)(dv * dx3);
+        gadd(3 * ip, dv * dx1);
+        gadd(3 * ip + 1, dv * dx2);
+        gadd(3 * ip + 2, dv * dx3);
       }
     }
+    flush();
+  }
   }
   // Q-water pairs seen from the water, Q atoms taken in turn by the lanes
   // (the Q atom is i of each pair): O with LJ + Coulomb, H Coulomb only (LJ
@@ -411,11 +471,12 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(co
             dv = -q2 * Vel;
           }
         }
-        g[3 * ip] = g[3 * ip] + (double)(dv * ex1);
-        g[3 * ip + 1] = g[3 * ip + 1] + (double)(dv * ex2);
-        g[3 * ip + 2] = g[3 * ip + 2] + (double)(dv * ex3);
+        gadd(3 * ip, dv * ex1);
+        gadd(3 * ip + 1, dv * ex2);
+        gadd(3 * ip + 2, dv * ex3);
       }
     }
+    flush();
   }
   // LRF Taylor expansion (FP64), atom `sub` of the water on lanes 0..2
   if (active && S.use_lrf && sub < 3 && (S.term_mask & TM_LRF)) {
@@ -425,7 +486,7 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(co
       const double xa[3] = {S.x[3 * i], S.x[3 * i + 1], S.x[3 * i + 2]};
       double df[3];
       const double V = lrf_taylor_atom(L, xa, df);
-      e_lrf = 0.5 * S.crg[i] * V;
+      if constexpr (EN) e_lrf = 0.5 * S.crg[i] * V;
       for (int c = 0; c < 3; ++c) {
         const double f = S.crg[i] * df[c];
         if (sub == 0) g[c] = g[c] - f;
@@ -476,7 +537,7 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(co
       }
   }
   for (int k = 0; k < 9; ++k) g[k] = seg8_sum(g[k]);
-  if (en) {
+  if constexpr (EN) {
     e_el = seg8_sum(e_el);
     e_vdw = seg8_sum(e_vdw);
     e_lrf = seg8_sum(e_lrf);
@@ -485,11 +546,24 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(co
   }
   if (active && sub == 0) {
     for (int k = 0; k < 9; ++k) S.d[3 * io + k] = g[k];
-    if (en) {
+    if constexpr (EN) {
       double* ew = S.e_water + (size_t)iw * kEWater;
       ew[0] = e_el; ew[1] = e_vdw; ew[2] = e_lrf; ew[3] = e_rad; ew[4] = e_fix;
     }
   }
+}
+
+template <typename P, int MAPPED, int PF, int W, int AA>
+__global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(const DevSys* __restrict__ sys,
+                                                                              const int2* __restrict__ map,
+                                                                              int en) {
+  const DevSys* Sp;
+  int base;
+  if (!block_work<MAPPED>(sys, map, blockDim.x >> 3, Sp, base)) return;
+  const DevSys& S = *Sp;
+  if (base + ((threadIdx.x >> 6) << 3) >= S.nwat) return;   // whole wavefronts past the last water
+  if (en) water_forces_body<P, PF, AA, true>(S, base);
+  else water_forces_body<P, PF, AA, false>(S, base);
 }
 template __global__ void k_water_forces<double, 0, 0, 6, 0>(const DevSys*, const int2*, int);
 template __global__ void k_water_forces<double, 0, 0, 6, 1>(const DevSys*, const int2*, int);
@@ -921,6 +995,7 @@ __global__ __launch_bounds__(kIntegrateBlock) void k_integrate(const DevSys* __r
         if (S.has_shake) S.v[ic] = (S.x[ic] - S.xx[ic]) / dt;
         vv[c] = S.v[ic];
         S.xf[ic] = (float)S.x[ic];
+        if (S.srec && i < S.nat_solute) S.srec[(size_t)kSrec * S.srec_of_atom[i] + c] = (float)S.x[ic];
         if (S.xw && i >= S.nat_solute) {
           const int rel = i - S.nat_solute;
           S.xw[16 * (rel / 3) + 3 * (rel % 3) + c] = (float)S.x[ic];
