@@ -1,6 +1,6 @@
 """Benchmark: aggregate ns/day of batched FEP lambda windows on one MI355X.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3|c5]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3|c4|c5]
                 [--accuracy-steps N [--accuracy-control]]
 
 Default workload (c3): three solvated systems (TIP3P spheres of radius 18,
@@ -12,7 +12,11 @@ instance (pair lists every 10 steps, all forces, leapfrog/Berendsen, SHAKE,
 temperature), coordinates resident in HBM, per-state Q energies recorded on
 the device every 10 steps.  Config c5: one EVB-style system (R = 40 A
 sphere, 5000-atom solute, 24 Q atoms, 4 states with Q-atom types and
-off-diagonal couplings) as 16 mapping windows.
+off-diagonal couplings) as 16 mapping windows.  Config c4: one system (R =
+30 A sphere, 2000-atom solute) as 11 lambda windows with the learned far
+field (trained on the GPU on 200 frames of exact far-field targets), its
+gradient error against exact all-pairs evaluation next to LRF and the plain
+cut-off, and dG over the windows with each far-field treatment.
 
 value = instances x steps x dt_fs x 1e-6 x 86400 / wall seconds of the
 timed region (mixed precision: FP32 pair arithmetic, FP64 accumulation).
@@ -144,9 +148,75 @@ def build_c5(d, equil=True, windows=16):
     return inputs, [list(range(windows))], [lam], cfg
 
 
-def c5_surrogate(inp, nframes=40, stride=25):
-    """The far-field surrogate of the EVB workload: trained on the GPU on
-    exact FP64 far-field targets of `nframes` frames of the first window."""
+def build_c4(d, equil=True, windows=11):
+    """The learned far-field workload: one system (R = 30 A TIP3P sphere
+    around a 2000-atom solute, 12 Q atoms, 2 FEP states, built like the
+    batched-window systems) as `windows` lambda windows (lambda = 0, 0.1,
+    ..., 1); the windows also as plain cut-off inputs (LRF off), and the
+    system, its restart and the directory for the far-field comparison."""
+    from qsource_amd import equilibrate, gensys
+    t0 = time.time()
+    s = gensys.solvated_system(radius=30.0, n_solute=2000, seed=4000, n_q=12, n_states=2)
+    log("c4 system: R=30 A, 2000 solute atoms, %d atoms, %d waters, built in %.1f s" %
+        (s.natom, s.nwat, time.time() - t0))
+    lams = [round(k / (windows - 1), 4) for k in range(windows)]
+    rst = equilibrate.equilibrate(s, d, "c4", lambdas=[0.5, 0.5], seed=31, progress=log) if equil else None
+    inputs, cut = [], []
+    for k, lam in enumerate(lams):
+        kw = dict(steps=1000000, output=1000000, lambdas=[1.0 - lam, lam], initial_temperature=300.0,
+                  random_seed=301 + k)
+        if rst:
+            kw["restart"] = os.path.basename(rst)
+        inputs.append(gensys.write_system(s, d, "c4_w%02d" % k, **kw)["inp"])
+        cut.append(gensys.write_system(s, d, "c4_cut_w%02d" % k, lrf=False, **kw)["inp"])
+    cfg = {"workload": "learned_far_field_windows", "systems": "R30/2000 solute atoms",
+           "atoms_per_system": [s.natom], "waters": s.nwat, "windows_per_system": windows, "instances": windows,
+           "q_atoms": 12, "states": 2, "stepsize_fs": 1.0, "nbcycle": 10, "energy_interval": IENE,
+           "far_field": "learned surrogate (BF16-MFMA MLP) in place of LRF"}
+    lam = np.array([[1.0 - l, l] for l in lams])
+    extra = {"system": s, "restart": rst, "dir": d, "cutoff_inputs": cut}
+    return inputs, [list(range(windows))], [lam], cfg, extra
+
+
+def c4_farfield(extra, model, inputs, groups, lams, ser_sur, steps, warmup):
+    """Far-field treatments of the c4 system compared: gradient errors
+    against exact all-pairs evaluation on held-out frames (farfield.
+    force_errors), and dG over the windows with the surrogate, with LRF and
+    with the plain cut-off (independent mixed-precision trajectories of the
+    same length, block-bootstrap errors)."""
+    from qsource_amd import engine as qe
+    from qsource_amd import farfield
+    t0 = time.time()
+    tin = farfield.treatment_inputs(extra["system"], extra["dir"], "c4_ff", extra["restart"], [0.5, 0.5])
+    frames = farfield.held_out_frames(tin["lrf"], nframes=10, stride=100, gap=5)
+    fe = farfield.force_errors(tin, frames, model)
+    log("far-field force errors vs exact (%.1f s): %s" % (time.time() - t0, json.dumps(fe)))
+    res = {"force_error_vs_exact": fe}
+    sers = {"surrogate": ser_sur}
+    for name, inp in (("lrf", inputs), ("cutoff", extra["cutoff_inputs"])):
+        e, wall, warm = run_batch(inp, qe.QGPU_MIXED, steps, warmup)
+        sers[name] = series(e, groups, warm // IENE)
+        e.close()
+        log("c4 %s batch: %d steps" % (name, steps))
+    dg = {}
+    for name, ser in sers.items():
+        z = dg_of(ser, lams)[0]
+        b = bootstrap_dg(ser, lams)[0]
+        dg[name] = {"dG_zwanzig": z[0], "dG_bar": z[1], "stderr": float(np.std(b)) if b is not None else None}
+    for name in ("surrogate", "cutoff"):
+        a, r = dg[name], dg["lrf"]
+        dg[name]["ddG_vs_lrf"] = a["dG_zwanzig"] - r["dG_zwanzig"]
+        dg[name]["stderr_ddG_vs_lrf"] = (float(np.hypot(a["stderr"], r["stderr"]))
+                                         if a["stderr"] is not None and r["stderr"] is not None else None)
+    res["dG_windows"] = dg
+    res["frames_per_window"] = int(min(len(w) for w in ser_sur[0]))
+    return res
+
+
+def far_surrogate(inp, nframes=40, stride=25):
+    """A far-field surrogate trained on the GPU on exact FP64 far-field
+    targets of `nframes` frames, `stride` steps apart, of the FP64 run of
+    window `inp`."""
     from qsource_amd import farfield
     t0 = time.time()
     Xs, Ys, _ = farfield.sample_frames(inp, nframes, stride)
@@ -444,7 +514,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--config", default="c3", choices=["c3", "c5"])
+    ap.add_argument("--config", default="c3", choices=["c3", "c4", "c5"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-procs", type=int, default=8)
     ap.add_argument("--profile-steps", type=int, default=20)
@@ -470,8 +540,11 @@ def main():
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "fp32-pair/fp64-accumulate", "data": "synthetic"}
     with tempfile.TemporaryDirectory() as d:
+        c4x = None
         if args.config == "c3":
             inputs, groups, lams, cfg = build_c3(d)
+        elif args.config == "c4":
+            inputs, groups, lams, cfg, c4x = build_c4(d)
         else:
             inputs, groups, lams, cfg = build_c5(d)
         n = len(inputs)
@@ -485,7 +558,10 @@ def main():
             log("CPU baseline: %s" % cpu)
         model = None
         if args.config == "c5":
-            model, cfg["surrogate"] = c5_surrogate(inputs[0])
+            model, cfg["surrogate"] = far_surrogate(inputs[0])
+        elif args.config == "c4":
+            # 200 frames of the middle window's FP64 run
+            model, cfg["surrogate"] = far_surrogate(inputs[len(inputs) // 2], nframes=200, stride=25)
         log("mixed batch: %d instances" % n)
         em, wall, warm = run_batch(inputs, qe.QGPU_MIXED, args.steps, args.warmup, far_model=model)
         skip = warm // IENE
@@ -509,6 +585,9 @@ def main():
         roof = roofs.get(dom, roofs["water_forces"])
         roof = dict(roof, dominant_kernel=dom)
         em.close()
+        farfield_res = None
+        if c4x is not None:
+            farfield_res = c4_farfield(c4x, model, inputs, groups, lams, ser_m, args.steps, args.warmup)
         ferr = force_error([inputs[k] for k in mid], states)
         log("force RMS error per term (mixed vs FP64, same coordinates): %s" %
             [{t: round(v["rms_rel"], 9) for t, v in f["per_term"].items()} for f in ferr])
@@ -567,6 +646,8 @@ def main():
                 "autotune": tuned, "plain_fp64_one_window_at_a_time_ns_per_day": plain,
                 "plain_fp64_ns_per_day_per_system": plain_per_system,
                 "speedup_vs_plain_fp64": value / plain, "accuracy": acc})
+    if farfield_res is not None:
+        out["farfield"] = farfield_res
     print(json.dumps(out))
 
 
