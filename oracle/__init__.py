@@ -41,7 +41,8 @@ def _load():
                      "oracle_final_energy", "oracle_get_xv", "oracle_get_d", "oracle_get_energies",
                      "oracle_get_qenergies", "oracle_list_sizes", "oracle_get_list", "oracle_get_lrf",
                      "oracle_temperature", "oracle_theta_corr", "oracle_nshells", "oracle_offdiag",
-                     "oracle_sizes", "oracle_pot_energy_nonbonds", "oracle_shake", "oracle_pot_energy"]:
+                     "oracle_sizes", "oracle_pot_energy_nonbonds", "oracle_shake", "oracle_pot_energy",
+                     "oracle_box"]:
             getattr(lib, name).argtypes = None
         lib.oracle_temperature.restype = ctypes.c_double
         _lib = lib
@@ -169,6 +170,13 @@ class Oracle:
         out = np.zeros(max(n, 1))
         self.lib.oracle_theta_corr(self.h, _dp(out))
         return out[:n]
+
+    def box(self):
+        """Periodic box: (box lengths, box centre), or None for a sphere."""
+        out = np.zeros(6)
+        if not self.lib.oracle_box(self.h, _dp(out)):
+            return None
+        return out[:3].copy(), out[3:].copy()
 
     def offdiag(self) -> np.ndarray:
         """(noffd, 2): H_ij and r_kl of each [off_diagonals] line at the

@@ -24,6 +24,7 @@
 //   qbond 17192-17245, qangle 17022-17114, qtorsion 17395-17540,
 //   qimproper 17249-17392
 //   newvel_ber 6194-6223, shake 17581-17679, temperature 6098-6190
+#include <array>
 #include "../qsource_amd/csrc/host/qsys.hpp"
 
 #include <cmath>
@@ -39,8 +40,9 @@ namespace {
 
 const double kPi = 4.0 * std::atan(1.0);
 
-struct NB { int i, j, ljcod; };
-struct NBQ { int iq, j, ljcod; };  // Q-atom number, partner atom (or first atom of water)
+// cgp: periodic box only, the charge-group pair of the entry (its shift)
+struct NB { int i, j, ljcod; int cgp = -1; };
+struct NBQ { int iq, j, ljcod; int cgp = -1; };  // Q-atom number, partner atom (or first atom of water)
 
 struct LRF {
   double cent[3];
@@ -69,6 +71,11 @@ struct Oracle {
   std::vector<NB> nbpp, nbpw, nbww;
   std::vector<NBQ> nbqp, nbqw;
   bool qp_done = false, qw_done = false;
+  // periodic box: the switching atoms of each listed charge-group pair
+  // (nbpp_cgp, nbpw_cgp) and the partner atom of each Q-solute group
+  // (nbqp_cgp); shifts formed from them at every force evaluation
+  std::vector<std::array<int, 2>> pp_cgp, pw_cgp;
+  std::vector<int> qp_cgp;
   std::vector<LRF> lrf;
   Energies E;
   std::vector<QEnergies> EQ;
@@ -126,6 +133,12 @@ struct Oracle {
       if ((p.first == i && p.second == j) || (p.first == j && p.second == i)) r = true;
     return r;
   }
+  // the box shift of one component: boxlength * nint(d / boxlength)
+  // (Fortran nint rounds half away from zero, as std::round)
+  double bshift(double dd, int k) const { return s.boxl[k] * std::round(dd * s.inv_boxl[k]); }
+  // minimum image of one component
+  double mimage(double dd, int k) const { return dd - s.boxl[k] * std::round(dd * s.inv_boxl[k]); }
+
   static bool skip_parity(int ig, int jg) {
     // 1-based parity rule (src/md.f90:10977-10980) with 0-based arguments
     int a = ig + 1, b = jg + 1;
@@ -155,11 +168,18 @@ struct Oracle {
   void lrf_update(int g1, int g2) {
     const auto& T = s.top;
     LRF& L = lrf[g2];
+    // periodic box: the shift of group 1's switching atom against group 2's
+    // centre (src/md.f90:5904-5917)
+    double sh[3] = {0.0, 0.0, 0.0};
+    if (s.pbc) {
+      const int w3 = 3 * T.cgp[g1].iswitch;
+      for (int k = 0; k < 3; ++k) sh[k] = bshift(x[w3 + k] - L.cent[k], k);
+    }
     for (int ia = T.cgp[g1].first; ia <= T.cgp[g1].last; ++ia) {
       int i = T.cgpatom[ia];
       if (s.iqatom[i] != 0) continue;
       int i3 = 3 * i;
-      double dr[3] = {x[i3] - L.cent[0] - 0.0, x[i3 + 1] - L.cent[1] - 0.0, x[i3 + 2] - L.cent[2] - 0.0};
+      double dr[3] = {x[i3] - L.cent[0] - sh[0], x[i3 + 1] - L.cent[1] - sh[1], x[i3 + 2] - L.cent[2] - sh[2]};
       double r2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
       double f0 = s.crg[i] / (r2 * std::sqrt(r2));
       double f1 = 3.0 * f0 / r2;
@@ -194,17 +214,25 @@ struct Oracle {
     const int ns = s.nat_solute;
     for (int iw = 0; iw < s.nwat; ++iw) {
       int is = ns + 3 * iw;
-      if (s.excl[is]) continue;
+      if (!s.pbc && s.excl[is]) continue;
       int is3 = 3 * is;
       int ig = s.iwhich_cgp[is];
       for (int jw = 0; jw < s.nwat; ++jw) {
         int ja = ns + 3 * jw;
-        if (s.excl[ja]) continue;
+        if (!s.pbc && s.excl[ja]) continue;
         int jg = s.iwhich_cgp[ja];
         int j3 = 3 * ja;
         if (skip_parity(iw, jw) || iw == jw) continue;
-        double r2 = (x[is3] - x[j3]) * (x[is3] - x[j3]) + (x[is3 + 1] - x[j3 + 1]) * (x[is3 + 1] - x[j3 + 1])
-                  + (x[is3 + 2] - x[j3 + 2]) * (x[is3 + 2] - x[j3 + 2]);
+        double r2;
+        if (s.pbc) {
+          // nbwwlist_box(_lrf) (src/md.f90:10793-10895, 11023-11148)
+          const double dx = mimage(x[is3] - x[j3], 0), dy = mimage(x[is3 + 1] - x[j3 + 1], 1),
+                       dz = mimage(x[is3 + 2] - x[j3 + 2], 2);
+          r2 = dx * dx + dy * dy + dz * dz;
+        } else {
+          r2 = (x[is3] - x[j3]) * (x[is3] - x[j3]) + (x[is3 + 1] - x[j3 + 1]) * (x[is3 + 1] - x[j3 + 1])
+             + (x[is3 + 2] - x[j3 + 2]) * (x[is3 + 2] - x[j3 + 2]);
+        }
         if (r2 <= rcut2) {
           for (int la = 0; la < 3; ++la)
             for (int ka = 0; ka < 3; ++ka)
@@ -222,6 +250,15 @@ struct Oracle {
   // same lists as their _lrf forms and collect no multipoles
   void make_pair_lists() {
     if (s.inp.use_lrf) cgp_centers();
+    if (s.pbc) {
+      // periodic box (src/md.f90:6003-6029)
+      nbpplist_box();
+      nbpwlist_box();
+      nbqplist_box();
+      nbwwlist_lrf();
+      nbqwlist_box();
+      return;
+    }
     nbpplist_lrf();
     nbpwlist_lrf();
     nbqplist();
@@ -229,15 +266,75 @@ struct Oracle {
     nbqwlist();
   }
 
+  // put_back_in_box (src/md.f90:18743-18930): the box centre follows the
+  // solute (or all atoms) unless it is rigid; molecules whose centre of
+  // mass left the box move by a box length per axis; with LRF the moved
+  // molecules' group centres are formed again
+  void put_back_in_box() {
+    const auto& T = s.top;
+    double boxc[3];
+    if (!s.inp.rigid_box_centre) {
+      const int last = s.nat_solute > 0 ? s.nat_solute : s.natom;
+      boxc[0] = boxc[1] = boxc[2] = 0.0;
+      for (int i = 0; i < last; ++i)
+        for (int k = 0; k < 3; ++k) boxc[k] = boxc[k] + x[3 * i + k];
+      for (int k = 0; k < 3; ++k) boxc[k] = boxc[k] / (double)last;
+      for (int k = 0; k < 3; ++k) s.boxc[k] = boxc[k];
+    } else {
+      for (int k = 0; k < 3; ++k) boxc[k] = s.boxc[k];
+    }
+    double hi[3], lo[3];
+    for (int k = 0; k < 3; ++k) { hi[k] = boxc[k] + s.boxl[k] / 2; lo[k] = boxc[k] - s.boxl[k] / 2; }
+    const int m0 = s.inp.put_solute_back ? 0 : s.nsolute_mol;
+    const int m1 = s.inp.put_solvent_back ? s.nmol : s.nsolute_mol;
+    std::vector<char> moved(s.nmol, 0);
+    for (int m = m0; m < m1; ++m) {
+      double cm[3] = {0.0, 0.0, 0.0};
+      const int a0 = T.istart_mol[m], a1 = T.istart_mol[m + 1];
+      for (int j = a0; j < a1; ++j)
+        for (int k = 0; k < 3; ++k) cm[k] = cm[k] + x[3 * j + k] * s.mass[j];
+      for (int k = 0; k < 3; ++k) cm[k] = cm[k] * s.mol_minv[m];
+      for (int k = 0; k < 3; ++k) {
+        if (cm[k] > hi[k]) {
+          for (int j = a0; j < a1; ++j) x[3 * j + k] = x[3 * j + k] - s.boxl[k];
+          moved[m] = 1;
+        } else if (cm[k] < lo[k]) {
+          for (int j = a0; j < a1; ++j) x[3 * j + k] = x[3 * j + k] + s.boxl[k];
+          moved[m] = 1;
+        }
+      }
+    }
+    if (s.inp.use_lrf)
+      for (int m = m0; m < m1; ++m) {
+        if (!moved[m]) continue;
+        const int g0 = s.iwhich_cgp[T.istart_mol[m]], g1 = s.iwhich_cgp[T.istart_mol[m + 1] - 1];
+        for (int ig = g0; ig <= g1; ++ig) {
+          LRF& L = lrf[ig];
+          L.cent[0] = L.cent[1] = L.cent[2] = 0.0;
+          for (int a = T.cgp[ig].first; a <= T.cgp[ig].last; ++a)
+            for (int k = 0; k < 3; ++k) L.cent[k] = L.cent[k] + x[3 * T.cgpatom[a] + k];
+          for (int k = 0; k < 3; ++k) L.cent[k] = L.cent[k] / (double)(T.cgp[ig].last - T.cgp[ig].first + 1);
+        }
+      }
+  }
+
   // solute lists: filled in below
   void nbpplist_lrf();
   void nbpwlist_lrf();
   void nbqplist();
   void nbqwlist();
+  void nbpplist_box();
+  void nbpwlist_box();
+  void nbqplist_box();
+  void nbqwlist_box();
 
   // ---------------------------------------------------------------- forces
+  // with a periodic box (nonbond_ww_spc_box, src/md.f90:15123-15236) the
+  // shift of the water pair is formed from the O-O vector of its first row
+  // and kept for the pair's other rows
   void nonbond_ww_spc() {
     int ichg = 1;
+    double bs[3] = {0.0, 0.0, 0.0};
     const size_t n = nbww.size();
     for (size_t iw = 0; iw < n; iw += 3) {
       if (ichg > 3) ichg = 1;
@@ -248,6 +345,10 @@ struct Oracle {
         int ja = j + ip, j3 = 3 * ja;
         double c2 = s.chg_solv[ip];
         double dx1 = x[j3] - x[i3], dx2 = x[j3 + 1] - x[i3 + 1], dx3 = x[j3 + 2] - x[i3 + 2];
+        if (s.pbc) {
+          if (ichg == 1 && ip == 0) { bs[0] = bshift(dx1, 0); bs[1] = bshift(dx2, 1); bs[2] = bshift(dx3, 2); }
+          dx1 = dx1 - bs[0]; dx2 = dx2 - bs[1]; dx3 = dx3 - bs[2];
+        }
         double r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
         double r6 = r2 * r2 * r2;
         r2 = 1.0 / r2;
@@ -274,7 +375,8 @@ struct Oracle {
 
   void lrf_taylor() {
     for (int i = 0; i < s.natom; ++i) {
-      if (s.excl[i] || s.iqatom[i] != 0) continue;
+      // excluded atoms take part in a periodic box (src/md.f90:5837)
+      if ((!s.pbc && s.excl[i]) || s.iqatom[i] != 0) continue;
       int i3 = 3 * i;
       const LRF& L = lrf[s.iwhich_cgp[i]];
       double dr[3] = {L.cent[0] - x[i3], L.cent[1] - x[i3 + 1], L.cent[2] - x[i3 + 2]};
@@ -525,6 +627,7 @@ struct Oracle {
     else nonbond_qp();
     if (s.natom > s.nat_solute) {
       if (s.top.solvent_type == 1) {
+        if (s.pbc) throw std::runtime_error("3-atom solvent in a periodic box is not supported");
         nonbond_3atomsolvent();
         nonbond_qw_3atom();
       } else {
@@ -558,9 +661,10 @@ struct Oracle {
       E.p_improper = improper(0, s.nimps_solute);
       E.w_improper = improper(s.nimps_solute, (int)s.imp.size());
     }
-    fix_shell();
+    // no shell or solvent restraints in a periodic box (src/md.f90:16114-16126)
+    if (!s.pbc) fix_shell();
     p_restrain();
-    if (s.nwat > 0) {
+    if (s.nwat > 0 && !s.pbc) {
       restrain_solvent();
       if (s.wpol_restr) watpol();
     }
@@ -679,7 +783,10 @@ struct Oracle {
 
   // one md_run iteration (src/md.f90:6471-6612) without output
   bool step() {
-    if (istep % s.inp.nbcycle == 0) make_pair_lists();
+    if (istep % s.inp.nbcycle == 0) {
+      if (s.pbc && (s.inp.put_solute_back || s.inp.put_solvent_back)) put_back_in_box();
+      make_pair_lists();
+    }
     pot_energy();
     newvel_ber(s.dt, Tscale_solute, 0, s.nat_solute);
     newvel_ber(s.dt, Tscale_solvent, s.nat_solute, s.natom);
@@ -825,10 +932,156 @@ void Oracle::nbqwlist() {
   }
   qw_done = true;
 }
+// ------------------------------------------------------------- periodic box
+// nbpplist_box(_lrf) (src/md.f90:7810-7997, 8190-8396): charge groups by
+// the minimum-image distance of their switching atoms, no excluded atoms;
+// each listed group pair keeps its switching atoms for the shift.
+void Oracle::nbpplist_box() {
+  nbpp.clear();
+  pp_cgp.clear();
+  const auto& T = s.top;
+  const double rcut2 = s.inp.rcpp * s.inp.rcpp, rclrf2 = s.inp.rclrf * s.inp.rclrf;
+  for (int ig = 0; ig < T.ncgp_solute; ++ig) {
+    const int ig_sw = T.cgp[ig].iswitch, i3 = 3 * ig_sw;
+    for (int jg = 0; jg < T.ncgp_solute; ++jg) {
+      if (skip_parity(ig, jg)) continue;
+      const int jg_sw = T.cgp[jg].iswitch, j3 = 3 * jg_sw;
+      const double dx = mimage(x[i3] - x[j3], 0), dy = mimage(x[i3 + 1] - x[j3 + 1], 1),
+                   dz = mimage(x[i3 + 2] - x[j3 + 2], 2);
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      if (r2 <= rcut2) {
+        pp_cgp.push_back({ig_sw, jg_sw});
+        const int cg = (int)pp_cgp.size() - 1;
+        for (int a = T.cgp[ig].first; a <= T.cgp[ig].last; ++a) {
+          int i = T.cgpatom[a];
+          if (s.iqatom[i] != 0) continue;
+          for (int b = T.cgp[jg].first; b <= T.cgp[jg].last; ++b) {
+            int j = T.cgpatom[b];
+            if (ig == jg && i >= j) continue;
+            if (s.iqatom[j] != 0) continue;
+            int lj = ljcod(s.iac[i], s.iac[j]);
+            if (s.crg[i] * s.crg[j] == 0.0 && typ(i).avdw[lj - 1] * typ(j).avdw[lj - 1] == 0.0 &&
+                typ(i).bvdw[lj - 1] * typ(j).bvdw[lj - 1] == 0.0)
+              continue;
+            if (excluded_pair(i, j)) continue;
+            nbpp.push_back({i, j, pair14(i, j) ? 3 : lj, cg});
+          }
+        }
+      } else if (s.inp.use_lrf && r2 <= rclrf2) {
+        lrf_update(ig, jg);
+        lrf_update(jg, ig);
+      }
+    }
+  }
+}
+
+// nbpwlist_box(_lrf) (src/md.f90:9169-9299, 9431-9580)
+void Oracle::nbpwlist_box() {
+  nbpw.clear();
+  pw_cgp.clear();
+  const auto& T = s.top;
+  const double rcut2 = s.inp.rcpw * s.inp.rcpw, rclrf2 = s.inp.rclrf * s.inp.rclrf;
+  for (int ig = 0; ig < T.ncgp_solute; ++ig) {
+    const int ig_sw = T.cgp[ig].iswitch, i3 = 3 * ig_sw;
+    for (int jw = 0; jw < s.nwat; ++jw) {
+      const int ja = s.nat_solute + 3 * jw, j3 = 3 * ja, jg = s.iwhich_cgp[ja];
+      const double dx = mimage(x[i3] - x[j3], 0), dy = mimage(x[i3 + 1] - x[j3 + 1], 1),
+                   dz = mimage(x[i3 + 2] - x[j3 + 2], 2);
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      if (r2 <= rcut2) {
+        pw_cgp.push_back({ig_sw, ja});
+        const int cg = (int)pw_cgp.size() - 1;
+        for (int a = T.cgp[ig].first; a <= T.cgp[ig].last; ++a) {
+          int i = T.cgpatom[a];
+          if (s.iqatom[i] != 0) continue;
+          for (int k = 0; k < 3; ++k) {
+            int j = ja + k;
+            int lj = ljcod(s.iac[i], s.iac[j]);
+            if (s.crg[i] * s.crg[j] == 0.0 && typ(i).avdw[lj - 1] * typ(j).avdw[lj - 1] == 0.0 &&
+                typ(i).bvdw[lj - 1] * typ(j).bvdw[lj - 1] == 0.0)
+              continue;
+            nbpw.push_back({i, j, lj, cg});
+          }
+        }
+      } else if (s.inp.use_lrf && r2 <= rclrf2) {
+        lrf_update(ig, jg);
+        lrf_update(jg, ig);
+      }
+    }
+  }
+}
+
+// nbqplist_box (src/md.f90:10327-10553): a solute charge group enters when
+// any of its atoms lies within Rcq of the Q switching atom (minimum image);
+// that atom gives the group's shift.  Rebuilt at every list update (built
+// once only for Rcq < 0, which this restatement refuses).
+void Oracle::nbqplist_box() {
+  nbqp.clear();
+  qp_cgp.clear();
+  if (s.nqat == 0) return;
+  const auto& T = s.top;
+  const double rcut2 = s.inp.rcq * s.inp.rcq;
+  const int q3 = 3 * s.qswitch;
+  for (int ig = 0; ig < T.ncgp_solute; ++ig) {
+    bool inside = false;
+    for (int a = T.cgp[ig].first; a <= T.cgp[ig].last && !inside; ++a) {
+      const int i = T.cgpatom[a], i3 = 3 * i;
+      const double dx = mimage(x[i3] - x[q3], 0), dy = mimage(x[i3 + 1] - x[q3 + 1], 1),
+                   dz = mimage(x[i3 + 2] - x[q3 + 2], 2);
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      if (r2 <= rcut2) {
+        inside = true;
+        qp_cgp.push_back(i);
+      }
+    }
+    if (!inside) continue;
+    const int cg = (int)qp_cgp.size() - 1;
+    for (int a = T.cgp[ig].first; a <= T.cgp[ig].last; ++a) {
+      int i = T.cgpatom[a];
+      bool conn = false;
+      for (int st = 0; st < s.nstates && !conn; ++st)
+        for (int iq = 0; iq < s.nqat && !conn; ++iq) conn = qconn_of(s, st, iq, i) <= 3;
+      if (conn) continue;
+      for (int iq = 0; iq < s.nqat; ++iq) {
+        int lj = ljcod(s.iac[i], s.iac[s.iqseq[iq]]);
+        if (qconn_of(s, 0, iq, i) == 4) lj = 3;
+        nbqp.push_back({iq, i, lj, cg});
+      }
+    }
+  }
+}
+
+// nbqwlist_box (src/md.f90:10554-10700): waters whose O lies within Rcq of
+// the Q switching atom (minimum image); rebuilt at every list update
+void Oracle::nbqwlist_box() {
+  nbqw.clear();
+  if (s.nqat == 0) return;
+  const double rcut2 = s.inp.rcq * s.inp.rcq;
+  const int q3 = 3 * s.qswitch;
+  for (int iw = 0; iw < s.nwat; ++iw) {
+    const int ia = s.nat_solute + 3 * iw, i3 = 3 * ia;
+    const double dx = mimage(x[i3] - x[q3], 0), dy = mimage(x[i3 + 1] - x[q3 + 1], 1),
+                 dz = mimage(x[i3 + 2] - x[q3 + 2], 2);
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    if (r2 <= rcut2)
+      for (int iq = 0; iq < s.nqat; ++iq)
+        for (int is = 0; is < 3; ++is)
+          nbqw.push_back({iq, ia + is, ljcod(s.iac[s.nat_solute + is], s.iac[s.iqseq[iq]])});
+  }
+}
+
+// With a periodic box (nonbond_pp_box, src/md.f90:13159-13280) every listed
+// group pair's shift is formed from its switching atoms first.
 void Oracle::nonbond_pp() {
+  std::vector<std::array<double, 3>> sh(s.pbc ? pp_cgp.size() : 0);
+  for (size_t g = 0; g < sh.size(); ++g) {
+    const int a3 = 3 * pp_cgp[g][0], b3 = 3 * pp_cgp[g][1];
+    for (int k = 0; k < 3; ++k) sh[g][k] = bshift(x[b3 + k] - x[a3 + k], k);
+  }
   for (const NB& p : nbpp) {
     int i = p.i, j = p.j, i3 = 3 * i, j3 = 3 * j;
     double dx1 = x[j3] - x[i3], dx2 = x[j3 + 1] - x[i3 + 1], dx3 = x[j3 + 2] - x[i3 + 2];
+    if (s.pbc) { dx1 = dx1 - sh[p.cgp][0]; dx2 = dx2 - sh[p.cgp][1]; dx3 = dx3 - sh[p.cgp][2]; }
     double r2 = 1.0 / (dx1 * dx1 + dx2 * dx2 + dx3 * dx3);
     double r = std::sqrt(r2);
     double r6 = r2 * r2 * r2;
@@ -843,10 +1096,17 @@ void Oracle::nonbond_pp() {
     E.pp_vdw = E.pp_vdw + Va - Vb;
   }
 }
+// periodic box: nonbond_pw_box (src/md.f90:13369-13490)
 void Oracle::nonbond_pw() {
+  std::vector<std::array<double, 3>> sh(s.pbc ? pw_cgp.size() : 0);
+  for (size_t g = 0; g < sh.size(); ++g) {
+    const int a3 = 3 * pw_cgp[g][0], b3 = 3 * pw_cgp[g][1];
+    for (int k = 0; k < 3; ++k) sh[g][k] = bshift(x[b3 + k] - x[a3 + k], k);
+  }
   for (const NB& p : nbpw) {
     int i = p.i, j = p.j, i3 = 3 * i, j3 = 3 * j;
     double dx1 = x[j3] - x[i3], dx2 = x[j3 + 1] - x[i3 + 1], dx3 = x[j3 + 2] - x[i3 + 2];
+    if (s.pbc) { dx1 = dx1 - sh[p.cgp][0]; dx2 = dx2 - sh[p.cgp][1]; dx3 = dx3 - sh[p.cgp][2]; }
     double aLJ = typ(i).avdw[p.ljcod - 1] * typ(j).avdw[p.ljcod - 1];
     double bLJ = typ(i).bvdw[p.ljcod - 1] * typ(j).bvdw[p.ljcod - 1];
     double r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
@@ -864,12 +1124,20 @@ void Oracle::nonbond_pw() {
     E.pw_vdw = E.pw_vdw + Va - Vb;
   }
 }
+// periodic box: nonbond_qp_box (src/md.f90:13925-14096), shifts of the
+// listed groups against the Q switching atom
 void Oracle::nonbond_qp() {
   const int ns = s.nstates;
+  std::vector<std::array<double, 3>> sh(s.pbc ? qp_cgp.size() : 0);
+  for (size_t g = 0; g < sh.size(); ++g) {
+    const int a3 = 3 * qp_cgp[g], q3 = 3 * s.qswitch;
+    for (int k = 0; k < 3; ++k) sh[g][k] = bshift(x[a3 + k] - x[q3 + k], k);
+  }
   for (const NBQ& p : nbqp) {
     int iq = p.iq, i = s.iqseq[iq], j = p.j, i3 = 3 * i, j3 = 3 * j;
     int iLJ = p.ljcod;
     double dx1 = x[j3] - x[i3], dx2 = x[j3 + 1] - x[i3 + 1], dx3 = x[j3 + 2] - x[i3 + 2];
+    if (s.pbc) { dx1 = dx1 - sh[p.cgp][0]; dx2 = dx2 - sh[p.cgp][1]; dx3 = dx3 - sh[p.cgp][2]; }
     double r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
     r2 = 1.0 / r2;
     double r6 = r2 * r2 * r2;
@@ -890,12 +1158,20 @@ void Oracle::nonbond_qp() {
   }
 }
 
+// periodic box: nonbond_qw_spc_box (src/md.f90:14574-14748), the shift of
+// the water O against the Q switching atom for all three water atoms
 void Oracle::nonbond_qw_spc() {
   const int ns = s.nstates;
   for (size_t iw = 0; iw < nbqw.size(); iw += 3) {
     int iq = nbqw[iw].iq, i = s.iqseq[iq], i3 = 3 * i;
     int j = nbqw[iw].j, j3 = 3 * j;
+    double bs[3] = {0.0, 0.0, 0.0};
+    if (s.pbc) {
+      const int q3 = 3 * s.qswitch;
+      for (int k = 0; k < 3; ++k) bs[k] = bshift(x[j3 + k] - x[q3 + k], k);
+    }
     double dx1 = x[j3] - x[i3], dx2 = x[j3 + 1] - x[i3 + 1], dx3 = x[j3 + 2] - x[i3 + 2];
+    if (s.pbc) { dx1 = dx1 - bs[0]; dx2 = dx2 - bs[1]; dx3 = dx3 - bs[2]; }
     double r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
     double r6_hc = r2 * r2 * r2;
     r2 = 1.0 / r2;
@@ -932,6 +1208,7 @@ void Oracle::nonbond_qw_spc() {
     for (int ip = 1; ip < 3; ++ip) {
       int ja = j + ip, ja3 = 3 * ja;
       double ex1 = x[ja3] - x[i3], ex2 = x[ja3 + 1] - x[i3 + 1], ex3 = x[ja3 + 2] - x[i3 + 2];
+      if (s.pbc) { ex1 = ex1 - bs[0]; ex2 = ex2 - bs[1]; ex3 = ex3 - bs[2]; }
       double q2 = 1.0 / (ex1 * ex1 + ex2 * ex2 + ex3 * ex3);
       double rr = std::sqrt(q2);
       double dvh = 0;
@@ -1183,12 +1460,18 @@ double Oracle::torsion(int a, int b) {
 // type of each state and soft-core r^6 + alpha.
 void Oracle::nonbond_qp_qvdw() {
   const int ns = s.nstates;
+  std::vector<std::array<double, 3>> sh(s.pbc ? qp_cgp.size() : 0);
+  for (size_t g = 0; g < sh.size(); ++g) {
+    const int a3 = 3 * qp_cgp[g], q3 = 3 * s.qswitch;
+    for (int k = 0; k < 3; ++k) sh[g][k] = bshift(x[a3 + k] - x[q3 + k], k);
+  }
   for (const NBQ& p : nbqp) {
     int iq = p.iq, i = s.iqseq[iq], j = p.j, i3 = 3 * i, j3 = 3 * j;
     int iacj = s.iac[j];
     int iLJ = p.ljcod;
     int qLJ = iLJ == 2 ? 1 : iLJ;
     double dx1 = x[j3] - x[i3], dx2 = x[j3 + 1] - x[i3 + 1], dx3 = x[j3 + 2] - x[i3 + 2];
+    if (s.pbc) { dx1 = dx1 - sh[p.cgp][0]; dx2 = dx2 - sh[p.cgp][1]; dx3 = dx3 - sh[p.cgp][2]; }
     double r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
     double r6 = r2 * r2 * r2;
     r2 = 1.0 / r2;
@@ -1546,6 +1829,8 @@ void Oracle::p_restrain() {
         if (!use(i)) continue;
         const int i3 = 3 * i;
         double dr[3] = {x[i3] - s.xtop[i3], x[i3 + 1] - s.xtop[i3 + 1], x[i3 + 2] - s.xtop[i3 + 2]};
+        // periodic box: the minimal periodic distance (src/md.f90:15561-15565)
+        if (s.pbc) for (int c = 0; c < 3; ++c) dr[c] = mimage(dr[c], c);
         double r2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
         E.rst_protein = E.rst_protein + 0.5 * fk * r2;
         for (int c = 0; c < 3; ++c) d[i3 + c] = d[i3 + c] + fk * dr[c];
@@ -1563,6 +1848,7 @@ void Oracle::p_restrain() {
   for (const RstDis& r : s.rstdis) {
     const int i3 = 3 * r.i, j3 = 3 * r.j;
     double dr[3] = {x[j3] - x[i3], x[j3 + 1] - x[i3 + 1], x[j3 + 2] - x[i3 + 2]};
+    if (s.pbc) for (int c = 0; c < 3; ++c) dr[c] = mimage(dr[c], c);   // src/md.f90:15630-15634
     const double wgt = wgt_of(r.ipsi);
     double b = std::sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
     double db;
@@ -1581,6 +1867,8 @@ void Oracle::p_restrain() {
     const int i3 = 3 * r.i, j3 = 3 * r.j, k3 = 3 * r.k;
     double dr[3], dr2[3];
     for (int c = 0; c < 3; ++c) { dr[c] = x[i3 + c] - x[j3 + c]; dr2[c] = x[k3 + c] - x[j3 + c]; }
+    if (s.pbc)   // src/md.f90:15694-15702
+      for (int c = 0; c < 3; ++c) { dr[c] = mimage(dr[c], c); dr2[c] = mimage(dr2[c], c); }
     const double wgt = wgt_of(r.ipsi);
     double r2ij = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
     double r2jk = dr2[0] * dr2[0] + dr2[1] * dr2[1] + dr2[2] * dr2[2];
@@ -1608,7 +1896,9 @@ void Oracle::p_restrain() {
     }
     add_state(r.ipsi, edum);
   }
+  // wall restraints only for spheres (src/md.f90:15768)
   for (const RstWal& r : s.rstwal) {
+    if (s.pbc) break;
     const double fk = r.fk;
     for (int i = r.i; i <= r.j; ++i) {
       if (!(s.heavy[i] || r.ih == 1)) continue;
@@ -1911,6 +2201,12 @@ void oracle_theta_corr(void* h, double* out) {
   for (size_t k = 0; k < o->s.wshell.size(); ++k) out[k] = o->s.wshell[k].theta_corr;
 }
 int oracle_nshells(void* h) { return (int)static_cast<Oracle*>(h)->s.wshell.size(); }
+// periodic box: box lengths and centre (6 doubles); returns 1 for a box, 0 for a sphere
+int oracle_box(void* h, double* out) {
+  const Setup& s = static_cast<Oracle*>(h)->s;
+  for (int k = 0; k < 3; ++k) { out[k] = s.boxl[k]; out[3 + k] = s.boxc[k]; }
+  return s.pbc ? 1 : 0;
+}
 int oracle_monitor(void* h, double* out, int cap) {
   Oracle* o = static_cast<Oracle*>(h);
   return oracle_monitor_eval(o->s, o->x.data(), out, cap);

@@ -413,6 +413,7 @@ void Batch::upload(int k) {
   const qsys::Setup& s = *setups_[k];
   const qsys::Topology& t = s.top;
   if (s.nstates > kMaxQStates) throw std::runtime_error("too many FEP states");
+  if (s.pbc) throw std::runtime_error("periodic boxes are not run by the GPU engine yet");
   auto* I = new Inst();
   inst_.push_back(I);
   I->groups = device_groups(s);

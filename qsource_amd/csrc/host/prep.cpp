@@ -310,6 +310,16 @@ Setup prepare(const std::string& input_path) {
   s.nwat = (s.natom - s.nat_solute) / s.solv_atom;
   s.nmol = (int)t.istart_mol.size();
   t.istart_mol.push_back(s.natom);
+  // periodic box (src/md.f90:17869-17905): the same boundary in topology and
+  // input, box from the topology (or the restart below), Q-atom cut-off
+  // default -1 (every atom)
+  s.pbc = t.use_pbc;
+  if (s.pbc != in.box) throw std::runtime_error("Must have same boundary (sphere or box) in topology and input file");
+  if (s.pbc) {
+    for (int k = 0; k < 3; ++k) { s.boxl[k] = t.boxlength[k]; s.boxc[k] = t.boxcentre[k]; }
+    if (!in.have_rcq) in.rcq = -1.0;
+    if (s.nwat > 0 && t.solvent_type != 0) throw std::runtime_error("periodic boxes need SPC solvent (solvent_type 0)");
+  }
 
   // topology(): unit conversions and LJ code table (src/md.f90:17819-17905)
   for (auto& a : t.anglib) a.ang0 = kDeg2Rad * a.ang0;
@@ -343,11 +353,12 @@ Setup prepare(const std::string& input_path) {
   for (int i = 0; i < s.natom; ++i) s.mass[i] = t.types[s.iac[i]].mass;
   s.freeze = in.freeze;
   s.rexcl_i = in.have_shell_radius ? in.shell_radius : 0.85;
-  if (s.rexcl_i < 0) throw std::runtime_error("inner radius of restrained shell must be >= 0");
+  if (s.rexcl_i < 0 && !s.pbc) throw std::runtime_error("inner radius of restrained shell must be >= 0");
   s.shell.assign(s.natom, 0);
-  if (s.rexcl_i > t.rexcl_o) throw std::runtime_error("inner radius of restrained shell must be < exclusion radius");
+  if (s.rexcl_i > t.rexcl_o && !s.pbc)
+    throw std::runtime_error("inner radius of restrained shell must be < exclusion radius");
   if (s.rexcl_i <= 1.0) s.rexcl_i = s.rexcl_i * t.rexcl_o;
-  {
+  if (!s.pbc) {
     double rin2 = s.rexcl_i * s.rexcl_i;
     for (int ig = 0; ig < t.ncgp_solute; ++ig) {
       int sw = t.cgp[ig].iswitch;
@@ -364,7 +375,10 @@ Setup prepare(const std::string& input_path) {
   std::vector<double> theta_corr_rst;
   bool restart = !in.restart_file.empty();
   if (restart) {
-    read_restart(join(dir, in.restart_file), s.natom, s.x, s.v, &theta_corr_rst);
+    double box[6];
+    read_restart(join(dir, in.restart_file), s.natom, s.x, s.v, &theta_corr_rst, s.pbc ? box : nullptr);
+    if (s.pbc)
+      for (int k = 0; k < 3; ++k) { s.boxl[k] = box[k]; s.boxc[k] = box[3 + k]; }
   } else {
     s.x = s.xtop;
     s.v.assign(3 * s.natom, 0.0);
@@ -381,6 +395,14 @@ Setup prepare(const std::string& input_path) {
   std::vector<FepData::QShake> s_qshake;
   if (!in.fep_file.empty()) {
     FepData f = read_fep(join(dir, in.fep_file), s.natom);
+    if (s.pbc) {
+      if (f.qswitch < 0) throw std::runtime_error("Section PBC (switching_atom) is required when using periodic boundary");
+      s.qswitch = f.qswitch;
+      // Rcq < 0 (every atom, the lists built once) leaves the reference's
+      // Q-solute groups without a shift record (src/md.f90:10359-10400)
+      if (in.rcq < 0 && !f.iqseq.empty())
+        throw std::runtime_error("a periodic box with Q atoms needs a positive q_atom cut-off");
+    }
     s.nstates = f.nstates;
     for (auto& pr : f.mon_pairs) {
       Setup::MonPair m;
@@ -511,7 +533,7 @@ Setup prepare(const std::string& input_path) {
   s.fkwpol = in.fkwpol == -1 ? 20.0 : in.fkwpol;
   s.dwmz = in.dwmz == -1 ? 0.26 * std::exp(-0.19 * (s.rwat - 15.0)) + 0.74 : in.dwmz;
   s.awmz = in.awmz == -1 ? 0.2 / (1.0 + std::exp(0.4 * (s.rwat - 25.0))) + 0.3 : in.awmz;
-  s.wpol_restr = in.wpol_restr;
+  s.wpol_restr = in.wpol_restr && !s.pbc;   // no solvent restraints in a periodic box
   // wat_shells (src/md.f90:18189-18368)
   if (s.nwat > 0 && s.wpol_restr) {
     int bndcodw = t.bnd.back().cod, angcodw = t.ang.back().cod;
@@ -538,6 +560,28 @@ Setup prepare(const std::string& input_path) {
       rout = rout - dr;
     }
     s.n_max_insh = (int)(nmax * 1.5);
+  }
+  if (s.pbc) {
+    // inverse box lengths and the cut-off checks (src/md.f90:17873-17905)
+    for (int k = 0; k < 3; ++k) s.inv_boxl[k] = s.boxl[k] == 0.0 ? 0.0 : 1.0 / s.boxl[k];
+    if (s.boxl[0] == 0.0 || s.boxl[1] == 0.0 || s.boxl[2] == 0.0)
+      for (int k = 0; k < 3; ++k) s.inv_boxl[k] = 0.0;
+    const double bmin = std::min(s.boxl[0], std::min(s.boxl[1], s.boxl[2]));
+    if (!(bmin > in.rcpp * 2)) throw std::runtime_error("Solute-solute cut-off radii too large");
+    if (!(bmin > in.rcww * 2)) throw std::runtime_error("Solvent-solvent cut-off radii too large");
+    if (!(bmin > in.rcpw * 2)) throw std::runtime_error("Solute-solvent cut-off radii too large");
+    if (!(bmin > in.rcq * 2)) throw std::runtime_error("Q-atom cut-off radii too large");
+    if (in.use_lrf && !(bmin > in.rclrf * 2)) throw std::runtime_error("LRF cut-off radii too large");
+    // per molecule 1/mass (prep_sim, src/md.f90:16769-16784) and the solute
+    // molecules (src/md.f90:16758-16759)
+    s.mol_minv.assign(s.nmol, 0.0);
+    for (int m = 0; m < s.nmol; ++m) {
+      double ms = 0.0;
+      for (int j = t.istart_mol[m]; j < t.istart_mol[m + 1]; ++j) ms = ms + t.types[s.iac[j]].mass;
+      s.mol_minv[m] = 1.0 / ms;
+    }
+    const int nres = (int)t.res_start.size();
+    s.nsolute_mol = s.nmol - (nres - t.nres_solute);
   }
   s.iwhich_cgp.assign(s.natom, 0);
   for (int ig = 0; ig < (int)t.cgp.size(); ++ig)

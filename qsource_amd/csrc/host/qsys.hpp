@@ -71,6 +71,7 @@ struct Topology {
   std::vector<int> istart_mol;  // 0-based start atom of each molecule
   int solvent_type = 0;
   bool use_pbc = false;
+  double boxlength[3] = {0, 0, 0}, boxcentre[3] = {0, 0, 0};   // periodic box (topology)
   double rexcl_o = 0, rexcl_i = 0, rwat = 0;
   double xpcent[3] = {0,0,0}, xwcent[3] = {0,0,0};
   int nexats = 0, nexwat = 0;
@@ -117,6 +118,9 @@ struct MDInput {
   std::vector<RstWal> rstwal;
   // residue:atom texts of distance restraints, resolved against the topology
   std::vector<std::pair<std::string, std::string>> rstdis_text;
+  // [PBC] (src/md.f90:4516-4598)
+  bool box = false, rigid_box_centre = false, put_solute_back = false, put_solvent_back = false;
+  bool have_rcq = false;
 };
 
 // Q-bonded terms of the FEP file (src/qatom.f90:920-1790): atoms 0-based,
@@ -167,6 +171,7 @@ struct FepData {
   struct ExSpec { int i, j; std::vector<char> flag; };
   std::vector<ExSpec> exspec;
   bool qq_lib_charges = false;
+  int qswitch = -1;                       // [PBC] switching_atom (0-based), periodic box only
   // [shake_constraints]: atom pair (0-based) and distance per state
   struct QShake { int i, j; std::vector<double> dist; };
   std::vector<QShake> qshake;
@@ -258,6 +263,14 @@ struct Setup {
   std::vector<RstAng> rstang;
   std::vector<RstWal> rstwal;
   std::vector<double> mass;           // topology type mass per atom (iaclib(iac)%mass)
+  // periodic box: box lengths, centre and inverse lengths (src/md.f90:17873-17880),
+  // the Q-atom switching atom, per molecule 1/mass, and the number of solute
+  // molecules (nmol - (nres - nres_solute))
+  bool pbc = false;
+  double boxl[3] = {0, 0, 0}, boxc[3] = {0, 0, 0}, inv_boxl[3] = {0, 0, 0};
+  int qswitch = -1;
+  std::vector<double> mol_minv;
+  int nsolute_mol = 0;
   bool freeze = false;
 };
 
@@ -265,10 +278,14 @@ struct Setup {
 Topology read_topology(const std::string& path);
 MDInput read_input(const std::string& path);
 FepData read_fep(const std::string& path, int natom);
+// Restart files (write_xfin src/md.f90:18705-18736, prep_coord :16477-16577):
+// x, v, then theta_corr of the polarisation shells (sphere) or the box
+// lengths and centre (periodic box, box != nullptr: 6 doubles).
 bool read_restart(const std::string& path, int natom, std::vector<double>& x,
-                  std::vector<double>& v, std::vector<double>* theta_corr);
+                  std::vector<double>& v, std::vector<double>* theta_corr, double* box = nullptr);
 void write_restart(const std::string& path, const std::vector<double>& x,
-                   const std::vector<double>& v, const std::vector<double>* theta_corr);
+                   const std::vector<double>& v, const std::vector<double>* theta_corr,
+                   const double* box = nullptr);
 
 // Trajectory atom mask of the [trajectory_atoms] rows (mask_add/update,
 // src/mask.f90:129-361): per atom 1 if written; "all" when rows is empty.

@@ -330,7 +330,17 @@ Topology read_topology(const std::string& path) {
     std::string l = r.next();
     std::vector<std::string> tk;
     Records::tokens(l, tk);
-    if (!tk.empty() && tk[0] == "PBC") throw std::runtime_error("periodic boundary topologies are not supported");
+    if (!tk.empty() && tk[0] == "PBC") {
+      // periodic box: box lengths and centre; no excluded atoms
+      // (src/topo.f90:1022-1032)
+      t.use_pbc = true;
+      auto bl = r.items(3);
+      for (int k = 0; k < 3; ++k) t.boxlength[k] = to_d(bl[k]);
+      auto bc = r.items(3);
+      for (int k = 0; k < 3; ++k) t.boxcentre[k] = to_d(bc[k]);
+      t.excl.assign(N, 0);
+      return t;
+    }
     auto rv = lead_reals(l);
     if (t.version < 5.01) {
       t.rexcl_o = rv.at(0); t.rexcl_i = rv.at(1); t.rwat = rv.at(2);
@@ -460,7 +470,18 @@ MDInput read_input(const std::string& path) {
   Sections S(path);
   MDInput m;
   std::string v;
-  if (S.has("PBC")) throw std::runtime_error("periodic boundary runs are not supported");
+  if (S.has("PBC")) {
+    // [PBC] (src/md.f90:4516-4598); the NPT ensemble and a changed box size
+    // are not implemented
+    m.box = true;
+    if (S.get("PBC", "rigid_box_centre", v)) parse_onoff(v, m.rigid_box_centre);
+    bool cp = false;
+    if (S.get("PBC", "constant_pressure", v)) parse_onoff(v, cp);
+    if (cp) throw std::runtime_error("constant pressure (MC_volume) is not supported");
+    if (S.get("PBC", "control_box", v)) throw std::runtime_error("control_box is not supported");
+    if (S.get("PBC", "put_solvent_back_in_box", v)) parse_onoff(v, m.put_solvent_back);
+    if (S.get("PBC", "put_solute_back_in_box", v)) parse_onoff(v, m.put_solute_back);
+  }
   if (!S.has("MD")) throw std::runtime_error("input has no [MD] section");
   if (!S.get("MD", "steps", v)) throw std::runtime_error("[MD] steps missing");
   m.nsteps = (int)to_i(v);
@@ -487,7 +508,7 @@ MDInput read_input(const std::string& path) {
     if (S.get("CUT-OFFS", "solute_solute", v)) m.rcpp = to_d(v);
     if (S.get("CUT-OFFS", "solvent_solvent", v)) m.rcww = to_d(v);
     if (S.get("CUT-OFFS", "solute_solvent", v)) m.rcpw = to_d(v);
-    if (S.get("CUT-OFFS", "q_atom", v)) m.rcq = to_d(v);
+    if (S.get("CUT-OFFS", "q_atom", v)) { m.rcq = to_d(v); m.have_rcq = true; }
     if (m.use_lrf && S.get("CUT-OFFS", "lrf", v)) m.rclrf = to_d(v);
   }
   if (m.use_lrf && (m.rclrf < m.rcpp || m.rclrf < m.rcpw || m.rclrf < m.rcww))
@@ -673,6 +694,12 @@ FepData read_fep(const std::string& path, int natom) {
   int nq = (int)f.iqseq.size();
   f.qcrg.assign(nq, std::vector<double>(f.nstates, 0.0));
   if (S.get("FEP", "qq_use_library_charges", v)) parse_onoff(v, f.qq_lib_charges);
+  // the Q atoms' switching atom for periodic boxes (src/qatom.f90:672-690)
+  if (S.get("PBC", "switching_atom", v)) {
+    const int q = (int)to_i(v);
+    if (q < 1 || q > natom) throw std::runtime_error("invalid PBC switching atom");
+    f.qswitch = q - 1;
+  }
   // monitored atom groups and group pairs (qatom_load_fep, src/qatom.f90:1987-2020)
   if (S.has("MONITOR_GROUPS"))
     for (auto& l : S.sec.at("MONITOR_GROUPS")) {
@@ -846,7 +873,7 @@ void write_rec(FILE* fp, const void* p, int32_t n) {
 }  // namespace
 
 bool read_restart(const std::string& path, int natom, std::vector<double>& x,
-                  std::vector<double>& v, std::vector<double>* theta_corr) {
+                  std::vector<double>& v, std::vector<double>* theta_corr, double* box) {
   FILE* fp = std::fopen(path.c_str(), "rb");
   if (!fp) throw std::runtime_error("cannot open restart " + path);
   std::vector<char> b;
@@ -864,7 +891,13 @@ bool read_restart(const std::string& path, int natom, std::vector<double>& x,
   };
   arr(x);
   arr(v);
-  if (theta_corr) {
+  if (box) {
+    // box lengths, then box centre (3 doubles each)
+    for (int k = 0; k < 2; ++k) {
+      if (!read_rec(fp, b) || b.size() != 24) { std::fclose(fp); throw std::runtime_error("restart has no box record"); }
+      std::memcpy(box + 3 * k, b.data(), 24);
+    }
+  } else if (theta_corr) {
     theta_corr->clear();
     if (read_rec(fp, b) && b.size() >= 4) {
       int32_t n;
@@ -880,7 +913,7 @@ bool read_restart(const std::string& path, int natom, std::vector<double>& x,
 }
 
 void write_restart(const std::string& path, const std::vector<double>& x,
-                   const std::vector<double>& v, const std::vector<double>* theta_corr) {
+                   const std::vector<double>& v, const std::vector<double>* theta_corr, const double* box) {
   FILE* fp = std::fopen(path.c_str(), "wb");
   if (!fp) throw std::runtime_error("cannot write " + path);
   int32_t canary = -1337;
@@ -898,6 +931,10 @@ void write_restart(const std::string& path, const std::vector<double>& x,
     std::memcpy(b.data(), &n, 4);
     std::memcpy(b.data() + 4, theta_corr->data(), 8 * theta_corr->size());
     write_rec(fp, b.data(), (int32_t)b.size());
+  }
+  if (box) {
+    write_rec(fp, box, 24);
+    write_rec(fp, box + 3, 24);
   }
   std::fclose(fp);
 }

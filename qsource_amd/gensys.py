@@ -130,6 +130,9 @@ class System:
     solvent_type: int = 0
     # excluded atoms (topology flags, one per atom), None = none
     excl: Optional[np.ndarray] = None
+    # periodic box: edge lengths (x, y, z) and centre; None = sphere
+    box: Optional[np.ndarray] = None
+    box_centre: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(3))
 
     @property
     def natom(self) -> int:
@@ -191,6 +194,32 @@ def water_lattice(radius: float, rng, avoid: Optional[np.ndarray] = None,
     rot = _random_rotations(rng, nw)
     w = O[:, None, :] + np.einsum("nij,kj->nki", rot, local)
     return w
+
+
+def water_box(length: float, rng, avoid: Optional[np.ndarray] = None, avoid_r: float = 3.0):
+    """Waters filling a periodic cube of edge `length` centred at the
+    origin: a cubic lattice of round(length * DENSITY^(1/3)) sites per edge
+    (spacing length / n, so the lattice continues across the faces), random
+    orientations; sites within avoid_r of a solute atom are dropped.
+    Returns (nw,3,3) coordinates O,H1,H2."""
+    n = max(1, int(round(length * DENSITY ** (1.0 / 3.0))))
+    a = length / n
+    g = -length / 2 + (np.arange(n) + 0.5) * a
+    O = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    if avoid is not None and len(avoid):
+        keep = np.zeros(len(O), dtype=bool)
+        for b0 in range(0, len(O), 1024):
+            d = O[b0:b0 + 1024, None, :] - avoid[None, :, :]
+            d -= length * np.round(d / length)
+            keep[b0:b0 + 1024] = (d ** 2).sum(-1).min(1) > avoid_r ** 2
+        O = O[keep]
+    nw = O.shape[0]
+    half = math.radians(THETA_HOH) / 2
+    local = np.array([[0, 0, 0],
+                      [R_OH * math.sin(half), 0, R_OH * math.cos(half)],
+                      [-R_OH * math.sin(half), 0, R_OH * math.cos(half)]])
+    rot = _random_rotations(rng, nw)
+    return O[:, None, :] + np.einsum("nij,kj->nki", rot, local)
 
 
 def _branched_solute(n_atoms: int, rmax: float, rng, bond=1.53, min_dist=3.2):
@@ -392,7 +421,9 @@ def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
                     q_shake: bool = False, solvent_allatom: bool = False,
                     solute_rmax: Optional[float] = None, exclude_outside: bool = False,
                     exclude_water_radius: Optional[float] = None, hydrogens: bool = False,
-                    monitor: bool = False, group_size: int = 4) -> System:
+                    monitor: bool = False, group_size: int = 4, box: Optional[float] = None,
+                    solute_offset: Optional[Sequence[float]] = None,
+                    shift: Optional[Sequence[float]] = None) -> System:
     """A TIP3P sphere of the given radius around an optional branched solute
     (charges N(0,q_sd) neutralised per 4-atom charge group, sigma~U[2.5,3.8],
     eps~U[0.05,0.2] quantised into at most n_types atom types), with an
@@ -417,7 +448,10 @@ def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
     monitor: three monitored atom groups (the Q atoms, the solute atoms
     bonded to them, and the first eight solute atoms) and their three
     pairs; group_size: the most atoms of a solute charge group (connected
-    atoms in bond-tree order)."""
+    atoms in bond-tree order); box: a periodic cube of this edge instead of
+    the sphere (water_box; the solute grows within radius - 5 of its
+    centre, moved by solute_offset before the waters are placed; every atom
+    moved by shift at the end, so molecules may start outside the box)."""
     rng = np.random.default_rng(seed)
     # --- solute
     if n_solute > 0:
@@ -436,7 +470,12 @@ def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
         sx = np.zeros((0, 3))
         sb, angles0, tors0, dist, groups0 = [], [], [], {}, []
     ns = n_solute
-    wat = water_lattice(radius, rng, avoid=sx if ns else None)
+    if box is not None:
+        if solute_offset is not None and ns:
+            sx = sx + np.asarray(solute_offset, dtype=float)
+        wat = water_box(box, rng, avoid=sx if ns else None)
+    else:
+        wat = water_lattice(radius, rng, avoid=sx if ns else None)
     nw = wat.shape[0]
     x = np.concatenate([sx, wat.reshape(-1, 3)], 0)
     N = ns + 3 * nw
@@ -569,6 +608,10 @@ def solvated_system(radius: float, n_solute: int, seed: int, n_q: int = 0,
         res_start=np.array(res_start), res_names=res_names,
         nres_solute=1 if ns else 0, mol_start=np.array(mol_start),
         rexcl_o=radius, rwat=radius)
+    if box is not None:
+        sysm.box = np.array([box, box, box], dtype=float)
+        if shift is not None:
+            sysm.x = sysm.x + np.asarray(shift, dtype=float)
 
     if n_q > 0 and n_states > 0:
         # Q region: the n_q solute atoms nearest the centre, in whole
@@ -981,6 +1024,14 @@ def write_top(s: System, path: str) -> None:
     L.append("%8d = No. of SYBYL atom types:" % nt)
     L += _fmt_list(["" for _ in range(nt)], 13, "%-5s ")
     L.append("%8d = solvent type (0=SPC,1=3-atom,2=general)" % s.solvent_type)
+    if s.box is not None:
+        # periodic box (src/topo.f90:1350-1360): kind, edge lengths, centre
+        L.append("%8s = kind of boundary" % "PBC")
+        L.append("%8.3f%8.3f%8.3f = Size of box, x y z" % tuple(s.box))
+        L.append("%8.3f%8.3f%8.3f = Centre coordinate of box" % tuple(s.box_centre))
+        with open(path, "w") as f:
+            f.write("\n".join(L) + "\n")
+        return
     L.append("%8.3f%8.3f = Exclusion, solvent radii" % (s.rexcl_o, s.rwat))
     L.append("%8.3f%8.3f%8.3f = Solute centre" % tuple(s.xpcent))
     L.append("%8.3f%8.3f%8.3f = Solvent centre" % tuple(s.xwcent))
@@ -1059,6 +1110,9 @@ def write_fep(s: System, path: str) -> None:
                 L += ["", "[%s_couplings]" % name]
                 for term, bond, flag in rows:
                     L.append("%d %d %d" % (term, bond, flag))
+    if s.box is not None:
+        # the Q atoms' switching atom for the periodic lists (src/qatom.f90:672-690)
+        L += ["", "[PBC]", "switching_atom %d" % int(s.qatoms[0])]
     with open(path, "w") as f:
         f.write("\n".join(L) + "\n")
 
@@ -1080,10 +1134,13 @@ def write_inp(path: str, top: str, final: str, *, steps: int, stepsize: float = 
               exclude_bonded: bool = False,
               temp_interval: Optional[int] = None, restraints: Optional[dict] = None,
               trajectory: Optional[str] = None, trajectory_interval: int = 0,
-              trajectory_atoms: Optional[Sequence[str]] = None) -> None:
+              trajectory_atoms: Optional[Sequence[str]] = None,
+              pbc: Optional[dict] = None) -> None:
     """qdyn input file; `restraints` maps sequence/atom/distance/angle/wall
     to lists of rows written into the [<name>_restraints] sections;
-    `trajectory_atoms` rows form the [trajectory_atoms] mask."""
+    `trajectory_atoms` rows form the [trajectory_atoms] mask; `pbc` (a dict
+    of [PBC] keys, e.g. put_solvent_back_in_box) writes a periodic-box input:
+    a [PBC] section in place of [sphere] and [solvent]."""
     on = lambda b: "on" if b else "off"
     L = ["[MD]", "steps %d" % steps, "stepsize %.6g" % stepsize,
          "temperature %.6g" % temperature, "bath_coupling %.6g" % bath_coupling,
@@ -1096,23 +1153,26 @@ def write_inp(path: str, top: str, final: str, *, steps: int, stepsize: float = 
           "solute_solvent %.6g" % rcpw, "q_atom %.6g" % rcq]
     if lrf:
         L.append("lrf %.6g" % rclrf)
-    L += ["", "[sphere]"]
-    if shell_radius is not None:
-        L.append("shell_radius %.6g" % shell_radius)
-    if shell_force is not None:
-        L.append("shell_force %.6g" % shell_force)
-    if excluded_force is not None:
-        L.append("excluded_force %.6g" % excluded_force)
-    if excluded_freeze:
-        L.append("excluded_freeze on")
-    if exclude_bonded:
-        L.append("exclude_bonded on")
-    L += ["", "[solvent]"]
-    if radial_force is not None:
-        L.append("radial_force %.6g" % radial_force)
-    L.append("polarisation %s" % on(polarisation))
-    if polarisation_force is not None:
-        L.append("polarisation_force %.6g" % polarisation_force)
+    if pbc is not None:
+        L += ["", "[PBC]"] + ["%s %s" % (k, on(v) if isinstance(v, bool) else v) for k, v in pbc.items()]
+    else:
+        L += ["", "[sphere]"]
+        if shell_radius is not None:
+            L.append("shell_radius %.6g" % shell_radius)
+        if shell_force is not None:
+            L.append("shell_force %.6g" % shell_force)
+        if excluded_force is not None:
+            L.append("excluded_force %.6g" % excluded_force)
+        if excluded_freeze:
+            L.append("excluded_freeze on")
+        if exclude_bonded:
+            L.append("exclude_bonded on")
+        L += ["", "[solvent]"]
+        if radial_force is not None:
+            L.append("radial_force %.6g" % radial_force)
+        L.append("polarisation %s" % on(polarisation))
+        if polarisation_force is not None:
+            L.append("polarisation_force %.6g" % polarisation_force)
     L += ["", "[intervals]", "non_bond %d" % non_bond, "output %d" % output]
     if temp_interval is not None:
         L.append("temperature %d" % temp_interval)
@@ -1152,7 +1212,10 @@ def _rec(payload: bytes) -> bytes:
 
 
 def write_restart(path: str, x: np.ndarray, v: np.ndarray,
-                  theta_corr: Optional[np.ndarray] = None) -> None:
+                  theta_corr: Optional[np.ndarray] = None, box: Optional[np.ndarray] = None) -> None:
+    """qdyn restart (write_xfin, src/md.f90:18705-18736): x, v, then the
+    polarisation shells' theta_corr (sphere) or the box lengths and centre
+    (periodic box, box = 6 values)."""
     x = np.ascontiguousarray(x, dtype="<f8").reshape(-1)
     v = np.ascontiguousarray(v, dtype="<f8").reshape(-1)
     nat3 = x.size
@@ -1162,12 +1225,17 @@ def write_restart(path: str, x: np.ndarray, v: np.ndarray,
     if theta_corr is not None:
         tc = np.ascontiguousarray(theta_corr, dtype="<f8")
         out += _rec(struct.pack("<i", tc.size) + tc.tobytes())
+    if box is not None:
+        b = np.ascontiguousarray(box, dtype="<f8").reshape(-1)
+        out += _rec(b[:3].tobytes()) + _rec(b[3:6].tobytes())
     with open(path, "wb") as f:
         f.write(out)
 
 
 def read_restart(path: str):
-    """Returns (x (N,3), v (N,3), theta_corr or None)."""
+    """Returns (x (N,3), v (N,3), theta_corr or None); a periodic box's
+    restart (two 24-byte box records after v) gives theta_corr None (see
+    read_restart_box)."""
     with open(path, "rb") as f:
         data = f.read()
     recs = []
@@ -1182,10 +1250,26 @@ def read_restart(path: str):
     x = np.frombuffer(recs[1], dtype="<f8", offset=4, count=nat3).copy()
     v = np.frombuffer(recs[2], dtype="<f8", offset=4, count=nat3).copy()
     tc = None
-    if len(recs) > 3:
+    if len(recs) > 3 and not (len(recs) == 5 and len(recs[3]) == 24 and len(recs[4]) == 24):
         nsh = struct.unpack_from("<i", recs[3])[0]
         tc = np.frombuffer(recs[3], dtype="<f8", offset=4, count=nsh).copy()
     return x.reshape(-1, 3), v.reshape(-1, 3), tc
+
+
+def read_restart_box(path: str) -> Optional[np.ndarray]:
+    """The box lengths and centre (6 values) of a periodic box's restart,
+    or None."""
+    with open(path, "rb") as f:
+        data = f.read()
+    recs = []
+    p = 0
+    while p < len(data):
+        (n,) = struct.unpack_from("<i", data, p)
+        recs.append(data[p + 4:p + 4 + n])
+        p += 8 + n
+    if len(recs) == 5 and len(recs[3]) == 24 and len(recs[4]) == 24:
+        return np.concatenate([np.frombuffer(recs[3], "<f8"), np.frombuffer(recs[4], "<f8")])
+    return None
 
 
 def mapping_lambdas(n_states: int, windows: int) -> np.ndarray:
@@ -1227,6 +1311,8 @@ def write_system(s: System, directory: str, name: str, **inp) -> dict:
         inp.setdefault("fep", name + ".fep")
     inpp = os.path.join(directory, name + ".inp")
     inp.setdefault("steps", 100)
+    if s.box is not None:
+        inp.setdefault("pbc", {"put_solvent_back_in_box": True, "put_solute_back_in_box": False})
     if s.restraints:
         inp.setdefault("restraints", s.restraints)
     write_inp(inpp, name + ".top", name + ".re", **inp)

@@ -14,12 +14,14 @@ import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 FIXTURE = os.path.join(HERE, "prep_SPH")          # eq1.inp, lig_w.top, lig_w.fep
+FIXTURE_PBC = os.path.join(HERE, "prep_PBC")      # lig_w.top, lig_w.fep of the periodic box
 REFERENCE = "/root/reference/tests/basic_tests"   # eq1.inp; prep_SPH/lig_w.{top,fep}
 FILES = ("eq1.inp", "lig_w.top", "lig_w.fep")
 
 # eval_test.sh:96-101: the Q-surr. row of the step-0 Q-atom energies of
-# eq1 that every SPH run must print
+# eq1 that every SPH (PBC) run must print
 INIT_QSURR_SPH = "Q-surr. 1 1.0000      3.12    139.43"
+INIT_QSURR_PBC = "Q-surr. 1 1.0000    -31.30    228.64"
 
 
 def reference_present() -> bool:
@@ -29,7 +31,7 @@ def reference_present() -> bool:
 
 
 def sph_input(inp: str, data_dir: str, dest_dir: str, final: str = "eq1.re",
-              integrator: str = "leap-frog", thermostat: str = "berendsen") -> str:
+              integrator: str = "leap-frog", thermostat: str = "berendsen", name: str = None) -> str:
     """Writes `inp` into dest_dir as run_test.sh prepares it for the sphere:
     the boundary becomes [sphere] with shell_radius 16 and shell_force 20,
     the Q-atom and LRF cut-offs 99, and the integrator and thermostat the
@@ -41,7 +43,6 @@ def sph_input(inp: str, data_dir: str, dest_dir: str, final: str = "eq1.re",
              ("invalid-qatom", "q_atom\t99"), ("invalid-lrf", "lrf\t99"),
              ("invalid-integrator", integrator), ("invalid-thermostat", thermostat)]
     for a, b in subst:
-        assert a in t, a
         t = t.replace(a, b)
     out = []
     for line in t.splitlines():
@@ -51,9 +52,26 @@ def sph_input(inp: str, data_dir: str, dest_dir: str, final: str = "eq1.re",
         elif len(w) == 2 and w[0] == "final":
             line = "%-24s %s" % (w[0], os.path.join(os.path.abspath(dest_dir), final))
         out.append(line)
-    path = os.path.join(dest_dir, os.path.basename(inp))
+    path = os.path.join(dest_dir, name or os.path.basename(inp))
     with open(path, "w") as f:
         f.write("\n".join(out) + "\n")
+    return path
+
+
+def pbc_input(inp: str, data_dir: str, dest_dir: str, final: str = "eq1.re",
+              integrator: str = "leap-frog", thermostat: str = "berendsen") -> str:
+    """As sph_input for the periodic box (run_test.sh:179-182): the boundary
+    becomes [PBC] with the solvent (not the solute) put back in the box at
+    list updates, the Q-atom and LRF cut-offs 24."""
+    t = open(inp).read()
+    t = t.replace("invalid-boundary", "[PBC]\nput_solvent_back_in_box\ton\nput_solute_back_in_box\toff")
+    t = t.replace("invalid-qatom", "q_atom\t24").replace("invalid-lrf", "lrf\t24")
+    tmp = os.path.join(dest_dir, "_pbc_" + os.path.basename(inp))
+    with open(tmp, "w") as f:
+        f.write(t)
+    path = sph_input(tmp, data_dir, dest_dir, final=final, integrator=integrator, thermostat=thermostat,
+                     name=os.path.basename(inp))
+    os.remove(tmp)
     return path
 
 

@@ -44,6 +44,11 @@ import basic_tests as bt  # noqa: E402
 QDYN = os.path.join(ROOT, "oracle", "_ref", "qdyn5")
 OUT = os.path.dirname(os.path.abspath(__file__))
 
+# periodic-box runs: 30 steps of 0.5 fs at 50 K, lists every 5 steps, Q-atom
+# and LRF cut-offs inside half the box edge
+BOX_INP = dict(steps=30, stepsize=0.5, temperature=50.0, initial_temperature=50.0, bath_coupling=5.0,
+               non_bond=5, rcq=12.0, rclrf=13.0, lambdas=[0.6, 0.4])
+
 CASES = {
     # name: (builder kwargs, input kwargs)
     "c1_r15_100": (dict(radius=15.0, n_solute=0, seed=1001), dict(steps=100)),
@@ -170,6 +175,26 @@ CASES = {
                               dict(steps=30, lambdas=[0.5, 0.5], stepsize=0.5, temperature=50.0,
                                    bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
                                    energy_interval=1)),
+    # periodic box (28 A cube, 60-atom solute, 6 Q atoms, 2 states): LRF
+    # with shifted multipoles and without; the box centre floating with an
+    # off-centre solute; a rigid box centre with every atom shifted and the
+    # solute put back too; Q-atom types with soft-core and an energy file;
+    # input restraints (minimum-image distances, no wall restraint)
+    "c15_box28_q6_30": (dict(radius=14.0, n_solute=60, seed=9101, n_q=6, n_states=2, box=28.0),
+                        dict(BOX_INP)),
+    "c15_box28_q6_nolrf_30": (dict(radius=14.0, n_solute=60, seed=9101, n_q=6, n_states=2, box=28.0),
+                              dict(BOX_INP, lrf=False)),
+    "c15_box28_q6_float_30": (dict(radius=14.0, n_solute=60, seed=9101, n_q=6, n_states=2, box=28.0,
+                                   solute_offset=(4.0, -3.0, 2.5)), dict(BOX_INP)),
+    "c15_box28_q6_rigid_30": (dict(radius=14.0, n_solute=60, seed=9101, n_q=6, n_states=2, box=28.0,
+                                   shift=(3.0, -2.5, 1.5)),
+                              dict(BOX_INP, pbc={"rigid_box_centre": True, "put_solvent_back_in_box": True,
+                                                 "put_solute_back_in_box": True})),
+    "c15_box28_q6_qvdw_30": (dict(radius=14.0, n_solute=60, seed=9102, n_q=6, n_states=2, box=28.0, q_types=True,
+                                  softcore=20.0, solute_offset=(1.0, 2.0, -2.0)),
+                             dict(BOX_INP, energy="case.en", energy_interval=1)),
+    "c15_box28_q6_rst_30": (dict(radius=14.0, n_solute=60, seed=9103, n_q=6, n_states=2, box=28.0,
+                                 restraints=True, solute_offset=(-3.0, 0.0, 2.0)), dict(BOX_INP)),
 }
 
 def parse_monitor(log: str):
@@ -303,12 +328,42 @@ def make_prep_sph(only):
         print(name, len(blocks), "blocks", x.shape, sha[:12])
 
 
+def make_prep_pbc(only):
+    """The periodic-box basic test system (prep_PBC): copies its topology and
+    FEP file beside basic_tests.py and runs reference qdyn5 on eq1 as
+    run_test.sh prepares it for the box with leap-frog and Berendsen; keeps
+    every printed summary block, SHA-256 digests of the final x and v bytes
+    (FP64, bit-exact pins without storing 12 262 x 6 doubles) and the box."""
+    name = "prep_pbc_eq1"
+    if only and name not in only:
+        return
+    if not bt.reference_present():
+        sys.exit("reference basic tests not found under " + bt.REFERENCE)
+    os.makedirs(bt.FIXTURE_PBC, exist_ok=True)
+    for f in ("lig_w.top", "lig_w.fep"):
+        shutil.copyfile(os.path.join(bt.REFERENCE, "prep_PBC", f), os.path.join(bt.FIXTURE_PBC, f))
+    with tempfile.TemporaryDirectory() as d:
+        inp = bt.pbc_input(os.path.join(bt.FIXTURE, "eq1.inp"), bt.FIXTURE_PBC, d)
+        r = subprocess.run([QDYN, inp], cwd=d, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        if "terminated normally" not in r.stdout:
+            sys.exit("reference run failed for %s:\n%s" % (name, r.stdout[-3000:]))
+        blocks = summary_blocks(r.stdout)
+        x, v, _ = gensys.read_restart(os.path.join(d, "eq1.re"))
+        box = gensys.read_restart_box(os.path.join(d, "eq1.re"))
+        sha = hashlib.sha256(open(os.path.join(bt.FIXTURE_PBC, "lig_w.top"), "rb").read()).hexdigest()
+        digest = lambda a: hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+        np.savez_compressed(os.path.join(OUT, name + ".npz"), blocks=np.array(blocks), box=box,
+                            x_sha=np.array(digest(x)), v_sha=np.array(digest(v)), top_sha=np.array(sha))
+        print(name, len(blocks), "blocks", x.shape, sha[:12])
+
+
 def main():
     if not os.path.exists(QDYN):
         sys.exit("reference qdyn5 not built: make -f oracle/Makefile.ref OUT=oracle/_ref")
     only = set(sys.argv[1:])   # case names to (re)make; all when none given
     make_write_out_cases(only)
     make_prep_sph(only)
+    make_prep_pbc(only)
     for name, (sysargs, inpargs) in CASES.items():
         if only and name not in only:
             continue
@@ -331,6 +386,9 @@ def main():
                 extra = dict(eq_frames=rows, offd_frames=offd)
             if sysargs.get("monitor"):
                 extra["monitor"] = parse_monitor(r.stdout)
+            box = gensys.read_restart_box(paths["re"])
+            if box is not None:
+                extra["box"] = box
             if inpargs.get("trajectory"):
                 raw = lambda f: np.frombuffer(open(os.path.join(d, f), "rb").read(), dtype=np.uint8)
                 extra.update(re_bytes=raw(name + ".re"), en_bytes=raw(inpargs["energy"]),

@@ -18,6 +18,11 @@ from qsource_amd import gensys
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
+# periodic-box runs: 30 steps of 0.5 fs at 50 K, lists every 5 steps, Q-atom
+# and LRF cut-offs inside half the box edge
+BOX_INP = dict(steps=30, stepsize=0.5, temperature=50.0, initial_temperature=50.0, bath_coupling=5.0,
+               non_bond=5, rcq=12.0, rclrf=13.0, lambdas=[0.6, 0.4])
+
 CASES = {
     "c1_r15_100": (dict(radius=15.0, n_solute=0, seed=1001), dict(steps=100)),
     "c1_r10_250": (dict(radius=10.0, n_solute=0, seed=1001), dict(steps=250)),
@@ -139,6 +144,26 @@ CASES = {
                               dict(steps=30, lambdas=[0.5, 0.5], stepsize=0.5, temperature=50.0,
                                    bath_coupling=2.0, initial_temperature=50.0, energy="case.en",
                                    energy_interval=1)),
+    # periodic box (28 A cube, 60-atom solute, 6 Q atoms, 2 states): LRF
+    # with shifted multipoles and without; the box centre floating with an
+    # off-centre solute; a rigid box centre with every atom shifted and the
+    # solute put back too; Q-atom types with soft-core and an energy file;
+    # input restraints (minimum-image distances, no wall restraint)
+    "c15_box28_q6_30": (dict(radius=14.0, n_solute=60, seed=9101, n_q=6, n_states=2, box=28.0),
+                        dict(BOX_INP)),
+    "c15_box28_q6_nolrf_30": (dict(radius=14.0, n_solute=60, seed=9101, n_q=6, n_states=2, box=28.0),
+                              dict(BOX_INP, lrf=False)),
+    "c15_box28_q6_float_30": (dict(radius=14.0, n_solute=60, seed=9101, n_q=6, n_states=2, box=28.0,
+                                   solute_offset=(4.0, -3.0, 2.5)), dict(BOX_INP)),
+    "c15_box28_q6_rigid_30": (dict(radius=14.0, n_solute=60, seed=9101, n_q=6, n_states=2, box=28.0,
+                                   shift=(3.0, -2.5, 1.5)),
+                              dict(BOX_INP, pbc={"rigid_box_centre": True, "put_solvent_back_in_box": True,
+                                                 "put_solute_back_in_box": True})),
+    "c15_box28_q6_qvdw_30": (dict(radius=14.0, n_solute=60, seed=9102, n_q=6, n_states=2, box=28.0, q_types=True,
+                                  softcore=20.0, solute_offset=(1.0, 2.0, -2.0)),
+                             dict(BOX_INP, energy="case.en", energy_interval=1)),
+    "c15_box28_q6_rst_30": (dict(radius=14.0, n_solute=60, seed=9103, n_q=6, n_states=2, box=28.0,
+                                 restraints=True, solute_offset=(-3.0, 0.0, 2.0)), dict(BOX_INP)),
 }
 
 ORACLE_COLS = {"ww_el": "ww_el", "ww_vdw": "ww_vdw", "lrf": "lrf", "rst_total": "rst_total",
@@ -176,6 +201,10 @@ def test_oracle_matches_reference_qdyn(name, tmp_path):
     x, v = o.state()
     assert np.array_equal(x, g["x"].ravel()), "final coordinates differ from reference qdyn"
     assert np.array_equal(v, g["v"].ravel()), "final velocities differ from reference qdyn"
+    if "box" in g.files:
+        # periodic box: the restart's box lengths and (moving) centre
+        bl, bc = o.box()
+        assert np.array_equal(np.concatenate([bl, bc]), g["box"])
 
 
 ENERGY_FILE_CASES = sorted(k for k, (_, inp) in CASES.items() if inp.get("energy"))

@@ -1,6 +1,7 @@
-"""The oracle on the reference's own basic test system (tests/basic_tests:
-prep_SPH, eq1.inp), prepared as run_test.sh does for the sphere with
-leap-frog and Berendsen (run_test.sh:176-189).
+"""The oracle on the reference's own basic test systems (tests/basic_tests:
+prep_SPH and prep_PBC, eq1.inp), prepared as run_test.sh does for the
+sphere and the periodic box with leap-frog and Berendsen
+(run_test.sh:176-189).
 
 * Read in place from the reference's tree (skipped where it is absent): the
   step-0 Q-surr. row equals eval_test.sh's known answer
@@ -109,4 +110,71 @@ def test_sph_eq1_run_matches_reference_qdyn(tmp_path):
     assert np.array_equal(x, g["x"].ravel()) and np.array_equal(v, g["v"].ravel())
     assert np.array_equal(o.theta_corr(), g["theta_corr"])
     assert np.abs(g["theta_corr"]).max() > 0   # the istep = 100 update happened
+    o.close()
+
+
+PBC_BENCH = ["PBC_%s_%s_benchmark.en" % (i, t) for i in ("leap-frog", "velocity-verlet")
+             for t in ("berendsen", "langevin", "nose-hoover")]
+
+
+@needs_reference
+def test_pbc_initial_energy_in_place(tmp_path):
+    """The periodic box: the step-0 Q-surr. known answer (eval_test.sh:99-101)
+    and the exact columns of row 1 of the six PBC_*_benchmark.en files."""
+    inp = bt.pbc_input(os.path.join(bt.REFERENCE, "eq1.inp"), os.path.join(bt.REFERENCE, "prep_PBC"),
+                       str(tmp_path))
+    E, EQ, z = _step0(inp)
+    text = md.write_out(E, EQ, [1.0], step=0, final=False, use_lrf=True, **z)
+    qsurr = [l for l in text.splitlines() if l.startswith("Q-surr.")]
+    assert qsurr == [bt.INIT_QSURR_PBC], qsurr
+    col = _columns(E, EQ)
+    for f in PBC_BENCH:
+        row = bt.benchmark_row(os.path.join(bt.REFERENCE, f), 1)
+        for c in EXACT:
+            lo, hi = row[c]
+            assert lo == hi, (f, c)
+            assert "%.2f" % col[c] == "%.2f" % lo or (lo == 0 and abs(col[c]) < 0.005), (f, c, col[c], lo)
+        lo, hi = row["POT"]
+        # the box's POT bounds are one printed value
+        assert ("%.2f" % col["POT"] == "%.2f" % lo) if lo == hi else lo <= col["POT"] <= hi, (f, col["POT"], lo, hi)
+
+
+@needs_reference
+def test_pbc_fixture_copies_are_the_reference_files():
+    for f in ("lig_w.top", "lig_w.fep"):
+        assert filecmp.cmp(os.path.join(bt.FIXTURE_PBC, f), os.path.join(bt.REFERENCE, "prep_PBC", f),
+                           shallow=False)
+
+
+def test_pbc_eq1_run_matches_reference_qdyn(tmp_path):
+    """100 steps of the periodic box with the solvent put back at every list
+    update: every printed summary block byte for byte, and x, v and the box
+    (its centre follows the solute) bit for bit as reference qdyn5 wrote
+    them."""
+    g = np.load(os.path.join(GOLD, "prep_pbc_eq1.npz"), allow_pickle=False)
+    top = os.path.join(bt.FIXTURE_PBC, "lig_w.top")
+    assert hashlib.sha256(open(top, "rb").read()).hexdigest() == str(g["top_sha"])
+    blocks = [str(b) for b in g["blocks"]]
+    inp = bt.pbc_input(os.path.join(bt.FIXTURE, "eq1.inp"), bt.FIXTURE_PBC, str(tmp_path))
+    s = md.read_settings(inp)
+    nsteps, iout = s["steps"], s["iout"]
+    o = Oracle(inp)
+    z = o.sizes()
+    assert (o.natom, z["nat_solute"], z["nwat"], o.nstates) == (12262, 46, 4072, 1)
+    printed = []
+    for k in range(nsteps):
+        o.run(1)
+        if k % iout == 0:
+            printed.append(md.write_out(o.energies(), o.qenergies(), [1.0], step=k, final=False, **z))
+    o.final_energy()
+    printed.append(md.write_out(o.energies(), o.qenergies(), [1.0], step=nsteps, final=True, **z))
+    assert len(printed) == len(blocks)
+    for k, (a, b) in enumerate(zip(printed, blocks)):
+        assert a == b, "block %d:\n%s\n---- reference ----\n%s" % (k, a, b)
+    x, v = o.state()
+    digest = lambda a: hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+    assert digest(x) == str(g["x_sha"]) and digest(v) == str(g["v_sha"])
+    bl, bc = o.box()
+    assert np.array_equal(np.concatenate([bl, bc]), g["box"])
+    assert not np.array_equal(bc, [10.052, 6.221, -1.995])   # the centre moved with the solute
     o.close()
