@@ -350,6 +350,15 @@ int qgpu_trj_close(qgpu_trj* t);
 int qgpu_write_restart(const char* path, int natom, const double* x, const double* v, int nshell,
                        const double* theta_corr);
 
+/* Restart file of a periodic box (write_xfin, src/md.f90:18705-18736):
+ * canary -1337, x, v, then the box lengths and the box centre (box: 6 FP64). */
+int qgpu_write_restart_box(const char* path, int natom, const double* x, const double* v, const double* box);
+
+/* Periodic box of instance k (src/md.f90:18732-18735): out[0..2] the box
+ * lengths, out[3..5] the current box centre (it follows the solute at list
+ * updates unless rigid_box_centre is on); *is_box = 0 for a sphere. */
+int qgpu_box(qgpu_handle* h, int k, double* out, int* is_box);
+
 /* Message of the last failure in this thread ("" when none). */
 const char* qgpu_last_error(void);
 

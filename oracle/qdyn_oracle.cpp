@@ -1964,8 +1964,13 @@ static int oracle_monitor_eval(const Setup& s, const double* x, double* out, int
         const int qatomj = s.iqatom[atomj];
         int iacj = s.iac[atomj];
         double qj = s.crg[atomj];
-        const double dx1 = x[3 * atomi] - x[3 * atomj], dx2 = x[3 * atomi + 1] - x[3 * atomj + 1],
-                     dx3 = x[3 * atomi + 2] - x[3 * atomj + 2];
+        double dx1 = x[3 * atomi] - x[3 * atomj], dx2 = x[3 * atomi + 1] - x[3 * atomj + 1],
+               dx3 = x[3 * atomi + 2] - x[3 * atomj + 2];
+        if (s.pbc) {   // src/md.f90:11269-11273
+          dx1 = dx1 - s.boxl[0] * std::round(dx1 * s.inv_boxl[0]);
+          dx2 = dx2 - s.boxl[1] * std::round(dx2 * s.inv_boxl[1]);
+          dx3 = dx3 - s.boxl[2] * std::round(dx3 * s.inv_boxl[2]);
+        }
         const double r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
         const double r = std::sqrt(1.0 / r2);
         double r6 = r2 * r2 * r2;

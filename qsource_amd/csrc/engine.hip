@@ -37,6 +37,7 @@ template <typename P, int MAPPED, int PF> __global__ void k_solute_forces(const 
 template <typename P, int NS> __global__ void k_q_forces(const DevSys*, int);
 __global__ void k_q_reduce(const DevSys*, int);
 __global__ void k_q_lists(const DevSys*);
+__global__ void k_put_back(const DevSys*);
 __global__ void k_monitor(const DevSys*);
 __global__ void k_freeze_x(const DevSys*);
 __global__ void k_terms(const DevSys*);
@@ -169,6 +170,30 @@ static ListRows longest_rows(const qsys::Setup& s, const DevGroups& G) {
   ListRows out;
   const qsys::Topology& t = s.top;
   if (s.nwat == 0) return out;
+  if (s.pbc) {
+    // periodic box: every pair by its minimum image
+    auto d2 = [&](int a, int b) {
+      double r2 = 0.0;
+      for (int c = 0; c < 3; ++c) {
+        double d = s.x[3 * a + c] - s.x[3 * b + c];
+        d = d - s.boxl[c] * std::round(d * s.inv_boxl[c]);
+        r2 += d * d;
+      }
+      return r2;
+    };
+    const double rww2 = s.inp.rcww * s.inp.rcww, rpw2 = s.inp.rcpw * s.inp.rcpw;
+    std::vector<int> nww(s.nwat, 0), nwp(s.nwat, 0), npw(t.ncgp_solute, 0);
+    for (int w = 0; w < s.nwat; ++w) {
+      const int o = s.nat_solute + 3 * w;
+      for (int v = w + 1; v < s.nwat; ++v)
+        if (d2(o, s.nat_solute + 3 * v) <= rww2) { ++nww[w]; ++nww[v]; }
+      for (int g = 0; g < t.ncgp_solute; ++g)
+        if (d2(o, t.cgp[g].iswitch) <= rpw2) { nwp[w] += G.dev_first[g + 1] - G.dev_first[g]; ++npw[g]; }
+    }
+    for (int w = 0; w < s.nwat; ++w) { out.ww = std::max(out.ww, nww[w]); out.wp = std::max(out.wp, nwp[w]); }
+    for (int g = 0; g < t.ncgp_solute; ++g) out.pw = std::max(out.pw, npw[g]);
+    return out;
+  }
   const double rww = s.inp.rcww, rpw = s.inp.rcpw;
   const double cell = std::max(1.0, std::max(rww, rpw));
   double lo[3] = {1e300, 1e300, 1e300};
@@ -275,7 +300,8 @@ void Batch::init(const std::vector<std::string>& inputs, int device) {
   }
   allatom_ = stype == 1;
   for (auto* s : setups_) {
-    q_dynamic_ = q_dynamic_ || (s->nqat > 0 && !(s->inp.rcq > s->top.rexcl_o));
+    q_dynamic_ = q_dynamic_ || (s->nqat > 0 && (s->pbc || !(s->inp.rcq > s->top.rexcl_o)));
+    put_back_ = put_back_ || (s->pbc && (s->inp.put_solute_back || s->inp.put_solvent_back));
     freeze_ = freeze_ || s->inp.freeze;
   }
   nbcycle_ = setups_[0]->inp.nbcycle;
@@ -413,7 +439,7 @@ void Batch::upload(int k) {
   const qsys::Setup& s = *setups_[k];
   const qsys::Topology& t = s.top;
   if (s.nstates > kMaxQStates) throw std::runtime_error("too many FEP states");
-  if (s.pbc) throw std::runtime_error("periodic boxes are not run by the GPU engine yet");
+  if (s.pbc && s.top.solvent_type != 0) throw std::runtime_error("periodic boxes need SPC solvent");
   auto* I = new Inst();
   inst_.push_back(I);
   I->groups = device_groups(s);
@@ -715,7 +741,9 @@ void Batch::upload(int k) {
     // Rcq > rexcl_o: the sets above hold for the whole run (nbqplist and
     // nbqwlist return after their first call); otherwise k_q_lists rebuilds
     // them at every list update, in capacity-sized arrays
-    D.q_dynamic = (nq > 0 && !(s.inp.rcq > t.rexcl_o)) ? 1 : 0;
+    // a periodic box rebuilds them at every list update (nbqplist_box and
+    // nbqwlist_box, with a positive Rcq)
+    D.q_dynamic = (nq > 0 && (s.pbc || !(s.inp.rcq > t.rexcl_o))) ? 1 : 0;
     D.rcq2 = rcq2;
     {
       std::vector<int> cnt = {(int)qp_atoms.size(), (int)qw_waters.size()};
@@ -728,6 +756,22 @@ void Batch::upload(int k) {
     D.qp_cand = I->upload(qp_cand);
     D.qp_gin = I->alloc<unsigned char>(std::max(1, G.nsolute));
     for (int c = 0; c < 3; ++c) D.xpcent[c] = t.xpcent[c];
+    // periodic box
+    D.pbc = s.pbc ? 1 : 0;
+    for (int c = 0; c < 3; ++c) {
+      D.boxl[c] = s.boxl[c]; D.inv_boxl[c] = s.inv_boxl[c];
+      D.boxl_f[c] = (float)s.boxl[c]; D.inv_boxl_f[c] = (float)s.inv_boxl[c];
+    }
+    D.qswitch = s.pbc ? std::max(0, s.qswitch) : 0;
+    D.qp_shatom = I->alloc<int>(std::max(1, G.nsolute));
+    D.boxc = I->upload(std::vector<double>{s.boxc[0], s.boxc[1], s.boxc[2], -1.0});
+    D.rigid_boxc = s.inp.rigid_box_centre ? 1 : 0;
+    D.put_solute_back = s.inp.put_solute_back ? 1 : 0;
+    D.put_solvent_back = s.inp.put_solvent_back ? 1 : 0;
+    D.nmol = s.nmol;
+    D.nsolute_mol = s.nsolute_mol;
+    D.mol_start = I->upload(t.istart_mol);
+    D.mol_minv = I->upload(s.pbc ? s.mol_minv : std::vector<double>(1, 0.0));
     D.qp_atoms = I->upload(qp_atoms);
     D.qw_waters = I->upload(qw_waters);
     D.e_q = I->alloc<double>((size_t)std::max(1, nq * s.nstates) * 4);
@@ -876,8 +920,9 @@ void Batch::upload(int k) {
     for (const qsys::RstAng& r : s.rstang)
       add_xt({XT_RST_ANG, r.i, r.j, r.k, 0, r.ipsi, 0, 0, 0, 0}, {r.ang * deg2rad, r.fk, 0, 0, 0, 0, 0, 0},
              {r.i, r.j, r.k});
+    // wall restraints only for spheres (src/md.f90:15768)
     for (const qsys::RstWal& r : s.rstwal)
-      for (int i = r.i; i <= r.j; ++i)
+      for (int i = r.i; i <= r.j && !s.pbc; ++i)
         if (s.heavy[i] || r.ih == 1)
           add_xt({XT_RST_WALL, i, 0, 0, 0, -1, 0, 0, 0, 0}, {r.d, r.fk, r.aMorse, r.dMorse, 0, 0, 0, 0}, {i});
     D.nxt = (int)xatoms.size();
@@ -1242,6 +1287,15 @@ void Batch::join() {
   ls_ = stream_;
 }
 
+// put_back_in_box of the periodic-box instances at a step's list update
+// (md_run, src/md.f90:6483-6485), then the FP32 coordinate copies
+void Batch::launch_put_back() {
+  if (!put_back_) return;
+  const int ni = (int)h_sys_.size();
+  hipLaunchKernelGGL(k_put_back, dim3(1, ni), dim3(256), 0, stream_, d_sys_);
+  hipLaunchKernelGGL(k_x_to_f32, dim3((3 * max_natom_ + 255) / 256, ni), dim3(256), 0, stream_, d_sys_);
+}
+
 void Batch::launch_lists() {
   const int ni = (int)h_sys_.size();
   dim3 g0((max_ncgp_ + 255) / 256, ni);
@@ -1484,7 +1538,10 @@ void Batch::forces() {
 }
 
 void Batch::one_step(bool energies) {
-  if (istep_ % nbcycle_ == 0) launch_lists();
+  if (istep_ % nbcycle_ == 0) {
+    launch_put_back();
+    launch_lists();
+  }
   launch_forces(true, istep_, energies);
   launch_integrate(istep_);
   ++istep_;
@@ -1502,7 +1559,10 @@ void Batch::launch_cycle_graph(bool final_energies) {
     hipGraph_t g = nullptr;
     HIPCHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < nbcycle_; ++k) {
-      if (k == 0) launch_lists();
+      if (k == 0) {
+        launch_put_back();
+        launch_lists();
+      }
       launch_forces(true, istep_ + k, fin && k == nbcycle_ - 1);
       launch_integrate(istep_ + k);
     }
@@ -1655,6 +1715,16 @@ void Batch::step(int nsteps) {
   }
   HIPCHECK(hipEventRecord(ev1, stream_));
   HIPCHECK(hipStreamSynchronize(stream_));
+}
+
+// Periodic box of instance k: box lengths and the current centre (6
+// values); false for a sphere.
+bool Batch::box(int k, double* out) {
+  HIPCHECK(hipStreamSynchronize(stream_));
+  const DevSys& D = h_sys_[k];
+  for (int c = 0; c < 3; ++c) out[c] = D.boxl[c];
+  HIPCHECK(hipMemcpy(out + 3, D.boxc, 3 * sizeof(double), hipMemcpyDeviceToHost));
+  return D.pbc != 0;
 }
 
 void Batch::get_state(int k, double* x, double* v) {
@@ -2394,6 +2464,22 @@ int qgpu_write_restart(const char* path, int natom, const double* x, const doubl
     std::vector<double> tc;
     if (nshell > 0) tc.assign(theta_corr, theta_corr + nshell);
     qsys::write_restart(path, xv, vv, nshell > 0 ? &tc : nullptr);
+  });
+}
+
+int qgpu_write_restart_box(const char* path, int natom, const double* x, const double* v, const double* box) {
+  return guarded([&] {
+    if (!path || !x || !v || !box || natom <= 0) throw std::runtime_error("invalid restart arguments");
+    std::vector<double> xv(x, x + 3 * (size_t)natom), vv(v, v + 3 * (size_t)natom);
+    qsys::write_restart(path, xv, vv, nullptr, box);
+  });
+}
+
+int qgpu_box(qgpu_handle* h, int k, double* out, int* is_box) {
+  return guarded([&] {
+    CHECK_K(h, k);
+    if (!out || !is_box) throw std::runtime_error("null output");
+    *is_box = h->b->box(k, out) ? 1 : 0;
   });
 }
 

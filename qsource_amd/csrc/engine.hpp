@@ -199,6 +199,21 @@ struct DevSys {
   const unsigned char* qp_cand;  // nat_solute: atom may enter the Q-solute list
   unsigned char* qp_gin;         // ncgp_solute: group's switch atom within Rcq (k_q_lists)
   double xpcent[3];              // solute centre (topology)
+  // periodic box (pbc != 0): edge lengths and their inverses (FP64 and FP32
+  // copies), the Q atoms' switching atom, per solute device group the atom
+  // whose image gives the group's Q-atom shift (-1: group not in the Q-solute
+  // list; k_q_lists), the box centre (moves with put_back_in_box), and the
+  // molecule table put_back_in_box works on
+  int pbc;
+  double boxl[3], inv_boxl[3];
+  float boxl_f[3], inv_boxl_f[3];
+  int qswitch;
+  int* qp_shatom;
+  double* boxc;
+  int rigid_boxc, put_solute_back, put_solvent_back;
+  int nmol, nsolute_mol;
+  const int* mol_start;          // nmol + 1 (0-based first atom of each molecule, then natom)
+  const double* mol_minv;        // 1 / mass of each molecule
   int* qp_atoms;                 // solute atoms in the Q-solute list, ascending
   int* qw_waters;                // waters in the Q-water list, ascending
   // Q-atom LJ types (qvdw_flag): qiac [iq*nstates+s]; qavdw/qbvdw [type*3 +
@@ -331,6 +346,7 @@ class Batch {
   void make_lists();
   void forces();
   void forces_terms(int mask);
+  bool box(int k, double* out);
   void get_state(int k, double* x, double* v);
   void set_state(int k, const double* x, const double* v);
   void get_gradient(int k, double* d);
@@ -422,6 +438,7 @@ class Batch {
   int nbcycle_ = 10;
   bool allatom_ = false;   // all-atom solvent LJ (topology solvent type 1)
   bool q_dynamic_ = false; // some instance rebuilds its Q lists at list updates
+  bool put_back_ = false;  // some periodic-box instance puts molecules back at list updates
   bool freeze_ = false;    // some instance has excluded_freeze on
   int iene_ = 0;
   long ebase_ = 0;         // step of the last set_energy_interval
@@ -445,6 +462,7 @@ class Batch {
  private:
   bool record_possible(long s) const;
   void launch_tuned(int kernel, const LaunchShape& sh);
+  void launch_put_back();
   const std::pair<int2*, int>& work_map(int kernel, int map, int block);
   LaunchShape shape_[TK_COUNT];
   bool tuned_ = false;

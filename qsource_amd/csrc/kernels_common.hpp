@@ -213,6 +213,22 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 __device__ __forceinline__ double block_sum_256(double v, double* red) { return block_sum<4>(v, red); }
 __device__ __forceinline__ double block_sum_1024(double v, double* red) { return block_sum<16>(v, red); }
 
+// Periodic box: the shift boxlength * nint(d / boxlength) of one component
+// of a difference vector (nint rounds half away from zero, as round does).
+// FP64 in the reference's form; FP32 from the FP32 copies for the
+// mixed-precision pair terms (a listed pair's switching atoms are well
+// within half a box apart, so both pick the same image).
+__device__ __forceinline__ double pbc_shift(const DevSys& S, double d, int k) {
+  return S.boxl[k] * round(d * S.inv_boxl[k]);
+}
+__device__ __forceinline__ float pbc_shift(const DevSys& S, float d, int k) {
+  return S.boxl_f[k] * roundf(d * S.inv_boxl_f[k]);
+}
+// the minimum image of one component (FP64, as the list builders form it)
+__device__ __forceinline__ double pbc_image(const DevSys& S, double d, int k) {
+  return d - S.boxl[k] * round(d * S.inv_boxl[k]);
+}
+
 // The reference visits a pair of charge groups / waters (a, b), 1-based,
 // only when (a < b and a+b even) or (a > b and a+b odd)
 // (src/md.f90:7700-7702, 10977-10980).  With 0-based indices the parity of
@@ -375,15 +391,30 @@ struct LrfRaw {
   }
 };
 
+// Periodic box: the shift of the partner group whose switching atom is sw
+// against centre c (lrf_update, src/md.f90:5904-5917), zero for a sphere.
+__device__ __forceinline__ void lrf_box_shift(const DevSys& S, int sw, double c0, double c1, double c2,
+                                              double* sh) {
+  if (S.pbc) {
+    sh[0] = pbc_shift(S, S.x[3 * sw] - c0, 0);
+    sh[1] = pbc_shift(S, S.x[3 * sw + 1] - c1, 1);
+    sh[2] = pbc_shift(S, S.x[3 * sw + 2] - c2, 2);
+  } else {
+    sh[0] = sh[1] = sh[2] = 0.0;
+  }
+}
+
 // The FP32 raw moments of the non-Q atoms of charge group g (as seen from
 // centre c), added to part.
 __device__ __forceinline__ void lrf_raw_group(const DevSys& S, int g, double c0, double c1, double c2,
                                               LrfRaw<float>& part) {
+  double sh[3];
+  lrf_box_shift(S, S.cgp_sw[g], c0, c1, c2, sh);
   for (int a = S.cgp_first[g]; a <= S.cgp_last[g]; ++a) {
     const int i = S.cgpatom[a];
     if (S.flags[i] & F_Q) continue;
-    part.add((float)S.crg[i], (float)(S.x[3 * i] - c0), (float)(S.x[3 * i + 1] - c1),
-             (float)(S.x[3 * i + 2] - c2));
+    part.add((float)S.crg[i], (float)(S.x[3 * i] - c0 - sh[0]), (float)(S.x[3 * i + 1] - c1 - sh[1]),
+             (float)(S.x[3 * i + 2] - c2 - sh[2]));
   }
 }
 
@@ -392,10 +423,15 @@ template <typename P>
 __device__ __forceinline__ void lrf_gather_group(const DevSys& S, int g, double c0, double c1,
                                                  double c2, LrfAcc& acc) {
   if constexpr (sizeof(P) == 8) {
+    double sh[3];
+    lrf_box_shift(S, S.cgp_sw[g], c0, c1, c2, sh);
     for (int a = S.cgp_first[g]; a <= S.cgp_last[g]; ++a) {
       const int i = S.cgpatom[a];
       if (S.flags[i] & F_Q) continue;
-      acc.add(S.crg[i], S.x[3 * i] - c0, S.x[3 * i + 1] - c1, S.x[3 * i + 2] - c2);
+      if (S.pbc)
+        acc.add(S.crg[i], S.x[3 * i] - c0 - sh[0], S.x[3 * i + 1] - c1 - sh[1], S.x[3 * i + 2] - c2 - sh[2]);
+      else
+        acc.add(S.crg[i], S.x[3 * i] - c0, S.x[3 * i + 1] - c1, S.x[3 * i + 2] - c2);
     }
   } else {
     // mixed precision: the group's moments in FP32 (raw form, then the
@@ -415,29 +451,47 @@ __device__ __forceinline__ void lrf_gather_group(const DevSys& S, int g, double 
 template <typename P>
 __device__ __forceinline__ void lrf_gather_water(const DevSys& S, int jo, int jw, double c0, double c1,
                                                  double c2, const float* cf, LrfAcc& acc) {
+  double sh[3];
+  lrf_box_shift(S, jo, c0, c1, c2, sh);
   if constexpr (sizeof(P) == 8) {
     (void)jw;
     const gconst<double> xj = gptr(S.x) + 3 * jo;
-    for (int k = 0; k < 3; ++k)
-      acc.add(S.chg_solv[k], xj[3 * k] - c0, xj[3 * k + 1] - c1, xj[3 * k + 2] - c2);
+    for (int k = 0; k < 3; ++k) {
+      if (S.pbc)
+        acc.add(S.chg_solv[k], xj[3 * k] - c0 - sh[0], xj[3 * k + 1] - c1 - sh[1], xj[3 * k + 2] - c2 - sh[2]);
+      else
+        acc.add(S.chg_solv[k], xj[3 * k] - c0, xj[3 * k + 1] - c1, xj[3 * k + 2] - c2);
+    }
   } else {
-    (void)jo;
     float xj[9];
     water_xf(S, jw, xj);
+    const float sf[3] = {(float)sh[0], (float)sh[1], (float)sh[2]};
     LrfRaw<float> part;
     part.zero();
     for (int k = 0; k < 3; ++k)
-      part.add((float)S.chg_solv[k], xj[3 * k] - cf[0], xj[3 * k + 1] - cf[1], xj[3 * k + 2] - cf[2]);
+      part.add((float)S.chg_solv[k], xj[3 * k] - cf[0] - sf[0], xj[3 * k + 1] - cf[1] - sf[1],
+               xj[3 * k + 2] - cf[2] - sf[2]);
     acc.add(part.terms());
   }
 }
 
 // The raw moments of standard water jw's three atoms about the FP32 centre
-// cf, added to part (FP32).
+// cf, added to part (FP32); in a periodic box moved by the shift of the
+// water's O against the FP64 centre c.
 constexpr int kLrfPend = 8;   // partner tiles whose far waters' FP32 raw moments are summed before conversion
-__device__ __forceinline__ void lrf_raw_water(const DevSys& S, int jw, const float* cf, LrfRaw<float>& part) {
+__device__ __forceinline__ void lrf_raw_water(const DevSys& S, int jw, const double* c, const float* cf,
+                                              LrfRaw<float>& part) {
   float xj[9];
   water_xf(S, jw, xj);
+  if (S.pbc) {
+    double sh[3];
+    lrf_box_shift(S, S.nat_solute + 3 * jw, c[0], c[1], c[2], sh);
+    const float sf[3] = {(float)sh[0], (float)sh[1], (float)sh[2]};
+    for (int k = 0; k < 3; ++k)
+      part.add((float)S.chg_solv[k], xj[3 * k] - cf[0] - sf[0], xj[3 * k + 1] - cf[1] - sf[1],
+               xj[3 * k + 2] - cf[2] - sf[2]);
+    return;
+  }
   for (int k = 0; k < 3; ++k)
     part.add((float)S.chg_solv[k], xj[3 * k] - cf[0], xj[3 * k + 1] - cf[1], xj[3 * k + 2] - cf[2]);
 }

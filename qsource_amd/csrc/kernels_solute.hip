@@ -90,6 +90,7 @@ __global__ __launch_bounds__(256) void k_solute_lists(const DevSys* __restrict__
       int js = S.cgp_sw[jg];
       if (!(S.flags[js] & F_EXCL)) {
         double a = xs0 - S.x[3 * js], b = xs1 - S.x[3 * js + 1], c = xs2 - S.x[3 * js + 2];
+        if (S.pbc) { a = pbc_image(S, a, 0); b = pbc_image(S, b, 1); c = pbc_image(S, c, 2); }
         double r2 = a * a + b * b + c * c;
         inside = r2 <= S.rcpp2;
         far = !inside && r2 <= S.rclrf2;
@@ -114,6 +115,7 @@ __global__ __launch_bounds__(256) void k_solute_lists(const DevSys* __restrict__
     int jo = S.nat_solute + 3 * jw;
     if (jw < S.nwat && !(S.flags[jo] & F_EXCL)) {
       double a = xs0 - S.x[3 * jo], b = xs1 - S.x[3 * jo + 1], c = xs2 - S.x[3 * jo + 2];
+      if (S.pbc) { a = pbc_image(S, a, 0); b = pbc_image(S, b, 1); c = pbc_image(S, c, 2); }
       double r2 = a * a + b * b + c * c;
       inside = r2 <= S.rcpw2;
       far = !inside && r2 <= S.rclrf2;
@@ -199,6 +201,15 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(c
       const bool same = jg == g;
       const bool own = same || owns_pair(g, jg);
       const int fj = S.cgp_first[jg];
+      // periodic box (nonbond_pp_box :13159-13280): the group pair's shift
+      // from the two switching atoms, as seen from this atom's group
+      P ps0 = 0, ps1 = 0, ps2 = 0;
+      if (S.pbc) {
+        const int swi = S.cgp_sw[g], swj = S.cgp_sw[jg];
+        ps0 = pbc_shift(S, X[3 * swj] - X[3 * swi], 0);
+        ps1 = pbc_shift(S, X[3 * swj + 1] - X[3 * swi + 1], 1);
+        ps2 = pbc_shift(S, X[3 * swj + 2] - X[3 * swi + 2], 2);
+      }
       while (todo) {
         const int b = __builtin_ctz(todo);
         todo &= todo - 1;
@@ -210,6 +221,7 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(c
         // dx = x_j - x_i; the reference pair (ia, ja) has dx of the same sign
         // when i is first, and the opposite sign otherwise
         P dx1 = X[3 * j] - xi0, dx2 = X[3 * j + 1] - xi1, dx3 = X[3 * j + 2] - xi2;
+        if (S.pbc) { dx1 = dx1 - ps0; dx2 = dx2 - ps1; dx3 = dx3 - ps2; }
         if (!me_first) { dx1 = -dx1; dx2 = -dx2; dx3 = -dx3; }
         P r;
         const P r2 = inv_r2(pnorm2(dx1, dx2, dx3), r);
@@ -255,6 +267,15 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(c
     // standard waters carry the solvent charges (checked at upload)
     const P qw3[3] = {(P)S.chg_solv[0], (P)S.chg_solv[1], (P)S.chg_solv[2]};
     auto pw_water = [&](const int jo, const P* xw) {
+      // periodic box (nonbond_pw_box): the shift of the water O against the
+      // group's switching atom
+      P ws0 = 0, ws1 = 0, ws2 = 0;
+      if (S.pbc) {
+        const int sw = S.cgp_sw[g];
+        ws0 = pbc_shift(S, xw[0] - X[3 * sw], 0);
+        ws1 = pbc_shift(S, xw[1] - X[3 * sw + 1], 1);
+        ws2 = pbc_shift(S, xw[2] - X[3 * sw + 2], 2);
+      }
       for (int ip = 0; ip < 3; ++ip) {
         const int j = jo + ip;
         P aLJ, bLJ;
@@ -269,7 +290,8 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(c
           aLJ = (P)(S.avdw[ti * 3 + lj] * S.avdw[tj * 3 + lj]);
           bLJ = (P)(S.bvdw[ti * 3 + lj] * S.bvdw[tj * 3 + lj]);
         }
-        const P dx1 = xw[3 * ip] - xi0, dx2 = xw[3 * ip + 1] - xi1, dx3 = xw[3 * ip + 2] - xi2;
+        P dx1 = xw[3 * ip] - xi0, dx2 = xw[3 * ip + 1] - xi1, dx3 = xw[3 * ip + 2] - xi2;
+        if (S.pbc) { dx1 = dx1 - ws0; dx2 = dx2 - ws1; dx3 = dx3 - ws2; }
         P r;
         const P r2 = inv_r2(pnorm2(dx1, dx2, dx3), r);
         const P r6 = r2 * r2 * r2;
@@ -329,12 +351,22 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? 6 : 1) void k_solute_forces(c
     }
     // Q-solute pairs seen from the solute atom (the Q atom is i of the pair)
     if ((fl & F_QP) && (S.term_mask & TM_QX)) {
+      // periodic box (nonbond_qp_box :13925-14096): the group's shift atom
+      // against the Q switching atom
+      P qs0 = 0, qs1 = 0, qs2 = 0;
+      if (S.pbc) {
+        const int sa = S.qp_shatom[g], qw = S.qswitch;
+        qs0 = pbc_shift(S, X[3 * sa] - X[3 * qw], 0);
+        qs1 = pbc_shift(S, X[3 * sa + 1] - X[3 * qw + 1], 1);
+        qs2 = pbc_shift(S, X[3 * sa + 2] - X[3 * qw + 2], 2);
+      }
       for (int iq = sub; iq < S.nqat; iq += kSFLanes) {
         const int qa = S.iqseq[iq];
         const int lj = S.qp_lj[(size_t)i * S.nqat + iq];
         const int l = lj - 1;
         const int tq = S.iac[qa];
-        const P dx1 = xi0 - X[3 * qa], dx2 = xi1 - X[3 * qa + 1], dx3 = xi2 - X[3 * qa + 2];
+        P dx1 = xi0 - X[3 * qa], dx2 = xi1 - X[3 * qa + 1], dx3 = xi2 - X[3 * qa + 2];
+        if (S.pbc) { dx1 = dx1 - qs0; dx2 = dx2 - qs1; dx3 = dx3 - qs2; }
         const P r2sq = pnorm2(dx1, dx2, dx3);
         P r;
         const P r2 = inv_r2(r2sq, r);
@@ -449,7 +481,17 @@ __global__ __launch_bounds__(256) void k_q_forces(const DevSys* __restrict__ sys
     const int lj = S.qp_lj[(size_t)j * S.nqat + iq];
     const int l = lj - 1;
     const int tj = S.iac[j];
-    P dx1 = (P)(S.x[3 * j] - xq0), dx2 = (P)(S.x[3 * j + 1] - xq1), dx3 = (P)(S.x[3 * j + 2] - xq2);
+    P dx1, dx2, dx3;
+    if (S.pbc) {
+      // periodic box (nonbond_qp_box): the shift of the partner's group
+      const int sa = S.qp_shatom[S.iwhich_cgp[j]], qw = S.qswitch;
+      const double s0 = pbc_shift(S, S.x[3 * sa] - S.x[3 * qw], 0);
+      const double s1 = pbc_shift(S, S.x[3 * sa + 1] - S.x[3 * qw + 1], 1);
+      const double s2 = pbc_shift(S, S.x[3 * sa + 2] - S.x[3 * qw + 2], 2);
+      dx1 = (P)(S.x[3 * j] - xq0 - s0); dx2 = (P)(S.x[3 * j + 1] - xq1 - s1); dx3 = (P)(S.x[3 * j + 2] - xq2 - s2);
+    } else {
+      dx1 = (P)(S.x[3 * j] - xq0); dx2 = (P)(S.x[3 * j + 1] - xq1); dx3 = (P)(S.x[3 * j + 2] - xq2);
+    }
     const P r2sq = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
     P r;
     P r2 = inv_r2(r2sq, r);
@@ -494,7 +536,21 @@ __global__ __launch_bounds__(256) void k_q_forces(const DevSys* __restrict__ sys
   const P cO = (P)S.chg_solv[0];
   for (int k = chunk * 64 + lane; k < nqw; k += stride) {
     const int jo = S.nat_solute + 3 * S.qw_waters[k];
-    P dx1 = (P)(S.x[3 * jo] - xq0), dx2 = (P)(S.x[3 * jo + 1] - xq1), dx3 = (P)(S.x[3 * jo + 2] - xq2);
+    // periodic box (nonbond_qw_spc_box): the water O's shift against the Q
+    // switching atom, for all three water atoms
+    double ws0 = 0.0, ws1 = 0.0, ws2 = 0.0;
+    if (S.pbc) {
+      const int qw = S.qswitch;
+      ws0 = pbc_shift(S, S.x[3 * jo] - S.x[3 * qw], 0);
+      ws1 = pbc_shift(S, S.x[3 * jo + 1] - S.x[3 * qw + 1], 1);
+      ws2 = pbc_shift(S, S.x[3 * jo + 2] - S.x[3 * qw + 2], 2);
+    }
+    P dx1, dx2, dx3;
+    if (S.pbc) {
+      dx1 = (P)(S.x[3 * jo] - xq0 - ws0); dx2 = (P)(S.x[3 * jo + 1] - xq1 - ws1); dx3 = (P)(S.x[3 * jo + 2] - xq2 - ws2);
+    } else {
+      dx1 = (P)(S.x[3 * jo] - xq0); dx2 = (P)(S.x[3 * jo + 1] - xq1); dx3 = (P)(S.x[3 * jo + 2] - xq2);
+    }
     P r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
     P r6_hc = r2 * r2 * r2;
     P r;
@@ -527,7 +583,12 @@ __global__ __launch_bounds__(256) void k_q_forces(const DevSys* __restrict__ sys
     g2 = g2 - (double)(dv * dx3);
     for (int ip = 1; ip < 3; ++ip) {
       const int ja = jo + ip;
-      P ex1 = (P)(S.x[3 * ja] - xq0), ex2 = (P)(S.x[3 * ja + 1] - xq1), ex3 = (P)(S.x[3 * ja + 2] - xq2);
+      P ex1, ex2, ex3;
+      if (S.pbc) {
+        ex1 = (P)(S.x[3 * ja] - xq0 - ws0); ex2 = (P)(S.x[3 * ja + 1] - xq1 - ws1); ex3 = (P)(S.x[3 * ja + 2] - xq2 - ws2);
+      } else {
+        ex1 = (P)(S.x[3 * ja] - xq0); ex2 = (P)(S.x[3 * ja + 1] - xq1); ex3 = (P)(S.x[3 * ja + 2] - xq2);
+      }
       P rr;
       const P h2 = ex1 * ex1 + ex2 * ex2 + ex3 * ex3;
       P q2 = inv_r2(h2, rr);
@@ -616,6 +677,10 @@ __device__ __forceinline__ int ordered_compact(bool take, int* wave_cnt, int& to
   return pos;
 }
 
+// Periodic box (nbqplist_box :10327-10553, nbqwlist_box :10554-10700): a
+// solute charge group enters when any of its atoms (in the group's order)
+// lies within Rcq of the Q switching atom (minimum image), and that atom is
+// kept as the group's shift atom (qp_shatom); a water when its O does.
 __global__ __launch_bounds__(1024) void k_q_lists(const DevSys* __restrict__ sys) {
   const DevSys& S = sys[blockIdx.y];
   if (!S.q_dynamic) return;
@@ -623,12 +688,29 @@ __global__ __launch_bounds__(1024) void k_q_lists(const DevSys* __restrict__ sys
   unsigned char* fl = const_cast<unsigned char*>(S.flags);
   const double p0 = S.xpcent[0], p1 = S.xpcent[1], p2 = S.xpcent[2];
   auto inside = [&](int a) {
+    if (S.pbc) {
+      const int q3 = 3 * S.qswitch;
+      const double d0 = pbc_image(S, S.x[3 * a] - S.x[q3], 0), d1 = pbc_image(S, S.x[3 * a + 1] - S.x[q3 + 1], 1),
+                   d2 = pbc_image(S, S.x[3 * a + 2] - S.x[q3 + 2], 2);
+      return d0 * d0 + d1 * d1 + d2 * d2 <= S.rcq2;
+    }
     const double d0 = S.x[3 * a] - p0, d1 = S.x[3 * a + 1] - p1, d2 = S.x[3 * a + 2] - p2;
     return !(fl[a] & F_EXCL) && d0 * d0 + d1 * d1 + d2 * d2 <= S.rcq2;
   };
   // groups first (their switch atoms' flags are read before any flag of
   // this instance is rewritten), then the atoms
-  for (int g = threadIdx.x; g < S.ncgp_solute; g += blockDim.x) S.qp_gin[g] = inside(S.cgp_sw[g]) ? 1 : 0;
+  for (int g = threadIdx.x; g < S.ncgp_solute; g += blockDim.x) {
+    if (S.pbc) {
+      // the whole group's atoms (all slices of a split group see the same)
+      int sa = -1;
+      for (int a = S.cgp_cfirst[g]; a <= S.cgp_clast[g] && sa < 0; ++a)
+        if (inside(S.cgpatom[a])) sa = S.cgpatom[a];
+      S.qp_shatom[g] = sa;
+      S.qp_gin[g] = sa >= 0 ? 1 : 0;
+    } else {
+      S.qp_gin[g] = inside(S.cgp_sw[g]) ? 1 : 0;
+    }
+  }
   __threadfence_block();
   __syncthreads();
   int nqp = 0;
@@ -681,7 +763,8 @@ __global__ __launch_bounds__(256) void k_monitor(const DevSys* __restrict__ sys)
     const int i = p / nj, j = p % nj;
     const int a = ai[i], b = aj[j];
     const int qa = S.iqatom_d[a], qb = S.iqatom_d[b];
-    const double dx1 = pos(a, 0) - pos(b, 0), dx2 = pos(a, 1) - pos(b, 1), dx3 = pos(a, 2) - pos(b, 2);
+    double dx1 = pos(a, 0) - pos(b, 0), dx2 = pos(a, 1) - pos(b, 1), dx3 = pos(a, 2) - pos(b, 2);
+    if (S.pbc) { dx1 = pbc_image(S, dx1, 0); dx2 = pbc_image(S, dx2, 1); dx3 = pbc_image(S, dx3, 2); }   // :11269-11273
     const double r2 = dx1 * dx1 + dx2 * dx2 + dx3 * dx3;
     const double r = sqrt(1.0 / r2);
     const double r6_hc = r2 * r2 * r2;
@@ -997,7 +1080,9 @@ __device__ void xterm(const DevSys& S, int n) {
     case XT_RST_SEQ: {
       // sequence restraint of one atom to its topology position (:15549-15575)
       const int i3 = 3 * R[1];
-      const double d0 = x[i3] - S.xtop[i3], d1 = x[i3 + 1] - S.xtop[i3 + 1], d2 = x[i3 + 2] - S.xtop[i3 + 2];
+      double d0 = x[i3] - S.xtop[i3], d1 = x[i3 + 1] - S.xtop[i3 + 1], d2 = x[i3 + 2] - S.xtop[i3 + 2];
+      // periodic box: the minimal periodic distance (:15561-15565)
+      if (S.pbc) { d0 = pbc_image(S, d0, 0); d1 = pbc_image(S, d1, 1); d2 = pbc_image(S, d2, 2); }
       e = 0.5 * P[0] * (d0 * d0 + d1 * d1 + d2 * d2);
       put_slot(S, s0, P[0] * d0, P[0] * d1, P[0] * d2);
       break;
@@ -1040,6 +1125,7 @@ __device__ void xterm(const DevSys& S, int n) {
       const double wgt = R[5] != 0 ? S.lambda[R[5] - 1] : 1.0;
       double dr[3];
       for (int c = 0; c < 3; ++c) dr[c] = x[j3 + c] - x[i3 + c];
+      if (S.pbc) for (int c = 0; c < 3; ++c) dr[c] = pbc_image(S, dr[c], c);   // :15630-15634
       const double b = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
       double db = 0.0;
       if (b < P[0]) db = b - P[0];
@@ -1056,6 +1142,8 @@ __device__ void xterm(const DevSys& S, int n) {
       const double wgt = R[5] != 0 ? S.lambda[R[5] - 1] : 1.0;
       double dr[3], dr2[3];
       for (int c = 0; c < 3; ++c) { dr[c] = x[i3 + c] - x[j3 + c]; dr2[c] = x[k3 + c] - x[j3 + c]; }
+      if (S.pbc)   // :15694-15702
+        for (int c = 0; c < 3; ++c) { dr[c] = pbc_image(S, dr[c], c); dr2[c] = pbc_image(S, dr2[c], c); }
       const double r2ij = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
       const double r2jk = dr2[0] * dr2[0] + dr2[1] * dr2[1] + dr2[2] * dr2[2];
       const double rij = sqrt(r2ij), rjk = sqrt(r2jk);

@@ -58,6 +58,60 @@ __global__ void k_freeze_x(const DevSys* __restrict__ sys) {
   S.xq[c] = (S.flags[c / 3] & F_EXCL) ? S.xtop[c] : S.x[c];
 }
 
+// put_back_in_box (src/md.f90:18743-18930), periodic box, at list updates:
+// the box centre follows the solute (all atoms without one) unless it is
+// rigid; every molecule of the chosen kinds whose centre of mass lies
+// beyond a face moves by one box length along that axis.  One workgroup per
+// instance: the centre is summed in atom order by three threads, then each
+// thread takes molecules (each summed in atom order), as the reference does.
+// The FP32 copies are refreshed afterwards (k_x_to_f32); the LRF centres
+// are formed again by the list build that follows.
+// boxc[3] holds the step of the last put-back, so that a list update redone
+// after a list regrowth does not put molecules back a second time.
+__global__ __launch_bounds__(256) void k_put_back(const DevSys* __restrict__ sys) {
+  const DevSys& S = sys[blockIdx.y];
+  if (!S.pbc || !(S.put_solute_back || S.put_solvent_back)) return;
+  if (halted(S, PH_LISTS)) return;
+  const double st = (double)*S.step_ctr;
+  if (S.boxc[3] == st) return;
+  __shared__ double boxc[3];
+  const int t = threadIdx.x;
+  if (t < 3) {
+    double c;
+    if (!S.rigid_boxc) {
+      const int last = S.nat_solute > 0 ? S.nat_solute : S.natom;
+      c = 0.0;
+      for (int i = 0; i < last; ++i) c = c + S.x[3 * i + t];
+      c = c / (double)last;
+      S.boxc[t] = c;
+    } else {
+      c = S.boxc[t];
+    }
+    boxc[t] = c;
+  }
+  __syncthreads();
+  double hi[3], lo[3];
+  for (int k = 0; k < 3; ++k) { hi[k] = boxc[k] + S.boxl[k] / 2; lo[k] = boxc[k] - S.boxl[k] / 2; }
+  const int m0 = S.put_solute_back ? 0 : S.nsolute_mol;
+  const int m1 = S.put_solvent_back ? S.nmol : S.nsolute_mol;
+  for (int m = m0 + t; m < m1; m += blockDim.x) {
+    const int a0 = S.mol_start[m], a1 = S.mol_start[m + 1];
+    double cm[3] = {0.0, 0.0, 0.0};
+    for (int j = a0; j < a1; ++j)
+      for (int k = 0; k < 3; ++k) cm[k] = cm[k] + S.x[3 * j + k] * S.mass[j];
+    for (int k = 0; k < 3; ++k) {
+      cm[k] = cm[k] * S.mol_minv[m];
+      double mv = 0.0;
+      if (cm[k] > hi[k]) mv = -S.boxl[k];
+      else if (cm[k] < lo[k]) mv = S.boxl[k];
+      if (mv != 0.0)
+        for (int j = a0; j < a1; ++j) S.x[3 * j + k] = S.x[3 * j + k] + mv;
+    }
+  }
+  __syncthreads();
+  if (t == 0) S.boxc[3] = st;
+}
+
 // ---------------------------------------------------------------------------
 // Water lists (nbwwlist_lrf src/md.f90:10897-11021 and the water side of
 // nbpwlist_lrf :9300-9429).  kWLLanes lanes per water (8 in the FP64
@@ -91,6 +145,7 @@ __global__ __launch_bounds__(256) void k_water_lists(const DevSys* __restrict__ 
   double* Lg = S.lrf + (size_t)ig * kLrfStride;
   const double c0 = Lg[0], c1 = Lg[1], c2 = Lg[2];
   const float cf[3] = {(float)c0, (float)c1, (float)c2};
+  const double cc[3] = {c0, c1, c2};
   const unsigned long long below = (1ull << lane) - 1ull;
   const unsigned long long mine = ((1ull << kWLLanes) - 1ull) << seg;
   LrfAcc acc;
@@ -113,7 +168,8 @@ __global__ __launch_bounds__(256) void k_water_lists(const DevSys* __restrict__ 
     if (active && jw < S.nwat && jw != iw) {
       const int jo = S.nat_solute + 3 * jw;
       if (!(gfl[jo] & F_EXCL)) {
-        const double a = xi0 - gx[3 * jo], b = xi1 - gx[3 * jo + 1], c = xi2 - gx[3 * jo + 2];
+        double a = xi0 - gx[3 * jo], b = xi1 - gx[3 * jo + 1], c = xi2 - gx[3 * jo + 2];
+        if (S.pbc) { a = pbc_image(S, a, 0); b = pbc_image(S, b, 1); c = pbc_image(S, c, 2); }
         const double r2 = a * a + b * b + c * c;
         inside = r2 <= S.rcww2;
         far = !inside && r2 <= S.rclrf2;
@@ -129,7 +185,7 @@ __global__ __launch_bounds__(256) void k_water_lists(const DevSys* __restrict__ 
       const int jo = S.nat_solute + 3 * jw;
       if constexpr (sizeof(P) == 4) {
         if (S.std_water) {
-          lrf_raw_water(S, jw, cf, pend);
+          lrf_raw_water(S, jw, cc, cf, pend);
         } else {
           LrfRaw<float> part;
           part.zero();
@@ -163,7 +219,8 @@ __global__ __launch_bounds__(256) void k_water_lists(const DevSys* __restrict__ 
       const int sw = S.cgp_sw[g];
       if (!(gfl[sw] & F_EXCL)) {
         // r2 as the reference forms it: solute switch atom minus water O
-        const double a = gx[3 * sw] - xi0, b = gx[3 * sw + 1] - xi1, c = gx[3 * sw + 2] - xi2;
+        double a = gx[3 * sw] - xi0, b = gx[3 * sw + 1] - xi1, c = gx[3 * sw + 2] - xi2;
+        if (S.pbc) { a = pbc_image(S, a, 0); b = pbc_image(S, b, 1); c = pbc_image(S, c, 2); }
         const double r2 = a * a + b * b + c * c;
         inside = r2 <= S.rcpw2;
         far = !inside && r2 <= S.rclrf2;
@@ -223,7 +280,12 @@ template __global__ void k_water_lists<float, 1>(const DevSys*, const int2*);
 // energies are not read launch with en = 0 (the gradient is the same).  The
 // two cases are separate instantiations of the body (EN), so a force-only
 // step carries no energy arithmetic.
-template <typename P, int PF, int AA, bool EN>
+// BOX: periodic box (nonbond_ww_spc_box :15123-15236, nonbond_pw_box
+// :13369-13490, nonbond_qw_spc_box :14574-14748): each water pair's shift
+// from its O-O vector, each solute group's from the group's switching atom
+// and the water O, the Q atoms' from the water O and the Q switching atom;
+// no solvent restraints.
+template <typename P, int PF, int AA, bool EN, bool BOX>
 __device__ __forceinline__ void water_forces_body(const DevSys& S, const int base) {
   constexpr bool F32 = sizeof(P) == 4;
   const int t = threadIdx.x;
@@ -278,12 +340,19 @@ __device__ __forceinline__ void water_forces_body(const DevSys& S, const int bas
   const gconst<int> row = gptr(S.ww_nb) + (size_t)iw * S.ww_cap;
   auto ww_pair = [&](const int jw, const P* xj) {
     P el = 0, vdw = 0;
+    P s0 = 0, s1 = 0, s2 = 0;
+    if constexpr (BOX) {
+      s0 = pbc_shift(S, xj[0] - xi[0], 0);
+      s1 = pbc_shift(S, xj[1] - xi[1], 1);
+      s2 = pbc_shift(S, xj[2] - xi[2], 2);
+    }
     for (int la = 0; la < 3; ++la) {
       P gl0 = 0, gl1 = 0, gl2 = 0;
       for (int ip = 0; ip < 3; ++ip) {
-        const P dx1 = xj[3 * ip] - xi[3 * la];
-        const P dx2 = xj[3 * ip + 1] - xi[3 * la + 1];
-        const P dx3 = xj[3 * ip + 2] - xi[3 * la + 2];
+        P dx1 = xj[3 * ip] - xi[3 * la];
+        P dx2 = xj[3 * ip + 1] - xi[3 * la + 1];
+        P dx3 = xj[3 * ip + 2] - xi[3 * la + 2];
+        if constexpr (BOX) { dx1 = dx1 - s0; dx2 = dx2 - s1; dx3 = dx3 - s2; }
         const P r2sq = pnorm2(dx1, dx2, dx3);
         P r;
         const P r2 = inv_r2(r2sq, r);
@@ -365,12 +434,20 @@ __device__ __forceinline__ void water_forces_body(const DevSys& S, const int bas
       const int gs = wrow[k];
       const int a0 = S.cgp_first[gs], a1 = S.cgp_last[gs];
       const gconst<f32v4> rec = gptr(reinterpret_cast<const f32v4*>(S.srec));
+      P s0 = 0, s1 = 0, s2 = 0;
+      if constexpr (BOX) {
+        const int sw = S.cgp_sw[gs];
+        s0 = pbc_shift(S, xi[0] - X[3 * sw], 0);
+        s1 = pbc_shift(S, xi[1] - X[3 * sw + 1], 1);
+        s2 = pbc_shift(S, xi[2] - X[3 * sw + 2], 2);
+      }
       for (int a = a0; a <= a1; ++a) {
         const f32v4 r0 = rec[3 * (size_t)a], r1 = rec[3 * (size_t)a + 1], r2v = rec[3 * (size_t)a + 2];
         if (r2v.z != 0.0f) continue;   // a Q atom
         const P ra[3] = {(P)r1.x, (P)r1.y, (P)r1.z}, rb[3] = {(P)r1.w, (P)r2v.x, (P)r2v.y};
         for (int ip = 0; ip < 3; ++ip) {
-          const P dx1 = xi[3 * ip] - (P)r0.x, dx2 = xi[3 * ip + 1] - (P)r0.y, dx3 = xi[3 * ip + 2] - (P)r0.z;
+          P dx1 = xi[3 * ip] - (P)r0.x, dx2 = xi[3 * ip + 1] - (P)r0.y, dx3 = xi[3 * ip + 2] - (P)r0.z;
+          if constexpr (BOX) { dx1 = dx1 - s0; dx2 = dx2 - s1; dx3 = dx3 - s2; }
           P r;
           const P r2 = inv_r2(pnorm2(dx1, dx2, dx3), r);
           const P r6 = r2 * r2 * r2;
@@ -389,6 +466,13 @@ __device__ __forceinline__ void water_forces_body(const DevSys& S, const int bas
   } else {
   for (int k = sub; k < nwp; k += 8) {
     const int gs = wrow[k];
+    P s0 = 0, s1 = 0, s2 = 0;
+    if constexpr (BOX) {
+      const int sw = S.cgp_sw[gs];
+      s0 = pbc_shift(S, xi[0] - X[3 * sw], 0);
+      s1 = pbc_shift(S, xi[1] - X[3 * sw + 1], 1);
+      s2 = pbc_shift(S, xi[2] - X[3 * sw + 2], 2);
+    }
     for (int a = S.cgp_first[gs]; a <= S.cgp_last[gs]; ++a) {
       const int i = S.cgpatom[a];
       if (S.flags[i] & F_Q) continue;
@@ -398,7 +482,8 @@ __device__ __forceinline__ void water_forces_body(const DevSys& S, const int bas
       const P* wlj = S.std_water ? solute_water_lj<P>(S) + 6 * (size_t)i : nullptr;
       for (int ip = 0; ip < 3; ++ip) {
         const int j = io + ip;
-        const P dx1 = xi[3 * ip] - xs0, dx2 = xi[3 * ip + 1] - xs1, dx3 = xi[3 * ip + 2] - xs2;
+        P dx1 = xi[3 * ip] - xs0, dx2 = xi[3 * ip + 1] - xs1, dx3 = xi[3 * ip + 2] - xs2;
+        if constexpr (BOX) { dx1 = dx1 - s0; dx2 = dx2 - s1; dx3 = dx3 - s2; }
         P aLJ, bLJ;
         if (wlj) {
           aLJ = wlj[ip];
@@ -434,13 +519,21 @@ __device__ __forceinline__ void water_forces_body(const DevSys& S, const int bas
   if (active && (S.flags[io] & F_QW) && (S.term_mask & TM_QX)) {
     const int ns = S.nstates;
     const bool qv = S.qvdw != 0;
+    P q0 = 0, q1 = 0, q2s = 0;
+    if constexpr (BOX) {
+      const int qs = S.qswitch;
+      q0 = pbc_shift(S, xi[0] - X[3 * qs], 0);
+      q1 = pbc_shift(S, xi[1] - X[3 * qs + 1], 1);
+      q2s = pbc_shift(S, xi[2] - X[3 * qs + 2], 2);
+    }
     for (int iq = sub; iq < S.nqat; iq += 8) {
       const int qi = S.iqseq[iq];
       const P xq0 = X[3 * qi], xq1 = X[3 * qi + 1], xq2 = X[3 * qi + 2];
       const gconst<P> R = gptr(qw_rec<P>(S)) + (size_t)iq * kQwRec;
       const P qe = R[0], lsum = R[1];
       for (int ip = 0; ip < 3; ++ip) {
-        const P ex1 = xi[3 * ip] - xq0, ex2 = xi[3 * ip + 1] - xq1, ex3 = xi[3 * ip + 2] - xq2;
+        P ex1 = xi[3 * ip] - xq0, ex2 = xi[3 * ip + 1] - xq1, ex3 = xi[3 * ip + 2] - xq2;
+        if constexpr (BOX) { ex1 = ex1 - q0; ex2 = ex2 - q1; ex3 = ex3 - q2s; }
         const P h2 = pnorm2(ex1, ex2, ex3);
         P rr;
         const P q2 = inv_r2(h2, rr);
@@ -496,7 +589,7 @@ __device__ __forceinline__ void water_forces_body(const DevSys& S, const int bas
     }
   }
   // radial solvent restraint on the water O (switching atom), lane 3 (FP64)
-  if (active && sub == 3 && !(S.flags[io] & F_EXCL) && (S.term_mask & TM_OTHER)) {
+  if (!BOX && active && sub == 3 && !(S.flags[io] & F_EXCL) && (S.term_mask & TM_OTHER)) {
     const double shift = S.fk_wsphere != 0.0 ? sqrt(qsys::kBoltz * S.scal[2] / S.fk_wsphere) : 0.0;
     const double dr0 = S.x[3 * io] - S.xwcent[0], dr1 = S.x[3 * io + 1] - S.xwcent[1],
                  dr2 = S.x[3 * io + 2] - S.xwcent[2];
@@ -562,8 +655,13 @@ __global__ __launch_bounds__(256, sizeof(P) == 4 ? W : 1) void k_water_forces(co
   if (!block_work<MAPPED>(sys, map, blockDim.x >> 3, Sp, base)) return;
   const DevSys& S = *Sp;
   if (base + ((threadIdx.x >> 6) << 3) >= S.nwat) return;   // whole wavefronts past the last water
-  if (en) water_forces_body<P, PF, AA, true>(S, base);
-  else water_forces_body<P, PF, AA, false>(S, base);
+  if (S.pbc) {
+    if (en) water_forces_body<P, PF, AA, true, true>(S, base);
+    else water_forces_body<P, PF, AA, false, true>(S, base);
+  } else {
+    if (en) water_forces_body<P, PF, AA, true, false>(S, base);
+    else water_forces_body<P, PF, AA, false, false>(S, base);
+  }
 }
 template __global__ void k_water_forces<double, 0, 0, 6, 0>(const DevSys*, const int2*, int);
 template __global__ void k_water_forces<double, 0, 0, 6, 1>(const DevSys*, const int2*, int);

@@ -38,6 +38,7 @@ SYMBOLS = [
     "qgpu_far_samples", "qgpu_far_lrf_estimate", "qgpu_set_far_model",
     "qgpu_profile_kernels", "qgpu_list_counts", "qgpu_offdiag_states", "qgpu_monitor",
     "qgpu_trj_create", "qgpu_trj_natoms", "qgpu_trj_write", "qgpu_trj_close", "qgpu_write_restart",
+    "qgpu_write_restart_box", "qgpu_box",
     "qgpu_autotune", "qgpu_set_launch_shape",
     "qgpu_instance_status", "qgpu_list_capacity", "qgpu_set_list_capacity", "qgpu_pot_energy_terms",
     "qgpu_work_counts", "qgpu_mlp_gradient_noalloc", "qgpu_system_sizes", "qgpu_offdiag",
@@ -121,6 +122,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.qgpu_trj_write.argtypes = [P, dp]
     lib.qgpu_trj_close.argtypes = [P]
     lib.qgpu_write_restart.argtypes = [ctypes.c_char_p, ctypes.c_int, dp, dp, ctypes.c_int, dp]
+    lib.qgpu_write_restart_box.argtypes = [ctypes.c_char_p, ctypes.c_int, dp, dp, dp]
+    lib.qgpu_box.argtypes = [P, ctypes.c_int, dp, ctypes.POINTER(ctypes.c_int)]
     lib.qgpu_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
@@ -272,6 +275,14 @@ class Engine:
         t = ctypes.c_double()
         self._check(self.lib.qgpu_temperature(self.h, k, ctypes.byref(t)))
         return t.value
+
+    def box(self, k: int = 0):
+        """Periodic box of instance k: (lengths, current centre), or None for
+        a sphere."""
+        out = np.zeros(6)
+        isb = ctypes.c_int()
+        self._check(self.lib.qgpu_box(self.h, k, _dp(out), ctypes.byref(isb)))
+        return (out[:3].copy(), out[3:].copy()) if isb.value else None
 
     def theta_corr(self, k: int = 0) -> np.ndarray:
         n = ctypes.c_int()
@@ -493,11 +504,18 @@ class Trajectory:
             pass
 
 
-def write_restart(path: str, x: np.ndarray, v: np.ndarray, theta_corr: Optional[np.ndarray] = None):
-    """Restart file as qdyn's write_xfin writes it."""
+def write_restart(path: str, x: np.ndarray, v: np.ndarray, theta_corr: Optional[np.ndarray] = None,
+                  box: Optional[np.ndarray] = None):
+    """Restart file as qdyn's write_xfin writes it; `box` (6 values: lengths
+    and centre) for a periodic box."""
     lib = load_library()
     x = np.ascontiguousarray(x, dtype=np.float64).ravel()
     v = np.ascontiguousarray(v, dtype=np.float64).ravel()
+    if box is not None:
+        b = np.ascontiguousarray(box, dtype=np.float64).ravel()
+        if lib.qgpu_write_restart_box(os.fsencode(path), x.size // 3, _dp(x), _dp(v), _dp(b)) != 0:
+            raise QgpuError(lib.qgpu_last_error().decode())
+        return
     tc = np.ascontiguousarray(theta_corr if theta_corr is not None else np.zeros(0), dtype=np.float64)
     if lib.qgpu_write_restart(os.fsencode(path), x.size // 3, _dp(x), _dp(v), tc.size,
                               _dp(tc) if tc.size else None) != 0:

@@ -219,6 +219,12 @@ def _abnormal(log: TextIO, msg: str, at_step: int, restart: Optional[str]) -> No
     log.write("!" * 79 + "\nABNORMAL TERMINATION of Qdyn5\nTerminating due to " + msg + "\n" + "!" * 79 + "\n")
 
 
+def _box(e, k):
+    """The restart's box record of a periodic-box instance (lengths, centre)."""
+    b = e.box(k)
+    return None if b is None else np.concatenate(b)
+
+
 def run(inputs: Sequence[str], precision: int = qe.QGPU_MIXED,
         far_model: Optional["qe.FarFieldMLP"] = None, version: str = "qgpu",
         logs: Optional[Sequence[TextIO]] = None) -> List[Dict[str, float]]:
@@ -284,7 +290,7 @@ def run(inputs: Sequence[str], precision: int = qe.QGPU_MIXED,
                         at = e.status(k)[1]
                         if s["final"]:
                             x, v = e.state(k)
-                            qe.write_restart(s["final"], x, v, e.theta_corr(k))
+                            qe.write_restart(s["final"], x, v, e.theta_corr(k), box=_box(e, k))
                         if logs is not None:
                             _abnormal(logs[k], str(err), at, s["final"])
                     raise
@@ -305,7 +311,7 @@ def run(inputs: Sequence[str], precision: int = qe.QGPU_MIXED,
                 for k, s in enumerate(st):
                     if s["final"]:
                         x, v = e.state(k)
-                        qe.write_restart(s["final"], x, v, e.theta_corr(k))
+                        qe.write_restart(s["final"], x, v, e.theta_corr(k), box=_box(e, k))
         for t in trj:
             if t is not None:
                 t.close()
@@ -328,7 +334,7 @@ def run(inputs: Sequence[str], precision: int = qe.QGPU_MIXED,
         for k, s in enumerate(st):
             if s["final"]:
                 x, v = e.state(k)
-                qe.write_restart(s["final"], x, v, e.theta_corr(k))
+                qe.write_restart(s["final"], x, v, e.theta_corr(k), box=_box(e, k))
         return [e.energies(k) for k in range(n)]
     finally:
         e.close()

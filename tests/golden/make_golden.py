@@ -195,6 +195,8 @@ CASES = {
                              dict(BOX_INP, energy="case.en", energy_interval=1)),
     "c15_box28_q6_rst_30": (dict(radius=14.0, n_solute=60, seed=9103, n_q=6, n_states=2, box=28.0,
                                  restraints=True, solute_offset=(-3.0, 0.0, 2.0)), dict(BOX_INP)),
+    "c15_box28_q6_monitor_30": (dict(radius=14.0, n_solute=60, seed=9104, n_q=6, n_states=2, box=28.0,
+                                     monitor=True, solute_offset=(2.0, -4.0, 1.0)), dict(BOX_INP)),
 }
 
 def parse_monitor(log: str):
