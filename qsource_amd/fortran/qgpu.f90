@@ -161,6 +161,23 @@ module qgpu
       real(c_double), intent(in) :: x(*), v(*), theta_corr(*)
     end function
 
+    ! periodic box: restart with the box record, and the box of instance k
+    integer(c_int) function qgpu_write_restart_box(path, natom, x, v, box) &
+        bind(C, name='qgpu_write_restart_box')
+      import :: c_int, c_char, c_double
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: natom
+      real(c_double), intent(in) :: x(*), v(*), box(6)
+    end function
+
+    integer(c_int) function qgpu_box(h, k, box, is_box) bind(C, name='qgpu_box')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: k
+      real(c_double), intent(out) :: box(6)
+      integer(c_int), intent(out) :: is_box
+    end function
+
     integer(c_int) function qgpu_trj_create(input_path, trj_path, trj) bind(C, name='qgpu_trj_create')
       import :: c_int, c_char, c_ptr
       character(kind=c_char), intent(in) :: input_path(*)
