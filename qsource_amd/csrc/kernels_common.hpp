@@ -393,9 +393,12 @@ struct LrfRaw {
 
 // Periodic box: the shift of the partner group whose switching atom is sw
 // against centre c (lrf_update, src/md.f90:5904-5917), zero for a sphere.
+// BOX (here and in the gathers below): the system is a periodic box, a
+// compile-time flag so that the sphere's list builds carry no box arithmetic.
+template <bool BOX>
 __device__ __forceinline__ void lrf_box_shift(const DevSys& S, int sw, double c0, double c1, double c2,
                                               double* sh) {
-  if (S.pbc) {
+  if constexpr (BOX) {
     sh[0] = pbc_shift(S, S.x[3 * sw] - c0, 0);
     sh[1] = pbc_shift(S, S.x[3 * sw + 1] - c1, 1);
     sh[2] = pbc_shift(S, S.x[3 * sw + 2] - c2, 2);
@@ -406,10 +409,11 @@ __device__ __forceinline__ void lrf_box_shift(const DevSys& S, int sw, double c0
 
 // The FP32 raw moments of the non-Q atoms of charge group g (as seen from
 // centre c), added to part.
+template <bool BOX>
 __device__ __forceinline__ void lrf_raw_group(const DevSys& S, int g, double c0, double c1, double c2,
                                               LrfRaw<float>& part) {
   double sh[3];
-  lrf_box_shift(S, S.cgp_sw[g], c0, c1, c2, sh);
+  lrf_box_shift<BOX>(S, S.cgp_sw[g], c0, c1, c2, sh);
   for (int a = S.cgp_first[g]; a <= S.cgp_last[g]; ++a) {
     const int i = S.cgpatom[a];
     if (S.flags[i] & F_Q) continue;
@@ -419,16 +423,16 @@ __device__ __forceinline__ void lrf_raw_group(const DevSys& S, int g, double c0,
 }
 
 // Add the non-Q atoms of charge group g (as seen from centre c) to acc.
-template <typename P>
+template <typename P, bool BOX>
 __device__ __forceinline__ void lrf_gather_group(const DevSys& S, int g, double c0, double c1,
                                                  double c2, LrfAcc& acc) {
   if constexpr (sizeof(P) == 8) {
     double sh[3];
-    lrf_box_shift(S, S.cgp_sw[g], c0, c1, c2, sh);
+    lrf_box_shift<BOX>(S, S.cgp_sw[g], c0, c1, c2, sh);
     for (int a = S.cgp_first[g]; a <= S.cgp_last[g]; ++a) {
       const int i = S.cgpatom[a];
       if (S.flags[i] & F_Q) continue;
-      if (S.pbc)
+      if constexpr (BOX)
         acc.add(S.crg[i], S.x[3 * i] - c0 - sh[0], S.x[3 * i + 1] - c1 - sh[1], S.x[3 * i + 2] - c2 - sh[2]);
       else
         acc.add(S.crg[i], S.x[3 * i] - c0, S.x[3 * i + 1] - c1, S.x[3 * i + 2] - c2);
@@ -438,7 +442,7 @@ __device__ __forceinline__ void lrf_gather_group(const DevSys& S, int g, double 
     // components), accumulated in FP64
     LrfRaw<float> part;
     part.zero();
-    lrf_raw_group(S, g, c0, c1, c2, part);
+    lrf_raw_group<BOX>(S, g, c0, c1, c2, part);
     acc.add(part.terms());
   }
 }
@@ -448,16 +452,16 @@ __device__ __forceinline__ void lrf_gather_group(const DevSys& S, int g, double 
 // the same terms in the same order, without the group/atom indirection.
 // In mixed precision the partner's offsets from the centre come from the
 // FP32 coordinate copy and the FP32 centre (cf).
-template <typename P>
+template <typename P, bool BOX>
 __device__ __forceinline__ void lrf_gather_water(const DevSys& S, int jo, int jw, double c0, double c1,
                                                  double c2, const float* cf, LrfAcc& acc) {
   double sh[3];
-  lrf_box_shift(S, jo, c0, c1, c2, sh);
+  lrf_box_shift<BOX>(S, jo, c0, c1, c2, sh);
   if constexpr (sizeof(P) == 8) {
     (void)jw;
     const gconst<double> xj = gptr(S.x) + 3 * jo;
     for (int k = 0; k < 3; ++k) {
-      if (S.pbc)
+      if constexpr (BOX)
         acc.add(S.chg_solv[k], xj[3 * k] - c0 - sh[0], xj[3 * k + 1] - c1 - sh[1], xj[3 * k + 2] - c2 - sh[2]);
       else
         acc.add(S.chg_solv[k], xj[3 * k] - c0, xj[3 * k + 1] - c1, xj[3 * k + 2] - c2);
@@ -479,13 +483,14 @@ __device__ __forceinline__ void lrf_gather_water(const DevSys& S, int jo, int jw
 // cf, added to part (FP32); in a periodic box moved by the shift of the
 // water's O against the FP64 centre c.
 constexpr int kLrfPend = 8;   // partner tiles whose far waters' FP32 raw moments are summed before conversion
+template <bool BOX>
 __device__ __forceinline__ void lrf_raw_water(const DevSys& S, int jw, const double* c, const float* cf,
                                               LrfRaw<float>& part) {
   float xj[9];
   water_xf(S, jw, xj);
-  if (S.pbc) {
+  if constexpr (BOX) {
     double sh[3];
-    lrf_box_shift(S, S.nat_solute + 3 * jw, c[0], c[1], c[2], sh);
+    lrf_box_shift<BOX>(S, S.nat_solute + 3 * jw, c[0], c[1], c[2], sh);
     const float sf[3] = {(float)sh[0], (float)sh[1], (float)sh[2]};
     for (int k = 0; k < 3; ++k)
       part.add((float)S.chg_solv[k], xj[3 * k] - cf[0] - sf[0], xj[3 * k + 1] - cf[1] - sf[1],

@@ -60,13 +60,11 @@ __device__ ulonglong2 pair_masks(const DevSys& S, int g, int jg) {
 // ---------------------------------------------------------------------------
 // Solute lists (nbpplist_lrf src/md.f90:7998-8188 and the solute side of
 // nbpwlist_lrf :9300-9429): one wavefront per solute charge group; P as in
-// k_water_lists (precision of the LRF multipole terms).
-template <typename P, int MAPPED>
-__global__ __launch_bounds__(256) void k_solute_lists(const DevSys* __restrict__ sys, const int2* __restrict__ map) {
-  const DevSys* Sp;
-  int base;
-  if (!block_work<MAPPED>(sys, map, blockDim.x >> 6, Sp, base)) return;
-  const DevSys& S = *Sp;
+// k_water_lists (precision of the LRF multipole terms); BOX: periodic box
+// (nbpplist_box_lrf and nbpwlist_box_lrf: minimum-image distances, shifted
+// LRF partners).
+template <typename P, bool BOX>
+__device__ __forceinline__ void solute_lists_body(const DevSys& S, const int base) {
   const int lane = threadIdx.x & 63;
   const int g = base + (threadIdx.x >> 6);
   if (g >= S.ncgp_solute) return;
@@ -90,7 +88,7 @@ __global__ __launch_bounds__(256) void k_solute_lists(const DevSys* __restrict__
       int js = S.cgp_sw[jg];
       if (!(S.flags[js] & F_EXCL)) {
         double a = xs0 - S.x[3 * js], b = xs1 - S.x[3 * js + 1], c = xs2 - S.x[3 * js + 2];
-        if (S.pbc) { a = pbc_image(S, a, 0); b = pbc_image(S, b, 1); c = pbc_image(S, c, 2); }
+        if constexpr (BOX) { a = pbc_image(S, a, 0); b = pbc_image(S, b, 1); c = pbc_image(S, c, 2); }
         double r2 = a * a + b * b + c * c;
         inside = r2 <= S.rcpp2;
         far = !inside && r2 <= S.rclrf2;
@@ -105,7 +103,7 @@ __global__ __launch_bounds__(256) void k_solute_lists(const DevSys* __restrict__
       }
     }
     n_pp += __popcll(m);
-    if (far && S.use_lrf) lrf_gather_group<P>(S, jg, c0, c1, c2, acc);
+    if (far && S.use_lrf) lrf_gather_group<P, BOX>(S, jg, c0, c1, c2, acc);
   }
   int n_pw = 0;
   int* wrow = S.pw_nb + (size_t)g * S.pw_cap;
@@ -115,7 +113,7 @@ __global__ __launch_bounds__(256) void k_solute_lists(const DevSys* __restrict__
     int jo = S.nat_solute + 3 * jw;
     if (jw < S.nwat && !(S.flags[jo] & F_EXCL)) {
       double a = xs0 - S.x[3 * jo], b = xs1 - S.x[3 * jo + 1], c = xs2 - S.x[3 * jo + 2];
-      if (S.pbc) { a = pbc_image(S, a, 0); b = pbc_image(S, b, 1); c = pbc_image(S, c, 2); }
+      if constexpr (BOX) { a = pbc_image(S, a, 0); b = pbc_image(S, b, 1); c = pbc_image(S, c, 2); }
       double r2 = a * a + b * b + c * c;
       inside = r2 <= S.rcpw2;
       far = !inside && r2 <= S.rclrf2;
@@ -127,8 +125,8 @@ __global__ __launch_bounds__(256) void k_solute_lists(const DevSys* __restrict__
     }
     n_pw += __popcll(m);
     if (far && S.use_lrf) {
-      if (S.std_water) lrf_gather_water<P>(S, jo, jw, c0, c1, c2, cf, acc);
-      else lrf_gather_group<P>(S, S.iwhich_cgp[jo], c0, c1, c2, acc);
+      if (S.std_water) lrf_gather_water<P, BOX>(S, jo, jw, c0, c1, c2, cf, acc);
+      else lrf_gather_group<P, BOX>(S, S.iwhich_cgp[jo], c0, c1, c2, acc);
     }
   }
   acc.wave_reduce();
@@ -141,6 +139,14 @@ __global__ __launch_bounds__(256) void k_solute_lists(const DevSys* __restrict__
       record_failure(S, ST_LIST_OVERFLOW, PH_LISTS);
     }
   }
+}
+template <typename P, int MAPPED>
+__global__ __launch_bounds__(256) void k_solute_lists(const DevSys* __restrict__ sys, const int2* __restrict__ map) {
+  const DevSys* Sp;
+  int base;
+  if (!block_work<MAPPED>(sys, map, blockDim.x >> 6, Sp, base)) return;
+  if (Sp->pbc) solute_lists_body<P, true>(*Sp, base);
+  else solute_lists_body<P, false>(*Sp, base);
 }
 template __global__ void k_solute_lists<double, 0>(const DevSys*, const int2*);
 template __global__ void k_solute_lists<float, 0>(const DevSys*, const int2*);
@@ -453,10 +459,8 @@ template __global__ void k_solute_forces<float, 1, 1>(const DevSys*, const int2*
 // energies to q_part, and k_q_reduce sums the chunks in a fixed order.
 // NS bounds the number of FEP states at compile time so that the per-state
 // accumulators stay in registers.
-template <typename P, int NS>
-__global__ __launch_bounds__(256) void k_q_forces(const DevSys* __restrict__ sys, int shadow) {
-  const DevSys& S = sys[blockIdx.y];
-  if (shadow && !shadow_record_step(S)) return;
+template <typename P, int NS, bool BOX>
+__device__ __forceinline__ void q_forces_body(const DevSys& S, const int shadow) {
   const int lane = threadIdx.x & 63;
   const int wv = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int iq = wv / kQChunks, chunk = wv % kQChunks;
@@ -482,7 +486,7 @@ __global__ __launch_bounds__(256) void k_q_forces(const DevSys* __restrict__ sys
     const int l = lj - 1;
     const int tj = S.iac[j];
     P dx1, dx2, dx3;
-    if (S.pbc) {
+    if constexpr (BOX) {
       // periodic box (nonbond_qp_box): the shift of the partner's group
       const int sa = S.qp_shatom[S.iwhich_cgp[j]], qw = S.qswitch;
       const double s0 = pbc_shift(S, S.x[3 * sa] - S.x[3 * qw], 0);
@@ -539,14 +543,14 @@ __global__ __launch_bounds__(256) void k_q_forces(const DevSys* __restrict__ sys
     // periodic box (nonbond_qw_spc_box): the water O's shift against the Q
     // switching atom, for all three water atoms
     double ws0 = 0.0, ws1 = 0.0, ws2 = 0.0;
-    if (S.pbc) {
+    if constexpr (BOX) {
       const int qw = S.qswitch;
       ws0 = pbc_shift(S, S.x[3 * jo] - S.x[3 * qw], 0);
       ws1 = pbc_shift(S, S.x[3 * jo + 1] - S.x[3 * qw + 1], 1);
       ws2 = pbc_shift(S, S.x[3 * jo + 2] - S.x[3 * qw + 2], 2);
     }
     P dx1, dx2, dx3;
-    if (S.pbc) {
+    if constexpr (BOX) {
       dx1 = (P)(S.x[3 * jo] - xq0 - ws0); dx2 = (P)(S.x[3 * jo + 1] - xq1 - ws1); dx3 = (P)(S.x[3 * jo + 2] - xq2 - ws2);
     } else {
       dx1 = (P)(S.x[3 * jo] - xq0); dx2 = (P)(S.x[3 * jo + 1] - xq1); dx3 = (P)(S.x[3 * jo + 2] - xq2);
@@ -584,7 +588,7 @@ __global__ __launch_bounds__(256) void k_q_forces(const DevSys* __restrict__ sys
     for (int ip = 1; ip < 3; ++ip) {
       const int ja = jo + ip;
       P ex1, ex2, ex3;
-      if (S.pbc) {
+      if constexpr (BOX) {
         ex1 = (P)(S.x[3 * ja] - xq0 - ws0); ex2 = (P)(S.x[3 * ja + 1] - xq1 - ws1); ex3 = (P)(S.x[3 * ja + 2] - xq2 - ws2);
       } else {
         ex1 = (P)(S.x[3 * ja] - xq0); ex2 = (P)(S.x[3 * ja + 1] - xq1); ex3 = (P)(S.x[3 * ja + 2] - xq2);
@@ -647,6 +651,13 @@ __global__ __launch_bounds__(256) void k_q_forces(const DevSys* __restrict__ sys
       if (lane == 0) out[3 + st * 4 + c] = v;
     }
   }
+}
+template <typename P, int NS>
+__global__ __launch_bounds__(256) void k_q_forces(const DevSys* __restrict__ sys, int shadow) {
+  const DevSys& S = sys[blockIdx.y];
+  if (shadow && !shadow_record_step(S)) return;
+  if (S.pbc) q_forces_body<P, NS, true>(S, shadow);
+  else q_forces_body<P, NS, false>(S, shadow);
 }
 template __global__ void k_q_forces<double, 2>(const DevSys*, int);
 template __global__ void k_q_forces<float, 2>(const DevSys*, int);

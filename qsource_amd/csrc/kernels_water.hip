@@ -124,14 +124,11 @@ __global__ __launch_bounds__(256) void k_put_back(const DevSys* __restrict__ sys
 // charge-group centre (20 FP64 accumulators per lane, summed over the 8
 // lanes at the end).  P = float evaluates each partner group's multipole
 // terms in FP32 before adding them to the FP64 sums (mixed precision); list
-// membership is decided in FP64 in both.
-template <typename P, int MAPPED>
-__global__ __launch_bounds__(256) void k_water_lists(const DevSys* __restrict__ sys, const int2* __restrict__ map) {
+// membership is decided in FP64 in both.  BOX: periodic box (nbwwlist_box,
+// nbpwlist_box_lrf: minimum-image distances, shifted LRF partners).
+template <typename P, bool BOX>
+__device__ __forceinline__ void water_lists_body(const DevSys& S, const int base) {
   constexpr int kWLLanes = sizeof(P) == 4 ? kWLLanes32 : kWLLanes64;
-  const DevSys* Sp;
-  int base;
-  if (!block_work<MAPPED>(sys, map, blockDim.x / kWLLanes, Sp, base)) return;
-  const DevSys& S = *Sp;
   const int t = threadIdx.x;
   if (base + (t >> 6) * (64 / kWLLanes) >= S.nwat) return;   // whole wavefronts past the last water
   const int lane = t & 63, sub = t & (kWLLanes - 1), seg = lane & ~(kWLLanes - 1);
@@ -169,7 +166,7 @@ __global__ __launch_bounds__(256) void k_water_lists(const DevSys* __restrict__ 
       const int jo = S.nat_solute + 3 * jw;
       if (!(gfl[jo] & F_EXCL)) {
         double a = xi0 - gx[3 * jo], b = xi1 - gx[3 * jo + 1], c = xi2 - gx[3 * jo + 2];
-        if (S.pbc) { a = pbc_image(S, a, 0); b = pbc_image(S, b, 1); c = pbc_image(S, c, 2); }
+        if constexpr (BOX) { a = pbc_image(S, a, 0); b = pbc_image(S, b, 1); c = pbc_image(S, c, 2); }
         const double r2 = a * a + b * b + c * c;
         inside = r2 <= S.rcww2;
         far = !inside && r2 <= S.rclrf2;
@@ -185,16 +182,16 @@ __global__ __launch_bounds__(256) void k_water_lists(const DevSys* __restrict__ 
       const int jo = S.nat_solute + 3 * jw;
       if constexpr (sizeof(P) == 4) {
         if (S.std_water) {
-          lrf_raw_water(S, jw, cc, cf, pend);
+          lrf_raw_water<BOX>(S, jw, cc, cf, pend);
         } else {
           LrfRaw<float> part;
           part.zero();
-          lrf_raw_group(S, S.iwhich_cgp[jo], c0, c1, c2, part);
+          lrf_raw_group<BOX>(S, S.iwhich_cgp[jo], c0, c1, c2, part);
           sum32.add(part.terms());
         }
       } else {
-        if (S.std_water) lrf_gather_water<P>(S, jo, jw, c0, c1, c2, cf, acc);
-        else lrf_gather_group<P>(S, S.iwhich_cgp[jo], c0, c1, c2, acc);
+        if (S.std_water) lrf_gather_water<P, BOX>(S, jo, jw, c0, c1, c2, cf, acc);
+        else lrf_gather_group<P, BOX>(S, S.iwhich_cgp[jo], c0, c1, c2, acc);
       }
     }
     if constexpr (sizeof(P) == 4) {
@@ -220,7 +217,7 @@ __global__ __launch_bounds__(256) void k_water_lists(const DevSys* __restrict__ 
       if (!(gfl[sw] & F_EXCL)) {
         // r2 as the reference forms it: solute switch atom minus water O
         double a = gx[3 * sw] - xi0, b = gx[3 * sw + 1] - xi1, c = gx[3 * sw + 2] - xi2;
-        if (S.pbc) { a = pbc_image(S, a, 0); b = pbc_image(S, b, 1); c = pbc_image(S, c, 2); }
+        if constexpr (BOX) { a = pbc_image(S, a, 0); b = pbc_image(S, b, 1); c = pbc_image(S, c, 2); }
         const double r2 = a * a + b * b + c * c;
         inside = r2 <= S.rcpw2;
         far = !inside && r2 <= S.rclrf2;
@@ -232,7 +229,7 @@ __global__ __launch_bounds__(256) void k_water_lists(const DevSys* __restrict__ 
       if (pos < S.wp_cap) wrow[pos] = g;
     }
     n_wp += __popcll(m);
-    if (far && S.use_lrf) lrf_gather_group<P>(S, g, c0, c1, c2, acc);
+    if (far && S.use_lrf) lrf_gather_group<P, BOX>(S, g, c0, c1, c2, acc);
   }
   acc.segn_reduce<kWLLanes>();
   if (iw < S.nwat && sub == 0) {
@@ -250,6 +247,15 @@ __global__ __launch_bounds__(256) void k_water_lists(const DevSys* __restrict__ 
       }
     }
   }
+}
+template <typename P, int MAPPED>
+__global__ __launch_bounds__(256) void k_water_lists(const DevSys* __restrict__ sys, const int2* __restrict__ map) {
+  constexpr int kWLLanes = sizeof(P) == 4 ? kWLLanes32 : kWLLanes64;
+  const DevSys* Sp;
+  int base;
+  if (!block_work<MAPPED>(sys, map, blockDim.x / kWLLanes, Sp, base)) return;
+  if (Sp->pbc) water_lists_body<P, true>(*Sp, base);
+  else water_lists_body<P, false>(*Sp, base);
 }
 template __global__ void k_water_lists<double, 0>(const DevSys*, const int2*);
 template __global__ void k_water_lists<float, 0>(const DevSys*, const int2*);
