@@ -178,6 +178,14 @@ class Oracle:
             return None
         return out[:3].copy(), out[3:].copy()
 
+    def volume_moves(self):
+        """Constant pressure: (tries, accepted, per try 0 = rejected,
+        1 = accepted with deltaW <= 0, 2 = accepted by the random draw)."""
+        n = self.lib.oracle_volume_moves(self.h, (ctypes.c_int * 2)(), ctypes.c_int(2))
+        out = (ctypes.c_int * (n + 2))()
+        self.lib.oracle_volume_moves(self.h, out, ctypes.c_int(n + 2))
+        return out[0], out[1], [int(f) for f in out[2:n + 2]]
+
     def offdiag(self) -> np.ndarray:
         """(noffd, 2): H_ij and r_kl of each [off_diagonals] line at the
         current coordinates."""

@@ -89,6 +89,11 @@ struct Oracle {
   std::vector<double> cgp_cent_dummy;
   int last_shake_iter = 0;
   std::string err;
+  // constant pressure: volume moves tried and accepted, the accept flag of
+  // each try, and randm's argument (the stream itself is s.rng)
+  int volume_try = 0, volume_acc = 0;
+  std::vector<int> volume_log;
+  int pressure_seed = 0;
 
   const Topology& t() const { return s.top; }
   int ncgp() const { return (int)s.top.cgp.size(); }
@@ -110,6 +115,7 @@ struct Oracle {
     list_sh.assign(s.wshell.size(), {});
     nsort.assign(s.wshell.size(), {});
     Ekinmax = 1000.0 * s.ndegf * kBoltz * s.inp.temp0 / 2.0 / (double)s.natom;
+    pressure_seed = s.inp.pressure_seed;
     temperature();
   }
 
@@ -781,11 +787,138 @@ struct Oracle {
     }
   }
 
+  // new_potential (src/md.f90:19276-19434, geometric rule): the nonbonded,
+  // LRF, restraint and Q-Q energies at the current coordinates with the
+  // lists as they are; the potential takes the bonded and Q-bonded energies
+  // of `old` (the last pot_energy), without its solvent torsions and
+  // impropers, as the reference sums it
+  void new_potential(const Energies& old) {
+    E.potential = 0; E.pp_el = 0; E.pp_vdw = 0; E.pw_el = 0; E.pw_vdw = 0; E.ww_el = 0; E.ww_vdw = 0;
+    E.qx_el = 0; E.qx_vdw = 0; E.rst_protein = 0; E.lrf = 0;
+    E.p_bond = 0; E.w_bond = 0; E.p_angle = 0; E.w_angle = 0; E.p_torsion = 0; E.w_torsion = 0;
+    E.p_improper = 0; E.w_improper = 0;
+    for (auto& q : EQ) {
+      q.qq_el = q.qq_vdw = q.qp_el = q.qp_vdw = q.qw_el = q.qw_vdw = 0;
+      q.restraint = 0;
+    }
+    std::fill(d.begin(), d.end(), 0.0);
+    pot_energy_nonbonds();
+    E.p_bond = bond(0, s.nbonds_solute);
+    E.w_bond = bond(s.nbonds_solute, (int)s.bnd.size());
+    E.p_angle = angle(0, s.nangles_solute);
+    E.w_angle = angle(s.nangles_solute, (int)s.ang.size());
+    E.p_torsion = torsion(0, s.ntors_solute);
+    E.w_torsion = torsion(s.ntors_solute, (int)s.tor.size());
+    if (s.ff_type == 2) {
+      E.p_improper = improper2(0, s.nimps_solute);
+      E.w_improper = improper2(s.nimps_solute, (int)s.imp.size());
+    } else {
+      E.p_improper = improper(0, s.nimps_solute);
+      E.w_improper = improper(s.nimps_solute, (int)s.imp.size());
+    }
+    p_restrain();
+    nonbond_qq();
+    for (int st = 0; st < s.nstates; ++st) {
+      QEnergies& q = EQ[st];
+      q.qx_el = q.qq_el + q.qp_el + q.qw_el;
+      q.qx_vdw = q.qq_vdw + q.qp_vdw + q.qw_vdw;
+      E.qx_el = E.qx_el + q.qx_el * s.lambda[st];
+      E.qx_vdw = E.qx_vdw + q.qx_vdw * s.lambda[st];
+      E.rst_protein = E.rst_protein + q.restraint * s.lambda[st];
+    }
+    E.potential = old.p_bond + old.w_bond + old.p_angle + old.w_angle + old.p_torsion + old.p_improper
+                + E.pp_el + E.pp_vdw + E.pw_el + E.pw_vdw + E.ww_el + E.ww_vdw + old.q_bond + old.q_angle
+                + old.q_torsion + old.q_improper + E.qx_el + E.qx_vdw + E.rst_protein + E.lrf;
+  }
+
+  // MC_volume (src/md.f90:18931-19272), molecule-based scaling: a random
+  // volume change of at most max_volume_displ, every molecule's centre of
+  // mass scaled about the box centre, the lists and LRF built again (with
+  // LRF on), and the move accepted by the Metropolis test on
+  // deltaE + P deltaV - nmol kT ln(V'/V).  A rejected move restores x, the
+  // box, and (with LRF) every list and the multipoles except the Q-water
+  // list, which the reference keeps from the scaled coordinates.
+  bool mc_volume() {
+    const std::vector<double> old_x = x;
+    const Energies previous_E = E;
+    double old_boxl[3], old_inv[3];
+    for (int k = 0; k < 3; ++k) { old_boxl[k] = s.boxl[k]; old_inv[k] = s.inv_boxl[k]; }
+    const bool lrf_on = s.inp.use_lrf;
+    std::vector<NB> o_pp, o_pw, o_ww;
+    std::vector<NBQ> o_qp;
+    std::vector<std::array<int, 2>> o_ppc, o_pwc;
+    std::vector<int> o_qpc;
+    std::vector<LRF> o_lrf;
+    if (lrf_on) {
+      o_pp = nbpp; o_pw = nbpw; o_ww = nbww; o_qp = nbqp;
+      o_ppc = pp_cgp; o_pwc = pw_cgp; o_qpc = qp_cgp; o_lrf = lrf;
+    }
+    new_potential(previous_E);
+    const Energies old_E = E;
+    const double old_V = old_boxl[0] * old_boxl[1] * old_boxl[2];
+    double r = s.rng.next(pressure_seed);
+    r = r * 2 - 1;
+    const double deltaV = r * s.inp.max_vol_displ;
+    const double new_V = deltaV + old_V;
+    const double cubr = std::pow(new_V / old_V, 1.0 / 3.0);
+    for (int k = 0; k < 3; ++k) s.boxl[k] = s.boxl[k] * cubr;
+    for (int k = 0; k < 3; ++k) s.inv_boxl[k] = 1.0 / s.boxl[k];
+    const double bmin = std::min(s.boxl[0], std::min(s.boxl[1], s.boxl[2]));
+    const MDInput& in = s.inp;
+    const char* small = nullptr;
+    if (!(bmin > in.rcpp * 2)) small = "Solute-solute cut-off radii too large";
+    else if (!(bmin > in.rcww * 2)) small = "Solvent-solvent cut-off radii too large";
+    else if (!(bmin > in.rcpw * 2)) small = "Solute-solvent cut-off radii too large";
+    else if (!(bmin > in.rcq * 2)) small = "Q-atom cut-off radii too large";
+    else if (in.use_lrf && !(bmin > in.rclrf * 2)) small = "LRF cut-off radii too large";
+    if (small) { err = small; return false; }
+    const auto& T = s.top;
+    for (int m = 0; m < s.nmol; ++m) {
+      double cm[3] = {0.0, 0.0, 0.0};
+      const int a0 = T.istart_mol[m], a1 = T.istart_mol[m + 1];
+      for (int j = a0; j < a1; ++j)
+        for (int k = 0; k < 3; ++k) cm[k] = cm[k] + x[3 * j + k] * s.mass[j];
+      for (int k = 0; k < 3; ++k) cm[k] = cm[k] * s.mol_minv[m];
+      double mv[3];
+      for (int k = 0; k < 3; ++k)
+        mv[k] = ((cm[k] - s.boxc[k]) * s.boxl[k] / old_boxl[k] + s.boxc[k]) - cm[k];
+      for (int j = a0; j < a1; ++j)
+        for (int k = 0; k < 3; ++k) x[3 * j + k] = x[3 * j + k] + mv[k];
+    }
+    if (lrf_on) make_pair_lists();
+    new_potential(previous_E);
+    const double deltaE = E.potential - old_E.potential;
+    const double deltaW = deltaE + in.pressure * deltaV - s.nmol * kBoltz * in.temp0 * std::log(new_V / old_V);
+    bool acc = true, drawn = false;
+    if (!(deltaW <= 0.0)) {
+      const double u = s.rng.next(pressure_seed);
+      acc = !(u > std::exp(-deltaW / kBoltz / in.temp0));
+      drawn = true;
+    }
+    volume_try += 1;
+    volume_log.push_back(acc ? (drawn ? 2 : 1) : 0);
+    if (acc) {
+      volume_acc += 1;
+    } else {
+      x = old_x;
+      E = previous_E;
+      for (int k = 0; k < 3; ++k) { s.boxl[k] = old_boxl[k]; s.inv_boxl[k] = old_inv[k]; }
+      if (lrf_on) {
+        nbpp = o_pp; nbpw = o_pw; nbww = o_ww; nbqp = o_qp;
+        pp_cgp = o_ppc; pw_cgp = o_pwc; qp_cgp = o_qpc; lrf = o_lrf;
+      }
+    }
+    return true;
+  }
+
   // one md_run iteration (src/md.f90:6471-6612) without output
   bool step() {
     if (istep % s.inp.nbcycle == 0) {
       if (s.pbc && (s.inp.put_solute_back || s.inp.put_solvent_back)) put_back_in_box();
       make_pair_lists();
+      // a volume move at list updates (src/md.f90:6538-6544)
+      if (s.pbc && s.inp.constant_pressure && istep % s.inp.ivolume_cycle == 0 && istep > 0)
+        if (!mc_volume()) return false;
     }
     pot_energy();
     newvel_ber(s.dt, Tscale_solute, 0, s.nat_solute);
@@ -2211,6 +2344,16 @@ int oracle_box(void* h, double* out) {
   const Setup& s = static_cast<Oracle*>(h)->s;
   for (int k = 0; k < 3; ++k) { out[k] = s.boxl[k]; out[3 + k] = s.boxc[k]; }
   return s.pbc ? 1 : 0;
+}
+// volume moves: tries, accepted, then per try 0 (rejected), 1 (accepted,
+// deltaW <= 0) or 2 (accepted by the random draw), as many as cap allows;
+// returns the number of tries
+int oracle_volume_moves(void* h, int* out, int cap) {
+  const Oracle* o = static_cast<Oracle*>(h);
+  if (cap > 0) out[0] = o->volume_try;
+  if (cap > 1) out[1] = o->volume_acc;
+  for (int k = 0; k < (int)o->volume_log.size() && k + 2 < cap; ++k) out[k + 2] = o->volume_log[k];
+  return o->volume_try;
 }
 int oracle_monitor(void* h, double* out, int cap) {
   Oracle* o = static_cast<Oracle*>(h);

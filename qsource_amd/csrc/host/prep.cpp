@@ -814,6 +814,7 @@ Setup prepare(const std::string& input_path) {
         s.v[3 * i + j] = (a - 6.0) * sd + 0.0;
       }
     }
+    s.rng = rnd;
     {
       std::vector<double> xx = s.x;
       if (shake_cpu(s, xx, s.x) < 0) throw std::runtime_error("initial shake failure");

@@ -121,6 +121,11 @@ struct MDInput {
   // [PBC] (src/md.f90:4516-4598)
   bool box = false, rigid_box_centre = false, put_solute_back = false, put_solvent_back = false;
   bool have_rcq = false;
+  // constant pressure (MC_volume): volume moves of at most max_vol_displ
+  // A^3 every ivolume_cycle steps; pressure in kcal/mol/A^3
+  bool constant_pressure = false;
+  double max_vol_displ = 0.0, pressure = 1.0;
+  int pressure_seed = 3781, ivolume_cycle = 10;
 };
 
 // Q-bonded terms of the FEP file (src/qatom.f90:920-1790): atoms 0-based,
@@ -191,6 +196,12 @@ struct WaterShell {
 
 // Q-Q / Q-neighbour pair for one state (NBQ_TYPE, src/md.f90:347-352).
 struct NbqPair { int iq, j, jq, ljcod; double el_scale; };
+
+// qdyn's integer LCG (randm, src/md.f90:17543-17577) with its saved state.
+struct Randm {
+  int irand = 0; bool started = false;
+  double next(int& ig);
+};
 
 // Everything md_run needs, after initialize/topology/prep_coord/get_fep/
 // prep_sim/init_shake/make_nbqqlist/shrink_topology (src/qdyn.f90:108-132).
@@ -272,6 +283,9 @@ struct Setup {
   std::vector<double> mol_minv;
   int nsolute_mol = 0;
   bool freeze = false;
+  // randm's saved state after the initialisation (the Maxwell velocities
+  // draw from it); the volume moves continue the same stream
+  Randm rng;
 };
 
 // Readers.  Each throws std::runtime_error with a message on bad input.
@@ -306,11 +320,6 @@ class DcdWriter {
   std::vector<int> atoms_;
 };
 
-// qdyn's integer LCG (randm, src/md.f90:17543-17577) with its saved state.
-struct Randm {
-  int irand = 0; bool started = false;
-  double next(int& ig);
-};
 
 // Build the Setup from an input file (paths in it are relative to its
 // directory).  Runs the whole qdyn initialisation sequence, including

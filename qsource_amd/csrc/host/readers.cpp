@@ -471,14 +471,25 @@ MDInput read_input(const std::string& path) {
   MDInput m;
   std::string v;
   if (S.has("PBC")) {
-    // [PBC] (src/md.f90:4516-4598); the NPT ensemble and a changed box size
-    // are not implemented
+    // [PBC] (src/md.f90:4516-4598); a changed box size (control_box) and
+    // atom-based volume scaling are not implemented
     m.box = true;
     if (S.get("PBC", "rigid_box_centre", v)) parse_onoff(v, m.rigid_box_centre);
-    bool cp = false;
-    if (S.get("PBC", "constant_pressure", v)) parse_onoff(v, cp);
-    if (cp) throw std::runtime_error("constant pressure (MC_volume) is not supported");
-    if (S.get("PBC", "control_box", v)) throw std::runtime_error("control_box is not supported");
+    if (S.get("PBC", "constant_pressure", v)) parse_onoff(v, m.constant_pressure);
+    if (m.constant_pressure) {
+      if (!S.get("PBC", "max_volume_displ", v))
+        throw std::runtime_error("maximum volume displacement not specified (section PBC)");
+      m.max_vol_displ = to_d(v);
+      if (S.get("PBC", "pressure_seed", v)) m.pressure_seed = (int)to_i(v);
+      if (S.get("PBC", "pressure", v)) m.pressure = to_d(v);
+      // bar to the internal unit (src/md.f90:4553)
+      m.pressure = m.pressure * 1.43836e-5;
+      bool atom_based = false;
+      if (S.get("PBC", "atom_based_scaling", v)) parse_onoff(v, atom_based);
+      if (atom_based) throw std::runtime_error("atom-based volume scaling is not supported");
+    } else if (S.get("PBC", "control_box", v)) {
+      throw std::runtime_error("control_box is not supported");
+    }
     if (S.get("PBC", "put_solvent_back_in_box", v)) parse_onoff(v, m.put_solvent_back);
     if (S.get("PBC", "put_solute_back_in_box", v)) parse_onoff(v, m.put_solute_back);
   }
@@ -534,6 +545,9 @@ MDInput read_input(const std::string& path) {
   }
   if (S.has("INTERVALS")) {
     if (S.get("INTERVALS", "non_bond", v)) m.nbcycle = (int)to_i(v);
+    // volume moves every volume_change steps, read only under constant
+    // pressure (src/md.f90:4955-4960)
+    if (m.constant_pressure && S.get("INTERVALS", "volume_change", v)) m.ivolume_cycle = (int)to_i(v);
     if (S.get("INTERVALS", "output", v)) m.iout_cycle = (int)to_i(v);
     m.itemp_cycle = m.iout_cycle;
     if (S.get("INTERVALS", "temperature", v)) m.itemp_cycle = (int)to_i(v);

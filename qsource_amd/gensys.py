@@ -1135,12 +1135,13 @@ def write_inp(path: str, top: str, final: str, *, steps: int, stepsize: float = 
               temp_interval: Optional[int] = None, restraints: Optional[dict] = None,
               trajectory: Optional[str] = None, trajectory_interval: int = 0,
               trajectory_atoms: Optional[Sequence[str]] = None,
-              pbc: Optional[dict] = None) -> None:
+              pbc: Optional[dict] = None, volume_change: Optional[int] = None) -> None:
     """qdyn input file; `restraints` maps sequence/atom/distance/angle/wall
     to lists of rows written into the [<name>_restraints] sections;
     `trajectory_atoms` rows form the [trajectory_atoms] mask; `pbc` (a dict
     of [PBC] keys, e.g. put_solvent_back_in_box) writes a periodic-box input:
-    a [PBC] section in place of [sphere] and [solvent]."""
+    a [PBC] section in place of [sphere] and [solvent]; `volume_change` is
+    the interval of constant-pressure volume moves."""
     on = lambda b: "on" if b else "off"
     L = ["[MD]", "steps %d" % steps, "stepsize %.6g" % stepsize,
          "temperature %.6g" % temperature, "bath_coupling %.6g" % bath_coupling,
@@ -1176,6 +1177,8 @@ def write_inp(path: str, top: str, final: str, *, steps: int, stepsize: float = 
     L += ["", "[intervals]", "non_bond %d" % non_bond, "output %d" % output]
     if temp_interval is not None:
         L.append("temperature %d" % temp_interval)
+    if volume_change is not None:
+        L.append("volume_change %d" % volume_change)
     if energy:
         L.append("energy %d" % energy_interval)
     if trajectory:

@@ -61,7 +61,8 @@ def read_settings(path: str) -> Dict:
     return dict(steps=int(get("md", "steps", 0)), itrj=int(get("intervals", "trajectory", 0)),
                 iene=int(get("intervals", "energy", 0)),
                 iout=int(get("intervals", "output", IOUT_DEFAULT)),
-                use_lrf=get("lrf", "lrf") is None or onoff(get("lrf", "lrf")),
+                use_lrf=get("md", "lrf") is None or onoff(get("md", "lrf")),
+                npt=onoff(get("pbc", "constant_pressure")),
                 force_rms=onoff(get("intervals", "force_rms")) or onoff(get("md", "force_rms")),
                 final=res(get("files", "final")),
                 trajectory=res(get("files", "trajectory")), energy=res(get("files", "energy")),
@@ -106,6 +107,19 @@ def _centered(msg: str, fill: str = "=") -> str:
     """centered_heading (src/misc.f90:46-54)."""
     n = (78 - len(msg)) // 2
     return fill * n + " " + msg + " " + fill * max(0, n - 1)
+
+
+def volume_summary(volume: float, tries: int, accepted: int) -> str:
+    """The volume-change summary write_out prints after the final energies
+    of a constant-pressure run (src/md.f90:18678-18690): the final volume,
+    then the moves tried and accepted and their ratio."""
+    ratio = accepted / tries if tries else float("nan")
+    r = "       NaN" if tries == 0 else _f(ratio, 10, 3)
+    L = ["=========================== VOLUME CHANGE SUMMARY ===========================",
+         "Final volume: " + _f(volume, 10, 3), "",
+         " " * 16 + _a("total", 10) + _a("accepted", 10) + _a("ratio", 10),
+         _t17("Attempts") + _i(tries, 10) + _i(accepted, 10) + r, "=" * 79]
+    return "\n".join(L)
 
 
 def _reverting(items: List[str], per_record: int) -> List[str]:
@@ -329,6 +343,10 @@ def run(inputs: Sequence[str], precision: int = qe.QGPU_MIXED,
         if logs is not None:
             for k in range(n):
                 logs[k].write("\n" + summary(k, nsteps, True))
+                if st[k]["npt"]:
+                    bl, _ = e.box(k)
+                    tries, acc, _ = e.volume_moves(k)
+                    logs[k].write("\n" + volume_summary(bl[0] * bl[1] * bl[2], tries, acc) + "\n")
         # write_xfin comes after the closing pot_energy, whose water
         # polarisation update (istep = nsteps) the restart's theta_corr holds
         for k, s in enumerate(st):

@@ -75,6 +75,56 @@ def pbc_input(inp: str, data_dir: str, dest_dir: str, final: str = "eq1.re",
     return path
 
 
+FIXTURE_NPT = os.path.join(HERE, "test5_NPT")      # eq1.inp, lig_w.fep of the constant-pressure test
+REFERENCE_NPT = "/root/reference/tests/test5"
+# the constant-pressure test shortened: 100 steps, lists and volume moves
+# every 10 steps (nine moves)
+NPT_SHORT = dict(steps=100, non_bond=10, volume_change=10)
+
+
+def npt_input(dest_dir: str, final: str = "eq1.re", **over) -> str:
+    """The reference's constant-pressure test input (tests/test5/eq1.inp:
+    the prep_PBC ligand in its 50 A water box, 0.1 fs from 0.1 K, LRF on,
+    volume moves of up to 5 A^3 at 1.01 bar from pressure seed 1337) written
+    into dest_dir with the topology (identical to prep_PBC's) and its own
+    FEP file by absolute path.  `over` replaces or adds keys, e.g.
+    steps=100, non_bond=10, volume_change=10."""
+    sect = {"steps": "MD", "non_bond": "intervals", "output": "intervals", "volume_change": "intervals",
+            "max_volume_displ": "PBC", "temperature": "MD", "stepsize": "MD"}
+    lines = open(os.path.join(FIXTURE_NPT, "eq1.inp")).read().splitlines()
+    out, cur, done = [], None, set()
+    files = {"topology": os.path.join(FIXTURE_PBC, "lig_w.top"), "fep": os.path.join(FIXTURE_NPT, "lig_w.fep"),
+             "final": os.path.join(os.path.abspath(dest_dir), final)}
+
+    def flush(section):
+        blank = 0
+        while out and not out[-1].strip():
+            out.pop()
+            blank += 1
+        for k, v in over.items():
+            if sect.get(k) == section and k not in done:
+                out.append("%-24s %s" % (k, v))
+                done.add(k)
+        out.extend([""] * blank)
+
+    for line in lines:
+        w = line.split()
+        if line.strip().startswith("["):
+            flush(cur)
+            cur = line.strip()[1:-1]
+        elif len(w) >= 2 and w[0] in files:
+            line = "%-24s %s" % (w[0], files[w[0]])
+        elif len(w) >= 2 and w[0] in over:
+            line = "%-24s %s" % (w[0], over[w[0]])
+            done.add(w[0])
+        out.append(line)
+    flush(cur)
+    path = os.path.join(dest_dir, "npt_eq1.inp")
+    with open(path, "w") as f:
+        f.write("\n".join(out) + "\n")
+    return path
+
+
 def benchmark_row(path: str, row: int = 1):
     """Row `row` of a reference benchmark energy file (the lower and upper
     bound of QEL QVdW EL VdW QBND QANG QTOR QIMP BND ANG TOR IMP TOT KIN

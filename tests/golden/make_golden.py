@@ -48,6 +48,9 @@ OUT = os.path.dirname(os.path.abspath(__file__))
 # and LRF cut-offs inside half the box edge
 BOX_INP = dict(steps=30, stepsize=0.5, temperature=50.0, initial_temperature=50.0, bath_coupling=5.0,
                non_bond=5, rcq=12.0, rclrf=13.0, lambdas=[0.6, 0.4])
+# constant pressure in the box: volume moves of up to 60 A^3 at 1 bar
+NPT = {"constant_pressure": True, "max_volume_displ": 60, "pressure_seed": 4242, "pressure": 1.0,
+       "put_solvent_back_in_box": True, "put_solute_back_in_box": False}
 
 CASES = {
     # name: (builder kwargs, input kwargs)
@@ -197,7 +200,42 @@ CASES = {
                                  restraints=True, solute_offset=(-3.0, 0.0, 2.0)), dict(BOX_INP)),
     "c15_box28_q6_monitor_30": (dict(radius=14.0, n_solute=60, seed=9104, n_q=6, n_states=2, box=28.0,
                                      monitor=True, solute_offset=(2.0, -4.0, 1.0)), dict(BOX_INP)),
+    # constant pressure (MC_volume) every 5 steps: with LRF (a rejected
+    # move), at 300 K with moves accepted by the random draw, and without
+    # LRF every 10 steps at 500 bar (lists kept through the moves)
+    "c15_box28_q6_npt_30": (dict(radius=14.0, n_solute=60, seed=9105, n_q=6, n_states=2, box=28.0,
+                                 solute_offset=(1.5, -2.0, 0.5)),
+                            dict(BOX_INP, pbc=dict(NPT, max_volume_displ=4), volume_change=5)),
+    "c15_box28_q6_npt_hot_30": (dict(radius=14.0, n_solute=60, seed=9107, n_q=6, n_states=2, box=28.0,
+                                     solute_offset=(1.5, -2.0, 0.5)),
+                                dict(BOX_INP, temperature=300.0, initial_temperature=300.0,
+                                     pbc=dict(NPT, max_volume_displ=3, pressure=500.0, pressure_seed=7),
+                                     volume_change=5)),
+    "c15_box28_q6_npt_nolrf_30": (dict(radius=14.0, n_solute=60, seed=9106, n_q=6, n_states=2, box=28.0),
+                                  dict(BOX_INP, lrf=False, pbc=dict(NPT, pressure_seed=99, pressure=500.0),
+                                       volume_change=10)),
 }
+
+def parse_volume_moves(log: str):
+    """MC_volume's verdicts in the order printed (src/md.f90:19189-19196):
+    1 for "Volume change accepted", 0 for "Volume change rejected"."""
+    out = []
+    for line in log.splitlines():
+        if line.startswith("Volume change accepted"):
+            out.append(1)
+        elif line.startswith("Volume change rejected"):
+            out.append(0)
+    return np.array(out, dtype=np.int32)
+
+
+def volume_summary_block(log: str) -> str:
+    """The volume-change summary of the final write_out (src/md.f90:
+    18678-18690), from its heading to its closing line of '='."""
+    lines = log.splitlines()
+    i = next(k for k, l in enumerate(lines) if "VOLUME CHANGE SUMMARY" in l)
+    j = next(k for k in range(i + 1, len(lines)) if lines[k] == "=" * 79)
+    return "\n".join(lines[i:j + 1])
+
 
 def parse_monitor(log: str):
     """The monitored-pair table of every energy summary (write_out's formats
@@ -359,6 +397,35 @@ def make_prep_pbc(only):
         print(name, len(blocks), "blocks", x.shape, sha[:12])
 
 
+def make_test5_npt(only):
+    """The reference's constant-pressure test (tests/test5: the prep_PBC
+    system with its own FEP file and eq1.inp), shortened to 100 steps with
+    lists and volume moves every 10 steps; copies eq1.inp and the FEP file
+    beside basic_tests.py and keeps the printed summary blocks, the verdict
+    of every volume move, SHA-256 digests of the final x and v bytes and the
+    final box."""
+    name = "test5_npt_eq1"
+    if only and name not in only:
+        return
+    os.makedirs(bt.FIXTURE_NPT, exist_ok=True)
+    shutil.copyfile(os.path.join(bt.REFERENCE_NPT, "eq1.inp"), os.path.join(bt.FIXTURE_NPT, "eq1.inp"))
+    shutil.copyfile(os.path.join(bt.REFERENCE_NPT, "prep", "lig_w.fep"), os.path.join(bt.FIXTURE_NPT, "lig_w.fep"))
+    with tempfile.TemporaryDirectory() as d:
+        inp = bt.npt_input(d, **bt.NPT_SHORT)
+        r = subprocess.run([QDYN, inp], cwd=d, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        if "terminated normally" not in r.stdout:
+            sys.exit("reference run failed for %s:\n%s" % (name, r.stdout[-3000:]))
+        blocks = summary_blocks(r.stdout)
+        x, v, _ = gensys.read_restart(os.path.join(d, "eq1.re"))
+        box = gensys.read_restart_box(os.path.join(d, "eq1.re"))
+        digest = lambda a: hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+        np.savez_compressed(os.path.join(OUT, name + ".npz"), blocks=np.array(blocks), box=box,
+                            volume_moves=parse_volume_moves(r.stdout),
+                            volume_summary=np.array(volume_summary_block(r.stdout)),
+                            x_sha=np.array(digest(x)), v_sha=np.array(digest(v)))
+        print(name, len(blocks), "blocks", parse_volume_moves(r.stdout))
+
+
 def main():
     if not os.path.exists(QDYN):
         sys.exit("reference qdyn5 not built: make -f oracle/Makefile.ref OUT=oracle/_ref")
@@ -366,6 +433,7 @@ def main():
     make_write_out_cases(only)
     make_prep_sph(only)
     make_prep_pbc(only)
+    make_test5_npt(only)
     for name, (sysargs, inpargs) in CASES.items():
         if only and name not in only:
             continue
@@ -388,6 +456,9 @@ def main():
                 extra = dict(eq_frames=rows, offd_frames=offd)
             if sysargs.get("monitor"):
                 extra["monitor"] = parse_monitor(r.stdout)
+            if inpargs.get("pbc", {}).get("constant_pressure"):
+                extra["volume_moves"] = parse_volume_moves(r.stdout)
+                extra["volume_summary"] = np.array(volume_summary_block(r.stdout))
             box = gensys.read_restart_box(paths["re"])
             if box is not None:
                 extra["box"] = box

@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 from oracle import Oracle
-from qsource_amd import gensys
+from qsource_amd import gensys, md
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -22,6 +22,9 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 # and LRF cut-offs inside half the box edge
 BOX_INP = dict(steps=30, stepsize=0.5, temperature=50.0, initial_temperature=50.0, bath_coupling=5.0,
                non_bond=5, rcq=12.0, rclrf=13.0, lambdas=[0.6, 0.4])
+# constant pressure in the box: volume moves of up to 60 A^3 at 1 bar
+NPT = {"constant_pressure": True, "max_volume_displ": 60, "pressure_seed": 4242, "pressure": 1.0,
+       "put_solvent_back_in_box": True, "put_solute_back_in_box": False}
 
 CASES = {
     "c1_r15_100": (dict(radius=15.0, n_solute=0, seed=1001), dict(steps=100)),
@@ -166,6 +169,20 @@ CASES = {
                                  restraints=True, solute_offset=(-3.0, 0.0, 2.0)), dict(BOX_INP)),
     "c15_box28_q6_monitor_30": (dict(radius=14.0, n_solute=60, seed=9104, n_q=6, n_states=2, box=28.0,
                                      monitor=True, solute_offset=(2.0, -4.0, 1.0)), dict(BOX_INP)),
+    # constant pressure (MC_volume) every 5 steps: with LRF (a rejected
+    # move), at 300 K with moves accepted by the random draw, and without
+    # LRF every 10 steps at 500 bar (lists kept through the moves)
+    "c15_box28_q6_npt_30": (dict(radius=14.0, n_solute=60, seed=9105, n_q=6, n_states=2, box=28.0,
+                                 solute_offset=(1.5, -2.0, 0.5)),
+                            dict(BOX_INP, pbc=dict(NPT, max_volume_displ=4), volume_change=5)),
+    "c15_box28_q6_npt_hot_30": (dict(radius=14.0, n_solute=60, seed=9107, n_q=6, n_states=2, box=28.0,
+                                     solute_offset=(1.5, -2.0, 0.5)),
+                                dict(BOX_INP, temperature=300.0, initial_temperature=300.0,
+                                     pbc=dict(NPT, max_volume_displ=3, pressure=500.0, pressure_seed=7),
+                                     volume_change=5)),
+    "c15_box28_q6_npt_nolrf_30": (dict(radius=14.0, n_solute=60, seed=9106, n_q=6, n_states=2, box=28.0),
+                                  dict(BOX_INP, lrf=False, pbc=dict(NPT, pressure_seed=99, pressure=500.0),
+                                       volume_change=10)),
 }
 
 ORACLE_COLS = {"ww_el": "ww_el", "ww_vdw": "ww_vdw", "lrf": "lrf", "rst_total": "rst_total",
@@ -207,6 +224,15 @@ def test_oracle_matches_reference_qdyn(name, tmp_path):
         # periodic box: the restart's box lengths and (moving) centre
         bl, bc = o.box()
         assert np.array_equal(np.concatenate([bl, bc]), g["box"])
+    if "volume_moves" in g.files:
+        # constant pressure: the verdict of every volume move as printed
+        tries, acc, log = o.volume_moves()
+        assert [1 if f else 0 for f in log] == list(g["volume_moves"]) and tries == len(log)
+        assert acc == sum(1 for f in log if f)
+        assert 0 in log or name != "c15_box28_q6_npt_30"        # a rejected move with LRF
+        assert 2 in log or name != "c15_box28_q6_npt_hot_30"    # moves accepted by the random draw
+        bl, _ = o.box()
+        assert md.volume_summary(bl[0] * bl[1] * bl[2], tries, acc) == str(g["volume_summary"])
 
 
 ENERGY_FILE_CASES = sorted(k for k, (_, inp) in CASES.items() if inp.get("energy"))

@@ -16,6 +16,11 @@ sphere and the periodic box with leap-frog and Berendsen
   final restart (x, v, water-polarisation theta_corr) bit for bit as the
   reference's serial qdyn5 wrote it, and md.write_out of the oracle's
   energies reproduces every summary block qdyn5 printed, byte for byte.
+* The reference's constant-pressure test (tests/test5: the prep_PBC system,
+  its own FEP file and eq1.inp with volume moves; copies in
+  tests/golden/test5_NPT), shortened to 100 steps with lists and volume
+  moves every 10 steps: the verdict of every move, the printed summary
+  blocks and the volume-change summary, and x, v and the box bit for bit.
 """
 import filecmp
 import hashlib
@@ -177,4 +182,48 @@ def test_pbc_eq1_run_matches_reference_qdyn(tmp_path):
     bl, bc = o.box()
     assert np.array_equal(np.concatenate([bl, bc]), g["box"])
     assert not np.array_equal(bc, [10.052, 6.221, -1.995])   # the centre moved with the solute
+    o.close()
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(bt.REFERENCE_NPT, "eq1.inp")), reason="reference test5 not present")
+def test_npt_fixture_copies_are_the_reference_files():
+    assert filecmp.cmp(os.path.join(bt.FIXTURE_NPT, "eq1.inp"), os.path.join(bt.REFERENCE_NPT, "eq1.inp"),
+                       shallow=False)
+    assert filecmp.cmp(os.path.join(bt.FIXTURE_NPT, "lig_w.fep"),
+                       os.path.join(bt.REFERENCE_NPT, "prep", "lig_w.fep"), shallow=False)
+    assert filecmp.cmp(os.path.join(bt.FIXTURE_PBC, "lig_w.top"),
+                       os.path.join(bt.REFERENCE_NPT, "prep", "lig_w.top"), shallow=False)
+
+
+def test_npt_eq1_run_matches_reference_qdyn(tmp_path):
+    """Nine volume moves (four accepted, the rest rejected and undone) in
+    100 steps of the 50 A box: each verdict, every printed block, the
+    volume-change summary and the final x, v and box as reference qdyn5
+    produced them."""
+    g = np.load(os.path.join(GOLD, "test5_npt_eq1.npz"), allow_pickle=False)
+    inp = bt.npt_input(str(tmp_path), **bt.NPT_SHORT)
+    s = md.read_settings(inp)
+    nsteps, iout = s["steps"], s["iout"]
+    assert s["npt"]
+    o = Oracle(inp)
+    z = o.sizes()
+    printed = []
+    for k in range(nsteps):
+        o.run(1)
+        if k % iout == 0:
+            printed.append(md.write_out(o.energies(), o.qenergies(), [1.0], step=k, final=False, **z))
+    o.final_energy()
+    printed.append(md.write_out(o.energies(), o.qenergies(), [1.0], step=nsteps, final=True, **z))
+    blocks = [str(b) for b in g["blocks"]]
+    assert len(printed) == len(blocks)
+    for k, (a, b) in enumerate(zip(printed, blocks)):
+        assert a == b, "block %d:\n%s\n---- reference ----\n%s" % (k, a, b)
+    tries, acc, log = o.volume_moves()
+    assert [1 if f else 0 for f in log] == list(g["volume_moves"]) and 0 in log
+    bl, bc = o.box()
+    assert md.volume_summary(bl[0] * bl[1] * bl[2], tries, acc) == str(g["volume_summary"])
+    x, v = o.state()
+    digest = lambda a: hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+    assert digest(x) == str(g["x_sha"]) and digest(v) == str(g["v_sha"])
+    assert np.array_equal(np.concatenate([bl, bc]), g["box"])
     o.close()
