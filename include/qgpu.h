@@ -359,6 +359,14 @@ int qgpu_write_restart_box(const char* path, int natom, const double* x, const d
  * updates unless rigid_box_centre is on); *is_box = 0 for a sphere. */
 int qgpu_box(qgpu_handle* h, int k, double* out, int* is_box);
 
+/* Constant-pressure volume moves of instance k so far (MC_volume,
+ * src/md.f90:18931-19272, called by md_run at :6538-6544; the counters
+ * volume_try and volume_acc that write_out prints at :18678-18686):
+ * *tries = the number of moves; out[0] = tries, out[1] = accepted, then per
+ * move 0 (rejected), 1 (accepted, deltaW <= 0) or 2 (accepted by the random
+ * draw), as many as cap allows. */
+int qgpu_volume_moves(qgpu_handle* h, int k, int* out, int cap, int* tries);
+
 /* Message of the last failure in this thread ("" when none). */
 const char* qgpu_last_error(void);
 

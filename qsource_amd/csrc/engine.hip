@@ -36,7 +36,8 @@ template <typename P, int MAPPED> __global__ void k_solute_lists(const DevSys*, 
 template <typename P, int MAPPED, int PF> __global__ void k_solute_forces(const DevSys*, const int2*, int);
 template <typename P, int NS> __global__ void k_q_forces(const DevSys*, int);
 __global__ void k_q_reduce(const DevSys*, int);
-__global__ void k_q_lists(const DevSys*);
+__global__ void k_q_lists(const DevSys*, int);
+__global__ void k_mc_scale(const DevSys*);
 __global__ void k_put_back(const DevSys*);
 __global__ void k_monitor(const DevSys*);
 __global__ void k_freeze_x(const DevSys*);
@@ -308,6 +309,21 @@ void Batch::init(const std::vector<std::string>& inputs, int device) {
   for (auto* s : setups_)
     if (s->inp.nbcycle != nbcycle_ || nbcycle_ < 1)
       throw std::runtime_error("all instances of a batch need the same non_bond interval");
+  // constant pressure: one volume-move interval and one list treatment
+  // (LRF or not) for the batch's constant-pressure instances, whose moves
+  // are tried together
+  int npt_lrf = -1;
+  for (auto* s : setups_) {
+    if (!(s->pbc && s->inp.constant_pressure)) continue;
+    if (s->inp.ivolume_cycle < 1) throw std::runtime_error("volume_change must be at least 1");
+    if (npt_ && s->inp.ivolume_cycle != ivol_)
+      throw std::runtime_error("constant-pressure instances of a batch need the same volume_change interval");
+    if (npt_lrf >= 0 && npt_lrf != (s->inp.use_lrf ? 1 : 0))
+      throw std::runtime_error("constant-pressure instances of a batch must all use LRF or all not");
+    npt_ = true;
+    ivol_ = s->inp.ivolume_cycle;
+    npt_lrf = s->inp.use_lrf ? 1 : 0;
+  }
   const char* ng = std::getenv("QGPU_NO_GRAPH");
   use_graph_ = !(ng && ng[0] == '1');
   const char* at = std::getenv("QGPU_AUTOTUNE");
@@ -331,6 +347,14 @@ void Batch::init(const std::vector<std::string>& inputs, int device) {
   for (size_t k = 0; k < h_sys_.size(); ++k) {
     h_sys_[k].ke_part = inst_[k]->alloc<double>((size_t)kblocks * 6);
     h_sys_[k].ke_blocks = kblocks;
+  }
+  vol_.resize(ni);
+  for (size_t k = 0; k < ni; ++k) {
+    const qsys::Setup& s = *setups_[k];
+    if (!(s.pbc && s.inp.constant_pressure)) continue;
+    vol_[k].on = true;
+    vol_[k].seed = s.inp.pressure_seed;
+    vol_[k].x_saved = inst_[k]->alloc<double>(3 * (size_t)s.natom);
   }
   HIPCHECK(hipMalloc(&d_sys_, h_sys_.size() * sizeof(DevSys)));
   sync_sys();
@@ -762,6 +786,8 @@ void Batch::upload(int k) {
       D.boxl[c] = s.boxl[c]; D.inv_boxl[c] = s.inv_boxl[c];
       D.boxl_f[c] = (float)s.boxl[c]; D.inv_boxl_f[c] = (float)s.inv_boxl[c];
     }
+    D.mc_move = 0;
+    for (int c = 0; c < 3; ++c) D.boxl_old[c] = s.boxl[c];
     D.qswitch = s.pbc ? std::max(0, s.qswitch) : 0;
     D.qp_shatom = I->alloc<int>(std::max(1, G.nsolute));
     D.boxc = I->upload(std::vector<double>{s.boxc[0], s.boxc[1], s.boxc[2], -1.0});
@@ -1296,7 +1322,7 @@ void Batch::launch_put_back() {
   hipLaunchKernelGGL(k_x_to_f32, dim3((3 * max_natom_ + 255) / 256, ni), dim3(256), 0, stream_, d_sys_);
 }
 
-void Batch::launch_lists() {
+void Batch::launch_lists(bool q_waters) {
   const int ni = (int)h_sys_.size();
   dim3 g0((max_ncgp_ + 255) / 256, ni);
   { pb(0); hipLaunchKernelGGL(k_cgp_centers, g0, dim3(256), 0, stream_, d_sys_); pe(0); }
@@ -1304,7 +1330,7 @@ void Batch::launch_lists() {
   // stream; solute lists (pp, solute side of pw, solute multipoles) beside
   fork();
   if (max_ncgp_solute_ > 0) { pb(2); launch_tuned(TK_SOLUTE_LISTS, shape_[TK_SOLUTE_LISTS]); pe(2); }
-  if (q_dynamic_) hipLaunchKernelGGL(k_q_lists, dim3(1, ni), dim3(1024), 0, ls_, d_sys_);
+  if (q_dynamic_) hipLaunchKernelGGL(k_q_lists, dim3(1, ni), dim3(1024), 0, ls_, d_sys_, q_waters ? 1 : 0);
   const hipStream_t side = ls_;
   ls_ = stream_;
   if (max_nwat_ > 0) { pb(1); launch_tuned(TK_WATER_LISTS, shape_[TK_WATER_LISTS]); pe(1); }
@@ -1541,10 +1567,165 @@ void Batch::one_step(bool energies) {
   if (istep_ % nbcycle_ == 0) {
     launch_put_back();
     launch_lists();
+    // a volume move at this list update (md_run, src/md.f90:6538-6544)
+    if (npt_ && istep_ > 0 && istep_ % ivol_ == 0) mc_volume();
   }
-  launch_forces(true, istep_, energies);
+  launch_forces(true, istep_, energies || npt_energy_step(istep_));
   launch_integrate(istep_);
   ++istep_;
+}
+
+// Whether step s is the step before a volume move: its pot_energy's bonded
+// and Q-bonded energies enter the move's potentials, so it forms energies.
+bool Batch::npt_energy_step(long s) const {
+  const long n = s + 1;
+  return npt_ && n % nbcycle_ == 0 && n % ivol_ == 0;
+}
+
+// new_potential (src/md.f90:19276-19434) of instance k after a force
+// evaluation with energies: its nonbonded, LRF, Q and restraint energies,
+// with the bonded and Q-bonded energies of old_e (qgpu_energies order; the
+// last pot_energy), summed as the reference sums them (no solvent torsions
+// or impropers).
+double Batch::new_potential(int k, const double* old_e) {
+  double e[QGPU_E_COUNT];
+  energies(k, e, QGPU_E_COUNT);
+  return old_e[QGPU_E_P_BOND] + old_e[QGPU_E_W_BOND] + old_e[QGPU_E_P_ANGLE] + old_e[QGPU_E_W_ANGLE] +
+         old_e[QGPU_E_P_TORSION] + old_e[QGPU_E_P_IMPROPER] + e[QGPU_E_PP_EL] + e[QGPU_E_PP_VDW] +
+         e[QGPU_E_PW_EL] + e[QGPU_E_PW_VDW] + e[QGPU_E_WW_EL] + e[QGPU_E_WW_VDW] + old_e[QGPU_E_Q_BOND] +
+         old_e[QGPU_E_Q_ANGLE] + old_e[QGPU_E_Q_TORSION] + old_e[QGPU_E_Q_IMPROPER] + e[QGPU_E_QX_EL] +
+         e[QGPU_E_QX_VDW] + e[QGPU_E_RST_PROTEIN] + e[QGPU_E_LRF];
+}
+
+// MC_volume (src/md.f90:18931-19272) of the constant-pressure instances,
+// at the list update of a volume-move step (after put_back_in_box and the
+// lists).  The host reads the potentials back, draws the move from the
+// instance's randm stream and judges it; the device scales the molecules
+// (k_mc_scale) and builds the lists again.  Per instance: the potential
+// with the lists of this update; a volume change of at most
+// max_volume_displ, the box edges scaled by the cube root of the volume
+// ratio; every molecule's centre of mass scaled about the box centre; with
+// LRF, every list and the multipoles built again; the potential again; the
+// move accepted when deltaW = deltaE + P deltaV - nmol kT ln(V'/V) <= 0,
+// else when a second draw does not exceed exp(-deltaW / kT).  A rejected
+// move restores x and the box and (with LRF) builds every list again from
+// the restored coordinates except the Q-water list, which the reference
+// keeps from the scaled ones.  Instances without constant pressure keep
+// their state: their lists are built again from unchanged coordinates.
+void Batch::mc_volume() {
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(stream_));
+  if (check_failures()) make_lists();   // list rows regrown: this update's lists built again
+  const int ni = size();
+  std::vector<std::array<double, QGPU_E_COUNT>> prev(ni);
+  std::vector<double> old_pot(ni, 0.0), old_v(ni, 0.0), dv(ni, 0.0), new_v(ni, 0.0);
+  bool lrf_on = false;
+  for (int k = 0; k < ni; ++k) {
+    if (!vol_[k].on) continue;
+    energies(k, prev[k].data(), QGPU_E_COUNT);
+    const DevSys& D = h_sys_[k];
+    HIPCHECK(hipMemcpyAsync(vol_[k].x_saved, D.x, 3 * (size_t)D.natom * sizeof(double), hipMemcpyDeviceToDevice,
+                            stream_));
+    lrf_on = setups_[k]->inp.use_lrf;
+  }
+  forces();
+  for (int k = 0; k < ni; ++k)
+    if (vol_[k].on) old_pot[k] = new_potential(k, prev[k].data());
+  // the moves: new box lengths, checked against the cut-offs
+  for (int k = 0; k < ni; ++k) {
+    if (!vol_[k].on) continue;
+    DevSys& D = h_sys_[k];
+    qsys::Setup& s = *setups_[k];
+    old_v[k] = D.boxl[0] * D.boxl[1] * D.boxl[2];
+    double r = s.rng.next(vol_[k].seed);
+    r = r * 2 - 1;
+    dv[k] = r * s.inp.max_vol_displ;
+    new_v[k] = dv[k] + old_v[k];
+    const double cubr = std::pow(new_v[k] / old_v[k], 1.0 / 3.0);
+    for (int c = 0; c < 3; ++c) {
+      D.boxl_old[c] = D.boxl[c];
+      D.boxl[c] = D.boxl[c] * cubr;
+      D.inv_boxl[c] = 1.0 / D.boxl[c];
+      D.boxl_f[c] = (float)D.boxl[c];
+      D.inv_boxl_f[c] = (float)D.inv_boxl[c];
+    }
+    const double bmin = std::min(D.boxl[0], std::min(D.boxl[1], D.boxl[2]));
+    const qsys::MDInput& in = s.inp;
+    const char* small = nullptr;
+    if (!(bmin > in.rcpp * 2)) small = "Solute-solute cut-off radii too large";
+    else if (!(bmin > in.rcww * 2)) small = "Solvent-solvent cut-off radii too large";
+    else if (!(bmin > in.rcpw * 2)) small = "Solute-solvent cut-off radii too large";
+    else if (!(bmin > in.rcq * 2)) small = "Q-atom cut-off radii too large";
+    else if (in.use_lrf && !(bmin > in.rclrf * 2)) small = "LRF cut-off radii too large";
+    if (small) {
+      dead_ = "run stopped: instance " + std::to_string(k) + ": " + small + " at the volume move of step " +
+              std::to_string(istep_);
+      throw std::runtime_error(dead_);
+    }
+    D.mc_move = 1;
+  }
+  sync_sys();
+  hipLaunchKernelGGL(k_mc_scale, dim3(1, ni), dim3(256), 0, stream_, d_sys_);
+  HIPCHECK(hipGetLastError());
+  refresh_xf();
+  for (auto& D : h_sys_) D.mc_move = 0;
+  sync_sys();
+  if (lrf_on) make_lists();
+  forces();
+  // Metropolis test
+  bool any_rejected = false;
+  for (int k = 0; k < ni; ++k) {
+    if (!vol_[k].on) continue;
+    qsys::Setup& s = *setups_[k];
+    const qsys::MDInput& in = s.inp;
+    const double delta_e = new_potential(k, prev[k].data()) - old_pot[k];
+    const double delta_w =
+        delta_e + in.pressure * dv[k] - s.nmol * qsys::kBoltz * in.temp0 * std::log(new_v[k] / old_v[k]);
+    bool acc = true, drawn = false;
+    if (!(delta_w <= 0.0)) {
+      const double u = s.rng.next(vol_[k].seed);
+      acc = !(u > std::exp(-delta_w / qsys::kBoltz / in.temp0));
+      drawn = true;
+    }
+    vol_[k].tries += 1;
+    vol_[k].log.push_back(acc ? (drawn ? 2 : 1) : 0);
+    if (acc) {
+      vol_[k].accepted += 1;
+      continue;
+    }
+    any_rejected = true;
+    DevSys& D = h_sys_[k];
+    HIPCHECK(hipMemcpyAsync(D.x, vol_[k].x_saved, 3 * (size_t)D.natom * sizeof(double), hipMemcpyDeviceToDevice,
+                            stream_));
+    for (int c = 0; c < 3; ++c) {
+      D.boxl[c] = D.boxl_old[c];
+      D.inv_boxl[c] = 1.0 / D.boxl[c];
+      D.boxl_f[c] = (float)D.boxl[c];
+      D.inv_boxl_f[c] = (float)D.inv_boxl[c];
+    }
+  }
+  if (!any_rejected) return;
+  HIPCHECK(hipStreamSynchronize(stream_));
+  sync_sys();
+  refresh_xf();
+  if (lrf_on) {
+    launch_lists(false);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(stream_));
+    if (check_failures()) {
+      launch_lists(false);
+      HIPCHECK(hipStreamSynchronize(stream_));
+      check_failures();
+    }
+  }
+}
+
+int Batch::volume_moves(int k, int* out, int cap) const {
+  const VolumeMoves& m = vol_[k];
+  if (cap > 0) out[0] = m.tries;
+  if (cap > 1) out[1] = m.accepted;
+  for (int i = 0; i < (int)m.log.size() && i + 2 < cap; ++i) out[i + 2] = m.log[i];
+  return m.tries;
 }
 
 // One list-update cycle (NBcycle steps) captured as a hipGraph: the kernels
@@ -1620,12 +1801,14 @@ void Batch::profile(int nsteps, double* ms, int cap) {
 void Batch::launch_steps(int nsteps, bool final_energies) {
   // energy partials only in the last step, when final_energies
   auto last = [&](int n) { return final_energies && n == nsteps - 1; };
+  // volume moves need the host at their list updates: no captured cycles
+  const bool graph = use_graph_ && !npt_;
   int n = 0;
-  while (n < nsteps && (!use_graph_ || istep_ % nbcycle_ != 0 || nsteps - n < nbcycle_)) {
+  while (n < nsteps && (!graph || istep_ % nbcycle_ != 0 || nsteps - n < nbcycle_)) {
     one_step(last(n));
     ++n;
   }
-  while (use_graph_ && nsteps - n >= nbcycle_) {
+  while (graph && nsteps - n >= nbcycle_) {
     launch_cycle_graph(final_energies && nsteps - n == nbcycle_);
     n += nbcycle_;
   }
@@ -2480,6 +2663,14 @@ int qgpu_box(qgpu_handle* h, int k, double* out, int* is_box) {
     CHECK_K(h, k);
     if (!out || !is_box) throw std::runtime_error("null output");
     *is_box = h->b->box(k, out) ? 1 : 0;
+  });
+}
+
+int qgpu_volume_moves(qgpu_handle* h, int k, int* out, int cap, int* tries) {
+  return guarded([&] {
+    CHECK_K(h, k);
+    if (!tries || (cap > 0 && !out)) throw std::runtime_error("null output");
+    *tries = h->b->volume_moves(k, out, cap);
   });
 }
 

@@ -214,6 +214,10 @@ struct DevSys {
   int nmol, nsolute_mol;
   const int* mol_start;          // nmol + 1 (0-based first atom of each molecule, then natom)
   const double* mol_minv;        // 1 / mass of each molecule
+  // constant pressure: a volume move scales this instance's molecules
+  // (mc_move != 0) from the box lengths boxl_old to boxl (k_mc_scale)
+  int mc_move;
+  double boxl_old[3];
   int* qp_atoms;                 // solute atoms in the Q-solute list, ascending
   int* qw_waters;                // waters in the Q-water list, ascending
   // Q-atom LJ types (qvdw_flag): qiac [iq*nstates+s]; qavdw/qbvdw [type*3 +
@@ -364,6 +368,10 @@ class Batch {
   // e7 = pp, pw, ww (el, vdW) and LRF, eq4 = per state qp and qw (el, vdW)
   void pot_energy_nonbonds(int k, const double* x, double* d, double* e7, double* eq4);
   int istep() const { return istep_; }
+  // constant pressure: volume moves of instance k so far (tries, accepted)
+  // and per try 0 (rejected), 1 (accepted, deltaW <= 0) or 2 (accepted by
+  // the random draw); returns the number of tries
+  int volume_moves(int k, int* out, int cap) const;
   // failure record of instance k: ST_* bits and the step they were found at
   void status(int k, int* bits, int* at_step);
   int list_regrowths() const { return list_regrowths_; }
@@ -454,7 +462,9 @@ class Batch {
   unsigned long long* d_fail_ = nullptr;   // batch-wide failure key
   std::string dead_;                   // set when a failure stopped the batch for good
   void upload(int k);
-  void launch_lists();
+  // every list and the LRF multipoles; q_waters = false leaves the
+  // Q-water list as it is
+  void launch_lists(bool q_waters = true);
   void launch_forces(bool step, long s, bool energies = true);
  public:
   int monitor(int k, double* out, int cap);   // monitored pair energies of instance k
@@ -469,6 +479,22 @@ class Batch {
   bool autotune_on_ = true;       // tune at the first step (QGPU_AUTOTUNE=0 switches it off)
   std::vector<std::pair<std::array<int, 3>, std::pair<int2*, int>>> maps_;
   void launch_integrate(long s);
+  // constant pressure (MC_volume): some periodic-box instance tries a
+  // volume move every ivol_ steps at its list update; per instance the
+  // move's saved coordinates, randm's argument and the move record
+  bool npt_ = false;
+  int ivol_ = 0;
+  struct VolumeMoves {
+    bool on = false;
+    double* x_saved = nullptr;
+    int seed = 0;
+    std::vector<int> log;
+    int tries = 0, accepted = 0;
+  };
+  std::vector<VolumeMoves> vol_;
+  bool npt_energy_step(long s) const;
+  void mc_volume();
+  double new_potential(int k, const double* old_e);
   void ensure_far();
   void launch_far_model();
   class Mlp* far_model_ = nullptr;

@@ -692,7 +692,9 @@ __device__ __forceinline__ int ordered_compact(bool take, int* wave_cnt, int& to
 // solute charge group enters when any of its atoms (in the group's order)
 // lies within Rcq of the Q switching atom (minimum image), and that atom is
 // kept as the group's shift atom (qp_shatom); a water when its O does.
-__global__ __launch_bounds__(1024) void k_q_lists(const DevSys* __restrict__ sys) {
+// with_waters = 0 rebuilds the Q-solute list only (a rejected volume move
+// keeps the Q-water list of the scaled coordinates, as MC_volume does).
+__global__ __launch_bounds__(1024) void k_q_lists(const DevSys* __restrict__ sys, int with_waters) {
   const DevSys& S = sys[blockIdx.y];
   if (!S.q_dynamic) return;
   __shared__ int wave_cnt[16];
@@ -734,6 +736,10 @@ __global__ __launch_bounds__(1024) void k_q_lists(const DevSys* __restrict__ sys
     }
     const int pos = ordered_compact(take, wave_cnt, nqp);
     if (take) S.qp_atoms[pos] = i;
+  }
+  if (!with_waters) {
+    if (threadIdx.x == 0) S.q_counts[0] = nqp;
+    return;
   }
   int nqw = 0;
   for (int base = 0; base < S.nwat; base += blockDim.x) {

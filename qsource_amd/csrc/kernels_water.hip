@@ -112,6 +112,30 @@ __global__ __launch_bounds__(256) void k_put_back(const DevSys* __restrict__ sys
   if (t == 0) S.boxc[3] = st;
 }
 
+// A constant-pressure volume move (MC_volume, src/md.f90:19081-19107,
+// molecule-based scaling): every molecule of an instance with mc_move set
+// moves so that its centre of mass scales about the box centre by the
+// ratio of the new to the old box length per axis, ((cm - c) L / L_old +
+// c) - cm.  One workgroup per instance, a thread per molecule (its centre
+// summed in atom order).  The FP32 copies are refreshed afterwards.
+__global__ __launch_bounds__(256) void k_mc_scale(const DevSys* __restrict__ sys) {
+  const DevSys& S = sys[blockIdx.y];
+  if (!S.mc_move) return;
+  for (int m = threadIdx.x; m < S.nmol; m += blockDim.x) {
+    const int a0 = S.mol_start[m], a1 = S.mol_start[m + 1];
+    double cm[3] = {0.0, 0.0, 0.0};
+    for (int j = a0; j < a1; ++j)
+      for (int k = 0; k < 3; ++k) cm[k] = cm[k] + S.x[3 * j + k] * S.mass[j];
+    double mv[3];
+    for (int k = 0; k < 3; ++k) {
+      cm[k] = cm[k] * S.mol_minv[m];
+      mv[k] = ((cm[k] - S.boxc[k]) * S.boxl[k] / S.boxl_old[k] + S.boxc[k]) - cm[k];
+    }
+    for (int j = a0; j < a1; ++j)
+      for (int k = 0; k < 3; ++k) S.x[3 * j + k] = S.x[3 * j + k] + mv[k];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Water lists (nbwwlist_lrf src/md.f90:10897-11021 and the water side of
 // nbpwlist_lrf :9300-9429).  kWLLanes lanes per water (8 in the FP64

@@ -38,7 +38,7 @@ SYMBOLS = [
     "qgpu_far_samples", "qgpu_far_lrf_estimate", "qgpu_set_far_model",
     "qgpu_profile_kernels", "qgpu_list_counts", "qgpu_offdiag_states", "qgpu_monitor",
     "qgpu_trj_create", "qgpu_trj_natoms", "qgpu_trj_write", "qgpu_trj_close", "qgpu_write_restart",
-    "qgpu_write_restart_box", "qgpu_box",
+    "qgpu_write_restart_box", "qgpu_box", "qgpu_volume_moves",
     "qgpu_autotune", "qgpu_set_launch_shape",
     "qgpu_instance_status", "qgpu_list_capacity", "qgpu_set_list_capacity", "qgpu_pot_energy_terms",
     "qgpu_work_counts", "qgpu_mlp_gradient_noalloc", "qgpu_system_sizes", "qgpu_offdiag",
@@ -124,6 +124,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.qgpu_write_restart.argtypes = [ctypes.c_char_p, ctypes.c_int, dp, dp, ctypes.c_int, dp]
     lib.qgpu_write_restart_box.argtypes = [ctypes.c_char_p, ctypes.c_int, dp, dp, dp]
     lib.qgpu_box.argtypes = [P, ctypes.c_int, dp, ctypes.POINTER(ctypes.c_int)]
+    ip = ctypes.POINTER(ctypes.c_int)
+    lib.qgpu_volume_moves.argtypes = [P, ctypes.c_int, ip, ctypes.c_int, ip]
     lib.qgpu_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
@@ -283,6 +285,16 @@ class Engine:
         isb = ctypes.c_int()
         self._check(self.lib.qgpu_box(self.h, k, _dp(out), ctypes.byref(isb)))
         return (out[:3].copy(), out[3:].copy()) if isb.value else None
+
+    def volume_moves(self, k: int = 0):
+        """Constant-pressure volume moves of instance k: (tries, accepted,
+        per move 0 = rejected, 1 = accepted with deltaW <= 0, 2 = accepted
+        by the random draw)."""
+        n = ctypes.c_int()
+        self._check(self.lib.qgpu_volume_moves(self.h, k, None, 0, ctypes.byref(n)))
+        out = (ctypes.c_int * (n.value + 2))()
+        self._check(self.lib.qgpu_volume_moves(self.h, k, out, n.value + 2, ctypes.byref(n)))
+        return out[0], out[1], [int(f) for f in out[2:n.value + 2]]
 
     def theta_corr(self, k: int = 0) -> np.ndarray:
         n = ctypes.c_int()

@@ -14,7 +14,13 @@ qdyn by the CPU tests) and the reference's own basic PBC test system:
   most 1e-5 of the gradient RMS;
 * the reference basic PBC system (12 262 atoms, 49.6 A box, LRF, the
   solvent put back every 25 steps): 100 FP64 steps against the oracle;
-* the monitored group pairs of a box case against the oracle.
+* the monitored group pairs of a box case against the oracle;
+* constant pressure (MC_volume): every golden NPT case as an FP64 instance
+  with the oracle's verdict for every volume move and the same state, box
+  and energies within 1e-9; two NPT cases as one batch equal to their
+  single runs; the reference's NPT test (tests/test5, nine moves) over 100
+  FP64 steps against the oracle; md.run printing the reference's
+  volume-change summary; a mixed-precision NPT run.
 """
 import os
 import sys
@@ -35,6 +41,7 @@ from test_oracle_reference import CASES  # noqa: E402
 pytestmark = pytest.mark.gpu
 
 BOX_CASES = sorted(k for k, (sa, _) in CASES.items() if sa.get("box"))
+NPT_CASES = sorted(k for k, (_, ia) in CASES.items() if ia.get("pbc", {}).get("constant_pressure"))
 TERMS = ["pp_el", "pp_vdw", "pw_el", "pw_vdw", "ww_el", "ww_vdw", "lrf", "p_bond", "p_angle", "p_torsion",
          "qx_el", "qx_vdw", "rst_protein", "potential", "kinetic"]
 
@@ -155,3 +162,83 @@ def test_reference_pbc_system_100_steps(tmp_path):
     em.step(100)
     assert np.isfinite(em.energies(0)["potential"])
     em.close()
+
+
+def _moves(e, o):
+    mg, mo = e.volume_moves(0), o.volume_moves()
+    assert mg == mo, (mg, mo)
+    return mg
+
+
+@pytest.mark.parametrize("name", NPT_CASES)
+def test_npt_case_fp64_vs_oracle(name, tmp_path):
+    _say("case: %s, constant pressure, FP64 single instance vs oracle" % name)
+    inp, n = _write(name, str(tmp_path))
+    e = qe.Engine([inp], qe.QGPU_FP64)
+    e.step(n)
+    o = Oracle(inp)
+    o.run(n)
+    tries, acc, log = _moves(e, o)
+    assert tries > 0
+    _compare(e, 0, o)
+    o.close()
+    e.close()
+
+
+def test_npt_batch_equals_single_instances(tmp_path):
+    names = ["c15_box28_q6_npt_30", "c15_box28_q6_npt_hot_30", "c15_box28_q6_30"]
+    _say("case: FP64 batch %s vs single instances" % names)
+    inps = [_write(n, str(tmp_path))[0] for n in names]
+    eb = qe.Engine(inps, qe.QGPU_FP64)
+    eb.step(30)
+    for k, inp in enumerate(inps):
+        es = qe.Engine([inp], qe.QGPU_FP64)
+        es.step(30)
+        xb, vb = eb.state(k)
+        xs, vs = es.state(0)
+        assert np.abs(xb - xs).max() <= 1e-12 * np.abs(xs).max(), names[k]
+        assert np.abs(vb - vs).max() <= 1e-12 * np.abs(vs).max(), names[k]
+        assert eb.volume_moves(k) == es.volume_moves(0), names[k]
+        assert np.array_equal(eb.box(k)[0], es.box(0)[0]), names[k]
+        es.close()
+    eb.close()
+
+
+def test_reference_npt_system_100_steps(tmp_path):
+    _say("case: reference NPT test (tests/test5 eq1, 12262 atoms, 9 volume moves), FP64 100 steps vs oracle")
+    inp = bt.npt_input(str(tmp_path), **bt.NPT_SHORT)
+    g = np.load(os.path.join(HERE, "golden", "test5_npt_eq1.npz"), allow_pickle=False)
+    e = qe.Engine([inp], qe.QGPU_FP64)
+    e.step(100)
+    o = Oracle(inp)
+    o.run(100)
+    tries, acc, log = _moves(e, o)
+    assert [1 if f else 0 for f in log] == list(g["volume_moves"])
+    _compare(e, 0, o)
+    o.close()
+    e.close()
+
+
+def test_npt_md_run_prints_volume_summary(tmp_path):
+    import io
+
+    from qsource_amd import md
+    name = "c15_box28_q6_npt_hot_30"
+    _say("case: md.run of %s prints the reference's volume-change summary" % name)
+    inp, _ = _write(name, str(tmp_path))
+    g = np.load(os.path.join(HERE, "golden", name + ".npz"), allow_pickle=False)
+    log = io.StringIO()
+    md.run([inp], precision=qe.QGPU_FP64, logs=[log])
+    assert str(g["volume_summary"]) in log.getvalue()
+
+
+def test_npt_mixed_precision_run(tmp_path):
+    name = "c15_box28_q6_npt_30"
+    _say("case: %s, mixed precision" % name)
+    inp, n = _write(name, str(tmp_path))
+    e = qe.Engine([inp], qe.QGPU_MIXED)
+    e.step(n)
+    tries, acc, log = e.volume_moves(0)
+    assert tries == 5   # list updates at steps 5, 10, 15, 20, 25
+    assert np.isfinite(e.energies(0)["potential"])
+    e.close()
