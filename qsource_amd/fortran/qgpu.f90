@@ -178,6 +178,16 @@ module qgpu
       integer(c_int), intent(out) :: is_box
     end function
 
+    ! constant pressure: out(1) tries, out(2) accepted, then each move's
+    ! verdict (0 rejected, 1 accepted, 2 accepted by the random draw)
+    integer(c_int) function qgpu_volume_moves(h, k, out, cap, tries) bind(C, name='qgpu_volume_moves')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: k, cap
+      integer(c_int), intent(out) :: out(*)
+      integer(c_int), intent(out) :: tries
+    end function
+
     integer(c_int) function qgpu_trj_create(input_path, trj_path, trj) bind(C, name='qgpu_trj_create')
       import :: c_int, c_char, c_ptr
       character(kind=c_char), intent(in) :: input_path(*)
