@@ -94,6 +94,9 @@ struct Oracle {
   int volume_try = 0, volume_acc = 0;
   std::vector<int> volume_log;
   int pressure_seed = 0;
+  // randm's argument for the Langevin forces; the Nose-Hoover chain
+  int iseed = 0;
+  std::vector<double> xnh, vnh, gnh;
 
   const Topology& t() const { return s.top; }
   int ncgp() const { return (int)s.top.cgp.size(); }
@@ -116,6 +119,8 @@ struct Oracle {
     nsort.assign(s.wshell.size(), {});
     Ekinmax = 1000.0 * s.ndegf * kBoltz * s.inp.temp0 / 2.0 / (double)s.natom;
     pressure_seed = s.inp.pressure_seed;
+    iseed = s.inp.iseed;
+    xnh.assign(s.qnh.size(), 0.0); vnh.assign(s.qnh.size(), 0.0); gnh.assign(s.qnh.size(), 0.0);
     temperature();
   }
 
@@ -702,6 +707,8 @@ struct Oracle {
   }
 
   // ---------------------------------------------------------------- update
+  // newvel_ber (src/md.f90:6194-6223); velocity-Verlet repeats the velocity
+  // update after the position update
   void newvel_ber(double step, double tscale, int a0, int a1) {
     for (int i = a0; i < a1; ++i) {
       int i3 = 3 * i;
@@ -710,7 +717,82 @@ struct Oracle {
         xx[i3 + k] = x[i3 + k];
         x[i3 + k] = x[i3 + k] + v[i3 + k] * s.dt;
       }
+      if (s.velocity_verlet)
+        for (int k = 0; k < 3; ++k) v[i3 + k] = (v[i3 + k] - d[i3 + k] * s.winv[i] * step) * tscale;
     }
+  }
+
+  // newvel_lan (src/md.f90:6225-6271): random forces drawn atom by atom
+  // from randm's stream, uniform (rands (u - 1/2)) or Gaussian (gauss with
+  // standard deviation rands), then friction-scaled leapfrog
+  void newvel_lan(double step, double lfriction) {
+    std::vector<double> rande(3 * (size_t)s.natom);
+    for (int i = 0; i < s.natom; ++i)
+      for (int j = 0; j < 3; ++j) {
+        if (!s.langevin_gauss) {
+          const double u = s.rng.next(iseed);
+          rande[3 * i + j] = s.rands[i] * (u - 0.5);
+        } else {
+          double a = 0.0;
+          for (int k = 0; k < 12; ++k) a = a + s.rng.next(iseed);
+          rande[3 * i + j] = (a - 6.0) * s.rands[i] + 0.0;
+        }
+      }
+    for (int i = 0; i < s.natom; ++i) {
+      int i3 = 3 * i;
+      for (int k = 0; k < 3; ++k) {
+        v[i3 + k] = (v[i3 + k] * lfriction) - (d[i3 + k] - rande[i3 + k]) * s.winv[i] * step;
+        xx[i3 + k] = x[i3 + k];
+        x[i3 + k] = x[i3 + k] + v[i3 + k] * s.dt;
+      }
+      if (s.velocity_verlet)
+        for (int k = 0; k < 3; ++k)
+          v[i3 + k] = (v[i3 + k] * lfriction) - (d[i3 + k] - rande[i3 + k]) * s.winv[i] * step;
+    }
+  }
+
+  // newvel_nos (src/md.f90:6273-6301)
+  void newvel_nos(double step) {
+    for (int i = 0; i < s.natom; ++i) {
+      int i3 = 3 * i;
+      for (int k = 0; k < 3; ++k) {
+        v[i3 + k] = (v[i3 + k] - d[i3 + k] * s.winv[i] * step);
+        xx[i3 + k] = x[i3 + k];
+        x[i3 + k] = x[i3 + k] + v[i3 + k] * s.dt;
+      }
+      if (s.velocity_verlet)
+        for (int k = 0; k < 3; ++k) v[i3 + k] = (v[i3 + k] - d[i3 + k] * s.winv[i] * step);
+    }
+  }
+
+  // nh_prop (src/md.f90:6303-6344): half-step propagation of the
+  // Nose-Hoover chain from the last temperature, velocities scaled by
+  // exp(-vnh_1 dt/2)
+  void nh_prop() {
+    const int n = (int)s.qnh.size();
+    const double dt2 = 0.5 * s.dt, dt4 = 0.5 * dt2, dt8 = 0.5 * dt4, T0 = s.inp.temp0;
+    const std::vector<double>& q = s.qnh;
+    gnh[0] = s.ndegf * kBoltz * (Temp - T0) / q[0];
+    for (int i = 1; i < n; ++i) gnh[i] = (q[i - 1] * (vnh[i - 1] * vnh[i - 1]) - s.kbt) / q[i];
+    vnh[n - 1] = vnh[n - 1] + gnh[n - 1] * dt4;
+    xnh[n - 1] = xnh[n - 1] + vnh[n - 1] * dt2;
+    double expf = 0.0;
+    for (int i = n - 2; i >= 0; --i) {
+      expf = std::exp(-(dt8 * vnh[i + 1]));
+      vnh[i] = (vnh[i] * expf + gnh[i] * dt4) * expf;
+      xnh[i] = xnh[i] + vnh[i] * dt2;
+    }
+    const double sc = std::exp(-(vnh[0] * dt2));
+    for (double& w : v) w = w * sc;
+    gnh[0] = s.ndegf * kBoltz * (sc * sc * Temp - T0) / q[0];
+    vnh[0] = (vnh[0] * expf + gnh[0] * dt4) * expf;
+    for (int i = 1; i < n - 1; ++i) {
+      expf = std::exp(-(dt8 * vnh[i + 1]));
+      gnh[i] = (q[i - 1] * (vnh[i - 1] * vnh[i - 1]) - s.kbt) / q[i];
+      vnh[i] = (vnh[i] * expf + gnh[i] * dt4) * expf;
+    }
+    gnh[n - 1] = (q[n - 2] * (vnh[n - 2] * vnh[n - 2]) - s.kbt) / q[n - 1];
+    vnh[n - 1] = vnh[n - 1] + gnh[n - 1] * dt4;
   }
 
   int shake() {
@@ -921,13 +1003,22 @@ struct Oracle {
         if (!mc_volume()) return false;
     }
     pot_energy();
-    newvel_ber(s.dt, Tscale_solute, 0, s.nat_solute);
-    newvel_ber(s.dt, Tscale_solvent, s.nat_solute, s.natom);
+    // thermostat (src/md.f90:6578-6588)
+    if (s.thermostat == 0) {
+      newvel_ber(s.dv_mod, Tscale_solute, 0, s.nat_solute);
+      newvel_ber(s.dv_mod, Tscale_solvent, s.nat_solute, s.natom);
+    } else if (s.thermostat == 1) {
+      newvel_lan(s.dv_mod, s.dv_friction);
+    } else {
+      nh_prop();
+      newvel_nos(s.dv_mod);
+    }
     if (s.shake_constraints > 0) {
       int n = shake();
       if (n < 0) { err = "shake failure"; return false; }
       last_shake_iter = n / s.nmol;
       for (size_t k = 0; k < x.size(); ++k) v[k] = (x[k] - xx[k]) / s.dt;
+      if (s.thermostat == 2) nh_prop();   // velocities scaled again after SHAKE
     }
     temperature();
     ++istep;

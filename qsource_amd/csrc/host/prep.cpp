@@ -663,6 +663,27 @@ Setup prepare(const std::string& input_path) {
     } else {
       s.detail_temps = true;
     }
+    // thermostat and integrator constants (src/md.f90:4646-4708,
+    // 4416-4424, 6446-6462)
+    s.velocity_verlet = in.integrator == "velocity-verlet";
+    s.dv_mod = s.velocity_verlet ? 0.5 * s.dt : s.dt;
+    if (in.thermostat == "langevin") {
+      if (in.freeze) throw std::runtime_error("the Langevin thermostat with excluded_freeze is not supported");
+      s.thermostat = 1;
+      s.friction = in.have_friction ? in.friction : 1 / s.tau_t;
+      s.langevin_gauss = in.langevin_gauss;
+      s.dv_friction = (1.0 - s.friction * s.dv_mod);
+      const double gkt = s.langevin_gauss ? 2.0 * s.friction * kBoltz * in.temp0 / s.dv_mod
+                                          : 24.0 * s.friction * kBoltz * in.temp0 / s.dv_mod;
+      s.rands.resize(s.natom);
+      for (int i = 0; i < s.natom; ++i) s.rands[i] = std::sqrt(gkt / s.winv[i]);
+    } else if (in.thermostat == "nose-hoover") {
+      s.thermostat = 2;
+      s.kbt = kBoltz * in.temp0;
+      const double nhq = in.have_nhq ? in.nhq : s.kbt / (s.tau_t * s.tau_t);
+      s.qnh.assign(in.nhchains, nhq);
+      s.qnh[0] = s.ndegf * nhq;
+    }
     // clear angles whose i-k pair is constrained
     for (auto& m : s.shake_mol)
       for (auto& c : m.bond)

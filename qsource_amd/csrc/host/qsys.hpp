@@ -108,6 +108,12 @@ struct MDInput {
   std::string top_file, restart_file, final_file, energy_file, fep_file;
   std::vector<double> lambdas;
   std::string thermostat = "berendsen", integrator = "leap-frog";
+  // Langevin: friction (default 1/tau_T) and Gaussian instead of uniform
+  // random forces; Nose-Hoover: chain length and thermostat mass (default
+  // kT/tau_T^2)
+  bool have_friction = false, langevin_gauss = false, have_nhq = false;
+  double friction = 0.0, nhq = 0.0;
+  int nhchains = 10;
   std::string restraint_file;            // [files] restraint: xtop from a restart file
   std::string trj_file;                  // [files] trajectory
   std::vector<std::string> trajectory_atoms;  // [trajectory_atoms] mask rows
@@ -217,6 +223,18 @@ struct Setup {
   std::vector<int> ljcod;             // natyps*natyps, values 1 or 2
   int natyps = 0;
   double dt = 0, tau_t = 0;
+  // thermostat (0 Berendsen, 1 Langevin, 2 Nose-Hoover) and integrator
+  // (velocity-Verlet: two half-step velocity updates); dv_mod the velocity
+  // update's step (dt, or dt/2 with velocity-Verlet), the Langevin friction
+  // factor 1 - friction dv_mod and per-atom random force scale
+  // sqrt(gkT / winv), the Nose-Hoover chain masses and kT (src/md.f90:
+  // 4646-4708, 4416-4424, 6446-6462)
+  int thermostat = 0;
+  bool velocity_verlet = false;
+  double dv_mod = 0, friction = 0, dv_friction = 0, kbt = 0;
+  bool langevin_gauss = false;
+  std::vector<double> rands;
+  std::vector<double> qnh;
   // solvent
   double chg_solv[3] = {0,0,0}, alj_solv[3][2] = {{0}}, blj_solv[3][2] = {{0}};
   double rwat = 0, fk_wsphere = 0, dwmz = 0, awmz = 0, fkwpol = 0;

@@ -501,11 +501,25 @@ MDInput read_input(const std::string& path) {
   if (!S.get("MD", "temperature", v)) throw std::runtime_error("[MD] temperature missing");
   m.temp0 = to_d(v);
   if (S.get("MD", "bath_coupling", v)) m.tau_t = to_d(v);
+  // thermostat and integrator (src/md.f90:4646-4708); a name the reference
+  // does not recognise falls back to Berendsen only if it is "nose_hoover",
+  // as the reference's test of the name lets it
   if (S.get("MD", "thermostat", v)) m.thermostat = v;
-  if (m.thermostat != "berendsen")
-    throw std::runtime_error("only the berendsen thermostat is supported");
+  if (m.thermostat == "nose_hoover") m.thermostat = "berendsen";
+  if (m.thermostat != "berendsen" && m.thermostat != "langevin" && m.thermostat != "nose-hoover")
+    throw std::runtime_error("this thermostat does not exist in Q: " + m.thermostat);
+  if (m.thermostat == "langevin") {
+    if (S.get("MD", "langevin_friction", v)) { m.have_friction = true; m.friction = to_d(v); }
+    if (S.get("MD", "langevin_random", v)) parse_onoff(v, m.langevin_gauss);
+  }
+  if (m.thermostat == "nose-hoover") {
+    if (S.get("MD", "nhchains", v)) m.nhchains = (int)to_i(v);
+    if (S.get("MD", "nose-hoover_mass", v)) { m.have_nhq = true; m.nhq = to_d(v); }
+    if (m.nhchains < 2) throw std::runtime_error("nhchains must be at least 2");
+  }
   if (S.get("MD", "integrator", v)) m.integrator = v;
-  if (m.integrator != "leap-frog") throw std::runtime_error("only the leap-frog integrator is supported");
+  if (m.integrator != "leap-frog" && m.integrator != "velocity-verlet")
+    throw std::runtime_error("no such integrator exists in Q: " + m.integrator);
   if (S.get("MD", "separate_scaling", v)) parse_onoff(v, m.separate_scaling);
   m.iseed = 1;
   if (S.get("MD", "random_seed", v)) m.iseed = (int)to_i(v);

@@ -1135,17 +1135,27 @@ def write_inp(path: str, top: str, final: str, *, steps: int, stepsize: float = 
               temp_interval: Optional[int] = None, restraints: Optional[dict] = None,
               trajectory: Optional[str] = None, trajectory_interval: int = 0,
               trajectory_atoms: Optional[Sequence[str]] = None,
-              pbc: Optional[dict] = None, volume_change: Optional[int] = None) -> None:
+              pbc: Optional[dict] = None, volume_change: Optional[int] = None,
+              thermostat: Optional[str] = None, integrator: Optional[str] = None,
+              md_extra: Optional[dict] = None) -> None:
     """qdyn input file; `restraints` maps sequence/atom/distance/angle/wall
     to lists of rows written into the [<name>_restraints] sections;
     `trajectory_atoms` rows form the [trajectory_atoms] mask; `pbc` (a dict
     of [PBC] keys, e.g. put_solvent_back_in_box) writes a periodic-box input:
     a [PBC] section in place of [sphere] and [solvent]; `volume_change` is
-    the interval of constant-pressure volume moves."""
+    the interval of constant-pressure volume moves; `thermostat`,
+    `integrator` and `md_extra` (more [MD] keys, e.g. nhchains) choose the
+    dynamics."""
     on = lambda b: "on" if b else "off"
     L = ["[MD]", "steps %d" % steps, "stepsize %.6g" % stepsize,
          "temperature %.6g" % temperature, "bath_coupling %.6g" % bath_coupling,
          "separate_scaling %s" % on(separate_scaling)]
+    if thermostat:
+        L.append("thermostat %s" % thermostat)
+    if integrator:
+        L.append("integrator %s" % integrator)
+    for k, v in (md_extra or {}).items():
+        L.append("%s %s" % (k, on(v) if isinstance(v, bool) else v))
     if initial_temperature is not None:
         L += ["random_seed %d" % random_seed, "initial_temperature %.6g" % initial_temperature]
     L += ["shake_solvent %s" % on(shake_solvent), "shake_solute %s" % on(shake_solute),

@@ -48,6 +48,9 @@ OUT = os.path.dirname(os.path.abspath(__file__))
 # and LRF cut-offs inside half the box edge
 BOX_INP = dict(steps=30, stepsize=0.5, temperature=50.0, initial_temperature=50.0, bath_coupling=5.0,
                non_bond=5, rcq=12.0, rclrf=13.0, lambdas=[0.6, 0.4])
+# 300 K runs of 1 fs steps from Maxwell velocities
+HOT_INP = dict(steps=30, lambdas=[0.6, 0.4], stepsize=1.0, temperature=300.0, bath_coupling=10.0,
+               initial_temperature=300.0)
 # constant pressure in the box: volume moves of up to 60 A^3 at 1 bar
 NPT = {"constant_pressure": True, "max_volume_displ": 60, "pressure_seed": 4242, "pressure": 1.0,
        "put_solvent_back_in_box": True, "put_solute_back_in_box": False}
@@ -214,6 +217,28 @@ CASES = {
     "c15_box28_q6_npt_nolrf_30": (dict(radius=14.0, n_solute=60, seed=9106, n_q=6, n_states=2, box=28.0),
                                   dict(BOX_INP, lrf=False, pbc=dict(NPT, pressure_seed=99, pressure=500.0),
                                        volume_change=10)),
+    # thermostats and integrators at 300 K, 1 fs: Langevin (uniform random
+    # forces) with SHAKE; Langevin with Gaussian forces, a set friction and
+    # velocity-Verlet; Nose-Hoover with SHAKE (the chain propagated again
+    # after SHAKE); Nose-Hoover with velocity-Verlet, 4 chains and a set
+    # mass; Berendsen with velocity-Verlet and SHAKE; Langevin at constant
+    # pressure (volume moves and random forces share randm's stream)
+    "c16_r14_q8_lan_30": (dict(radius=14.0, n_solute=40, seed=3101, n_q=8, n_states=2),
+                          dict(HOT_INP, thermostat="langevin")),
+    "c16_r14_q8_lan_gauss_vv_30": (dict(radius=14.0, n_solute=40, seed=3102, n_q=8, n_states=2),
+                                   dict(HOT_INP, thermostat="langevin", integrator="velocity-verlet",
+                                        md_extra={"langevin_random": True, "langevin_friction": 0.5})),
+    "c16_r14_q8_nh_shake_30": (dict(radius=14.0, n_solute=40, seed=3103, n_q=8, n_states=2),
+                               dict(HOT_INP, thermostat="nose-hoover", shake_hydrogens=True)),
+    "c16_r14_q8_nh_vv_30": (dict(radius=14.0, n_solute=40, seed=3104, n_q=8, n_states=2),
+                            dict(HOT_INP, thermostat="nose-hoover", integrator="velocity-verlet",
+                                 md_extra={"nhchains": 4, "nose-hoover_mass": 2.5})),
+    "c16_r14_q8_ber_vv_30": (dict(radius=14.0, n_solute=40, seed=3105, n_q=8, n_states=2),
+                             dict(HOT_INP, integrator="velocity-verlet", shake_hydrogens=True)),
+    "c16_box28_q6_lan_npt_30": (dict(radius=14.0, n_solute=60, seed=3106, n_q=6, n_states=2, box=28.0),
+                                dict(HOT_INP, stepsize=0.5, non_bond=5, rcq=12.0, rclrf=13.0, thermostat="langevin",
+                                     pbc=dict(NPT, max_volume_displ=3, pressure=500.0, pressure_seed=7),
+                                     volume_change=5)),
 }
 
 def parse_volume_moves(log: str):
@@ -426,6 +451,35 @@ def make_test5_npt(only):
         print(name, len(blocks), "blocks", parse_volume_moves(r.stdout))
 
 
+def make_sph_thermostats(only):
+    """The reference's basic sphere test (eq1 of tests/basic_tests) with each
+    integrator and thermostat of its test matrix (run_test.sh:193-200)
+    besides leap-frog/Berendsen (prep_sph_eq1): every printed summary block
+    and SHA-256 digests of the final x and v bytes, per combination."""
+    name = "sph_eq1_thermostats"
+    if only and name not in only:
+        return
+    out = {}
+    for integ in ("leap-frog", "velocity-verlet"):
+        for th in ("berendsen", "langevin", "nose-hoover"):
+            if (integ, th) == ("leap-frog", "berendsen"):
+                continue
+            key = "%s_%s" % (integ.replace("-", "_"), th.replace("-", "_"))
+            with tempfile.TemporaryDirectory() as d:
+                inp = bt.sph_input(os.path.join(bt.FIXTURE, "eq1.inp"), bt.FIXTURE, d, integrator=integ,
+                                   thermostat=th)
+                r = subprocess.run([QDYN, inp], cwd=d, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+                if "terminated normally" not in r.stdout:
+                    sys.exit("reference run failed for %s %s:\n%s" % (name, key, r.stdout[-3000:]))
+                x, v, _ = gensys.read_restart(os.path.join(d, "eq1.re"))
+                digest = lambda a: hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+                out[key + "_blocks"] = np.array(summary_blocks(r.stdout))
+                out[key + "_x_sha"] = np.array(digest(x))
+                out[key + "_v_sha"] = np.array(digest(v))
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **out)
+    print(name, sorted(k[:-7] for k in out if k.endswith("_blocks")))
+
+
 def main():
     if not os.path.exists(QDYN):
         sys.exit("reference qdyn5 not built: make -f oracle/Makefile.ref OUT=oracle/_ref")
@@ -434,6 +488,7 @@ def main():
     make_prep_sph(only)
     make_prep_pbc(only)
     make_test5_npt(only)
+    make_sph_thermostats(only)
     for name, (sysargs, inpargs) in CASES.items():
         if only and name not in only:
             continue

@@ -22,6 +22,9 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 # and LRF cut-offs inside half the box edge
 BOX_INP = dict(steps=30, stepsize=0.5, temperature=50.0, initial_temperature=50.0, bath_coupling=5.0,
                non_bond=5, rcq=12.0, rclrf=13.0, lambdas=[0.6, 0.4])
+# 300 K runs of 1 fs steps from Maxwell velocities
+HOT_INP = dict(steps=30, lambdas=[0.6, 0.4], stepsize=1.0, temperature=300.0, bath_coupling=10.0,
+               initial_temperature=300.0)
 # constant pressure in the box: volume moves of up to 60 A^3 at 1 bar
 NPT = {"constant_pressure": True, "max_volume_displ": 60, "pressure_seed": 4242, "pressure": 1.0,
        "put_solvent_back_in_box": True, "put_solute_back_in_box": False}
@@ -183,6 +186,28 @@ CASES = {
     "c15_box28_q6_npt_nolrf_30": (dict(radius=14.0, n_solute=60, seed=9106, n_q=6, n_states=2, box=28.0),
                                   dict(BOX_INP, lrf=False, pbc=dict(NPT, pressure_seed=99, pressure=500.0),
                                        volume_change=10)),
+    # thermostats and integrators at 300 K, 1 fs: Langevin (uniform random
+    # forces) with SHAKE; Langevin with Gaussian forces, a set friction and
+    # velocity-Verlet; Nose-Hoover with SHAKE (the chain propagated again
+    # after SHAKE); Nose-Hoover with velocity-Verlet, 4 chains and a set
+    # mass; Berendsen with velocity-Verlet and SHAKE; Langevin at constant
+    # pressure (volume moves and random forces share randm's stream)
+    "c16_r14_q8_lan_30": (dict(radius=14.0, n_solute=40, seed=3101, n_q=8, n_states=2),
+                          dict(HOT_INP, thermostat="langevin")),
+    "c16_r14_q8_lan_gauss_vv_30": (dict(radius=14.0, n_solute=40, seed=3102, n_q=8, n_states=2),
+                                   dict(HOT_INP, thermostat="langevin", integrator="velocity-verlet",
+                                        md_extra={"langevin_random": True, "langevin_friction": 0.5})),
+    "c16_r14_q8_nh_shake_30": (dict(radius=14.0, n_solute=40, seed=3103, n_q=8, n_states=2),
+                               dict(HOT_INP, thermostat="nose-hoover", shake_hydrogens=True)),
+    "c16_r14_q8_nh_vv_30": (dict(radius=14.0, n_solute=40, seed=3104, n_q=8, n_states=2),
+                            dict(HOT_INP, thermostat="nose-hoover", integrator="velocity-verlet",
+                                 md_extra={"nhchains": 4, "nose-hoover_mass": 2.5})),
+    "c16_r14_q8_ber_vv_30": (dict(radius=14.0, n_solute=40, seed=3105, n_q=8, n_states=2),
+                             dict(HOT_INP, integrator="velocity-verlet", shake_hydrogens=True)),
+    "c16_box28_q6_lan_npt_30": (dict(radius=14.0, n_solute=60, seed=3106, n_q=6, n_states=2, box=28.0),
+                                dict(HOT_INP, stepsize=0.5, non_bond=5, rcq=12.0, rclrf=13.0, thermostat="langevin",
+                                     pbc=dict(NPT, max_volume_displ=3, pressure=500.0, pressure_seed=7),
+                                     volume_change=5)),
 }
 
 ORACLE_COLS = {"ww_el": "ww_el", "ww_vdw": "ww_vdw", "lrf": "lrf", "rst_total": "rst_total",

@@ -16,6 +16,9 @@ sphere and the periodic box with leap-frog and Berendsen
   final restart (x, v, water-polarisation theta_corr) bit for bit as the
   reference's serial qdyn5 wrote it, and md.write_out of the oracle's
   energies reproduces every summary block qdyn5 printed, byte for byte.
+* The same sphere input with every other integrator and thermostat of the
+  reference's test matrix (run_test.sh:193-200: velocity-Verlet,
+  Langevin, Nose-Hoover): every printed block and x, v bit for bit.
 * The reference's constant-pressure test (tests/test5: the prep_PBC system,
   its own FEP file and eq1.inp with volume moves; copies in
   tests/golden/test5_NPT), shortened to 100 steps with lists and volume
@@ -226,4 +229,39 @@ def test_npt_eq1_run_matches_reference_qdyn(tmp_path):
     digest = lambda a: hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
     assert digest(x) == str(g["x_sha"]) and digest(v) == str(g["v_sha"])
     assert np.array_equal(np.concatenate([bl, bc]), g["box"])
+    o.close()
+
+
+THERMO = [(i, t) for i in ("leap-frog", "velocity-verlet") for t in ("berendsen", "langevin", "nose-hoover")
+          if (i, t) != ("leap-frog", "berendsen")]
+
+
+@pytest.mark.parametrize("integrator,thermostat", THERMO)
+def test_sph_eq1_thermostats_match_reference_qdyn(integrator, thermostat, tmp_path):
+    """100 steps of the basic sphere test with the given integrator and
+    thermostat (Langevin: uniform random forces from randm's stream after
+    the Maxwell draws; Nose-Hoover: a 10-chain thermostat of mass kT/tau^2):
+    every printed block byte for byte and the final x and v bit for bit."""
+    g = np.load(os.path.join(GOLD, "sph_eq1_thermostats.npz"), allow_pickle=False)
+    key = "%s_%s" % (integrator.replace("-", "_"), thermostat.replace("-", "_"))
+    inp = bt.sph_input(os.path.join(bt.FIXTURE, "eq1.inp"), bt.FIXTURE, str(tmp_path), integrator=integrator,
+                       thermostat=thermostat)
+    s = md.read_settings(inp)
+    nsteps, iout = s["steps"], s["iout"]
+    o = Oracle(inp)
+    z = o.sizes()
+    printed = []
+    for k in range(nsteps):
+        o.run(1)
+        if k % iout == 0:
+            printed.append(md.write_out(o.energies(), o.qenergies(), [1.0], step=k, final=False, **z))
+    o.final_energy()
+    printed.append(md.write_out(o.energies(), o.qenergies(), [1.0], step=nsteps, final=True, **z))
+    blocks = [str(b) for b in g[key + "_blocks"]]
+    assert len(printed) == len(blocks)
+    for k, (a, b) in enumerate(zip(printed, blocks)):
+        assert a == b, "block %d:\n%s\n---- reference ----\n%s" % (k, a, b)
+    x, v = o.state()
+    digest = lambda a: hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+    assert digest(x) == str(g[key + "_x_sha"]) and digest(v) == str(g[key + "_v_sha"])
     o.close()
