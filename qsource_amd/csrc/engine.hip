@@ -38,6 +38,7 @@ template <typename P, int NS> __global__ void k_q_forces(const DevSys*, int);
 __global__ void k_q_reduce(const DevSys*, int);
 __global__ void k_q_lists(const DevSys*, int);
 __global__ void k_mc_scale(const DevSys*);
+__global__ void k_nh_prop(const DevSys*);
 __global__ void k_put_back(const DevSys*);
 __global__ void k_monitor(const DevSys*);
 __global__ void k_freeze_x(const DevSys*);
@@ -304,6 +305,7 @@ void Batch::init(const std::vector<std::string>& inputs, int device) {
     q_dynamic_ = q_dynamic_ || (s->nqat > 0 && (s->pbc || !(s->inp.rcq > s->top.rexcl_o)));
     put_back_ = put_back_ || (s->pbc && (s->inp.put_solute_back || s->inp.put_solvent_back));
     freeze_ = freeze_ || s->inp.freeze;
+    nose_hoover_ = nose_hoover_ || s->thermostat == 2;
   }
   nbcycle_ = setups_[0]->inp.nbcycle;
   for (auto* s : setups_)
@@ -1023,6 +1025,39 @@ void Batch::upload(int k) {
   }
   // integrator
   D.dt = s.dt; D.temp0 = s.inp.temp0; D.tau_t = s.tau_t;
+  // thermostat and integrator
+  D.thermostat = s.thermostat;
+  D.vverlet = s.velocity_verlet ? 1 : 0;
+  D.dv_mod = s.dv_mod;
+  D.dv_friction = s.dv_friction;
+  D.lgauss = s.langevin_gauss ? 1 : 0;
+  D.rands = nullptr; D.rng = nullptr; D.rng_jump = nullptr;
+  D.rng_step[0] = 1; D.rng_step[1] = 0;
+  if (s.thermostat == 1) {
+    // randm as an affine map mod 1e8 (state' = 31415821 state + 1): the
+    // jump to atom i's first draw and over a whole step's draws
+    const long long m = 100000000LL, mult = 31415821LL;
+    const int per_atom = s.langevin_gauss ? 36 : 3;
+    std::vector<int> jump(2 * (size_t)s.natom);
+    long long A = 1, C = 0;
+    for (int i = 0; i < s.natom; ++i) {
+      jump[2 * i] = (int)A;
+      jump[2 * i + 1] = (int)C;
+      for (int k = 0; k < per_atom; ++k) { A = A * mult % m; C = (C * mult + 1) % m; }
+    }
+    D.rng_step[0] = A; D.rng_step[1] = C;
+    D.rng_jump = I->upload(jump);
+    D.rands = I->upload(s.rands);
+    // the stream continues from the Maxwell draws; unstarted, the first
+    // draw seeds it from random_seed as randm does
+    qsys::Randm r = s.rng;
+    if (!r.started) { r.started = true; r.irand = std::abs(s.inp.iseed) % 100000000; }
+    D.rng = I->upload(std::vector<int>{r.irand});
+  }
+  D.nhn = s.thermostat == 2 ? (int)s.qnh.size() : 0;
+  D.qnh = D.nhn ? I->upload(s.qnh) : nullptr;
+  D.nh = D.nhn ? I->upload(std::vector<double>(3 * (size_t)D.nhn + 2, 0.0)) : nullptr;
+  D.kbt = s.kbt;
   D.separate_scaling = s.separate_scaling; D.detail_temps = s.detail_temps;
   D.ndegf = s.ndegf; D.ndegfree = s.ndegfree; D.ndegf_solute = s.ndegf_solute;
   D.ndegf_solvent = s.ndegf_solvent; D.ndegfree_solute = s.ndegfree_solute;
@@ -1485,6 +1520,8 @@ void Batch::launch_integrate(long s) {
   const int blocks = (std::max(max_units_, 1) + kIntegrateBlock - 1) / kIntegrateBlock;
   // energy-file frame of this step (the kernel checks the step counter)
   if (record_possible(s)) { pb(10); hipLaunchKernelGGL(k_energy_reduce, dim3(1, ni), dim3(256), 0, stream_, d_sys_, 1); pe(10); }
+  // Nose-Hoover chains: this step's velocity scales
+  if (nose_hoover_) hipLaunchKernelGGL(k_nh_prop, dim3(1, ni), dim3(64), 0, stream_, d_sys_);
   { pb(11); hipLaunchKernelGGL(k_integrate, dim3(blocks, ni), dim3(kIntegrateBlock), 0, stream_, d_sys_); pe(11); }
   { pb(12); hipLaunchKernelGGL(k_temperature, dim3(1, ni), dim3(256), 0, stream_, d_sys_, blocks); pe(12); }
 }
@@ -1636,6 +1673,11 @@ void Batch::mc_volume() {
     if (!vol_[k].on) continue;
     DevSys& D = h_sys_[k];
     qsys::Setup& s = *setups_[k];
+    // Langevin instances: randm's stream is on the device
+    if (D.rng) {
+      HIPCHECK(hipMemcpy(&s.rng.irand, D.rng, sizeof(int), hipMemcpyDeviceToHost));
+      s.rng.started = true;
+    }
     old_v[k] = D.boxl[0] * D.boxl[1] * D.boxl[2];
     double r = s.rng.next(vol_[k].seed);
     r = r * 2 - 1;
@@ -1689,6 +1731,7 @@ void Batch::mc_volume() {
     }
     vol_[k].tries += 1;
     vol_[k].log.push_back(acc ? (drawn ? 2 : 1) : 0);
+    if (h_sys_[k].rng) HIPCHECK(hipMemcpy(h_sys_[k].rng, &s.rng.irand, sizeof(int), hipMemcpyHostToDevice));
     if (acc) {
       vol_[k].accepted += 1;
       continue;

@@ -274,6 +274,28 @@ struct DevSys {
   // integrator
   double dt;
   double temp0, tau_t;
+  // thermostat (0 Berendsen, 1 Langevin, 2 Nose-Hoover) and velocity-Verlet
+  // (the velocity update applied again after the position update); dv_mod
+  // the velocity update's step
+  int thermostat, vverlet;
+  double dv_mod;
+  // Langevin: friction factor, per-atom random force scale, Gaussian (1)
+  // or uniform (0) forces; randm's state (rng[0], advanced once per step by
+  // the step's draws: rng_step = {A, C}, state' = (A state + C) mod 1e8)
+  // and per atom the jump to its first draw (rng_jump[2i], rng_jump[2i+1])
+  double dv_friction;
+  const double* rands;
+  int lgauss;
+  int* rng;
+  const int* rng_jump;
+  long long rng_step[2];
+  // Nose-Hoover chain: nhn links with masses qnh; nh holds xnh, vnh, Gnh
+  // (nhn each) and the step's two velocity scales (before the update, and
+  // after SHAKE), formed by k_nh_prop before k_integrate
+  int nhn;
+  const double* qnh;
+  double* nh;
+  double kbt;
   int separate_scaling, detail_temps;
   int ndegf, ndegfree, ndegf_solute, ndegf_solvent, ndegfree_solute, ndegfree_solvent;
   // SHAKE units: every atom belongs to exactly one unit (a SHAKE molecule or
@@ -448,6 +470,7 @@ class Batch {
   bool q_dynamic_ = false; // some instance rebuilds its Q lists at list updates
   bool put_back_ = false;  // some periodic-box instance puts molecules back at list updates
   bool freeze_ = false;    // some instance has excluded_freeze on
+  bool nose_hoover_ = false;   // some instance has a Nose-Hoover thermostat
   int iene_ = 0;
   long ebase_ = 0;         // step of the last set_energy_interval
   bool use_graph_ = true;

@@ -1024,6 +1024,85 @@ __global__ __launch_bounds__(256) void k_shake(const DevSys* __restrict__ sys, c
   else atomicAdd(out, n);
 }
 
+// randm (src/md.f90:17543-17577) as the affine map state' = (31415821 state
+// + 1) mod 1e8, and its real value: state/10*10 / 1e8, 0 outside (0, 1].
+__device__ __forceinline__ double randm_next(long long& st) {
+  st = (st * 31415821LL + 1) % 100000000LL;
+  double r = (double)(st / 10) * 10 / 100000000.0;
+  if (r <= 0.0 || r > 1.0) r = 0.0;
+  return r;
+}
+
+// Langevin random forces of atom i (newvel_lan, src/md.f90:6232-6248): its
+// draws start after those of atoms 0..i-1 of this step; uniform
+// rands (u - 1/2) or Gaussian (gauss: twelve draws, (sum - 6) rands).
+__device__ __forceinline__ void langevin_forces(const DevSys& S, int i, double* re) {
+  long long st = ((long long)S.rng_jump[2 * i] * (long long)S.rng[0] + (long long)S.rng_jump[2 * i + 1]) %
+                 100000000LL;
+  const double sd = S.rands[i];
+  for (int c = 0; c < 3; ++c) {
+    if (!S.lgauss) {
+      re[c] = sd * (randm_next(st) - 0.5);
+    } else {
+      double a = 0.0;
+      for (int k = 0; k < 12; ++k) a = a + randm_next(st);
+      re[c] = (a - 6.0) * sd + 0.0;
+    }
+  }
+}
+
+// The thermostat's velocity update of one component (newvel_ber,
+// newvel_lan, newvel_nos: src/md.f90:6194-6301) with gradient g, inverse
+// mass w, Berendsen scale ts and Langevin random force re.
+__device__ __forceinline__ double vel_update(const DevSys& S, double v, double g, double w, double ts, double re) {
+  if (S.thermostat == 1) return (v * S.dv_friction) - (g - re) * w * S.dv_mod;
+  if (S.thermostat == 2) return (v - g * w * S.dv_mod);
+  return (v - g * w * S.dv_mod) * ts;
+}
+
+// nh_prop (src/md.f90:6303-6344) for the Nose-Hoover instances, before
+// k_integrate: the chain's half-step from the last temperature and the
+// velocity scale exp(-vnh_1 dt/2); with SHAKE the second propagation md_run
+// makes after SHAKE (from the same temperature) too.  Neither reads the
+// velocities, so both scales are formed here: nh[3n] before the velocity
+// update, nh[3n+1] after SHAKE.  One thread per instance.
+__device__ double nh_half(const DevSys& S) {
+  const int n = S.nhn;
+  double* xnh = S.nh;
+  double* vnh = S.nh + n;
+  double* gnh = S.nh + 2 * n;
+  const double* q = S.qnh;
+  const double dt2 = 0.5 * S.dt, dt4 = 0.5 * dt2, dt8 = 0.5 * dt4, T0 = S.temp0, temp = S.scal[3];
+  gnh[0] = S.ndegf * qsys::kBoltz * (temp - T0) / q[0];
+  for (int i = 1; i < n; ++i) gnh[i] = (q[i - 1] * (vnh[i - 1] * vnh[i - 1]) - S.kbt) / q[i];
+  vnh[n - 1] = vnh[n - 1] + gnh[n - 1] * dt4;
+  xnh[n - 1] = xnh[n - 1] + vnh[n - 1] * dt2;
+  double expf = 0.0;
+  for (int i = n - 2; i >= 0; --i) {
+    expf = exp(-(dt8 * vnh[i + 1]));
+    vnh[i] = (vnh[i] * expf + gnh[i] * dt4) * expf;
+    xnh[i] = xnh[i] + vnh[i] * dt2;
+  }
+  const double sc = exp(-(vnh[0] * dt2));
+  gnh[0] = S.ndegf * qsys::kBoltz * (sc * sc * temp - T0) / q[0];
+  vnh[0] = (vnh[0] * expf + gnh[0] * dt4) * expf;
+  for (int i = 1; i < n - 1; ++i) {
+    expf = exp(-(dt8 * vnh[i + 1]));
+    gnh[i] = (q[i - 1] * (vnh[i - 1] * vnh[i - 1]) - S.kbt) / q[i];
+    vnh[i] = (vnh[i] * expf + gnh[i] * dt4) * expf;
+  }
+  gnh[n - 1] = (q[n - 2] * (vnh[n - 2] * vnh[n - 2]) - S.kbt) / q[n - 1];
+  vnh[n - 1] = vnh[n - 1] + gnh[n - 1] * dt4;
+  return sc;
+}
+__global__ void k_nh_prop(const DevSys* __restrict__ sys) {
+  const DevSys& S = sys[blockIdx.y];
+  if (threadIdx.x != 0 || S.thermostat != 2 || halted(S, PH_INTEGRATE)) return;
+  const int n = S.nhn;
+  S.nh[3 * n] = nh_half(S);
+  S.nh[3 * n + 1] = S.has_shake ? nh_half(S) : 1.0;
+}
+
 __global__ __launch_bounds__(kIntegrateBlock) void k_integrate(const DevSys* __restrict__ sys) {
   const DevSys& S = sys[blockIdx.y];
   if (halted(S, PH_INTEGRATE)) return;
@@ -1035,18 +1114,26 @@ __global__ __launch_bounds__(kIntegrateBlock) void k_integrate(const DevSys* __r
     const double dt = S.dt;
     const int a = S.unit_atoms[S.unit_first[u]];
     const double ts = a < S.nat_solute ? S.scal[0] : S.scal[1];
-    double x[9], xo[9], v[9], w[3], gd[9];
+    // Nose-Hoover: the velocities scaled by the chain first
+    const double s1 = S.thermostat == 2 ? S.nh[3 * S.nhn] : 1.0;
+    double x[9], xo[9], v[9], w[3], gd[9], re[9];
     for (int k = 0; k < 3; ++k) w[k] = S.winv[a + k];
     for (int k = 0; k < 3; ++k) step_gradient(S, a + k, gd + 3 * k);
+    for (int c = 0; c < 9; ++c) re[c] = 0.0;
+    if (S.thermostat == 1)
+      for (int k = 0; k < 3; ++k) langevin_forces(S, a + k, re + 3 * k);
     for (int c = 0; c < 9; ++c) {
       const int ic = 3 * a + c;
       // excluded_freeze: an excluded atom restarts from rest at its
       // topology position (fix_shell, src/md.f90:2831-2839)
       const bool frz = S.freeze && (S.flags[a + c / 3] & F_EXCL);
-      double vv = ((frz ? 0.0 : S.v[ic]) - gd[c] * w[c / 3] * dt) * ts;
-      v[c] = vv;
+      double v0 = frz ? 0.0 : S.v[ic];
+      if (S.thermostat == 2) v0 = v0 * s1;
+      double vv = vel_update(S, v0, gd[c], w[c / 3], ts, re[c]);
       xo[c] = frz ? S.xtop[ic] : S.x[ic];
       x[c] = xo[c] + vv * dt;
+      if (S.vverlet) vv = vel_update(S, vv, gd[c], w[c / 3], ts, re[c]);
+      v[c] = vv;
     }
     const int c0 = S.con_first[u];
     const double d2[3] = {S.sh_d2[c0], S.sh_d2[c0 + 1], S.sh_d2[c0 + 2]};
@@ -1075,9 +1162,13 @@ __global__ __launch_bounds__(kIntegrateBlock) void k_integrate(const DevSys* __r
       if (ready == 7u) break;
       if (nits >= S.shake_maxiter) { record_failure(S, ST_SHAKE, PH_INTEGRATE); break; }
     }
+    const double s2 = S.thermostat == 2 ? S.nh[3 * S.nhn + 1] : 1.0;
     for (int c = 0; c < 9; ++c) {
       const int ic = 3 * a + c;
-      if (S.has_shake) v[c] = (x[c] - xo[c]) / dt;
+      if (S.has_shake) {
+        v[c] = (x[c] - xo[c]) / dt;
+        if (S.thermostat == 2) v[c] = v[c] * s2;
+      }
       S.x[ic] = x[c];
       S.xf[ic] = (float)x[c];
       S.v[ic] = v[c];
@@ -1095,32 +1186,41 @@ __global__ __launch_bounds__(kIntegrateBlock) void k_integrate(const DevSys* __r
   } else if (u < S.nunits) {
     const double dt = S.dt;
     const double ts_solute = S.scal[0], ts_solvent = S.scal[1];
+    const double s1 = S.thermostat == 2 ? S.nh[3 * S.nhn] : 1.0;
     const int a0 = S.unit_first[u], a1 = S.unit_first[u + 1];
     for (int k = a0; k < a1; ++k) {
       const int i = S.unit_atoms[k];
-      double gd[3];
+      double gd[3], re[3] = {0.0, 0.0, 0.0};
       step_gradient(S, i, gd);
+      if (S.thermostat == 1) langevin_forces(S, i, re);
       const double ts = i < S.nat_solute ? ts_solute : ts_solvent;
       const double w = S.winv[i];
       const bool frz = S.freeze && (S.flags[i] & F_EXCL);
       for (int c = 0; c < 3; ++c) {
         const int ic = 3 * i + c;
-        double vv = ((frz ? 0.0 : S.v[ic]) - gd[c] * w * dt) * ts;
+        double v0 = frz ? 0.0 : S.v[ic];
+        if (S.thermostat == 2) v0 = v0 * s1;
+        double vv = vel_update(S, v0, gd[c], w, ts, re[c]);
         const double xo = frz ? S.xtop[ic] : S.x[ic];
-        S.v[ic] = vv;
         S.xx[ic] = xo;
         S.x[ic] = xo + vv * dt;
+        if (S.vverlet) vv = vel_update(S, vv, gd[c], w, ts, re[c]);
+        S.v[ic] = vv;
         S.d[ic] = gd[c];
       }
     }
     const int c0 = S.con_first[u], c1 = S.con_first[u + 1];
     if (c1 > c0 && shake_molecule(S, S.x, S.xx, c0, c1) < 0) record_failure(S, ST_SHAKE, PH_INTEGRATE);
+    const double s2 = S.thermostat == 2 ? S.nh[3 * S.nhn + 1] : 1.0;
     for (int k = a0; k < a1; ++k) {
       const int i = S.unit_atoms[k];
       double vv[3];
       for (int c = 0; c < 3; ++c) {
         const int ic = 3 * i + c;
-        if (S.has_shake) S.v[ic] = (S.x[ic] - S.xx[ic]) / dt;
+        if (S.has_shake) {
+          S.v[ic] = (S.x[ic] - S.xx[ic]) / dt;
+          if (S.thermostat == 2) S.v[ic] = S.v[ic] * s2;
+        }
         vv[c] = S.v[ic];
         S.xf[ic] = (float)S.x[ic];
         if (S.srec && i < S.nat_solute) S.srec[(size_t)kSrec * S.srec_of_atom[i] + c] = (float)S.x[ic];
@@ -1184,6 +1284,9 @@ __global__ __launch_bounds__(256) void k_temperature(const DevSys* __restrict__ 
     tsu = tsv;
   }
   S.scal[0] = tsu; S.scal[1] = tsv; S.scal[2] = tfree; S.scal[3] = temp;
+  // Langevin: randm's state after this step's draws
+  if (S.thermostat == 1)
+    S.rng[0] = (int)((S.rng_step[0] * (long long)S.rng[0] + S.rng_step[1]) % 100000000LL);
   // a NaN temperature stops the run after this step (md_run's test,
   // src/md.f90:6654-6656, made at every step)
   if (temp != temp) record_failure(S, ST_NAN, PH_TEMPERATURE);
