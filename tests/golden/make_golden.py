@@ -480,6 +480,48 @@ def make_sph_thermostats(only):
     print(name, sorted(k[:-7] for k in out if k.endswith("_blocks")))
 
 
+QFEP = os.path.join(ROOT, "oracle", "_ref", "qfep5")
+QFEP_KT = 0.596
+QFEP_LAMBDAS = [0.0, 0.25, 0.5, 0.75, 1.0]
+
+
+def make_qfep(only):
+    """Five lambda windows of a small two-state system run by reference
+    qdyn5 with an energy frame every step (the engine writes these files
+    byte for byte the same), then reference qfep5 on them: keeps the energy
+    files and qfep's Part 1 table (lambda, dGf, sum dGf, dGr, sum dGr, <dG>
+    per file)."""
+    name = "qfep_5win"
+    if only and name not in only:
+        return
+    if not os.path.exists(QFEP):
+        sys.exit("reference qfep5 not built: make -f oracle/Makefile.ref OUT=oracle/_ref")
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        s = gensys.solvated_system(radius=12.0, n_solute=40, seed=2401, n_q=8, n_states=2)
+        files = []
+        for k, lam in enumerate(QFEP_LAMBDAS):
+            p = gensys.write_system(s, d, "w%d" % k, steps=60, stepsize=0.5, temperature=50.0, bath_coupling=2.0,
+                                    initial_temperature=50.0, lambdas=[1 - lam, lam], energy="w%d.en" % k,
+                                    energy_interval=1)
+            r = subprocess.run([QDYN, p["inp"]], cwd=d, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+            if "terminated normally" not in r.stdout:
+                sys.exit("reference run failed for %s window %d" % (name, k))
+            files.append("w%d.en" % k)
+            out["en%d" % k] = np.frombuffer(open(os.path.join(d, files[-1]), "rb").read(), dtype=np.uint8)
+        # qfep's prompts: files; states, off-diagonals; kT, points to skip;
+        # gap bins; points per bin; alpha of state 2; off-diagonal count;
+        # reaction coordinate; file names
+        stdin = "%d\n2 0\n%.3f 0\n1\n1\n0.0\n0\n1 -1\n%s\n" % (len(files), QFEP_KT, "\n".join(files))
+        r = subprocess.run([QFEP], cwd=d, input=stdin, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        lines = r.stdout.splitlines()
+        i = next(k for k, l in enumerate(lines) if l.startswith("# lambda(1)      dGf"))
+        table = [[float(t) for t in lines[i + 1 + k].split()] for k in range(len(files))]
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), part1=np.array(table), kT=np.array(QFEP_KT),
+                        lambdas=np.array(QFEP_LAMBDAS), **out)
+    print(name, np.array(table)[:, 5])
+
+
 def main():
     if not os.path.exists(QDYN):
         sys.exit("reference qdyn5 not built: make -f oracle/Makefile.ref OUT=oracle/_ref")
@@ -489,6 +531,7 @@ def main():
     make_prep_pbc(only)
     make_test5_npt(only)
     make_sph_thermostats(only)
+    make_qfep(only)
     for name, (sysargs, inpargs) in CASES.items():
         if only and name not in only:
             continue

@@ -4,7 +4,11 @@ Pins: a constant energy gap gives dG = gap for Zwanzig and BAR exactly;
 Gaussian gaps (linear response) give dG = mu - sigma^2/(2kT) in the limit
 of many samples; BAR and Zwanzig agree on overlapping windows; the energy
 file round-trips and holds qdyn's record layout (header + per-state records
-of 4 + 8*15 bytes + an empty off-diagonal record per frame).
+of 4 + 8*15 bytes + an empty off-diagonal record per frame); on five lambda
+windows written by reference qdyn5 (the engine writes such files byte for
+byte the same), fep.read_energy_file and fep.zwanzig reproduce every
+number of reference qfep5's Part 1 table (dGf, its running sum, dGr, its
+running sum, <dG>) at qfep's printed precision.
 """
 import os
 import struct
@@ -80,3 +84,30 @@ def test_energy_file_offdiag_roundtrip(tmp_path):
     fep.write_energy_file(p2, lam, g["eq_frames"], offd=g["offd_frames"])
     _, rows2, od2 = fep.read_energy_frames(p2, 4)
     assert np.array_equal(rows2, g["eq_frames"]) and np.array_equal(od2, g["offd_frames"])
+
+
+def test_zwanzig_matches_reference_qfep(tmp_path):
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "qfep_5win.npz"),
+                allow_pickle=False)
+    lams = g["lambdas"]
+    lam, tots = [], []
+    for k in range(len(lams)):
+        path = os.path.join(str(tmp_path), "w%d.en" % k)
+        with open(path, "wb") as f:
+            f.write(g["en%d" % k].tobytes())
+        lk, tk = fep.read_energy_file(path, 2)
+        assert np.allclose(lk, [1 - lams[k], lams[k]])
+        lam.append(lk)
+        tots.append(tk)
+    kt = float(g["kT"])
+    z = fep.zwanzig(np.array(lam), tots, temperature=kt / fep.GAS_CONSTANT)
+    n = len(lams)
+    dgf, dgr, dg = z["dGf"], z["dGr"], z["dG"]
+    sum_f = np.concatenate([[0.0], np.cumsum(dgf[:-1])])
+    sum_r = np.array([sum(dgr[j] for j in range(k + 1, n)) for k in range(n)])
+    # qfep prints per file k: lambda(1), dGf(k-1), sum dGf up to k, dGr(k+1), sum dGr from k, <dG>(k)
+    for k, row in enumerate(g["part1"]):
+        ours = [lam[k][0], dgf[k - 1] if k > 0 else 0.0, sum_f[k], dgr[k + 1] if k + 1 < n else 0.0, sum_r[k],
+                dg[k]]
+        for a, b in zip(ours, row):
+            assert abs(round(a, 3) - b) <= 0.0010001, (k, ours, list(row))
