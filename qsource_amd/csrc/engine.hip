@@ -1102,6 +1102,11 @@ void Batch::upload(int k) {
   max_units_ = std::max(max_units_, D.nunits);
   D.unit_first = I->upload(ufirst);
   D.unit_atoms = I->upload(uatoms);
+  {
+    std::vector<int> head(D.nunits);
+    for (int u = 0; u < D.nunits; ++u) head[u] = uatoms[ufirst[u]];
+    D.unit_head = I->upload(head);
+  }
   D.con_first = I->upload(cfirst);
   D.sh_i = I->upload(shi);
   D.sh_j = I->upload(shj);

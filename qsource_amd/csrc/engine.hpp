@@ -307,6 +307,7 @@ struct DevSys {
   // 1 for a water unit (atoms a, a+1, a+2 with constraints (a,a+1),
   // (a,a+2), (a+1,a+2) in that order): integrated entirely in registers
   const signed char* unit_kind;
+  const int* unit_head;          // first atom of each unit
   const double* sh_d2;
   int shake_maxiter;             // SHAKE_MAX_ITER (1000); QGPU_SHAKE_MAXITER lowers it to force a failure
   // energies: eacc[term] partial sums written per item, reduced on demand

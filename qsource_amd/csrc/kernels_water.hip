@@ -968,6 +968,36 @@ __device__ __forceinline__ void step_gradient(const DevSys& S, int a, double* g)
   g[2] = g[2] + g2;
 }
 
+// step_gradient for the three atoms of water unit atoms a..a+2, with every
+// independent load of the water issued first (gradient, shell, slots) and
+// the dependent ones (polarisation gradient, slot sums) after them; the
+// same sums in the same order as step_gradient.
+__device__ __forceinline__ void water_step_gradient(const DevSys& S, int a, double* gd) {
+  for (int c = 0; c < 9; ++c) gd[c] = S.d[3 * a + c];
+  const int iw = (a - S.nat_solute) / 3;
+  const int sh = (S.wpol_restr && a >= S.nat_solute) ? S.shell_of[iw] : -1;
+  int so[3];
+  for (int k = 0; k < 3; ++k) so[k] = S.slot_of_atom[a + k];
+  if (sh >= 0) {
+    const double* w = S.wpol_g + 9 * (size_t)iw;
+    double wv[9];
+    for (int c = 0; c < 9; ++c) wv[c] = w[c];
+    for (int c = 0; c < 9; ++c) gd[c] = gd[c] + wv[c];
+  }
+  if (!(S.term_mask & TM_OTHER)) return;
+  for (int k = 0; k < 3; ++k) {
+    const int t = so[k];
+    if (t < 0) continue;
+    double g0 = 0, g1 = 0, g2 = 0;
+    const gconst<double> o = gptr(S.slot_g) + 3 * (size_t)S.slot_first[t];
+    const int n = S.slot_first[t + 1] - S.slot_first[t];
+    for (int j = 0; j < n; ++j) { g0 = g0 + o[3 * j]; g1 = g1 + o[3 * j + 1]; g2 = g2 + o[3 * j + 2]; }
+    gd[3 * k] = gd[3 * k] + g0;
+    gd[3 * k + 1] = gd[3 * k + 1] + g1;
+    gd[3 * k + 2] = gd[3 * k + 2] + g2;
+  }
+}
+
 // shake (src/md.f90:17581-17679) of one molecule, constraints c0..c1 of
 // the instance's SHAKE tables (at most 64), on coordinates x with the
 // pre-step coordinates xx: returns the molecule's iteration count, or -1
@@ -1112,13 +1142,16 @@ __global__ __launch_bounds__(kIntegrateBlock) void k_integrate(const DevSys* __r
     // water: the same arithmetic as the generic path below, with the 9
     // coordinates, the 9 pre-step coordinates and the velocities in registers
     const double dt = S.dt;
-    const int a = S.unit_atoms[S.unit_first[u]];
+    const int a = S.unit_head[u];
+    const int c0 = S.con_first[u];
+    const double d2[3] = {S.sh_d2[c0], S.sh_d2[c0 + 1], S.sh_d2[c0 + 2]};
     const double ts = a < S.nat_solute ? S.scal[0] : S.scal[1];
     // Nose-Hoover: the velocities scaled by the chain first
     const double s1 = S.thermostat == 2 ? S.nh[3 * S.nhn] : 1.0;
     double x[9], xo[9], v[9], w[3], gd[9], re[9];
     for (int k = 0; k < 3; ++k) w[k] = S.winv[a + k];
-    for (int k = 0; k < 3; ++k) step_gradient(S, a + k, gd + 3 * k);
+    for (int c = 0; c < 9; ++c) { v[c] = S.v[3 * a + c]; xo[c] = S.x[3 * a + c]; }
+    water_step_gradient(S, a, gd);
     for (int c = 0; c < 9; ++c) re[c] = 0.0;
     if (S.thermostat == 1)
       for (int k = 0; k < 3; ++k) langevin_forces(S, a + k, re + 3 * k);
@@ -1127,16 +1160,14 @@ __global__ __launch_bounds__(kIntegrateBlock) void k_integrate(const DevSys* __r
       // excluded_freeze: an excluded atom restarts from rest at its
       // topology position (fix_shell, src/md.f90:2831-2839)
       const bool frz = S.freeze && (S.flags[a + c / 3] & F_EXCL);
-      double v0 = frz ? 0.0 : S.v[ic];
+      double v0 = frz ? 0.0 : v[c];
       if (S.thermostat == 2) v0 = v0 * s1;
       double vv = vel_update(S, v0, gd[c], w[c / 3], ts, re[c]);
-      xo[c] = frz ? S.xtop[ic] : S.x[ic];
+      if (frz) xo[c] = S.xtop[ic];
       x[c] = xo[c] + vv * dt;
       if (S.vverlet) vv = vel_update(S, vv, gd[c], w[c / 3], ts, re[c]);
       v[c] = vv;
     }
-    const int c0 = S.con_first[u];
-    const double d2[3] = {S.sh_d2[c0], S.sh_d2[c0 + 1], S.sh_d2[c0 + 2]};
     const int ci[3] = {0, 0, 1}, cj[3] = {1, 2, 2};
     unsigned ready = 0u;
     int nits = 0;
